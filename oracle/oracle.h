@@ -1,0 +1,132 @@
+/*
+ * CPU oracle for the WRF-ARW dynamics step -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C restatement of the reference routines on the path
+ * (dyn_em/module_advect_em.F, module_small_step_em.F, module_em.F,
+ *  module_big_step_utilities_em.F, share/module_bc.F, dyn_em/solve_em.F),
+ * loop by loop and in the reference's floating-point operation order, so
+ * that an IEEE FP32 build reproduces the reference's default-REAL arithmetic
+ * and a FP64 build (-DWREAL=double) plays the role of the reference's -r8
+ * build.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+ * may load it, and only as the checker; the product path never calls it.
+ *
+ * Build: oracle/Makefile (gcc -O2 -ffp-contract=off -fno-fast-math).
+ * Every function cites the reference file:line it follows.
+ */
+#ifndef WRF_ORACLE_H
+#define WRF_ORACLE_H
+
+#include <stddef.h>
+#include <math.h>
+#include "../include/wrfmi.h"
+
+#ifndef WREAL
+#define WREAL float
+#endif
+typedef WREAL wreal;
+#define R(x) ((wreal)(x))
+
+typedef wrfmi_dims dims_t;
+typedef wrfmi_config cfg_t;
+
+/* state with WREAL arrays; member order identical to wrfmi_state */
+typedef struct ostate {
+#define O_PTR(n) wreal *n;
+  WRFMI_FIELDS_3D(O_PTR)
+  WRFMI_FIELDS_2D(O_PTR)
+  WRFMI_FIELDS_1D(O_PTR)
+#undef O_PTR
+#define O_R(n) wreal n;
+  WRFMI_SCALARS_R(O_R)
+#undef O_R
+  void *a, *alpha, *gamma;
+  int n_moist, n_species;
+  wreal *q[WRFMI_MAX_SPECIES];
+  wreal *q_old[WRFMI_MAX_SPECIES];
+  wreal *q_tend[WRFMI_MAX_SPECIES];
+} ostate;
+
+#define NI_ (d->ime - d->ims + 1)
+#define NK_ (d->kme - d->kms + 1)
+#define A3(a, i, k, j) ((a)[(((size_t)((j) - d->jms)) * NK_ + (size_t)((k) - d->kms)) * NI_ + (size_t)((i) - d->ims)])
+#define A2(a, i, j) ((a)[((size_t)((j) - d->jms)) * NI_ + (size_t)((i) - d->ims)])
+#define A1(a, k) ((a)[(k) - d->kms])
+
+#define IMIN(a, b) ((a) < (b) ? (a) : (b))
+#define IMAX(a, b) ((a) > (b) ? (a) : (b))
+
+/* share/module_model_constants.F:17-47 (REAL parameters, evaluated in WREAL) */
+#define K_G R(9.81)
+#define K_RD R(287.)
+#define K_CP (R(7.) * K_RD / R(2.))
+#define K_RV R(461.6)
+#define K_CV (K_CP - K_RD)
+#define K_CVPM (-K_CV / K_CP)
+#define K_P1000 R(100000.)
+#define K_T0 R(300.)
+#define K_P0 K_P1000
+#define K_CPOVCV (K_CP / (K_CP - K_RD))
+#define K_RVOVRD (K_RV / K_RD)
+#define K_RERADIUS (R(1.) / R(6370.0e03))
+#define K_CB R(25.)
+#define K_W_ALPHA R(0.3)
+#define K_W_BETA R(1.0)
+
+/* Fortran SIGN(1.,x): magnitude 1 with the sign bit of x */
+static inline wreal sgn1(wreal x) { return (wreal)copysign(1.0, (double)x); }
+
+/* ---- share/module_bc.F ---- */
+void o_set_physical_bc3d(wreal *dat, char var, const cfg_t *c, const dims_t *d);
+void o_set_physical_bc2d(wreal *dat, char var, const cfg_t *c, const dims_t *d);
+
+/* ---- rk_step_prep and callees ---- */
+void o_rk_step_prep(ostate *s, const cfg_t *c, const dims_t *d, int rk_step);
+void o_rk_phys_bc_dry_1(ostate *s, const cfg_t *c, const dims_t *d);
+
+/* ---- advection (order 5 horizontal / 3 vertical, plus 2..6 rejected) ---- */
+int o_advect_u(const wreal *u, const wreal *u_old, wreal *tend, const wreal *ru, const wreal *rv,
+               const wreal *rom, const wreal *c1, const wreal *c2, const wreal *mut, int time_step,
+               const cfg_t *c, const wreal *msfux, const wreal *msfuy, const wreal *msfvx,
+               const wreal *msfvy, const wreal *msftx, const wreal *msfty, const wreal *fzm,
+               const wreal *fzp, wreal rdx, wreal rdy, const wreal *rdzw, const dims_t *d);
+int o_advect_v(const wreal *v, const wreal *v_old, wreal *tend, const wreal *ru, const wreal *rv,
+               const wreal *rom, const wreal *c1, const wreal *c2, const wreal *mut, int time_step,
+               const cfg_t *c, const wreal *msfux, const wreal *msfuy, const wreal *msfvx,
+               const wreal *msfvy, const wreal *msftx, const wreal *msfty, const wreal *fzm,
+               const wreal *fzp, wreal rdx, wreal rdy, const wreal *rdzw, const dims_t *d);
+int o_advect_w(const wreal *w, const wreal *w_old, wreal *tend, const wreal *ru, const wreal *rv,
+               const wreal *rom, const wreal *c1, const wreal *c2, const wreal *mut, int time_step,
+               const cfg_t *c, const wreal *msfux, const wreal *msfuy, const wreal *msfvx,
+               const wreal *msfvy, const wreal *msftx, const wreal *msfty, const wreal *fzm,
+               const wreal *fzp, wreal rdx, wreal rdy, const wreal *rdzu, const dims_t *d);
+int o_advect_scalar(const wreal *field, const wreal *field_old, wreal *tend, const wreal *ru,
+                    const wreal *rv, const wreal *rom, const wreal *c1, const wreal *c2,
+                    const wreal *mut, int time_step, const cfg_t *c, const wreal *msfux,
+                    const wreal *msfuy, const wreal *msfvx, const wreal *msfvy, const wreal *msftx,
+                    const wreal *msfty, const wreal *fzm, const wreal *fzp, wreal rdx, wreal rdy,
+                    const wreal *rdzw, const dims_t *d);
+
+/* ---- rk_tendency and rk_addtend_dry ---- */
+int o_rk_tendency(ostate *s, const cfg_t *c, const dims_t *d, int rk_step);
+void o_rk_addtend_dry(ostate *s, const dims_t *d, int rk_step);
+
+/* ---- small step ---- */
+void o_small_step_prep(ostate *s, const dims_t *d, int rk_step);
+void o_calc_p_rho(ostate *s, const cfg_t *c, const dims_t *d, int step);
+void o_calc_coef_w(ostate *s, const cfg_t *c, const dims_t *d, wreal dts);
+void o_advance_uv(ostate *s, const cfg_t *c, const dims_t *d, wreal dts);
+void o_advance_mu_t(ostate *s, const cfg_t *c, const dims_t *d, wreal dts);
+void o_advance_w(ostate *s, const cfg_t *c, const dims_t *d, wreal dts);
+void o_sumflux(ostate *s, const dims_t *d, int iteration, int number_of_small_timesteps);
+void o_calc_mu_uv_1(ostate *s, const cfg_t *c, const dims_t *d);
+void o_small_step_finish(ostate *s, const dims_t *d, int nsmall, wreal dts, int rk_step, int rk_order);
+
+/* ---- scalars and EOS ---- */
+int o_rk_scalar_tend(ostate *s, const cfg_t *c, const dims_t *d, int ispec);
+void o_rk_update_scalar(ostate *s, const dims_t *d, int ispec, int rk_step, wreal dt_rk);
+void o_calc_p_rho_phi(ostate *s, const cfg_t *c, const dims_t *d);
+
+/* ---- one RK3 step (dyn_em/solve_em.F) ---- */
+int o_solve_em_step(ostate *s, const cfg_t *c, const dims_t *d);
+
+#endif

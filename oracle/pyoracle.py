@@ -1,0 +1,87 @@
+"""ctypes front end of the CPU oracle -- TEST INFRASTRUCTURE ONLY.
+
+Loads oracle/liboracle_f32.so (reference-order FP32 restatement) or
+liboracle_f64.so (same source in FP64, the -r8 analogue), and the reference's own
+advect_scalar from oracle/_ref/libref_advect.so when that was built.
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from wrf_amd.fields import (FIELDS_1D, FIELDS_2D, FIELDS_3D, MAX_SPECIES, SCALARS_R, State32, State64)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _load(name):
+    path = os.path.join(HERE, name)
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} is not built (run `make -C oracle`)")
+    return ctypes.CDLL(path)
+
+
+class Oracle:
+    def __init__(self, precision: str = "f32"):
+        self.precision = precision
+        self.lib = _load("liboracle_f32.so" if precision == "f32" else "liboracle_f64.so")
+        self.dtype = np.float32 if precision == "f32" else np.float64
+        self.ctype = ctypes.c_float if precision == "f32" else ctypes.c_double
+        self.State = State32 if precision == "f32" else State64
+        self.lib.oracle_state_sizeof.restype = ctypes.c_size_t
+        if self.lib.oracle_state_sizeof() != ctypes.sizeof(self.State):
+            raise RuntimeError("oracle state layout does not match wrf_amd.fields")
+        self.lib.oracle_solve_em_steps.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+
+    # ------------------------------------------------------------------ state plumbing
+    def to_native(self, hs):
+        """Copy a wrf_amd.grid.HostState into oracle-precision arrays; returns (arrays, struct)."""
+        arrs = {k: np.array(v, dtype=self.dtype, copy=True, order="C") for k, v in hs.arrays().items()}
+        st = self.State()
+        P = ctypes.POINTER(self.ctype)
+        for n in FIELDS_3D + FIELDS_2D + FIELDS_1D:
+            setattr(st, n, arrs[n].ctypes.data_as(P))
+        for n in SCALARS_R:
+            setattr(st, n, hs.scal[n])
+        st.a = arrs["a"].ctypes.data
+        st.alpha = arrs["alpha"].ctypes.data
+        st.gamma = arrs["gamma"].ctypes.data
+        st.n_moist, st.n_species = hs.n_moist, hs.n_species
+        for m in range(hs.n_species):
+            st.q[m] = arrs[f"q{m}"].ctypes.data_as(P)
+            st.q_old[m] = arrs[f"q_old{m}"].ctypes.data_as(P)
+            st.q_tend[m] = arrs[f"q_tend{m}"].ctypes.data_as(P)
+        return arrs, st
+
+    def run(self, hs, nsteps: int = 1):
+        """Advance a copy of `hs` by `nsteps` RK3 steps; returns the arrays dict."""
+        arrs, st = self.to_native(hs)
+        rc = self.lib.oracle_solve_em_steps(ctypes.byref(st), ctypes.byref(hs.cfg), ctypes.byref(hs.dims), nsteps)
+        if rc != 0:
+            raise RuntimeError(f"oracle solve_em step failed with status {rc}")
+        return arrs
+
+
+class RefAdvect:
+    """The reference's own advect_scalar (compiled in place, ADVECT_KERNEL mode)."""
+
+    def __init__(self):
+        self.lib = _load(os.path.join("_ref", "libref_advect.so"))
+        self.fn = self.lib.ref_advect_scalar
+        self.fn.restype = None
+
+    def advect_scalar(self, field, field_old, tend, ru, rv, rom, c1, c2, mut, time_step, cfg, msf, fzm, fzp,
+                      rdx, rdy, rdzw, dims):
+        flags = (ctypes.c_int * 15)(cfg.periodic_x, cfg.periodic_y, cfg.symmetric_xs, cfg.symmetric_xe,
+                                    cfg.symmetric_ys, cfg.symmetric_ye, cfg.open_xs, cfg.open_xe, cfg.open_ys,
+                                    cfg.open_ye, cfg.specified, cfg.nested, cfg.polar, cfg.h_sca_adv_order,
+                                    cfg.v_sca_adv_order)
+        d = (ctypes.c_int * 18)(*[getattr(dims, n) for n, _ in dims._fields_])
+        f = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+        self.fn(f(field), f(field_old), f(tend), f(ru), f(rv), f(rom), f(c1), f(c2), f(mut),
+                ctypes.c_int(time_step), flags, f(msf["msfux"]), f(msf["msfuy"]), f(msf["msfvx"]), f(msf["msfvy"]),
+                f(msf["msftx"]), f(msf["msfty"]), f(fzm), f(fzp), ctypes.c_float(rdx), ctypes.c_float(rdy),
+                f(rdzw), d)

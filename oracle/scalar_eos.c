@@ -1,0 +1,93 @@
+/*
+ * Oracle: scalar transport and the equation of state.  TEST INFRASTRUCTURE.
+ *   rk_scalar_tend    dyn_em/module_em.F:1096-1442 (advect_scalar branch, no diffusion)
+ *   rk_update_scalar  dyn_em/module_em.F:1587-1799
+ *   calc_p_rho_phi    dyn_em/module_big_step_utilities_em.F:953-1223 (non-hydrostatic)
+ * Species s->q[m] (m < n_moist are moist species, q[0] = QV); at the solve_em.F
+ * call sites (2215, 2309, 2783, 2849) ru/rv/ww are the time-averaged ru_m/rv_m/ww_m,
+ * mut is muts, scalar_1 is the *_old array and mu_old/mu_new are mu_1/mu_2.
+ */
+#include <stdlib.h>
+#include "oracle.h"
+
+/* advect_tend is zeroed over the tile (zero_tend, module_em.F:1224) and then filled */
+int o_rk_scalar_tend(ostate *s, const cfg_t *c, const dims_t *d, int m) {
+  int adv_opt = m < s->n_moist ? c->moist_adv_opt : c->scalar_adv_opt;
+  if (adv_opt != 0 || c->zadvect_implicit || c->diff_opt != 0 || c->diff_6th_opt != 0) return WRFMI_E_BADOPT;
+  for (int j = d->jts; j <= d->jte; j++)
+    for (int k = d->kts; k <= d->kte; k++)
+      for (int i = d->its; i <= d->ite; i++) A3(s->advect_tend, i, k, j) = R(0.);
+  return o_advect_scalar(s->q[m], s->q[m], s->advect_tend, s->ru_m, s->rv_m, s->ww_m, s->c1h, s->c2h, s->muts,
+                         c->time_step, c, s->msfux, s->msfuy, s->msfvx, s->msfvy, s->msftx, s->msfty, s->fnm, s->fnp,
+                         s->rdx, s->rdy, s->rdnw, d);
+}
+
+void o_rk_update_scalar(ostate *s, const dims_t *d, int m, int rk_step, wreal dt) {
+  const int ide = d->ide, jde = d->jde, kde = d->kde;
+  const int its = d->its, ite = d->ite, jts = d->jts, jte = d->jte, kts = d->kts, kte = d->kte;
+  wreal *sc1 = s->q_old[m], *sc2 = s->q[m], *sct = s->q_tend[m];
+  for (int j = jts; j <= IMIN(jte, jde - 1); j++)
+    for (int i = its; i <= IMIN(ite, ide - 1); i++) {
+      wreal muold = A2(s->mu_1, i, j) + A2(s->mub, i, j);
+      wreal munew = A2(s->mu_2, i, j) + A2(s->mub, i, j);
+      for (int k = kts; k <= IMIN(kte, kde - 1); k++) {
+        /* tendency = advect_tend*msfty (+ sc_tend): periodic/open bounds equal the tile */
+        wreal tendency = A3(s->advect_tend, i, k, j) * A2(s->msfty, i, j);
+        tendency = tendency + A3(sct, i, k, j);
+        if (rk_step == 1) A3(sc1, i, k, j) = A3(sc2, i, k, j);
+        A3(sc2, i, k, j) = ((A1(s->c1h, k) * muold + A1(s->c2h, k)) * A3(sc1, i, k, j) + dt * tendency) /
+                           (A1(s->c1h, k) * munew + A1(s->c2h, k));
+      }
+    }
+}
+
+void o_calc_p_rho_phi(ostate *s, const cfg_t *c, const dims_t *d) {
+  const int ide = d->ide, jde = d->jde, kde = d->kde;
+  const int its = d->its, ite = d->ite, jts = d->jts, jte = d->jte, kts = d->kts, kte = d->kte;
+  int itf = IMIN(ite, ide - 1), jtf = IMIN(jte, jde - 1), ktf = IMIN(kte, kde - 1);
+  wreal *al = s->al, *alb = s->alb, *mu = s->mu_2, *muts = s->muts, *ph = s->ph_2, *phb = s->phb, *p = s->p, *pb = s->pb, *t = s->t_2;
+  wreal ptop = s->p_top;
+  for (int j = jts; j <= jtf; j++)
+    for (int k = kts; k <= ktf; k++)
+      for (int i = its; i <= itf; i++) {
+        if (c->hypsometric_opt == 1) {
+          A3(al, i, k, j) = -R(1.) / (A1(s->c1h, k) * A2(muts, i, j) + A1(s->c2h, k)) *
+                            (A3(alb, i, k, j) * (A1(s->c1h, k) * A2(mu, i, j)) + A1(s->rdnw, k) * (A3(ph, i, k + 1, j) - A3(ph, i, k, j)));
+        } else {
+          wreal pfu = A1(s->c3f, k + 1) * A2(muts, i, j) + A1(s->c4f, k + 1) + ptop;
+          wreal pfd = A1(s->c3f, k) * A2(muts, i, j) + A1(s->c4f, k) + ptop;
+          wreal phm = A1(s->c3h, k) * A2(muts, i, j) + A1(s->c4h, k) + ptop;
+#if WREAL_IS_DOUBLE
+          wreal lg = log(pfd / pfu);
+#else
+          wreal lg = logf(pfd / pfu);
+#endif
+          A3(al, i, k, j) = (A3(ph, i, k + 1, j) - A3(ph, i, k, j) + A3(phb, i, k + 1, j) - A3(phb, i, k, j)) / phm / lg - A3(alb, i, k, j);
+        }
+      }
+  for (int j = jts; j <= jtf; j++)
+    for (int k = kts; k <= ktf; k++)
+      for (int i = its; i <= itf; i++) {
+        wreal temp;
+        if (s->n_moist >= 1) {
+          if (c->use_theta_m == 1) {
+            temp = (K_RD * (K_T0 + A3(t, i, k, j))) / (K_P0 * (A3(al, i, k, j) + A3(alb, i, k, j)));
+          } else {
+            wreal qvf = R(1.) + K_RVOVRD * A3(s->q[0], i, k, j);
+            temp = (K_RD * (K_T0 + A3(t, i, k, j)) * qvf) / (K_P0 * (A3(al, i, k, j) + A3(alb, i, k, j)));
+          }
+#if WREAL_IS_DOUBLE
+          wreal pw = pow(temp, K_CPOVCV);
+#else
+          wreal pw = powf(temp, K_CPOVCV);
+#endif
+          A3(p, i, k, j) = pw * K_P0 - A3(pb, i, k, j);
+        } else {
+#if WREAL_IS_DOUBLE
+          A3(p, i, k, j) = K_P0 * pow((K_RD * (K_T0 + A3(t, i, k, j))) / (K_P0 * (A3(al, i, k, j) + A3(alb, i, k, j))), K_CPOVCV) - A3(pb, i, k, j);
+#else
+          A3(p, i, k, j) = K_P0 * powf((K_RD * (K_T0 + A3(t, i, k, j))) / (K_P0 * (A3(al, i, k, j) + A3(alb, i, k, j))), K_CPOVCV) - A3(pb, i, k, j);
+#endif
+        }
+      }
+}

@@ -1,0 +1,139 @@
+/*
+ * Oracle: one RK3 dynamics time step in the call order of dyn_em/solve_em.F.
+ * TEST INFRASTRUCTURE.
+ *   acoustic step count   solve_em.F:446-465
+ *   RK substep sizes      solve_em.F:601-615
+ *   rk_step_prep + bc     solve_em.F:652-782 (the rho halo fill is omitted: rho is not read)
+ *   rk_tendency           solve_em.F:876-911
+ *   rk_addtend_dry        solve_em.F:967-980
+ *   small_step_prep, calc_p_rho(0), calc_coef_w   solve_em.F:1089-1147
+ *   halo fills                                    solve_em.F:1181-1264
+ *   acoustic loop          solve_em.F:1266-1719
+ *   calc_mu_uv_1, small_step_finish, rk3 fills    solve_em.F:1732-1800
+ *   scalar advection/update                       solve_em.F:2198-2364, 2777-2950
+ *   calc_p_rho_phi                                solve_em.F:2959-2979
+ *   rk_phys_bc_dry_2 + species fills (rk < 3)     solve_em.F:3240-3340
+ *   end-of-step calc_p_rho_phi + set_phys_bc_dry_2 + species fills  solve_em.F:4180-4540
+ * Physics is off: the *_tendf, mu_tendf, h_diabatic and species tendencies are read
+ * as given (module_first_rk_step_part1.F:182 zeroes them before the physics fills them).
+ */
+#include <stdlib.h>
+#include <math.h>
+#include "oracle.h"
+
+int o_num_sound_steps(const cfg_t *c, const ostate *s) {
+  if (c->time_step_sound != 0) return c->time_step_sound;
+  wreal spacing = s->dx < s->dy ? s->dx : s->dy;
+  int n = 2 * ((int)(R(300.) * s->dt / spacing - R(0.01)) + 1);
+  return n > 4 ? n : 4;
+}
+
+static void bc_species(ostate *s, const cfg_t *c, const dims_t *d) {
+  for (int m = 0; m < s->n_species; m++) o_set_physical_bc3d(s->q[m], 'p', c, d);
+}
+
+int o_solve_em_step(ostate *s, const cfg_t *c, const dims_t *d) {
+  int rc;
+  if (c->rk_ord != 3 || c->specified || c->nested || c->polar) return WRFMI_E_BADOPT;
+  const int rk_order = 3;
+  int nsound = o_num_sound_steps(c, s);
+  wreal dts = s->dt / (wreal)nsound;
+  for (int rk_step = 1; rk_step <= rk_order; rk_step++) {
+    wreal dt_rk, dts_rk;
+    int nsmall;
+    if (rk_step == 1) { dt_rk = s->dt / R(3.); dts_rk = dt_rk; nsmall = 1; }
+    else if (rk_step == 2) { dt_rk = R(0.5) * s->dt; dts_rk = dts; nsmall = nsound / 2; }
+    else { dt_rk = s->dt; dts_rk = dts; nsmall = nsound; }
+
+    o_rk_step_prep(s, c, d, rk_step);
+    o_rk_phys_bc_dry_1(s, c, d);
+    o_set_physical_bc3d(s->al, 'p', c, d);
+    o_set_physical_bc3d(s->ph_2, 'w', c, d);
+
+    if ((rc = o_rk_tendency(s, c, d, rk_step))) return rc;
+    o_rk_addtend_dry(s, d, rk_step);
+
+    o_small_step_prep(s, d, rk_step);
+    o_calc_p_rho(s, c, d, 0);
+    if (c->non_hydrostatic) o_calc_coef_w(s, c, d, dts_rk);
+
+    o_set_physical_bc3d(s->ru_tend, 'u', c, d);
+    o_set_physical_bc3d(s->rv_tend, 'v', c, d);
+    o_set_physical_bc3d(s->ph_2, 'w', c, d);
+    o_set_physical_bc3d(s->al, 'p', c, d);
+    o_set_physical_bc3d(s->p, 'p', c, d);
+    o_set_physical_bc3d(s->t_1, 'p', c, d);
+    o_set_physical_bc3d(s->t_save, 't', c, d);
+    o_set_physical_bc2d(s->mu_1, 't', c, d);
+    o_set_physical_bc2d(s->mu_2, 't', c, d);
+    o_set_physical_bc2d(s->mudf, 't', c, d);
+
+    for (int it = 1; it <= nsmall; it++) {
+      o_advance_uv(s, c, d, dts_rk);
+      o_advance_mu_t(s, c, d, dts_rk);
+      if (c->non_hydrostatic) o_advance_w(s, c, d, dts_rk);
+      o_sumflux(s, d, it, nsmall);
+      o_calc_p_rho(s, c, d, it);
+      o_set_physical_bc3d(s->ph_2, 'w', c, d);
+      o_set_physical_bc3d(s->al, 'p', c, d);
+      o_set_physical_bc3d(s->p, 'p', c, d);
+      o_set_physical_bc2d(s->muts, 't', c, d);
+      o_set_physical_bc2d(s->mu_2, 't', c, d);
+      o_set_physical_bc2d(s->mudf, 't', c, d);
+    }
+
+    o_calc_mu_uv_1(s, c, d);
+    o_small_step_finish(s, d, nsmall, dts_rk, rk_step, rk_order);
+    if (rk_step == rk_order) {
+      o_set_physical_bc3d(s->ru_m, 'u', c, d);
+      o_set_physical_bc3d(s->rv_m, 'v', c, d);
+      o_set_physical_bc3d(s->ww_m, 'w', c, d);
+      o_set_physical_bc2d(s->mut, 't', c, d);
+      o_set_physical_bc2d(s->muts, 't', c, d);
+    }
+
+    for (int m = 0; m < s->n_species; m++) {
+      if ((rc = o_rk_scalar_tend(s, c, d, m))) return rc;
+      o_rk_update_scalar(s, d, m, rk_step, dt_rk);
+    }
+
+    o_calc_p_rho_phi(s, c, d);
+
+    if (rk_step < rk_order) {
+      o_set_physical_bc3d(s->u_2, 'u', c, d);
+      o_set_physical_bc3d(s->v_2, 'v', c, d);
+      o_set_physical_bc3d(s->w_2, 'w', c, d);
+      o_set_physical_bc3d(s->t_2, 'p', c, d);
+      o_set_physical_bc3d(s->ph_2, 'w', c, d);
+      o_set_physical_bc2d(s->mu_2, 't', c, d);
+      bc_species(s, c, d);
+    }
+  }
+  /* solve_em.F:4180-4207 recomputes calc_p_rho_phi from unchanged inputs (same values) */
+  o_calc_p_rho_phi(s, c, d);
+  o_set_physical_bc3d(s->u_1, 'u', c, d);
+  o_set_physical_bc3d(s->u_2, 'u', c, d);
+  o_set_physical_bc3d(s->v_1, 'v', c, d);
+  o_set_physical_bc3d(s->v_2, 'v', c, d);
+  o_set_physical_bc3d(s->w_1, 'w', c, d);
+  o_set_physical_bc3d(s->w_2, 'w', c, d);
+  o_set_physical_bc3d(s->t_1, 'p', c, d);
+  o_set_physical_bc3d(s->t_2, 'p', c, d);
+  o_set_physical_bc3d(s->ph_1, 'w', c, d);
+  o_set_physical_bc3d(s->ph_2, 'w', c, d);
+  o_set_physical_bc2d(s->mu_1, 't', c, d);
+  o_set_physical_bc2d(s->mu_2, 't', c, d);
+  bc_species(s, c, d);
+  return 0;
+}
+
+/* ctypes helpers */
+size_t oracle_state_sizeof(void) { return sizeof(ostate); }
+int oracle_wreal_bytes(void) { return (int)sizeof(wreal); }
+int oracle_solve_em_steps(ostate *s, const cfg_t *c, const dims_t *d, int nsteps) {
+  for (int n = 0; n < nsteps; n++) {
+    int rc = o_solve_em_step(s, c, d);
+    if (rc) return rc;
+  }
+  return 0;
+}

@@ -1,0 +1,514 @@
+"""Domain set-up for the WRF-ARW dynamics step: index sextets, options, and the
+analytic initial states of the benchmark configurations.
+
+Arrays are numpy float32 in the reference's Fortran layout ``(ims:ime, kms:kme,
+jms:jme)`` with i fastest; in C order that is shape ``(nj, nk, ni)`` indexed
+``[j - jms, k - kms, i - ims]``.  The initial state follows the steps of the
+reference's idealised initialisation (dyn_em/module_initialize_ideal.F:611-1130:
+eta levels and their metrics, hybrid coefficients, dry base state from a dry
+sounding, moist full state from the moist sounding, hydrostatic geopotential,
+warm bubble) with an analytic Weisman-Klemp sounding in place of the
+input_sounding file, computed in float64 and rounded once to float32.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+from .fields import FIELDS_1D, FIELDS_2D, FIELDS_3D, MAX_SPECIES, Config, Dims
+
+# share/module_model_constants.F:17-47
+G = 9.81
+R_D = 287.0
+CP = 7.0 * R_D / 2.0
+R_V = 461.6
+CV = CP - R_D
+CVPM = -CV / CP
+RCP = R_D / CP
+P1000 = 1.0e5
+T0 = 300.0
+RVOVRD = R_V / R_D
+BDYZONE = 4  # share/module_bc.F:33
+
+
+@dataclasses.dataclass
+class DomainSpec:
+    nx: int = 40              # mass points in x (e_we = nx + 1)
+    ny: int = 40
+    nz: int = 40              # mass levels (e_vert = nz + 1)
+    dx: float = 2000.0
+    dy: float = 2000.0
+    dt: float = 12.0
+    ztop: float = 20000.0
+    time_step_sound: int = 6
+    bc: str = "periodic"      # "periodic" | "open"
+    hybrid_opt: int = 0
+    etac: float = 0.2
+    n_passive: int = 0        # passive scalars after QV
+    n_bubbles: int = 1
+    bubble_amp: float = 3.0
+    noise_amp: float = 0.1
+    noise_levels: int = 10
+    seed: int = 1001
+    pad_i: int = 32           # pad the memory i-extent to a multiple of this (alignment)
+    precision_mode: int = 0
+
+    @property
+    def e_we(self):
+        return self.nx + 1
+
+    @property
+    def e_sn(self):
+        return self.ny + 1
+
+    @property
+    def e_vert(self):
+        return self.nz + 1
+
+
+def make_dims(spec: DomainSpec) -> Dims:
+    """Serial WRF domain/memory/tile sextets (frame/module_domain.F:686-687: halo = bdyzone).
+    The tile is the whole patch, as in a single-tile run."""
+    d = Dims()
+    d.ids, d.ide = 1, spec.e_we
+    d.jds, d.jde = 1, spec.e_sn
+    d.kds, d.kde = 1, spec.e_vert
+    d.ims, d.jms, d.kms = d.ids - BDYZONE, d.jds - BDYZONE, d.kds
+    ime = d.ide + BDYZONE
+    ni = ime - d.ims + 1
+    if spec.pad_i > 1:
+        ni = -(-ni // spec.pad_i) * spec.pad_i
+    d.ime = d.ims + ni - 1
+    d.jme = d.jde + BDYZONE
+    d.kme = d.kde
+    d.its, d.ite, d.jts, d.jte, d.kts, d.kte = d.ids, d.ide, d.jds, d.jde, d.kds, d.kde
+    return d
+
+
+def make_config(spec: DomainSpec) -> Config:
+    """Namelist/Registry defaults (Registry/Registry.EM_COMMON:2840-2902) with physics,
+    diffusion and damping off, as the benchmark configurations state."""
+    c = Config()
+    per = spec.bc == "periodic"
+    c.periodic_x = c.periodic_y = int(per)
+    c.open_xs = c.open_xe = c.open_ys = c.open_ye = int(spec.bc == "open")
+    c.rk_ord = 3
+    c.h_mom_adv_order, c.v_mom_adv_order = 5, 3
+    c.h_sca_adv_order, c.v_sca_adv_order = 5, 3
+    c.moist_adv_opt = c.scalar_adv_opt = 0
+    c.momentum_adv_opt = 1
+    c.non_hydrostatic = 1
+    c.top_lid = 0
+    c.phi_adv_z = 1
+    c.hypsometric_opt = 2
+    c.use_theta_m = 1
+    c.damp_opt = 0
+    c.w_damping = 0
+    c.diff_opt = 0
+    c.diff_6th_opt = 0
+    c.zadvect_implicit = 0
+    c.pert_coriolis = 0
+    c.map_proj = 0
+    c.time_step_sound = spec.time_step_sound
+    c.time_step = int(round(spec.dt))
+    c.epssm, c.smdiv, c.emdiv = 0.1, 0.1, 0.01
+    c.dampcoef, c.zdamp, c.w_crit_cfl = 0.2, 5000.0, 1.2
+    c.precision_mode = spec.precision_mode
+    return c
+
+
+def shape3(d: Dims):
+    return (d.jme - d.jms + 1, d.kme - d.kms + 1, d.ime - d.ims + 1)
+
+
+def shape2(d: Dims):
+    return (d.jme - d.jms + 1, d.ime - d.ims + 1)
+
+
+def shape1(d: Dims):
+    return (d.kme - d.kms + 1,)
+
+
+# ----------------------------------------------------------------------------- sounding
+def wk_theta(z):
+    """Weisman-Klemp (1982) potential temperature: 300 K at the ground, 343 K at 12 km,
+    isothermal (213 K) stratosphere above."""
+    z = np.asarray(z, dtype=np.float64)
+    ztr, th0, thtr, ttr = 12000.0, 300.0, 343.0, 213.0
+    lo = th0 + (thtr - th0) * (np.clip(z, 0, ztr) / ztr) ** 1.25
+    hi = thtr * np.exp(G * (z - ztr) / (CP * ttr))
+    return np.where(z <= ztr, lo, hi)
+
+
+def wk_rh(z):
+    z = np.asarray(z, dtype=np.float64)
+    ztr = 12000.0
+    return np.where(z <= ztr, 1.0 - 0.75 * (np.clip(z, 0, ztr) / ztr) ** 1.25, 0.25)
+
+
+def _qvs(t, p):
+    es = 611.2 * np.exp(17.67 * (t - 273.15) / (t - 29.65))
+    return 0.622 * es / np.maximum(p - es, 1.0)
+
+
+def sounding(moist: bool, zmax=30000.0, dz=5.0, psfc=1.0e5, qv_max=0.014):
+    """Hydrostatic integration of the analytic sounding on a fine z grid.
+    Returns z, p (full), pd (dry hydrostatic), theta, qv."""
+    z = np.arange(0.0, zmax + dz, dz)
+    th = wk_theta(z)
+    rh = wk_rh(z) if moist else np.zeros_like(z)
+    n = z.size
+    p = np.empty(n)
+    qv = np.zeros(n)
+    rho = np.empty(n)
+    p[0] = psfc
+    for it in range(n):
+        if it > 0:
+            # trapezoidal step with a fixed-point iteration (cf. get_sounding's 10 iterations)
+            pg = p[it - 1] - G * rho[it - 1] * dz
+            for _ in range(10):
+                t = th[it] * (pg / P1000) ** RCP
+                q = min(rh[it] * _qvs(t, pg), qv_max) if moist else 0.0
+                rh_ = pg / (R_D * t * (1.0 + RVOVRD * q) / (1.0 + q))
+                pg = p[it - 1] - 0.5 * G * (rho[it - 1] + rh_) * dz
+            p[it] = pg
+        t = th[it] * (p[it] / P1000) ** RCP
+        qv[it] = min(rh[it] * _qvs(t, p[it]), qv_max) if moist else 0.0
+        rho[it] = p[it] / (R_D * t * (1.0 + RVOVRD * qv[it]) / (1.0 + qv[it]))
+    # dry hydrostatic pressure: remove the weight of the vapour above each level
+    wv = G * rho * qv / (1.0 + qv)
+    col = np.concatenate([np.cumsum((0.5 * (wv[1:] + wv[:-1]) * dz)[::-1])[::-1], [0.0]])
+    pd = p - col
+    return z, p, pd, th, qv
+
+
+def hodograph(z):
+    """Quarter-circle hodograph: radius 8 m/s to 2 km, then straight to 32 m/s at 6 km."""
+    z = np.asarray(z, dtype=np.float64)
+    ang = 0.5 * np.pi * np.clip(z, 0, 2000.0) / 2000.0
+    u = 8.0 * (1.0 - np.cos(ang))
+    v = 8.0 * np.sin(ang)
+    up = 8.0 + (32.0 - 8.0) * np.clip((z - 2000.0) / 4000.0, 0.0, 1.0)
+    u = np.where(z <= 2000.0, u, up)
+    v = np.where(z <= 2000.0, v, 8.0)
+    return u, v
+
+
+def _interp_p(var, pin, plev):
+    """interp_0 by pressure (pressure decreases with height)."""
+    return np.interp(-np.asarray(plev), -pin, var)
+
+
+# ----------------------------------------------------------------------------- state
+class HostState:
+    """Host-side arrays of one domain (numpy float32, Fortran layout)."""
+
+    def __init__(self, spec: DomainSpec):
+        self.spec = spec
+        self.dims = make_dims(spec)
+        self.cfg = make_config(spec)
+        d = self.dims
+        self.a3 = {n: np.zeros(shape3(d), np.float32) for n in FIELDS_3D}
+        self.a2 = {n: np.zeros(shape2(d), np.float32) for n in FIELDS_2D}
+        self.a1 = {n: np.zeros(shape1(d), np.float32) for n in FIELDS_1D}
+        self.scal = {}
+        self.n_moist = 1
+        self.n_species = 1 + spec.n_passive
+        if self.n_species > MAX_SPECIES:
+            raise ValueError("too many species")
+        self.q = [np.zeros(shape3(d), np.float32) for _ in range(self.n_species)]
+        self.q_old = [np.zeros(shape3(d), np.float32) for _ in range(self.n_species)]
+        self.q_tend = [np.zeros(shape3(d), np.float32) for _ in range(self.n_species)]
+        self.coef_dtype = np.float64 if spec.precision_mode == 1 else np.float32
+        self.a = np.zeros(shape3(d), self.coef_dtype)
+        self.alpha = np.zeros(shape3(d), self.coef_dtype)
+        self.gamma = np.zeros(shape3(d), self.coef_dtype)
+
+    # all arrays, name -> ndarray (species as q0.., q_old0.., q_tend0..)
+    def arrays(self):
+        out = {}
+        out.update(self.a3)
+        out.update(self.a2)
+        out.update(self.a1)
+        for m in range(self.n_species):
+            out[f"q{m}"] = self.q[m]
+            out[f"q_old{m}"] = self.q_old[m]
+            out[f"q_tend{m}"] = self.q_tend[m]
+        out["a"], out["alpha"], out["gamma"] = self.a, self.alpha, self.gamma
+        return out
+
+    def copy(self):
+        other = HostState.__new__(HostState)
+        other.spec, other.dims, other.cfg = self.spec, self.dims, self.cfg
+        other.a3 = {k: v.copy() for k, v in self.a3.items()}
+        other.a2 = {k: v.copy() for k, v in self.a2.items()}
+        other.a1 = {k: v.copy() for k, v in self.a1.items()}
+        other.scal = dict(self.scal)
+        other.n_moist, other.n_species = self.n_moist, self.n_species
+        other.q = [x.copy() for x in self.q]
+        other.q_old = [x.copy() for x in self.q_old]
+        other.q_tend = [x.copy() for x in self.q_tend]
+        other.coef_dtype = self.coef_dtype
+        other.a, other.alpha, other.gamma = self.a.copy(), self.alpha.copy(), self.gamma.copy()
+        return other
+
+
+def _halo_fill_np(a: np.ndarray, d: Dims, periodic: bool, stag_i: bool, stag_j: bool):
+    """Initial halo fill (periodic wrap or zero-gradient copy) of a (nj, [nk,] ni) array."""
+    nx, ny = d.ide - d.ids, d.jde - d.jds
+    i_idx = np.arange(d.ims, d.ime + 1)
+    j_idx = np.arange(d.jms, d.jme + 1)
+    if periodic:
+        si = np.where(i_idx < d.ids, i_idx + nx, np.where(i_idx > d.ide - 1 + (1 if stag_i else 0), i_idx - nx, i_idx))
+        if stag_i:
+            si = np.where(i_idx == d.ide, d.ids, si)
+        sj = np.where(j_idx < d.jds, j_idx + ny, np.where(j_idx > d.jde - 1 + (1 if stag_j else 0), j_idx - ny, j_idx))
+        if stag_j:
+            sj = np.where(j_idx == d.jde, d.jds, sj)
+    else:
+        si = np.clip(i_idx, d.ids, d.ide if stag_i else d.ide - 1)
+        sj = np.clip(j_idx, d.jds, d.jde if stag_j else d.jde - 1)
+    si = np.clip(si, d.ims, d.ime) - d.ims
+    sj = np.clip(sj, d.jms, d.jme) - d.jms
+    if a.ndim == 3:
+        a[...] = a[sj][:, :, si]
+    else:
+        a[...] = a[sj][:, si]
+
+
+def build_initial_state(spec: DomainSpec) -> HostState:
+    """Analytic supercell-type initial state (see the module docstring)."""
+    hs = HostState(spec)
+    d = hs.dims
+    kde = d.kde
+    nz = spec.nz
+    rng = np.random.Generator(np.random.Philox(spec.seed))
+
+    # eta levels and metrics (module_initialize_ideal.F:617-730), float64
+    znw = 1.0 - np.arange(kde, dtype=np.float64) / (kde - 1)  # index k-1
+    dnw = znw[1:] - znw[:-1]
+    rdnw = 1.0 / dnw
+    znu = 0.5 * (znw[1:] + znw[:-1])
+    dn = np.zeros(kde)
+    rdn = np.zeros(kde)
+    fnp = np.zeros(kde)
+    fnm = np.zeros(kde)
+    for k in range(2, kde):  # Fortran k = 2..kde-1 -> index k-1
+        dn[k - 1] = 0.5 * (dnw[k - 1] + dnw[k - 2])
+        rdn[k - 1] = 1.0 / dn[k - 1]
+        fnp[k - 1] = 0.5 * dnw[k - 1] / dn[k - 1]
+        fnm[k - 1] = 0.5 * dnw[k - 2] / dn[k - 1]
+    cof1 = (2.0 * dn[1] + dn[2]) / (dn[1] + dn[2]) * dnw[0] / dn[1]
+    cof2 = dn[1] / (dn[1] + dn[2]) * dnw[0] / dn[2]
+    cf1 = fnp[1] + cof1
+    cf2 = fnm[1] - cof1 - cof2
+    cf3 = cof2
+    cfn = (0.5 * dnw[kde - 2] + dn[kde - 2]) / dn[kde - 2]
+    cfn1 = -0.5 * dnw[kde - 2] / dn[kde - 2]
+
+    # soundings
+    zs, pdry, _, thdry, _ = sounding(moist=False)
+    zm, pm, pdm, thm, qvm = sounding(moist=True)
+    p_top = float(np.interp(spec.ztop, zs, pdry))
+
+    # hybrid coefficients (module_initialize_ideal.F:791-877)
+    if spec.hybrid_opt == 0:
+        c3f = znw.copy()
+    elif spec.hybrid_opt == 2:
+        e = spec.etac
+        b1 = 2.0 * e ** 2 * (1.0 - e)
+        b2 = -e * (4.0 - 3.0 * e - e ** 3)
+        b3 = 2.0 * (1.0 - e ** 3)
+        b4 = -(1.0 - e ** 2)
+        b5 = (1.0 - e) ** 4
+        c3f = (b1 + b2 * znw + b3 * znw ** 2 + b4 * znw ** 3) / b5
+        c3f = np.where(znw < e, 0.0, c3f)
+        c3f[0], c3f[-1] = 1.0, 0.0
+    else:
+        raise ValueError("hybrid_opt must be 0 or 2")
+    c4f = (znw - c3f) * (P1000 - p_top)
+    c3h = (c3f[1:] + c3f[:-1]) * 0.5
+    c4h = (znu - c3h) * (P1000 - p_top)
+    c1f = np.zeros(kde)
+    for k in range(2, kde):
+        c1f[k - 1] = (c3h[k - 1] - c3h[k - 2]) / (znu[k - 1] - znu[k - 2])
+    c1f[0] = 1.0
+    c1f[kde - 1] = 1.0 if spec.hybrid_opt in (0, 1) else 0.0
+    c2f = (1.0 - c1f) * (P1000 - p_top)
+    c1h = (c3f[1:] - c3f[:-1]) / (znw[1:] - znw[:-1])
+    c2h = (1.0 - c1h) * (P1000 - p_top)
+    if spec.hybrid_opt == 0:
+        c3h = znu.copy()
+        c4h[:] = 0.0
+        c1f[:] = 1.0
+        c2f[:] = 0.0
+        c1h[:] = 1.0
+        c2h[:] = 0.0
+
+    def put1(name, vals):  # vals indexed k-1 for k = 1..len
+        arr = hs.a1[name]
+        arr[: len(vals)] = np.asarray(vals, dtype=np.float64).astype(np.float32)
+
+    put1("znw", znw); put1("dnw", dnw); put1("rdnw", rdnw); put1("znu", znu)
+    put1("dn", dn); put1("rdn", rdn); put1("fnp", fnp); put1("fnm", fnm)
+    put1("c1h", c1h); put1("c2h", c2h); put1("c3h", c3h); put1("c4h", c4h)
+    put1("c1f", c1f); put1("c2f", c2f); put1("c3f", c3f); put1("c4f", c4f)
+    hs.scal.update(dict(dt=spec.dt, dx=spec.dx, dy=spec.dy, rdx=1.0 / spec.dx, rdy=1.0 / spec.dy,
+                        cf1=cf1, cf2=cf2, cf3=cf3, cfn=cfn, cfn1=cfn1, p_top=p_top))
+
+    # base state: flat terrain (ht = 0, phb(1) = 0), dry sounding
+    p_surf = float(np.interp(0.0, zs, pdry))
+    mub = p_surf - p_top
+    pb = c3h * (p_surf - p_top) + c4h + p_top
+    t_init = _interp_p(thdry, pdry, pb) - T0
+    alb = (R_D / P1000) * (t_init + T0) * (pb / P1000) ** CVPM
+    phb = np.zeros(kde)
+    for kk in range(2, kde + 1):
+        k = kk - 1
+        phb[kk - 1] = phb[kk - 2] - dnw[kk - 2] * (c1h[k - 1] * mub + c2h[k - 1]) * alb[kk - 2]
+
+    # moist full state, horizontally uniform column
+    pd_surf = float(np.interp(0.0, zm, pdm))
+    mu1 = pd_surf - p_top - mub
+    plev = c3h * (pd_surf - p_top) + c4h + p_top
+    qv = _interp_p(qvm, pdm, plev)
+    th = _interp_p(thm, pdm, plev) - T0
+
+    ny_m, nx_m = spec.ny, spec.nx
+    # 3-D working arrays over mass points, shape (ny, nz, nx), float64
+    T = np.broadcast_to(th[None, :, None], (ny_m, nz, nx_m)).copy()
+    QV = np.broadcast_to(qv[None, :, None], (ny_m, nz, nx_m)).copy()
+
+    def hydro_column(T, QV):
+        """p, alt, al, ph_1 from theta and qv (module_initialize_ideal.F:1000-1085)."""
+        p = np.zeros_like(T)
+        kk = nz - 1
+        k = kk + 1  # Fortran kk = kte-1 (top mass level), k = kk + 1
+        # top layer: the same qv average as pg_buoy_w's cqw(kde-1) so that the model's
+        # top-level vertical pressure gradient balances exactly (the reference's ideal
+        # init uses qv(kde-1) alone, module_initialize_ideal.F:1033)
+        qvf1 = 0.5 * (QV[:, kk, :] + QV[:, kk - 1, :])
+        qvf2 = 1.0 / (1.0 + qvf1)
+        qvf1 = qvf1 * qvf2
+        p[:, kk, :] = -0.5 * ((c1f[k] * mu1) + qvf1 * (c1f[k] * mub + c2f[k])) / rdnw[kk] / qvf2
+        for kk in range(nz - 2, -1, -1):
+            k = kk + 1
+            qvf1 = 0.5 * (QV[:, kk, :] + QV[:, kk + 1, :])
+            qvf2 = 1.0 / (1.0 + qvf1)
+            qvf1 = qvf1 * qvf2
+            p[:, kk, :] = p[:, kk + 1, :] - ((c1f[k] * mu1) + qvf1 * (c1f[k] * mub + c2f[k])) / qvf2 / rdn[kk + 1]
+        qvf = 1.0 + RVOVRD * QV
+        alt = (R_D / P1000) * (T + T0) * qvf * ((p + pb[None, :, None]) / P1000) ** CVPM
+        al = alt - alb[None, :, None]
+        # geopotential from the log-pressure hypsometric relation that the model's
+        # equation of state inverts (calc_p_rho_phi, hypsometric_opt = 2), so that the
+        # start-of-run diagnosis of al and p (start_em.F:794-830) returns this state
+        muts = mub + mu1
+        pf = c3f * muts + c4f + p_top
+        phm = c3h * muts + c4h + p_top
+        dlog = np.log(pf[:-1] / pf[1:])
+        ph_tot = np.zeros((ny_m, nz + 1, nx_m))
+        for k in range(nz):
+            ph_tot[:, k + 1, :] = ph_tot[:, k, :] + alt[:, k, :] * phm[k] * dlog[k]
+        ph = ph_tot - phb[None, :, None]
+        al = (ph_tot[:, 1:, :] - ph_tot[:, :-1, :]) / phm[None, :, None] / dlog[None, :, None] - alb[None, :, None]
+        alt = al + alb[None, :, None]
+        p = P1000 * ((R_D * (T0 + T * (1.0 + RVOVRD * QV) + T0 * RVOVRD * QV)) / (P1000 * alt)) ** (CP / CV) - pb[None, :, None]
+        return p, alt, al, ph
+
+    p, alt, al, ph = hydro_column(T, QV)
+
+    # warm bubble(s): delt*cos^2(pi/2*rad), 10 km / 1.5 km radii, centred at z = 1.5 km
+    zmid = 0.5 * (ph[:, :-1, :] + ph[:, 1:, :] + phb[None, :-1, None] + phb[None, 1:, None]) / G
+    xs = (np.arange(nx_m) + 0.5) * spec.dx
+    ys = (np.arange(ny_m) + 0.5) * spec.dy
+    if spec.n_bubbles == 1:
+        centres = [(0.5 * nx_m * spec.dx, 0.5 * ny_m * spec.dy, spec.bubble_amp)]
+    else:
+        centres = [(rng.uniform(0, nx_m * spec.dx), rng.uniform(0, ny_m * spec.dy), spec.bubble_amp)
+                   for _ in range(spec.n_bubbles)]
+    Lx, Ly = nx_m * spec.dx, ny_m * spec.dy
+    for (xc, yc, amp) in centres:
+        dxr = xs - xc
+        dyr = ys - yc
+        if spec.bc == "periodic":
+            dxr = (dxr + 0.5 * Lx) % Lx - 0.5 * Lx
+            dyr = (dyr + 0.5 * Ly) % Ly - 0.5 * Ly
+        xr = dxr / 10000.0
+        yr = dyr / 10000.0
+        zr = (zmid - 1500.0) / 1500.0
+        rad = np.sqrt(xr[None, None, :] ** 2 + yr[:, None, None] ** 2 + zr ** 2)
+        T = T + np.where(rad <= 1.0, amp * np.cos(0.5 * np.pi * rad) ** 2, 0.0)
+    # seeded +/- noise on the lowest levels
+    if spec.noise_amp > 0:
+        nl = min(spec.noise_levels, nz)
+        T[:, :nl, :] += rng.uniform(-spec.noise_amp, spec.noise_amp, size=(ny_m, nl, nx_m))
+    p, alt, al, ph = hydro_column(T, QV)
+    # moist potential temperature (use_theta_m = 1)
+    Tm = (T + T0) * (1.0 + RVOVRD * QV) - T0
+
+    # winds from the hodograph at base-state mass-level heights
+    zmass = 0.5 * (phb[:-1] + phb[1:]) / G
+    uz, vz = hodograph(zmass)
+
+    # place into memory arrays
+    js, is_ = slice(d.jds - d.jms, d.jds - d.jms + ny_m), slice(d.ids - d.ims, d.ids - d.ims + nx_m)
+    ks = slice(0, nz)
+    A3, A2 = hs.a3, hs.a2
+
+    def put3_mass(name, v, kslice=ks):
+        A3[name][js, kslice, is_] = v.astype(np.float32)
+
+    put3_mass("t_2", Tm); put3_mass("t_1", Tm)
+    put3_mass("p", p); put3_mass("al", al); put3_mass("alt", alt)
+    put3_mass("ph_2", ph, slice(0, nz + 1)); put3_mass("ph_1", ph, slice(0, nz + 1))
+    A3["phb"][:, : nz + 1, :] = phb.astype(np.float32)[None, :, None]
+    A3["pb"][:, :nz, :] = pb.astype(np.float32)[None, :, None]
+    A3["alb"][:, :nz, :] = alb.astype(np.float32)[None, :, None]
+    A3["u_2"][:, :nz, :] = uz.astype(np.float32)[None, :, None]
+    A3["v_2"][:, :nz, :] = vz.astype(np.float32)[None, :, None]
+    A3["u_1"][...] = A3["u_2"]
+    A3["v_1"][...] = A3["v_2"]
+    A2["mub"][...] = np.float32(mub)
+    A2["mu_2"][...] = np.float32(mu1)
+    A2["mu_1"][...] = np.float32(mu1)
+    for n in ("msfux", "msfuy", "msfvx", "msfvx_inv", "msfvy", "msftx", "msfty", "cosa"):
+        A2[n][...] = 1.0
+    hs.q[0][js, ks, is_] = QV.astype(np.float32)
+    # passive scalars: log-normal per point, median 1e-5 kg/kg, sigma 1.0, clipped at 0
+    for m in range(1, hs.n_species):
+        vals = np.exp(np.log(1.0e-5) + 1.0 * rng.standard_normal((ny_m, nz, nx_m)))
+        hs.q[m][js, ks, is_] = np.maximum(vals, 0.0).astype(np.float32)
+
+    # halo fills of everything that carries a halo
+    per = spec.bc == "periodic"
+    stag = {"u_1": (1, 0), "u_2": (1, 0), "v_1": (0, 1), "v_2": (0, 1)}
+    for n in ("u_1", "u_2", "v_1", "v_2", "w_1", "w_2", "t_1", "t_2", "ph_1", "ph_2", "phb", "pb", "alb",
+              "p", "al", "alt"):
+        si, sj = stag.get(n, (0, 0))
+        _halo_fill_np(A3[n], d, per, bool(si), bool(sj))
+    for m in range(hs.n_species):
+        _halo_fill_np(hs.q[m], d, per, False, False)
+        hs.q_old[m][...] = hs.q[m]
+    for n in ("mub", "mu_1", "mu_2"):
+        _halo_fill_np(A2[n], d, per, False, False)
+    return hs
+
+
+# ----------------------------------------------------------------------------- presets
+def supercell_c0(**kw) -> DomainSpec:
+    """41x41x40 mass points, dx=2 km, dt=12 s, 6 sound steps, open boundaries (seed 1001)."""
+    base = dict(nx=41, ny=41, nz=40, dx=2000.0, dy=2000.0, dt=12.0, ztop=20000.0, time_step_sound=6,
+                bc="open", n_bubbles=1, bubble_amp=3.0, noise_amp=0.1, seed=1001)
+    base.update(kw)
+    return DomainSpec(**base)
+
+
+def large_c1(**kw) -> DomainSpec:
+    """400x400x80, dx=1 km, dt=6 s, doubly periodic, 6 bubbles, qv + 6 passive scalars (seed 1002)."""
+    base = dict(nx=400, ny=400, nz=80, dx=1000.0, dy=1000.0, dt=6.0, ztop=20000.0, time_step_sound=6,
+                bc="periodic", n_bubbles=6, bubble_amp=3.0, noise_amp=0.1, n_passive=6, seed=1002)
+    base.update(kw)
+    return DomainSpec(**base)
