@@ -132,6 +132,61 @@ int wrfmi_solve_em_step(wrfmi_state *st, const wrfmi_config *cfg,
 int wrfmi_solve_em_step_graph(wrfmi_state *st, const wrfmi_config *cfg,
                               const wrfmi_dims *d, void *stream, int sync);
 
+/* ---- per-routine entry points -------------------------------------------
+ * The calls solve_em makes inside one RK substep, each with the halo fills the
+ * reference makes right after it.  Each enqueues on `stream` and returns; the
+ * whole step above is exactly this sequence.  rk_step is 1..3; dts is the
+ * acoustic step of the substep (dt/3 at rk_step 1, dt/time_step_sound after). */
+
+/* rk_step_prep (dyn_em/module_em.F:37-186, called at solve_em.F:657),
+ * rk_phys_bc_dry_1 (module_bc_em.F:960-1041, called at solve_em.F:726) and
+ * the al, ph_2 fills of solve_em.F:760-782. */
+int wrfmi_rk_step_prep(wrfmi_state *st, const wrfmi_config *cfg, const wrfmi_dims *d,
+                       int rk_step, void *stream);
+/* rk_tendency (module_em.F:190-955; advect_u/v/w, advect_scalar(theta), rhs_ph,
+ * horizontal_pressure_gradient, pg_buoy_w, w_damp, coriolis, curvature) and
+ * rk_addtend_dry (module_em.F:959-1092), called at solve_em.F:881 and 967. */
+int wrfmi_rk_tendency(wrfmi_state *st, const wrfmi_config *cfg, const wrfmi_dims *d,
+                      int rk_step, void *stream);
+/* small_step_prep (module_small_step_em.F:16-290), calc_p_rho step 0
+ * (438-568), calc_coef_w (570-652), called at solve_em.F:1089-1124, and the
+ * fills of solve_em.F:1181-1264. */
+int wrfmi_small_step_prep(wrfmi_state *st, const wrfmi_config *cfg, const wrfmi_dims *d,
+                          int rk_step, float dts, void *stream);
+/* advance_uv (module_small_step_em.F:654-967), called at solve_em.F:1279. */
+int wrfmi_advance_uv(wrfmi_state *st, const wrfmi_config *cfg, const wrfmi_dims *d,
+                     float dts, void *stream);
+/* One column pass: advance_mu_t (969-1175), advance_w (1178-1469), sumflux
+ * (1473-1633), calc_p_rho (438-568) for acoustic substep `iteration` of
+ * `n_small` (called at solve_em.F:1392, 1499, 1566, 1623), and the fills
+ * that follow them (solve_em.F:1640-1719). */
+int wrfmi_advance_mu_t_w(wrfmi_state *st, const wrfmi_config *cfg, const wrfmi_dims *d,
+                         float dts, int iteration, int n_small, void *stream);
+/* calc_mu_uv_1 (module_big_step_utilities_em.F:184-321), small_step_finish
+ * (module_small_step_em.F:295-434), and at rk_step 3 the ru_m, rv_m, ww_m,
+ * mut, muts fills (called at solve_em.F:1732-1800). */
+int wrfmi_small_step_finish(wrfmi_state *st, const wrfmi_config *cfg, const wrfmi_dims *d,
+                            int rk_step, int n_small, float dts, void *stream);
+/* rk_scalar_tend (module_em.F:1096-1442, advect_scalar with ru_m, rv_m, ww_m)
+ * followed by rk_update_scalar (1587-1799) for species `ispec`
+ * (called at solve_em.F:2215-2309 and 2781-2856). */
+int wrfmi_rk_scalar_step(wrfmi_state *st, const wrfmi_config *cfg, const wrfmi_dims *d,
+                         int ispec, int rk_step, float dt_rk, void *stream);
+/* calc_p_rho_phi (module_big_step_utilities_em.F:953-1223), called at solve_em.F:2965. */
+int wrfmi_calc_p_rho_phi(wrfmi_state *st, const wrfmi_config *cfg, const wrfmi_dims *d,
+                         void *stream);
+/* rk_phys_bc_dry_2 (module_bc_em.F:1045-1104, called at solve_em.F:3245) plus
+ * the species fills that follow it (kind 0), or set_phys_bc_dry_2 and the
+ * species fills at the end of the step (solve_em.F:4461-4540, kind 1). */
+int wrfmi_phys_bc_dry_2(wrfmi_state *st, const wrfmi_config *cfg, const wrfmi_dims *d,
+                        int kind, void *stream);
+/* set_physical_bc3d / set_physical_bc2d (share/module_bc.F:651-1113, 202-647)
+ * on one field; var is the reference's staggering letter 'u','v','w','t','p'. */
+int wrfmi_set_physical_bc3d(float *field, char var, const wrfmi_config *cfg,
+                            const wrfmi_dims *d, void *stream);
+int wrfmi_set_physical_bc2d(float *field, char var, const wrfmi_config *cfg,
+                            const wrfmi_dims *d, void *stream);
+
 /* Non-finite scan of the prognostic state (u_2, v_2, w_2, t_2, ph_2, mu_2);
  * returns WRFMI_E_NONFINITE and the count in *n_bad when any is found. */
 int wrfmi_check_finite(const wrfmi_state *st, const wrfmi_dims *d, long long *n_bad);

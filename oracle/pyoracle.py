@@ -1,6 +1,7 @@
 """ctypes front end of the CPU oracle -- TEST INFRASTRUCTURE ONLY.
 
-Loads oracle/liboracle_f32.so (reference-order FP32 restatement) or
+Loads oracle/liboracle_f32.so (reference-order FP32 restatement),
+liboracle_f32cr.so (the same with the EOS's LOG and ** rounded once from double),
 liboracle_f64.so (same source in FP64, the -r8 analogue), and the reference's own
 advect_scalar from oracle/_ref/libref_advect.so when that was built.
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this.
@@ -27,10 +28,12 @@ def _load(name):
 class Oracle:
     def __init__(self, precision: str = "f32"):
         self.precision = precision
-        self.lib = _load("liboracle_f32.so" if precision == "f32" else "liboracle_f64.so")
-        self.dtype = np.float32 if precision == "f32" else np.float64
-        self.ctype = ctypes.c_float if precision == "f32" else ctypes.c_double
-        self.State = State32 if precision == "f32" else State64
+        libs = {"f32": "liboracle_f32.so", "f32cr": "liboracle_f32cr.so", "f64": "liboracle_f64.so"}
+        self.lib = _load(libs[precision])
+        single = precision != "f64"
+        self.dtype = np.float32 if single else np.float64
+        self.ctype = ctypes.c_float if single else ctypes.c_double
+        self.State = State32 if single else State64
         self.lib.oracle_state_sizeof.restype = ctypes.c_size_t
         if self.lib.oracle_state_sizeof() != ctypes.sizeof(self.State):
             raise RuntimeError("oracle state layout does not match wrf_amd.fields")
@@ -56,6 +59,10 @@ class Oracle:
             st.q_tend[m] = arrs[f"q_tend{m}"].ctypes.data_as(P)
         return arrs, st
 
+    def session(self, hs):
+        """A persistent oracle state (copy of `hs`) advanced routine by routine."""
+        return OracleSession(self, hs)
+
     def run(self, hs, nsteps: int = 1):
         """Advance a copy of `hs` by `nsteps` RK3 steps; returns the arrays dict."""
         arrs, st = self.to_native(hs)
@@ -63,6 +70,36 @@ class Oracle:
         if rc != 0:
             raise RuntimeError(f"oracle solve_em step failed with status {rc}")
         return arrs
+
+
+class OracleSession:
+    """Oracle arrays + native struct kept alive between per-routine calls
+    (oracle_<name> in solve_step.c, same names as the wrfmi_<name> entry points)."""
+
+    _ARGS = {"rk_step_prep": "i", "rk_tendency": "i", "small_step_prep": "id", "advance_uv": "d",
+             "advance_mu_t_w": "dii", "small_step_finish": "iid", "rk_scalar_step": "iid",
+             "calc_p_rho_phi": "", "phys_bc_dry_2": "i"}
+
+    def __init__(self, oracle: Oracle, hs):
+        self.o = oracle
+        self.cfg, self.dims = hs.cfg, hs.dims
+        self.arrays, self.st = oracle.to_native(hs)
+        for name, kinds in self._ARGS.items():
+            fn = getattr(oracle.lib, "oracle_" + name)
+            fn.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_int if k == "i" else ctypes.c_double for k in kinds]
+            fn.restype = ctypes.c_int
+
+    def phase(self, name: str, *args):
+        rc = getattr(self.o.lib, "oracle_" + name)(ctypes.byref(self.st), ctypes.byref(self.cfg),
+                                                   ctypes.byref(self.dims), *args)
+        if rc != 0:
+            raise RuntimeError(f"oracle {name} failed with status {rc}")
+
+    def step(self, nsteps: int = 1):
+        rc = self.o.lib.oracle_solve_em_steps(ctypes.byref(self.st), ctypes.byref(self.cfg), ctypes.byref(self.dims),
+                                              nsteps)
+        if rc != 0:
+            raise RuntimeError(f"oracle solve_em step failed with status {rc}")
 
 
 class RefAdvect:

@@ -41,6 +41,20 @@ void o_rk_update_scalar(ostate *s, const dims_t *d, int m, int rk_step, wreal dt
     }
 }
 
+/* LOG and ** of calc_p_rho_phi.  Default: the libm routine of the working precision, as a
+ * Fortran compiler calls it (logf/powf for default REAL).  With EOS_CORRECTLY_ROUNDED the FP32
+ * build instead evaluates them in double and rounds once, which is what the GPU kernels do. */
+#if WREAL_IS_DOUBLE
+#define EOS_LOG(x) log(x)
+#define EOS_POW(x, y) pow(x, y)
+#elif defined(EOS_CORRECTLY_ROUNDED)
+#define EOS_LOG(x) ((float)log((double)(x)))
+#define EOS_POW(x, y) ((float)pow((double)(x), (double)(y)))
+#else
+#define EOS_LOG(x) logf(x)
+#define EOS_POW(x, y) powf(x, y)
+#endif
+
 void o_calc_p_rho_phi(ostate *s, const cfg_t *c, const dims_t *d) {
   const int ide = d->ide, jde = d->jde, kde = d->kde;
   const int its = d->its, ite = d->ite, jts = d->jts, jte = d->jte, kts = d->kts, kte = d->kte;
@@ -57,11 +71,7 @@ void o_calc_p_rho_phi(ostate *s, const cfg_t *c, const dims_t *d) {
           wreal pfu = A1(s->c3f, k + 1) * A2(muts, i, j) + A1(s->c4f, k + 1) + ptop;
           wreal pfd = A1(s->c3f, k) * A2(muts, i, j) + A1(s->c4f, k) + ptop;
           wreal phm = A1(s->c3h, k) * A2(muts, i, j) + A1(s->c4h, k) + ptop;
-#if WREAL_IS_DOUBLE
-          wreal lg = log(pfd / pfu);
-#else
-          wreal lg = logf(pfd / pfu);
-#endif
+          wreal lg = EOS_LOG(pfd / pfu);
           A3(al, i, k, j) = (A3(ph, i, k + 1, j) - A3(ph, i, k, j) + A3(phb, i, k + 1, j) - A3(phb, i, k, j)) / phm / lg - A3(alb, i, k, j);
         }
       }
@@ -76,18 +86,10 @@ void o_calc_p_rho_phi(ostate *s, const cfg_t *c, const dims_t *d) {
             wreal qvf = R(1.) + K_RVOVRD * A3(s->q[0], i, k, j);
             temp = (K_RD * (K_T0 + A3(t, i, k, j)) * qvf) / (K_P0 * (A3(al, i, k, j) + A3(alb, i, k, j)));
           }
-#if WREAL_IS_DOUBLE
-          wreal pw = pow(temp, K_CPOVCV);
-#else
-          wreal pw = powf(temp, K_CPOVCV);
-#endif
+          wreal pw = EOS_POW(temp, K_CPOVCV);
           A3(p, i, k, j) = pw * K_P0 - A3(pb, i, k, j);
         } else {
-#if WREAL_IS_DOUBLE
-          A3(p, i, k, j) = K_P0 * pow((K_RD * (K_T0 + A3(t, i, k, j))) / (K_P0 * (A3(al, i, k, j) + A3(alb, i, k, j))), K_CPOVCV) - A3(pb, i, k, j);
-#else
-          A3(p, i, k, j) = K_P0 * powf((K_RD * (K_T0 + A3(t, i, k, j))) / (K_P0 * (A3(al, i, k, j) + A3(alb, i, k, j))), K_CPOVCV) - A3(pb, i, k, j);
-#endif
+          A3(p, i, k, j) = K_P0 * EOS_POW((K_RD * (K_T0 + A3(t, i, k, j))) / (K_P0 * (A3(al, i, k, j) + A3(alb, i, k, j))), K_CPOVCV) - A3(pb, i, k, j);
         }
       }
 }

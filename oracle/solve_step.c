@@ -137,3 +137,99 @@ int oracle_solve_em_steps(ostate *s, const cfg_t *c, const dims_t *d, int nsteps
   }
   return 0;
 }
+
+/* Per-routine groups, in the order and with the halo fills of o_solve_em_step, for the
+ * phase-by-phase parity tests of the GPU entry points of the same names. */
+int oracle_rk_step_prep(ostate *s, const cfg_t *c, const dims_t *d, int rk_step) {
+  o_rk_step_prep(s, c, d, rk_step);
+  o_rk_phys_bc_dry_1(s, c, d);
+  o_set_physical_bc3d(s->al, 'p', c, d);
+  o_set_physical_bc3d(s->ph_2, 'w', c, d);
+  return 0;
+}
+int oracle_rk_tendency(ostate *s, const cfg_t *c, const dims_t *d, int rk_step) {
+  int rc = o_rk_tendency(s, c, d, rk_step);
+  if (rc) return rc;
+  o_rk_addtend_dry(s, d, rk_step);
+  return 0;
+}
+int oracle_small_step_prep(ostate *s, const cfg_t *c, const dims_t *d, int rk_step, double dts) {
+  o_small_step_prep(s, d, rk_step);
+  o_calc_p_rho(s, c, d, 0);
+  if (c->non_hydrostatic) o_calc_coef_w(s, c, d, (wreal)dts);
+  o_set_physical_bc3d(s->ru_tend, 'u', c, d);
+  o_set_physical_bc3d(s->rv_tend, 'v', c, d);
+  o_set_physical_bc3d(s->ph_2, 'w', c, d);
+  o_set_physical_bc3d(s->al, 'p', c, d);
+  o_set_physical_bc3d(s->p, 'p', c, d);
+  o_set_physical_bc3d(s->t_1, 'p', c, d);
+  o_set_physical_bc3d(s->t_save, 't', c, d);
+  o_set_physical_bc2d(s->mu_1, 't', c, d);
+  o_set_physical_bc2d(s->mu_2, 't', c, d);
+  o_set_physical_bc2d(s->mudf, 't', c, d);
+  return 0;
+}
+int oracle_advance_uv(ostate *s, const cfg_t *c, const dims_t *d, double dts) {
+  o_advance_uv(s, c, d, (wreal)dts);
+  return 0;
+}
+int oracle_advance_mu_t_w(ostate *s, const cfg_t *c, const dims_t *d, double dts, int iteration, int n_small) {
+  o_advance_mu_t(s, c, d, (wreal)dts);
+  if (c->non_hydrostatic) o_advance_w(s, c, d, (wreal)dts);
+  o_sumflux(s, d, iteration, n_small);
+  o_calc_p_rho(s, c, d, iteration);
+  o_set_physical_bc3d(s->ph_2, 'w', c, d);
+  o_set_physical_bc3d(s->al, 'p', c, d);
+  o_set_physical_bc3d(s->p, 'p', c, d);
+  o_set_physical_bc2d(s->muts, 't', c, d);
+  o_set_physical_bc2d(s->mu_2, 't', c, d);
+  o_set_physical_bc2d(s->mudf, 't', c, d);
+  return 0;
+}
+int oracle_small_step_finish(ostate *s, const cfg_t *c, const dims_t *d, int rk_step, int n_small, double dts) {
+  o_calc_mu_uv_1(s, c, d);
+  o_small_step_finish(s, d, n_small, (wreal)dts, rk_step, 3);
+  if (rk_step == 3) {
+    o_set_physical_bc3d(s->ru_m, 'u', c, d);
+    o_set_physical_bc3d(s->rv_m, 'v', c, d);
+    o_set_physical_bc3d(s->ww_m, 'w', c, d);
+    o_set_physical_bc2d(s->mut, 't', c, d);
+    o_set_physical_bc2d(s->muts, 't', c, d);
+  }
+  return 0;
+}
+int oracle_rk_scalar_step(ostate *s, const cfg_t *c, const dims_t *d, int ispec, int rk_step, double dt_rk) {
+  int rc = o_rk_scalar_tend(s, c, d, ispec);
+  if (rc) return rc;
+  o_rk_update_scalar(s, d, ispec, rk_step, (wreal)dt_rk);
+  return 0;
+}
+int oracle_calc_p_rho_phi(ostate *s, const cfg_t *c, const dims_t *d) {
+  o_calc_p_rho_phi(s, c, d);
+  return 0;
+}
+int oracle_phys_bc_dry_2(ostate *s, const cfg_t *c, const dims_t *d, int kind) {
+  if (kind == 0) {
+    o_set_physical_bc3d(s->u_2, 'u', c, d);
+    o_set_physical_bc3d(s->v_2, 'v', c, d);
+    o_set_physical_bc3d(s->w_2, 'w', c, d);
+    o_set_physical_bc3d(s->t_2, 'p', c, d);
+    o_set_physical_bc3d(s->ph_2, 'w', c, d);
+    o_set_physical_bc2d(s->mu_2, 't', c, d);
+  } else {
+    o_set_physical_bc3d(s->u_1, 'u', c, d);
+    o_set_physical_bc3d(s->u_2, 'u', c, d);
+    o_set_physical_bc3d(s->v_1, 'v', c, d);
+    o_set_physical_bc3d(s->v_2, 'v', c, d);
+    o_set_physical_bc3d(s->w_1, 'w', c, d);
+    o_set_physical_bc3d(s->w_2, 'w', c, d);
+    o_set_physical_bc3d(s->t_1, 'p', c, d);
+    o_set_physical_bc3d(s->t_2, 'p', c, d);
+    o_set_physical_bc3d(s->ph_1, 'w', c, d);
+    o_set_physical_bc3d(s->ph_2, 'w', c, d);
+    o_set_physical_bc2d(s->mu_1, 't', c, d);
+    o_set_physical_bc2d(s->mu_2, 't', c, d);
+  }
+  bc_species(s, c, d);
+  return 0;
+}

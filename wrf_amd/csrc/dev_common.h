@@ -1,0 +1,96 @@
+// Device-side helpers shared by the gfx950 kernels of the WRF-ARW dynamics step.
+//
+// Layout: every 3-D field is the reference's Fortran array (ims:ime, kms:kme, jms:jme),
+// i fastest.  Offsets are 32-bit (the host rejects domains of 2^31 elements or more).
+// All kernels are compiled with -ffp-contract=off and written in the reference's
+// floating-point operation order, so that FP32 results match a reference-order
+// IEEE FP32 evaluation bit for bit (transcendentals aside).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "wrfmi.h"
+
+namespace wrfmi {
+
+struct Dm {
+  int ids, ide, jds, jde, kds, kde;
+  int ims, ime, jms, jme, kms, kme;
+  int ni, nk, sj;  // memory extents in i and k, stride of j (= ni*nk)
+};
+
+struct KArgs {
+  wrfmi_state s;   // device pointers
+  wrfmi_config c;
+  Dm d;
+  float dts;       // small (acoustic) time step of this RK substep
+  float dt_rk;     // RK substep length
+  float dt;        // full time step
+  int rk_step;
+  int iteration;   // acoustic substep number (1-based)
+  int nsmall;      // acoustic substeps in this RK substep
+  int species;     // species index for the scalar kernels
+};
+
+// One field of a batched halo fill (set_physical_bc3d/2d on up to 16 fields per launch)
+struct HaloField {
+  float *p;
+  int kind;  // 0 mass ('p','t'), 1 u-staggered ('u'), 2 v-staggered ('v'), 3 w levels ('w')
+  int is3d;
+};
+struct HaloArgs {
+  HaloField f[16];
+  int n;
+  Dm d;
+  int periodic_x, periodic_y, open_xs, open_xe, open_ys, open_ye;
+};
+
+#define DEVI __device__ __forceinline__
+
+DEVI int X3(const Dm &d, int i, int k, int j) { return (i - d.ims) + d.ni * (k - d.kms) + d.sj * (j - d.jms); }
+DEVI int X2(const Dm &d, int i, int j) { return (i - d.ims) + d.ni * (j - d.jms); }
+DEVI int X1(const Dm &d, int k) { return k - d.kms; }
+
+template <typename T> DEVI T imin(T a, T b) { return a < b ? a : b; }
+template <typename T> DEVI T imax(T a, T b) { return a > b ? a : b; }
+
+// share/module_model_constants.F:17-47, evaluated in FP32 as the reference's REAL parameters
+constexpr float K_G = 9.81f;
+constexpr float K_RD = 287.f;
+constexpr float K_CP = 7.f * 287.f / 2.f;
+constexpr float K_RV = 461.6f;
+constexpr float K_CPOVCV = K_CP / (K_CP - K_RD);
+constexpr float K_RVOVRD = K_RV / K_RD;
+constexpr float K_T0 = 300.f;
+constexpr float K_P0 = 100000.f;
+constexpr float K_RERADIUS = 1.f / 6370.0e03f;
+constexpr float K_CB = 25.f;
+constexpr float K_W_ALPHA = 0.3f;
+constexpr float K_W_BETA = 1.0f;
+
+// Fortran SIGN(1.,x) for REAL x: 1 with the sign bit of x
+DEVI float sgn1(float x) { return __builtin_copysignf(1.f, x); }
+
+// Flux operators of module_advect_em.F:199-213.  ts = SIGN(1,time_step) as a REAL.
+DEVI float flux4(float qm2, float qm1, float q0, float qp1) {
+  return (7.f * (q0 + qm1) - (qp1 + qm2)) / 12.0f;
+}
+DEVI float flux3(float qm2, float qm1, float q0, float qp1, float ua, float ts) {
+  return flux4(qm2, qm1, q0, qp1) + ts * sgn1(ua) * ((qp1 - qm2) - 3.f * (q0 - qm1)) / 12.0f;
+}
+DEVI float flux6(float qm3, float qm2, float qm1, float q0, float qp1, float qp2) {
+  return (37.f * (q0 + qm1) - 8.f * (qp1 + qm2) + (qp2 + qm3)) / 60.0f;
+}
+DEVI float flux5(float qm3, float qm2, float qm1, float q0, float qp1, float qp2, float ua, float ts) {
+  return flux6(qm3, qm2, qm1, q0, qp1, qp2) - ts * sgn1(ua) * ((qp2 - qm3) - 5.f * (qp1 - qm2) + 10.f * (q0 - qm1)) / 60.0f;
+}
+
+// Error-free transformation used for the compensated column sums (precision mode 1)
+DEVI void two_sum_acc(float &s, float &comp, float x) {
+  float t = s + x;
+  float bp = t - s;
+  float e = (s - (t - bp)) + (x - bp);
+  s = t;
+  comp = comp + e;
+}
+
+}  // namespace wrfmi

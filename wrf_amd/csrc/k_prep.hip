@@ -1,0 +1,152 @@
+// rk_step_prep on the GPU: one thread per (i,j) column, marching in k.
+//   calculate_full   dyn_em/module_big_step_utilities_em.F:3587-3636
+//   calc_mu_uv       dyn_em/module_big_step_utilities_em.F:26-180
+//   couple_momentum  dyn_em/module_big_step_utilities_em.F:329-399
+//   calc_ww_cp       dyn_em/module_big_step_utilities_em.F:640-782 (column scan)
+//   calc_cq          dyn_em/module_big_step_utilities_em.F:787-906
+//   calc_alt/calc_php dyn_em/module_big_step_utilities_em.F:910-949, 1227-1266
+// The thread grid spans i = ids-1..ide, j = jds-1..jde (mut is also needed one point
+// into the halo, as calculate_full computes it there).
+#include "kernels.h"
+
+namespace wrfmi {
+
+template <bool COMP>
+__global__ void __launch_bounds__(256) k_step_prep(KArgs a) {
+  const Dm &d = a.d;
+  const wrfmi_state &s = a.s;
+  const int i = d.ids - 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int j = d.jds - 1 + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  const int ide = d.ide, jde = d.jde, kde = d.kde;
+  if (i > ide || j > jde) return;
+  const bool mi = i <= ide - 1, mj = j <= jde - 1;
+  float mut = 0.f;
+  if (mi && mj) {
+    mut = s.mub[X2(d, i, j)] + s.mu_2[X2(d, i, j)];
+    s.mut[X2(d, i, j)] = mut;
+  }
+  if (i < d.ids || j < d.jds) return;
+  const float *mu = s.mu_2, *mub = s.mub;
+  const int ktf = kde - 1;
+  // calc_mu_uv (whole-patch tile: its == ids, ite == ide)
+  float muu = 0.f, muv = 0.f;
+  if (mj) {
+    if (i > d.ids && i < ide) {
+      muu = 0.5f * (((mu[X2(d, i, j)] + mu[X2(d, i - 1, j)]) + mub[X2(d, i, j)]) + mub[X2(d, i - 1, j)]);
+    } else if (i == d.ids) {
+      int im = a.c.periodic_x ? i - 1 : i;
+      muu = 0.5f * (((mu[X2(d, i, j)] + mu[X2(d, im, j)]) + mub[X2(d, i, j)]) + mub[X2(d, im, j)]);
+    } else {  // i == ide
+      int im = a.c.periodic_x ? i : i - 1;
+      muu = 0.5f * (((mu[X2(d, i - 1, j)] + mu[X2(d, im, j)]) + mub[X2(d, i - 1, j)]) + mub[X2(d, im, j)]);
+    }
+    s.muu[X2(d, i, j)] = muu;
+  }
+  if (mi) {
+    if (j > d.jds && j < jde) {
+      muv = 0.5f * (((mu[X2(d, i, j)] + mu[X2(d, i, j - 1)]) + mub[X2(d, i, j)]) + mub[X2(d, i, j - 1)]);
+    } else if (j == d.jds) {
+      int jm = a.c.periodic_y ? j - 1 : j;
+      muv = 0.5f * (((mu[X2(d, i, j)] + mu[X2(d, i, jm)]) + mub[X2(d, i, j)]) + mub[X2(d, i, jm)]);
+    } else {
+      int jm = a.c.periodic_y ? j : j - 1;
+      muv = 0.5f * (((mu[X2(d, i, j - 1)] + mu[X2(d, i, jm)]) + mub[X2(d, i, j - 1)]) + mub[X2(d, i, jm)]);
+    }
+    s.muv[X2(d, i, j)] = muv;
+  }
+  const float *c1h = s.c1h, *c2h = s.c2h, *c1f = s.c1f, *c2f = s.c2f;
+  const int nm = s.n_moist;
+  // couple_momentum, calc_cq, calc_alt, calc_php
+  for (int k = 1; k <= ktf; k++) {
+    const int o = X3(d, i, k, j);
+    const float c1 = c1h[X1(d, k)], c2 = c2h[X1(d, k)];
+    if (mj) {
+      s.ru[o] = s.u_2[o] * (c1 * muu + c2) / s.msfuy[X2(d, i, j)];
+      float cq;
+      if (nm >= 1) {
+        float qtot = 0.f;
+        for (int m = 0; m < nm; m++) qtot = qtot + s.q[m][o] + s.q[m][o - 1];
+        cq = 1.f / (1.f + 0.5f * qtot);
+      } else {
+        cq = 1.f;
+      }
+      s.cqu[o] = cq;
+    }
+    if (mi) {
+      s.rv[o] = s.v_2[o] * (c1 * muv + c2) * s.msfvx_inv[X2(d, i, j)];
+      float cq;
+      if (nm >= 1) {
+        float qtot = 0.f;
+        for (int m = 0; m < nm; m++) qtot = qtot + s.q[m][o] + s.q[m][o - d.sj];
+        cq = 1.f / (1.f + 0.5f * qtot);
+      } else {
+        cq = 1.f;
+      }
+      s.cqv[o] = cq;
+    }
+    if (mi && mj) {
+      if (k >= 2) {
+        float cw;
+        if (nm >= 1) {
+          float qtot = 0.f;
+          for (int m = 0; m < nm; m++) qtot = qtot + s.q[m][o] + s.q[m][o - d.ni];
+          cw = 0.5f * qtot;
+        } else {
+          cw = 0.f;
+        }
+        s.cqw[o] = cw;
+      }
+      s.alt[o] = s.al[o] + s.alb[o];
+      s.php[o] = 0.5f * (((s.phb[o] + s.phb[o + d.ni]) + s.ph_2[o]) + s.ph_2[o + d.ni]);
+    }
+  }
+  if (mi && mj) {
+    const float msfty = s.msfty[X2(d, i, j)];
+    for (int k = 1; k <= kde; k++) {
+      const int o = X3(d, i, k, j);
+      s.rw[o] = s.w_2[o] * (c1f[X1(d, k)] * mut + c2f[X1(d, k)]) / msfty;
+    }
+  }
+  // calc_ww_cp: ww(1) = ww(kte) = 0 on i = its..ite; the scan on mass columns
+  if (mj) {
+    s.ww[X3(d, i, 1, j)] = 0.f;
+    s.ww[X3(d, i, kde, j)] = 0.f;
+  }
+  if (mi && mj) {
+    // full column mass at the u points i, i+1 and v points j, j+1 (calc_ww_cp's own muu/muv)
+    const float lmuu0 = 0.5f * (((mu[X2(d, i, j)] + mub[X2(d, i, j)]) + mu[X2(d, i - 1, j)]) + mub[X2(d, i - 1, j)]);
+    const float lmuu1 = 0.5f * (((mu[X2(d, i + 1, j)] + mub[X2(d, i + 1, j)]) + mu[X2(d, i, j)]) + mub[X2(d, i, j)]);
+    const float lmuv0 = 0.5f * (((mu[X2(d, i, j)] + mub[X2(d, i, j)]) + mu[X2(d, i, j - 1)]) + mub[X2(d, i, j - 1)]);
+    const float lmuv1 = 0.5f * (((mu[X2(d, i, j + 1)] + mub[X2(d, i, j + 1)]) + mu[X2(d, i, j)]) + mub[X2(d, i, j)]);
+    const float msfuy0 = s.msfuy[X2(d, i, j)], msfuy1 = s.msfuy[X2(d, i + 1, j)];
+    const float mvi0 = s.msfvx_inv[X2(d, i, j)], mvi1 = s.msfvx_inv[X2(d, i, j + 1)];
+    const float msftx = s.msftx[X2(d, i, j)];
+    const float rdx = s.rdx, rdy = s.rdy;
+    float dmdt = 0.f, comp = 0.f;
+    // pass 1: divv(k) and its column sum; divv(k) is parked in ww(k+1) until the scan reads it
+    for (int k = 1; k <= ktf; k++) {
+      const int o = X3(d, i, k, j);
+      const float c1 = c1h[X1(d, k)], c2 = c2h[X1(d, k)];
+      const float tx = (c1 * lmuu1 + c2) * s.u_2[o + 1] / msfuy1 - (c1 * lmuu0 + c2) * s.u_2[o] / msfuy0;
+      const float ty = (c1 * lmuv1 + c2) * s.v_2[o + d.sj] * mvi1 - (c1 * lmuv0 + c2) * s.v_2[o] * mvi0;
+      const float divv = msftx * s.dnw[X1(d, k)] * (rdx * tx + rdy * ty);
+      if (COMP) two_sum_acc(dmdt, comp, divv);
+      else dmdt = dmdt + divv;
+      if (k <= ktf - 1) s.ww[o + d.ni] = divv;
+    }
+    if (COMP) dmdt = dmdt + comp;
+    float prev = 0.f;
+    for (int k = 2; k <= ktf; k++) {
+      const int o = X3(d, i, k, j);
+      const float dv = s.ww[o];
+      const float cur = prev - s.dnw[X1(d, k - 1)] * c1h[X1(d, k - 1)] * dmdt - dv;
+      s.ww[o] = cur;
+      prev = cur;
+    }
+  }
+}
+
+template __global__ void k_step_prep<false>(KArgs);
+template __global__ void k_step_prep<true>(KArgs);
+
+}  // namespace wrfmi

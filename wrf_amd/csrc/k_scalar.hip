@@ -1,0 +1,233 @@
+// Scalar transport, the equation of state and the lateral halo fills on the GPU.
+//   k_adv_scalar    : rk_scalar_tend's advect_scalar (dyn_em/module_em.F:1096-1442,
+//                     module_advect_em.F:3029-4359) with the time-averaged ru_m/rv_m/ww_m
+//   k_update_scalar : rk_update_scalar (dyn_em/module_em.F:1587-1799)
+//   k_p_rho_phi     : calc_p_rho_phi, non-hydrostatic (module_big_step_utilities_em.F:953-1223)
+//   k_halo          : set_physical_bc3d / set_physical_bc2d (share/module_bc.F:202-1113)
+//                     for up to 16 fields per launch
+#include "kernels.h"
+
+namespace wrfmi {
+
+__global__ void __launch_bounds__(256) k_adv_scalar(KArgs a) {
+  const Dm &d = a.d;
+  const wrfmi_state &s = a.s;
+  const int i = d.ids + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int j = d.jds + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  const int ids = d.ids, ide = d.ide, jds = d.jds, jde = d.jde, kde = d.kde;
+  const int its = ids, ite = ide, jts = jds, jte = jde, kts = d.kds, kte = kde;
+  if (i > ide || j > jde) return;
+  const wrfmi_config &c = a.c;
+  const float ts = c.time_step >= 0 ? 1.f : -1.f;
+  const int ktf = kte - 1;
+  const float *f = s.q[a.species], *ru = s.ru_m, *rv = s.rv_m, *rom = s.ww_m;
+  float *tend = s.advect_tend;
+  const float rdx = s.rdx, rdy = s.rdy;
+  const bool mass = i <= ide - 1 && j <= jde - 1;
+  if (!mass) {
+    for (int k = kts; k <= kte; k++) tend[X3(d, i, k, j)] = 0.f;
+    return;
+  }
+  const bool dxs = !(c.periodic_x || c.symmetric_xs || its > ids + 3), dxe = !(c.periodic_x || c.symmetric_xe || ite < ide - 3);
+  const bool dys = !(c.periodic_y || c.symmetric_ys || jts > jds + 3), dye = !(c.periodic_y || c.symmetric_ye || jte < jde - 4);
+  int jy0 = jts, jy1 = imin(jte, jde - 1), jf0 = jy0, jf1 = jy1 + 1;
+  if (dys) { jy0 = imax(jts, jds + 1); jf0 = jds + 3; }
+  if (dye) { jy1 = imin(jte, jde - 2); jf1 = jde - 3; }
+  int ix0 = its, ix1 = imin(ite, ide - 1), if0 = ix0, if1 = ix1 + 1;
+  if (dxs) { ix0 = imax(ids + 1, its); if0 = imin(ix0 + 2, ids + 3); }
+  if (dxe) { ix1 = imin(ide - 2, ite); if1 = ide - 3; }
+  auto fqy = [&](int k, int jj) -> float {
+    if (jj >= jf0 && jj <= jf1) {
+      const float vel = rv[X3(d, i, k, jj)];
+      return vel * flux5(f[X3(d, i, k, jj - 3)], f[X3(d, i, k, jj - 2)], f[X3(d, i, k, jj - 1)], f[X3(d, i, k, jj)],
+                         f[X3(d, i, k, jj + 1)], f[X3(d, i, k, jj + 2)], vel, ts);
+    } else if (jj == jds + 1 || jj == jde - 1) {
+      return 0.5f * rv[X3(d, i, k, jj)] * (f[X3(d, i, k, jj)] + f[X3(d, i, k, jj - 1)]);
+    } else {
+      const float vel = rv[X3(d, i, k, jj)];
+      return vel * flux3(f[X3(d, i, k, jj - 2)], f[X3(d, i, k, jj - 1)], f[X3(d, i, k, jj)], f[X3(d, i, k, jj + 1)], vel, ts);
+    }
+  };
+  auto fqx = [&](int k, int ii) -> float {
+    if (ii >= if0 && ii <= if1) {
+      const float vel = ru[X3(d, ii, k, j)];
+      return vel * flux5(f[X3(d, ii - 3, k, j)], f[X3(d, ii - 2, k, j)], f[X3(d, ii - 1, k, j)], f[X3(d, ii, k, j)],
+                         f[X3(d, ii + 1, k, j)], f[X3(d, ii + 2, k, j)], vel, ts);
+    } else if (ii == ids + 1 || ii == ide - 1) {
+      return 0.5f * (ru[X3(d, ii, k, j)]) * (f[X3(d, ii, k, j)] + f[X3(d, ii - 1, k, j)]);
+    } else {
+      const float vel = ru[X3(d, ii, k, j)];
+      return vel * flux3(f[X3(d, ii - 2, k, j)], f[X3(d, ii - 1, k, j)], f[X3(d, ii, k, j)], f[X3(d, ii + 1, k, j)], vel, ts);
+    }
+  };
+  auto vflux = [&](int k) -> float {
+    if (k == kts || k == kte) return 0.f;
+    if (k == kts + 1 || k == ktf)
+      return rom[X3(d, i, k, j)] * (s.fnm[X1(d, k)] * f[X3(d, i, k, j)] + s.fnp[X1(d, k)] * f[X3(d, i, k - 1, j)]);
+    const float vel = rom[X3(d, i, k, j)];
+    return vel * flux3(f[X3(d, i, k - 2, j)], f[X3(d, i, k - 1, j)], f[X3(d, i, k, j)], f[X3(d, i, k + 1, j)], -vel, ts);
+  };
+  auto P = [&](const float *arr, int ii, int k, int jj) -> float { return arr[X3(d, ii, k, jj)]; };
+  float vf_lo = 0.f;
+  for (int k = kts; k <= kte; k++) {
+    float t = 0.f;
+    if (k <= ktf) {
+      if (j >= jy0 && j <= jy1) t = t - s.msftx[X2(d, i, j)] * rdy * (fqy(k, j + 1) - fqy(k, j));
+      if (i >= ix0 && i <= ix1) t = t - s.msftx[X2(d, i, j)] * rdx * (fqx(k, i + 1) - fqx(k, i));
+      if (c.open_xs && i == its) {
+        float ub = 0.5f * (P(ru, its, k, j) + P(ru, its + 1, k, j));
+        if (ub > 0.f) ub = 0.f;
+        t = t - rdx * (ub * (P(f, its + 1, k, j) - P(f, its, k, j)) + P(f, its, k, j) * (P(ru, its + 1, k, j) - P(ru, its, k, j)));
+      }
+      if (c.open_xe && i == ide - 1) {
+        float ub = 0.5f * (P(ru, ite - 1, k, j) + P(ru, ite, k, j));
+        if (ub < 0.f) ub = 0.f;
+        t = t - rdx * (ub * (P(f, i, k, j) - P(f, i - 1, k, j)) + P(f, i, k, j) * (P(ru, ite, k, j) - P(ru, ite - 1, k, j)));
+      }
+      if (c.open_ys && j == jts) {
+        float vb = 0.5f * (P(rv, i, k, jts) + P(rv, i, k, jts + 1));
+        if (vb > 0.f) vb = 0.f;
+        t = t - rdy * (vb * (P(f, i, k, jts + 1) - P(f, i, k, jts)) + P(f, i, k, jts) * (P(rv, i, k, jts + 1) - P(rv, i, k, jts)));
+      }
+      if (c.open_ye && j == jde - 1) {
+        float vb = 0.5f * (P(rv, i, k, jte - 1) + P(rv, i, k, jte));
+        if (vb < 0.f) vb = 0.f;
+        t = t - rdy * (vb * (P(f, i, k, j) - P(f, i, k, j - 1)) + P(f, i, k, j) * (P(rv, i, k, jte) - P(rv, i, k, jte - 1)));
+      }
+      const float vf_hi = vflux(k + 1);
+      t = t - s.rdnw[X1(d, k)] * (vf_hi - vf_lo);
+      vf_lo = vf_hi;
+    }
+    tend[X3(d, i, k, j)] = t;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_update_scalar(KArgs a) {
+  const Dm &d = a.d;
+  const wrfmi_state &s = a.s;
+  const int i = d.ids + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int k = d.kds + (int)blockIdx.y;
+  const int j = d.jds + (int)blockIdx.z;
+  if (i > d.ide - 1 || k > d.kde - 1) return;
+  const int o = X3(d, i, k, j), o2 = X2(d, i, j);
+  float *sc1 = s.q_old[a.species], *sc2 = s.q[a.species];
+  const float muold = s.mu_1[o2] + s.mub[o2];
+  const float munew = s.mu_2[o2] + s.mub[o2];
+  float tendency = s.advect_tend[o] * s.msfty[o2];
+  tendency = tendency + s.q_tend[a.species][o];
+  float q1;
+  if (a.rk_step == 1) {
+    q1 = sc2[o];
+    sc1[o] = q1;
+  } else {
+    q1 = sc1[o];
+  }
+  const float c1 = s.c1h[X1(d, k)], c2 = s.c2h[X1(d, k)];
+  sc2[o] = ((c1 * muold + c2) * q1 + a.dt_rk * tendency) / (c1 * munew + c2);
+}
+
+// calc_p_rho_phi, non-hydrostatic; logarithm and power in FP64 then rounded once to FP32
+__global__ void __launch_bounds__(256) k_p_rho_phi(KArgs a) {
+  const Dm &d = a.d;
+  const wrfmi_state &s = a.s;
+  const int i = d.ids + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int k = d.kds + (int)blockIdx.y;
+  const int j = d.jds + (int)blockIdx.z;
+  if (i > d.ide - 1 || k > d.kde - 1) return;
+  const int o = X3(d, i, k, j), o2 = X2(d, i, j);
+  const float muts = s.muts[o2], mu = s.mu_2[o2];
+  float al;
+  if (a.c.hypsometric_opt == 1) {
+    const float c1 = s.c1h[X1(d, k)], c2 = s.c2h[X1(d, k)];
+    al = -1.f / (c1 * muts + c2) * (s.alb[o] * (c1 * mu) + s.rdnw[X1(d, k)] * (s.ph_2[o + d.ni] - s.ph_2[o]));
+  } else {
+    const float ptop = s.p_top;
+    const float pfu = s.c3f[X1(d, k + 1)] * muts + s.c4f[X1(d, k + 1)] + ptop;
+    const float pfd = s.c3f[X1(d, k)] * muts + s.c4f[X1(d, k)] + ptop;
+    const float phm = s.c3h[X1(d, k)] * muts + s.c4h[X1(d, k)] + ptop;
+    const float lg = (float)log((double)(pfd / pfu));
+    al = (s.ph_2[o + d.ni] - s.ph_2[o] + s.phb[o + d.ni] - s.phb[o]) / phm / lg - s.alb[o];
+  }
+  s.al[o] = al;
+  const float t = s.t_2[o];
+  float temp;
+  if (s.n_moist >= 1) {
+    if (a.c.use_theta_m == 1) {
+      temp = (K_RD * (K_T0 + t)) / (K_P0 * (al + s.alb[o]));
+    } else {
+      const float qvf = 1.f + K_RVOVRD * s.q[0][o];
+      temp = (K_RD * (K_T0 + t) * qvf) / (K_P0 * (al + s.alb[o]));
+    }
+    const float pw = (float)pow((double)temp, (double)K_CPOVCV);
+    s.p[o] = pw * K_P0 - s.pb[o];
+  } else {
+    const float pw = (float)pow((double)((K_RD * (K_T0 + t)) / (K_P0 * (al + s.alb[o]))), (double)K_CPOVCV);
+    s.p[o] = K_P0 * pw - s.pb[o];
+  }
+}
+
+// ----------------------------------------------------------------------------- halo fills
+// Value a filled halo cell ends up with after set_physical_bc's x pass followed by its
+// y pass (periodic wrap, or the open-boundary zero-gradient copy), computed directly
+// from the source cell so that all cells of all fields fill in one launch.
+__global__ void __launch_bounds__(256) k_halo(HaloArgs h) {
+  const Dm &d = h.d;
+  const int i = d.ims + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int j = d.jms + (int)blockIdx.y;
+  const HaloField F = h.f[blockIdx.z];
+  if (i > d.ime || j > d.jme) return;
+  const int ids = d.ids, ide = d.ide, jds = d.jds, jde = d.jde;
+  const int istag = F.kind == 1 ? 0 : -1, jstag = F.kind == 2 ? 0 : -1;
+  const int kend = F.is3d ? (F.kind == 3 ? d.kde : d.kde - 1) : 1;
+  const int nx = ide - ids, ny = jde - jds;
+  // x pass: which i are written, over which j, and from where
+  int xsrc = i;
+  bool xfill = false;
+  int xj0 = 0, xj1 = -1;
+  if (h.periodic_x) {
+    if (i <= ids - 1 && i >= ids - 4) { xsrc = i + nx; xfill = true; }
+    else if (i >= ide && i <= ide + 4 + istag) { xsrc = i - nx; xfill = true; }
+    xj0 = jds; xj1 = jde + jstag;
+  } else {
+    if (h.open_xs && i <= ids - 1 && i >= ids - 3) { xsrc = ids; xfill = true; }
+    if (h.open_xe) {
+      if (istag == -1 && i >= ide && i <= ide + 2) { xsrc = ide - 1; xfill = true; }
+      if (istag == 0 && i >= ide + 1 && i <= ide + 3) { xsrc = ide; xfill = true; }
+    }
+    if (F.is3d) {
+      xj0 = jds - 4;
+      xj1 = jde + jstag + 4;
+    } else {
+      xj0 = jds; xj1 = jde + jstag;
+    }
+  }
+  // y pass over every i of the memory row
+  int ysrc = j;
+  bool yfill = false;
+  if (h.periodic_y) {
+    if (j <= jds - 1 && j >= jds - 4) { ysrc = j + ny; yfill = true; }
+    else if (j >= jde && j <= jde + 4 + jstag) { ysrc = j - ny; yfill = true; }
+  } else {
+    if (h.open_ys && j <= jds - 1 && j >= jds - 3) { ysrc = jds; yfill = true; }
+    if (h.open_ye) {
+      if (jstag == -1 && j >= jde && j <= jde + 2) { ysrc = jde - 1; yfill = true; }
+      if (jstag == 0 && j >= jde + 1 && j <= jde + 3) { ysrc = jde; yfill = true; }
+    }
+  }
+  int si = i, sj = j;
+  if (yfill) {
+    sj = ysrc;
+    if (xfill && sj >= xj0 && sj <= xj1) si = xsrc;
+  } else if (xfill && j >= xj0 && j <= xj1) {
+    si = xsrc;
+  } else {
+    return;
+  }
+  if (F.is3d) {
+    for (int k = d.kds; k <= kend; k++) F.p[X3(d, i, k, j)] = F.p[X3(d, si, k, sj)];
+  } else {
+    F.p[X2(d, i, j)] = F.p[X2(d, si, sj)];
+  }
+}
+
+}  // namespace wrfmi

@@ -1,0 +1,580 @@
+// The acoustic (small) step on the GPU, dyn_em/module_small_step_em.F.
+//   k_small_prep   : small_step_prep (16-290) + calc_p_rho(step 0) (438-568) + calc_coef_w (570-652)
+//   k_advance_uv   : advance_uv (654-967), one thread per (i,k,j) point
+//   k_acoustic_col : advance_mu_t (969-1175) + advance_w (1178-1469) + sumflux (1473-1633)
+//                    + calc_p_rho (438-568), one thread per (i,j) column
+//   k_small_finish : calc_mu_uv_1 (module_big_step_utilities_em.F:184-321) + small_step_finish (295-434)
+// Arguments are bound as at the solve_em.F call sites (1089-1757): the coupled
+// perturbations live in u_2..ph_2, the time-t values in *_save, ww1 is the large-step
+// omega, t_2save holds t_2ave.  CT is the type of the tridiagonal coefficients
+// a, alpha, gamma (float: reference order; double: FP64 coefficient build and solve).
+#include "kernels.h"
+
+namespace wrfmi {
+
+template <typename CT>
+__global__ void __launch_bounds__(256) k_small_prep(KArgs a) {
+  const Dm &d = a.d;
+  const wrfmi_state &s = a.s;
+  const int i = d.ids + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int j = d.jds + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  const int ide = d.ide, jde = d.jde, kde = d.kde;
+  if (i > ide || j > jde) return;
+  const bool mi = i <= ide - 1, mj = j <= jde - 1, mass = mi && mj;
+  const int k_end = kde - 1;
+  const float *c1h = s.c1h, *c2h = s.c2h, *c1f = s.c1f, *c2f = s.c2f;
+  const int o2 = X2(d, i, j);
+  float muts = 0.f, muus = 0.f, muvs = 0.f;
+  if (a.rk_step == 1) {
+    if (mass) {
+      s.mu_1[o2] = s.mu_2[o2];
+      s.mudf[o2] = 0.f;
+      for (int k = 1; k <= k_end; k++) {
+        const int o = X3(d, i, k, j);
+        s.t_1[o] = s.t_2[o];
+      }
+      for (int k = 1; k <= kde; k++) {
+        const int o = X3(d, i, k, j);
+        s.w_1[o] = s.w_2[o];
+        s.ph_1[o] = s.ph_2[o];
+      }
+      muts = s.mub[o2] + s.mu_2[o2];
+      s.muts[o2] = muts;
+    }
+    if (mj)
+      for (int k = 1; k <= k_end; k++) {
+        const int o = X3(d, i, k, j);
+        s.u_1[o] = s.u_2[o];
+      }
+    if (mi)
+      for (int k = 1; k <= k_end; k++) {
+        const int o = X3(d, i, k, j);
+        s.v_1[o] = s.v_2[o];
+      }
+    if (mj) { muus = s.muu[o2]; s.muus[o2] = muus; }
+    if (mi) { muvs = s.muv[o2]; s.muvs[o2] = muvs; }
+    if (mass) {
+      s.mu_save[o2] = s.mu_2[o2];
+      s.mu_2[o2] = 0.f;
+    }
+  } else {
+    if (mass) { muts = s.mub[o2] + s.mu_1[o2]; s.muts[o2] = muts; }
+    if (mj) {
+      muus = 0.5f * (((s.mub[o2] + s.mu_1[o2]) + s.mub[X2(d, i - 1, j)]) + s.mu_1[X2(d, i - 1, j)]);
+      s.muus[o2] = muus;
+    }
+    if (mi) {
+      muvs = 0.5f * (((s.mub[o2] + s.mu_1[o2]) + s.mub[X2(d, i, j - 1)]) + s.mu_1[X2(d, i, j - 1)]);
+      s.muvs[o2] = muvs;
+    }
+    if (mass) {
+      s.mu_save[o2] = s.mu_2[o2];
+      s.mu_2[o2] = s.mu_1[o2] - s.mu_2[o2];
+    }
+  }
+  if (mj) {
+    const float muu = s.muu[o2], msfuy = s.msfuy[o2];
+    for (int k = 1; k <= k_end; k++) {
+      const int o = X3(d, i, k, j);
+      const float u2 = s.u_2[o];
+      s.u_save[o] = u2;
+      s.u_2[o] = ((c1h[X1(d, k)] * muus + c2h[X1(d, k)]) * s.u_1[o] - (c1h[X1(d, k)] * muu + c2h[X1(d, k)]) * u2) / msfuy;
+    }
+  }
+  if (mi) {
+    const float muv = s.muv[o2], mvi = s.msfvx_inv[o2];
+    for (int k = 1; k <= k_end; k++) {
+      const int o = X3(d, i, k, j);
+      const float v2 = s.v_2[o];
+      s.v_save[o] = v2;
+      s.v_2[o] = ((c1h[X1(d, k)] * muvs + c2h[X1(d, k)]) * s.v_1[o] - (c1h[X1(d, k)] * muv + c2h[X1(d, k)]) * v2) * mvi;
+    }
+  }
+  if (!mass) return;
+  const float mut = s.mut[o2], msfty = s.msfty[o2];
+  s.ww1[X3(d, i, kde, j)] = 0.f;
+  s.ww1[X3(d, i, 1, j)] = 0.f;
+  for (int k = 1; k <= k_end; k++) {
+    const int o = X3(d, i, k, j);
+    s.c2a[o] = K_CPOVCV * (s.pb[o] + s.p[o]) / s.alt[o];
+    const float t2 = s.t_2[o];
+    s.t_save[o] = t2;
+    s.t_2[o] = (c1h[X1(d, k)] * muts + c2h[X1(d, k)]) * s.t_1[o] - (c1h[X1(d, k)] * mut + c2h[X1(d, k)]) * t2;
+  }
+  for (int k = 1; k <= kde; k++) {
+    const int o = X3(d, i, k, j);
+    const float w2 = s.w_2[o];
+    s.w_save[o] = w2;
+    s.w_2[o] = ((c1f[X1(d, k)] * muts + c2f[X1(d, k)]) * s.w_1[o] - (c1f[X1(d, k)] * mut + c2f[X1(d, k)]) * w2) / msfty;
+    const float ph2 = s.ph_2[o];
+    s.ph_save[o] = ph2;
+    s.ph_2[o] = s.ph_1[o] - ph2;
+    s.ww1[o] = s.ww[o];
+  }
+  // calc_p_rho, step 0 (mu = mu_2, mut = muts, t_1 = t_save)
+  const float mu = s.mu_2[o2];
+  if (a.c.non_hydrostatic) {
+    for (int k = 1; k <= k_end; k++) {
+      const int o = X3(d, i, k, j);
+      const float c1 = c1h[X1(d, k)], c2 = c2h[X1(d, k)];
+      const float alt = s.alt[o];
+      const float al = -1.f / (c1 * muts + c2) * (alt * (c1 * mu) + s.rdnw[X1(d, k)] * (s.ph_2[o + d.ni] - s.ph_2[o]));
+      s.al[o] = al;
+      const float t1 = s.t_save[o];
+      const float pp = s.c2a[o] * (alt * (s.t_2[o] - (c1 * mu) * t1) / ((c1 * muts + c2) * (K_T0 + t1)) - al);
+      s.p[o] = pp;
+      s.pm1[o] = pp;
+    }
+  }
+  // calc_coef_w
+  if (a.c.non_hydrostatic) {
+    CT *A = (CT *)s.a, *AL = (CT *)s.alpha, *GA = (CT *)s.gamma;
+    const CT lid = a.c.top_lid ? CT(0) : CT(1);
+    const CT epssm = (CT)a.c.epssm;
+    CT cof = (CT(.5) * (CT)a.dts * (CT)K_G * (CT(1) + epssm));
+    cof = cof * cof;
+    const CT MUT = (CT)mut;
+    auto cH = [&](int k) -> CT { return (CT)c1h[X1(d, k)] * MUT + (CT)c2h[X1(d, k)]; };
+    auto cF = [&](int k) -> CT { return (CT)c1f[X1(d, k)] * MUT + (CT)c2f[X1(d, k)]; };
+    const CT *dummy = nullptr;
+    (void)dummy;
+    auto rdnw = [&](int k) -> CT { return (CT)s.rdnw[X1(d, k)]; };
+    auto rdn = [&](int k) -> CT { return (CT)s.rdn[X1(d, k)]; };
+    auto C2A = [&](int k) -> CT { return (CT)s.c2a[X3(d, i, k, j)]; };
+    auto CQW = [&](int k) -> CT { return (CT)s.cqw[X3(d, i, k, j)]; };
+    {
+      const int k = kde - 1;
+      A[X3(d, i, 2, j)] = CT(0);
+      A[X3(d, i, kde, j)] = -CT(2) * cof * (rdnw(kde - 1) * rdnw(kde - 1)) * C2A(kde - 1) * lid / (cH(k) * cF(k));
+      GA[X3(d, i, 1, j)] = CT(0);
+    }
+    for (int kk = 3; kk <= kde - 1; kk++) {
+      const int k = kk - 1;
+      A[X3(d, i, kk, j)] = -CQW(kk) * cof * rdn(kk) * rdnw(kk - 1) * C2A(kk - 1) / (cH(k) * cF(k));
+    }
+    CT gam = CT(0);
+    for (int k = 2; k <= kde - 1; k++) {
+      const CT b = CT(1) + CQW(k) * cof * rdn(k) *
+                               (rdnw(k) * C2A(k) / (cH(k) * cF(k)) + rdnw(k - 1) * C2A(k - 1) / (cH(k - 1) * cF(k)));
+      const CT c = -CQW(k) * cof * rdn(k) * rdnw(k) * C2A(k) / (cH(k) * cF(k + 1));
+      const CT alpha = CT(1) / (b - A[X3(d, i, k, j)] * gam);
+      AL[X3(d, i, k, j)] = alpha;
+      gam = c * alpha;
+      GA[X3(d, i, k, j)] = gam;
+    }
+    {
+      const int k = kde;
+      const CT b = CT(1) + CT(2) * cof * (rdnw(kde - 1) * rdnw(kde - 1)) * C2A(kde - 1) / (cH(k - 1) * cF(k));
+      const CT c = CT(0);
+      const CT alpha = CT(1) / (b - A[X3(d, i, kde, j)] * gam);
+      AL[X3(d, i, kde, j)] = alpha;
+      GA[X3(d, i, kde, j)] = c * alpha;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_advance_uv(KArgs a) {
+  const Dm &d = a.d;
+  const wrfmi_state &s = a.s;
+  const int i = d.ids + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int k = d.kds + (int)blockIdx.y;
+  const int j = d.jds + (int)blockIdx.z;
+  const int ids = d.ids, ide = d.ide, jds = d.jds, jde = d.jde, kde = d.kde;
+  const int its = ids, ite = ide, jts = jds, jte = jde;
+  if (i > ide || k > kde - 1) return;
+  const wrfmi_config &c = a.c;
+  const int k_start = d.kds;
+  const int i_start = its, i_end = imin(ite, ide - 1), j_start = jts, j_end = imin(jte, jde - 1), i_endu = ite, j_endv = jte;
+  int i_start_up = i_start, i_end_up = i_endu, j_start_vp = j_start, j_end_vp = j_endv;
+  if ((c.open_xs || c.symmetric_xs) && its == ids) i_start_up++;
+  if ((c.open_xe || c.symmetric_xe) && ite == ide) i_end_up--;
+  if ((c.open_ys || c.symmetric_ys || c.polar) && jts == jds) j_start_vp++;
+  if ((c.open_ye || c.symmetric_ye || c.polar) && jte == jde) j_end_vp--;
+  int i_start_u_tend = i_start, i_end_u_tend = i_endu, j_start_v_tend = j_start, j_end_v_tend = j_endv;
+  if (c.symmetric_xs && its == ids) i_start_u_tend++;
+  if (c.symmetric_xe && ite == ide) i_end_u_tend--;
+  if (c.symmetric_ys && jts == jds) j_start_v_tend++;
+  if (c.symmetric_ye && jte == jde) j_end_v_tend--;
+  const float rdx = s.rdx, rdy = s.rdy, dx = 1.f / rdx, dy = 1.f / rdy, dts = a.dts;
+  const float cf1 = s.cf1, cf2 = s.cf2, cf3 = s.cf3, emdiv = c.emdiv;
+  const float *p = s.p, *pb = s.pb, *ph = s.ph_2, *php = s.php, *alt = s.alt, *al = s.al, *mu = s.mu_2, *mudf = s.mudf;
+  const float c1 = s.c1h[X1(d, k)], c2 = s.c2h[X1(d, k)];
+  const bool nh = c.non_hydrostatic;
+  const int o = X3(d, i, k, j);
+  // u
+  if (j >= j_start && j <= j_end) {
+    float u = s.u_2[o];
+    if (i >= i_start_u_tend && i <= i_end_u_tend) u = u + dts * s.ru_tend[o];
+    if (i >= i_start_up && i <= i_end_up) {
+      const float mudf_xy = -emdiv * dx * (mudf[X2(d, i, j)] - mudf[X2(d, i - 1, j)]) / s.msfuy[X2(d, i, j)];
+      const float m = s.msfux[X2(d, i, j)] / s.msfuy[X2(d, i, j)];
+      float dpxy = m * .5f * rdx * (c1 * s.muu[X2(d, i, j)] + c2) *
+                   (((ph[X3(d, i, k + 1, j)] - ph[X3(d, i - 1, k + 1, j)]) + (ph[o] - ph[X3(d, i - 1, k, j)])) +
+                    (alt[o] + alt[X3(d, i - 1, k, j)]) * (p[o] - p[X3(d, i - 1, k, j)]) +
+                    (al[o] + al[X3(d, i - 1, k, j)]) * (pb[o] - pb[X3(d, i - 1, k, j)]));
+      if (nh) {
+        auto dpn = [&](int kk) -> float {
+          if (kk == 1)
+            return .5f * (cf1 * (p[X3(d, i, 1, j)] + p[X3(d, i - 1, 1, j)]) + cf2 * (p[X3(d, i, 2, j)] + p[X3(d, i - 1, 2, j)]) +
+                          cf3 * (p[X3(d, i, 3, j)] + p[X3(d, i - 1, 3, j)]));
+          if (kk == kde) {
+            if (!c.top_lid) return 0.f;
+            return .5f * (cf1 * (p[X3(d, i - 1, kde - 1, j)] + p[X3(d, i, kde - 1, j)]) + cf2 * (p[X3(d, i - 1, kde - 2, j)] + p[X3(d, i, kde - 2, j)]) +
+                          cf3 * (p[X3(d, i - 1, kde - 3, j)] + p[X3(d, i, kde - 3, j)]));
+          }
+          return .5f * (s.fnm[X1(d, kk)] * (p[X3(d, i, kk, j)] + p[X3(d, i - 1, kk, j)]) + s.fnp[X1(d, kk)] * (p[X3(d, i, kk - 1, j)] + p[X3(d, i - 1, kk - 1, j)]));
+        };
+        dpxy = dpxy + m * rdx * (php[o] - php[X3(d, i - 1, k, j)]) *
+                          (s.rdnw[X1(d, k)] * (dpn(k + 1) - dpn(k)) - .5f * ((c1 * mu[X2(d, i - 1, j)]) + (c1 * mu[X2(d, i, j)])));
+      }
+      u = u - dts * s.cqu[o] * dpxy + (c1 * mudf_xy);
+    }
+    s.u_2[o] = u;
+  }
+  // v
+  if (i <= i_end && j >= j_start_v_tend && j <= j_end_v_tend) {
+    float v = s.v_2[o];
+    v = v + dts * s.rv_tend[o];
+    if (j >= j_start_vp && j <= j_end_vp) {
+      const float mudf_xy = -emdiv * dy * (mudf[X2(d, i, j)] - mudf[X2(d, i, j - 1)]) * s.msfvx_inv[X2(d, i, j)];
+      const float m = s.msfvy[X2(d, i, j)] / s.msfvx[X2(d, i, j)];
+      float dpxy = m * .5f * rdy * (c1 * s.muv[X2(d, i, j)] + c2) *
+                   (((ph[X3(d, i, k + 1, j)] - ph[X3(d, i, k + 1, j - 1)]) + (ph[o] - ph[X3(d, i, k, j - 1)])) +
+                    (alt[o] + alt[X3(d, i, k, j - 1)]) * (p[o] - p[X3(d, i, k, j - 1)]) +
+                    (al[o] + al[X3(d, i, k, j - 1)]) * (pb[o] - pb[X3(d, i, k, j - 1)]));
+      if (nh) {
+        auto dpn = [&](int kk) -> float {
+          if (kk == 1)
+            return .5f * (cf1 * (p[X3(d, i, 1, j)] + p[X3(d, i, 1, j - 1)]) + cf2 * (p[X3(d, i, 2, j)] + p[X3(d, i, 2, j - 1)]) +
+                          cf3 * (p[X3(d, i, 3, j)] + p[X3(d, i, 3, j - 1)]));
+          if (kk == kde) {
+            if (!c.top_lid) return 0.f;
+            return .5f * (cf1 * (p[X3(d, i, kde - 1, j - 1)] + p[X3(d, i, kde - 1, j)]) + cf2 * (p[X3(d, i, kde - 2, j - 1)] + p[X3(d, i, kde - 2, j)]) +
+                          cf3 * (p[X3(d, i, kde - 3, j - 1)] + p[X3(d, i, kde - 3, j)]));
+          }
+          return .5f * (s.fnm[X1(d, kk)] * (p[X3(d, i, kk, j)] + p[X3(d, i, kk, j - 1)]) + s.fnp[X1(d, kk)] * (p[X3(d, i, kk - 1, j)] + p[X3(d, i, kk - 1, j - 1)]));
+        };
+        dpxy = dpxy + m * rdy * (php[o] - php[X3(d, i, k, j - 1)]) *
+                          (s.rdnw[X1(d, k)] * (dpn(k + 1) - dpn(k)) - .5f * ((c1 * mu[X2(d, i, j - 1)]) + (c1 * mu[X2(d, i, j)])));
+      }
+      v = v - dts * s.cqv[o] * dpxy + (c1 * mudf_xy);
+    }
+    s.v_2[o] = v;
+  }
+  (void)k_start;
+}
+
+// One thread per (i,j) column.  Mass columns run advance_mu_t, advance_w and
+// calc_p_rho; every column of the tile accumulates its share of sumflux.
+template <typename CT, bool COMP>
+__global__ void __launch_bounds__(256) k_acoustic_col(KArgs a) {
+  const Dm &d = a.d;
+  const wrfmi_state &s = a.s;
+  const int i = d.ids + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int j = d.jds + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  const int ide = d.ide, jde = d.jde, kde = d.kde;
+  if (i > ide || j > jde) return;
+  const bool mi = i <= ide - 1, mj = j <= jde - 1, mass = mi && mj;
+  const wrfmi_config &c = a.c;
+  const int k_end = kde - 1;
+  const float dts = a.dts, epssm = c.epssm;
+  const float *c1h = s.c1h, *c2h = s.c2h, *c1f = s.c1f, *c2f = s.c2f;
+  const int o2 = X2(d, i, j);
+  if (mass) {
+    // ---------------- advance_mu_t
+    const float *u = s.u_2, *u_1 = s.u_save, *v = s.v_2, *v_1 = s.v_save;
+    float *ww = s.ww;
+    const float msftx = s.msftx[o2], msfty = s.msfty[o2], rdx = s.rdx, rdy = s.rdy;
+    const float muu0 = s.muu[o2], muu1 = s.muu[X2(d, i + 1, j)], muv0 = s.muv[o2], muv1 = s.muv[X2(d, i, j + 1)];
+    const float msfuy0 = s.msfuy[o2], msfuy1 = s.msfuy[X2(d, i + 1, j)];
+    const float mvi0 = s.msfvx_inv[o2], mvi1 = s.msfvx_inv[X2(d, i, j + 1)];
+    float dmdt = 0.f, comp = 0.f;
+    // pass 1: dvdxi(k) and its column integral; dvdxi(k) is parked in ww(k+1)
+    for (int k = 1; k <= k_end; k++) {
+      const int o = X3(d, i, k, j);
+      const float cc = c1h[X1(d, k)], c2 = c2h[X1(d, k)];
+      const float dv = msftx * msfty *
+                       (rdy * ((v[o + d.sj] + (cc * muv1 + c2) * v_1[o + d.sj] * mvi1) - (v[o] + (cc * muv0 + c2) * v_1[o] * mvi0)) +
+                        rdx * ((u[o + 1] + (cc * muu1 + c2) * u_1[o + 1] / msfuy1) - (u[o] + (cc * muu0 + c2) * u_1[o] / msfuy0)));
+      if (COMP) two_sum_acc(dmdt, comp, s.dnw[X1(d, k)] * dv);
+      else dmdt = dmdt + s.dnw[X1(d, k)] * dv;
+      if (k <= k_end - 1) ww[o + d.ni] = dv;
+    }
+    if (COMP) dmdt = dmdt + comp;
+    const float mu_tend = s.mu_tend[o2];
+    float mu = s.mu_2[o2];
+    float muave = mu;
+    mu = mu + dts * (dmdt + mu_tend);
+    s.mu_2[o2] = mu;
+    s.mudf[o2] = (dmdt + mu_tend);
+    const float muts = s.mut[o2] + mu;
+    s.muts[o2] = muts;
+    muave = .5f * ((1.f + epssm) * mu + (1.f - epssm) * muave);
+    s.muave[o2] = muave;
+    {
+      float prev = ww[X3(d, i, 1, j)];
+      for (int kk = 2; kk <= k_end; kk++) {
+        const int o = X3(d, i, kk, j);
+        const int k = kk - 1;
+        const float dv = ww[o];
+        const float cur = prev - s.dnw[X1(d, kk - 1)] * ((c1h[X1(d, k)] * dmdt) + dv + (c1h[X1(d, k)] * mu_tend)) / msfty;
+        ww[o] = cur;
+        prev = cur;
+      }
+      for (int k = 1; k <= k_end; k++) {
+        const int o = X3(d, i, k, j);
+        ww[o] = ww[o] - s.ww1[o];
+      }
+    }
+    // theta
+    const float *t_1 = s.t_save;
+    float *t = s.t_2;
+    for (int k = 1; k <= k_end; k++) {
+      const int o = X3(d, i, k, j);
+      const float tv = t[o];
+      s.t_2save[o] = tv;
+      t[o] = tv + msfty * dts * s.t_tend[o];
+    }
+    {
+      float wdtn_lo = 0.f;
+      for (int k = 1; k <= k_end; k++) {
+        const int o = X3(d, i, k, j);
+        float wdtn_hi = 0.f;
+        if (k + 1 <= k_end) wdtn_hi = ww[o + d.ni] * (s.fnm[X1(d, k + 1)] * t_1[o + d.ni] + s.fnp[X1(d, k + 1)] * t_1[o]);
+        t[o] = t[o] - dts * msfty *
+                          (msftx * (.5f * rdy * (v[o + d.sj] * (t_1[o + d.sj] + t_1[o]) - v[o] * (t_1[o] + t_1[o - d.sj])) +
+                                    .5f * rdx * (u[o + 1] * (t_1[o + 1] + t_1[o]) - u[o] * (t_1[o] + t_1[o - 1]))) +
+                           s.rdnw[X1(d, k)] * (wdtn_hi - wdtn_lo));
+        wdtn_lo = wdtn_hi;
+      }
+    }
+    // ---------------- advance_w
+    if (c.non_hydrostatic) {
+      float *w = s.w_2, *ph = s.ph_2;
+      const float *ph_1 = s.ph_save, *phb = s.phb, *ph_tend = s.ph_tend, *rw_tend = s.rw_tend;
+      const float *c2a = s.c2a, *cqw = s.cqw, *alt = s.alt, *t_2 = s.t_2, *ht = s.ht;
+      const CT *A = (const CT *)s.a, *AL = (const CT *)s.alpha, *GA = (const CT *)s.gamma;
+      const float mut = s.mut[o2];
+      const float msft_inv = 1.f / msfty;
+      const float cf1 = s.cf1, cf2 = s.cf2, cf3 = s.cf3;
+      float *t2ave = s.t_2save;
+      auto t2ave_at = [&](int k) -> float {
+        const int o = X3(d, i, k, j);
+        float v1 = .5f * ((1.f + epssm) * t_2[o] + (1.f - epssm) * t2ave[o]);
+        v1 = (v1 + (c1h[X1(d, k)] * muave) * K_T0) / ((c1h[X1(d, k)] * muts + c2h[X1(d, k)]) * (K_T0 + t_1[o]));
+        t2ave[o] = v1;
+        return v1;
+      };
+      auto wdwn_at = [&](int kp1) -> float {  // wdwn(k+1) for k = kp1-1
+        const int k = kp1 - 1;
+        const int o = X3(d, i, k, j);
+        if (c.phi_adv_z == 2)
+          return s.rdnw[X1(d, k)] * (ph_1[o + d.ni] - ph_1[o] + phb[o + d.ni] - phb[o]);
+        return .5f * (ww[o + d.ni] + ww[o]) * s.rdnw[X1(d, k)] * (ph_1[o + d.ni] - ph_1[o] + phb[o + d.ni] - phb[o]);
+      };
+      // final rhs(k) (k = 2..kde); reads w and ph at level k before they are overwritten
+      auto rhs_at = [&](int k) -> float {
+        const int o = X3(d, i, k, j);
+        float r = dts * (ph_tend[o] + .5f * K_G * (1.f - epssm) * w[o]);
+        if (k <= k_end) {
+          if (c.phi_adv_z == 2)
+            r = r - dts * ww[o] * (s.fnm[X1(d, k)] * wdwn_at(k + 1) + s.fnp[X1(d, k)] * wdwn_at(k));
+          else
+            r = r - dts * (s.fnm[X1(d, k)] * wdwn_at(k + 1) + s.fnp[X1(d, k)] * wdwn_at(k));
+        }
+        r = ph[o] + msfty * r / (c1f[X1(d, k)] * mut + c2f[X1(d, k)]);
+        if (c.top_lid && k == k_end + 1) r = 0.f;
+        return r;
+      };
+      // surface w from the terrain slope
+      {
+        const int o = X3(d, i, 1, j);
+        w[o] = msfty * .5f * rdy *
+                   ((ht[X2(d, i, j + 1)] - ht[o2]) * (cf1 * v[X3(d, i, 1, j + 1)] + cf2 * v[X3(d, i, 2, j + 1)] + cf3 * v[X3(d, i, 3, j + 1)]) +
+                    (ht[o2] - ht[X2(d, i, j - 1)]) * (cf1 * v[X3(d, i, 1, j)] + cf2 * v[X3(d, i, 2, j)] + cf3 * v[X3(d, i, 3, j)])) +
+               msftx * .5f * rdx *
+                   ((ht[X2(d, i + 1, j)] - ht[o2]) * (cf1 * u[X3(d, i + 1, 1, j)] + cf2 * u[X3(d, i + 1, 2, j)] + cf3 * u[X3(d, i + 1, 3, j)]) +
+                    (ht[o2] - ht[X2(d, i - 1, j)]) * (cf1 * u[X3(d, i, 1, j)] + cf2 * u[X3(d, i, 2, j)] + cf3 * u[X3(d, i, 3, j)]));
+      }
+      // forward pass: predictor + forward elimination; rhs(k) is stored in ph(k) once ph(k) is no longer needed
+      float t2a_lo = t2ave_at(1);
+      float phl_m1 = ph[X3(d, i, 1, j)], phl_0 = ph[X3(d, i, 2, j)];  // old ph at k-1, k
+      float rhs_m1 = 0.f;                                                // rhs(1) = 0
+      float rhs_0 = rhs_at(2);
+      CT wfe = (CT)w[X3(d, i, 1, j)];
+      for (int k = 2; k <= k_end + 1; k++) {
+        const int o = X3(d, i, k, j);
+        float wv;
+        if (k <= k_end) {
+          const float t2a_hi = t2ave_at(k);
+          const float phl_p1 = ph[o + d.ni];
+          const float rhs_p1 = rhs_at(k + 1);
+          wv = w[o] + dts * rw_tend[o] +
+               msft_inv * cqw[o] *
+                   (+.5f * dts * K_G * s.rdn[X1(d, k)] *
+                    (c2a[o] * s.rdnw[X1(d, k)] / (c1h[X1(d, k)] * mut + c2h[X1(d, k)]) *
+                         ((1.f + epssm) * (rhs_p1 - rhs_0) + (1.f - epssm) * (phl_p1 - phl_0)) -
+                     c2a[o - d.ni] * s.rdnw[X1(d, k - 1)] / (c1h[X1(d, k - 1)] * mut + c2h[X1(d, k - 1)]) *
+                         ((1.f + epssm) * (rhs_0 - rhs_m1) + (1.f - epssm) * (phl_0 - phl_m1)))) +
+               dts * K_G * msft_inv *
+                   (s.rdn[X1(d, k)] * (c2a[o] * alt[o] * t2a_hi - c2a[o - d.ni] * alt[o - d.ni] * t2a_lo) - (c1f[X1(d, k)] * muave));
+          ph[o - d.ni] = (k - 1 >= 2) ? rhs_m1 : ph[o - d.ni];
+          t2a_lo = t2a_hi;
+          phl_m1 = phl_0; phl_0 = phl_p1;
+          rhs_m1 = rhs_0; rhs_0 = rhs_p1;
+        } else {
+          wv = w[o] + dts * rw_tend[o] +
+               msft_inv * (-.5f * dts * K_G / (c1h[X1(d, k - 1)] * mut + c2h[X1(d, k - 1)]) * (s.rdnw[X1(d, k - 1)] * s.rdnw[X1(d, k - 1)]) * 2.f *
+                               c2a[o - d.ni] * ((1.f + epssm) * (rhs_0 - rhs_m1) + (1.f - epssm) * (phl_0 - phl_m1)) -
+                           dts * K_G * (2.f * s.rdnw[X1(d, k - 1)] * c2a[o - d.ni] * alt[o - d.ni] * t2a_lo + (c1f[X1(d, k)] * muave)));
+          if (c.top_lid) wv = 0.f;
+          ph[o - d.ni] = rhs_m1;
+          ph[o] = rhs_0;
+        }
+        // w(k) = (w(k) - a(k) w(k-1)) alpha(k)
+        if (sizeof(CT) == sizeof(float)) {
+          const float wf = (wv - (float)A[X3(d, i, k, j)] * (float)wfe) * (float)AL[X3(d, i, k, j)];
+          wfe = (CT)wf;
+          w[o] = wf;
+        } else {
+          wfe = ((CT)wv - A[X3(d, i, k, j)] * wfe) * AL[X3(d, i, k, j)];
+          w[o] = (float)wfe;
+          s.advect_tend[o] = (float)((double)wfe - (double)(float)wfe);  // low-order part kept for the back sweep
+        }
+      }
+      // backward pass: back substitution, then ph = rhs + (dts g (1+epssm)/2) msfty w / (c1f muts + c2f)
+      CT wup = wfe;
+      {
+        const int k = k_end + 1;
+        const int o = X3(d, i, k, j);
+        ph[o] = ph[o] + msfty * .5f * dts * K_G * (1.f + epssm) * w[o] / (c1f[X1(d, k)] * muts + c2f[X1(d, k)]);
+      }
+      for (int k = k_end; k >= 2; k--) {
+        const int o = X3(d, i, k, j);
+        float wk;
+        if (sizeof(CT) == sizeof(float)) {
+          wk = w[o] - (float)GA[X3(d, i, k, j)] * (float)wup;
+          wup = (CT)wk;
+        } else {
+          const CT wd = (CT)w[o] + (CT)s.advect_tend[o];
+          const CT wn = wd - GA[X3(d, i, k, j)] * wup;
+          wup = wn;
+          wk = (float)wn;
+        }
+        w[o] = wk;
+        ph[o] = ph[o] + msfty * .5f * dts * K_G * (1.f + epssm) * wk / (c1f[X1(d, k)] * muts + c2f[X1(d, k)]);
+      }
+      if (c.damp_opt == 3) {
+        // Rayleigh damping of w (advance_w, module_small_step_em.F:1445-1458) is applied to the
+        // final w before ph is formed there; it is not part of the implemented option set
+      }
+    }
+    // ---------------- calc_p_rho (step = iteration >= 1), mut = muts, t_1 = t_save
+    if (c.non_hydrostatic) {
+      const float *ph = s.ph_2;
+      for (int k = 1; k <= k_end; k++) {
+        const int o = X3(d, i, k, j);
+        const float cc = c1h[X1(d, k)], c2 = c2h[X1(d, k)];
+        const float alt = s.alt[o];
+        const float al = -1.f / (cc * muts + c2) * (alt * (cc * mu) + s.rdnw[X1(d, k)] * (ph[o + d.ni] - ph[o]));
+        s.al[o] = al;
+        const float t1 = t_1[o];
+        float pp = s.c2a[o] * (alt * (s.t_2[o] - (cc * mu) * t1) / ((cc * muts + c2) * (K_T0 + t1)) - al);
+        const float ptmp = pp;
+        pp = pp + c.smdiv * (pp - s.pm1[o]);
+        s.p[o] = pp;
+        s.pm1[o] = ptmp;
+      }
+    }
+  }
+  // ---------------- sumflux
+  {
+    const int it = a.iteration, ns = a.nsmall;
+    const float nsf = (float)ns;
+    for (int k = 1; k <= kde; k++) {
+      const int o = X3(d, i, k, j);
+      float rum = it == 1 ? 0.f : s.ru_m[o];
+      float rvm = it == 1 ? 0.f : s.rv_m[o];
+      float wwm = it == 1 ? 0.f : s.ww_m[o];
+      const bool kin = k <= kde - 1;
+      if (mj && kin) rum = rum + s.u_2[o];
+      if (mi && kin) rvm = rvm + s.v_2[o];
+      if (mass) wwm = wwm + s.ww[o];
+      if (it == ns) {
+        if (mj && kin) rum = rum / nsf + (c1h[X1(d, k)] * s.muu[o2] + c2h[X1(d, k)]) * s.u_save[o] / s.msfuy[o2];
+        if (mi && kin) rvm = rvm / nsf + (c1h[X1(d, k)] * s.muv[o2] + c2h[X1(d, k)]) * s.v_save[o] * s.msfvx_inv[o2];
+        if (mass) wwm = wwm / nsf + s.ww1[o];
+      }
+      s.ru_m[o] = rum;
+      s.rv_m[o] = rvm;
+      s.ww_m[o] = wwm;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_small_finish(KArgs a) {
+  const Dm &d = a.d;
+  const wrfmi_state &s = a.s;
+  const int i = d.ids + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int j = d.jds + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  const int ide = d.ide, jde = d.jde, kde = d.kde;
+  if (i > ide || j > jde) return;
+  const bool mi = i <= ide - 1, mj = j <= jde - 1, mass = mi && mj;
+  const float *c1h = s.c1h, *c2h = s.c2h, *c1f = s.c1f, *c2f = s.c2f;
+  const float *muts = s.muts;
+  const int o2 = X2(d, i, j);
+  // calc_mu_uv_1(muts -> muus, muvs)
+  float muus = 0.f, muvs = 0.f;
+  if (mj) {
+    if (i > d.ids && i < ide) muus = 0.5f * (muts[o2] + muts[X2(d, i - 1, j)]);
+    else if (i == d.ids) muus = 0.5f * (muts[o2] + muts[X2(d, a.c.periodic_x ? i - 1 : i, j)]);
+    else muus = 0.5f * (muts[X2(d, i - 1, j)] + muts[X2(d, a.c.periodic_x ? i : i - 1, j)]);
+    s.muus[o2] = muus;
+  }
+  if (mi) {
+    if (j > d.jds && j < jde) muvs = 0.5f * (muts[o2] + muts[X2(d, i, j - 1)]);
+    else if (j == d.jds) muvs = 0.5f * (muts[o2] + muts[X2(d, i, a.c.periodic_y ? j - 1 : j)]);
+    else muvs = 0.5f * (muts[X2(d, i, j - 1)] + muts[X2(d, i, a.c.periodic_y ? j : j - 1)]);
+    s.muvs[o2] = muvs;
+  }
+  // small_step_finish
+  if (mi) {
+    const float muv = s.muv[o2], msfvx = s.msfvx[o2];
+    for (int k = d.kds; k <= kde - 1; k++) {
+      const int o = X3(d, i, k, j);
+      s.v_2[o] = (msfvx * s.v_2[o] + s.v_save[o] * (c1h[X1(d, k)] * muv + c2h[X1(d, k)])) / (c1h[X1(d, k)] * muvs + c2h[X1(d, k)]);
+    }
+  }
+  if (mj) {
+    const float muu = s.muu[o2], msfuy = s.msfuy[o2];
+    for (int k = d.kds; k <= kde - 1; k++) {
+      const int o = X3(d, i, k, j);
+      s.u_2[o] = (msfuy * s.u_2[o] + s.u_save[o] * (c1h[X1(d, k)] * muu + c2h[X1(d, k)])) / (c1h[X1(d, k)] * muus + c2h[X1(d, k)]);
+    }
+  }
+  if (!mass) return;
+  const float mut = s.mut[o2], mts = muts[o2], msfty = s.msfty[o2];
+  for (int k = d.kds; k <= kde; k++) {
+    const int o = X3(d, i, k, j);
+    s.w_2[o] = (msfty * s.w_2[o] + s.w_save[o] * (c1f[X1(d, k)] * mut + c2f[X1(d, k)])) / (c1f[X1(d, k)] * mts + c2f[X1(d, k)]);
+    s.ph_2[o] = s.ph_2[o] + s.ph_save[o];
+    s.ww[o] = s.ww[o] + s.ww1[o];
+  }
+  const bool last = a.rk_step >= a.c.rk_ord;
+  for (int k = d.kds; k <= kde - 1; k++) {
+    const int o = X3(d, i, k, j);
+    const float cm = c1h[X1(d, k)] * mut + c2h[X1(d, k)];
+    if (!last)
+      s.t_2[o] = (s.t_2[o] + s.t_save[o] * cm) / (c1h[X1(d, k)] * mts + c2h[X1(d, k)]);
+    else
+      s.t_2[o] = (s.t_2[o] - a.dts * (float)a.nsmall * cm * s.h_diabatic[o] + s.t_save[o] * cm) / (c1h[X1(d, k)] * mts + c2h[X1(d, k)]);
+  }
+  s.mu_2[o2] = s.mu_2[o2] + s.mu_save[o2];
+}
+
+template __global__ void k_small_prep<float>(KArgs);
+template __global__ void k_small_prep<double>(KArgs);
+template __global__ void k_acoustic_col<float, false>(KArgs);
+template __global__ void k_acoustic_col<double, true>(KArgs);
+
+}  // namespace wrfmi

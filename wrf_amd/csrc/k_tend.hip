@@ -1,0 +1,872 @@
+// rk_tendency + rk_addtend_dry on the GPU (dyn_em/module_em.F:190-1092), physics off.
+// Four column kernels (one thread per (i,j), marching in k), one per tendency family:
+//   k_tend_u : ru_tend = advect_u + horizontal_pressure_gradient(x) + coriolis + curvature + addtend
+//   k_tend_v : rv_tend = advect_v + horizontal_pressure_gradient(y) + coriolis + curvature + addtend
+//   k_tend_w : rw_tend = advect_w + pg_buoy_w (+ cqw := 1/(1+cqw)) + w_damp + coriolis + curvature + addtend
+//   k_tend_t : t_tend = advect_scalar(t) + addtend;  ph_tend = rhs_ph + addtend;
+//              plus zero_tend of u/v/w/ph/t_save, mu_tend, mu_save
+// Every output point accumulates its contributions in the reference's order
+// (zero_tend, then each routine of rk_tendency in call order, then rk_addtend_dry),
+// so the result does not depend on which thread computes which face flux.
+// Routines and line ranges: advect_u module_advect_em.F:126-1526, advect_v 1530-3024,
+// advect_scalar 3029-4359, advect_w 4364-6064; rhs_ph module_big_step_utilities_em.F:1365-2178,
+// horizontal_pressure_gradient 2183-2415, pg_buoy_w 2419-2499, w_damp 2503-2711,
+// coriolis 3640-3850, curvature 4175-4469.
+#include "kernels.h"
+
+namespace wrfmi {
+
+struct Geo {  // boundary-dependent flags of the whole-patch tile (its = ids, ite = ide ...)
+  bool per_x, per_y, sym_xs, sym_xe, sym_ys, sym_ye, open_xs, open_xe, open_ys, open_ye, spec, nested, polar;
+};
+DEVI Geo geo(const wrfmi_config &c) {
+  Geo g;
+  g.per_x = c.periodic_x; g.per_y = c.periodic_y;
+  g.sym_xs = c.symmetric_xs; g.sym_xe = c.symmetric_xe; g.sym_ys = c.symmetric_ys; g.sym_ye = c.symmetric_ye;
+  g.open_xs = c.open_xs; g.open_xe = c.open_xe; g.open_ys = c.open_ys; g.open_ye = c.open_ye;
+  g.nested = c.nested; g.spec = c.specified || c.nested; g.polar = c.polar;
+  return g;
+}
+
+// ------------------------------------------------------------------------------ u
+__global__ void __launch_bounds__(256) k_tend_u(KArgs a) {
+  const Dm &d = a.d;
+  const wrfmi_state &s = a.s;
+  const int i = d.ids + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int j = d.jds + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  const int ids = d.ids, ide = d.ide, jds = d.jds, jde = d.jde, kde = d.kde;
+  const int its = ids, ite = ide, jts = jds, jte = jde, kts = d.kds, kte = kde;
+  if (i > ide || j > jde) return;
+  const Geo g = geo(a.c);
+  const float ts = a.c.time_step >= 0 ? 1.f : -1.f;
+  const int ktf = kte - 1;
+  const float *u = s.u_2, *ru = s.ru, *rv = s.rv, *rom = s.ww, *rw = s.rw;
+  const float rdx = s.rdx, rdy = s.rdy;
+  const bool dxs = !(g.per_x || g.sym_xs || its > ids + 3), dxe = !(g.per_x || g.sym_xe || ite < ide - 2);
+  const bool dys = !(g.per_y || g.sym_ys || jts > jds + 3), dye = !(g.per_y || g.sym_ye || jte < jde - 4);
+  // y flux: columns iy0..iy1, tendency rows jy0..jy1, full-order faces jf0..jf1
+  int iy0 = its, iy1 = ite;
+  if (g.open_xs || g.spec) iy0 = imax(ids + 1, its);
+  if (g.open_xe || g.spec) iy1 = imin(ide - 1, ite);
+  if (g.per_x) { iy0 = its; iy1 = ite; }
+  int jy0 = jts, jy1 = imin(jte, jde - 1), jf0 = jy0, jf1 = jy1 + 1;
+  if (dys) { jy0 = imax(jts, jds + 1); jf0 = jds + 3; }
+  if (dye) { jy1 = imin(jte, jde - 2); jf1 = jde - 3; }
+  // x flux
+  int ix0 = its, ix1 = ite, if0 = ix0, if1 = ix1 + 1;
+  if (dxs) { ix0 = imax(ids + 1, its); if0 = ids + 3; }
+  if (dxe) { ix1 = imin(ide - 1, ite); if1 = ide - 2; }
+  // vertical
+  int iv0 = its, iv1 = ite;
+  if (g.open_ys || g.spec) iv0 = imax(ids + 1, its);
+  if (g.open_ye || g.spec) iv1 = imin(ide - 1, ite);
+  if (g.per_x) { iv0 = its; iv1 = ite; }
+  // pressure gradient
+  int ih0 = its, ih1 = ite;
+  if ((g.open_xs || g.spec || g.nested) && its == ids) ih0 = its + 1;
+  if ((g.open_xe || g.spec || g.nested) && ite == ide) ih1 = ite - 1;
+  if (g.per_x) { ih0 = its; ih1 = ite; }
+  // coriolis
+  int ic0 = its, ic1 = ite;
+  if (g.open_xs || g.spec || g.nested) ic0 = imax(ids + 1, its);
+  if (g.open_xe || g.spec || g.nested) ic1 = imin(ide - 1, ite);
+  if (g.per_x) { ic0 = its; ic1 = ite; }
+  // curvature: vxgm stencil ranges and its copy rules
+  const bool cpx_s = (g.open_xs || (g.spec && !g.per_x) || g.nested) && its == ids;
+  const bool cpx_e = (g.open_xe || (g.spec && !g.per_x) || g.nested) && ite == ide;
+  const bool cpy_s = (g.open_ys || g.spec || g.nested || g.polar) && jts == jds;
+  const bool cpy_e = (g.open_ye || g.spec || g.nested || g.polar) && jte == jde;
+  int ik0 = its, ik1 = ite;
+  if (g.open_xs || g.spec || g.nested) ik0 = imax(ids + 1, its);
+  if (g.open_xe || g.spec || g.nested) ik1 = imin(ide - 1, ite);
+  if (g.per_x) { ik0 = its; ik1 = ite; }
+  const float *msfvx = s.msfvx, *msfuy = s.msfuy, *vv = s.v_2;
+  auto vx_raw = [&](int ii, int k, int jj) -> float {
+    return 0.5f * (u[X3(d, ii, k, jj)] + u[X3(d, ii + 1, k, jj)]) * (msfvx[X2(d, ii, jj + 1)] - msfvx[X2(d, ii, jj)]) * rdy -
+           0.5f * (vv[X3(d, ii, k, jj)] + vv[X3(d, ii, k, jj + 1)]) * (msfuy[X2(d, ii + 1, jj)] - msfuy[X2(d, ii, jj)]) * rdx;
+  };
+  auto vx = [&](int ii, int k, int jj) -> float {
+    if (cpy_s && jj == jts - 1) jj = jts;
+    else if (cpy_e && jj == jte) jj = jte - 1;
+    if (jj >= jts && jj <= jte - 1) {
+      if (cpx_s && ii == its - 1) ii = its;
+      else if (cpx_e && ii == ite) ii = ite - 1;
+    }
+    return vx_raw(ii, k, jj);
+  };
+  auto fqy = [&](int k, int jj) -> float {
+    if (jj >= jf0 && jj <= jf1) {
+      const float vel = 0.5f * (rv[X3(d, i, k, jj)] + rv[X3(d, i - 1, k, jj)]);
+      return vel * flux5(u[X3(d, i, k, jj - 3)], u[X3(d, i, k, jj - 2)], u[X3(d, i, k, jj - 1)], u[X3(d, i, k, jj)],
+                         u[X3(d, i, k, jj + 1)], u[X3(d, i, k, jj + 2)], vel, ts);
+    } else if (jj == jds + 1 || jj == jde - 1) {
+      return 0.25f * (rv[X3(d, i, k, jj)] + rv[X3(d, i - 1, k, jj)]) * (u[X3(d, i, k, jj)] + u[X3(d, i, k, jj - 1)]);
+    } else {
+      const float vel = 0.5f * (rv[X3(d, i, k, jj)] + rv[X3(d, i - 1, k, jj)]);
+      return vel * flux3(u[X3(d, i, k, jj - 2)], u[X3(d, i, k, jj - 1)], u[X3(d, i, k, jj)], u[X3(d, i, k, jj + 1)], vel, ts);
+    }
+  };
+  auto fqx = [&](int k, int ii) -> float {
+    if (ii >= if0 && ii <= if1) {
+      const float vel = 0.5f * (ru[X3(d, ii, k, j)] + ru[X3(d, ii - 1, k, j)]);
+      return vel * flux5(u[X3(d, ii - 3, k, j)], u[X3(d, ii - 2, k, j)], u[X3(d, ii - 1, k, j)], u[X3(d, ii, k, j)],
+                         u[X3(d, ii + 1, k, j)], u[X3(d, ii + 2, k, j)], vel, ts);
+    } else if ((dxs && ix0 == ids + 1 && ii == ids + 1)) {
+      float ub = u[X3(d, ii - 1, k, j)];
+      if (g.spec && u[X3(d, ii, k, j)] < 0.f) ub = u[X3(d, ii, k, j)];
+      return 0.25f * (ru[X3(d, ii, k, j)] + ru[X3(d, ii - 1, k, j)]) * (u[X3(d, ii, k, j)] + ub);
+    } else if (dxe && ix1 == ide - 1 && ii == ide) {
+      float ub = u[X3(d, ii, k, j)];
+      if (g.spec && u[X3(d, ii - 1, k, j)] > 0.f) ub = u[X3(d, ii - 1, k, j)];
+      return 0.25f * (ru[X3(d, ii, k, j)] + ru[X3(d, ii - 1, k, j)]) * (u[X3(d, ii - 1, k, j)] + ub);
+    } else {
+      const float vel = 0.5f * (ru[X3(d, ii, k, j)] + ru[X3(d, ii - 1, k, j)]);
+      return vel * flux3(u[X3(d, ii - 2, k, j)], u[X3(d, ii - 1, k, j)], u[X3(d, ii, k, j)], u[X3(d, ii + 1, k, j)], vel, ts);
+    }
+  };
+  auto vflux = [&](int k) -> float {
+    if (k == kts || k == kte) return 0.f;
+    if (k == kts + 1 || k == ktf)
+      return 0.5f * (rom[X3(d, i, k, j)] + rom[X3(d, i - 1, k, j)]) *
+             (s.fnm[X1(d, k)] * u[X3(d, i, k, j)] + s.fnp[X1(d, k)] * u[X3(d, i, k - 1, j)]);
+    const float vel = 0.5f * (rom[X3(d, i - 1, k, j)] + rom[X3(d, i, k, j)]);
+    return vel * flux3(u[X3(d, i, k - 2, j)], u[X3(d, i, k - 1, j)], u[X3(d, i, k, j)], u[X3(d, i, k + 1, j)], -vel, ts);
+  };
+  const bool row_ok = j <= imin(jte, jde - 1);
+  const float *p = s.p, *ph = s.ph_2, *al = s.al, *alt = s.alt, *pb = s.pb, *php = s.php, *mu = s.mu_2;
+  const float cf1 = s.cf1, cf2 = s.cf2, cf3 = s.cf3, cfn = s.cfn, cfn1 = s.cfn1;
+  const bool nh = a.c.non_hydrostatic;
+  auto dpn = [&](int k) -> float {  // vertically interpolated p at w level k, averaged to the u point
+    if (k == 1)
+      return .5f * (cf1 * (p[X3(d, i - 1, 1, j)] + p[X3(d, i, 1, j)]) + cf2 * (p[X3(d, i - 1, 2, j)] + p[X3(d, i, 2, j)]) +
+                    cf3 * (p[X3(d, i - 1, 3, j)] + p[X3(d, i, 3, j)]));
+    if (k == kde) {
+      if (!a.c.top_lid) return 0.f;
+      return .5f * (cfn * (p[X3(d, i - 1, kde - 1, j)] + p[X3(d, i, kde - 1, j)]) + cfn1 * (p[X3(d, i - 1, kde - 2, j)] + p[X3(d, i, kde - 2, j)]));
+    }
+    return .5f * (s.fnm[X1(d, k)] * (p[X3(d, i - 1, k, j)] + p[X3(d, i, k, j)]) + s.fnp[X1(d, k)] * (p[X3(d, i - 1, k - 1, j)] + p[X3(d, i, k - 1, j)]));
+  };
+  float vf_lo = vflux(kts);
+  float dpn_lo = (nh && row_ok && i >= ih0 && i <= ih1) ? dpn(1) : 0.f;
+  for (int k = kts; k <= kte; k++) {
+    const int o = X3(d, i, k, j);
+    float t = 0.f;
+    float vf_hi = 0.f, dpn_hi = 0.f;
+    if (row_ok && k <= ktf) {
+      // advect_u: y flux divergence
+      if (i >= iy0 && i <= iy1 && j >= jy0 && j <= jy1) {
+        const float mrdy = s.msfux[X2(d, i, j)] * rdy;
+        t = t - mrdy * (fqy(k, j + 1) - fqy(k, j));
+      }
+      // x flux divergence
+      if (i >= ix0 && i <= ix1) {
+        const float mrdx = s.msfux[X2(d, i, j)] * rdx;
+        t = t - mrdx * (fqx(k, i + 1) - fqx(k, i));
+      }
+      const float c1 = s.c1h[X1(d, k)], c2 = s.c2h[X1(d, k)];
+      // radiative lateral boundary condition in x
+      if (g.open_xs && i == its) {
+        float ub = ru[o] - K_CB * (c1 * s.mut[X2(d, its, j)] + c2);
+        if (ub > 0.f) ub = 0.f;
+        t = t - rdx * ub * (u[X3(d, its + 1, k, j)] - u[o]);
+      }
+      if (g.open_xe && i == ite) {
+        float ub = ru[o] + K_CB * (c1 * s.mut[X2(d, ite - 1, j)] + c2);
+        if (ub < 0.f) ub = 0.f;
+        t = t - rdx * ub * (u[o] - u[X3(d, ite - 1, k, j)]);
+      }
+      {
+        int io0 = its, io1 = imin(ite, ide), imn = ids, imx = ide - 1;
+        if (g.open_xs) io0 = imax(ids + 1, its);
+        if (g.open_xe) io1 = imin(ite, ide - 1);
+        const int ip = imin(imx, i), im = imax(imn, i - 1);
+        if (g.open_ys && j == jts && i >= io0 && i <= io1) {
+          const float mrdy = s.msfux[X2(d, i, jts)] * rdy;
+          const float vw = 0.5f * (rv[X3(d, ip, k, jts)] + rv[X3(d, im, k, jts)]);
+          const float vb = vw < 0.f ? vw : 0.f;
+          const float dvm = rv[X3(d, ip, k, jts + 1)] - rv[X3(d, ip, k, jts)];
+          const float dvp = rv[X3(d, im, k, jts + 1)] - rv[X3(d, im, k, jts)];
+          t = t - mrdy * (vb * (u[X3(d, i, k, jts + 1)] - u[X3(d, i, k, jts)]) + 0.5f * u[X3(d, i, k, jts)] * (dvm + dvp));
+        }
+        if (g.open_ye && j == jte - 1 && i >= io0 && i <= io1) {
+          const float mrdy = s.msfux[X2(d, i, jte - 1)] * rdy;
+          const float vw = 0.5f * (rv[X3(d, ip, k, jte)] + rv[X3(d, im, k, jte)]);
+          const float vb = vw > 0.f ? vw : 0.f;
+          const float dvm = rv[X3(d, ip, k, jte)] - rv[X3(d, ip, k, jte - 1)];
+          const float dvp = rv[X3(d, im, k, jte)] - rv[X3(d, im, k, jte - 1)];
+          t = t - mrdy * (vb * (u[X3(d, i, k, jte - 1)] - u[X3(d, i, k, jte - 2)]) + 0.5f * u[X3(d, i, k, jte - 1)] * (dvm + dvp));
+        }
+      }
+      // vertical flux divergence (order 3)
+      if (i >= iv0 && i <= iv1) {
+        vf_hi = vflux(k + 1);
+        t = t - s.rdnw[X1(d, k)] * (vf_hi - vf_lo);
+      }
+      // horizontal_pressure_gradient (x)
+      if (i >= ih0 && i <= ih1) {
+        const float m = s.msfux[X2(d, i, j)] / s.msfuy[X2(d, i, j)];
+        float dpx = m * .5f * rdx * (c1 * s.muu[X2(d, i, j)] + c2) *
+                    ((ph[X3(d, i, k + 1, j)] - ph[X3(d, i - 1, k + 1, j)] + ph[o] - ph[X3(d, i - 1, k, j)]) +
+                     (alt[o] + alt[X3(d, i - 1, k, j)]) * (p[o] - p[X3(d, i - 1, k, j)]) +
+                     (al[o] + al[X3(d, i - 1, k, j)]) * (pb[o] - pb[X3(d, i - 1, k, j)]));
+        if (nh) {
+          dpn_hi = dpn(k + 1);
+          dpx = dpx + m * rdx * (php[o] - php[X3(d, i - 1, k, j)]) *
+                          (s.rdnw[X1(d, k)] * (dpn_hi - dpn_lo) - .5f * ((c1 * mu[X2(d, i - 1, j)]) + (c1 * mu[X2(d, i, j)])));
+          t = t - s.cqu[o] * dpx;
+        } else {
+          t = t - dpx;
+        }
+      }
+      // coriolis
+      if (i >= ic0 && i <= ic1) {
+        t = t + (s.msfux[X2(d, i, j)] / s.msfuy[X2(d, i, j)]) * 0.5f * (s.f[X2(d, i, j)] + s.f[X2(d, i - 1, j)]) * 0.25f *
+                    (rv[X3(d, i - 1, k, j + 1)] + rv[X3(d, i, k, j + 1)] + rv[X3(d, i - 1, k, j)] + rv[X3(d, i, k, j)]) -
+            0.5f * (s.e[X2(d, i, j)] + s.e[X2(d, i - 1, j)]) * 0.5f * (s.cosa[X2(d, i, j)] + s.cosa[X2(d, i - 1, j)]) * 0.25f *
+                (rw[X3(d, i - 1, k + 1, j)] + rw[X3(d, i - 1, k, j)] + rw[X3(d, i, k + 1, j)] + rw[X3(d, i, k, j)]);
+      }
+      // curvature
+      if (i >= ik0 && i <= ik1) {
+        t = t + 0.5f * (vx(i, k, j) + vx(i - 1, k, j)) * 0.25f *
+                    (rv[X3(d, i - 1, k, j + 1)] + rv[X3(d, i, k, j + 1)] + rv[X3(d, i - 1, k, j)] + rv[X3(d, i, k, j)]) -
+            u[o] * K_RERADIUS * 0.25f * (rw[X3(d, i - 1, k + 1, j)] + rw[X3(d, i - 1, k, j)] + rw[X3(d, i, k + 1, j)] + rw[X3(d, i, k, j)]);
+      }
+    }
+    // rk_addtend_dry
+    if (row_ok && k <= kte - 1) {
+      const float msf = s.msfuy[X2(d, i, j)];
+      float tf = s.ru_tendf[o];
+      if (a.rk_step == 1) {
+        tf = tf + 0.f * msf;  // u_save was zeroed by rk_tendency (module_em.F:395)
+        s.ru_tendf[o] = tf;
+      }
+      t = t + tf / msf;
+    }
+    s.ru_tend[o] = t;
+    vf_lo = vf_hi;
+    dpn_lo = dpn_hi;
+  }
+}
+
+// ------------------------------------------------------------------------------ v
+__global__ void __launch_bounds__(256) k_tend_v(KArgs a) {
+  const Dm &d = a.d;
+  const wrfmi_state &s = a.s;
+  const int i = d.ids + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int j = d.jds + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  const int ids = d.ids, ide = d.ide, jds = d.jds, jde = d.jde, kde = d.kde;
+  const int its = ids, ite = ide, jts = jds, jte = jde, kts = d.kds, kte = kde;
+  if (i > ide || j > jde) return;
+  const Geo g = geo(a.c);
+  const float ts = a.c.time_step >= 0 ? 1.f : -1.f;
+  const int ktf = kte - 1;
+  const float *v = s.v_2, *ru = s.ru, *rv = s.rv, *rom = s.ww, *rw = s.rw;
+  const float rdx = s.rdx, rdy = s.rdy;
+  const bool dxs = !(g.per_x || g.sym_xs || its > ids + 3), dxe = !(g.per_x || g.sym_xe || ite < ide - 3);
+  const bool dys = !(g.per_y || g.sym_ys || jts > jds + 3), dye = !(g.per_y || g.sym_ye || jte < jde - 3);
+  const bool col_ok = i <= imin(ite, ide - 1);
+  // y flux
+  int jy0 = jts, jy1 = jte, jf0 = jts, jf1 = jte + 1;
+  if (dys) { jy0 = imax(jts, jds + 1); jf0 = jds + 3; }
+  if (dye) { jy1 = imin(jte, jde - 1); jf1 = jde - 2; }
+  // x flux
+  int jx0 = jts, jx1 = jte;
+  if (g.open_ys || g.spec) jx0 = imax(jds + 1, jts);
+  if (g.open_ye || g.spec) jx1 = imin(jde - 1, jte);
+  int ix0 = its, ix1 = imin(ite, ide - 1), if0 = ix0, if1 = ix1 + 1;
+  if (dxs) { ix0 = imax(ids + 1, its); if0 = imin(ix0 + 2, ids + 3); }
+  if (dxe) { ix1 = imin(ide - 2, ite); if1 = ide - 3; }
+  // vertical
+  int jv0 = jts, jv1 = jte;
+  if (g.open_ys || g.spec) jv0 = imax(jds + 1, jts);
+  if (g.open_ye || g.spec) jv1 = imin(jde - 1, jte);
+  // pressure gradient
+  int jh0 = jts, jh1 = jte;
+  if ((g.open_ys || g.spec || g.nested || g.polar) && jts == jds) jh0 = jts + 1;
+  if ((g.open_ye || g.spec || g.nested || g.polar) && jte == jde) jh1 = jte - 1;
+  // coriolis / curvature rows
+  int jc0 = jts, jc1 = jte;
+  if (g.open_ys || g.spec || g.nested || g.polar) jc0 = imax(jds + 1, jts);
+  if (g.open_ye || g.spec || g.nested || g.polar) jc1 = imin(jde - 1, jte);
+  // curvature vxgm
+  const bool cpx_s = (g.open_xs || (g.spec && !g.per_x) || g.nested) && its == ids;
+  const bool cpx_e = (g.open_xe || (g.spec && !g.per_x) || g.nested) && ite == ide;
+  const bool cpy_s = (g.open_ys || g.spec || g.nested || g.polar) && jts == jds;
+  const bool cpy_e = (g.open_ye || g.spec || g.nested || g.polar) && jte == jde;
+  const float *msfvx = s.msfvx, *msfuy = s.msfuy, *uu = s.u_2;
+  auto vx_raw = [&](int ii, int k, int jj) -> float {
+    return 0.5f * (uu[X3(d, ii, k, jj)] + uu[X3(d, ii + 1, k, jj)]) * (msfvx[X2(d, ii, jj + 1)] - msfvx[X2(d, ii, jj)]) * rdy -
+           0.5f * (v[X3(d, ii, k, jj)] + v[X3(d, ii, k, jj + 1)]) * (msfuy[X2(d, ii + 1, jj)] - msfuy[X2(d, ii, jj)]) * rdx;
+  };
+  auto vx = [&](int ii, int k, int jj) -> float {
+    if (cpy_s && jj == jts - 1) jj = jts;
+    else if (cpy_e && jj == jte) jj = jte - 1;
+    if (jj >= jts && jj <= jte - 1) {
+      if (cpx_s && ii == its - 1) ii = its;
+      else if (cpx_e && ii == ite) ii = ite - 1;
+    }
+    return vx_raw(ii, k, jj);
+  };
+  auto fqy = [&](int k, int jj) -> float {
+    if (jj >= jf0 && jj <= jf1) {
+      const float vel = 0.5f * (rv[X3(d, i, k, jj)] + rv[X3(d, i, k, jj - 1)]);
+      return vel * flux5(v[X3(d, i, k, jj - 3)], v[X3(d, i, k, jj - 2)], v[X3(d, i, k, jj - 1)], v[X3(d, i, k, jj)],
+                         v[X3(d, i, k, jj + 1)], v[X3(d, i, k, jj + 2)], vel, ts);
+    } else if (jj == jds + 1) {
+      float vb = v[X3(d, i, k, jj - 1)];
+      if (g.spec && v[X3(d, i, k, jj)] < 0.f) vb = v[X3(d, i, k, jj)];
+      return 0.25f * (rv[X3(d, i, k, jj)] + rv[X3(d, i, k, jj - 1)]) * (v[X3(d, i, k, jj)] + vb);
+    } else if (jj == jde) {
+      float vb = v[X3(d, i, k, jj)];
+      if (g.spec && v[X3(d, i, k, jj - 1)] > 0.f) vb = v[X3(d, i, k, jj - 1)];
+      return 0.25f * (rv[X3(d, i, k, jj)] + rv[X3(d, i, k, jj - 1)]) * (vb + v[X3(d, i, k, jj - 1)]);
+    } else {
+      const float vel = 0.5f * (rv[X3(d, i, k, jj)] + rv[X3(d, i, k, jj - 1)]);
+      return vel * flux3(v[X3(d, i, k, jj - 2)], v[X3(d, i, k, jj - 1)], v[X3(d, i, k, jj)], v[X3(d, i, k, jj + 1)], vel, ts);
+    }
+  };
+  auto fqx = [&](int k, int ii) -> float {
+    if (ii >= if0 && ii <= if1) {
+      const float vel = 0.5f * (ru[X3(d, ii, k, j)] + ru[X3(d, ii, k, j - 1)]);
+      return vel * flux5(v[X3(d, ii - 3, k, j)], v[X3(d, ii - 2, k, j)], v[X3(d, ii - 1, k, j)], v[X3(d, ii, k, j)],
+                         v[X3(d, ii + 1, k, j)], v[X3(d, ii + 2, k, j)], vel, ts);
+    } else if (ii == ids + 1 || ii == ide - 1) {
+      return 0.25f * (ru[X3(d, ii, k, j)] + ru[X3(d, ii, k, j - 1)]) * (v[X3(d, ii, k, j)] + v[X3(d, ii - 1, k, j)]);
+    } else {
+      const float vel = 0.5f * (ru[X3(d, ii, k, j)] + ru[X3(d, ii, k, j - 1)]);
+      return vel * flux3(v[X3(d, ii - 2, k, j)], v[X3(d, ii - 1, k, j)], v[X3(d, ii, k, j)], v[X3(d, ii + 1, k, j)], vel, ts);
+    }
+  };
+  auto vflux = [&](int k) -> float {
+    if (k == kts || k == kte) return 0.f;
+    if (k == kts + 1 || k == ktf)
+      return 0.5f * (rom[X3(d, i, k, j)] + rom[X3(d, i, k, j - 1)]) *
+             (s.fnm[X1(d, k)] * v[X3(d, i, k, j)] + s.fnp[X1(d, k)] * v[X3(d, i, k - 1, j)]);
+    const float vel = 0.5f * (rom[X3(d, i, k, j)] + rom[X3(d, i, k, j - 1)]);
+    return vel * flux3(v[X3(d, i, k - 2, j)], v[X3(d, i, k - 1, j)], v[X3(d, i, k, j)], v[X3(d, i, k + 1, j)], -vel, ts);
+  };
+  const float *p = s.p, *ph = s.ph_2, *al = s.al, *alt = s.alt, *pb = s.pb, *php = s.php, *mu = s.mu_2;
+  const float cf1 = s.cf1, cf2 = s.cf2, cf3 = s.cf3, cfn = s.cfn, cfn1 = s.cfn1;
+  const bool nh = a.c.non_hydrostatic;
+  auto dpn = [&](int k) -> float {
+    if (k == 1)
+      return .5f * (cf1 * (p[X3(d, i, 1, j - 1)] + p[X3(d, i, 1, j)]) + cf2 * (p[X3(d, i, 2, j - 1)] + p[X3(d, i, 2, j)]) +
+                    cf3 * (p[X3(d, i, 3, j - 1)] + p[X3(d, i, 3, j)]));
+    if (k == kde) {
+      if (!a.c.top_lid) return 0.f;
+      return .5f * (cfn * (p[X3(d, i, kde - 1, j - 1)] + p[X3(d, i, kde - 1, j)]) + cfn1 * (p[X3(d, i, kde - 2, j - 1)] + p[X3(d, i, kde - 2, j)]));
+    }
+    return .5f * (s.fnm[X1(d, k)] * (p[X3(d, i, k, j - 1)] + p[X3(d, i, k, j)]) + s.fnp[X1(d, k)] * (p[X3(d, i, k - 1, j - 1)] + p[X3(d, i, k - 1, j)]));
+  };
+  float vf_lo = vflux(kts);
+  float dpn_lo = (nh && col_ok && j >= jh0 && j <= jh1) ? dpn(1) : 0.f;
+  for (int k = kts; k <= kte; k++) {
+    const int o = X3(d, i, k, j);
+    float t = 0.f, vf_hi = 0.f, dpn_hi = 0.f;
+    if (col_ok && k <= ktf) {
+      if (j >= jy0 && j <= jy1) {
+        const float mrdy = s.msfvy[X2(d, i, j)] * rdy;
+        t = t - mrdy * (fqy(k, j + 1) - fqy(k, j));
+      }
+      if (j >= jx0 && j <= jx1 && i >= ix0 && i <= ix1) {
+        const float mrdx = s.msfvy[X2(d, i, j)] * rdx;
+        t = t - mrdx * (fqx(k, i + 1) - fqx(k, i));
+      }
+      const float c1 = s.c1h[X1(d, k)], c2 = s.c2h[X1(d, k)];
+      if (g.open_ys && j == jts) {
+        float vb = rv[o] - K_CB * (c1 * s.mut[X2(d, i, jts)] + c2);
+        if (vb > 0.f) vb = 0.f;
+        t = t - rdy * vb * (v[X3(d, i, k, jts + 1)] - v[o]);
+      }
+      if (g.open_ye && j == jte) {
+        float vb = rv[o] + K_CB * (c1 * s.mut[X2(d, i, jte - 1)] + c2);
+        if (vb < 0.f) vb = 0.f;
+        t = t - rdy * vb * (v[o] - v[X3(d, i, k, jte - 1)]);
+      }
+      {
+        int jo0 = jts, jo1 = imin(jte, jde), jmn = jds, jmx = jde - 1;
+        if (g.open_ys) jo0 = imax(jds + 1, jts);
+        if (g.open_ye) jo1 = imin(jte, jde - 1);
+        const int jp = imin(jmx, j), jm = imax(jmn, j - 1);
+        if (g.open_xs && i == its && j >= jo0 && j <= jo1) {
+          const float mrdx = s.msfvy[X2(d, its, j)] * rdx;
+          const float uw = 0.5f * (ru[X3(d, its, k, jp)] + ru[X3(d, its, k, jm)]);
+          const float ub = uw < 0.f ? uw : 0.f;
+          const float dup = ru[X3(d, its + 1, k, jp)] - ru[X3(d, its, k, jp)];
+          const float dum = ru[X3(d, its + 1, k, jm)] - ru[X3(d, its, k, jm)];
+          t = t - mrdx * (ub * (v[X3(d, its + 1, k, j)] - v[X3(d, its, k, j)]) + 0.5f * v[X3(d, its, k, j)] * (dup + dum));
+        }
+        if (g.open_xe && i == ite - 1 && j >= jo0 && j <= jo1) {
+          const float mrdx = s.msfvy[X2(d, ite - 1, j)] * rdx;
+          const float uw = 0.5f * (ru[X3(d, ite, k, jp)] + ru[X3(d, ite, k, jm)]);
+          const float ub = uw > 0.f ? uw : 0.f;
+          const float dup = ru[X3(d, ite, k, jp)] - ru[X3(d, ite - 1, k, jp)];
+          const float dum = ru[X3(d, ite, k, jm)] - ru[X3(d, ite - 1, k, jm)];
+          t = t - mrdx * (ub * (v[X3(d, ite - 1, k, j)] - v[X3(d, ite - 2, k, j)]) + 0.5f * v[X3(d, ite - 1, k, j)] * (dup + dum));
+        }
+      }
+      if (j >= jv0 && j <= jv1) {
+        vf_hi = vflux(k + 1);
+        t = t - (s.msfvy[X2(d, i, j)] / s.msfvx[X2(d, i, j)]) * s.rdnw[X1(d, k)] * (vf_hi - vf_lo);
+      }
+      if (j >= jh0 && j <= jh1) {
+        const float m = s.msfvy[X2(d, i, j)] / s.msfvx[X2(d, i, j)];
+        float dpy = m * .5f * rdy * (c1 * s.muv[X2(d, i, j)] + c2) *
+                    ((ph[X3(d, i, k + 1, j)] - ph[X3(d, i, k + 1, j - 1)] + ph[o] - ph[X3(d, i, k, j - 1)]) +
+                     (alt[o] + alt[X3(d, i, k, j - 1)]) * (p[o] - p[X3(d, i, k, j - 1)]) +
+                     (al[o] + al[X3(d, i, k, j - 1)]) * (pb[o] - pb[X3(d, i, k, j - 1)]));
+        if (nh) {
+          dpn_hi = dpn(k + 1);
+          dpy = dpy + m * rdy * (php[o] - php[X3(d, i, k, j - 1)]) *
+                          (s.rdnw[X1(d, k)] * (dpn_hi - dpn_lo) - .5f * ((c1 * mu[X2(d, i, j - 1)]) + (c1 * mu[X2(d, i, j)])));
+          t = t - s.cqv[o] * dpy;
+        } else {
+          t = t - dpy;
+        }
+      }
+      if (j >= jc0 && j <= jc1) {
+        const float m = s.msfvy[X2(d, i, j)] / s.msfvx[X2(d, i, j)];
+        t = t - m * 0.5f * (s.f[X2(d, i, j)] + s.f[X2(d, i, j - 1)]) * 0.25f *
+                    (ru[X3(d, i, k, j)] + ru[X3(d, i + 1, k, j)] + ru[X3(d, i, k, j - 1)] + ru[X3(d, i + 1, k, j - 1)]) +
+            m * 0.5f * (s.e[X2(d, i, j)] + s.e[X2(d, i, j - 1)]) * 0.5f * (s.sina[X2(d, i, j)] + s.sina[X2(d, i, j - 1)]) * 0.25f *
+                (rw[X3(d, i, k + 1, j - 1)] + rw[X3(d, i, k, j - 1)] + rw[X3(d, i, k + 1, j)] + rw[X3(d, i, k, j)]);
+        t = t - 0.5f * (vx(i, k, j) + vx(i, k, j - 1)) * 0.25f *
+                    (ru[X3(d, i, k, j)] + ru[X3(d, i + 1, k, j)] + ru[X3(d, i, k, j - 1)] + ru[X3(d, i + 1, k, j - 1)]) -
+            m * v[o] * K_RERADIUS * 0.25f * (rw[X3(d, i, k + 1, j - 1)] + rw[X3(d, i, k, j - 1)] + rw[X3(d, i, k + 1, j)] + rw[X3(d, i, k, j)]);
+      }
+    }
+    if (col_ok && k <= kte - 1) {
+      float tf = s.rv_tendf[o];
+      if (a.rk_step == 1) {
+        tf = tf + 0.f * s.msfvx[X2(d, i, j)];  // v_save was zeroed by rk_tendency
+        s.rv_tendf[o] = tf;
+      }
+      t = t + tf * s.msfvx_inv[X2(d, i, j)];
+    }
+    s.rv_tend[o] = t;
+    vf_lo = vf_hi;
+    dpn_lo = dpn_hi;
+  }
+}
+
+// ------------------------------------------------------------------------------ w
+__global__ void __launch_bounds__(256) k_tend_w(KArgs a) {
+  const Dm &d = a.d;
+  const wrfmi_state &s = a.s;
+  const int i = d.ids + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int j = d.jds + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  const int ids = d.ids, ide = d.ide, jds = d.jds, jde = d.jde, kde = d.kde;
+  const int its = ids, ite = ide, jts = jds, jte = jde, kts = d.kds, kte = kde;
+  if (i > ide || j > jde) return;
+  const Geo g = geo(a.c);
+  const float ts = a.c.time_step >= 0 ? 1.f : -1.f;
+  const int ktf = kte - 1;
+  const float *w = s.w_2, *ru = s.ru, *rv = s.rv, *rom = s.ww;
+  const float *fzm = s.fnm, *fzp = s.fnp;
+  const float rdx = s.rdx, rdy = s.rdy;
+  const bool mass = i <= imin(ite, ide - 1) && j <= imin(jte, jde - 1);
+  const bool dxs = !(g.per_x || g.sym_xs || its > ids + 3), dxe = !(g.per_x || g.sym_xe || ite < ide - 3);
+  const bool dys = !(g.per_y || g.sym_ys || jts > jds + 3), dye = !(g.per_y || g.sym_ye || jte < jde - 4);
+  int jy0 = jts, jy1 = imin(jte, jde - 1), jf0 = jy0, jf1 = jy1 + 1;
+  if (dys) { jy0 = imax(jts, jds + 1); jf0 = jds + 3; }
+  if (dye) { jy1 = imin(jte, jde - 2); jf1 = jde - 3; }
+  int ix0 = its, ix1 = imin(ite, ide - 1), if0 = ix0, if1 = ix1 + 1;
+  if (dxs) { ix0 = imax(ids + 1, its); if0 = imin(ix0 + 2, ids + 3); }
+  if (dxe) { ix1 = imin(ide - 2, ite); if1 = ide - 3; }
+  // face velocity at w level k (k <= ktf) or at the lid (k = ktf+1)
+  auto wvel = [&](const float *rr, int ii, int k, int jj) -> float {
+    if (k <= ktf) return fzm[X1(d, k)] * rr[X3(d, ii, k, jj)] + fzp[X1(d, k)] * rr[X3(d, ii, k - 1, jj)];
+    return (2.f - fzm[X1(d, k - 1)]) * rr[X3(d, ii, k - 1, jj)] - fzp[X1(d, k - 1)] * rr[X3(d, ii, k - 2, jj)];
+  };
+  auto fqy = [&](int k, int jj) -> float {
+    if (jj >= jf0 && jj <= jf1) {
+      const float vel = wvel(rv, i, k, jj);
+      return vel * flux5(w[X3(d, i, k, jj - 3)], w[X3(d, i, k, jj - 2)], w[X3(d, i, k, jj - 1)], w[X3(d, i, k, jj)],
+                         w[X3(d, i, k, jj + 1)], w[X3(d, i, k, jj + 2)], vel, ts);
+    } else if (jj == jds + 1 || jj == jde - 1) {
+      return 0.5f * (wvel(rv, i, k, jj)) * (w[X3(d, i, k, jj)] + w[X3(d, i, k, jj - 1)]);
+    } else {
+      const float vel = wvel(rv, i, k, jj);
+      return vel * flux3(w[X3(d, i, k, jj - 2)], w[X3(d, i, k, jj - 1)], w[X3(d, i, k, jj)], w[X3(d, i, k, jj + 1)], vel, ts);
+    }
+  };
+  auto fqx = [&](int k, int ii) -> float {
+    if (ii >= if0 && ii <= if1) {
+      const float vel = wvel(ru, ii, k, j);
+      return vel * flux5(w[X3(d, ii - 3, k, j)], w[X3(d, ii - 2, k, j)], w[X3(d, ii - 1, k, j)], w[X3(d, ii, k, j)],
+                         w[X3(d, ii + 1, k, j)], w[X3(d, ii + 2, k, j)], vel, ts);
+    } else if (ii == ids + 1 || ii == ide - 1) {
+      return 0.5f * (wvel(ru, ii, k, j)) * (w[X3(d, ii, k, j)] + w[X3(d, ii - 1, k, j)]);
+    } else {
+      const float vel = wvel(ru, ii, k, j);
+      return vel * flux3(w[X3(d, ii - 2, k, j)], w[X3(d, ii - 1, k, j)], w[X3(d, ii, k, j)], w[X3(d, ii + 1, k, j)], vel, ts);
+    }
+  };
+  auto vflux = [&](int k) -> float {  // k in kts+1 .. ktf+1
+    if (k == kts + 1 || k == ktf + 1)
+      return 0.25f * (rom[X3(d, i, k, j)] + rom[X3(d, i, k - 1, j)]) * (w[X3(d, i, k, j)] + w[X3(d, i, k - 1, j)]);
+    const float vel = 0.5f * (rom[X3(d, i, k, j)] + rom[X3(d, i, k - 1, j)]);
+    return vel * flux3(w[X3(d, i, k - 2, j)], w[X3(d, i, k - 1, j)], w[X3(d, i, k, j)], w[X3(d, i, k + 1, j)], -vel, ts);
+  };
+  const float msfty = mass ? s.msfty[X2(d, i, j)] : 1.f;
+  const float mu = mass ? s.mu_2[X2(d, i, j)] : 0.f, mub = mass ? s.mub[X2(d, i, j)] : 0.f;
+  const float mut = mass ? s.mut[X2(d, i, j)] : 0.f;
+  // pg_buoy_w reads the moisture coefficient cqw(kde-1) before it is replaced by 1/(1+cqw)
+  const float cqw_top = mass ? s.cqw[X3(d, i, kde - 1, j)] : 0.f;
+  float vf_lo = 0.f;
+  if (mass) vf_lo = vflux(kts + 1);
+  for (int k = kts; k <= kte; k++) {
+    const int o = X3(d, i, k, j);
+    float t = 0.f;
+    if (mass) {
+      if (k >= kts + 1) {
+        if (j >= jy0 && j <= jy1) {
+          const float mrdy = s.msftx[X2(d, i, j)] * rdy;
+          t = t - mrdy * (fqy(k, j + 1) - fqy(k, j));
+        }
+        if (i >= ix0 && i <= ix1) {
+          const float mrdx = s.msftx[X2(d, i, j)] * rdx;
+          t = t - mrdx * (fqx(k, i + 1) - fqx(k, i));
+        }
+        // open boundaries
+        if (g.open_xs && i == its) {
+          float uw, dru;
+          if (k <= ktf) {
+            uw = 0.5f * (fzm[X1(d, k)] * (ru[X3(d, its, k, j)] + ru[X3(d, its + 1, k, j)]) + fzp[X1(d, k)] * (ru[X3(d, its, k - 1, j)] + ru[X3(d, its + 1, k - 1, j)]));
+            dru = fzm[X1(d, k)] * (ru[X3(d, its + 1, k, j)] - ru[X3(d, its, k, j)]) + fzp[X1(d, k)] * (ru[X3(d, its + 1, k - 1, j)] - ru[X3(d, its, k - 1, j)]);
+          } else {
+            uw = 0.5f * ((2.f - fzm[X1(d, k - 1)]) * (ru[X3(d, its, k - 1, j)] + ru[X3(d, its + 1, k - 1, j)]) - fzp[X1(d, k - 1)] * (ru[X3(d, its, k - 2, j)] + ru[X3(d, its + 1, k - 2, j)]));
+            dru = (2.f - fzm[X1(d, k - 1)]) * (ru[X3(d, its + 1, k - 1, j)] - ru[X3(d, its, k - 1, j)]) - fzp[X1(d, k - 1)] * (ru[X3(d, its + 1, k - 2, j)] - ru[X3(d, its, k - 2, j)]);
+          }
+          const float ub = uw < 0.f ? uw : 0.f;
+          t = t - rdx * (ub * (w[X3(d, its + 1, k, j)] - w[X3(d, its, k, j)]) + w[X3(d, its, k, j)] * dru);
+        }
+        if (g.open_xe && i == imin(ite, ide - 1)) {
+          const int ie = i;
+          float uw, dru;
+          if (k <= ktf) {
+            uw = 0.5f * (fzm[X1(d, k)] * (ru[X3(d, ite - 1, k, j)] + ru[X3(d, ite, k, j)]) + fzp[X1(d, k)] * (ru[X3(d, ite - 1, k - 1, j)] + ru[X3(d, ite, k - 1, j)]));
+            dru = fzm[X1(d, k)] * (ru[X3(d, ite, k, j)] - ru[X3(d, ite - 1, k, j)]) + fzp[X1(d, k)] * (ru[X3(d, ite, k - 1, j)] - ru[X3(d, ite - 1, k - 1, j)]);
+          } else {
+            uw = 0.5f * ((2.f - fzm[X1(d, k - 1)]) * (ru[X3(d, ite - 1, k - 1, j)] + ru[X3(d, ite, k - 1, j)]) - fzp[X1(d, k - 1)] * (ru[X3(d, ite - 1, k - 2, j)] + ru[X3(d, ite, k - 2, j)]));
+            dru = (2.f - fzm[X1(d, k - 1)]) * (ru[X3(d, ite, k - 1, j)] - ru[X3(d, ite - 1, k - 1, j)]) - fzp[X1(d, k - 1)] * (ru[X3(d, ite, k - 2, j)] - ru[X3(d, ite - 1, k - 2, j)]);
+          }
+          const float ub = uw > 0.f ? uw : 0.f;
+          t = t - rdx * (ub * (w[X3(d, ie, k, j)] - w[X3(d, ie - 1, k, j)]) + w[X3(d, ie, k, j)] * dru);
+        }
+        if (g.open_ys && j == jts) {
+          float vw, drv;
+          if (k <= ktf) {
+            vw = 0.5f * (fzm[X1(d, k)] * (rv[X3(d, i, k, jts)] + rv[X3(d, i, k, jts + 1)]) + fzp[X1(d, k)] * (rv[X3(d, i, k - 1, jts)] + rv[X3(d, i, k - 1, jts + 1)]));
+            drv = fzm[X1(d, k)] * (rv[X3(d, i, k, jts + 1)] - rv[X3(d, i, k, jts)]) + fzp[X1(d, k)] * (rv[X3(d, i, k - 1, jts + 1)] - rv[X3(d, i, k - 1, jts)]);
+          } else {
+            vw = 0.5f * ((2.f - fzm[X1(d, k - 1)]) * (rv[X3(d, i, k - 1, jts)] + rv[X3(d, i, k - 1, jts + 1)]) - fzp[X1(d, k - 1)] * (rv[X3(d, i, k - 2, jts)] + rv[X3(d, i, k - 2, jts + 1)]));
+            drv = (2.f - fzm[X1(d, k - 1)]) * (rv[X3(d, i, k - 1, jts + 1)] - rv[X3(d, i, k - 1, jts)]) - fzp[X1(d, k - 1)] * (rv[X3(d, i, k - 2, jts + 1)] - rv[X3(d, i, k - 2, jts)]);
+          }
+          const float vb = vw < 0.f ? vw : 0.f;
+          t = t - rdy * (vb * (w[X3(d, i, k, jts + 1)] - w[X3(d, i, k, jts)]) + w[X3(d, i, k, jts)] * drv);
+        }
+        if (g.open_ye && j == imin(jte, jde - 1)) {
+          const int je = j;
+          float vw, drv;
+          if (k <= ktf) {
+            vw = 0.5f * (fzm[X1(d, k)] * (rv[X3(d, i, k, jte - 1)] + rv[X3(d, i, k, jte)]) + fzp[X1(d, k)] * (rv[X3(d, i, k - 1, jte - 1)] + rv[X3(d, i, k - 1, jte)]));
+            drv = fzm[X1(d, k)] * (rv[X3(d, i, k, jte)] - rv[X3(d, i, k, jte - 1)]) + fzp[X1(d, k)] * (rv[X3(d, i, k - 1, jte)] - rv[X3(d, i, k - 1, jte - 1)]);
+          } else {
+            vw = 0.5f * ((2.f - fzm[X1(d, k - 1)]) * (rv[X3(d, i, k - 1, jte - 1)] + rv[X3(d, i, k - 1, jte)]) - fzp[X1(d, k - 1)] * (rv[X3(d, i, k - 2, jte - 1)] + rv[X3(d, i, k - 2, jte)]));
+            drv = (2.f - fzm[X1(d, k - 1)]) * (rv[X3(d, i, k - 1, jte)] - rv[X3(d, i, k - 1, jte - 1)]) - fzp[X1(d, k - 1)] * (rv[X3(d, i, k - 2, jte)] - rv[X3(d, i, k - 2, jte - 1)]);
+          }
+          const float vb = vw > 0.f ? vw : 0.f;
+          t = t - rdy * (vb * (w[X3(d, i, k, je)] - w[X3(d, i, k, je - 1)]) + w[X3(d, i, k, je)] * drv);
+        }
+        // vertical advection of w (order 3) with the lid flux term
+        if (k <= ktf) {
+          const float vf_hi = vflux(k + 1);
+          t = t - s.rdn[X1(d, k)] * (vf_hi - vf_lo);
+          vf_lo = vf_hi;
+        } else {
+          t = t + 2.f * s.rdn[X1(d, k - 1)] * (vf_lo);
+        }
+        // pg_buoy_w
+        if (k == kde) {
+          const float cq1 = 1.f / (1.f + cqw_top);
+          const float cq2 = cqw_top * cq1;
+          t = t + (1.f / msfty) * K_G *
+                      (cq1 * 2.f * s.rdnw[X1(d, k - 1)] * (-s.p[X3(d, i, k - 1, j)]) - (s.c1f[X1(d, k)] * mu) -
+                       cq2 * (s.c1f[X1(d, k)] * mub + s.c2f[X1(d, k)]));
+        } else {
+          const float cqw = s.cqw[o];
+          const float cq1 = 1.f / (1.f + cqw);
+          const float cq2 = cqw * cq1;
+          s.cqw[o] = cq1;
+          t = t + (1.f / msfty) * K_G *
+                      (cq1 * s.rdn[X1(d, k)] * (s.p[o] - s.p[X3(d, i, k - 1, j)]) - (s.c1f[X1(d, k)] * mu) -
+                       cq2 * (s.c1f[X1(d, k)] * mub + s.c2f[X1(d, k)]));
+        }
+        // w_damp
+        if (a.c.w_damping == 1 && k <= kde - 1) {
+          const float w_damp_on = a.c.zadvect_implicit > 0 ? a.c.w_crit_cfl : K_W_BETA;
+          const float cf = s.c1f[X1(d, k)] * mut + s.c2f[X1(d, k)];
+          const float vert_cfl = fabsf(s.ww[o] / cf * s.rdnw[X1(d, k)] * a.dt);
+          if (vert_cfl > w_damp_on) t = t - sgn1(w[o]) * K_W_ALPHA * (vert_cfl - a.c.w_crit_cfl) * cf;
+        }
+        // coriolis and curvature
+        if (k <= ktf) {
+          const float fm = fzm[X1(d, k)], fp = fzp[X1(d, k)];
+          t = t + s.e[X2(d, i, j)] *
+                      (s.cosa[X2(d, i, j)] * 0.5f * (fm * (ru[o] + ru[o + 1]) + fp * (ru[o - d.ni] + ru[o - d.ni + 1])) -
+                       (s.msftx[X2(d, i, j)] / msfty) * s.sina[X2(d, i, j)] * 0.5f * (fm * (rv[o] + rv[o + d.sj]) + fp * (rv[o - d.ni] + rv[o - d.ni + d.sj])));
+          const float *uu = s.u_2, *vv = s.v_2;
+          t = t + K_RERADIUS * (0.5f * (fm * (ru[o] + ru[o + 1]) + fp * (ru[o - d.ni] + ru[o - d.ni + 1])) * 0.5f *
+                                    (fm * (uu[o] + uu[o + 1]) + fp * (uu[o - d.ni] + uu[o - d.ni + 1])) +
+                                (s.msftx[X2(d, i, j)] / msfty) * 0.5f * (fm * (rv[o] + rv[o + d.sj]) + fp * (rv[o - d.ni] + rv[o - d.ni + d.sj])) *
+                                    0.5f * (fm * (vv[o] + vv[o + d.sj]) + fp * (vv[o - d.ni] + vv[o - d.ni + d.sj])));
+        }
+      }
+      // rk_addtend_dry (rw and ph parts share this loop in the reference; ph is in k_tend_t)
+      float tf = s.rw_tendf[o];
+      if (a.rk_step == 1) {
+        tf = tf + 0.f * msfty;  // w_save was zeroed by rk_tendency
+        s.rw_tendf[o] = tf;
+      }
+      t = t + tf / msfty;
+    }
+    s.rw_tend[o] = t;
+  }
+}
+
+// ------------------------------------------------------------------------------ t, ph, zeroing
+__global__ void __launch_bounds__(256) k_tend_t(KArgs a) {
+  const Dm &d = a.d;
+  const wrfmi_state &s = a.s;
+  const int i = d.ids + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int j = d.jds + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  const int ids = d.ids, ide = d.ide, jds = d.jds, jde = d.jde, kde = d.kde;
+  const int its = ids, ite = ide, jts = jds, jte = jde, kts = d.kds, kte = kde;
+  if (i > ide || j > jde) return;
+  const Geo g = geo(a.c);
+  const float ts = a.c.time_step >= 0 ? 1.f : -1.f;
+  const int ktf = kte - 1;
+  const float *f = s.t_2, *ru = s.ru, *rv = s.rv, *rom = s.ww;
+  const float rdx = s.rdx, rdy = s.rdy;
+  const bool mass = i <= imin(ite, ide - 1) && j <= imin(jte, jde - 1);
+  const bool dxs = !(g.per_x || g.sym_xs || its > ids + 3), dxe = !(g.per_x || g.sym_xe || ite < ide - 3);
+  const bool dys = !(g.per_y || g.sym_ys || jts > jds + 3), dye = !(g.per_y || g.sym_ye || jte < jde - 4);
+  int jy0 = jts, jy1 = imin(jte, jde - 1), jf0 = jy0, jf1 = jy1 + 1;
+  if (dys) { jy0 = imax(jts, jds + 1); jf0 = jds + 3; }
+  if (dye) { jy1 = imin(jte, jde - 2); jf1 = jde - 3; }
+  int ix0 = its, ix1 = imin(ite, ide - 1), if0 = ix0, if1 = ix1 + 1;
+  if (dxs) { ix0 = imax(ids + 1, its); if0 = imin(ix0 + 2, ids + 3); }
+  if (dxe) { ix1 = imin(ide - 2, ite); if1 = ide - 3; }
+  auto fqy = [&](int k, int jj) -> float {
+    if (jj >= jf0 && jj <= jf1) {
+      const float vel = rv[X3(d, i, k, jj)];
+      return vel * flux5(f[X3(d, i, k, jj - 3)], f[X3(d, i, k, jj - 2)], f[X3(d, i, k, jj - 1)], f[X3(d, i, k, jj)],
+                         f[X3(d, i, k, jj + 1)], f[X3(d, i, k, jj + 2)], vel, ts);
+    } else if (jj == jds + 1 || jj == jde - 1) {
+      return 0.5f * rv[X3(d, i, k, jj)] * (f[X3(d, i, k, jj)] + f[X3(d, i, k, jj - 1)]);
+    } else {
+      const float vel = rv[X3(d, i, k, jj)];
+      return vel * flux3(f[X3(d, i, k, jj - 2)], f[X3(d, i, k, jj - 1)], f[X3(d, i, k, jj)], f[X3(d, i, k, jj + 1)], vel, ts);
+    }
+  };
+  auto fqx = [&](int k, int ii) -> float {
+    if (ii >= if0 && ii <= if1) {
+      const float vel = ru[X3(d, ii, k, j)];
+      return vel * flux5(f[X3(d, ii - 3, k, j)], f[X3(d, ii - 2, k, j)], f[X3(d, ii - 1, k, j)], f[X3(d, ii, k, j)],
+                         f[X3(d, ii + 1, k, j)], f[X3(d, ii + 2, k, j)], vel, ts);
+    } else if (ii == ids + 1 || ii == ide - 1) {
+      return 0.5f * (ru[X3(d, ii, k, j)]) * (f[X3(d, ii, k, j)] + f[X3(d, ii - 1, k, j)]);
+    } else {
+      const float vel = ru[X3(d, ii, k, j)];
+      return vel * flux3(f[X3(d, ii - 2, k, j)], f[X3(d, ii - 1, k, j)], f[X3(d, ii, k, j)], f[X3(d, ii + 1, k, j)], vel, ts);
+    }
+  };
+  auto vflux = [&](int k) -> float {
+    if (k == kts || k == kte) return 0.f;
+    if (k == kts + 1 || k == ktf)
+      return rom[X3(d, i, k, j)] * (s.fnm[X1(d, k)] * f[X3(d, i, k, j)] + s.fnp[X1(d, k)] * f[X3(d, i, k - 1, j)]);
+    const float vel = rom[X3(d, i, k, j)];
+    return vel * flux3(f[X3(d, i, k - 2, j)], f[X3(d, i, k - 1, j)], f[X3(d, i, k, j)], f[X3(d, i, k + 1, j)], -vel, ts);
+  };
+  // rhs_ph pieces
+  const float *u = s.u_2, *v = s.v_2, *ph = s.ph_2, *phb = s.phb, *w = s.w_2;
+  const float cfn = s.cfn, cfn1 = s.cfn1;
+  const bool spec = g.spec;
+  const bool ord6 = a.c.h_sca_adv_order <= 6 && a.c.h_sca_adv_order > 4;
+  int jp0 = jts, jp1 = imin(jte, jde - 1);
+  if (g.open_ys || spec) jp0 = imax(jts, jds + 3);
+  if (g.open_ye || spec) jp1 = imin(jp1, jde - 4);
+  int ip0 = its, ip1 = imin(ite, ide - 1);
+  if (g.open_xs || spec) ip0 = imax(its, ids + 3);
+  if (g.open_xe || spec) ip1 = imin(ip1, ide - 4);
+  auto vface = [&](int k, int jj) -> float {
+    const float cc = s.c1f[X1(d, k)] * s.muv[X2(d, i, jj)] + s.c2f[X1(d, k)];
+    if (k < kte) return cc * (v[X3(d, i, k, jj)] + v[X3(d, i, k - 1, jj)]) * s.msfvy[X2(d, i, jj)];
+    return cc * (cfn * v[X3(d, i, k - 1, jj)] + cfn1 * v[X3(d, i, k - 2, jj)]) * s.msfvy[X2(d, i, jj)];
+  };
+  auto uface = [&](int k, int ii) -> float {
+    const float cc = s.c1f[X1(d, k)] * s.muu[X2(d, ii, j)] + s.c2f[X1(d, k)];
+    if (k < kte) return cc * (u[X3(d, ii, k, j)] + u[X3(d, ii, k - 1, j)]) * s.msfux[X2(d, ii, j)];
+    return cc * (cfn * u[X3(d, ii, k - 1, j)] + cfn1 * u[X3(d, ii, k - 2, j)]) * s.msfux[X2(d, ii, j)];
+  };
+  auto P = [&](const float *arr, int ii, int k, int jj) -> float { return arr[X3(d, ii, k, jj)]; };
+  const float msfty = mass ? s.msfty[X2(d, i, j)] : 1.f;
+  const float mut = mass ? s.mut[X2(d, i, j)] : 0.f;
+  float vf_lo = vflux(kts);
+  float wd_lo = 0.f;  // rhs_ph wdwn(k) carried upward
+  if (mass) {
+    const int k = 2;
+    if (a.c.phi_adv_z == 2)
+      wd_lo = s.rdnw[X1(d, k - 1)] * (P(ph, i, k, j) - P(ph, i, k - 1, j) + P(phb, i, k, j) - P(phb, i, k - 1, j));
+    else
+      wd_lo = .5f * (P(rom, i, k, j) + P(rom, i, k - 1, j)) * s.rdnw[X1(d, k - 1)] *
+              (P(ph, i, k, j) - P(ph, i, k - 1, j) + P(phb, i, k, j) - P(phb, i, k - 1, j));
+  }
+  for (int k = kts; k <= kte; k++) {
+    const int o = X3(d, i, k, j);
+    // zero_tend of the *_save arrays over the whole tile
+    s.u_save[o] = 0.f;
+    s.v_save[o] = 0.f;
+    s.w_save[o] = 0.f;
+    s.ph_save[o] = 0.f;
+    s.t_save[o] = 0.f;
+    float t = 0.f, pt = 0.f, vf_hi = 0.f;
+    if (mass) {
+      if (k <= ktf) {
+        // advect_scalar(t)
+        if (j >= jy0 && j <= jy1) {
+          const float mrdy = s.msftx[X2(d, i, j)] * rdy;
+          t = t - mrdy * (fqy(k, j + 1) - fqy(k, j));
+        }
+        if (i >= ix0 && i <= ix1) {
+          const float mrdx = s.msftx[X2(d, i, j)] * rdx;
+          t = t - mrdx * (fqx(k, i + 1) - fqx(k, i));
+        }
+        if (g.open_xs && i == its) {
+          float ub = 0.5f * (P(ru, its, k, j) + P(ru, its + 1, k, j));
+          if (ub > 0.f) ub = 0.f;
+          t = t - rdx * (ub * (P(f, its + 1, k, j) - P(f, its, k, j)) + P(f, its, k, j) * (P(ru, its + 1, k, j) - P(ru, its, k, j)));
+        }
+        if (g.open_xe && i == imin(ite, ide - 1)) {
+          float ub = 0.5f * (P(ru, ite - 1, k, j) + P(ru, ite, k, j));
+          if (ub < 0.f) ub = 0.f;
+          t = t - rdx * (ub * (P(f, i, k, j) - P(f, i - 1, k, j)) + P(f, i, k, j) * (P(ru, ite, k, j) - P(ru, ite - 1, k, j)));
+        }
+        if (g.open_ys && j == jts) {
+          float vb = 0.5f * (P(rv, i, k, jts) + P(rv, i, k, jts + 1));
+          if (vb > 0.f) vb = 0.f;
+          t = t - rdy * (vb * (P(f, i, k, jts + 1) - P(f, i, k, jts)) + P(f, i, k, jts) * (P(rv, i, k, jts + 1) - P(rv, i, k, jts)));
+        }
+        if (g.open_ye && j == imin(jte, jde - 1)) {
+          float vb = 0.5f * (P(rv, i, k, jte - 1) + P(rv, i, k, jte));
+          if (vb < 0.f) vb = 0.f;
+          t = t - rdy * (vb * (P(f, i, k, j) - P(f, i, k, j - 1)) + P(f, i, k, j) * (P(rv, i, k, jte) - P(rv, i, k, jte - 1)));
+        }
+        vf_hi = vflux(k + 1);
+        t = t - s.rdnw[X1(d, k)] * (vf_hi - vf_lo);
+        // rk_addtend_dry (theta)
+        float tf = s.t_tendf[o];
+        if (a.rk_step == 1) {
+          tf = tf + 0.f;  // t_save was zeroed by rk_tendency
+          s.t_tendf[o] = tf;
+        }
+        t = t + tf / msfty + (s.c1h[X1(d, k)] * mut + s.c2h[X1(d, k)]) * s.h_diabatic[o] / msfty;
+      }
+      // rhs_ph
+      if (k >= 2 && k <= kte - 1) {
+        float wd_hi;
+        if (a.c.phi_adv_z == 2) {
+          wd_hi = s.rdnw[X1(d, k)] * (P(ph, i, k + 1, j) - P(ph, i, k, j) + P(phb, i, k + 1, j) - P(phb, i, k, j));
+          pt = pt - P(rom, i, k, j) * (s.fnm[X1(d, k)] * wd_hi + s.fnp[X1(d, k)] * wd_lo);
+        } else {
+          wd_hi = .5f * (P(rom, i, k + 1, j) + P(rom, i, k, j)) * s.rdnw[X1(d, k)] *
+                  (P(ph, i, k + 1, j) - P(ph, i, k, j) + P(phb, i, k + 1, j) - P(phb, i, k, j));
+          pt = pt - (s.fnm[X1(d, k)] * wd_hi + s.fnp[X1(d, k)] * wd_lo);
+        }
+        wd_lo = wd_hi;
+      }
+      if (a.c.non_hydrostatic) {
+        if (k == kde) pt = 0.f;
+        if (k >= 2) pt = pt + (s.c1f[X1(d, k)] * mut + s.c2f[X1(d, k)]) * K_G * w[o] / msfty;
+      }
+      if (ord6 && k >= 2) {
+        const float cy = k < kte ? (0.25f * rdy / msfty) : (0.5f * rdy / msfty);
+        const float cx = k < kte ? (0.25f * rdx / msfty) : (0.5f * rdx / msfty);
+        if (j >= jp0 && j <= jp1) {
+          const float S = 45.f * (P(ph, i, k, j + 1) - P(ph, i, k, j - 1)) - 9.f * (P(ph, i, k, j + 2) - P(ph, i, k, j - 2)) +
+                          (P(ph, i, k, j + 3) - P(ph, i, k, j - 3)) + 45.f * (P(phb, i, k, j + 1) - P(phb, i, k, j - 1)) -
+                          9.f * (P(phb, i, k, j + 2) - P(phb, i, k, j - 2)) + (P(phb, i, k, j + 3) - P(phb, i, k, j - 3));
+          pt = pt - cy * ((vface(k, j + 1) + vface(k, j)) * (1.f / 60.f) * S);
+        }
+        if (((g.open_ys || spec) && j == jds + 2) || ((g.open_ye || spec) && j == jde - 3)) {
+          const float S = 8.f * (P(ph, i, k, j + 1) - P(ph, i, k, j - 1)) - (P(ph, i, k, j + 2) - P(ph, i, k, j - 2)) +
+                          8.f * (P(phb, i, k, j + 1) - P(phb, i, k, j - 1)) - (P(phb, i, k, j + 2) - P(phb, i, k, j - 2));
+          pt = pt - cy * ((vface(k, j + 1) + vface(k, j)) * (1.f / 12.f) * S);
+        }
+        if (((g.open_ys || spec) && j == jds + 1) || ((g.open_ye || spec) && j == jde - 2)) {
+          const float dA = P(phb, i, k, j + 1) - P(phb, i, k, j) + P(ph, i, k, j + 1) - P(ph, i, k, j);
+          const float dB = P(phb, i, k, j) - P(phb, i, k, j - 1) + P(ph, i, k, j) - P(ph, i, k, j - 1);
+          pt = pt - cy * (vface(k, j + 1) * dA + vface(k, j) * dB);
+        }
+        if (i >= ip0 && i <= ip1) {
+          const float S = 45.f * (P(ph, i + 1, k, j) - P(ph, i - 1, k, j)) - 9.f * (P(ph, i + 2, k, j) - P(ph, i - 2, k, j)) +
+                          (P(ph, i + 3, k, j) - P(ph, i - 3, k, j)) + 45.f * (P(phb, i + 1, k, j) - P(phb, i - 1, k, j)) -
+                          9.f * (P(phb, i + 2, k, j) - P(phb, i - 2, k, j)) + (P(phb, i + 3, k, j) - P(phb, i - 3, k, j));
+          pt = pt - cx * ((uface(k, i + 1) + uface(k, i)) * (1.f / 60.f) * S);
+        }
+        if ((g.open_xs && i == ids + 2) || (g.open_xe && i == ide - 3)) {
+          const float S = 8.f * (P(ph, i + 1, k, j) - P(ph, i - 1, k, j)) - (P(ph, i + 2, k, j) - P(ph, i - 2, k, j)) +
+                          8.f * (P(phb, i + 1, k, j) - P(phb, i - 1, k, j)) - (P(phb, i + 2, k, j) - P(phb, i - 2, k, j));
+          pt = pt - cx * ((uface(k, i + 1) + uface(k, i)) * (1.f / 12.f) * S);
+        }
+        if (((g.open_xs || spec) && i == ids + 1) || ((g.open_xe || spec) && i == ide - 2)) {
+          const float dA = P(phb, i + 1, k, j) - P(phb, i, k, j) + P(ph, i + 1, k, j) - P(ph, i, k, j);
+          const float dB = P(phb, i, k, j) - P(phb, i - 1, k, j) + P(ph, i, k, j) - P(ph, i - 1, k, j);
+          pt = pt - cx * (uface(k, i + 1) * dA + uface(k, i) * dB);
+        }
+      }
+      if (k >= 2) {
+        const float cc = s.c1f[X1(d, k)] * mut + s.c2f[X1(d, k)];
+        if (g.open_ys && j == jts) {
+          const int kz = imin(k, kde - 1);
+          const float vb = .5f * (s.fnm[X1(d, kz)] * (P(v, i, kz, j + 1) + P(v, i, kz, j)) + s.fnp[X1(d, kz)] * (P(v, i, kz - 1, j + 1) + P(v, i, kz - 1, j)));
+          const float vl = vb < 0.f ? vb : 0.f;
+          pt = pt - rdy * cc * (vl * (P(ph, i, k, j + 1) - P(ph, i, k, j)));
+        }
+        if (g.open_ye && j == jte - 1) {
+          const int kz = imin(k, kde - 1);
+          const float vb = .5f * (s.fnm[X1(d, kz)] * (P(v, i, kz, j + 1) + P(v, i, kz, j)) + s.fnp[X1(d, kz)] * (P(v, i, kz - 1, j + 1) + P(v, i, kz - 1, j)));
+          const float vr = vb > 0.f ? vb : 0.f;
+          pt = pt - rdy * cc * (vr * (P(ph, i, k, j) - P(ph, i, k, j - 1)));
+        }
+        if (g.open_xs && i == its) {
+          const int kz = k;
+          const float ub = .5f * (s.fnm[X1(d, kz)] * (P(u, i + 1, kz, j) + P(u, i, kz, j)) + s.fnp[X1(d, kz)] * (P(u, i + 1, kz - 1, j) + P(u, i, kz - 1, j)));
+          const float ul = ub < 0.f ? ub : 0.f;
+          pt = pt - (s.msftx[X2(d, i, j)] / msfty) * rdx * cc * (ul * (P(ph, i + 1, k, j) - P(ph, i, k, j)));
+        }
+        if (g.open_xe && i == ite - 1) {
+          const int kz = k < kde ? k : k - 1;
+          const float ub = .5f * (s.fnm[X1(d, kz)] * (P(u, i + 1, kz, j) + P(u, i, kz, j)) + s.fnp[X1(d, kz)] * (P(u, i + 1, kz - 1, j) + P(u, i, kz - 1, j)));
+          const float ur = ub > 0.f ? ub : 0.f;
+          pt = pt - (s.msftx[X2(d, i, j)] / msfty) * rdx * cc * (ur * (P(ph, i, k, j) - P(ph, i - 1, k, j)));
+        }
+      }
+      float ptf = s.ph_tendf[o];
+      if (a.rk_step == 1) {
+        ptf = ptf + 0.f;  // ph_save was zeroed by rk_tendency
+        s.ph_tendf[o] = ptf;
+      }
+      pt = pt + ptf / msfty;
+    }
+    s.t_tend[o] = t;
+    s.ph_tend[o] = pt;
+    vf_lo = vf_hi;
+  }
+  s.mu_tend[X2(d, i, j)] = 0.f;
+  s.mu_save[X2(d, i, j)] = 0.f;
+  if (mass) s.mu_tend[X2(d, i, j)] = 0.f + s.mu_tendf[X2(d, i, j)];
+}
+
+}  // namespace wrfmi

@@ -1,0 +1,20 @@
+// Kernel entry points of the dynamics step, shared by the kernel sources and the host driver.
+#pragma once
+#include "dev_common.h"
+
+namespace wrfmi {
+template <bool COMP> __global__ void k_step_prep(KArgs);
+__global__ void k_tend_u(KArgs);
+__global__ void k_tend_v(KArgs);
+__global__ void k_tend_w(KArgs);
+__global__ void k_tend_t(KArgs);
+template <typename CT> __global__ void k_small_prep(KArgs);
+__global__ void k_advance_uv(KArgs);
+template <typename CT, bool COMP> __global__ void k_acoustic_col(KArgs);
+__global__ void k_small_finish(KArgs);
+__global__ void k_adv_scalar(KArgs);
+__global__ void k_update_scalar(KArgs);
+__global__ void k_p_rho_phi(KArgs);
+__global__ void k_halo(HaloArgs);
+__global__ void k_count_nonfinite(const float *a, Dm d, int kend, int is3d, unsigned long long *cnt);
+}  // namespace wrfmi

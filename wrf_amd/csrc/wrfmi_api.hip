@@ -1,0 +1,590 @@
+// C-ABI of the MI355X WRF-ARW dynamics step (declarations in include/wrfmi.h).
+// Host driver: the call order of dyn_em/solve_em.F:578-4540 for the implemented options,
+// expressed as a sequence of kernel launches on one HIP stream (optionally replayed from a
+// captured hipGraph), with halo fills at the points where the reference calls
+// set_physical_bc3d/2d.
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+#include <math.h>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+#include "kernels.h"
+
+
+using namespace wrfmi;
+
+static thread_local std::string g_err;
+static std::mutex g_mu;
+
+static int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+static int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+#define HIPCHK(x)                                                                              \
+  do {                                                                                         \
+    hipError_t e_ = (x);                                                                       \
+    if (e_ != hipSuccess) return fail(WRFMI_E_DEVICE, "%s: %s", #x, hipGetErrorString(e_));   \
+  } while (0)
+
+// ------------------------------------------------------------------ registry of host arrays
+struct Mirror { void *dev; size_t bytes; };
+static std::map<uintptr_t, Mirror> g_reg;  // host base address -> mirror
+
+static void *translate(void *p) {
+  if (!p) return p;
+  uintptr_t a = (uintptr_t)p;
+  auto it = g_reg.upper_bound(a);
+  if (it == g_reg.begin()) return p;
+  --it;
+  if (a >= it->first && a < it->first + it->second.bytes) return (char *)it->second.dev + (a - it->first);
+  return p;
+}
+
+extern "C" int wrfmi_register_field(void *host_ptr, size_t bytes) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!host_ptr || !bytes) return fail(WRFMI_E_POINTER, "register: null pointer or zero size");
+  if (g_reg.count((uintptr_t)host_ptr)) return 0;
+  void *dev = nullptr;
+  HIPCHK(hipMalloc(&dev, bytes));
+  HIPCHK(hipMemcpy(dev, host_ptr, bytes, hipMemcpyHostToDevice));
+  g_reg[(uintptr_t)host_ptr] = Mirror{dev, bytes};
+  return 0;
+}
+extern "C" int wrfmi_unregister_field(void *host_ptr) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto it = g_reg.find((uintptr_t)host_ptr);
+  if (it == g_reg.end()) return fail(WRFMI_E_POINTER, "unregister: pointer was not registered");
+  HIPCHK(hipFree(it->second.dev));
+  g_reg.erase(it);
+  return 0;
+}
+extern "C" int wrfmi_sync_to_device(void *host_ptr) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto it = g_reg.find((uintptr_t)host_ptr);
+  if (it == g_reg.end()) return fail(WRFMI_E_POINTER, "sync_to_device: pointer was not registered");
+  HIPCHK(hipMemcpy(it->second.dev, host_ptr, it->second.bytes, hipMemcpyHostToDevice));
+  return 0;
+}
+extern "C" int wrfmi_sync_to_host(void *host_ptr) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto it = g_reg.find((uintptr_t)host_ptr);
+  if (it == g_reg.end()) return fail(WRFMI_E_POINTER, "sync_to_host: pointer was not registered");
+  HIPCHK(hipDeviceSynchronize());
+  HIPCHK(hipMemcpy(host_ptr, it->second.dev, it->second.bytes, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+// ------------------------------------------------------------------ library
+static hipStream_t g_stream = nullptr;
+static unsigned long long *g_counter = nullptr;
+
+extern "C" int wrfmi_init(void) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!g_stream) HIPCHK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+  if (!g_counter) HIPCHK(hipMalloc(&g_counter, sizeof(unsigned long long)));
+  return 0;
+}
+struct GraphEntry;
+static std::vector<GraphEntry *> g_graphs;
+static void free_graphs();
+extern "C" int wrfmi_finalize(void) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  free_graphs();
+  for (auto &kv : g_reg) (void)hipFree(kv.second.dev);
+  g_reg.clear();
+  if (g_counter) { (void)hipFree(g_counter); g_counter = nullptr; }
+  if (g_stream) { (void)hipStreamDestroy(g_stream); g_stream = nullptr; }
+  return 0;
+}
+extern "C" const char *wrfmi_last_error(void) { return g_err.c_str(); }
+extern "C" const char *wrfmi_version(void) { return "1.0"; }
+extern "C" int wrfmi_state_field_count(void) {
+  int n = 0;
+#define CNT(x) n++;
+  WRFMI_FIELDS_3D(CNT) WRFMI_FIELDS_2D(CNT) WRFMI_FIELDS_1D(CNT)
+#undef CNT
+  return n;
+}
+extern "C" size_t wrfmi_state_sizeof(void) { return sizeof(wrfmi_state); }
+
+// ------------------------------------------------------------------ validation
+static int check_cfg(const wrfmi_config *c) {
+  if (c->rk_ord != 3) return fail(WRFMI_E_BADOPT, "rk_ord=%d: only the third-order Runge-Kutta step is implemented", c->rk_ord);
+  if (c->h_mom_adv_order != 5 || c->v_mom_adv_order != 3 || c->h_sca_adv_order != 5 || c->v_sca_adv_order != 3)
+    return fail(WRFMI_E_BADOPT, "advection orders h/v = %d/%d (momentum), %d/%d (scalar): only 5/3 is implemented",
+                c->h_mom_adv_order, c->v_mom_adv_order, c->h_sca_adv_order, c->v_sca_adv_order);
+  if (c->specified || c->nested || c->polar || c->symmetric_xs || c->symmetric_xe || c->symmetric_ys || c->symmetric_ye)
+    return fail(WRFMI_E_BADOPT, "lateral boundaries: periodic and open are implemented (not specified/nested/polar/symmetric)");
+  if (!c->periodic_x && !(c->open_xs && c->open_xe)) return fail(WRFMI_E_BADOPT, "x boundaries must be periodic or open on both sides");
+  if (!c->periodic_y && !(c->open_ys && c->open_ye)) return fail(WRFMI_E_BADOPT, "y boundaries must be periodic or open on both sides");
+  if (c->moist_adv_opt != 0 || c->scalar_adv_opt != 0 || c->momentum_adv_opt != 1)
+    return fail(WRFMI_E_BADOPT, "advection options: only the original (non-PD, non-WENO) fluxes are implemented");
+  if (!c->non_hydrostatic) return fail(WRFMI_E_BADOPT, "hydrostatic mode is not implemented");
+  if (c->diff_opt != 0 || c->diff_6th_opt != 0 || c->damp_opt == 3 || c->zadvect_implicit || c->pert_coriolis || c->map_proj == 6)
+    return fail(WRFMI_E_BADOPT, "diffusion, damp_opt=3, IEVA, perturbation Coriolis and lat-lon grids are outside the implemented set");
+  if (c->hypsometric_opt != 1 && c->hypsometric_opt != 2) return fail(WRFMI_E_BADOPT, "hypsometric_opt must be 1 or 2");
+  if (c->precision_mode != 0 && c->precision_mode != 1) return fail(WRFMI_E_BADOPT, "precision_mode must be 0 or 1");
+  return 0;
+}
+
+static int make_dm(const wrfmi_dims *w, Dm *d) {
+  if (w->its != w->ids || w->ite != w->ide || w->jts != w->jds || w->jte != w->jde || w->kts != w->kds || w->kte != w->kde)
+    return fail(WRFMI_E_DIMS, "the GPU path processes the whole patch: tile must equal the domain");
+  if (w->ids - w->ims < 4 || w->jds - w->jms < 4 || w->ime - w->ide < 4 || w->jme - w->jde < 4)
+    return fail(WRFMI_E_DIMS, "memory dims need a halo of at least 4 (bdyzone) around the domain");
+  if (w->kms != w->kds || w->kme < w->kde) return fail(WRFMI_E_DIMS, "memory k range must start at kds and reach kde");
+  if (w->kde - w->kds < 4 || w->ide - w->ids < 7 || w->jde - w->jds < 7) return fail(WRFMI_E_DIMS, "domain too small");
+  d->ids = w->ids; d->ide = w->ide; d->jds = w->jds; d->jde = w->jde; d->kds = w->kds; d->kde = w->kde;
+  d->ims = w->ims; d->ime = w->ime; d->jms = w->jms; d->jme = w->jme; d->kms = w->kms; d->kme = w->kme;
+  long long ni = w->ime - w->ims + 1, nk = w->kme - w->kms + 1, nj = w->jme - w->jms + 1;
+  if (ni * nk * nj >= (1LL << 31)) return fail(WRFMI_E_DIMS, "domain of 2^31 or more elements per field");
+  d->ni = (int)ni; d->nk = (int)nk; d->sj = (int)(ni * nk);
+  return 0;
+}
+
+static int translate_state(const wrfmi_state *in, wrfmi_state *out) {
+  *out = *in;
+#define TR(n) out->n = (float *)translate((void *)in->n); if (!out->n) return fail(WRFMI_E_POINTER, "state field %s is null", #n);
+  WRFMI_FIELDS_3D(TR) WRFMI_FIELDS_2D(TR) WRFMI_FIELDS_1D(TR)
+#undef TR
+  out->a = translate(in->a); out->alpha = translate(in->alpha); out->gamma = translate(in->gamma);
+  if (!out->a || !out->alpha || !out->gamma) return fail(WRFMI_E_POINTER, "tridiagonal coefficient arrays are null");
+  if (in->n_moist < 0 || in->n_species < in->n_moist || in->n_species > WRFMI_MAX_SPECIES)
+    return fail(WRFMI_E_BADOPT, "n_moist=%d n_species=%d out of range", in->n_moist, in->n_species);
+  for (int m = 0; m < in->n_species; m++) {
+    out->q[m] = (float *)translate(in->q[m]);
+    out->q_old[m] = (float *)translate(in->q_old[m]);
+    out->q_tend[m] = (float *)translate(in->q_tend[m]);
+    if (!out->q[m] || !out->q_old[m] || !out->q_tend[m]) return fail(WRFMI_E_POINTER, "species %d arrays are null", m);
+  }
+  // every pointer must now be device-accessible
+  hipPointerAttribute_t at;
+  if (hipPointerGetAttributes(&at, out->u_2) != hipSuccess || at.type != hipMemoryTypeDevice) {
+    (void)hipGetLastError();
+    return fail(WRFMI_E_POINTER, "u_2 is neither a device pointer nor a registered host array");
+  }
+  return 0;
+}
+
+// ------------------------------------------------------------------ launch helpers
+static inline dim3 col_grid(const Dm &d, int i0, int j0, dim3 blk) {
+  int ni = d.ide - i0 + 1, nj = d.jde - j0 + 1;
+  return dim3((ni + blk.x - 1) / blk.x, (nj + blk.y - 1) / blk.y, 1);
+}
+static const dim3 kColBlk(64, 4, 1);
+
+struct Launcher {
+  hipStream_t st;
+  KArgs a;
+  HaloArgs h;
+  int err = 0;
+  void halo_begin() { h.n = 0; }
+  void halo_add(float *p, char kind, int is3d) {
+    int k = kind == 'u' ? 1 : kind == 'v' ? 2 : kind == 'w' ? 3 : 0;
+    h.f[h.n++] = HaloField{p, k, is3d};
+    if (h.n == 16) halo_flush();
+  }
+  void halo_flush() {
+    if (h.n == 0) return;
+    const Dm &d = a.d;
+    dim3 blk(256, 1, 1);
+    dim3 grd((d.ime - d.ims + 1 + 255) / 256, d.jme - d.jms + 1, h.n);
+    hipLaunchKernelGGL(k_halo, grd, blk, 0, st, h);
+    h.n = 0;
+  }
+};
+
+// ------------------------------------------------------------------ phases of one RK substep
+// Each enqueues the kernels of one group of solve_em calls, then the halo fills the
+// reference makes after that group.
+static int num_sound_steps(const wrfmi_config &c, const wrfmi_state &s) {
+  // solve_em.F:446-463
+  if (c.time_step_sound != 0) return c.time_step_sound;
+  float spacing = s.dx < s.dy ? s.dx : s.dy;
+  int n = 2 * ((int)(300.f * s.dt / spacing - 0.01f) + 1);
+  return n < 4 ? 4 : n;
+}
+static inline dim3 g_prep(const Dm &d) { return col_grid(d, d.ids - 1, d.jds - 1, kColBlk); }
+static inline dim3 g_col(const Dm &d) { return col_grid(d, d.ids, d.jds, kColBlk); }
+static inline dim3 g_pt(const Dm &d) { return dim3(((d.ide - d.ids + 1) + 63) / 64, d.kde - d.kds, d.jde - d.jds + 1); }
+static inline dim3 g_mass(const Dm &d) { return dim3(((d.ide - d.ids) + 63) / 64, d.kde - d.kds, d.jde - d.jds); }
+
+static void ph_step_prep(Launcher &L) {
+  const KArgs &a = L.a;
+  const wrfmi_state &s = a.s;
+  if (a.c.precision_mode == 1) hipLaunchKernelGGL(k_step_prep<true>, g_prep(a.d), kColBlk, 0, L.st, a);
+  else hipLaunchKernelGGL(k_step_prep<false>, g_prep(a.d), kColBlk, 0, L.st, a);
+  L.halo_begin();
+  L.halo_add(s.ru, 'u', 1); L.halo_add(s.rv, 'v', 1); L.halo_add(s.rw, 'w', 1); L.halo_add(s.ww, 'w', 1);
+  L.halo_add(s.php, 'w', 1); L.halo_add(s.alt, 't', 1); L.halo_add(s.p, 'p', 1);
+  L.halo_add(s.muu, 'u', 0); L.halo_add(s.muv, 'v', 0); L.halo_add(s.mut, 't', 0);
+  L.halo_add(s.al, 'p', 1); L.halo_add(s.ph_2, 'w', 1);
+  L.halo_flush();
+}
+static void ph_tendency(Launcher &L) {
+  const KArgs &a = L.a;
+  hipLaunchKernelGGL(k_tend_u, g_col(a.d), kColBlk, 0, L.st, a);
+  hipLaunchKernelGGL(k_tend_v, g_col(a.d), kColBlk, 0, L.st, a);
+  hipLaunchKernelGGL(k_tend_w, g_col(a.d), kColBlk, 0, L.st, a);
+  hipLaunchKernelGGL(k_tend_t, g_col(a.d), kColBlk, 0, L.st, a);
+}
+static void ph_small_prep(Launcher &L) {
+  const KArgs &a = L.a;
+  const wrfmi_state &s = a.s;
+  if (a.c.precision_mode == 1) hipLaunchKernelGGL(k_small_prep<double>, g_col(a.d), kColBlk, 0, L.st, a);
+  else hipLaunchKernelGGL(k_small_prep<float>, g_col(a.d), kColBlk, 0, L.st, a);
+  L.halo_begin();
+  L.halo_add(s.ru_tend, 'u', 1); L.halo_add(s.rv_tend, 'v', 1); L.halo_add(s.ph_2, 'w', 1);
+  L.halo_add(s.al, 'p', 1); L.halo_add(s.p, 'p', 1); L.halo_add(s.t_1, 'p', 1); L.halo_add(s.t_save, 't', 1);
+  L.halo_add(s.mu_1, 't', 0); L.halo_add(s.mu_2, 't', 0); L.halo_add(s.mudf, 't', 0);
+  L.halo_flush();
+}
+static void ph_advance_uv(Launcher &L) {
+  hipLaunchKernelGGL(k_advance_uv, g_pt(L.a.d), dim3(64, 1, 1), 0, L.st, L.a);
+}
+static void ph_advance_mu_t_w(Launcher &L) {
+  const KArgs &a = L.a;
+  const wrfmi_state &s = a.s;
+  if (a.c.precision_mode == 1) hipLaunchKernelGGL((k_acoustic_col<double, true>), g_col(a.d), kColBlk, 0, L.st, a);
+  else hipLaunchKernelGGL((k_acoustic_col<float, false>), g_col(a.d), kColBlk, 0, L.st, a);
+  L.halo_begin();
+  L.halo_add(s.ph_2, 'w', 1); L.halo_add(s.al, 'p', 1); L.halo_add(s.p, 'p', 1);
+  L.halo_add(s.muts, 't', 0); L.halo_add(s.mu_2, 't', 0); L.halo_add(s.mudf, 't', 0);
+  L.halo_flush();
+}
+static void ph_small_finish(Launcher &L) {
+  const KArgs &a = L.a;
+  const wrfmi_state &s = a.s;
+  hipLaunchKernelGGL(k_small_finish, g_col(a.d), kColBlk, 0, L.st, a);
+  if (a.rk_step == 3) {
+    L.halo_begin();
+    L.halo_add(s.ru_m, 'u', 1); L.halo_add(s.rv_m, 'v', 1); L.halo_add(s.ww_m, 'w', 1);
+    L.halo_add(s.mut, 't', 0); L.halo_add(s.muts, 't', 0);
+    L.halo_flush();
+  }
+}
+static void ph_scalar(Launcher &L) {
+  hipLaunchKernelGGL(k_adv_scalar, g_col(L.a.d), kColBlk, 0, L.st, L.a);
+  hipLaunchKernelGGL(k_update_scalar, g_mass(L.a.d), dim3(64, 1, 1), 0, L.st, L.a);
+}
+static void ph_p_rho_phi(Launcher &L) {
+  hipLaunchKernelGGL(k_p_rho_phi, g_mass(L.a.d), dim3(64, 1, 1), 0, L.st, L.a);
+}
+static void ph_bc_dry_2(Launcher &L, int kind) {
+  const wrfmi_state &s = L.a.s;
+  L.halo_begin();
+  if (kind == 0) {
+    L.halo_add(s.u_2, 'u', 1); L.halo_add(s.v_2, 'v', 1); L.halo_add(s.w_2, 'w', 1);
+    L.halo_add(s.t_2, 'p', 1); L.halo_add(s.ph_2, 'w', 1); L.halo_add(s.mu_2, 't', 0);
+  } else {
+    L.halo_add(s.u_1, 'u', 1); L.halo_add(s.u_2, 'u', 1); L.halo_add(s.v_1, 'v', 1); L.halo_add(s.v_2, 'v', 1);
+    L.halo_add(s.w_1, 'w', 1); L.halo_add(s.w_2, 'w', 1); L.halo_add(s.t_1, 'p', 1); L.halo_add(s.t_2, 'p', 1);
+    L.halo_add(s.ph_1, 'w', 1); L.halo_add(s.ph_2, 'w', 1); L.halo_add(s.mu_1, 't', 0); L.halo_add(s.mu_2, 't', 0);
+  }
+  for (int m = 0; m < s.n_species; m++) L.halo_add(s.q[m], 'p', 1);
+  L.halo_flush();
+}
+
+// RK substep sizes of solve_em.F:601-615
+static void set_substep(KArgs &a, int rk, int nsound) {
+  const float dt = a.s.dt;
+  a.rk_step = rk;
+  a.dt = dt;
+  if (rk == 1) { a.dt_rk = dt / 3.f; a.dts = a.dt_rk; a.nsmall = 1; }
+  else if (rk == 2) { a.dt_rk = 0.5f * dt; a.dts = dt / (float)nsound; a.nsmall = nsound / 2; }
+  else { a.dt_rk = dt; a.dts = dt / (float)nsound; a.nsmall = nsound; }
+}
+
+static void enqueue_step(Launcher &L) {
+  KArgs &a = L.a;
+  const int nsound = num_sound_steps(a.c, a.s);
+  for (int rk = 1; rk <= 3; rk++) {
+    set_substep(a, rk, nsound);
+    ph_step_prep(L);
+    ph_tendency(L);
+    ph_small_prep(L);
+    for (int it = 1; it <= a.nsmall; it++) {
+      a.iteration = it;
+      ph_advance_uv(L);
+      ph_advance_mu_t_w(L);
+    }
+    ph_small_finish(L);
+    for (int m = 0; m < a.s.n_species; m++) {
+      a.species = m;
+      ph_scalar(L);
+    }
+    ph_p_rho_phi(L);
+    if (rk < 3) ph_bc_dry_2(L, 0);
+  }
+  // solve_em.F:4195 recomputes calc_p_rho_phi from inputs that have not changed since the
+  // last substep's call (same values), then set_phys_bc_dry_2 and the species fills follow.
+  ph_bc_dry_2(L, 1);
+}
+
+static int prepare(const wrfmi_state *st_in, const wrfmi_config *cfg, const wrfmi_dims *dims, Launcher &L, wrfmi_state &s) {
+  int rc;
+  if ((rc = check_cfg(cfg))) return rc;
+  Dm d;
+  if ((rc = make_dm(dims, &d))) return rc;
+  if ((rc = translate_state(st_in, &s))) return rc;
+  memset(&L.a, 0, sizeof(L.a));
+  L.a.s = s;
+  L.a.c = *cfg;
+  L.a.d = d;
+  L.h.d = d;
+  L.h.periodic_x = cfg->periodic_x; L.h.periodic_y = cfg->periodic_y;
+  L.h.open_xs = cfg->open_xs; L.h.open_xe = cfg->open_xe; L.h.open_ys = cfg->open_ys; L.h.open_ye = cfg->open_ye;
+  return 0;
+}
+
+extern "C" int wrfmi_solve_em_step(wrfmi_state *st, const wrfmi_config *cfg, const wrfmi_dims *dims, void *stream, int sync) {
+  if (!st || !cfg || !dims) return fail(WRFMI_E_POINTER, "null argument");
+  Launcher L;
+  wrfmi_state s;
+  int rc;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if ((rc = prepare(st, cfg, dims, L, s))) return rc;
+  }
+  L.st = (hipStream_t)stream;
+  enqueue_step(L);
+  HIPCHK(hipGetLastError());
+  if (sync) HIPCHK(hipStreamSynchronize(L.st));
+  return 0;
+}
+
+// ------------------------------------------------------------------ hipGraph replay
+struct GraphEntry {
+  wrfmi_state s;
+  wrfmi_config c;
+  wrfmi_dims d;
+  hipGraph_t g;
+  hipGraphExec_t x;
+};
+static void free_graphs() {
+  for (GraphEntry *e : g_graphs) {
+    (void)hipGraphExecDestroy(e->x);
+    (void)hipGraphDestroy(e->g);
+    delete e;
+  }
+  g_graphs.clear();
+}
+
+extern "C" int wrfmi_solve_em_step_graph(wrfmi_state *st, const wrfmi_config *cfg, const wrfmi_dims *dims, void *stream, int sync) {
+  if (!st || !cfg || !dims) return fail(WRFMI_E_POINTER, "null argument");
+  hipStream_t hs = (hipStream_t)stream;
+  GraphEntry *hit = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    for (GraphEntry *e : g_graphs)
+      if (!memcmp(&e->s, st, sizeof(*st)) && !memcmp(&e->c, cfg, sizeof(*cfg)) && !memcmp(&e->d, dims, sizeof(*dims))) hit = e;
+  }
+  if (!hit) {
+    Launcher L;
+    wrfmi_state s;
+    int rc;
+    {
+      std::lock_guard<std::mutex> lk(g_mu);
+      if ((rc = prepare(st, cfg, dims, L, s))) return rc;
+      if (!g_stream) HIPCHK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+    }
+    hipStream_t cs = g_stream;
+    L.st = cs;
+    hipGraph_t g;
+    HIPCHK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    enqueue_step(L);
+    HIPCHK(hipStreamEndCapture(cs, &g));
+    hipGraphExec_t x;
+    HIPCHK(hipGraphInstantiate(&x, g, nullptr, nullptr, 0));
+    hit = new GraphEntry{*st, *cfg, *dims, g, x};
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_graphs.push_back(hit);
+  }
+  HIPCHK(hipGraphLaunch(hit->x, hs));
+  if (sync) HIPCHK(hipStreamSynchronize(hs));
+  return 0;
+}
+
+// ------------------------------------------------------------------ per-routine entry points
+// Common wrapper: validate, translate pointers, set the substep fields, enqueue one phase.
+template <typename F>
+static int run_phase(wrfmi_state *st, const wrfmi_config *cfg, const wrfmi_dims *dims, void *stream, F &&body) {
+  if (!st || !cfg || !dims) return fail(WRFMI_E_POINTER, "null argument");
+  Launcher L;
+  wrfmi_state s;
+  int rc;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if ((rc = prepare(st, cfg, dims, L, s))) return rc;
+  }
+  L.st = (hipStream_t)stream;
+  if ((rc = body(L))) return rc;
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+static int check_rk(int rk_step) {
+  return (rk_step < 1 || rk_step > 3) ? fail(WRFMI_E_BADOPT, "rk_step=%d outside 1..3", rk_step) : 0;
+}
+
+extern "C" int wrfmi_rk_step_prep(wrfmi_state *st, const wrfmi_config *cfg, const wrfmi_dims *d, int rk_step, void *stream) {
+  return run_phase(st, cfg, d, stream, [&](Launcher &L) {
+    if (int rc = check_rk(rk_step)) return rc;
+    L.a.rk_step = rk_step;
+    ph_step_prep(L);
+    return 0;
+  });
+}
+extern "C" int wrfmi_rk_tendency(wrfmi_state *st, const wrfmi_config *cfg, const wrfmi_dims *d, int rk_step, void *stream) {
+  return run_phase(st, cfg, d, stream, [&](Launcher &L) {
+    if (int rc = check_rk(rk_step)) return rc;
+    L.a.rk_step = rk_step;
+    ph_tendency(L);
+    return 0;
+  });
+}
+extern "C" int wrfmi_small_step_prep(wrfmi_state *st, const wrfmi_config *cfg, const wrfmi_dims *d, int rk_step, float dts,
+                                     void *stream) {
+  return run_phase(st, cfg, d, stream, [&](Launcher &L) {
+    if (int rc = check_rk(rk_step)) return rc;
+    L.a.rk_step = rk_step;
+    L.a.dts = dts;
+    ph_small_prep(L);
+    return 0;
+  });
+}
+extern "C" int wrfmi_advance_uv(wrfmi_state *st, const wrfmi_config *cfg, const wrfmi_dims *d, float dts, void *stream) {
+  return run_phase(st, cfg, d, stream, [&](Launcher &L) {
+    L.a.dts = dts;
+    ph_advance_uv(L);
+    return 0;
+  });
+}
+extern "C" int wrfmi_advance_mu_t_w(wrfmi_state *st, const wrfmi_config *cfg, const wrfmi_dims *d, float dts, int iteration,
+                                    int n_small, void *stream) {
+  return run_phase(st, cfg, d, stream, [&](Launcher &L) {
+    if (iteration < 1 || iteration > n_small) return fail(WRFMI_E_BADOPT, "iteration %d outside 1..%d", iteration, n_small);
+    L.a.dts = dts;
+    L.a.iteration = iteration;
+    L.a.nsmall = n_small;
+    ph_advance_mu_t_w(L);
+    return 0;
+  });
+}
+extern "C" int wrfmi_small_step_finish(wrfmi_state *st, const wrfmi_config *cfg, const wrfmi_dims *d, int rk_step, int n_small,
+                                       float dts, void *stream) {
+  return run_phase(st, cfg, d, stream, [&](Launcher &L) {
+    if (int rc = check_rk(rk_step)) return rc;
+    L.a.rk_step = rk_step;
+    L.a.nsmall = n_small;
+    L.a.dts = dts;
+    ph_small_finish(L);
+    return 0;
+  });
+}
+extern "C" int wrfmi_rk_scalar_step(wrfmi_state *st, const wrfmi_config *cfg, const wrfmi_dims *d, int ispec, int rk_step,
+                                    float dt_rk, void *stream) {
+  return run_phase(st, cfg, d, stream, [&](Launcher &L) {
+    if (int rc = check_rk(rk_step)) return rc;
+    if (ispec < 0 || ispec >= L.a.s.n_species) return fail(WRFMI_E_BADOPT, "species %d outside 0..%d", ispec, L.a.s.n_species - 1);
+    L.a.rk_step = rk_step;
+    L.a.species = ispec;
+    L.a.dt_rk = dt_rk;
+    ph_scalar(L);
+    return 0;
+  });
+}
+extern "C" int wrfmi_calc_p_rho_phi(wrfmi_state *st, const wrfmi_config *cfg, const wrfmi_dims *d, void *stream) {
+  return run_phase(st, cfg, d, stream, [&](Launcher &L) {
+    ph_p_rho_phi(L);
+    return 0;
+  });
+}
+extern "C" int wrfmi_phys_bc_dry_2(wrfmi_state *st, const wrfmi_config *cfg, const wrfmi_dims *d, int kind, void *stream) {
+  return run_phase(st, cfg, d, stream, [&](Launcher &L) {
+    if (kind != 0 && kind != 1) return fail(WRFMI_E_BADOPT, "kind must be 0 or 1");
+    ph_bc_dry_2(L, kind);
+    return 0;
+  });
+}
+
+static int bc_single(float *field, char var, const wrfmi_config *cfg, const wrfmi_dims *dims, void *stream, int is3d) {
+  if (!field || !cfg || !dims) return fail(WRFMI_E_POINTER, "null argument");
+  if (var != 'u' && var != 'v' && var != 'w' && var != 't' && var != 'p')
+    return fail(WRFMI_E_BADOPT, "staggering letter '%c' is not one of u v w t p", var);
+  int rc;
+  Launcher L;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if ((rc = check_cfg(cfg))) return rc;
+    if ((rc = make_dm(dims, &L.a.d))) return rc;
+    field = (float *)translate(field);
+  }
+  L.h.d = L.a.d;
+  L.h.periodic_x = cfg->periodic_x; L.h.periodic_y = cfg->periodic_y;
+  L.h.open_xs = cfg->open_xs; L.h.open_xe = cfg->open_xe; L.h.open_ys = cfg->open_ys; L.h.open_ye = cfg->open_ye;
+  L.st = (hipStream_t)stream;
+  L.halo_begin();
+  L.halo_add(field, var, is3d);
+  L.halo_flush();
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+extern "C" int wrfmi_set_physical_bc3d(float *field, char var, const wrfmi_config *cfg, const wrfmi_dims *d, void *stream) {
+  return bc_single(field, var, cfg, d, stream, 1);
+}
+extern "C" int wrfmi_set_physical_bc2d(float *field, char var, const wrfmi_config *cfg, const wrfmi_dims *d, void *stream) {
+  return bc_single(field, var, cfg, d, stream, 0);
+}
+
+// ------------------------------------------------------------------ diagnostics
+namespace wrfmi {
+__global__ void k_count_nonfinite(const float *a, Dm d, int kend, int is3d, unsigned long long *cnt) {
+  const int i = d.ids + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int j = d.jds + (int)blockIdx.y;
+  if (i > d.ide - 1 || j > d.jde - 1) return;
+  unsigned long long n = 0;
+  if (is3d) {
+    for (int k = d.kds; k <= kend; k++) {
+      float v = a[X3(d, i, k, j)];
+      if (!isfinite(v)) n++;
+    }
+  } else if (!isfinite(a[X2(d, i, j)])) {
+    n = 1;
+  }
+  if (n) atomicAdd(cnt, n);
+}
+}  // namespace wrfmi
+
+extern "C" int wrfmi_check_finite(const wrfmi_state *st_in, const wrfmi_dims *dims, long long *n_bad) {
+  if (!st_in || !dims || !n_bad) return fail(WRFMI_E_POINTER, "null argument");
+  int rc;
+  Dm d;
+  wrfmi_state s;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if ((rc = make_dm(dims, &d))) return rc;
+    if ((rc = translate_state(st_in, &s))) return rc;
+    if (!g_counter) HIPCHK(hipMalloc(&g_counter, sizeof(unsigned long long)));
+  }
+  unsigned long long zero = 0, cnt = 0;
+  HIPCHK(hipMemcpy(g_counter, &zero, sizeof zero, hipMemcpyHostToDevice));
+  dim3 blk(64, 1, 1), grd((d.ide - d.ids + 63) / 64, d.jde - d.jds, 1);
+  const float *f3[] = {s.u_2, s.v_2, s.w_2, s.t_2, s.ph_2};
+  const int ke[] = {d.kde - 1, d.kde - 1, d.kde, d.kde - 1, d.kde};
+  for (int n = 0; n < 5; n++) hipLaunchKernelGGL(k_count_nonfinite, grd, blk, 0, 0, f3[n], d, ke[n], 1, g_counter);
+  hipLaunchKernelGGL(k_count_nonfinite, grd, blk, 0, 0, (const float *)s.mu_2, d, 1, 0, g_counter);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpy(&cnt, g_counter, sizeof cnt, hipMemcpyDeviceToHost));
+  *n_bad = (long long)cnt;
+  return cnt ? fail(WRFMI_E_NONFINITE, "%llu non-finite values in the prognostic state", cnt) : 0;
+}

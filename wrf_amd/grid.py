@@ -51,6 +51,7 @@ class DomainSpec:
     bubble_amp: float = 3.0
     noise_amp: float = 0.1
     noise_levels: int = 10
+    wind_scale: float = 1.0   # multiplies the hodograph winds (0 gives a resting atmosphere)
     seed: int = 1001
     pad_i: int = 32           # pad the memory i-extent to a multiple of this (alignment)
     precision_mode: int = 0
@@ -452,6 +453,7 @@ def build_initial_state(spec: DomainSpec) -> HostState:
     # winds from the hodograph at base-state mass-level heights
     zmass = 0.5 * (phb[:-1] + phb[1:]) / G
     uz, vz = hodograph(zmass)
+    uz, vz = uz * spec.wind_scale, vz * spec.wind_scale
 
     # place into memory arrays
     js, is_ = slice(d.jds - d.jms, d.jds - d.jms + ny_m), slice(d.ids - d.ims, d.ids - d.ims + nx_m)
