@@ -1,0 +1,163 @@
+#!/usr/bin/env python3
+"""Throughput of the MI355X WRF-ARW dynamics step (dyn_em/solve_em.F RK3 step) on the
+400x400x80 doubly periodic workload: dx = 1 km, dt = 6 s, 6 acoustic substeps, qv plus
+6 passive scalars, FP32.  Prints one JSON line.
+
+    python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--no-cpu-baseline]
+
+value = mass-point RK3 steps per second (millions) with the state resident in HBM.
+roofline = the acoustic column kernel (advance_mu_t + advance_w + sumflux + calc_p_rho),
+timed with HIP events on its launch stream over the timed steps, against 8 TB/s.
+cpu_baseline = the FP32 C restatement of the same step (the oracle: one core), timed on
+a 100x100x80 sub-domain of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "Mpoint-RK3-steps/s at 400x400x80 + % HBM peak; ensemble member-steps/s"
+# Algorithmic HBM bytes per mass point and RK3 step: 4 * (327 + 12*n_s + 51*N_ac) with
+# N_ac = 1 + n_sound/2 + n_sound acoustic substeps and n_s advected species.
+# Acoustic column kernel, per launch and mass point (FP32 form): reads u, v, u_save, v_save,
+# t_2, t_save, t_tend, ww1, w, rw_tend, ph, ph_save, phb, ph_tend, c2a, cqw, alt, ru_m, rv_m,
+# ww_m, pm1, a, alpha, gamma (24); writes ww, t_2, w, ph, ru_m, rv_m, ww_m, al, p, pm1 (10).
+ACOUSTIC_FIELDS = 34
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def b_alg(n_sound, n_species):
+    n_ac = 1 + n_sound // 2 + n_sound
+    return 4 * (327 + 12 * n_species + 51 * n_ac)
+
+
+def cpu_baseline(seconds_target=15.0):
+    """Oracle (FP32 C restatement, one thread) on a 100x100x80 periodic sub-domain of the
+    benchmark workload; returns the cpu_baseline object."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from pyoracle import Oracle
+    from wrf_amd import grid
+    spec = grid.large_c1(nx=100, ny=100, n_bubbles=2)
+    hs = grid.build_initial_state(spec)
+    o = Oracle("f32")
+    sess = o.session(hs)
+    t0 = time.perf_counter()
+    sess.step(1)
+    t1 = time.perf_counter()
+    n = max(1, min(20, int(seconds_target / max(t1 - t0, 1e-3))))
+    t0 = time.perf_counter()
+    sess.step(n)
+    dt = time.perf_counter() - t0
+    pts = spec.nx * spec.ny * spec.nz
+    return {"value": pts * n / dt / 1e6, "unit": "Mpoint-RK3-steps/s", "cores": 1, "kind": "port",
+            "sample": f"FP32 C restatement of the step (oracle), 1 thread, 100x100x80 periodic sub-domain of the "
+                      f"workload (qv + 6 scalars), {n} RK3 steps after 1 untimed, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--nx", type=int, default=400)
+    ap.add_argument("--ny", type=int, default=400)
+    ap.add_argument("--nz", type=int, default=80)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--graph", action="store_true", help="also time hipGraph replay of the step (extra field)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on one GPU: the dynamics step is built and measured for a single MI355X "
+              "(multi-GPU sharding is a design note, not part of this build). Use --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from wrf_amd import grid, runtime
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(1)
+    spec = grid.large_c1(nx=args.nx, ny=args.ny, nz=args.nz)
+    log(f"building the {spec.nx}x{spec.ny}x{spec.nz} initial state ...")
+    t0 = time.perf_counter()
+    hs = grid.build_initial_state(spec)
+    log(f"  done in {time.perf_counter() - t0:.1f} s; uploading")
+    dev = runtime.DeviceDomain(hs)
+    log(f"  {dev.hbm_bytes() / 1e9:.2f} GB resident in HBM")
+    pts = spec.nx * spec.ny * spec.nz
+    n_sound = runtime.num_sound_steps(hs.cfg, hs.scal["dt"], hs.scal["dx"], hs.scal["dy"])
+    stream = torch.cuda.current_stream()
+
+    for w in range(args.warmup):
+        dev.step(1, stream=stream)
+    torch.cuda.synchronize()
+    log(f"warm-up done ({args.warmup} steps); timing {args.steps} steps")
+
+    with runtime.KernelTimer(["k_acoustic_col"]) as timer:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        dev.step(args.steps, stream=stream, sync=False)
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        ktimes = timer.collect()
+    ms_per_step = 1e3 * elapsed / args.steps
+    value = pts * args.steps / elapsed / 1e6
+    nbad = dev.check_finite()
+    if nbad:
+        print(f"non-finite values after the timed steps: {nbad}", file=sys.stderr)
+        sys.exit(1)
+    log(f"  {ms_per_step:.3f} ms/step, {value:.1f} Mpoint-steps/s")
+
+    n_launch, tot_ms = ktimes["k_acoustic_col"]
+    avg_ms = tot_ms / max(n_launch, 1)
+    per_launch_bytes = ACOUSTIC_FIELDS * 4 * pts
+    achieved = per_launch_bytes / (avg_ms * 1e-3) / 1e9
+    balg = b_alg(n_sound, hs.n_species)
+    step_gbs = balg * pts / (ms_per_step * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Mpoint-RK3-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: analytic Weisman-Klemp sounding, quarter-circle hodograph, 6 seeded warm bubbles, "
+                "seeded theta noise, log-normal passive scalars (seed 1002)",
+        "config": {"workload": f"WRF-ARW RK3 dynamics step, {spec.nx}x{spec.ny}x{spec.nz} mass points, doubly "
+                               f"periodic, dx=1 km, dt=6 s, {n_sound} acoustic substeps, qv + {spec.n_passive} "
+                               f"passive scalars, 5th/3rd-order advection, FP32",
+                   "nx": spec.nx, "ny": spec.ny, "nz": spec.nz, "n_species": hs.n_species, "n_sound": n_sound},
+        "pct_hbm_peak": round(100.0 * step_gbs / HBM_PEAK_GBS, 2),
+        "alg_bytes_per_point_step": balg,
+        "roofline": {"bound": "hbm", "kernel": "k_acoustic_col", "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None, "avg_launch_ms": round(avg_ms, 4), "launches": n_launch,
+                     "alg_bytes_per_launch": per_launch_bytes},
+    }
+    if args.graph:
+        dev.step(1, graph=True, stream=stream)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        dev.step(args.steps, graph=True, stream=stream, sync=False)
+        torch.cuda.synchronize()
+        out["graph_ms_per_step"] = round(1e3 * (time.perf_counter() - t0) / args.steps, 4)
+    if not args.no_cpu_baseline:
+        log("timing the CPU baseline ...")
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -187,6 +187,20 @@ int wrfmi_set_physical_bc3d(float *field, char var, const wrfmi_config *cfg,
 int wrfmi_set_physical_bc2d(float *field, char var, const wrfmi_config *cfg,
                             const wrfmi_dims *d, void *stream);
 
+/* ---- kernel timing (diagnostic; no counterpart in the reference) ----------
+ * While enabled, every launch of a selected kernel by the eager entry points
+ * above is bracketed by a HIP event pair on its own stream; the graph entry
+ * point is refused.  wrfmi_timing_enable(mask) clears the accumulators and
+ * selects kernel n when bit n of mask is set (0 switches timing off);
+ * wrfmi_timing_collect() waits for the recorded events and adds their elapsed
+ * times; wrfmi_timing_get() returns launches and total milliseconds of kernel
+ * `id` (0 .. wrfmi_kernel_count()-1, named by wrfmi_kernel_name). */
+int wrfmi_kernel_count(void);
+const char *wrfmi_kernel_name(int id);
+int wrfmi_timing_enable(unsigned kernel_mask);
+int wrfmi_timing_collect(void);
+int wrfmi_timing_get(int id, long long *launches, double *total_ms);
+
 /* Non-finite scan of the prognostic state (u_2, v_2, w_2, t_2, ph_2, mu_2);
  * returns WRFMI_E_NONFINITE and the count in *n_bad when any is found. */
 int wrfmi_check_finite(const wrfmi_state *st, const wrfmi_dims *d, long long *n_bad);

@@ -37,6 +37,7 @@ typedef int hipError_t;
 enum { hipSuccess = 0, hipErrorNotSupported = 801 };
 typedef struct emu_stream *hipStream_t;
 typedef struct emu_graph *hipGraph_t;
+typedef struct emu_event *hipEvent_t;
 typedef struct emu_graph_exec *hipGraphExec_t;
 enum hipMemcpyKind { hipMemcpyHostToDevice = 1, hipMemcpyDeviceToHost = 2 };
 enum { hipStreamNonBlocking = 1, hipStreamCaptureModeThreadLocal = 1 };
@@ -54,6 +55,11 @@ inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
 inline hipError_t hipStreamCreateWithFlags(hipStream_t *s, int) { *s = nullptr; return hipSuccess; }
 inline hipError_t hipStreamDestroy(hipStream_t) { return hipSuccess; }
 inline hipError_t hipPointerGetAttributes(hipPointerAttribute_t *a, const void *) { a->type = hipMemoryTypeDevice; return hipSuccess; }
+// events are not emulated (kernel timing mode is a GPU-only diagnostic)
+inline hipError_t hipEventCreate(hipEvent_t *e) { *e = nullptr; return hipErrorNotSupported; }
+inline hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipErrorNotSupported; }
+inline hipError_t hipEventSynchronize(hipEvent_t) { return hipErrorNotSupported; }
+inline hipError_t hipEventElapsedTime(float *ms, hipEvent_t, hipEvent_t) { *ms = 0.f; return hipErrorNotSupported; }
 // graph capture is not emulated: the capturing entry point reports an error
 inline hipError_t hipStreamBeginCapture(hipStream_t, int) { return hipErrorNotSupported; }
 inline hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t *) { return hipErrorNotSupported; }

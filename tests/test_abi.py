@@ -1,0 +1,82 @@
+"""The C-ABI library loads on a host without a GPU, exports every entry point that
+include/wrfmi.h declares, agrees with the Python struct mirrors, and rejects options
+outside the implemented set before touching the device."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from wrf_amd import fields, grid, runtime
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_entry_points():
+    text = open(os.path.join(ROOT, "include", "wrfmi.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(wrfmi_\w+)\s*\(", text)))
+
+
+def test_header_declares_entry_points():
+    names = declared_entry_points()
+    assert "wrfmi_solve_em_step" in names and "wrfmi_advance_mu_t_w" in names
+    assert len(names) >= 25
+
+
+def test_library_exports_every_declared_symbol():
+    lib = runtime.load_library()
+    missing = [n for n in declared_entry_points() if not hasattr(lib, n)]
+    assert not missing, f"not exported: {missing}"
+
+
+def test_struct_layouts_match():
+    lib = runtime.load_library()
+    assert lib.wrfmi_state_sizeof() == ctypes.sizeof(fields.State32)
+    assert lib.wrfmi_state_field_count() == fields.state_field_count()
+    assert lib.wrfmi_version() == b"1.0"
+    names = [lib.wrfmi_kernel_name(k).decode() for k in range(lib.wrfmi_kernel_count())]
+    assert "k_acoustic_col" in names and "k_halo" in names
+
+
+@pytest.mark.parametrize("field,value,msg", [
+    ("rk_ord", 2, "rk_ord"),
+    ("h_mom_adv_order", 6, "orders"),
+    ("specified", 1, "lateral"),
+    ("diff_opt", 1, "diffusion"),
+    ("damp_opt", 3, "damp_opt"),
+    ("moist_adv_opt", 1, "advection options"),
+])
+def test_unsupported_options_are_rejected(field, value, msg):
+    lib = runtime.load_library()
+    spec = grid.DomainSpec(nx=10, ny=10, nz=6)
+    cfg = grid.make_config(spec)
+    dims = grid.make_dims(spec)
+    setattr(cfg, field, value)
+    st = fields.State32()
+    rc = lib.wrfmi_solve_em_step(ctypes.byref(st), ctypes.byref(cfg), ctypes.byref(dims), None, 1)
+    assert rc == runtime.WRFMI_E_BADOPT
+    assert msg in lib.wrfmi_last_error().decode()
+
+
+def test_bad_dims_are_rejected():
+    lib = runtime.load_library()
+    spec = grid.DomainSpec(nx=10, ny=10, nz=6)
+    cfg = grid.make_config(spec)
+    dims = grid.make_dims(spec)
+    dims.ims = dims.ids - 2  # halo narrower than the 4 points the stencils need
+    st = fields.State32()
+    rc = lib.wrfmi_solve_em_step(ctypes.byref(st), ctypes.byref(cfg), ctypes.byref(dims), None, 1)
+    assert rc == runtime.WRFMI_E_DIMS
+
+
+def test_step_phase_sequence():
+    spec = grid.supercell_c0()
+    hs_cfg = grid.make_config(spec)
+    ph = runtime.step_phases(hs_cfg, dict(dt=12.0, dx=2000.0, dy=2000.0), 1)
+    names = [n for n, _ in ph]
+    # 1 + 3 + 6 acoustic substeps for time_step_sound = 6
+    assert names.count("advance_mu_t_w") == 10
+    assert names.count("rk_scalar_step") == 3
+    assert names[-2:] == ["calc_p_rho_phi", "phys_bc_dry_2"]
+    assert runtime.num_sound_steps(type(hs_cfg)(), 6.0, 1000.0, 1000.0) == 4

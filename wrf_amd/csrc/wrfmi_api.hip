@@ -183,11 +183,52 @@ static inline dim3 col_grid(const Dm &d, int i0, int j0, dim3 blk) {
 }
 static const dim3 kColBlk(64, 4, 1);
 
+// ------------------------------------------------------------------ kernel timing (diagnostic)
+enum KernelId {
+  KID_STEP_PREP, KID_TEND_U, KID_TEND_V, KID_TEND_W, KID_TEND_T, KID_SMALL_PREP, KID_ADVANCE_UV,
+  KID_ACOUSTIC_COL, KID_SMALL_FINISH, KID_ADV_SCALAR, KID_UPDATE_SCALAR, KID_P_RHO_PHI, KID_HALO, KID_COUNT
+};
+static const char *kKernelNames[KID_COUNT] = {
+  "k_step_prep", "k_tend_u", "k_tend_v", "k_tend_w", "k_tend_t", "k_small_prep", "k_advance_uv",
+  "k_acoustic_col", "k_small_finish", "k_adv_scalar", "k_update_scalar", "k_p_rho_phi", "k_halo"};
+struct EvPair { int id; hipEvent_t a, b; };
+static unsigned g_timing = 0;  // bit n set: time launches of kernel n
+static std::vector<EvPair> g_ev_pending;
+static std::vector<hipEvent_t> g_ev_free;
+static long long g_t_count[KID_COUNT];
+static double g_t_ms[KID_COUNT];
+
+static hipEvent_t ev_get() {
+  if (!g_ev_free.empty()) {
+    hipEvent_t e = g_ev_free.back();
+    g_ev_free.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
 struct Launcher {
   hipStream_t st;
   KArgs a;
   HaloArgs h;
   int err = 0;
+  // launch with an optional event pair around it (kernel timing mode)
+  template <typename K, typename... A>
+  void launch(int id, K kern, dim3 g, dim3 b, A... args) {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (g_timing & (1u << id)) {
+      e0 = ev_get();
+      e1 = ev_get();
+      if (e0 && e1) (void)hipEventRecord(e0, st);
+    }
+    hipLaunchKernelGGL(kern, g, b, 0, st, args...);
+    if (e0 && e1) {
+      (void)hipEventRecord(e1, st);
+      g_ev_pending.push_back(EvPair{id, e0, e1});
+    }
+  }
   void halo_begin() { h.n = 0; }
   void halo_add(float *p, char kind, int is3d) {
     int k = kind == 'u' ? 1 : kind == 'v' ? 2 : kind == 'w' ? 3 : 0;
@@ -199,7 +240,7 @@ struct Launcher {
     const Dm &d = a.d;
     dim3 blk(256, 1, 1);
     dim3 grd((d.ime - d.ims + 1 + 255) / 256, d.jme - d.jms + 1, h.n);
-    hipLaunchKernelGGL(k_halo, grd, blk, 0, st, h);
+    launch(KID_HALO, k_halo, grd, blk, h);
     h.n = 0;
   }
 };
@@ -222,8 +263,8 @@ static inline dim3 g_mass(const Dm &d) { return dim3(((d.ide - d.ids) + 63) / 64
 static void ph_step_prep(Launcher &L) {
   const KArgs &a = L.a;
   const wrfmi_state &s = a.s;
-  if (a.c.precision_mode == 1) hipLaunchKernelGGL(k_step_prep<true>, g_prep(a.d), kColBlk, 0, L.st, a);
-  else hipLaunchKernelGGL(k_step_prep<false>, g_prep(a.d), kColBlk, 0, L.st, a);
+  if (a.c.precision_mode == 1) L.launch(KID_STEP_PREP, k_step_prep<true>, g_prep(a.d), kColBlk, a);
+  else L.launch(KID_STEP_PREP, k_step_prep<false>, g_prep(a.d), kColBlk, a);
   L.halo_begin();
   L.halo_add(s.ru, 'u', 1); L.halo_add(s.rv, 'v', 1); L.halo_add(s.rw, 'w', 1); L.halo_add(s.ww, 'w', 1);
   L.halo_add(s.php, 'w', 1); L.halo_add(s.alt, 't', 1); L.halo_add(s.p, 'p', 1);
@@ -233,16 +274,16 @@ static void ph_step_prep(Launcher &L) {
 }
 static void ph_tendency(Launcher &L) {
   const KArgs &a = L.a;
-  hipLaunchKernelGGL(k_tend_u, g_col(a.d), kColBlk, 0, L.st, a);
-  hipLaunchKernelGGL(k_tend_v, g_col(a.d), kColBlk, 0, L.st, a);
-  hipLaunchKernelGGL(k_tend_w, g_col(a.d), kColBlk, 0, L.st, a);
-  hipLaunchKernelGGL(k_tend_t, g_col(a.d), kColBlk, 0, L.st, a);
+  L.launch(KID_TEND_U, k_tend_u, g_col(a.d), kColBlk, a);
+  L.launch(KID_TEND_V, k_tend_v, g_col(a.d), kColBlk, a);
+  L.launch(KID_TEND_W, k_tend_w, g_col(a.d), kColBlk, a);
+  L.launch(KID_TEND_T, k_tend_t, g_col(a.d), kColBlk, a);
 }
 static void ph_small_prep(Launcher &L) {
   const KArgs &a = L.a;
   const wrfmi_state &s = a.s;
-  if (a.c.precision_mode == 1) hipLaunchKernelGGL(k_small_prep<double>, g_col(a.d), kColBlk, 0, L.st, a);
-  else hipLaunchKernelGGL(k_small_prep<float>, g_col(a.d), kColBlk, 0, L.st, a);
+  if (a.c.precision_mode == 1) L.launch(KID_SMALL_PREP, k_small_prep<double>, g_col(a.d), kColBlk, a);
+  else L.launch(KID_SMALL_PREP, k_small_prep<float>, g_col(a.d), kColBlk, a);
   L.halo_begin();
   L.halo_add(s.ru_tend, 'u', 1); L.halo_add(s.rv_tend, 'v', 1); L.halo_add(s.ph_2, 'w', 1);
   L.halo_add(s.al, 'p', 1); L.halo_add(s.p, 'p', 1); L.halo_add(s.t_1, 'p', 1); L.halo_add(s.t_save, 't', 1);
@@ -250,13 +291,13 @@ static void ph_small_prep(Launcher &L) {
   L.halo_flush();
 }
 static void ph_advance_uv(Launcher &L) {
-  hipLaunchKernelGGL(k_advance_uv, g_pt(L.a.d), dim3(64, 1, 1), 0, L.st, L.a);
+  L.launch(KID_ADVANCE_UV, k_advance_uv, g_pt(L.a.d), dim3(64, 1, 1), L.a);
 }
 static void ph_advance_mu_t_w(Launcher &L) {
   const KArgs &a = L.a;
   const wrfmi_state &s = a.s;
-  if (a.c.precision_mode == 1) hipLaunchKernelGGL((k_acoustic_col<double, true>), g_col(a.d), kColBlk, 0, L.st, a);
-  else hipLaunchKernelGGL((k_acoustic_col<float, false>), g_col(a.d), kColBlk, 0, L.st, a);
+  if (a.c.precision_mode == 1) L.launch(KID_ACOUSTIC_COL, k_acoustic_col<double, true>, g_col(a.d), kColBlk, a);
+  else L.launch(KID_ACOUSTIC_COL, k_acoustic_col<float, false>, g_col(a.d), kColBlk, a);
   L.halo_begin();
   L.halo_add(s.ph_2, 'w', 1); L.halo_add(s.al, 'p', 1); L.halo_add(s.p, 'p', 1);
   L.halo_add(s.muts, 't', 0); L.halo_add(s.mu_2, 't', 0); L.halo_add(s.mudf, 't', 0);
@@ -265,7 +306,7 @@ static void ph_advance_mu_t_w(Launcher &L) {
 static void ph_small_finish(Launcher &L) {
   const KArgs &a = L.a;
   const wrfmi_state &s = a.s;
-  hipLaunchKernelGGL(k_small_finish, g_col(a.d), kColBlk, 0, L.st, a);
+  L.launch(KID_SMALL_FINISH, k_small_finish, g_col(a.d), kColBlk, a);
   if (a.rk_step == 3) {
     L.halo_begin();
     L.halo_add(s.ru_m, 'u', 1); L.halo_add(s.rv_m, 'v', 1); L.halo_add(s.ww_m, 'w', 1);
@@ -274,11 +315,11 @@ static void ph_small_finish(Launcher &L) {
   }
 }
 static void ph_scalar(Launcher &L) {
-  hipLaunchKernelGGL(k_adv_scalar, g_col(L.a.d), kColBlk, 0, L.st, L.a);
-  hipLaunchKernelGGL(k_update_scalar, g_mass(L.a.d), dim3(64, 1, 1), 0, L.st, L.a);
+  L.launch(KID_ADV_SCALAR, k_adv_scalar, g_col(L.a.d), kColBlk, L.a);
+  L.launch(KID_UPDATE_SCALAR, k_update_scalar, g_mass(L.a.d), dim3(64, 1, 1), L.a);
 }
 static void ph_p_rho_phi(Launcher &L) {
-  hipLaunchKernelGGL(k_p_rho_phi, g_mass(L.a.d), dim3(64, 1, 1), 0, L.st, L.a);
+  L.launch(KID_P_RHO_PHI, k_p_rho_phi, g_mass(L.a.d), dim3(64, 1, 1), L.a);
 }
 static void ph_bc_dry_2(Launcher &L, int kind) {
   const wrfmi_state &s = L.a.s;
@@ -382,6 +423,7 @@ static void free_graphs() {
 
 extern "C" int wrfmi_solve_em_step_graph(wrfmi_state *st, const wrfmi_config *cfg, const wrfmi_dims *dims, void *stream, int sync) {
   if (!st || !cfg || !dims) return fail(WRFMI_E_POINTER, "null argument");
+  if (g_timing) return fail(WRFMI_E_BADOPT, "kernel timing mode records events on eager launches only");
   hipStream_t hs = (hipStream_t)stream;
   GraphEntry *hit = nullptr;
   {
@@ -587,4 +629,38 @@ extern "C" int wrfmi_check_finite(const wrfmi_state *st_in, const wrfmi_dims *di
   HIPCHK(hipMemcpy(&cnt, g_counter, sizeof cnt, hipMemcpyDeviceToHost));
   *n_bad = (long long)cnt;
   return cnt ? fail(WRFMI_E_NONFINITE, "%llu non-finite values in the prognostic state", cnt) : 0;
+}
+
+// ------------------------------------------------------------------ kernel timing entry points
+extern "C" int wrfmi_kernel_count(void) { return KID_COUNT; }
+extern "C" const char *wrfmi_kernel_name(int id) { return (id >= 0 && id < KID_COUNT) ? kKernelNames[id] : ""; }
+extern "C" int wrfmi_timing_enable(unsigned kernel_mask) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  HIPCHK(hipDeviceSynchronize());
+  for (EvPair &e : g_ev_pending) { g_ev_free.push_back(e.a); g_ev_free.push_back(e.b); }
+  g_ev_pending.clear();
+  for (int n = 0; n < KID_COUNT; n++) { g_t_count[n] = 0; g_t_ms[n] = 0.0; }
+  g_timing = kernel_mask;
+  return 0;
+}
+extern "C" int wrfmi_timing_collect(void) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  for (EvPair &e : g_ev_pending) {
+    HIPCHK(hipEventSynchronize(e.b));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e.a, e.b));
+    g_t_count[e.id]++;
+    g_t_ms[e.id] += ms;
+    g_ev_free.push_back(e.a);
+    g_ev_free.push_back(e.b);
+  }
+  g_ev_pending.clear();
+  return 0;
+}
+extern "C" int wrfmi_timing_get(int id, long long *launches, double *total_ms) {
+  if (id < 0 || id >= KID_COUNT || !launches || !total_ms) return fail(WRFMI_E_BADOPT, "bad kernel id or null output");
+  std::lock_guard<std::mutex> lk(g_mu);
+  *launches = g_t_count[id];
+  *total_ms = g_t_ms[id];
+  return 0;
 }

@@ -123,6 +123,11 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         fn.argtypes = [vp, ctypes.c_char, vp, vp, vp]
         fn.restype = ctypes.c_int
     lib.wrfmi_check_finite.argtypes = [vp, vp, ctypes.POINTER(ctypes.c_longlong)]
+    lib.wrfmi_kernel_count.restype = ctypes.c_int
+    lib.wrfmi_kernel_name.restype = ctypes.c_char_p
+    lib.wrfmi_kernel_name.argtypes = [ctypes.c_int]
+    lib.wrfmi_timing_enable.argtypes = [ctypes.c_uint]
+    lib.wrfmi_timing_get.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_double)]
     if lib.wrfmi_state_sizeof() != ctypes.sizeof(State32):
         raise RuntimeError("wrfmi_state layout of the library does not match wrf_amd.fields")
     _lib = lib
@@ -132,6 +137,40 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
 def check(rc: int):
     if rc != WRFMI_OK:
         raise WrfmiError(rc, load_library().wrfmi_last_error().decode())
+
+
+class KernelTimer:
+    """Per-kernel HIP-event timing of the eager entry points (wrfmi_timing_*)."""
+
+    def __init__(self, kernels=None):
+        """kernels: names to time (None = all)."""
+        self.lib = load_library()
+        names = [self.lib.wrfmi_kernel_name(k).decode() for k in range(self.lib.wrfmi_kernel_count())]
+        if kernels is None:
+            self.mask = (1 << len(names)) - 1
+        else:
+            unknown = set(kernels) - set(names)
+            if unknown:
+                raise ValueError(f"unknown kernels {sorted(unknown)}")
+            self.mask = sum(1 << names.index(k) for k in kernels)
+
+    def __enter__(self):
+        check(self.lib.wrfmi_timing_enable(self.mask))
+        return self
+
+    def __exit__(self, *exc):
+        self.lib.wrfmi_timing_enable(0)
+        return False
+
+    def collect(self) -> dict:
+        """name -> (launches, total_ms) of the events recorded so far."""
+        check(self.lib.wrfmi_timing_collect())
+        out = {}
+        for kid in range(self.lib.wrfmi_kernel_count()):
+            n, ms = ctypes.c_longlong(0), ctypes.c_double(0.0)
+            check(self.lib.wrfmi_timing_get(kid, ctypes.byref(n), ctypes.byref(ms)))
+            out[self.lib.wrfmi_kernel_name(kid).decode()] = (n.value, ms.value)
+        return out
 
 
 class DeviceDomain:
@@ -207,6 +246,9 @@ class DeviceDomain:
         self.torch.cuda.synchronize(self.device)
         keys = names if names is not None else list(self.t)
         return {k: self.t[k].cpu().numpy() for k in keys}
+
+    def download(self) -> dict:
+        return self.to_host()
 
     def hbm_bytes(self) -> int:
         return sum(v.numel() * v.element_size() for v in self.t.values())
