@@ -72,6 +72,8 @@ def main():
     ap.add_argument("--nz", type=int, default=80)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--graph", action="store_true", help="also time hipGraph replay of the step (extra field)")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="after the timed steps, copy a buffer of this many MiB once (calibrates HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU: the dynamics step is built and measured for a single MI355X "
@@ -146,6 +148,14 @@ def main():
                      "traffic": None, "avg_launch_ms": round(avg_ms, 4), "launches": n_launch,
                      "alg_bytes_per_launch": per_launch_bytes},
     }
+    if args.copy_probe_mib:
+        src = torch.ones(args.copy_probe_mib << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        out["copy_probe_bytes"] = 2 * src.numel() * 4
+        del src, dst
     if args.graph:
         dev.step(1, graph=True, stream=stream)
         torch.cuda.synchronize()

@@ -4,10 +4,13 @@ from __future__ import annotations
 import numpy as np
 
 # The reference evaluates LOG and ** (vspow) in FP32 libm; the GPU evaluates them in
-# double and rounds once.  Outputs of calc_p_rho_phi may therefore differ by a few ulp of
-# the full (base + perturbation) value at isolated points.
+# double and rounds once.  Against the libm oracle, outputs of calc_p_rho_phi may differ by
+# a few ulp of the full (base + perturbation) value at isolated points: 1 ulp of LOG moves
+# alpha by up to ~1 ulp, which ** (exponent cp/cv = 1.4) and the product with p0 carry into
+# p at up to ~8 ulp.  Against the once-rounded oracle the bound is the rare case where a
+# double result rounds differently to float on the two sides.
 EOS_FIELDS = {"p": "pb", "al": "alb"}
-EOS_ULPS = 4
+EOS_ULPS = 8
 
 
 def rel_l2(a, b) -> float:
@@ -29,7 +32,7 @@ def interior(a, d, kind="mass", nk=None):
     return a[j0:j0 + nj, i0:i0 + ni]
 
 
-def compare_exact(got: dict, exp: dict, names, eos_tol: bool = False, base: dict | None = None):
+def compare_exact(got: dict, exp: dict, names, eos_tol: bool = False, base: dict | None = None, ulps: int = EOS_ULPS):
     """Bitwise comparison; returns a list of failure descriptions."""
     fails = []
     for k in names:
@@ -44,7 +47,7 @@ def compare_exact(got: dict, exp: dict, names, eos_tol: bool = False, base: dict
         if eos_tol and k in EOS_FIELDS and base is not None:
             full = np.abs(e.astype(np.float64) + base[EOS_FIELDS[k]].astype(np.float64))
             ulp = np.spacing(full.astype(np.float32)).astype(np.float64)
-            bad &= np.abs(g.astype(np.float64) - e.astype(np.float64)) > EOS_ULPS * ulp
+            bad &= np.abs(g.astype(np.float64) - e.astype(np.float64)) > ulps * ulp
         n = int(bad.sum())
         if n:
             idx = tuple(int(x[0]) for x in np.nonzero(bad))
@@ -53,7 +56,7 @@ def compare_exact(got: dict, exp: dict, names, eos_tol: bool = False, base: dict
     return fails
 
 
-def phase_parity(backend, session, phases, nsteps: int = 1, stop_at_first: bool = True):
+def phase_parity(backend, session, phases, nsteps: int = 1, stop_at_first: bool = True, eos_ulps: int = EOS_ULPS):
     """Run the per-routine calls on the backend and on the oracle from identical inputs
     (the oracle's state is uploaded before every call) and compare every array bitwise.
     Returns a list of failure descriptions."""
@@ -65,7 +68,8 @@ def phase_parity(backend, session, phases, nsteps: int = 1, stop_at_first: bool 
             backend.phase(name, *args)
             session.phase(name, *args)
             got = backend.download()
-            f = compare_exact(got, session.arrays, names, eos_tol=(name == "calc_p_rho_phi"), base=session.arrays)
+            f = compare_exact(got, session.arrays, names, eos_tol=(name == "calc_p_rho_phi"), base=session.arrays,
+                              ulps=eos_ulps)
             if f:
                 fails.append(f"step {step + 1} call {n} {name}{args}: " + "; ".join(f))
                 if stop_at_first:
