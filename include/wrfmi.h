@@ -105,7 +105,10 @@ size_t wrfmi_state_sizeof(void);
 /* ---- host-array residency (device mirrors keyed by host address) ---------
  * Replaces the per-call data movement a drop-in would otherwise need: a host
  * that owns the Fortran arrays registers each once, syncs at solve start/end
- * and at I/O steps, and passes its own host pointers to every entry point. */
+ * and at I/O steps, and passes its own host pointers to every entry point.
+ * Registering an address again refreshes its mirror from the host (or replaces
+ * it when the size changed); a range overlapping another registered array is
+ * refused.  Unregister an array before freeing it. */
 int wrfmi_register_field(void *host_ptr, size_t bytes);
 int wrfmi_unregister_field(void *host_ptr);
 int wrfmi_sync_to_device(void *host_ptr);

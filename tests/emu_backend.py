@@ -23,6 +23,7 @@ class EmuDomain:
         lib = ctypes.CDLL(EMU_LIB)
         vp = ctypes.c_void_p
         lib.wrfmi_register_field.argtypes = [vp, ctypes.c_size_t]
+        lib.wrfmi_unregister_field.argtypes = [vp]
         lib.wrfmi_sync_to_device.argtypes = [vp]
         lib.wrfmi_sync_to_host.argtypes = [vp]
         lib.wrfmi_last_error.restype = ctypes.c_char_p
@@ -49,6 +50,12 @@ class EmuDomain:
             st.q_old[m] = self.arrays[f"q_old{m}"].ctypes.data_as(P)
             st.q_tend[m] = self.arrays[f"q_tend{m}"].ctypes.data_as(P)
         self.state = st
+
+    def __del__(self):
+        arrays = getattr(self, "arrays", None)
+        if arrays:
+            for v in arrays.values():
+                self.lib.wrfmi_unregister_field(ctypes.c_void_p(v.ctypes.data))
 
     def _chk(self, rc):
         if rc != 0:

@@ -8,6 +8,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path[:0] = [os.path.dirname(HERE), os.path.join(os.path.dirname(HERE), "oracle"), HERE]
 
+import numpy as np  # noqa: E402
 from emu_backend import EmuDomain  # noqa: E402
 from parity import phase_parity  # noqa: E402
 from pyoracle import Oracle  # noqa: E402
@@ -20,15 +21,54 @@ CASES = {
 }
 
 
+def registry_check():
+    """Host-array registry: a second registration of the same address refreshes the
+    mirror; an overlapping range is refused; unregistering frees the address."""
+    import ctypes
+
+    import numpy as np
+    from emu_backend import EMU_LIB
+    lib = ctypes.CDLL(EMU_LIB)
+    vp = ctypes.c_void_p
+    lib.wrfmi_register_field.argtypes = [vp, ctypes.c_size_t]
+    lib.wrfmi_unregister_field.argtypes = [vp]
+    lib.wrfmi_sync_to_host.argtypes = [vp]
+    lib.wrfmi_sync_to_device.argtypes = [vp]
+    a = np.arange(100, dtype=np.float32)
+    assert lib.wrfmi_register_field(a.ctypes.data, a.nbytes) == 0
+    a[:] = 7.0
+    assert lib.wrfmi_register_field(a.ctypes.data, a.nbytes) == 0  # refresh from the host
+    a[:] = 0.0
+    assert lib.wrfmi_sync_to_host(a.ctypes.data) == 0
+    assert np.all(a == 7.0)
+    assert lib.wrfmi_register_field(a[10:].ctypes.data, 40) == 4  # overlaps: WRFMI_E_POINTER
+    assert lib.wrfmi_unregister_field(a.ctypes.data) == 0
+    assert lib.wrfmi_unregister_field(a.ctypes.data) == 4
+    assert lib.wrfmi_register_field(a[10:].ctypes.data, 40) == 0
+    print("OK registry", flush=True)
+
+
 def main():
+    if sys.argv[1] == "registry":
+        registry_check()
+        return
     case, order, nsteps = sys.argv[1], int(sys.argv[2]), int(sys.argv[3])
     spec = grid.DomainSpec(**CASES[case])
     hs = grid.build_initial_state(spec)
     emu = EmuDomain(hs, order=order)
-    oracle = Oracle("f32")
-    sess = oracle.session(hs)
     phases = runtime.step_phases(hs.cfg, hs.scal, hs.n_species)
-    fails = phase_parity(emu, sess, phases, nsteps=nsteps)
+    # bitwise against the oracle whose EOS rounds LOG and ** once from double, as the
+    # kernels do; then the whole step through wrfmi_solve_em_step
+    sess = Oracle("f32cr").session(hs)
+    fails = phase_parity(emu, sess, phases, nsteps=nsteps, eos_ulps=0)
+    if not fails:
+        emu2 = EmuDomain(hs, order=order)
+        emu2.step(nsteps)
+        got = emu2.download()
+        exp = Oracle("f32cr").run(hs, nsteps)
+        bad = [k for k in exp if not np.array_equal(got[k], exp[k])]
+        if bad:
+            fails.append(f"whole step differs in {bad}")
     for f in fails:
         print("MISMATCH", f)
     print("OK" if not fails else "FAILED", case, "order", order, flush=True)

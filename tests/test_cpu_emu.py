@@ -31,3 +31,10 @@ def test_emulated_kernels_match_oracle(emu_env, case, order, nsteps):
                        env=emu_env, capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stdout[-4000:] + r.stderr[-4000:]
     assert "OK" in r.stdout
+
+
+def test_emulated_host_array_registry(emu_env):
+    r = subprocess.run([sys.executable, os.path.join(HERE, "emu_phase_check.py"), "registry"],
+                       env=emu_env, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-4000:] + r.stderr[-4000:]
+    assert "OK registry" in r.stdout

@@ -1,0 +1,242 @@
+"""Pins of the CPU oracle (oracle/*.c) -- the checks that make it a trustworthy checker.
+
+* advect_scalar: bit for bit against the reference's own advect_scalar, compiled in place
+  (oracle/_ref, when the reference tree is mounted) and through the committed fixture
+  tests/golden/advect_scalar_ref.npz that the same build wrote.
+* The whole step and the routines the reference cannot be compiled for here (they need
+  the Registry-generated module_configure): properties the equations fix --
+  a resting hydrostatic atmosphere stays at rest; a horizontally uniform state stays
+  uniform; mirror and x/y-transpose symmetry of a centred bubble without wind;
+  translation invariance on a periodic domain; conservation of dry-air and species mass
+  on a periodic domain; the equation of state reproduces the independently built
+  hydrostatic initial state; FP32 and FP64 builds agree to FP32 accuracy.
+* The analytic sounding: against the reference's test/em_quarter_ss/input_sounding.
+"""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+from parity import interior, rel_l2
+from wrf_amd import grid
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = os.path.join(HERE, "golden")
+MSF = ("msfux", "msfuy", "msfvx", "msfvy", "msftx", "msfty")
+
+
+def _oracle_advect_scalar(o, g, n, tend):
+    d = grid.Dims(*[int(x) for x in g[f"c{n}_dims"]])
+    spec = grid.DomainSpec(bc="periodic" if int(g[f"c{n}_bc"]) == 0 else "open")
+    c = grid.make_config(spec)
+    ts = int(g[f"c{n}_time_step"])
+    c.time_step = ts
+    P = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    L = o.lib
+    L.o_advect_scalar.argtypes = ([ctypes.c_void_p] * 9 + [ctypes.c_int, ctypes.c_void_p] + [ctypes.c_void_p] * 8
+                                  + [ctypes.c_float, ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p])
+    a = {k: np.ascontiguousarray(g[f"c{n}_{k}"]) for k in
+         ("field", "field_old", "ru", "rv", "rom", "c1", "c2", "mut", "fzm", "fzp", "rdzw") + MSF}
+    rc = L.o_advect_scalar(P(a["field"]), P(a["field_old"]), P(tend), P(a["ru"]), P(a["rv"]), P(a["rom"]), P(a["c1"]),
+                           P(a["c2"]), P(a["mut"]), ts, ctypes.byref(c), *[P(a[m]) for m in MSF], P(a["fzm"]),
+                           P(a["fzp"]), float(g[f"c{n}_rdx"]), float(g[f"c{n}_rdy"]), P(a["rdzw"]), ctypes.byref(d))
+    assert rc == 0
+
+
+@pytest.mark.parametrize("n", range(6))
+def test_advect_scalar_matches_reference_fixture(oracle32, n):
+    g = np.load(os.path.join(GOLDEN, "advect_scalar_ref.npz"))
+    tend = np.zeros_like(g[f"c{n}_tendency"])
+    _oracle_advect_scalar(oracle32, g, n, tend)
+    assert np.array_equal(tend.view(np.int32), g[f"c{n}_tendency"].view(np.int32))
+    assert np.count_nonzero(tend) > 100
+
+
+def test_advect_scalar_matches_reference_build(oracle32):
+    """Fresh random inputs through the reference build itself (needs oracle/_ref)."""
+    ref_lib = os.path.join(os.path.dirname(HERE), "oracle", "_ref", "libref_advect.so")
+    if not os.path.exists(ref_lib):
+        pytest.skip("reference advection kernel not built (no reference tree)")
+    from pyoracle import RefAdvect
+    ref = RefAdvect()
+    rng = np.random.default_rng(77)
+    for bc, ts in (("periodic", 6), ("open", -12)):
+        spec = grid.DomainSpec(nx=15, ny=12, nz=9, bc=bc)
+        d, c = grid.make_dims(spec), grid.make_config(spec)
+        c.time_step = ts
+        s3, s2, s1 = grid.shape3(d), grid.shape2(d), grid.shape1(d)
+        u = lambda sh, lo, hi: rng.uniform(lo, hi, sh).astype(np.float32)
+        g = {"c0_field": u(s3, 0, 1e-2), "c0_field_old": u(s3, 0, 1e-2), "c0_ru": u(s3, -5e4, 5e4),
+             "c0_rv": u(s3, -5e4, 5e4), "c0_rom": u(s3, -80, 80), "c0_mut": u(s2, 8e4, 9e4),
+             "c0_c1": u(s1, 0.5, 1), "c0_c2": u(s1, 0, 1e3), "c0_fzm": u(s1, 0.4, 0.6), "c0_rdzw": u(s1, -40, -10),
+             "c0_time_step": np.int32(ts), "c0_rdx": np.float32(1e-3), "c0_rdy": np.float32(1e-3),
+             "c0_bc": np.int32(0 if bc == "periodic" else 1),
+             "c0_dims": np.array([getattr(d, n) for n, _ in d._fields_], np.int32)}
+        g["c0_fzp"] = (1 - g["c0_fzm"]).astype(np.float32)
+        for m in MSF:
+            g["c0_" + m] = u(s2, 0.9, 1.1)
+        t_ref = np.zeros(s3, np.float32)
+        ref.advect_scalar(g["c0_field"], g["c0_field_old"], t_ref, g["c0_ru"], g["c0_rv"], g["c0_rom"], g["c0_c1"],
+                          g["c0_c2"], g["c0_mut"], ts, c, {m: g["c0_" + m] for m in MSF}, g["c0_fzm"], g["c0_fzp"],
+                          1e-3, 1e-3, g["c0_rdzw"], d)
+        t_or = np.zeros(s3, np.float32)
+        _oracle_advect_scalar(oracle32, g, 0, t_or)
+        assert np.array_equal(t_ref.view(np.int32), t_or.view(np.int32)), bc
+
+
+def test_sounding_matches_reference_input_sounding():
+    a = np.loadtxt(os.path.join(GOLDEN, "em_quarter_ss_input_sounding.txt"), skiprows=1)
+    z, th = a[:, 0], a[:, 1]
+    trop = z <= 12000.0
+    # the file holds theta to 4 decimals
+    assert np.abs(grid.wk_theta(z[trop]) - th[trop]).max() < 1e-4
+    assert abs(float(np.loadtxt(os.path.join(GOLDEN, "em_quarter_ss_input_sounding.txt"), max_rows=1)[1]) - 300.0) < 1e-9
+
+
+def _run(o, spec, nsteps):
+    hs = grid.build_initial_state(spec)
+    return hs, o.run(hs, nsteps)
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_resting_atmosphere_stays_at_rest(prec):
+    from pyoracle import Oracle
+    spec = grid.DomainSpec(nx=12, ny=10, nz=20, bc="periodic", bubble_amp=0.0, noise_amp=0.0, wind_scale=0.0, seed=1)
+    hs, a = _run(Oracle(prec), spec, 10)
+    d = hs.dims
+    w = interior(a["w_2"], d, "w")
+    u = interior(a["u_2"], d, "u")
+    assert np.abs(w).max() < 1e-3
+    assert np.abs(u).max() < 1e-3
+    assert rel_l2(interior(a["t_2"], d), interior(hs.a3["t_2"], d)) < 1e-6
+
+
+def test_uniform_state_stays_uniform(oracle32):
+    spec = grid.DomainSpec(nx=10, ny=9, nz=14, bc="periodic", bubble_amp=0.0, noise_amp=0.0, seed=2)
+    hs, a = _run(oracle32, spec, 3)
+    d = hs.dims
+    for k, kind in (("u_2", "u"), ("v_2", "v"), ("w_2", "w"), ("t_2", "mass"), ("ph_2", "w"), ("p", "mass")):
+        x = interior(a[k], d, kind)
+        col = x[:1, :, :1]
+        assert np.array_equal(x, np.broadcast_to(col, x.shape)), k
+    assert not np.array_equal(a["u_2"], hs.a3["u_2"])
+
+
+def _sym_state(bc):
+    spec = grid.DomainSpec(nx=16, ny=16, nz=12, dx=2000.0, dy=2000.0, bc=bc, wind_scale=0.0, noise_amp=0.0,
+                           bubble_amp=3.0, n_bubbles=1, seed=3)
+    return spec
+
+
+@pytest.mark.parametrize("bc", ["periodic", "open"])
+def test_mirror_and_transpose_symmetry(oracle32, bc):
+    """One step: rounding-level asymmetries later grow fast, because the upwind sign of
+    the 5th-order fluxes flips with the sign of near-zero velocities on the mirror plane."""
+    hs, a = _run(oracle32, _sym_state(bc), 1)
+    d = hs.dims
+    t = interior(a["t_2"], d) - interior(hs.a3["t_2"], d)
+    w = interior(a["w_2"], d, "w")
+    u, v = interior(a["u_2"], d, "u"), interior(a["v_2"], d, "v")
+    p = interior(a["p"], d)
+    scale_t, scale_w, scale_u = np.abs(t).max(), np.abs(w).max(), np.abs(u).max()
+    assert scale_w > 1e-3 and scale_u > 1e-4
+    tol = 1e-4
+    # mirror in x: mass points i -> nx-1-i; u at face i -> -u at face nx-i
+    assert np.abs(t - t[:, :, ::-1]).max() <= tol * scale_t
+    assert np.abs(w - w[:, :, ::-1]).max() <= tol * scale_w
+    assert np.abs(u + u[:, :, ::-1]).max() <= tol * scale_u
+    # mirror in y
+    assert np.abs(t - t[::-1]).max() <= tol * scale_t
+    assert np.abs(v + v[::-1]).max() <= tol * scale_u
+    # x <-> y transpose: u(j, k, i) = v(i, k, j)
+    assert np.abs(t - t.transpose(2, 1, 0)).max() <= tol * scale_t
+    assert np.abs(u - v.transpose(2, 1, 0)).max() <= tol * scale_u
+    assert np.abs(p - p.transpose(2, 1, 0)).max() <= tol * np.abs(p).max()
+
+
+def test_periodic_translation_invariance(oracle32):
+    spec = grid.DomainSpec(nx=14, ny=12, nz=10, bc="periodic", n_passive=1, n_bubbles=2, seed=4)
+    hs = grid.build_initial_state(spec)
+    d = hs.dims
+    di, dj = 5, 3
+    sh = hs.copy()
+    i0, j0, nx, ny = d.ids - d.ims, d.jds - d.jms, spec.nx, spec.ny
+
+    def roll(x):
+        if x.ndim == 1:
+            return
+        blk = x[j0:j0 + ny, ..., i0:i0 + nx] if x.ndim == 3 else x[j0:j0 + ny, i0:i0 + nx]
+        blk[...] = np.roll(np.roll(blk, dj, axis=0), di, axis=-1)
+
+    stag = {"u_1": (1, 0), "u_2": (1, 0), "v_1": (0, 1), "v_2": (0, 1)}
+    for name, x in sh.arrays().items():
+        if x.ndim > 1:
+            roll(x)
+            si, sj = stag.get(name, (0, 0))
+            grid._halo_fill_np(x, d, True, bool(si), bool(sj))
+    a = oracle32.run(hs, 2)
+    b = oracle32.run(sh, 2)
+    for k, kind in (("u_2", "u"), ("v_2", "v"), ("w_2", "w"), ("t_2", "mass"), ("mu_2", "mass"), ("q1", "mass")):
+        xa = interior(a[k], d, kind)
+        xb = interior(b[k], d, kind)
+        xa = xa[:ny, ..., :nx] if xa.ndim == 3 else xa[:ny, :nx]
+        xb = xb[:ny, ..., :nx] if xb.ndim == 3 else xb[:ny, :nx]
+        assert np.array_equal(np.roll(np.roll(xa, dj, axis=0), di, axis=-1), xb), k
+
+
+def test_periodic_mass_conservation(oracle32):
+    spec = grid.DomainSpec(nx=24, ny=20, nz=16, dx=1000.0, dy=1000.0, dt=6.0, bc="periodic", n_passive=2,
+                           n_bubbles=2, seed=1002)
+    hs = grid.build_initial_state(spec)
+    d = hs.dims
+
+    def budgets(a):
+        mu = interior(a["mu_2"], d).astype(np.float64)
+        mub = interior(a["mub"], d).astype(np.float64)
+        c1h = a["c1h"][: d.kde - 1].astype(np.float64)
+        c2h = a["c2h"][: d.kde - 1].astype(np.float64)
+        dnw = a["dnw"][: d.kde - 1].astype(np.float64)
+        col = (c1h[None, :, None] * (mu + mub)[:, None, :] + c2h[None, :, None]) * (-dnw)[None, :, None]
+        return float(np.sum(mu)), [np.sum(col * interior(a[f"q{m}"], d).astype(np.float64)) for m in range(hs.n_species)]
+
+    m0, q0 = budgets(hs.arrays())
+    a = oracle32.run(hs, 5)
+    m1, q1 = budgets(a)
+    mtot = float(np.sum(interior(hs.a2["mub"], d).astype(np.float64)))
+    assert abs(m1 - m0) / mtot < 1e-9
+    for x0, x1 in zip(q0, q1):
+        assert abs(x1 - x0) / abs(x0) < 1e-6
+    assert not np.array_equal(a["q1"], hs.q[1])
+
+
+def test_eos_reproduces_hydrostatic_initial_state(oracle32):
+    """calc_p_rho_phi (C, FP32) on the initial state built in float64 numpy from the
+    sounding (an independent evaluation of the same relations) returns its p and al."""
+    hs = grid.build_initial_state(grid.supercell_c0())
+    # calc_p_rho_phi reads the full column mass of the acoustic steps (muts = mub + mu)
+    hs.a2["muts"][...] = hs.a2["mub"] + hs.a2["mu_2"]
+    sess = oracle32.session(hs)
+    sess.phase("calc_p_rho_phi")
+    d = hs.dims
+    p0, pb = interior(hs.a3["p"], d).astype(np.float64), interior(hs.a3["pb"], d).astype(np.float64)
+    p1 = interior(sess.arrays["p"], d).astype(np.float64)
+    ulp = np.spacing(np.abs(p0 + pb).astype(np.float32)).astype(np.float64)
+    # FP32 conditioning: Delta(ph+phb) over a layer and log(pfd/pfu) near 1 each lose
+    # ~4-5 bits, so tens of ulps are expected; a wrong level, sign or exponent gives 1e4+
+    assert np.max(np.abs(p1 - p0) / ulp) < 256
+    al0, alb = interior(hs.a3["al"], d).astype(np.float64), interior(hs.a3["alb"], d).astype(np.float64)
+    al1 = interior(sess.arrays["al"], d).astype(np.float64)
+    ulpa = np.spacing(np.abs(al0 + alb).astype(np.float32)).astype(np.float64)
+    assert np.max(np.abs(al1 - al0) / ulpa) < 256
+
+
+def test_fp32_and_fp64_builds_agree(oracle32, oracle64):
+    hs = grid.build_initial_state(grid.supercell_c0())
+    a, b = oracle32.run(hs, 1), oracle64.run(hs, 1)
+    d = hs.dims
+    bounds = dict(u_2=1e-4, v_2=1e-4, t_2=1e-6, mu_2=1e-3, ph_2=1e-4, p=1e-2, al=1e-3, w_2=1e-1, q0=1e-5)
+    kinds = dict(u_2="u", v_2="v", w_2="w", ph_2="w")
+    for k, bound in bounds.items():
+        err = rel_l2(interior(a[k], d, kinds.get(k, "mass")), interior(b[k], d, kinds.get(k, "mass")))
+        assert err < bound, (k, err)

@@ -8,6 +8,7 @@
 #include <stdio.h>
 #include <string.h>
 #include <math.h>
+#include <iterator>
 #include <map>
 #include <mutex>
 #include <string>
@@ -53,7 +54,25 @@ static void *translate(void *p) {
 extern "C" int wrfmi_register_field(void *host_ptr, size_t bytes) {
   std::lock_guard<std::mutex> lk(g_mu);
   if (!host_ptr || !bytes) return fail(WRFMI_E_POINTER, "register: null pointer or zero size");
-  if (g_reg.count((uintptr_t)host_ptr)) return 0;
+  const uintptr_t a = (uintptr_t)host_ptr;
+  auto it = g_reg.find(a);
+  if (it != g_reg.end()) {
+    // registering the same array again refreshes its mirror from the host
+    if (it->second.bytes == bytes) {
+      HIPCHK(hipMemcpy(it->second.dev, host_ptr, bytes, hipMemcpyHostToDevice));
+      return 0;
+    }
+    HIPCHK(hipFree(it->second.dev));
+    g_reg.erase(it);
+  }
+  auto nx = g_reg.lower_bound(a);
+  if (nx != g_reg.end() && nx->first < a + bytes)
+    return fail(WRFMI_E_POINTER, "register: range overlaps another registered array (unregister it first)");
+  if (nx != g_reg.begin()) {
+    auto pv = std::prev(nx);
+    if (pv->first + pv->second.bytes > a)
+      return fail(WRFMI_E_POINTER, "register: range overlaps another registered array (unregister it first)");
+  }
   void *dev = nullptr;
   HIPCHK(hipMalloc(&dev, bytes));
   HIPCHK(hipMemcpy(dev, host_ptr, bytes, hipMemcpyHostToDevice));
