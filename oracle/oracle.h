@@ -12,6 +12,26 @@
  *
  * Build: oracle/Makefile (gcc -O2 -ffp-contract=off -fno-fast-math).
  * Every function cites the reference file:line it follows.
+ *
+ * How each part is pinned (tests/test_oracle.py):
+ *   o_advect_scalar  bit for bit against the reference's own advect_scalar, compiled in
+ *                    place (oracle/_ref) and through tests/golden/advect_scalar_ref.npz:
+ *                    periodic and open, both signs of time_step, field_old = field and not.
+ *   o_advect_u/_v    bit for bit equal to o_advect_scalar in uniform flow (same fluxes,
+ *                    shifted stencil), plus symmetry/translation/rest-state checks.
+ *   o_advect_w       horizontal part equal to o_advect_scalar to rounding in uniform
+ *                    flow; vertical part bit for bit on levels 2..kde-3.
+ *   whole step       resting atmosphere stays at rest (FP32 and FP64); uniform state stays
+ *                    uniform; mirror and x/y-transpose symmetry of a centred bubble
+ *                    (periodic and open); periodic translation invariance (bitwise);
+ *                    dry-air and species mass conservation; the EOS reproduces the
+ *                    independently built hydrostatic state; FP32 and FP64 builds agree.
+ * Parity unpinned (no reference build, fixture or closed form covers them, and the
+ * benchmark configurations do not exercise them): the w_damping = 1 branch of w_damp,
+ * damp_opt = 3 in advance_w, top_lid = 1, phi_adv_z = 2, hypsometric_opt = 1,
+ * use_theta_m = 0, Coriolis with f or e non-zero, map factors other than 1, terrain
+ * (the surface-w term of advance_w and the cf1..cf3 extrapolations with ht != 0), and
+ * advect_w's lowest and top three levels.
  */
 #ifndef WRF_ORACLE_H
 #define WRF_ORACLE_H
