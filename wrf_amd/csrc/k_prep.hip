@@ -1,67 +1,67 @@
-// rk_step_prep on the GPU: one thread per (i,j) column, marching in k.
+// rk_step_prep on the GPU.
+//   k_step_prep : every point-wise part, one thread per (i,k,j) point
+//   k_ww_scan   : calc_ww_cp's column sum and upward scan, one thread per (i,j) column
 //   calculate_full   dyn_em/module_big_step_utilities_em.F:3587-3636
 //   calc_mu_uv       dyn_em/module_big_step_utilities_em.F:26-180
 //   couple_momentum  dyn_em/module_big_step_utilities_em.F:329-399
 //   calc_ww_cp       dyn_em/module_big_step_utilities_em.F:640-782 (column scan)
 //   calc_cq          dyn_em/module_big_step_utilities_em.F:787-906
 //   calc_alt/calc_php dyn_em/module_big_step_utilities_em.F:910-949, 1227-1266
-// The thread grid spans i = ids-1..ide, j = jds-1..jde (mut is also needed one point
-// into the halo, as calculate_full computes it there).
 #include "kernels.h"
 
 namespace wrfmi {
 
-template <bool COMP>
+// calc_mu_uv's muu, muv at (i, j) (whole-patch tile: its == ids, ite == ide)
+DEVI float prep_muu(const KArgs &a, int i, int j) {
+  const Dm &d = a.d;
+  const float *mu = a.s.mu_2, *mub = a.s.mub;
+  if (i > d.ids && i < d.ide) return 0.5f * (((mu[X2(d, i, j)] + mu[X2(d, i - 1, j)]) + mub[X2(d, i, j)]) + mub[X2(d, i - 1, j)]);
+  if (i == d.ids) {
+    const int im = a.c.periodic_x ? i - 1 : i;
+    return 0.5f * (((mu[X2(d, i, j)] + mu[X2(d, im, j)]) + mub[X2(d, i, j)]) + mub[X2(d, im, j)]);
+  }
+  const int im = a.c.periodic_x ? i : i - 1;  // i == ide
+  return 0.5f * (((mu[X2(d, i - 1, j)] + mu[X2(d, im, j)]) + mub[X2(d, i - 1, j)]) + mub[X2(d, im, j)]);
+}
+DEVI float prep_muv(const KArgs &a, int i, int j) {
+  const Dm &d = a.d;
+  const float *mu = a.s.mu_2, *mub = a.s.mub;
+  if (j > d.jds && j < d.jde) return 0.5f * (((mu[X2(d, i, j)] + mu[X2(d, i, j - 1)]) + mub[X2(d, i, j)]) + mub[X2(d, i, j - 1)]);
+  if (j == d.jds) {
+    const int jm = a.c.periodic_y ? j - 1 : j;
+    return 0.5f * (((mu[X2(d, i, j)] + mu[X2(d, i, jm)]) + mub[X2(d, i, j)]) + mub[X2(d, i, jm)]);
+  }
+  const int jm = a.c.periodic_y ? j : j - 1;
+  return 0.5f * (((mu[X2(d, i, j - 1)] + mu[X2(d, i, jm)]) + mub[X2(d, i, j - 1)]) + mub[X2(d, i, jm)]);
+}
+
+// Point part: one thread per (i, k, j) with i = ids-1..ide, j = jds-1..jde (mut is also
+// needed one point into the halo, as calculate_full computes it there); the thread of
+// level 1 writes the 2-D fields.
 __global__ void __launch_bounds__(256) k_step_prep(KArgs a) {
   const Dm &d = a.d;
   const wrfmi_state &s = a.s;
   const int i = d.ids - 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
   const int j = d.jds - 1 + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  const int k = d.kds + (int)blockIdx.z;
   const int ide = d.ide, jde = d.jde, kde = d.kde;
-  if (i > ide || j > jde) return;
+  if (i > ide || j > jde || k > kde) return;
   const bool mi = i <= ide - 1, mj = j <= jde - 1;
+  const bool top = k == d.kds;
   float mut = 0.f;
   if (mi && mj) {
     mut = s.mub[X2(d, i, j)] + s.mu_2[X2(d, i, j)];
-    s.mut[X2(d, i, j)] = mut;
+    if (top) s.mut[X2(d, i, j)] = mut;
   }
   if (i < d.ids || j < d.jds) return;
-  const float *mu = s.mu_2, *mub = s.mub;
   const int ktf = kde - 1;
-  // calc_mu_uv (whole-patch tile: its == ids, ite == ide)
-  float muu = 0.f, muv = 0.f;
-  if (mj) {
-    if (i > d.ids && i < ide) {
-      muu = 0.5f * (((mu[X2(d, i, j)] + mu[X2(d, i - 1, j)]) + mub[X2(d, i, j)]) + mub[X2(d, i - 1, j)]);
-    } else if (i == d.ids) {
-      int im = a.c.periodic_x ? i - 1 : i;
-      muu = 0.5f * (((mu[X2(d, i, j)] + mu[X2(d, im, j)]) + mub[X2(d, i, j)]) + mub[X2(d, im, j)]);
-    } else {  // i == ide
-      int im = a.c.periodic_x ? i : i - 1;
-      muu = 0.5f * (((mu[X2(d, i - 1, j)] + mu[X2(d, im, j)]) + mub[X2(d, i - 1, j)]) + mub[X2(d, im, j)]);
-    }
-    s.muu[X2(d, i, j)] = muu;
-  }
-  if (mi) {
-    if (j > d.jds && j < jde) {
-      muv = 0.5f * (((mu[X2(d, i, j)] + mu[X2(d, i, j - 1)]) + mub[X2(d, i, j)]) + mub[X2(d, i, j - 1)]);
-    } else if (j == d.jds) {
-      int jm = a.c.periodic_y ? j - 1 : j;
-      muv = 0.5f * (((mu[X2(d, i, j)] + mu[X2(d, i, jm)]) + mub[X2(d, i, j)]) + mub[X2(d, i, jm)]);
-    } else {
-      int jm = a.c.periodic_y ? j : j - 1;
-      muv = 0.5f * (((mu[X2(d, i, j - 1)] + mu[X2(d, i, jm)]) + mub[X2(d, i, j - 1)]) + mub[X2(d, i, jm)]);
-    }
-    s.muv[X2(d, i, j)] = muv;
-  }
-  const float *c1h = s.c1h, *c2h = s.c2h, *c1f = s.c1f, *c2f = s.c2f;
+  const int o = X3(d, i, k, j);
   const int nm = s.n_moist;
-  // couple_momentum, calc_cq, calc_alt, calc_php
-  for (int k = 1; k <= ktf; k++) {
-    const int o = X3(d, i, k, j);
-    const float c1 = c1h[X1(d, k)], c2 = c2h[X1(d, k)];
-    if (mj) {
-      s.ru[o] = s.u_2[o] * (c1 * muu + c2) / s.msfuy[X2(d, i, j)];
+  if (mj) {
+    const float muu = prep_muu(a, i, j);
+    if (top) s.muu[X2(d, i, j)] = muu;
+    if (k <= ktf) {
+      s.ru[o] = s.u_2[o] * (s.c1h[X1(d, k)] * muu + s.c2h[X1(d, k)]) / s.msfuy[X2(d, i, j)];
       float cq;
       if (nm >= 1) {
         float qtot = 0.f;
@@ -72,8 +72,12 @@ __global__ void __launch_bounds__(256) k_step_prep(KArgs a) {
       }
       s.cqu[o] = cq;
     }
-    if (mi) {
-      s.rv[o] = s.v_2[o] * (c1 * muv + c2) * s.msfvx_inv[X2(d, i, j)];
+  }
+  if (mi) {
+    const float muv = prep_muv(a, i, j);
+    if (top) s.muv[X2(d, i, j)] = muv;
+    if (k <= ktf) {
+      s.rv[o] = s.v_2[o] * (s.c1h[X1(d, k)] * muv + s.c2h[X1(d, k)]) * s.msfvx_inv[X2(d, i, j)];
       float cq;
       if (nm >= 1) {
         float qtot = 0.f;
@@ -84,69 +88,80 @@ __global__ void __launch_bounds__(256) k_step_prep(KArgs a) {
       }
       s.cqv[o] = cq;
     }
-    if (mi && mj) {
-      if (k >= 2) {
-        float cw;
-        if (nm >= 1) {
-          float qtot = 0.f;
-          for (int m = 0; m < nm; m++) qtot = qtot + s.q[m][o] + s.q[m][o - d.ni];
-          cw = 0.5f * qtot;
-        } else {
-          cw = 0.f;
-        }
-        s.cqw[o] = cw;
+  }
+  if (mi && mj) {
+    if (k >= 2 && k <= ktf) {
+      float cw;
+      if (nm >= 1) {
+        float qtot = 0.f;
+        for (int m = 0; m < nm; m++) qtot = qtot + s.q[m][o] + s.q[m][o - d.ni];
+        cw = 0.5f * qtot;
+      } else {
+        cw = 0.f;
       }
+      s.cqw[o] = cw;
+    }
+    if (k <= ktf) {
       s.alt[o] = s.al[o] + s.alb[o];
       s.php[o] = 0.5f * (((s.phb[o] + s.phb[o + d.ni]) + s.ph_2[o]) + s.ph_2[o + d.ni]);
     }
+    s.rw[o] = s.w_2[o] * (s.c1f[X1(d, k)] * mut + s.c2f[X1(d, k)]) / s.msfty[X2(d, i, j)];
   }
-  if (mi && mj) {
-    const float msfty = s.msfty[X2(d, i, j)];
-    for (int k = 1; k <= kde; k++) {
-      const int o = X3(d, i, k, j);
-      s.rw[o] = s.w_2[o] * (c1f[X1(d, k)] * mut + c2f[X1(d, k)]) / msfty;
-    }
-  }
-  // calc_ww_cp: ww(1) = ww(kte) = 0 on i = its..ite; the scan on mass columns
+}
+
+// calc_ww_cp (module_big_step_utilities_em.F:640-782): column divergence of the coupled
+// horizontal momentum and the upward scan for omega; one thread per (i, j) column.
+template <bool COMP>
+__global__ void __launch_bounds__(256) k_ww_scan(KArgs a) {
+  const Dm &d = a.d;
+  const wrfmi_state &s = a.s;
+  const int i = d.ids + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int j = d.jds + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  const int ide = d.ide, jde = d.jde, kde = d.kde;
+  if (i > ide || j > jde) return;
+  const bool mi = i <= ide - 1, mj = j <= jde - 1;
+  const float *mu = s.mu_2, *mub = s.mub;
+  const float *c1h = s.c1h, *c2h = s.c2h;
+  const int ktf = kde - 1;
+  // ww(1) = ww(kte) = 0 on i = its..ite
   if (mj) {
     s.ww[X3(d, i, 1, j)] = 0.f;
     s.ww[X3(d, i, kde, j)] = 0.f;
   }
-  if (mi && mj) {
-    // full column mass at the u points i, i+1 and v points j, j+1 (calc_ww_cp's own muu/muv)
-    const float lmuu0 = 0.5f * (((mu[X2(d, i, j)] + mub[X2(d, i, j)]) + mu[X2(d, i - 1, j)]) + mub[X2(d, i - 1, j)]);
-    const float lmuu1 = 0.5f * (((mu[X2(d, i + 1, j)] + mub[X2(d, i + 1, j)]) + mu[X2(d, i, j)]) + mub[X2(d, i, j)]);
-    const float lmuv0 = 0.5f * (((mu[X2(d, i, j)] + mub[X2(d, i, j)]) + mu[X2(d, i, j - 1)]) + mub[X2(d, i, j - 1)]);
-    const float lmuv1 = 0.5f * (((mu[X2(d, i, j + 1)] + mub[X2(d, i, j + 1)]) + mu[X2(d, i, j)]) + mub[X2(d, i, j)]);
-    const float msfuy0 = s.msfuy[X2(d, i, j)], msfuy1 = s.msfuy[X2(d, i + 1, j)];
-    const float mvi0 = s.msfvx_inv[X2(d, i, j)], mvi1 = s.msfvx_inv[X2(d, i, j + 1)];
-    const float msftx = s.msftx[X2(d, i, j)];
-    const float rdx = s.rdx, rdy = s.rdy;
-    float dmdt = 0.f, comp = 0.f;
-    // pass 1: divv(k) and its column sum; divv(k) is parked in ww(k+1) until the scan reads it
-    for (int k = 1; k <= ktf; k++) {
-      const int o = X3(d, i, k, j);
-      const float c1 = c1h[X1(d, k)], c2 = c2h[X1(d, k)];
-      const float tx = (c1 * lmuu1 + c2) * s.u_2[o + 1] / msfuy1 - (c1 * lmuu0 + c2) * s.u_2[o] / msfuy0;
-      const float ty = (c1 * lmuv1 + c2) * s.v_2[o + d.sj] * mvi1 - (c1 * lmuv0 + c2) * s.v_2[o] * mvi0;
-      const float divv = msftx * s.dnw[X1(d, k)] * (rdx * tx + rdy * ty);
-      if (COMP) two_sum_acc(dmdt, comp, divv);
-      else dmdt = dmdt + divv;
-      if (k <= ktf - 1) s.ww[o + d.ni] = divv;
-    }
-    if (COMP) dmdt = dmdt + comp;
-    float prev = 0.f;
-    for (int k = 2; k <= ktf; k++) {
-      const int o = X3(d, i, k, j);
-      const float dv = s.ww[o];
-      const float cur = prev - s.dnw[X1(d, k - 1)] * c1h[X1(d, k - 1)] * dmdt - dv;
-      s.ww[o] = cur;
-      prev = cur;
-    }
+  if (!(mi && mj)) return;
+  // full column mass at the u points i, i+1 and v points j, j+1 (calc_ww_cp's own muu/muv)
+  const float lmuu0 = 0.5f * (((mu[X2(d, i, j)] + mub[X2(d, i, j)]) + mu[X2(d, i - 1, j)]) + mub[X2(d, i - 1, j)]);
+  const float lmuu1 = 0.5f * (((mu[X2(d, i + 1, j)] + mub[X2(d, i + 1, j)]) + mu[X2(d, i, j)]) + mub[X2(d, i, j)]);
+  const float lmuv0 = 0.5f * (((mu[X2(d, i, j)] + mub[X2(d, i, j)]) + mu[X2(d, i, j - 1)]) + mub[X2(d, i, j - 1)]);
+  const float lmuv1 = 0.5f * (((mu[X2(d, i, j + 1)] + mub[X2(d, i, j + 1)]) + mu[X2(d, i, j)]) + mub[X2(d, i, j)]);
+  const float msfuy0 = s.msfuy[X2(d, i, j)], msfuy1 = s.msfuy[X2(d, i + 1, j)];
+  const float mvi0 = s.msfvx_inv[X2(d, i, j)], mvi1 = s.msfvx_inv[X2(d, i, j + 1)];
+  const float msftx = s.msftx[X2(d, i, j)];
+  const float rdx = s.rdx, rdy = s.rdy;
+  float dmdt = 0.f, comp = 0.f;
+  // pass 1: divv(k) and its column sum; divv(k) is parked in ww(k+1) until the scan reads it
+  for (int k = 1; k <= ktf; k++) {
+    const int o = X3(d, i, k, j);
+    const float c1 = c1h[X1(d, k)], c2 = c2h[X1(d, k)];
+    const float tx = (c1 * lmuu1 + c2) * s.u_2[o + 1] / msfuy1 - (c1 * lmuu0 + c2) * s.u_2[o] / msfuy0;
+    const float ty = (c1 * lmuv1 + c2) * s.v_2[o + d.sj] * mvi1 - (c1 * lmuv0 + c2) * s.v_2[o] * mvi0;
+    const float divv = msftx * s.dnw[X1(d, k)] * (rdx * tx + rdy * ty);
+    if (COMP) two_sum_acc(dmdt, comp, divv);
+    else dmdt = dmdt + divv;
+    if (k <= ktf - 1) s.ww[o + d.ni] = divv;
+  }
+  if (COMP) dmdt = dmdt + comp;
+  float prev = 0.f;
+  for (int k = 2; k <= ktf; k++) {
+    const int o = X3(d, i, k, j);
+    const float dv = s.ww[o];
+    const float cur = prev - s.dnw[X1(d, k - 1)] * c1h[X1(d, k - 1)] * dmdt - dv;
+    s.ww[o] = cur;
+    prev = cur;
   }
 }
 
-template __global__ void k_step_prep<false>(KArgs);
-template __global__ void k_step_prep<true>(KArgs);
+template __global__ void k_ww_scan<false>(KArgs);
+template __global__ void k_ww_scan<true>(KArgs);
 
 }  // namespace wrfmi

@@ -9,14 +9,17 @@
 
 namespace wrfmi {
 
+// One thread per (i, k, j) point: the horizontal fluxes at both faces in x and y and the
+// vertical fluxes at both w levels of the point (the column form carried the lower one).
 __global__ void __launch_bounds__(256) k_adv_scalar(KArgs a) {
   const Dm &d = a.d;
   const wrfmi_state &s = a.s;
   const int i = d.ids + (int)(blockIdx.x * blockDim.x + threadIdx.x);
   const int j = d.jds + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  const int k = d.kds + (int)blockIdx.z;
   const int ids = d.ids, ide = d.ide, jds = d.jds, jde = d.jde, kde = d.kde;
   const int its = ids, ite = ide, jts = jds, jte = jde, kts = d.kds, kte = kde;
-  if (i > ide || j > jde) return;
+  if (i > ide || j > jde || k > kte) return;
   const wrfmi_config &c = a.c;
   const float ts = c.time_step >= 0 ? 1.f : -1.f;
   const int ktf = kte - 1;
@@ -25,7 +28,7 @@ __global__ void __launch_bounds__(256) k_adv_scalar(KArgs a) {
   const float rdx = s.rdx, rdy = s.rdy;
   const bool mass = i <= ide - 1 && j <= jde - 1;
   if (!mass) {
-    for (int k = kts; k <= kte; k++) tend[X3(d, i, k, j)] = 0.f;
+    tend[X3(d, i, k, j)] = 0.f;
     return;
   }
   const bool dxs = !(c.periodic_x || c.symmetric_xs || its > ids + 3), dxe = !(c.periodic_x || c.symmetric_xe || ite < ide - 3);
@@ -68,8 +71,7 @@ __global__ void __launch_bounds__(256) k_adv_scalar(KArgs a) {
     return vel * flux3(f[X3(d, i, k - 2, j)], f[X3(d, i, k - 1, j)], f[X3(d, i, k, j)], f[X3(d, i, k + 1, j)], -vel, ts);
   };
   auto P = [&](const float *arr, int ii, int k, int jj) -> float { return arr[X3(d, ii, k, jj)]; };
-  float vf_lo = 0.f;
-  for (int k = kts; k <= kte; k++) {
+  {
     float t = 0.f;
     if (k <= ktf) {
       if (j >= jy0 && j <= jy1) t = t - s.msftx[X2(d, i, j)] * rdy * (fqy(k, j + 1) - fqy(k, j));
@@ -95,8 +97,8 @@ __global__ void __launch_bounds__(256) k_adv_scalar(KArgs a) {
         t = t - rdy * (vb * (P(f, i, k, j) - P(f, i, k, j - 1)) + P(f, i, k, j) * (P(rv, i, k, jte) - P(rv, i, k, jte - 1)));
       }
       const float vf_hi = vflux(k + 1);
+      const float vf_lo = vflux(k);
       t = t - s.rdnw[X1(d, k)] * (vf_hi - vf_lo);
-      vf_lo = vf_hi;
     }
     tend[X3(d, i, k, j)] = t;
   }

@@ -1,5 +1,7 @@
 // The acoustic (small) step on the GPU, dyn_em/module_small_step_em.F.
-//   k_small_prep   : small_step_prep (16-290) + calc_p_rho(step 0) (438-568) + calc_coef_w (570-652)
+//   k_small_prep   : small_step_prep (16-290), one thread per (i,k,j) point
+//   k_p_rho0       : calc_p_rho at step 0 (438-568), one thread per point
+//   k_coef_w       : small_step_prep's 2-D updates + calc_coef_w (570-652), one thread per column
 //   k_advance_uv   : advance_uv (654-967), one thread per (i,k,j) point
 //   k_acoustic_col : advance_mu_t (969-1175) + advance_w (1178-1469) + sumflux (1473-1633)
 //                    + calc_p_rho (438-568), one thread per (i,j) column
@@ -12,8 +14,106 @@
 
 namespace wrfmi {
 
-template <typename CT>
+// small_step_prep, point part: one thread per (i, k, j).  The 2-D fields it updates
+// (muts, muus, muvs, mu_1, mu_2, mu_save, mudf) are written afterwards by k_coef_w, so that
+// every thread here reads their values from before the update.
+struct MuS {  // small_step_prep's column masses of one (i, j)
+  float muts, muus, muvs, mu_new;
+};
+DEVI MuS small_mu(const KArgs &a, int i, int j) {
+  const Dm &d = a.d;
+  const wrfmi_state &s = a.s;
+  const int o2 = X2(d, i, j);
+  MuS m;
+  if (a.rk_step == 1) {
+    m.muts = s.mub[o2] + s.mu_2[o2];
+    m.muus = s.muu[o2];
+    m.muvs = s.muv[o2];
+    m.mu_new = 0.f;
+  } else {
+    m.muts = s.mub[o2] + s.mu_1[o2];
+    m.muus = 0.5f * (((s.mub[o2] + s.mu_1[o2]) + s.mub[X2(d, i - 1, j)]) + s.mu_1[X2(d, i - 1, j)]);
+    m.muvs = 0.5f * (((s.mub[o2] + s.mu_1[o2]) + s.mub[X2(d, i, j - 1)]) + s.mu_1[X2(d, i, j - 1)]);
+    m.mu_new = s.mu_1[o2] - s.mu_2[o2];
+  }
+  return m;
+}
+
 __global__ void __launch_bounds__(256) k_small_prep(KArgs a) {
+  const Dm &d = a.d;
+  const wrfmi_state &s = a.s;
+  const int i = d.ids + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int j = d.jds + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  const int k = d.kds + (int)blockIdx.z;
+  const int ide = d.ide, jde = d.jde, kde = d.kde;
+  if (i > ide || j > jde || k > kde) return;
+  const bool mi = i <= ide - 1, mj = j <= jde - 1, mass = mi && mj;
+  const int k_end = kde - 1;
+  const float *c1h = s.c1h, *c2h = s.c2h, *c1f = s.c1f, *c2f = s.c2f;
+  const int o2 = X2(d, i, j), o = X3(d, i, k, j);
+  const bool rk1 = a.rk_step == 1;
+  const MuS m = small_mu(a, i, j);
+  if (mj && k <= k_end) {
+    const float u2 = s.u_2[o];
+    float u1;
+    if (rk1) { u1 = u2; s.u_1[o] = u1; } else { u1 = s.u_1[o]; }
+    s.u_save[o] = u2;
+    s.u_2[o] = ((c1h[X1(d, k)] * m.muus + c2h[X1(d, k)]) * u1 - (c1h[X1(d, k)] * s.muu[o2] + c2h[X1(d, k)]) * u2) / s.msfuy[o2];
+  }
+  if (mi && k <= k_end) {
+    const float v2 = s.v_2[o];
+    float v1;
+    if (rk1) { v1 = v2; s.v_1[o] = v1; } else { v1 = s.v_1[o]; }
+    s.v_save[o] = v2;
+    s.v_2[o] = ((c1h[X1(d, k)] * m.muvs + c2h[X1(d, k)]) * v1 - (c1h[X1(d, k)] * s.muv[o2] + c2h[X1(d, k)]) * v2) * s.msfvx_inv[o2];
+  }
+  if (!mass) return;
+  const float mut = s.mut[o2], msfty = s.msfty[o2];
+  if (k <= k_end) {
+    s.c2a[o] = K_CPOVCV * (s.pb[o] + s.p[o]) / s.alt[o];
+    const float t2 = s.t_2[o];
+    float t1;
+    if (rk1) { t1 = t2; s.t_1[o] = t1; } else { t1 = s.t_1[o]; }
+    s.t_save[o] = t2;
+    s.t_2[o] = (c1h[X1(d, k)] * m.muts + c2h[X1(d, k)]) * t1 - (c1h[X1(d, k)] * mut + c2h[X1(d, k)]) * t2;
+  }
+  {
+    const float w2 = s.w_2[o], ph2 = s.ph_2[o];
+    float w1, ph1;
+    if (rk1) { w1 = w2; ph1 = ph2; s.w_1[o] = w1; s.ph_1[o] = ph1; } else { w1 = s.w_1[o]; ph1 = s.ph_1[o]; }
+    s.w_save[o] = w2;
+    s.w_2[o] = ((c1f[X1(d, k)] * m.muts + c2f[X1(d, k)]) * w1 - (c1f[X1(d, k)] * mut + c2f[X1(d, k)]) * w2) / msfty;
+    s.ph_save[o] = ph2;
+    s.ph_2[o] = ph1 - ph2;
+    s.ww1[o] = s.ww[o];
+  }
+}
+
+// calc_p_rho at step 0 (mu = the new mu_2, mut = muts, t_1 = t_save): one thread per point
+__global__ void __launch_bounds__(256) k_p_rho0(KArgs a) {
+  const Dm &d = a.d;
+  const wrfmi_state &s = a.s;
+  const int i = d.ids + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int j = d.jds + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  const int k = d.kds + (int)blockIdx.z;
+  if (i > d.ide - 1 || j > d.jde - 1 || k > d.kde - 1) return;
+  if (!a.c.non_hydrostatic) return;
+  const int o = X3(d, i, k, j);
+  const MuS m = small_mu(a, i, j);
+  const float mu = m.mu_new, muts = m.muts;
+  const float c1 = s.c1h[X1(d, k)], c2 = s.c2h[X1(d, k)];
+  const float alt = s.alt[o];
+  const float al = -1.f / (c1 * muts + c2) * (alt * (c1 * mu) + s.rdnw[X1(d, k)] * (s.ph_2[o + d.ni] - s.ph_2[o]));
+  s.al[o] = al;
+  const float t1 = s.t_save[o];
+  const float pp = s.c2a[o] * (alt * (s.t_2[o] - (c1 * mu) * t1) / ((c1 * muts + c2) * (K_T0 + t1)) - al);
+  s.p[o] = pp;
+  s.pm1[o] = pp;
+}
+
+// small_step_prep's 2-D updates and calc_coef_w: one thread per (i, j) column
+template <typename CT>
+__global__ void __launch_bounds__(256) k_coef_w(KArgs a) {
   const Dm &d = a.d;
   const wrfmi_state &s = a.s;
   const int i = d.ids + (int)(blockIdx.x * blockDim.x + threadIdx.x);
@@ -21,155 +121,59 @@ __global__ void __launch_bounds__(256) k_small_prep(KArgs a) {
   const int ide = d.ide, jde = d.jde, kde = d.kde;
   if (i > ide || j > jde) return;
   const bool mi = i <= ide - 1, mj = j <= jde - 1, mass = mi && mj;
-  const int k_end = kde - 1;
   const float *c1h = s.c1h, *c2h = s.c2h, *c1f = s.c1f, *c2f = s.c2f;
   const int o2 = X2(d, i, j);
-  float muts = 0.f, muus = 0.f, muvs = 0.f;
-  if (a.rk_step == 1) {
-    if (mass) {
+  const MuS m = small_mu(a, i, j);
+  if (mass) {
+    if (a.rk_step == 1) {
       s.mu_1[o2] = s.mu_2[o2];
       s.mudf[o2] = 0.f;
-      for (int k = 1; k <= k_end; k++) {
-        const int o = X3(d, i, k, j);
-        s.t_1[o] = s.t_2[o];
-      }
-      for (int k = 1; k <= kde; k++) {
-        const int o = X3(d, i, k, j);
-        s.w_1[o] = s.w_2[o];
-        s.ph_1[o] = s.ph_2[o];
-      }
-      muts = s.mub[o2] + s.mu_2[o2];
-      s.muts[o2] = muts;
     }
-    if (mj)
-      for (int k = 1; k <= k_end; k++) {
-        const int o = X3(d, i, k, j);
-        s.u_1[o] = s.u_2[o];
-      }
-    if (mi)
-      for (int k = 1; k <= k_end; k++) {
-        const int o = X3(d, i, k, j);
-        s.v_1[o] = s.v_2[o];
-      }
-    if (mj) { muus = s.muu[o2]; s.muus[o2] = muus; }
-    if (mi) { muvs = s.muv[o2]; s.muvs[o2] = muvs; }
-    if (mass) {
-      s.mu_save[o2] = s.mu_2[o2];
-      s.mu_2[o2] = 0.f;
-    }
-  } else {
-    if (mass) { muts = s.mub[o2] + s.mu_1[o2]; s.muts[o2] = muts; }
-    if (mj) {
-      muus = 0.5f * (((s.mub[o2] + s.mu_1[o2]) + s.mub[X2(d, i - 1, j)]) + s.mu_1[X2(d, i - 1, j)]);
-      s.muus[o2] = muus;
-    }
-    if (mi) {
-      muvs = 0.5f * (((s.mub[o2] + s.mu_1[o2]) + s.mub[X2(d, i, j - 1)]) + s.mu_1[X2(d, i, j - 1)]);
-      s.muvs[o2] = muvs;
-    }
-    if (mass) {
-      s.mu_save[o2] = s.mu_2[o2];
-      s.mu_2[o2] = s.mu_1[o2] - s.mu_2[o2];
-    }
+    s.muts[o2] = m.muts;
+    s.mu_save[o2] = s.mu_2[o2];
+    s.mu_2[o2] = m.mu_new;
   }
-  if (mj) {
-    const float muu = s.muu[o2], msfuy = s.msfuy[o2];
-    for (int k = 1; k <= k_end; k++) {
-      const int o = X3(d, i, k, j);
-      const float u2 = s.u_2[o];
-      s.u_save[o] = u2;
-      s.u_2[o] = ((c1h[X1(d, k)] * muus + c2h[X1(d, k)]) * s.u_1[o] - (c1h[X1(d, k)] * muu + c2h[X1(d, k)]) * u2) / msfuy;
-    }
+  if (mj) s.muus[o2] = m.muus;
+  if (mi) s.muvs[o2] = m.muvs;
+  if (!mass || !a.c.non_hydrostatic) return;
+  const float mut = s.mut[o2];
+  CT *A = (CT *)s.a, *AL = (CT *)s.alpha, *GA = (CT *)s.gamma;
+  const CT lid = a.c.top_lid ? CT(0) : CT(1);
+  const CT epssm = (CT)a.c.epssm;
+  CT cof = (CT(.5) * (CT)a.dts * (CT)K_G * (CT(1) + epssm));
+  cof = cof * cof;
+  const CT MUT = (CT)mut;
+  auto cH = [&](int k) -> CT { return (CT)c1h[X1(d, k)] * MUT + (CT)c2h[X1(d, k)]; };
+  auto cF = [&](int k) -> CT { return (CT)c1f[X1(d, k)] * MUT + (CT)c2f[X1(d, k)]; };
+  auto rdnw = [&](int k) -> CT { return (CT)s.rdnw[X1(d, k)]; };
+  auto rdn = [&](int k) -> CT { return (CT)s.rdn[X1(d, k)]; };
+  auto C2A = [&](int k) -> CT { return (CT)s.c2a[X3(d, i, k, j)]; };
+  auto CQW = [&](int k) -> CT { return (CT)s.cqw[X3(d, i, k, j)]; };
+  {
+    const int k = kde - 1;
+    A[X3(d, i, 2, j)] = CT(0);
+    A[X3(d, i, kde, j)] = -CT(2) * cof * (rdnw(kde - 1) * rdnw(kde - 1)) * C2A(kde - 1) * lid / (cH(k) * cF(k));
+    GA[X3(d, i, 1, j)] = CT(0);
   }
-  if (mi) {
-    const float muv = s.muv[o2], mvi = s.msfvx_inv[o2];
-    for (int k = 1; k <= k_end; k++) {
-      const int o = X3(d, i, k, j);
-      const float v2 = s.v_2[o];
-      s.v_save[o] = v2;
-      s.v_2[o] = ((c1h[X1(d, k)] * muvs + c2h[X1(d, k)]) * s.v_1[o] - (c1h[X1(d, k)] * muv + c2h[X1(d, k)]) * v2) * mvi;
-    }
+  CT gam = CT(0);
+  for (int k = 2; k <= kde - 1; k++) {
+    const CT ak = k >= 3 ? -CQW(k) * cof * rdn(k) * rdnw(k - 1) * C2A(k - 1) / (cH(k - 1) * cF(k - 1)) : CT(0);
+    if (k >= 3) A[X3(d, i, k, j)] = ak;
+    const CT b = CT(1) + CQW(k) * cof * rdn(k) *
+                             (rdnw(k) * C2A(k) / (cH(k) * cF(k)) + rdnw(k - 1) * C2A(k - 1) / (cH(k - 1) * cF(k)));
+    const CT c = -CQW(k) * cof * rdn(k) * rdnw(k) * C2A(k) / (cH(k) * cF(k + 1));
+    const CT alpha = CT(1) / (b - ak * gam);
+    AL[X3(d, i, k, j)] = alpha;
+    gam = c * alpha;
+    GA[X3(d, i, k, j)] = gam;
   }
-  if (!mass) return;
-  const float mut = s.mut[o2], msfty = s.msfty[o2];
-  s.ww1[X3(d, i, kde, j)] = 0.f;
-  s.ww1[X3(d, i, 1, j)] = 0.f;
-  for (int k = 1; k <= k_end; k++) {
-    const int o = X3(d, i, k, j);
-    s.c2a[o] = K_CPOVCV * (s.pb[o] + s.p[o]) / s.alt[o];
-    const float t2 = s.t_2[o];
-    s.t_save[o] = t2;
-    s.t_2[o] = (c1h[X1(d, k)] * muts + c2h[X1(d, k)]) * s.t_1[o] - (c1h[X1(d, k)] * mut + c2h[X1(d, k)]) * t2;
-  }
-  for (int k = 1; k <= kde; k++) {
-    const int o = X3(d, i, k, j);
-    const float w2 = s.w_2[o];
-    s.w_save[o] = w2;
-    s.w_2[o] = ((c1f[X1(d, k)] * muts + c2f[X1(d, k)]) * s.w_1[o] - (c1f[X1(d, k)] * mut + c2f[X1(d, k)]) * w2) / msfty;
-    const float ph2 = s.ph_2[o];
-    s.ph_save[o] = ph2;
-    s.ph_2[o] = s.ph_1[o] - ph2;
-    s.ww1[o] = s.ww[o];
-  }
-  // calc_p_rho, step 0 (mu = mu_2, mut = muts, t_1 = t_save)
-  const float mu = s.mu_2[o2];
-  if (a.c.non_hydrostatic) {
-    for (int k = 1; k <= k_end; k++) {
-      const int o = X3(d, i, k, j);
-      const float c1 = c1h[X1(d, k)], c2 = c2h[X1(d, k)];
-      const float alt = s.alt[o];
-      const float al = -1.f / (c1 * muts + c2) * (alt * (c1 * mu) + s.rdnw[X1(d, k)] * (s.ph_2[o + d.ni] - s.ph_2[o]));
-      s.al[o] = al;
-      const float t1 = s.t_save[o];
-      const float pp = s.c2a[o] * (alt * (s.t_2[o] - (c1 * mu) * t1) / ((c1 * muts + c2) * (K_T0 + t1)) - al);
-      s.p[o] = pp;
-      s.pm1[o] = pp;
-    }
-  }
-  // calc_coef_w
-  if (a.c.non_hydrostatic) {
-    CT *A = (CT *)s.a, *AL = (CT *)s.alpha, *GA = (CT *)s.gamma;
-    const CT lid = a.c.top_lid ? CT(0) : CT(1);
-    const CT epssm = (CT)a.c.epssm;
-    CT cof = (CT(.5) * (CT)a.dts * (CT)K_G * (CT(1) + epssm));
-    cof = cof * cof;
-    const CT MUT = (CT)mut;
-    auto cH = [&](int k) -> CT { return (CT)c1h[X1(d, k)] * MUT + (CT)c2h[X1(d, k)]; };
-    auto cF = [&](int k) -> CT { return (CT)c1f[X1(d, k)] * MUT + (CT)c2f[X1(d, k)]; };
-    const CT *dummy = nullptr;
-    (void)dummy;
-    auto rdnw = [&](int k) -> CT { return (CT)s.rdnw[X1(d, k)]; };
-    auto rdn = [&](int k) -> CT { return (CT)s.rdn[X1(d, k)]; };
-    auto C2A = [&](int k) -> CT { return (CT)s.c2a[X3(d, i, k, j)]; };
-    auto CQW = [&](int k) -> CT { return (CT)s.cqw[X3(d, i, k, j)]; };
-    {
-      const int k = kde - 1;
-      A[X3(d, i, 2, j)] = CT(0);
-      A[X3(d, i, kde, j)] = -CT(2) * cof * (rdnw(kde - 1) * rdnw(kde - 1)) * C2A(kde - 1) * lid / (cH(k) * cF(k));
-      GA[X3(d, i, 1, j)] = CT(0);
-    }
-    for (int kk = 3; kk <= kde - 1; kk++) {
-      const int k = kk - 1;
-      A[X3(d, i, kk, j)] = -CQW(kk) * cof * rdn(kk) * rdnw(kk - 1) * C2A(kk - 1) / (cH(k) * cF(k));
-    }
-    CT gam = CT(0);
-    for (int k = 2; k <= kde - 1; k++) {
-      const CT b = CT(1) + CQW(k) * cof * rdn(k) *
-                               (rdnw(k) * C2A(k) / (cH(k) * cF(k)) + rdnw(k - 1) * C2A(k - 1) / (cH(k - 1) * cF(k)));
-      const CT c = -CQW(k) * cof * rdn(k) * rdnw(k) * C2A(k) / (cH(k) * cF(k + 1));
-      const CT alpha = CT(1) / (b - A[X3(d, i, k, j)] * gam);
-      AL[X3(d, i, k, j)] = alpha;
-      gam = c * alpha;
-      GA[X3(d, i, k, j)] = gam;
-    }
-    {
-      const int k = kde;
-      const CT b = CT(1) + CT(2) * cof * (rdnw(kde - 1) * rdnw(kde - 1)) * C2A(kde - 1) / (cH(k - 1) * cF(k));
-      const CT c = CT(0);
-      const CT alpha = CT(1) / (b - A[X3(d, i, kde, j)] * gam);
-      AL[X3(d, i, kde, j)] = alpha;
-      GA[X3(d, i, kde, j)] = c * alpha;
-    }
+  {
+    const int k = kde;
+    const CT b = CT(1) + CT(2) * cof * (rdnw(kde - 1) * rdnw(kde - 1)) * C2A(kde - 1) / (cH(k - 1) * cF(k));
+    const CT c = CT(0);
+    const CT alpha = CT(1) / (b - A[X3(d, i, kde, j)] * gam);
+    AL[X3(d, i, kde, j)] = alpha;
+    GA[X3(d, i, kde, j)] = c * alpha;
   }
 }
 
@@ -512,68 +516,62 @@ __global__ void __launch_bounds__(256) k_acoustic_col(KArgs a) {
   }
 }
 
+// One thread per (i, k, j); the thread of level kds also writes the 2-D fields.
 __global__ void __launch_bounds__(256) k_small_finish(KArgs a) {
   const Dm &d = a.d;
   const wrfmi_state &s = a.s;
   const int i = d.ids + (int)(blockIdx.x * blockDim.x + threadIdx.x);
   const int j = d.jds + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  const int k = d.kds + (int)blockIdx.z;
   const int ide = d.ide, jde = d.jde, kde = d.kde;
-  if (i > ide || j > jde) return;
+  if (i > ide || j > jde || k > kde) return;
   const bool mi = i <= ide - 1, mj = j <= jde - 1, mass = mi && mj;
   const float *c1h = s.c1h, *c2h = s.c2h, *c1f = s.c1f, *c2f = s.c2f;
   const float *muts = s.muts;
   const int o2 = X2(d, i, j);
+  const int o = X3(d, i, k, j);
+  const bool top = k == d.kds;
   // calc_mu_uv_1(muts -> muus, muvs)
   float muus = 0.f, muvs = 0.f;
   if (mj) {
     if (i > d.ids && i < ide) muus = 0.5f * (muts[o2] + muts[X2(d, i - 1, j)]);
     else if (i == d.ids) muus = 0.5f * (muts[o2] + muts[X2(d, a.c.periodic_x ? i - 1 : i, j)]);
     else muus = 0.5f * (muts[X2(d, i - 1, j)] + muts[X2(d, a.c.periodic_x ? i : i - 1, j)]);
-    s.muus[o2] = muus;
+    if (top) s.muus[o2] = muus;
   }
   if (mi) {
     if (j > d.jds && j < jde) muvs = 0.5f * (muts[o2] + muts[X2(d, i, j - 1)]);
     else if (j == d.jds) muvs = 0.5f * (muts[o2] + muts[X2(d, i, a.c.periodic_y ? j - 1 : j)]);
     else muvs = 0.5f * (muts[X2(d, i, j - 1)] + muts[X2(d, i, a.c.periodic_y ? j : j - 1)]);
-    s.muvs[o2] = muvs;
+    if (top) s.muvs[o2] = muvs;
   }
   // small_step_finish
-  if (mi) {
+  if (mi && k <= kde - 1) {
     const float muv = s.muv[o2], msfvx = s.msfvx[o2];
-    for (int k = d.kds; k <= kde - 1; k++) {
-      const int o = X3(d, i, k, j);
-      s.v_2[o] = (msfvx * s.v_2[o] + s.v_save[o] * (c1h[X1(d, k)] * muv + c2h[X1(d, k)])) / (c1h[X1(d, k)] * muvs + c2h[X1(d, k)]);
-    }
+    s.v_2[o] = (msfvx * s.v_2[o] + s.v_save[o] * (c1h[X1(d, k)] * muv + c2h[X1(d, k)])) / (c1h[X1(d, k)] * muvs + c2h[X1(d, k)]);
   }
-  if (mj) {
+  if (mj && k <= kde - 1) {
     const float muu = s.muu[o2], msfuy = s.msfuy[o2];
-    for (int k = d.kds; k <= kde - 1; k++) {
-      const int o = X3(d, i, k, j);
-      s.u_2[o] = (msfuy * s.u_2[o] + s.u_save[o] * (c1h[X1(d, k)] * muu + c2h[X1(d, k)])) / (c1h[X1(d, k)] * muus + c2h[X1(d, k)]);
-    }
+    s.u_2[o] = (msfuy * s.u_2[o] + s.u_save[o] * (c1h[X1(d, k)] * muu + c2h[X1(d, k)])) / (c1h[X1(d, k)] * muus + c2h[X1(d, k)]);
   }
   if (!mass) return;
   const float mut = s.mut[o2], mts = muts[o2], msfty = s.msfty[o2];
-  for (int k = d.kds; k <= kde; k++) {
-    const int o = X3(d, i, k, j);
-    s.w_2[o] = (msfty * s.w_2[o] + s.w_save[o] * (c1f[X1(d, k)] * mut + c2f[X1(d, k)])) / (c1f[X1(d, k)] * mts + c2f[X1(d, k)]);
-    s.ph_2[o] = s.ph_2[o] + s.ph_save[o];
-    s.ww[o] = s.ww[o] + s.ww1[o];
-  }
-  const bool last = a.rk_step >= a.c.rk_ord;
-  for (int k = d.kds; k <= kde - 1; k++) {
-    const int o = X3(d, i, k, j);
+  s.w_2[o] = (msfty * s.w_2[o] + s.w_save[o] * (c1f[X1(d, k)] * mut + c2f[X1(d, k)])) / (c1f[X1(d, k)] * mts + c2f[X1(d, k)]);
+  s.ph_2[o] = s.ph_2[o] + s.ph_save[o];
+  s.ww[o] = s.ww[o] + s.ww1[o];
+  if (k <= kde - 1) {
+    const bool last = a.rk_step >= a.c.rk_ord;
     const float cm = c1h[X1(d, k)] * mut + c2h[X1(d, k)];
     if (!last)
       s.t_2[o] = (s.t_2[o] + s.t_save[o] * cm) / (c1h[X1(d, k)] * mts + c2h[X1(d, k)]);
     else
       s.t_2[o] = (s.t_2[o] - a.dts * (float)a.nsmall * cm * s.h_diabatic[o] + s.t_save[o] * cm) / (c1h[X1(d, k)] * mts + c2h[X1(d, k)]);
   }
-  s.mu_2[o2] = s.mu_2[o2] + s.mu_save[o2];
+  if (top) s.mu_2[o2] = s.mu_2[o2] + s.mu_save[o2];
 }
 
-template __global__ void k_small_prep<float>(KArgs);
-template __global__ void k_small_prep<double>(KArgs);
+template __global__ void k_coef_w<float>(KArgs);
+template __global__ void k_coef_w<double>(KArgs);
 template __global__ void k_acoustic_col<float, false>(KArgs);
 template __global__ void k_acoustic_col<double, true>(KArgs);
 

@@ -34,9 +34,10 @@ __global__ void __launch_bounds__(256) k_tend_u(KArgs a) {
   const wrfmi_state &s = a.s;
   const int i = d.ids + (int)(blockIdx.x * blockDim.x + threadIdx.x);
   const int j = d.jds + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  const int k = d.kds + (int)blockIdx.z;
   const int ids = d.ids, ide = d.ide, jds = d.jds, jde = d.jde, kde = d.kde;
   const int its = ids, ite = ide, jts = jds, jte = jde, kts = d.kds, kte = kde;
-  if (i > ide || j > jde) return;
+  if (i > ide || j > jde || k > kte) return;
   const Geo g = geo(a.c);
   const float ts = a.c.time_step >= 0 ? 1.f : -1.f;
   const int ktf = kte - 1;
@@ -146,13 +147,13 @@ __global__ void __launch_bounds__(256) k_tend_u(KArgs a) {
     }
     return .5f * (s.fnm[X1(d, k)] * (p[X3(d, i - 1, k, j)] + p[X3(d, i, k, j)]) + s.fnp[X1(d, k)] * (p[X3(d, i - 1, k - 1, j)] + p[X3(d, i, k - 1, j)]));
   };
-  float vf_lo = vflux(kts);
-  float dpn_lo = (nh && row_ok && i >= ih0 && i <= ih1) ? dpn(1) : 0.f;
-  for (int k = kts; k <= kte; k++) {
+  {
     const int o = X3(d, i, k, j);
     float t = 0.f;
     float vf_hi = 0.f, dpn_hi = 0.f;
     if (row_ok && k <= ktf) {
+      const float vf_lo = (i >= iv0 && i <= iv1) ? vflux(k) : 0.f;
+      const float dpn_lo = (nh && i >= ih0 && i <= ih1) ? dpn(k) : 0.f;
       // advect_u: y flux divergence
       if (i >= iy0 && i <= iy1 && j >= jy0 && j <= jy1) {
         const float mrdy = s.msfux[X2(d, i, j)] * rdy;
@@ -243,8 +244,6 @@ __global__ void __launch_bounds__(256) k_tend_u(KArgs a) {
       t = t + tf / msf;
     }
     s.ru_tend[o] = t;
-    vf_lo = vf_hi;
-    dpn_lo = dpn_hi;
   }
 }
 
@@ -254,9 +253,10 @@ __global__ void __launch_bounds__(256) k_tend_v(KArgs a) {
   const wrfmi_state &s = a.s;
   const int i = d.ids + (int)(blockIdx.x * blockDim.x + threadIdx.x);
   const int j = d.jds + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  const int k = d.kds + (int)blockIdx.z;
   const int ids = d.ids, ide = d.ide, jds = d.jds, jde = d.jde, kde = d.kde;
   const int its = ids, ite = ide, jts = jds, jte = jde, kts = d.kds, kte = kde;
-  if (i > ide || j > jde) return;
+  if (i > ide || j > jde || k > kte) return;
   const Geo g = geo(a.c);
   const float ts = a.c.time_step >= 0 ? 1.f : -1.f;
   const int ktf = kte - 1;
@@ -358,12 +358,12 @@ __global__ void __launch_bounds__(256) k_tend_v(KArgs a) {
     }
     return .5f * (s.fnm[X1(d, k)] * (p[X3(d, i, k, j - 1)] + p[X3(d, i, k, j)]) + s.fnp[X1(d, k)] * (p[X3(d, i, k - 1, j - 1)] + p[X3(d, i, k - 1, j)]));
   };
-  float vf_lo = vflux(kts);
-  float dpn_lo = (nh && col_ok && j >= jh0 && j <= jh1) ? dpn(1) : 0.f;
-  for (int k = kts; k <= kte; k++) {
+  {
     const int o = X3(d, i, k, j);
     float t = 0.f, vf_hi = 0.f, dpn_hi = 0.f;
     if (col_ok && k <= ktf) {
+      const float vf_lo = (j >= jv0 && j <= jv1) ? vflux(k) : 0.f;
+      const float dpn_lo = (nh && j >= jh0 && j <= jh1) ? dpn(k) : 0.f;
       if (j >= jy0 && j <= jy1) {
         const float mrdy = s.msfvy[X2(d, i, j)] * rdy;
         t = t - mrdy * (fqy(k, j + 1) - fqy(k, j));
@@ -444,8 +444,6 @@ __global__ void __launch_bounds__(256) k_tend_v(KArgs a) {
       t = t + tf * s.msfvx_inv[X2(d, i, j)];
     }
     s.rv_tend[o] = t;
-    vf_lo = vf_hi;
-    dpn_lo = dpn_hi;
   }
 }
 
@@ -455,9 +453,10 @@ __global__ void __launch_bounds__(256) k_tend_w(KArgs a) {
   const wrfmi_state &s = a.s;
   const int i = d.ids + (int)(blockIdx.x * blockDim.x + threadIdx.x);
   const int j = d.jds + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  const int k = d.kds + (int)blockIdx.z;
   const int ids = d.ids, ide = d.ide, jds = d.jds, jde = d.jde, kde = d.kde;
   const int its = ids, ite = ide, jts = jds, jte = jde, kts = d.kds, kte = kde;
-  if (i > ide || j > jde) return;
+  if (i > ide || j > jde || k > kte) return;
   const Geo g = geo(a.c);
   const float ts = a.c.time_step >= 0 ? 1.f : -1.f;
   const int ktf = kte - 1;
@@ -511,11 +510,7 @@ __global__ void __launch_bounds__(256) k_tend_w(KArgs a) {
   const float msfty = mass ? s.msfty[X2(d, i, j)] : 1.f;
   const float mu = mass ? s.mu_2[X2(d, i, j)] : 0.f, mub = mass ? s.mub[X2(d, i, j)] : 0.f;
   const float mut = mass ? s.mut[X2(d, i, j)] : 0.f;
-  // pg_buoy_w reads the moisture coefficient cqw(kde-1) before it is replaced by 1/(1+cqw)
-  const float cqw_top = mass ? s.cqw[X3(d, i, kde - 1, j)] : 0.f;
-  float vf_lo = 0.f;
-  if (mass) vf_lo = vflux(kts + 1);
-  for (int k = kts; k <= kte; k++) {
+  {
     const int o = X3(d, i, k, j);
     float t = 0.f;
     if (mass) {
@@ -582,13 +577,22 @@ __global__ void __launch_bounds__(256) k_tend_w(KArgs a) {
         // vertical advection of w (order 3) with the lid flux term
         if (k <= ktf) {
           const float vf_hi = vflux(k + 1);
-          t = t - s.rdn[X1(d, k)] * (vf_hi - vf_lo);
-          vf_lo = vf_hi;
+          t = t - s.rdn[X1(d, k)] * (vf_hi - vflux(k));
         } else {
-          t = t + 2.f * s.rdn[X1(d, k - 1)] * (vf_lo);
+          t = t + 2.f * s.rdn[X1(d, k - 1)] * (vflux(kde));
         }
         // pg_buoy_w
         if (k == kde) {
+          // The top row reads the moisture coefficient cqw(kde-1) of calc_cq before the
+          // thread of level kde-1 replaces it by 1/(1+cqw): recompute it as calc_cq does
+          // (module_big_step_utilities_em.F:787-906) from the moist species.
+          float cqw_top = 0.f;
+          if (s.n_moist >= 1) {
+            const int ot = X3(d, i, kde - 1, j);
+            float qtot = 0.f;
+            for (int m = 0; m < s.n_moist; m++) qtot = qtot + s.q[m][ot] + s.q[m][ot - d.ni];
+            cqw_top = 0.5f * qtot;
+          }
           const float cq1 = 1.f / (1.f + cqw_top);
           const float cq2 = cqw_top * cq1;
           t = t + (1.f / msfty) * K_G *
@@ -641,9 +645,10 @@ __global__ void __launch_bounds__(256) k_tend_t(KArgs a) {
   const wrfmi_state &s = a.s;
   const int i = d.ids + (int)(blockIdx.x * blockDim.x + threadIdx.x);
   const int j = d.jds + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  const int k = d.kds + (int)blockIdx.z;
   const int ids = d.ids, ide = d.ide, jds = d.jds, jde = d.jde, kde = d.kde;
   const int its = ids, ite = ide, jts = jds, jte = jde, kts = d.kds, kte = kde;
-  if (i > ide || j > jde) return;
+  if (i > ide || j > jde || k > kte) return;
   const Geo g = geo(a.c);
   const float ts = a.c.time_step >= 0 ? 1.f : -1.f;
   const int ktf = kte - 1;
@@ -713,17 +718,14 @@ __global__ void __launch_bounds__(256) k_tend_t(KArgs a) {
   auto P = [&](const float *arr, int ii, int k, int jj) -> float { return arr[X3(d, ii, k, jj)]; };
   const float msfty = mass ? s.msfty[X2(d, i, j)] : 1.f;
   const float mut = mass ? s.mut[X2(d, i, j)] : 0.f;
-  float vf_lo = vflux(kts);
-  float wd_lo = 0.f;  // rhs_ph wdwn(k) carried upward
-  if (mass) {
-    const int k = 2;
+  // rhs_ph's wdwn at w level kk (kk = 2..kde-1)
+  auto wdwn = [&](int kk) -> float {
     if (a.c.phi_adv_z == 2)
-      wd_lo = s.rdnw[X1(d, k - 1)] * (P(ph, i, k, j) - P(ph, i, k - 1, j) + P(phb, i, k, j) - P(phb, i, k - 1, j));
-    else
-      wd_lo = .5f * (P(rom, i, k, j) + P(rom, i, k - 1, j)) * s.rdnw[X1(d, k - 1)] *
-              (P(ph, i, k, j) - P(ph, i, k - 1, j) + P(phb, i, k, j) - P(phb, i, k - 1, j));
-  }
-  for (int k = kts; k <= kte; k++) {
+      return s.rdnw[X1(d, kk - 1)] * (P(ph, i, kk, j) - P(ph, i, kk - 1, j) + P(phb, i, kk, j) - P(phb, i, kk - 1, j));
+    return .5f * (P(rom, i, kk, j) + P(rom, i, kk - 1, j)) * s.rdnw[X1(d, kk - 1)] *
+           (P(ph, i, kk, j) - P(ph, i, kk - 1, j) + P(phb, i, kk, j) - P(phb, i, kk - 1, j));
+  };
+  {
     const int o = X3(d, i, k, j);
     // zero_tend of the *_save arrays over the whole tile
     s.u_save[o] = 0.f;
@@ -764,7 +766,7 @@ __global__ void __launch_bounds__(256) k_tend_t(KArgs a) {
           t = t - rdy * (vb * (P(f, i, k, j) - P(f, i, k, j - 1)) + P(f, i, k, j) * (P(rv, i, k, jte) - P(rv, i, k, jte - 1)));
         }
         vf_hi = vflux(k + 1);
-        t = t - s.rdnw[X1(d, k)] * (vf_hi - vf_lo);
+        t = t - s.rdnw[X1(d, k)] * (vf_hi - vflux(k));
         // rk_addtend_dry (theta)
         float tf = s.t_tendf[o];
         if (a.rk_step == 1) {
@@ -775,16 +777,11 @@ __global__ void __launch_bounds__(256) k_tend_t(KArgs a) {
       }
       // rhs_ph
       if (k >= 2 && k <= kte - 1) {
-        float wd_hi;
-        if (a.c.phi_adv_z == 2) {
-          wd_hi = s.rdnw[X1(d, k)] * (P(ph, i, k + 1, j) - P(ph, i, k, j) + P(phb, i, k + 1, j) - P(phb, i, k, j));
+        const float wd_hi = wdwn(k + 1), wd_lo = wdwn(k);
+        if (a.c.phi_adv_z == 2)
           pt = pt - P(rom, i, k, j) * (s.fnm[X1(d, k)] * wd_hi + s.fnp[X1(d, k)] * wd_lo);
-        } else {
-          wd_hi = .5f * (P(rom, i, k + 1, j) + P(rom, i, k, j)) * s.rdnw[X1(d, k)] *
-                  (P(ph, i, k + 1, j) - P(ph, i, k, j) + P(phb, i, k + 1, j) - P(phb, i, k, j));
+        else
           pt = pt - (s.fnm[X1(d, k)] * wd_hi + s.fnp[X1(d, k)] * wd_lo);
-        }
-        wd_lo = wd_hi;
       }
       if (a.c.non_hydrostatic) {
         if (k == kde) pt = 0.f;
@@ -862,11 +859,12 @@ __global__ void __launch_bounds__(256) k_tend_t(KArgs a) {
     }
     s.t_tend[o] = t;
     s.ph_tend[o] = pt;
-    vf_lo = vf_hi;
   }
-  s.mu_tend[X2(d, i, j)] = 0.f;
-  s.mu_save[X2(d, i, j)] = 0.f;
-  if (mass) s.mu_tend[X2(d, i, j)] = 0.f + s.mu_tendf[X2(d, i, j)];
+  if (k == kts) {
+    s.mu_tend[X2(d, i, j)] = 0.f;
+    s.mu_save[X2(d, i, j)] = 0.f;
+    if (mass) s.mu_tend[X2(d, i, j)] = 0.f + s.mu_tendf[X2(d, i, j)];
+  }
 }
 
 }  // namespace wrfmi

@@ -3,12 +3,15 @@
 #include "dev_common.h"
 
 namespace wrfmi {
-template <bool COMP> __global__ void k_step_prep(KArgs);
+__global__ void k_step_prep(KArgs);
+template <bool COMP> __global__ void k_ww_scan(KArgs);
 __global__ void k_tend_u(KArgs);
 __global__ void k_tend_v(KArgs);
 __global__ void k_tend_w(KArgs);
 __global__ void k_tend_t(KArgs);
-template <typename CT> __global__ void k_small_prep(KArgs);
+__global__ void k_small_prep(KArgs);
+__global__ void k_p_rho0(KArgs);
+template <typename CT> __global__ void k_coef_w(KArgs);
 __global__ void k_advance_uv(KArgs);
 template <typename CT, bool COMP> __global__ void k_acoustic_col(KArgs);
 __global__ void k_small_finish(KArgs);

@@ -205,11 +205,13 @@ static const dim3 kColBlk(64, 4, 1);
 // ------------------------------------------------------------------ kernel timing (diagnostic)
 enum KernelId {
   KID_STEP_PREP, KID_TEND_U, KID_TEND_V, KID_TEND_W, KID_TEND_T, KID_SMALL_PREP, KID_ADVANCE_UV,
-  KID_ACOUSTIC_COL, KID_SMALL_FINISH, KID_ADV_SCALAR, KID_UPDATE_SCALAR, KID_P_RHO_PHI, KID_HALO, KID_COUNT
+  KID_ACOUSTIC_COL, KID_SMALL_FINISH, KID_ADV_SCALAR, KID_UPDATE_SCALAR, KID_P_RHO_PHI, KID_HALO, KID_P_RHO0,
+  KID_COEF_W, KID_WW_SCAN, KID_COUNT
 };
 static const char *kKernelNames[KID_COUNT] = {
   "k_step_prep", "k_tend_u", "k_tend_v", "k_tend_w", "k_tend_t", "k_small_prep", "k_advance_uv",
-  "k_acoustic_col", "k_small_finish", "k_adv_scalar", "k_update_scalar", "k_p_rho_phi", "k_halo"};
+  "k_acoustic_col", "k_small_finish", "k_adv_scalar", "k_update_scalar", "k_p_rho_phi", "k_halo", "k_p_rho0",
+  "k_coef_w", "k_ww_scan"};
 struct EvPair { int id; hipEvent_t a, b; };
 static unsigned g_timing = 0;  // bit n set: time launches of kernel n
 static std::vector<EvPair> g_ev_pending;
@@ -277,13 +279,20 @@ static int num_sound_steps(const wrfmi_config &c, const wrfmi_state &s) {
 static inline dim3 g_prep(const Dm &d) { return col_grid(d, d.ids - 1, d.jds - 1, kColBlk); }
 static inline dim3 g_col(const Dm &d) { return col_grid(d, d.ids, d.jds, kColBlk); }
 static inline dim3 g_pt(const Dm &d) { return dim3(((d.ide - d.ids + 1) + 63) / 64, d.kde - d.kds, d.jde - d.jds + 1); }
+// (i, j) tiles of kColBlk over i = ids..ide, j = jds..jde, and nk levels in z
+static inline dim3 g_pt3(const Dm &d, int nk) {
+  const dim3 g = col_grid(d, d.ids, d.jds, kColBlk);
+  return dim3(g.x, g.y, nk);
+}
 static inline dim3 g_mass(const Dm &d) { return dim3(((d.ide - d.ids) + 63) / 64, d.kde - d.kds, d.jde - d.jds); }
 
 static void ph_step_prep(Launcher &L) {
   const KArgs &a = L.a;
   const wrfmi_state &s = a.s;
-  if (a.c.precision_mode == 1) L.launch(KID_STEP_PREP, k_step_prep<true>, g_prep(a.d), kColBlk, a);
-  else L.launch(KID_STEP_PREP, k_step_prep<false>, g_prep(a.d), kColBlk, a);
+  const dim3 gp = g_prep(a.d);
+  L.launch(KID_STEP_PREP, k_step_prep, dim3(gp.x, gp.y, a.d.kde - a.d.kds + 1), kColBlk, a);
+  if (a.c.precision_mode == 1) L.launch(KID_WW_SCAN, k_ww_scan<true>, g_col(a.d), kColBlk, a);
+  else L.launch(KID_WW_SCAN, k_ww_scan<false>, g_col(a.d), kColBlk, a);
   L.halo_begin();
   L.halo_add(s.ru, 'u', 1); L.halo_add(s.rv, 'v', 1); L.halo_add(s.rw, 'w', 1); L.halo_add(s.ww, 'w', 1);
   L.halo_add(s.php, 'w', 1); L.halo_add(s.alt, 't', 1); L.halo_add(s.p, 'p', 1);
@@ -293,16 +302,20 @@ static void ph_step_prep(Launcher &L) {
 }
 static void ph_tendency(Launcher &L) {
   const KArgs &a = L.a;
-  L.launch(KID_TEND_U, k_tend_u, g_col(a.d), kColBlk, a);
-  L.launch(KID_TEND_V, k_tend_v, g_col(a.d), kColBlk, a);
-  L.launch(KID_TEND_W, k_tend_w, g_col(a.d), kColBlk, a);
-  L.launch(KID_TEND_T, k_tend_t, g_col(a.d), kColBlk, a);
+  const int nk = a.d.kde - a.d.kds + 1;
+  L.launch(KID_TEND_U, k_tend_u, g_pt3(a.d, nk), kColBlk, a);
+  L.launch(KID_TEND_V, k_tend_v, g_pt3(a.d, nk), kColBlk, a);
+  L.launch(KID_TEND_W, k_tend_w, g_pt3(a.d, nk), kColBlk, a);
+  L.launch(KID_TEND_T, k_tend_t, g_pt3(a.d, nk), kColBlk, a);
 }
 static void ph_small_prep(Launcher &L) {
   const KArgs &a = L.a;
   const wrfmi_state &s = a.s;
-  if (a.c.precision_mode == 1) L.launch(KID_SMALL_PREP, k_small_prep<double>, g_col(a.d), kColBlk, a);
-  else L.launch(KID_SMALL_PREP, k_small_prep<float>, g_col(a.d), kColBlk, a);
+  const int nk = a.d.kde - a.d.kds + 1;
+  L.launch(KID_SMALL_PREP, k_small_prep, g_pt3(a.d, nk), kColBlk, a);
+  L.launch(KID_P_RHO0, k_p_rho0, g_pt3(a.d, nk - 1), kColBlk, a);
+  if (a.c.precision_mode == 1) L.launch(KID_COEF_W, k_coef_w<double>, g_col(a.d), kColBlk, a);
+  else L.launch(KID_COEF_W, k_coef_w<float>, g_col(a.d), kColBlk, a);
   L.halo_begin();
   L.halo_add(s.ru_tend, 'u', 1); L.halo_add(s.rv_tend, 'v', 1); L.halo_add(s.ph_2, 'w', 1);
   L.halo_add(s.al, 'p', 1); L.halo_add(s.p, 'p', 1); L.halo_add(s.t_1, 'p', 1); L.halo_add(s.t_save, 't', 1);
@@ -325,7 +338,7 @@ static void ph_advance_mu_t_w(Launcher &L) {
 static void ph_small_finish(Launcher &L) {
   const KArgs &a = L.a;
   const wrfmi_state &s = a.s;
-  L.launch(KID_SMALL_FINISH, k_small_finish, g_col(a.d), kColBlk, a);
+  L.launch(KID_SMALL_FINISH, k_small_finish, g_pt3(a.d, a.d.kde - a.d.kds + 1), kColBlk, a);
   if (a.rk_step == 3) {
     L.halo_begin();
     L.halo_add(s.ru_m, 'u', 1); L.halo_add(s.rv_m, 'v', 1); L.halo_add(s.ww_m, 'w', 1);
@@ -334,7 +347,7 @@ static void ph_small_finish(Launcher &L) {
   }
 }
 static void ph_scalar(Launcher &L) {
-  L.launch(KID_ADV_SCALAR, k_adv_scalar, g_col(L.a.d), kColBlk, L.a);
+  L.launch(KID_ADV_SCALAR, k_adv_scalar, g_pt3(L.a.d, L.a.d.kde - L.a.d.kds + 1), kColBlk, L.a);
   L.launch(KID_UPDATE_SCALAR, k_update_scalar, g_mass(L.a.d), dim3(64, 1, 1), L.a);
 }
 static void ph_p_rho_phi(Launcher &L) {
