@@ -29,6 +29,10 @@ struct KArgs {
   int iteration;   // acoustic substep number (1-based)
   int nsmall;      // acoustic substeps in this RK substep
   int species;     // species index for the scalar kernels
+  // library-owned 3-D work arrays, same layout as a grid field
+  float *scr_dv;   // advance_mu_t's column-mass divergence dvdxi
+  float *scr_rhs;  // advance_w's right-hand side rhs
+  float *scr_lo;   // low-order part of the FP64 forward-elimination value (precision mode 1)
 };
 
 // One field of a batched halo fill (set_physical_bc3d/2d on up to 16 fields per launch)

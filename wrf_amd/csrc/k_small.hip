@@ -3,8 +3,8 @@
 //   k_p_rho0       : calc_p_rho at step 0 (438-568), one thread per point
 //   k_coef_w       : small_step_prep's 2-D updates + calc_coef_w (570-652), one thread per column
 //   k_advance_uv   : advance_uv (654-967), one thread per (i,k,j) point
-//   k_acoustic_col : advance_mu_t (969-1175) + advance_w (1178-1469) + sumflux (1473-1633)
-//                    + calc_p_rho (438-568), one thread per (i,j) column
+//   k_ac_*         : advance_mu_t (969-1175) + advance_w (1178-1469) + sumflux (1473-1633)
+//                    + calc_p_rho (438-568), in six kernels (see below)
 //   k_small_finish : calc_mu_uv_1 (module_big_step_utilities_em.F:184-321) + small_step_finish (295-434)
 // Arguments are bound as at the solve_em.F call sites (1089-1757): the coupled
 // perturbations live in u_2..ph_2, the time-t values in *_save, ww1 is the large-step
@@ -270,250 +270,295 @@ __global__ void __launch_bounds__(256) k_advance_uv(KArgs a) {
 
 // One thread per (i,j) column.  Mass columns run advance_mu_t, advance_w and
 // calc_p_rho; every column of the tile accumulates its share of sumflux.
-template <typename CT, bool COMP>
-__global__ void __launch_bounds__(256) k_acoustic_col(KArgs a) {
+// ---------------------------------------------------------------------------------------
+// The column part of one acoustic substep: advance_mu_t + advance_w + sumflux + calc_p_rho,
+// split at the two column dependencies (the mass-divergence integral before the omega scan,
+// and the tridiagonal forward/backward sweeps) into point kernels and two thin column
+// kernels.  Per substep, in order:
+//   k_ac_div      (point)  dvdxi                                    -> scr_dv
+//   k_ac_mu_ww    (column) dmdt, mu_2/mudf/muts/muave, omega scan     -> ww
+//   k_ac_theta    (point)  theta update, t_2ave, advance_w's rhs      -> t_2, t_2save, scr_rhs
+//   k_ac_wpred    (point)  surface w and the explicit w predictor     -> w
+//   k_ac_wsolve   (column) tridiagonal forward elimination + back substitution -> w
+//   k_ac_finish   (point)  ph from rhs and w, calc_p_rho, sumflux
+// Each point kernel only reads what earlier kernels wrote, so a thread never depends on
+// another thread of its own launch.
+
+// dvdxi of advance_mu_t (module_small_step_em.F:1092-1099) at mass point (i, k, j)
+__global__ void __launch_bounds__(256) k_ac_div(KArgs a) {
   const Dm &d = a.d;
   const wrfmi_state &s = a.s;
   const int i = d.ids + (int)(blockIdx.x * blockDim.x + threadIdx.x);
   const int j = d.jds + (int)(blockIdx.y * blockDim.y + threadIdx.y);
-  const int ide = d.ide, jde = d.jde, kde = d.kde;
-  if (i > ide || j > jde) return;
+  const int k = d.kds + (int)blockIdx.z;
+  if (i > d.ide - 1 || j > d.jde - 1 || k > d.kde - 1) return;
+  const int o2 = X2(d, i, j), o = X3(d, i, k, j);
+  const float *u = s.u_2, *u_1 = s.u_save, *v = s.v_2, *v_1 = s.v_save;
+  const float msftx = s.msftx[o2], msfty = s.msfty[o2], rdx = s.rdx, rdy = s.rdy;
+  const float muu0 = s.muu[o2], muu1 = s.muu[X2(d, i + 1, j)], muv0 = s.muv[o2], muv1 = s.muv[X2(d, i, j + 1)];
+  const float msfuy0 = s.msfuy[o2], msfuy1 = s.msfuy[X2(d, i + 1, j)];
+  const float mvi0 = s.msfvx_inv[o2], mvi1 = s.msfvx_inv[X2(d, i, j + 1)];
+  const float cc = s.c1h[X1(d, k)], c2 = s.c2h[X1(d, k)];
+  a.scr_dv[o] = msftx * msfty *
+                (rdy * ((v[o + d.sj] + (cc * muv1 + c2) * v_1[o + d.sj] * mvi1) - (v[o] + (cc * muv0 + c2) * v_1[o] * mvi0)) +
+                 rdx * ((u[o + 1] + (cc * muu1 + c2) * u_1[o + 1] / msfuy1) - (u[o] + (cc * muu0 + c2) * u_1[o] / msfuy0)));
+}
+
+// advance_mu_t's column integral, mass update and omega scan (module_small_step_em.F:1100-1121)
+template <bool COMP>
+__global__ void __launch_bounds__(256) k_ac_mu_ww(KArgs a) {
+  const Dm &d = a.d;
+  const wrfmi_state &s = a.s;
+  const int i = d.ids + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int j = d.jds + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  if (i > d.ide - 1 || j > d.jde - 1) return;
+  const int k_end = d.kde - 1;
+  const int o2 = X2(d, i, j);
+  const float *dvp = a.scr_dv, *dnw = s.dnw, *c1h = s.c1h, *ww1 = s.ww1;
+  float *ww = s.ww;
+  const float dts = a.dts, epssm = a.c.epssm, msfty = s.msfty[o2];
+  float dmdt = 0.f, comp = 0.f;
+  for (int k = 1; k <= k_end; k++) {
+    const float x = dnw[X1(d, k)] * dvp[X3(d, i, k, j)];
+    if (COMP) two_sum_acc(dmdt, comp, x);
+    else dmdt = dmdt + x;
+  }
+  if (COMP) dmdt = dmdt + comp;
+  const float mu_tend = s.mu_tend[o2];
+  float mu = s.mu_2[o2];
+  float muave = mu;
+  mu = mu + dts * (dmdt + mu_tend);
+  s.mu_2[o2] = mu;
+  s.mudf[o2] = (dmdt + mu_tend);
+  s.muts[o2] = s.mut[o2] + mu;
+  muave = .5f * ((1.f + epssm) * mu + (1.f - epssm) * muave);
+  s.muave[o2] = muave;
+  const int o1 = X3(d, i, 1, j);
+  float prev = ww[o1];
+  ww[o1] = prev - ww1[o1];
+  for (int kk = 2; kk <= k_end; kk++) {
+    const int o = X3(d, i, kk, j);
+    const int k = kk - 1;
+    const float cur = prev - dnw[X1(d, kk - 1)] * ((c1h[X1(d, k)] * dmdt) + dvp[X3(d, i, k, j)] + (c1h[X1(d, k)] * mu_tend)) / msfty;
+    prev = cur;
+    ww[o] = cur - ww1[o];
+  }
+}
+
+// theta update of advance_mu_t (1125-1174), t_2ave and rhs of advance_w (1312-1371)
+__global__ void __launch_bounds__(256) k_ac_theta(KArgs a) {
+  const Dm &d = a.d;
+  const wrfmi_state &s = a.s;
+  const int i = d.ids + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int j = d.jds + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  const int k = d.kds + (int)blockIdx.z;
+  const int kde = d.kde, k_end = kde - 1;
+  if (i > d.ide - 1 || j > d.jde - 1 || k > kde) return;
+  const wrfmi_config &c = a.c;
+  const int o2 = X2(d, i, j), o = X3(d, i, k, j);
+  const float dts = a.dts, epssm = c.epssm;
+  const float msftx = s.msftx[o2], msfty = s.msfty[o2], rdx = s.rdx, rdy = s.rdy;
+  const float *ww = s.ww, *t_1 = s.t_save;
+  if (k <= k_end) {
+    const float *u = s.u_2, *v = s.v_2;
+    const float tv = s.t_2[o];
+    float t = tv + msfty * dts * s.t_tend[o];
+    const float wdtn_hi = (k + 1 <= k_end) ? ww[o + d.ni] * (s.fnm[X1(d, k + 1)] * t_1[o + d.ni] + s.fnp[X1(d, k + 1)] * t_1[o]) : 0.f;
+    const float wdtn_lo = (k >= 2) ? ww[o] * (s.fnm[X1(d, k)] * t_1[o] + s.fnp[X1(d, k)] * t_1[o - d.ni]) : 0.f;
+    t = t - dts * msfty *
+                (msftx * (.5f * rdy * (v[o + d.sj] * (t_1[o + d.sj] + t_1[o]) - v[o] * (t_1[o] + t_1[o - d.sj])) +
+                          .5f * rdx * (u[o + 1] * (t_1[o + 1] + t_1[o]) - u[o] * (t_1[o] + t_1[o - 1]))) +
+                 s.rdnw[X1(d, k)] * (wdtn_hi - wdtn_lo));
+    s.t_2[o] = t;
+    if (c.non_hydrostatic) {
+      // t_2ave (advance_w 1312-1320): the old theta is t_2save's value from advance_mu_t
+      const float muave = s.muave[o2], muts = s.muts[o2];
+      float v1 = .5f * ((1.f + epssm) * t + (1.f - epssm) * tv);
+      v1 = (v1 + (s.c1h[X1(d, k)] * muave) * K_T0) / ((s.c1h[X1(d, k)] * muts + s.c2h[X1(d, k)]) * (K_T0 + t_1[o]));
+      s.t_2save[o] = v1;
+    } else {
+      s.t_2save[o] = tv;
+    }
+  }
+  if (!c.non_hydrostatic || k < 2) return;
+  // rhs(k), k = 2..kde (advance_w 1324-1371)
+  const float *ph_1 = s.ph_save, *phb = s.phb;
+  auto wdwn_at = [&](int kp1) -> float {  // wdwn(k+1) for k = kp1-1
+    const int kk = kp1 - 1;
+    const int oo = X3(d, i, kk, j);
+    if (c.phi_adv_z == 2)
+      return s.rdnw[X1(d, kk)] * (ph_1[oo + d.ni] - ph_1[oo] + phb[oo + d.ni] - phb[oo]);
+    return .5f * (ww[oo + d.ni] + ww[oo]) * s.rdnw[X1(d, kk)] * (ph_1[oo + d.ni] - ph_1[oo] + phb[oo + d.ni] - phb[oo]);
+  };
+  const float mut = s.mut[o2];
+  float r = dts * (s.ph_tend[o] + .5f * K_G * (1.f - epssm) * s.w_2[o]);
+  if (k <= k_end) {
+    if (c.phi_adv_z == 2)
+      r = r - dts * ww[o] * (s.fnm[X1(d, k)] * wdwn_at(k + 1) + s.fnp[X1(d, k)] * wdwn_at(k));
+    else
+      r = r - dts * (s.fnm[X1(d, k)] * wdwn_at(k + 1) + s.fnp[X1(d, k)] * wdwn_at(k));
+  }
+  r = s.ph_2[o] + msfty * r / (s.c1f[X1(d, k)] * mut + s.c2f[X1(d, k)]);
+  if (c.top_lid && k == k_end + 1) r = 0.f;
+  a.scr_rhs[o] = r;
+}
+
+// surface w (advance_w 1383-1395) and the explicit w predictor (1402-1431), k = 1..kde
+__global__ void __launch_bounds__(256) k_ac_wpred(KArgs a) {
+  const Dm &d = a.d;
+  const wrfmi_state &s = a.s;
+  const int i = d.ids + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int j = d.jds + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  const int k = d.kds + (int)blockIdx.z;
+  const int kde = d.kde, k_end = kde - 1;
+  if (i > d.ide - 1 || j > d.jde - 1 || k > kde) return;
+  const wrfmi_config &c = a.c;
+  if (!c.non_hydrostatic) return;
+  const int o2 = X2(d, i, j), o = X3(d, i, k, j);
+  const float dts = a.dts, epssm = c.epssm;
+  const float msftx = s.msftx[o2], msfty = s.msfty[o2], rdx = s.rdx, rdy = s.rdy;
+  float *w = s.w_2;
+  if (k == 1) {
+    const float *u = s.u_2, *v = s.v_2, *ht = s.ht;
+    const float cf1 = s.cf1, cf2 = s.cf2, cf3 = s.cf3;
+    w[o] = msfty * .5f * rdy *
+               ((ht[X2(d, i, j + 1)] - ht[o2]) * (cf1 * v[X3(d, i, 1, j + 1)] + cf2 * v[X3(d, i, 2, j + 1)] + cf3 * v[X3(d, i, 3, j + 1)]) +
+                (ht[o2] - ht[X2(d, i, j - 1)]) * (cf1 * v[X3(d, i, 1, j)] + cf2 * v[X3(d, i, 2, j)] + cf3 * v[X3(d, i, 3, j)])) +
+           msftx * .5f * rdx *
+               ((ht[X2(d, i + 1, j)] - ht[o2]) * (cf1 * u[X3(d, i + 1, 1, j)] + cf2 * u[X3(d, i + 1, 2, j)] + cf3 * u[X3(d, i + 1, 3, j)]) +
+                (ht[o2] - ht[X2(d, i - 1, j)]) * (cf1 * u[X3(d, i, 1, j)] + cf2 * u[X3(d, i, 2, j)] + cf3 * u[X3(d, i, 3, j)]));
+    return;
+  }
+  const float *ph = s.ph_2, *rhs = a.scr_rhs, *c2a = s.c2a, *alt = s.alt, *t2ave = s.t_2save;
+  const float *c1h = s.c1h, *c2h = s.c2h, *c1f = s.c1f;
+  const float mut = s.mut[o2], muave = s.muave[o2];
+  const float msft_inv = 1.f / msfty;
+  const float rhs_m1 = k >= 3 ? rhs[o - d.ni] : 0.f;  // rhs(1) = 0
+  const float rhs_0 = rhs[o];
+  const float phl_m1 = ph[o - d.ni], phl_0 = ph[o];
+  const float t2a_lo = t2ave[o - d.ni];
+  float wv;
+  if (k <= k_end) {
+    const float rhs_p1 = rhs[o + d.ni], phl_p1 = ph[o + d.ni], t2a_hi = t2ave[o];
+    wv = w[o] + dts * s.rw_tend[o] +
+         msft_inv * s.cqw[o] *
+             (+.5f * dts * K_G * s.rdn[X1(d, k)] *
+              (c2a[o] * s.rdnw[X1(d, k)] / (c1h[X1(d, k)] * mut + c2h[X1(d, k)]) *
+                   ((1.f + epssm) * (rhs_p1 - rhs_0) + (1.f - epssm) * (phl_p1 - phl_0)) -
+               c2a[o - d.ni] * s.rdnw[X1(d, k - 1)] / (c1h[X1(d, k - 1)] * mut + c2h[X1(d, k - 1)]) *
+                   ((1.f + epssm) * (rhs_0 - rhs_m1) + (1.f - epssm) * (phl_0 - phl_m1)))) +
+         dts * K_G * msft_inv *
+             (s.rdn[X1(d, k)] * (c2a[o] * alt[o] * t2a_hi - c2a[o - d.ni] * alt[o - d.ni] * t2a_lo) - (c1f[X1(d, k)] * muave));
+  } else {
+    wv = w[o] + dts * s.rw_tend[o] +
+         msft_inv * (-.5f * dts * K_G / (c1h[X1(d, k - 1)] * mut + c2h[X1(d, k - 1)]) * (s.rdnw[X1(d, k - 1)] * s.rdnw[X1(d, k - 1)]) * 2.f *
+                         c2a[o - d.ni] * ((1.f + epssm) * (rhs_0 - rhs_m1) + (1.f - epssm) * (phl_0 - phl_m1)) -
+                     dts * K_G * (2.f * s.rdnw[X1(d, k - 1)] * c2a[o - d.ni] * alt[o - d.ni] * t2a_lo + (c1f[X1(d, k)] * muave)));
+    if (c.top_lid) wv = 0.f;
+  }
+  w[o] = wv;
+}
+
+// tridiagonal forward elimination and back substitution (advance_w 1433-1443) per column;
+// CT = double keeps the forward value in FP64, its low-order part parked in scr_lo
+template <typename CT>
+__global__ void __launch_bounds__(256) k_ac_wsolve(KArgs a) {
+  const Dm &d = a.d;
+  const wrfmi_state &s = a.s;
+  const int i = d.ids + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int j = d.jds + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  const int kde = d.kde, k_end = kde - 1;
+  if (i > d.ide - 1 || j > d.jde - 1) return;
+  if (!a.c.non_hydrostatic) return;
+  float *w = s.w_2;
+  float *lo = a.scr_lo;
+  const CT *A = (const CT *)s.a, *AL = (const CT *)s.alpha, *GA = (const CT *)s.gamma;
+  CT wfe = (CT)w[X3(d, i, 1, j)];
+  for (int k = 2; k <= k_end + 1; k++) {
+    const int o = X3(d, i, k, j);
+    const float wv = w[o];
+    if (sizeof(CT) == sizeof(float)) {
+      const float wf = (wv - (float)A[o] * (float)wfe) * (float)AL[o];
+      wfe = (CT)wf;
+      w[o] = wf;
+    } else {
+      wfe = ((CT)wv - A[o] * wfe) * AL[o];
+      w[o] = (float)wfe;
+      lo[o] = (float)((double)wfe - (double)(float)wfe);
+    }
+  }
+  CT wup = wfe;
+  for (int k = k_end; k >= 2; k--) {
+    const int o = X3(d, i, k, j);
+    float wk;
+    if (sizeof(CT) == sizeof(float)) {
+      wk = w[o] - (float)GA[o] * (float)wup;
+      wup = (CT)wk;
+    } else {
+      const CT wd = (CT)w[o] + (CT)lo[o];
+      const CT wn = wd - GA[o] * wup;
+      wup = wn;
+      wk = (float)wn;
+    }
+    w[o] = wk;
+  }
+}
+
+// ph from rhs and the final w (advance_w 1460-1465), calc_p_rho for step >= 1 (438-568)
+// and sumflux (1473-1633); one thread per (i, k, j) with i <= ide, j <= jde, k <= kde
+__global__ void __launch_bounds__(256) k_ac_finish(KArgs a) {
+  const Dm &d = a.d;
+  const wrfmi_state &s = a.s;
+  const int i = d.ids + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int j = d.jds + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  const int k = d.kds + (int)blockIdx.z;
+  const int ide = d.ide, jde = d.jde, kde = d.kde, k_end = kde - 1;
+  if (i > ide || j > jde || k > kde) return;
   const bool mi = i <= ide - 1, mj = j <= jde - 1, mass = mi && mj;
   const wrfmi_config &c = a.c;
-  const int k_end = kde - 1;
-  const float dts = a.dts, epssm = c.epssm;
-  const float *c1h = s.c1h, *c2h = s.c2h, *c1f = s.c1f, *c2f = s.c2f;
-  const int o2 = X2(d, i, j);
-  if (mass) {
-    // ---------------- advance_mu_t
-    const float *u = s.u_2, *u_1 = s.u_save, *v = s.v_2, *v_1 = s.v_save;
-    float *ww = s.ww;
-    const float msftx = s.msftx[o2], msfty = s.msfty[o2], rdx = s.rdx, rdy = s.rdy;
-    const float muu0 = s.muu[o2], muu1 = s.muu[X2(d, i + 1, j)], muv0 = s.muv[o2], muv1 = s.muv[X2(d, i, j + 1)];
-    const float msfuy0 = s.msfuy[o2], msfuy1 = s.msfuy[X2(d, i + 1, j)];
-    const float mvi0 = s.msfvx_inv[o2], mvi1 = s.msfvx_inv[X2(d, i, j + 1)];
-    float dmdt = 0.f, comp = 0.f;
-    // pass 1: dvdxi(k) and its column integral; dvdxi(k) is parked in ww(k+1)
-    for (int k = 1; k <= k_end; k++) {
-      const int o = X3(d, i, k, j);
+  const int o2 = X2(d, i, j), o = X3(d, i, k, j);
+  const float *c1h = s.c1h, *c2h = s.c2h, *c1f = s.c1f;
+  if (mass && c.non_hydrostatic) {
+    const float dts = a.dts, epssm = c.epssm, msfty = s.msfty[o2], muts = s.muts[o2];
+    const float *rhs = a.scr_rhs, *w = s.w_2;
+    auto ph_at = [&](int kk, int oo) -> float {  // final ph at level kk >= 2
+      return rhs[oo] + msfty * .5f * dts * K_G * (1.f + epssm) * w[oo] / (c1f[X1(d, kk)] * muts + s.c2f[X1(d, kk)]);
+    };
+    const float ph0 = k >= 2 ? ph_at(k, o) : s.ph_2[o];
+    if (k >= 2) s.ph_2[o] = ph0;
+    if (k <= k_end) {
+      const float ph1 = ph_at(k + 1, o + d.ni);
+      const float mu = s.mu_2[o2];
       const float cc = c1h[X1(d, k)], c2 = c2h[X1(d, k)];
-      const float dv = msftx * msfty *
-                       (rdy * ((v[o + d.sj] + (cc * muv1 + c2) * v_1[o + d.sj] * mvi1) - (v[o] + (cc * muv0 + c2) * v_1[o] * mvi0)) +
-                        rdx * ((u[o + 1] + (cc * muu1 + c2) * u_1[o + 1] / msfuy1) - (u[o] + (cc * muu0 + c2) * u_1[o] / msfuy0)));
-      if (COMP) two_sum_acc(dmdt, comp, s.dnw[X1(d, k)] * dv);
-      else dmdt = dmdt + s.dnw[X1(d, k)] * dv;
-      if (k <= k_end - 1) ww[o + d.ni] = dv;
-    }
-    if (COMP) dmdt = dmdt + comp;
-    const float mu_tend = s.mu_tend[o2];
-    float mu = s.mu_2[o2];
-    float muave = mu;
-    mu = mu + dts * (dmdt + mu_tend);
-    s.mu_2[o2] = mu;
-    s.mudf[o2] = (dmdt + mu_tend);
-    const float muts = s.mut[o2] + mu;
-    s.muts[o2] = muts;
-    muave = .5f * ((1.f + epssm) * mu + (1.f - epssm) * muave);
-    s.muave[o2] = muave;
-    {
-      float prev = ww[X3(d, i, 1, j)];
-      for (int kk = 2; kk <= k_end; kk++) {
-        const int o = X3(d, i, kk, j);
-        const int k = kk - 1;
-        const float dv = ww[o];
-        const float cur = prev - s.dnw[X1(d, kk - 1)] * ((c1h[X1(d, k)] * dmdt) + dv + (c1h[X1(d, k)] * mu_tend)) / msfty;
-        ww[o] = cur;
-        prev = cur;
-      }
-      for (int k = 1; k <= k_end; k++) {
-        const int o = X3(d, i, k, j);
-        ww[o] = ww[o] - s.ww1[o];
-      }
-    }
-    // theta
-    const float *t_1 = s.t_save;
-    float *t = s.t_2;
-    for (int k = 1; k <= k_end; k++) {
-      const int o = X3(d, i, k, j);
-      const float tv = t[o];
-      s.t_2save[o] = tv;
-      t[o] = tv + msfty * dts * s.t_tend[o];
-    }
-    {
-      float wdtn_lo = 0.f;
-      for (int k = 1; k <= k_end; k++) {
-        const int o = X3(d, i, k, j);
-        float wdtn_hi = 0.f;
-        if (k + 1 <= k_end) wdtn_hi = ww[o + d.ni] * (s.fnm[X1(d, k + 1)] * t_1[o + d.ni] + s.fnp[X1(d, k + 1)] * t_1[o]);
-        t[o] = t[o] - dts * msfty *
-                          (msftx * (.5f * rdy * (v[o + d.sj] * (t_1[o + d.sj] + t_1[o]) - v[o] * (t_1[o] + t_1[o - d.sj])) +
-                                    .5f * rdx * (u[o + 1] * (t_1[o + 1] + t_1[o]) - u[o] * (t_1[o] + t_1[o - 1]))) +
-                           s.rdnw[X1(d, k)] * (wdtn_hi - wdtn_lo));
-        wdtn_lo = wdtn_hi;
-      }
-    }
-    // ---------------- advance_w
-    if (c.non_hydrostatic) {
-      float *w = s.w_2, *ph = s.ph_2;
-      const float *ph_1 = s.ph_save, *phb = s.phb, *ph_tend = s.ph_tend, *rw_tend = s.rw_tend;
-      const float *c2a = s.c2a, *cqw = s.cqw, *alt = s.alt, *t_2 = s.t_2, *ht = s.ht;
-      const CT *A = (const CT *)s.a, *AL = (const CT *)s.alpha, *GA = (const CT *)s.gamma;
-      const float mut = s.mut[o2];
-      const float msft_inv = 1.f / msfty;
-      const float cf1 = s.cf1, cf2 = s.cf2, cf3 = s.cf3;
-      float *t2ave = s.t_2save;
-      auto t2ave_at = [&](int k) -> float {
-        const int o = X3(d, i, k, j);
-        float v1 = .5f * ((1.f + epssm) * t_2[o] + (1.f - epssm) * t2ave[o]);
-        v1 = (v1 + (c1h[X1(d, k)] * muave) * K_T0) / ((c1h[X1(d, k)] * muts + c2h[X1(d, k)]) * (K_T0 + t_1[o]));
-        t2ave[o] = v1;
-        return v1;
-      };
-      auto wdwn_at = [&](int kp1) -> float {  // wdwn(k+1) for k = kp1-1
-        const int k = kp1 - 1;
-        const int o = X3(d, i, k, j);
-        if (c.phi_adv_z == 2)
-          return s.rdnw[X1(d, k)] * (ph_1[o + d.ni] - ph_1[o] + phb[o + d.ni] - phb[o]);
-        return .5f * (ww[o + d.ni] + ww[o]) * s.rdnw[X1(d, k)] * (ph_1[o + d.ni] - ph_1[o] + phb[o + d.ni] - phb[o]);
-      };
-      // final rhs(k) (k = 2..kde); reads w and ph at level k before they are overwritten
-      auto rhs_at = [&](int k) -> float {
-        const int o = X3(d, i, k, j);
-        float r = dts * (ph_tend[o] + .5f * K_G * (1.f - epssm) * w[o]);
-        if (k <= k_end) {
-          if (c.phi_adv_z == 2)
-            r = r - dts * ww[o] * (s.fnm[X1(d, k)] * wdwn_at(k + 1) + s.fnp[X1(d, k)] * wdwn_at(k));
-          else
-            r = r - dts * (s.fnm[X1(d, k)] * wdwn_at(k + 1) + s.fnp[X1(d, k)] * wdwn_at(k));
-        }
-        r = ph[o] + msfty * r / (c1f[X1(d, k)] * mut + c2f[X1(d, k)]);
-        if (c.top_lid && k == k_end + 1) r = 0.f;
-        return r;
-      };
-      // surface w from the terrain slope
-      {
-        const int o = X3(d, i, 1, j);
-        w[o] = msfty * .5f * rdy *
-                   ((ht[X2(d, i, j + 1)] - ht[o2]) * (cf1 * v[X3(d, i, 1, j + 1)] + cf2 * v[X3(d, i, 2, j + 1)] + cf3 * v[X3(d, i, 3, j + 1)]) +
-                    (ht[o2] - ht[X2(d, i, j - 1)]) * (cf1 * v[X3(d, i, 1, j)] + cf2 * v[X3(d, i, 2, j)] + cf3 * v[X3(d, i, 3, j)])) +
-               msftx * .5f * rdx *
-                   ((ht[X2(d, i + 1, j)] - ht[o2]) * (cf1 * u[X3(d, i + 1, 1, j)] + cf2 * u[X3(d, i + 1, 2, j)] + cf3 * u[X3(d, i + 1, 3, j)]) +
-                    (ht[o2] - ht[X2(d, i - 1, j)]) * (cf1 * u[X3(d, i, 1, j)] + cf2 * u[X3(d, i, 2, j)] + cf3 * u[X3(d, i, 3, j)]));
-      }
-      // forward pass: predictor + forward elimination; rhs(k) is stored in ph(k) once ph(k) is no longer needed
-      float t2a_lo = t2ave_at(1);
-      float phl_m1 = ph[X3(d, i, 1, j)], phl_0 = ph[X3(d, i, 2, j)];  // old ph at k-1, k
-      float rhs_m1 = 0.f;                                                // rhs(1) = 0
-      float rhs_0 = rhs_at(2);
-      CT wfe = (CT)w[X3(d, i, 1, j)];
-      for (int k = 2; k <= k_end + 1; k++) {
-        const int o = X3(d, i, k, j);
-        float wv;
-        if (k <= k_end) {
-          const float t2a_hi = t2ave_at(k);
-          const float phl_p1 = ph[o + d.ni];
-          const float rhs_p1 = rhs_at(k + 1);
-          wv = w[o] + dts * rw_tend[o] +
-               msft_inv * cqw[o] *
-                   (+.5f * dts * K_G * s.rdn[X1(d, k)] *
-                    (c2a[o] * s.rdnw[X1(d, k)] / (c1h[X1(d, k)] * mut + c2h[X1(d, k)]) *
-                         ((1.f + epssm) * (rhs_p1 - rhs_0) + (1.f - epssm) * (phl_p1 - phl_0)) -
-                     c2a[o - d.ni] * s.rdnw[X1(d, k - 1)] / (c1h[X1(d, k - 1)] * mut + c2h[X1(d, k - 1)]) *
-                         ((1.f + epssm) * (rhs_0 - rhs_m1) + (1.f - epssm) * (phl_0 - phl_m1)))) +
-               dts * K_G * msft_inv *
-                   (s.rdn[X1(d, k)] * (c2a[o] * alt[o] * t2a_hi - c2a[o - d.ni] * alt[o - d.ni] * t2a_lo) - (c1f[X1(d, k)] * muave));
-          ph[o - d.ni] = (k - 1 >= 2) ? rhs_m1 : ph[o - d.ni];
-          t2a_lo = t2a_hi;
-          phl_m1 = phl_0; phl_0 = phl_p1;
-          rhs_m1 = rhs_0; rhs_0 = rhs_p1;
-        } else {
-          wv = w[o] + dts * rw_tend[o] +
-               msft_inv * (-.5f * dts * K_G / (c1h[X1(d, k - 1)] * mut + c2h[X1(d, k - 1)]) * (s.rdnw[X1(d, k - 1)] * s.rdnw[X1(d, k - 1)]) * 2.f *
-                               c2a[o - d.ni] * ((1.f + epssm) * (rhs_0 - rhs_m1) + (1.f - epssm) * (phl_0 - phl_m1)) -
-                           dts * K_G * (2.f * s.rdnw[X1(d, k - 1)] * c2a[o - d.ni] * alt[o - d.ni] * t2a_lo + (c1f[X1(d, k)] * muave)));
-          if (c.top_lid) wv = 0.f;
-          ph[o - d.ni] = rhs_m1;
-          ph[o] = rhs_0;
-        }
-        // w(k) = (w(k) - a(k) w(k-1)) alpha(k)
-        if (sizeof(CT) == sizeof(float)) {
-          const float wf = (wv - (float)A[X3(d, i, k, j)] * (float)wfe) * (float)AL[X3(d, i, k, j)];
-          wfe = (CT)wf;
-          w[o] = wf;
-        } else {
-          wfe = ((CT)wv - A[X3(d, i, k, j)] * wfe) * AL[X3(d, i, k, j)];
-          w[o] = (float)wfe;
-          s.advect_tend[o] = (float)((double)wfe - (double)(float)wfe);  // low-order part kept for the back sweep
-        }
-      }
-      // backward pass: back substitution, then ph = rhs + (dts g (1+epssm)/2) msfty w / (c1f muts + c2f)
-      CT wup = wfe;
-      {
-        const int k = k_end + 1;
-        const int o = X3(d, i, k, j);
-        ph[o] = ph[o] + msfty * .5f * dts * K_G * (1.f + epssm) * w[o] / (c1f[X1(d, k)] * muts + c2f[X1(d, k)]);
-      }
-      for (int k = k_end; k >= 2; k--) {
-        const int o = X3(d, i, k, j);
-        float wk;
-        if (sizeof(CT) == sizeof(float)) {
-          wk = w[o] - (float)GA[X3(d, i, k, j)] * (float)wup;
-          wup = (CT)wk;
-        } else {
-          const CT wd = (CT)w[o] + (CT)s.advect_tend[o];
-          const CT wn = wd - GA[X3(d, i, k, j)] * wup;
-          wup = wn;
-          wk = (float)wn;
-        }
-        w[o] = wk;
-        ph[o] = ph[o] + msfty * .5f * dts * K_G * (1.f + epssm) * wk / (c1f[X1(d, k)] * muts + c2f[X1(d, k)]);
-      }
-      if (c.damp_opt == 3) {
-        // Rayleigh damping of w (advance_w, module_small_step_em.F:1445-1458) is applied to the
-        // final w before ph is formed there; it is not part of the implemented option set
-      }
-    }
-    // ---------------- calc_p_rho (step = iteration >= 1), mut = muts, t_1 = t_save
-    if (c.non_hydrostatic) {
-      const float *ph = s.ph_2;
-      for (int k = 1; k <= k_end; k++) {
-        const int o = X3(d, i, k, j);
-        const float cc = c1h[X1(d, k)], c2 = c2h[X1(d, k)];
-        const float alt = s.alt[o];
-        const float al = -1.f / (cc * muts + c2) * (alt * (cc * mu) + s.rdnw[X1(d, k)] * (ph[o + d.ni] - ph[o]));
-        s.al[o] = al;
-        const float t1 = t_1[o];
-        float pp = s.c2a[o] * (alt * (s.t_2[o] - (cc * mu) * t1) / ((cc * muts + c2) * (K_T0 + t1)) - al);
-        const float ptmp = pp;
-        pp = pp + c.smdiv * (pp - s.pm1[o]);
-        s.p[o] = pp;
-        s.pm1[o] = ptmp;
-      }
+      const float alt = s.alt[o];
+      const float al = -1.f / (cc * muts + c2) * (alt * (cc * mu) + s.rdnw[X1(d, k)] * (ph1 - ph0));
+      s.al[o] = al;
+      const float t1 = s.t_save[o];
+      float pp = s.c2a[o] * (alt * (s.t_2[o] - (cc * mu) * t1) / ((cc * muts + c2) * (K_T0 + t1)) - al);
+      const float ptmp = pp;
+      pp = pp + c.smdiv * (pp - s.pm1[o]);
+      s.p[o] = pp;
+      s.pm1[o] = ptmp;
     }
   }
-  // ---------------- sumflux
-  {
-    const int it = a.iteration, ns = a.nsmall;
-    const float nsf = (float)ns;
-    for (int k = 1; k <= kde; k++) {
-      const int o = X3(d, i, k, j);
-      float rum = it == 1 ? 0.f : s.ru_m[o];
-      float rvm = it == 1 ? 0.f : s.rv_m[o];
-      float wwm = it == 1 ? 0.f : s.ww_m[o];
-      const bool kin = k <= kde - 1;
-      if (mj && kin) rum = rum + s.u_2[o];
-      if (mi && kin) rvm = rvm + s.v_2[o];
-      if (mass) wwm = wwm + s.ww[o];
-      if (it == ns) {
-        if (mj && kin) rum = rum / nsf + (c1h[X1(d, k)] * s.muu[o2] + c2h[X1(d, k)]) * s.u_save[o] / s.msfuy[o2];
-        if (mi && kin) rvm = rvm / nsf + (c1h[X1(d, k)] * s.muv[o2] + c2h[X1(d, k)]) * s.v_save[o] * s.msfvx_inv[o2];
-        if (mass) wwm = wwm / nsf + s.ww1[o];
-      }
-      s.ru_m[o] = rum;
-      s.rv_m[o] = rvm;
-      s.ww_m[o] = wwm;
-    }
+  // sumflux
+  const int it = a.iteration, ns = a.nsmall;
+  const float nsf = (float)ns;
+  float rum = it == 1 ? 0.f : s.ru_m[o];
+  float rvm = it == 1 ? 0.f : s.rv_m[o];
+  float wwm = it == 1 ? 0.f : s.ww_m[o];
+  const bool kin = k <= kde - 1;
+  if (mj && kin) rum = rum + s.u_2[o];
+  if (mi && kin) rvm = rvm + s.v_2[o];
+  if (mass) wwm = wwm + s.ww[o];
+  if (it == ns) {
+    if (mj && kin) rum = rum / nsf + (c1h[X1(d, k)] * s.muu[o2] + c2h[X1(d, k)]) * s.u_save[o] / s.msfuy[o2];
+    if (mi && kin) rvm = rvm / nsf + (c1h[X1(d, k)] * s.muv[o2] + c2h[X1(d, k)]) * s.v_save[o] * s.msfvx_inv[o2];
+    if (mass) wwm = wwm / nsf + s.ww1[o];
   }
+  s.ru_m[o] = rum;
+  s.rv_m[o] = rvm;
+  s.ww_m[o] = wwm;
 }
 
 // One thread per (i, k, j); the thread of level kds also writes the 2-D fields.
@@ -572,7 +617,9 @@ __global__ void __launch_bounds__(256) k_small_finish(KArgs a) {
 
 template __global__ void k_coef_w<float>(KArgs);
 template __global__ void k_coef_w<double>(KArgs);
-template __global__ void k_acoustic_col<float, false>(KArgs);
-template __global__ void k_acoustic_col<double, true>(KArgs);
+template __global__ void k_ac_mu_ww<false>(KArgs);
+template __global__ void k_ac_mu_ww<true>(KArgs);
+template __global__ void k_ac_wsolve<float>(KArgs);
+template __global__ void k_ac_wsolve<double>(KArgs);
 
 }  // namespace wrfmi

@@ -13,7 +13,12 @@ __global__ void k_small_prep(KArgs);
 __global__ void k_p_rho0(KArgs);
 template <typename CT> __global__ void k_coef_w(KArgs);
 __global__ void k_advance_uv(KArgs);
-template <typename CT, bool COMP> __global__ void k_acoustic_col(KArgs);
+__global__ void k_ac_div(KArgs);
+template <bool COMP> __global__ void k_ac_mu_ww(KArgs);
+__global__ void k_ac_theta(KArgs);
+__global__ void k_ac_wpred(KArgs);
+template <typename CT> __global__ void k_ac_wsolve(KArgs);
+__global__ void k_ac_finish(KArgs);
 __global__ void k_small_finish(KArgs);
 __global__ void k_adv_scalar(KArgs);
 __global__ void k_update_scalar(KArgs);

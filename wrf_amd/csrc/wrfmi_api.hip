@@ -103,6 +103,38 @@ extern "C" int wrfmi_sync_to_host(void *host_ptr) {
   return 0;
 }
 
+// ------------------------------------------------------------------ work arrays
+// Library-owned 3-D work arrays (one grid field each), kept per field size until
+// wrfmi_finalize.  Allocated before any launch is enqueued or captured.
+struct Scratch { float *dv, *rhs, *lo; };
+static std::map<size_t, Scratch> g_scratch;
+
+static int get_scratch(size_t nelem, Scratch *out) {
+  auto it = g_scratch.find(nelem);
+  if (it == g_scratch.end()) {
+    Scratch sc{nullptr, nullptr, nullptr};
+    const size_t bytes = nelem * sizeof(float);
+    if (hipMalloc(&sc.dv, bytes) != hipSuccess || hipMalloc(&sc.rhs, bytes) != hipSuccess ||
+        hipMalloc(&sc.lo, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      if (sc.dv) (void)hipFree(sc.dv);
+      if (sc.rhs) (void)hipFree(sc.rhs);
+      return fail(WRFMI_E_DEVICE, "cannot allocate %zu bytes of work arrays", 3 * bytes);
+    }
+    it = g_scratch.emplace(nelem, sc).first;
+  }
+  *out = it->second;
+  return 0;
+}
+static void free_scratch() {
+  for (auto &kv : g_scratch) {
+    (void)hipFree(kv.second.dv);
+    (void)hipFree(kv.second.rhs);
+    (void)hipFree(kv.second.lo);
+  }
+  g_scratch.clear();
+}
+
 // ------------------------------------------------------------------ library
 static hipStream_t g_stream = nullptr;
 static unsigned long long *g_counter = nullptr;
@@ -119,6 +151,7 @@ static void free_graphs();
 extern "C" int wrfmi_finalize(void) {
   std::lock_guard<std::mutex> lk(g_mu);
   free_graphs();
+  free_scratch();
   for (auto &kv : g_reg) (void)hipFree(kv.second.dev);
   g_reg.clear();
   if (g_counter) { (void)hipFree(g_counter); g_counter = nullptr; }
@@ -205,13 +238,13 @@ static const dim3 kColBlk(64, 4, 1);
 // ------------------------------------------------------------------ kernel timing (diagnostic)
 enum KernelId {
   KID_STEP_PREP, KID_TEND_U, KID_TEND_V, KID_TEND_W, KID_TEND_T, KID_SMALL_PREP, KID_ADVANCE_UV,
-  KID_ACOUSTIC_COL, KID_SMALL_FINISH, KID_ADV_SCALAR, KID_UPDATE_SCALAR, KID_P_RHO_PHI, KID_HALO, KID_P_RHO0,
-  KID_COEF_W, KID_WW_SCAN, KID_COUNT
+  KID_AC_DIV, KID_SMALL_FINISH, KID_ADV_SCALAR, KID_UPDATE_SCALAR, KID_P_RHO_PHI, KID_HALO, KID_P_RHO0,
+  KID_COEF_W, KID_WW_SCAN, KID_AC_MU_WW, KID_AC_THETA, KID_AC_WPRED, KID_AC_WSOLVE, KID_AC_FINISH, KID_COUNT
 };
 static const char *kKernelNames[KID_COUNT] = {
   "k_step_prep", "k_tend_u", "k_tend_v", "k_tend_w", "k_tend_t", "k_small_prep", "k_advance_uv",
-  "k_acoustic_col", "k_small_finish", "k_adv_scalar", "k_update_scalar", "k_p_rho_phi", "k_halo", "k_p_rho0",
-  "k_coef_w", "k_ww_scan"};
+  "k_ac_div", "k_small_finish", "k_adv_scalar", "k_update_scalar", "k_p_rho_phi", "k_halo", "k_p_rho0",
+  "k_coef_w", "k_ww_scan", "k_ac_mu_ww", "k_ac_theta", "k_ac_wpred", "k_ac_wsolve", "k_ac_finish"};
 struct EvPair { int id; hipEvent_t a, b; };
 static unsigned g_timing = 0;  // bit n set: time launches of kernel n
 static std::vector<EvPair> g_ev_pending;
@@ -328,8 +361,16 @@ static void ph_advance_uv(Launcher &L) {
 static void ph_advance_mu_t_w(Launcher &L) {
   const KArgs &a = L.a;
   const wrfmi_state &s = a.s;
-  if (a.c.precision_mode == 1) L.launch(KID_ACOUSTIC_COL, k_acoustic_col<double, true>, g_col(a.d), kColBlk, a);
-  else L.launch(KID_ACOUSTIC_COL, k_acoustic_col<float, false>, g_col(a.d), kColBlk, a);
+  const bool p64 = a.c.precision_mode == 1;
+  const int nk = a.d.kde - a.d.kds + 1;
+  L.launch(KID_AC_DIV, k_ac_div, g_pt3(a.d, nk - 1), kColBlk, a);
+  if (p64) L.launch(KID_AC_MU_WW, k_ac_mu_ww<true>, g_col(a.d), kColBlk, a);
+  else L.launch(KID_AC_MU_WW, k_ac_mu_ww<false>, g_col(a.d), kColBlk, a);
+  L.launch(KID_AC_THETA, k_ac_theta, g_pt3(a.d, nk), kColBlk, a);
+  L.launch(KID_AC_WPRED, k_ac_wpred, g_pt3(a.d, nk), kColBlk, a);
+  if (p64) L.launch(KID_AC_WSOLVE, k_ac_wsolve<double>, g_col(a.d), kColBlk, a);
+  else L.launch(KID_AC_WSOLVE, k_ac_wsolve<float>, g_col(a.d), kColBlk, a);
+  L.launch(KID_AC_FINISH, k_ac_finish, g_pt3(a.d, nk), kColBlk, a);
   L.halo_begin();
   L.halo_add(s.ph_2, 'w', 1); L.halo_add(s.al, 'p', 1); L.halo_add(s.p, 'p', 1);
   L.halo_add(s.muts, 't', 0); L.halo_add(s.mu_2, 't', 0); L.halo_add(s.mudf, 't', 0);
@@ -414,6 +455,11 @@ static int prepare(const wrfmi_state *st_in, const wrfmi_config *cfg, const wrfm
   L.a.s = s;
   L.a.c = *cfg;
   L.a.d = d;
+  Scratch sc;
+  if ((rc = get_scratch((size_t)d.sj * (size_t)(d.jme - d.jms + 1), &sc))) return rc;
+  L.a.scr_dv = sc.dv;
+  L.a.scr_rhs = sc.rhs;
+  L.a.scr_lo = sc.lo;
   L.h.d = d;
   L.h.periodic_x = cfg->periodic_x; L.h.periodic_y = cfg->periodic_y;
   L.h.open_xs = cfg->open_xs; L.h.open_xe = cfg->open_xe; L.h.open_ys = cfg->open_ys; L.h.open_ye = cfg->open_ye;
