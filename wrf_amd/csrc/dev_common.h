@@ -74,18 +74,28 @@ constexpr float K_W_BETA = 1.0f;
 // Fortran SIGN(1.,x) for REAL x: 1 with the sign bit of x
 DEVI float sgn1(float x) { return __builtin_copysignf(1.f, x); }
 
+// x/12 and x/60 correctly rounded, as a product in double rounded once to float.
+// For float x the exact quotient x/c (c = 12 = 3*4, c = 60 = 15*4) lies at least
+// 2^-28 (relative) away from every midpoint between adjacent normal floats: x - c*m is a
+// non-zero multiple of a power of two no smaller than 2^(E-22) for a midpoint m of binade
+// E.  The double product x*(1/c) is within 2^-52 (relative) of x/c, so it rounds to the
+// same float as the IEEE division (results in the subnormal range, |x/c| < 2^-126, excepted;
+// zero is exact).  Three instructions instead of the IEEE division sequence.
+DEVI float div12(float x) { return (float)((double)x * (1.0 / 12.0)); }
+DEVI float div60(float x) { return (float)((double)x * (1.0 / 60.0)); }
+
 // Flux operators of module_advect_em.F:199-213.  ts = SIGN(1,time_step) as a REAL.
 DEVI float flux4(float qm2, float qm1, float q0, float qp1) {
-  return (7.f * (q0 + qm1) - (qp1 + qm2)) / 12.0f;
+  return div12(7.f * (q0 + qm1) - (qp1 + qm2));
 }
 DEVI float flux3(float qm2, float qm1, float q0, float qp1, float ua, float ts) {
-  return flux4(qm2, qm1, q0, qp1) + ts * sgn1(ua) * ((qp1 - qm2) - 3.f * (q0 - qm1)) / 12.0f;
+  return flux4(qm2, qm1, q0, qp1) + div12(ts * sgn1(ua) * ((qp1 - qm2) - 3.f * (q0 - qm1)));
 }
 DEVI float flux6(float qm3, float qm2, float qm1, float q0, float qp1, float qp2) {
-  return (37.f * (q0 + qm1) - 8.f * (qp1 + qm2) + (qp2 + qm3)) / 60.0f;
+  return div60(37.f * (q0 + qm1) - 8.f * (qp1 + qm2) + (qp2 + qm3));
 }
 DEVI float flux5(float qm3, float qm2, float qm1, float q0, float qp1, float qp2, float ua, float ts) {
-  return flux6(qm3, qm2, qm1, q0, qp1, qp2) - ts * sgn1(ua) * ((qp2 - qm3) - 5.f * (qp1 - qm2) + 10.f * (q0 - qm1)) / 60.0f;
+  return flux6(qm3, qm2, qm1, q0, qp1, qp2) - div60(ts * sgn1(ua) * ((qp2 - qm3) - 5.f * (qp1 - qm2) + 10.f * (q0 - qm1)));
 }
 
 // Error-free transformation used for the compensated column sums (precision mode 1)
