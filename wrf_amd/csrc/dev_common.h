@@ -107,4 +107,58 @@ DEVI void two_sum_acc(float &s, float &comp, float x) {
   comp = comp + e;
 }
 
+// Software-pipelined column march: visits k = k0..k1 in order and calls body(k, vals),
+// where vals[f] is field f's value at level k, loaded kPF levels ahead of its use.  The
+// loads of one level do not depend on the body's results, so up to kPF levels of loads are
+// in flight while the body works through the column (a thread per column otherwise waits
+// a full memory latency per level).  load(f, k) must only read data the body never writes.
+constexpr int kPF = 8;
+template <int NF, typename T, typename Load, typename Body>
+DEVI void column_march_up(int k0, int k1, Load &&load, Body &&body) {
+  T ring[kPF][NF];
+#pragma unroll
+  for (int r = 0; r < kPF; r++)
+#pragma unroll
+    for (int f = 0; f < NF; f++) ring[r][f] = (k0 + r <= k1) ? load(f, k0 + r) : T(0);
+  for (int kb = k0; kb <= k1; kb += kPF) {
+#pragma unroll
+    for (int r = 0; r < kPF; r++) {
+      const int k = kb + r;
+      if (k <= k1) {
+        T cur[NF];
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+          cur[f] = ring[r][f];
+          if (k + kPF <= k1) ring[r][f] = load(f, k + kPF);
+        }
+        body(k, cur);
+      }
+    }
+  }
+}
+// The same, visiting k = k1 down to k0.
+template <int NF, typename T, typename Load, typename Body>
+DEVI void column_march_down(int k0, int k1, Load &&load, Body &&body) {
+  T ring[kPF][NF];
+#pragma unroll
+  for (int r = 0; r < kPF; r++)
+#pragma unroll
+    for (int f = 0; f < NF; f++) ring[r][f] = (k1 - r >= k0) ? load(f, k1 - r) : T(0);
+  for (int kb = k1; kb >= k0; kb -= kPF) {
+#pragma unroll
+    for (int r = 0; r < kPF; r++) {
+      const int k = kb - r;
+      if (k >= k0) {
+        T cur[NF];
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+          cur[f] = ring[r][f];
+          if (k - kPF >= k0) ring[r][f] = load(f, k - kPF);
+        }
+        body(k, cur);
+      }
+    }
+  }
+}
+
 }  // namespace wrfmi

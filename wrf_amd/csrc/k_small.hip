@@ -314,15 +314,17 @@ __global__ void __launch_bounds__(256) k_ac_mu_ww(KArgs a) {
   if (i > d.ide - 1 || j > d.jde - 1) return;
   const int k_end = d.kde - 1;
   const int o2 = X2(d, i, j);
+  const int ob = X3(d, i, 0, j);  // level k of the column is at ob + k*ni
   const float *dvp = a.scr_dv, *dnw = s.dnw, *c1h = s.c1h, *ww1 = s.ww1;
   float *ww = s.ww;
+  const int ni = d.ni, x1 = -d.kms;
   const float dts = a.dts, epssm = a.c.epssm, msfty = s.msfty[o2];
   float dmdt = 0.f, comp = 0.f;
-  for (int k = 1; k <= k_end; k++) {
-    const float x = dnw[X1(d, k)] * dvp[X3(d, i, k, j)];
+  column_march_up<1, float>(1, k_end, [&](int, int k) { return dvp[ob + k * ni]; }, [&](int k, const float *v) {
+    const float x = dnw[k + x1] * v[0];
     if (COMP) two_sum_acc(dmdt, comp, x);
     else dmdt = dmdt + x;
-  }
+  });
   if (COMP) dmdt = dmdt + comp;
   const float mu_tend = s.mu_tend[o2];
   float mu = s.mu_2[o2];
@@ -333,16 +335,17 @@ __global__ void __launch_bounds__(256) k_ac_mu_ww(KArgs a) {
   s.muts[o2] = s.mut[o2] + mu;
   muave = .5f * ((1.f + epssm) * mu + (1.f - epssm) * muave);
   s.muave[o2] = muave;
-  const int o1 = X3(d, i, 1, j);
-  float prev = ww[o1];
-  ww[o1] = prev - ww1[o1];
-  for (int kk = 2; kk <= k_end; kk++) {
-    const int o = X3(d, i, kk, j);
-    const int k = kk - 1;
-    const float cur = prev - dnw[X1(d, kk - 1)] * ((c1h[X1(d, k)] * dmdt) + dvp[X3(d, i, k, j)] + (c1h[X1(d, k)] * mu_tend)) / msfty;
-    prev = cur;
-    ww[o] = cur - ww1[o];
-  }
+  // scan: level kk reads dvdxi(kk-1); fields: 0 = dvdxi(kk-1), 1 = ww1(kk)
+  float prev = ww[ob + ni];
+  ww[ob + ni] = prev - ww1[ob + ni];
+  column_march_up<2, float>(
+      2, k_end, [&](int f, int kk) { return f == 0 ? dvp[ob + (kk - 1) * ni] : ww1[ob + kk * ni]; },
+      [&](int kk, const float *v) {
+        const int k = kk - 1;
+        const float cur = prev - dnw[kk - 1 + x1] * ((c1h[k + x1] * dmdt) + v[0] + (c1h[k + x1] * mu_tend)) / msfty;
+        prev = cur;
+        ww[ob + kk * ni] = cur - v[1];
+      });
 }
 
 // theta update of advance_mu_t (1125-1174), t_2ave and rhs of advance_w (1312-1371)
@@ -473,35 +476,47 @@ __global__ void __launch_bounds__(256) k_ac_wsolve(KArgs a) {
   float *w = s.w_2;
   float *lo = a.scr_lo;
   const CT *A = (const CT *)s.a, *AL = (const CT *)s.alpha, *GA = (const CT *)s.gamma;
-  CT wfe = (CT)w[X3(d, i, 1, j)];
-  for (int k = 2; k <= k_end + 1; k++) {
-    const int o = X3(d, i, k, j);
-    const float wv = w[o];
-    if (sizeof(CT) == sizeof(float)) {
-      const float wf = (wv - (float)A[o] * (float)wfe) * (float)AL[o];
-      wfe = (CT)wf;
-      w[o] = wf;
-    } else {
-      wfe = ((CT)wv - A[o] * wfe) * AL[o];
-      w[o] = (float)wfe;
-      lo[o] = (float)((double)wfe - (double)(float)wfe);
-    }
-  }
+  const int ob = X3(d, i, 0, j), ni = d.ni;
+  CT wfe = (CT)w[ob + ni];
+  // forward elimination: fields 0 = w (predictor), 1 = a, 2 = alpha
+  column_march_up<3, CT>(
+      2, k_end + 1,
+      [&](int f, int k) -> CT { const int o = ob + k * ni; return f == 0 ? (CT)w[o] : f == 1 ? A[o] : AL[o]; },
+      [&](int k, const CT *v) {
+        const int o = ob + k * ni;
+        if (sizeof(CT) == sizeof(float)) {
+          const float wf = ((float)v[0] - (float)v[1] * (float)wfe) * (float)v[2];
+          wfe = (CT)wf;
+          w[o] = wf;
+        } else {
+          wfe = (v[0] - v[1] * wfe) * v[2];
+          w[o] = (float)wfe;
+          lo[o] = (float)((double)wfe - (double)(float)wfe);
+        }
+      });
+  // back substitution: fields 0 = forward value (w, plus its low part in FP64), 1 = gamma
   CT wup = wfe;
-  for (int k = k_end; k >= 2; k--) {
-    const int o = X3(d, i, k, j);
-    float wk;
-    if (sizeof(CT) == sizeof(float)) {
-      wk = w[o] - (float)GA[o] * (float)wup;
-      wup = (CT)wk;
-    } else {
-      const CT wd = (CT)w[o] + (CT)lo[o];
-      const CT wn = wd - GA[o] * wup;
-      wup = wn;
-      wk = (float)wn;
-    }
-    w[o] = wk;
-  }
+  column_march_down<2, CT>(
+      2, k_end,
+      [&](int f, int k) -> CT {
+        const int o = ob + k * ni;
+        if (f == 1) return GA[o];
+        if (sizeof(CT) == sizeof(float)) return (CT)w[o];
+        return (CT)w[o] + (CT)lo[o];
+      },
+      [&](int k, const CT *v) {
+        const int o = ob + k * ni;
+        float wk;
+        if (sizeof(CT) == sizeof(float)) {
+          wk = (float)v[0] - (float)v[1] * (float)wup;
+          wup = (CT)wk;
+        } else {
+          const CT wn = v[0] - v[1] * wup;
+          wup = wn;
+          wk = (float)wn;
+        }
+        w[o] = wk;
+      });
 }
 
 // ph from rhs and the final w (advance_w 1460-1465), calc_p_rho for step >= 1 (438-568)
