@@ -33,6 +33,7 @@ struct KArgs {
   float *scr_dv;   // advance_mu_t's column-mass divergence dvdxi
   float *scr_rhs;  // advance_w's right-hand side rhs
   float *scr_lo;   // low-order part of the FP64 forward-elimination value (precision mode 1)
+  int tnx, tny, tnz;  // tiles of the current launch in i, j and k (see tile3)
 };
 
 // One field of a batched halo fill (set_physical_bc3d/2d on up to 16 fields per launch)
@@ -50,9 +51,31 @@ struct HaloArgs {
 
 #define DEVI __device__ __forceinline__
 
+// XCD-aware tile order.  Kernels are launched on a 1-D grid of tnx*tny*tnz workgroups and
+// the hardware deals workgroup b to XCD b % 8.  Each XCD is given a contiguous run of the
+// logical tile sequence, ordered k fastest, then i, then j, so that the tiles which read
+// each other's halo rows and levels run on one XCD close together in time and share its L2.
+struct Tile3 {
+  int bx, by, bz;
+};
+constexpr unsigned kXCD = 8;
+DEVI Tile3 tile3_of(unsigned b, int tnx, int tny, int tnz) {
+  const unsigned n = (unsigned)tnx * (unsigned)tny * (unsigned)tnz;
+  const unsigned x = b % kXCD, r = b / kXCD;
+  const unsigned L = x * (n / kXCD) + (x < n % kXCD ? x : n % kXCD) + r;
+  Tile3 t;
+  t.bz = (int)(L % (unsigned)tnz);
+  const unsigned q = L / (unsigned)tnz;
+  t.bx = (int)(q % (unsigned)tnx);
+  t.by = (int)(q / (unsigned)tnx);
+  return t;
+}
+
 DEVI int X3(const Dm &d, int i, int k, int j) { return (i - d.ims) + d.ni * (k - d.kms) + d.sj * (j - d.jms); }
 DEVI int X2(const Dm &d, int i, int j) { return (i - d.ims) + d.ni * (j - d.jms); }
 DEVI int X1(const Dm &d, int k) { return k - d.kms; }
+
+DEVI Tile3 tile3(const KArgs &a) { return tile3_of(blockIdx.x, a.tnx, a.tny, a.tnz); }
 
 template <typename T> DEVI T imin(T a, T b) { return a < b ? a : b; }
 template <typename T> DEVI T imax(T a, T b) { return a > b ? a : b; }

@@ -39,11 +39,12 @@ DEVI float prep_muv(const KArgs &a, int i, int j) {
 // needed one point into the halo, as calculate_full computes it there); the thread of
 // level 1 writes the 2-D fields.
 __global__ void __launch_bounds__(256) k_step_prep(KArgs a) {
+  const Tile3 T = tile3(a);
   const Dm &d = a.d;
   const wrfmi_state &s = a.s;
-  const int i = d.ids - 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  const int j = d.jds - 1 + (int)(blockIdx.y * blockDim.y + threadIdx.y);
-  const int k = d.kds + (int)blockIdx.z;
+  const int i = d.ids - 1 + (int)(T.bx * blockDim.x + threadIdx.x);
+  const int j = d.jds - 1 + (int)(T.by * blockDim.y + threadIdx.y);
+  const int k = d.kds + (int)T.bz;
   const int ide = d.ide, jde = d.jde, kde = d.kde;
   if (i > ide || j > jde || k > kde) return;
   const bool mi = i <= ide - 1, mj = j <= jde - 1;
@@ -113,10 +114,11 @@ __global__ void __launch_bounds__(256) k_step_prep(KArgs a) {
 // horizontal momentum and the upward scan for omega; one thread per (i, j) column.
 template <bool COMP>
 __global__ void __launch_bounds__(256) k_ww_scan(KArgs a) {
+  const Tile3 T = tile3(a);
   const Dm &d = a.d;
   const wrfmi_state &s = a.s;
-  const int i = d.ids + (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  const int j = d.jds + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
+  const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
   const int ide = d.ide, jde = d.jde, kde = d.kde;
   if (i > ide || j > jde) return;
   const bool mi = i <= ide - 1, mj = j <= jde - 1;

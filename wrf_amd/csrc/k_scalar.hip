@@ -12,11 +12,12 @@ namespace wrfmi {
 // One thread per (i, k, j) point: the horizontal fluxes at both faces in x and y and the
 // vertical fluxes at both w levels of the point (the column form carried the lower one).
 __global__ void __launch_bounds__(256) k_adv_scalar(KArgs a) {
+  const Tile3 T = tile3(a);
   const Dm &d = a.d;
   const wrfmi_state &s = a.s;
-  const int i = d.ids + (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  const int j = d.jds + (int)(blockIdx.y * blockDim.y + threadIdx.y);
-  const int k = d.kds + (int)blockIdx.z;
+  const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
+  const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
+  const int k = d.kds + (int)T.bz;
   const int ids = d.ids, ide = d.ide, jds = d.jds, jde = d.jde, kde = d.kde;
   const int its = ids, ite = ide, jts = jds, jte = jde, kts = d.kds, kte = kde;
   if (i > ide || j > jde || k > kte) return;
@@ -105,12 +106,13 @@ __global__ void __launch_bounds__(256) k_adv_scalar(KArgs a) {
 }
 
 __global__ void __launch_bounds__(256) k_update_scalar(KArgs a) {
+  const Tile3 T = tile3(a);
   const Dm &d = a.d;
   const wrfmi_state &s = a.s;
-  const int i = d.ids + (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  const int k = d.kds + (int)blockIdx.y;
-  const int j = d.jds + (int)blockIdx.z;
-  if (i > d.ide - 1 || k > d.kde - 1) return;
+  const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
+  const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
+  const int k = d.kds + (int)T.bz;
+  if (i > d.ide - 1 || j > d.jde - 1 || k > d.kde - 1) return;
   const int o = X3(d, i, k, j), o2 = X2(d, i, j);
   float *sc1 = s.q_old[a.species], *sc2 = s.q[a.species];
   const float muold = s.mu_1[o2] + s.mub[o2];
@@ -130,12 +132,13 @@ __global__ void __launch_bounds__(256) k_update_scalar(KArgs a) {
 
 // calc_p_rho_phi, non-hydrostatic; logarithm and power in FP64 then rounded once to FP32
 __global__ void __launch_bounds__(256) k_p_rho_phi(KArgs a) {
+  const Tile3 T = tile3(a);
   const Dm &d = a.d;
   const wrfmi_state &s = a.s;
-  const int i = d.ids + (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  const int k = d.kds + (int)blockIdx.y;
-  const int j = d.jds + (int)blockIdx.z;
-  if (i > d.ide - 1 || k > d.kde - 1) return;
+  const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
+  const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
+  const int k = d.kds + (int)T.bz;
+  if (i > d.ide - 1 || j > d.jde - 1 || k > d.kde - 1) return;
   const int o = X3(d, i, k, j), o2 = X2(d, i, j);
   const float muts = s.muts[o2], mu = s.mu_2[o2];
   float al;

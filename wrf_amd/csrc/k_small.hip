@@ -40,11 +40,12 @@ DEVI MuS small_mu(const KArgs &a, int i, int j) {
 }
 
 __global__ void __launch_bounds__(256) k_small_prep(KArgs a) {
+  const Tile3 T = tile3(a);
   const Dm &d = a.d;
   const wrfmi_state &s = a.s;
-  const int i = d.ids + (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  const int j = d.jds + (int)(blockIdx.y * blockDim.y + threadIdx.y);
-  const int k = d.kds + (int)blockIdx.z;
+  const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
+  const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
+  const int k = d.kds + (int)T.bz;
   const int ide = d.ide, jde = d.jde, kde = d.kde;
   if (i > ide || j > jde || k > kde) return;
   const bool mi = i <= ide - 1, mj = j <= jde - 1, mass = mi && mj;
@@ -91,11 +92,12 @@ __global__ void __launch_bounds__(256) k_small_prep(KArgs a) {
 
 // calc_p_rho at step 0 (mu = the new mu_2, mut = muts, t_1 = t_save): one thread per point
 __global__ void __launch_bounds__(256) k_p_rho0(KArgs a) {
+  const Tile3 T = tile3(a);
   const Dm &d = a.d;
   const wrfmi_state &s = a.s;
-  const int i = d.ids + (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  const int j = d.jds + (int)(blockIdx.y * blockDim.y + threadIdx.y);
-  const int k = d.kds + (int)blockIdx.z;
+  const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
+  const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
+  const int k = d.kds + (int)T.bz;
   if (i > d.ide - 1 || j > d.jde - 1 || k > d.kde - 1) return;
   if (!a.c.non_hydrostatic) return;
   const int o = X3(d, i, k, j);
@@ -114,10 +116,11 @@ __global__ void __launch_bounds__(256) k_p_rho0(KArgs a) {
 // small_step_prep's 2-D updates and calc_coef_w: one thread per (i, j) column
 template <typename CT>
 __global__ void __launch_bounds__(256) k_coef_w(KArgs a) {
+  const Tile3 T = tile3(a);
   const Dm &d = a.d;
   const wrfmi_state &s = a.s;
-  const int i = d.ids + (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  const int j = d.jds + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
+  const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
   const int ide = d.ide, jde = d.jde, kde = d.kde;
   if (i > ide || j > jde) return;
   const bool mi = i <= ide - 1, mj = j <= jde - 1, mass = mi && mj;
@@ -178,14 +181,15 @@ __global__ void __launch_bounds__(256) k_coef_w(KArgs a) {
 }
 
 __global__ void __launch_bounds__(256) k_advance_uv(KArgs a) {
+  const Tile3 T = tile3(a);
   const Dm &d = a.d;
   const wrfmi_state &s = a.s;
-  const int i = d.ids + (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  const int k = d.kds + (int)blockIdx.y;
-  const int j = d.jds + (int)blockIdx.z;
+  const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
+  const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
+  const int k = d.kds + (int)T.bz;
   const int ids = d.ids, ide = d.ide, jds = d.jds, jde = d.jde, kde = d.kde;
   const int its = ids, ite = ide, jts = jds, jte = jde;
-  if (i > ide || k > kde - 1) return;
+  if (i > ide || j > jde || k > kde - 1) return;
   const wrfmi_config &c = a.c;
   const int k_start = d.kds;
   const int i_start = its, i_end = imin(ite, ide - 1), j_start = jts, j_end = imin(jte, jde - 1), i_endu = ite, j_endv = jte;
@@ -286,11 +290,12 @@ __global__ void __launch_bounds__(256) k_advance_uv(KArgs a) {
 
 // dvdxi of advance_mu_t (module_small_step_em.F:1092-1099) at mass point (i, k, j)
 __global__ void __launch_bounds__(256) k_ac_div(KArgs a) {
+  const Tile3 T = tile3(a);
   const Dm &d = a.d;
   const wrfmi_state &s = a.s;
-  const int i = d.ids + (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  const int j = d.jds + (int)(blockIdx.y * blockDim.y + threadIdx.y);
-  const int k = d.kds + (int)blockIdx.z;
+  const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
+  const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
+  const int k = d.kds + (int)T.bz;
   if (i > d.ide - 1 || j > d.jde - 1 || k > d.kde - 1) return;
   const int o2 = X2(d, i, j), o = X3(d, i, k, j);
   const float *u = s.u_2, *u_1 = s.u_save, *v = s.v_2, *v_1 = s.v_save;
@@ -307,10 +312,11 @@ __global__ void __launch_bounds__(256) k_ac_div(KArgs a) {
 // advance_mu_t's column integral, mass update and omega scan (module_small_step_em.F:1100-1121)
 template <bool COMP>
 __global__ void __launch_bounds__(256) k_ac_mu_ww(KArgs a) {
+  const Tile3 T = tile3(a);
   const Dm &d = a.d;
   const wrfmi_state &s = a.s;
-  const int i = d.ids + (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  const int j = d.jds + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
+  const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
   if (i > d.ide - 1 || j > d.jde - 1) return;
   const int k_end = d.kde - 1;
   const int o2 = X2(d, i, j);
@@ -350,11 +356,12 @@ __global__ void __launch_bounds__(256) k_ac_mu_ww(KArgs a) {
 
 // theta update of advance_mu_t (1125-1174), t_2ave and rhs of advance_w (1312-1371)
 __global__ void __launch_bounds__(256) k_ac_theta(KArgs a) {
+  const Tile3 T = tile3(a);
   const Dm &d = a.d;
   const wrfmi_state &s = a.s;
-  const int i = d.ids + (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  const int j = d.jds + (int)(blockIdx.y * blockDim.y + threadIdx.y);
-  const int k = d.kds + (int)blockIdx.z;
+  const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
+  const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
+  const int k = d.kds + (int)T.bz;
   const int kde = d.kde, k_end = kde - 1;
   if (i > d.ide - 1 || j > d.jde - 1 || k > kde) return;
   const wrfmi_config &c = a.c;
@@ -408,11 +415,12 @@ __global__ void __launch_bounds__(256) k_ac_theta(KArgs a) {
 
 // surface w (advance_w 1383-1395) and the explicit w predictor (1402-1431), k = 1..kde
 __global__ void __launch_bounds__(256) k_ac_wpred(KArgs a) {
+  const Tile3 T = tile3(a);
   const Dm &d = a.d;
   const wrfmi_state &s = a.s;
-  const int i = d.ids + (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  const int j = d.jds + (int)(blockIdx.y * blockDim.y + threadIdx.y);
-  const int k = d.kds + (int)blockIdx.z;
+  const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
+  const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
+  const int k = d.kds + (int)T.bz;
   const int kde = d.kde, k_end = kde - 1;
   if (i > d.ide - 1 || j > d.jde - 1 || k > kde) return;
   const wrfmi_config &c = a.c;
@@ -466,10 +474,11 @@ __global__ void __launch_bounds__(256) k_ac_wpred(KArgs a) {
 // CT = double keeps the forward value in FP64, its low-order part parked in scr_lo
 template <typename CT>
 __global__ void __launch_bounds__(256) k_ac_wsolve(KArgs a) {
+  const Tile3 T = tile3(a);
   const Dm &d = a.d;
   const wrfmi_state &s = a.s;
-  const int i = d.ids + (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  const int j = d.jds + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
+  const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
   const int kde = d.kde, k_end = kde - 1;
   if (i > d.ide - 1 || j > d.jde - 1) return;
   if (!a.c.non_hydrostatic) return;
@@ -522,11 +531,12 @@ __global__ void __launch_bounds__(256) k_ac_wsolve(KArgs a) {
 // ph from rhs and the final w (advance_w 1460-1465), calc_p_rho for step >= 1 (438-568)
 // and sumflux (1473-1633); one thread per (i, k, j) with i <= ide, j <= jde, k <= kde
 __global__ void __launch_bounds__(256) k_ac_finish(KArgs a) {
+  const Tile3 T = tile3(a);
   const Dm &d = a.d;
   const wrfmi_state &s = a.s;
-  const int i = d.ids + (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  const int j = d.jds + (int)(blockIdx.y * blockDim.y + threadIdx.y);
-  const int k = d.kds + (int)blockIdx.z;
+  const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
+  const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
+  const int k = d.kds + (int)T.bz;
   const int ide = d.ide, jde = d.jde, kde = d.kde, k_end = kde - 1;
   if (i > ide || j > jde || k > kde) return;
   const bool mi = i <= ide - 1, mj = j <= jde - 1, mass = mi && mj;
@@ -578,11 +588,12 @@ __global__ void __launch_bounds__(256) k_ac_finish(KArgs a) {
 
 // One thread per (i, k, j); the thread of level kds also writes the 2-D fields.
 __global__ void __launch_bounds__(256) k_small_finish(KArgs a) {
+  const Tile3 T = tile3(a);
   const Dm &d = a.d;
   const wrfmi_state &s = a.s;
-  const int i = d.ids + (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  const int j = d.jds + (int)(blockIdx.y * blockDim.y + threadIdx.y);
-  const int k = d.kds + (int)blockIdx.z;
+  const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
+  const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
+  const int k = d.kds + (int)T.bz;
   const int ide = d.ide, jde = d.jde, kde = d.kde;
   if (i > ide || j > jde || k > kde) return;
   const bool mi = i <= ide - 1, mj = j <= jde - 1, mass = mi && mj;

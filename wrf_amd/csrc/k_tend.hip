@@ -30,11 +30,12 @@ DEVI Geo geo(const wrfmi_config &c) {
 
 // ------------------------------------------------------------------------------ u
 __global__ void __launch_bounds__(256) k_tend_u(KArgs a) {
+  const Tile3 T = tile3(a);
   const Dm &d = a.d;
   const wrfmi_state &s = a.s;
-  const int i = d.ids + (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  const int j = d.jds + (int)(blockIdx.y * blockDim.y + threadIdx.y);
-  const int k = d.kds + (int)blockIdx.z;
+  const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
+  const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
+  const int k = d.kds + (int)T.bz;
   const int ids = d.ids, ide = d.ide, jds = d.jds, jde = d.jde, kde = d.kde;
   const int its = ids, ite = ide, jts = jds, jte = jde, kts = d.kds, kte = kde;
   if (i > ide || j > jde || k > kte) return;
@@ -249,11 +250,12 @@ __global__ void __launch_bounds__(256) k_tend_u(KArgs a) {
 
 // ------------------------------------------------------------------------------ v
 __global__ void __launch_bounds__(256) k_tend_v(KArgs a) {
+  const Tile3 T = tile3(a);
   const Dm &d = a.d;
   const wrfmi_state &s = a.s;
-  const int i = d.ids + (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  const int j = d.jds + (int)(blockIdx.y * blockDim.y + threadIdx.y);
-  const int k = d.kds + (int)blockIdx.z;
+  const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
+  const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
+  const int k = d.kds + (int)T.bz;
   const int ids = d.ids, ide = d.ide, jds = d.jds, jde = d.jde, kde = d.kde;
   const int its = ids, ite = ide, jts = jds, jte = jde, kts = d.kds, kte = kde;
   if (i > ide || j > jde || k > kte) return;
@@ -449,11 +451,12 @@ __global__ void __launch_bounds__(256) k_tend_v(KArgs a) {
 
 // ------------------------------------------------------------------------------ w
 __global__ void __launch_bounds__(256) k_tend_w(KArgs a) {
+  const Tile3 T = tile3(a);
   const Dm &d = a.d;
   const wrfmi_state &s = a.s;
-  const int i = d.ids + (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  const int j = d.jds + (int)(blockIdx.y * blockDim.y + threadIdx.y);
-  const int k = d.kds + (int)blockIdx.z;
+  const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
+  const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
+  const int k = d.kds + (int)T.bz;
   const int ids = d.ids, ide = d.ide, jds = d.jds, jde = d.jde, kde = d.kde;
   const int its = ids, ite = ide, jts = jds, jte = jde, kts = d.kds, kte = kde;
   if (i > ide || j > jde || k > kte) return;
@@ -641,11 +644,12 @@ __global__ void __launch_bounds__(256) k_tend_w(KArgs a) {
 
 // ------------------------------------------------------------------------------ t, ph, zeroing
 __global__ void __launch_bounds__(256) k_tend_t(KArgs a) {
+  const Tile3 T = tile3(a);
   const Dm &d = a.d;
   const wrfmi_state &s = a.s;
-  const int i = d.ids + (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  const int j = d.jds + (int)(blockIdx.y * blockDim.y + threadIdx.y);
-  const int k = d.kds + (int)blockIdx.z;
+  const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
+  const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
+  const int k = d.kds + (int)T.bz;
   const int ids = d.ids, ide = d.ide, jds = d.jds, jde = d.jde, kde = d.kde;
   const int its = ids, ite = ide, jts = jds, jte = jde, kts = d.kds, kte = kde;
   if (i > ide || j > jde || k > kte) return;

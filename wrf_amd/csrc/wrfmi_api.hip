@@ -283,6 +283,13 @@ struct Launcher {
       g_ev_pending.push_back(EvPair{id, e0, e1});
     }
   }
+  // a KArgs kernel on a tnx x tny x tnz tile grid, launched 1-D for the XCD-aware order
+  void launchK(int id, void (*kern)(KArgs), dim3 g, dim3 b, KArgs args) {
+    args.tnx = (int)g.x;
+    args.tny = (int)g.y;
+    args.tnz = (int)g.z;
+    launch(id, kern, dim3(g.x * g.y * g.z, 1, 1), b, args);
+  }
   void halo_begin() { h.n = 0; }
   void halo_add(float *p, char kind, int is3d) {
     int k = kind == 'u' ? 1 : kind == 'v' ? 2 : kind == 'w' ? 3 : 0;
@@ -311,21 +318,19 @@ static int num_sound_steps(const wrfmi_config &c, const wrfmi_state &s) {
 }
 static inline dim3 g_prep(const Dm &d) { return col_grid(d, d.ids - 1, d.jds - 1, kColBlk); }
 static inline dim3 g_col(const Dm &d) { return col_grid(d, d.ids, d.jds, kColBlk); }
-static inline dim3 g_pt(const Dm &d) { return dim3(((d.ide - d.ids + 1) + 63) / 64, d.kde - d.kds, d.jde - d.jds + 1); }
 // (i, j) tiles of kColBlk over i = ids..ide, j = jds..jde, and nk levels in z
 static inline dim3 g_pt3(const Dm &d, int nk) {
   const dim3 g = col_grid(d, d.ids, d.jds, kColBlk);
   return dim3(g.x, g.y, nk);
 }
-static inline dim3 g_mass(const Dm &d) { return dim3(((d.ide - d.ids) + 63) / 64, d.kde - d.kds, d.jde - d.jds); }
 
 static void ph_step_prep(Launcher &L) {
   const KArgs &a = L.a;
   const wrfmi_state &s = a.s;
   const dim3 gp = g_prep(a.d);
-  L.launch(KID_STEP_PREP, k_step_prep, dim3(gp.x, gp.y, a.d.kde - a.d.kds + 1), kColBlk, a);
-  if (a.c.precision_mode == 1) L.launch(KID_WW_SCAN, k_ww_scan<true>, g_col(a.d), kColBlk, a);
-  else L.launch(KID_WW_SCAN, k_ww_scan<false>, g_col(a.d), kColBlk, a);
+  L.launchK(KID_STEP_PREP, k_step_prep, dim3(gp.x, gp.y, a.d.kde - a.d.kds + 1), kColBlk, a);
+  if (a.c.precision_mode == 1) L.launchK(KID_WW_SCAN, k_ww_scan<true>, g_col(a.d), kColBlk, a);
+  else L.launchK(KID_WW_SCAN, k_ww_scan<false>, g_col(a.d), kColBlk, a);
   L.halo_begin();
   L.halo_add(s.ru, 'u', 1); L.halo_add(s.rv, 'v', 1); L.halo_add(s.rw, 'w', 1); L.halo_add(s.ww, 'w', 1);
   L.halo_add(s.php, 'w', 1); L.halo_add(s.alt, 't', 1); L.halo_add(s.p, 'p', 1);
@@ -336,19 +341,19 @@ static void ph_step_prep(Launcher &L) {
 static void ph_tendency(Launcher &L) {
   const KArgs &a = L.a;
   const int nk = a.d.kde - a.d.kds + 1;
-  L.launch(KID_TEND_U, k_tend_u, g_pt3(a.d, nk), kColBlk, a);
-  L.launch(KID_TEND_V, k_tend_v, g_pt3(a.d, nk), kColBlk, a);
-  L.launch(KID_TEND_W, k_tend_w, g_pt3(a.d, nk), kColBlk, a);
-  L.launch(KID_TEND_T, k_tend_t, g_pt3(a.d, nk), kColBlk, a);
+  L.launchK(KID_TEND_U, k_tend_u, g_pt3(a.d, nk), kColBlk, a);
+  L.launchK(KID_TEND_V, k_tend_v, g_pt3(a.d, nk), kColBlk, a);
+  L.launchK(KID_TEND_W, k_tend_w, g_pt3(a.d, nk), kColBlk, a);
+  L.launchK(KID_TEND_T, k_tend_t, g_pt3(a.d, nk), kColBlk, a);
 }
 static void ph_small_prep(Launcher &L) {
   const KArgs &a = L.a;
   const wrfmi_state &s = a.s;
   const int nk = a.d.kde - a.d.kds + 1;
-  L.launch(KID_SMALL_PREP, k_small_prep, g_pt3(a.d, nk), kColBlk, a);
-  L.launch(KID_P_RHO0, k_p_rho0, g_pt3(a.d, nk - 1), kColBlk, a);
-  if (a.c.precision_mode == 1) L.launch(KID_COEF_W, k_coef_w<double>, g_col(a.d), kColBlk, a);
-  else L.launch(KID_COEF_W, k_coef_w<float>, g_col(a.d), kColBlk, a);
+  L.launchK(KID_SMALL_PREP, k_small_prep, g_pt3(a.d, nk), kColBlk, a);
+  L.launchK(KID_P_RHO0, k_p_rho0, g_pt3(a.d, nk - 1), kColBlk, a);
+  if (a.c.precision_mode == 1) L.launchK(KID_COEF_W, k_coef_w<double>, g_col(a.d), kColBlk, a);
+  else L.launchK(KID_COEF_W, k_coef_w<float>, g_col(a.d), kColBlk, a);
   L.halo_begin();
   L.halo_add(s.ru_tend, 'u', 1); L.halo_add(s.rv_tend, 'v', 1); L.halo_add(s.ph_2, 'w', 1);
   L.halo_add(s.al, 'p', 1); L.halo_add(s.p, 'p', 1); L.halo_add(s.t_1, 'p', 1); L.halo_add(s.t_save, 't', 1);
@@ -356,21 +361,21 @@ static void ph_small_prep(Launcher &L) {
   L.halo_flush();
 }
 static void ph_advance_uv(Launcher &L) {
-  L.launch(KID_ADVANCE_UV, k_advance_uv, g_pt(L.a.d), dim3(64, 1, 1), L.a);
+  L.launchK(KID_ADVANCE_UV, k_advance_uv, g_pt3(L.a.d, L.a.d.kde - L.a.d.kds), kColBlk, L.a);
 }
 static void ph_advance_mu_t_w(Launcher &L) {
   const KArgs &a = L.a;
   const wrfmi_state &s = a.s;
   const bool p64 = a.c.precision_mode == 1;
   const int nk = a.d.kde - a.d.kds + 1;
-  L.launch(KID_AC_DIV, k_ac_div, g_pt3(a.d, nk - 1), kColBlk, a);
-  if (p64) L.launch(KID_AC_MU_WW, k_ac_mu_ww<true>, g_col(a.d), kColBlk, a);
-  else L.launch(KID_AC_MU_WW, k_ac_mu_ww<false>, g_col(a.d), kColBlk, a);
-  L.launch(KID_AC_THETA, k_ac_theta, g_pt3(a.d, nk), kColBlk, a);
-  L.launch(KID_AC_WPRED, k_ac_wpred, g_pt3(a.d, nk), kColBlk, a);
-  if (p64) L.launch(KID_AC_WSOLVE, k_ac_wsolve<double>, g_col(a.d), kColBlk, a);
-  else L.launch(KID_AC_WSOLVE, k_ac_wsolve<float>, g_col(a.d), kColBlk, a);
-  L.launch(KID_AC_FINISH, k_ac_finish, g_pt3(a.d, nk), kColBlk, a);
+  L.launchK(KID_AC_DIV, k_ac_div, g_pt3(a.d, nk - 1), kColBlk, a);
+  if (p64) L.launchK(KID_AC_MU_WW, k_ac_mu_ww<true>, g_col(a.d), kColBlk, a);
+  else L.launchK(KID_AC_MU_WW, k_ac_mu_ww<false>, g_col(a.d), kColBlk, a);
+  L.launchK(KID_AC_THETA, k_ac_theta, g_pt3(a.d, nk), kColBlk, a);
+  L.launchK(KID_AC_WPRED, k_ac_wpred, g_pt3(a.d, nk), kColBlk, a);
+  if (p64) L.launchK(KID_AC_WSOLVE, k_ac_wsolve<double>, g_col(a.d), kColBlk, a);
+  else L.launchK(KID_AC_WSOLVE, k_ac_wsolve<float>, g_col(a.d), kColBlk, a);
+  L.launchK(KID_AC_FINISH, k_ac_finish, g_pt3(a.d, nk), kColBlk, a);
   L.halo_begin();
   L.halo_add(s.ph_2, 'w', 1); L.halo_add(s.al, 'p', 1); L.halo_add(s.p, 'p', 1);
   L.halo_add(s.muts, 't', 0); L.halo_add(s.mu_2, 't', 0); L.halo_add(s.mudf, 't', 0);
@@ -379,7 +384,7 @@ static void ph_advance_mu_t_w(Launcher &L) {
 static void ph_small_finish(Launcher &L) {
   const KArgs &a = L.a;
   const wrfmi_state &s = a.s;
-  L.launch(KID_SMALL_FINISH, k_small_finish, g_pt3(a.d, a.d.kde - a.d.kds + 1), kColBlk, a);
+  L.launchK(KID_SMALL_FINISH, k_small_finish, g_pt3(a.d, a.d.kde - a.d.kds + 1), kColBlk, a);
   if (a.rk_step == 3) {
     L.halo_begin();
     L.halo_add(s.ru_m, 'u', 1); L.halo_add(s.rv_m, 'v', 1); L.halo_add(s.ww_m, 'w', 1);
@@ -388,11 +393,11 @@ static void ph_small_finish(Launcher &L) {
   }
 }
 static void ph_scalar(Launcher &L) {
-  L.launch(KID_ADV_SCALAR, k_adv_scalar, g_pt3(L.a.d, L.a.d.kde - L.a.d.kds + 1), kColBlk, L.a);
-  L.launch(KID_UPDATE_SCALAR, k_update_scalar, g_mass(L.a.d), dim3(64, 1, 1), L.a);
+  L.launchK(KID_ADV_SCALAR, k_adv_scalar, g_pt3(L.a.d, L.a.d.kde - L.a.d.kds + 1), kColBlk, L.a);
+  L.launchK(KID_UPDATE_SCALAR, k_update_scalar, g_pt3(L.a.d, L.a.d.kde - L.a.d.kds), kColBlk, L.a);
 }
 static void ph_p_rho_phi(Launcher &L) {
-  L.launch(KID_P_RHO_PHI, k_p_rho_phi, g_mass(L.a.d), dim3(64, 1, 1), L.a);
+  L.launchK(KID_P_RHO_PHI, k_p_rho_phi, g_pt3(L.a.d, L.a.d.kde - L.a.d.kds), kColBlk, L.a);
 }
 static void ph_bc_dry_2(Launcher &L, int kind) {
   const wrfmi_state &s = L.a.s;
