@@ -17,6 +17,7 @@
 #define __host__
 #define __forceinline__ inline
 #define __launch_bounds__(...)
+#define WRFMI_GLOBAL_AS
 
 struct dim3 {
   unsigned x, y, z;

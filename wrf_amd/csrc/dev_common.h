@@ -18,6 +18,52 @@ struct Dm {
   int ni, nk, sj;  // memory extents in i and k, stride of j (= ni*nk)
 };
 
+// Element offsets are carried as 32-bit byte offsets (Off) and added to the uniform base
+// pointer of each array (FPtr), so that every access is one SGPR-base + VGPR-offset load
+// with no 64-bit address arithmetic.  The host keeps fields below 2^29 elements.
+struct Off {
+  unsigned b;  // byte offset of a 4-byte element
+};
+#define HDI __host__ __device__ __forceinline__
+HDI Off operator+(Off o, int n) { return Off{o.b + 4u * (unsigned)n}; }
+HDI Off operator-(Off o, int n) { return Off{o.b - 4u * (unsigned)n}; }
+// Every array lives in device global memory (address space 1); saying so lets the compiler
+// use global (not flat) loads and stores with an SGPR base.
+#ifndef WRFMI_GLOBAL_AS
+#define WRFMI_GLOBAL_AS __attribute__((address_space(1)))
+#endif
+template <typename T>
+struct FPtr {
+  T *p;
+  HDI FPtr() : p(nullptr) {}
+  HDI FPtr(T *q) : p(q) {}
+  // element at Off o; 8-byte elements (the FP64 tridiagonal coefficients) sit at twice the
+  // byte offset of the 4-byte element with the same index
+  HDI T &operator[](Off o) const {
+    WRFMI_GLOBAL_AS char *g = (WRFMI_GLOBAL_AS char *)(char *)p;
+    return *(T *)(WRFMI_GLOBAL_AS T *)(g + (sizeof(T) == 8 ? 2u * o.b : o.b));
+  }
+  HDI T &operator[](int i) const { return *(T *)((WRFMI_GLOBAL_AS T *)p + i); }
+};
+
+// Device view of wrfmi_state: same layout, arrays as FPtr
+struct DState {
+#define WRFMI_DPTR(n) FPtr<float> n;
+  WRFMI_FIELDS_3D(WRFMI_DPTR)
+  WRFMI_FIELDS_2D(WRFMI_DPTR)
+  WRFMI_FIELDS_1D(WRFMI_DPTR)
+#undef WRFMI_DPTR
+#define WRFMI_DR(n) float n;
+  WRFMI_SCALARS_R(WRFMI_DR)
+#undef WRFMI_DR
+  void *a, *alpha, *gamma;
+  int n_moist, n_species;
+  FPtr<float> q[WRFMI_MAX_SPECIES];
+  FPtr<float> q_old[WRFMI_MAX_SPECIES];
+  FPtr<float> q_tend[WRFMI_MAX_SPECIES];
+};
+static_assert(sizeof(DState) == sizeof(wrfmi_state), "DState must mirror wrfmi_state");
+
 struct KArgs {
   wrfmi_state s;   // device pointers
   wrfmi_config c;
@@ -30,9 +76,9 @@ struct KArgs {
   int nsmall;      // acoustic substeps in this RK substep
   int species;     // species index for the scalar kernels
   // library-owned 3-D work arrays, same layout as a grid field
-  float *scr_dv;   // advance_mu_t's column-mass divergence dvdxi
-  float *scr_rhs;  // advance_w's right-hand side rhs
-  float *scr_lo;   // low-order part of the FP64 forward-elimination value (precision mode 1)
+  FPtr<float> scr_dv;   // advance_mu_t's column-mass divergence dvdxi
+  FPtr<float> scr_rhs;  // advance_w's right-hand side rhs
+  FPtr<float> scr_lo;   // low-order part of the FP64 forward-elimination value (precision mode 1)
   int tnx, tny, tnz;  // tiles of the current launch in i, j and k (see tile3)
 };
 
@@ -71,9 +117,12 @@ DEVI Tile3 tile3_of(unsigned b, int tnx, int tny, int tnz) {
   return t;
 }
 
-DEVI int X3(const Dm &d, int i, int k, int j) { return (i - d.ims) + d.ni * (k - d.kms) + d.sj * (j - d.jms); }
-DEVI int X2(const Dm &d, int i, int j) { return (i - d.ims) + d.ni * (j - d.jms); }
-DEVI int X1(const Dm &d, int k) { return k - d.kms; }
+DEVI int X3i(const Dm &d, int i, int k, int j) { return (i - d.ims) + d.ni * (k - d.kms) + d.sj * (j - d.jms); }
+DEVI int X2i(const Dm &d, int i, int j) { return (i - d.ims) + d.ni * (j - d.jms); }
+DEVI Off X3(const Dm &d, int i, int k, int j) { return Off{4u * (unsigned)X3i(d, i, k, j)}; }
+DEVI Off X2(const Dm &d, int i, int j) { return Off{4u * (unsigned)X2i(d, i, j)}; }
+DEVI Off X1(const Dm &d, int k) { return Off{4u * (unsigned)(k - d.kms)}; }
+DEVI const DState &dstate(const KArgs &a) { return *reinterpret_cast<const DState *>(&a.s); }
 
 DEVI Tile3 tile3(const KArgs &a) { return tile3_of(blockIdx.x, a.tnx, a.tny, a.tnz); }
 

@@ -14,7 +14,7 @@ namespace wrfmi {
 // calc_mu_uv's muu, muv at (i, j) (whole-patch tile: its == ids, ite == ide)
 DEVI float prep_muu(const KArgs &a, int i, int j) {
   const Dm &d = a.d;
-  const float *mu = a.s.mu_2, *mub = a.s.mub;
+  const FPtr<float> mu = dstate(a).mu_2, mub = dstate(a).mub;
   if (i > d.ids && i < d.ide) return 0.5f * (((mu[X2(d, i, j)] + mu[X2(d, i - 1, j)]) + mub[X2(d, i, j)]) + mub[X2(d, i - 1, j)]);
   if (i == d.ids) {
     const int im = a.c.periodic_x ? i - 1 : i;
@@ -25,7 +25,7 @@ DEVI float prep_muu(const KArgs &a, int i, int j) {
 }
 DEVI float prep_muv(const KArgs &a, int i, int j) {
   const Dm &d = a.d;
-  const float *mu = a.s.mu_2, *mub = a.s.mub;
+  const FPtr<float> mu = dstate(a).mu_2, mub = dstate(a).mub;
   if (j > d.jds && j < d.jde) return 0.5f * (((mu[X2(d, i, j)] + mu[X2(d, i, j - 1)]) + mub[X2(d, i, j)]) + mub[X2(d, i, j - 1)]);
   if (j == d.jds) {
     const int jm = a.c.periodic_y ? j - 1 : j;
@@ -41,7 +41,7 @@ DEVI float prep_muv(const KArgs &a, int i, int j) {
 __global__ void __launch_bounds__(256) k_step_prep(KArgs a) {
   const Tile3 T = tile3(a);
   const Dm &d = a.d;
-  const wrfmi_state &s = a.s;
+  const DState &s = dstate(a);
   const int i = d.ids - 1 + (int)(T.bx * blockDim.x + threadIdx.x);
   const int j = d.jds - 1 + (int)(T.by * blockDim.y + threadIdx.y);
   const int k = d.kds + (int)T.bz;
@@ -56,7 +56,7 @@ __global__ void __launch_bounds__(256) k_step_prep(KArgs a) {
   }
   if (i < d.ids || j < d.jds) return;
   const int ktf = kde - 1;
-  const int o = X3(d, i, k, j);
+  const Off o = X3(d, i, k, j);
   const int nm = s.n_moist;
   if (mj) {
     const float muu = prep_muu(a, i, j);
@@ -116,14 +116,14 @@ template <bool COMP>
 __global__ void __launch_bounds__(256) k_ww_scan(KArgs a) {
   const Tile3 T = tile3(a);
   const Dm &d = a.d;
-  const wrfmi_state &s = a.s;
+  const DState &s = dstate(a);
   const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
   const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
   const int ide = d.ide, jde = d.jde, kde = d.kde;
   if (i > ide || j > jde) return;
   const bool mi = i <= ide - 1, mj = j <= jde - 1;
-  const float *mu = s.mu_2, *mub = s.mub;
-  const float *c1h = s.c1h, *c2h = s.c2h;
+  const FPtr<float> mu = s.mu_2, mub = s.mub;
+  const FPtr<float> c1h = s.c1h, c2h = s.c2h;
   const int ktf = kde - 1;
   // ww(1) = ww(kte) = 0 on i = its..ite
   if (mj) {
@@ -143,7 +143,7 @@ __global__ void __launch_bounds__(256) k_ww_scan(KArgs a) {
   float dmdt = 0.f, comp = 0.f;
   // pass 1: divv(k) and its column sum; divv(k) is parked in ww(k+1) until the scan reads it
   for (int k = 1; k <= ktf; k++) {
-    const int o = X3(d, i, k, j);
+    const Off o = X3(d, i, k, j);
     const float c1 = c1h[X1(d, k)], c2 = c2h[X1(d, k)];
     const float tx = (c1 * lmuu1 + c2) * s.u_2[o + 1] / msfuy1 - (c1 * lmuu0 + c2) * s.u_2[o] / msfuy0;
     const float ty = (c1 * lmuv1 + c2) * s.v_2[o + d.sj] * mvi1 - (c1 * lmuv0 + c2) * s.v_2[o] * mvi0;
@@ -155,7 +155,7 @@ __global__ void __launch_bounds__(256) k_ww_scan(KArgs a) {
   if (COMP) dmdt = dmdt + comp;
   float prev = 0.f;
   for (int k = 2; k <= ktf; k++) {
-    const int o = X3(d, i, k, j);
+    const Off o = X3(d, i, k, j);
     const float dv = s.ww[o];
     const float cur = prev - s.dnw[X1(d, k - 1)] * c1h[X1(d, k - 1)] * dmdt - dv;
     s.ww[o] = cur;

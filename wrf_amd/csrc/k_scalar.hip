@@ -14,7 +14,7 @@ namespace wrfmi {
 __global__ void __launch_bounds__(256) k_adv_scalar(KArgs a) {
   const Tile3 T = tile3(a);
   const Dm &d = a.d;
-  const wrfmi_state &s = a.s;
+  const DState &s = dstate(a);
   const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
   const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
   const int k = d.kds + (int)T.bz;
@@ -24,8 +24,8 @@ __global__ void __launch_bounds__(256) k_adv_scalar(KArgs a) {
   const wrfmi_config &c = a.c;
   const float ts = c.time_step >= 0 ? 1.f : -1.f;
   const int ktf = kte - 1;
-  const float *f = s.q[a.species], *ru = s.ru_m, *rv = s.rv_m, *rom = s.ww_m;
-  float *tend = s.advect_tend;
+  const FPtr<float> f = s.q[a.species], ru = s.ru_m, rv = s.rv_m, rom = s.ww_m;
+  FPtr<float> tend = s.advect_tend;
   const float rdx = s.rdx, rdy = s.rdy;
   const bool mass = i <= ide - 1 && j <= jde - 1;
   if (!mass) {
@@ -71,7 +71,7 @@ __global__ void __launch_bounds__(256) k_adv_scalar(KArgs a) {
     const float vel = rom[X3(d, i, k, j)];
     return vel * flux3(f[X3(d, i, k - 2, j)], f[X3(d, i, k - 1, j)], f[X3(d, i, k, j)], f[X3(d, i, k + 1, j)], -vel, ts);
   };
-  auto P = [&](const float *arr, int ii, int k, int jj) -> float { return arr[X3(d, ii, k, jj)]; };
+  auto P = [&](FPtr<float> arr, int ii, int k, int jj) -> float { return arr[X3(d, ii, k, jj)]; };
   {
     float t = 0.f;
     if (k <= ktf) {
@@ -108,13 +108,13 @@ __global__ void __launch_bounds__(256) k_adv_scalar(KArgs a) {
 __global__ void __launch_bounds__(256) k_update_scalar(KArgs a) {
   const Tile3 T = tile3(a);
   const Dm &d = a.d;
-  const wrfmi_state &s = a.s;
+  const DState &s = dstate(a);
   const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
   const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
   const int k = d.kds + (int)T.bz;
   if (i > d.ide - 1 || j > d.jde - 1 || k > d.kde - 1) return;
-  const int o = X3(d, i, k, j), o2 = X2(d, i, j);
-  float *sc1 = s.q_old[a.species], *sc2 = s.q[a.species];
+  const Off o = X3(d, i, k, j), o2 = X2(d, i, j);
+  FPtr<float> sc1 = s.q_old[a.species], sc2 = s.q[a.species];
   const float muold = s.mu_1[o2] + s.mub[o2];
   const float munew = s.mu_2[o2] + s.mub[o2];
   float tendency = s.advect_tend[o] * s.msfty[o2];
@@ -134,12 +134,12 @@ __global__ void __launch_bounds__(256) k_update_scalar(KArgs a) {
 __global__ void __launch_bounds__(256) k_p_rho_phi(KArgs a) {
   const Tile3 T = tile3(a);
   const Dm &d = a.d;
-  const wrfmi_state &s = a.s;
+  const DState &s = dstate(a);
   const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
   const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
   const int k = d.kds + (int)T.bz;
   if (i > d.ide - 1 || j > d.jde - 1 || k > d.kde - 1) return;
-  const int o = X3(d, i, k, j), o2 = X2(d, i, j);
+  const Off o = X3(d, i, k, j), o2 = X2(d, i, j);
   const float muts = s.muts[o2], mu = s.mu_2[o2];
   float al;
   if (a.c.hypsometric_opt == 1) {
@@ -229,9 +229,9 @@ __global__ void __launch_bounds__(256) k_halo(HaloArgs h) {
     return;
   }
   if (F.is3d) {
-    for (int k = d.kds; k <= kend; k++) F.p[X3(d, i, k, j)] = F.p[X3(d, si, k, sj)];
+    for (int k = d.kds; k <= kend; k++) F.p[X3i(d, i, k, j)] = F.p[X3i(d, si, k, sj)];
   } else {
-    F.p[X2(d, i, j)] = F.p[X2(d, si, sj)];
+    F.p[X2i(d, i, j)] = F.p[X2i(d, si, sj)];
   }
 }
 

@@ -22,8 +22,8 @@ struct MuS {  // small_step_prep's column masses of one (i, j)
 };
 DEVI MuS small_mu(const KArgs &a, int i, int j) {
   const Dm &d = a.d;
-  const wrfmi_state &s = a.s;
-  const int o2 = X2(d, i, j);
+  const DState &s = dstate(a);
+  const Off o2 = X2(d, i, j);
   MuS m;
   if (a.rk_step == 1) {
     m.muts = s.mub[o2] + s.mu_2[o2];
@@ -42,7 +42,7 @@ DEVI MuS small_mu(const KArgs &a, int i, int j) {
 __global__ void __launch_bounds__(256) k_small_prep(KArgs a) {
   const Tile3 T = tile3(a);
   const Dm &d = a.d;
-  const wrfmi_state &s = a.s;
+  const DState &s = dstate(a);
   const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
   const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
   const int k = d.kds + (int)T.bz;
@@ -50,8 +50,8 @@ __global__ void __launch_bounds__(256) k_small_prep(KArgs a) {
   if (i > ide || j > jde || k > kde) return;
   const bool mi = i <= ide - 1, mj = j <= jde - 1, mass = mi && mj;
   const int k_end = kde - 1;
-  const float *c1h = s.c1h, *c2h = s.c2h, *c1f = s.c1f, *c2f = s.c2f;
-  const int o2 = X2(d, i, j), o = X3(d, i, k, j);
+  const FPtr<float> c1h = s.c1h, c2h = s.c2h, c1f = s.c1f, c2f = s.c2f;
+  const Off o2 = X2(d, i, j), o = X3(d, i, k, j);
   const bool rk1 = a.rk_step == 1;
   const MuS m = small_mu(a, i, j);
   if (mj && k <= k_end) {
@@ -94,13 +94,13 @@ __global__ void __launch_bounds__(256) k_small_prep(KArgs a) {
 __global__ void __launch_bounds__(256) k_p_rho0(KArgs a) {
   const Tile3 T = tile3(a);
   const Dm &d = a.d;
-  const wrfmi_state &s = a.s;
+  const DState &s = dstate(a);
   const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
   const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
   const int k = d.kds + (int)T.bz;
   if (i > d.ide - 1 || j > d.jde - 1 || k > d.kde - 1) return;
   if (!a.c.non_hydrostatic) return;
-  const int o = X3(d, i, k, j);
+  const Off o = X3(d, i, k, j);
   const MuS m = small_mu(a, i, j);
   const float mu = m.mu_new, muts = m.muts;
   const float c1 = s.c1h[X1(d, k)], c2 = s.c2h[X1(d, k)];
@@ -118,14 +118,14 @@ template <typename CT>
 __global__ void __launch_bounds__(256) k_coef_w(KArgs a) {
   const Tile3 T = tile3(a);
   const Dm &d = a.d;
-  const wrfmi_state &s = a.s;
+  const DState &s = dstate(a);
   const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
   const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
   const int ide = d.ide, jde = d.jde, kde = d.kde;
   if (i > ide || j > jde) return;
   const bool mi = i <= ide - 1, mj = j <= jde - 1, mass = mi && mj;
-  const float *c1h = s.c1h, *c2h = s.c2h, *c1f = s.c1f, *c2f = s.c2f;
-  const int o2 = X2(d, i, j);
+  const FPtr<float> c1h = s.c1h, c2h = s.c2h, c1f = s.c1f, c2f = s.c2f;
+  const Off o2 = X2(d, i, j);
   const MuS m = small_mu(a, i, j);
   if (mass) {
     if (a.rk_step == 1) {
@@ -140,7 +140,7 @@ __global__ void __launch_bounds__(256) k_coef_w(KArgs a) {
   if (mi) s.muvs[o2] = m.muvs;
   if (!mass || !a.c.non_hydrostatic) return;
   const float mut = s.mut[o2];
-  CT *A = (CT *)s.a, *AL = (CT *)s.alpha, *GA = (CT *)s.gamma;
+  const FPtr<CT> A((CT *)s.a), AL((CT *)s.alpha), GA((CT *)s.gamma);
   const CT lid = a.c.top_lid ? CT(0) : CT(1);
   const CT epssm = (CT)a.c.epssm;
   CT cof = (CT(.5) * (CT)a.dts * (CT)K_G * (CT(1) + epssm));
@@ -183,7 +183,7 @@ __global__ void __launch_bounds__(256) k_coef_w(KArgs a) {
 __global__ void __launch_bounds__(256) k_advance_uv(KArgs a) {
   const Tile3 T = tile3(a);
   const Dm &d = a.d;
-  const wrfmi_state &s = a.s;
+  const DState &s = dstate(a);
   const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
   const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
   const int k = d.kds + (int)T.bz;
@@ -205,10 +205,10 @@ __global__ void __launch_bounds__(256) k_advance_uv(KArgs a) {
   if (c.symmetric_ye && jte == jde) j_end_v_tend--;
   const float rdx = s.rdx, rdy = s.rdy, dx = 1.f / rdx, dy = 1.f / rdy, dts = a.dts;
   const float cf1 = s.cf1, cf2 = s.cf2, cf3 = s.cf3, emdiv = c.emdiv;
-  const float *p = s.p, *pb = s.pb, *ph = s.ph_2, *php = s.php, *alt = s.alt, *al = s.al, *mu = s.mu_2, *mudf = s.mudf;
+  const FPtr<float> p = s.p, pb = s.pb, ph = s.ph_2, php = s.php, alt = s.alt, al = s.al, mu = s.mu_2, mudf = s.mudf;
   const float c1 = s.c1h[X1(d, k)], c2 = s.c2h[X1(d, k)];
   const bool nh = c.non_hydrostatic;
-  const int o = X3(d, i, k, j);
+  const Off o = X3(d, i, k, j);
   // u
   if (j >= j_start && j <= j_end) {
     float u = s.u_2[o];
@@ -292,13 +292,13 @@ __global__ void __launch_bounds__(256) k_advance_uv(KArgs a) {
 __global__ void __launch_bounds__(256) k_ac_div(KArgs a) {
   const Tile3 T = tile3(a);
   const Dm &d = a.d;
-  const wrfmi_state &s = a.s;
+  const DState &s = dstate(a);
   const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
   const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
   const int k = d.kds + (int)T.bz;
   if (i > d.ide - 1 || j > d.jde - 1 || k > d.kde - 1) return;
-  const int o2 = X2(d, i, j), o = X3(d, i, k, j);
-  const float *u = s.u_2, *u_1 = s.u_save, *v = s.v_2, *v_1 = s.v_save;
+  const Off o2 = X2(d, i, j), o = X3(d, i, k, j);
+  const FPtr<float> u = s.u_2, u_1 = s.u_save, v = s.v_2, v_1 = s.v_save;
   const float msftx = s.msftx[o2], msfty = s.msfty[o2], rdx = s.rdx, rdy = s.rdy;
   const float muu0 = s.muu[o2], muu1 = s.muu[X2(d, i + 1, j)], muv0 = s.muv[o2], muv1 = s.muv[X2(d, i, j + 1)];
   const float msfuy0 = s.msfuy[o2], msfuy1 = s.msfuy[X2(d, i + 1, j)];
@@ -314,15 +314,15 @@ template <bool COMP>
 __global__ void __launch_bounds__(256) k_ac_mu_ww(KArgs a) {
   const Tile3 T = tile3(a);
   const Dm &d = a.d;
-  const wrfmi_state &s = a.s;
+  const DState &s = dstate(a);
   const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
   const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
   if (i > d.ide - 1 || j > d.jde - 1) return;
   const int k_end = d.kde - 1;
-  const int o2 = X2(d, i, j);
-  const int ob = X3(d, i, 0, j);  // level k of the column is at ob + k*ni
-  const float *dvp = a.scr_dv, *dnw = s.dnw, *c1h = s.c1h, *ww1 = s.ww1;
-  float *ww = s.ww;
+  const Off o2 = X2(d, i, j);
+  const Off ob = X3(d, i, 0, j);  // level k of the column is at ob + k*ni
+  const FPtr<float> dvp = a.scr_dv, dnw = s.dnw, c1h = s.c1h, ww1 = s.ww1;
+  FPtr<float> ww = s.ww;
   const int ni = d.ni, x1 = -d.kms;
   const float dts = a.dts, epssm = a.c.epssm, msfty = s.msfty[o2];
   float dmdt = 0.f, comp = 0.f;
@@ -358,19 +358,19 @@ __global__ void __launch_bounds__(256) k_ac_mu_ww(KArgs a) {
 __global__ void __launch_bounds__(256) k_ac_theta(KArgs a) {
   const Tile3 T = tile3(a);
   const Dm &d = a.d;
-  const wrfmi_state &s = a.s;
+  const DState &s = dstate(a);
   const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
   const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
   const int k = d.kds + (int)T.bz;
   const int kde = d.kde, k_end = kde - 1;
   if (i > d.ide - 1 || j > d.jde - 1 || k > kde) return;
   const wrfmi_config &c = a.c;
-  const int o2 = X2(d, i, j), o = X3(d, i, k, j);
+  const Off o2 = X2(d, i, j), o = X3(d, i, k, j);
   const float dts = a.dts, epssm = c.epssm;
   const float msftx = s.msftx[o2], msfty = s.msfty[o2], rdx = s.rdx, rdy = s.rdy;
-  const float *ww = s.ww, *t_1 = s.t_save;
+  const FPtr<float> ww = s.ww, t_1 = s.t_save;
   if (k <= k_end) {
-    const float *u = s.u_2, *v = s.v_2;
+    const FPtr<float> u = s.u_2, v = s.v_2;
     const float tv = s.t_2[o];
     float t = tv + msfty * dts * s.t_tend[o];
     const float wdtn_hi = (k + 1 <= k_end) ? ww[o + d.ni] * (s.fnm[X1(d, k + 1)] * t_1[o + d.ni] + s.fnp[X1(d, k + 1)] * t_1[o]) : 0.f;
@@ -392,10 +392,10 @@ __global__ void __launch_bounds__(256) k_ac_theta(KArgs a) {
   }
   if (!c.non_hydrostatic || k < 2) return;
   // rhs(k), k = 2..kde (advance_w 1324-1371)
-  const float *ph_1 = s.ph_save, *phb = s.phb;
+  const FPtr<float> ph_1 = s.ph_save, phb = s.phb;
   auto wdwn_at = [&](int kp1) -> float {  // wdwn(k+1) for k = kp1-1
     const int kk = kp1 - 1;
-    const int oo = X3(d, i, kk, j);
+    const Off oo = X3(d, i, kk, j);
     if (c.phi_adv_z == 2)
       return s.rdnw[X1(d, kk)] * (ph_1[oo + d.ni] - ph_1[oo] + phb[oo + d.ni] - phb[oo]);
     return .5f * (ww[oo + d.ni] + ww[oo]) * s.rdnw[X1(d, kk)] * (ph_1[oo + d.ni] - ph_1[oo] + phb[oo + d.ni] - phb[oo]);
@@ -417,7 +417,7 @@ __global__ void __launch_bounds__(256) k_ac_theta(KArgs a) {
 __global__ void __launch_bounds__(256) k_ac_wpred(KArgs a) {
   const Tile3 T = tile3(a);
   const Dm &d = a.d;
-  const wrfmi_state &s = a.s;
+  const DState &s = dstate(a);
   const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
   const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
   const int k = d.kds + (int)T.bz;
@@ -425,12 +425,12 @@ __global__ void __launch_bounds__(256) k_ac_wpred(KArgs a) {
   if (i > d.ide - 1 || j > d.jde - 1 || k > kde) return;
   const wrfmi_config &c = a.c;
   if (!c.non_hydrostatic) return;
-  const int o2 = X2(d, i, j), o = X3(d, i, k, j);
+  const Off o2 = X2(d, i, j), o = X3(d, i, k, j);
   const float dts = a.dts, epssm = c.epssm;
   const float msftx = s.msftx[o2], msfty = s.msfty[o2], rdx = s.rdx, rdy = s.rdy;
-  float *w = s.w_2;
+  FPtr<float> w = s.w_2;
   if (k == 1) {
-    const float *u = s.u_2, *v = s.v_2, *ht = s.ht;
+    const FPtr<float> u = s.u_2, v = s.v_2, ht = s.ht;
     const float cf1 = s.cf1, cf2 = s.cf2, cf3 = s.cf3;
     w[o] = msfty * .5f * rdy *
                ((ht[X2(d, i, j + 1)] - ht[o2]) * (cf1 * v[X3(d, i, 1, j + 1)] + cf2 * v[X3(d, i, 2, j + 1)] + cf3 * v[X3(d, i, 3, j + 1)]) +
@@ -440,8 +440,8 @@ __global__ void __launch_bounds__(256) k_ac_wpred(KArgs a) {
                 (ht[o2] - ht[X2(d, i - 1, j)]) * (cf1 * u[X3(d, i, 1, j)] + cf2 * u[X3(d, i, 2, j)] + cf3 * u[X3(d, i, 3, j)]));
     return;
   }
-  const float *ph = s.ph_2, *rhs = a.scr_rhs, *c2a = s.c2a, *alt = s.alt, *t2ave = s.t_2save;
-  const float *c1h = s.c1h, *c2h = s.c2h, *c1f = s.c1f;
+  const FPtr<float> ph = s.ph_2, rhs = a.scr_rhs, c2a = s.c2a, alt = s.alt, t2ave = s.t_2save;
+  const FPtr<float> c1h = s.c1h, c2h = s.c2h, c1f = s.c1f;
   const float mut = s.mut[o2], muave = s.muave[o2];
   const float msft_inv = 1.f / msfty;
   const float rhs_m1 = k >= 3 ? rhs[o - d.ni] : 0.f;  // rhs(1) = 0
@@ -476,23 +476,24 @@ template <typename CT>
 __global__ void __launch_bounds__(256) k_ac_wsolve(KArgs a) {
   const Tile3 T = tile3(a);
   const Dm &d = a.d;
-  const wrfmi_state &s = a.s;
+  const DState &s = dstate(a);
   const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
   const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
   const int kde = d.kde, k_end = kde - 1;
   if (i > d.ide - 1 || j > d.jde - 1) return;
   if (!a.c.non_hydrostatic) return;
-  float *w = s.w_2;
-  float *lo = a.scr_lo;
-  const CT *A = (const CT *)s.a, *AL = (const CT *)s.alpha, *GA = (const CT *)s.gamma;
-  const int ob = X3(d, i, 0, j), ni = d.ni;
+  FPtr<float> w = s.w_2;
+  FPtr<float> lo = a.scr_lo;
+  const FPtr<CT> A((CT *)s.a), AL((CT *)s.alpha), GA((CT *)s.gamma);
+  const Off ob = X3(d, i, 0, j);
+  const int ni = d.ni;
   CT wfe = (CT)w[ob + ni];
   // forward elimination: fields 0 = w (predictor), 1 = a, 2 = alpha
   column_march_up<3, CT>(
       2, k_end + 1,
-      [&](int f, int k) -> CT { const int o = ob + k * ni; return f == 0 ? (CT)w[o] : f == 1 ? A[o] : AL[o]; },
+      [&](int f, int k) -> CT { const Off o = ob + k * ni; return f == 0 ? (CT)w[o] : f == 1 ? A[o] : AL[o]; },
       [&](int k, const CT *v) {
-        const int o = ob + k * ni;
+        const Off o = ob + k * ni;
         if (sizeof(CT) == sizeof(float)) {
           const float wf = ((float)v[0] - (float)v[1] * (float)wfe) * (float)v[2];
           wfe = (CT)wf;
@@ -508,13 +509,13 @@ __global__ void __launch_bounds__(256) k_ac_wsolve(KArgs a) {
   column_march_down<2, CT>(
       2, k_end,
       [&](int f, int k) -> CT {
-        const int o = ob + k * ni;
+        const Off o = ob + k * ni;
         if (f == 1) return GA[o];
         if (sizeof(CT) == sizeof(float)) return (CT)w[o];
         return (CT)w[o] + (CT)lo[o];
       },
       [&](int k, const CT *v) {
-        const int o = ob + k * ni;
+        const Off o = ob + k * ni;
         float wk;
         if (sizeof(CT) == sizeof(float)) {
           wk = (float)v[0] - (float)v[1] * (float)wup;
@@ -533,7 +534,7 @@ __global__ void __launch_bounds__(256) k_ac_wsolve(KArgs a) {
 __global__ void __launch_bounds__(256) k_ac_finish(KArgs a) {
   const Tile3 T = tile3(a);
   const Dm &d = a.d;
-  const wrfmi_state &s = a.s;
+  const DState &s = dstate(a);
   const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
   const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
   const int k = d.kds + (int)T.bz;
@@ -541,12 +542,12 @@ __global__ void __launch_bounds__(256) k_ac_finish(KArgs a) {
   if (i > ide || j > jde || k > kde) return;
   const bool mi = i <= ide - 1, mj = j <= jde - 1, mass = mi && mj;
   const wrfmi_config &c = a.c;
-  const int o2 = X2(d, i, j), o = X3(d, i, k, j);
-  const float *c1h = s.c1h, *c2h = s.c2h, *c1f = s.c1f;
+  const Off o2 = X2(d, i, j), o = X3(d, i, k, j);
+  const FPtr<float> c1h = s.c1h, c2h = s.c2h, c1f = s.c1f;
   if (mass && c.non_hydrostatic) {
     const float dts = a.dts, epssm = c.epssm, msfty = s.msfty[o2], muts = s.muts[o2];
-    const float *rhs = a.scr_rhs, *w = s.w_2;
-    auto ph_at = [&](int kk, int oo) -> float {  // final ph at level kk >= 2
+    const FPtr<float> rhs = a.scr_rhs, w = s.w_2;
+    auto ph_at = [&](int kk, Off oo) -> float {  // final ph at level kk >= 2
       return rhs[oo] + msfty * .5f * dts * K_G * (1.f + epssm) * w[oo] / (c1f[X1(d, kk)] * muts + s.c2f[X1(d, kk)]);
     };
     const float ph0 = k >= 2 ? ph_at(k, o) : s.ph_2[o];
@@ -590,17 +591,17 @@ __global__ void __launch_bounds__(256) k_ac_finish(KArgs a) {
 __global__ void __launch_bounds__(256) k_small_finish(KArgs a) {
   const Tile3 T = tile3(a);
   const Dm &d = a.d;
-  const wrfmi_state &s = a.s;
+  const DState &s = dstate(a);
   const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
   const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
   const int k = d.kds + (int)T.bz;
   const int ide = d.ide, jde = d.jde, kde = d.kde;
   if (i > ide || j > jde || k > kde) return;
   const bool mi = i <= ide - 1, mj = j <= jde - 1, mass = mi && mj;
-  const float *c1h = s.c1h, *c2h = s.c2h, *c1f = s.c1f, *c2f = s.c2f;
-  const float *muts = s.muts;
-  const int o2 = X2(d, i, j);
-  const int o = X3(d, i, k, j);
+  const FPtr<float> c1h = s.c1h, c2h = s.c2h, c1f = s.c1f, c2f = s.c2f;
+  const FPtr<float> muts = s.muts;
+  const Off o2 = X2(d, i, j);
+  const Off o = X3(d, i, k, j);
   const bool top = k == d.kds;
   // calc_mu_uv_1(muts -> muus, muvs)
   float muus = 0.f, muvs = 0.f;

@@ -32,7 +32,7 @@ DEVI Geo geo(const wrfmi_config &c) {
 __global__ void __launch_bounds__(256) k_tend_u(KArgs a) {
   const Tile3 T = tile3(a);
   const Dm &d = a.d;
-  const wrfmi_state &s = a.s;
+  const DState &s = dstate(a);
   const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
   const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
   const int k = d.kds + (int)T.bz;
@@ -42,7 +42,7 @@ __global__ void __launch_bounds__(256) k_tend_u(KArgs a) {
   const Geo g = geo(a.c);
   const float ts = a.c.time_step >= 0 ? 1.f : -1.f;
   const int ktf = kte - 1;
-  const float *u = s.u_2, *ru = s.ru, *rv = s.rv, *rom = s.ww, *rw = s.rw;
+  const FPtr<float> u = s.u_2, ru = s.ru, rv = s.rv, rom = s.ww, rw = s.rw;
   const float rdx = s.rdx, rdy = s.rdy;
   const bool dxs = !(g.per_x || g.sym_xs || its > ids + 3), dxe = !(g.per_x || g.sym_xe || ite < ide - 2);
   const bool dys = !(g.per_y || g.sym_ys || jts > jds + 3), dye = !(g.per_y || g.sym_ye || jte < jde - 4);
@@ -82,7 +82,7 @@ __global__ void __launch_bounds__(256) k_tend_u(KArgs a) {
   if (g.open_xs || g.spec || g.nested) ik0 = imax(ids + 1, its);
   if (g.open_xe || g.spec || g.nested) ik1 = imin(ide - 1, ite);
   if (g.per_x) { ik0 = its; ik1 = ite; }
-  const float *msfvx = s.msfvx, *msfuy = s.msfuy, *vv = s.v_2;
+  const FPtr<float> msfvx = s.msfvx, msfuy = s.msfuy, vv = s.v_2;
   auto vx_raw = [&](int ii, int k, int jj) -> float {
     return 0.5f * (u[X3(d, ii, k, jj)] + u[X3(d, ii + 1, k, jj)]) * (msfvx[X2(d, ii, jj + 1)] - msfvx[X2(d, ii, jj)]) * rdy -
            0.5f * (vv[X3(d, ii, k, jj)] + vv[X3(d, ii, k, jj + 1)]) * (msfuy[X2(d, ii + 1, jj)] - msfuy[X2(d, ii, jj)]) * rdx;
@@ -135,7 +135,7 @@ __global__ void __launch_bounds__(256) k_tend_u(KArgs a) {
     return vel * flux3(u[X3(d, i, k - 2, j)], u[X3(d, i, k - 1, j)], u[X3(d, i, k, j)], u[X3(d, i, k + 1, j)], -vel, ts);
   };
   const bool row_ok = j <= imin(jte, jde - 1);
-  const float *p = s.p, *ph = s.ph_2, *al = s.al, *alt = s.alt, *pb = s.pb, *php = s.php, *mu = s.mu_2;
+  const FPtr<float> p = s.p, ph = s.ph_2, al = s.al, alt = s.alt, pb = s.pb, php = s.php, mu = s.mu_2;
   const float cf1 = s.cf1, cf2 = s.cf2, cf3 = s.cf3, cfn = s.cfn, cfn1 = s.cfn1;
   const bool nh = a.c.non_hydrostatic;
   auto dpn = [&](int k) -> float {  // vertically interpolated p at w level k, averaged to the u point
@@ -149,7 +149,7 @@ __global__ void __launch_bounds__(256) k_tend_u(KArgs a) {
     return .5f * (s.fnm[X1(d, k)] * (p[X3(d, i - 1, k, j)] + p[X3(d, i, k, j)]) + s.fnp[X1(d, k)] * (p[X3(d, i - 1, k - 1, j)] + p[X3(d, i, k - 1, j)]));
   };
   {
-    const int o = X3(d, i, k, j);
+    const Off o = X3(d, i, k, j);
     float t = 0.f;
     float vf_hi = 0.f, dpn_hi = 0.f;
     if (row_ok && k <= ktf) {
@@ -252,7 +252,7 @@ __global__ void __launch_bounds__(256) k_tend_u(KArgs a) {
 __global__ void __launch_bounds__(256) k_tend_v(KArgs a) {
   const Tile3 T = tile3(a);
   const Dm &d = a.d;
-  const wrfmi_state &s = a.s;
+  const DState &s = dstate(a);
   const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
   const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
   const int k = d.kds + (int)T.bz;
@@ -262,7 +262,7 @@ __global__ void __launch_bounds__(256) k_tend_v(KArgs a) {
   const Geo g = geo(a.c);
   const float ts = a.c.time_step >= 0 ? 1.f : -1.f;
   const int ktf = kte - 1;
-  const float *v = s.v_2, *ru = s.ru, *rv = s.rv, *rom = s.ww, *rw = s.rw;
+  const FPtr<float> v = s.v_2, ru = s.ru, rv = s.rv, rom = s.ww, rw = s.rw;
   const float rdx = s.rdx, rdy = s.rdy;
   const bool dxs = !(g.per_x || g.sym_xs || its > ids + 3), dxe = !(g.per_x || g.sym_xe || ite < ide - 3);
   const bool dys = !(g.per_y || g.sym_ys || jts > jds + 3), dye = !(g.per_y || g.sym_ye || jte < jde - 3);
@@ -295,7 +295,7 @@ __global__ void __launch_bounds__(256) k_tend_v(KArgs a) {
   const bool cpx_e = (g.open_xe || (g.spec && !g.per_x) || g.nested) && ite == ide;
   const bool cpy_s = (g.open_ys || g.spec || g.nested || g.polar) && jts == jds;
   const bool cpy_e = (g.open_ye || g.spec || g.nested || g.polar) && jte == jde;
-  const float *msfvx = s.msfvx, *msfuy = s.msfuy, *uu = s.u_2;
+  const FPtr<float> msfvx = s.msfvx, msfuy = s.msfuy, uu = s.u_2;
   auto vx_raw = [&](int ii, int k, int jj) -> float {
     return 0.5f * (uu[X3(d, ii, k, jj)] + uu[X3(d, ii + 1, k, jj)]) * (msfvx[X2(d, ii, jj + 1)] - msfvx[X2(d, ii, jj)]) * rdy -
            0.5f * (v[X3(d, ii, k, jj)] + v[X3(d, ii, k, jj + 1)]) * (msfuy[X2(d, ii + 1, jj)] - msfuy[X2(d, ii, jj)]) * rdx;
@@ -347,7 +347,7 @@ __global__ void __launch_bounds__(256) k_tend_v(KArgs a) {
     const float vel = 0.5f * (rom[X3(d, i, k, j)] + rom[X3(d, i, k, j - 1)]);
     return vel * flux3(v[X3(d, i, k - 2, j)], v[X3(d, i, k - 1, j)], v[X3(d, i, k, j)], v[X3(d, i, k + 1, j)], -vel, ts);
   };
-  const float *p = s.p, *ph = s.ph_2, *al = s.al, *alt = s.alt, *pb = s.pb, *php = s.php, *mu = s.mu_2;
+  const FPtr<float> p = s.p, ph = s.ph_2, al = s.al, alt = s.alt, pb = s.pb, php = s.php, mu = s.mu_2;
   const float cf1 = s.cf1, cf2 = s.cf2, cf3 = s.cf3, cfn = s.cfn, cfn1 = s.cfn1;
   const bool nh = a.c.non_hydrostatic;
   auto dpn = [&](int k) -> float {
@@ -361,7 +361,7 @@ __global__ void __launch_bounds__(256) k_tend_v(KArgs a) {
     return .5f * (s.fnm[X1(d, k)] * (p[X3(d, i, k, j - 1)] + p[X3(d, i, k, j)]) + s.fnp[X1(d, k)] * (p[X3(d, i, k - 1, j - 1)] + p[X3(d, i, k - 1, j)]));
   };
   {
-    const int o = X3(d, i, k, j);
+    const Off o = X3(d, i, k, j);
     float t = 0.f, vf_hi = 0.f, dpn_hi = 0.f;
     if (col_ok && k <= ktf) {
       const float vf_lo = (j >= jv0 && j <= jv1) ? vflux(k) : 0.f;
@@ -453,7 +453,7 @@ __global__ void __launch_bounds__(256) k_tend_v(KArgs a) {
 __global__ void __launch_bounds__(256) k_tend_w(KArgs a) {
   const Tile3 T = tile3(a);
   const Dm &d = a.d;
-  const wrfmi_state &s = a.s;
+  const DState &s = dstate(a);
   const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
   const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
   const int k = d.kds + (int)T.bz;
@@ -463,8 +463,8 @@ __global__ void __launch_bounds__(256) k_tend_w(KArgs a) {
   const Geo g = geo(a.c);
   const float ts = a.c.time_step >= 0 ? 1.f : -1.f;
   const int ktf = kte - 1;
-  const float *w = s.w_2, *ru = s.ru, *rv = s.rv, *rom = s.ww;
-  const float *fzm = s.fnm, *fzp = s.fnp;
+  const FPtr<float> w = s.w_2, ru = s.ru, rv = s.rv, rom = s.ww;
+  const FPtr<float> fzm = s.fnm, fzp = s.fnp;
   const float rdx = s.rdx, rdy = s.rdy;
   const bool mass = i <= imin(ite, ide - 1) && j <= imin(jte, jde - 1);
   const bool dxs = !(g.per_x || g.sym_xs || its > ids + 3), dxe = !(g.per_x || g.sym_xe || ite < ide - 3);
@@ -476,7 +476,7 @@ __global__ void __launch_bounds__(256) k_tend_w(KArgs a) {
   if (dxs) { ix0 = imax(ids + 1, its); if0 = imin(ix0 + 2, ids + 3); }
   if (dxe) { ix1 = imin(ide - 2, ite); if1 = ide - 3; }
   // face velocity at w level k (k <= ktf) or at the lid (k = ktf+1)
-  auto wvel = [&](const float *rr, int ii, int k, int jj) -> float {
+  auto wvel = [&](FPtr<float> rr, int ii, int k, int jj) -> float {
     if (k <= ktf) return fzm[X1(d, k)] * rr[X3(d, ii, k, jj)] + fzp[X1(d, k)] * rr[X3(d, ii, k - 1, jj)];
     return (2.f - fzm[X1(d, k - 1)]) * rr[X3(d, ii, k - 1, jj)] - fzp[X1(d, k - 1)] * rr[X3(d, ii, k - 2, jj)];
   };
@@ -514,7 +514,7 @@ __global__ void __launch_bounds__(256) k_tend_w(KArgs a) {
   const float mu = mass ? s.mu_2[X2(d, i, j)] : 0.f, mub = mass ? s.mub[X2(d, i, j)] : 0.f;
   const float mut = mass ? s.mut[X2(d, i, j)] : 0.f;
   {
-    const int o = X3(d, i, k, j);
+    const Off o = X3(d, i, k, j);
     float t = 0.f;
     if (mass) {
       if (k >= kts + 1) {
@@ -591,7 +591,7 @@ __global__ void __launch_bounds__(256) k_tend_w(KArgs a) {
           // (module_big_step_utilities_em.F:787-906) from the moist species.
           float cqw_top = 0.f;
           if (s.n_moist >= 1) {
-            const int ot = X3(d, i, kde - 1, j);
+            const Off ot = X3(d, i, kde - 1, j);
             float qtot = 0.f;
             for (int m = 0; m < s.n_moist; m++) qtot = qtot + s.q[m][ot] + s.q[m][ot - d.ni];
             cqw_top = 0.5f * qtot;
@@ -623,7 +623,7 @@ __global__ void __launch_bounds__(256) k_tend_w(KArgs a) {
           t = t + s.e[X2(d, i, j)] *
                       (s.cosa[X2(d, i, j)] * 0.5f * (fm * (ru[o] + ru[o + 1]) + fp * (ru[o - d.ni] + ru[o - d.ni + 1])) -
                        (s.msftx[X2(d, i, j)] / msfty) * s.sina[X2(d, i, j)] * 0.5f * (fm * (rv[o] + rv[o + d.sj]) + fp * (rv[o - d.ni] + rv[o - d.ni + d.sj])));
-          const float *uu = s.u_2, *vv = s.v_2;
+          const FPtr<float> uu = s.u_2, vv = s.v_2;
           t = t + K_RERADIUS * (0.5f * (fm * (ru[o] + ru[o + 1]) + fp * (ru[o - d.ni] + ru[o - d.ni + 1])) * 0.5f *
                                     (fm * (uu[o] + uu[o + 1]) + fp * (uu[o - d.ni] + uu[o - d.ni + 1])) +
                                 (s.msftx[X2(d, i, j)] / msfty) * 0.5f * (fm * (rv[o] + rv[o + d.sj]) + fp * (rv[o - d.ni] + rv[o - d.ni + d.sj])) *
@@ -646,7 +646,7 @@ __global__ void __launch_bounds__(256) k_tend_w(KArgs a) {
 __global__ void __launch_bounds__(256) k_tend_t(KArgs a) {
   const Tile3 T = tile3(a);
   const Dm &d = a.d;
-  const wrfmi_state &s = a.s;
+  const DState &s = dstate(a);
   const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
   const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
   const int k = d.kds + (int)T.bz;
@@ -656,7 +656,7 @@ __global__ void __launch_bounds__(256) k_tend_t(KArgs a) {
   const Geo g = geo(a.c);
   const float ts = a.c.time_step >= 0 ? 1.f : -1.f;
   const int ktf = kte - 1;
-  const float *f = s.t_2, *ru = s.ru, *rv = s.rv, *rom = s.ww;
+  const FPtr<float> f = s.t_2, ru = s.ru, rv = s.rv, rom = s.ww;
   const float rdx = s.rdx, rdy = s.rdy;
   const bool mass = i <= imin(ite, ide - 1) && j <= imin(jte, jde - 1);
   const bool dxs = !(g.per_x || g.sym_xs || its > ids + 3), dxe = !(g.per_x || g.sym_xe || ite < ide - 3);
@@ -699,7 +699,7 @@ __global__ void __launch_bounds__(256) k_tend_t(KArgs a) {
     return vel * flux3(f[X3(d, i, k - 2, j)], f[X3(d, i, k - 1, j)], f[X3(d, i, k, j)], f[X3(d, i, k + 1, j)], -vel, ts);
   };
   // rhs_ph pieces
-  const float *u = s.u_2, *v = s.v_2, *ph = s.ph_2, *phb = s.phb, *w = s.w_2;
+  const FPtr<float> u = s.u_2, v = s.v_2, ph = s.ph_2, phb = s.phb, w = s.w_2;
   const float cfn = s.cfn, cfn1 = s.cfn1;
   const bool spec = g.spec;
   const bool ord6 = a.c.h_sca_adv_order <= 6 && a.c.h_sca_adv_order > 4;
@@ -719,7 +719,7 @@ __global__ void __launch_bounds__(256) k_tend_t(KArgs a) {
     if (k < kte) return cc * (u[X3(d, ii, k, j)] + u[X3(d, ii, k - 1, j)]) * s.msfux[X2(d, ii, j)];
     return cc * (cfn * u[X3(d, ii, k - 1, j)] + cfn1 * u[X3(d, ii, k - 2, j)]) * s.msfux[X2(d, ii, j)];
   };
-  auto P = [&](const float *arr, int ii, int k, int jj) -> float { return arr[X3(d, ii, k, jj)]; };
+  auto P = [&](FPtr<float> arr, int ii, int k, int jj) -> float { return arr[X3(d, ii, k, jj)]; };
   const float msfty = mass ? s.msfty[X2(d, i, j)] : 1.f;
   const float mut = mass ? s.mut[X2(d, i, j)] : 0.f;
   // rhs_ph's wdwn at w level kk (kk = 2..kde-1)
@@ -730,7 +730,7 @@ __global__ void __launch_bounds__(256) k_tend_t(KArgs a) {
            (P(ph, i, kk, j) - P(ph, i, kk - 1, j) + P(phb, i, kk, j) - P(phb, i, kk - 1, j));
   };
   {
-    const int o = X3(d, i, k, j);
+    const Off o = X3(d, i, k, j);
     // zero_tend of the *_save arrays over the whole tile
     s.u_save[o] = 0.f;
     s.v_save[o] = 0.f;

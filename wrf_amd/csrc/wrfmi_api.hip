@@ -462,9 +462,9 @@ static int prepare(const wrfmi_state *st_in, const wrfmi_config *cfg, const wrfm
   L.a.d = d;
   Scratch sc;
   if ((rc = get_scratch((size_t)d.sj * (size_t)(d.jme - d.jms + 1), &sc))) return rc;
-  L.a.scr_dv = sc.dv;
-  L.a.scr_rhs = sc.rhs;
-  L.a.scr_lo = sc.lo;
+  L.a.scr_dv = FPtr<float>(sc.dv);
+  L.a.scr_rhs = FPtr<float>(sc.rhs);
+  L.a.scr_lo = FPtr<float>(sc.lo);
   L.h.d = d;
   L.h.periodic_x = cfg->periodic_x; L.h.periodic_y = cfg->periodic_y;
   L.h.open_xs = cfg->open_xs; L.h.open_xe = cfg->open_xe; L.h.open_ys = cfg->open_ys; L.h.open_ye = cfg->open_ye;
@@ -680,10 +680,10 @@ __global__ void k_count_nonfinite(const float *a, Dm d, int kend, int is3d, unsi
   unsigned long long n = 0;
   if (is3d) {
     for (int k = d.kds; k <= kend; k++) {
-      float v = a[X3(d, i, k, j)];
+      float v = a[X3i(d, i, k, j)];
       if (!isfinite(v)) n++;
     }
-  } else if (!isfinite(a[X2(d, i, j)])) {
+  } else if (!isfinite(a[X2i(d, i, j)])) {
     n = 1;
   }
   if (n) atomicAdd(cnt, n);
