@@ -126,6 +126,27 @@ DEVI const DState &dstate(const KArgs &a) { return *reinterpret_cast<const DStat
 
 DEVI Tile3 tile3(const KArgs &a) { return tile3_of(blockIdx.x, a.tnx, a.tny, a.tnz); }
 
+// Lateral-boundary flags of the whole-patch tile (its = ids, ite = ide, ...).  The host
+// accepts periodic or open boundaries only, so symmetric, specified, nested and polar are
+// compile-time false here; PER = true is the doubly periodic specialisation, in which every
+// open-boundary and degraded-order branch is compiled out.
+struct Geo {
+  bool per_x, per_y, sym_xs, sym_xe, sym_ys, sym_ye, open_xs, open_xe, open_ys, open_ye, spec, nested, polar;
+};
+template <bool PER>
+DEVI Geo geo(const wrfmi_config &c) {
+  Geo g;
+  g.per_x = PER ? true : (bool)c.periodic_x;
+  g.per_y = PER ? true : (bool)c.periodic_y;
+  g.sym_xs = g.sym_xe = g.sym_ys = g.sym_ye = false;
+  g.open_xs = PER ? false : (bool)c.open_xs;
+  g.open_xe = PER ? false : (bool)c.open_xe;
+  g.open_ys = PER ? false : (bool)c.open_ys;
+  g.open_ye = PER ? false : (bool)c.open_ye;
+  g.nested = g.spec = g.polar = false;
+  return g;
+}
+
 template <typename T> DEVI T imin(T a, T b) { return a < b ? a : b; }
 template <typename T> DEVI T imax(T a, T b) { return a > b ? a : b; }
 

@@ -11,6 +11,7 @@ namespace wrfmi {
 
 // One thread per (i, k, j) point: the horizontal fluxes at both faces in x and y and the
 // vertical fluxes at both w levels of the point (the column form carried the lower one).
+template <bool PER>
 __global__ void __launch_bounds__(256) k_adv_scalar(KArgs a) {
   const Tile3 T = tile3(a);
   const Dm &d = a.d;
@@ -22,6 +23,7 @@ __global__ void __launch_bounds__(256) k_adv_scalar(KArgs a) {
   const int its = ids, ite = ide, jts = jds, jte = jde, kts = d.kds, kte = kde;
   if (i > ide || j > jde || k > kte) return;
   const wrfmi_config &c = a.c;
+  const Geo g = geo<PER>(c);
   const float ts = c.time_step >= 0 ? 1.f : -1.f;
   const int ktf = kte - 1;
   const FPtr<float> f = s.q[a.species], ru = s.ru_m, rv = s.rv_m, rom = s.ww_m;
@@ -32,8 +34,8 @@ __global__ void __launch_bounds__(256) k_adv_scalar(KArgs a) {
     tend[X3(d, i, k, j)] = 0.f;
     return;
   }
-  const bool dxs = !(c.periodic_x || c.symmetric_xs || its > ids + 3), dxe = !(c.periodic_x || c.symmetric_xe || ite < ide - 3);
-  const bool dys = !(c.periodic_y || c.symmetric_ys || jts > jds + 3), dye = !(c.periodic_y || c.symmetric_ye || jte < jde - 4);
+  const bool dxs = !(g.per_x || g.sym_xs || its > ids + 3), dxe = !(g.per_x || g.sym_xe || ite < ide - 3);
+  const bool dys = !(g.per_y || g.sym_ys || jts > jds + 3), dye = !(g.per_y || g.sym_ye || jte < jde - 4);
   int jy0 = jts, jy1 = imin(jte, jde - 1), jf0 = jy0, jf1 = jy1 + 1;
   if (dys) { jy0 = imax(jts, jds + 1); jf0 = jds + 3; }
   if (dye) { jy1 = imin(jte, jde - 2); jf1 = jde - 3; }
@@ -77,22 +79,22 @@ __global__ void __launch_bounds__(256) k_adv_scalar(KArgs a) {
     if (k <= ktf) {
       if (j >= jy0 && j <= jy1) t = t - s.msftx[X2(d, i, j)] * rdy * (fqy(k, j + 1) - fqy(k, j));
       if (i >= ix0 && i <= ix1) t = t - s.msftx[X2(d, i, j)] * rdx * (fqx(k, i + 1) - fqx(k, i));
-      if (c.open_xs && i == its) {
+      if (g.open_xs && i == its) {
         float ub = 0.5f * (P(ru, its, k, j) + P(ru, its + 1, k, j));
         if (ub > 0.f) ub = 0.f;
         t = t - rdx * (ub * (P(f, its + 1, k, j) - P(f, its, k, j)) + P(f, its, k, j) * (P(ru, its + 1, k, j) - P(ru, its, k, j)));
       }
-      if (c.open_xe && i == ide - 1) {
+      if (g.open_xe && i == ide - 1) {
         float ub = 0.5f * (P(ru, ite - 1, k, j) + P(ru, ite, k, j));
         if (ub < 0.f) ub = 0.f;
         t = t - rdx * (ub * (P(f, i, k, j) - P(f, i - 1, k, j)) + P(f, i, k, j) * (P(ru, ite, k, j) - P(ru, ite - 1, k, j)));
       }
-      if (c.open_ys && j == jts) {
+      if (g.open_ys && j == jts) {
         float vb = 0.5f * (P(rv, i, k, jts) + P(rv, i, k, jts + 1));
         if (vb > 0.f) vb = 0.f;
         t = t - rdy * (vb * (P(f, i, k, jts + 1) - P(f, i, k, jts)) + P(f, i, k, jts) * (P(rv, i, k, jts + 1) - P(rv, i, k, jts)));
       }
-      if (c.open_ye && j == jde - 1) {
+      if (g.open_ye && j == jde - 1) {
         float vb = 0.5f * (P(rv, i, k, jte - 1) + P(rv, i, k, jte));
         if (vb < 0.f) vb = 0.f;
         t = t - rdy * (vb * (P(f, i, k, j) - P(f, i, k, j - 1)) + P(f, i, k, j) * (P(rv, i, k, jte) - P(rv, i, k, jte - 1)));
@@ -234,5 +236,8 @@ __global__ void __launch_bounds__(256) k_halo(HaloArgs h) {
     F.p[X2i(d, i, j)] = F.p[X2i(d, si, sj)];
   }
 }
+
+template __global__ void k_adv_scalar<false>(KArgs);
+template __global__ void k_adv_scalar<true>(KArgs);
 
 }  // namespace wrfmi

@@ -180,6 +180,7 @@ __global__ void __launch_bounds__(256) k_coef_w(KArgs a) {
   }
 }
 
+template <bool PER>
 __global__ void __launch_bounds__(256) k_advance_uv(KArgs a) {
   const Tile3 T = tile3(a);
   const Dm &d = a.d;
@@ -191,18 +192,19 @@ __global__ void __launch_bounds__(256) k_advance_uv(KArgs a) {
   const int its = ids, ite = ide, jts = jds, jte = jde;
   if (i > ide || j > jde || k > kde - 1) return;
   const wrfmi_config &c = a.c;
+  const Geo g = geo<PER>(c);
   const int k_start = d.kds;
   const int i_start = its, i_end = imin(ite, ide - 1), j_start = jts, j_end = imin(jte, jde - 1), i_endu = ite, j_endv = jte;
   int i_start_up = i_start, i_end_up = i_endu, j_start_vp = j_start, j_end_vp = j_endv;
-  if ((c.open_xs || c.symmetric_xs) && its == ids) i_start_up++;
-  if ((c.open_xe || c.symmetric_xe) && ite == ide) i_end_up--;
-  if ((c.open_ys || c.symmetric_ys || c.polar) && jts == jds) j_start_vp++;
-  if ((c.open_ye || c.symmetric_ye || c.polar) && jte == jde) j_end_vp--;
+  if ((g.open_xs || g.sym_xs) && its == ids) i_start_up++;
+  if ((g.open_xe || g.sym_xe) && ite == ide) i_end_up--;
+  if ((g.open_ys || g.sym_ys || g.polar) && jts == jds) j_start_vp++;
+  if ((g.open_ye || g.sym_ye || g.polar) && jte == jde) j_end_vp--;
   int i_start_u_tend = i_start, i_end_u_tend = i_endu, j_start_v_tend = j_start, j_end_v_tend = j_endv;
-  if (c.symmetric_xs && its == ids) i_start_u_tend++;
-  if (c.symmetric_xe && ite == ide) i_end_u_tend--;
-  if (c.symmetric_ys && jts == jds) j_start_v_tend++;
-  if (c.symmetric_ye && jte == jde) j_end_v_tend--;
+  if (g.sym_xs && its == ids) i_start_u_tend++;
+  if (g.sym_xe && ite == ide) i_end_u_tend--;
+  if (g.sym_ys && jts == jds) j_start_v_tend++;
+  if (g.sym_ye && jte == jde) j_end_v_tend--;
   const float rdx = s.rdx, rdy = s.rdy, dx = 1.f / rdx, dy = 1.f / rdy, dts = a.dts;
   const float cf1 = s.cf1, cf2 = s.cf2, cf3 = s.cf3, emdiv = c.emdiv;
   const FPtr<float> p = s.p, pb = s.pb, ph = s.ph_2, php = s.php, alt = s.alt, al = s.al, mu = s.mu_2, mudf = s.mudf;
@@ -642,6 +644,8 @@ __global__ void __launch_bounds__(256) k_small_finish(KArgs a) {
   if (top) s.mu_2[o2] = s.mu_2[o2] + s.mu_save[o2];
 }
 
+template __global__ void k_advance_uv<false>(KArgs);
+template __global__ void k_advance_uv<true>(KArgs);
 template __global__ void k_coef_w<float>(KArgs);
 template __global__ void k_coef_w<double>(KArgs);
 template __global__ void k_ac_mu_ww<false>(KArgs);

@@ -16,19 +16,8 @@
 
 namespace wrfmi {
 
-struct Geo {  // boundary-dependent flags of the whole-patch tile (its = ids, ite = ide ...)
-  bool per_x, per_y, sym_xs, sym_xe, sym_ys, sym_ye, open_xs, open_xe, open_ys, open_ye, spec, nested, polar;
-};
-DEVI Geo geo(const wrfmi_config &c) {
-  Geo g;
-  g.per_x = c.periodic_x; g.per_y = c.periodic_y;
-  g.sym_xs = c.symmetric_xs; g.sym_xe = c.symmetric_xe; g.sym_ys = c.symmetric_ys; g.sym_ye = c.symmetric_ye;
-  g.open_xs = c.open_xs; g.open_xe = c.open_xe; g.open_ys = c.open_ys; g.open_ye = c.open_ye;
-  g.nested = c.nested; g.spec = c.specified || c.nested; g.polar = c.polar;
-  return g;
-}
-
 // ------------------------------------------------------------------------------ u
+template <bool PER>
 __global__ void __launch_bounds__(256) k_tend_u(KArgs a) {
   const Tile3 T = tile3(a);
   const Dm &d = a.d;
@@ -39,7 +28,7 @@ __global__ void __launch_bounds__(256) k_tend_u(KArgs a) {
   const int ids = d.ids, ide = d.ide, jds = d.jds, jde = d.jde, kde = d.kde;
   const int its = ids, ite = ide, jts = jds, jte = jde, kts = d.kds, kte = kde;
   if (i > ide || j > jde || k > kte) return;
-  const Geo g = geo(a.c);
+  const Geo g = geo<PER>(a.c);
   const float ts = a.c.time_step >= 0 ? 1.f : -1.f;
   const int ktf = kte - 1;
   const FPtr<float> u = s.u_2, ru = s.ru, rv = s.rv, rom = s.ww, rw = s.rw;
@@ -249,6 +238,7 @@ __global__ void __launch_bounds__(256) k_tend_u(KArgs a) {
 }
 
 // ------------------------------------------------------------------------------ v
+template <bool PER>
 __global__ void __launch_bounds__(256) k_tend_v(KArgs a) {
   const Tile3 T = tile3(a);
   const Dm &d = a.d;
@@ -259,7 +249,7 @@ __global__ void __launch_bounds__(256) k_tend_v(KArgs a) {
   const int ids = d.ids, ide = d.ide, jds = d.jds, jde = d.jde, kde = d.kde;
   const int its = ids, ite = ide, jts = jds, jte = jde, kts = d.kds, kte = kde;
   if (i > ide || j > jde || k > kte) return;
-  const Geo g = geo(a.c);
+  const Geo g = geo<PER>(a.c);
   const float ts = a.c.time_step >= 0 ? 1.f : -1.f;
   const int ktf = kte - 1;
   const FPtr<float> v = s.v_2, ru = s.ru, rv = s.rv, rom = s.ww, rw = s.rw;
@@ -450,6 +440,7 @@ __global__ void __launch_bounds__(256) k_tend_v(KArgs a) {
 }
 
 // ------------------------------------------------------------------------------ w
+template <bool PER>
 __global__ void __launch_bounds__(256) k_tend_w(KArgs a) {
   const Tile3 T = tile3(a);
   const Dm &d = a.d;
@@ -460,7 +451,7 @@ __global__ void __launch_bounds__(256) k_tend_w(KArgs a) {
   const int ids = d.ids, ide = d.ide, jds = d.jds, jde = d.jde, kde = d.kde;
   const int its = ids, ite = ide, jts = jds, jte = jde, kts = d.kds, kte = kde;
   if (i > ide || j > jde || k > kte) return;
-  const Geo g = geo(a.c);
+  const Geo g = geo<PER>(a.c);
   const float ts = a.c.time_step >= 0 ? 1.f : -1.f;
   const int ktf = kte - 1;
   const FPtr<float> w = s.w_2, ru = s.ru, rv = s.rv, rom = s.ww;
@@ -643,6 +634,7 @@ __global__ void __launch_bounds__(256) k_tend_w(KArgs a) {
 }
 
 // ------------------------------------------------------------------------------ t, ph, zeroing
+template <bool PER>
 __global__ void __launch_bounds__(256) k_tend_t(KArgs a) {
   const Tile3 T = tile3(a);
   const Dm &d = a.d;
@@ -653,7 +645,7 @@ __global__ void __launch_bounds__(256) k_tend_t(KArgs a) {
   const int ids = d.ids, ide = d.ide, jds = d.jds, jde = d.jde, kde = d.kde;
   const int its = ids, ite = ide, jts = jds, jte = jde, kts = d.kds, kte = kde;
   if (i > ide || j > jde || k > kte) return;
-  const Geo g = geo(a.c);
+  const Geo g = geo<PER>(a.c);
   const float ts = a.c.time_step >= 0 ? 1.f : -1.f;
   const int ktf = kte - 1;
   const FPtr<float> f = s.t_2, ru = s.ru, rv = s.rv, rom = s.ww;
@@ -870,5 +862,14 @@ __global__ void __launch_bounds__(256) k_tend_t(KArgs a) {
     if (mass) s.mu_tend[X2(d, i, j)] = 0.f + s.mu_tendf[X2(d, i, j)];
   }
 }
+
+template __global__ void k_tend_u<false>(KArgs);
+template __global__ void k_tend_u<true>(KArgs);
+template __global__ void k_tend_v<false>(KArgs);
+template __global__ void k_tend_v<true>(KArgs);
+template __global__ void k_tend_w<false>(KArgs);
+template __global__ void k_tend_w<true>(KArgs);
+template __global__ void k_tend_t<false>(KArgs);
+template __global__ void k_tend_t<true>(KArgs);
 
 }  // namespace wrfmi

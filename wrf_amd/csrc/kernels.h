@@ -5,14 +5,14 @@
 namespace wrfmi {
 __global__ void k_step_prep(KArgs);
 template <bool COMP> __global__ void k_ww_scan(KArgs);
-__global__ void k_tend_u(KArgs);
-__global__ void k_tend_v(KArgs);
-__global__ void k_tend_w(KArgs);
-__global__ void k_tend_t(KArgs);
+template <bool PER> __global__ void k_tend_u(KArgs);
+template <bool PER> __global__ void k_tend_v(KArgs);
+template <bool PER> __global__ void k_tend_w(KArgs);
+template <bool PER> __global__ void k_tend_t(KArgs);
 __global__ void k_small_prep(KArgs);
 __global__ void k_p_rho0(KArgs);
 template <typename CT> __global__ void k_coef_w(KArgs);
-__global__ void k_advance_uv(KArgs);
+template <bool PER> __global__ void k_advance_uv(KArgs);
 __global__ void k_ac_div(KArgs);
 template <bool COMP> __global__ void k_ac_mu_ww(KArgs);
 __global__ void k_ac_theta(KArgs);
@@ -20,7 +20,7 @@ __global__ void k_ac_wpred(KArgs);
 template <typename CT> __global__ void k_ac_wsolve(KArgs);
 __global__ void k_ac_finish(KArgs);
 __global__ void k_small_finish(KArgs);
-__global__ void k_adv_scalar(KArgs);
+template <bool PER> __global__ void k_adv_scalar(KArgs);
 __global__ void k_update_scalar(KArgs);
 __global__ void k_p_rho_phi(KArgs);
 __global__ void k_halo(HaloArgs);

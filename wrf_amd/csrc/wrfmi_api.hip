@@ -341,10 +341,11 @@ static void ph_step_prep(Launcher &L) {
 static void ph_tendency(Launcher &L) {
   const KArgs &a = L.a;
   const int nk = a.d.kde - a.d.kds + 1;
-  L.launchK(KID_TEND_U, k_tend_u, g_pt3(a.d, nk), kColBlk, a);
-  L.launchK(KID_TEND_V, k_tend_v, g_pt3(a.d, nk), kColBlk, a);
-  L.launchK(KID_TEND_W, k_tend_w, g_pt3(a.d, nk), kColBlk, a);
-  L.launchK(KID_TEND_T, k_tend_t, g_pt3(a.d, nk), kColBlk, a);
+  const bool per = a.c.periodic_x && a.c.periodic_y;
+  L.launchK(KID_TEND_U, per ? k_tend_u<true> : k_tend_u<false>, g_pt3(a.d, nk), kColBlk, a);
+  L.launchK(KID_TEND_V, per ? k_tend_v<true> : k_tend_v<false>, g_pt3(a.d, nk), kColBlk, a);
+  L.launchK(KID_TEND_W, per ? k_tend_w<true> : k_tend_w<false>, g_pt3(a.d, nk), kColBlk, a);
+  L.launchK(KID_TEND_T, per ? k_tend_t<true> : k_tend_t<false>, g_pt3(a.d, nk), kColBlk, a);
 }
 static void ph_small_prep(Launcher &L) {
   const KArgs &a = L.a;
@@ -361,7 +362,8 @@ static void ph_small_prep(Launcher &L) {
   L.halo_flush();
 }
 static void ph_advance_uv(Launcher &L) {
-  L.launchK(KID_ADVANCE_UV, k_advance_uv, g_pt3(L.a.d, L.a.d.kde - L.a.d.kds), kColBlk, L.a);
+  const bool per = L.a.c.periodic_x && L.a.c.periodic_y;
+  L.launchK(KID_ADVANCE_UV, per ? k_advance_uv<true> : k_advance_uv<false>, g_pt3(L.a.d, L.a.d.kde - L.a.d.kds), kColBlk, L.a);
 }
 static void ph_advance_mu_t_w(Launcher &L) {
   const KArgs &a = L.a;
@@ -393,7 +395,8 @@ static void ph_small_finish(Launcher &L) {
   }
 }
 static void ph_scalar(Launcher &L) {
-  L.launchK(KID_ADV_SCALAR, k_adv_scalar, g_pt3(L.a.d, L.a.d.kde - L.a.d.kds + 1), kColBlk, L.a);
+  const bool per = L.a.c.periodic_x && L.a.c.periodic_y;
+  L.launchK(KID_ADV_SCALAR, per ? k_adv_scalar<true> : k_adv_scalar<false>, g_pt3(L.a.d, L.a.d.kde - L.a.d.kds + 1), kColBlk, L.a);
   L.launchK(KID_UPDATE_SCALAR, k_update_scalar, g_pt3(L.a.d, L.a.d.kde - L.a.d.kds), kColBlk, L.a);
 }
 static void ph_p_rho_phi(Launcher &L) {
