@@ -205,8 +205,8 @@ DEVI void two_sum_acc(float &s, float &comp, float x) {
 // loads of one level do not depend on the body's results, so up to kPF levels of loads are
 // in flight while the body works through the column (a thread per column otherwise waits
 // a full memory latency per level).  load(f, k) must only read data the body never writes.
-constexpr int kPF = 8;
-template <int NF, typename T, typename Load, typename Body>
+constexpr int kPFdef = 8;
+template <int NF, typename T, int kPF = kPFdef, typename Load, typename Body>
 DEVI void column_march_up(int k0, int k1, Load &&load, Body &&body) {
   T ring[kPF][NF];
 #pragma unroll
@@ -230,7 +230,7 @@ DEVI void column_march_up(int k0, int k1, Load &&load, Body &&body) {
   }
 }
 // The same, visiting k = k1 down to k0.
-template <int NF, typename T, typename Load, typename Body>
+template <int NF, typename T, int kPF = kPFdef, typename Load, typename Body>
 DEVI void column_march_down(int k0, int k1, Load &&load, Body &&body) {
   T ring[kPF][NF];
 #pragma unroll

@@ -177,12 +177,32 @@ __global__ void __launch_bounds__(256) k_p_rho_phi(KArgs a) {
 // Value a filled halo cell ends up with after set_physical_bc's x pass followed by its
 // y pass (periodic wrap, or the open-boundary zero-gradient copy), computed directly
 // from the source cell so that all cells of all fields fill in one launch.
+// One thread per (halo cell, level, field).  The halo ring of the memory patch is
+// enumerated as: the rows below jds and from jde up (every i), then the columns left of ids
+// and from ide on for the rows jds..jde-1.
 __global__ void __launch_bounds__(256) k_halo(HaloArgs h) {
   const Dm &d = h.d;
-  const int i = d.ims + (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  const int j = d.jms + (int)blockIdx.y;
-  const HaloField F = h.f[blockIdx.z];
-  if (i > d.ime || j > d.jme) return;
+  const unsigned W = (unsigned)d.ni, B = (unsigned)(d.jds - d.jms), Tn = (unsigned)(d.jme - d.jde + 1);
+  const unsigned Lw = (unsigned)(d.ids - d.ims), LR = Lw + (unsigned)(d.ime - d.ide + 1);
+  const unsigned nring = (B + Tn) * W + (unsigned)(d.jde - d.jds) * LR, nk = (unsigned)d.kde;
+  const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nring * nk * (unsigned)h.n) return;
+  const unsigned fidx = t / (nring * nk), rem = t % (nring * nk);
+  const int k = d.kds + (int)(rem / nring);
+  unsigned c = rem % nring;
+  int i, j;
+  if (c < B * W) {
+    j = d.jms + (int)(c / W); i = d.ims + (int)(c % W);
+  } else if (c < (B + Tn) * W) {
+    c -= B * W;
+    j = d.jde + (int)(c / W); i = d.ims + (int)(c % W);
+  } else {
+    c -= (B + Tn) * W;
+    j = d.jds + (int)(c / LR);
+    const unsigned cc = c % LR;
+    i = cc < Lw ? d.ims + (int)cc : d.ide + (int)(cc - Lw);
+  }
+  const HaloField F = h.f[fidx];
   const int ids = d.ids, ide = d.ide, jds = d.jds, jde = d.jde;
   const int istag = F.kind == 1 ? 0 : -1, jstag = F.kind == 2 ? 0 : -1;
   const int kend = F.is3d ? (F.kind == 3 ? d.kde : d.kde - 1) : 1;
@@ -230,11 +250,9 @@ __global__ void __launch_bounds__(256) k_halo(HaloArgs h) {
   } else {
     return;
   }
-  if (F.is3d) {
-    for (int k = d.kds; k <= kend; k++) F.p[X3i(d, i, k, j)] = F.p[X3i(d, si, k, sj)];
-  } else {
-    F.p[X2i(d, i, j)] = F.p[X2i(d, si, sj)];
-  }
+  if (k > kend) return;
+  if (F.is3d) F.p[X3i(d, i, k, j)] = F.p[X3i(d, si, k, sj)];
+  else F.p[X2i(d, i, j)] = F.p[X2i(d, si, sj)];
 }
 
 template __global__ void k_adv_scalar<false>(KArgs);

@@ -279,18 +279,40 @@ __global__ void __launch_bounds__(256) k_advance_uv(KArgs a) {
 // ---------------------------------------------------------------------------------------
 // The column part of one acoustic substep: advance_mu_t + advance_w + sumflux + calc_p_rho,
 // split at the two column dependencies (the mass-divergence integral before the omega scan,
-// and the tridiagonal forward/backward sweeps) into point kernels and two thin column
-// kernels.  Per substep, in order:
-//   k_ac_div      (point)  dvdxi                                    -> scr_dv
-//   k_ac_mu_ww    (column) dmdt, mu_2/mudf/muts/muave, omega scan     -> ww
-//   k_ac_theta    (point)  theta update, t_2ave, advance_w's rhs      -> t_2, t_2save, scr_rhs
-//   k_ac_wpred    (point)  surface w and the explicit w predictor     -> w
-//   k_ac_wsolve   (column) tridiagonal forward elimination + back substitution -> w
-//   k_ac_finish   (point)  ph from rhs and w, calc_p_rho, sumflux
+// and the tridiagonal forward/backward sweeps) into point kernels and two column kernels.
+// Per substep, in order:
+//   k_ac_div    (point)  dvdxi -> scr_dv; sumflux of ru_m, rv_m
+//   k_ac_mu_ww  (column) dmdt, mu_2/mudf/muts/muave, omega scan -> ww
+//   k_ac_theta  (point)  theta update, t_2ave, advance_w's rhs -> t_2, t_2save, scr_rhs;
+//                        sumflux of ww_m
+//   k_ac_wpred  (point)  surface w and the explicit w predictor -> w
+//   k_ac_wsolve (column) tridiagonal forward elimination and back substitution, ph,
+//                        calc_p_rho -> w, ph, al, p, pm1
 // Each point kernel only reads what earlier kernels wrote, so a thread never depends on
 // another thread of its own launch.
 
-// dvdxi of advance_mu_t (module_small_step_em.F:1092-1099) at mass point (i, k, j)
+// sumflux's running sums at (i, k, j) (module_small_step_em.F:1473-1633) for the x and y
+// mass fluxes; ww_m at mass points is summed in k_ac_theta, once omega is final
+DEVI void sumflux_uv(const KArgs &a, const DState &s, int i, int j, int k, Off o, Off o2) {
+  const Dm &d = a.d;
+  const bool mi = i <= d.ide - 1, mj = j <= d.jde - 1, kin = k <= d.kde - 1;
+  const int it = a.iteration, ns = a.nsmall;
+  const float nsf = (float)ns;
+  float rum = it == 1 ? 0.f : s.ru_m[o];
+  float rvm = it == 1 ? 0.f : s.rv_m[o];
+  if (mj && kin) rum = rum + s.u_2[o];
+  if (mi && kin) rvm = rvm + s.v_2[o];
+  if (it == ns) {
+    if (mj && kin) rum = rum / nsf + (s.c1h[X1(d, k)] * s.muu[o2] + s.c2h[X1(d, k)]) * s.u_save[o] / s.msfuy[o2];
+    if (mi && kin) rvm = rvm / nsf + (s.c1h[X1(d, k)] * s.muv[o2] + s.c2h[X1(d, k)]) * s.v_save[o] * s.msfvx_inv[o2];
+  }
+  s.ru_m[o] = rum;
+  s.rv_m[o] = rvm;
+  if (!(mi && mj)) s.ww_m[o] = it == 1 ? 0.f : s.ww_m[o];
+}
+
+// dvdxi of advance_mu_t (module_small_step_em.F:1092-1099) at mass points, and the x/y parts
+// of sumflux; one thread per (i, k, j) with i <= ide, j <= jde, k <= kde
 __global__ void __launch_bounds__(256) k_ac_div(KArgs a) {
   const Tile3 T = tile3(a);
   const Dm &d = a.d;
@@ -298,8 +320,10 @@ __global__ void __launch_bounds__(256) k_ac_div(KArgs a) {
   const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
   const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
   const int k = d.kds + (int)T.bz;
-  if (i > d.ide - 1 || j > d.jde - 1 || k > d.kde - 1) return;
+  if (i > d.ide || j > d.jde || k > d.kde) return;
   const Off o2 = X2(d, i, j), o = X3(d, i, k, j);
+  sumflux_uv(a, s, i, j, k, o, o2);
+  if (i > d.ide - 1 || j > d.jde - 1 || k > d.kde - 1) return;
   const FPtr<float> u = s.u_2, u_1 = s.u_save, v = s.v_2, v_1 = s.v_save;
   const float msftx = s.msftx[o2], msfty = s.msfty[o2], rdx = s.rdx, rdy = s.rdy;
   const float muu0 = s.muu[o2], muu1 = s.muu[X2(d, i + 1, j)], muv0 = s.muv[o2], muv1 = s.muv[X2(d, i, j + 1)];
@@ -371,6 +395,13 @@ __global__ void __launch_bounds__(256) k_ac_theta(KArgs a) {
   const float dts = a.dts, epssm = c.epssm;
   const float msftx = s.msftx[o2], msfty = s.msfty[o2], rdx = s.rdx, rdy = s.rdy;
   const FPtr<float> ww = s.ww, t_1 = s.t_save;
+  {  // sumflux of omega (module_small_step_em.F:1473-1633) at this mass point
+    const int it = a.iteration, ns = a.nsmall;
+    float wwm = it == 1 ? 0.f : s.ww_m[o];
+    wwm = wwm + ww[o];
+    if (it == ns) wwm = wwm / (float)ns + s.ww1[o];
+    s.ww_m[o] = wwm;
+  }
   if (k <= k_end) {
     const FPtr<float> u = s.u_2, v = s.v_2;
     const float tv = s.t_2[o];
@@ -472,8 +503,9 @@ __global__ void __launch_bounds__(256) k_ac_wpred(KArgs a) {
   w[o] = wv;
 }
 
-// tridiagonal forward elimination and back substitution (advance_w 1433-1443) per column;
-// CT = double keeps the forward value in FP64, its low-order part parked in scr_lo
+// Per column: the tridiagonal forward elimination and back substitution (advance_w
+// 1433-1443), ph from rhs and the final w (1460-1465) and calc_p_rho for step >= 1
+// (438-568).  CT = double keeps the forward value in FP64, its low-order part parked in scr_lo.
 template <typename CT>
 __global__ void __launch_bounds__(256) k_ac_wsolve(KArgs a) {
   const Tile3 T = tile3(a);
@@ -483,11 +515,13 @@ __global__ void __launch_bounds__(256) k_ac_wsolve(KArgs a) {
   const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
   const int kde = d.kde, k_end = kde - 1;
   if (i > d.ide - 1 || j > d.jde - 1) return;
-  if (!a.c.non_hydrostatic) return;
-  FPtr<float> w = s.w_2;
+  const wrfmi_config &c = a.c;
+  if (!c.non_hydrostatic) return;
+  FPtr<float> w = s.w_2, ph = s.ph_2;
   FPtr<float> lo = a.scr_lo;
+  const FPtr<float> rhs = a.scr_rhs;
   const FPtr<CT> A((CT *)s.a), AL((CT *)s.alpha), GA((CT *)s.gamma);
-  const Off ob = X3(d, i, 0, j);
+  const Off ob = X3(d, i, 0, j), o2 = X2(d, i, j);
   const int ni = d.ni;
   CT wfe = (CT)w[ob + ni];
   // forward elimination: fields 0 = w (predictor), 1 = a, 2 = alpha
@@ -506,87 +540,66 @@ __global__ void __launch_bounds__(256) k_ac_wsolve(KArgs a) {
           lo[o] = (float)((double)wfe - (double)(float)wfe);
         }
       });
-  // back substitution: fields 0 = forward value (w, plus its low part in FP64), 1 = gamma
+  // downward: back substitution, ph, calc_p_rho.  Fields: 0 = forward value of w (plus its
+  // low part in FP64), 1 = gamma, 2 = rhs, 3 = t_2, 4 = t_save, 5 = alt, 6 = c2a, 7 = pm1
+  const float dts = a.dts, epssm = c.epssm, msfty = s.msfty[o2], muts = s.muts[o2], mu = s.mu_2[o2];
+  const FPtr<float> c1h = s.c1h, c2h = s.c2h, c1f = s.c1f, c2f = s.c2f, rdnw = s.rdnw;
+  const float smdiv = c.smdiv;
   CT wup = wfe;
-  column_march_down<2, CT>(
-      2, k_end,
+  float ph_up = 0.f;  // final ph one level above
+  column_march_down<8, CT, 4>(
+      1, k_end + 1,
       [&](int f, int k) -> CT {
         const Off o = ob + k * ni;
-        if (f == 1) return GA[o];
-        if (sizeof(CT) == sizeof(float)) return (CT)w[o];
-        return (CT)w[o] + (CT)lo[o];
+        switch (f) {
+          case 0:
+            if (sizeof(CT) == sizeof(float) || k > k_end || k < 2) return (CT)w[o];
+            return (CT)w[o] + (CT)lo[o];
+          case 1: return GA[o];
+          case 2: return (CT)rhs[o];
+          case 3: return (CT)s.t_2[o];
+          case 4: return (CT)s.t_save[o];
+          case 5: return (CT)s.alt[o];
+          case 6: return (CT)s.c2a[o];
+          default: return (CT)s.pm1[o];
+        }
       },
       [&](int k, const CT *v) {
         const Off o = ob + k * ni;
-        float wk;
-        if (sizeof(CT) == sizeof(float)) {
-          wk = (float)v[0] - (float)v[1] * (float)wup;
-          wup = (CT)wk;
+        float phk;
+        if (k >= 2) {
+          float wk;
+          if (k == k_end + 1) {
+            wk = (float)wup;  // top: the forward value
+          } else if (sizeof(CT) == sizeof(float)) {
+            wk = (float)v[0] - (float)v[1] * (float)wup;
+            wup = (CT)wk;
+            w[o] = wk;
+          } else {
+            const CT wn = v[0] - v[1] * wup;
+            wup = wn;
+            wk = (float)wn;
+            w[o] = wk;
+          }
+          phk = (float)v[2] + msfty * .5f * dts * K_G * (1.f + epssm) * wk / (c1f[X1(d, k)] * muts + c2f[X1(d, k)]);
+          ph[o] = phk;
         } else {
-          const CT wn = v[0] - v[1] * wup;
-          wup = wn;
-          wk = (float)wn;
+          phk = ph[o];
         }
-        w[o] = wk;
+        if (k <= k_end) {
+          const float cc = c1h[X1(d, k)], c2 = c2h[X1(d, k)];
+          const float alt = (float)v[5];
+          const float al = -1.f / (cc * muts + c2) * (alt * (cc * mu) + rdnw[X1(d, k)] * (ph_up - phk));
+          s.al[o] = al;
+          const float t1 = (float)v[4];
+          float pp = (float)v[6] * (alt * ((float)v[3] - (cc * mu) * t1) / ((cc * muts + c2) * (K_T0 + t1)) - al);
+          const float ptmp = pp;
+          pp = pp + smdiv * (pp - (float)v[7]);
+          s.p[o] = pp;
+          s.pm1[o] = ptmp;
+        }
+        ph_up = phk;
       });
-}
-
-// ph from rhs and the final w (advance_w 1460-1465), calc_p_rho for step >= 1 (438-568)
-// and sumflux (1473-1633); one thread per (i, k, j) with i <= ide, j <= jde, k <= kde
-__global__ void __launch_bounds__(256) k_ac_finish(KArgs a) {
-  const Tile3 T = tile3(a);
-  const Dm &d = a.d;
-  const DState &s = dstate(a);
-  const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
-  const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
-  const int k = d.kds + (int)T.bz;
-  const int ide = d.ide, jde = d.jde, kde = d.kde, k_end = kde - 1;
-  if (i > ide || j > jde || k > kde) return;
-  const bool mi = i <= ide - 1, mj = j <= jde - 1, mass = mi && mj;
-  const wrfmi_config &c = a.c;
-  const Off o2 = X2(d, i, j), o = X3(d, i, k, j);
-  const FPtr<float> c1h = s.c1h, c2h = s.c2h, c1f = s.c1f;
-  if (mass && c.non_hydrostatic) {
-    const float dts = a.dts, epssm = c.epssm, msfty = s.msfty[o2], muts = s.muts[o2];
-    const FPtr<float> rhs = a.scr_rhs, w = s.w_2;
-    auto ph_at = [&](int kk, Off oo) -> float {  // final ph at level kk >= 2
-      return rhs[oo] + msfty * .5f * dts * K_G * (1.f + epssm) * w[oo] / (c1f[X1(d, kk)] * muts + s.c2f[X1(d, kk)]);
-    };
-    const float ph0 = k >= 2 ? ph_at(k, o) : s.ph_2[o];
-    if (k >= 2) s.ph_2[o] = ph0;
-    if (k <= k_end) {
-      const float ph1 = ph_at(k + 1, o + d.ni);
-      const float mu = s.mu_2[o2];
-      const float cc = c1h[X1(d, k)], c2 = c2h[X1(d, k)];
-      const float alt = s.alt[o];
-      const float al = -1.f / (cc * muts + c2) * (alt * (cc * mu) + s.rdnw[X1(d, k)] * (ph1 - ph0));
-      s.al[o] = al;
-      const float t1 = s.t_save[o];
-      float pp = s.c2a[o] * (alt * (s.t_2[o] - (cc * mu) * t1) / ((cc * muts + c2) * (K_T0 + t1)) - al);
-      const float ptmp = pp;
-      pp = pp + c.smdiv * (pp - s.pm1[o]);
-      s.p[o] = pp;
-      s.pm1[o] = ptmp;
-    }
-  }
-  // sumflux
-  const int it = a.iteration, ns = a.nsmall;
-  const float nsf = (float)ns;
-  float rum = it == 1 ? 0.f : s.ru_m[o];
-  float rvm = it == 1 ? 0.f : s.rv_m[o];
-  float wwm = it == 1 ? 0.f : s.ww_m[o];
-  const bool kin = k <= kde - 1;
-  if (mj && kin) rum = rum + s.u_2[o];
-  if (mi && kin) rvm = rvm + s.v_2[o];
-  if (mass) wwm = wwm + s.ww[o];
-  if (it == ns) {
-    if (mj && kin) rum = rum / nsf + (c1h[X1(d, k)] * s.muu[o2] + c2h[X1(d, k)]) * s.u_save[o] / s.msfuy[o2];
-    if (mi && kin) rvm = rvm / nsf + (c1h[X1(d, k)] * s.muv[o2] + c2h[X1(d, k)]) * s.v_save[o] * s.msfvx_inv[o2];
-    if (mass) wwm = wwm / nsf + s.ww1[o];
-  }
-  s.ru_m[o] = rum;
-  s.rv_m[o] = rvm;
-  s.ww_m[o] = wwm;
 }
 
 // One thread per (i, k, j); the thread of level kds also writes the 2-D fields.

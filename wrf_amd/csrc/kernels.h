@@ -18,7 +18,6 @@ template <bool COMP> __global__ void k_ac_mu_ww(KArgs);
 __global__ void k_ac_theta(KArgs);
 __global__ void k_ac_wpred(KArgs);
 template <typename CT> __global__ void k_ac_wsolve(KArgs);
-__global__ void k_ac_finish(KArgs);
 __global__ void k_small_finish(KArgs);
 template <bool PER> __global__ void k_adv_scalar(KArgs);
 __global__ void k_update_scalar(KArgs);

@@ -239,12 +239,12 @@ static const dim3 kColBlk(64, 4, 1);
 enum KernelId {
   KID_STEP_PREP, KID_TEND_U, KID_TEND_V, KID_TEND_W, KID_TEND_T, KID_SMALL_PREP, KID_ADVANCE_UV,
   KID_AC_DIV, KID_SMALL_FINISH, KID_ADV_SCALAR, KID_UPDATE_SCALAR, KID_P_RHO_PHI, KID_HALO, KID_P_RHO0,
-  KID_COEF_W, KID_WW_SCAN, KID_AC_MU_WW, KID_AC_THETA, KID_AC_WPRED, KID_AC_WSOLVE, KID_AC_FINISH, KID_COUNT
+  KID_COEF_W, KID_WW_SCAN, KID_AC_MU_WW, KID_AC_THETA, KID_AC_WPRED, KID_AC_WSOLVE, KID_COUNT
 };
 static const char *kKernelNames[KID_COUNT] = {
   "k_step_prep", "k_tend_u", "k_tend_v", "k_tend_w", "k_tend_t", "k_small_prep", "k_advance_uv",
   "k_ac_div", "k_small_finish", "k_adv_scalar", "k_update_scalar", "k_p_rho_phi", "k_halo", "k_p_rho0",
-  "k_coef_w", "k_ww_scan", "k_ac_mu_ww", "k_ac_theta", "k_ac_wpred", "k_ac_wsolve", "k_ac_finish"};
+  "k_coef_w", "k_ww_scan", "k_ac_mu_ww", "k_ac_theta", "k_ac_wpred", "k_ac_wsolve"};
 struct EvPair { int id; hipEvent_t a, b; };
 static unsigned g_timing = 0;  // bit n set: time launches of kernel n
 static std::vector<EvPair> g_ev_pending;
@@ -299,9 +299,10 @@ struct Launcher {
   void halo_flush() {
     if (h.n == 0) return;
     const Dm &d = a.d;
-    dim3 blk(256, 1, 1);
-    dim3 grd((d.ime - d.ims + 1 + 255) / 256, d.jme - d.jms + 1, h.n);
-    launch(KID_HALO, k_halo, grd, blk, h);
+    const long ring = (long)(d.jds - d.jms + d.jme - d.jde + 1) * d.ni +
+                      (long)(d.jde - d.jds) * (d.ids - d.ims + d.ime - d.ide + 1);
+    const long nthreads = ring * d.kde * h.n;
+    launch(KID_HALO, k_halo, dim3((unsigned)((nthreads + 255) / 256), 1, 1), dim3(256, 1, 1), h);
     h.n = 0;
   }
 };
@@ -370,14 +371,13 @@ static void ph_advance_mu_t_w(Launcher &L) {
   const wrfmi_state &s = a.s;
   const bool p64 = a.c.precision_mode == 1;
   const int nk = a.d.kde - a.d.kds + 1;
-  L.launchK(KID_AC_DIV, k_ac_div, g_pt3(a.d, nk - 1), kColBlk, a);
+  L.launchK(KID_AC_DIV, k_ac_div, g_pt3(a.d, nk), kColBlk, a);
   if (p64) L.launchK(KID_AC_MU_WW, k_ac_mu_ww<true>, g_col(a.d), kColBlk, a);
   else L.launchK(KID_AC_MU_WW, k_ac_mu_ww<false>, g_col(a.d), kColBlk, a);
   L.launchK(KID_AC_THETA, k_ac_theta, g_pt3(a.d, nk), kColBlk, a);
   L.launchK(KID_AC_WPRED, k_ac_wpred, g_pt3(a.d, nk), kColBlk, a);
   if (p64) L.launchK(KID_AC_WSOLVE, k_ac_wsolve<double>, g_col(a.d), kColBlk, a);
   else L.launchK(KID_AC_WSOLVE, k_ac_wsolve<float>, g_col(a.d), kColBlk, a);
-  L.launchK(KID_AC_FINISH, k_ac_finish, g_pt3(a.d, nk), kColBlk, a);
   L.halo_begin();
   L.halo_add(s.ph_2, 'w', 1); L.halo_add(s.al, 'p', 1); L.halo_add(s.p, 'p', 1);
   L.halo_add(s.muts, 't', 0); L.halo_add(s.mu_2, 't', 0); L.halo_add(s.mudf, 't', 0);
