@@ -180,6 +180,9 @@ __global__ void __launch_bounds__(256) k_coef_w(KArgs a) {
   }
 }
 
+// advance_uv, one thread per (i, j) column segment of kAdvUvK levels: the column constants
+// (map-factor ratios, column masses, the mudf term) are formed once, and the vertically
+// interpolated pressure dpn at the lower w level is carried up from the level below.
 template <bool PER>
 __global__ void __launch_bounds__(256) k_advance_uv(KArgs a) {
   const Tile3 T = tile3(a);
@@ -187,13 +190,13 @@ __global__ void __launch_bounds__(256) k_advance_uv(KArgs a) {
   const DState &s = dstate(a);
   const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
   const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
-  const int k = d.kds + (int)T.bz;
+  const int k0 = d.kds + (int)T.bz * kAdvUvK;
   const int ids = d.ids, ide = d.ide, jds = d.jds, jde = d.jde, kde = d.kde;
   const int its = ids, ite = ide, jts = jds, jte = jde;
-  if (i > ide || j > jde || k > kde - 1) return;
+  if (i > ide || j > jde || k0 > kde - 1) return;
+  const int k1 = imin(k0 + kAdvUvK - 1, kde - 1);
   const wrfmi_config &c = a.c;
   const Geo g = geo<PER>(c);
-  const int k_start = d.kds;
   const int i_start = its, i_end = imin(ite, ide - 1), j_start = jts, j_end = imin(jte, jde - 1), i_endu = ite, j_endv = jte;
   int i_start_up = i_start, i_end_up = i_endu, j_start_vp = j_start, j_end_vp = j_endv;
   if ((g.open_xs || g.sym_xs) && its == ids) i_start_up++;
@@ -208,74 +211,97 @@ __global__ void __launch_bounds__(256) k_advance_uv(KArgs a) {
   const float rdx = s.rdx, rdy = s.rdy, dx = 1.f / rdx, dy = 1.f / rdy, dts = a.dts;
   const float cf1 = s.cf1, cf2 = s.cf2, cf3 = s.cf3, emdiv = c.emdiv;
   const FPtr<float> p = s.p, pb = s.pb, ph = s.ph_2, php = s.php, alt = s.alt, al = s.al, mu = s.mu_2, mudf = s.mudf;
-  const float c1 = s.c1h[X1(d, k)], c2 = s.c2h[X1(d, k)];
   const bool nh = c.non_hydrostatic;
-  const Off o = X3(d, i, k, j);
+  const Off o2 = X2(d, i, j);
   // u
   if (j >= j_start && j <= j_end) {
-    float u = s.u_2[o];
-    if (i >= i_start_u_tend && i <= i_end_u_tend) u = u + dts * s.ru_tend[o];
-    if (i >= i_start_up && i <= i_end_up) {
-      const float mudf_xy = -emdiv * dx * (mudf[X2(d, i, j)] - mudf[X2(d, i - 1, j)]) / s.msfuy[X2(d, i, j)];
-      const float m = s.msfux[X2(d, i, j)] / s.msfuy[X2(d, i, j)];
-      float dpxy = m * .5f * rdx * (c1 * s.muu[X2(d, i, j)] + c2) *
-                   (((ph[X3(d, i, k + 1, j)] - ph[X3(d, i - 1, k + 1, j)]) + (ph[o] - ph[X3(d, i - 1, k, j)])) +
-                    (alt[o] + alt[X3(d, i - 1, k, j)]) * (p[o] - p[X3(d, i - 1, k, j)]) +
-                    (al[o] + al[X3(d, i - 1, k, j)]) * (pb[o] - pb[X3(d, i - 1, k, j)]));
-      if (nh) {
-        auto dpn = [&](int kk) -> float {
-          if (kk == 1)
-            return .5f * (cf1 * (p[X3(d, i, 1, j)] + p[X3(d, i - 1, 1, j)]) + cf2 * (p[X3(d, i, 2, j)] + p[X3(d, i - 1, 2, j)]) +
-                          cf3 * (p[X3(d, i, 3, j)] + p[X3(d, i - 1, 3, j)]));
-          if (kk == kde) {
-            if (!c.top_lid) return 0.f;
-            return .5f * (cf1 * (p[X3(d, i - 1, kde - 1, j)] + p[X3(d, i, kde - 1, j)]) + cf2 * (p[X3(d, i - 1, kde - 2, j)] + p[X3(d, i, kde - 2, j)]) +
-                          cf3 * (p[X3(d, i - 1, kde - 3, j)] + p[X3(d, i, kde - 3, j)]));
-          }
-          return .5f * (s.fnm[X1(d, kk)] * (p[X3(d, i, kk, j)] + p[X3(d, i - 1, kk, j)]) + s.fnp[X1(d, kk)] * (p[X3(d, i, kk - 1, j)] + p[X3(d, i - 1, kk - 1, j)]));
-        };
-        dpxy = dpxy + m * rdx * (php[o] - php[X3(d, i - 1, k, j)]) *
-                          (s.rdnw[X1(d, k)] * (dpn(k + 1) - dpn(k)) - .5f * ((c1 * mu[X2(d, i - 1, j)]) + (c1 * mu[X2(d, i, j)])));
-      }
-      u = u - dts * s.cqu[o] * dpxy + (c1 * mudf_xy);
+    const bool tend_on = i >= i_start_u_tend && i <= i_end_u_tend, pg_on = i >= i_start_up && i <= i_end_up;
+    float mudf_xy = 0.f, m = 0.f, muu = 0.f, mu0 = 0.f, mu1 = 0.f;
+    if (pg_on) {
+      mudf_xy = -emdiv * dx * (mudf[o2] - mudf[X2(d, i - 1, j)]) / s.msfuy[o2];
+      m = s.msfux[o2] / s.msfuy[o2];
+      muu = s.muu[o2];
+      mu0 = mu[X2(d, i - 1, j)];
+      mu1 = mu[o2];
     }
-    s.u_2[o] = u;
+    // dpn at w level kk, averaged to the u point (kk = 1: cf extrapolation; kk = kde: lid)
+    auto dpn = [&](int kk) -> float {
+      if (kk == 1)
+        return .5f * (cf1 * (p[X3(d, i, 1, j)] + p[X3(d, i - 1, 1, j)]) + cf2 * (p[X3(d, i, 2, j)] + p[X3(d, i - 1, 2, j)]) +
+                      cf3 * (p[X3(d, i, 3, j)] + p[X3(d, i - 1, 3, j)]));
+      if (kk == kde) {
+        if (!c.top_lid) return 0.f;
+        return .5f * (cf1 * (p[X3(d, i - 1, kde - 1, j)] + p[X3(d, i, kde - 1, j)]) + cf2 * (p[X3(d, i - 1, kde - 2, j)] + p[X3(d, i, kde - 2, j)]) +
+                      cf3 * (p[X3(d, i - 1, kde - 3, j)] + p[X3(d, i, kde - 3, j)]));
+      }
+      return .5f * (s.fnm[X1(d, kk)] * (p[X3(d, i, kk, j)] + p[X3(d, i - 1, kk, j)]) + s.fnp[X1(d, kk)] * (p[X3(d, i, kk - 1, j)] + p[X3(d, i - 1, kk - 1, j)]));
+    };
+    float dpn_lo = (pg_on && nh) ? dpn(k0) : 0.f;
+    for (int k = k0; k <= k1; k++) {
+      const Off o = X3(d, i, k, j);
+      const float c1 = s.c1h[X1(d, k)], c2 = s.c2h[X1(d, k)];
+      float u = s.u_2[o];
+      if (tend_on) u = u + dts * s.ru_tend[o];
+      if (pg_on) {
+        float dpxy = m * .5f * rdx * (c1 * muu + c2) *
+                     (((ph[o + d.ni] - ph[X3(d, i - 1, k + 1, j)]) + (ph[o] - ph[o - 1])) +
+                      (alt[o] + alt[o - 1]) * (p[o] - p[o - 1]) +
+                      (al[o] + al[o - 1]) * (pb[o] - pb[o - 1]));
+        if (nh) {
+          const float dpn_hi = dpn(k + 1);
+          dpxy = dpxy + m * rdx * (php[o] - php[o - 1]) * (s.rdnw[X1(d, k)] * (dpn_hi - dpn_lo) - .5f * ((c1 * mu0) + (c1 * mu1)));
+          dpn_lo = dpn_hi;
+        }
+        u = u - dts * s.cqu[o] * dpxy + (c1 * mudf_xy);
+      }
+      s.u_2[o] = u;
+    }
   }
   // v
   if (i <= i_end && j >= j_start_v_tend && j <= j_end_v_tend) {
-    float v = s.v_2[o];
-    v = v + dts * s.rv_tend[o];
-    if (j >= j_start_vp && j <= j_end_vp) {
-      const float mudf_xy = -emdiv * dy * (mudf[X2(d, i, j)] - mudf[X2(d, i, j - 1)]) * s.msfvx_inv[X2(d, i, j)];
-      const float m = s.msfvy[X2(d, i, j)] / s.msfvx[X2(d, i, j)];
-      float dpxy = m * .5f * rdy * (c1 * s.muv[X2(d, i, j)] + c2) *
-                   (((ph[X3(d, i, k + 1, j)] - ph[X3(d, i, k + 1, j - 1)]) + (ph[o] - ph[X3(d, i, k, j - 1)])) +
-                    (alt[o] + alt[X3(d, i, k, j - 1)]) * (p[o] - p[X3(d, i, k, j - 1)]) +
-                    (al[o] + al[X3(d, i, k, j - 1)]) * (pb[o] - pb[X3(d, i, k, j - 1)]));
-      if (nh) {
-        auto dpn = [&](int kk) -> float {
-          if (kk == 1)
-            return .5f * (cf1 * (p[X3(d, i, 1, j)] + p[X3(d, i, 1, j - 1)]) + cf2 * (p[X3(d, i, 2, j)] + p[X3(d, i, 2, j - 1)]) +
-                          cf3 * (p[X3(d, i, 3, j)] + p[X3(d, i, 3, j - 1)]));
-          if (kk == kde) {
-            if (!c.top_lid) return 0.f;
-            return .5f * (cf1 * (p[X3(d, i, kde - 1, j - 1)] + p[X3(d, i, kde - 1, j)]) + cf2 * (p[X3(d, i, kde - 2, j - 1)] + p[X3(d, i, kde - 2, j)]) +
-                          cf3 * (p[X3(d, i, kde - 3, j - 1)] + p[X3(d, i, kde - 3, j)]));
-          }
-          return .5f * (s.fnm[X1(d, kk)] * (p[X3(d, i, kk, j)] + p[X3(d, i, kk, j - 1)]) + s.fnp[X1(d, kk)] * (p[X3(d, i, kk - 1, j)] + p[X3(d, i, kk - 1, j - 1)]));
-        };
-        dpxy = dpxy + m * rdy * (php[o] - php[X3(d, i, k, j - 1)]) *
-                          (s.rdnw[X1(d, k)] * (dpn(k + 1) - dpn(k)) - .5f * ((c1 * mu[X2(d, i, j - 1)]) + (c1 * mu[X2(d, i, j)])));
-      }
-      v = v - dts * s.cqv[o] * dpxy + (c1 * mudf_xy);
+    const bool pg_on = j >= j_start_vp && j <= j_end_vp;
+    float mudf_xy = 0.f, m = 0.f, muv = 0.f, mu0 = 0.f, mu1 = 0.f;
+    if (pg_on) {
+      mudf_xy = -emdiv * dy * (mudf[o2] - mudf[X2(d, i, j - 1)]) * s.msfvx_inv[o2];
+      m = s.msfvy[o2] / s.msfvx[o2];
+      muv = s.muv[o2];
+      mu0 = mu[X2(d, i, j - 1)];
+      mu1 = mu[o2];
     }
-    s.v_2[o] = v;
+    auto dpn = [&](int kk) -> float {
+      if (kk == 1)
+        return .5f * (cf1 * (p[X3(d, i, 1, j)] + p[X3(d, i, 1, j - 1)]) + cf2 * (p[X3(d, i, 2, j)] + p[X3(d, i, 2, j - 1)]) +
+                      cf3 * (p[X3(d, i, 3, j)] + p[X3(d, i, 3, j - 1)]));
+      if (kk == kde) {
+        if (!c.top_lid) return 0.f;
+        return .5f * (cf1 * (p[X3(d, i, kde - 1, j - 1)] + p[X3(d, i, kde - 1, j)]) + cf2 * (p[X3(d, i, kde - 2, j - 1)] + p[X3(d, i, kde - 2, j)]) +
+                      cf3 * (p[X3(d, i, kde - 3, j - 1)] + p[X3(d, i, kde - 3, j)]));
+      }
+      return .5f * (s.fnm[X1(d, kk)] * (p[X3(d, i, kk, j)] + p[X3(d, i, kk, j - 1)]) + s.fnp[X1(d, kk)] * (p[X3(d, i, kk - 1, j)] + p[X3(d, i, kk - 1, j - 1)]));
+    };
+    float dpn_lo = (pg_on && nh) ? dpn(k0) : 0.f;
+    for (int k = k0; k <= k1; k++) {
+      const Off o = X3(d, i, k, j);
+      const float c1 = s.c1h[X1(d, k)], c2 = s.c2h[X1(d, k)];
+      float v = s.v_2[o];
+      v = v + dts * s.rv_tend[o];
+      if (pg_on) {
+        float dpxy = m * .5f * rdy * (c1 * muv + c2) *
+                     (((ph[o + d.ni] - ph[X3(d, i, k + 1, j - 1)]) + (ph[o] - ph[o - d.sj])) +
+                      (alt[o] + alt[o - d.sj]) * (p[o] - p[o - d.sj]) +
+                      (al[o] + al[o - d.sj]) * (pb[o] - pb[o - d.sj]));
+        if (nh) {
+          const float dpn_hi = dpn(k + 1);
+          dpxy = dpxy + m * rdy * (php[o] - php[o - d.sj]) * (s.rdnw[X1(d, k)] * (dpn_hi - dpn_lo) - .5f * ((c1 * mu0) + (c1 * mu1)));
+          dpn_lo = dpn_hi;
+        }
+        v = v - dts * s.cqv[o] * dpxy + (c1 * mudf_xy);
+      }
+      s.v_2[o] = v;
+    }
   }
-  (void)k_start;
 }
 
-// One thread per (i,j) column.  Mass columns run advance_mu_t, advance_w and
-// calc_p_rho; every column of the tile accumulates its share of sumflux.
 // ---------------------------------------------------------------------------------------
 // The column part of one acoustic substep: advance_mu_t + advance_w + sumflux + calc_p_rho,
 // split at the two column dependencies (the mass-divergence integral before the omega scan,

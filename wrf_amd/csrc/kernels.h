@@ -3,6 +3,7 @@
 #include "dev_common.h"
 
 namespace wrfmi {
+constexpr int kAdvUvK = 8;  // levels per thread in k_advance_uv
 __global__ void k_step_prep(KArgs);
 template <bool COMP> __global__ void k_ww_scan(KArgs);
 template <bool PER> __global__ void k_tend_u(KArgs);

@@ -364,7 +364,8 @@ static void ph_small_prep(Launcher &L) {
 }
 static void ph_advance_uv(Launcher &L) {
   const bool per = L.a.c.periodic_x && L.a.c.periodic_y;
-  L.launchK(KID_ADVANCE_UV, per ? k_advance_uv<true> : k_advance_uv<false>, g_pt3(L.a.d, L.a.d.kde - L.a.d.kds), kColBlk, L.a);
+  L.launchK(KID_ADVANCE_UV, per ? k_advance_uv<true> : k_advance_uv<false>,
+            g_pt3(L.a.d, (L.a.d.kde - L.a.d.kds + kAdvUvK - 1) / kAdvUvK), kColBlk, L.a);
 }
 static void ph_advance_mu_t_w(Launcher &L) {
   const KArgs &a = L.a;
