@@ -7,6 +7,7 @@
 // Nothing of the product links this file; the product is the hipcc build for gfx950.
 #pragma once
 #include <cmath>
+#include <functional>
 #include <cstddef>
 #include <cstdint>
 #include <cstdlib>
@@ -17,6 +18,7 @@
 #define __host__
 #define __forceinline__ inline
 #define __launch_bounds__(...)
+#define __shared__ static
 #define WRFMI_GLOBAL_AS
 
 struct dim3 {
@@ -71,6 +73,10 @@ inline hipError_t hipGraphDestroy(hipGraph_t) { return hipSuccess; }
 
 // Launch order: 0 = blocks and threads in increasing order, 1 = decreasing order.
 int emu_launch_order();
+// Runs the nt threads of one workgroup as fibers that meet at every __syncthreads().
+void emu_run_block(long nt, bool rev, dim3 b, const std::function<void()> &body);
+void __syncthreads();
+float __shfl_down(float v, unsigned delta, int width = 64);
 
 template <typename... P, typename... A>
 void emu_launch(void (*kern)(P...), dim3 g, dim3 b, A &&...args) {
@@ -81,11 +87,7 @@ void emu_launch(void (*kern)(P...), dim3 g, dim3 b, A &&...args) {
   for (long ob = 0; ob < nb; ob++) {
     const long bb = rev ? nb - 1 - ob : ob;
     blockIdx = emu_uint3{(unsigned)(bb % g.x), (unsigned)((bb / g.x) % g.y), (unsigned)(bb / ((long)g.x * g.y))};
-    for (long ot = 0; ot < nt; ot++) {
-      const long tt = rev ? nt - 1 - ot : ot;
-      threadIdx = emu_uint3{(unsigned)(tt % b.x), (unsigned)((tt / b.x) % b.y), (unsigned)(tt / ((long)b.x * b.y))};
-      kern(args...);
-    }
+    emu_run_block(nt, rev, b, [&]() { kern(args...); });
   }
 }
 #define hipLaunchKernelGGL(K, G, B, SHMEM, STREAM, ...) emu_launch(K, G, B, __VA_ARGS__)

@@ -9,19 +9,23 @@
 
 namespace wrfmi {
 
-// One thread per (i, k, j) point: the horizontal fluxes at both faces in x and y and the
-// vertical fluxes at both w levels of the point (the column form carried the lower one).
+// advect_scalar with each face flux computed once.  A block is a tile of 63 x 4 columns
+// (wave lane = i, wave = j row) marching up kAdvK levels: lane l computes the x flux at its
+// own face i and takes the face i+1 flux from lane l+1 (lane 63 only computes); each row
+// computes the y flux at its face j and reads face j+1 from the row above through LDS (the
+// top row also computes face j+4); the vertical flux is carried up the column.  Every flux
+// is the same expression, in the same order, as in advect_scalar.
 template <bool PER>
 __global__ void __launch_bounds__(256) k_adv_scalar(KArgs a) {
   const Tile3 T = tile3(a);
   const Dm &d = a.d;
   const DState &s = dstate(a);
-  const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
-  const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
-  const int k = d.kds + (int)T.bz;
+  const int lane = (int)threadIdx.x, row = (int)threadIdx.y;
+  const int i = d.ids + T.bx * kAdvTileI + lane;
+  const int j = d.jds + T.by * (int)blockDim.y + row;
+  const int k0 = d.kds + T.bz * kAdvK;
   const int ids = d.ids, ide = d.ide, jds = d.jds, jde = d.jde, kde = d.kde;
   const int its = ids, ite = ide, jts = jds, jte = jde, kts = d.kds, kte = kde;
-  if (i > ide || j > jde || k > kte) return;
   const wrfmi_config &c = a.c;
   const Geo g = geo<PER>(c);
   const float ts = c.time_step >= 0 ? 1.f : -1.f;
@@ -29,11 +33,9 @@ __global__ void __launch_bounds__(256) k_adv_scalar(KArgs a) {
   const FPtr<float> f = s.q[a.species], ru = s.ru_m, rv = s.rv_m, rom = s.ww_m;
   FPtr<float> tend = s.advect_tend;
   const float rdx = s.rdx, rdy = s.rdy;
+  const bool inside = i <= ide && j <= jde;          // a point of the u/v-staggered patch
+  const bool writer = inside && lane < kAdvTileI;   // lane 63 of each row only computes a face
   const bool mass = i <= ide - 1 && j <= jde - 1;
-  if (!mass) {
-    tend[X3(d, i, k, j)] = 0.f;
-    return;
-  }
   const bool dxs = !(g.per_x || g.sym_xs || its > ids + 3), dxe = !(g.per_x || g.sym_xe || ite < ide - 3);
   const bool dys = !(g.per_y || g.sym_ys || jts > jds + 3), dye = !(g.per_y || g.sym_ye || jte < jde - 4);
   int jy0 = jts, jy1 = imin(jte, jde - 1), jf0 = jy0, jf1 = jy1 + 1;
@@ -74,36 +76,57 @@ __global__ void __launch_bounds__(256) k_adv_scalar(KArgs a) {
     return vel * flux3(f[X3(d, i, k - 2, j)], f[X3(d, i, k - 1, j)], f[X3(d, i, k, j)], f[X3(d, i, k + 1, j)], -vel, ts);
   };
   auto P = [&](FPtr<float> arr, int ii, int k, int jj) -> float { return arr[X3(d, ii, k, jj)]; };
-  {
-    float t = 0.f;
-    if (k <= ktf) {
-      if (j >= jy0 && j <= jy1) t = t - s.msftx[X2(d, i, j)] * rdy * (fqy(k, j + 1) - fqy(k, j));
-      if (i >= ix0 && i <= ix1) t = t - s.msftx[X2(d, i, j)] * rdx * (fqx(k, i + 1) - fqx(k, i));
-      if (g.open_xs && i == its) {
-        float ub = 0.5f * (P(ru, its, k, j) + P(ru, its + 1, k, j));
-        if (ub > 0.f) ub = 0.f;
-        t = t - rdx * (ub * (P(f, its + 1, k, j) - P(f, its, k, j)) + P(f, its, k, j) * (P(ru, its + 1, k, j) - P(ru, its, k, j)));
+  // which faces this thread supplies: x face i (rows of mass points), y face j (columns of
+  // mass points), and for the top row of the block y face j+1 as well
+  const bool fx_on = i <= ide && j <= jde - 1 && i >= ix0 && i <= ix1 + 1;
+  const bool fy_on = i <= ide - 1 && j <= jde && j >= jy0 && j <= jy1 + 1;
+  const bool fy2_on = row == (int)blockDim.y - 1 && i <= ide - 1 && j + 1 <= jde && j + 1 >= jy0 && j + 1 <= jy1 + 1;
+  const bool xdiv = mass && i >= ix0 && i <= ix1, ydiv = mass && j >= jy0 && j <= jy1;
+  const float msftx = mass ? s.msftx[X2(d, i, j)] : 0.f;
+  __shared__ float fy_s[2][kAdvRows + 1][64];
+  float vf_lo = (mass && k0 <= ktf) ? vflux(k0) : 0.f;
+  for (int kk = 0; kk < kAdvK; kk++) {
+    const int k = k0 + kk;
+    if (k > kte) break;  // uniform over the block
+    const float fx = (fx_on && k <= ktf) ? fqx(k, i) : 0.f;
+    const float fx_up = __shfl_down(fx, 1, 64);
+    const float fy = (fy_on && k <= ktf) ? fqy(k, j) : 0.f;
+    float(*buf)[64] = fy_s[kk & 1];
+    buf[row][lane] = fy;
+    if (row == (int)blockDim.y - 1) buf[row + 1][lane] = (fy2_on && k <= ktf) ? fqy(k, j + 1) : 0.f;
+    __syncthreads();
+    const float fy_up = buf[row + 1][lane];
+    if (writer) {
+      float t = 0.f;
+      if (mass && k <= ktf) {
+        if (ydiv) t = t - msftx * rdy * (fy_up - fy);
+        if (xdiv) t = t - msftx * rdx * (fx_up - fx);
+        if (g.open_xs && i == its) {
+          float ub = 0.5f * (P(ru, its, k, j) + P(ru, its + 1, k, j));
+          if (ub > 0.f) ub = 0.f;
+          t = t - rdx * (ub * (P(f, its + 1, k, j) - P(f, its, k, j)) + P(f, its, k, j) * (P(ru, its + 1, k, j) - P(ru, its, k, j)));
+        }
+        if (g.open_xe && i == ide - 1) {
+          float ub = 0.5f * (P(ru, ite - 1, k, j) + P(ru, ite, k, j));
+          if (ub < 0.f) ub = 0.f;
+          t = t - rdx * (ub * (P(f, i, k, j) - P(f, i - 1, k, j)) + P(f, i, k, j) * (P(ru, ite, k, j) - P(ru, ite - 1, k, j)));
+        }
+        if (g.open_ys && j == jts) {
+          float vb = 0.5f * (P(rv, i, k, jts) + P(rv, i, k, jts + 1));
+          if (vb > 0.f) vb = 0.f;
+          t = t - rdy * (vb * (P(f, i, k, jts + 1) - P(f, i, k, jts)) + P(f, i, k, jts) * (P(rv, i, k, jts + 1) - P(rv, i, k, jts)));
+        }
+        if (g.open_ye && j == jde - 1) {
+          float vb = 0.5f * (P(rv, i, k, jte - 1) + P(rv, i, k, jte));
+          if (vb < 0.f) vb = 0.f;
+          t = t - rdy * (vb * (P(f, i, k, j) - P(f, i, k, j - 1)) + P(f, i, k, j) * (P(rv, i, k, jte) - P(rv, i, k, jte - 1)));
+        }
+        const float vf_hi = vflux(k + 1);
+        t = t - s.rdnw[X1(d, k)] * (vf_hi - vf_lo);
+        vf_lo = vf_hi;
       }
-      if (g.open_xe && i == ide - 1) {
-        float ub = 0.5f * (P(ru, ite - 1, k, j) + P(ru, ite, k, j));
-        if (ub < 0.f) ub = 0.f;
-        t = t - rdx * (ub * (P(f, i, k, j) - P(f, i - 1, k, j)) + P(f, i, k, j) * (P(ru, ite, k, j) - P(ru, ite - 1, k, j)));
-      }
-      if (g.open_ys && j == jts) {
-        float vb = 0.5f * (P(rv, i, k, jts) + P(rv, i, k, jts + 1));
-        if (vb > 0.f) vb = 0.f;
-        t = t - rdy * (vb * (P(f, i, k, jts + 1) - P(f, i, k, jts)) + P(f, i, k, jts) * (P(rv, i, k, jts + 1) - P(rv, i, k, jts)));
-      }
-      if (g.open_ye && j == jde - 1) {
-        float vb = 0.5f * (P(rv, i, k, jte - 1) + P(rv, i, k, jte));
-        if (vb < 0.f) vb = 0.f;
-        t = t - rdy * (vb * (P(f, i, k, j) - P(f, i, k, j - 1)) + P(f, i, k, j) * (P(rv, i, k, jte) - P(rv, i, k, jte - 1)));
-      }
-      const float vf_hi = vflux(k + 1);
-      const float vf_lo = vflux(k);
-      t = t - s.rdnw[X1(d, k)] * (vf_hi - vf_lo);
+      tend[X3(d, i, k, j)] = t;
     }
-    tend[X3(d, i, k, j)] = t;
   }
 }
 

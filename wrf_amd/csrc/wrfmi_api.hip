@@ -397,7 +397,10 @@ static void ph_small_finish(Launcher &L) {
 }
 static void ph_scalar(Launcher &L) {
   const bool per = L.a.c.periodic_x && L.a.c.periodic_y;
-  L.launchK(KID_ADV_SCALAR, per ? k_adv_scalar<true> : k_adv_scalar<false>, g_pt3(L.a.d, L.a.d.kde - L.a.d.kds + 1), kColBlk, L.a);
+  const Dm &d = L.a.d;
+  const dim3 ga((d.ide - d.ids + 1 + kAdvTileI - 1) / kAdvTileI, (d.jde - d.jds + 1 + kAdvRows - 1) / kAdvRows,
+                (d.kde - d.kds + 1 + kAdvK - 1) / kAdvK);
+  L.launchK(KID_ADV_SCALAR, per ? k_adv_scalar<true> : k_adv_scalar<false>, ga, dim3(64, kAdvRows, 1), L.a);
   L.launchK(KID_UPDATE_SCALAR, k_update_scalar, g_pt3(L.a.d, L.a.d.kde - L.a.d.kds), kColBlk, L.a);
 }
 static void ph_p_rho_phi(Launcher &L) {
