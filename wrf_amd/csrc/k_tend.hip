@@ -24,10 +24,11 @@ __global__ void __launch_bounds__(256) k_tend_u(KArgs a) {
   const DState &s = dstate(a);
   const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
   const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
-  const int k = d.kds + (int)T.bz;
+  const int k0 = d.kds + (int)T.bz * kTendK;
   const int ids = d.ids, ide = d.ide, jds = d.jds, jde = d.jde, kde = d.kde;
   const int its = ids, ite = ide, jts = jds, jte = jde, kts = d.kds, kte = kde;
-  if (i > ide || j > jde || k > kte) return;
+  if (i > ide || j > jde || k0 > kte) return;
+  const int k1 = imin(k0 + kTendK - 1, kte);
   const Geo g = geo<PER>(a.c);
   const float ts = a.c.time_step >= 0 ? 1.f : -1.f;
   const int ktf = kte - 1;
@@ -137,13 +138,14 @@ __global__ void __launch_bounds__(256) k_tend_u(KArgs a) {
     }
     return .5f * (s.fnm[X1(d, k)] * (p[X3(d, i - 1, k, j)] + p[X3(d, i, k, j)]) + s.fnp[X1(d, k)] * (p[X3(d, i - 1, k - 1, j)] + p[X3(d, i, k - 1, j)]));
   };
-  {
+  // the vertical flux and dpn at the lower w level are carried up the column segment
+  float vf_lo = (row_ok && k0 <= ktf && i >= iv0 && i <= iv1) ? vflux(k0) : 0.f;
+  float dpn_lo = (row_ok && k0 <= ktf && nh && i >= ih0 && i <= ih1) ? dpn(k0) : 0.f;
+  for (int k = k0; k <= k1; k++) {
     const Off o = X3(d, i, k, j);
     float t = 0.f;
     float vf_hi = 0.f, dpn_hi = 0.f;
     if (row_ok && k <= ktf) {
-      const float vf_lo = (i >= iv0 && i <= iv1) ? vflux(k) : 0.f;
-      const float dpn_lo = (nh && i >= ih0 && i <= ih1) ? dpn(k) : 0.f;
       // advect_u: y flux divergence
       if (i >= iy0 && i <= iy1 && j >= jy0 && j <= jy1) {
         const float mrdy = s.msfux[X2(d, i, j)] * rdy;
@@ -234,6 +236,8 @@ __global__ void __launch_bounds__(256) k_tend_u(KArgs a) {
       t = t + tf / msf;
     }
     s.ru_tend[o] = t;
+    vf_lo = vf_hi;
+    dpn_lo = dpn_hi;
   }
 }
 
@@ -245,10 +249,11 @@ __global__ void __launch_bounds__(256) k_tend_v(KArgs a) {
   const DState &s = dstate(a);
   const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
   const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
-  const int k = d.kds + (int)T.bz;
+  const int k0 = d.kds + (int)T.bz * kTendK;
   const int ids = d.ids, ide = d.ide, jds = d.jds, jde = d.jde, kde = d.kde;
   const int its = ids, ite = ide, jts = jds, jte = jde, kts = d.kds, kte = kde;
-  if (i > ide || j > jde || k > kte) return;
+  if (i > ide || j > jde || k0 > kte) return;
+  const int k1 = imin(k0 + kTendK - 1, kte);
   const Geo g = geo<PER>(a.c);
   const float ts = a.c.time_step >= 0 ? 1.f : -1.f;
   const int ktf = kte - 1;
@@ -350,12 +355,12 @@ __global__ void __launch_bounds__(256) k_tend_v(KArgs a) {
     }
     return .5f * (s.fnm[X1(d, k)] * (p[X3(d, i, k, j - 1)] + p[X3(d, i, k, j)]) + s.fnp[X1(d, k)] * (p[X3(d, i, k - 1, j - 1)] + p[X3(d, i, k - 1, j)]));
   };
-  {
+  float vf_lo = (col_ok && k0 <= ktf && j >= jv0 && j <= jv1) ? vflux(k0) : 0.f;
+  float dpn_lo = (col_ok && k0 <= ktf && nh && j >= jh0 && j <= jh1) ? dpn(k0) : 0.f;
+  for (int k = k0; k <= k1; k++) {
     const Off o = X3(d, i, k, j);
     float t = 0.f, vf_hi = 0.f, dpn_hi = 0.f;
     if (col_ok && k <= ktf) {
-      const float vf_lo = (j >= jv0 && j <= jv1) ? vflux(k) : 0.f;
-      const float dpn_lo = (nh && j >= jh0 && j <= jh1) ? dpn(k) : 0.f;
       if (j >= jy0 && j <= jy1) {
         const float mrdy = s.msfvy[X2(d, i, j)] * rdy;
         t = t - mrdy * (fqy(k, j + 1) - fqy(k, j));
@@ -436,6 +441,8 @@ __global__ void __launch_bounds__(256) k_tend_v(KArgs a) {
       t = t + tf * s.msfvx_inv[X2(d, i, j)];
     }
     s.rv_tend[o] = t;
+    vf_lo = vf_hi;
+    dpn_lo = dpn_hi;
   }
 }
 
@@ -447,10 +454,11 @@ __global__ void __launch_bounds__(256) k_tend_w(KArgs a) {
   const DState &s = dstate(a);
   const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
   const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
-  const int k = d.kds + (int)T.bz;
+  const int k0 = d.kds + (int)T.bz * kTendK;
   const int ids = d.ids, ide = d.ide, jds = d.jds, jde = d.jde, kde = d.kde;
   const int its = ids, ite = ide, jts = jds, jte = jde, kts = d.kds, kte = kde;
-  if (i > ide || j > jde || k > kte) return;
+  if (i > ide || j > jde || k0 > kte) return;
+  const int k1 = imin(k0 + kTendK - 1, kte);
   const Geo g = geo<PER>(a.c);
   const float ts = a.c.time_step >= 0 ? 1.f : -1.f;
   const int ktf = kte - 1;
@@ -504,7 +512,8 @@ __global__ void __launch_bounds__(256) k_tend_w(KArgs a) {
   const float msfty = mass ? s.msfty[X2(d, i, j)] : 1.f;
   const float mu = mass ? s.mu_2[X2(d, i, j)] : 0.f, mub = mass ? s.mub[X2(d, i, j)] : 0.f;
   const float mut = mass ? s.mut[X2(d, i, j)] : 0.f;
-  {
+  float vf_c = (mass && imax(k0, kts + 1) <= kte) ? vflux(imax(k0, kts + 1)) : 0.f;  // vflux at level k
+  for (int k = k0; k <= k1; k++) {
     const Off o = X3(d, i, k, j);
     float t = 0.f;
     if (mass) {
@@ -571,9 +580,10 @@ __global__ void __launch_bounds__(256) k_tend_w(KArgs a) {
         // vertical advection of w (order 3) with the lid flux term
         if (k <= ktf) {
           const float vf_hi = vflux(k + 1);
-          t = t - s.rdn[X1(d, k)] * (vf_hi - vflux(k));
+          t = t - s.rdn[X1(d, k)] * (vf_hi - vf_c);
+          vf_c = vf_hi;
         } else {
-          t = t + 2.f * s.rdn[X1(d, k - 1)] * (vflux(kde));
+          t = t + 2.f * s.rdn[X1(d, k - 1)] * (vf_c);
         }
         // pg_buoy_w
         if (k == kde) {
@@ -641,10 +651,11 @@ __global__ void __launch_bounds__(256) k_tend_t(KArgs a) {
   const DState &s = dstate(a);
   const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
   const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
-  const int k = d.kds + (int)T.bz;
+  const int k0 = d.kds + (int)T.bz * kTendK;
   const int ids = d.ids, ide = d.ide, jds = d.jds, jde = d.jde, kde = d.kde;
   const int its = ids, ite = ide, jts = jds, jte = jde, kts = d.kds, kte = kde;
-  if (i > ide || j > jde || k > kte) return;
+  if (i > ide || j > jde || k0 > kte) return;
+  const int k1 = imin(k0 + kTendK - 1, kte);
   const Geo g = geo<PER>(a.c);
   const float ts = a.c.time_step >= 0 ? 1.f : -1.f;
   const int ktf = kte - 1;
@@ -721,7 +732,9 @@ __global__ void __launch_bounds__(256) k_tend_t(KArgs a) {
     return .5f * (P(rom, i, kk, j) + P(rom, i, kk - 1, j)) * s.rdnw[X1(d, kk - 1)] *
            (P(ph, i, kk, j) - P(ph, i, kk - 1, j) + P(phb, i, kk, j) - P(phb, i, kk - 1, j));
   };
-  {
+  float vf_c = mass ? vflux(k0) : 0.f;                                  // vflux at level k
+  float wd_c = (mass && k0 >= 2 && k0 <= kte - 1) ? wdwn(k0) : 0.f;     // rhs_ph's wdwn at level k
+  for (int k = k0; k <= k1; k++) {
     const Off o = X3(d, i, k, j);
     // zero_tend of the *_save arrays over the whole tile
     s.u_save[o] = 0.f;
@@ -762,7 +775,8 @@ __global__ void __launch_bounds__(256) k_tend_t(KArgs a) {
           t = t - rdy * (vb * (P(f, i, k, j) - P(f, i, k, j - 1)) + P(f, i, k, j) * (P(rv, i, k, jte) - P(rv, i, k, jte - 1)));
         }
         vf_hi = vflux(k + 1);
-        t = t - s.rdnw[X1(d, k)] * (vf_hi - vflux(k));
+        t = t - s.rdnw[X1(d, k)] * (vf_hi - vf_c);
+        vf_c = vf_hi;
         // rk_addtend_dry (theta)
         float tf = s.t_tendf[o];
         if (a.rk_step == 1) {
@@ -773,7 +787,8 @@ __global__ void __launch_bounds__(256) k_tend_t(KArgs a) {
       }
       // rhs_ph
       if (k >= 2 && k <= kte - 1) {
-        const float wd_hi = wdwn(k + 1), wd_lo = wdwn(k);
+        const float wd_hi = wdwn(k + 1), wd_lo = k == 2 || k == k0 ? wdwn(k) : wd_c;
+        wd_c = wd_hi;
         if (a.c.phi_adv_z == 2)
           pt = pt - P(rom, i, k, j) * (s.fnm[X1(d, k)] * wd_hi + s.fnp[X1(d, k)] * wd_lo);
         else
@@ -856,7 +871,7 @@ __global__ void __launch_bounds__(256) k_tend_t(KArgs a) {
     s.t_tend[o] = t;
     s.ph_tend[o] = pt;
   }
-  if (k == kts) {
+  if (k0 == kts) {
     s.mu_tend[X2(d, i, j)] = 0.f;
     s.mu_save[X2(d, i, j)] = 0.f;
     if (mass) s.mu_tend[X2(d, i, j)] = 0.f + s.mu_tendf[X2(d, i, j)];

@@ -5,6 +5,7 @@
 namespace wrfmi {
 constexpr int kAdvUvK = 8;  // levels per thread in k_advance_uv
 constexpr int kAdvK = 8;     // levels per block in k_adv_scalar
+constexpr int kTendK = 8;    // levels per thread in k_tend_u/v/w/t
 constexpr int kAdvTileI = 63;  // i points per wave in k_adv_scalar (lane 63 supplies a face)
 constexpr int kAdvRows = 4;    // j rows (waves) per block in k_adv_scalar
 __global__ void k_step_prep(KArgs);

@@ -343,10 +343,11 @@ static void ph_tendency(Launcher &L) {
   const KArgs &a = L.a;
   const int nk = a.d.kde - a.d.kds + 1;
   const bool per = a.c.periodic_x && a.c.periodic_y;
-  L.launchK(KID_TEND_U, per ? k_tend_u<true> : k_tend_u<false>, g_pt3(a.d, nk), kColBlk, a);
-  L.launchK(KID_TEND_V, per ? k_tend_v<true> : k_tend_v<false>, g_pt3(a.d, nk), kColBlk, a);
-  L.launchK(KID_TEND_W, per ? k_tend_w<true> : k_tend_w<false>, g_pt3(a.d, nk), kColBlk, a);
-  L.launchK(KID_TEND_T, per ? k_tend_t<true> : k_tend_t<false>, g_pt3(a.d, nk), kColBlk, a);
+  const dim3 gt = g_pt3(a.d, (nk + kTendK - 1) / kTendK);
+  L.launchK(KID_TEND_U, per ? k_tend_u<true> : k_tend_u<false>, gt, kColBlk, a);
+  L.launchK(KID_TEND_V, per ? k_tend_v<true> : k_tend_v<false>, gt, kColBlk, a);
+  L.launchK(KID_TEND_W, per ? k_tend_w<true> : k_tend_w<false>, gt, kColBlk, a);
+  L.launchK(KID_TEND_T, per ? k_tend_t<true> : k_tend_t<false>, gt, kColBlk, a);
 }
 static void ph_small_prep(Launcher &L) {
   const KArgs &a = L.a;
