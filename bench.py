@@ -4,11 +4,14 @@
 6 passive scalars, FP32.  Prints one JSON line.
 
     python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--no-cpu-baseline]
+                    [--ensemble-members 64] [--ensemble-steps 10]
 
 value = mass-point RK3 steps per second (millions) with the state resident in HBM.
 roofline = the kernel with the largest share of one profiled (untimed) step, timed with HIP
 events on its launch stream over the timed steps, against 8 TB/s; its algorithmic bytes are
 the 3-D arrays it must read and write once per launch.
+ensemble = member-steps/s of the 64-member uneven ensemble, member by member with the plain
+kernels and batched (one launch per kernel over all members, hipGraph replay).
 cpu_baseline = the FP32 C restatement of the same step (the oracle: one core), timed on
 a 100x100x80 sub-domain of the same workload.
 """
@@ -82,6 +85,60 @@ def cpu_baseline(seconds_target=15.0):
                       f"workload (qv + 6 scalars), {n} RK3 steps after 1 untimed, {dt:.1f} s"}
 
 
+def ensemble_leg(runtime, grid, torch, n_members, steps, log):
+    """The 64-member uneven ensemble (nx, ny from {64..256}, nz = 50, dx = 2 km, dt = 12 s):
+    member-steps/s with the plain kernels launched member by member (wrfmi_solve_em_step per
+    member) and with the batched step (one launch per kernel over all members, replayed from
+    a hipGraph: wrfmi_ensemble_step_graph)."""
+    specs = grid.ensemble_c3(n_members)
+    log(f"building the {n_members}-member ensemble ...")
+    t0 = time.perf_counter()
+    members = []
+    for sp in specs:
+        hs = grid.build_initial_state(sp)
+        members.append(runtime.DeviceDomain(hs))
+        del hs
+    log(f"  done in {time.perf_counter() - t0:.1f} s")
+    ens = runtime.DeviceEnsemble(members)
+    pts = ens.points()
+    stream = torch.cuda.current_stream()
+    for m in members:
+        m.step(1, stream=stream, sync=False)
+    ens.step(1, stream=stream, graph=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for m in members:
+            m.step(1, stream=stream, sync=False)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ens.step(steps, stream=stream, sync=False)
+    torch.cuda.synchronize()
+    t_eager = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ens.step(steps, stream=stream, graph=True, sync=False)
+    torch.cuda.synchronize()
+    t_graph = time.perf_counter() - t0
+    nbad = sum(m.check_finite() for m in members)
+    if nbad:
+        raise RuntimeError(f"ensemble: {nbad} non-finite values after the timed steps")
+    ms = lambda t: n_members * steps / t
+    out = {"members": n_members, "steps": steps, "mass_points": pts,
+           "sizes": "nx, ny from {64, 96, 128, 192, 256} (seed 1004), nz = 50, open boundaries, 6 acoustic substeps",
+           "member_steps_per_s_plain": round(ms(t_plain), 2),
+           "member_steps_per_s_batched": round(ms(t_eager), 2),
+           "member_steps_per_s_batched_graph": round(ms(t_graph), 2),
+           "mpoint_steps_per_s_batched_graph": round(pts * steps / t_graph / 1e6, 1),
+           "speedup_vs_plain": round(t_plain / t_graph, 3)}
+    ens.close()
+    del ens, members
+    torch.cuda.empty_cache()
+    log(f"  ensemble: plain {out['member_steps_per_s_plain']} member-steps/s, batched graph "
+        f"{out['member_steps_per_s_batched_graph']} ({out['speedup_vs_plain']}x)")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,6 +149,9 @@ def main():
     ap.add_argument("--nz", type=int, default=80)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--graph", action="store_true", help="also time hipGraph replay of the step (extra field)")
+    ap.add_argument("--ensemble-members", type=int, default=64,
+                    help="members of the uneven-ensemble leg (0 skips it)")
+    ap.add_argument("--ensemble-steps", type=int, default=10)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after the timed steps, copy a buffer of this many MiB once (calibrates HBM counters)")
     args = ap.parse_args()
@@ -191,6 +251,10 @@ def main():
         dev.step(args.steps, graph=True, stream=stream, sync=False)
         torch.cuda.synchronize()
         out["graph_ms_per_step"] = round(1e3 * (time.perf_counter() - t0) / args.steps, 4)
+    if args.ensemble_members > 0:
+        del dev
+        torch.cuda.empty_cache()
+        out["ensemble"] = ensemble_leg(runtime, grid, torch, args.ensemble_members, args.ensemble_steps, log)
     if not args.no_cpu_baseline:
         log("timing the CPU baseline ...")
         out["cpu_baseline"] = cpu_baseline()

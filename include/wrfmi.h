@@ -135,6 +135,25 @@ int wrfmi_solve_em_step(wrfmi_state *st, const wrfmi_config *cfg,
 int wrfmi_solve_em_step_graph(wrfmi_state *st, const wrfmi_config *cfg,
                               const wrfmi_dims *d, void *stream, int sync);
 
+/* ---- batched ensembles ---------------------------------------------------
+ * No counterpart in the reference, which advances one domain per solve_em call
+ * (ensembles run as separate executions).  An ensemble is n_members independent
+ * domains, of any sizes, each advanced by exactly the step of
+ * wrfmi_solve_em_step (bit for bit), with one launch per kernel of the step
+ * covering the tiles of every member.  states/cfgs/dims are arrays of
+ * n_members structs, copied at creation: later steps use those pointers.
+ * Members must share the boundary type, precision_mode, n_species and the
+ * number of acoustic steps (WRFMI_E_BADOPT otherwise).  Each member gets its
+ * own work arrays.  _step enqueues the step eagerly; _step_graph replays a
+ * hipGraph captured on its first call. */
+typedef struct wrfmi_ensemble wrfmi_ensemble;
+int wrfmi_ensemble_create(int n_members, const wrfmi_state *states,
+                          const wrfmi_config *cfgs, const wrfmi_dims *dims,
+                          wrfmi_ensemble **out);
+int wrfmi_ensemble_step(wrfmi_ensemble *e, void *stream, int sync);
+int wrfmi_ensemble_step_graph(wrfmi_ensemble *e, void *stream, int sync);
+int wrfmi_ensemble_destroy(wrfmi_ensemble *e);
+
 /* ---- per-routine entry points -------------------------------------------
  * The calls solve_em makes inside one RK substep, each with the halo fills the
  * reference makes right after it.  Each enqueues on `stream` and returns; the

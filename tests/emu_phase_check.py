@@ -48,9 +48,66 @@ def registry_check():
     print("OK registry", flush=True)
 
 
+def ensemble_check(order: int, nsteps: int):
+    """Batched ensemble (wrfmi_ensemble_*): members of different sizes advanced together
+    are bit for bit the members advanced one by one; mismatched members are refused."""
+    import ctypes
+
+    from wrf_amd.fields import Config, Dims, State32
+    specs = [grid.DomainSpec(nx=14, ny=11, nz=10, bc="open", n_bubbles=1, seed=21),
+             grid.DomainSpec(nx=9, ny=13, nz=10, bc="open", n_bubbles=1, bubble_amp=1.5, seed=22),
+             grid.DomainSpec(nx=12, ny=9, nz=12, bc="open", n_bubbles=1, bubble_amp=3.5, seed=23)]
+    hss = [grid.build_initial_state(s) for s in specs]
+    singles = []
+    for hs in hss:
+        e = EmuDomain(hs, order=order)
+        e.step(nsteps)
+        singles.append(e.download())
+        del e
+    members = [EmuDomain(hs, order=order) for hs in hss]
+    lib = members[0].lib
+    vp = ctypes.c_void_p
+    lib.wrfmi_ensemble_create.argtypes = [ctypes.c_int, vp, vp, vp, ctypes.POINTER(vp)]
+    lib.wrfmi_ensemble_step.argtypes = [vp, vp, ctypes.c_int]
+    lib.wrfmi_ensemble_destroy.argtypes = [vp]
+    n = len(members)
+    states = (State32 * n)(*[m.state for m in members])
+    cfgs = (Config * n)(*[m.cfg for m in members])
+    dims = (Dims * n)(*[m.dims for m in members])
+    ens = vp()
+    rc = lib.wrfmi_ensemble_create(n, ctypes.byref(states), ctypes.byref(cfgs), ctypes.byref(dims), ctypes.byref(ens))
+    assert rc == 0, lib.wrfmi_last_error().decode()
+    for _ in range(nsteps):
+        assert lib.wrfmi_ensemble_step(ens, None, 1) == 0, lib.wrfmi_last_error().decode()
+    fails = []
+    for m, (mem, exp) in enumerate(zip(members, singles)):
+        got = mem.download()
+        bad = [k for k in exp if not np.array_equal(got[k], exp[k])]
+        if bad:
+            fails.append(f"member {m} differs from its single-domain step in {bad}")
+    assert lib.wrfmi_ensemble_destroy(ens) == 0
+    assert lib.wrfmi_ensemble_destroy(ens) == 4  # no longer known: WRFMI_E_POINTER
+    # a periodic member among open ones runs other kernels: refused
+    hp = grid.build_initial_state(grid.DomainSpec(nx=10, ny=10, nz=10, bc="periodic", n_bubbles=1, seed=24))
+    mp = EmuDomain(hp, order=order)
+    states2 = (State32 * 2)(members[0].state, mp.state)
+    cfgs2 = (Config * 2)(members[0].cfg, mp.cfg)
+    dims2 = (Dims * 2)(members[0].dims, mp.dims)
+    rc = lib.wrfmi_ensemble_create(2, ctypes.byref(states2), ctypes.byref(cfgs2), ctypes.byref(dims2), ctypes.byref(ens))
+    if rc != 1:
+        fails.append(f"mismatched members: expected WRFMI_E_BADOPT, got {rc}")
+    for f in fails:
+        print("MISMATCH", f)
+    print("OK ensemble" if not fails else "FAILED ensemble", "order", order, flush=True)
+    sys.exit(1 if fails else 0)
+
+
 def main():
     if sys.argv[1] == "registry":
         registry_check()
+        return
+    if sys.argv[1] == "ensemble":
+        ensemble_check(int(sys.argv[2]), int(sys.argv[3]))
         return
     case, order, nsteps = sys.argv[1], int(sys.argv[2]), int(sys.argv[3])
     spec = grid.DomainSpec(**CASES[case])

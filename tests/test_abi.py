@@ -36,7 +36,18 @@ def test_struct_layouts_match():
     assert lib.wrfmi_state_field_count() == fields.state_field_count()
     assert lib.wrfmi_version() == b"1.0"
     names = [lib.wrfmi_kernel_name(k).decode() for k in range(lib.wrfmi_kernel_count())]
-    assert "k_acoustic_col" in names and "k_halo" in names
+    assert {"k_advance_uv", "k_ac_wsolve", "k_adv_scalar", "k_halo"} <= set(names)
+
+
+def test_ensemble_arguments_checked_before_the_device():
+    lib = runtime.load_library()
+    out = ctypes.c_void_p()
+    st, cfg, dims = fields.State32(), fields.Config(), fields.Dims()
+    rc = lib.wrfmi_ensemble_create(0, ctypes.byref(st), ctypes.byref(cfg), ctypes.byref(dims), ctypes.byref(out))
+    assert rc == runtime.WRFMI_E_BADOPT
+    rc = lib.wrfmi_ensemble_create(1, None, ctypes.byref(cfg), ctypes.byref(dims), ctypes.byref(out))
+    assert rc == runtime.WRFMI_E_POINTER
+    assert lib.wrfmi_ensemble_step(ctypes.c_void_p(1234), None, 0) == runtime.WRFMI_E_POINTER
 
 
 @pytest.mark.parametrize("field,value,msg", [

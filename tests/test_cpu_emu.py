@@ -38,3 +38,11 @@ def test_emulated_host_array_registry(emu_env):
                        env=emu_env, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout[-4000:] + r.stderr[-4000:]
     assert "OK registry" in r.stdout
+
+
+@pytest.mark.parametrize("order", [0, 1])
+def test_emulated_ensemble_matches_single_domains(emu_env, order):
+    r = subprocess.run([sys.executable, os.path.join(HERE, "emu_phase_check.py"), "ensemble", str(order), "2"],
+                       env=emu_env, capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stdout[-4000:] + r.stderr[-4000:]
+    assert "OK ensemble" in r.stdout

@@ -95,6 +95,18 @@ struct HaloArgs {
   int periodic_x, periodic_y, open_xs, open_xe, open_ys, open_ye;
 };
 
+// Ensemble launches: one launch covers the workgroups of every member.  m[] holds each
+// member's arguments, off[] the prefix sums of the members' workgroup counts (n + 1
+// entries) and lut[] the member of every logical workgroup (total entries).
+template <typename T>
+struct Batch {
+  const T *m;
+  const unsigned *off;
+  const unsigned *lut;
+  int n;
+  unsigned total;
+};
+
 #define DEVI __device__ __forceinline__
 
 // XCD-aware tile order.  Kernels are launched on a 1-D grid of tnx*tny*tnz workgroups and
@@ -105,16 +117,21 @@ struct Tile3 {
   int bx, by, bz;
 };
 constexpr unsigned kXCD = 8;
-DEVI Tile3 tile3_of(unsigned b, int tnx, int tny, int tnz) {
-  const unsigned n = (unsigned)tnx * (unsigned)tny * (unsigned)tnz;
+// logical index of workgroup b among n, contiguous per XCD
+DEVI unsigned xcd_logical(unsigned b, unsigned n) {
   const unsigned x = b % kXCD, r = b / kXCD;
-  const unsigned L = x * (n / kXCD) + (x < n % kXCD ? x : n % kXCD) + r;
+  return x * (n / kXCD) + (x < n % kXCD ? x : n % kXCD) + r;
+}
+DEVI Tile3 tile_of_logical(unsigned L, int tnx, int tnz) {
   Tile3 t;
   t.bz = (int)(L % (unsigned)tnz);
   const unsigned q = L / (unsigned)tnz;
   t.bx = (int)(q % (unsigned)tnx);
   t.by = (int)(q / (unsigned)tnx);
   return t;
+}
+DEVI Tile3 tile3_of(unsigned b, int tnx, int tny, int tnz) {
+  return tile_of_logical(xcd_logical(b, (unsigned)tnx * (unsigned)tny * (unsigned)tnz), tnx, tnz);
 }
 
 DEVI int X3i(const Dm &d, int i, int k, int j) { return (i - d.ims) + d.ni * (k - d.kms) + d.sj * (j - d.jms); }
@@ -125,6 +142,33 @@ DEVI Off X1(const Dm &d, int k) { return Off{4u * (unsigned)(k - d.kms)}; }
 DEVI const DState &dstate(const KArgs &a) { return *reinterpret_cast<const DState *>(&a.s); }
 
 DEVI Tile3 tile3(const KArgs &a) { return tile3_of(blockIdx.x, a.tnx, a.tny, a.tnz); }
+
+typedef Batch<KArgs> BatchArgs;
+typedef Batch<HaloArgs> BatchHalo;
+
+DEVI const KArgs &kargs(const KArgs &a, Tile3 &T) {
+  T = tile3(a);
+  return a;
+}
+// Halo launches: the arguments and the workgroup index within the member
+DEVI const HaloArgs &hargs(const HaloArgs &h, unsigned &blk) {
+  blk = blockIdx.x;
+  return h;
+}
+DEVI const HaloArgs &hargs(const BatchHalo &b, unsigned &blk) {
+  const unsigned m = b.lut[blockIdx.x];
+  blk = blockIdx.x - b.off[m];
+  return b.m[m];
+}
+// A member of an ensemble launch: the XCD-aware order runs over the concatenated tiles of
+// all members, so each XCD takes a contiguous run of members.
+DEVI const KArgs &kargs(const BatchArgs &b, Tile3 &T) {
+  const unsigned L = xcd_logical(blockIdx.x, b.total);
+  const unsigned m = b.lut[L];
+  const KArgs &a = b.m[m];
+  T = tile_of_logical(L - b.off[m], a.tnx, a.tnz);
+  return a;
+}
 
 // Lateral-boundary flags of the whole-patch tile (its = ids, ite = ide, ...).  The host
 // accepts periodic or open boundaries only, so symmetric, specified, nested and polar are

@@ -38,8 +38,10 @@ DEVI float prep_muv(const KArgs &a, int i, int j) {
 // Point part: one thread per (i, k, j) with i = ids-1..ide, j = jds-1..jde (mut is also
 // needed one point into the halo, as calculate_full computes it there); the thread of
 // level 1 writes the 2-D fields.
-__global__ void __launch_bounds__(256) k_step_prep(KArgs a) {
-  const Tile3 T = tile3(a);
+template <typename ARGS>
+__global__ void __launch_bounds__(256) k_step_prep(ARGS args) {
+  Tile3 T;
+  const KArgs &a = kargs(args, T);
   const Dm &d = a.d;
   const DState &s = dstate(a);
   const int i = d.ids - 1 + (int)(T.bx * blockDim.x + threadIdx.x);
@@ -112,9 +114,10 @@ __global__ void __launch_bounds__(256) k_step_prep(KArgs a) {
 
 // calc_ww_cp (module_big_step_utilities_em.F:640-782): column divergence of the coupled
 // horizontal momentum and the upward scan for omega; one thread per (i, j) column.
-template <bool COMP>
-__global__ void __launch_bounds__(256) k_ww_scan(KArgs a) {
-  const Tile3 T = tile3(a);
+template <typename ARGS, bool COMP>
+__global__ void __launch_bounds__(256) k_ww_scan(ARGS args) {
+  Tile3 T;
+  const KArgs &a = kargs(args, T);
   const Dm &d = a.d;
   const DState &s = dstate(a);
   const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
@@ -163,7 +166,9 @@ __global__ void __launch_bounds__(256) k_ww_scan(KArgs a) {
   }
 }
 
-template __global__ void k_ww_scan<false>(KArgs);
-template __global__ void k_ww_scan<true>(KArgs);
+WRFMI_INST_P(k_ww_scan, false)
+WRFMI_INST_P(k_ww_scan, true)
+
+WRFMI_INST(k_step_prep)
 
 }  // namespace wrfmi

@@ -15,9 +15,10 @@ namespace wrfmi {
 // computes the y flux at its face j and reads face j+1 from the row above through LDS (the
 // top row also computes face j+4); the vertical flux is carried up the column.  Every flux
 // is the same expression, in the same order, as in advect_scalar.
-template <bool PER>
-__global__ void __launch_bounds__(256) k_adv_scalar(KArgs a) {
-  const Tile3 T = tile3(a);
+template <typename ARGS, bool PER>
+__global__ void __launch_bounds__(256) k_adv_scalar(ARGS args) {
+  Tile3 T;
+  const KArgs &a = kargs(args, T);
   const Dm &d = a.d;
   const DState &s = dstate(a);
   const int lane = (int)threadIdx.x, row = (int)threadIdx.y;
@@ -130,8 +131,10 @@ __global__ void __launch_bounds__(256) k_adv_scalar(KArgs a) {
   }
 }
 
-__global__ void __launch_bounds__(256) k_update_scalar(KArgs a) {
-  const Tile3 T = tile3(a);
+template <typename ARGS>
+__global__ void __launch_bounds__(256) k_update_scalar(ARGS args) {
+  Tile3 T;
+  const KArgs &a = kargs(args, T);
   const Dm &d = a.d;
   const DState &s = dstate(a);
   const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
@@ -156,8 +159,10 @@ __global__ void __launch_bounds__(256) k_update_scalar(KArgs a) {
 }
 
 // calc_p_rho_phi, non-hydrostatic; logarithm and power in FP64 then rounded once to FP32
-__global__ void __launch_bounds__(256) k_p_rho_phi(KArgs a) {
-  const Tile3 T = tile3(a);
+template <typename ARGS>
+__global__ void __launch_bounds__(256) k_p_rho_phi(ARGS args) {
+  Tile3 T;
+  const KArgs &a = kargs(args, T);
   const Dm &d = a.d;
   const DState &s = dstate(a);
   const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
@@ -203,12 +208,15 @@ __global__ void __launch_bounds__(256) k_p_rho_phi(KArgs a) {
 // One thread per (halo cell, level, field).  The halo ring of the memory patch is
 // enumerated as: the rows below jds and from jde up (every i), then the columns left of ids
 // and from ide on for the rows jds..jde-1.
-__global__ void __launch_bounds__(256) k_halo(HaloArgs h) {
+template <typename ARGS>
+__global__ void __launch_bounds__(256) k_halo(ARGS args) {
+  unsigned blk;
+  const HaloArgs &h = hargs(args, blk);
   const Dm &d = h.d;
   const unsigned W = (unsigned)d.ni, B = (unsigned)(d.jds - d.jms), Tn = (unsigned)(d.jme - d.jde + 1);
   const unsigned Lw = (unsigned)(d.ids - d.ims), LR = Lw + (unsigned)(d.ime - d.ide + 1);
   const unsigned nring = (B + Tn) * W + (unsigned)(d.jde - d.jds) * LR, nk = (unsigned)d.kde;
-  const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+  const unsigned t = blk * blockDim.x + threadIdx.x;
   if (t >= nring * nk * (unsigned)h.n) return;
   const unsigned fidx = t / (nring * nk), rem = t % (nring * nk);
   const int k = d.kds + (int)(rem / nring);
@@ -278,7 +286,12 @@ __global__ void __launch_bounds__(256) k_halo(HaloArgs h) {
   else F.p[X2i(d, i, j)] = F.p[X2i(d, si, sj)];
 }
 
-template __global__ void k_adv_scalar<false>(KArgs);
-template __global__ void k_adv_scalar<true>(KArgs);
+WRFMI_INST_P(k_adv_scalar, false)
+WRFMI_INST_P(k_adv_scalar, true)
+
+WRFMI_INST(k_update_scalar)
+WRFMI_INST(k_p_rho_phi)
+template __global__ void k_halo<HaloArgs>(HaloArgs);
+template __global__ void k_halo<BatchHalo>(BatchHalo);
 
 }  // namespace wrfmi

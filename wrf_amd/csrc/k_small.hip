@@ -39,8 +39,10 @@ DEVI MuS small_mu(const KArgs &a, int i, int j) {
   return m;
 }
 
-__global__ void __launch_bounds__(256) k_small_prep(KArgs a) {
-  const Tile3 T = tile3(a);
+template <typename ARGS>
+__global__ void __launch_bounds__(256) k_small_prep(ARGS args) {
+  Tile3 T;
+  const KArgs &a = kargs(args, T);
   const Dm &d = a.d;
   const DState &s = dstate(a);
   const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
@@ -91,8 +93,10 @@ __global__ void __launch_bounds__(256) k_small_prep(KArgs a) {
 }
 
 // calc_p_rho at step 0 (mu = the new mu_2, mut = muts, t_1 = t_save): one thread per point
-__global__ void __launch_bounds__(256) k_p_rho0(KArgs a) {
-  const Tile3 T = tile3(a);
+template <typename ARGS>
+__global__ void __launch_bounds__(256) k_p_rho0(ARGS args) {
+  Tile3 T;
+  const KArgs &a = kargs(args, T);
   const Dm &d = a.d;
   const DState &s = dstate(a);
   const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
@@ -114,9 +118,10 @@ __global__ void __launch_bounds__(256) k_p_rho0(KArgs a) {
 }
 
 // small_step_prep's 2-D updates and calc_coef_w: one thread per (i, j) column
-template <typename CT>
-__global__ void __launch_bounds__(256) k_coef_w(KArgs a) {
-  const Tile3 T = tile3(a);
+template <typename ARGS, typename CT>
+__global__ void __launch_bounds__(256) k_coef_w(ARGS args) {
+  Tile3 T;
+  const KArgs &a = kargs(args, T);
   const Dm &d = a.d;
   const DState &s = dstate(a);
   const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
@@ -183,9 +188,10 @@ __global__ void __launch_bounds__(256) k_coef_w(KArgs a) {
 // advance_uv, one thread per (i, j) column segment of kAdvUvK levels: the column constants
 // (map-factor ratios, column masses, the mudf term) are formed once, and the vertically
 // interpolated pressure dpn at the lower w level is carried up from the level below.
-template <bool PER>
-__global__ void __launch_bounds__(256) k_advance_uv(KArgs a) {
-  const Tile3 T = tile3(a);
+template <typename ARGS, bool PER>
+__global__ void __launch_bounds__(256) k_advance_uv(ARGS args) {
+  Tile3 T;
+  const KArgs &a = kargs(args, T);
   const Dm &d = a.d;
   const DState &s = dstate(a);
   const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
@@ -339,8 +345,10 @@ DEVI void sumflux_uv(const KArgs &a, const DState &s, int i, int j, int k, Off o
 
 // dvdxi of advance_mu_t (module_small_step_em.F:1092-1099) at mass points, and the x/y parts
 // of sumflux; one thread per (i, k, j) with i <= ide, j <= jde, k <= kde
-__global__ void __launch_bounds__(256) k_ac_div(KArgs a) {
-  const Tile3 T = tile3(a);
+template <typename ARGS>
+__global__ void __launch_bounds__(256) k_ac_div(ARGS args) {
+  Tile3 T;
+  const KArgs &a = kargs(args, T);
   const Dm &d = a.d;
   const DState &s = dstate(a);
   const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
@@ -362,9 +370,10 @@ __global__ void __launch_bounds__(256) k_ac_div(KArgs a) {
 }
 
 // advance_mu_t's column integral, mass update and omega scan (module_small_step_em.F:1100-1121)
-template <bool COMP>
-__global__ void __launch_bounds__(256) k_ac_mu_ww(KArgs a) {
-  const Tile3 T = tile3(a);
+template <typename ARGS, bool COMP>
+__global__ void __launch_bounds__(256) k_ac_mu_ww(ARGS args) {
+  Tile3 T;
+  const KArgs &a = kargs(args, T);
   const Dm &d = a.d;
   const DState &s = dstate(a);
   const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
@@ -407,8 +416,10 @@ __global__ void __launch_bounds__(256) k_ac_mu_ww(KArgs a) {
 }
 
 // theta update of advance_mu_t (1125-1174), t_2ave and rhs of advance_w (1312-1371)
-__global__ void __launch_bounds__(256) k_ac_theta(KArgs a) {
-  const Tile3 T = tile3(a);
+template <typename ARGS>
+__global__ void __launch_bounds__(256) k_ac_theta(ARGS args) {
+  Tile3 T;
+  const KArgs &a = kargs(args, T);
   const Dm &d = a.d;
   const DState &s = dstate(a);
   const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
@@ -473,8 +484,10 @@ __global__ void __launch_bounds__(256) k_ac_theta(KArgs a) {
 }
 
 // surface w (advance_w 1383-1395) and the explicit w predictor (1402-1431), k = 1..kde
-__global__ void __launch_bounds__(256) k_ac_wpred(KArgs a) {
-  const Tile3 T = tile3(a);
+template <typename ARGS>
+__global__ void __launch_bounds__(256) k_ac_wpred(ARGS args) {
+  Tile3 T;
+  const KArgs &a = kargs(args, T);
   const Dm &d = a.d;
   const DState &s = dstate(a);
   const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
@@ -532,9 +545,10 @@ __global__ void __launch_bounds__(256) k_ac_wpred(KArgs a) {
 // Per column: the tridiagonal forward elimination and back substitution (advance_w
 // 1433-1443), ph from rhs and the final w (1460-1465) and calc_p_rho for step >= 1
 // (438-568).  CT = double keeps the forward value in FP64, its low-order part parked in scr_lo.
-template <typename CT>
-__global__ void __launch_bounds__(256) k_ac_wsolve(KArgs a) {
-  const Tile3 T = tile3(a);
+template <typename ARGS, typename CT>
+__global__ void __launch_bounds__(256) k_ac_wsolve(ARGS args) {
+  Tile3 T;
+  const KArgs &a = kargs(args, T);
   const Dm &d = a.d;
   const DState &s = dstate(a);
   const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
@@ -629,8 +643,10 @@ __global__ void __launch_bounds__(256) k_ac_wsolve(KArgs a) {
 }
 
 // One thread per (i, k, j); the thread of level kds also writes the 2-D fields.
-__global__ void __launch_bounds__(256) k_small_finish(KArgs a) {
-  const Tile3 T = tile3(a);
+template <typename ARGS>
+__global__ void __launch_bounds__(256) k_small_finish(ARGS args) {
+  Tile3 T;
+  const KArgs &a = kargs(args, T);
   const Dm &d = a.d;
   const DState &s = dstate(a);
   const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
@@ -683,13 +699,20 @@ __global__ void __launch_bounds__(256) k_small_finish(KArgs a) {
   if (top) s.mu_2[o2] = s.mu_2[o2] + s.mu_save[o2];
 }
 
-template __global__ void k_advance_uv<false>(KArgs);
-template __global__ void k_advance_uv<true>(KArgs);
-template __global__ void k_coef_w<float>(KArgs);
-template __global__ void k_coef_w<double>(KArgs);
-template __global__ void k_ac_mu_ww<false>(KArgs);
-template __global__ void k_ac_mu_ww<true>(KArgs);
-template __global__ void k_ac_wsolve<float>(KArgs);
-template __global__ void k_ac_wsolve<double>(KArgs);
+WRFMI_INST_P(k_advance_uv, false)
+WRFMI_INST_P(k_advance_uv, true)
+WRFMI_INST_P(k_coef_w, float)
+WRFMI_INST_P(k_coef_w, double)
+WRFMI_INST_P(k_ac_mu_ww, false)
+WRFMI_INST_P(k_ac_mu_ww, true)
+WRFMI_INST_P(k_ac_wsolve, float)
+WRFMI_INST_P(k_ac_wsolve, double)
+
+WRFMI_INST(k_small_prep)
+WRFMI_INST(k_p_rho0)
+WRFMI_INST(k_ac_div)
+WRFMI_INST(k_ac_theta)
+WRFMI_INST(k_ac_wpred)
+WRFMI_INST(k_small_finish)
 
 }  // namespace wrfmi

@@ -17,9 +17,10 @@
 namespace wrfmi {
 
 // ------------------------------------------------------------------------------ u
-template <bool PER>
-__global__ void __launch_bounds__(256) k_tend_u(KArgs a) {
-  const Tile3 T = tile3(a);
+template <typename ARGS, bool PER>
+__global__ void __launch_bounds__(256) k_tend_u(ARGS args) {
+  Tile3 T;
+  const KArgs &a = kargs(args, T);
   const Dm &d = a.d;
   const DState &s = dstate(a);
   const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
@@ -242,9 +243,10 @@ __global__ void __launch_bounds__(256) k_tend_u(KArgs a) {
 }
 
 // ------------------------------------------------------------------------------ v
-template <bool PER>
-__global__ void __launch_bounds__(256) k_tend_v(KArgs a) {
-  const Tile3 T = tile3(a);
+template <typename ARGS, bool PER>
+__global__ void __launch_bounds__(256) k_tend_v(ARGS args) {
+  Tile3 T;
+  const KArgs &a = kargs(args, T);
   const Dm &d = a.d;
   const DState &s = dstate(a);
   const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
@@ -447,9 +449,10 @@ __global__ void __launch_bounds__(256) k_tend_v(KArgs a) {
 }
 
 // ------------------------------------------------------------------------------ w
-template <bool PER>
-__global__ void __launch_bounds__(256) k_tend_w(KArgs a) {
-  const Tile3 T = tile3(a);
+template <typename ARGS, bool PER>
+__global__ void __launch_bounds__(256) k_tend_w(ARGS args) {
+  Tile3 T;
+  const KArgs &a = kargs(args, T);
   const Dm &d = a.d;
   const DState &s = dstate(a);
   const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
@@ -644,9 +647,10 @@ __global__ void __launch_bounds__(256) k_tend_w(KArgs a) {
 }
 
 // ------------------------------------------------------------------------------ t, ph, zeroing
-template <bool PER>
-__global__ void __launch_bounds__(256) k_tend_t(KArgs a) {
-  const Tile3 T = tile3(a);
+template <typename ARGS, bool PER>
+__global__ void __launch_bounds__(256) k_tend_t(ARGS args) {
+  Tile3 T;
+  const KArgs &a = kargs(args, T);
   const Dm &d = a.d;
   const DState &s = dstate(a);
   const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
@@ -878,13 +882,13 @@ __global__ void __launch_bounds__(256) k_tend_t(KArgs a) {
   }
 }
 
-template __global__ void k_tend_u<false>(KArgs);
-template __global__ void k_tend_u<true>(KArgs);
-template __global__ void k_tend_v<false>(KArgs);
-template __global__ void k_tend_v<true>(KArgs);
-template __global__ void k_tend_w<false>(KArgs);
-template __global__ void k_tend_w<true>(KArgs);
-template __global__ void k_tend_t<false>(KArgs);
-template __global__ void k_tend_t<true>(KArgs);
+WRFMI_INST_P(k_tend_u, false)
+WRFMI_INST_P(k_tend_u, true)
+WRFMI_INST_P(k_tend_v, false)
+WRFMI_INST_P(k_tend_v, true)
+WRFMI_INST_P(k_tend_w, false)
+WRFMI_INST_P(k_tend_w, true)
+WRFMI_INST_P(k_tend_t, false)
+WRFMI_INST_P(k_tend_t, true)
 
 }  // namespace wrfmi

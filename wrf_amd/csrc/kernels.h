@@ -8,25 +8,35 @@ constexpr int kAdvK = 8;     // levels per block in k_adv_scalar
 constexpr int kTendK = 8;    // levels per thread in k_tend_u/v/w/t
 constexpr int kAdvTileI = 63;  // i points per wave in k_adv_scalar (lane 63 supplies a face)
 constexpr int kAdvRows = 4;    // j rows (waves) per block in k_adv_scalar
-__global__ void k_step_prep(KArgs);
-template <bool COMP> __global__ void k_ww_scan(KArgs);
-template <bool PER> __global__ void k_tend_u(KArgs);
-template <bool PER> __global__ void k_tend_v(KArgs);
-template <bool PER> __global__ void k_tend_w(KArgs);
-template <bool PER> __global__ void k_tend_t(KArgs);
-__global__ void k_small_prep(KArgs);
-__global__ void k_p_rho0(KArgs);
-template <typename CT> __global__ void k_coef_w(KArgs);
-template <bool PER> __global__ void k_advance_uv(KArgs);
-__global__ void k_ac_div(KArgs);
-template <bool COMP> __global__ void k_ac_mu_ww(KArgs);
-__global__ void k_ac_theta(KArgs);
-__global__ void k_ac_wpred(KArgs);
-template <typename CT> __global__ void k_ac_wsolve(KArgs);
-__global__ void k_small_finish(KArgs);
-template <bool PER> __global__ void k_adv_scalar(KArgs);
-__global__ void k_update_scalar(KArgs);
-__global__ void k_p_rho_phi(KArgs);
-__global__ void k_halo(HaloArgs);
+// Every state kernel is a template on its argument type: KArgs for one domain, BatchArgs
+// for an ensemble launch over the tiles of all members (k_halo: HaloArgs / BatchHalo).
+template <typename A> __global__ void k_step_prep(A);
+template <typename A, bool COMP> __global__ void k_ww_scan(A);
+template <typename A, bool PER> __global__ void k_tend_u(A);
+template <typename A, bool PER> __global__ void k_tend_v(A);
+template <typename A, bool PER> __global__ void k_tend_w(A);
+template <typename A, bool PER> __global__ void k_tend_t(A);
+template <typename A> __global__ void k_small_prep(A);
+template <typename A> __global__ void k_p_rho0(A);
+template <typename A, typename CT> __global__ void k_coef_w(A);
+template <typename A, bool PER> __global__ void k_advance_uv(A);
+template <typename A> __global__ void k_ac_div(A);
+template <typename A, bool COMP> __global__ void k_ac_mu_ww(A);
+template <typename A> __global__ void k_ac_theta(A);
+template <typename A> __global__ void k_ac_wpred(A);
+template <typename A, typename CT> __global__ void k_ac_wsolve(A);
+template <typename A> __global__ void k_small_finish(A);
+template <typename A, bool PER> __global__ void k_adv_scalar(A);
+template <typename A> __global__ void k_update_scalar(A);
+template <typename A> __global__ void k_p_rho_phi(A);
+template <typename A> __global__ void k_halo(A);
 __global__ void k_count_nonfinite(const float *a, Dm d, int kend, int is3d, unsigned long long *cnt);
 }  // namespace wrfmi
+
+// explicit instantiations, in the source file that defines the kernel
+#define WRFMI_INST(K) \
+  template __global__ void K<KArgs>(KArgs); \
+  template __global__ void K<BatchArgs>(BatchArgs);
+#define WRFMI_INST_P(K, P) \
+  template __global__ void K<KArgs, P>(KArgs); \
+  template __global__ void K<BatchArgs, P>(BatchArgs);

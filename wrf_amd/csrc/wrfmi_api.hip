@@ -148,9 +148,11 @@ extern "C" int wrfmi_init(void) {
 struct GraphEntry;
 static std::vector<GraphEntry *> g_graphs;
 static void free_graphs();
+static void free_ensembles();
 extern "C" int wrfmi_finalize(void) {
   std::lock_guard<std::mutex> lk(g_mu);
   free_graphs();
+  free_ensembles();
   free_scratch();
   for (auto &kv : g_reg) (void)hipFree(kv.second.dev);
   g_reg.clear();
@@ -263,11 +265,35 @@ static hipEvent_t ev_get() {
   return e;
 }
 
+// The two compiled forms of a state kernel: one domain, and an ensemble launch.
+struct KRef {
+  void (*one)(KArgs);
+  void (*batch)(BatchArgs);
+};
+#define KR(K) KRef{K<KArgs>, K<BatchArgs>}
+#define KRP(K, P) KRef{K<KArgs, P>, K<BatchArgs, P>}
+
+// One launch of an ensemble member, as recorded for a batched launch (wrfmi_ensemble_*).
+struct RecOp {
+  int id;
+  bool halo;
+  void *kern;      // the BatchArgs / BatchHalo form
+  dim3 blk;
+  unsigned wgs;    // workgroups of this member
+  size_t arg;      // index into RecMember::k or ::h
+};
+struct RecMember {
+  std::vector<RecOp> ops;
+  std::vector<KArgs> k;
+  std::vector<HaloArgs> h;
+};
+
 struct Launcher {
   hipStream_t st;
   KArgs a;
   HaloArgs h;
   int err = 0;
+  RecMember *rec = nullptr;  // set: record the launches instead of enqueueing them
   // launch with an optional event pair around it (kernel timing mode)
   template <typename K, typename... A>
   void launch(int id, K kern, dim3 g, dim3 b, A... args) {
@@ -283,12 +309,18 @@ struct Launcher {
       g_ev_pending.push_back(EvPair{id, e0, e1});
     }
   }
-  // a KArgs kernel on a tnx x tny x tnz tile grid, launched 1-D for the XCD-aware order
-  void launchK(int id, void (*kern)(KArgs), dim3 g, dim3 b, KArgs args) {
+  // a state kernel on a tnx x tny x tnz tile grid, launched 1-D for the XCD-aware order
+  void launchK(int id, KRef kern, dim3 g, dim3 b, KArgs args) {
     args.tnx = (int)g.x;
     args.tny = (int)g.y;
     args.tnz = (int)g.z;
-    launch(id, kern, dim3(g.x * g.y * g.z, 1, 1), b, args);
+    const unsigned n = g.x * g.y * g.z;
+    if (rec) {
+      rec->ops.push_back(RecOp{id, false, (void *)kern.batch, b, n, rec->k.size()});
+      rec->k.push_back(args);
+      return;
+    }
+    launch(id, kern.one, dim3(n, 1, 1), b, args);
   }
   void halo_begin() { h.n = 0; }
   void halo_add(float *p, char kind, int is3d) {
@@ -298,11 +330,17 @@ struct Launcher {
   }
   void halo_flush() {
     if (h.n == 0) return;
-    const Dm &d = a.d;
+    const Dm &d = h.d;
     const long ring = (long)(d.jds - d.jms + d.jme - d.jde + 1) * d.ni +
                       (long)(d.jde - d.jds) * (d.ids - d.ims + d.ime - d.ide + 1);
     const long nthreads = ring * d.kde * h.n;
-    launch(KID_HALO, k_halo, dim3((unsigned)((nthreads + 255) / 256), 1, 1), dim3(256, 1, 1), h);
+    const unsigned wgs = (unsigned)((nthreads + 255) / 256);
+    if (rec) {
+      rec->ops.push_back(RecOp{KID_HALO, true, (void *)k_halo<BatchHalo>, dim3(256, 1, 1), wgs, rec->h.size()});
+      rec->h.push_back(h);
+    } else {
+      launch(KID_HALO, k_halo<HaloArgs>, dim3(wgs, 1, 1), dim3(256, 1, 1), h);
+    }
     h.n = 0;
   }
 };
@@ -329,9 +367,9 @@ static void ph_step_prep(Launcher &L) {
   const KArgs &a = L.a;
   const wrfmi_state &s = a.s;
   const dim3 gp = g_prep(a.d);
-  L.launchK(KID_STEP_PREP, k_step_prep, dim3(gp.x, gp.y, a.d.kde - a.d.kds + 1), kColBlk, a);
-  if (a.c.precision_mode == 1) L.launchK(KID_WW_SCAN, k_ww_scan<true>, g_col(a.d), kColBlk, a);
-  else L.launchK(KID_WW_SCAN, k_ww_scan<false>, g_col(a.d), kColBlk, a);
+  L.launchK(KID_STEP_PREP, KR(k_step_prep), dim3(gp.x, gp.y, a.d.kde - a.d.kds + 1), kColBlk, a);
+  if (a.c.precision_mode == 1) L.launchK(KID_WW_SCAN, KRP(k_ww_scan, true), g_col(a.d), kColBlk, a);
+  else L.launchK(KID_WW_SCAN, KRP(k_ww_scan, false), g_col(a.d), kColBlk, a);
   L.halo_begin();
   L.halo_add(s.ru, 'u', 1); L.halo_add(s.rv, 'v', 1); L.halo_add(s.rw, 'w', 1); L.halo_add(s.ww, 'w', 1);
   L.halo_add(s.php, 'w', 1); L.halo_add(s.alt, 't', 1); L.halo_add(s.p, 'p', 1);
@@ -344,19 +382,19 @@ static void ph_tendency(Launcher &L) {
   const int nk = a.d.kde - a.d.kds + 1;
   const bool per = a.c.periodic_x && a.c.periodic_y;
   const dim3 gt = g_pt3(a.d, (nk + kTendK - 1) / kTendK);
-  L.launchK(KID_TEND_U, per ? k_tend_u<true> : k_tend_u<false>, gt, kColBlk, a);
-  L.launchK(KID_TEND_V, per ? k_tend_v<true> : k_tend_v<false>, gt, kColBlk, a);
-  L.launchK(KID_TEND_W, per ? k_tend_w<true> : k_tend_w<false>, gt, kColBlk, a);
-  L.launchK(KID_TEND_T, per ? k_tend_t<true> : k_tend_t<false>, gt, kColBlk, a);
+  L.launchK(KID_TEND_U, per ? KRP(k_tend_u, true) : KRP(k_tend_u, false), gt, kColBlk, a);
+  L.launchK(KID_TEND_V, per ? KRP(k_tend_v, true) : KRP(k_tend_v, false), gt, kColBlk, a);
+  L.launchK(KID_TEND_W, per ? KRP(k_tend_w, true) : KRP(k_tend_w, false), gt, kColBlk, a);
+  L.launchK(KID_TEND_T, per ? KRP(k_tend_t, true) : KRP(k_tend_t, false), gt, kColBlk, a);
 }
 static void ph_small_prep(Launcher &L) {
   const KArgs &a = L.a;
   const wrfmi_state &s = a.s;
   const int nk = a.d.kde - a.d.kds + 1;
-  L.launchK(KID_SMALL_PREP, k_small_prep, g_pt3(a.d, nk), kColBlk, a);
-  L.launchK(KID_P_RHO0, k_p_rho0, g_pt3(a.d, nk - 1), kColBlk, a);
-  if (a.c.precision_mode == 1) L.launchK(KID_COEF_W, k_coef_w<double>, g_col(a.d), kColBlk, a);
-  else L.launchK(KID_COEF_W, k_coef_w<float>, g_col(a.d), kColBlk, a);
+  L.launchK(KID_SMALL_PREP, KR(k_small_prep), g_pt3(a.d, nk), kColBlk, a);
+  L.launchK(KID_P_RHO0, KR(k_p_rho0), g_pt3(a.d, nk - 1), kColBlk, a);
+  if (a.c.precision_mode == 1) L.launchK(KID_COEF_W, KRP(k_coef_w, double), g_col(a.d), kColBlk, a);
+  else L.launchK(KID_COEF_W, KRP(k_coef_w, float), g_col(a.d), kColBlk, a);
   L.halo_begin();
   L.halo_add(s.ru_tend, 'u', 1); L.halo_add(s.rv_tend, 'v', 1); L.halo_add(s.ph_2, 'w', 1);
   L.halo_add(s.al, 'p', 1); L.halo_add(s.p, 'p', 1); L.halo_add(s.t_1, 'p', 1); L.halo_add(s.t_save, 't', 1);
@@ -365,7 +403,7 @@ static void ph_small_prep(Launcher &L) {
 }
 static void ph_advance_uv(Launcher &L) {
   const bool per = L.a.c.periodic_x && L.a.c.periodic_y;
-  L.launchK(KID_ADVANCE_UV, per ? k_advance_uv<true> : k_advance_uv<false>,
+  L.launchK(KID_ADVANCE_UV, per ? KRP(k_advance_uv, true) : KRP(k_advance_uv, false),
             g_pt3(L.a.d, (L.a.d.kde - L.a.d.kds + kAdvUvK - 1) / kAdvUvK), kColBlk, L.a);
 }
 static void ph_advance_mu_t_w(Launcher &L) {
@@ -373,13 +411,13 @@ static void ph_advance_mu_t_w(Launcher &L) {
   const wrfmi_state &s = a.s;
   const bool p64 = a.c.precision_mode == 1;
   const int nk = a.d.kde - a.d.kds + 1;
-  L.launchK(KID_AC_DIV, k_ac_div, g_pt3(a.d, nk), kColBlk, a);
-  if (p64) L.launchK(KID_AC_MU_WW, k_ac_mu_ww<true>, g_col(a.d), kColBlk, a);
-  else L.launchK(KID_AC_MU_WW, k_ac_mu_ww<false>, g_col(a.d), kColBlk, a);
-  L.launchK(KID_AC_THETA, k_ac_theta, g_pt3(a.d, nk), kColBlk, a);
-  L.launchK(KID_AC_WPRED, k_ac_wpred, g_pt3(a.d, nk), kColBlk, a);
-  if (p64) L.launchK(KID_AC_WSOLVE, k_ac_wsolve<double>, g_col(a.d), kColBlk, a);
-  else L.launchK(KID_AC_WSOLVE, k_ac_wsolve<float>, g_col(a.d), kColBlk, a);
+  L.launchK(KID_AC_DIV, KR(k_ac_div), g_pt3(a.d, nk), kColBlk, a);
+  if (p64) L.launchK(KID_AC_MU_WW, KRP(k_ac_mu_ww, true), g_col(a.d), kColBlk, a);
+  else L.launchK(KID_AC_MU_WW, KRP(k_ac_mu_ww, false), g_col(a.d), kColBlk, a);
+  L.launchK(KID_AC_THETA, KR(k_ac_theta), g_pt3(a.d, nk), kColBlk, a);
+  L.launchK(KID_AC_WPRED, KR(k_ac_wpred), g_pt3(a.d, nk), kColBlk, a);
+  if (p64) L.launchK(KID_AC_WSOLVE, KRP(k_ac_wsolve, double), g_col(a.d), kColBlk, a);
+  else L.launchK(KID_AC_WSOLVE, KRP(k_ac_wsolve, float), g_col(a.d), kColBlk, a);
   L.halo_begin();
   L.halo_add(s.ph_2, 'w', 1); L.halo_add(s.al, 'p', 1); L.halo_add(s.p, 'p', 1);
   L.halo_add(s.muts, 't', 0); L.halo_add(s.mu_2, 't', 0); L.halo_add(s.mudf, 't', 0);
@@ -388,7 +426,7 @@ static void ph_advance_mu_t_w(Launcher &L) {
 static void ph_small_finish(Launcher &L) {
   const KArgs &a = L.a;
   const wrfmi_state &s = a.s;
-  L.launchK(KID_SMALL_FINISH, k_small_finish, g_pt3(a.d, a.d.kde - a.d.kds + 1), kColBlk, a);
+  L.launchK(KID_SMALL_FINISH, KR(k_small_finish), g_pt3(a.d, a.d.kde - a.d.kds + 1), kColBlk, a);
   if (a.rk_step == 3) {
     L.halo_begin();
     L.halo_add(s.ru_m, 'u', 1); L.halo_add(s.rv_m, 'v', 1); L.halo_add(s.ww_m, 'w', 1);
@@ -401,11 +439,11 @@ static void ph_scalar(Launcher &L) {
   const Dm &d = L.a.d;
   const dim3 ga((d.ide - d.ids + 1 + kAdvTileI - 1) / kAdvTileI, (d.jde - d.jds + 1 + kAdvRows - 1) / kAdvRows,
                 (d.kde - d.kds + 1 + kAdvK - 1) / kAdvK);
-  L.launchK(KID_ADV_SCALAR, per ? k_adv_scalar<true> : k_adv_scalar<false>, ga, dim3(64, kAdvRows, 1), L.a);
-  L.launchK(KID_UPDATE_SCALAR, k_update_scalar, g_pt3(L.a.d, L.a.d.kde - L.a.d.kds), kColBlk, L.a);
+  L.launchK(KID_ADV_SCALAR, per ? KRP(k_adv_scalar, true) : KRP(k_adv_scalar, false), ga, dim3(64, kAdvRows, 1), L.a);
+  L.launchK(KID_UPDATE_SCALAR, KR(k_update_scalar), g_pt3(L.a.d, L.a.d.kde - L.a.d.kds), kColBlk, L.a);
 }
 static void ph_p_rho_phi(Launcher &L) {
-  L.launchK(KID_P_RHO_PHI, k_p_rho_phi, g_pt3(L.a.d, L.a.d.kde - L.a.d.kds), kColBlk, L.a);
+  L.launchK(KID_P_RHO_PHI, KR(k_p_rho_phi), g_pt3(L.a.d, L.a.d.kde - L.a.d.kds), kColBlk, L.a);
 }
 static void ph_bc_dry_2(Launcher &L, int kind) {
   const wrfmi_state &s = L.a.s;
@@ -458,7 +496,10 @@ static void enqueue_step(Launcher &L) {
   ph_bc_dry_2(L, 1);
 }
 
-static int prepare(const wrfmi_state *st_in, const wrfmi_config *cfg, const wrfmi_dims *dims, Launcher &L, wrfmi_state &s) {
+// Validate and translate the arguments into L; the work arrays are the library's shared
+// ones for this field size unless own_scratch is given (ensemble members each have theirs).
+static int prepare(const wrfmi_state *st_in, const wrfmi_config *cfg, const wrfmi_dims *dims, Launcher &L, wrfmi_state &s,
+                   const Scratch *own_scratch = nullptr) {
   int rc;
   if ((rc = check_cfg(cfg))) return rc;
   Dm d;
@@ -469,7 +510,8 @@ static int prepare(const wrfmi_state *st_in, const wrfmi_config *cfg, const wrfm
   L.a.c = *cfg;
   L.a.d = d;
   Scratch sc;
-  if ((rc = get_scratch((size_t)d.sj * (size_t)(d.jme - d.jms + 1), &sc))) return rc;
+  if (own_scratch) sc = *own_scratch;
+  else if ((rc = get_scratch((size_t)d.sj * (size_t)(d.jme - d.jms + 1), &sc))) return rc;
   L.a.scr_dv = FPtr<float>(sc.dv);
   L.a.scr_rhs = FPtr<float>(sc.rhs);
   L.a.scr_lo = FPtr<float>(sc.lo);
@@ -546,6 +588,207 @@ extern "C" int wrfmi_solve_em_step_graph(wrfmi_state *st, const wrfmi_config *cf
   HIPCHK(hipGraphLaunch(hit->x, hs));
   if (sync) HIPCHK(hipStreamSynchronize(hs));
   return 0;
+}
+
+// ------------------------------------------------------------------ batched ensembles
+// An ensemble advances many independent domains (members, of any sizes) with one launch per
+// kernel of the step: the launches of every member are recorded once, at creation, and each
+// group of corresponding launches becomes one launch over the concatenated tiles of all
+// members.  The per-member arguments, the workgroup offsets and the workgroup -> member table
+// live in one device allocation that the kernels read (Batch<T> in dev_common.h).
+struct EnsOp {
+  int id;
+  bool halo;
+  void *kern;
+  dim3 blk;
+  unsigned total;
+  BatchArgs ka;
+  BatchHalo ha;
+};
+struct wrfmi_ensemble {
+  int n = 0;
+  std::vector<EnsOp> ops;
+  std::vector<float *> work;  // per-member work arrays
+  void *table = nullptr;      // device: member arguments, offsets, tables
+  hipGraph_t g = nullptr;
+  hipGraphExec_t x = nullptr;
+};
+static void ens_free(wrfmi_ensemble *e) {
+  if (e->x) (void)hipGraphExecDestroy(e->x);
+  if (e->g) (void)hipGraphDestroy(e->g);
+  for (float *p : e->work) (void)hipFree(p);
+  if (e->table) (void)hipFree(e->table);
+  delete e;
+}
+static std::vector<wrfmi_ensemble *> g_ensembles;
+static void free_ensembles() {
+  for (wrfmi_ensemble *e : g_ensembles) ens_free(e);
+  g_ensembles.clear();
+}
+
+extern "C" int wrfmi_ensemble_create(int n_members, const wrfmi_state *states, const wrfmi_config *cfgs,
+                                     const wrfmi_dims *dims, wrfmi_ensemble **out) {
+  if (!states || !cfgs || !dims || !out) return fail(WRFMI_E_POINTER, "null argument");
+  if (n_members < 1) return fail(WRFMI_E_BADOPT, "an ensemble needs at least one member");
+  *out = nullptr;
+  std::lock_guard<std::mutex> lk(g_mu);
+  wrfmi_ensemble *e = new wrfmi_ensemble;
+  e->n = n_members;
+  std::vector<RecMember> rec(n_members);
+  int rc = 0;
+  for (int m = 0; m < n_members && !rc; m++) {
+    Dm d;
+    if ((rc = make_dm(&dims[m], &d))) break;
+    const size_t bytes = (size_t)d.sj * (size_t)(d.jme - d.jms + 1) * sizeof(float);
+    Scratch sc{nullptr, nullptr, nullptr};
+    for (float **p : {&sc.dv, &sc.rhs, &sc.lo}) {
+      if (hipMalloc(p, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        rc = fail(WRFMI_E_DEVICE, "member %d: cannot allocate %zu bytes of work arrays", m, bytes);
+        break;
+      }
+      e->work.push_back(*p);
+    }
+    if (rc) break;
+    Launcher L;
+    wrfmi_state s;
+    if ((rc = prepare(&states[m], &cfgs[m], &dims[m], L, s, &sc))) {
+      std::string msg = g_err;
+      rc = fail(rc, "member %d: %s", m, msg.c_str());
+      break;
+    }
+    L.rec = &rec[m];
+    enqueue_step(L);
+  }
+  // every member must have produced the same launch sequence (same kernels and blocks):
+  // members share the boundary type, precision mode, species count and acoustic steps
+  for (int m = 1; m < n_members && !rc; m++) {
+    bool same = rec[m].ops.size() == rec[0].ops.size();
+    for (size_t o = 0; same && o < rec[0].ops.size(); o++)
+      same = rec[m].ops[o].kern == rec[0].ops[o].kern && rec[m].ops[o].blk.x == rec[0].ops[o].blk.x &&
+             rec[m].ops[o].blk.y == rec[0].ops[o].blk.y && rec[m].ops[o].halo == rec[0].ops[o].halo &&
+             (!rec[m].ops[o].halo || rec[m].h[rec[m].ops[o].arg].n == rec[0].h[rec[0].ops[o].arg].n);
+    if (!same)
+      rc = fail(WRFMI_E_BADOPT, "member %d: members must share the boundary type, precision mode, number of species "
+                "and number of acoustic steps", m);
+  }
+  if (rc) {
+    ens_free(e);
+    return rc;
+  }
+  // host image of the device table
+  std::vector<char> img;
+  auto put = [&](const void *p, size_t n) {
+    size_t at = (img.size() + 255) & ~(size_t)255;
+    img.resize(at + n);
+    memcpy(img.data() + at, p, n);
+    return at;
+  };
+  std::map<std::vector<unsigned>, std::pair<size_t, size_t>> shapes;  // member wgs -> (off, lut)
+  struct Pending { size_t args, off, lut; };
+  std::vector<Pending> pend;
+  const size_t nops = rec[0].ops.size();
+  for (size_t o = 0; o < nops; o++) {
+    const RecOp &op0 = rec[0].ops[o];
+    std::vector<unsigned> w(n_members);
+    for (int m = 0; m < n_members; m++) w[m] = rec[m].ops[o].wgs;
+    auto it = shapes.find(w);
+    if (it == shapes.end()) {
+      std::vector<unsigned> off(n_members + 1, 0), lut;
+      for (int m = 0; m < n_members; m++) {
+        off[m + 1] = off[m] + w[m];
+        lut.insert(lut.end(), w[m], (unsigned)m);
+      }
+      const size_t oo = put(off.data(), off.size() * sizeof(unsigned));
+      const size_t lo = put(lut.data(), lut.size() * sizeof(unsigned));
+      it = shapes.emplace(w, std::make_pair(oo, lo)).first;
+    }
+    size_t ao;
+    if (op0.halo) {
+      std::vector<HaloArgs> hv(n_members);
+      for (int m = 0; m < n_members; m++) hv[m] = rec[m].h[rec[m].ops[o].arg];
+      ao = put(hv.data(), hv.size() * sizeof(HaloArgs));
+    } else {
+      std::vector<KArgs> kv(n_members);
+      for (int m = 0; m < n_members; m++) kv[m] = rec[m].k[rec[m].ops[o].arg];
+      ao = put(kv.data(), kv.size() * sizeof(KArgs));
+    }
+    unsigned total = 0;
+    for (unsigned v : w) total += v;
+    EnsOp eo{op0.id, op0.halo, op0.kern, op0.blk, total, {}, {}};
+    e->ops.push_back(eo);
+    pend.push_back(Pending{ao, it->second.first, it->second.second});
+  }
+  if (hipMalloc(&e->table, img.size()) != hipSuccess || hipMemcpy(e->table, img.data(), img.size(), hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipGetLastError();
+    ens_free(e);
+    return fail(WRFMI_E_DEVICE, "cannot set up the ensemble table (%zu bytes)", img.size());
+  }
+  char *base = (char *)e->table;
+  for (size_t o = 0; o < nops; o++) {
+    EnsOp &eo = e->ops[o];
+    const unsigned *off = (const unsigned *)(base + pend[o].off), *lut = (const unsigned *)(base + pend[o].lut);
+    if (eo.halo) eo.ha = BatchHalo{(const HaloArgs *)(base + pend[o].args), off, lut, n_members, eo.total};
+    else eo.ka = BatchArgs{(const KArgs *)(base + pend[o].args), off, lut, n_members, eo.total};
+  }
+  g_ensembles.push_back(e);
+  *out = e;
+  return 0;
+}
+
+static void ens_enqueue(wrfmi_ensemble *e, hipStream_t st) {
+  Launcher L;
+  L.st = st;
+  for (const EnsOp &o : e->ops) {
+    if (o.halo) L.launch(o.id, (void (*)(BatchHalo))o.kern, dim3(o.total, 1, 1), o.blk, o.ha);
+    else L.launch(o.id, (void (*)(BatchArgs))o.kern, dim3(o.total, 1, 1), o.blk, o.ka);
+  }
+}
+static bool ens_known(wrfmi_ensemble *e) {
+  for (wrfmi_ensemble *k : g_ensembles)
+    if (k == e) return true;
+  return false;
+}
+
+extern "C" int wrfmi_ensemble_step(wrfmi_ensemble *e, void *stream, int sync) {
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!e || !ens_known(e)) return fail(WRFMI_E_POINTER, "not an ensemble of this library");
+  }
+  ens_enqueue(e, (hipStream_t)stream);
+  HIPCHK(hipGetLastError());
+  if (sync) HIPCHK(hipStreamSynchronize((hipStream_t)stream));
+  return 0;
+}
+
+extern "C" int wrfmi_ensemble_step_graph(wrfmi_ensemble *e, void *stream, int sync) {
+  if (g_timing) return fail(WRFMI_E_BADOPT, "kernel timing mode records events on eager launches only");
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!e || !ens_known(e)) return fail(WRFMI_E_POINTER, "not an ensemble of this library");
+    if (!e->x) {
+      if (!g_stream) HIPCHK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+      HIPCHK(hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal));
+      ens_enqueue(e, g_stream);
+      HIPCHK(hipStreamEndCapture(g_stream, &e->g));
+      HIPCHK(hipGraphInstantiate(&e->x, e->g, nullptr, nullptr, 0));
+    }
+  }
+  HIPCHK(hipGraphLaunch(e->x, (hipStream_t)stream));
+  if (sync) HIPCHK(hipStreamSynchronize((hipStream_t)stream));
+  return 0;
+}
+
+extern "C" int wrfmi_ensemble_destroy(wrfmi_ensemble *e) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  for (size_t n = 0; n < g_ensembles.size(); n++)
+    if (g_ensembles[n] == e) {
+      HIPCHK(hipDeviceSynchronize());
+      ens_free(e);
+      g_ensembles.erase(g_ensembles.begin() + n);
+      return 0;
+    }
+  return fail(WRFMI_E_POINTER, "not an ensemble of this library");
 }
 
 // ------------------------------------------------------------------ per-routine entry points

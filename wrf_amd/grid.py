@@ -383,26 +383,29 @@ def build_initial_state(spec: DomainSpec) -> HostState:
     QV = np.broadcast_to(qv[None, :, None], (ny_m, nz, nx_m)).copy()
 
     def hydro_column(T, QV):
-        """p, alt, al, ph_1 from theta and qv (module_initialize_ideal.F:1000-1085)."""
+        """p, alt, al, ph_1 from theta and qv (module_initialize_ideal.F:1000-1085).
+        The vertical recurrences run on level-major copies (contiguous level slices)."""
+        T = np.ascontiguousarray(T.transpose(1, 0, 2))
+        QV = np.ascontiguousarray(QV.transpose(1, 0, 2))
         p = np.zeros_like(T)
         kk = nz - 1
         k = kk + 1  # Fortran kk = kte-1 (top mass level), k = kk + 1
         # top layer: the same qv average as pg_buoy_w's cqw(kde-1) so that the model's
         # top-level vertical pressure gradient balances exactly (the reference's ideal
         # init uses qv(kde-1) alone, module_initialize_ideal.F:1033)
-        qvf1 = 0.5 * (QV[:, kk, :] + QV[:, kk - 1, :])
+        qvf1 = 0.5 * (QV[kk] + QV[kk - 1])
         qvf2 = 1.0 / (1.0 + qvf1)
         qvf1 = qvf1 * qvf2
-        p[:, kk, :] = -0.5 * ((c1f[k] * mu1) + qvf1 * (c1f[k] * mub + c2f[k])) / rdnw[kk] / qvf2
+        p[kk] = -0.5 * ((c1f[k] * mu1) + qvf1 * (c1f[k] * mub + c2f[k])) / rdnw[kk] / qvf2
         for kk in range(nz - 2, -1, -1):
             k = kk + 1
-            qvf1 = 0.5 * (QV[:, kk, :] + QV[:, kk + 1, :])
+            qvf1 = 0.5 * (QV[kk] + QV[kk + 1])
             qvf2 = 1.0 / (1.0 + qvf1)
             qvf1 = qvf1 * qvf2
-            p[:, kk, :] = p[:, kk + 1, :] - ((c1f[k] * mu1) + qvf1 * (c1f[k] * mub + c2f[k])) / qvf2 / rdn[kk + 1]
+            p[kk] = p[kk + 1] - ((c1f[k] * mu1) + qvf1 * (c1f[k] * mub + c2f[k])) / qvf2 / rdn[kk + 1]
         qvf = 1.0 + RVOVRD * QV
-        alt = (R_D / P1000) * (T + T0) * qvf * ((p + pb[None, :, None]) / P1000) ** CVPM
-        al = alt - alb[None, :, None]
+        alt = (R_D / P1000) * (T + T0) * qvf * ((p + pb[:, None, None]) / P1000) ** CVPM
+        al = alt - alb[:, None, None]
         # geopotential from the log-pressure hypsometric relation that the model's
         # equation of state inverts (calc_p_rho_phi, hypsometric_opt = 2), so that the
         # start-of-run diagnosis of al and p (start_em.F:794-830) returns this state
@@ -410,14 +413,15 @@ def build_initial_state(spec: DomainSpec) -> HostState:
         pf = c3f * muts + c4f + p_top
         phm = c3h * muts + c4h + p_top
         dlog = np.log(pf[:-1] / pf[1:])
-        ph_tot = np.zeros((ny_m, nz + 1, nx_m))
+        ph_tot = np.zeros((nz + 1, ny_m, nx_m))
         for k in range(nz):
-            ph_tot[:, k + 1, :] = ph_tot[:, k, :] + alt[:, k, :] * phm[k] * dlog[k]
-        ph = ph_tot - phb[None, :, None]
-        al = (ph_tot[:, 1:, :] - ph_tot[:, :-1, :]) / phm[None, :, None] / dlog[None, :, None] - alb[None, :, None]
-        alt = al + alb[None, :, None]
-        p = P1000 * ((R_D * (T0 + T * (1.0 + RVOVRD * QV) + T0 * RVOVRD * QV)) / (P1000 * alt)) ** (CP / CV) - pb[None, :, None]
-        return p, alt, al, ph
+            ph_tot[k + 1] = ph_tot[k] + alt[k] * phm[k] * dlog[k]
+        ph = ph_tot - phb[:, None, None]
+        al = (ph_tot[1:] - ph_tot[:-1]) / phm[:, None, None] / dlog[:, None, None] - alb[:, None, None]
+        alt = al + alb[:, None, None]
+        p = P1000 * ((R_D * (T0 + T * (1.0 + RVOVRD * QV) + T0 * RVOVRD * QV)) / (P1000 * alt)) ** (CP / CV) - pb[:, None, None]
+        tr = lambda a: np.ascontiguousarray(a.transpose(1, 0, 2))
+        return tr(p), tr(alt), tr(al), tr(ph)
 
     p, alt, al, ph = hydro_column(T, QV)
 
@@ -514,3 +518,23 @@ def large_c1(**kw) -> DomainSpec:
                 bc="periodic", n_bubbles=6, bubble_amp=3.0, noise_amp=0.1, n_passive=6, seed=1002)
     base.update(kw)
     return DomainSpec(**base)
+
+
+ENSEMBLE_SIZES = (64, 96, 128, 192, 256)
+
+
+def ensemble_c3(n_members: int = 64, seed: int = 1004, **kw) -> list:
+    """Uneven ensemble: member m has nx, ny drawn uniformly from ENSEMBLE_SIZES (generator
+    seeded with `seed`), nz = 50, dx = 2 km, dt = 12 s, the 41x41x40 case's sounding and open
+    boundaries, a bubble amplitude drawn from U[1, 4] K and +/-0.2 K theta noise, with its
+    own seed seed*1000 + m.  Returns the members' DomainSpecs."""
+    rng = np.random.Generator(np.random.Philox(seed))
+    out = []
+    for m in range(n_members):
+        nx, ny = (int(v) for v in rng.choice(ENSEMBLE_SIZES, size=2))
+        amp = float(rng.uniform(1.0, 4.0))
+        base = dict(nx=nx, ny=ny, nz=50, dx=2000.0, dy=2000.0, dt=12.0, ztop=20000.0, time_step_sound=6,
+                    bc="open", n_bubbles=1, bubble_amp=amp, noise_amp=0.2, seed=seed * 1000 + m)
+        base.update(kw)
+        out.append(DomainSpec(**base))
+    return out

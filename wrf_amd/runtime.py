@@ -122,6 +122,10 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     for fn in (lib.wrfmi_set_physical_bc3d, lib.wrfmi_set_physical_bc2d):
         fn.argtypes = [vp, ctypes.c_char, vp, vp, vp]
         fn.restype = ctypes.c_int
+    lib.wrfmi_ensemble_create.argtypes = [ctypes.c_int, vp, vp, vp, ctypes.POINTER(vp)]
+    for fn in (lib.wrfmi_ensemble_step, lib.wrfmi_ensemble_step_graph):
+        fn.argtypes = [vp, vp, ctypes.c_int]
+    lib.wrfmi_ensemble_destroy.argtypes = [vp]
     lib.wrfmi_check_finite.argtypes = [vp, vp, ctypes.POINTER(ctypes.c_longlong)]
     lib.wrfmi_kernel_count.restype = ctypes.c_int
     lib.wrfmi_kernel_name.restype = ctypes.c_char_p
@@ -253,3 +257,43 @@ class DeviceDomain:
     def hbm_bytes(self) -> int:
         return sum(v.numel() * v.element_size() for v in self.t.values())
 
+
+
+class DeviceEnsemble:
+    """Independent domains (members, of any sizes) advanced together by the batched entry
+    points wrfmi_ensemble_step / _graph: one launch per kernel of the step over the tiles
+    of all members.  ``members`` are DeviceDomains (their HBM fields are used in place)."""
+
+    def __init__(self, members):
+        self.lib = load_library()
+        self.members = list(members)
+        n = len(self.members)
+        self._states = (State32 * n)(*[m.state for m in self.members])
+        self._cfgs = (Config * n)(*[m.cfg for m in self.members])
+        self._dims = (Dims * n)(*[m.dims for m in self.members])
+        self.handle = ctypes.c_void_p()
+        check(self.lib.wrfmi_ensemble_create(n, ctypes.byref(self._states), ctypes.byref(self._cfgs),
+                                             ctypes.byref(self._dims), ctypes.byref(self.handle)))
+
+    def step(self, nsteps: int = 1, graph: bool = False, stream=None, sync: bool = True):
+        """Advance every member by `nsteps` RK3 steps."""
+        fn = self.lib.wrfmi_ensemble_step_graph if graph else self.lib.wrfmi_ensemble_step
+        h = self.members[0].stream_handle(stream)
+        for n in range(nsteps):
+            check(fn(self.handle, ctypes.c_void_p(h), int(sync and n == nsteps - 1)))
+
+    def close(self):
+        if self.handle:
+            check(self.lib.wrfmi_ensemble_destroy(self.handle))
+            self.handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def points(self) -> int:
+        """Mass points over all members."""
+        return sum((m.dims.ide - m.dims.ids) * (m.dims.jde - m.dims.jds) * (m.dims.kde - m.dims.kds)
+                   for m in self.members)
