@@ -20,6 +20,7 @@
 #define __launch_bounds__(...)
 #define __shared__ static
 #define WRFMI_GLOBAL_AS
+#define WRFMI_CONST_AS
 
 struct dim3 {
   unsigned x, y, z;

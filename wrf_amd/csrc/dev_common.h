@@ -46,13 +46,34 @@ struct FPtr {
   HDI T &operator[](int i) const { return *(T *)((WRFMI_GLOBAL_AS T *)p + i); }
 };
 
-// Device view of wrfmi_state: same layout, arrays as FPtr
+// Arrays that no kernel writes (the 1-D level coefficients).  Loads through the constant
+// address space become scalar loads when the index is uniform across the wave; those
+// complete on the scalar counter, so waiting for one does not also wait for the vector
+// loads in flight (vector loads retire in order: a wait for the newest waits for all).
+#ifndef WRFMI_CONST_AS
+#define WRFMI_CONST_AS __attribute__((address_space(4)))
+#endif
+template <typename T>
+struct CPtr {
+  const T *p;
+  HDI CPtr() : p(nullptr) {}
+  HDI CPtr(const T *q) : p(q) {}
+  HDI T operator[](Off o) const {
+    const WRFMI_CONST_AS char *g = (const WRFMI_CONST_AS char *)(const char *)p;
+    return *(const WRFMI_CONST_AS T *)(g + o.b);
+  }
+  HDI T operator[](int i) const { return ((const WRFMI_CONST_AS T *)p)[i]; }
+};
+
+// Device view of wrfmi_state: same layout, arrays as FPtr (1-D level arrays as CPtr)
 struct DState {
 #define WRFMI_DPTR(n) FPtr<float> n;
+#define WRFMI_CPTR(n) CPtr<float> n;
   WRFMI_FIELDS_3D(WRFMI_DPTR)
   WRFMI_FIELDS_2D(WRFMI_DPTR)
-  WRFMI_FIELDS_1D(WRFMI_DPTR)
+  WRFMI_FIELDS_1D(WRFMI_CPTR)
 #undef WRFMI_DPTR
+#undef WRFMI_CPTR
 #define WRFMI_DR(n) float n;
   WRFMI_SCALARS_R(WRFMI_DR)
 #undef WRFMI_DR
@@ -249,6 +270,10 @@ DEVI void two_sum_acc(float &s, float &comp, float x) {
 // loads of one level do not depend on the body's results, so up to kPF levels of loads are
 // in flight while the body works through the column (a thread per column otherwise waits
 // a full memory latency per level).  load(f, k) must only read data the body never writes.
+// The loads are unconditional (levels past k1 are clamped to k1): a load under a branch
+// makes the compiler copy its destination at the join, and such a copy waits for every
+// load in flight.  Full blocks of kPF levels run without any branch; the last partial
+// block only calls the body.
 constexpr int kPFdef = 8;
 template <int NF, typename T, int kPF = kPFdef, typename Load, typename Body>
 DEVI void column_march_up(int k0, int k1, Load &&load, Body &&body) {
@@ -256,22 +281,23 @@ DEVI void column_march_up(int k0, int k1, Load &&load, Body &&body) {
 #pragma unroll
   for (int r = 0; r < kPF; r++)
 #pragma unroll
-    for (int f = 0; f < NF; f++) ring[r][f] = (k0 + r <= k1) ? load(f, k0 + r) : T(0);
-  for (int kb = k0; kb <= k1; kb += kPF) {
+    for (int f = 0; f < NF; f++) ring[r][f] = load(f, imin(k0 + r, k1));
+  int kb = k0;
+  for (; kb + kPF - 1 <= k1; kb += kPF) {
 #pragma unroll
     for (int r = 0; r < kPF; r++) {
-      const int k = kb + r;
-      if (k <= k1) {
-        T cur[NF];
+      T cur[NF];
 #pragma unroll
-        for (int f = 0; f < NF; f++) {
-          cur[f] = ring[r][f];
-          if (k + kPF <= k1) ring[r][f] = load(f, k + kPF);
-        }
-        body(k, cur);
+      for (int f = 0; f < NF; f++) {
+        cur[f] = ring[r][f];
+        ring[r][f] = load(f, imin(kb + r + kPF, k1));
       }
+      body(kb + r, cur);
     }
   }
+#pragma unroll
+  for (int r = 0; r < kPF - 1; r++)
+    if (kb + r <= k1) body(kb + r, ring[r]);
 }
 // The same, visiting k = k1 down to k0.
 template <int NF, typename T, int kPF = kPFdef, typename Load, typename Body>
@@ -280,22 +306,23 @@ DEVI void column_march_down(int k0, int k1, Load &&load, Body &&body) {
 #pragma unroll
   for (int r = 0; r < kPF; r++)
 #pragma unroll
-    for (int f = 0; f < NF; f++) ring[r][f] = (k1 - r >= k0) ? load(f, k1 - r) : T(0);
-  for (int kb = k1; kb >= k0; kb -= kPF) {
+    for (int f = 0; f < NF; f++) ring[r][f] = load(f, imax(k1 - r, k0));
+  int kb = k1;
+  for (; kb - kPF + 1 >= k0; kb -= kPF) {
 #pragma unroll
     for (int r = 0; r < kPF; r++) {
-      const int k = kb - r;
-      if (k >= k0) {
-        T cur[NF];
+      T cur[NF];
 #pragma unroll
-        for (int f = 0; f < NF; f++) {
-          cur[f] = ring[r][f];
-          if (k - kPF >= k0) ring[r][f] = load(f, k - kPF);
-        }
-        body(k, cur);
+      for (int f = 0; f < NF; f++) {
+        cur[f] = ring[r][f];
+        ring[r][f] = load(f, imax(kb - r - kPF, k0));
       }
+      body(kb - r, cur);
     }
   }
+#pragma unroll
+  for (int r = 0; r < kPF - 1; r++)
+    if (kb - r >= k0) body(kb - r, ring[r]);
 }
 
 }  // namespace wrfmi

@@ -126,7 +126,7 @@ __global__ void __launch_bounds__(256) k_ww_scan(ARGS args) {
   if (i > ide || j > jde) return;
   const bool mi = i <= ide - 1, mj = j <= jde - 1;
   const FPtr<float> mu = s.mu_2, mub = s.mub;
-  const FPtr<float> c1h = s.c1h, c2h = s.c2h;
+  const CPtr<float> c1h = s.c1h, c2h = s.c2h;
   const int ktf = kde - 1;
   // ww(1) = ww(kte) = 0 on i = its..ite
   if (mj) {

@@ -52,7 +52,7 @@ __global__ void __launch_bounds__(256) k_small_prep(ARGS args) {
   if (i > ide || j > jde || k > kde) return;
   const bool mi = i <= ide - 1, mj = j <= jde - 1, mass = mi && mj;
   const int k_end = kde - 1;
-  const FPtr<float> c1h = s.c1h, c2h = s.c2h, c1f = s.c1f, c2f = s.c2f;
+  const CPtr<float> c1h = s.c1h, c2h = s.c2h, c1f = s.c1f, c2f = s.c2f;
   const Off o2 = X2(d, i, j), o = X3(d, i, k, j);
   const bool rk1 = a.rk_step == 1;
   const MuS m = small_mu(a, i, j);
@@ -129,7 +129,7 @@ __global__ void __launch_bounds__(256) k_coef_w(ARGS args) {
   const int ide = d.ide, jde = d.jde, kde = d.kde;
   if (i > ide || j > jde) return;
   const bool mi = i <= ide - 1, mj = j <= jde - 1, mass = mi && mj;
-  const FPtr<float> c1h = s.c1h, c2h = s.c2h, c1f = s.c1f, c2f = s.c2f;
+  const CPtr<float> c1h = s.c1h, c2h = s.c2h, c1f = s.c1f, c2f = s.c2f;
   const Off o2 = X2(d, i, j);
   const MuS m = small_mu(a, i, j);
   if (mass) {
@@ -382,7 +382,8 @@ __global__ void __launch_bounds__(256) k_ac_mu_ww(ARGS args) {
   const int k_end = d.kde - 1;
   const Off o2 = X2(d, i, j);
   const Off ob = X3(d, i, 0, j);  // level k of the column is at ob + k*ni
-  const FPtr<float> dvp = a.scr_dv, dnw = s.dnw, c1h = s.c1h, ww1 = s.ww1;
+  const FPtr<float> dvp = a.scr_dv, ww1 = s.ww1;
+  const CPtr<float> dnw = s.dnw, c1h = s.c1h;
   FPtr<float> ww = s.ww;
   const int ni = d.ni, x1 = -d.kms;
   const float dts = a.dts, epssm = a.c.epssm, msfty = s.msfty[o2];
@@ -513,7 +514,7 @@ __global__ void __launch_bounds__(256) k_ac_wpred(ARGS args) {
     return;
   }
   const FPtr<float> ph = s.ph_2, rhs = a.scr_rhs, c2a = s.c2a, alt = s.alt, t2ave = s.t_2save;
-  const FPtr<float> c1h = s.c1h, c2h = s.c2h, c1f = s.c1f;
+  const CPtr<float> c1h = s.c1h, c2h = s.c2h, c1f = s.c1f;
   const float mut = s.mut[o2], muave = s.muave[o2];
   const float msft_inv = 1.f / msfty;
   const float rhs_m1 = k >= 3 ? rhs[o - d.ni] : 0.f;  // rhs(1) = 0
@@ -583,7 +584,7 @@ __global__ void __launch_bounds__(256) k_ac_wsolve(ARGS args) {
   // downward: back substitution, ph, calc_p_rho.  Fields: 0 = forward value of w (plus its
   // low part in FP64), 1 = gamma, 2 = rhs, 3 = t_2, 4 = t_save, 5 = alt, 6 = c2a, 7 = pm1
   const float dts = a.dts, epssm = c.epssm, msfty = s.msfty[o2], muts = s.muts[o2], mu = s.mu_2[o2];
-  const FPtr<float> c1h = s.c1h, c2h = s.c2h, c1f = s.c1f, c2f = s.c2f, rdnw = s.rdnw;
+  const CPtr<float> c1h = s.c1h, c2h = s.c2h, c1f = s.c1f, c2f = s.c2f, rdnw = s.rdnw;
   const float smdiv = c.smdiv;
   CT wup = wfe;
   float ph_up = 0.f;  // final ph one level above
@@ -655,7 +656,7 @@ __global__ void __launch_bounds__(256) k_small_finish(ARGS args) {
   const int ide = d.ide, jde = d.jde, kde = d.kde;
   if (i > ide || j > jde || k > kde) return;
   const bool mi = i <= ide - 1, mj = j <= jde - 1, mass = mi && mj;
-  const FPtr<float> c1h = s.c1h, c2h = s.c2h, c1f = s.c1f, c2f = s.c2f;
+  const CPtr<float> c1h = s.c1h, c2h = s.c2h, c1f = s.c1f, c2f = s.c2f;
   const FPtr<float> muts = s.muts;
   const Off o2 = X2(d, i, j);
   const Off o = X3(d, i, k, j);

@@ -466,7 +466,7 @@ __global__ void __launch_bounds__(256) k_tend_w(ARGS args) {
   const float ts = a.c.time_step >= 0 ? 1.f : -1.f;
   const int ktf = kte - 1;
   const FPtr<float> w = s.w_2, ru = s.ru, rv = s.rv, rom = s.ww;
-  const FPtr<float> fzm = s.fnm, fzp = s.fnp;
+  const CPtr<float> fzm = s.fnm, fzp = s.fnp;
   const float rdx = s.rdx, rdy = s.rdy;
   const bool mass = i <= imin(ite, ide - 1) && j <= imin(jte, jde - 1);
   const bool dxs = !(g.per_x || g.sym_xs || its > ids + 3), dxe = !(g.per_x || g.sym_xe || ite < ide - 3);
