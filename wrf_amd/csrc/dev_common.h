@@ -44,6 +44,19 @@ struct FPtr {
     return *(T *)(WRFMI_GLOBAL_AS T *)(g + (sizeof(T) == 8 ? 2u * o.b : o.b));
   }
   HDI T &operator[](int i) const { return *(T *)((WRFMI_GLOBAL_AS T *)p + i); }
+  // 16-byte access of four consecutive i points at Off o (o.b must be a multiple of 16)
+  template <typename V>
+  HDI V ld(Off o) const {
+    return *(const V *)(const WRFMI_GLOBAL_AS V *)((WRFMI_GLOBAL_AS char *)(char *)p + o.b);
+  }
+  template <typename V>
+  HDI void st(Off o, const V &v) const {
+    *(V *)(WRFMI_GLOBAL_AS V *)((WRFMI_GLOBAL_AS char *)(char *)p + o.b) = v;
+  }
+};
+// Four consecutive i points of one lane ("vector" kernels: 16-byte loads and stores)
+struct alignas(16) F4 {
+  float v[4];
 };
 
 // Arrays that no kernel writes (the 1-D level coefficients).  Loads through the constant

@@ -131,6 +131,108 @@ __global__ void __launch_bounds__(256) k_adv_scalar(ARGS args) {
   }
 }
 
+// advect_scalar for the doubly periodic case with four consecutive i points per lane
+// (16-byte loads).  Same fluxes and the same operation order as k_adv_scalar, element by
+// element: x faces from a 12-point window (the lane's points and four on each side), y
+// faces j and j+1 from the rows j-3..j+3, the vertical flux carried up the column with the
+// values of q at k-1..k+2 held in registers.  The lane's points start at an i whose memory
+// offset is a multiple of four floats; the host checks the alignment and the padding the
+// windows need.
+template <typename ARGS>
+__global__ void __launch_bounds__(256) k_adv_scalar_v(ARGS args) {
+  Tile3 T;
+  const KArgs &a = kargs(args, T);
+  const Dm &d = a.d;
+  const DState &s = dstate(a);
+  const int ia0 = d.ims + 4 * ((d.ids - d.ims) >> 2);
+  const int i0 = ia0 + kVecW * (T.bx * 64 + (int)threadIdx.x);
+  const int j = d.jds + T.by * (int)blockDim.y + (int)threadIdx.y;
+  const int k0 = d.kds + T.bz * kAdvVK;
+  const int ids = d.ids, ide = d.ide, jde = d.jde, kts = d.kds, kte = d.kde, ktf = kte - 1;
+  if (i0 > ide || j > jde) return;
+  const int k1 = imin(k0 + kAdvVK - 1, kte);
+  const float ts = a.c.time_step >= 0 ? 1.f : -1.f;
+  const FPtr<float> f = s.q[a.species], ru = s.ru_m, rv = s.rv_m, rom = s.ww_m;
+  const FPtr<float> tend = s.advect_tend;
+  const float rdx = s.rdx, rdy = s.rdy;
+  const int ni = d.ni, sj = d.sj;
+  const Off ob = X3(d, i0, 0, j);  // level k at ob + k*ni
+  auto at = [&](int k) { return ob + k * ni; };
+  const F4 msftx = s.msftx.ld<F4>(X2(d, i0, j));
+  const bool jmass = j <= jde - 1;
+  // vertical flux at w level kp from q at kp-2..kp+1 (vflux of k_adv_scalar)
+  auto vflux = [&](int kp, const F4 &qm2, const F4 &qm1, const F4 &q0, const F4 &qp1, const F4 &w) {
+    const float fm = s.fnm[X1(d, imin(kp, kte))], fp = s.fnp[X1(d, imin(kp, kte))];
+    const bool zero = kp == kts || kp == kte, low = kp == kts + 1 || kp == ktf;
+    F4 r;
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+      const float lo = w.v[n] * (fm * q0.v[n] + fp * qm1.v[n]);
+      const float hi = w.v[n] * flux3(qm2.v[n], qm1.v[n], q0.v[n], qp1.v[n], -w.v[n], ts);
+      r.v[n] = zero ? 0.f : (low ? lo : hi);
+    }
+    return r;
+  };
+  auto ldq = [&](int k) { return f.ld<F4>(at(imax(kts, imin(k, ktf)))); };
+  F4 qm1 = ldq(k0 - 1), q0 = ldq(k0), qp1 = ldq(k0 + 1);
+  F4 vf_lo = vflux(k0, ldq(k0 - 2), qm1, q0, qp1, rom.ld<F4>(at(imin(k0, kte))));
+#pragma unroll
+  for (int kk = 0; kk < kAdvVK; kk++) {
+    const int k = k0 + kk;
+    const Off o = at(imin(k, ktf));
+    // x window: points i0-4 .. i0+7
+    const F4 wl = f.ld<F4>(o - 4), wr = f.ld<F4>(o + 4);
+    const F4 ru0 = ru.ld<F4>(o), ru1 = ru.ld<F4>(o + 4);
+    // y rows j-3 .. j+3 (row j is q0)
+    const F4 ym3 = f.ld<F4>(o - 3 * sj), ym2 = f.ld<F4>(o - 2 * sj), ym1 = f.ld<F4>(o - sj);
+    const F4 yp1 = f.ld<F4>(o + sj), yp2 = f.ld<F4>(o + 2 * sj), yp3 = f.ld<F4>(o + 3 * sj);
+    const F4 rv0 = rv.ld<F4>(o), rv1 = rv.ld<F4>(o + sj);
+    const F4 qp2 = ldq(k + 2);
+    const F4 wp = rom.ld<F4>(at(imin(k + 1, kte)));
+    float W[12];
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+      W[n] = wl.v[n];
+      W[n + 4] = q0.v[n];
+      W[n + 8] = wr.v[n];
+    }
+    float fx[5];
+#pragma unroll
+    for (int n = 0; n < 5; n++) {
+      const float vel = n < 4 ? ru0.v[n] : ru1.v[0];
+      fx[n] = vel * flux5(W[n + 1], W[n + 2], W[n + 3], W[n + 4], W[n + 5], W[n + 6], vel, ts);
+    }
+    const F4 vf_hi = vflux(k + 1, qm1, q0, qp1, qp2, wp);
+    const float rdnw = s.rdnw[X1(d, imin(k, ktf))];
+    F4 t;
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+      const float fy0 = rv0.v[n] * flux5(ym3.v[n], ym2.v[n], ym1.v[n], q0.v[n], yp1.v[n], yp2.v[n], rv0.v[n], ts);
+      const float fy1 = rv1.v[n] * flux5(ym2.v[n], ym1.v[n], q0.v[n], yp1.v[n], yp2.v[n], yp3.v[n], rv1.v[n], ts);
+      float tt = 0.f;
+      tt = tt - msftx.v[n] * rdy * (fy1 - fy0);
+      tt = tt - msftx.v[n] * rdx * (fx[n + 1] - fx[n]);
+      tt = tt - rdnw * (vf_hi.v[n] - vf_lo.v[n]);
+      const bool mass = jmass && i0 + n <= ide - 1 && k <= ktf;
+      t.v[n] = mass ? tt : 0.f;
+    }
+    qm1 = q0;
+    q0 = qp1;
+    qp1 = qp2;
+    vf_lo = vf_hi;
+    if (k <= k1) {
+      const Off os = at(k);
+      if (i0 >= ids && i0 + 3 <= ide) {
+        tend.st<F4>(os, t);
+      } else {
+#pragma unroll
+        for (int n = 0; n < 4; n++)
+          if (i0 + n >= ids && i0 + n <= ide) tend[os + n] = t.v[n];
+      }
+    }
+  }
+}
+
 template <typename ARGS>
 __global__ void __launch_bounds__(256) k_update_scalar(ARGS args) {
   Tile3 T;
@@ -289,6 +391,7 @@ __global__ void __launch_bounds__(256) k_halo(ARGS args) {
 WRFMI_INST_P(k_adv_scalar, false)
 WRFMI_INST_P(k_adv_scalar, true)
 
+WRFMI_INST(k_adv_scalar_v)
 WRFMI_INST(k_update_scalar)
 WRFMI_INST(k_p_rho_phi)
 template __global__ void k_halo<HaloArgs>(HaloArgs);

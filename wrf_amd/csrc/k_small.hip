@@ -219,91 +219,85 @@ __global__ void __launch_bounds__(256) k_advance_uv(ARGS args) {
   const FPtr<float> p = s.p, pb = s.pb, ph = s.ph_2, php = s.php, alt = s.alt, al = s.al, mu = s.mu_2, mudf = s.mudf;
   const bool nh = c.non_hydrostatic;
   const Off o2 = X2(d, i, j);
-  // u
-  if (j >= j_start && j <= j_end) {
-    const bool tend_on = i >= i_start_u_tend && i <= i_end_u_tend, pg_on = i >= i_start_up && i <= i_end_up;
-    float mudf_xy = 0.f, m = 0.f, muu = 0.f, mu0 = 0.f, mu1 = 0.f;
-    if (pg_on) {
-      mudf_xy = -emdiv * dx * (mudf[o2] - mudf[X2(d, i - 1, j)]) / s.msfuy[o2];
-      m = s.msfux[o2] / s.msfuy[o2];
-      muu = s.muu[o2];
-      mu0 = mu[X2(d, i - 1, j)];
-      mu1 = mu[o2];
-    }
-    // dpn at w level kk, averaged to the u point (kk = 1: cf extrapolation; kk = kde: lid)
-    auto dpn = [&](int kk) -> float {
-      if (kk == 1)
-        return .5f * (cf1 * (p[X3(d, i, 1, j)] + p[X3(d, i - 1, 1, j)]) + cf2 * (p[X3(d, i, 2, j)] + p[X3(d, i - 1, 2, j)]) +
-                      cf3 * (p[X3(d, i, 3, j)] + p[X3(d, i - 1, 3, j)]));
-      if (kk == kde) {
-        if (!c.top_lid) return 0.f;
-        return .5f * (cf1 * (p[X3(d, i - 1, kde - 1, j)] + p[X3(d, i, kde - 1, j)]) + cf2 * (p[X3(d, i - 1, kde - 2, j)] + p[X3(d, i, kde - 2, j)]) +
-                      cf3 * (p[X3(d, i - 1, kde - 3, j)] + p[X3(d, i, kde - 3, j)]));
-      }
-      return .5f * (s.fnm[X1(d, kk)] * (p[X3(d, i, kk, j)] + p[X3(d, i - 1, kk, j)]) + s.fnp[X1(d, kk)] * (p[X3(d, i, kk - 1, j)] + p[X3(d, i - 1, kk - 1, j)]));
-    };
-    float dpn_lo = (pg_on && nh) ? dpn(k0) : 0.f;
-    for (int k = k0; k <= k1; k++) {
-      const Off o = X3(d, i, k, j);
-      const float c1 = s.c1h[X1(d, k)], c2 = s.c2h[X1(d, k)];
-      float u = s.u_2[o];
-      if (tend_on) u = u + dts * s.ru_tend[o];
-      if (pg_on) {
-        float dpxy = m * .5f * rdx * (c1 * muu + c2) *
-                     (((ph[o + d.ni] - ph[X3(d, i - 1, k + 1, j)]) + (ph[o] - ph[o - 1])) +
-                      (alt[o] + alt[o - 1]) * (p[o] - p[o - 1]) +
-                      (al[o] + al[o - 1]) * (pb[o] - pb[o - 1]));
-        if (nh) {
-          const float dpn_hi = dpn(k + 1);
-          dpxy = dpxy + m * rdx * (php[o] - php[o - 1]) * (s.rdnw[X1(d, k)] * (dpn_hi - dpn_lo) - .5f * ((c1 * mu0) + (c1 * mu1)));
-          dpn_lo = dpn_hi;
-        }
-        u = u - dts * s.cqu[o] * dpxy + (c1 * mudf_xy);
-      }
-      s.u_2[o] = u;
-    }
+  // Every lane of the (i <= ide, j <= jde) patch evaluates both updates with loads that
+  // stay inside the memory patch, and only the stores are masked: loads under per-lane
+  // conditions would make the compiler wait for all loads in flight at each join.
+  const bool u_on = j >= j_start && j <= j_end;
+  const bool v_on = i <= i_end && j >= j_start_v_tend && j <= j_end_v_tend;
+  const bool utend_on = i >= i_start_u_tend && i <= i_end_u_tend, upg_on = i >= i_start_up && i <= i_end_up;
+  const bool vpg_on = j >= j_start_vp && j <= j_end_vp;
+  // column constants (map-factor ratios, column masses, the mudf term)
+  const float mudf_xu = -emdiv * dx * (mudf[o2] - mudf[X2(d, i - 1, j)]) / s.msfuy[o2];
+  const float mu_m = s.msfux[o2] / s.msfuy[o2], muu = s.muu[o2], mu0u = mu[X2(d, i - 1, j)], mu1 = mu[o2];
+  const float mudf_yv = -emdiv * dy * (mudf[o2] - mudf[X2(d, i, j - 1)]) * s.msfvx_inv[o2];
+  const float mv_m = s.msfvy[o2] / s.msfvx[o2], muv = s.muv[o2], mu0v = mu[X2(d, i, j - 1)];
+  // dpn at w level kk averaged to the u (sh = 1) or v (sh = sj) point, general form
+  auto dpn_gen = [&](int kk, int sh) -> float {
+    const Off q = X3(d, i, kk, j);
+    return .5f * (s.fnm[X1(d, kk)] * (p[q] + p[q - sh]) + s.fnp[X1(d, kk)] * (p[q - d.ni] + p[q - d.ni - sh]));
+  };
+  // kk = 1: extrapolation with cf1..cf3; kk = kde: lid (zero without top_lid)
+  auto dpn_bot = [&](int sh) -> float {
+    const Off q = X3(d, i, 1, j);
+    return .5f * (cf1 * (p[q] + p[q - sh]) + cf2 * (p[q + d.ni] + p[q + d.ni - sh]) + cf3 * (p[q + 2 * d.ni] + p[q + 2 * d.ni - sh]));
+  };
+  auto dpn_top = [&](int sh) -> float {
+    const Off q = X3(d, i, kde - 1, j);
+    return .5f * (cf1 * (p[q - sh] + p[q]) + cf2 * (p[q - d.ni - sh] + p[q - d.ni]) + cf3 * (p[q - 2 * d.ni - sh] + p[q - 2 * d.ni]));
+  };
+  float dpnu_lo, dpnv_lo, dpnu_top = 0.f, dpnv_top = 0.f;
+  if (k0 == 1) {
+    dpnu_lo = dpn_bot(1);
+    dpnv_lo = dpn_bot(d.sj);
+  } else {
+    dpnu_lo = dpn_gen(k0, 1);
+    dpnv_lo = dpn_gen(k0, d.sj);
   }
-  // v
-  if (i <= i_end && j >= j_start_v_tend && j <= j_end_v_tend) {
-    const bool pg_on = j >= j_start_vp && j <= j_end_vp;
-    float mudf_xy = 0.f, m = 0.f, muv = 0.f, mu0 = 0.f, mu1 = 0.f;
-    if (pg_on) {
-      mudf_xy = -emdiv * dy * (mudf[o2] - mudf[X2(d, i, j - 1)]) * s.msfvx_inv[o2];
-      m = s.msfvy[o2] / s.msfvx[o2];
-      muv = s.muv[o2];
-      mu0 = mu[X2(d, i, j - 1)];
-      mu1 = mu[o2];
+  if (c.top_lid && k1 == kde - 1) {
+    dpnu_top = dpn_top(1);
+    dpnv_top = dpn_top(d.sj);
+  }
+#pragma unroll
+  for (int kk = 0; kk < kAdvUvK; kk++) {
+    const int k = imin(k0 + kk, k1);  // levels past k1 repeat k1 and are not stored
+    const Off o = X3(d, i, k, j);
+    const float c1 = s.c1h[X1(d, k)], c2 = s.c2h[X1(d, k)], rdnw = s.rdnw[X1(d, k)];
+    const int kp = imin(k + 1, kde);
+    const bool top = k + 1 == kde;
+    // u
+    {
+      float u = s.u_2[o];
+      const float ut = s.ru_tend[o];
+      u = utend_on ? u + dts * ut : u;
+      float dpxy = mu_m * .5f * rdx * (c1 * muu + c2) *
+                   (((ph[o + d.ni] - ph[X3(d, i - 1, k + 1, j)]) + (ph[o] - ph[o - 1])) +
+                    (alt[o] + alt[o - 1]) * (p[o] - p[o - 1]) +
+                    (al[o] + al[o - 1]) * (pb[o] - pb[o - 1]));
+      const float g_hi = dpn_gen(kp, 1);
+      const float dpn_hi = top ? dpnu_top : g_hi;
+      const float dpxn = dpxy + mu_m * rdx * (php[o] - php[o - 1]) * (rdnw * (dpn_hi - dpnu_lo) - .5f * ((c1 * mu0u) + (c1 * mu1)));
+      dpxy = nh ? dpxn : dpxy;
+      dpnu_lo = dpn_hi;
+      const float upg = u - dts * s.cqu[o] * dpxy + (c1 * mudf_xu);
+      u = upg_on ? upg : u;
+      if (u_on && k0 + kk <= k1) s.u_2[o] = u;
     }
-    auto dpn = [&](int kk) -> float {
-      if (kk == 1)
-        return .5f * (cf1 * (p[X3(d, i, 1, j)] + p[X3(d, i, 1, j - 1)]) + cf2 * (p[X3(d, i, 2, j)] + p[X3(d, i, 2, j - 1)]) +
-                      cf3 * (p[X3(d, i, 3, j)] + p[X3(d, i, 3, j - 1)]));
-      if (kk == kde) {
-        if (!c.top_lid) return 0.f;
-        return .5f * (cf1 * (p[X3(d, i, kde - 1, j - 1)] + p[X3(d, i, kde - 1, j)]) + cf2 * (p[X3(d, i, kde - 2, j - 1)] + p[X3(d, i, kde - 2, j)]) +
-                      cf3 * (p[X3(d, i, kde - 3, j - 1)] + p[X3(d, i, kde - 3, j)]));
-      }
-      return .5f * (s.fnm[X1(d, kk)] * (p[X3(d, i, kk, j)] + p[X3(d, i, kk, j - 1)]) + s.fnp[X1(d, kk)] * (p[X3(d, i, kk - 1, j)] + p[X3(d, i, kk - 1, j - 1)]));
-    };
-    float dpn_lo = (pg_on && nh) ? dpn(k0) : 0.f;
-    for (int k = k0; k <= k1; k++) {
-      const Off o = X3(d, i, k, j);
-      const float c1 = s.c1h[X1(d, k)], c2 = s.c2h[X1(d, k)];
+    // v
+    {
       float v = s.v_2[o];
       v = v + dts * s.rv_tend[o];
-      if (pg_on) {
-        float dpxy = m * .5f * rdy * (c1 * muv + c2) *
-                     (((ph[o + d.ni] - ph[X3(d, i, k + 1, j - 1)]) + (ph[o] - ph[o - d.sj])) +
-                      (alt[o] + alt[o - d.sj]) * (p[o] - p[o - d.sj]) +
-                      (al[o] + al[o - d.sj]) * (pb[o] - pb[o - d.sj]));
-        if (nh) {
-          const float dpn_hi = dpn(k + 1);
-          dpxy = dpxy + m * rdy * (php[o] - php[o - d.sj]) * (s.rdnw[X1(d, k)] * (dpn_hi - dpn_lo) - .5f * ((c1 * mu0) + (c1 * mu1)));
-          dpn_lo = dpn_hi;
-        }
-        v = v - dts * s.cqv[o] * dpxy + (c1 * mudf_xy);
-      }
-      s.v_2[o] = v;
+      float dpxy = mv_m * .5f * rdy * (c1 * muv + c2) *
+                   (((ph[o + d.ni] - ph[X3(d, i, k + 1, j - 1)]) + (ph[o] - ph[o - d.sj])) +
+                    (alt[o] + alt[o - d.sj]) * (p[o] - p[o - d.sj]) +
+                    (al[o] + al[o - d.sj]) * (pb[o] - pb[o - d.sj]));
+      const float g_hi = dpn_gen(kp, d.sj);
+      const float dpn_hi = top ? dpnv_top : g_hi;
+      const float dpyn = dpxy + mv_m * rdy * (php[o] - php[o - d.sj]) * (rdnw * (dpn_hi - dpnv_lo) - .5f * ((c1 * mu0v) + (c1 * mu1)));
+      dpxy = nh ? dpyn : dpxy;
+      dpnv_lo = dpn_hi;
+      const float vpg = v - dts * s.cqv[o] * dpxy + (c1 * mudf_yv);
+      v = vpg_on ? vpg : v;
+      if (v_on && k0 + kk <= k1) s.v_2[o] = v;
     }
   }
 }

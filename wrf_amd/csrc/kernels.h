@@ -8,6 +8,9 @@ constexpr int kAdvK = 8;     // levels per block in k_adv_scalar
 constexpr int kTendK = 8;    // levels per thread in k_tend_u/v/w/t
 constexpr int kAdvTileI = 63;  // i points per wave in k_adv_scalar (lane 63 supplies a face)
 constexpr int kAdvRows = 4;    // j rows (waves) per block in k_adv_scalar
+constexpr int kVecW = 4;       // i points per lane in the vector kernels (16-byte accesses)
+constexpr int kVecTileI = 64 * kVecW;  // i points per wave in the vector kernels
+constexpr int kAdvVK = 4;      // levels per thread in k_adv_scalar_v
 // Every state kernel is a template on its argument type: KArgs for one domain, BatchArgs
 // for an ensemble launch over the tiles of all members (k_halo: HaloArgs / BatchHalo).
 template <typename A> __global__ void k_step_prep(A);
@@ -27,6 +30,7 @@ template <typename A> __global__ void k_ac_wpred(A);
 template <typename A, typename CT> __global__ void k_ac_wsolve(A);
 template <typename A> __global__ void k_small_finish(A);
 template <typename A, bool PER> __global__ void k_adv_scalar(A);
+template <typename A> __global__ void k_adv_scalar_v(A);
 template <typename A> __global__ void k_update_scalar(A);
 template <typename A> __global__ void k_p_rho_phi(A);
 template <typename A> __global__ void k_halo(A);

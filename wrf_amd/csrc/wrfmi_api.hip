@@ -294,6 +294,7 @@ struct Launcher {
   HaloArgs h;
   int err = 0;
   RecMember *rec = nullptr;  // set: record the launches instead of enqueueing them
+  bool vec = false;          // the layout allows the four-points-per-lane kernels
   // launch with an optional event pair around it (kernel timing mode)
   template <typename K, typename... A>
   void launch(int id, K kern, dim3 g, dim3 b, A... args) {
@@ -439,7 +440,14 @@ static void ph_scalar(Launcher &L) {
   const Dm &d = L.a.d;
   const dim3 ga((d.ide - d.ids + 1 + kAdvTileI - 1) / kAdvTileI, (d.jde - d.jds + 1 + kAdvRows - 1) / kAdvRows,
                 (d.kde - d.kds + 1 + kAdvK - 1) / kAdvK);
-  L.launchK(KID_ADV_SCALAR, per ? KRP(k_adv_scalar, true) : KRP(k_adv_scalar, false), ga, dim3(64, kAdvRows, 1), L.a);
+  if (per && L.vec) {
+    const int ia0 = d.ims + 4 * ((d.ids - d.ims) >> 2);
+    const dim3 gv((d.ide - ia0 + kVecTileI) / kVecTileI, (d.jde - d.jds + kAdvRows) / kAdvRows,
+                  (d.kde - d.kds + kAdvVK) / kAdvVK);
+    L.launchK(KID_ADV_SCALAR, KR(k_adv_scalar_v), gv, dim3(64, kAdvRows, 1), L.a);
+  } else {
+    L.launchK(KID_ADV_SCALAR, per ? KRP(k_adv_scalar, true) : KRP(k_adv_scalar, false), ga, dim3(64, kAdvRows, 1), L.a);
+  }
   L.launchK(KID_UPDATE_SCALAR, KR(k_update_scalar), g_pt3(L.a.d, L.a.d.kde - L.a.d.kds), kColBlk, L.a);
 }
 static void ph_p_rho_phi(Launcher &L) {
@@ -496,6 +504,27 @@ static void enqueue_step(Launcher &L) {
   ph_bc_dry_2(L, 1);
 }
 
+// The four-points-per-lane kernels read 16-byte groups starting at an i whose offset in the
+// row is a multiple of four, and windows reaching 7 points past ide: rows must be a multiple
+// of four floats long with that much room, and every array 16-byte aligned.
+static bool vec_layout(const wrfmi_state &s, const Dm &d, const KArgs &a) {
+  if (d.ni % 4 != 0 || d.ni < d.ide - d.ims + 8) return false;
+  bool ok = true;
+  auto al = [&](const void *p) { ok = ok && ((uintptr_t)p & 15) == 0; };
+#define AL(n) al(s.n);
+  WRFMI_FIELDS_3D(AL) WRFMI_FIELDS_2D(AL)
+#undef AL
+  for (int m = 0; m < s.n_species; m++) {
+    al(s.q[m]);
+    al(s.q_old[m]);
+    al(s.q_tend[m]);
+  }
+  al(a.scr_dv.p);
+  al(a.scr_rhs.p);
+  al(a.scr_lo.p);
+  return ok;
+}
+
 // Validate and translate the arguments into L; the work arrays are the library's shared
 // ones for this field size unless own_scratch is given (ensemble members each have theirs).
 static int prepare(const wrfmi_state *st_in, const wrfmi_config *cfg, const wrfmi_dims *dims, Launcher &L, wrfmi_state &s,
@@ -518,6 +547,7 @@ static int prepare(const wrfmi_state *st_in, const wrfmi_config *cfg, const wrfm
   L.h.d = d;
   L.h.periodic_x = cfg->periodic_x; L.h.periodic_y = cfg->periodic_y;
   L.h.open_xs = cfg->open_xs; L.h.open_xe = cfg->open_xe; L.h.open_ys = cfg->open_ys; L.h.open_ye = cfg->open_ye;
+  L.vec = vec_layout(s, d, L.a);
   return 0;
 }
 
