@@ -18,6 +18,12 @@ CASES = {
     "periodic": dict(nx=14, ny=11, nz=10, bc="periodic", n_passive=1, n_bubbles=2, bubble_amp=2.0, seed=11),
     "open": dict(nx=13, ny=12, nz=9, bc="open", n_bubbles=1, seed=12),
     "periodic_pm1": dict(nx=12, ny=10, nz=9, bc="periodic", n_bubbles=1, seed=13, precision_mode=1),
+    # rows of 23 and 22 floats: not a multiple of four, so the one-point kernels run throughout
+    "periodic_unaligned": dict(nx=14, ny=11, nz=10, bc="periodic", n_passive=1, n_bubbles=2, bubble_amp=2.0, seed=14, pad_i=1),
+    "open_unaligned": dict(nx=13, ny=12, nz=9, bc="open", n_bubbles=1, seed=15, pad_i=1),
+    # rows of 32 floats ending 6 points after ide: the four-point windows run past the row
+    "periodic_tight": dict(nx=21, ny=9, nz=8, bc="periodic", n_passive=1, n_bubbles=1, seed=17, pad_i=4),
+    "open_tight": dict(nx=21, ny=10, nz=8, bc="open", n_bubbles=1, seed=18, pad_i=4),
 }
 
 

@@ -20,6 +20,9 @@ pytestmark = pytest.mark.gpu
 CASES = {
     "periodic": dict(nx=24, ny=20, nz=16, bc="periodic", n_passive=2, n_bubbles=2, bubble_amp=2.0, seed=11),
     "open": dict(nx=22, ny=19, nz=14, bc="open", n_bubbles=1, seed=12),
+    # rows of 30 floats (not a multiple of four): the one-point kernel forms throughout
+    "periodic_unaligned": dict(nx=21, ny=17, nz=12, bc="periodic", n_passive=1, n_bubbles=2, bubble_amp=2.0, seed=16,
+                               pad_i=1),
 }
 KINDS = dict(u_2="u", v_2="v", w_2="w", t_2="mass", mu_2="mass", p="mass", ph_2="w", al="mass", q0="mass")
 

@@ -228,6 +228,23 @@ DEVI Geo geo(const wrfmi_config &c) {
 template <typename T> DEVI T imin(T a, T b) { return a < b ? a : b; }
 template <typename T> DEVI T imax(T a, T b) { return a > b ? a : b; }
 
+// Vector kernels: a lane holds the four points i0..i0+3, where i0 - ims is a multiple of four
+// (so that the lane's memory is a 16-byte group); vec_ia0 is the first such i at or below ids.
+DEVI int vec_ia0(const Dm &d) { return d.ims + 4 * ((d.ids - d.ims) >> 2); }
+// store of the lane's points that lie in [lo, hi]
+DEVI void st4m(const FPtr<float> &p, Off o, const F4 &v, int i0, int lo, int hi) {
+  if (i0 >= lo && i0 + 3 <= hi) {
+    p.st<F4>(o, v);
+  } else {
+#pragma unroll
+    for (int n = 0; n < 4; n++)
+      if (i0 + n >= lo && i0 + n <= hi) p[o + n] = v.v[n];
+  }
+}
+// neighbours in i of point n from the lane's group c and the groups l (i0-4..) and r (i0+4..)
+DEVI float xm1(const F4 &l, const F4 &c, int n) { return n ? c.v[n - 1] : l.v[3]; }
+DEVI float xp1(const F4 &c, const F4 &r, int n) { return n < 3 ? c.v[n + 1] : r.v[0]; }
+
 // share/module_model_constants.F:17-47, evaluated in FP32 as the reference's REAL parameters
 constexpr float K_G = 9.81f;
 constexpr float K_RD = 287.f;

@@ -537,6 +537,246 @@ __global__ void __launch_bounds__(256) k_ac_wpred(ARGS args) {
   w[o] = wv;
 }
 
+// ---------------------------------------------------------------------------------------
+// Four-points-per-lane forms of k_ac_div, k_ac_theta and k_ac_wpred (16-byte loads and
+// stores; used when the host finds the layout aligned).  Element by element they evaluate
+// the same expressions in the same order as the one-point kernels above; i neighbours come
+// from the groups of four on either side.
+
+template <typename ARGS>
+__global__ void __launch_bounds__(256) k_ac_div_v(ARGS args) {
+  Tile3 T;
+  const KArgs &a = kargs(args, T);
+  const Dm &d = a.d;
+  const DState &s = dstate(a);
+  const int i0 = vec_ia0(d) + kVecW * (T.bx * 64 + (int)threadIdx.x);
+  const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
+  const int k = d.kds + (int)T.bz;
+  const int ids = d.ids, ide = d.ide, jde = d.jde, kde = d.kde;
+  if (i0 > ide || j > jde || k > kde) return;
+  const Off o2 = X2(d, i0, j), o = X3(d, i0, k, j);
+  const bool mj = j <= jde - 1, kin = k <= kde - 1;
+  const int it = a.iteration, ns = a.nsmall;
+  const float nsf = (float)ns;
+  const float c1 = s.c1h[X1(d, imin(k, kde - 1))], c2 = s.c2h[X1(d, imin(k, kde - 1))];
+  const F4 u = s.u_2.ld<F4>(o), v = s.v_2.ld<F4>(o), us = s.u_save.ld<F4>(o), vs = s.v_save.ld<F4>(o);
+  const F4 rum0 = s.ru_m.ld<F4>(o), rvm0 = s.rv_m.ld<F4>(o);
+  const F4 muu = s.muu.ld<F4>(o2), msfuy = s.msfuy.ld<F4>(o2), muv = s.muv.ld<F4>(o2), mvi = s.msfvx_inv.ld<F4>(o2);
+  // sumflux of ru_m, rv_m (and ww_m outside the mass points), as sumflux_uv
+  F4 rum, rvm;
+#pragma unroll
+  for (int n = 0; n < 4; n++) {
+    const bool mi = i0 + n <= ide - 1;
+    float ru_ = it == 1 ? 0.f : rum0.v[n];
+    float rv_ = it == 1 ? 0.f : rvm0.v[n];
+    if (mj && kin) ru_ = ru_ + u.v[n];
+    if (mi && kin) rv_ = rv_ + v.v[n];
+    if (it == ns) {
+      if (mj && kin) ru_ = ru_ / nsf + (c1 * muu.v[n] + c2) * us.v[n] / msfuy.v[n];
+      if (mi && kin) rv_ = rv_ / nsf + (c1 * muv.v[n] + c2) * vs.v[n] * mvi.v[n];
+    }
+    rum.v[n] = ru_;
+    rvm.v[n] = rv_;
+  }
+  st4m(s.ru_m, o, rum, i0, ids, ide);
+  st4m(s.rv_m, o, rvm, i0, ids, ide);
+  if (it == 1) {
+#pragma unroll
+    for (int n = 0; n < 4; n++)
+      if (i0 + n >= ids && i0 + n <= ide && !(i0 + n <= ide - 1 && mj)) s.ww_m[o + n] = 0.f;
+  }
+  if (!mj || !kin) return;
+  // dvdxi at the mass points
+  const F4 ur = s.u_2.ld<F4>(o + 4), usr = s.u_save.ld<F4>(o + 4);
+  const F4 muur = s.muu.ld<F4>(o2 + 4), msfuyr = s.msfuy.ld<F4>(o2 + 4);
+  const F4 vn = s.v_2.ld<F4>(o + d.sj), vsn = s.v_save.ld<F4>(o + d.sj);
+  const F4 muvn = s.muv.ld<F4>(o2 + d.ni), mvin = s.msfvx_inv.ld<F4>(o2 + d.ni);
+  const F4 msftx = s.msftx.ld<F4>(o2), msfty = s.msfty.ld<F4>(o2);
+  const float rdx = s.rdx, rdy = s.rdy;
+  F4 dv;
+#pragma unroll
+  for (int n = 0; n < 4; n++) {
+    const float u1 = xp1(u, ur, n), us1 = xp1(us, usr, n), muu1 = xp1(muu, muur, n), msfuy1 = xp1(msfuy, msfuyr, n);
+    dv.v[n] = msftx.v[n] * msfty.v[n] *
+              (rdy * ((vn.v[n] + (c1 * muvn.v[n] + c2) * vsn.v[n] * mvin.v[n]) - (v.v[n] + (c1 * muv.v[n] + c2) * vs.v[n] * mvi.v[n])) +
+               rdx * ((u1 + (c1 * muu1 + c2) * us1 / msfuy1) - (u.v[n] + (c1 * muu.v[n] + c2) * us.v[n] / msfuy.v[n])));
+  }
+  st4m(a.scr_dv, o, dv, i0, ids, ide - 1);
+}
+
+template <typename ARGS>
+__global__ void __launch_bounds__(256) k_ac_theta_v(ARGS args) {
+  Tile3 T;
+  const KArgs &a = kargs(args, T);
+  const Dm &d = a.d;
+  const DState &s = dstate(a);
+  const int i0 = vec_ia0(d) + kVecW * (T.bx * 64 + (int)threadIdx.x);
+  const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
+  const int k = d.kds + (int)T.bz;
+  const int ids = d.ids, ide = d.ide, jde = d.jde, kde = d.kde, k_end = kde - 1;
+  if (i0 > ide - 1 || j > jde - 1 || k > kde) return;
+  const wrfmi_config &c = a.c;
+  const Off o2 = X2(d, i0, j), o = X3(d, i0, k, j);
+  const int ni = d.ni, sj = d.sj;
+  const int kp = imin(k + 1, kde), km = imax(k - 1, d.kds);  // clamped neighbour levels
+  const Off op = o + (kp - k) * ni, om = o - (k - km) * ni;
+  const float dts = a.dts, epssm = c.epssm, rdx = s.rdx, rdy = s.rdy;
+  const F4 msftx = s.msftx.ld<F4>(o2), msfty = s.msfty.ld<F4>(o2);
+  const F4 ww = s.ww.ld<F4>(o);
+  {  // sumflux of omega at the mass points
+    const int it = a.iteration, ns = a.nsmall;
+    const F4 wwm0 = s.ww_m.ld<F4>(o), ww1 = s.ww1.ld<F4>(o);
+    F4 wwm;
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+      float x = it == 1 ? 0.f : wwm0.v[n];
+      x = x + ww.v[n];
+      if (it == ns) x = x / (float)ns + ww1.v[n];
+      wwm.v[n] = x;
+    }
+    st4m(s.ww_m, o, wwm, i0, ids, ide - 1);
+  }
+  const F4 t1 = s.t_save.ld<F4>(o);
+  if (k <= k_end) {
+    const F4 tv = s.t_2.ld<F4>(o), tt = s.t_tend.ld<F4>(o);
+    const F4 wwp = s.ww.ld<F4>(op), t1p = s.t_save.ld<F4>(op), t1m = s.t_save.ld<F4>(om);
+    const F4 t1l = s.t_save.ld<F4>(o - 4), t1r = s.t_save.ld<F4>(o + 4);
+    const F4 t1n = s.t_save.ld<F4>(o + sj), t1s = s.t_save.ld<F4>(o - sj);
+    const F4 u = s.u_2.ld<F4>(o), ur = s.u_2.ld<F4>(o + 4), v = s.v_2.ld<F4>(o), vn = s.v_2.ld<F4>(o + sj);
+    const float fnm_p = s.fnm[X1(d, kp)], fnp_p = s.fnp[X1(d, kp)], fnm_0 = s.fnm[X1(d, k)], fnp_0 = s.fnp[X1(d, k)];
+    const float rdnw = s.rdnw[X1(d, k)], c1 = s.c1h[X1(d, k)], c2 = s.c2h[X1(d, k)];
+    F4 tn, t2s;
+    const F4 muave = s.muave.ld<F4>(o2), muts = s.muts.ld<F4>(o2);
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+      float t = tv.v[n] + msfty.v[n] * dts * tt.v[n];
+      const float wdtn_hi = (k + 1 <= k_end) ? wwp.v[n] * (fnm_p * t1p.v[n] + fnp_p * t1.v[n]) : 0.f;
+      const float wdtn_lo = (k >= 2) ? ww.v[n] * (fnm_0 * t1.v[n] + fnp_0 * t1m.v[n]) : 0.f;
+      t = t - dts * msfty.v[n] *
+                  (msftx.v[n] * (.5f * rdy * (vn.v[n] * (t1n.v[n] + t1.v[n]) - v.v[n] * (t1.v[n] + t1s.v[n])) +
+                                 .5f * rdx * (xp1(u, ur, n) * (xp1(t1, t1r, n) + t1.v[n]) - u.v[n] * (t1.v[n] + xm1(t1l, t1, n)))) +
+                   rdnw * (wdtn_hi - wdtn_lo));
+      tn.v[n] = t;
+      float v1 = .5f * ((1.f + epssm) * t + (1.f - epssm) * tv.v[n]);
+      v1 = (v1 + (c1 * muave.v[n]) * K_T0) / ((c1 * muts.v[n] + c2) * (K_T0 + t1.v[n]));
+      t2s.v[n] = c.non_hydrostatic ? v1 : tv.v[n];
+    }
+    st4m(s.t_2, o, tn, i0, ids, ide - 1);
+    st4m(s.t_2save, o, t2s, i0, ids, ide - 1);
+  }
+  if (!c.non_hydrostatic || k < 2) return;
+  // rhs(k), k = 2..kde
+  const F4 ph1 = s.ph_save.ld<F4>(o), ph1p = s.ph_save.ld<F4>(op), ph1m = s.ph_save.ld<F4>(om);
+  const F4 pb0 = s.phb.ld<F4>(o), pbp = s.phb.ld<F4>(op), pbm = s.phb.ld<F4>(om);
+  const F4 wwp = s.ww.ld<F4>(op), wwm = s.ww.ld<F4>(om);
+  const F4 pht = s.ph_tend.ld<F4>(o), w2 = s.w_2.ld<F4>(o), ph2 = s.ph_2.ld<F4>(o), mut = s.mut.ld<F4>(o2);
+  const float rdnw_k = s.rdnw[X1(d, imin(k, k_end))], rdnw_m = s.rdnw[X1(d, k - 1)];
+  const float fnm_0 = s.fnm[X1(d, k)], fnp_0 = s.fnp[X1(d, k)], c1f = s.c1f[X1(d, k)], c2f = s.c2f[X1(d, k)];
+  const bool z2 = c.phi_adv_z == 2;
+  F4 rr;
+#pragma unroll
+  for (int n = 0; n < 4; n++) {
+    // wdwn(k+1) (level k) and wdwn(k) (level k-1)
+    const float dph_hi = ph1p.v[n] - ph1.v[n] + pbp.v[n] - pb0.v[n];
+    const float dph_lo = ph1.v[n] - ph1m.v[n] + pb0.v[n] - pbm.v[n];
+    const float wd_hi = z2 ? rdnw_k * dph_hi : .5f * (wwp.v[n] + ww.v[n]) * rdnw_k * dph_hi;
+    const float wd_lo = z2 ? rdnw_m * dph_lo : .5f * (ww.v[n] + wwm.v[n]) * rdnw_m * dph_lo;
+    float r = dts * (pht.v[n] + .5f * K_G * (1.f - epssm) * w2.v[n]);
+    if (k <= k_end) {
+      if (z2) r = r - dts * ww.v[n] * (fnm_0 * wd_hi + fnp_0 * wd_lo);
+      else r = r - dts * (fnm_0 * wd_hi + fnp_0 * wd_lo);
+    }
+    r = ph2.v[n] + msfty.v[n] * r / (c1f * mut.v[n] + c2f);
+    if (c.top_lid && k == k_end + 1) r = 0.f;
+    rr.v[n] = r;
+  }
+  st4m(a.scr_rhs, o, rr, i0, ids, ide - 1);
+}
+
+template <typename ARGS>
+__global__ void __launch_bounds__(256) k_ac_wpred_v(ARGS args) {
+  Tile3 T;
+  const KArgs &a = kargs(args, T);
+  const Dm &d = a.d;
+  const DState &s = dstate(a);
+  const int i0 = vec_ia0(d) + kVecW * (T.bx * 64 + (int)threadIdx.x);
+  const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
+  const int k = d.kds + (int)T.bz;
+  const int ids = d.ids, ide = d.ide, jde = d.jde, kde = d.kde, k_end = kde - 1;
+  if (i0 > ide - 1 || j > jde - 1 || k > kde) return;
+  const wrfmi_config &c = a.c;
+  if (!c.non_hydrostatic) return;
+  const Off o2 = X2(d, i0, j), o = X3(d, i0, k, j);
+  const int ni = d.ni, sj = d.sj;
+  const float dts = a.dts, epssm = c.epssm, rdx = s.rdx, rdy = s.rdy;
+  const F4 msftx = s.msftx.ld<F4>(o2), msfty = s.msfty.ld<F4>(o2);
+  FPtr<float> w = s.w_2;
+  if (k == 1) {
+    const FPtr<float> u = s.u_2, v = s.v_2, ht = s.ht;
+    const float cf1 = s.cf1, cf2 = s.cf2, cf3 = s.cf3;
+    const F4 h0 = ht.ld<F4>(o2), hl = ht.ld<F4>(o2 - 4), hr = ht.ld<F4>(o2 + 4), hn = ht.ld<F4>(o2 + ni), hs = ht.ld<F4>(o2 - ni);
+    const F4 v1n = v.ld<F4>(o + sj), v2n = v.ld<F4>(o + sj + ni), v3n = v.ld<F4>(o + sj + 2 * ni);
+    const F4 v1 = v.ld<F4>(o), v2 = v.ld<F4>(o + ni), v3 = v.ld<F4>(o + 2 * ni);
+    const F4 u1 = u.ld<F4>(o), u2 = u.ld<F4>(o + ni), u3 = u.ld<F4>(o + 2 * ni);
+    const F4 u1r = u.ld<F4>(o + 4), u2r = u.ld<F4>(o + ni + 4), u3r = u.ld<F4>(o + 2 * ni + 4);
+    F4 wv;
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+      wv.v[n] = msfty.v[n] * .5f * rdy *
+                    ((hn.v[n] - h0.v[n]) * (cf1 * v1n.v[n] + cf2 * v2n.v[n] + cf3 * v3n.v[n]) +
+                     (h0.v[n] - hs.v[n]) * (cf1 * v1.v[n] + cf2 * v2.v[n] + cf3 * v3.v[n])) +
+                msftx.v[n] * .5f * rdx *
+                    ((xp1(h0, hr, n) - h0.v[n]) * (cf1 * xp1(u1, u1r, n) + cf2 * xp1(u2, u2r, n) + cf3 * xp1(u3, u3r, n)) +
+                     (h0.v[n] - xm1(hl, h0, n)) * (cf1 * u1.v[n] + cf2 * u2.v[n] + cf3 * u3.v[n]));
+    }
+    st4m(w, o, wv, i0, ids, ide - 1);
+    return;
+  }
+  const FPtr<float> ph = s.ph_2, rhs = a.scr_rhs, c2a = s.c2a, alt = s.alt, t2ave = s.t_2save;
+  const CPtr<float> c1h = s.c1h, c2h = s.c2h, c1f = s.c1f;
+  const F4 mut = s.mut.ld<F4>(o2), muave = s.muave.ld<F4>(o2);
+  const Off om = o - ni, op = o + (k <= k_end ? ni : 0);
+  const F4 rhsm = rhs.ld<F4>(om), rhs0 = rhs.ld<F4>(o), rhsp = rhs.ld<F4>(op);
+  const F4 phm = ph.ld<F4>(om), ph0 = ph.ld<F4>(o), php_ = ph.ld<F4>(op);
+  const F4 t2lo = t2ave.ld<F4>(om), t2hi = t2ave.ld<F4>(o);
+  const F4 c2a0 = c2a.ld<F4>(o), c2am = c2a.ld<F4>(om), alt0 = alt.ld<F4>(o), altm = alt.ld<F4>(om);
+  const F4 w0 = w.ld<F4>(o), rwt = s.rw_tend.ld<F4>(o), cqw = s.cqw.ld<F4>(o);
+  const int kc = imin(k, k_end);
+  const float rdn = s.rdn[X1(d, kc)], rdnw0 = s.rdnw[X1(d, kc)], rdnwm = s.rdnw[X1(d, k - 1)];
+  const float c1h0 = c1h[X1(d, kc)], c2h0 = c2h[X1(d, kc)], c1hm = c1h[X1(d, k - 1)], c2hm = c2h[X1(d, k - 1)];
+  const float c1fk = c1f[X1(d, k)];
+  F4 wv;
+#pragma unroll
+  for (int n = 0; n < 4; n++) {
+    const float msft_inv = 1.f / msfty.v[n];
+    const float rhs_m1 = k >= 3 ? rhsm.v[n] : 0.f;  // rhs(1) = 0
+    const float rhs_0 = rhs0.v[n];
+    const float phl_m1 = phm.v[n], phl_0 = ph0.v[n];
+    const float t2a_lo = t2lo.v[n];
+    float x;
+    if (k <= k_end) {
+      const float rhs_p1 = rhsp.v[n], phl_p1 = php_.v[n], t2a_hi = t2hi.v[n];
+      x = w0.v[n] + dts * rwt.v[n] +
+          msft_inv * cqw.v[n] *
+              (+.5f * dts * K_G * rdn *
+               (c2a0.v[n] * rdnw0 / (c1h0 * mut.v[n] + c2h0) *
+                    ((1.f + epssm) * (rhs_p1 - rhs_0) + (1.f - epssm) * (phl_p1 - phl_0)) -
+                c2am.v[n] * rdnwm / (c1hm * mut.v[n] + c2hm) *
+                    ((1.f + epssm) * (rhs_0 - rhs_m1) + (1.f - epssm) * (phl_0 - phl_m1)))) +
+          dts * K_G * msft_inv *
+              (rdn * (c2a0.v[n] * alt0.v[n] * t2a_hi - c2am.v[n] * altm.v[n] * t2a_lo) - (c1fk * muave.v[n]));
+    } else {
+      x = w0.v[n] + dts * rwt.v[n] +
+          msft_inv * (-.5f * dts * K_G / (c1hm * mut.v[n] + c2hm) * (rdnwm * rdnwm) * 2.f *
+                          c2am.v[n] * ((1.f + epssm) * (rhs_0 - rhs_m1) + (1.f - epssm) * (phl_0 - phl_m1)) -
+                      dts * K_G * (2.f * rdnwm * c2am.v[n] * altm.v[n] * t2a_lo + (c1fk * muave.v[n])));
+      if (c.top_lid) x = 0.f;
+    }
+    wv.v[n] = x;
+  }
+  st4m(w, o, wv, i0, ids, ide - 1);
+}
+
 // Per column: the tridiagonal forward elimination and back substitution (advance_w
 // 1433-1443), ph from rhs and the final w (1460-1465) and calc_p_rho for step >= 1
 // (438-568).  CT = double keeps the forward value in FP64, its low-order part parked in scr_lo.
@@ -708,6 +948,9 @@ WRFMI_INST(k_p_rho0)
 WRFMI_INST(k_ac_div)
 WRFMI_INST(k_ac_theta)
 WRFMI_INST(k_ac_wpred)
+WRFMI_INST(k_ac_div_v)
+WRFMI_INST(k_ac_theta_v)
+WRFMI_INST(k_ac_wpred_v)
 WRFMI_INST(k_small_finish)
 
 }  // namespace wrfmi

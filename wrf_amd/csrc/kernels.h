@@ -27,6 +27,9 @@ template <typename A> __global__ void k_ac_div(A);
 template <typename A, bool COMP> __global__ void k_ac_mu_ww(A);
 template <typename A> __global__ void k_ac_theta(A);
 template <typename A> __global__ void k_ac_wpred(A);
+template <typename A> __global__ void k_ac_div_v(A);
+template <typename A> __global__ void k_ac_theta_v(A);
+template <typename A> __global__ void k_ac_wpred_v(A);
 template <typename A, typename CT> __global__ void k_ac_wsolve(A);
 template <typename A> __global__ void k_small_finish(A);
 template <typename A, bool PER> __global__ void k_adv_scalar(A);

@@ -364,6 +364,12 @@ static inline dim3 g_pt3(const Dm &d, int nk) {
   return dim3(g.x, g.y, nk);
 }
 
+// (i, j) tiles of four-points-per-lane kernels (kVecTileI x 4 points from vec_ia0), nk levels
+static inline dim3 g_vec3(const Dm &d, int nk) {
+  const int ia0 = d.ims + 4 * ((d.ids - d.ims) >> 2);
+  return dim3((d.ide - ia0 + kVecTileI) / kVecTileI, (d.jde - d.jds + 4) / 4, nk);
+}
+
 static void ph_step_prep(Launcher &L) {
   const KArgs &a = L.a;
   const wrfmi_state &s = a.s;
@@ -412,11 +418,17 @@ static void ph_advance_mu_t_w(Launcher &L) {
   const wrfmi_state &s = a.s;
   const bool p64 = a.c.precision_mode == 1;
   const int nk = a.d.kde - a.d.kds + 1;
-  L.launchK(KID_AC_DIV, KR(k_ac_div), g_pt3(a.d, nk), kColBlk, a);
+  if (L.vec) L.launchK(KID_AC_DIV, KR(k_ac_div_v), g_vec3(a.d, nk), kColBlk, a);
+  else L.launchK(KID_AC_DIV, KR(k_ac_div), g_pt3(a.d, nk), kColBlk, a);
   if (p64) L.launchK(KID_AC_MU_WW, KRP(k_ac_mu_ww, true), g_col(a.d), kColBlk, a);
   else L.launchK(KID_AC_MU_WW, KRP(k_ac_mu_ww, false), g_col(a.d), kColBlk, a);
-  L.launchK(KID_AC_THETA, KR(k_ac_theta), g_pt3(a.d, nk), kColBlk, a);
-  L.launchK(KID_AC_WPRED, KR(k_ac_wpred), g_pt3(a.d, nk), kColBlk, a);
+  if (L.vec) {
+    L.launchK(KID_AC_THETA, KR(k_ac_theta_v), g_vec3(a.d, nk), kColBlk, a);
+    L.launchK(KID_AC_WPRED, KR(k_ac_wpred_v), g_vec3(a.d, nk), kColBlk, a);
+  } else {
+    L.launchK(KID_AC_THETA, KR(k_ac_theta), g_pt3(a.d, nk), kColBlk, a);
+    L.launchK(KID_AC_WPRED, KR(k_ac_wpred), g_pt3(a.d, nk), kColBlk, a);
+  }
   if (p64) L.launchK(KID_AC_WSOLVE, KRP(k_ac_wsolve, double), g_col(a.d), kColBlk, a);
   else L.launchK(KID_AC_WSOLVE, KRP(k_ac_wsolve, float), g_col(a.d), kColBlk, a);
   L.halo_begin();
@@ -505,10 +517,12 @@ static void enqueue_step(Launcher &L) {
 }
 
 // The four-points-per-lane kernels read 16-byte groups starting at an i whose offset in the
-// row is a multiple of four, and windows reaching 7 points past ide: rows must be a multiple
-// of four floats long with that much room, and every array 16-byte aligned.
+// row is a multiple of four: rows must be a multiple of four floats long and every array
+// 16-byte aligned.  Their windows reach up to 7 points past ide, which may run past the end
+// of the row into the next one (inside the array: no kernel reads the last row j = jme); such
+// values only feed points that are not stored.
 static bool vec_layout(const wrfmi_state &s, const Dm &d, const KArgs &a) {
-  if (d.ni % 4 != 0 || d.ni < d.ide - d.ims + 8) return false;
+  if (d.ni % 4 != 0) return false;
   bool ok = true;
   auto al = [&](const void *p) { ok = ok && ((uintptr_t)p & 15) == 0; };
 #define AL(n) al(s.n);
@@ -665,6 +679,7 @@ extern "C" int wrfmi_ensemble_create(int n_members, const wrfmi_state *states, c
   wrfmi_ensemble *e = new wrfmi_ensemble;
   e->n = n_members;
   std::vector<RecMember> rec(n_members);
+  std::vector<Launcher> Ls(n_members);
   int rc = 0;
   for (int m = 0; m < n_members && !rc; m++) {
     Dm d;
@@ -680,15 +695,21 @@ extern "C" int wrfmi_ensemble_create(int n_members, const wrfmi_state *states, c
       e->work.push_back(*p);
     }
     if (rc) break;
-    Launcher L;
     wrfmi_state s;
-    if ((rc = prepare(&states[m], &cfgs[m], &dims[m], L, s, &sc))) {
+    if ((rc = prepare(&states[m], &cfgs[m], &dims[m], Ls[m], s, &sc))) {
       std::string msg = g_err;
       rc = fail(rc, "member %d: %s", m, msg.c_str());
       break;
     }
-    L.rec = &rec[m];
-    enqueue_step(L);
+  }
+  // one kernel form for all members: the four-points-per-lane kernels only if every
+  // member's layout allows them
+  bool all_vec = true;
+  for (int m = 0; m < n_members && !rc; m++) all_vec = all_vec && Ls[m].vec;
+  for (int m = 0; m < n_members && !rc; m++) {
+    Ls[m].vec = all_vec;
+    Ls[m].rec = &rec[m];
+    enqueue_step(Ls[m]);
   }
   // every member must have produced the same launch sequence (same kernels and blocks):
   // members share the boundary type, precision mode, species count and acoustic steps
