@@ -260,6 +260,38 @@ __global__ void __launch_bounds__(256) k_update_scalar(ARGS args) {
   sc2[o] = ((c1 * muold + c2) * q1 + a.dt_rk * tendency) / (c1 * munew + c2);
 }
 
+// rk_update_scalar with four i points per lane (16-byte loads and stores)
+template <typename ARGS>
+__global__ void __launch_bounds__(256) k_update_scalar_v(ARGS args) {
+  Tile3 T;
+  const KArgs &a = kargs(args, T);
+  const Dm &d = a.d;
+  const DState &s = dstate(a);
+  const int i0 = vec_ia0(d) + kVecW * (T.bx * 64 + (int)threadIdx.x);
+  const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
+  const int k = d.kds + (int)T.bz;
+  if (i0 > d.ide - 1 || j > d.jde - 1 || k > d.kde - 1) return;
+  const Off o = X3(d, i0, k, j), o2 = X2(d, i0, j);
+  const FPtr<float> sc1 = s.q_old[a.species], sc2 = s.q[a.species];
+  const F4 mu1 = s.mu_1.ld<F4>(o2), mu2 = s.mu_2.ld<F4>(o2), mub = s.mub.ld<F4>(o2), msfty = s.msfty.ld<F4>(o2);
+  const F4 adv = s.advect_tend.ld<F4>(o), qt = s.q_tend[a.species].ld<F4>(o);
+  const F4 q2 = sc2.ld<F4>(o), q1o = sc1.ld<F4>(o);
+  const float c1 = s.c1h[X1(d, k)], c2 = s.c2h[X1(d, k)];
+  const bool rk1 = a.rk_step == 1;
+  F4 qn;
+#pragma unroll
+  for (int n = 0; n < 4; n++) {
+    const float muold = mu1.v[n] + mub.v[n];
+    const float munew = mu2.v[n] + mub.v[n];
+    float tendency = adv.v[n] * msfty.v[n];
+    tendency = tendency + qt.v[n];
+    const float q1 = rk1 ? q2.v[n] : q1o.v[n];
+    qn.v[n] = ((c1 * muold + c2) * q1 + a.dt_rk * tendency) / (c1 * munew + c2);
+  }
+  if (rk1) st4m(sc1, o, q2, i0, d.ids, d.ide - 1);
+  st4m(sc2, o, qn, i0, d.ids, d.ide - 1);
+}
+
 // calc_p_rho_phi, non-hydrostatic; logarithm and power in FP64 then rounded once to FP32
 template <typename ARGS>
 __global__ void __launch_bounds__(256) k_p_rho_phi(ARGS args) {
@@ -393,6 +425,7 @@ WRFMI_INST_P(k_adv_scalar, true)
 
 WRFMI_INST(k_adv_scalar_v)
 WRFMI_INST(k_update_scalar)
+WRFMI_INST(k_update_scalar_v)
 WRFMI_INST(k_p_rho_phi)
 template __global__ void k_halo<HaloArgs>(HaloArgs);
 template __global__ void k_halo<BatchHalo>(BatchHalo);

@@ -302,6 +302,161 @@ __global__ void __launch_bounds__(256) k_advance_uv(ARGS args) {
   }
 }
 
+// advance_uv with four i points per lane (16-byte loads): the same expressions as
+// k_advance_uv element by element.  Values at i-1 come from the group of four below (o-4),
+// values at j-1 from the row below; p and ph at level k+1 are carried to the next level.
+template <typename ARGS, bool PER>
+__global__ void __launch_bounds__(256) k_advance_uv_v(ARGS args) {
+  Tile3 T;
+  const KArgs &a = kargs(args, T);
+  const Dm &d = a.d;
+  const DState &s = dstate(a);
+  const int i0 = vec_ia0(d) + kVecW * (T.bx * 64 + (int)threadIdx.x);
+  const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
+  const int k0 = d.kds + (int)T.bz * kAdvUvK;
+  const int ids = d.ids, ide = d.ide, jds = d.jds, jde = d.jde, kde = d.kde;
+  const int its = ids, ite = ide, jts = jds, jte = jde;
+  if (i0 > ide || j > jde || k0 > kde - 1) return;
+  const int k1 = imin(k0 + kAdvUvK - 1, kde - 1);
+  const wrfmi_config &c = a.c;
+  const Geo g = geo<PER>(c);
+  const int i_start = its, i_end = imin(ite, ide - 1), j_start = jts, j_end = imin(jte, jde - 1), i_endu = ite, j_endv = jte;
+  int i_start_up = i_start, i_end_up = i_endu, j_start_vp = j_start, j_end_vp = j_endv;
+  if ((g.open_xs || g.sym_xs) && its == ids) i_start_up++;
+  if ((g.open_xe || g.sym_xe) && ite == ide) i_end_up--;
+  if ((g.open_ys || g.sym_ys || g.polar) && jts == jds) j_start_vp++;
+  if ((g.open_ye || g.sym_ye || g.polar) && jte == jde) j_end_vp--;
+  int i_start_u_tend = i_start, i_end_u_tend = i_endu, j_start_v_tend = j_start, j_end_v_tend = j_endv;
+  if (g.sym_xs && its == ids) i_start_u_tend++;
+  if (g.sym_xe && ite == ide) i_end_u_tend--;
+  if (g.sym_ys && jts == jds) j_start_v_tend++;
+  if (g.sym_ye && jte == jde) j_end_v_tend--;
+  const float rdx = s.rdx, rdy = s.rdy, dx = 1.f / rdx, dy = 1.f / rdy, dts = a.dts;
+  const float cf1 = s.cf1, cf2 = s.cf2, cf3 = s.cf3, emdiv = c.emdiv;
+  const FPtr<float> p = s.p, pb = s.pb, ph = s.ph_2, php = s.php, alt = s.alt, al = s.al, mu = s.mu_2, mudf = s.mudf;
+  const bool nh = c.non_hydrostatic;
+  const int ni = d.ni, sj = d.sj;
+  const Off o2 = X2(d, i0, j), ob = X3(d, i0, 0, j);  // level k at ob + k*ni
+  const bool u_on = j >= j_start && j <= j_end;
+  const bool v_on = j >= j_start_v_tend && j <= j_end_v_tend;
+  const bool vpg_on = j >= j_start_vp && j <= j_end_vp;
+  // column constants
+  const F4 mdf = mudf.ld<F4>(o2), mdfl = mudf.ld<F4>(o2 - 4), mdfs = mudf.ld<F4>(o2 - ni);
+  const F4 msfuy = s.msfuy.ld<F4>(o2), msfux = s.msfux.ld<F4>(o2), muu = s.muu.ld<F4>(o2);
+  const F4 muc = mu.ld<F4>(o2), mul = mu.ld<F4>(o2 - 4), mus = mu.ld<F4>(o2 - ni);
+  const F4 mvi = s.msfvx_inv.ld<F4>(o2), msfvy = s.msfvy.ld<F4>(o2), msfvx = s.msfvx.ld<F4>(o2), muv = s.muv.ld<F4>(o2);
+  float mudf_xu[4], mu_m[4], mu0u[4], mudf_yv[4], mv_m[4];
+#pragma unroll
+  for (int n = 0; n < 4; n++) {
+    mudf_xu[n] = -emdiv * dx * (mdf.v[n] - xm1(mdfl, mdf, n)) / msfuy.v[n];
+    mu_m[n] = msfux.v[n] / msfuy.v[n];
+    mu0u[n] = xm1(mul, muc, n);
+    mudf_yv[n] = -emdiv * dy * (mdf.v[n] - mdfs.v[n]) * mvi.v[n];
+    mv_m[n] = msfvy.v[n] / msfvx.v[n];
+  }
+  // dpn at the u points (from the groups c and l) and at the v points (c and s)
+  F4 dpnu_lo, dpnv_lo, dpnu_top, dpnv_top;
+  if (k0 == 1) {
+    const Off q = ob + ni;
+    const F4 c1 = p.ld<F4>(q), c2 = p.ld<F4>(q + ni), c3 = p.ld<F4>(q + 2 * ni);
+    const F4 l1 = p.ld<F4>(q - 4), l2 = p.ld<F4>(q + ni - 4), l3 = p.ld<F4>(q + 2 * ni - 4);
+    const F4 s1 = p.ld<F4>(q - sj), s2 = p.ld<F4>(q + ni - sj), s3 = p.ld<F4>(q + 2 * ni - sj);
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+      dpnu_lo.v[n] = .5f * (cf1 * (c1.v[n] + xm1(l1, c1, n)) + cf2 * (c2.v[n] + xm1(l2, c2, n)) + cf3 * (c3.v[n] + xm1(l3, c3, n)));
+      dpnv_lo.v[n] = .5f * (cf1 * (c1.v[n] + s1.v[n]) + cf2 * (c2.v[n] + s2.v[n]) + cf3 * (c3.v[n] + s3.v[n]));
+    }
+  } else {
+    const Off q = ob + k0 * ni;
+    const F4 c0 = p.ld<F4>(q), cm = p.ld<F4>(q - ni), l0 = p.ld<F4>(q - 4), lm = p.ld<F4>(q - ni - 4);
+    const F4 s0 = p.ld<F4>(q - sj), sm = p.ld<F4>(q - ni - sj);
+    const float fm = s.fnm[X1(d, k0)], fp = s.fnp[X1(d, k0)];
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+      dpnu_lo.v[n] = .5f * (fm * (c0.v[n] + xm1(l0, c0, n)) + fp * (cm.v[n] + xm1(lm, cm, n)));
+      dpnv_lo.v[n] = .5f * (fm * (c0.v[n] + s0.v[n]) + fp * (cm.v[n] + sm.v[n]));
+    }
+  }
+#pragma unroll
+  for (int n = 0; n < 4; n++) dpnu_top.v[n] = dpnv_top.v[n] = 0.f;
+  if (c.top_lid && k1 == kde - 1) {
+    const Off q = ob + (kde - 1) * ni;
+    const F4 c1 = p.ld<F4>(q), c2 = p.ld<F4>(q - ni), c3 = p.ld<F4>(q - 2 * ni);
+    const F4 l1 = p.ld<F4>(q - 4), l2 = p.ld<F4>(q - ni - 4), l3 = p.ld<F4>(q - 2 * ni - 4);
+    const F4 s1 = p.ld<F4>(q - sj), s2 = p.ld<F4>(q - ni - sj), s3 = p.ld<F4>(q - 2 * ni - sj);
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+      dpnu_top.v[n] = .5f * (cf1 * (xm1(l1, c1, n) + c1.v[n]) + cf2 * (xm1(l2, c2, n) + c2.v[n]) + cf3 * (xm1(l3, c3, n) + c3.v[n]));
+      dpnv_top.v[n] = .5f * (cf1 * (s1.v[n] + c1.v[n]) + cf2 * (s2.v[n] + c2.v[n]) + cf3 * (s3.v[n] + c3.v[n]));
+    }
+  }
+  auto lv = [&](int k) { return ob + imin(k, kde) * ni; };
+#pragma unroll 1
+  for (int kk = 0; kk < kAdvUvK; kk++) {
+    const int k = k0 + kk;
+    const int kc = imin(k, k1);
+    const Off o = lv(kc), op = lv(k + 1);
+    const float c1 = s.c1h[X1(d, kc)], c2 = s.c2h[X1(d, kc)], rdnw = s.rdnw[X1(d, kc)];
+    const int kp = imin(k + 1, kde);
+    const float fm = s.fnm[X1(d, kp)], fp = s.fnp[X1(d, kp)];
+    const bool top = k + 1 == kde;
+    const F4 pc = p.ld<F4>(o), pcp = p.ld<F4>(op), hc = ph.ld<F4>(o), hcp = ph.ld<F4>(op);
+    const F4 altc = alt.ld<F4>(o), alc = al.ld<F4>(o), pbc = pb.ld<F4>(o), phpc = php.ld<F4>(o);
+    // u: neighbours at i-1
+    {
+      const F4 pl = p.ld<F4>(o - 4), plp = p.ld<F4>(op - 4), hl = ph.ld<F4>(o - 4), hlp = ph.ld<F4>(op - 4);
+      const F4 altl = alt.ld<F4>(o - 4), all = al.ld<F4>(o - 4), pbl = pb.ld<F4>(o - 4), phpl = php.ld<F4>(o - 4);
+      const F4 u0 = s.u_2.ld<F4>(o), ut = s.ru_tend.ld<F4>(o), cqu = s.cqu.ld<F4>(o);
+      F4 un;
+#pragma unroll
+      for (int n = 0; n < 4; n++) {
+        const int i = i0 + n;
+        const bool utend_on = i >= i_start_u_tend && i <= i_end_u_tend, upg_on = i >= i_start_up && i <= i_end_up;
+        float u = u0.v[n];
+        u = utend_on ? u + dts * ut.v[n] : u;
+        float dpxy = mu_m[n] * .5f * rdx * (c1 * muu.v[n] + c2) *
+                     (((hcp.v[n] - xm1(hlp, hcp, n)) + (hc.v[n] - xm1(hl, hc, n))) +
+                      (altc.v[n] + xm1(altl, altc, n)) * (pc.v[n] - xm1(pl, pc, n)) +
+                      (alc.v[n] + xm1(all, alc, n)) * (pbc.v[n] - xm1(pbl, pbc, n)));
+        const float g_hi = .5f * (fm * (pcp.v[n] + xm1(plp, pcp, n)) + fp * (pc.v[n] + xm1(pl, pc, n)));
+        const float dpn_hi = top ? dpnu_top.v[n] : g_hi;
+        const float dpxn = dpxy + mu_m[n] * rdx * (phpc.v[n] - xm1(phpl, phpc, n)) *
+                                      (rdnw * (dpn_hi - dpnu_lo.v[n]) - .5f * ((c1 * mu0u[n]) + (c1 * muc.v[n])));
+        dpxy = nh ? dpxn : dpxy;
+        dpnu_lo.v[n] = dpn_hi;
+        const float upg = u - dts * cqu.v[n] * dpxy + (c1 * mudf_xu[n]);
+        un.v[n] = upg_on ? upg : u;
+      }
+      if (u_on && k <= k1) st4m(s.u_2, o, un, i0, ids, ide);
+    }
+    // v: neighbours at j-1
+    {
+      const F4 ps = p.ld<F4>(o - sj), psp = p.ld<F4>(op - sj), hs = ph.ld<F4>(o - sj), hsp = ph.ld<F4>(op - sj);
+      const F4 alts = alt.ld<F4>(o - sj), als = al.ld<F4>(o - sj), pbs = pb.ld<F4>(o - sj), phps = php.ld<F4>(o - sj);
+      const F4 v0 = s.v_2.ld<F4>(o), vt = s.rv_tend.ld<F4>(o), cqv = s.cqv.ld<F4>(o);
+      F4 vn;
+#pragma unroll
+      for (int n = 0; n < 4; n++) {
+        float v = v0.v[n];
+        v = v + dts * vt.v[n];
+        float dpxy = mv_m[n] * .5f * rdy * (c1 * muv.v[n] + c2) *
+                     (((hcp.v[n] - hsp.v[n]) + (hc.v[n] - hs.v[n])) +
+                      (altc.v[n] + alts.v[n]) * (pc.v[n] - ps.v[n]) +
+                      (alc.v[n] + als.v[n]) * (pbc.v[n] - pbs.v[n]));
+        const float g_hi = .5f * (fm * (pcp.v[n] + psp.v[n]) + fp * (pc.v[n] + ps.v[n]));
+        const float dpn_hi = top ? dpnv_top.v[n] : g_hi;
+        const float dpyn = dpxy + mv_m[n] * rdy * (phpc.v[n] - phps.v[n]) *
+                                      (rdnw * (dpn_hi - dpnv_lo.v[n]) - .5f * ((c1 * mus.v[n]) + (c1 * muc.v[n])));
+        dpxy = nh ? dpyn : dpxy;
+        dpnv_lo.v[n] = dpn_hi;
+        const float vpg = v - dts * cqv.v[n] * dpxy + (c1 * mudf_yv[n]);
+        vn.v[n] = vpg_on ? vpg : v;
+      }
+      if (v_on && k <= k1) st4m(s.v_2, o, vn, i0, ids, i_end);
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------------------
 // The column part of one acoustic substep: advance_mu_t + advance_w + sumflux + calc_p_rho,
 // split at the two column dependencies (the mass-divergence integral before the omega scan,
@@ -535,6 +690,140 @@ __global__ void __launch_bounds__(256) k_ac_wpred(ARGS args) {
     if (c.top_lid) wv = 0.f;
   }
   w[o] = wv;
+}
+
+// ---------------------------------------------------------------------------------------
+// Four-points-per-lane forms of small_step_prep and calc_p_rho step 0 (16-byte accesses;
+// element by element the expressions of k_small_prep and k_p_rho0).
+struct MuS4 {
+  F4 muts, muus, muvs, mu_new;
+};
+DEVI MuS4 small_mu4(const KArgs &a, int i0, int j) {
+  const Dm &d = a.d;
+  const DState &s = dstate(a);
+  const Off o2 = X2(d, i0, j);
+  const F4 mub = s.mub.ld<F4>(o2), mu1 = s.mu_1.ld<F4>(o2), mu2 = s.mu_2.ld<F4>(o2);
+  const F4 mubl = s.mub.ld<F4>(o2 - 4), mu1l = s.mu_1.ld<F4>(o2 - 4), mubs = s.mub.ld<F4>(o2 - d.ni), mu1s = s.mu_1.ld<F4>(o2 - d.ni);
+  const F4 muu = s.muu.ld<F4>(o2), muv = s.muv.ld<F4>(o2);
+  const bool rk1 = a.rk_step == 1;
+  MuS4 m;
+#pragma unroll
+  for (int n = 0; n < 4; n++) {
+    m.muts.v[n] = rk1 ? mub.v[n] + mu2.v[n] : mub.v[n] + mu1.v[n];
+    m.muus.v[n] = rk1 ? muu.v[n] : 0.5f * (((mub.v[n] + mu1.v[n]) + xm1(mubl, mub, n)) + xm1(mu1l, mu1, n));
+    m.muvs.v[n] = rk1 ? muv.v[n] : 0.5f * (((mub.v[n] + mu1.v[n]) + mubs.v[n]) + mu1s.v[n]);
+    m.mu_new.v[n] = rk1 ? 0.f : mu1.v[n] - mu2.v[n];
+  }
+  return m;
+}
+
+template <typename ARGS>
+__global__ void __launch_bounds__(256) k_small_prep_v(ARGS args) {
+  Tile3 T;
+  const KArgs &a = kargs(args, T);
+  const Dm &d = a.d;
+  const DState &s = dstate(a);
+  const int i0 = vec_ia0(d) + kVecW * (T.bx * 64 + (int)threadIdx.x);
+  const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
+  const int k = d.kds + (int)T.bz;
+  const int ids = d.ids, ide = d.ide, jde = d.jde, kde = d.kde, k_end = kde - 1;
+  if (i0 > ide || j > jde || k > kde) return;
+  const bool mj = j <= jde - 1, kin = k <= k_end;
+  const int kc = imin(k, k_end);
+  const float c1 = s.c1h[X1(d, kc)], c2 = s.c2h[X1(d, kc)], c1f = s.c1f[X1(d, k)], c2f = s.c2f[X1(d, k)];
+  const Off o2 = X2(d, i0, j), o = X3(d, i0, k, j);
+  const bool rk1 = a.rk_step == 1;
+  const MuS4 m = small_mu4(a, i0, j);
+  if (mj && kin) {
+    const F4 u2 = s.u_2.ld<F4>(o), u1o = s.u_1.ld<F4>(o), muu = s.muu.ld<F4>(o2), msfuy = s.msfuy.ld<F4>(o2);
+    F4 un;
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+      const float u1 = rk1 ? u2.v[n] : u1o.v[n];
+      un.v[n] = ((c1 * m.muus.v[n] + c2) * u1 - (c1 * muu.v[n] + c2) * u2.v[n]) / msfuy.v[n];
+    }
+    if (rk1) st4m(s.u_1, o, u2, i0, ids, ide);
+    st4m(s.u_save, o, u2, i0, ids, ide);
+    st4m(s.u_2, o, un, i0, ids, ide);
+  }
+  if (kin) {
+    const F4 v2 = s.v_2.ld<F4>(o), v1o = s.v_1.ld<F4>(o), muv = s.muv.ld<F4>(o2), mvi = s.msfvx_inv.ld<F4>(o2);
+    F4 vn;
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+      const float v1 = rk1 ? v2.v[n] : v1o.v[n];
+      vn.v[n] = ((c1 * m.muvs.v[n] + c2) * v1 - (c1 * muv.v[n] + c2) * v2.v[n]) * mvi.v[n];
+    }
+    if (rk1) st4m(s.v_1, o, v2, i0, ids, ide - 1);
+    st4m(s.v_save, o, v2, i0, ids, ide - 1);
+    st4m(s.v_2, o, vn, i0, ids, ide - 1);
+  }
+  if (!mj || i0 > ide - 1) return;
+  const F4 mut = s.mut.ld<F4>(o2), msfty = s.msfty.ld<F4>(o2);
+  if (kin) {
+    const F4 pb = s.pb.ld<F4>(o), pp = s.p.ld<F4>(o), alt = s.alt.ld<F4>(o);
+    const F4 t2 = s.t_2.ld<F4>(o), t1o = s.t_1.ld<F4>(o);
+    F4 c2a, tn;
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+      c2a.v[n] = K_CPOVCV * (pb.v[n] + pp.v[n]) / alt.v[n];
+      const float t1 = rk1 ? t2.v[n] : t1o.v[n];
+      tn.v[n] = (c1 * m.muts.v[n] + c2) * t1 - (c1 * mut.v[n] + c2) * t2.v[n];
+    }
+    st4m(s.c2a, o, c2a, i0, ids, ide - 1);
+    if (rk1) st4m(s.t_1, o, t2, i0, ids, ide - 1);
+    st4m(s.t_save, o, t2, i0, ids, ide - 1);
+    st4m(s.t_2, o, tn, i0, ids, ide - 1);
+  }
+  {
+    const F4 w2 = s.w_2.ld<F4>(o), ph2 = s.ph_2.ld<F4>(o), w1o = s.w_1.ld<F4>(o), ph1o = s.ph_1.ld<F4>(o), ww = s.ww.ld<F4>(o);
+    F4 wn, phn;
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+      const float w1 = rk1 ? w2.v[n] : w1o.v[n];
+      const float ph1 = rk1 ? ph2.v[n] : ph1o.v[n];
+      wn.v[n] = ((c1f * m.muts.v[n] + c2f) * w1 - (c1f * mut.v[n] + c2f) * w2.v[n]) / msfty.v[n];
+      phn.v[n] = ph1 - ph2.v[n];
+    }
+    if (rk1) {
+      st4m(s.w_1, o, w2, i0, ids, ide - 1);
+      st4m(s.ph_1, o, ph2, i0, ids, ide - 1);
+    }
+    st4m(s.w_save, o, w2, i0, ids, ide - 1);
+    st4m(s.w_2, o, wn, i0, ids, ide - 1);
+    st4m(s.ph_save, o, ph2, i0, ids, ide - 1);
+    st4m(s.ph_2, o, phn, i0, ids, ide - 1);
+    st4m(s.ww1, o, ww, i0, ids, ide - 1);
+  }
+}
+
+template <typename ARGS>
+__global__ void __launch_bounds__(256) k_p_rho0_v(ARGS args) {
+  Tile3 T;
+  const KArgs &a = kargs(args, T);
+  const Dm &d = a.d;
+  const DState &s = dstate(a);
+  const int i0 = vec_ia0(d) + kVecW * (T.bx * 64 + (int)threadIdx.x);
+  const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
+  const int k = d.kds + (int)T.bz;
+  if (i0 > d.ide - 1 || j > d.jde - 1 || k > d.kde - 1) return;
+  if (!a.c.non_hydrostatic) return;
+  const Off o = X3(d, i0, k, j);
+  const MuS4 m = small_mu4(a, i0, j);
+  const float c1 = s.c1h[X1(d, k)], c2 = s.c2h[X1(d, k)], rdnw = s.rdnw[X1(d, k)];
+  const F4 alt = s.alt.ld<F4>(o), php = s.ph_2.ld<F4>(o + d.ni), ph = s.ph_2.ld<F4>(o), t1 = s.t_save.ld<F4>(o);
+  const F4 t2 = s.t_2.ld<F4>(o), c2a = s.c2a.ld<F4>(o);
+  F4 aln, pn;
+#pragma unroll
+  for (int n = 0; n < 4; n++) {
+    const float mu = m.mu_new.v[n], muts = m.muts.v[n];
+    const float al = -1.f / (c1 * muts + c2) * (alt.v[n] * (c1 * mu) + rdnw * (php.v[n] - ph.v[n]));
+    aln.v[n] = al;
+    pn.v[n] = c2a.v[n] * (alt.v[n] * (t2.v[n] - (c1 * mu) * t1.v[n]) / ((c1 * muts + c2) * (K_T0 + t1.v[n])) - al);
+  }
+  st4m(s.al, o, aln, i0, d.ids, d.ide - 1);
+  st4m(s.p, o, pn, i0, d.ids, d.ide - 1);
+  st4m(s.pm1, o, pn, i0, d.ids, d.ide - 1);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -936,6 +1225,8 @@ __global__ void __launch_bounds__(256) k_small_finish(ARGS args) {
 
 WRFMI_INST_P(k_advance_uv, false)
 WRFMI_INST_P(k_advance_uv, true)
+WRFMI_INST_P(k_advance_uv_v, false)
+WRFMI_INST_P(k_advance_uv_v, true)
 WRFMI_INST_P(k_coef_w, float)
 WRFMI_INST_P(k_coef_w, double)
 WRFMI_INST_P(k_ac_mu_ww, false)
@@ -949,6 +1240,8 @@ WRFMI_INST(k_ac_div)
 WRFMI_INST(k_ac_theta)
 WRFMI_INST(k_ac_wpred)
 WRFMI_INST(k_ac_div_v)
+WRFMI_INST(k_small_prep_v)
+WRFMI_INST(k_p_rho0_v)
 WRFMI_INST(k_ac_theta_v)
 WRFMI_INST(k_ac_wpred_v)
 WRFMI_INST(k_small_finish)

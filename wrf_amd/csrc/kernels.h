@@ -23,11 +23,14 @@ template <typename A> __global__ void k_small_prep(A);
 template <typename A> __global__ void k_p_rho0(A);
 template <typename A, typename CT> __global__ void k_coef_w(A);
 template <typename A, bool PER> __global__ void k_advance_uv(A);
+template <typename A, bool PER> __global__ void k_advance_uv_v(A);
 template <typename A> __global__ void k_ac_div(A);
 template <typename A, bool COMP> __global__ void k_ac_mu_ww(A);
 template <typename A> __global__ void k_ac_theta(A);
 template <typename A> __global__ void k_ac_wpred(A);
 template <typename A> __global__ void k_ac_div_v(A);
+template <typename A> __global__ void k_small_prep_v(A);
+template <typename A> __global__ void k_p_rho0_v(A);
 template <typename A> __global__ void k_ac_theta_v(A);
 template <typename A> __global__ void k_ac_wpred_v(A);
 template <typename A, typename CT> __global__ void k_ac_wsolve(A);
@@ -35,6 +38,7 @@ template <typename A> __global__ void k_small_finish(A);
 template <typename A, bool PER> __global__ void k_adv_scalar(A);
 template <typename A> __global__ void k_adv_scalar_v(A);
 template <typename A> __global__ void k_update_scalar(A);
+template <typename A> __global__ void k_update_scalar_v(A);
 template <typename A> __global__ void k_p_rho_phi(A);
 template <typename A> __global__ void k_halo(A);
 __global__ void k_count_nonfinite(const float *a, Dm d, int kend, int is3d, unsigned long long *cnt);

@@ -398,8 +398,13 @@ static void ph_small_prep(Launcher &L) {
   const KArgs &a = L.a;
   const wrfmi_state &s = a.s;
   const int nk = a.d.kde - a.d.kds + 1;
-  L.launchK(KID_SMALL_PREP, KR(k_small_prep), g_pt3(a.d, nk), kColBlk, a);
-  L.launchK(KID_P_RHO0, KR(k_p_rho0), g_pt3(a.d, nk - 1), kColBlk, a);
+  if (L.vec) {
+    L.launchK(KID_SMALL_PREP, KR(k_small_prep_v), g_vec3(a.d, nk), kColBlk, a);
+    L.launchK(KID_P_RHO0, KR(k_p_rho0_v), g_vec3(a.d, nk - 1), kColBlk, a);
+  } else {
+    L.launchK(KID_SMALL_PREP, KR(k_small_prep), g_pt3(a.d, nk), kColBlk, a);
+    L.launchK(KID_P_RHO0, KR(k_p_rho0), g_pt3(a.d, nk - 1), kColBlk, a);
+  }
   if (a.c.precision_mode == 1) L.launchK(KID_COEF_W, KRP(k_coef_w, double), g_col(a.d), kColBlk, a);
   else L.launchK(KID_COEF_W, KRP(k_coef_w, float), g_col(a.d), kColBlk, a);
   L.halo_begin();
@@ -410,8 +415,11 @@ static void ph_small_prep(Launcher &L) {
 }
 static void ph_advance_uv(Launcher &L) {
   const bool per = L.a.c.periodic_x && L.a.c.periodic_y;
-  L.launchK(KID_ADVANCE_UV, per ? KRP(k_advance_uv, true) : KRP(k_advance_uv, false),
-            g_pt3(L.a.d, (L.a.d.kde - L.a.d.kds + kAdvUvK - 1) / kAdvUvK), kColBlk, L.a);
+  const int nkb = (L.a.d.kde - L.a.d.kds + kAdvUvK - 1) / kAdvUvK;
+  if (L.vec)
+    L.launchK(KID_ADVANCE_UV, per ? KRP(k_advance_uv_v, true) : KRP(k_advance_uv_v, false), g_vec3(L.a.d, nkb), kColBlk, L.a);
+  else
+    L.launchK(KID_ADVANCE_UV, per ? KRP(k_advance_uv, true) : KRP(k_advance_uv, false), g_pt3(L.a.d, nkb), kColBlk, L.a);
 }
 static void ph_advance_mu_t_w(Launcher &L) {
   const KArgs &a = L.a;
@@ -460,7 +468,8 @@ static void ph_scalar(Launcher &L) {
   } else {
     L.launchK(KID_ADV_SCALAR, per ? KRP(k_adv_scalar, true) : KRP(k_adv_scalar, false), ga, dim3(64, kAdvRows, 1), L.a);
   }
-  L.launchK(KID_UPDATE_SCALAR, KR(k_update_scalar), g_pt3(L.a.d, L.a.d.kde - L.a.d.kds), kColBlk, L.a);
+  if (L.vec) L.launchK(KID_UPDATE_SCALAR, KR(k_update_scalar_v), g_vec3(L.a.d, L.a.d.kde - L.a.d.kds), kColBlk, L.a);
+  else L.launchK(KID_UPDATE_SCALAR, KR(k_update_scalar), g_pt3(L.a.d, L.a.d.kde - L.a.d.kds), kColBlk, L.a);
 }
 static void ph_p_rho_phi(Launcher &L) {
   L.launchK(KID_P_RHO_PHI, KR(k_p_rho_phi), g_pt3(L.a.d, L.a.d.kde - L.a.d.kds), kColBlk, L.a);
