@@ -152,6 +152,7 @@ def main():
     ap.add_argument("--ensemble-members", type=int, default=64,
                     help="members of the uneven-ensemble leg (0 skips it)")
     ap.add_argument("--ensemble-steps", type=int, default=10)
+    ap.add_argument("--ensemble-only", action="store_true", help="run only the ensemble leg (profiling)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after the timed steps, copy a buffer of this many MiB once (calibrates HBM counters)")
     args = ap.parse_args()
@@ -166,6 +167,9 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
+    if args.ensemble_only:
+        print(json.dumps({"ensemble": ensemble_leg(runtime, grid, torch, args.ensemble_members, args.ensemble_steps, log)}))
+        return
     spec = grid.large_c1(nx=args.nx, ny=args.ny, nz=args.nz)
     log(f"building the {spec.nx}x{spec.ny}x{spec.nz} initial state ...")
     t0 = time.perf_counter()
@@ -204,6 +208,13 @@ def main():
         sys.exit(1)
     log(f"  {ms_per_step:.3f} ms/step, {value:.1f} Mpoint-steps/s")
 
+    # measured HBM bytes per launch of the dominant kernel (rocprofv3 FETCH_SIZE + WRITE_SIZE,
+    # calibrated against a device copy) for this workload, from the committed profile
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "hbm_traffic_400x400x80.json")
+    if (spec.nx, spec.ny, spec.nz) == (400, 400, 80) and os.path.exists(tfile):
+        with open(tfile) as fh:
+            traffic = json.load(fh)["kernels"].get(dominant, {}).get("bytes_per_launch")
     n_launch, tot_ms = ktimes[dominant]
     avg_ms = tot_ms / max(n_launch, 1)
     per_launch_bytes = KERNEL_FIELDS[dominant] * 4 * pts
@@ -232,7 +243,7 @@ def main():
         "alg_bytes_per_point_step": balg,
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "avg_launch_ms": round(avg_ms, 4), "launches": n_launch,
+                     "traffic": traffic, "avg_launch_ms": round(avg_ms, 4), "launches": n_launch,
                      "alg_bytes_per_launch": per_launch_bytes},
         "kernel_ms_per_step": {k: round(v[1], 3) for k, v in sorted(per_kernel.items(), key=lambda kv: -kv[1][1])},
     }

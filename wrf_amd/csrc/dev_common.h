@@ -132,11 +132,14 @@ struct HaloArgs {
 // Ensemble launches: one launch covers the workgroups of every member.  m[] holds each
 // member's arguments, off[] the prefix sums of the members' workgroup counts (n + 1
 // entries) and lut[] the member of every logical workgroup (total entries).
+// The tables are read through the constant address space: the kernels store to the state
+// arrays, and a table in ordinary global memory would have to be read again after every
+// store (it might alias them); constant memory cannot, so its values stay in registers.
 template <typename T>
 struct Batch {
-  const T *m;
-  const unsigned *off;
-  const unsigned *lut;
+  const WRFMI_CONST_AS T *m;
+  const WRFMI_CONST_AS unsigned *off;
+  const WRFMI_CONST_AS unsigned *lut;
   int n;
   unsigned total;
 };
@@ -192,14 +195,14 @@ DEVI const HaloArgs &hargs(const HaloArgs &h, unsigned &blk) {
 DEVI const HaloArgs &hargs(const BatchHalo &b, unsigned &blk) {
   const unsigned m = b.lut[blockIdx.x];
   blk = blockIdx.x - b.off[m];
-  return b.m[m];
+  return *(const HaloArgs *)(b.m + m);
 }
 // A member of an ensemble launch: the XCD-aware order runs over the concatenated tiles of
 // all members, so each XCD takes a contiguous run of members.
 DEVI const KArgs &kargs(const BatchArgs &b, Tile3 &T) {
   const unsigned L = xcd_logical(blockIdx.x, b.total);
   const unsigned m = b.lut[L];
-  const KArgs &a = b.m[m];
+  const KArgs &a = *(const KArgs *)(b.m + m);
   T = tile_of_logical(L - b.off[m], a.tnx, a.tnz);
   return a;
 }

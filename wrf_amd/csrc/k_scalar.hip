@@ -139,7 +139,7 @@ __global__ void __launch_bounds__(256) k_adv_scalar(ARGS args) {
 // offset is a multiple of four floats; the host checks the alignment and the padding the
 // windows need.
 template <typename ARGS>
-__global__ void __launch_bounds__(256) k_adv_scalar_v(ARGS args) {
+__global__ void __launch_bounds__(256) k_adv_scalar_x4(ARGS args) {
   Tile3 T;
   const KArgs &a = kargs(args, T);
   const Dm &d = a.d;
@@ -262,7 +262,7 @@ __global__ void __launch_bounds__(256) k_update_scalar(ARGS args) {
 
 // rk_update_scalar with four i points per lane (16-byte loads and stores)
 template <typename ARGS>
-__global__ void __launch_bounds__(256) k_update_scalar_v(ARGS args) {
+__global__ void __launch_bounds__(256) k_update_scalar_x4(ARGS args) {
   Tile3 T;
   const KArgs &a = kargs(args, T);
   const Dm &d = a.d;
@@ -339,7 +339,7 @@ __global__ void __launch_bounds__(256) k_p_rho_phi(ARGS args) {
 // Value a filled halo cell ends up with after set_physical_bc's x pass followed by its
 // y pass (periodic wrap, or the open-boundary zero-gradient copy), computed directly
 // from the source cell so that all cells of all fields fill in one launch.
-// One thread per (halo cell, level, field).  The halo ring of the memory patch is
+// One thread per (halo cell, group of kHaloK levels, field).  The halo ring of the memory patch is
 // enumerated as: the rows below jds and from jde up (every i), then the columns left of ids
 // and from ide on for the rows jds..jde-1.
 template <typename ARGS>
@@ -349,11 +349,11 @@ __global__ void __launch_bounds__(256) k_halo(ARGS args) {
   const Dm &d = h.d;
   const unsigned W = (unsigned)d.ni, B = (unsigned)(d.jds - d.jms), Tn = (unsigned)(d.jme - d.jde + 1);
   const unsigned Lw = (unsigned)(d.ids - d.ims), LR = Lw + (unsigned)(d.ime - d.ide + 1);
-  const unsigned nring = (B + Tn) * W + (unsigned)(d.jde - d.jds) * LR, nk = (unsigned)d.kde;
+  const unsigned nring = (B + Tn) * W + (unsigned)(d.jde - d.jds) * LR, nkc = (unsigned)(d.kde + kHaloK - 1) / kHaloK;
   const unsigned t = blk * blockDim.x + threadIdx.x;
-  if (t >= nring * nk * (unsigned)h.n) return;
-  const unsigned fidx = t / (nring * nk), rem = t % (nring * nk);
-  const int k = d.kds + (int)(rem / nring);
+  if (t >= nring * nkc * (unsigned)h.n) return;
+  const unsigned fidx = t / (nring * nkc), rem = t % (nring * nkc);
+  const int k0 = d.kds + (int)(rem / nring) * kHaloK;
   unsigned c = rem % nring;
   int i, j;
   if (c < B * W) {
@@ -415,17 +415,25 @@ __global__ void __launch_bounds__(256) k_halo(ARGS args) {
   } else {
     return;
   }
-  if (k > kend) return;
-  if (F.is3d) F.p[X3i(d, i, k, j)] = F.p[X3i(d, si, k, sj)];
-  else F.p[X2i(d, i, j)] = F.p[X2i(d, si, sj)];
+  if (!F.is3d) {
+    if (k0 == d.kds) F.p[X2i(d, i, j)] = F.p[X2i(d, si, sj)];
+    return;
+  }
+  // kHaloK levels per thread: all loads first (levels past kend repeat kend), then the stores
+  float v[kHaloK];
+#pragma unroll
+  for (int kk = 0; kk < kHaloK; kk++) v[kk] = F.p[X3i(d, si, imin(k0 + kk, kend), sj)];
+#pragma unroll
+  for (int kk = 0; kk < kHaloK; kk++)
+    if (k0 + kk <= kend) F.p[X3i(d, i, k0 + kk, j)] = v[kk];
 }
 
 WRFMI_INST_P(k_adv_scalar, false)
 WRFMI_INST_P(k_adv_scalar, true)
 
-WRFMI_INST(k_adv_scalar_v)
+WRFMI_INST(k_adv_scalar_x4)
 WRFMI_INST(k_update_scalar)
-WRFMI_INST(k_update_scalar_v)
+WRFMI_INST(k_update_scalar_x4)
 WRFMI_INST(k_p_rho_phi)
 template __global__ void k_halo<HaloArgs>(HaloArgs);
 template __global__ void k_halo<BatchHalo>(BatchHalo);

@@ -306,7 +306,7 @@ __global__ void __launch_bounds__(256) k_advance_uv(ARGS args) {
 // k_advance_uv element by element.  Values at i-1 come from the group of four below (o-4),
 // values at j-1 from the row below; p and ph at level k+1 are carried to the next level.
 template <typename ARGS, bool PER>
-__global__ void __launch_bounds__(256) k_advance_uv_v(ARGS args) {
+__global__ void __launch_bounds__(256) k_advance_uv_x4(ARGS args) {
   Tile3 T;
   const KArgs &a = kargs(args, T);
   const Dm &d = a.d;
@@ -718,7 +718,7 @@ DEVI MuS4 small_mu4(const KArgs &a, int i0, int j) {
 }
 
 template <typename ARGS>
-__global__ void __launch_bounds__(256) k_small_prep_v(ARGS args) {
+__global__ void __launch_bounds__(256) k_small_prep_x4(ARGS args) {
   Tile3 T;
   const KArgs &a = kargs(args, T);
   const Dm &d = a.d;
@@ -798,7 +798,7 @@ __global__ void __launch_bounds__(256) k_small_prep_v(ARGS args) {
 }
 
 template <typename ARGS>
-__global__ void __launch_bounds__(256) k_p_rho0_v(ARGS args) {
+__global__ void __launch_bounds__(256) k_p_rho0_x4(ARGS args) {
   Tile3 T;
   const KArgs &a = kargs(args, T);
   const Dm &d = a.d;
@@ -833,7 +833,7 @@ __global__ void __launch_bounds__(256) k_p_rho0_v(ARGS args) {
 // from the groups of four on either side.
 
 template <typename ARGS>
-__global__ void __launch_bounds__(256) k_ac_div_v(ARGS args) {
+__global__ void __launch_bounds__(256) k_ac_div_x4(ARGS args) {
   Tile3 T;
   const KArgs &a = kargs(args, T);
   const Dm &d = a.d;
@@ -894,7 +894,7 @@ __global__ void __launch_bounds__(256) k_ac_div_v(ARGS args) {
 }
 
 template <typename ARGS>
-__global__ void __launch_bounds__(256) k_ac_theta_v(ARGS args) {
+__global__ void __launch_bounds__(256) k_ac_theta_x4(ARGS args) {
   Tile3 T;
   const KArgs &a = kargs(args, T);
   const Dm &d = a.d;
@@ -983,7 +983,7 @@ __global__ void __launch_bounds__(256) k_ac_theta_v(ARGS args) {
 }
 
 template <typename ARGS>
-__global__ void __launch_bounds__(256) k_ac_wpred_v(ARGS args) {
+__global__ void __launch_bounds__(256) k_ac_wpred_x4(ARGS args) {
   Tile3 T;
   const KArgs &a = kargs(args, T);
   const Dm &d = a.d;
@@ -1225,8 +1225,8 @@ __global__ void __launch_bounds__(256) k_small_finish(ARGS args) {
 
 WRFMI_INST_P(k_advance_uv, false)
 WRFMI_INST_P(k_advance_uv, true)
-WRFMI_INST_P(k_advance_uv_v, false)
-WRFMI_INST_P(k_advance_uv_v, true)
+WRFMI_INST_P(k_advance_uv_x4, false)
+WRFMI_INST_P(k_advance_uv_x4, true)
 WRFMI_INST_P(k_coef_w, float)
 WRFMI_INST_P(k_coef_w, double)
 WRFMI_INST_P(k_ac_mu_ww, false)
@@ -1239,11 +1239,11 @@ WRFMI_INST(k_p_rho0)
 WRFMI_INST(k_ac_div)
 WRFMI_INST(k_ac_theta)
 WRFMI_INST(k_ac_wpred)
-WRFMI_INST(k_ac_div_v)
-WRFMI_INST(k_small_prep_v)
-WRFMI_INST(k_p_rho0_v)
-WRFMI_INST(k_ac_theta_v)
-WRFMI_INST(k_ac_wpred_v)
+WRFMI_INST(k_ac_div_x4)
+WRFMI_INST(k_small_prep_x4)
+WRFMI_INST(k_p_rho0_x4)
+WRFMI_INST(k_ac_theta_x4)
+WRFMI_INST(k_ac_wpred_x4)
 WRFMI_INST(k_small_finish)
 
 }  // namespace wrfmi

@@ -10,7 +10,8 @@ constexpr int kAdvTileI = 63;  // i points per wave in k_adv_scalar (lane 63 sup
 constexpr int kAdvRows = 4;    // j rows (waves) per block in k_adv_scalar
 constexpr int kVecW = 4;       // i points per lane in the vector kernels (16-byte accesses)
 constexpr int kVecTileI = 64 * kVecW;  // i points per wave in the vector kernels
-constexpr int kAdvVK = 4;      // levels per thread in k_adv_scalar_v
+constexpr int kAdvVK = 4;      // levels per thread in k_adv_scalar_x4
+constexpr int kHaloK = 8;      // levels per thread in k_halo
 // Every state kernel is a template on its argument type: KArgs for one domain, BatchArgs
 // for an ensemble launch over the tiles of all members (k_halo: HaloArgs / BatchHalo).
 template <typename A> __global__ void k_step_prep(A);
@@ -23,22 +24,22 @@ template <typename A> __global__ void k_small_prep(A);
 template <typename A> __global__ void k_p_rho0(A);
 template <typename A, typename CT> __global__ void k_coef_w(A);
 template <typename A, bool PER> __global__ void k_advance_uv(A);
-template <typename A, bool PER> __global__ void k_advance_uv_v(A);
+template <typename A, bool PER> __global__ void k_advance_uv_x4(A);
 template <typename A> __global__ void k_ac_div(A);
 template <typename A, bool COMP> __global__ void k_ac_mu_ww(A);
 template <typename A> __global__ void k_ac_theta(A);
 template <typename A> __global__ void k_ac_wpred(A);
-template <typename A> __global__ void k_ac_div_v(A);
-template <typename A> __global__ void k_small_prep_v(A);
-template <typename A> __global__ void k_p_rho0_v(A);
-template <typename A> __global__ void k_ac_theta_v(A);
-template <typename A> __global__ void k_ac_wpred_v(A);
+template <typename A> __global__ void k_ac_div_x4(A);
+template <typename A> __global__ void k_small_prep_x4(A);
+template <typename A> __global__ void k_p_rho0_x4(A);
+template <typename A> __global__ void k_ac_theta_x4(A);
+template <typename A> __global__ void k_ac_wpred_x4(A);
 template <typename A, typename CT> __global__ void k_ac_wsolve(A);
 template <typename A> __global__ void k_small_finish(A);
 template <typename A, bool PER> __global__ void k_adv_scalar(A);
-template <typename A> __global__ void k_adv_scalar_v(A);
+template <typename A> __global__ void k_adv_scalar_x4(A);
 template <typename A> __global__ void k_update_scalar(A);
-template <typename A> __global__ void k_update_scalar_v(A);
+template <typename A> __global__ void k_update_scalar_x4(A);
 template <typename A> __global__ void k_p_rho_phi(A);
 template <typename A> __global__ void k_halo(A);
 __global__ void k_count_nonfinite(const float *a, Dm d, int kend, int is3d, unsigned long long *cnt);

@@ -334,7 +334,7 @@ struct Launcher {
     const Dm &d = h.d;
     const long ring = (long)(d.jds - d.jms + d.jme - d.jde + 1) * d.ni +
                       (long)(d.jde - d.jds) * (d.ids - d.ims + d.ime - d.ide + 1);
-    const long nthreads = ring * d.kde * h.n;
+    const long nthreads = ring * ((d.kde + kHaloK - 1) / kHaloK) * h.n;
     const unsigned wgs = (unsigned)((nthreads + 255) / 256);
     if (rec) {
       rec->ops.push_back(RecOp{KID_HALO, true, (void *)k_halo<BatchHalo>, dim3(256, 1, 1), wgs, rec->h.size()});
@@ -399,8 +399,8 @@ static void ph_small_prep(Launcher &L) {
   const wrfmi_state &s = a.s;
   const int nk = a.d.kde - a.d.kds + 1;
   if (L.vec) {
-    L.launchK(KID_SMALL_PREP, KR(k_small_prep_v), g_vec3(a.d, nk), kColBlk, a);
-    L.launchK(KID_P_RHO0, KR(k_p_rho0_v), g_vec3(a.d, nk - 1), kColBlk, a);
+    L.launchK(KID_SMALL_PREP, KR(k_small_prep_x4), g_vec3(a.d, nk), kColBlk, a);
+    L.launchK(KID_P_RHO0, KR(k_p_rho0_x4), g_vec3(a.d, nk - 1), kColBlk, a);
   } else {
     L.launchK(KID_SMALL_PREP, KR(k_small_prep), g_pt3(a.d, nk), kColBlk, a);
     L.launchK(KID_P_RHO0, KR(k_p_rho0), g_pt3(a.d, nk - 1), kColBlk, a);
@@ -417,7 +417,7 @@ static void ph_advance_uv(Launcher &L) {
   const bool per = L.a.c.periodic_x && L.a.c.periodic_y;
   const int nkb = (L.a.d.kde - L.a.d.kds + kAdvUvK - 1) / kAdvUvK;
   if (L.vec)
-    L.launchK(KID_ADVANCE_UV, per ? KRP(k_advance_uv_v, true) : KRP(k_advance_uv_v, false), g_vec3(L.a.d, nkb), kColBlk, L.a);
+    L.launchK(KID_ADVANCE_UV, per ? KRP(k_advance_uv_x4, true) : KRP(k_advance_uv_x4, false), g_vec3(L.a.d, nkb), kColBlk, L.a);
   else
     L.launchK(KID_ADVANCE_UV, per ? KRP(k_advance_uv, true) : KRP(k_advance_uv, false), g_pt3(L.a.d, nkb), kColBlk, L.a);
 }
@@ -426,13 +426,13 @@ static void ph_advance_mu_t_w(Launcher &L) {
   const wrfmi_state &s = a.s;
   const bool p64 = a.c.precision_mode == 1;
   const int nk = a.d.kde - a.d.kds + 1;
-  if (L.vec) L.launchK(KID_AC_DIV, KR(k_ac_div_v), g_vec3(a.d, nk), kColBlk, a);
+  if (L.vec) L.launchK(KID_AC_DIV, KR(k_ac_div_x4), g_vec3(a.d, nk), kColBlk, a);
   else L.launchK(KID_AC_DIV, KR(k_ac_div), g_pt3(a.d, nk), kColBlk, a);
   if (p64) L.launchK(KID_AC_MU_WW, KRP(k_ac_mu_ww, true), g_col(a.d), kColBlk, a);
   else L.launchK(KID_AC_MU_WW, KRP(k_ac_mu_ww, false), g_col(a.d), kColBlk, a);
   if (L.vec) {
-    L.launchK(KID_AC_THETA, KR(k_ac_theta_v), g_vec3(a.d, nk), kColBlk, a);
-    L.launchK(KID_AC_WPRED, KR(k_ac_wpred_v), g_vec3(a.d, nk), kColBlk, a);
+    L.launchK(KID_AC_THETA, KR(k_ac_theta_x4), g_vec3(a.d, nk), kColBlk, a);
+    L.launchK(KID_AC_WPRED, KR(k_ac_wpred_x4), g_vec3(a.d, nk), kColBlk, a);
   } else {
     L.launchK(KID_AC_THETA, KR(k_ac_theta), g_pt3(a.d, nk), kColBlk, a);
     L.launchK(KID_AC_WPRED, KR(k_ac_wpred), g_pt3(a.d, nk), kColBlk, a);
@@ -464,11 +464,11 @@ static void ph_scalar(Launcher &L) {
     const int ia0 = d.ims + 4 * ((d.ids - d.ims) >> 2);
     const dim3 gv((d.ide - ia0 + kVecTileI) / kVecTileI, (d.jde - d.jds + kAdvRows) / kAdvRows,
                   (d.kde - d.kds + kAdvVK) / kAdvVK);
-    L.launchK(KID_ADV_SCALAR, KR(k_adv_scalar_v), gv, dim3(64, kAdvRows, 1), L.a);
+    L.launchK(KID_ADV_SCALAR, KR(k_adv_scalar_x4), gv, dim3(64, kAdvRows, 1), L.a);
   } else {
     L.launchK(KID_ADV_SCALAR, per ? KRP(k_adv_scalar, true) : KRP(k_adv_scalar, false), ga, dim3(64, kAdvRows, 1), L.a);
   }
-  if (L.vec) L.launchK(KID_UPDATE_SCALAR, KR(k_update_scalar_v), g_vec3(L.a.d, L.a.d.kde - L.a.d.kds), kColBlk, L.a);
+  if (L.vec) L.launchK(KID_UPDATE_SCALAR, KR(k_update_scalar_x4), g_vec3(L.a.d, L.a.d.kde - L.a.d.kds), kColBlk, L.a);
   else L.launchK(KID_UPDATE_SCALAR, KR(k_update_scalar), g_pt3(L.a.d, L.a.d.kde - L.a.d.kds), kColBlk, L.a);
 }
 static void ph_p_rho_phi(Launcher &L) {
@@ -787,9 +787,10 @@ extern "C" int wrfmi_ensemble_create(int n_members, const wrfmi_state *states, c
   char *base = (char *)e->table;
   for (size_t o = 0; o < nops; o++) {
     EnsOp &eo = e->ops[o];
-    const unsigned *off = (const unsigned *)(base + pend[o].off), *lut = (const unsigned *)(base + pend[o].lut);
-    if (eo.halo) eo.ha = BatchHalo{(const HaloArgs *)(base + pend[o].args), off, lut, n_members, eo.total};
-    else eo.ka = BatchArgs{(const KArgs *)(base + pend[o].args), off, lut, n_members, eo.total};
+    auto off = (const WRFMI_CONST_AS unsigned *)(base + pend[o].off);
+    auto lut = (const WRFMI_CONST_AS unsigned *)(base + pend[o].lut);
+    if (eo.halo) eo.ha = BatchHalo{(const WRFMI_CONST_AS HaloArgs *)(base + pend[o].args), off, lut, n_members, eo.total};
+    else eo.ka = BatchArgs{(const WRFMI_CONST_AS KArgs *)(base + pend[o].args), off, lut, n_members, eo.total};
   }
   g_ensembles.push_back(e);
   *out = e;
