@@ -1,7 +1,7 @@
 // Device-side helpers shared by the gfx950 kernels of the WRF-ARW dynamics step.
 //
 // Layout: every 3-D field is the reference's Fortran array (ims:ime, kms:kme, jms:jme),
-// i fastest.  Offsets are 32-bit (the host rejects domains of 2^31 elements or more).
+// i fastest.  Offsets are 32-bit byte offsets (the host rejects fields of 2^29 elements or more).
 // All kernels are compiled with -ffp-contract=off and written in the reference's
 // floating-point operation order, so that FP32 results match a reference-order
 // IEEE FP32 evaluation bit for bit (transcendentals aside).

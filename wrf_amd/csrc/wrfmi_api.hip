@@ -201,7 +201,9 @@ static int make_dm(const wrfmi_dims *w, Dm *d) {
   d->ids = w->ids; d->ide = w->ide; d->jds = w->jds; d->jde = w->jde; d->kds = w->kds; d->kde = w->kde;
   d->ims = w->ims; d->ime = w->ime; d->jms = w->jms; d->jme = w->jme; d->kms = w->kms; d->kme = w->kme;
   long long ni = w->ime - w->ims + 1, nk = w->kme - w->kms + 1, nj = w->jme - w->jms + 1;
-  if (ni * nk * nj >= (1LL << 31)) return fail(WRFMI_E_DIMS, "domain of 2^31 or more elements per field");
+  // kernels address elements by 32-bit byte offsets, twice the element offset for the FP64
+  // tridiagonal coefficients: fields stay below 2^29 elements
+  if (ni * nk * nj >= (1LL << 29)) return fail(WRFMI_E_DIMS, "domain of 2^29 or more elements per field");
   d->ni = (int)ni; d->nk = (int)nk; d->sj = (int)(ni * nk);
   return 0;
 }
