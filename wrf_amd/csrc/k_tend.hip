@@ -242,6 +242,192 @@ __global__ void __launch_bounds__(256) k_tend_u(ARGS args) {
   }
 }
 
+// k_tend_u for the doubly periodic case with four i points per lane (16-byte loads), one
+// level per thread.  Element by element the same terms in the same order as k_tend_u with
+// PER = true (5th-order fluxes on every face, no lateral-boundary terms); the vertical
+// flux and dpn at both w levels of the level are formed directly.
+template <typename ARGS>
+__global__ void __launch_bounds__(256) k_tend_u_x4(ARGS args) {
+  Tile3 T;
+  const KArgs &a = kargs(args, T);
+  const Dm &d = a.d;
+  const DState &s = dstate(a);
+  const int i0 = vec_ia0(d) + kVecW * (T.bx * 64 + (int)threadIdx.x);
+  const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
+  const int k = d.kds + (int)T.bz;
+  const int ids = d.ids, ide = d.ide, jde = d.jde, kde = d.kde, kts = d.kds, kte = kde, ktf = kte - 1;
+  if (i0 > ide || j > jde || k > kte) return;
+  const float ts = a.c.time_step >= 0 ? 1.f : -1.f;
+  const float rdx = s.rdx, rdy = s.rdy;
+  const int ni = d.ni, sj = d.sj;
+  const FPtr<float> u = s.u_2, ru = s.ru, rv = s.rv, rom = s.ww, rw = s.rw;
+  const Off o2 = X2(d, i0, j), ob = X3(d, i0, 0, j);  // level kk at ob + kk*ni
+  auto lv = [&](int kk) { return ob + imax(kts, imin(kk, kte)) * ni; };
+  const Off o = lv(k);
+  const bool row_ok = j <= jde - 1;
+  const bool rk1 = a.rk_step == 1;
+  F4 t;
+#pragma unroll
+  for (int n = 0; n < 4; n++) t.v[n] = 0.f;
+  if (row_ok && k <= ktf) {
+    const float c1 = s.c1h[X1(d, k)], c2 = s.c2h[X1(d, k)], rdnw = s.rdnw[X1(d, k)];
+    const F4 msfux = s.msfux.ld<F4>(o2), msfuy = s.msfuy.ld<F4>(o2), msfuyl = s.msfuy.ld<F4>(o2 - 4), msfuyr = s.msfuy.ld<F4>(o2 + 4);
+    // advect_u: y faces j and j+1 at the u points
+    {
+      const F4 ym3 = u.ld<F4>(o - 3 * sj), ym2 = u.ld<F4>(o - 2 * sj), ym1 = u.ld<F4>(o - sj), y0 = u.ld<F4>(o);
+      const F4 yp1 = u.ld<F4>(o + sj), yp2 = u.ld<F4>(o + 2 * sj), yp3 = u.ld<F4>(o + 3 * sj);
+      const F4 rv0 = rv.ld<F4>(o), rv0l = rv.ld<F4>(o - 4), rv1 = rv.ld<F4>(o + sj), rv1l = rv.ld<F4>(o + sj - 4);
+#pragma unroll
+      for (int n = 0; n < 4; n++) {
+        const float vel0 = 0.5f * (rv0.v[n] + xm1(rv0l, rv0, n));
+        const float vel1 = 0.5f * (rv1.v[n] + xm1(rv1l, rv1, n));
+        const float f0 = vel0 * flux5(ym3.v[n], ym2.v[n], ym1.v[n], y0.v[n], yp1.v[n], yp2.v[n], vel0, ts);
+        const float f1 = vel1 * flux5(ym2.v[n], ym1.v[n], y0.v[n], yp1.v[n], yp2.v[n], yp3.v[n], vel1, ts);
+        const float mrdy = msfux.v[n] * rdy;
+        t.v[n] = t.v[n] - mrdy * (f1 - f0);
+      }
+    }
+    // x faces i0 .. i0+4
+    {
+      const F4 wl = u.ld<F4>(o - 4), wc = u.ld<F4>(o), wr = u.ld<F4>(o + 4);
+      const F4 rl = ru.ld<F4>(o - 4), rc = ru.ld<F4>(o), rr = ru.ld<F4>(o + 4);
+      float W[12], R[6];
+#pragma unroll
+      for (int n = 0; n < 4; n++) {
+        W[n] = wl.v[n];
+        W[n + 4] = wc.v[n];
+        W[n + 8] = wr.v[n];
+        R[n + 1] = rc.v[n];
+      }
+      R[0] = rl.v[3];
+      R[5] = rr.v[0];
+      float fx[5];
+#pragma unroll
+      for (int n = 0; n < 5; n++) {
+        const float vel = 0.5f * (R[n + 1] + R[n]);
+        fx[n] = vel * flux5(W[n + 1], W[n + 2], W[n + 3], W[n + 4], W[n + 5], W[n + 6], vel, ts);
+      }
+#pragma unroll
+      for (int n = 0; n < 4; n++) {
+        const float mrdx = msfux.v[n] * rdx;
+        t.v[n] = t.v[n] - mrdx * (fx[n + 1] - fx[n]);
+      }
+    }
+    // vertical flux divergence (order 3) from the fluxes at w levels k and k+1
+    {
+      const F4 qm2 = u.ld<F4>(lv(k - 2)), qm1 = u.ld<F4>(lv(k - 1)), q0 = u.ld<F4>(o), qp1 = u.ld<F4>(lv(k + 1)),
+               qp2 = u.ld<F4>(lv(k + 2));
+      const F4 w0 = rom.ld<F4>(o), w0l = rom.ld<F4>(o - 4), w1 = rom.ld<F4>(lv(k + 1)), w1l = rom.ld<F4>(lv(k + 1) - 4);
+      auto vflux = [&](int kk, float wc_, float wl_, float qm2_, float qm1_, float q0_, float qp1_) {
+        const int kc = imin(kk, kte);
+        const float lo = 0.5f * (wc_ + wl_) * (s.fnm[X1(d, kc)] * q0_ + s.fnp[X1(d, kc)] * qm1_);
+        const float vel = 0.5f * (wl_ + wc_);
+        const float hi = vel * flux3(qm2_, qm1_, q0_, qp1_, -vel, ts);
+        return (kk == kts || kk == kte) ? 0.f : ((kk == kts + 1 || kk == ktf) ? lo : hi);
+      };
+#pragma unroll
+      for (int n = 0; n < 4; n++) {
+        const float vf_lo = vflux(k, w0.v[n], xm1(w0l, w0, n), qm2.v[n], qm1.v[n], q0.v[n], qp1.v[n]);
+        const float vf_hi = vflux(k + 1, w1.v[n], xm1(w1l, w1, n), qm1.v[n], q0.v[n], qp1.v[n], qp2.v[n]);
+        t.v[n] = t.v[n] - rdnw * (vf_hi - vf_lo);
+      }
+    }
+    // horizontal_pressure_gradient (x)
+    {
+      const FPtr<float> p = s.p, ph = s.ph_2, al = s.al, alt = s.alt, pb = s.pb, php = s.php;
+      const F4 muu = s.muu.ld<F4>(o2), mu = s.mu_2.ld<F4>(o2), mul = s.mu_2.ld<F4>(o2 - 4);
+      const Off op = lv(k + 1), om = lv(k - 1);
+      const F4 php1 = ph.ld<F4>(op), php1l = ph.ld<F4>(op - 4), ph0 = ph.ld<F4>(o), ph0l = ph.ld<F4>(o - 4);
+      const F4 altc = alt.ld<F4>(o), altl = alt.ld<F4>(o - 4), pc = p.ld<F4>(o), pl = p.ld<F4>(o - 4);
+      const F4 alc = al.ld<F4>(o), all = al.ld<F4>(o - 4), pbc = pb.ld<F4>(o), pbl = pb.ld<F4>(o - 4);
+      const F4 phpc = php.ld<F4>(o), phpl = php.ld<F4>(o - 4), cqu = s.cqu.ld<F4>(o);
+      const F4 pp = p.ld<F4>(op), ppl = p.ld<F4>(op - 4), pm = p.ld<F4>(om), pml = p.ld<F4>(om - 4);
+      const bool nh = a.c.non_hydrostatic;
+      const float fm0 = s.fnm[X1(d, k)], fp0 = s.fnp[X1(d, k)], fm1 = s.fnm[X1(d, imin(k + 1, kte))], fp1 = s.fnp[X1(d, imin(k + 1, kte))];
+      // dpn at w level k (extrapolated at k = 1) and k+1 (the lid at kde)
+      F4 dlo, dhi;
+#pragma unroll
+      for (int n = 0; n < 4; n++) {
+        dlo.v[n] = .5f * (fm0 * (xm1(pl, pc, n) + pc.v[n]) + fp0 * (xm1(pml, pm, n) + pm.v[n]));
+        dhi.v[n] = .5f * (fm1 * (xm1(ppl, pp, n) + pp.v[n]) + fp1 * (xm1(pl, pc, n) + pc.v[n]));
+      }
+      if (k == 1) {
+        const float cf1 = s.cf1, cf2 = s.cf2, cf3 = s.cf3;
+        const Off q = ob + ni;
+        const F4 c1_ = p.ld<F4>(q), c2_ = p.ld<F4>(q + ni), c3_ = p.ld<F4>(q + 2 * ni);
+        const F4 l1 = p.ld<F4>(q - 4), l2 = p.ld<F4>(q + ni - 4), l3 = p.ld<F4>(q + 2 * ni - 4);
+#pragma unroll
+        for (int n = 0; n < 4; n++)
+          dlo.v[n] = .5f * (cf1 * (xm1(l1, c1_, n) + c1_.v[n]) + cf2 * (xm1(l2, c2_, n) + c2_.v[n]) + cf3 * (xm1(l3, c3_, n) + c3_.v[n]));
+      }
+      if (k + 1 == kde) {
+        F4 top;
+#pragma unroll
+        for (int n = 0; n < 4; n++) top.v[n] = 0.f;
+        if (a.c.top_lid) {
+          const float cfn = s.cfn, cfn1 = s.cfn1;
+          const Off q = ob + (kde - 1) * ni;
+          const F4 c1_ = p.ld<F4>(q), c2_ = p.ld<F4>(q - ni), l1 = p.ld<F4>(q - 4), l2 = p.ld<F4>(q - ni - 4);
+#pragma unroll
+          for (int n = 0; n < 4; n++) top.v[n] = .5f * (cfn * (xm1(l1, c1_, n) + c1_.v[n]) + cfn1 * (xm1(l2, c2_, n) + c2_.v[n]));
+        }
+        dhi = top;
+      }
+#pragma unroll
+      for (int n = 0; n < 4; n++) {
+        const float m = msfux.v[n] / msfuy.v[n];
+        float dpx = m * .5f * rdx * (c1 * muu.v[n] + c2) *
+                    ((php1.v[n] - xm1(php1l, php1, n) + ph0.v[n] - xm1(ph0l, ph0, n)) +
+                     (altc.v[n] + xm1(altl, altc, n)) * (pc.v[n] - xm1(pl, pc, n)) +
+                     (alc.v[n] + xm1(all, alc, n)) * (pbc.v[n] - xm1(pbl, pbc, n)));
+        if (nh) {
+          dpx = dpx + m * rdx * (phpc.v[n] - xm1(phpl, phpc, n)) *
+                          (rdnw * (dhi.v[n] - dlo.v[n]) - .5f * ((c1 * xm1(mul, mu, n)) + (c1 * mu.v[n])));
+          t.v[n] = t.v[n] - cqu.v[n] * dpx;
+        } else {
+          t.v[n] = t.v[n] - dpx;
+        }
+      }
+    }
+    // coriolis and curvature
+    {
+      const F4 f = s.f.ld<F4>(o2), fl = s.f.ld<F4>(o2 - 4), e = s.e.ld<F4>(o2), el = s.e.ld<F4>(o2 - 4);
+      const F4 ca = s.cosa.ld<F4>(o2), cal = s.cosa.ld<F4>(o2 - 4);
+      const F4 rv0 = rv.ld<F4>(o), rv0l = rv.ld<F4>(o - 4), rv1 = rv.ld<F4>(o + sj), rv1l = rv.ld<F4>(o + sj - 4);
+      const F4 rw0 = rw.ld<F4>(o), rw0l = rw.ld<F4>(o - 4), rw1 = rw.ld<F4>(lv(k + 1)), rw1l = rw.ld<F4>(lv(k + 1) - 4);
+      const F4 wl = u.ld<F4>(o - 4), wc = u.ld<F4>(o), wr = u.ld<F4>(o + 4);
+      const F4 v0 = s.v_2.ld<F4>(o), v0l = s.v_2.ld<F4>(o - 4), v1 = s.v_2.ld<F4>(o + sj), v1l = s.v_2.ld<F4>(o + sj - 4);
+      const F4 mx0 = s.msfvx.ld<F4>(o2), mx0l = s.msfvx.ld<F4>(o2 - 4), mx1 = s.msfvx.ld<F4>(o2 + ni), mx1l = s.msfvx.ld<F4>(o2 + ni - 4);
+#pragma unroll
+      for (int n = 0; n < 4; n++) {
+        const float rvs = xm1(rv1l, rv1, n) + rv1.v[n] + xm1(rv0l, rv0, n) + rv0.v[n];
+        const float rws = xm1(rw1l, rw1, n) + xm1(rw0l, rw0, n) + rw1.v[n] + rw0.v[n];
+        t.v[n] = t.v[n] + (msfux.v[n] / msfuy.v[n]) * 0.5f * (f.v[n] + xm1(fl, f, n)) * 0.25f * rvs -
+                 0.5f * (e.v[n] + xm1(el, e, n)) * 0.5f * (ca.v[n] + xm1(cal, ca, n)) * 0.25f * rws;
+        // vx at i and i-1
+        const float ui = wc.v[n], uim = xm1(wl, wc, n), uip = xp1(wc, wr, n);
+        const float vx_i = 0.5f * (ui + uip) * (mx1.v[n] - mx0.v[n]) * rdy -
+                           0.5f * (v0.v[n] + v1.v[n]) * (xp1(msfuy, msfuyr, n) - msfuy.v[n]) * rdx;
+        const float vx_im = 0.5f * (uim + ui) * (xm1(mx1l, mx1, n) - xm1(mx0l, mx0, n)) * rdy -
+                            0.5f * (xm1(v0l, v0, n) + xm1(v1l, v1, n)) * (msfuy.v[n] - xm1(msfuyl, msfuy, n)) * rdx;
+        t.v[n] = t.v[n] + 0.5f * (vx_i + vx_im) * 0.25f * rvs - ui * K_RERADIUS * 0.25f * rws;
+      }
+    }
+  }
+  // rk_addtend_dry
+  if (row_ok && k <= kte - 1) {
+    const F4 msf = s.msfuy.ld<F4>(o2);
+    F4 tf = s.ru_tendf.ld<F4>(o);
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+      if (rk1) tf.v[n] = tf.v[n] + 0.f * msf.v[n];  // u_save was zeroed by rk_tendency (module_em.F:395)
+      t.v[n] = t.v[n] + tf.v[n] / msf.v[n];
+    }
+    if (rk1) st4m(s.ru_tendf, o, tf, i0, ids, ide);
+  }
+  st4m(s.ru_tend, o, t, i0, ids, ide);
+}
+
 // ------------------------------------------------------------------------------ v
 template <typename ARGS, bool PER>
 __global__ void __launch_bounds__(256) k_tend_v(ARGS args) {
@@ -446,6 +632,189 @@ __global__ void __launch_bounds__(256) k_tend_v(ARGS args) {
     vf_lo = vf_hi;
     dpn_lo = dpn_hi;
   }
+}
+
+// k_tend_v for the doubly periodic case with four i points per lane, one level per thread;
+// element by element the terms of k_tend_v with PER = true, in its order.
+template <typename ARGS>
+__global__ void __launch_bounds__(256) k_tend_v_x4(ARGS args) {
+  Tile3 T;
+  const KArgs &a = kargs(args, T);
+  const Dm &d = a.d;
+  const DState &s = dstate(a);
+  const int i0 = vec_ia0(d) + kVecW * (T.bx * 64 + (int)threadIdx.x);
+  const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
+  const int k = d.kds + (int)T.bz;
+  const int ids = d.ids, ide = d.ide, jde = d.jde, kde = d.kde, kts = d.kds, kte = kde, ktf = kte - 1;
+  if (i0 > ide || j > jde || k > kte) return;
+  const float ts = a.c.time_step >= 0 ? 1.f : -1.f;
+  const float rdx = s.rdx, rdy = s.rdy;
+  const int ni = d.ni, sj = d.sj;
+  const FPtr<float> v = s.v_2, ru = s.ru, rv = s.rv, rom = s.ww, rw = s.rw;
+  const Off o2 = X2(d, i0, j), ob = X3(d, i0, 0, j);
+  auto lv = [&](int kk) { return ob + imax(kts, imin(kk, kte)) * ni; };
+  const Off o = lv(k);
+  const bool rk1 = a.rk_step == 1;
+  F4 t;
+#pragma unroll
+  for (int n = 0; n < 4; n++) t.v[n] = 0.f;
+  if (k <= ktf) {
+    const float c1 = s.c1h[X1(d, k)], c2 = s.c2h[X1(d, k)], rdnw = s.rdnw[X1(d, k)];
+    const F4 msfvy = s.msfvy.ld<F4>(o2), msfvx = s.msfvx.ld<F4>(o2);
+    // advect_v: y faces j and j+1
+    {
+      const F4 ym3 = v.ld<F4>(o - 3 * sj), ym2 = v.ld<F4>(o - 2 * sj), ym1 = v.ld<F4>(o - sj), y0 = v.ld<F4>(o);
+      const F4 yp1 = v.ld<F4>(o + sj), yp2 = v.ld<F4>(o + 2 * sj), yp3 = v.ld<F4>(o + 3 * sj);
+      const F4 rvm = rv.ld<F4>(o - sj), rv0 = rv.ld<F4>(o), rv1 = rv.ld<F4>(o + sj);
+#pragma unroll
+      for (int n = 0; n < 4; n++) {
+        const float vel0 = 0.5f * (rv0.v[n] + rvm.v[n]);
+        const float vel1 = 0.5f * (rv1.v[n] + rv0.v[n]);
+        const float f0 = vel0 * flux5(ym3.v[n], ym2.v[n], ym1.v[n], y0.v[n], yp1.v[n], yp2.v[n], vel0, ts);
+        const float f1 = vel1 * flux5(ym2.v[n], ym1.v[n], y0.v[n], yp1.v[n], yp2.v[n], yp3.v[n], vel1, ts);
+        const float mrdy = msfvy.v[n] * rdy;
+        t.v[n] = t.v[n] - mrdy * (f1 - f0);
+      }
+    }
+    // x faces i0 .. i0+4 (face ii between mass points ii-1 and ii)
+    {
+      const F4 wl = v.ld<F4>(o - 4), wc = v.ld<F4>(o), wr = v.ld<F4>(o + 4);
+      const F4 r0 = ru.ld<F4>(o), r0r = ru.ld<F4>(o + 4), rm = ru.ld<F4>(o - sj), rmr = ru.ld<F4>(o - sj + 4);
+      float W[12];
+#pragma unroll
+      for (int n = 0; n < 4; n++) {
+        W[n] = wl.v[n];
+        W[n + 4] = wc.v[n];
+        W[n + 8] = wr.v[n];
+      }
+      float fx[5];
+#pragma unroll
+      for (int n = 0; n < 5; n++) {
+        const float rj = n < 4 ? r0.v[n] : r0r.v[0], rjm = n < 4 ? rm.v[n] : rmr.v[0];
+        const float vel = 0.5f * (rj + rjm);
+        fx[n] = vel * flux5(W[n + 1], W[n + 2], W[n + 3], W[n + 4], W[n + 5], W[n + 6], vel, ts);
+      }
+#pragma unroll
+      for (int n = 0; n < 4; n++) {
+        const float mrdx = msfvy.v[n] * rdx;
+        t.v[n] = t.v[n] - mrdx * (fx[n + 1] - fx[n]);
+      }
+    }
+    // vertical flux divergence
+    {
+      const F4 qm2 = v.ld<F4>(lv(k - 2)), qm1 = v.ld<F4>(lv(k - 1)), q0 = v.ld<F4>(o), qp1 = v.ld<F4>(lv(k + 1)),
+               qp2 = v.ld<F4>(lv(k + 2));
+      const F4 w0 = rom.ld<F4>(o), w0s = rom.ld<F4>(o - sj), w1 = rom.ld<F4>(lv(k + 1)), w1s = rom.ld<F4>(lv(k + 1) - sj);
+      auto vflux = [&](int kk, float wc_, float ws_, float qm2_, float qm1_, float q0_, float qp1_) {
+        const int kc = imin(kk, kte);
+        const float vel = 0.5f * (wc_ + ws_);
+        const float lo = vel * (s.fnm[X1(d, kc)] * q0_ + s.fnp[X1(d, kc)] * qm1_);
+        const float hi = vel * flux3(qm2_, qm1_, q0_, qp1_, -vel, ts);
+        return (kk == kts || kk == kte) ? 0.f : ((kk == kts + 1 || kk == ktf) ? lo : hi);
+      };
+#pragma unroll
+      for (int n = 0; n < 4; n++) {
+        const float vf_lo = vflux(k, w0.v[n], w0s.v[n], qm2.v[n], qm1.v[n], q0.v[n], qp1.v[n]);
+        const float vf_hi = vflux(k + 1, w1.v[n], w1s.v[n], qm1.v[n], q0.v[n], qp1.v[n], qp2.v[n]);
+        t.v[n] = t.v[n] - (msfvy.v[n] / msfvx.v[n]) * rdnw * (vf_hi - vf_lo);
+      }
+    }
+    // horizontal_pressure_gradient (y)
+    {
+      const FPtr<float> p = s.p, ph = s.ph_2, al = s.al, alt = s.alt, pb = s.pb, php = s.php;
+      const F4 muv = s.muv.ld<F4>(o2), mu = s.mu_2.ld<F4>(o2), mus = s.mu_2.ld<F4>(o2 - ni);
+      const Off op = lv(k + 1), om = lv(k - 1);
+      const F4 h1 = ph.ld<F4>(op), h1s = ph.ld<F4>(op - sj), h0 = ph.ld<F4>(o), h0s = ph.ld<F4>(o - sj);
+      const F4 altc = alt.ld<F4>(o), alts = alt.ld<F4>(o - sj), pc = p.ld<F4>(o), ps = p.ld<F4>(o - sj);
+      const F4 alc = al.ld<F4>(o), als = al.ld<F4>(o - sj), pbc = pb.ld<F4>(o), pbs = pb.ld<F4>(o - sj);
+      const F4 phpc = php.ld<F4>(o), phps = php.ld<F4>(o - sj), cqv = s.cqv.ld<F4>(o);
+      const F4 pp = p.ld<F4>(op), pps = p.ld<F4>(op - sj), pm = p.ld<F4>(om), pms = p.ld<F4>(om - sj);
+      const bool nh = a.c.non_hydrostatic;
+      const float fm0 = s.fnm[X1(d, k)], fp0 = s.fnp[X1(d, k)], fm1 = s.fnm[X1(d, imin(k + 1, kte))], fp1 = s.fnp[X1(d, imin(k + 1, kte))];
+      F4 dlo, dhi;
+#pragma unroll
+      for (int n = 0; n < 4; n++) {
+        dlo.v[n] = .5f * (fm0 * (ps.v[n] + pc.v[n]) + fp0 * (pms.v[n] + pm.v[n]));
+        dhi.v[n] = .5f * (fm1 * (pps.v[n] + pp.v[n]) + fp1 * (ps.v[n] + pc.v[n]));
+      }
+      if (k == 1) {
+        const float cf1 = s.cf1, cf2 = s.cf2, cf3 = s.cf3;
+        const Off q = ob + ni;
+        const F4 c1_ = p.ld<F4>(q), c2_ = p.ld<F4>(q + ni), c3_ = p.ld<F4>(q + 2 * ni);
+        const F4 s1 = p.ld<F4>(q - sj), s2 = p.ld<F4>(q + ni - sj), s3 = p.ld<F4>(q + 2 * ni - sj);
+#pragma unroll
+        for (int n = 0; n < 4; n++)
+          dlo.v[n] = .5f * (cf1 * (s1.v[n] + c1_.v[n]) + cf2 * (s2.v[n] + c2_.v[n]) + cf3 * (s3.v[n] + c3_.v[n]));
+      }
+      if (k + 1 == kde) {
+        F4 top;
+#pragma unroll
+        for (int n = 0; n < 4; n++) top.v[n] = 0.f;
+        if (a.c.top_lid) {
+          const float cfn = s.cfn, cfn1 = s.cfn1;
+          const Off q = ob + (kde - 1) * ni;
+          const F4 c1_ = p.ld<F4>(q), c2_ = p.ld<F4>(q - ni), s1 = p.ld<F4>(q - sj), s2 = p.ld<F4>(q - ni - sj);
+#pragma unroll
+          for (int n = 0; n < 4; n++) top.v[n] = .5f * (cfn * (s1.v[n] + c1_.v[n]) + cfn1 * (s2.v[n] + c2_.v[n]));
+        }
+        dhi = top;
+      }
+#pragma unroll
+      for (int n = 0; n < 4; n++) {
+        const float m = msfvy.v[n] / msfvx.v[n];
+        float dpy = m * .5f * rdy * (c1 * muv.v[n] + c2) *
+                    ((h1.v[n] - h1s.v[n] + h0.v[n] - h0s.v[n]) +
+                     (altc.v[n] + alts.v[n]) * (pc.v[n] - ps.v[n]) +
+                     (alc.v[n] + als.v[n]) * (pbc.v[n] - pbs.v[n]));
+        if (nh) {
+          dpy = dpy + m * rdy * (phpc.v[n] - phps.v[n]) * (rdnw * (dhi.v[n] - dlo.v[n]) - .5f * ((c1 * mus.v[n]) + (c1 * mu.v[n])));
+          t.v[n] = t.v[n] - cqv.v[n] * dpy;
+        } else {
+          t.v[n] = t.v[n] - dpy;
+        }
+      }
+    }
+    // coriolis and curvature
+    {
+      const F4 f = s.f.ld<F4>(o2), fs = s.f.ld<F4>(o2 - ni), e = s.e.ld<F4>(o2), es = s.e.ld<F4>(o2 - ni);
+      const F4 sa = s.sina.ld<F4>(o2), sas = s.sina.ld<F4>(o2 - ni);
+      const F4 r0 = ru.ld<F4>(o), r0r = ru.ld<F4>(o + 4), rm = ru.ld<F4>(o - sj), rmr = ru.ld<F4>(o - sj + 4);
+      const F4 rw0 = rw.ld<F4>(o), rw0s = rw.ld<F4>(o - sj), rw1 = rw.ld<F4>(lv(k + 1)), rw1s = rw.ld<F4>(lv(k + 1) - sj);
+      const F4 u0 = s.u_2.ld<F4>(o), u0r = s.u_2.ld<F4>(o + 4), um = s.u_2.ld<F4>(o - sj), umr = s.u_2.ld<F4>(o - sj + 4);
+      const F4 vm = v.ld<F4>(o - sj), v0 = v.ld<F4>(o), vp = v.ld<F4>(o + sj);
+      const F4 mxm = s.msfvx.ld<F4>(o2 - ni), mx0 = msfvx, mxp = s.msfvx.ld<F4>(o2 + ni);
+      const F4 my0 = s.msfuy.ld<F4>(o2), my0r = s.msfuy.ld<F4>(o2 + 4), mym = s.msfuy.ld<F4>(o2 - ni), mymr = s.msfuy.ld<F4>(o2 - ni + 4);
+#pragma unroll
+      for (int n = 0; n < 4; n++) {
+        const float m = msfvy.v[n] / msfvx.v[n];
+        const float rus = r0.v[n] + xp1(r0, r0r, n) + rm.v[n] + xp1(rm, rmr, n);
+        const float rws = rw1s.v[n] + rw0s.v[n] + rw1.v[n] + rw0.v[n];
+        t.v[n] = t.v[n] - m * 0.5f * (f.v[n] + fs.v[n]) * 0.25f * rus +
+                 m * 0.5f * (e.v[n] + es.v[n]) * 0.5f * (sa.v[n] + sas.v[n]) * 0.25f * rws;
+        const float vx_j = 0.5f * (u0.v[n] + xp1(u0, u0r, n)) * (mxp.v[n] - mx0.v[n]) * rdy -
+                           0.5f * (v0.v[n] + vp.v[n]) * (xp1(my0, my0r, n) - my0.v[n]) * rdx;
+        const float vx_jm = 0.5f * (um.v[n] + xp1(um, umr, n)) * (mx0.v[n] - mxm.v[n]) * rdy -
+                            0.5f * (vm.v[n] + v0.v[n]) * (xp1(mym, mymr, n) - mym.v[n]) * rdx;
+        t.v[n] = t.v[n] - 0.5f * (vx_j + vx_jm) * 0.25f * rus - m * v0.v[n] * K_RERADIUS * 0.25f * rws;
+      }
+    }
+  }
+  // rk_addtend_dry
+  if (k <= kte - 1) {
+    const F4 mvx = s.msfvx.ld<F4>(o2), mvi = s.msfvx_inv.ld<F4>(o2);
+    F4 tf = s.rv_tendf.ld<F4>(o);
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+      if (rk1) tf.v[n] = tf.v[n] + 0.f * mvx.v[n];  // v_save was zeroed by rk_tendency
+      t.v[n] = t.v[n] + tf.v[n] * mvi.v[n];
+    }
+    if (rk1) st4m(s.rv_tendf, o, tf, i0, ids, ide - 1);
+  }
+  // col_ok: only i <= ide-1 carry the tendency; i = ide is stored as zero
+#pragma unroll
+  for (int n = 0; n < 4; n++)
+    if (i0 + n > ide - 1) t.v[n] = 0.f;
+  st4m(s.rv_tend, o, t, i0, ids, ide);
 }
 
 // ------------------------------------------------------------------------------ w
@@ -884,8 +1253,10 @@ __global__ void __launch_bounds__(256) k_tend_t(ARGS args) {
 
 WRFMI_INST_P(k_tend_u, false)
 WRFMI_INST_P(k_tend_u, true)
+WRFMI_INST(k_tend_u_x4)
 WRFMI_INST_P(k_tend_v, false)
 WRFMI_INST_P(k_tend_v, true)
+WRFMI_INST(k_tend_v_x4)
 WRFMI_INST_P(k_tend_w, false)
 WRFMI_INST_P(k_tend_w, true)
 WRFMI_INST_P(k_tend_t, false)

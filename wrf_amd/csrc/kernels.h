@@ -17,7 +17,9 @@ constexpr int kHaloK = 8;      // levels per thread in k_halo
 template <typename A> __global__ void k_step_prep(A);
 template <typename A, bool COMP> __global__ void k_ww_scan(A);
 template <typename A, bool PER> __global__ void k_tend_u(A);
+template <typename A> __global__ void k_tend_u_x4(A);
 template <typename A, bool PER> __global__ void k_tend_v(A);
+template <typename A> __global__ void k_tend_v_x4(A);
 template <typename A, bool PER> __global__ void k_tend_w(A);
 template <typename A, bool PER> __global__ void k_tend_t(A);
 template <typename A> __global__ void k_small_prep(A);
