@@ -1015,6 +1015,197 @@ __global__ void __launch_bounds__(256) k_tend_w(ARGS args) {
   }
 }
 
+// k_tend_w for the doubly periodic case with four i points per lane, one level per thread;
+// element by element the terms of k_tend_w with PER = true, in its order.
+template <typename ARGS>
+__global__ void __launch_bounds__(256) k_tend_w_x4(ARGS args) {
+  Tile3 T;
+  const KArgs &a = kargs(args, T);
+  const Dm &d = a.d;
+  const DState &s = dstate(a);
+  const int i0 = vec_ia0(d) + kVecW * (T.bx * 64 + (int)threadIdx.x);
+  const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
+  const int k = d.kds + (int)T.bz;
+  const int ids = d.ids, ide = d.ide, jde = d.jde, kde = d.kde, kts = d.kds, kte = kde, ktf = kte - 1;
+  if (i0 > ide || j > jde || k > kte) return;
+  const float ts = a.c.time_step >= 0 ? 1.f : -1.f;
+  const float rdx = s.rdx, rdy = s.rdy;
+  const int ni = d.ni, sj = d.sj;
+  const FPtr<float> w = s.w_2, ru = s.ru, rv = s.rv, rom = s.ww;
+  const CPtr<float> fzm = s.fnm, fzp = s.fnp;
+  const Off o2 = X2(d, i0, j), ob = X3(d, i0, 0, j);
+  auto lv = [&](int kk) { return ob + imax(kts, imin(kk, kte)) * ni; };
+  const Off o = lv(k);
+  const bool rk1 = a.rk_step == 1;
+  const bool jmass = j <= jde - 1;
+  F4 t;
+#pragma unroll
+  for (int n = 0; n < 4; n++) t.v[n] = 0.f;
+  if (!jmass || i0 > ide - 1) {
+    st4m(s.rw_tend, o, t, i0, ids, ide);
+    return;
+  }
+  const F4 msfty = s.msfty.ld<F4>(o2), msftx = s.msftx.ld<F4>(o2);
+  if (k >= kts + 1) {
+    // face velocities at w level k: fzm(k) r(k) + fzp(k) r(k-1) below the lid, the
+    // extrapolation (2 - fzm(k-1)) r(k-1) - fzp(k-1) r(k-2) at k = ktf+1
+    const bool lid = k > ktf;
+    const float ca = lid ? 2.f - fzm[X1(d, k - 1)] : fzm[X1(d, k)];
+    const float cb = lid ? -fzp[X1(d, k - 1)] : fzp[X1(d, k)];
+    const Off oa = lid ? lv(k - 1) : o, obb = lid ? lv(k - 2) : lv(k - 1);
+    // y faces j and j+1
+    {
+      const F4 ym3 = w.ld<F4>(o - 3 * sj), ym2 = w.ld<F4>(o - 2 * sj), ym1 = w.ld<F4>(o - sj), y0 = w.ld<F4>(o);
+      const F4 yp1 = w.ld<F4>(o + sj), yp2 = w.ld<F4>(o + 2 * sj), yp3 = w.ld<F4>(o + 3 * sj);
+      const F4 r0a = rv.ld<F4>(oa), r0b = rv.ld<F4>(obb), r1a = rv.ld<F4>(oa + sj), r1b = rv.ld<F4>(obb + sj);
+#pragma unroll
+      for (int n = 0; n < 4; n++) {
+        const float vel0 = ca * r0a.v[n] + cb * r0b.v[n];
+        const float vel1 = ca * r1a.v[n] + cb * r1b.v[n];
+        const float f0 = vel0 * flux5(ym3.v[n], ym2.v[n], ym1.v[n], y0.v[n], yp1.v[n], yp2.v[n], vel0, ts);
+        const float f1 = vel1 * flux5(ym2.v[n], ym1.v[n], y0.v[n], yp1.v[n], yp2.v[n], yp3.v[n], vel1, ts);
+        const float mrdy = msftx.v[n] * rdy;
+        t.v[n] = t.v[n] - mrdy * (f1 - f0);
+      }
+    }
+    // x faces i0 .. i0+4
+    {
+      const F4 wl = w.ld<F4>(o - 4), wc = w.ld<F4>(o), wr = w.ld<F4>(o + 4);
+      const F4 rac = ru.ld<F4>(oa), rar = ru.ld<F4>(oa + 4), rbc = ru.ld<F4>(obb), rbr = ru.ld<F4>(obb + 4);
+      float W[12];
+#pragma unroll
+      for (int n = 0; n < 4; n++) {
+        W[n] = wl.v[n];
+        W[n + 4] = wc.v[n];
+        W[n + 8] = wr.v[n];
+      }
+      float fx[5];
+#pragma unroll
+      for (int n = 0; n < 5; n++) {
+        const float ra = n < 4 ? rac.v[n] : rar.v[0], rb = n < 4 ? rbc.v[n] : rbr.v[0];
+        const float vel = ca * ra + cb * rb;
+        fx[n] = vel * flux5(W[n + 1], W[n + 2], W[n + 3], W[n + 4], W[n + 5], W[n + 6], vel, ts);
+      }
+#pragma unroll
+      for (int n = 0; n < 4; n++) {
+        const float mrdx = msftx.v[n] * rdx;
+        t.v[n] = t.v[n] - mrdx * (fx[n + 1] - fx[n]);
+      }
+    }
+    // vertical advection of w (order 3) with the lid flux term
+    {
+      const F4 qm2 = w.ld<F4>(lv(k - 2)), qm1 = w.ld<F4>(lv(k - 1)), q0 = w.ld<F4>(o), qp1 = w.ld<F4>(lv(k + 1)),
+               qp2 = w.ld<F4>(lv(k + 2));
+      const F4 wm = rom.ld<F4>(lv(k - 1)), w0 = rom.ld<F4>(o), wp = rom.ld<F4>(lv(k + 1));
+      // vflux at w level kk in kts+1 .. ktf+1 from rom and w at kk and kk-1
+      auto vflux = [&](int kk, float rk, float rkm, float qm2_, float qm1_, float q0_, float qp1_) {
+        const float lo = 0.25f * (rk + rkm) * (q0_ + qm1_);
+        const float vel = 0.5f * (rk + rkm);
+        const float hi = vel * flux3(qm2_, qm1_, q0_, qp1_, -vel, ts);
+        return (kk == kts + 1 || kk == ktf + 1) ? lo : hi;
+      };
+      const float rdnk = s.rdn[X1(d, imin(k, ktf))], rdnm = s.rdn[X1(d, k - 1)];
+#pragma unroll
+      for (int n = 0; n < 4; n++) {
+        const float vf_c = vflux(k, w0.v[n], wm.v[n], qm2.v[n], qm1.v[n], q0.v[n], qp1.v[n]);
+        if (k <= ktf) {
+          const float vf_hi = vflux(k + 1, wp.v[n], w0.v[n], qm1.v[n], q0.v[n], qp1.v[n], qp2.v[n]);
+          t.v[n] = t.v[n] - rdnk * (vf_hi - vf_c);
+        } else {
+          t.v[n] = t.v[n] + 2.f * rdnm * (vf_c);
+        }
+      }
+    }
+    // pg_buoy_w
+    {
+      const F4 mu = s.mu_2.ld<F4>(o2), mub = s.mub.ld<F4>(o2);
+      const float c1f = s.c1f[X1(d, k)], c2f = s.c2f[X1(d, k)];
+      const F4 pm = s.p.ld<F4>(lv(k - 1));
+      if (k == kde) {
+        // cqw(kde-1) recomputed from the moist species (k_tend_w reads it before the level
+        // kde-1 thread replaces it by 1/(1+cqw))
+        F4 qtot;
+#pragma unroll
+        for (int n = 0; n < 4; n++) qtot.v[n] = 0.f;
+        const Off ot = lv(kde - 1);
+        for (int m = 0; m < s.n_moist; m++) {
+          const F4 qa = s.q[m].ld<F4>(ot), qb = s.q[m].ld<F4>(ot - ni);
+#pragma unroll
+          for (int n = 0; n < 4; n++) qtot.v[n] = qtot.v[n] + qa.v[n] + qb.v[n];
+        }
+        const float rdnw = s.rdnw[X1(d, k - 1)];
+#pragma unroll
+        for (int n = 0; n < 4; n++) {
+          const float cqw_top = s.n_moist >= 1 ? 0.5f * qtot.v[n] : 0.f;
+          const float cq1 = 1.f / (1.f + cqw_top);
+          const float cq2 = cqw_top * cq1;
+          t.v[n] = t.v[n] + (1.f / msfty.v[n]) * K_G *
+                                (cq1 * 2.f * rdnw * (-pm.v[n]) - (c1f * mu.v[n]) - cq2 * (c1f * mub.v[n] + c2f));
+        }
+      } else {
+        const F4 cqw = s.cqw.ld<F4>(o), pc = s.p.ld<F4>(o);
+        const float rdn = s.rdn[X1(d, k)];
+        F4 cq1v;
+#pragma unroll
+        for (int n = 0; n < 4; n++) {
+          const float cq1 = 1.f / (1.f + cqw.v[n]);
+          const float cq2 = cqw.v[n] * cq1;
+          cq1v.v[n] = cq1;
+          t.v[n] = t.v[n] + (1.f / msfty.v[n]) * K_G *
+                                (cq1 * rdn * (pc.v[n] - pm.v[n]) - (c1f * mu.v[n]) - cq2 * (c1f * mub.v[n] + c2f));
+        }
+        st4m(s.cqw, o, cq1v, i0, ids, ide - 1);
+      }
+    }
+    // w_damp
+    if (a.c.w_damping == 1 && k <= kde - 1) {
+      const float w_damp_on = a.c.zadvect_implicit > 0 ? a.c.w_crit_cfl : K_W_BETA;
+      const F4 mut = s.mut.ld<F4>(o2), wwc = s.ww.ld<F4>(o), wc = w.ld<F4>(o);
+      const float c1f = s.c1f[X1(d, k)], c2f = s.c2f[X1(d, k)], rdnw = s.rdnw[X1(d, k)];
+#pragma unroll
+      for (int n = 0; n < 4; n++) {
+        const float cf = c1f * mut.v[n] + c2f;
+        const float vert_cfl = fabsf(wwc.v[n] / cf * rdnw * a.dt);
+        if (vert_cfl > w_damp_on) t.v[n] = t.v[n] - sgn1(wc.v[n]) * K_W_ALPHA * (vert_cfl - a.c.w_crit_cfl) * cf;
+      }
+    }
+    // coriolis and curvature
+    if (k <= ktf) {
+      const float fm = fzm[X1(d, k)], fp = fzp[X1(d, k)];
+      const Off om = lv(k - 1);
+      const F4 e = s.e.ld<F4>(o2), cosa = s.cosa.ld<F4>(o2), sina = s.sina.ld<F4>(o2);
+      const F4 r0 = ru.ld<F4>(o), r0r = ru.ld<F4>(o + 4), rm = ru.ld<F4>(om), rmr = ru.ld<F4>(om + 4);
+      const F4 v0 = rv.ld<F4>(o), v0n = rv.ld<F4>(o + sj), vm = rv.ld<F4>(om), vmn = rv.ld<F4>(om + sj);
+      const F4 u0 = s.u_2.ld<F4>(o), u0r = s.u_2.ld<F4>(o + 4), um = s.u_2.ld<F4>(om), umr = s.u_2.ld<F4>(om + 4);
+      const F4 q0 = s.v_2.ld<F4>(o), q0n = s.v_2.ld<F4>(o + sj), qm = s.v_2.ld<F4>(om), qmn = s.v_2.ld<F4>(om + sj);
+#pragma unroll
+      for (int n = 0; n < 4; n++) {
+        const float rus = fm * (r0.v[n] + xp1(r0, r0r, n)) + fp * (rm.v[n] + xp1(rm, rmr, n));
+        const float rvs = fm * (v0.v[n] + v0n.v[n]) + fp * (vm.v[n] + vmn.v[n]);
+        const float mr = msftx.v[n] / msfty.v[n];
+        t.v[n] = t.v[n] + e.v[n] * (cosa.v[n] * 0.5f * rus - mr * sina.v[n] * 0.5f * rvs);
+        const float uus = fm * (u0.v[n] + xp1(u0, u0r, n)) + fp * (um.v[n] + xp1(um, umr, n));
+        const float vvs = fm * (q0.v[n] + q0n.v[n]) + fp * (qm.v[n] + qmn.v[n]);
+        t.v[n] = t.v[n] + K_RERADIUS * (0.5f * rus * 0.5f * uus + mr * 0.5f * rvs * 0.5f * vvs);
+      }
+    }
+  }
+  // rk_addtend_dry (rw part)
+  {
+    F4 tf = s.rw_tendf.ld<F4>(o);
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+      if (rk1) tf.v[n] = tf.v[n] + 0.f * msfty.v[n];  // w_save was zeroed by rk_tendency
+      t.v[n] = t.v[n] + tf.v[n] / msfty.v[n];
+    }
+    if (rk1) st4m(s.rw_tendf, o, tf, i0, ids, ide - 1);
+  }
+#pragma unroll
+  for (int n = 0; n < 4; n++)
+    if (i0 + n > ide - 1) t.v[n] = 0.f;
+  st4m(s.rw_tend, o, t, i0, ids, ide);
+}
+
 // ------------------------------------------------------------------------------ t, ph, zeroing
 template <typename ARGS, bool PER>
 __global__ void __launch_bounds__(256) k_tend_t(ARGS args) {
@@ -1259,6 +1450,7 @@ WRFMI_INST_P(k_tend_v, true)
 WRFMI_INST(k_tend_v_x4)
 WRFMI_INST_P(k_tend_w, false)
 WRFMI_INST_P(k_tend_w, true)
+WRFMI_INST(k_tend_w_x4)
 WRFMI_INST_P(k_tend_t, false)
 WRFMI_INST_P(k_tend_t, true)
 
