@@ -112,6 +112,134 @@ __global__ void __launch_bounds__(256) k_step_prep(ARGS args) {
   }
 }
 
+// k_step_prep with four i points per lane (16-byte accesses; element by element the
+// expressions of k_step_prep).  The lanes start one group of four below the first group
+// holding ids, so that mut is also formed at i = ids-1.
+template <typename ARGS>
+__global__ void __launch_bounds__(256) k_step_prep_x4(ARGS args) {
+  Tile3 T;
+  const KArgs &a = kargs(args, T);
+  const Dm &d = a.d;
+  const DState &s = dstate(a);
+  const int i0 = vec_ia0(d) - 4 + kVecW * (T.bx * 64 + (int)threadIdx.x);
+  const int j = d.jds - 1 + (int)(T.by * blockDim.y + threadIdx.y);
+  const int k = d.kds + (int)T.bz;
+  const int ids = d.ids, ide = d.ide, jds = d.jds, jde = d.jde, kde = d.kde;
+  if (i0 > ide || j > jde || k > kde) return;
+  const bool mj = j <= jde - 1;
+  const bool top = k == d.kds;
+  const Off o2 = X2(d, i0, j), o = X3(d, i0, k, j);
+  const F4 mub = s.mub.ld<F4>(o2), mu = s.mu_2.ld<F4>(o2);
+  F4 mut;
+#pragma unroll
+  for (int n = 0; n < 4; n++) mut.v[n] = mub.v[n] + mu.v[n];
+  if (top && mj) st4m(s.mut, o2, mut, i0, ids - 1, ide - 1);
+  if (j < jds || i0 + 3 < ids) return;
+  const int ktf = kde - 1;
+  const int nm = s.n_moist;
+  const int kc = imin(k, ktf);
+  const float c1 = s.c1h[X1(d, kc)], c2 = s.c2h[X1(d, kc)], c1f = s.c1f[X1(d, k)], c2f = s.c2f[X1(d, k)];
+  const bool px = a.c.periodic_x, py = a.c.periodic_y;
+  if (mj) {
+    // prep_muu at the u points
+    const F4 mul = s.mu_2.ld<F4>(o2 - 4), mubl = s.mub.ld<F4>(o2 - 4);
+    F4 muu;
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+      const int i = i0 + n;
+      const float m0 = mu.v[n], ml = xm1(mul, mu, n), b0 = mub.v[n], bl = xm1(mubl, mub, n);
+      if (i > ids && i < ide) muu.v[n] = 0.5f * (((m0 + ml) + b0) + bl);
+      else if (i == ids) muu.v[n] = 0.5f * (((m0 + (px ? ml : m0)) + b0) + (px ? bl : b0));
+      else muu.v[n] = 0.5f * (((ml + (px ? m0 : ml)) + bl) + (px ? b0 : bl));
+    }
+    if (top) st4m(s.muu, o2, muu, i0, ids, ide);
+    if (k <= ktf) {
+      const F4 u = s.u_2.ld<F4>(o), msfuy = s.msfuy.ld<F4>(o2);
+      F4 qtot, ru;
+#pragma unroll
+      for (int n = 0; n < 4; n++) qtot.v[n] = 0.f;
+      for (int m = 0; m < nm; m++) {
+        const F4 qc = s.q[m].ld<F4>(o), ql = s.q[m].ld<F4>(o - 4);
+#pragma unroll
+        for (int n = 0; n < 4; n++) qtot.v[n] = qtot.v[n] + qc.v[n] + xm1(ql, qc, n);
+      }
+      F4 cq;
+#pragma unroll
+      for (int n = 0; n < 4; n++) {
+        ru.v[n] = u.v[n] * (c1 * muu.v[n] + c2) / msfuy.v[n];
+        cq.v[n] = nm >= 1 ? 1.f / (1.f + 0.5f * qtot.v[n]) : 1.f;
+      }
+      st4m(s.ru, o, ru, i0, ids, ide);
+      st4m(s.cqu, o, cq, i0, ids, ide);
+    }
+  }
+  {
+    // prep_muv at the v points (i <= ide-1)
+    const F4 mus = s.mu_2.ld<F4>(o2 - d.ni), mubs = s.mub.ld<F4>(o2 - d.ni);
+    F4 muv;
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+      const float m0 = mu.v[n], ms = mus.v[n], b0 = mub.v[n], bs = mubs.v[n];
+      if (j > jds && j < jde) muv.v[n] = 0.5f * (((m0 + ms) + b0) + bs);
+      else if (j == jds) muv.v[n] = 0.5f * (((m0 + (py ? ms : m0)) + b0) + (py ? bs : b0));
+      else muv.v[n] = 0.5f * (((ms + (py ? m0 : ms)) + bs) + (py ? b0 : bs));
+    }
+    if (top) st4m(s.muv, o2, muv, i0, ids, ide - 1);
+    if (k <= ktf) {
+      const F4 v = s.v_2.ld<F4>(o), mvi = s.msfvx_inv.ld<F4>(o2);
+      F4 qtot, rv;
+#pragma unroll
+      for (int n = 0; n < 4; n++) qtot.v[n] = 0.f;
+      for (int m = 0; m < nm; m++) {
+        const F4 qc = s.q[m].ld<F4>(o), qs = s.q[m].ld<F4>(o - d.sj);
+#pragma unroll
+        for (int n = 0; n < 4; n++) qtot.v[n] = qtot.v[n] + qc.v[n] + qs.v[n];
+      }
+      F4 cq;
+#pragma unroll
+      for (int n = 0; n < 4; n++) {
+        rv.v[n] = v.v[n] * (c1 * muv.v[n] + c2) * mvi.v[n];
+        cq.v[n] = nm >= 1 ? 1.f / (1.f + 0.5f * qtot.v[n]) : 1.f;
+      }
+      st4m(s.rv, o, rv, i0, ids, ide - 1);
+      st4m(s.cqv, o, cq, i0, ids, ide - 1);
+    }
+  }
+  if (!mj) return;
+  if (k >= 2 && k <= ktf) {
+    F4 qtot, cw;
+#pragma unroll
+    for (int n = 0; n < 4; n++) qtot.v[n] = 0.f;
+    for (int m = 0; m < nm; m++) {
+      const F4 qc = s.q[m].ld<F4>(o), qb = s.q[m].ld<F4>(o - d.ni);
+#pragma unroll
+      for (int n = 0; n < 4; n++) qtot.v[n] = qtot.v[n] + qc.v[n] + qb.v[n];
+    }
+#pragma unroll
+    for (int n = 0; n < 4; n++) cw.v[n] = nm >= 1 ? 0.5f * qtot.v[n] : 0.f;
+    st4m(s.cqw, o, cw, i0, ids, ide - 1);
+  }
+  if (k <= ktf) {
+    const F4 al = s.al.ld<F4>(o), alb = s.alb.ld<F4>(o), phb = s.phb.ld<F4>(o), phbp = s.phb.ld<F4>(o + d.ni);
+    const F4 ph = s.ph_2.ld<F4>(o), php = s.ph_2.ld<F4>(o + d.ni);
+    F4 alt, phpn;
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+      alt.v[n] = al.v[n] + alb.v[n];
+      phpn.v[n] = 0.5f * (((phb.v[n] + phbp.v[n]) + ph.v[n]) + php.v[n]);
+    }
+    st4m(s.alt, o, alt, i0, ids, ide - 1);
+    st4m(s.php, o, phpn, i0, ids, ide - 1);
+  }
+  {
+    const F4 w = s.w_2.ld<F4>(o), msfty = s.msfty.ld<F4>(o2);
+    F4 rw;
+#pragma unroll
+    for (int n = 0; n < 4; n++) rw.v[n] = w.v[n] * (c1f * mut.v[n] + c2f) / msfty.v[n];
+    st4m(s.rw, o, rw, i0, ids, ide - 1);
+  }
+}
+
 // calc_ww_cp (module_big_step_utilities_em.F:640-782): column divergence of the coupled
 // horizontal momentum and the upward scan for omega; one thread per (i, j) column.
 template <typename ARGS, bool COMP>
@@ -170,5 +298,6 @@ WRFMI_INST_P(k_ww_scan, false)
 WRFMI_INST_P(k_ww_scan, true)
 
 WRFMI_INST(k_step_prep)
+WRFMI_INST(k_step_prep_x4)
 
 }  // namespace wrfmi

@@ -292,6 +292,64 @@ __global__ void __launch_bounds__(256) k_update_scalar_x4(ARGS args) {
   st4m(sc2, o, qn, i0, d.ids, d.ide - 1);
 }
 
+// calc_p_rho_phi with four i points per lane (element by element as k_p_rho_phi)
+template <typename ARGS>
+__global__ void __launch_bounds__(256) k_p_rho_phi_x4(ARGS args) {
+  Tile3 T;
+  const KArgs &a = kargs(args, T);
+  const Dm &d = a.d;
+  const DState &s = dstate(a);
+  const int i0 = vec_ia0(d) + kVecW * (T.bx * 64 + (int)threadIdx.x);
+  const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
+  const int k = d.kds + (int)T.bz;
+  if (i0 > d.ide - 1 || j > d.jde - 1 || k > d.kde - 1) return;
+  const Off o = X3(d, i0, k, j), o2 = X2(d, i0, j);
+  const F4 muts = s.muts.ld<F4>(o2), mu = s.mu_2.ld<F4>(o2);
+  const F4 php = s.ph_2.ld<F4>(o + d.ni), ph = s.ph_2.ld<F4>(o), alb = s.alb.ld<F4>(o);
+  const F4 pbp = s.phb.ld<F4>(o + d.ni), pb0 = s.phb.ld<F4>(o), t = s.t_2.ld<F4>(o), pbs = s.pb.ld<F4>(o);
+  const bool hyp1 = a.c.hypsometric_opt == 1;
+  const float c1 = s.c1h[X1(d, k)], c2 = s.c2h[X1(d, k)], rdnw = s.rdnw[X1(d, k)];
+  const float c3fp = s.c3f[X1(d, k + 1)], c4fp = s.c4f[X1(d, k + 1)], c3f = s.c3f[X1(d, k)], c4f = s.c4f[X1(d, k)];
+  const float c3h = s.c3h[X1(d, k)], c4h = s.c4h[X1(d, k)], ptop = s.p_top;
+  const bool moist = s.n_moist >= 1, thm = a.c.use_theta_m == 1;
+  F4 qv;
+  if (moist && !thm) qv = s.q[0].ld<F4>(o);
+  else
+#pragma unroll
+    for (int n = 0; n < 4; n++) qv.v[n] = 0.f;
+  F4 aln, pn;
+#pragma unroll
+  for (int n = 0; n < 4; n++) {
+    float al;
+    if (hyp1) {
+      al = -1.f / (c1 * muts.v[n] + c2) * (alb.v[n] * (c1 * mu.v[n]) + rdnw * (php.v[n] - ph.v[n]));
+    } else {
+      const float pfu = c3fp * muts.v[n] + c4fp + ptop;
+      const float pfd = c3f * muts.v[n] + c4f + ptop;
+      const float phm = c3h * muts.v[n] + c4h + ptop;
+      const float lg = (float)log((double)(pfd / pfu));
+      al = (php.v[n] - ph.v[n] + pbp.v[n] - pb0.v[n]) / phm / lg - alb.v[n];
+    }
+    aln.v[n] = al;
+    if (moist) {
+      float temp;
+      if (thm) {
+        temp = (K_RD * (K_T0 + t.v[n])) / (K_P0 * (al + alb.v[n]));
+      } else {
+        const float qvf = 1.f + K_RVOVRD * qv.v[n];
+        temp = (K_RD * (K_T0 + t.v[n]) * qvf) / (K_P0 * (al + alb.v[n]));
+      }
+      const float pw = (float)pow((double)temp, (double)K_CPOVCV);
+      pn.v[n] = pw * K_P0 - pbs.v[n];
+    } else {
+      const float pw = (float)pow((double)((K_RD * (K_T0 + t.v[n])) / (K_P0 * (al + alb.v[n]))), (double)K_CPOVCV);
+      pn.v[n] = K_P0 * pw - pbs.v[n];
+    }
+  }
+  st4m(s.al, o, aln, i0, d.ids, d.ide - 1);
+  st4m(s.p, o, pn, i0, d.ids, d.ide - 1);
+}
+
 // calc_p_rho_phi, non-hydrostatic; logarithm and power in FP64 then rounded once to FP32
 template <typename ARGS>
 __global__ void __launch_bounds__(256) k_p_rho_phi(ARGS args) {
@@ -435,6 +493,7 @@ WRFMI_INST(k_adv_scalar_x4)
 WRFMI_INST(k_update_scalar)
 WRFMI_INST(k_update_scalar_x4)
 WRFMI_INST(k_p_rho_phi)
+WRFMI_INST(k_p_rho_phi_x4)
 template __global__ void k_halo<HaloArgs>(HaloArgs);
 template __global__ void k_halo<BatchHalo>(BatchHalo);
 

@@ -692,6 +692,95 @@ __global__ void __launch_bounds__(256) k_ac_wpred(ARGS args) {
   w[o] = wv;
 }
 
+// calc_mu_uv_1 + small_step_finish with four i points per lane (16-byte accesses; element by
+// element the expressions of k_small_finish)
+template <typename ARGS>
+__global__ void __launch_bounds__(256) k_small_finish_x4(ARGS args) {
+  Tile3 T;
+  const KArgs &a = kargs(args, T);
+  const Dm &d = a.d;
+  const DState &s = dstate(a);
+  const int i0 = vec_ia0(d) + kVecW * (T.bx * 64 + (int)threadIdx.x);
+  const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
+  const int k = d.kds + (int)T.bz;
+  const int ids = d.ids, ide = d.ide, jds = d.jds, jde = d.jde, kde = d.kde;
+  if (i0 > ide || j > jde || k > kde) return;
+  const bool mj = j <= jde - 1, kin = k <= kde - 1;
+  const int kc = imin(k, kde - 1);
+  const float c1 = s.c1h[X1(d, kc)], c2 = s.c2h[X1(d, kc)], c1f = s.c1f[X1(d, k)], c2f = s.c2f[X1(d, k)];
+  const Off o2 = X2(d, i0, j), o = X3(d, i0, k, j);
+  const bool top = k == d.kds;
+  const F4 mts = s.muts.ld<F4>(o2), mtsl = s.muts.ld<F4>(o2 - 4), mtss = s.muts.ld<F4>(o2 - d.ni);
+  const bool px = a.c.periodic_x, py = a.c.periodic_y;
+  F4 muus, muvs;
+#pragma unroll
+  for (int n = 0; n < 4; n++) {
+    const int i = i0 + n;
+    const float c = mts.v[n], l = xm1(mtsl, mts, n);
+    if (i > ids && i < ide) muus.v[n] = 0.5f * (c + l);
+    else if (i == ids) muus.v[n] = 0.5f * (c + (px ? l : c));
+    else muus.v[n] = 0.5f * (l + (px ? c : l));
+    const float sv = mtss.v[n];
+    if (j > jds && j < jde) muvs.v[n] = 0.5f * (c + sv);
+    else if (j == jds) muvs.v[n] = 0.5f * (c + (py ? sv : c));
+    else muvs.v[n] = 0.5f * (sv + (py ? c : sv));
+  }
+  if (top) {
+    if (mj) st4m(s.muus, o2, muus, i0, ids, ide);
+    st4m(s.muvs, o2, muvs, i0, ids, ide - 1);
+  }
+  if (kin) {
+    const F4 muv = s.muv.ld<F4>(o2), msfvx = s.msfvx.ld<F4>(o2), v2 = s.v_2.ld<F4>(o), vs = s.v_save.ld<F4>(o);
+    F4 vn;
+#pragma unroll
+    for (int n = 0; n < 4; n++) vn.v[n] = (msfvx.v[n] * v2.v[n] + vs.v[n] * (c1 * muv.v[n] + c2)) / (c1 * muvs.v[n] + c2);
+    st4m(s.v_2, o, vn, i0, ids, ide - 1);
+  }
+  if (mj && kin) {
+    const F4 muu = s.muu.ld<F4>(o2), msfuy = s.msfuy.ld<F4>(o2), u2 = s.u_2.ld<F4>(o), us = s.u_save.ld<F4>(o);
+    F4 un;
+#pragma unroll
+    for (int n = 0; n < 4; n++) un.v[n] = (msfuy.v[n] * u2.v[n] + us.v[n] * (c1 * muu.v[n] + c2)) / (c1 * muus.v[n] + c2);
+    st4m(s.u_2, o, un, i0, ids, ide);
+  }
+  if (!mj || i0 > ide - 1) return;
+  const F4 mut = s.mut.ld<F4>(o2), msfty = s.msfty.ld<F4>(o2);
+  {
+    const F4 w2 = s.w_2.ld<F4>(o), ws = s.w_save.ld<F4>(o), ph2 = s.ph_2.ld<F4>(o), phs = s.ph_save.ld<F4>(o);
+    const F4 ww = s.ww.ld<F4>(o), ww1 = s.ww1.ld<F4>(o);
+    F4 wn, phn, wwn;
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+      wn.v[n] = (msfty.v[n] * w2.v[n] + ws.v[n] * (c1f * mut.v[n] + c2f)) / (c1f * mts.v[n] + c2f);
+      phn.v[n] = ph2.v[n] + phs.v[n];
+      wwn.v[n] = ww.v[n] + ww1.v[n];
+    }
+    st4m(s.w_2, o, wn, i0, ids, ide - 1);
+    st4m(s.ph_2, o, phn, i0, ids, ide - 1);
+    st4m(s.ww, o, wwn, i0, ids, ide - 1);
+  }
+  if (kin) {
+    const bool last = a.rk_step >= a.c.rk_ord;
+    const F4 t2 = s.t_2.ld<F4>(o), tsv = s.t_save.ld<F4>(o), hd = s.h_diabatic.ld<F4>(o);
+    F4 tn;
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+      const float cm = c1 * mut.v[n] + c2;
+      const float nl = (t2.v[n] + tsv.v[n] * cm) / (c1 * mts.v[n] + c2);
+      const float lt = (t2.v[n] - a.dts * (float)a.nsmall * cm * hd.v[n] + tsv.v[n] * cm) / (c1 * mts.v[n] + c2);
+      tn.v[n] = last ? lt : nl;
+    }
+    st4m(s.t_2, o, tn, i0, ids, ide - 1);
+  }
+  if (top) {
+    const F4 mu2 = s.mu_2.ld<F4>(o2), mus = s.mu_save.ld<F4>(o2);
+    F4 mn;
+#pragma unroll
+    for (int n = 0; n < 4; n++) mn.v[n] = mu2.v[n] + mus.v[n];
+    st4m(s.mu_2, o2, mn, i0, ids, ide - 1);
+  }
+}
+
 // ---------------------------------------------------------------------------------------
 // Four-points-per-lane forms of small_step_prep and calc_p_rho step 0 (16-byte accesses;
 // element by element the expressions of k_small_prep and k_p_rho0).
@@ -1245,5 +1334,6 @@ WRFMI_INST(k_p_rho0_x4)
 WRFMI_INST(k_ac_theta_x4)
 WRFMI_INST(k_ac_wpred_x4)
 WRFMI_INST(k_small_finish)
+WRFMI_INST(k_small_finish_x4)
 
 }  // namespace wrfmi

@@ -15,6 +15,7 @@ constexpr int kHaloK = 8;      // levels per thread in k_halo
 // Every state kernel is a template on its argument type: KArgs for one domain, BatchArgs
 // for an ensemble launch over the tiles of all members (k_halo: HaloArgs / BatchHalo).
 template <typename A> __global__ void k_step_prep(A);
+template <typename A> __global__ void k_step_prep_x4(A);
 template <typename A, bool COMP> __global__ void k_ww_scan(A);
 template <typename A, bool PER> __global__ void k_tend_u(A);
 template <typename A> __global__ void k_tend_u_x4(A);
@@ -39,11 +40,13 @@ template <typename A> __global__ void k_ac_theta_x4(A);
 template <typename A> __global__ void k_ac_wpred_x4(A);
 template <typename A, typename CT> __global__ void k_ac_wsolve(A);
 template <typename A> __global__ void k_small_finish(A);
+template <typename A> __global__ void k_small_finish_x4(A);
 template <typename A, bool PER> __global__ void k_adv_scalar(A);
 template <typename A> __global__ void k_adv_scalar_x4(A);
 template <typename A> __global__ void k_update_scalar(A);
 template <typename A> __global__ void k_update_scalar_x4(A);
 template <typename A> __global__ void k_p_rho_phi(A);
+template <typename A> __global__ void k_p_rho_phi_x4(A);
 template <typename A> __global__ void k_halo(A);
 __global__ void k_count_nonfinite(const float *a, Dm d, int kend, int is3d, unsigned long long *cnt);
 }  // namespace wrfmi

@@ -376,7 +376,14 @@ static void ph_step_prep(Launcher &L) {
   const KArgs &a = L.a;
   const wrfmi_state &s = a.s;
   const dim3 gp = g_prep(a.d);
-  L.launchK(KID_STEP_PREP, KR(k_step_prep), dim3(gp.x, gp.y, a.d.kde - a.d.kds + 1), kColBlk, a);
+  if (L.vec) {
+    // one extra group of four below ids (mut at ids-1), rows from jds-1
+    const int ia0 = a.d.ims + 4 * ((a.d.ids - a.d.ims) >> 2) - 4;
+    L.launchK(KID_STEP_PREP, KR(k_step_prep_x4),
+              dim3((a.d.ide - ia0 + kVecTileI) / kVecTileI, (a.d.jde - a.d.jds + 1 + 4) / 4, a.d.kde - a.d.kds + 1), kColBlk, a);
+  } else {
+    L.launchK(KID_STEP_PREP, KR(k_step_prep), dim3(gp.x, gp.y, a.d.kde - a.d.kds + 1), kColBlk, a);
+  }
   if (a.c.precision_mode == 1) L.launchK(KID_WW_SCAN, KRP(k_ww_scan, true), g_col(a.d), kColBlk, a);
   else L.launchK(KID_WW_SCAN, KRP(k_ww_scan, false), g_col(a.d), kColBlk, a);
   L.halo_begin();
@@ -452,7 +459,8 @@ static void ph_advance_mu_t_w(Launcher &L) {
 static void ph_small_finish(Launcher &L) {
   const KArgs &a = L.a;
   const wrfmi_state &s = a.s;
-  L.launchK(KID_SMALL_FINISH, KR(k_small_finish), g_pt3(a.d, a.d.kde - a.d.kds + 1), kColBlk, a);
+  if (L.vec) L.launchK(KID_SMALL_FINISH, KR(k_small_finish_x4), g_vec3(a.d, a.d.kde - a.d.kds + 1), kColBlk, a);
+  else L.launchK(KID_SMALL_FINISH, KR(k_small_finish), g_pt3(a.d, a.d.kde - a.d.kds + 1), kColBlk, a);
   if (a.rk_step == 3) {
     L.halo_begin();
     L.halo_add(s.ru_m, 'u', 1); L.halo_add(s.rv_m, 'v', 1); L.halo_add(s.ww_m, 'w', 1);
@@ -477,7 +485,8 @@ static void ph_scalar(Launcher &L) {
   else L.launchK(KID_UPDATE_SCALAR, KR(k_update_scalar), g_pt3(L.a.d, L.a.d.kde - L.a.d.kds), kColBlk, L.a);
 }
 static void ph_p_rho_phi(Launcher &L) {
-  L.launchK(KID_P_RHO_PHI, KR(k_p_rho_phi), g_pt3(L.a.d, L.a.d.kde - L.a.d.kds), kColBlk, L.a);
+  if (L.vec) L.launchK(KID_P_RHO_PHI, KR(k_p_rho_phi_x4), g_vec3(L.a.d, L.a.d.kde - L.a.d.kds), kColBlk, L.a);
+  else L.launchK(KID_P_RHO_PHI, KR(k_p_rho_phi), g_pt3(L.a.d, L.a.d.kde - L.a.d.kds), kColBlk, L.a);
 }
 static void ph_bc_dry_2(Launcher &L, int kind) {
   const wrfmi_state &s = L.a.s;
