@@ -1442,6 +1442,222 @@ __global__ void __launch_bounds__(256) k_tend_t(ARGS args) {
   }
 }
 
+// k_tend_t for the doubly periodic case with four i points per lane, one level per thread:
+// zero_tend of the *_save arrays, advect_scalar(theta) + rk_addtend_dry into t_tend, rhs_ph
+// (order-6 horizontal form) + rk_addtend_dry into ph_tend, and mu_tend/mu_save at level
+// kts.  Element by element the terms of k_tend_t with PER = true, in its order.
+template <typename ARGS>
+__global__ void __launch_bounds__(256) k_tend_t_x4(ARGS args) {
+  Tile3 T;
+  const KArgs &a = kargs(args, T);
+  const Dm &d = a.d;
+  const DState &s = dstate(a);
+  const int i0 = vec_ia0(d) + kVecW * (T.bx * 64 + (int)threadIdx.x);
+  const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
+  const int k = d.kds + (int)T.bz;
+  const int ids = d.ids, ide = d.ide, jde = d.jde, kde = d.kde, kts = d.kds, kte = kde, ktf = kte - 1;
+  if (i0 > ide || j > jde || k > kte) return;
+  const float ts = a.c.time_step >= 0 ? 1.f : -1.f;
+  const float rdx = s.rdx, rdy = s.rdy;
+  const int ni = d.ni, sj = d.sj;
+  const FPtr<float> f = s.t_2, ru = s.ru, rv = s.rv, rom = s.ww, ph = s.ph_2, phb = s.phb;
+  const Off o2 = X2(d, i0, j), ob = X3(d, i0, 0, j);
+  auto lv = [&](int kk) { return ob + imax(kts, imin(kk, kte)) * ni; };
+  const Off o = lv(k);
+  const bool rk1 = a.rk_step == 1;
+  F4 z;
+#pragma unroll
+  for (int n = 0; n < 4; n++) z.v[n] = 0.f;
+  // zero_tend of the *_save arrays over the whole tile
+  st4m(s.u_save, o, z, i0, ids, ide);
+  st4m(s.v_save, o, z, i0, ids, ide);
+  st4m(s.w_save, o, z, i0, ids, ide);
+  st4m(s.ph_save, o, z, i0, ids, ide);
+  st4m(s.t_save, o, z, i0, ids, ide);
+  if (k == kts) {
+    st4m(s.mu_tend, o2, z, i0, ids, ide);
+    st4m(s.mu_save, o2, z, i0, ids, ide);
+  }
+  F4 t = z, pt = z;
+  const bool jmass = j <= jde - 1;
+  if (jmass && i0 <= ide - 1) {
+    const F4 msftx = s.msftx.ld<F4>(o2), msfty = s.msfty.ld<F4>(o2), mut = s.mut.ld<F4>(o2);
+    if (k <= ktf) {
+      // advect_scalar(t): y faces j and j+1
+      {
+        const F4 ym3 = f.ld<F4>(o - 3 * sj), ym2 = f.ld<F4>(o - 2 * sj), ym1 = f.ld<F4>(o - sj), y0 = f.ld<F4>(o);
+        const F4 yp1 = f.ld<F4>(o + sj), yp2 = f.ld<F4>(o + 2 * sj), yp3 = f.ld<F4>(o + 3 * sj);
+        const F4 rv0 = rv.ld<F4>(o), rv1 = rv.ld<F4>(o + sj);
+#pragma unroll
+        for (int n = 0; n < 4; n++) {
+          const float f0 = rv0.v[n] * flux5(ym3.v[n], ym2.v[n], ym1.v[n], y0.v[n], yp1.v[n], yp2.v[n], rv0.v[n], ts);
+          const float f1 = rv1.v[n] * flux5(ym2.v[n], ym1.v[n], y0.v[n], yp1.v[n], yp2.v[n], yp3.v[n], rv1.v[n], ts);
+          const float mrdy = msftx.v[n] * rdy;
+          t.v[n] = t.v[n] - mrdy * (f1 - f0);
+        }
+      }
+      // x faces i0 .. i0+4
+      {
+        const F4 wl = f.ld<F4>(o - 4), wc = f.ld<F4>(o), wr = f.ld<F4>(o + 4);
+        const F4 rc = ru.ld<F4>(o), rr = ru.ld<F4>(o + 4);
+        float W[12];
+#pragma unroll
+        for (int n = 0; n < 4; n++) {
+          W[n] = wl.v[n];
+          W[n + 4] = wc.v[n];
+          W[n + 8] = wr.v[n];
+        }
+        float fx[5];
+#pragma unroll
+        for (int n = 0; n < 5; n++) {
+          const float vel = n < 4 ? rc.v[n] : rr.v[0];
+          fx[n] = vel * flux5(W[n + 1], W[n + 2], W[n + 3], W[n + 4], W[n + 5], W[n + 6], vel, ts);
+        }
+#pragma unroll
+        for (int n = 0; n < 4; n++) {
+          const float mrdx = msftx.v[n] * rdx;
+          t.v[n] = t.v[n] - mrdx * (fx[n + 1] - fx[n]);
+        }
+      }
+      // vertical
+      {
+        const F4 qm2 = f.ld<F4>(lv(k - 2)), qm1 = f.ld<F4>(lv(k - 1)), q0 = f.ld<F4>(o), qp1 = f.ld<F4>(lv(k + 1)),
+                 qp2 = f.ld<F4>(lv(k + 2));
+        const F4 w0 = rom.ld<F4>(o), w1 = rom.ld<F4>(lv(k + 1));
+        auto vflux = [&](int kk, float w_, float qm2_, float qm1_, float q0_, float qp1_) {
+          const int kc = imin(kk, kte);
+          const float lo = w_ * (s.fnm[X1(d, kc)] * q0_ + s.fnp[X1(d, kc)] * qm1_);
+          const float hi = w_ * flux3(qm2_, qm1_, q0_, qp1_, -w_, ts);
+          return (kk == kts || kk == kte) ? 0.f : ((kk == kts + 1 || kk == ktf) ? lo : hi);
+        };
+        const float rdnw = s.rdnw[X1(d, k)];
+#pragma unroll
+        for (int n = 0; n < 4; n++) {
+          const float vf_c = vflux(k, w0.v[n], qm2.v[n], qm1.v[n], q0.v[n], qp1.v[n]);
+          const float vf_hi = vflux(k + 1, w1.v[n], qm1.v[n], q0.v[n], qp1.v[n], qp2.v[n]);
+          t.v[n] = t.v[n] - rdnw * (vf_hi - vf_c);
+        }
+      }
+      // rk_addtend_dry (theta)
+      {
+        F4 tf = s.t_tendf.ld<F4>(o);
+        const F4 hd = s.h_diabatic.ld<F4>(o);
+        const float c1 = s.c1h[X1(d, k)], c2 = s.c2h[X1(d, k)];
+#pragma unroll
+        for (int n = 0; n < 4; n++) {
+          if (rk1) tf.v[n] = tf.v[n] + 0.f;  // t_save was zeroed by rk_tendency
+          t.v[n] = t.v[n] + tf.v[n] / msfty.v[n] + (c1 * mut.v[n] + c2) * hd.v[n] / msfty.v[n];
+        }
+        if (rk1) st4m(s.t_tendf, o, tf, i0, ids, ide - 1);
+      }
+    }
+    // rhs_ph
+    const bool z2 = a.c.phi_adv_z == 2;
+    if (k >= 2 && k <= kte - 1) {
+      const F4 hm = ph.ld<F4>(lv(k - 1)), h0 = ph.ld<F4>(o), hp = ph.ld<F4>(lv(k + 1));
+      const F4 bm = phb.ld<F4>(lv(k - 1)), b0 = phb.ld<F4>(o), bp = phb.ld<F4>(lv(k + 1));
+      const F4 rm = rom.ld<F4>(lv(k - 1)), r0 = rom.ld<F4>(o), rp = rom.ld<F4>(lv(k + 1));
+      const float rdnw_k = s.rdnw[X1(d, k)], rdnw_m = s.rdnw[X1(d, k - 1)];
+      const float fm = s.fnm[X1(d, k)], fp = s.fnp[X1(d, k)];
+#pragma unroll
+      for (int n = 0; n < 4; n++) {
+        // wdwn at w levels k+1 and k
+        const float dhi = hp.v[n] - h0.v[n] + bp.v[n] - b0.v[n], dlo = h0.v[n] - hm.v[n] + b0.v[n] - bm.v[n];
+        const float wd_hi = z2 ? rdnw_k * dhi : .5f * (rp.v[n] + r0.v[n]) * rdnw_k * dhi;
+        const float wd_lo = z2 ? rdnw_m * dlo : .5f * (r0.v[n] + rm.v[n]) * rdnw_m * dlo;
+        if (z2) pt.v[n] = pt.v[n] - r0.v[n] * (fm * wd_hi + fp * wd_lo);
+        else pt.v[n] = pt.v[n] - (fm * wd_hi + fp * wd_lo);
+      }
+    }
+    if (a.c.non_hydrostatic) {
+      const F4 w = s.w_2.ld<F4>(o);
+      const float c1f = s.c1f[X1(d, k)], c2f = s.c2f[X1(d, k)];
+#pragma unroll
+      for (int n = 0; n < 4; n++) {
+        if (k == kde) pt.v[n] = 0.f;
+        if (k >= 2) pt.v[n] = pt.v[n] + (c1f * mut.v[n] + c2f) * K_G * w.v[n] / msfty.v[n];
+      }
+    }
+    const bool ord6 = a.c.h_sca_adv_order <= 6 && a.c.h_sca_adv_order > 4;
+    if (ord6 && k >= 2) {
+      const float c1f = s.c1f[X1(d, k)], c2f = s.c2f[X1(d, k)], cfn = s.cfn, cfn1 = s.cfn1;
+      const bool below = k < kte;
+      // v and u at the faces: levels k and k-1, or k-1 and k-2 at the lid
+      const Off fa = below ? o : lv(k - 1), fb = below ? lv(k - 1) : lv(k - 2);
+      const float ca = below ? 1.f : cfn, cb = below ? 1.f : cfn1;
+      // y: rows j-3 .. j+3 of ph and phb
+      {
+        const F4 p1 = ph.ld<F4>(o + sj), m1 = ph.ld<F4>(o - sj), p2 = ph.ld<F4>(o + 2 * sj), m2 = ph.ld<F4>(o - 2 * sj);
+        const F4 p3 = ph.ld<F4>(o + 3 * sj), m3 = ph.ld<F4>(o - 3 * sj);
+        const F4 bp1 = phb.ld<F4>(o + sj), bm1 = phb.ld<F4>(o - sj), bp2 = phb.ld<F4>(o + 2 * sj), bm2 = phb.ld<F4>(o - 2 * sj);
+        const F4 bp3 = phb.ld<F4>(o + 3 * sj), bm3 = phb.ld<F4>(o - 3 * sj);
+        const F4 va0 = s.v_2.ld<F4>(fa), vb0 = s.v_2.ld<F4>(fb), va1 = s.v_2.ld<F4>(fa + sj), vb1 = s.v_2.ld<F4>(fb + sj);
+        const F4 muv0 = s.muv.ld<F4>(o2), muv1 = s.muv.ld<F4>(o2 + ni), my0 = s.msfvy.ld<F4>(o2), my1 = s.msfvy.ld<F4>(o2 + ni);
+#pragma unroll
+        for (int n = 0; n < 4; n++) {
+          const float cy = below ? (0.25f * rdy / msfty.v[n]) : (0.5f * rdy / msfty.v[n]);
+          const float S = 45.f * (p1.v[n] - m1.v[n]) - 9.f * (p2.v[n] - m2.v[n]) + (p3.v[n] - m3.v[n]) +
+                          45.f * (bp1.v[n] - bm1.v[n]) - 9.f * (bp2.v[n] - bm2.v[n]) + (bp3.v[n] - bm3.v[n]);
+          const float cc1 = c1f * muv1.v[n] + c2f, cc0 = c1f * muv0.v[n] + c2f;
+          const float vf1 = below ? cc1 * (va1.v[n] + vb1.v[n]) * my1.v[n] : cc1 * (ca * va1.v[n] + cb * vb1.v[n]) * my1.v[n];
+          const float vf0 = below ? cc0 * (va0.v[n] + vb0.v[n]) * my0.v[n] : cc0 * (ca * va0.v[n] + cb * vb0.v[n]) * my0.v[n];
+          pt.v[n] = pt.v[n] - cy * ((vf1 + vf0) * (1.f / 60.f) * S);
+        }
+      }
+      // x: points i-3 .. i+3 from the groups l, c, r
+      {
+        const F4 hl = ph.ld<F4>(o - 4), hc = ph.ld<F4>(o), hr = ph.ld<F4>(o + 4);
+        const F4 bl = phb.ld<F4>(o - 4), bc = phb.ld<F4>(o), br = phb.ld<F4>(o + 4);
+        const F4 ua = s.u_2.ld<F4>(fa), uar = s.u_2.ld<F4>(fa + 4), ub = s.u_2.ld<F4>(fb), ubr = s.u_2.ld<F4>(fb + 4);
+        const F4 muu = s.muu.ld<F4>(o2), muur = s.muu.ld<F4>(o2 + 4), mx = s.msfux.ld<F4>(o2), mxr = s.msfux.ld<F4>(o2 + 4);
+        float H[12], B[12];
+#pragma unroll
+        for (int n = 0; n < 4; n++) {
+          H[n] = hl.v[n]; H[n + 4] = hc.v[n]; H[n + 8] = hr.v[n];
+          B[n] = bl.v[n]; B[n + 4] = bc.v[n]; B[n + 8] = br.v[n];
+        }
+#pragma unroll
+        for (int n = 0; n < 4; n++) {
+          const int c = n + 4;  // index of point i in H/B
+          const float cx = below ? (0.25f * rdx / msfty.v[n]) : (0.5f * rdx / msfty.v[n]);
+          const float S = 45.f * (H[c + 1] - H[c - 1]) - 9.f * (H[c + 2] - H[c - 2]) + (H[c + 3] - H[c - 3]) +
+                          45.f * (B[c + 1] - B[c - 1]) - 9.f * (B[c + 2] - B[c - 2]) + (B[c + 3] - B[c - 3]);
+          const float cc1 = c1f * xp1(muu, muur, n) + c2f, cc0 = c1f * muu.v[n] + c2f;
+          const float uf1 = below ? cc1 * (xp1(ua, uar, n) + xp1(ub, ubr, n)) * xp1(mx, mxr, n)
+                                  : cc1 * (ca * xp1(ua, uar, n) + cb * xp1(ub, ubr, n)) * xp1(mx, mxr, n);
+          const float uf0 = below ? cc0 * (ua.v[n] + ub.v[n]) * mx.v[n] : cc0 * (ca * ua.v[n] + cb * ub.v[n]) * mx.v[n];
+          pt.v[n] = pt.v[n] - cx * ((uf1 + uf0) * (1.f / 60.f) * S);
+        }
+      }
+    }
+    // rk_addtend_dry (ph)
+    {
+      F4 ptf = s.ph_tendf.ld<F4>(o);
+#pragma unroll
+      for (int n = 0; n < 4; n++) {
+        if (rk1) ptf.v[n] = ptf.v[n] + 0.f;  // ph_save was zeroed by rk_tendency
+        pt.v[n] = pt.v[n] + ptf.v[n] / msfty.v[n];
+      }
+      if (rk1) st4m(s.ph_tendf, o, ptf, i0, ids, ide - 1);
+    }
+    if (k == kts) {
+      const F4 mtf = s.mu_tendf.ld<F4>(o2);
+      F4 mt;
+#pragma unroll
+      for (int n = 0; n < 4; n++) mt.v[n] = 0.f + mtf.v[n];
+      st4m(s.mu_tend, o2, mt, i0, ids, ide - 1);
+    }
+  }
+#pragma unroll
+  for (int n = 0; n < 4; n++)
+    if (i0 + n > ide - 1 || !jmass) {
+      t.v[n] = 0.f;
+      pt.v[n] = 0.f;
+    }
+  st4m(s.t_tend, o, t, i0, ids, ide);
+  st4m(s.ph_tend, o, pt, i0, ids, ide);
+}
+
 WRFMI_INST_P(k_tend_u, false)
 WRFMI_INST_P(k_tend_u, true)
 WRFMI_INST(k_tend_u_x4)
@@ -1453,5 +1669,6 @@ WRFMI_INST_P(k_tend_w, true)
 WRFMI_INST(k_tend_w_x4)
 WRFMI_INST_P(k_tend_t, false)
 WRFMI_INST_P(k_tend_t, true)
+WRFMI_INST(k_tend_t_x4)
 
 }  // namespace wrfmi

@@ -404,7 +404,8 @@ static void ph_tendency(Launcher &L) {
   else L.launchK(KID_TEND_V, per ? KRP(k_tend_v, true) : KRP(k_tend_v, false), gt, kColBlk, a);
   if (per && L.vec) L.launchK(KID_TEND_W, KR(k_tend_w_x4), g_vec3(a.d, nk), kColBlk, a);
   else L.launchK(KID_TEND_W, per ? KRP(k_tend_w, true) : KRP(k_tend_w, false), gt, kColBlk, a);
-  L.launchK(KID_TEND_T, per ? KRP(k_tend_t, true) : KRP(k_tend_t, false), gt, kColBlk, a);
+  if (per && L.vec) L.launchK(KID_TEND_T, KR(k_tend_t_x4), g_vec3(a.d, nk), kColBlk, a);
+  else L.launchK(KID_TEND_T, per ? KRP(k_tend_t, true) : KRP(k_tend_t, false), gt, kColBlk, a);
 }
 static void ph_small_prep(Launcher &L) {
   const KArgs &a = L.a;
