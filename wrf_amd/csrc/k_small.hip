@@ -466,9 +466,9 @@ __global__ void __launch_bounds__(256) k_advance_uv_x4(ARGS args) {
 //   k_ac_mu_ww  (column) dmdt, mu_2/mudf/muts/muave, omega scan -> ww
 //   k_ac_theta  (point)  theta update, t_2ave, advance_w's rhs -> t_2, t_2save, scr_rhs;
 //                        sumflux of ww_m
-//   k_ac_wpred  (point)  surface w and the explicit w predictor -> w
-//   k_ac_wsolve (column) tridiagonal forward elimination and back substitution, ph,
-//                        calc_p_rho -> w, ph, al, p, pm1
+//   k_ac_wpred  (point)  surface w at level kds -> w
+//   k_ac_wsolve (column) explicit w predictor (levels 2..kde), tridiagonal forward
+//                        elimination and back substitution, ph, calc_p_rho -> w, ph, al, p, pm1
 // Each point kernel only reads what earlier kernels wrote, so a thread never depends on
 // another thread of its own launch.
 
@@ -633,7 +633,8 @@ __global__ void __launch_bounds__(256) k_ac_theta(ARGS args) {
   a.scr_rhs[o] = r;
 }
 
-// surface w (advance_w 1383-1395) and the explicit w predictor (1402-1431), k = 1..kde
+// surface w (advance_w 1383-1395): one thread per (i, j), level kds only; the explicit
+// predictor of levels 2..kde (1402-1431) is formed inside k_ac_wsolve
 template <typename ARGS>
 __global__ void __launch_bounds__(256) k_ac_wpred(ARGS args) {
   Tile3 T;
@@ -642,16 +643,12 @@ __global__ void __launch_bounds__(256) k_ac_wpred(ARGS args) {
   const DState &s = dstate(a);
   const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
   const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
-  const int k = d.kds + (int)T.bz;
-  const int kde = d.kde, k_end = kde - 1;
-  if (i > d.ide - 1 || j > d.jde - 1 || k > kde) return;
-  const wrfmi_config &c = a.c;
-  if (!c.non_hydrostatic) return;
-  const Off o2 = X2(d, i, j), o = X3(d, i, k, j);
-  const float dts = a.dts, epssm = c.epssm;
+  if (i > d.ide - 1 || j > d.jde - 1) return;
+  if (!a.c.non_hydrostatic) return;
+  const Off o2 = X2(d, i, j), o = X3(d, i, d.kds, j);
   const float msftx = s.msftx[o2], msfty = s.msfty[o2], rdx = s.rdx, rdy = s.rdy;
   FPtr<float> w = s.w_2;
-  if (k == 1) {
+  {
     const FPtr<float> u = s.u_2, v = s.v_2, ht = s.ht;
     const float cf1 = s.cf1, cf2 = s.cf2, cf3 = s.cf3;
     w[o] = msfty * .5f * rdy *
@@ -660,36 +657,7 @@ __global__ void __launch_bounds__(256) k_ac_wpred(ARGS args) {
            msftx * .5f * rdx *
                ((ht[X2(d, i + 1, j)] - ht[o2]) * (cf1 * u[X3(d, i + 1, 1, j)] + cf2 * u[X3(d, i + 1, 2, j)] + cf3 * u[X3(d, i + 1, 3, j)]) +
                 (ht[o2] - ht[X2(d, i - 1, j)]) * (cf1 * u[X3(d, i, 1, j)] + cf2 * u[X3(d, i, 2, j)] + cf3 * u[X3(d, i, 3, j)]));
-    return;
   }
-  const FPtr<float> ph = s.ph_2, rhs = a.scr_rhs, c2a = s.c2a, alt = s.alt, t2ave = s.t_2save;
-  const CPtr<float> c1h = s.c1h, c2h = s.c2h, c1f = s.c1f;
-  const float mut = s.mut[o2], muave = s.muave[o2];
-  const float msft_inv = 1.f / msfty;
-  const float rhs_m1 = k >= 3 ? rhs[o - d.ni] : 0.f;  // rhs(1) = 0
-  const float rhs_0 = rhs[o];
-  const float phl_m1 = ph[o - d.ni], phl_0 = ph[o];
-  const float t2a_lo = t2ave[o - d.ni];
-  float wv;
-  if (k <= k_end) {
-    const float rhs_p1 = rhs[o + d.ni], phl_p1 = ph[o + d.ni], t2a_hi = t2ave[o];
-    wv = w[o] + dts * s.rw_tend[o] +
-         msft_inv * s.cqw[o] *
-             (+.5f * dts * K_G * s.rdn[X1(d, k)] *
-              (c2a[o] * s.rdnw[X1(d, k)] / (c1h[X1(d, k)] * mut + c2h[X1(d, k)]) *
-                   ((1.f + epssm) * (rhs_p1 - rhs_0) + (1.f - epssm) * (phl_p1 - phl_0)) -
-               c2a[o - d.ni] * s.rdnw[X1(d, k - 1)] / (c1h[X1(d, k - 1)] * mut + c2h[X1(d, k - 1)]) *
-                   ((1.f + epssm) * (rhs_0 - rhs_m1) + (1.f - epssm) * (phl_0 - phl_m1)))) +
-         dts * K_G * msft_inv *
-             (s.rdn[X1(d, k)] * (c2a[o] * alt[o] * t2a_hi - c2a[o - d.ni] * alt[o - d.ni] * t2a_lo) - (c1f[X1(d, k)] * muave));
-  } else {
-    wv = w[o] + dts * s.rw_tend[o] +
-         msft_inv * (-.5f * dts * K_G / (c1h[X1(d, k - 1)] * mut + c2h[X1(d, k - 1)]) * (s.rdnw[X1(d, k - 1)] * s.rdnw[X1(d, k - 1)]) * 2.f *
-                         c2a[o - d.ni] * ((1.f + epssm) * (rhs_0 - rhs_m1) + (1.f - epssm) * (phl_0 - phl_m1)) -
-                     dts * K_G * (2.f * s.rdnw[X1(d, k - 1)] * c2a[o - d.ni] * alt[o - d.ni] * t2a_lo + (c1f[X1(d, k)] * muave)));
-    if (c.top_lid) wv = 0.f;
-  }
-  w[o] = wv;
 }
 
 // calc_mu_uv_1 + small_step_finish with four i points per lane (16-byte accesses; element by
@@ -1079,17 +1047,15 @@ __global__ void __launch_bounds__(256) k_ac_wpred_x4(ARGS args) {
   const DState &s = dstate(a);
   const int i0 = vec_ia0(d) + kVecW * (T.bx * 64 + (int)threadIdx.x);
   const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
-  const int k = d.kds + (int)T.bz;
-  const int ids = d.ids, ide = d.ide, jde = d.jde, kde = d.kde, k_end = kde - 1;
-  if (i0 > ide - 1 || j > jde - 1 || k > kde) return;
-  const wrfmi_config &c = a.c;
-  if (!c.non_hydrostatic) return;
-  const Off o2 = X2(d, i0, j), o = X3(d, i0, k, j);
+  const int ids = d.ids, ide = d.ide, jde = d.jde;
+  if (i0 > ide - 1 || j > jde - 1) return;
+  if (!a.c.non_hydrostatic) return;
+  const Off o2 = X2(d, i0, j), o = X3(d, i0, d.kds, j);
   const int ni = d.ni, sj = d.sj;
-  const float dts = a.dts, epssm = c.epssm, rdx = s.rdx, rdy = s.rdy;
+  const float rdx = s.rdx, rdy = s.rdy;
   const F4 msftx = s.msftx.ld<F4>(o2), msfty = s.msfty.ld<F4>(o2);
   FPtr<float> w = s.w_2;
-  if (k == 1) {
+  {
     const FPtr<float> u = s.u_2, v = s.v_2, ht = s.ht;
     const float cf1 = s.cf1, cf2 = s.cf2, cf3 = s.cf3;
     const F4 h0 = ht.ld<F4>(o2), hl = ht.ld<F4>(o2 - 4), hr = ht.ld<F4>(o2 + 4), hn = ht.ld<F4>(o2 + ni), hs = ht.ld<F4>(o2 - ni);
@@ -1108,55 +1074,12 @@ __global__ void __launch_bounds__(256) k_ac_wpred_x4(ARGS args) {
                      (h0.v[n] - xm1(hl, h0, n)) * (cf1 * u1.v[n] + cf2 * u2.v[n] + cf3 * u3.v[n]));
     }
     st4m(w, o, wv, i0, ids, ide - 1);
-    return;
   }
-  const FPtr<float> ph = s.ph_2, rhs = a.scr_rhs, c2a = s.c2a, alt = s.alt, t2ave = s.t_2save;
-  const CPtr<float> c1h = s.c1h, c2h = s.c2h, c1f = s.c1f;
-  const F4 mut = s.mut.ld<F4>(o2), muave = s.muave.ld<F4>(o2);
-  const Off om = o - ni, op = o + (k <= k_end ? ni : 0);
-  const F4 rhsm = rhs.ld<F4>(om), rhs0 = rhs.ld<F4>(o), rhsp = rhs.ld<F4>(op);
-  const F4 phm = ph.ld<F4>(om), ph0 = ph.ld<F4>(o), php_ = ph.ld<F4>(op);
-  const F4 t2lo = t2ave.ld<F4>(om), t2hi = t2ave.ld<F4>(o);
-  const F4 c2a0 = c2a.ld<F4>(o), c2am = c2a.ld<F4>(om), alt0 = alt.ld<F4>(o), altm = alt.ld<F4>(om);
-  const F4 w0 = w.ld<F4>(o), rwt = s.rw_tend.ld<F4>(o), cqw = s.cqw.ld<F4>(o);
-  const int kc = imin(k, k_end);
-  const float rdn = s.rdn[X1(d, kc)], rdnw0 = s.rdnw[X1(d, kc)], rdnwm = s.rdnw[X1(d, k - 1)];
-  const float c1h0 = c1h[X1(d, kc)], c2h0 = c2h[X1(d, kc)], c1hm = c1h[X1(d, k - 1)], c2hm = c2h[X1(d, k - 1)];
-  const float c1fk = c1f[X1(d, k)];
-  F4 wv;
-#pragma unroll
-  for (int n = 0; n < 4; n++) {
-    const float msft_inv = 1.f / msfty.v[n];
-    const float rhs_m1 = k >= 3 ? rhsm.v[n] : 0.f;  // rhs(1) = 0
-    const float rhs_0 = rhs0.v[n];
-    const float phl_m1 = phm.v[n], phl_0 = ph0.v[n];
-    const float t2a_lo = t2lo.v[n];
-    float x;
-    if (k <= k_end) {
-      const float rhs_p1 = rhsp.v[n], phl_p1 = php_.v[n], t2a_hi = t2hi.v[n];
-      x = w0.v[n] + dts * rwt.v[n] +
-          msft_inv * cqw.v[n] *
-              (+.5f * dts * K_G * rdn *
-               (c2a0.v[n] * rdnw0 / (c1h0 * mut.v[n] + c2h0) *
-                    ((1.f + epssm) * (rhs_p1 - rhs_0) + (1.f - epssm) * (phl_p1 - phl_0)) -
-                c2am.v[n] * rdnwm / (c1hm * mut.v[n] + c2hm) *
-                    ((1.f + epssm) * (rhs_0 - rhs_m1) + (1.f - epssm) * (phl_0 - phl_m1)))) +
-          dts * K_G * msft_inv *
-              (rdn * (c2a0.v[n] * alt0.v[n] * t2a_hi - c2am.v[n] * altm.v[n] * t2a_lo) - (c1fk * muave.v[n]));
-    } else {
-      x = w0.v[n] + dts * rwt.v[n] +
-          msft_inv * (-.5f * dts * K_G / (c1hm * mut.v[n] + c2hm) * (rdnwm * rdnwm) * 2.f *
-                          c2am.v[n] * ((1.f + epssm) * (rhs_0 - rhs_m1) + (1.f - epssm) * (phl_0 - phl_m1)) -
-                      dts * K_G * (2.f * rdnwm * c2am.v[n] * altm.v[n] * t2a_lo + (c1fk * muave.v[n])));
-      if (c.top_lid) x = 0.f;
-    }
-    wv.v[n] = x;
-  }
-  st4m(w, o, wv, i0, ids, ide - 1);
 }
 
-// Per column: the tridiagonal forward elimination and back substitution (advance_w
-// 1433-1443), ph from rhs and the final w (1460-1465) and calc_p_rho for step >= 1
+// Per column: the explicit w predictor (advance_w 1402-1431), the tridiagonal forward
+// elimination and back substitution (1433-1443), ph from rhs and the final w (1460-1465)
+// and calc_p_rho for step >= 1
 // (438-568).  CT = double keeps the forward value in FP64, its low-order part parked in scr_lo.
 template <typename ARGS, typename CT>
 __global__ void __launch_bounds__(256) k_ac_wsolve(ARGS args) {
@@ -1176,27 +1099,78 @@ __global__ void __launch_bounds__(256) k_ac_wsolve(ARGS args) {
   const FPtr<CT> A((CT *)s.a), AL((CT *)s.alpha), GA((CT *)s.gamma);
   const Off ob = X3(d, i, 0, j), o2 = X2(d, i, j);
   const int ni = d.ni;
-  CT wfe = (CT)w[ob + ni];
-  // forward elimination: fields 0 = w (predictor), 1 = a, 2 = alpha
-  column_march_up<3, CT>(
+  CT wfe = (CT)w[ob + ni];  // surface w (level 1) from k_ac_wpred
+  // Upward: the explicit w predictor of advance_w (1402-1431) at level k, then the forward
+  // elimination.  Fields: 0 = w, 1 = a, 2 = alpha, 3 = rhs(k+1), 4 = ph(k+1), 5 = t_2ave,
+  // 6 = c2a, 7 = alt, 8 = rw_tend, 9 = cqw (k+1 clamped to kde); rhs, ph at k and k-1 and
+  // t_2ave, c2a, alt at k-1 are carried from the levels below.
+  const float dts = a.dts, epssm = c.epssm;
+  const FPtr<float> ph_c = s.ph_2, t2ave = s.t_2save, c2a = s.c2a, alt_f = s.alt;
+  const float msfty = s.msfty[o2], mut = s.mut[o2], muave = s.muave[o2];
+  const CPtr<float> c1h = s.c1h, c2h = s.c2h, c1f = s.c1f, c2f = s.c2f, rdnw = s.rdnw, rdn = s.rdn;
+  const float msft_inv = 1.f / msfty;
+  float rhs_m1 = 0.f, rhs_0 = rhs[ob + 2 * ni];  // rhs(1) = 0
+  float phl_m1 = ph_c[ob + ni], phl_0 = ph_c[ob + 2 * ni];
+  float t2a_lo = t2ave[ob + ni], c2a_lo = c2a[ob + ni], alt_lo = alt_f[ob + ni];
+  column_march_up<10, CT, 4>(
       2, k_end + 1,
-      [&](int f, int k) -> CT { const Off o = ob + k * ni; return f == 0 ? (CT)w[o] : f == 1 ? A[o] : AL[o]; },
+      [&](int f, int k) -> CT {
+        const Off o = ob + k * ni, op = ob + imin(k + 1, kde) * ni;
+        switch (f) {
+          case 0: return (CT)w[o];
+          case 1: return A[o];
+          case 2: return AL[o];
+          case 3: return (CT)rhs[op];
+          case 4: return (CT)ph_c[op];
+          case 5: return (CT)t2ave[o];
+          case 6: return (CT)c2a[o];
+          case 7: return (CT)alt_f[o];
+          case 8: return (CT)s.rw_tend[o];
+          default: return (CT)s.cqw[o];
+        }
+      },
       [&](int k, const CT *v) {
         const Off o = ob + k * ni;
+        const float w0 = (float)v[0], rhs_p1 = (float)v[3], phl_p1 = (float)v[4], t2a_hi = (float)v[5];
+        const float c2a0 = (float)v[6], alt0 = (float)v[7], rwt = (float)v[8], cqw = (float)v[9];
+        float wv;
+        if (k <= k_end) {
+          wv = w0 + dts * rwt +
+               msft_inv * cqw *
+                   (+.5f * dts * K_G * rdn[X1(d, k)] *
+                    (c2a0 * rdnw[X1(d, k)] / (c1h[X1(d, k)] * mut + c2h[X1(d, k)]) *
+                         ((1.f + epssm) * (rhs_p1 - rhs_0) + (1.f - epssm) * (phl_p1 - phl_0)) -
+                     c2a_lo * rdnw[X1(d, k - 1)] / (c1h[X1(d, k - 1)] * mut + c2h[X1(d, k - 1)]) *
+                         ((1.f + epssm) * (rhs_0 - rhs_m1) + (1.f - epssm) * (phl_0 - phl_m1)))) +
+               dts * K_G * msft_inv *
+                   (rdn[X1(d, k)] * (c2a0 * alt0 * t2a_hi - c2a_lo * alt_lo * t2a_lo) - (c1f[X1(d, k)] * muave));
+        } else {
+          wv = w0 + dts * rwt +
+               msft_inv * (-.5f * dts * K_G / (c1h[X1(d, k - 1)] * mut + c2h[X1(d, k - 1)]) * (rdnw[X1(d, k - 1)] * rdnw[X1(d, k - 1)]) * 2.f *
+                               c2a_lo * ((1.f + epssm) * (rhs_0 - rhs_m1) + (1.f - epssm) * (phl_0 - phl_m1)) -
+                           dts * K_G * (2.f * rdnw[X1(d, k - 1)] * c2a_lo * alt_lo * t2a_lo + (c1f[X1(d, k)] * muave)));
+          if (c.top_lid) wv = 0.f;
+        }
+        rhs_m1 = rhs_0;
+        rhs_0 = rhs_p1;
+        phl_m1 = phl_0;
+        phl_0 = phl_p1;
+        t2a_lo = t2a_hi;
+        c2a_lo = c2a0;
+        alt_lo = alt0;
         if (sizeof(CT) == sizeof(float)) {
-          const float wf = ((float)v[0] - (float)v[1] * (float)wfe) * (float)v[2];
+          const float wf = (wv - (float)v[1] * (float)wfe) * (float)v[2];
           wfe = (CT)wf;
           w[o] = wf;
         } else {
-          wfe = (v[0] - v[1] * wfe) * v[2];
+          wfe = ((CT)wv - v[1] * wfe) * v[2];
           w[o] = (float)wfe;
           lo[o] = (float)((double)wfe - (double)(float)wfe);
         }
       });
   // downward: back substitution, ph, calc_p_rho.  Fields: 0 = forward value of w (plus its
   // low part in FP64), 1 = gamma, 2 = rhs, 3 = t_2, 4 = t_save, 5 = alt, 6 = c2a, 7 = pm1
-  const float dts = a.dts, epssm = c.epssm, msfty = s.msfty[o2], muts = s.muts[o2], mu = s.mu_2[o2];
-  const CPtr<float> c1h = s.c1h, c2h = s.c2h, c1f = s.c1f, c2f = s.c2f, rdnw = s.rdnw;
+  const float muts = s.muts[o2], mu = s.mu_2[o2];
   const float smdiv = c.smdiv;
   CT wup = wfe;
   float ph_up = 0.f;  // final ph one level above

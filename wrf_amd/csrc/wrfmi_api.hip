@@ -445,10 +445,10 @@ static void ph_advance_mu_t_w(Launcher &L) {
   else L.launchK(KID_AC_MU_WW, KRP(k_ac_mu_ww, false), g_col(a.d), kColBlk, a);
   if (L.vec) {
     L.launchK(KID_AC_THETA, KR(k_ac_theta_x4), g_vec3(a.d, nk), kColBlk, a);
-    L.launchK(KID_AC_WPRED, KR(k_ac_wpred_x4), g_vec3(a.d, nk), kColBlk, a);
+    L.launchK(KID_AC_WPRED, KR(k_ac_wpred_x4), g_vec3(a.d, 1), kColBlk, a);
   } else {
     L.launchK(KID_AC_THETA, KR(k_ac_theta), g_pt3(a.d, nk), kColBlk, a);
-    L.launchK(KID_AC_WPRED, KR(k_ac_wpred), g_pt3(a.d, nk), kColBlk, a);
+    L.launchK(KID_AC_WPRED, KR(k_ac_wpred), g_pt3(a.d, 1), kColBlk, a);
   }
   if (p64) L.launchK(KID_AC_WSOLVE, KRP(k_ac_wsolve, double), g_col(a.d), kColBlk, a);
   else L.launchK(KID_AC_WSOLVE, KRP(k_ac_wsolve, float), g_col(a.d), kColBlk, a);
