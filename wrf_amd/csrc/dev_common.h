@@ -113,6 +113,8 @@ struct KArgs {
   FPtr<float> scr_dv;   // advance_mu_t's column-mass divergence dvdxi
   FPtr<float> scr_rhs;  // advance_w's right-hand side rhs
   FPtr<float> scr_lo;   // low-order part of the FP64 forward-elimination value (precision mode 1)
+  float *scr_qt;        // advect_tend of species 0 .. n_species-2 (all-species scalar kernels),
+  unsigned long long qt_stride;  // one grid field (elements) apart
   int tnx, tny, tnz;  // tiles of the current launch in i, j and k (see tile3)
 };
 

@@ -260,6 +260,116 @@ __global__ void __launch_bounds__(256) k_update_scalar(ARGS args) {
   sc2[o] = ((c1 * muold + c2) * q1 + a.dt_rk * tendency) / (c1 * munew + c2);
 }
 
+// advect_scalar for every species in one pass (doubly periodic, four i points per lane, one
+// level per thread): ru_m, rv_m and ww_m are read once for all species.  Species m's
+// tendency goes to the work array m (the last species to advect_tend, as after the
+// reference's species loop); element by element the fluxes of k_adv_scalar_x4.
+template <typename ARGS>
+__global__ void __launch_bounds__(256) k_adv_scalar_all_x4(ARGS args) {
+  Tile3 T;
+  const KArgs &a = kargs(args, T);
+  const Dm &d = a.d;
+  const DState &s = dstate(a);
+  const int i0 = vec_ia0(d) + kVecW * (T.bx * 64 + (int)threadIdx.x);
+  const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
+  const int k = d.kds + (int)T.bz;
+  const int ids = d.ids, ide = d.ide, jde = d.jde, kts = d.kds, kte = d.kde, ktf = kte - 1;
+  if (i0 > ide || j > jde || k > kte) return;
+  const float ts = a.c.time_step >= 0 ? 1.f : -1.f;
+  const float rdx = s.rdx, rdy = s.rdy;
+  const int ni = d.ni, sj = d.sj;
+  const Off ob = X3(d, i0, 0, j);
+  auto lq = [&](int kk) { return ob + imax(kts, imin(kk, ktf)) * ni; };  // q levels kts..ktf
+  auto lw = [&](int kk) { return ob + imax(kts, imin(kk, kte)) * ni; };  // w levels kts..kte
+  const Off o = lq(k), os = lw(k);
+  const F4 msftx = s.msftx.ld<F4>(X2(d, i0, j));
+  const F4 ru0 = s.ru_m.ld<F4>(o), ru1 = s.ru_m.ld<F4>(o + 4), rv0 = s.rv_m.ld<F4>(o), rv1 = s.rv_m.ld<F4>(o + sj);
+  const F4 w0 = s.ww_m.ld<F4>(lw(k)), w1 = s.ww_m.ld<F4>(lw(k + 1));
+  const float rdnw = s.rdnw[X1(d, imin(k, ktf))];
+  const float fm0 = s.fnm[X1(d, k)], fp0 = s.fnp[X1(d, k)], fm1 = s.fnm[X1(d, imin(k + 1, kte))], fp1 = s.fnp[X1(d, imin(k + 1, kte))];
+  auto vflux = [&](int kp, float fm, float fp, float w, float qm2, float qm1, float q0, float qp1) {
+    const bool zero = kp == kts || kp == kte, low = kp == kts + 1 || kp == ktf;
+    const float lo = w * (fm * q0 + fp * qm1);
+    const float hi = w * flux3(qm2, qm1, q0, qp1, -w, ts);
+    return zero ? 0.f : (low ? lo : hi);
+  };
+  const bool jmass = j <= jde - 1;
+  const int ns = s.n_species;
+  for (int m = 0; m < ns; m++) {
+    const FPtr<float> f = s.q[m];
+    const F4 q0 = f.ld<F4>(o), wl = f.ld<F4>(o - 4), wr = f.ld<F4>(o + 4);
+    const F4 ym3 = f.ld<F4>(o - 3 * sj), ym2 = f.ld<F4>(o - 2 * sj), ym1 = f.ld<F4>(o - sj);
+    const F4 yp1 = f.ld<F4>(o + sj), yp2 = f.ld<F4>(o + 2 * sj), yp3 = f.ld<F4>(o + 3 * sj);
+    const F4 qm2 = f.ld<F4>(lq(k - 2)), qm1 = f.ld<F4>(lq(k - 1)), qp1 = f.ld<F4>(lq(k + 1)), qp2 = f.ld<F4>(lq(k + 2));
+    float W[12];
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+      W[n] = wl.v[n];
+      W[n + 4] = q0.v[n];
+      W[n + 8] = wr.v[n];
+    }
+    float fx[5];
+#pragma unroll
+    for (int n = 0; n < 5; n++) {
+      const float vel = n < 4 ? ru0.v[n] : ru1.v[0];
+      fx[n] = vel * flux5(W[n + 1], W[n + 2], W[n + 3], W[n + 4], W[n + 5], W[n + 6], vel, ts);
+    }
+    F4 t;
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+      const float vf_lo = vflux(k, fm0, fp0, w0.v[n], qm2.v[n], qm1.v[n], q0.v[n], qp1.v[n]);
+      const float vf_hi = vflux(k + 1, fm1, fp1, w1.v[n], qm1.v[n], q0.v[n], qp1.v[n], qp2.v[n]);
+      const float fy0 = rv0.v[n] * flux5(ym3.v[n], ym2.v[n], ym1.v[n], q0.v[n], yp1.v[n], yp2.v[n], rv0.v[n], ts);
+      const float fy1 = rv1.v[n] * flux5(ym2.v[n], ym1.v[n], q0.v[n], yp1.v[n], yp2.v[n], yp3.v[n], rv1.v[n], ts);
+      float tt = 0.f;
+      tt = tt - msftx.v[n] * rdy * (fy1 - fy0);
+      tt = tt - msftx.v[n] * rdx * (fx[n + 1] - fx[n]);
+      tt = tt - rdnw * (vf_hi - vf_lo);
+      const bool mass = jmass && i0 + n <= ide - 1 && k <= ktf;
+      t.v[n] = mass ? tt : 0.f;
+    }
+    const FPtr<float> dst = m == ns - 1 ? s.advect_tend : FPtr<float>(a.scr_qt + (size_t)m * a.qt_stride);
+    st4m(dst, os, t, i0, ids, ide);
+  }
+}
+
+// rk_update_scalar for every species in one pass (four i points per lane), reading each
+// species' advection tendency where k_adv_scalar_all_x4 left it
+template <typename ARGS>
+__global__ void __launch_bounds__(256) k_update_scalar_all_x4(ARGS args) {
+  Tile3 T;
+  const KArgs &a = kargs(args, T);
+  const Dm &d = a.d;
+  const DState &s = dstate(a);
+  const int i0 = vec_ia0(d) + kVecW * (T.bx * 64 + (int)threadIdx.x);
+  const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
+  const int k = d.kds + (int)T.bz;
+  if (i0 > d.ide - 1 || j > d.jde - 1 || k > d.kde - 1) return;
+  const Off o = X3(d, i0, k, j), o2 = X2(d, i0, j);
+  const F4 mu1 = s.mu_1.ld<F4>(o2), mu2 = s.mu_2.ld<F4>(o2), mub = s.mub.ld<F4>(o2), msfty = s.msfty.ld<F4>(o2);
+  const float c1 = s.c1h[X1(d, k)], c2 = s.c2h[X1(d, k)];
+  const bool rk1 = a.rk_step == 1;
+  const int ns = s.n_species;
+  for (int m = 0; m < ns; m++) {
+    const FPtr<float> sc1 = s.q_old[m], sc2 = s.q[m];
+    const FPtr<float> src = m == ns - 1 ? s.advect_tend : FPtr<float>(a.scr_qt + (size_t)m * a.qt_stride);
+    const F4 adv = src.ld<F4>(o), qt = s.q_tend[m].ld<F4>(o);
+    const F4 q2 = sc2.ld<F4>(o), q1o = sc1.ld<F4>(o);
+    F4 qn;
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+      const float muold = mu1.v[n] + mub.v[n];
+      const float munew = mu2.v[n] + mub.v[n];
+      float tendency = adv.v[n] * msfty.v[n];
+      tendency = tendency + qt.v[n];
+      const float q1 = rk1 ? q2.v[n] : q1o.v[n];
+      qn.v[n] = ((c1 * muold + c2) * q1 + a.dt_rk * tendency) / (c1 * munew + c2);
+    }
+    if (rk1) st4m(sc1, o, q2, i0, d.ids, d.ide - 1);
+    st4m(sc2, o, qn, i0, d.ids, d.ide - 1);
+  }
+}
+
 // rk_update_scalar with four i points per lane (16-byte loads and stores)
 template <typename ARGS>
 __global__ void __launch_bounds__(256) k_update_scalar_x4(ARGS args) {
@@ -492,6 +602,8 @@ WRFMI_INST_P(k_adv_scalar, true)
 WRFMI_INST(k_adv_scalar_x4)
 WRFMI_INST(k_update_scalar)
 WRFMI_INST(k_update_scalar_x4)
+WRFMI_INST(k_adv_scalar_all_x4)
+WRFMI_INST(k_update_scalar_all_x4)
 WRFMI_INST(k_p_rho_phi)
 WRFMI_INST(k_p_rho_phi_x4)
 template __global__ void k_halo<HaloArgs>(HaloArgs);

@@ -46,6 +46,8 @@ template <typename A, bool PER> __global__ void k_adv_scalar(A);
 template <typename A> __global__ void k_adv_scalar_x4(A);
 template <typename A> __global__ void k_update_scalar(A);
 template <typename A> __global__ void k_update_scalar_x4(A);
+template <typename A> __global__ void k_adv_scalar_all_x4(A);
+template <typename A> __global__ void k_update_scalar_all_x4(A);
 template <typename A> __global__ void k_p_rho_phi(A);
 template <typename A> __global__ void k_p_rho_phi_x4(A);
 template <typename A> __global__ void k_halo(A);

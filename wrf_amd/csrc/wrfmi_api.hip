@@ -126,6 +126,24 @@ static int get_scratch(size_t nelem, Scratch *out) {
   *out = it->second;
   return 0;
 }
+// Advection tendencies of species 0 .. n-2 for the all-species scalar kernels: n-1 grid
+// fields per (field size, species count), allocated on first use.
+static std::map<std::pair<size_t, int>, float *> g_qt;
+static int get_qt(size_t nelem, int nspecies, float **out) {
+  const auto key = std::make_pair(nelem, nspecies);
+  auto it = g_qt.find(key);
+  if (it == g_qt.end()) {
+    float *p = nullptr;
+    const size_t bytes = nelem * sizeof(float) * (size_t)(nspecies - 1);
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(WRFMI_E_DEVICE, "cannot allocate %zu bytes of species work arrays", bytes);
+    }
+    it = g_qt.emplace(key, p).first;
+  }
+  *out = it->second;
+  return 0;
+}
 static void free_scratch() {
   for (auto &kv : g_scratch) {
     (void)hipFree(kv.second.dv);
@@ -133,6 +151,8 @@ static void free_scratch() {
     (void)hipFree(kv.second.lo);
   }
   g_scratch.clear();
+  for (auto &kv : g_qt) (void)hipFree(kv.second);
+  g_qt.clear();
 }
 
 // ------------------------------------------------------------------ library
@@ -297,6 +317,7 @@ struct Launcher {
   int err = 0;
   RecMember *rec = nullptr;  // set: record the launches instead of enqueueing them
   bool vec = false;          // the layout allows the four-points-per-lane kernels
+  bool scal_all = false;     // species advected and updated by the all-species kernels
   // launch with an optional event pair around it (kernel timing mode)
   template <typename K, typename... A>
   void launch(int id, K kern, dim3 g, dim3 b, A... args) {
@@ -485,6 +506,13 @@ static void ph_scalar(Launcher &L) {
   if (L.vec) L.launchK(KID_UPDATE_SCALAR, KR(k_update_scalar_x4), g_vec3(L.a.d, L.a.d.kde - L.a.d.kds), kColBlk, L.a);
   else L.launchK(KID_UPDATE_SCALAR, KR(k_update_scalar), g_pt3(L.a.d, L.a.d.kde - L.a.d.kds), kColBlk, L.a);
 }
+// every species' rk_scalar_tend and rk_update_scalar (species are independent: the order
+// over species does not change any value)
+static void ph_scalar_all(Launcher &L) {
+  const Dm &d = L.a.d;
+  L.launchK(KID_ADV_SCALAR, KR(k_adv_scalar_all_x4), g_vec3(d, d.kde - d.kds + 1), kColBlk, L.a);
+  L.launchK(KID_UPDATE_SCALAR, KR(k_update_scalar_all_x4), g_vec3(d, d.kde - d.kds), kColBlk, L.a);
+}
 static void ph_p_rho_phi(Launcher &L) {
   if (L.vec) L.launchK(KID_P_RHO_PHI, KR(k_p_rho_phi_x4), g_vec3(L.a.d, L.a.d.kde - L.a.d.kds), kColBlk, L.a);
   else L.launchK(KID_P_RHO_PHI, KR(k_p_rho_phi), g_pt3(L.a.d, L.a.d.kde - L.a.d.kds), kColBlk, L.a);
@@ -528,9 +556,13 @@ static void enqueue_step(Launcher &L) {
       ph_advance_mu_t_w(L);
     }
     ph_small_finish(L);
-    for (int m = 0; m < a.s.n_species; m++) {
-      a.species = m;
-      ph_scalar(L);
+    if (L.scal_all) {
+      ph_scalar_all(L);
+    } else {
+      for (int m = 0; m < a.s.n_species; m++) {
+        a.species = m;
+        ph_scalar(L);
+      }
     }
     ph_p_rho_phi(L);
     if (rk < 3) ph_bc_dry_2(L, 0);
@@ -566,7 +598,7 @@ static bool vec_layout(const wrfmi_state &s, const Dm &d, const KArgs &a) {
 // Validate and translate the arguments into L; the work arrays are the library's shared
 // ones for this field size unless own_scratch is given (ensemble members each have theirs).
 static int prepare(const wrfmi_state *st_in, const wrfmi_config *cfg, const wrfmi_dims *dims, Launcher &L, wrfmi_state &s,
-                   const Scratch *own_scratch = nullptr) {
+                   const Scratch *own_scratch = nullptr, float *own_qt = nullptr) {
   int rc;
   if ((rc = check_cfg(cfg))) return rc;
   Dm d;
@@ -586,6 +618,15 @@ static int prepare(const wrfmi_state *st_in, const wrfmi_config *cfg, const wrfm
   L.h.periodic_x = cfg->periodic_x; L.h.periodic_y = cfg->periodic_y;
   L.h.open_xs = cfg->open_xs; L.h.open_xe = cfg->open_xe; L.h.open_ys = cfg->open_ys; L.h.open_ye = cfg->open_ye;
   L.vec = vec_layout(s, d, L.a);
+  // the all-species scalar kernels (doubly periodic, aligned layout, more than one species)
+  L.scal_all = L.vec && cfg->periodic_x && cfg->periodic_y && s.n_species > 1;
+  if (L.scal_all) {
+    const size_t nelem = (size_t)d.sj * (size_t)(d.jme - d.jms + 1);
+    float *qt = own_qt;
+    if (!qt && (rc = get_qt(nelem, s.n_species, &qt))) return rc;
+    L.a.scr_qt = qt;
+    L.a.qt_stride = nelem;
+  }
   return 0;
 }
 
@@ -719,8 +760,18 @@ extern "C" int wrfmi_ensemble_create(int n_members, const wrfmi_state *states, c
       e->work.push_back(*p);
     }
     if (rc) break;
+    float *qt = nullptr;  // species work arrays of a doubly periodic member with several species
+    if (cfgs[m].periodic_x && cfgs[m].periodic_y && states[m].n_species > 1) {
+      const size_t qb = bytes * (size_t)(states[m].n_species - 1);
+      if (hipMalloc(&qt, qb) != hipSuccess) {
+        (void)hipGetLastError();
+        rc = fail(WRFMI_E_DEVICE, "member %d: cannot allocate %zu bytes of species work arrays", m, qb);
+        break;
+      }
+      e->work.push_back(qt);
+    }
     wrfmi_state s;
-    if ((rc = prepare(&states[m], &cfgs[m], &dims[m], Ls[m], s, &sc))) {
+    if ((rc = prepare(&states[m], &cfgs[m], &dims[m], Ls[m], s, &sc, qt))) {
       std::string msg = g_err;
       rc = fail(rc, "member %d: %s", m, msg.c_str());
       break;
@@ -732,6 +783,7 @@ extern "C" int wrfmi_ensemble_create(int n_members, const wrfmi_state *states, c
   for (int m = 0; m < n_members && !rc; m++) all_vec = all_vec && Ls[m].vec;
   for (int m = 0; m < n_members && !rc; m++) {
     Ls[m].vec = all_vec;
+    Ls[m].scal_all = Ls[m].scal_all && all_vec;
     Ls[m].rec = &rec[m];
     enqueue_step(Ls[m]);
   }
