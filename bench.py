@@ -4,7 +4,7 @@
 6 passive scalars, FP32.  Prints one JSON line.
 
     python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--no-cpu-baseline]
-                    [--ensemble-members 64] [--ensemble-steps 10]
+                    [--ensemble-members 64] [--ensemble-steps 10] [--terrain]
 
 value = mass-point RK3 steps per second (millions) with the state resident in HBM.
 roofline = the kernel with the largest share of one profiled (untimed) step, timed with HIP
@@ -12,6 +12,7 @@ events on its launch stream over the timed steps, against 8 TB/s; its algorithmi
 the 3-D arrays it must read and write once per launch.
 ensemble = member-steps/s of the 64-member uneven ensemble, member by member with the plain
 kernels and batched (one launch per kernel over all members, hipGraph replay).
+terrain (with --terrain) = the 1200x1200x80 terrain case, FP32 and precision forms.
 cpu_baseline = the FP32 C restatement of the same step (the oracle: one core), timed on
 a 100x100x80 sub-domain of the same workload.
 """
@@ -139,6 +140,52 @@ def ensemble_leg(runtime, grid, torch, n_members, steps, log):
     return out
 
 
+def terrain_leg(runtime, grid, torch, steps, log):
+    """The 1200x1200x80 terrain case (hybrid levels, 12 Witch-of-Agnesi ridges, periodic y /
+    open x, dx = 500 m, dt = 3 s, qv + 10 scalars): Mpoint-RK3-steps/s over `steps` steps
+    after one untimed step, in the FP32 form and in the precision form (float64 column
+    solve and compensated column sums, precision_mode 1)."""
+    import numpy as np
+    spec = grid.terrain_c4()
+    log(f"building the {spec.nx}x{spec.ny}x{spec.nz} terrain state ...")
+    t0 = time.perf_counter()
+    hs = grid.build_initial_state(spec)
+    log(f"  done in {time.perf_counter() - t0:.1f} s")
+    pts = spec.nx * spec.ny * spec.nz
+    n_sound = runtime.num_sound_steps(hs.cfg, hs.scal["dt"], hs.scal["dx"], hs.scal["dy"])
+    out = {"workload": f"{spec.nx}x{spec.ny}x{spec.nz}, hybrid_opt 2, {spec.n_ridges} Witch-of-Agnesi ridges, "
+                       f"periodic y / open x, dx=500 m, dt=3 s, {n_sound} acoustic substeps, qv + "
+                       f"{spec.n_passive} passive scalars (seed {spec.seed})",
+           "steps": steps, "alg_bytes_per_point_step": b_alg(n_sound, hs.n_species)}
+    stream = torch.cuda.current_stream()
+    for form in ("fp32", "precision"):
+        if form == "precision":
+            hs.cfg.precision_mode = 1
+            hs.coef_dtype = np.float64
+            hs.a, hs.alpha, hs.gamma = (np.zeros(hs.a.shape, np.float64) for _ in range(3))
+        dev = runtime.DeviceDomain(hs)
+        if form == "fp32":
+            out["hbm_gb"] = round(dev.hbm_bytes() / 1e9, 2)
+        dev.step(1, stream=stream)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        dev.step(steps, stream=stream, sync=False)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        nbad = dev.check_finite()
+        if nbad:
+            raise RuntimeError(f"terrain ({form}): {nbad} non-finite values after the timed steps")
+        out[f"{form}_ms_per_step"] = round(1e3 * dt / steps, 3)
+        out[f"{form}_mpoint_steps_per_s"] = round(pts * steps / dt / 1e6, 1)
+        out[f"{form}_pct_hbm_peak"] = round(100.0 * out["alg_bytes_per_point_step"] * pts * steps / dt / 1e9
+                                            / HBM_PEAK_GBS, 2)
+        log(f"  terrain {form}: {out[f'{form}_ms_per_step']} ms/step, {out[f'{form}_mpoint_steps_per_s']} "
+            f"Mpoint-steps/s")
+        del dev
+        torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -153,6 +200,10 @@ def main():
                     help="members of the uneven-ensemble leg (0 skips it)")
     ap.add_argument("--ensemble-steps", type=int, default=10)
     ap.add_argument("--ensemble-only", action="store_true", help="run only the ensemble leg (profiling)")
+    ap.add_argument("--terrain", action="store_true",
+                    help="add the 1200x1200x80 terrain leg (about 50 GB of HBM, a minute of host set-up)")
+    ap.add_argument("--terrain-steps", type=int, default=20)
+    ap.add_argument("--terrain-only", action="store_true", help="run only the terrain leg (profiling)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after the timed steps, copy a buffer of this many MiB once (calibrates HBM counters)")
     args = ap.parse_args()
@@ -169,6 +220,9 @@ def main():
         sys.exit(1)
     if args.ensemble_only:
         print(json.dumps({"ensemble": ensemble_leg(runtime, grid, torch, args.ensemble_members, args.ensemble_steps, log)}))
+        return
+    if args.terrain_only:
+        print(json.dumps({"terrain": terrain_leg(runtime, grid, torch, args.terrain_steps, log)}))
         return
     spec = grid.large_c1(nx=args.nx, ny=args.ny, nz=args.nz)
     log(f"building the {spec.nx}x{spec.ny}x{spec.nz} initial state ...")
@@ -266,6 +320,11 @@ def main():
         del dev
         torch.cuda.empty_cache()
         out["ensemble"] = ensemble_leg(runtime, grid, torch, args.ensemble_members, args.ensemble_steps, log)
+    if args.terrain:
+        if args.ensemble_members <= 0:
+            del dev
+            torch.cuda.empty_cache()
+        out["terrain"] = terrain_leg(runtime, grid, torch, args.terrain_steps, log)
     if not args.no_cpu_baseline:
         log("timing the CPU baseline ...")
         out["cpu_baseline"] = cpu_baseline()

@@ -24,6 +24,13 @@ CASES = {
     # rows of 32 floats ending 6 points after ide: the four-point windows run past the row
     "periodic_tight": dict(nx=21, ny=9, nz=8, bc="periodic", n_passive=1, n_bubbles=1, seed=17, pad_i=4),
     "open_tight": dict(nx=21, ny=10, nz=8, bc="open", n_bubbles=1, seed=18, pad_i=4),
+    # terrain on hybrid levels with one periodic and one open direction, both ways round
+    "terrain": dict(nx=16, ny=12, nz=10, dx=2000.0, dy=2000.0, dt=6.0, bc_x="open", bc_y="periodic",
+                    hybrid_opt=2, sounding="stable", n_ridges=3, ridge_a=(3000.0, 8000.0), n_passive=1,
+                    n_bubbles=0, noise_amp=0.0, seed=19),
+    "terrain_yopen": dict(nx=13, ny=14, nz=9, dx=2000.0, dy=2000.0, dt=6.0, bc_x="periodic", bc_y="open",
+                          hybrid_opt=2, sounding="stable", n_ridges=2, ridge_a=(3000.0, 8000.0), n_passive=2,
+                          n_bubbles=0, noise_amp=0.0, seed=20),
 }
 
 

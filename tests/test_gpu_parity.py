@@ -23,6 +23,13 @@ CASES = {
     # rows of 30 floats (not a multiple of four): the one-point kernel forms throughout
     "periodic_unaligned": dict(nx=21, ny=17, nz=12, bc="periodic", n_passive=1, n_bubbles=2, bubble_amp=2.0, seed=16,
                                pad_i=1),
+    # ridges on hybrid levels (hybrid_opt 2) with one open and one periodic direction
+    "terrain": dict(nx=26, ny=20, nz=16, dx=1000.0, dy=1000.0, dt=3.0, bc_x="open", bc_y="periodic", hybrid_opt=2,
+                    sounding="stable", n_ridges=3, ridge_a=(3000.0, 8000.0), n_passive=2, n_bubbles=0, noise_amp=0.0,
+                    seed=21),
+    "terrain_yopen": dict(nx=20, ny=23, nz=12, dx=1000.0, dy=1000.0, dt=3.0, bc_x="periodic", bc_y="open",
+                          hybrid_opt=2, sounding="stable", n_ridges=2, ridge_a=(3000.0, 8000.0), n_passive=1,
+                          n_bubbles=0, noise_amp=0.0, seed=22),
 }
 KINDS = dict(u_2="u", v_2="v", w_2="w", t_2="mass", mu_2="mass", p="mass", ph_2="w", al="mass", q0="mass")
 

@@ -43,7 +43,9 @@ class DomainSpec:
     dt: float = 12.0
     ztop: float = 20000.0
     time_step_sound: int = 6
-    bc: str = "periodic"      # "periodic" | "open"
+    bc: str = "periodic"      # "periodic" | "open" (both directions unless bc_x/bc_y say otherwise)
+    bc_x: str = ""            # per-direction override of bc
+    bc_y: str = ""
     hybrid_opt: int = 0
     etac: float = 0.2
     n_passive: int = 0        # passive scalars after QV
@@ -55,6 +57,26 @@ class DomainSpec:
     seed: int = 1001
     pad_i: int = 32           # pad the memory i-extent to a multiple of this (alignment)
     precision_mode: int = 0
+    # sounding "wk" (Weisman-Klemp, hodograph winds) or "stable" (constant Brunt-Vaisala
+    # frequency, uniform westerly); the stable sounding's parameters follow
+    sounding: str = "wk"
+    theta0: float = 288.0     # surface potential temperature (K)
+    brunt: float = 0.01       # N (1/s): theta = theta0 * exp(N^2 z / g)
+    qv0: float = 0.008        # surface vapour mixing ratio (kg/kg), e-folding height qv_scale
+    qv_scale: float = 2500.0
+    u0: float = 10.0          # uniform westerly (m/s)
+    # terrain: n_ridges Witch-of-Agnesi hills h_m / (1 + (r/a)^2) at seeded positions
+    n_ridges: int = 0
+    ridge_a: tuple = (5000.0, 20000.0)
+    ridge_h: tuple = (200.0, 1500.0)
+
+    @property
+    def bcx(self):
+        return self.bc_x or self.bc
+
+    @property
+    def bcy(self):
+        return self.bc_y or self.bc
 
     @property
     def e_we(self):
@@ -92,9 +114,13 @@ def make_config(spec: DomainSpec) -> Config:
     """Namelist/Registry defaults (Registry/Registry.EM_COMMON:2840-2902) with physics,
     diffusion and damping off, as the benchmark configurations state."""
     c = Config()
-    per = spec.bc == "periodic"
-    c.periodic_x = c.periodic_y = int(per)
-    c.open_xs = c.open_xe = c.open_ys = c.open_ye = int(spec.bc == "open")
+    for b in (spec.bcx, spec.bcy):
+        if b not in ("periodic", "open"):
+            raise ValueError(f"lateral boundary {b!r}: periodic or open")
+    c.periodic_x = int(spec.bcx == "periodic")
+    c.periodic_y = int(spec.bcy == "periodic")
+    c.open_xs = c.open_xe = int(spec.bcx == "open")
+    c.open_ys = c.open_ye = int(spec.bcy == "open")
     c.rk_ord = 3
     c.h_mom_adv_order, c.v_mom_adv_order = 5, 3
     c.h_sca_adv_order, c.v_sca_adv_order = 5, 3
@@ -185,6 +211,54 @@ def sounding(moist: bool, zmax=30000.0, dz=5.0, psfc=1.0e5, qv_max=0.014):
     return z, p, pd, th, qv
 
 
+def stable_sounding(spec: DomainSpec, moist: bool, zmax=30000.0, dz=5.0, psfc=1.0e5):
+    """Hydrostatic integration of a constant-N sounding, theta = theta0 * exp(N^2 z / g),
+    with vapour qv0 * exp(-z / qv_scale) when `moist`.  Same integration and returns as
+    `sounding` (z, p, pd, theta, qv)."""
+    z = np.arange(0.0, zmax + dz, dz)
+    th = spec.theta0 * np.exp(spec.brunt ** 2 * z / G)
+    qv = spec.qv0 * np.exp(-z / spec.qv_scale) if moist else np.zeros_like(z)
+    n = z.size
+    p = np.empty(n)
+    rho = np.empty(n)
+    p[0] = psfc
+    for it in range(n):
+        if it > 0:
+            pg = p[it - 1] - G * rho[it - 1] * dz
+            for _ in range(10):
+                t = th[it] * (pg / P1000) ** RCP
+                rh_ = pg / (R_D * t * (1.0 + RVOVRD * qv[it]) / (1.0 + qv[it]))
+                pg = p[it - 1] - 0.5 * G * (rho[it - 1] + rh_) * dz
+            p[it] = pg
+        t = th[it] * (p[it] / P1000) ** RCP
+        rho[it] = p[it] / (R_D * t * (1.0 + RVOVRD * qv[it]) / (1.0 + qv[it]))
+    wv = G * rho * qv / (1.0 + qv)
+    col = np.concatenate([np.cumsum((0.5 * (wv[1:] + wv[:-1]) * dz)[::-1])[::-1], [0.0]])
+    return z, p, p - col, th, qv
+
+
+def ridge_field(spec: DomainSpec, rng) -> np.ndarray:
+    """Terrain height (ny, nx) at mass points: the sum of `n_ridges` Witch-of-Agnesi hills
+    h_m / (1 + (r/a)^2), centres uniform over the domain, a and h_m uniform in ridge_a and
+    ridge_h; distances wrap in a periodic direction."""
+    nx, ny = spec.nx, spec.ny
+    Lx, Ly = nx * spec.dx, ny * spec.dy
+    xs = (np.arange(nx) + 0.5) * spec.dx
+    ys = (np.arange(ny) + 0.5) * spec.dy
+    ht = np.zeros((ny, nx))
+    for _ in range(spec.n_ridges):
+        xc, yc = rng.uniform(0.0, Lx), rng.uniform(0.0, Ly)
+        a = rng.uniform(*spec.ridge_a)
+        hm = rng.uniform(*spec.ridge_h)
+        dxr, dyr = xs - xc, ys - yc
+        if spec.bcx == "periodic":
+            dxr = (dxr + 0.5 * Lx) % Lx - 0.5 * Lx
+        if spec.bcy == "periodic":
+            dyr = (dyr + 0.5 * Ly) % Ly - 0.5 * Ly
+        ht += hm / (1.0 + (dxr[None, :] ** 2 + dyr[:, None] ** 2) / (a * a))
+    return ht
+
+
 def hodograph(z):
     """Quarter-circle hodograph: radius 8 m/s to 2 km, then straight to 32 m/s at 6 km."""
     z = np.asarray(z, dtype=np.float64)
@@ -256,37 +330,40 @@ class HostState:
         return other
 
 
-def _halo_fill_np(a: np.ndarray, d: Dims, periodic: bool, stag_i: bool, stag_j: bool):
-    """Initial halo fill (periodic wrap or zero-gradient copy) of a (nj, [nk,] ni) array."""
+def _halo_fill_np(a: np.ndarray, d: Dims, periodic, stag_i: bool, stag_j: bool):
+    """Initial halo fill (periodic wrap or zero-gradient copy) of a (nj, [nk,] ni) array.
+    `periodic` is one flag for both directions or a pair (periodic in x, periodic in y)."""
+    per_x, per_y = (periodic, periodic) if isinstance(periodic, (bool, int, np.bool_)) else periodic
     nx, ny = d.ide - d.ids, d.jde - d.jds
     i_idx = np.arange(d.ims, d.ime + 1)
     j_idx = np.arange(d.jms, d.jme + 1)
-    if periodic:
+    if per_x:
         si = np.where(i_idx < d.ids, i_idx + nx, np.where(i_idx > d.ide - 1 + (1 if stag_i else 0), i_idx - nx, i_idx))
         if stag_i:
             si = np.where(i_idx == d.ide, d.ids, si)
+    else:
+        si = np.clip(i_idx, d.ids, d.ide if stag_i else d.ide - 1)
+    if per_y:
         sj = np.where(j_idx < d.jds, j_idx + ny, np.where(j_idx > d.jde - 1 + (1 if stag_j else 0), j_idx - ny, j_idx))
         if stag_j:
             sj = np.where(j_idx == d.jde, d.jds, sj)
     else:
-        si = np.clip(i_idx, d.ids, d.ide if stag_i else d.ide - 1)
         sj = np.clip(j_idx, d.jds, d.jde if stag_j else d.jde - 1)
     si = np.clip(si, d.ims, d.ime) - d.ims
     sj = np.clip(sj, d.jms, d.jme) - d.jms
-    if a.ndim == 3:
-        a[...] = a[sj][:, :, si]
-    else:
-        a[...] = a[sj][:, si]
+    # copy only the halo strips: i first over every row, then whole rows in j (corners
+    # take a[sj, si]); every source index is a non-halo one
+    ih = np.nonzero(si != np.arange(si.size))[0]
+    jh = np.nonzero(sj != np.arange(sj.size))[0]
+    a[..., ih] = a[..., si[ih]]
+    a[jh] = a[sj[jh]]
 
 
-def build_initial_state(spec: DomainSpec) -> HostState:
-    """Analytic supercell-type initial state (see the module docstring)."""
-    hs = HostState(spec)
-    d = hs.dims
-    kde = d.kde
-    nz = spec.nz
-    rng = np.random.Generator(np.random.Philox(spec.seed))
-
+def _vertical_coords(hs: HostState, spec: DomainSpec, p_top: float):
+    """Eta levels, their metrics and the hybrid coefficients (module_initialize_ideal.F:
+    617-877), in float64; stores the 1-D arrays and the scalars of `hs` and returns the
+    float64 arrays by name."""
+    kde = hs.dims.kde
     # eta levels and metrics (module_initialize_ideal.F:617-730), float64
     znw = 1.0 - np.arange(kde, dtype=np.float64) / (kde - 1)  # index k-1
     dnw = znw[1:] - znw[:-1]
@@ -308,11 +385,6 @@ def build_initial_state(spec: DomainSpec) -> HostState:
     cf3 = cof2
     cfn = (0.5 * dnw[kde - 2] + dn[kde - 2]) / dn[kde - 2]
     cfn1 = -0.5 * dnw[kde - 2] / dn[kde - 2]
-
-    # soundings
-    zs, pdry, _, thdry, _ = sounding(moist=False)
-    zm, pm, pdm, thm, qvm = sounding(moist=True)
-    p_top = float(np.interp(spec.ztop, zs, pdry))
 
     # hybrid coefficients (module_initialize_ideal.F:791-877)
     if spec.hybrid_opt == 0:
@@ -358,6 +430,32 @@ def build_initial_state(spec: DomainSpec) -> HostState:
     put1("c1f", c1f); put1("c2f", c2f); put1("c3f", c3f); put1("c4f", c4f)
     hs.scal.update(dict(dt=spec.dt, dx=spec.dx, dy=spec.dy, rdx=1.0 / spec.dx, rdy=1.0 / spec.dy,
                         cf1=cf1, cf2=cf2, cf3=cf3, cfn=cfn, cfn1=cfn1, p_top=p_top))
+
+    return dict(znw=znw, dnw=dnw, rdnw=rdnw, znu=znu, dn=dn, rdn=rdn, fnp=fnp, fnm=fnm,
+                c1h=c1h, c2h=c2h, c3h=c3h, c4h=c4h, c1f=c1f, c2f=c2f, c3f=c3f, c4f=c4f)
+
+
+def build_initial_state(spec: DomainSpec) -> HostState:
+    """Analytic supercell-type initial state (see the module docstring); a spec with the
+    stable sounding goes to build_terrain_state."""
+    if spec.sounding == "stable":
+        return build_terrain_state(spec)
+    if spec.sounding != "wk":
+        raise ValueError(f"sounding {spec.sounding!r}: wk or stable")
+    hs = HostState(spec)
+    d = hs.dims
+    kde = d.kde
+    nz = spec.nz
+    rng = np.random.Generator(np.random.Philox(spec.seed))
+
+    # soundings
+    zs, pdry, _, thdry, _ = sounding(moist=False)
+    zm, pm, pdm, thm, qvm = sounding(moist=True)
+    p_top = float(np.interp(spec.ztop, zs, pdry))
+
+    v = _vertical_coords(hs, spec, p_top)
+    znw, dnw, rdnw, znu, dn, rdn = v["znw"], v["dnw"], v["rdnw"], v["znu"], v["dn"], v["rdn"]
+    c1h, c2h, c3h, c4h, c1f, c2f, c3f, c4f = (v[n] for n in ("c1h", "c2h", "c3h", "c4h", "c1f", "c2f", "c3f", "c4f"))
 
     # base state: flat terrain (ht = 0, phb(1) = 0), dry sounding
     p_surf = float(np.interp(0.0, zs, pdry))
@@ -438,8 +536,9 @@ def build_initial_state(spec: DomainSpec) -> HostState:
     for (xc, yc, amp) in centres:
         dxr = xs - xc
         dyr = ys - yc
-        if spec.bc == "periodic":
+        if spec.bcx == "periodic":
             dxr = (dxr + 0.5 * Lx) % Lx - 0.5 * Lx
+        if spec.bcy == "periodic":
             dyr = (dyr + 0.5 * Ly) % Ly - 0.5 * Ly
         xr = dxr / 10000.0
         yr = dyr / 10000.0
@@ -489,7 +588,7 @@ def build_initial_state(spec: DomainSpec) -> HostState:
         hs.q[m][js, ks, is_] = np.maximum(vals, 0.0).astype(np.float32)
 
     # halo fills of everything that carries a halo
-    per = spec.bc == "periodic"
+    per = (spec.bcx == "periodic", spec.bcy == "periodic")
     stag = {"u_1": (1, 0), "u_2": (1, 0), "v_1": (0, 1), "v_2": (0, 1)}
     for n in ("u_1", "u_2", "v_1", "v_2", "w_1", "w_2", "t_1", "t_2", "ph_1", "ph_2", "phb", "pb", "alb",
               "p", "al", "alt"):
@@ -500,6 +599,146 @@ def build_initial_state(spec: DomainSpec) -> HostState:
         hs.q_old[m][...] = hs.q[m]
     for n in ("mub", "mu_1", "mu_2"):
         _halo_fill_np(A2[n], d, per, False, False)
+    return hs
+
+
+def build_terrain_state(spec: DomainSpec) -> HostState:
+    """Stably stratified flow over terrain: the stable sounding, a uniform westerly u0 and
+    ridge_field terrain, on the hybrid levels of `spec`.  The steps are those of the
+    reference's idealised initialisation for its hill cases (module_initialize_ideal.F:
+    611-1130): dry base state from the dry sounding with p_surf(ht) and phb(1) = g*ht,
+    moist full state from the moist sounding, hydrostatic geopotential.  Every quantity
+    varies per column through ht, so the columns are built in blocks of rows, level by
+    level in float64, on a thread pool (numpy releases the GIL), and rounded once into the
+    float32 arrays.  Each block draws its scalars from its own generator, so the state
+    does not depend on the number of threads."""
+    import concurrent.futures
+    import os
+
+    hs = HostState(spec)
+    d = hs.dims
+    nz = spec.nz
+    rng = np.random.Generator(np.random.Philox(spec.seed))
+
+    zs, pdry, _, thdry, _ = stable_sounding(spec, moist=False)
+    zm, _, pdm, thm, qvm = stable_sounding(spec, moist=True)
+    p_top = float(np.interp(spec.ztop, zs, pdry))
+    v = _vertical_coords(hs, spec, p_top)
+    dnw, rdnw, rdn = v["dnw"], v["rdnw"], v["rdn"]
+    c1h, c2h, c3h, c4h, c1f, c2f, c3f, c4f = (v[n] for n in ("c1h", "c2h", "c3h", "c4h", "c1f", "c2f", "c3f", "c4f"))
+    ht_all = ridge_field(spec, rng)
+    A3, A2 = hs.a3, hs.a2
+    f32 = np.float32
+    i0 = d.ids - d.ims
+    is_ = slice(i0, i0 + spec.nx)
+    rows = 16
+    blocks = [(j, min(j + rows, spec.ny)) for j in range(0, spec.ny, rows)]
+    seeds = np.random.SeedSequence([spec.seed, 1]).spawn(len(blocks))
+
+    def build_rows(blk):
+        jb0, jb1 = blocks[blk]
+        js = slice(d.jds - d.jms + jb0, d.jds - d.jms + jb1)
+        ht = ht_all[jb0:jb1]
+        # dry base state (module_initialize_ideal.F:905-960, p_surf from the sounding at ht)
+        p_surf = np.interp(ht, zs, pdry)
+        mub = p_surf - p_top
+        pb = [c3h[k] * mub + c4h[k] + p_top for k in range(nz)]
+        alb = [(R_D / P1000) * _interp_p(thdry, pdry, pb[k]) * (pb[k] / P1000) ** CVPM for k in range(nz)]
+        phb = [G * ht]
+        for k in range(nz):
+            phb.append(phb[k] - dnw[k] * (c1h[k] * mub + c2h[k]) * alb[k])
+        # moist full state (module_initialize_ideal.F:965-1085)
+        pd_surf = np.interp(ht, zm, pdm)
+        mu1 = pd_surf - p_top - mub
+        plev = [c3h[k] * (pd_surf - p_top) + c4h[k] + p_top for k in range(nz)]
+        QV = [_interp_p(qvm, pdm, plev[k]) for k in range(nz)]
+        T = [_interp_p(thm, pdm, plev[k]) - T0 for k in range(nz)]
+        # first guess of p from the hydrostatic dry-mass relation, top down, giving alt
+        alt = [None] * nz
+        p_k = None
+        for kk in range(nz - 1, -1, -1):
+            k = kk + 1
+            lo = kk - 1 if kk == nz - 1 else kk + 1
+            qvf1 = 0.5 * (QV[kk] + QV[lo])
+            qvf2 = 1.0 / (1.0 + qvf1)
+            qvf1 = qvf1 * qvf2
+            if kk == nz - 1:
+                p_k = -0.5 * ((c1f[k] * mu1) + qvf1 * (c1f[k] * mub + c2f[k])) / rdnw[kk] / qvf2
+            else:
+                p_k = p_k - ((c1f[k] * mu1) + qvf1 * (c1f[k] * mub + c2f[k])) / qvf2 / rdn[kk + 1]
+            alt[kk] = (R_D / P1000) * (T[kk] + T0) * (1.0 + RVOVRD * QV[kk]) * ((p_k + pb[kk]) / P1000) ** CVPM
+        # geopotential from the hypsometric relation that calc_p_rho_phi inverts, then
+        # al, alt and p from it
+        muts = mub + mu1
+        ph_lo = G * ht
+        A3["ph_2"][js, 0, is_] = (ph_lo - phb[0]).astype(f32)
+        for k in range(nz):
+            pf_lo = c3f[k] * muts + c4f[k] + p_top
+            pf_hi = c3f[k + 1] * muts + c4f[k + 1] + p_top
+            phm = c3h[k] * muts + c4h[k] + p_top
+            dlog = np.log(pf_lo / pf_hi)
+            ph_hi = ph_lo + alt[k] * phm * dlog
+            al = (ph_hi - ph_lo) / phm / dlog - alb[k]
+            altk = al + alb[k]
+            tv = T0 + T[k] * (1.0 + RVOVRD * QV[k]) + T0 * RVOVRD * QV[k]
+            pk = P1000 * ((R_D * tv) / (P1000 * altk)) ** (CP / CV) - pb[k]
+            A3["ph_2"][js, k + 1, is_] = (ph_hi - phb[k + 1]).astype(f32)
+            A3["al"][js, k, is_] = al.astype(f32)
+            A3["alt"][js, k, is_] = altk.astype(f32)
+            A3["p"][js, k, is_] = pk.astype(f32)
+            A3["t_2"][js, k, is_] = ((T[k] + T0) * (1.0 + RVOVRD * QV[k]) - T0).astype(f32)
+            A3["pb"][js, k, is_] = pb[k].astype(f32)
+            A3["alb"][js, k, is_] = alb[k].astype(f32)
+            A3["phb"][js, k, is_] = phb[k].astype(f32)
+            hs.q[0][js, k, is_] = QV[k].astype(f32)
+            ph_lo = ph_hi
+        A3["phb"][js, nz, is_] = phb[nz].astype(f32)
+        A2["ht"][js, is_] = ht.astype(f32)
+        A2["mub"][js, is_] = mub.astype(f32)
+        A2["mu_2"][js, is_] = mu1.astype(f32)
+        # passive scalars: log-normal per point, median 1e-5 kg/kg, sigma 1.0 (positive,
+        # so the clip at 0 is implicit), drawn in float32
+        g = np.random.Generator(np.random.Philox(seeds[blk]))
+        for m in range(1, hs.n_species):
+            z = g.standard_normal((jb1 - jb0, nz, spec.nx), dtype=np.float32)
+            np.exp(z, out=z)
+            z *= f32(1.0e-5)
+            hs.q[m][js, :nz, is_] = z
+
+    per = (spec.bcx == "periodic", spec.bcy == "periodic")
+    stag = {"u_1": (1, 0), "u_2": (1, 0), "v_1": (0, 1), "v_2": (0, 1)}
+
+    def finish(name):
+        """Copies of the two time levels, then the halo fill of one array."""
+        if name in ("t_1", "ph_1"):
+            A3[name][...] = A3[name.replace("_1", "_2")]
+        elif name in ("u_1", "u_2"):
+            A3[name][:, :nz, :] = f32(spec.u0)
+        elif name == "mu_1":
+            A2["mu_1"][...] = A2["mu_2"]
+        elif name.startswith("msf") or name == "cosa":
+            A2[name][...] = 1.0
+            return
+        if name.startswith("q") and name[1:].isdigit():
+            m = int(name[1:])
+            _halo_fill_np(hs.q[m], d, per, False, False)
+            hs.q_old[m][...] = hs.q[m]
+        elif name in A2:
+            _halo_fill_np(A2[name], d, per, False, False)
+        else:
+            si, sj = stag.get(name, (0, 0))
+            _halo_fill_np(A3[name], d, per, bool(si), bool(sj))
+
+    workers = max(1, min(16, os.cpu_count() or 1))
+    with concurrent.futures.ThreadPoolExecutor(workers) as pool:
+        list(pool.map(build_rows, range(len(blocks))))
+        names = ["u_1", "u_2", "v_1", "v_2", "t_1", "t_2", "ph_1", "ph_2", "phb", "pb", "alb", "p", "al", "alt",
+                 "mub", "mu_1", "mu_2", "ht", "msfux", "msfuy", "msfvx", "msfvx_inv", "msfvy", "msftx", "msfty",
+                 "cosa"] + [f"q{m}" for m in range(hs.n_species)]
+        # the time-level copies read halo-free sources: run them before their sources' fills
+        first = ["t_1", "ph_1", "mu_1"]
+        list(pool.map(finish, first))
+        list(pool.map(finish, [n for n in names if n not in first]))
     return hs
 
 
@@ -538,3 +777,15 @@ def ensemble_c3(n_members: int = 64, seed: int = 1004, **kw) -> list:
         base.update(kw)
         out.append(DomainSpec(**base))
     return out
+
+
+def terrain_c4(**kw) -> DomainSpec:
+    """1200x1200x80 on hybrid levels (hybrid_opt 2), dx=500 m, dt=3 s, 6 sound steps,
+    periodic in y and open in x, 12 Witch-of-Agnesi ridges (half-width 5-20 km, height
+    200-1500 m), uniform 10 m/s westerly, N = 0.01 1/s, qv + 10 passive scalars (seed 1005)."""
+    base = dict(nx=1200, ny=1200, nz=80, dx=500.0, dy=500.0, dt=3.0, ztop=20000.0, time_step_sound=6,
+                bc="open", bc_x="open", bc_y="periodic", hybrid_opt=2, sounding="stable", n_ridges=12,
+                ridge_a=(5000.0, 20000.0), ridge_h=(200.0, 1500.0), u0=10.0, brunt=0.01, n_passive=10,
+                n_bubbles=0, noise_amp=0.0, seed=1005)
+    base.update(kw)
+    return DomainSpec(**base)
