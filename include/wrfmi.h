@@ -202,6 +202,10 @@ int wrfmi_calc_p_rho_phi(wrfmi_state *st, const wrfmi_config *cfg, const wrfmi_d
  * species fills at the end of the step (solve_em.F:4461-4540, kind 1). */
 int wrfmi_phys_bc_dry_2(wrfmi_state *st, const wrfmi_config *cfg, const wrfmi_dims *d,
                         int kind, void *stream);
+/* set_w_surface with fill_w_flag false (module_bc_em.F:1196-1295), called at
+ * solve_em.F:4740 after the RK loop: w_2 at level 1 from u_2, v_2 and ht, interior only. */
+int wrfmi_set_w_surface(wrfmi_state *st, const wrfmi_config *cfg, const wrfmi_dims *d,
+                        void *stream);
 /* set_physical_bc3d / set_physical_bc2d (share/module_bc.F:651-1113, 202-647)
  * on one field; var is the reference's staggering letter 'u','v','w','t','p'. */
 int wrfmi_set_physical_bc3d(float *field, char var, const wrfmi_config *cfg,

@@ -3,6 +3,7 @@
  *   set_physical_bc3d  share/module_bc.F:651-1113
  *   set_physical_bc2d  share/module_bc.F:202-647
  *   rk_phys_bc_dry_1   dyn_em/module_bc_em.F:960-1043
+ *   set_w_surface      dyn_em/module_bc_em.F:1196-1295 (fill_w_flag false)
  *   rk_step_prep       dyn_em/module_em.F:37-186 with calculate_full
  *                      (module_big_step_utilities_em.F:3587-3636), calc_mu_uv (26-180),
  *                      couple_momentum (329-399), calc_ww_cp (640-782), calc_cq (787-906),
@@ -159,6 +160,30 @@ static void set_bc(wreal *dat, char var_in, const cfg_t *c, const dims_t *d, int
 
 void o_set_physical_bc3d(wreal *dat, char var, const cfg_t *c, const dims_t *d) { set_bc(dat, var, c, d, 1); }
 void o_set_physical_bc2d(wreal *dat, char var, const cfg_t *c, const dims_t *d) { set_bc(dat, var, c, d, 0); }
+
+/* The kinematic lower boundary condition w(1) = msfty*v.dz/dy + msftx*u.dz/dx, with u and v
+ * extrapolated to the surface by cf1..cf3 and the height differences of ht taken on both
+ * sides of the point; a neighbour across a non-periodic edge is the point itself
+ * (dyn_em/module_bc_em.F:1196-1295, fill_w_flag false: only level 1 is written). */
+void o_set_w_surface(wreal *w, const ostate *s, const cfg_t *c, const dims_t *d) {
+  const wreal *ht = s->ht, *u = s->u_2, *v = s->v_2;
+  const wreal cf1 = s->cf1, cf2 = s->cf2, cf3 = s->cf3, rdx = s->rdx, rdy = s->rdy;
+  int jm1_limit = d->jds, jp1_limit = d->jde - 1, im1_limit = d->ids, ip1_limit = d->ide - 1;
+  if (c->periodic_x) { im1_limit = d->ids - 1; ip1_limit = d->ide; }
+  if (c->periodic_y) { jm1_limit = d->jds - 1; jp1_limit = d->jde; }
+  for (int j = d->jts; j <= IMIN(d->jte, d->jde - 1); j++) {
+    const int jm1 = IMAX(j - 1, jm1_limit), jp1 = IMIN(j + 1, jp1_limit);
+    for (int i = d->its; i <= IMIN(d->ite, d->ide - 1); i++) {
+      const int im1 = IMAX(i - 1, im1_limit), ip1 = IMIN(i + 1, ip1_limit);
+      A3(w, i, 1, j) = A2(s->msfty, i, j) * R(.5) * rdy *
+                           ((A2(ht, i, jp1) - A2(ht, i, j)) * (cf1 * A3(v, i, 1, j + 1) + cf2 * A3(v, i, 2, j + 1) + cf3 * A3(v, i, 3, j + 1)) +
+                            (A2(ht, i, j) - A2(ht, i, jm1)) * (cf1 * A3(v, i, 1, j) + cf2 * A3(v, i, 2, j) + cf3 * A3(v, i, 3, j))) +
+                       A2(s->msftx, i, j) * R(.5) * rdx *
+                           ((A2(ht, ip1, j) - A2(ht, i, j)) * (cf1 * A3(u, i + 1, 1, j) + cf2 * A3(u, i + 1, 2, j) + cf3 * A3(u, i + 1, 3, j)) +
+                            (A2(ht, i, j) - A2(ht, im1, j)) * (cf1 * A3(u, i, 1, j) + cf2 * A3(u, i, 2, j) + cf3 * A3(u, i, 3, j)));
+    }
+  }
+}
 
 /* dyn_em/module_bc_em.F:960-1043 (called with ru, rv, rw, ww, muu, muv, mut, php, alt, p) */
 void o_rk_phys_bc_dry_1(ostate *s, const cfg_t *c, const dims_t *d) {

@@ -29,9 +29,10 @@
  * Parity unpinned (no reference build, fixture or closed form covers them, and the
  * benchmark configurations do not exercise them): the w_damping = 1 branch of w_damp,
  * damp_opt = 3 in advance_w, top_lid = 1, phi_adv_z = 2, hypsometric_opt = 1,
- * use_theta_m = 0, Coriolis with f or e non-zero, map factors other than 1, terrain
- * (the surface-w term of advance_w and the cf1..cf3 extrapolations with ht != 0), and
- * advect_w's lowest and top three levels.
+ * use_theta_m = 0, Coriolis with f or e non-zero, map factors other than 1, and
+ * advect_w's lowest and top three levels.  The terrain terms (the surface-w term of
+ * advance_w and set_w_surface, with their cf1..cf3 extrapolations) are pinned by their
+ * closed form in uniform flow, U * (ht(i+1) - ht(i-1)) / (2 dx) (tests/test_oracle.py).
  */
 #ifndef WRF_ORACLE_H
 #define WRF_ORACLE_H
@@ -102,6 +103,7 @@ void o_set_physical_bc2d(wreal *dat, char var, const cfg_t *c, const dims_t *d);
 /* ---- rk_step_prep and callees ---- */
 void o_rk_step_prep(ostate *s, const cfg_t *c, const dims_t *d, int rk_step);
 void o_rk_phys_bc_dry_1(ostate *s, const cfg_t *c, const dims_t *d);
+void o_set_w_surface(wreal *w, const ostate *s, const cfg_t *c, const dims_t *d);
 
 /* ---- advection (order 5 horizontal / 3 vertical, plus 2..6 rejected) ---- */
 int o_advect_u(const wreal *u, const wreal *u_old, wreal *tend, const wreal *ru, const wreal *rv,

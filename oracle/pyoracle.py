@@ -78,7 +78,7 @@ class OracleSession:
 
     _ARGS = {"rk_step_prep": "i", "rk_tendency": "i", "small_step_prep": "id", "advance_uv": "d",
              "advance_mu_t_w": "dii", "small_step_finish": "iid", "rk_scalar_step": "iid",
-             "calc_p_rho_phi": "", "phys_bc_dry_2": "i"}
+             "calc_p_rho_phi": "", "phys_bc_dry_2": "i", "set_w_surface": ""}
 
     def __init__(self, oracle: Oracle, hs):
         self.o = oracle

@@ -14,6 +14,7 @@
  *   calc_p_rho_phi                                solve_em.F:2959-2979
  *   rk_phys_bc_dry_2 + species fills (rk < 3)     solve_em.F:3240-3340
  *   end-of-step calc_p_rho_phi + set_phys_bc_dry_2 + species fills  solve_em.F:4180-4540
+ *   set_w_surface (surface w from the final u, v and the terrain)   solve_em.F:4734-4750
  * Physics is off: the *_tendf, mu_tendf, h_diabatic and species tendencies are read
  * as given (module_first_rk_step_part1.F:182 zeroes them before the physics fills them).
  */
@@ -124,6 +125,8 @@ int o_solve_em_step(ostate *s, const cfg_t *c, const dims_t *d) {
   o_set_physical_bc2d(s->mu_1, 't', c, d);
   o_set_physical_bc2d(s->mu_2, 't', c, d);
   bc_species(s, c, d);
+  /* solve_em.F:4734-4750: the surface w is reset from the final u, v (interior only) */
+  o_set_w_surface(s->w_2, s, c, d);
   return 0;
 }
 
@@ -206,6 +209,10 @@ int oracle_rk_scalar_step(ostate *s, const cfg_t *c, const dims_t *d, int ispec,
 }
 int oracle_calc_p_rho_phi(ostate *s, const cfg_t *c, const dims_t *d) {
   o_calc_p_rho_phi(s, c, d);
+  return 0;
+}
+int oracle_set_w_surface(ostate *s, const cfg_t *c, const dims_t *d) {
+  o_set_w_surface(s->w_2, s, c, d);
   return 0;
 }
 int oracle_phys_bc_dry_2(ostate *s, const cfg_t *c, const dims_t *d, int kind) {

@@ -89,5 +89,5 @@ def test_step_phase_sequence():
     # 1 + 3 + 6 acoustic substeps for time_step_sound = 6
     assert names.count("advance_mu_t_w") == 10
     assert names.count("rk_scalar_step") == 3
-    assert names[-2:] == ["calc_p_rho_phi", "phys_bc_dry_2"]
+    assert names[-3:] == ["calc_p_rho_phi", "phys_bc_dry_2", "set_w_surface"]
     assert runtime.num_sound_steps(type(hs_cfg)(), 6.0, 1000.0, 1000.0) == 4

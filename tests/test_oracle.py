@@ -320,3 +320,60 @@ def test_advect_w_vertical_equals_pinned_advect_scalar(oracle32, ts, W):
     sl = (slice(j0, j0 + spec.ny), slice(1, d.kde - 3), slice(i0, i0 + spec.nx))
     assert np.count_nonzero(tsc[sl]) > 0.9 * tsc[sl].size
     assert np.array_equal(tw[sl], tsc[sl])
+
+
+@pytest.mark.parametrize("bc_x,bc_y", [("open", "periodic"), ("periodic", "open")])
+def test_set_w_surface_closed_form(oracle32, bc_x, bc_y):
+    """set_w_surface with u = U, v = 0 and unit map factors reduces to the centred slope
+    w(1) = U * (ht(i+1) - ht(i-1)) / (2 dx), and across an open edge to the one-sided
+    U * (ht(ids+1) - ht(ids)) / (2 dx); cf1 + cf2 + cf3 = 1 extrapolates a uniform u
+    exactly up to rounding.  The terrain initial state's w (start_em's call with the fill
+    w(k) = w(1) znw(k)^2) is the oracle's level-1 value bit for bit."""
+    spec = grid.terrain_c4(nx=30, ny=24, nz=10, dx=1000.0, dy=1000.0, bc_x=bc_x, bc_y=bc_y, ridge_a=(3000.0, 8000.0),
+                           n_passive=0, u0=12.5)
+    hs = grid.build_initial_state(spec)
+    d = hs.dims
+    sess = oracle32.session(hs)
+    sess.arrays["w_2"][...] = 0.0
+    sess.phase("set_w_surface")
+    w1 = interior(sess.arrays["w_2"], d, "w")[:, 0, :].astype(np.float64)
+    ht = hs.a2["ht"].astype(np.float64)
+    i0, j0 = d.ids - d.ims, d.jds - d.jms
+    nx, ny = spec.nx, spec.ny
+    ii = np.arange(nx) + i0
+    hp = ht[j0:j0 + ny][:, ii + 1]
+    hm = ht[j0:j0 + ny][:, ii - 1]
+    if bc_x == "open":
+        hm[:, 0] = ht[j0:j0 + ny, i0]
+        hp[:, -1] = ht[j0:j0 + ny, i0 + nx - 1]
+    exp = spec.u0 * (hp - hm) / (2.0 * spec.dx)
+    assert np.max(np.abs(exp)) > 0.5
+    assert np.max(np.abs(w1 - exp)) <= 1e-5 * np.max(np.abs(exp)), np.max(np.abs(w1 - exp))
+    # the initial state of both time levels carries the same surface w and its fill
+    w_init = interior(hs.a3["w_2"], d, "w")
+    assert np.array_equal(w_init[:, 0, :], interior(sess.arrays["w_2"], d, "w")[:, 0, :])
+    znw = hs.a1["znw"]
+    k = 5
+    assert np.array_equal(w_init[:, k, :], w_init[:, 0, :] * znw[k] * znw[k])
+    assert np.array_equal(hs.a3["w_1"], hs.a3["w_2"])
+
+
+def test_advance_w_surface_term_closed_form(oracle32):
+    """advance_w's lower boundary (module_small_step_em.F:1383-1395) has the same form as
+    set_w_surface on the raw i-1, i+1 and j-1, j+1 neighbours (halo values across an edge).
+    With the small-step u = U and v = 0 it is U * (ht(i+1) - ht(i-1)) / (2 dx)."""
+    spec = grid.terrain_c4(nx=26, ny=20, nz=10, dx=1000.0, dy=1000.0, ridge_a=(3000.0, 8000.0), n_passive=0)
+    hs = grid.build_initial_state(spec)
+    d = hs.dims
+    sess = oracle32.session(hs)
+    U = 7.25
+    sess.arrays["u_2"][...] = U
+    sess.arrays["v_2"][...] = 0.0
+    sess.phase("advance_mu_t_w", 0.5, 1, 1)
+    w1 = interior(sess.arrays["w_2"], d, "w")[:, 0, :].astype(np.float64)
+    ht = hs.a2["ht"].astype(np.float64)
+    i0, j0 = d.ids - d.ims, d.jds - d.jms
+    ii = np.arange(spec.nx) + i0
+    exp = U * (ht[j0:j0 + spec.ny][:, ii + 1] - ht[j0:j0 + spec.ny][:, ii - 1]) / (2.0 * spec.dx)
+    assert np.max(np.abs(exp)) > 0.3
+    assert np.max(np.abs(w1 - exp)) <= 1e-5 * np.max(np.abs(exp)), np.max(np.abs(w1 - exp))

@@ -294,6 +294,35 @@ __global__ void __launch_bounds__(256) k_ww_scan(ARGS args) {
   }
 }
 
+// set_w_surface with fill_w_flag false (dyn_em/module_bc_em.F:1196-1295): the kinematic
+// lower boundary condition w(1) = msfty v.dz/dy + msftx u.dz/dx, with u and v extrapolated
+// to the surface by cf1..cf3; across a non-periodic edge the neighbour is the point itself.
+// One thread per interior mass column.
+template <typename ARGS>
+__global__ void __launch_bounds__(256) k_set_w_surface(ARGS args) {
+  Tile3 T;
+  const KArgs &a = kargs(args, T);
+  const Dm &d = a.d;
+  const DState &s = dstate(a);
+  const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
+  const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
+  if (i > d.ide - 1 || j > d.jde - 1) return;
+  const bool px = a.c.periodic_x, py = a.c.periodic_y;
+  const int jm1 = imax(j - 1, py ? d.jds - 1 : d.jds), jp1 = imin(j + 1, py ? d.jde : d.jde - 1);
+  const int im1 = imax(i - 1, px ? d.ids - 1 : d.ids), ip1 = imin(i + 1, px ? d.ide : d.ide - 1);
+  const FPtr<float> u = s.u_2, v = s.v_2, ht = s.ht;
+  const float cf1 = s.cf1, cf2 = s.cf2, cf3 = s.cf3;
+  const Off o2 = X2(d, i, j);
+  const float h0 = ht[o2];
+  const float vs_n = cf1 * v[X3(d, i, 1, j + 1)] + cf2 * v[X3(d, i, 2, j + 1)] + cf3 * v[X3(d, i, 3, j + 1)];
+  const float vs_s = cf1 * v[X3(d, i, 1, j)] + cf2 * v[X3(d, i, 2, j)] + cf3 * v[X3(d, i, 3, j)];
+  const float us_e = cf1 * u[X3(d, i + 1, 1, j)] + cf2 * u[X3(d, i + 1, 2, j)] + cf3 * u[X3(d, i + 1, 3, j)];
+  const float us_w = cf1 * u[X3(d, i, 1, j)] + cf2 * u[X3(d, i, 2, j)] + cf3 * u[X3(d, i, 3, j)];
+  s.w_2[X3(d, i, 1, j)] = s.msfty[o2] * .5f * s.rdy * ((ht[X2(d, i, jp1)] - h0) * vs_n + (h0 - ht[X2(d, i, jm1)]) * vs_s) +
+                          s.msftx[o2] * .5f * s.rdx * ((ht[X2(d, ip1, j)] - h0) * us_e + (h0 - ht[X2(d, im1, j)]) * us_w);
+}
+
+WRFMI_INST(k_set_w_surface)
 WRFMI_INST_P(k_ww_scan, false)
 WRFMI_INST_P(k_ww_scan, true)
 

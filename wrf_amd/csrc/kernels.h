@@ -51,6 +51,7 @@ template <typename A> __global__ void k_update_scalar_all_x4(A);
 template <typename A> __global__ void k_p_rho_phi(A);
 template <typename A> __global__ void k_p_rho_phi_x4(A);
 template <typename A> __global__ void k_halo(A);
+template <typename A> __global__ void k_set_w_surface(A);
 __global__ void k_count_nonfinite(const float *a, Dm d, int kend, int is3d, unsigned long long *cnt);
 }  // namespace wrfmi
 

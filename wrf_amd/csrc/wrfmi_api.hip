@@ -263,12 +263,12 @@ static const dim3 kColBlk(64, 4, 1);
 enum KernelId {
   KID_STEP_PREP, KID_TEND_U, KID_TEND_V, KID_TEND_W, KID_TEND_T, KID_SMALL_PREP, KID_ADVANCE_UV,
   KID_AC_DIV, KID_SMALL_FINISH, KID_ADV_SCALAR, KID_UPDATE_SCALAR, KID_P_RHO_PHI, KID_HALO, KID_P_RHO0,
-  KID_COEF_W, KID_WW_SCAN, KID_AC_MU_WW, KID_AC_THETA, KID_AC_WPRED, KID_AC_WSOLVE, KID_COUNT
+  KID_COEF_W, KID_WW_SCAN, KID_AC_MU_WW, KID_AC_THETA, KID_AC_WPRED, KID_AC_WSOLVE, KID_W_SURFACE, KID_COUNT
 };
 static const char *kKernelNames[KID_COUNT] = {
   "k_step_prep", "k_tend_u", "k_tend_v", "k_tend_w", "k_tend_t", "k_small_prep", "k_advance_uv",
   "k_ac_div", "k_small_finish", "k_adv_scalar", "k_update_scalar", "k_p_rho_phi", "k_halo", "k_p_rho0",
-  "k_coef_w", "k_ww_scan", "k_ac_mu_ww", "k_ac_theta", "k_ac_wpred", "k_ac_wsolve"};
+  "k_coef_w", "k_ww_scan", "k_ac_mu_ww", "k_ac_theta", "k_ac_wpred", "k_ac_wsolve", "k_set_w_surface"};
 struct EvPair { int id; hipEvent_t a, b; };
 static unsigned g_timing = 0;  // bit n set: time launches of kernel n
 static std::vector<EvPair> g_ev_pending;
@@ -532,6 +532,11 @@ static void ph_bc_dry_2(Launcher &L, int kind) {
   L.halo_flush();
 }
 
+// set_w_surface after the RK loop (solve_em.F:4734-4750): interior columns only, no fill
+static void ph_w_surface(Launcher &L) {
+  L.launchK(KID_W_SURFACE, KR(k_set_w_surface), g_pt3(L.a.d, 1), kColBlk, L.a);
+}
+
 // RK substep sizes of solve_em.F:601-615
 static void set_substep(KArgs &a, int rk, int nsound) {
   const float dt = a.s.dt;
@@ -568,8 +573,10 @@ static void enqueue_step(Launcher &L) {
     if (rk < 3) ph_bc_dry_2(L, 0);
   }
   // solve_em.F:4195 recomputes calc_p_rho_phi from inputs that have not changed since the
-  // last substep's call (same values), then set_phys_bc_dry_2 and the species fills follow.
+  // last substep's call (same values), then set_phys_bc_dry_2 and the species fills follow,
+  // and the surface w is reset from the final u, v (solve_em.F:4740).
   ph_bc_dry_2(L, 1);
+  ph_w_surface(L);
 }
 
 // The four-points-per-lane kernels read 16-byte groups starting at an i whose offset in the
@@ -1017,6 +1024,13 @@ extern "C" int wrfmi_phys_bc_dry_2(wrfmi_state *st, const wrfmi_config *cfg, con
   return run_phase(st, cfg, d, stream, [&](Launcher &L) {
     if (kind != 0 && kind != 1) return fail(WRFMI_E_BADOPT, "kind must be 0 or 1");
     ph_bc_dry_2(L, kind);
+    return 0;
+  });
+}
+
+extern "C" int wrfmi_set_w_surface(wrfmi_state *st, const wrfmi_config *cfg, const wrfmi_dims *d, void *stream) {
+  return run_phase(st, cfg, d, stream, [&](Launcher &L) {
+    ph_w_surface(L);
     return 0;
   });
 }

@@ -739,7 +739,43 @@ def build_terrain_state(spec: DomainSpec) -> HostState:
         first = ["t_1", "ph_1", "mu_1"]
         list(pool.map(finish, first))
         list(pool.map(finish, [n for n in names if n not in first]))
+    # start_em.F:1480-1531: w of both time levels from the kinematic surface condition
+    for w, u, v in (("w_1", "u_1", "v_1"), ("w_2", "u_2", "v_2")):
+        set_w_surface_np(hs, A3[w], A3[u], A3[v], fill=True)
+        _halo_fill_np(A3[w], d, per, False, False)
     return hs
+
+
+def set_w_surface_np(hs: HostState, w, u, v, fill: bool):
+    """set_w_surface (dyn_em/module_bc_em.F:1196-1295) in float32, as start_em calls it
+    (start_em.F:1520-1531): w(1) = msfty*.5*rdy*(dht_n*v_n + dht_s*v_s) + msftx*.5*rdx*(...)
+    with u, v extrapolated to the surface by cf1..cf3, a neighbour across a non-periodic edge
+    being the point itself; with `fill`, w(k) = w(1)*znw(k)*znw(k) above.  Interior only."""
+    d, c = hs.dims, hs.cfg
+    f32 = np.float32
+    cf1, cf2, cf3 = (f32(hs.scal[n]) for n in ("cf1", "cf2", "cf3"))
+    rdx, rdy = f32(hs.scal["rdx"]), f32(hs.scal["rdy"])
+    i = np.arange(d.ids, d.ide)                 # i = ids..ide-1
+    j = np.arange(d.jds, d.jde)
+    im1 = np.maximum(i - 1, d.ids - 1 if c.periodic_x else d.ids) - d.ims
+    ip1 = np.minimum(i + 1, d.ide if c.periodic_x else d.ide - 1) - d.ims
+    jm1 = np.maximum(j - 1, d.jds - 1 if c.periodic_y else d.jds) - d.jms
+    jp1 = np.minimum(j + 1, d.jde if c.periodic_y else d.jde - 1) - d.jms
+    ii, jj = i - d.ims, j - d.jms
+    ht, msftx, msfty = hs.a2["ht"], hs.a2["msftx"], hs.a2["msfty"]
+    h0 = ht[np.ix_(jj, ii)]
+    srf = lambda a, jsel, isel: cf1 * a[jsel, 0][:, isel] + cf2 * a[jsel, 1][:, isel] + cf3 * a[jsel, 2][:, isel]
+    vs_n = srf(v, jj + 1, ii)
+    vs_s = srf(v, jj, ii)
+    us_e = srf(u, jj, ii + 1)
+    us_w = srf(u, jj, ii)
+    w1 = (msfty[np.ix_(jj, ii)] * f32(.5) * rdy * ((ht[np.ix_(jp1, ii)] - h0) * vs_n + (h0 - ht[np.ix_(jm1, ii)]) * vs_s) +
+          msftx[np.ix_(jj, ii)] * f32(.5) * rdx * ((ht[np.ix_(jj, ip1)] - h0) * us_e + (h0 - ht[np.ix_(jj, im1)]) * us_w))
+    w[jj[0]:jj[-1] + 1, 0, ii[0]:ii[-1] + 1] = w1
+    if fill:
+        znw = hs.a1["znw"]
+        for k in range(1, d.kde - d.kms + 1):   # Fortran k = kts+1..kte
+            w[jj[0]:jj[-1] + 1, k, ii[0]:ii[-1] + 1] = w1 * znw[k] * znw[k]
 
 
 # ----------------------------------------------------------------------------- presets

@@ -46,6 +46,7 @@ PHASE_ARGS = {
     "rk_scalar_step": "iif",      # ispec, rk_step, dt_rk
     "calc_p_rho_phi": "",
     "phys_bc_dry_2": "i",         # kind
+    "set_w_surface": "",
 }
 
 
@@ -86,6 +87,7 @@ def step_phases(cfg, scal: dict, n_species: int):
             out.append(("phys_bc_dry_2", (0,)))
     out.append(("calc_p_rho_phi", ()))
     out.append(("phys_bc_dry_2", (1,)))
+    out.append(("set_w_surface", ()))
     return out
 
 
