@@ -24,6 +24,9 @@ CASES = {
     # rows of 32 floats ending 6 points after ide: the four-point windows run past the row
     "periodic_tight": dict(nx=21, ny=9, nz=8, bc="periodic", n_passive=1, n_bubbles=1, seed=17, pad_i=4),
     "open_tight": dict(nx=21, ny=10, nz=8, bc="open", n_bubbles=1, seed=18, pad_i=4),
+    # rows of 101 and 256 points (one and exactly one 256-point wave row per grid row)
+    "periodic_wide": dict(nx=100, ny=7, nz=6, bc="periodic", n_passive=1, n_bubbles=1, seed=23),
+    "open_wide": dict(nx=255, ny=10, nz=5, bc="open", n_bubbles=1, seed=24),
     # terrain on hybrid levels with one periodic and one open direction, both ways round
     "terrain": dict(nx=16, ny=12, nz=10, dx=2000.0, dy=2000.0, dt=6.0, bc_x="open", bc_y="periodic",
                     hybrid_opt=2, sounding="stable", n_ridges=3, ridge_a=(3000.0, 8000.0), n_passive=1,

@@ -28,7 +28,8 @@ def emu_env():
 @pytest.mark.parametrize("case,order,nsteps", [("periodic", 0, 2), ("periodic", 1, 1), ("open", 0, 2), ("open", 1, 1),
                                                ("periodic_unaligned", 0, 1), ("open_unaligned", 1, 1),
                                                ("periodic_tight", 1, 1), ("open_tight", 0, 1),
-                                               ("terrain", 0, 2), ("terrain_yopen", 1, 1)])
+                                               ("terrain", 0, 2), ("terrain_yopen", 1, 1),
+                                               ("periodic_wide", 1, 1), ("open_wide", 0, 1)])
 def test_emulated_kernels_match_oracle(emu_env, case, order, nsteps):
     r = subprocess.run([sys.executable, os.path.join(HERE, "emu_phase_check.py"), case, str(order), str(nsteps)],
                        env=emu_env, capture_output=True, text=True, timeout=600)
@@ -49,3 +50,4 @@ def test_emulated_ensemble_matches_single_domains(emu_env, order):
                        env=emu_env, capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stdout[-4000:] + r.stderr[-4000:]
     assert "OK ensemble" in r.stdout
+

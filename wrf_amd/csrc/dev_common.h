@@ -246,6 +246,52 @@ DEVI void st4m(const FPtr<float> &p, Off o, const F4 &v, int i0, int lo, int hi)
       if (i0 + n >= lo && i0 + n <= hi) p[o + n] = v.v[n];
   }
 }
+// Boundary zone of a domain with an open direction.  Within kZone points of an open edge
+// the reference switches to lower-order fluxes and adds radiation terms; a four-point
+// kernel stores only the points outside the zone (zlo/zhi in i, zrow in j), and the
+// one-point kernel's zone form computes the zone itself: columns ids..ids+kZone-1 and
+// ide-kZone..ide (every row) if x is open, then rows jds..jds+kZone-1 and jde-kZone..jde
+// (the remaining columns) if y is open.  Point n of the zone is zone_point(n).
+constexpr int kZone = 4;
+DEVI int zlo(const KArgs &a, int lo) { return a.c.periodic_x ? lo : a.d.ids + kZone; }
+DEVI int zhi(const KArgs &a, int hi) { return a.c.periodic_x ? hi : a.d.ide - 1 - kZone; }
+DEVI bool zrow(const KArgs &a, int j) { return a.c.periodic_y || (j >= a.d.jds + kZone && j <= a.d.jde - 1 - kZone); }
+HDI unsigned zone_count(const Dm &d, bool px, bool py) {
+  const int zx = px ? 0 : kZone, zy = py ? 0 : kZone;
+  const int cx = zx ? 2 * zx + 1 : 0, ry = zy ? 2 * zy + 1 : 0;
+  const int ncol = d.ide - d.ids + 1, nrow = d.jde - d.jds + 1;
+  return (unsigned)(cx * nrow + ry * (ncol - cx));
+}
+DEVI bool zone_point(const Dm &d, bool px, bool py, unsigned n, int &i, int &j) {
+  const int zx = px ? 0 : kZone, zy = py ? 0 : kZone;
+  const int cx = zx ? 2 * zx + 1 : 0, ry = zy ? 2 * zy + 1 : 0;
+  const int ncol = d.ide - d.ids + 1, nrow = d.jde - d.jds + 1;
+  const unsigned na = (unsigned)(cx * nrow);
+  if (n < na) {
+    const int c = (int)(n % (unsigned)cx), r = (int)(n / (unsigned)cx);
+    i = c < zx ? d.ids + c : d.ide - zx + (c - zx);
+    j = d.jds + r;
+    return true;
+  }
+  n -= na;
+  const int ic = ncol - cx;
+  if (ry == 0 || ic <= 0 || n >= (unsigned)(ry * ic)) return false;
+  const int c = (int)(n % (unsigned)ic), r = (int)(n / (unsigned)ic);
+  i = d.ids + zx + c;
+  j = r < zy ? d.jds + r : d.jde - zy + (r - zy);
+  return true;
+}
+// (i, j) of a one-point kernel's thread: its tile position, or point n of the zone
+template <bool ZONE>
+DEVI bool point_ij(const KArgs &a, const Tile3 &T, int &i, int &j) {
+  if (!ZONE) {
+    i = a.d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
+    j = a.d.jds + (int)(T.by * blockDim.y + threadIdx.y);
+    return true;
+  }
+  const unsigned n = ((unsigned)T.bx * blockDim.y + threadIdx.y) * blockDim.x + threadIdx.x;
+  return zone_point(a.d, a.c.periodic_x, a.c.periodic_y, n, i, j);
+}
 // neighbours in i of point n from the lane's group c and the groups l (i0-4..) and r (i0+4..)
 DEVI float xm1(const F4 &l, const F4 &c, int n) { return n ? c.v[n - 1] : l.v[3]; }
 DEVI float xp1(const F4 &c, const F4 &r, int n) { return n < 3 ? c.v[n + 1] : r.v[0]; }

@@ -15,16 +15,19 @@ namespace wrfmi {
 // computes the y flux at its face j and reads face j+1 from the row above through LDS (the
 // top row also computes face j+4); the vertical flux is carried up the column.  Every flux
 // is the same expression, in the same order, as in advect_scalar.
-template <typename ARGS, bool PER>
+template <typename ARGS, bool PER, bool ZONE>
 __global__ void __launch_bounds__(256) k_adv_scalar(ARGS args) {
   Tile3 T;
   const KArgs &a = kargs(args, T);
   const Dm &d = a.d;
   const DState &s = dstate(a);
   const int lane = (int)threadIdx.x, row = (int)threadIdx.y;
-  const int i = d.ids + T.bx * kAdvTileI + lane;
-  const int j = d.jds + T.by * (int)blockDim.y + row;
-  const int k0 = d.kds + T.bz * kAdvK;
+  int i = d.ids + T.bx * kAdvTileI + lane;
+  int j = d.jds + T.by * (int)blockDim.y + row;
+  // the zone form: one point per thread, which forms both faces of its divergence itself
+  if (ZONE && !point_ij<true>(a, T, i, j)) return;
+  constexpr int KS = ZONE ? 1 : kAdvK;  // levels per thread (the zone form: one)
+  const int k0 = d.kds + T.bz * KS;
   const int ids = d.ids, ide = d.ide, jds = d.jds, jde = d.jde, kde = d.kde;
   const int its = ids, ite = ide, jts = jds, jte = jde, kts = d.kds, kte = kde;
   const wrfmi_config &c = a.c;
@@ -35,7 +38,7 @@ __global__ void __launch_bounds__(256) k_adv_scalar(ARGS args) {
   FPtr<float> tend = s.advect_tend;
   const float rdx = s.rdx, rdy = s.rdy;
   const bool inside = i <= ide && j <= jde;          // a point of the u/v-staggered patch
-  const bool writer = inside && lane < kAdvTileI;   // lane 63 of each row only computes a face
+  const bool writer = inside && (ZONE || lane < kAdvTileI);  // lane 63 of each row only computes a face
   const bool mass = i <= ide - 1 && j <= jde - 1;
   const bool dxs = !(g.per_x || g.sym_xs || its > ids + 3), dxe = !(g.per_x || g.sym_xe || ite < ide - 3);
   const bool dys = !(g.per_y || g.sym_ys || jts > jds + 3), dye = !(g.per_y || g.sym_ye || jte < jde - 4);
@@ -86,17 +89,25 @@ __global__ void __launch_bounds__(256) k_adv_scalar(ARGS args) {
   const float msftx = mass ? s.msftx[X2(d, i, j)] : 0.f;
   __shared__ float fy_s[2][kAdvRows + 1][64];
   float vf_lo = (mass && k0 <= ktf) ? vflux(k0) : 0.f;
-  for (int kk = 0; kk < kAdvK; kk++) {
+  for (int kk = 0; kk < KS; kk++) {
     const int k = k0 + kk;
     if (k > kte) break;  // uniform over the block
-    const float fx = (fx_on && k <= ktf) ? fqx(k, i) : 0.f;
-    const float fx_up = __shfl_down(fx, 1, 64);
-    const float fy = (fy_on && k <= ktf) ? fqy(k, j) : 0.f;
-    float(*buf)[64] = fy_s[kk & 1];
-    buf[row][lane] = fy;
-    if (row == (int)blockDim.y - 1) buf[row + 1][lane] = (fy2_on && k <= ktf) ? fqy(k, j + 1) : 0.f;
-    __syncthreads();
-    const float fy_up = buf[row + 1][lane];
+    float fx, fx_up, fy, fy_up;
+    if (ZONE) {
+      fx = (xdiv && k <= ktf) ? fqx(k, i) : 0.f;
+      fx_up = (xdiv && k <= ktf) ? fqx(k, i + 1) : 0.f;
+      fy = (ydiv && k <= ktf) ? fqy(k, j) : 0.f;
+      fy_up = (ydiv && k <= ktf) ? fqy(k, j + 1) : 0.f;
+    } else {
+      fx = (fx_on && k <= ktf) ? fqx(k, i) : 0.f;
+      fx_up = __shfl_down(fx, 1, 64);
+      fy = (fy_on && k <= ktf) ? fqy(k, j) : 0.f;
+      float(*buf)[64] = fy_s[kk & 1];
+      buf[row][lane] = fy;
+      if (row == (int)blockDim.y - 1) buf[row + 1][lane] = (fy2_on && k <= ktf) ? fqy(k, j + 1) : 0.f;
+      __syncthreads();
+      fy_up = buf[row + 1][lane];
+    }
     if (writer) {
       float t = 0.f;
       if (mass && k <= ktf) {
@@ -146,10 +157,11 @@ __global__ void __launch_bounds__(256) k_adv_scalar_x4(ARGS args) {
   const DState &s = dstate(a);
   const int ia0 = d.ims + 4 * ((d.ids - d.ims) >> 2);
   const int i0 = ia0 + kVecW * (T.bx * 64 + (int)threadIdx.x);
-  const int j = d.jds + T.by * (int)blockDim.y + (int)threadIdx.y;
+  const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
   const int k0 = d.kds + T.bz * kAdvVK;
   const int ids = d.ids, ide = d.ide, jde = d.jde, kts = d.kds, kte = d.kde, ktf = kte - 1;
-  if (i0 > ide || j > jde) return;
+  if (i0 > ide || j > jde || !zrow(a, j)) return;  // rows of an open y edge: the zone form
+  const int ilo = zlo(a, ids), ihi = zhi(a, ide);
   const int k1 = imin(k0 + kAdvVK - 1, kte);
   const float ts = a.c.time_step >= 0 ? 1.f : -1.f;
   const FPtr<float> f = s.q[a.species], ru = s.ru_m, rv = s.rv_m, rom = s.ww_m;
@@ -222,12 +234,12 @@ __global__ void __launch_bounds__(256) k_adv_scalar_x4(ARGS args) {
     vf_lo = vf_hi;
     if (k <= k1) {
       const Off os = at(k);
-      if (i0 >= ids && i0 + 3 <= ide) {
+      if (i0 >= ilo && i0 + 3 <= ihi) {
         tend.st<F4>(os, t);
       } else {
 #pragma unroll
         for (int n = 0; n < 4; n++)
-          if (i0 + n >= ids && i0 + n <= ide) tend[os + n] = t.v[n];
+          if (i0 + n >= ilo && i0 + n <= ihi) tend[os + n] = t.v[n];
       }
     }
   }
@@ -596,8 +608,9 @@ __global__ void __launch_bounds__(256) k_halo(ARGS args) {
     if (k0 + kk <= kend) F.p[X3i(d, i, k0 + kk, j)] = v[kk];
 }
 
-WRFMI_INST_P(k_adv_scalar, false)
-WRFMI_INST_P(k_adv_scalar, true)
+WRFMI_INST_P2(k_adv_scalar, false, false)
+WRFMI_INST_P2(k_adv_scalar, true, false)
+WRFMI_INST_P2(k_adv_scalar, false, true)
 
 WRFMI_INST(k_adv_scalar_x4)
 WRFMI_INST(k_update_scalar)

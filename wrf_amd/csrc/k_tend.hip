@@ -17,19 +17,20 @@
 namespace wrfmi {
 
 // ------------------------------------------------------------------------------ u
-template <typename ARGS, bool PER>
+template <typename ARGS, bool PER, bool ZONE>
 __global__ void __launch_bounds__(256) k_tend_u(ARGS args) {
   Tile3 T;
   const KArgs &a = kargs(args, T);
   const Dm &d = a.d;
   const DState &s = dstate(a);
-  const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
-  const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
-  const int k0 = d.kds + (int)T.bz * kTendK;
+  int i, j;
+  if (!point_ij<ZONE>(a, T, i, j)) return;
+  constexpr int KS = ZONE ? 1 : kTendK;  // levels per thread (the zone form: one)
+  const int k0 = d.kds + (int)T.bz * KS;
   const int ids = d.ids, ide = d.ide, jds = d.jds, jde = d.jde, kde = d.kde;
   const int its = ids, ite = ide, jts = jds, jte = jde, kts = d.kds, kte = kde;
   if (i > ide || j > jde || k0 > kte) return;
-  const int k1 = imin(k0 + kTendK - 1, kte);
+  const int k1 = imin(k0 + KS - 1, kte);
   const Geo g = geo<PER>(a.c);
   const float ts = a.c.time_step >= 0 ? 1.f : -1.f;
   const int ktf = kte - 1;
@@ -254,6 +255,7 @@ __global__ void __launch_bounds__(256) k_tend_u_x4(ARGS args) {
   const DState &s = dstate(a);
   const int i0 = vec_ia0(d) + kVecW * (T.bx * 64 + (int)threadIdx.x);
   const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
+  if (!zrow(a, j)) return;  // rows of an open y edge: the zone form
   const int k = d.kds + (int)T.bz;
   const int ids = d.ids, ide = d.ide, jde = d.jde, kde = d.kde, kts = d.kds, kte = kde, ktf = kte - 1;
   if (i0 > ide || j > jde || k > kte) return;
@@ -423,25 +425,26 @@ __global__ void __launch_bounds__(256) k_tend_u_x4(ARGS args) {
       if (rk1) tf.v[n] = tf.v[n] + 0.f * msf.v[n];  // u_save was zeroed by rk_tendency (module_em.F:395)
       t.v[n] = t.v[n] + tf.v[n] / msf.v[n];
     }
-    if (rk1) st4m(s.ru_tendf, o, tf, i0, ids, ide);
+    if (rk1) st4m(s.ru_tendf, o, tf, i0, zlo(a, ids), zhi(a, ide));
   }
-  st4m(s.ru_tend, o, t, i0, ids, ide);
+  st4m(s.ru_tend, o, t, i0, zlo(a, ids), zhi(a, ide));
 }
 
 // ------------------------------------------------------------------------------ v
-template <typename ARGS, bool PER>
+template <typename ARGS, bool PER, bool ZONE>
 __global__ void __launch_bounds__(256) k_tend_v(ARGS args) {
   Tile3 T;
   const KArgs &a = kargs(args, T);
   const Dm &d = a.d;
   const DState &s = dstate(a);
-  const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
-  const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
-  const int k0 = d.kds + (int)T.bz * kTendK;
+  int i, j;
+  if (!point_ij<ZONE>(a, T, i, j)) return;
+  constexpr int KS = ZONE ? 1 : kTendK;  // levels per thread (the zone form: one)
+  const int k0 = d.kds + (int)T.bz * KS;
   const int ids = d.ids, ide = d.ide, jds = d.jds, jde = d.jde, kde = d.kde;
   const int its = ids, ite = ide, jts = jds, jte = jde, kts = d.kds, kte = kde;
   if (i > ide || j > jde || k0 > kte) return;
-  const int k1 = imin(k0 + kTendK - 1, kte);
+  const int k1 = imin(k0 + KS - 1, kte);
   const Geo g = geo<PER>(a.c);
   const float ts = a.c.time_step >= 0 ? 1.f : -1.f;
   const int ktf = kte - 1;
@@ -644,6 +647,7 @@ __global__ void __launch_bounds__(256) k_tend_v_x4(ARGS args) {
   const DState &s = dstate(a);
   const int i0 = vec_ia0(d) + kVecW * (T.bx * 64 + (int)threadIdx.x);
   const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
+  if (!zrow(a, j)) return;  // rows of an open y edge: the zone form
   const int k = d.kds + (int)T.bz;
   const int ids = d.ids, ide = d.ide, jde = d.jde, kde = d.kde, kts = d.kds, kte = kde, ktf = kte - 1;
   if (i0 > ide || j > jde || k > kte) return;
@@ -808,29 +812,30 @@ __global__ void __launch_bounds__(256) k_tend_v_x4(ARGS args) {
       if (rk1) tf.v[n] = tf.v[n] + 0.f * mvx.v[n];  // v_save was zeroed by rk_tendency
       t.v[n] = t.v[n] + tf.v[n] * mvi.v[n];
     }
-    if (rk1) st4m(s.rv_tendf, o, tf, i0, ids, ide - 1);
+    if (rk1) st4m(s.rv_tendf, o, tf, i0, zlo(a, ids), zhi(a, ide - 1));
   }
   // col_ok: only i <= ide-1 carry the tendency; i = ide is stored as zero
 #pragma unroll
   for (int n = 0; n < 4; n++)
     if (i0 + n > ide - 1) t.v[n] = 0.f;
-  st4m(s.rv_tend, o, t, i0, ids, ide);
+  st4m(s.rv_tend, o, t, i0, zlo(a, ids), zhi(a, ide));
 }
 
 // ------------------------------------------------------------------------------ w
-template <typename ARGS, bool PER>
+template <typename ARGS, bool PER, bool ZONE>
 __global__ void __launch_bounds__(256) k_tend_w(ARGS args) {
   Tile3 T;
   const KArgs &a = kargs(args, T);
   const Dm &d = a.d;
   const DState &s = dstate(a);
-  const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
-  const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
-  const int k0 = d.kds + (int)T.bz * kTendK;
+  int i, j;
+  if (!point_ij<ZONE>(a, T, i, j)) return;
+  constexpr int KS = ZONE ? 1 : kTendK;  // levels per thread (the zone form: one)
+  const int k0 = d.kds + (int)T.bz * KS;
   const int ids = d.ids, ide = d.ide, jds = d.jds, jde = d.jde, kde = d.kde;
   const int its = ids, ite = ide, jts = jds, jte = jde, kts = d.kds, kte = kde;
   if (i > ide || j > jde || k0 > kte) return;
-  const int k1 = imin(k0 + kTendK - 1, kte);
+  const int k1 = imin(k0 + KS - 1, kte);
   const Geo g = geo<PER>(a.c);
   const float ts = a.c.time_step >= 0 ? 1.f : -1.f;
   const int ktf = kte - 1;
@@ -1025,6 +1030,7 @@ __global__ void __launch_bounds__(256) k_tend_w_x4(ARGS args) {
   const DState &s = dstate(a);
   const int i0 = vec_ia0(d) + kVecW * (T.bx * 64 + (int)threadIdx.x);
   const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
+  if (!zrow(a, j)) return;  // rows of an open y edge: the zone form
   const int k = d.kds + (int)T.bz;
   const int ids = d.ids, ide = d.ide, jde = d.jde, kde = d.kde, kts = d.kds, kte = kde, ktf = kte - 1;
   if (i0 > ide || j > jde || k > kte) return;
@@ -1042,7 +1048,7 @@ __global__ void __launch_bounds__(256) k_tend_w_x4(ARGS args) {
 #pragma unroll
   for (int n = 0; n < 4; n++) t.v[n] = 0.f;
   if (!jmass || i0 > ide - 1) {
-    st4m(s.rw_tend, o, t, i0, ids, ide);
+    st4m(s.rw_tend, o, t, i0, zlo(a, ids), zhi(a, ide));
     return;
   }
   const F4 msfty = s.msfty.ld<F4>(o2), msftx = s.msftx.ld<F4>(o2);
@@ -1154,7 +1160,7 @@ __global__ void __launch_bounds__(256) k_tend_w_x4(ARGS args) {
           t.v[n] = t.v[n] + (1.f / msfty.v[n]) * K_G *
                                 (cq1 * rdn * (pc.v[n] - pm.v[n]) - (c1f * mu.v[n]) - cq2 * (c1f * mub.v[n] + c2f));
         }
-        st4m(s.cqw, o, cq1v, i0, ids, ide - 1);
+        st4m(s.cqw, o, cq1v, i0, zlo(a, ids), zhi(a, ide - 1));
       }
     }
     // w_damp
@@ -1198,28 +1204,29 @@ __global__ void __launch_bounds__(256) k_tend_w_x4(ARGS args) {
       if (rk1) tf.v[n] = tf.v[n] + 0.f * msfty.v[n];  // w_save was zeroed by rk_tendency
       t.v[n] = t.v[n] + tf.v[n] / msfty.v[n];
     }
-    if (rk1) st4m(s.rw_tendf, o, tf, i0, ids, ide - 1);
+    if (rk1) st4m(s.rw_tendf, o, tf, i0, zlo(a, ids), zhi(a, ide - 1));
   }
 #pragma unroll
   for (int n = 0; n < 4; n++)
     if (i0 + n > ide - 1) t.v[n] = 0.f;
-  st4m(s.rw_tend, o, t, i0, ids, ide);
+  st4m(s.rw_tend, o, t, i0, zlo(a, ids), zhi(a, ide));
 }
 
 // ------------------------------------------------------------------------------ t, ph, zeroing
-template <typename ARGS, bool PER>
+template <typename ARGS, bool PER, bool ZONE>
 __global__ void __launch_bounds__(256) k_tend_t(ARGS args) {
   Tile3 T;
   const KArgs &a = kargs(args, T);
   const Dm &d = a.d;
   const DState &s = dstate(a);
-  const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
-  const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
-  const int k0 = d.kds + (int)T.bz * kTendK;
+  int i, j;
+  if (!point_ij<ZONE>(a, T, i, j)) return;
+  constexpr int KS = ZONE ? 1 : kTendK;  // levels per thread (the zone form: one)
+  const int k0 = d.kds + (int)T.bz * KS;
   const int ids = d.ids, ide = d.ide, jds = d.jds, jde = d.jde, kde = d.kde;
   const int its = ids, ite = ide, jts = jds, jte = jde, kts = d.kds, kte = kde;
   if (i > ide || j > jde || k0 > kte) return;
-  const int k1 = imin(k0 + kTendK - 1, kte);
+  const int k1 = imin(k0 + KS - 1, kte);
   const Geo g = geo<PER>(a.c);
   const float ts = a.c.time_step >= 0 ? 1.f : -1.f;
   const int ktf = kte - 1;
@@ -1454,6 +1461,7 @@ __global__ void __launch_bounds__(256) k_tend_t_x4(ARGS args) {
   const DState &s = dstate(a);
   const int i0 = vec_ia0(d) + kVecW * (T.bx * 64 + (int)threadIdx.x);
   const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
+  if (!zrow(a, j)) return;  // rows of an open y edge: the zone form
   const int k = d.kds + (int)T.bz;
   const int ids = d.ids, ide = d.ide, jde = d.jde, kde = d.kde, kts = d.kds, kte = kde, ktf = kte - 1;
   if (i0 > ide || j > jde || k > kte) return;
@@ -1469,14 +1477,14 @@ __global__ void __launch_bounds__(256) k_tend_t_x4(ARGS args) {
 #pragma unroll
   for (int n = 0; n < 4; n++) z.v[n] = 0.f;
   // zero_tend of the *_save arrays over the whole tile
-  st4m(s.u_save, o, z, i0, ids, ide);
-  st4m(s.v_save, o, z, i0, ids, ide);
-  st4m(s.w_save, o, z, i0, ids, ide);
-  st4m(s.ph_save, o, z, i0, ids, ide);
-  st4m(s.t_save, o, z, i0, ids, ide);
+  st4m(s.u_save, o, z, i0, zlo(a, ids), zhi(a, ide));
+  st4m(s.v_save, o, z, i0, zlo(a, ids), zhi(a, ide));
+  st4m(s.w_save, o, z, i0, zlo(a, ids), zhi(a, ide));
+  st4m(s.ph_save, o, z, i0, zlo(a, ids), zhi(a, ide));
+  st4m(s.t_save, o, z, i0, zlo(a, ids), zhi(a, ide));
   if (k == kts) {
-    st4m(s.mu_tend, o2, z, i0, ids, ide);
-    st4m(s.mu_save, o2, z, i0, ids, ide);
+    st4m(s.mu_tend, o2, z, i0, zlo(a, ids), zhi(a, ide));
+    st4m(s.mu_save, o2, z, i0, zlo(a, ids), zhi(a, ide));
   }
   F4 t = z, pt = z;
   const bool jmass = j <= jde - 1;
@@ -1548,7 +1556,7 @@ __global__ void __launch_bounds__(256) k_tend_t_x4(ARGS args) {
           if (rk1) tf.v[n] = tf.v[n] + 0.f;  // t_save was zeroed by rk_tendency
           t.v[n] = t.v[n] + tf.v[n] / msfty.v[n] + (c1 * mut.v[n] + c2) * hd.v[n] / msfty.v[n];
         }
-        if (rk1) st4m(s.t_tendf, o, tf, i0, ids, ide - 1);
+        if (rk1) st4m(s.t_tendf, o, tf, i0, zlo(a, ids), zhi(a, ide - 1));
       }
     }
     // rhs_ph
@@ -1638,14 +1646,14 @@ __global__ void __launch_bounds__(256) k_tend_t_x4(ARGS args) {
         if (rk1) ptf.v[n] = ptf.v[n] + 0.f;  // ph_save was zeroed by rk_tendency
         pt.v[n] = pt.v[n] + ptf.v[n] / msfty.v[n];
       }
-      if (rk1) st4m(s.ph_tendf, o, ptf, i0, ids, ide - 1);
+      if (rk1) st4m(s.ph_tendf, o, ptf, i0, zlo(a, ids), zhi(a, ide - 1));
     }
     if (k == kts) {
       const F4 mtf = s.mu_tendf.ld<F4>(o2);
       F4 mt;
 #pragma unroll
       for (int n = 0; n < 4; n++) mt.v[n] = 0.f + mtf.v[n];
-      st4m(s.mu_tend, o2, mt, i0, ids, ide - 1);
+      st4m(s.mu_tend, o2, mt, i0, zlo(a, ids), zhi(a, ide - 1));
     }
   }
 #pragma unroll
@@ -1654,21 +1662,25 @@ __global__ void __launch_bounds__(256) k_tend_t_x4(ARGS args) {
       t.v[n] = 0.f;
       pt.v[n] = 0.f;
     }
-  st4m(s.t_tend, o, t, i0, ids, ide);
-  st4m(s.ph_tend, o, pt, i0, ids, ide);
+  st4m(s.t_tend, o, t, i0, zlo(a, ids), zhi(a, ide));
+  st4m(s.ph_tend, o, pt, i0, zlo(a, ids), zhi(a, ide));
 }
 
-WRFMI_INST_P(k_tend_u, false)
-WRFMI_INST_P(k_tend_u, true)
+WRFMI_INST_P2(k_tend_u, false, false)
+WRFMI_INST_P2(k_tend_u, true, false)
+WRFMI_INST_P2(k_tend_u, false, true)
 WRFMI_INST(k_tend_u_x4)
-WRFMI_INST_P(k_tend_v, false)
-WRFMI_INST_P(k_tend_v, true)
+WRFMI_INST_P2(k_tend_v, false, false)
+WRFMI_INST_P2(k_tend_v, true, false)
+WRFMI_INST_P2(k_tend_v, false, true)
 WRFMI_INST(k_tend_v_x4)
-WRFMI_INST_P(k_tend_w, false)
-WRFMI_INST_P(k_tend_w, true)
+WRFMI_INST_P2(k_tend_w, false, false)
+WRFMI_INST_P2(k_tend_w, true, false)
+WRFMI_INST_P2(k_tend_w, false, true)
 WRFMI_INST(k_tend_w_x4)
-WRFMI_INST_P(k_tend_t, false)
-WRFMI_INST_P(k_tend_t, true)
+WRFMI_INST_P2(k_tend_t, false, false)
+WRFMI_INST_P2(k_tend_t, true, false)
+WRFMI_INST_P2(k_tend_t, false, true)
 WRFMI_INST(k_tend_t_x4)
 
 }  // namespace wrfmi

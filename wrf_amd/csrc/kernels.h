@@ -14,16 +14,18 @@ constexpr int kAdvVK = 4;      // levels per thread in k_adv_scalar_x4
 constexpr int kHaloK = 8;      // levels per thread in k_halo
 // Every state kernel is a template on its argument type: KArgs for one domain, BatchArgs
 // for an ensemble launch over the tiles of all members (k_halo: HaloArgs / BatchHalo).
+// ZONE = true is the one-point kernel's form over the boundary zone of a domain with an
+// open direction (zone_point), launched after the four-point form.
 template <typename A> __global__ void k_step_prep(A);
 template <typename A> __global__ void k_step_prep_x4(A);
 template <typename A, bool COMP> __global__ void k_ww_scan(A);
-template <typename A, bool PER> __global__ void k_tend_u(A);
+template <typename A, bool PER, bool ZONE = false> __global__ void k_tend_u(A);
 template <typename A> __global__ void k_tend_u_x4(A);
-template <typename A, bool PER> __global__ void k_tend_v(A);
+template <typename A, bool PER, bool ZONE = false> __global__ void k_tend_v(A);
 template <typename A> __global__ void k_tend_v_x4(A);
-template <typename A, bool PER> __global__ void k_tend_w(A);
+template <typename A, bool PER, bool ZONE = false> __global__ void k_tend_w(A);
 template <typename A> __global__ void k_tend_w_x4(A);
-template <typename A, bool PER> __global__ void k_tend_t(A);
+template <typename A, bool PER, bool ZONE = false> __global__ void k_tend_t(A);
 template <typename A> __global__ void k_tend_t_x4(A);
 template <typename A> __global__ void k_small_prep(A);
 template <typename A> __global__ void k_p_rho0(A);
@@ -42,7 +44,7 @@ template <typename A> __global__ void k_ac_wpred_x4(A);
 template <typename A, typename CT> __global__ void k_ac_wsolve(A);
 template <typename A> __global__ void k_small_finish(A);
 template <typename A> __global__ void k_small_finish_x4(A);
-template <typename A, bool PER> __global__ void k_adv_scalar(A);
+template <typename A, bool PER, bool ZONE = false> __global__ void k_adv_scalar(A);
 template <typename A> __global__ void k_adv_scalar_x4(A);
 template <typename A> __global__ void k_update_scalar(A);
 template <typename A> __global__ void k_update_scalar_x4(A);
@@ -62,3 +64,6 @@ __global__ void k_count_nonfinite(const float *a, Dm d, int kend, int is3d, unsi
 #define WRFMI_INST_P(K, P) \
   template __global__ void K<KArgs, P>(KArgs); \
   template __global__ void K<BatchArgs, P>(BatchArgs);
+#define WRFMI_INST_P2(K, P, Q) \
+  template __global__ void K<KArgs, P, Q>(KArgs); \
+  template __global__ void K<BatchArgs, P, Q>(BatchArgs);

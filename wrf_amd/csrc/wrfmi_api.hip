@@ -294,6 +294,7 @@ struct KRef {
 };
 #define KR(K) KRef{K<KArgs>, K<BatchArgs>}
 #define KRP(K, P) KRef{K<KArgs, P>, K<BatchArgs, P>}
+#define KRZ(K) KRef{K<KArgs, false, true>, K<BatchArgs, false, true>}
 
 // One launch of an ensemble member, as recorded for a batched launch (wrfmi_ensemble_*).
 struct RecOp {
@@ -318,6 +319,7 @@ struct Launcher {
   RecMember *rec = nullptr;  // set: record the launches instead of enqueueing them
   bool vec = false;          // the layout allows the four-points-per-lane kernels
   bool scal_all = false;     // species advected and updated by the all-species kernels
+  bool zone_ok = false;      // each open direction is wide enough for a boundary zone on each side
   // launch with an optional event pair around it (kernel timing mode)
   template <typename K, typename... A>
   void launch(int id, K kern, dim3 g, dim3 b, A... args) {
@@ -387,10 +389,15 @@ static inline dim3 g_pt3(const Dm &d, int nk) {
   return dim3(g.x, g.y, nk);
 }
 
-// (i, j) tiles of four-points-per-lane kernels (kVecTileI x 4 points from vec_ia0), nk levels
-static inline dim3 g_vec3(const Dm &d, int nk) {
-  const int ia0 = d.ims + 4 * ((d.ids - d.ims) >> 2);
-  return dim3((d.ide - ia0 + kVecTileI) / kVecTileI, (d.jde - d.jds + 4) / 4, nk);
+// (i, j) tiles of four-point kernels covering the points from i = ibase and the rows from
+// jds - jlow through jde, nk levels: each workgroup takes 256 points of `rows` rows
+static inline dim3 g_vec(const KArgs &a, int ibase, int jlow, int rows, int nk) {
+  const Dm &d = a.d;
+  const int w = kVecTileI;
+  return dim3((d.ide - ibase + w) / w, (d.jde - d.jds + 1 + jlow + rows - 1) / rows, nk);
+}
+static inline dim3 g_vec3(const KArgs &a, int nk) {
+  return g_vec(a, a.d.ims + 4 * ((a.d.ids - a.d.ims) >> 2), 0, 4, nk);
 }
 
 static void ph_step_prep(Launcher &L) {
@@ -400,8 +407,7 @@ static void ph_step_prep(Launcher &L) {
   if (L.vec) {
     // one extra group of four below ids (mut at ids-1), rows from jds-1
     const int ia0 = a.d.ims + 4 * ((a.d.ids - a.d.ims) >> 2) - 4;
-    L.launchK(KID_STEP_PREP, KR(k_step_prep_x4),
-              dim3((a.d.ide - ia0 + kVecTileI) / kVecTileI, (a.d.jde - a.d.jds + 1 + 4) / 4, a.d.kde - a.d.kds + 1), kColBlk, a);
+    L.launchK(KID_STEP_PREP, KR(k_step_prep_x4), g_vec(a, ia0, 1, 4, a.d.kde - a.d.kds + 1), kColBlk, a);
   } else {
     L.launchK(KID_STEP_PREP, KR(k_step_prep), dim3(gp.x, gp.y, a.d.kde - a.d.kds + 1), kColBlk, a);
   }
@@ -414,27 +420,44 @@ static void ph_step_prep(Launcher &L) {
   L.halo_add(s.al, 'p', 1); L.halo_add(s.ph_2, 'w', 1);
   L.halo_flush();
 }
+// the one-point zone form's grid: 256 zone points per workgroup, nkz level groups
+static inline dim3 g_zone(const KArgs &a, int nkz) {
+  const unsigned n = zone_count(a.d, a.c.periodic_x, a.c.periodic_y);
+  return dim3((n + 255) / 256, 1, nkz);
+}
 static void ph_tendency(Launcher &L) {
   const KArgs &a = L.a;
-  const int nk = a.d.kde - a.d.kds + 1;
+  const int nk = a.d.kde - a.d.kds + 1, nkt = (nk + kTendK - 1) / kTendK;
   const bool per = a.c.periodic_x && a.c.periodic_y;
-  const dim3 gt = g_pt3(a.d, (nk + kTendK - 1) / kTendK);
-  if (per && L.vec) L.launchK(KID_TEND_U, KR(k_tend_u_x4), g_vec3(a.d, nk), kColBlk, a);
-  else L.launchK(KID_TEND_U, per ? KRP(k_tend_u, true) : KRP(k_tend_u, false), gt, kColBlk, a);
-  if (per && L.vec) L.launchK(KID_TEND_V, KR(k_tend_v_x4), g_vec3(a.d, nk), kColBlk, a);
-  else L.launchK(KID_TEND_V, per ? KRP(k_tend_v, true) : KRP(k_tend_v, false), gt, kColBlk, a);
-  if (per && L.vec) L.launchK(KID_TEND_W, KR(k_tend_w_x4), g_vec3(a.d, nk), kColBlk, a);
-  else L.launchK(KID_TEND_W, per ? KRP(k_tend_w, true) : KRP(k_tend_w, false), gt, kColBlk, a);
-  if (per && L.vec) L.launchK(KID_TEND_T, KR(k_tend_t_x4), g_vec3(a.d, nk), kColBlk, a);
-  else L.launchK(KID_TEND_T, per ? KRP(k_tend_t, true) : KRP(k_tend_t, false), gt, kColBlk, a);
+  const dim3 gt = g_pt3(a.d, nkt);
+  if (L.vec && (per || L.zone_ok)) {
+    // four-point kernels; on a domain with an open direction they leave the boundary zone
+    // to the one-point kernels' zone form
+    L.launchK(KID_TEND_U, KR(k_tend_u_x4), g_vec3(a, nk), kColBlk, a);
+    L.launchK(KID_TEND_V, KR(k_tend_v_x4), g_vec3(a, nk), kColBlk, a);
+    L.launchK(KID_TEND_W, KR(k_tend_w_x4), g_vec3(a, nk), kColBlk, a);
+    L.launchK(KID_TEND_T, KR(k_tend_t_x4), g_vec3(a, nk), kColBlk, a);
+    if (!per) {
+      const dim3 gz = g_zone(a, nk);  // one level per thread
+      L.launchK(KID_TEND_U, KRZ(k_tend_u), gz, kColBlk, a);
+      L.launchK(KID_TEND_V, KRZ(k_tend_v), gz, kColBlk, a);
+      L.launchK(KID_TEND_W, KRZ(k_tend_w), gz, kColBlk, a);
+      L.launchK(KID_TEND_T, KRZ(k_tend_t), gz, kColBlk, a);
+    }
+    return;
+  }
+  L.launchK(KID_TEND_U, per ? KRP(k_tend_u, true) : KRP(k_tend_u, false), gt, kColBlk, a);
+  L.launchK(KID_TEND_V, per ? KRP(k_tend_v, true) : KRP(k_tend_v, false), gt, kColBlk, a);
+  L.launchK(KID_TEND_W, per ? KRP(k_tend_w, true) : KRP(k_tend_w, false), gt, kColBlk, a);
+  L.launchK(KID_TEND_T, per ? KRP(k_tend_t, true) : KRP(k_tend_t, false), gt, kColBlk, a);
 }
 static void ph_small_prep(Launcher &L) {
   const KArgs &a = L.a;
   const wrfmi_state &s = a.s;
   const int nk = a.d.kde - a.d.kds + 1;
   if (L.vec) {
-    L.launchK(KID_SMALL_PREP, KR(k_small_prep_x4), g_vec3(a.d, nk), kColBlk, a);
-    L.launchK(KID_P_RHO0, KR(k_p_rho0_x4), g_vec3(a.d, nk - 1), kColBlk, a);
+    L.launchK(KID_SMALL_PREP, KR(k_small_prep_x4), g_vec3(a, nk), kColBlk, a);
+    L.launchK(KID_P_RHO0, KR(k_p_rho0_x4), g_vec3(a, nk - 1), kColBlk, a);
   } else {
     L.launchK(KID_SMALL_PREP, KR(k_small_prep), g_pt3(a.d, nk), kColBlk, a);
     L.launchK(KID_P_RHO0, KR(k_p_rho0), g_pt3(a.d, nk - 1), kColBlk, a);
@@ -451,7 +474,7 @@ static void ph_advance_uv(Launcher &L) {
   const bool per = L.a.c.periodic_x && L.a.c.periodic_y;
   const int nkb = (L.a.d.kde - L.a.d.kds + kAdvUvK - 1) / kAdvUvK;
   if (L.vec)
-    L.launchK(KID_ADVANCE_UV, per ? KRP(k_advance_uv_x4, true) : KRP(k_advance_uv_x4, false), g_vec3(L.a.d, nkb), kColBlk, L.a);
+    L.launchK(KID_ADVANCE_UV, per ? KRP(k_advance_uv_x4, true) : KRP(k_advance_uv_x4, false), g_vec3(L.a, nkb), kColBlk, L.a);
   else
     L.launchK(KID_ADVANCE_UV, per ? KRP(k_advance_uv, true) : KRP(k_advance_uv, false), g_pt3(L.a.d, nkb), kColBlk, L.a);
 }
@@ -460,13 +483,13 @@ static void ph_advance_mu_t_w(Launcher &L) {
   const wrfmi_state &s = a.s;
   const bool p64 = a.c.precision_mode == 1;
   const int nk = a.d.kde - a.d.kds + 1;
-  if (L.vec) L.launchK(KID_AC_DIV, KR(k_ac_div_x4), g_vec3(a.d, nk), kColBlk, a);
+  if (L.vec) L.launchK(KID_AC_DIV, KR(k_ac_div_x4), g_vec3(a, nk), kColBlk, a);
   else L.launchK(KID_AC_DIV, KR(k_ac_div), g_pt3(a.d, nk), kColBlk, a);
   if (p64) L.launchK(KID_AC_MU_WW, KRP(k_ac_mu_ww, true), g_col(a.d), kColBlk, a);
   else L.launchK(KID_AC_MU_WW, KRP(k_ac_mu_ww, false), g_col(a.d), kColBlk, a);
   if (L.vec) {
-    L.launchK(KID_AC_THETA, KR(k_ac_theta_x4), g_vec3(a.d, nk), kColBlk, a);
-    L.launchK(KID_AC_WPRED, KR(k_ac_wpred_x4), g_vec3(a.d, 1), kColBlk, a);
+    L.launchK(KID_AC_THETA, KR(k_ac_theta_x4), g_vec3(a, nk), kColBlk, a);
+    L.launchK(KID_AC_WPRED, KR(k_ac_wpred_x4), g_vec3(a, 1), kColBlk, a);
   } else {
     L.launchK(KID_AC_THETA, KR(k_ac_theta), g_pt3(a.d, nk), kColBlk, a);
     L.launchK(KID_AC_WPRED, KR(k_ac_wpred), g_pt3(a.d, 1), kColBlk, a);
@@ -481,7 +504,7 @@ static void ph_advance_mu_t_w(Launcher &L) {
 static void ph_small_finish(Launcher &L) {
   const KArgs &a = L.a;
   const wrfmi_state &s = a.s;
-  if (L.vec) L.launchK(KID_SMALL_FINISH, KR(k_small_finish_x4), g_vec3(a.d, a.d.kde - a.d.kds + 1), kColBlk, a);
+  if (L.vec) L.launchK(KID_SMALL_FINISH, KR(k_small_finish_x4), g_vec3(a, a.d.kde - a.d.kds + 1), kColBlk, a);
   else L.launchK(KID_SMALL_FINISH, KR(k_small_finish), g_pt3(a.d, a.d.kde - a.d.kds + 1), kColBlk, a);
   if (a.rk_step == 3) {
     L.halo_begin();
@@ -495,26 +518,26 @@ static void ph_scalar(Launcher &L) {
   const Dm &d = L.a.d;
   const dim3 ga((d.ide - d.ids + 1 + kAdvTileI - 1) / kAdvTileI, (d.jde - d.jds + 1 + kAdvRows - 1) / kAdvRows,
                 (d.kde - d.kds + 1 + kAdvK - 1) / kAdvK);
-  if (per && L.vec) {
+  if (L.vec && (per || L.zone_ok)) {
     const int ia0 = d.ims + 4 * ((d.ids - d.ims) >> 2);
-    const dim3 gv((d.ide - ia0 + kVecTileI) / kVecTileI, (d.jde - d.jds + kAdvRows) / kAdvRows,
-                  (d.kde - d.kds + kAdvVK) / kAdvVK);
+    const dim3 gv = g_vec(L.a, ia0, 0, kAdvRows, (d.kde - d.kds + kAdvVK) / kAdvVK);
     L.launchK(KID_ADV_SCALAR, KR(k_adv_scalar_x4), gv, dim3(64, kAdvRows, 1), L.a);
+    if (!per) L.launchK(KID_ADV_SCALAR, KRZ(k_adv_scalar), g_zone(L.a, d.kde - d.kds + 1), dim3(64, kAdvRows, 1), L.a);
   } else {
     L.launchK(KID_ADV_SCALAR, per ? KRP(k_adv_scalar, true) : KRP(k_adv_scalar, false), ga, dim3(64, kAdvRows, 1), L.a);
   }
-  if (L.vec) L.launchK(KID_UPDATE_SCALAR, KR(k_update_scalar_x4), g_vec3(L.a.d, L.a.d.kde - L.a.d.kds), kColBlk, L.a);
+  if (L.vec) L.launchK(KID_UPDATE_SCALAR, KR(k_update_scalar_x4), g_vec3(L.a, L.a.d.kde - L.a.d.kds), kColBlk, L.a);
   else L.launchK(KID_UPDATE_SCALAR, KR(k_update_scalar), g_pt3(L.a.d, L.a.d.kde - L.a.d.kds), kColBlk, L.a);
 }
 // every species' rk_scalar_tend and rk_update_scalar (species are independent: the order
 // over species does not change any value)
 static void ph_scalar_all(Launcher &L) {
   const Dm &d = L.a.d;
-  L.launchK(KID_ADV_SCALAR, KR(k_adv_scalar_all_x4), g_vec3(d, d.kde - d.kds + 1), kColBlk, L.a);
-  L.launchK(KID_UPDATE_SCALAR, KR(k_update_scalar_all_x4), g_vec3(d, d.kde - d.kds), kColBlk, L.a);
+  L.launchK(KID_ADV_SCALAR, KR(k_adv_scalar_all_x4), g_vec3(L.a, d.kde - d.kds + 1), kColBlk, L.a);
+  L.launchK(KID_UPDATE_SCALAR, KR(k_update_scalar_all_x4), g_vec3(L.a, d.kde - d.kds), kColBlk, L.a);
 }
 static void ph_p_rho_phi(Launcher &L) {
-  if (L.vec) L.launchK(KID_P_RHO_PHI, KR(k_p_rho_phi_x4), g_vec3(L.a.d, L.a.d.kde - L.a.d.kds), kColBlk, L.a);
+  if (L.vec) L.launchK(KID_P_RHO_PHI, KR(k_p_rho_phi_x4), g_vec3(L.a, L.a.d.kde - L.a.d.kds), kColBlk, L.a);
   else L.launchK(KID_P_RHO_PHI, KR(k_p_rho_phi), g_pt3(L.a.d, L.a.d.kde - L.a.d.kds), kColBlk, L.a);
 }
 static void ph_bc_dry_2(Launcher &L, int kind) {
@@ -627,6 +650,7 @@ static int prepare(const wrfmi_state *st_in, const wrfmi_config *cfg, const wrfm
   L.vec = vec_layout(s, d, L.a);
   // the all-species scalar kernels (doubly periodic, aligned layout, more than one species)
   L.scal_all = L.vec && cfg->periodic_x && cfg->periodic_y && s.n_species > 1;
+  L.zone_ok = (cfg->periodic_x || d.ide - d.ids + 1 >= 2 * kZone + 2) && (cfg->periodic_y || d.jde - d.jds + 1 >= 2 * kZone + 2);
   if (L.scal_all) {
     const size_t nelem = (size_t)d.sj * (size_t)(d.jme - d.jms + 1);
     float *qt = own_qt;
@@ -788,9 +812,12 @@ extern "C" int wrfmi_ensemble_create(int n_members, const wrfmi_state *states, c
   // member's layout allows them
   bool all_vec = true;
   for (int m = 0; m < n_members && !rc; m++) all_vec = all_vec && Ls[m].vec;
+  bool all_zone = true;
+  for (int m = 0; m < n_members && !rc; m++) all_zone = all_zone && Ls[m].zone_ok;
   for (int m = 0; m < n_members && !rc; m++) {
     Ls[m].vec = all_vec;
     Ls[m].scal_all = Ls[m].scal_all && all_vec;
+    Ls[m].zone_ok = all_zone;
     Ls[m].rec = &rec[m];
     enqueue_step(Ls[m]);
   }
