@@ -122,9 +122,12 @@ int wrfmi_sync_to_host(void *host_ptr);
  * rk_addtend_dry, small_step_prep, calc_p_rho, calc_coef_w, the acoustic loop
  * (advance_uv, advance_mu_t, advance_w, sumflux, calc_p_rho), calc_mu_uv_1,
  * small_step_finish, rk_scalar_tend/rk_update_scalar for every species,
- * calc_p_rho_phi and the set_physical_bc3d/2d halo fills.  The physics
- * tendencies *_tendf, mu_tendf and h_diabatic are read as given (the reference
- * zeroes them at rk_step 1 when physics is off: module_first_rk_step_part1.F:182).
+ * calc_p_rho_phi, the set_physical_bc3d/2d halo fills and set_w_surface
+ * (solve_em.F:4740).  The physics tendencies *_tendf, mu_tendf and h_diabatic are
+ * read as given (the reference zeroes them at rk_step 1 when physics is off:
+ * module_first_rk_step_part1.F:182).  calc_coef_w's a and alpha are formed inside
+ * the column solve and not stored by this entry point (the per-routine
+ * wrfmi_small_step_prep stores them); gamma is stored as the reference does.
  * `stream` is a hipStream_t (NULL = the default stream); the call returns
  * after enqueueing unless `sync` is non-zero. */
 int wrfmi_solve_em_step(wrfmi_state *st, const wrfmi_config *cfg,

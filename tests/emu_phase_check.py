@@ -98,7 +98,9 @@ def ensemble_check(order: int, nsteps: int):
     fails = []
     for m, (mem, exp) in enumerate(zip(members, singles)):
         got = mem.download()
-        bad = [k for k in exp if not np.array_equal(got[k], exp[k])]
+        # a and alpha are calc_coef_w's work arrays: the per-routine entry points store them,
+        # the whole step forms them inside the column solve without storing them
+        bad = [k for k in exp if k not in ("a", "alpha") and not np.array_equal(got[k], exp[k])]
         if bad:
             fails.append(f"member {m} differs from its single-domain step in {bad}")
     assert lib.wrfmi_ensemble_destroy(ens) == 0
@@ -139,7 +141,9 @@ def main():
         emu2.step(nsteps)
         got = emu2.download()
         exp = Oracle("f32cr").run(hs, nsteps)
-        bad = [k for k in exp if not np.array_equal(got[k], exp[k])]
+        # a and alpha are calc_coef_w's work arrays: the per-routine entry points store them,
+        # the whole step forms them inside the column solve without storing them
+        bad = [k for k in exp if k not in ("a", "alpha") and not np.array_equal(got[k], exp[k])]
         if bad:
             fails.append(f"whole step differs in {bad}")
     for f in fails:

@@ -117,8 +117,10 @@ __global__ void __launch_bounds__(256) k_p_rho0(ARGS args) {
   s.pm1[o] = pp;
 }
 
-// small_step_prep's 2-D updates and calc_coef_w: one thread per (i, j) column
-template <typename ARGS, typename CT>
+// small_step_prep's 2-D updates and, with COEF, calc_coef_w: one thread per (i, j) column.
+// The whole-step driver launches it without COEF: k_ac_wsolve forms the same coefficients
+// on its way up each column from values it loads anyway.
+template <typename ARGS, typename CT, bool COEF>
 __global__ void __launch_bounds__(256) k_coef_w(ARGS args) {
   Tile3 T;
   const KArgs &a = kargs(args, T);
@@ -143,7 +145,7 @@ __global__ void __launch_bounds__(256) k_coef_w(ARGS args) {
   }
   if (mj) s.muus[o2] = m.muus;
   if (mi) s.muvs[o2] = m.muvs;
-  if (!mass || !a.c.non_hydrostatic) return;
+  if (!COEF || !mass || !a.c.non_hydrostatic) return;
   const float mut = s.mut[o2];
   const FPtr<CT> A((CT *)s.a), AL((CT *)s.alpha), GA((CT *)s.gamma);
   const CT lid = a.c.top_lid ? CT(0) : CT(1);
@@ -1096,14 +1098,16 @@ __global__ void __launch_bounds__(256) k_ac_wsolve(ARGS args) {
   FPtr<float> w = s.w_2, ph = s.ph_2;
   FPtr<float> lo = a.scr_lo;
   const FPtr<float> rhs = a.scr_rhs;
-  const FPtr<CT> A((CT *)s.a), AL((CT *)s.alpha), GA((CT *)s.gamma);
+  FPtr<CT> GA((CT *)s.gamma);
   const Off ob = X3(d, i, 0, j), o2 = X2(d, i, j);
   const int ni = d.ni;
   CT wfe = (CT)w[ob + ni];  // surface w (level 1) from k_ac_wpred
-  // Upward: the explicit w predictor of advance_w (1402-1431) at level k, then the forward
-  // elimination.  Fields: 0 = w, 1 = a, 2 = alpha, 3 = rhs(k+1), 4 = ph(k+1), 5 = t_2ave,
-  // 6 = c2a, 7 = alt, 8 = rw_tend, 9 = cqw (k+1 clamped to kde); rhs, ph at k and k-1 and
-  // t_2ave, c2a, alt at k-1 are carried from the levels below.
+  // Upward: the explicit w predictor of advance_w (1402-1431) at level k, calc_coef_w's
+  // coefficients a(k), alpha(k), gamma(k) (570-652, the same expressions in the same order
+  // as k_coef_w), then the forward elimination.  Fields: 0 = w, 1 = rhs(k+1), 2 = ph(k+1),
+  // 3 = t_2ave, 4 = c2a, 5 = alt, 6 = rw_tend, 7 = cqw (k+1 clamped to kde); rhs, ph at k
+  // and k-1 and t_2ave, c2a, alt at k-1 are carried from the levels below.  gamma is
+  // stored for the way down.
   const float dts = a.dts, epssm = c.epssm;
   const FPtr<float> ph_c = s.ph_2, t2ave = s.t_2save, c2a = s.c2a, alt_f = s.alt;
   const float msfty = s.msfty[o2], mut = s.mut[o2], muave = s.muave[o2];
@@ -1112,28 +1116,37 @@ __global__ void __launch_bounds__(256) k_ac_wsolve(ARGS args) {
   float rhs_m1 = 0.f, rhs_0 = rhs[ob + 2 * ni];  // rhs(1) = 0
   float phl_m1 = ph_c[ob + ni], phl_0 = ph_c[ob + 2 * ni];
   float t2a_lo = t2ave[ob + ni], c2a_lo = c2a[ob + ni], alt_lo = alt_f[ob + ni];
-  column_march_up<10, CT, 4>(
+  const CT lid = c.top_lid ? CT(0) : CT(1);
+  const CT epc = (CT)c.epssm;
+  CT cof = (CT(.5) * (CT)a.dts * (CT)K_G * (CT(1) + epc));
+  cof = cof * cof;
+  const CT MUT = (CT)mut;
+  auto cH = [&](int k) -> CT { return (CT)c1h[X1(d, k)] * MUT + (CT)c2h[X1(d, k)]; };
+  auto cF = [&](int k) -> CT { return (CT)c1f[X1(d, k)] * MUT + (CT)c2f[X1(d, k)]; };
+  auto rdnwC = [&](int k) -> CT { return (CT)rdnw[X1(d, k)]; };
+  auto rdnC = [&](int k) -> CT { return (CT)rdn[X1(d, k)]; };
+  CT gam = CT(0);
+  column_march_up<8, CT, 4>(
       2, k_end + 1,
       [&](int f, int k) -> CT {
         const Off o = ob + k * ni, op = ob + imin(k + 1, kde) * ni;
         switch (f) {
           case 0: return (CT)w[o];
-          case 1: return A[o];
-          case 2: return AL[o];
-          case 3: return (CT)rhs[op];
-          case 4: return (CT)ph_c[op];
-          case 5: return (CT)t2ave[o];
-          case 6: return (CT)c2a[o];
-          case 7: return (CT)alt_f[o];
-          case 8: return (CT)s.rw_tend[o];
+          case 1: return (CT)rhs[op];
+          case 2: return (CT)ph_c[op];
+          case 3: return (CT)t2ave[o];
+          case 4: return (CT)c2a[o];
+          case 5: return (CT)alt_f[o];
+          case 6: return (CT)s.rw_tend[o];
           default: return (CT)s.cqw[o];
         }
       },
       [&](int k, const CT *v) {
         const Off o = ob + k * ni;
-        const float w0 = (float)v[0], rhs_p1 = (float)v[3], phl_p1 = (float)v[4], t2a_hi = (float)v[5];
-        const float c2a0 = (float)v[6], alt0 = (float)v[7], rwt = (float)v[8], cqw = (float)v[9];
+        const float w0 = (float)v[0], rhs_p1 = (float)v[1], phl_p1 = (float)v[2], t2a_hi = (float)v[3];
+        const float c2a0 = (float)v[4], alt0 = (float)v[5], rwt = (float)v[6], cqw = (float)v[7];
         float wv;
+        CT ak, alpha;
         if (k <= k_end) {
           wv = w0 + dts * rwt +
                msft_inv * cqw *
@@ -1144,12 +1157,24 @@ __global__ void __launch_bounds__(256) k_ac_wsolve(ARGS args) {
                          ((1.f + epssm) * (rhs_0 - rhs_m1) + (1.f - epssm) * (phl_0 - phl_m1)))) +
                dts * K_G * msft_inv *
                    (rdn[X1(d, k)] * (c2a0 * alt0 * t2a_hi - c2a_lo * alt_lo * t2a_lo) - (c1f[X1(d, k)] * muave));
+          const CT CQW = v[7], C2A = v[4], C2Al = (CT)c2a_lo;
+          ak = k >= 3 ? -CQW * cof * rdnC(k) * rdnwC(k - 1) * C2Al / (cH(k - 1) * cF(k - 1)) : CT(0);
+          const CT b = CT(1) + CQW * cof * rdnC(k) *
+                                   (rdnwC(k) * C2A / (cH(k) * cF(k)) + rdnwC(k - 1) * C2Al / (cH(k - 1) * cF(k)));
+          const CT cc = -CQW * cof * rdnC(k) * rdnwC(k) * C2A / (cH(k) * cF(k + 1));
+          alpha = CT(1) / (b - ak * gam);
+          gam = cc * alpha;
+          GA[o] = gam;
         } else {
           wv = w0 + dts * rwt +
                msft_inv * (-.5f * dts * K_G / (c1h[X1(d, k - 1)] * mut + c2h[X1(d, k - 1)]) * (rdnw[X1(d, k - 1)] * rdnw[X1(d, k - 1)]) * 2.f *
                                c2a_lo * ((1.f + epssm) * (rhs_0 - rhs_m1) + (1.f - epssm) * (phl_0 - phl_m1)) -
                            dts * K_G * (2.f * rdnw[X1(d, k - 1)] * c2a_lo * alt_lo * t2a_lo + (c1f[X1(d, k)] * muave)));
           if (c.top_lid) wv = 0.f;
+          const CT C2Al = (CT)c2a_lo;
+          ak = -CT(2) * cof * (rdnwC(kde - 1) * rdnwC(kde - 1)) * C2Al * lid / (cH(kde - 1) * cF(kde - 1));
+          const CT b = CT(1) + CT(2) * cof * (rdnwC(kde - 1) * rdnwC(kde - 1)) * C2Al / (cH(kde - 1) * cF(kde));
+          alpha = CT(1) / (b - ak * gam);
         }
         rhs_m1 = rhs_0;
         rhs_0 = rhs_p1;
@@ -1159,11 +1184,11 @@ __global__ void __launch_bounds__(256) k_ac_wsolve(ARGS args) {
         c2a_lo = c2a0;
         alt_lo = alt0;
         if (sizeof(CT) == sizeof(float)) {
-          const float wf = (wv - (float)v[1] * (float)wfe) * (float)v[2];
+          const float wf = (wv - (float)ak * (float)wfe) * (float)alpha;
           wfe = (CT)wf;
           w[o] = wf;
         } else {
-          wfe = ((CT)wv - v[1] * wfe) * v[2];
+          wfe = ((CT)wv - ak * wfe) * alpha;
           w[o] = (float)wfe;
           lo[o] = (float)((double)wfe - (double)(float)wfe);
         }
@@ -1290,8 +1315,10 @@ WRFMI_INST_P(k_advance_uv, false)
 WRFMI_INST_P(k_advance_uv, true)
 WRFMI_INST_P(k_advance_uv_x4, false)
 WRFMI_INST_P(k_advance_uv_x4, true)
-WRFMI_INST_P(k_coef_w, float)
-WRFMI_INST_P(k_coef_w, double)
+WRFMI_INST_P2(k_coef_w, float, true)
+WRFMI_INST_P2(k_coef_w, double, true)
+WRFMI_INST_P2(k_coef_w, float, false)
+WRFMI_INST_P2(k_coef_w, double, false)
 WRFMI_INST_P(k_ac_mu_ww, false)
 WRFMI_INST_P(k_ac_mu_ww, true)
 WRFMI_INST_P(k_ac_wsolve, float)

@@ -29,7 +29,7 @@ template <typename A, bool PER, bool ZONE = false> __global__ void k_tend_t(A);
 template <typename A> __global__ void k_tend_t_x4(A);
 template <typename A> __global__ void k_small_prep(A);
 template <typename A> __global__ void k_p_rho0(A);
-template <typename A, typename CT> __global__ void k_coef_w(A);
+template <typename A, typename CT, bool COEF> __global__ void k_coef_w(A);
 template <typename A, bool PER> __global__ void k_advance_uv(A);
 template <typename A, bool PER> __global__ void k_advance_uv_x4(A);
 template <typename A> __global__ void k_ac_div(A);

@@ -295,6 +295,7 @@ struct KRef {
 #define KR(K) KRef{K<KArgs>, K<BatchArgs>}
 #define KRP(K, P) KRef{K<KArgs, P>, K<BatchArgs, P>}
 #define KRZ(K) KRef{K<KArgs, false, true>, K<BatchArgs, false, true>}
+#define KRC(K, P, Q) KRef{K<KArgs, P, Q>, K<BatchArgs, P, Q>}
 
 // One launch of an ensemble member, as recorded for a batched launch (wrfmi_ensemble_*).
 struct RecOp {
@@ -320,6 +321,7 @@ struct Launcher {
   bool vec = false;          // the layout allows the four-points-per-lane kernels
   bool scal_all = false;     // species advected and updated by the all-species kernels
   bool zone_ok = false;      // each open direction is wide enough for a boundary zone on each side
+  bool whole_step = false;   // enqueued by the whole-step driver (work arrays stay internal)
   // launch with an optional event pair around it (kernel timing mode)
   template <typename K, typename... A>
   void launch(int id, K kern, dim3 g, dim3 b, A... args) {
@@ -462,8 +464,11 @@ static void ph_small_prep(Launcher &L) {
     L.launchK(KID_SMALL_PREP, KR(k_small_prep), g_pt3(a.d, nk), kColBlk, a);
     L.launchK(KID_P_RHO0, KR(k_p_rho0), g_pt3(a.d, nk - 1), kColBlk, a);
   }
-  if (a.c.precision_mode == 1) L.launchK(KID_COEF_W, KRP(k_coef_w, double), g_col(a.d), kColBlk, a);
-  else L.launchK(KID_COEF_W, KRP(k_coef_w, float), g_col(a.d), kColBlk, a);
+  // the per-routine entry point stores calc_coef_w's a, alpha, gamma as the reference does;
+  // the whole step leaves them to k_ac_wsolve, which forms them on its way up each column
+  const bool p64 = a.c.precision_mode == 1;
+  if (L.whole_step) L.launchK(KID_COEF_W, p64 ? KRC(k_coef_w, double, false) : KRC(k_coef_w, float, false), g_col(a.d), kColBlk, a);
+  else L.launchK(KID_COEF_W, p64 ? KRC(k_coef_w, double, true) : KRC(k_coef_w, float, true), g_col(a.d), kColBlk, a);
   L.halo_begin();
   L.halo_add(s.ru_tend, 'u', 1); L.halo_add(s.rv_tend, 'v', 1); L.halo_add(s.ph_2, 'w', 1);
   L.halo_add(s.al, 'p', 1); L.halo_add(s.p, 'p', 1); L.halo_add(s.t_1, 'p', 1); L.halo_add(s.t_save, 't', 1);
@@ -572,6 +577,7 @@ static void set_substep(KArgs &a, int rk, int nsound) {
 
 static void enqueue_step(Launcher &L) {
   KArgs &a = L.a;
+  L.whole_step = true;
   const int nsound = num_sound_steps(a.c, a.s);
   for (int rk = 1; rk <= 3; rk++) {
     set_substep(a, rk, nsound);
