@@ -27,6 +27,9 @@ CASES = {
     # rows of 101 and 256 points (one and exactly one 256-point wave row per grid row)
     "periodic_wide": dict(nx=100, ny=7, nz=6, bc="periodic", n_passive=1, n_bubbles=1, seed=23),
     "open_wide": dict(nx=255, ny=10, nz=5, bc="open", n_bubbles=1, seed=24),
+    # an open direction of 9 points: too narrow for a boundary zone on each side, so the
+    # tendency and scalar-advection kernels keep their one-point forms there
+    "open_narrow": dict(nx=8, ny=12, nz=6, bc="open", n_passive=1, n_bubbles=1, seed=25),
     # terrain on hybrid levels with one periodic and one open direction, both ways round
     "terrain": dict(nx=16, ny=12, nz=10, dx=2000.0, dy=2000.0, dt=6.0, bc_x="open", bc_y="periodic",
                     hybrid_opt=2, sounding="stable", n_ridges=3, ridge_a=(3000.0, 8000.0), n_passive=1,

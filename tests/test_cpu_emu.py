@@ -29,7 +29,7 @@ def emu_env():
                                                ("periodic_unaligned", 0, 1), ("open_unaligned", 1, 1),
                                                ("periodic_tight", 1, 1), ("open_tight", 0, 1),
                                                ("terrain", 0, 2), ("terrain_yopen", 1, 1),
-                                               ("periodic_wide", 1, 1), ("open_wide", 0, 1)])
+                                               ("periodic_wide", 1, 1), ("open_wide", 0, 1), ("open_narrow", 1, 2)])
 def test_emulated_kernels_match_oracle(emu_env, case, order, nsteps):
     r = subprocess.run([sys.executable, os.path.join(HERE, "emu_phase_check.py"), case, str(order), str(nsteps)],
                        env=emu_env, capture_output=True, text=True, timeout=600)

@@ -366,7 +366,8 @@ __global__ void __launch_bounds__(256) k_update_scalar_all_x4(ARGS args) {
     const FPtr<float> sc1 = s.q_old[m], sc2 = s.q[m];
     const FPtr<float> src = m == ns - 1 ? s.advect_tend : FPtr<float>(a.scr_qt + (size_t)m * a.qt_stride);
     const F4 adv = src.ld<F4>(o), qt = s.q_tend[m].ld<F4>(o);
-    const F4 q2 = sc2.ld<F4>(o), q1o = sc1.ld<F4>(o);
+    // the old time level: the current values at rk_step 1 (then saved), the saved ones after
+    const F4 q1 = (rk1 ? sc2 : sc1).ld<F4>(o);
     F4 qn;
 #pragma unroll
     for (int n = 0; n < 4; n++) {
@@ -374,10 +375,9 @@ __global__ void __launch_bounds__(256) k_update_scalar_all_x4(ARGS args) {
       const float munew = mu2.v[n] + mub.v[n];
       float tendency = adv.v[n] * msfty.v[n];
       tendency = tendency + qt.v[n];
-      const float q1 = rk1 ? q2.v[n] : q1o.v[n];
-      qn.v[n] = ((c1 * muold + c2) * q1 + a.dt_rk * tendency) / (c1 * munew + c2);
+      qn.v[n] = ((c1 * muold + c2) * q1.v[n] + a.dt_rk * tendency) / (c1 * munew + c2);
     }
-    if (rk1) st4m(sc1, o, q2, i0, d.ids, d.ide - 1);
+    if (rk1) st4m(sc1, o, q1, i0, d.ids, d.ide - 1);
     st4m(sc2, o, qn, i0, d.ids, d.ide - 1);
   }
 }
@@ -397,9 +397,10 @@ __global__ void __launch_bounds__(256) k_update_scalar_x4(ARGS args) {
   const FPtr<float> sc1 = s.q_old[a.species], sc2 = s.q[a.species];
   const F4 mu1 = s.mu_1.ld<F4>(o2), mu2 = s.mu_2.ld<F4>(o2), mub = s.mub.ld<F4>(o2), msfty = s.msfty.ld<F4>(o2);
   const F4 adv = s.advect_tend.ld<F4>(o), qt = s.q_tend[a.species].ld<F4>(o);
-  const F4 q2 = sc2.ld<F4>(o), q1o = sc1.ld<F4>(o);
-  const float c1 = s.c1h[X1(d, k)], c2 = s.c2h[X1(d, k)];
   const bool rk1 = a.rk_step == 1;
+  // the old time level: the current values at rk_step 1 (then saved), the saved ones after
+  const F4 q1 = (rk1 ? sc2 : sc1).ld<F4>(o);
+  const float c1 = s.c1h[X1(d, k)], c2 = s.c2h[X1(d, k)];
   F4 qn;
 #pragma unroll
   for (int n = 0; n < 4; n++) {
@@ -407,10 +408,9 @@ __global__ void __launch_bounds__(256) k_update_scalar_x4(ARGS args) {
     const float munew = mu2.v[n] + mub.v[n];
     float tendency = adv.v[n] * msfty.v[n];
     tendency = tendency + qt.v[n];
-    const float q1 = rk1 ? q2.v[n] : q1o.v[n];
-    qn.v[n] = ((c1 * muold + c2) * q1 + a.dt_rk * tendency) / (c1 * munew + c2);
+    qn.v[n] = ((c1 * muold + c2) * q1.v[n] + a.dt_rk * tendency) / (c1 * munew + c2);
   }
-  if (rk1) st4m(sc1, o, q2, i0, d.ids, d.ide - 1);
+  if (rk1) st4m(sc1, o, q1, i0, d.ids, d.ide - 1);
   st4m(sc2, o, qn, i0, d.ids, d.ide - 1);
 }
 

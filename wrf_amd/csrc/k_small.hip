@@ -731,7 +731,8 @@ __global__ void __launch_bounds__(256) k_small_finish_x4(ARGS args) {
   }
   if (kin) {
     const bool last = a.rk_step >= a.c.rk_ord;
-    const F4 t2 = s.t_2.ld<F4>(o), tsv = s.t_save.ld<F4>(o), hd = s.h_diabatic.ld<F4>(o);
+    // h_diabatic enters at the last RK substep only; before that the load re-reads t_save
+    const F4 t2 = s.t_2.ld<F4>(o), tsv = s.t_save.ld<F4>(o), hd = (last ? s.h_diabatic : s.t_save).ld<F4>(o);
     F4 tn;
 #pragma unroll
     for (int n = 0; n < 4; n++) {
@@ -793,8 +794,10 @@ __global__ void __launch_bounds__(256) k_small_prep_x4(ARGS args) {
   const Off o2 = X2(d, i0, j), o = X3(d, i0, k, j);
   const bool rk1 = a.rk_step == 1;
   const MuS4 m = small_mu4(a, i0, j);
+  // at rk_step 1 the *_1 time level is the *_2 one (then saved): its loads read *_2 again,
+  // which the cache serves, instead of a second array from HBM
   if (mj && kin) {
-    const F4 u2 = s.u_2.ld<F4>(o), u1o = s.u_1.ld<F4>(o), muu = s.muu.ld<F4>(o2), msfuy = s.msfuy.ld<F4>(o2);
+    const F4 u2 = s.u_2.ld<F4>(o), u1o = (rk1 ? s.u_2 : s.u_1).ld<F4>(o), muu = s.muu.ld<F4>(o2), msfuy = s.msfuy.ld<F4>(o2);
     F4 un;
 #pragma unroll
     for (int n = 0; n < 4; n++) {
@@ -806,7 +809,7 @@ __global__ void __launch_bounds__(256) k_small_prep_x4(ARGS args) {
     st4m(s.u_2, o, un, i0, ids, ide);
   }
   if (kin) {
-    const F4 v2 = s.v_2.ld<F4>(o), v1o = s.v_1.ld<F4>(o), muv = s.muv.ld<F4>(o2), mvi = s.msfvx_inv.ld<F4>(o2);
+    const F4 v2 = s.v_2.ld<F4>(o), v1o = (rk1 ? s.v_2 : s.v_1).ld<F4>(o), muv = s.muv.ld<F4>(o2), mvi = s.msfvx_inv.ld<F4>(o2);
     F4 vn;
 #pragma unroll
     for (int n = 0; n < 4; n++) {
@@ -821,7 +824,7 @@ __global__ void __launch_bounds__(256) k_small_prep_x4(ARGS args) {
   const F4 mut = s.mut.ld<F4>(o2), msfty = s.msfty.ld<F4>(o2);
   if (kin) {
     const F4 pb = s.pb.ld<F4>(o), pp = s.p.ld<F4>(o), alt = s.alt.ld<F4>(o);
-    const F4 t2 = s.t_2.ld<F4>(o), t1o = s.t_1.ld<F4>(o);
+    const F4 t2 = s.t_2.ld<F4>(o), t1o = (rk1 ? s.t_2 : s.t_1).ld<F4>(o);
     F4 c2a, tn;
 #pragma unroll
     for (int n = 0; n < 4; n++) {
@@ -835,7 +838,7 @@ __global__ void __launch_bounds__(256) k_small_prep_x4(ARGS args) {
     st4m(s.t_2, o, tn, i0, ids, ide - 1);
   }
   {
-    const F4 w2 = s.w_2.ld<F4>(o), ph2 = s.ph_2.ld<F4>(o), w1o = s.w_1.ld<F4>(o), ph1o = s.ph_1.ld<F4>(o), ww = s.ww.ld<F4>(o);
+    const F4 w2 = s.w_2.ld<F4>(o), ph2 = s.ph_2.ld<F4>(o), w1o = (rk1 ? s.w_2 : s.w_1).ld<F4>(o), ph1o = (rk1 ? s.ph_2 : s.ph_1).ld<F4>(o), ww = s.ww.ld<F4>(o);
     F4 wn, phn;
 #pragma unroll
     for (int n = 0; n < 4; n++) {
