@@ -40,13 +40,11 @@ KERNEL_FIELDS = {
     "k_tend_t": 19,         # t ru rv ww ph phb u v w t_tendf ph_tendf h_diabatic -> t_tend ph_tend 5x*_save
     "k_small_prep": 26,     # 5x*_1 5x*_2 p pb alt ww -> 5x*_save 5x*_2 c2a ww1
     "k_p_rho0": 8,          # alt ph t t_save c2a -> al p pm1
-    "k_coef_w": 5,          # c2a cqw -> a alpha gamma
     "k_advance_uv": 14,     # u v ru_tend rv_tend p pb ph php alt al cqu cqv -> u v
     "k_ac_div": 9,          # u v u_save v_save ru_m rv_m -> dvdxi ru_m rv_m
     "k_ac_mu_ww": 3,        # dvdxi ww1 -> ww
     "k_ac_theta": 16,       # t t_tend t_save u v ww ph_tend w ph_save phb ph ww_m -> t t_2ave rhs ww_m
-    "k_ac_wpred": 9,        # rhs ph t_2ave c2a alt cqw w rw_tend -> w
-    "k_ac_wsolve": 15,      # w a alpha gamma rhs t t_save alt c2a pm1 -> w ph al p pm1
+    "k_ac_wsolve": 16,      # w rhs ph t_2ave c2a alt rw_tend cqw t t_save pm1 -> w ph al p pm1
     "k_small_finish": 19,   # 5x*_2 5x*_save ww ww1 h_diabatic -> 5x*_2 ww
     "k_adv_scalar": 5,      # q ru_m rv_m ww_m -> advect_tend
     "k_update_scalar": 5,   # advect_tend q_tend q q_old -> q

@@ -468,9 +468,9 @@ __global__ void __launch_bounds__(256) k_advance_uv_x4(ARGS args) {
 //   k_ac_mu_ww  (column) dmdt, mu_2/mudf/muts/muave, omega scan -> ww
 //   k_ac_theta  (point)  theta update, t_2ave, advance_w's rhs -> t_2, t_2save, scr_rhs;
 //                        sumflux of ww_m
-//   k_ac_wpred  (point)  surface w at level kds -> w
-//   k_ac_wsolve (column) explicit w predictor (levels 2..kde), tridiagonal forward
-//                        elimination and back substitution, ph, calc_p_rho -> w, ph, al, p, pm1
+//   k_ac_wsolve (column) surface w, explicit w predictor (levels 2..kde), calc_coef_w's
+//                        coefficients, tridiagonal forward elimination and back
+//                        substitution, ph, calc_p_rho -> w, ph, al, p, pm1 (gamma)
 // Each point kernel only reads what earlier kernels wrote, so a thread never depends on
 // another thread of its own launch.
 
@@ -635,32 +635,6 @@ __global__ void __launch_bounds__(256) k_ac_theta(ARGS args) {
   a.scr_rhs[o] = r;
 }
 
-// surface w (advance_w 1383-1395): one thread per (i, j), level kds only; the explicit
-// predictor of levels 2..kde (1402-1431) is formed inside k_ac_wsolve
-template <typename ARGS>
-__global__ void __launch_bounds__(256) k_ac_wpred(ARGS args) {
-  Tile3 T;
-  const KArgs &a = kargs(args, T);
-  const Dm &d = a.d;
-  const DState &s = dstate(a);
-  const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
-  const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
-  if (i > d.ide - 1 || j > d.jde - 1) return;
-  if (!a.c.non_hydrostatic) return;
-  const Off o2 = X2(d, i, j), o = X3(d, i, d.kds, j);
-  const float msftx = s.msftx[o2], msfty = s.msfty[o2], rdx = s.rdx, rdy = s.rdy;
-  FPtr<float> w = s.w_2;
-  {
-    const FPtr<float> u = s.u_2, v = s.v_2, ht = s.ht;
-    const float cf1 = s.cf1, cf2 = s.cf2, cf3 = s.cf3;
-    w[o] = msfty * .5f * rdy *
-               ((ht[X2(d, i, j + 1)] - ht[o2]) * (cf1 * v[X3(d, i, 1, j + 1)] + cf2 * v[X3(d, i, 2, j + 1)] + cf3 * v[X3(d, i, 3, j + 1)]) +
-                (ht[o2] - ht[X2(d, i, j - 1)]) * (cf1 * v[X3(d, i, 1, j)] + cf2 * v[X3(d, i, 2, j)] + cf3 * v[X3(d, i, 3, j)])) +
-           msftx * .5f * rdx *
-               ((ht[X2(d, i + 1, j)] - ht[o2]) * (cf1 * u[X3(d, i + 1, 1, j)] + cf2 * u[X3(d, i + 1, 2, j)] + cf3 * u[X3(d, i + 1, 3, j)]) +
-                (ht[o2] - ht[X2(d, i - 1, j)]) * (cf1 * u[X3(d, i, 1, j)] + cf2 * u[X3(d, i, 2, j)] + cf3 * u[X3(d, i, 3, j)]));
-  }
-}
 
 // calc_mu_uv_1 + small_step_finish with four i points per lane (16-byte accesses; element by
 // element the expressions of k_small_finish)
@@ -889,7 +863,7 @@ __global__ void __launch_bounds__(256) k_p_rho0_x4(ARGS args) {
 }
 
 // ---------------------------------------------------------------------------------------
-// Four-points-per-lane forms of k_ac_div, k_ac_theta and k_ac_wpred (16-byte loads and
+// Four-points-per-lane forms of k_ac_div and k_ac_theta (16-byte loads and
 // stores; used when the host finds the layout aligned).  Element by element they evaluate
 // the same expressions in the same order as the one-point kernels above; i neighbours come
 // from the groups of four on either side.
@@ -1044,43 +1018,6 @@ __global__ void __launch_bounds__(256) k_ac_theta_x4(ARGS args) {
   st4m(a.scr_rhs, o, rr, i0, ids, ide - 1);
 }
 
-template <typename ARGS>
-__global__ void __launch_bounds__(256) k_ac_wpred_x4(ARGS args) {
-  Tile3 T;
-  const KArgs &a = kargs(args, T);
-  const Dm &d = a.d;
-  const DState &s = dstate(a);
-  const int i0 = vec_ia0(d) + kVecW * (T.bx * 64 + (int)threadIdx.x);
-  const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
-  const int ids = d.ids, ide = d.ide, jde = d.jde;
-  if (i0 > ide - 1 || j > jde - 1) return;
-  if (!a.c.non_hydrostatic) return;
-  const Off o2 = X2(d, i0, j), o = X3(d, i0, d.kds, j);
-  const int ni = d.ni, sj = d.sj;
-  const float rdx = s.rdx, rdy = s.rdy;
-  const F4 msftx = s.msftx.ld<F4>(o2), msfty = s.msfty.ld<F4>(o2);
-  FPtr<float> w = s.w_2;
-  {
-    const FPtr<float> u = s.u_2, v = s.v_2, ht = s.ht;
-    const float cf1 = s.cf1, cf2 = s.cf2, cf3 = s.cf3;
-    const F4 h0 = ht.ld<F4>(o2), hl = ht.ld<F4>(o2 - 4), hr = ht.ld<F4>(o2 + 4), hn = ht.ld<F4>(o2 + ni), hs = ht.ld<F4>(o2 - ni);
-    const F4 v1n = v.ld<F4>(o + sj), v2n = v.ld<F4>(o + sj + ni), v3n = v.ld<F4>(o + sj + 2 * ni);
-    const F4 v1 = v.ld<F4>(o), v2 = v.ld<F4>(o + ni), v3 = v.ld<F4>(o + 2 * ni);
-    const F4 u1 = u.ld<F4>(o), u2 = u.ld<F4>(o + ni), u3 = u.ld<F4>(o + 2 * ni);
-    const F4 u1r = u.ld<F4>(o + 4), u2r = u.ld<F4>(o + ni + 4), u3r = u.ld<F4>(o + 2 * ni + 4);
-    F4 wv;
-#pragma unroll
-    for (int n = 0; n < 4; n++) {
-      wv.v[n] = msfty.v[n] * .5f * rdy *
-                    ((hn.v[n] - h0.v[n]) * (cf1 * v1n.v[n] + cf2 * v2n.v[n] + cf3 * v3n.v[n]) +
-                     (h0.v[n] - hs.v[n]) * (cf1 * v1.v[n] + cf2 * v2.v[n] + cf3 * v3.v[n])) +
-                msftx.v[n] * .5f * rdx *
-                    ((xp1(h0, hr, n) - h0.v[n]) * (cf1 * xp1(u1, u1r, n) + cf2 * xp1(u2, u2r, n) + cf3 * xp1(u3, u3r, n)) +
-                     (h0.v[n] - xm1(hl, h0, n)) * (cf1 * u1.v[n] + cf2 * u2.v[n] + cf3 * u3.v[n]));
-    }
-    st4m(w, o, wv, i0, ids, ide - 1);
-  }
-}
 
 // Per column: the explicit w predictor (advance_w 1402-1431), the tridiagonal forward
 // elimination and back substitution (1433-1443), ph from rhs and the final w (1460-1465)
@@ -1104,7 +1041,21 @@ __global__ void __launch_bounds__(256) k_ac_wsolve(ARGS args) {
   FPtr<CT> GA((CT *)s.gamma);
   const Off ob = X3(d, i, 0, j), o2 = X2(d, i, j);
   const int ni = d.ni;
-  CT wfe = (CT)w[ob + ni];  // surface w (level 1) from k_ac_wpred
+  // advance_w's lower boundary (1383-1395): w(1) from u, v extrapolated to the surface by
+  // cf1..cf3 and the terrain slopes on both sides of the point
+  float wsfc;
+  {
+    const FPtr<float> u = s.u_2, v = s.v_2, ht = s.ht;
+    const float cf1 = s.cf1, cf2 = s.cf2, cf3 = s.cf3, msftx = s.msftx[o2], msfty_s = s.msfty[o2];
+    wsfc = msfty_s * .5f * s.rdy *
+               ((ht[X2(d, i, j + 1)] - ht[o2]) * (cf1 * v[X3(d, i, 1, j + 1)] + cf2 * v[X3(d, i, 2, j + 1)] + cf3 * v[X3(d, i, 3, j + 1)]) +
+                (ht[o2] - ht[X2(d, i, j - 1)]) * (cf1 * v[X3(d, i, 1, j)] + cf2 * v[X3(d, i, 2, j)] + cf3 * v[X3(d, i, 3, j)])) +
+           msftx * .5f * s.rdx *
+               ((ht[X2(d, i + 1, j)] - ht[o2]) * (cf1 * u[X3(d, i + 1, 1, j)] + cf2 * u[X3(d, i + 1, 2, j)] + cf3 * u[X3(d, i + 1, 3, j)]) +
+                (ht[o2] - ht[X2(d, i - 1, j)]) * (cf1 * u[X3(d, i, 1, j)] + cf2 * u[X3(d, i, 2, j)] + cf3 * u[X3(d, i, 3, j)]));
+    w[ob + ni] = wsfc;
+  }
+  CT wfe = (CT)wsfc;
   // Upward: the explicit w predictor of advance_w (1402-1431) at level k, calc_coef_w's
   // coefficients a(k), alpha(k), gamma(k) (570-652, the same expressions in the same order
   // as k_coef_w), then the forward elimination.  Fields: 0 = w, 1 = rhs(k+1), 2 = ph(k+1),
@@ -1331,12 +1282,10 @@ WRFMI_INST(k_small_prep)
 WRFMI_INST(k_p_rho0)
 WRFMI_INST(k_ac_div)
 WRFMI_INST(k_ac_theta)
-WRFMI_INST(k_ac_wpred)
 WRFMI_INST(k_ac_div_x4)
 WRFMI_INST(k_small_prep_x4)
 WRFMI_INST(k_p_rho0_x4)
 WRFMI_INST(k_ac_theta_x4)
-WRFMI_INST(k_ac_wpred_x4)
 WRFMI_INST(k_small_finish)
 WRFMI_INST(k_small_finish_x4)
 

@@ -35,12 +35,10 @@ template <typename A, bool PER> __global__ void k_advance_uv_x4(A);
 template <typename A> __global__ void k_ac_div(A);
 template <typename A, bool COMP> __global__ void k_ac_mu_ww(A);
 template <typename A> __global__ void k_ac_theta(A);
-template <typename A> __global__ void k_ac_wpred(A);
 template <typename A> __global__ void k_ac_div_x4(A);
 template <typename A> __global__ void k_small_prep_x4(A);
 template <typename A> __global__ void k_p_rho0_x4(A);
 template <typename A> __global__ void k_ac_theta_x4(A);
-template <typename A> __global__ void k_ac_wpred_x4(A);
 template <typename A, typename CT> __global__ void k_ac_wsolve(A);
 template <typename A> __global__ void k_small_finish(A);
 template <typename A> __global__ void k_small_finish_x4(A);

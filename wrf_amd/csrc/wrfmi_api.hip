@@ -263,12 +263,12 @@ static const dim3 kColBlk(64, 4, 1);
 enum KernelId {
   KID_STEP_PREP, KID_TEND_U, KID_TEND_V, KID_TEND_W, KID_TEND_T, KID_SMALL_PREP, KID_ADVANCE_UV,
   KID_AC_DIV, KID_SMALL_FINISH, KID_ADV_SCALAR, KID_UPDATE_SCALAR, KID_P_RHO_PHI, KID_HALO, KID_P_RHO0,
-  KID_COEF_W, KID_WW_SCAN, KID_AC_MU_WW, KID_AC_THETA, KID_AC_WPRED, KID_AC_WSOLVE, KID_W_SURFACE, KID_COUNT
+  KID_COEF_W, KID_WW_SCAN, KID_AC_MU_WW, KID_AC_THETA, KID_AC_WSOLVE, KID_W_SURFACE, KID_COUNT
 };
 static const char *kKernelNames[KID_COUNT] = {
   "k_step_prep", "k_tend_u", "k_tend_v", "k_tend_w", "k_tend_t", "k_small_prep", "k_advance_uv",
   "k_ac_div", "k_small_finish", "k_adv_scalar", "k_update_scalar", "k_p_rho_phi", "k_halo", "k_p_rho0",
-  "k_coef_w", "k_ww_scan", "k_ac_mu_ww", "k_ac_theta", "k_ac_wpred", "k_ac_wsolve", "k_set_w_surface"};
+  "k_coef_w", "k_ww_scan", "k_ac_mu_ww", "k_ac_theta", "k_ac_wsolve", "k_set_w_surface"};
 struct EvPair { int id; hipEvent_t a, b; };
 static unsigned g_timing = 0;  // bit n set: time launches of kernel n
 static std::vector<EvPair> g_ev_pending;
@@ -492,13 +492,8 @@ static void ph_advance_mu_t_w(Launcher &L) {
   else L.launchK(KID_AC_DIV, KR(k_ac_div), g_pt3(a.d, nk), kColBlk, a);
   if (p64) L.launchK(KID_AC_MU_WW, KRP(k_ac_mu_ww, true), g_col(a.d), kColBlk, a);
   else L.launchK(KID_AC_MU_WW, KRP(k_ac_mu_ww, false), g_col(a.d), kColBlk, a);
-  if (L.vec) {
-    L.launchK(KID_AC_THETA, KR(k_ac_theta_x4), g_vec3(a, nk), kColBlk, a);
-    L.launchK(KID_AC_WPRED, KR(k_ac_wpred_x4), g_vec3(a, 1), kColBlk, a);
-  } else {
-    L.launchK(KID_AC_THETA, KR(k_ac_theta), g_pt3(a.d, nk), kColBlk, a);
-    L.launchK(KID_AC_WPRED, KR(k_ac_wpred), g_pt3(a.d, 1), kColBlk, a);
-  }
+  if (L.vec) L.launchK(KID_AC_THETA, KR(k_ac_theta_x4), g_vec3(a, nk), kColBlk, a);
+  else L.launchK(KID_AC_THETA, KR(k_ac_theta), g_pt3(a.d, nk), kColBlk, a);
   if (p64) L.launchK(KID_AC_WSOLVE, KRP(k_ac_wsolve, double), g_col(a.d), kColBlk, a);
   else L.launchK(KID_AC_WSOLVE, KRP(k_ac_wsolve, float), g_col(a.d), kColBlk, a);
   L.halo_begin();
