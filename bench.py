@@ -43,8 +43,8 @@ KERNEL_FIELDS = {
     "k_advance_uv": 14,     # u v ru_tend rv_tend p pb ph php alt al cqu cqv -> u v
     "k_ac_div": 9,          # u v u_save v_save ru_m rv_m -> dvdxi ru_m rv_m
     "k_ac_mu_ww": 3,        # dvdxi ww1 -> ww
-    "k_ac_theta": 16,       # t t_tend t_save u v ww ph_tend w ph_save phb ph ww_m -> t t_2ave rhs ww_m
-    "k_ac_wsolve": 16,      # w rhs ph t_2ave c2a alt rw_tend cqw t t_save pm1 -> w ph al p pm1
+    "k_ac_theta": 11,       # t t_tend t_save u v ww ww_m ww1 -> t t_2save ww_m
+    "k_ac_wsolve": 19,      # t_2save c2a alt rw_tend cqw w ph ph_tend ph_save phb ww t t_save pm1 -> w ph al p pm1
     "k_small_finish": 19,   # 5x*_2 5x*_save ww ww1 h_diabatic -> 5x*_2 ww
     "k_adv_scalar": 5,      # q ru_m rv_m ww_m -> advect_tend
     "k_update_scalar": 5,   # advect_tend q_tend q q_old -> q

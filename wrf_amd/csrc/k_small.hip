@@ -466,9 +466,8 @@ __global__ void __launch_bounds__(256) k_advance_uv_x4(ARGS args) {
 // Per substep, in order:
 //   k_ac_div    (point)  dvdxi -> scr_dv; sumflux of ru_m, rv_m
 //   k_ac_mu_ww  (column) dmdt, mu_2/mudf/muts/muave, omega scan -> ww
-//   k_ac_theta  (point)  theta update, t_2ave, advance_w's rhs -> t_2, t_2save, scr_rhs;
-//                        sumflux of ww_m
-//   k_ac_wsolve (column) surface w, explicit w predictor (levels 2..kde), calc_coef_w's
+//   k_ac_theta  (point)  theta update, t_2ave -> t_2, t_2save; sumflux of ww_m
+//   k_ac_wsolve (column) surface w, advance_w's rhs, explicit w predictor (levels 2..kde), calc_coef_w's
 //                        coefficients, tridiagonal forward elimination and back
 //                        substitution, ph, calc_p_rho -> w, ph, al, p, pm1 (gamma)
 // Each point kernel only reads what earlier kernels wrote, so a thread never depends on
@@ -612,27 +611,6 @@ __global__ void __launch_bounds__(256) k_ac_theta(ARGS args) {
       s.t_2save[o] = tv;
     }
   }
-  if (!c.non_hydrostatic || k < 2) return;
-  // rhs(k), k = 2..kde (advance_w 1324-1371)
-  const FPtr<float> ph_1 = s.ph_save, phb = s.phb;
-  auto wdwn_at = [&](int kp1) -> float {  // wdwn(k+1) for k = kp1-1
-    const int kk = kp1 - 1;
-    const Off oo = X3(d, i, kk, j);
-    if (c.phi_adv_z == 2)
-      return s.rdnw[X1(d, kk)] * (ph_1[oo + d.ni] - ph_1[oo] + phb[oo + d.ni] - phb[oo]);
-    return .5f * (ww[oo + d.ni] + ww[oo]) * s.rdnw[X1(d, kk)] * (ph_1[oo + d.ni] - ph_1[oo] + phb[oo + d.ni] - phb[oo]);
-  };
-  const float mut = s.mut[o2];
-  float r = dts * (s.ph_tend[o] + .5f * K_G * (1.f - epssm) * s.w_2[o]);
-  if (k <= k_end) {
-    if (c.phi_adv_z == 2)
-      r = r - dts * ww[o] * (s.fnm[X1(d, k)] * wdwn_at(k + 1) + s.fnp[X1(d, k)] * wdwn_at(k));
-    else
-      r = r - dts * (s.fnm[X1(d, k)] * wdwn_at(k + 1) + s.fnp[X1(d, k)] * wdwn_at(k));
-  }
-  r = s.ph_2[o] + msfty * r / (s.c1f[X1(d, k)] * mut + s.c2f[X1(d, k)]);
-  if (c.top_lid && k == k_end + 1) r = 0.f;
-  a.scr_rhs[o] = r;
 }
 
 
@@ -989,33 +967,6 @@ __global__ void __launch_bounds__(256) k_ac_theta_x4(ARGS args) {
     st4m(s.t_2, o, tn, i0, ids, ide - 1);
     st4m(s.t_2save, o, t2s, i0, ids, ide - 1);
   }
-  if (!c.non_hydrostatic || k < 2) return;
-  // rhs(k), k = 2..kde
-  const F4 ph1 = s.ph_save.ld<F4>(o), ph1p = s.ph_save.ld<F4>(op), ph1m = s.ph_save.ld<F4>(om);
-  const F4 pb0 = s.phb.ld<F4>(o), pbp = s.phb.ld<F4>(op), pbm = s.phb.ld<F4>(om);
-  const F4 wwp = s.ww.ld<F4>(op), wwm = s.ww.ld<F4>(om);
-  const F4 pht = s.ph_tend.ld<F4>(o), w2 = s.w_2.ld<F4>(o), ph2 = s.ph_2.ld<F4>(o), mut = s.mut.ld<F4>(o2);
-  const float rdnw_k = s.rdnw[X1(d, imin(k, k_end))], rdnw_m = s.rdnw[X1(d, k - 1)];
-  const float fnm_0 = s.fnm[X1(d, k)], fnp_0 = s.fnp[X1(d, k)], c1f = s.c1f[X1(d, k)], c2f = s.c2f[X1(d, k)];
-  const bool z2 = c.phi_adv_z == 2;
-  F4 rr;
-#pragma unroll
-  for (int n = 0; n < 4; n++) {
-    // wdwn(k+1) (level k) and wdwn(k) (level k-1)
-    const float dph_hi = ph1p.v[n] - ph1.v[n] + pbp.v[n] - pb0.v[n];
-    const float dph_lo = ph1.v[n] - ph1m.v[n] + pb0.v[n] - pbm.v[n];
-    const float wd_hi = z2 ? rdnw_k * dph_hi : .5f * (wwp.v[n] + ww.v[n]) * rdnw_k * dph_hi;
-    const float wd_lo = z2 ? rdnw_m * dph_lo : .5f * (ww.v[n] + wwm.v[n]) * rdnw_m * dph_lo;
-    float r = dts * (pht.v[n] + .5f * K_G * (1.f - epssm) * w2.v[n]);
-    if (k <= k_end) {
-      if (z2) r = r - dts * ww.v[n] * (fnm_0 * wd_hi + fnp_0 * wd_lo);
-      else r = r - dts * (fnm_0 * wd_hi + fnp_0 * wd_lo);
-    }
-    r = ph2.v[n] + msfty.v[n] * r / (c1f * mut.v[n] + c2f);
-    if (c.top_lid && k == k_end + 1) r = 0.f;
-    rr.v[n] = r;
-  }
-  st4m(a.scr_rhs, o, rr, i0, ids, ide - 1);
 }
 
 
@@ -1056,20 +1007,48 @@ __global__ void __launch_bounds__(256) k_ac_wsolve(ARGS args) {
     w[ob + ni] = wsfc;
   }
   CT wfe = (CT)wsfc;
-  // Upward: the explicit w predictor of advance_w (1402-1431) at level k, calc_coef_w's
-  // coefficients a(k), alpha(k), gamma(k) (570-652, the same expressions in the same order
-  // as k_coef_w), then the forward elimination.  Fields: 0 = w, 1 = rhs(k+1), 2 = ph(k+1),
-  // 3 = t_2ave, 4 = c2a, 5 = alt, 6 = rw_tend, 7 = cqw (k+1 clamped to kde); rhs, ph at k
-  // and k-1 and t_2ave, c2a, alt at k-1 are carried from the levels below.  gamma is
-  // stored for the way down.
+  // Upward: advance_w's right-hand side rhs(k+1) (1324-1371: phi advection by omega, the
+  // g*w term, rhs = ph + msfty*rhs/(c1f*mut+c2f); stored for the way down), the explicit
+  // w predictor (1402-1431) at level k, calc_coef_w's coefficients a(k), alpha(k),
+  // gamma(k) (570-652, the same expressions in the same order as k_coef_w; gamma stored
+  // for the way down), then the forward elimination.  Fields at level k: 0 = t_2ave,
+  // 1 = c2a, 2 = alt, 3 = rw_tend, 4 = cqw; at k+1 (clamped to kde): 5 = w, 6 = ph,
+  // 7 = ph_tend; at k+2 (clamped): 8 = ph_save, 9 = phb, 10 = omega.  w at k, rhs and ph at
+  // k and k-1, t_2ave, c2a and alt at k-1 and ph_save, phb and omega at k and k+1 are
+  // carried from the levels below.
   const float dts = a.dts, epssm = c.epssm;
   const FPtr<float> ph_c = s.ph_2, t2ave = s.t_2save, c2a = s.c2a, alt_f = s.alt;
+  const FPtr<float> ph1f = s.ph_save, phbf = s.phb, wwf = s.ww;
   const float msfty = s.msfty[o2], mut = s.mut[o2], muave = s.muave[o2];
   const CPtr<float> c1h = s.c1h, c2h = s.c2h, c1f = s.c1f, c2f = s.c2f, rdnw = s.rdnw, rdn = s.rdn;
+  const CPtr<float> fnm = s.fnm, fnp = s.fnp;
   const float msft_inv = 1.f / msfty;
-  float rhs_m1 = 0.f, rhs_0 = rhs[ob + 2 * ni];  // rhs(1) = 0
-  float phl_m1 = ph_c[ob + ni], phl_0 = ph_c[ob + 2 * ni];
-  float t2a_lo = t2ave[ob + ni], c2a_lo = c2a[ob + ni], alt_lo = alt_f[ob + ni];
+  const bool z2 = c.phi_adv_z == 2;
+  // rhs(kk) for kk = 2..kde from ph_save, phb, omega at kk-1, kk, min(kk+1, kde)
+  auto rhs_at = [&](int kk, float w2, float pht, float ph2, float ph1m, float ph1, float ph1p, float pbm, float pb0,
+                    float pbp, float wwm, float ww, float wwp) -> float {
+    const float rdnw_k = rdnw[X1(d, imin(kk, k_end))], rdnw_m = rdnw[X1(d, kk - 1)];
+    const float dph_hi = ph1p - ph1 + pbp - pb0;
+    const float dph_lo = ph1 - ph1m + pb0 - pbm;
+    const float wd_hi = z2 ? rdnw_k * dph_hi : .5f * (wwp + ww) * rdnw_k * dph_hi;
+    const float wd_lo = z2 ? rdnw_m * dph_lo : .5f * (ww + wwm) * rdnw_m * dph_lo;
+    float r = dts * (pht + .5f * K_G * (1.f - epssm) * w2);
+    if (kk <= k_end) {
+      if (z2) r = r - dts * ww * (fnm[X1(d, kk)] * wd_hi + fnp[X1(d, kk)] * wd_lo);
+      else r = r - dts * (fnm[X1(d, kk)] * wd_hi + fnp[X1(d, kk)] * wd_lo);
+    }
+    r = ph2 + msfty * r / (c1f[X1(d, kk)] * mut + c2f[X1(d, kk)]);
+    if (c.top_lid && kk == k_end + 1) r = 0.f;
+    return r;
+  };
+  const Off o1 = ob + ni, o2l = ob + 2 * ni, o3 = ob + imin(3, kde) * ni;
+  float w_k = w[o2l];  // w at the march's level (before the predictor)
+  float ph1_k = ph1f[o2l], ph1_k1 = ph1f[o3], pb_k = phbf[o2l], pb_k1 = phbf[o3], ww_k = wwf[o2l], ww_k1 = wwf[o3];
+  float rhs_m1 = 0.f;  // rhs(1) = 0
+  float rhs_0 = rhs_at(2, w_k, s.ph_tend[o2l], ph_c[o2l], ph1f[o1], ph1_k, ph1_k1, phbf[o1], pb_k, pb_k1, wwf[o1], ww_k, ww_k1);
+  rhs[o2l] = rhs_0;
+  float phl_m1 = ph_c[o1], phl_0 = ph_c[o2l];
+  float t2a_lo = t2ave[o1], c2a_lo = c2a[o1], alt_lo = alt_f[o1];
   const CT lid = c.top_lid ? CT(0) : CT(1);
   const CT epc = (CT)c.epssm;
   CT cof = (CT(.5) * (CT)a.dts * (CT)K_G * (CT(1) + epc));
@@ -1080,25 +1059,33 @@ __global__ void __launch_bounds__(256) k_ac_wsolve(ARGS args) {
   auto rdnwC = [&](int k) -> CT { return (CT)rdnw[X1(d, k)]; };
   auto rdnC = [&](int k) -> CT { return (CT)rdn[X1(d, k)]; };
   CT gam = CT(0);
-  column_march_up<8, CT, 4>(
+  column_march_up<11, float, 4>(
       2, k_end + 1,
-      [&](int f, int k) -> CT {
-        const Off o = ob + k * ni, op = ob + imin(k + 1, kde) * ni;
+      [&](int f, int k) -> float {
+        const Off o = ob + k * ni, op = ob + imin(k + 1, kde) * ni, opp = ob + imin(k + 2, kde) * ni;
         switch (f) {
-          case 0: return (CT)w[o];
-          case 1: return (CT)rhs[op];
-          case 2: return (CT)ph_c[op];
-          case 3: return (CT)t2ave[o];
-          case 4: return (CT)c2a[o];
-          case 5: return (CT)alt_f[o];
-          case 6: return (CT)s.rw_tend[o];
-          default: return (CT)s.cqw[o];
+          case 0: return t2ave[o];
+          case 1: return c2a[o];
+          case 2: return alt_f[o];
+          case 3: return s.rw_tend[o];
+          case 4: return s.cqw[o];
+          case 5: return w[op];
+          case 6: return ph_c[op];
+          case 7: return s.ph_tend[op];
+          case 8: return ph1f[opp];
+          case 9: return phbf[opp];
+          default: return wwf[opp];
         }
       },
-      [&](int k, const CT *v) {
+      [&](int k, const float *v) {
         const Off o = ob + k * ni;
-        const float w0 = (float)v[0], rhs_p1 = (float)v[1], phl_p1 = (float)v[2], t2a_hi = (float)v[3];
-        const float c2a0 = (float)v[4], alt0 = (float)v[5], rwt = (float)v[6], cqw = (float)v[7];
+        const float w0 = w_k, t2a_hi = v[0], c2a0 = v[1], alt0 = v[2], rwt = v[3], cqw = v[4];
+        const float w_p1 = v[5], phl_p1 = v[6];
+        float rhs_p1 = 0.f;
+        if (k + 1 <= kde) {
+          rhs_p1 = rhs_at(k + 1, w_p1, v[7], phl_p1, ph1_k, ph1_k1, v[8], pb_k, pb_k1, v[9], ww_k, ww_k1, v[10]);
+          rhs[o + ni] = rhs_p1;
+        }
         float wv;
         CT ak, alpha;
         if (k <= k_end) {
@@ -1111,7 +1098,7 @@ __global__ void __launch_bounds__(256) k_ac_wsolve(ARGS args) {
                          ((1.f + epssm) * (rhs_0 - rhs_m1) + (1.f - epssm) * (phl_0 - phl_m1)))) +
                dts * K_G * msft_inv *
                    (rdn[X1(d, k)] * (c2a0 * alt0 * t2a_hi - c2a_lo * alt_lo * t2a_lo) - (c1f[X1(d, k)] * muave));
-          const CT CQW = v[7], C2A = v[4], C2Al = (CT)c2a_lo;
+          const CT CQW = (CT)cqw, C2A = (CT)c2a0, C2Al = (CT)c2a_lo;
           ak = k >= 3 ? -CQW * cof * rdnC(k) * rdnwC(k - 1) * C2Al / (cH(k - 1) * cF(k - 1)) : CT(0);
           const CT b = CT(1) + CQW * cof * rdnC(k) *
                                    (rdnwC(k) * C2A / (cH(k) * cF(k)) + rdnwC(k - 1) * C2Al / (cH(k - 1) * cF(k)));
@@ -1137,6 +1124,13 @@ __global__ void __launch_bounds__(256) k_ac_wsolve(ARGS args) {
         t2a_lo = t2a_hi;
         c2a_lo = c2a0;
         alt_lo = alt0;
+        w_k = w_p1;
+        ph1_k = ph1_k1;
+        ph1_k1 = v[8];
+        pb_k = pb_k1;
+        pb_k1 = v[9];
+        ww_k = ww_k1;
+        ww_k1 = v[10];
         if (sizeof(CT) == sizeof(float)) {
           const float wf = (wv - (float)ak * (float)wfe) * (float)alpha;
           wfe = (CT)wf;
