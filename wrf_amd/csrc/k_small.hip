@@ -974,6 +974,11 @@ __global__ void __launch_bounds__(256) k_ac_theta_x4(ARGS args) {
 // elimination and back substitution (1433-1443), ph from rhs and the final w (1460-1465)
 // and calc_p_rho for step >= 1
 // (438-568).  CT = double keeps the forward value in FP64, its low-order part parked in scr_lo.
+// levels of loads in flight ahead of each march of k_ac_wsolve (2, 4, 6 and 8 time within
+// 2 % of each other at 400x400x80: the kernel is bound by the bytes it moves)
+#ifndef WRFMI_WSOLVE_PF
+#define WRFMI_WSOLVE_PF 4
+#endif
 template <typename ARGS, typename CT>
 __global__ void __launch_bounds__(256) k_ac_wsolve(ARGS args) {
   Tile3 T;
@@ -1059,7 +1064,7 @@ __global__ void __launch_bounds__(256) k_ac_wsolve(ARGS args) {
   auto rdnwC = [&](int k) -> CT { return (CT)rdnw[X1(d, k)]; };
   auto rdnC = [&](int k) -> CT { return (CT)rdn[X1(d, k)]; };
   CT gam = CT(0);
-  column_march_up<11, float, 4>(
+  column_march_up<11, float, WRFMI_WSOLVE_PF>(
       2, k_end + 1,
       [&](int f, int k) -> float {
         const Off o = ob + k * ni, op = ob + imin(k + 1, kde) * ni, opp = ob + imin(k + 2, kde) * ni;
@@ -1147,7 +1152,7 @@ __global__ void __launch_bounds__(256) k_ac_wsolve(ARGS args) {
   const float smdiv = c.smdiv;
   CT wup = wfe;
   float ph_up = 0.f;  // final ph one level above
-  column_march_down<8, CT, 4>(
+  column_march_down<8, CT, WRFMI_WSOLVE_PF>(
       1, k_end + 1,
       [&](int f, int k) -> CT {
         const Off o = ob + k * ni;
