@@ -393,20 +393,21 @@ __global__ void __launch_bounds__(256) k_advance_uv_x4(ARGS args) {
     }
   }
   auto lv = [&](int k) { return ob + imin(k, kde) * ni; };
+  // p and ph of the level itself (centre, i-1 and j-1 groups), carried from the level below
+  F4 pc = p.ld<F4>(lv(k0)), hc = ph.ld<F4>(lv(k0)), pl = p.ld<F4>(lv(k0) - 4), hl = ph.ld<F4>(lv(k0) - 4);
+  F4 ps = p.ld<F4>(lv(k0) - sj), hs = ph.ld<F4>(lv(k0) - sj);
 #pragma unroll 1
-  for (int kk = 0; kk < kAdvUvK; kk++) {
-    const int k = k0 + kk;
-    const int kc = imin(k, k1);
-    const Off o = lv(kc), op = lv(k + 1);
-    const float c1 = s.c1h[X1(d, kc)], c2 = s.c2h[X1(d, kc)], rdnw = s.rdnw[X1(d, kc)];
+  for (int k = k0; k <= k1; k++) {
+    const Off o = lv(k), op = lv(k + 1);
+    const float c1 = s.c1h[X1(d, k)], c2 = s.c2h[X1(d, k)], rdnw = s.rdnw[X1(d, k)];
     const int kp = imin(k + 1, kde);
     const float fm = s.fnm[X1(d, kp)], fp = s.fnp[X1(d, kp)];
     const bool top = k + 1 == kde;
-    const F4 pc = p.ld<F4>(o), pcp = p.ld<F4>(op), hc = ph.ld<F4>(o), hcp = ph.ld<F4>(op);
+    const F4 pcp = p.ld<F4>(op), hcp = ph.ld<F4>(op);
     const F4 altc = alt.ld<F4>(o), alc = al.ld<F4>(o), pbc = pb.ld<F4>(o), phpc = php.ld<F4>(o);
+    const F4 plp = p.ld<F4>(op - 4), hlp = ph.ld<F4>(op - 4), psp = p.ld<F4>(op - sj), hsp = ph.ld<F4>(op - sj);
     // u: neighbours at i-1
     {
-      const F4 pl = p.ld<F4>(o - 4), plp = p.ld<F4>(op - 4), hl = ph.ld<F4>(o - 4), hlp = ph.ld<F4>(op - 4);
       const F4 altl = alt.ld<F4>(o - 4), all = al.ld<F4>(o - 4), pbl = pb.ld<F4>(o - 4), phpl = php.ld<F4>(o - 4);
       const F4 u0 = s.u_2.ld<F4>(o), ut = s.ru_tend.ld<F4>(o), cqu = s.cqu.ld<F4>(o);
       F4 un;
@@ -429,11 +430,10 @@ __global__ void __launch_bounds__(256) k_advance_uv_x4(ARGS args) {
         const float upg = u - dts * cqu.v[n] * dpxy + (c1 * mudf_xu[n]);
         un.v[n] = upg_on ? upg : u;
       }
-      if (u_on && k <= k1) st4m(s.u_2, o, un, i0, ids, ide);
+      if (u_on) st4m(s.u_2, o, un, i0, ids, ide);
     }
     // v: neighbours at j-1
     {
-      const F4 ps = p.ld<F4>(o - sj), psp = p.ld<F4>(op - sj), hs = ph.ld<F4>(o - sj), hsp = ph.ld<F4>(op - sj);
       const F4 alts = alt.ld<F4>(o - sj), als = al.ld<F4>(o - sj), pbs = pb.ld<F4>(o - sj), phps = php.ld<F4>(o - sj);
       const F4 v0 = s.v_2.ld<F4>(o), vt = s.rv_tend.ld<F4>(o), cqv = s.cqv.ld<F4>(o);
       F4 vn;
@@ -454,8 +454,9 @@ __global__ void __launch_bounds__(256) k_advance_uv_x4(ARGS args) {
         const float vpg = v - dts * cqv.v[n] * dpxy + (c1 * mudf_yv[n]);
         vn.v[n] = vpg_on ? vpg : v;
       }
-      if (v_on && k <= k1) st4m(s.v_2, o, vn, i0, ids, i_end);
+      if (v_on) st4m(s.v_2, o, vn, i0, ids, i_end);
     }
+    pc = pcp; hc = hcp; pl = plp; hl = hlp; ps = psp; hs = hsp;
   }
 }
 
