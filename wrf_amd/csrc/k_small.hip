@@ -705,6 +705,7 @@ __global__ void __launch_bounds__(256) k_small_finish_x4(ARGS args) {
   }
 }
 
+
 // ---------------------------------------------------------------------------------------
 // Four-points-per-lane forms of small_step_prep and calc_p_rho step 0 (16-byte accesses;
 // element by element the expressions of k_small_prep and k_p_rho0).
@@ -812,6 +813,7 @@ __global__ void __launch_bounds__(256) k_small_prep_x4(ARGS args) {
   }
 }
 
+
 template <typename ARGS>
 __global__ void __launch_bounds__(256) k_p_rho0_x4(ARGS args) {
   Tile3 T;
@@ -855,58 +857,67 @@ __global__ void __launch_bounds__(256) k_ac_div_x4(ARGS args) {
   const DState &s = dstate(a);
   const int i0 = vec_ia0(d) + kVecW * (T.bx * 64 + (int)threadIdx.x);
   const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
-  const int k = d.kds + (int)T.bz;
+  const int k0 = d.kds + (int)T.bz * kDivK;
   const int ids = d.ids, ide = d.ide, jde = d.jde, kde = d.kde;
-  if (i0 > ide || j > jde || k > kde) return;
-  const Off o2 = X2(d, i0, j), o = X3(d, i0, k, j);
-  const bool mj = j <= jde - 1, kin = k <= kde - 1;
+  if (i0 > ide || j > jde || k0 > kde) return;
+  const int k1 = imin(k0 + kDivK - 1, kde);
+  const Off o2 = X2(d, i0, j), ob = X3(d, i0, 0, j);  // level k at ob + k*ni
+  const int ni = d.ni;
+  const bool mj = j <= jde - 1;
   const int it = a.iteration, ns = a.nsmall;
   const float nsf = (float)ns;
-  const float c1 = s.c1h[X1(d, imin(k, kde - 1))], c2 = s.c2h[X1(d, imin(k, kde - 1))];
-  const F4 u = s.u_2.ld<F4>(o), v = s.v_2.ld<F4>(o), us = s.u_save.ld<F4>(o), vs = s.v_save.ld<F4>(o);
-  const F4 rum0 = s.ru_m.ld<F4>(o), rvm0 = s.rv_m.ld<F4>(o);
-  const F4 muu = s.muu.ld<F4>(o2), msfuy = s.msfuy.ld<F4>(o2), muv = s.muv.ld<F4>(o2), mvi = s.msfvx_inv.ld<F4>(o2);
-  // sumflux of ru_m, rv_m (and ww_m outside the mass points), as sumflux_uv
-  F4 rum, rvm;
-#pragma unroll
-  for (int n = 0; n < 4; n++) {
-    const bool mi = i0 + n <= ide - 1;
-    float ru_ = it == 1 ? 0.f : rum0.v[n];
-    float rv_ = it == 1 ? 0.f : rvm0.v[n];
-    if (mj && kin) ru_ = ru_ + u.v[n];
-    if (mi && kin) rv_ = rv_ + v.v[n];
-    if (it == ns) {
-      if (mj && kin) ru_ = ru_ / nsf + (c1 * muu.v[n] + c2) * us.v[n] / msfuy.v[n];
-      if (mi && kin) rv_ = rv_ / nsf + (c1 * muv.v[n] + c2) * vs.v[n] * mvi.v[n];
-    }
-    rum.v[n] = ru_;
-    rvm.v[n] = rv_;
-  }
-  st4m(s.ru_m, o, rum, i0, ids, ide);
-  st4m(s.rv_m, o, rvm, i0, ids, ide);
-  if (it == 1) {
-#pragma unroll
-    for (int n = 0; n < 4; n++)
-      if (i0 + n >= ids && i0 + n <= ide && !(i0 + n <= ide - 1 && mj)) s.ww_m[o + n] = 0.f;
-  }
-  if (!mj || !kin) return;
-  // dvdxi at the mass points
-  const F4 ur = s.u_2.ld<F4>(o + 4), usr = s.u_save.ld<F4>(o + 4);
-  const F4 muur = s.muu.ld<F4>(o2 + 4), msfuyr = s.msfuy.ld<F4>(o2 + 4);
-  const F4 vn = s.v_2.ld<F4>(o + d.sj), vsn = s.v_save.ld<F4>(o + d.sj);
-  const F4 muvn = s.muv.ld<F4>(o2 + d.ni), mvin = s.msfvx_inv.ld<F4>(o2 + d.ni);
-  const F4 msftx = s.msftx.ld<F4>(o2), msfty = s.msfty.ld<F4>(o2);
   const float rdx = s.rdx, rdy = s.rdy;
-  F4 dv;
+  // the column's 2-D fields, loaded once for its kDivK levels
+  const F4 muu = s.muu.ld<F4>(o2), msfuy = s.msfuy.ld<F4>(o2), muv = s.muv.ld<F4>(o2), mvi = s.msfvx_inv.ld<F4>(o2);
+  const F4 muur = s.muu.ld<F4>(o2 + 4), msfuyr = s.msfuy.ld<F4>(o2 + 4);
+  const F4 muvn = s.muv.ld<F4>(o2 + ni), mvin = s.msfvx_inv.ld<F4>(o2 + ni);
+  const F4 msftx = s.msftx.ld<F4>(o2), msfty = s.msfty.ld<F4>(o2);
+#pragma unroll 1
+  for (int k = k0; k <= k1; k++) {
+    const Off o = ob + k * ni;
+    const bool kin = k <= kde - 1;
+    const float c1 = s.c1h[X1(d, imin(k, kde - 1))], c2 = s.c2h[X1(d, imin(k, kde - 1))];
+    const F4 u = s.u_2.ld<F4>(o), v = s.v_2.ld<F4>(o), us = s.u_save.ld<F4>(o), vs = s.v_save.ld<F4>(o);
+    const F4 rum0 = s.ru_m.ld<F4>(o), rvm0 = s.rv_m.ld<F4>(o);
+    // sumflux of ru_m, rv_m (and ww_m outside the mass points), as sumflux_uv
+    F4 rum, rvm;
 #pragma unroll
-  for (int n = 0; n < 4; n++) {
-    const float u1 = xp1(u, ur, n), us1 = xp1(us, usr, n), muu1 = xp1(muu, muur, n), msfuy1 = xp1(msfuy, msfuyr, n);
-    dv.v[n] = msftx.v[n] * msfty.v[n] *
-              (rdy * ((vn.v[n] + (c1 * muvn.v[n] + c2) * vsn.v[n] * mvin.v[n]) - (v.v[n] + (c1 * muv.v[n] + c2) * vs.v[n] * mvi.v[n])) +
-               rdx * ((u1 + (c1 * muu1 + c2) * us1 / msfuy1) - (u.v[n] + (c1 * muu.v[n] + c2) * us.v[n] / msfuy.v[n])));
+    for (int n = 0; n < 4; n++) {
+      const bool mi = i0 + n <= ide - 1;
+      float ru_ = it == 1 ? 0.f : rum0.v[n];
+      float rv_ = it == 1 ? 0.f : rvm0.v[n];
+      if (mj && kin) ru_ = ru_ + u.v[n];
+      if (mi && kin) rv_ = rv_ + v.v[n];
+      if (it == ns) {
+        if (mj && kin) ru_ = ru_ / nsf + (c1 * muu.v[n] + c2) * us.v[n] / msfuy.v[n];
+        if (mi && kin) rv_ = rv_ / nsf + (c1 * muv.v[n] + c2) * vs.v[n] * mvi.v[n];
+      }
+      rum.v[n] = ru_;
+      rvm.v[n] = rv_;
+    }
+    st4m(s.ru_m, o, rum, i0, ids, ide);
+    st4m(s.rv_m, o, rvm, i0, ids, ide);
+    if (it == 1) {
+#pragma unroll
+      for (int n = 0; n < 4; n++)
+        if (i0 + n >= ids && i0 + n <= ide && !(i0 + n <= ide - 1 && mj)) s.ww_m[o + n] = 0.f;
+    }
+    if (!mj || !kin) continue;
+    // dvdxi at the mass points
+    const F4 ur = s.u_2.ld<F4>(o + 4), usr = s.u_save.ld<F4>(o + 4);
+    const F4 vn = s.v_2.ld<F4>(o + d.sj), vsn = s.v_save.ld<F4>(o + d.sj);
+    F4 dv;
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+      const float u1 = xp1(u, ur, n), us1 = xp1(us, usr, n), muu1 = xp1(muu, muur, n), msfuy1 = xp1(msfuy, msfuyr, n);
+      dv.v[n] = msftx.v[n] * msfty.v[n] *
+                (rdy * ((vn.v[n] + (c1 * muvn.v[n] + c2) * vsn.v[n] * mvin.v[n]) - (v.v[n] + (c1 * muv.v[n] + c2) * vs.v[n] * mvi.v[n])) +
+                 rdx * ((u1 + (c1 * muu1 + c2) * us1 / msfuy1) - (u.v[n] + (c1 * muu.v[n] + c2) * us.v[n] / msfuy.v[n])));
+    }
+    st4m(a.scr_dv, o, dv, i0, ids, ide - 1);
   }
-  st4m(a.scr_dv, o, dv, i0, ids, ide - 1);
 }
+
 
 template <typename ARGS>
 __global__ void __launch_bounds__(256) k_ac_theta_x4(ARGS args) {

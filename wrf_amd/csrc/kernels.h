@@ -12,6 +12,7 @@ constexpr int kVecW = 4;       // i points per lane in the vector kernels (16-by
 constexpr int kVecTileI = 64 * kVecW;  // i points per wave in the vector kernels
 constexpr int kAdvVK = 4;      // levels per thread in k_adv_scalar_x4
 constexpr int kHaloK = 8;      // levels per thread in k_halo
+constexpr int kDivK = 4;       // levels per thread in k_ac_div_x4
 // Every state kernel is a template on its argument type: KArgs for one domain, BatchArgs
 // for an ensemble launch over the tiles of all members (k_halo: HaloArgs / BatchHalo).
 // ZONE = true is the one-point kernel's form over the boundary zone of a domain with an

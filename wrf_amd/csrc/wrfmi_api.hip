@@ -488,7 +488,7 @@ static void ph_advance_mu_t_w(Launcher &L) {
   const wrfmi_state &s = a.s;
   const bool p64 = a.c.precision_mode == 1;
   const int nk = a.d.kde - a.d.kds + 1;
-  if (L.vec) L.launchK(KID_AC_DIV, KR(k_ac_div_x4), g_vec3(a, nk), kColBlk, a);
+  if (L.vec) L.launchK(KID_AC_DIV, KR(k_ac_div_x4), g_vec3(a, (nk + kDivK - 1) / kDivK), kColBlk, a);
   else L.launchK(KID_AC_DIV, KR(k_ac_div), g_pt3(a.d, nk), kColBlk, a);
   if (p64) L.launchK(KID_AC_MU_WW, KRP(k_ac_mu_ww, true), g_col(a.d), kColBlk, a);
   else L.launchK(KID_AC_MU_WW, KRP(k_ac_mu_ww, false), g_col(a.d), kColBlk, a);
