@@ -173,6 +173,10 @@ DEVI Tile3 tile3_of(unsigned b, int tnx, int tny, int tnz) {
   return tile_of_logical(xcd_logical(b, (unsigned)tnx * (unsigned)tny * (unsigned)tnz), tnx, tnz);
 }
 
+// First i of the one-column-per-lane kernels' tiles: ids rounded down to a 32-point
+// (128-byte) boundary of the row, so that a wave's 64 points of one level start on a cache
+// line (rows are padded to a multiple of 32 points); lanes below ids return at once.
+HDI int col_ia(const Dm &d) { return d.ims + ((d.ids - d.ims) & ~31); }
 DEVI int X3i(const Dm &d, int i, int k, int j) { return (i - d.ims) + d.ni * (k - d.kms) + d.sj * (j - d.jms); }
 DEVI int X2i(const Dm &d, int i, int j) { return (i - d.ims) + d.ni * (j - d.jms); }
 DEVI Off X3(const Dm &d, int i, int k, int j) { return Off{4u * (unsigned)X3i(d, i, k, j)}; }

@@ -248,10 +248,10 @@ __global__ void __launch_bounds__(256) k_ww_scan(ARGS args) {
   const KArgs &a = kargs(args, T);
   const Dm &d = a.d;
   const DState &s = dstate(a);
-  const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
+  const int i = col_ia(d) + (int)(T.bx * blockDim.x + threadIdx.x);
   const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
   const int ide = d.ide, jde = d.jde, kde = d.kde;
-  if (i > ide || j > jde) return;
+  if (i < d.ids || i > ide || j > jde) return;
   const bool mi = i <= ide - 1, mj = j <= jde - 1;
   const FPtr<float> mu = s.mu_2, mub = s.mub;
   const CPtr<float> c1h = s.c1h, c2h = s.c2h;

@@ -126,10 +126,10 @@ __global__ void __launch_bounds__(256) k_coef_w(ARGS args) {
   const KArgs &a = kargs(args, T);
   const Dm &d = a.d;
   const DState &s = dstate(a);
-  const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
+  const int i = col_ia(d) + (int)(T.bx * blockDim.x + threadIdx.x);
   const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
   const int ide = d.ide, jde = d.jde, kde = d.kde;
-  if (i > ide || j > jde) return;
+  if (i < d.ids || i > ide || j > jde) return;
   const bool mi = i <= ide - 1, mj = j <= jde - 1, mass = mi && mj;
   const CPtr<float> c1h = s.c1h, c2h = s.c2h, c1f = s.c1f, c2f = s.c2f;
   const Off o2 = X2(d, i, j);
@@ -527,9 +527,9 @@ __global__ void __launch_bounds__(256) k_ac_mu_ww(ARGS args) {
   const KArgs &a = kargs(args, T);
   const Dm &d = a.d;
   const DState &s = dstate(a);
-  const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
+  const int i = col_ia(d) + (int)(T.bx * blockDim.x + threadIdx.x);
   const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
-  if (i > d.ide - 1 || j > d.jde - 1) return;
+  if (i < d.ids || i > d.ide - 1 || j > d.jde - 1) return;
   const int k_end = d.kde - 1;
   const Off o2 = X2(d, i, j);
   const Off ob = X3(d, i, 0, j);  // level k of the column is at ob + k*ni
@@ -997,10 +997,10 @@ __global__ void __launch_bounds__(256) k_ac_wsolve(ARGS args) {
   const KArgs &a = kargs(args, T);
   const Dm &d = a.d;
   const DState &s = dstate(a);
-  const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
+  const int i = col_ia(d) + (int)(T.bx * blockDim.x + threadIdx.x);
   const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
   const int kde = d.kde, k_end = kde - 1;
-  if (i > d.ide - 1 || j > d.jde - 1) return;
+  if (i < d.ids || i > d.ide - 1 || j > d.jde - 1) return;
   const wrfmi_config &c = a.c;
   if (!c.non_hydrostatic) return;
   FPtr<float> w = s.w_2, ph = s.ph_2;

@@ -384,7 +384,7 @@ static int num_sound_steps(const wrfmi_config &c, const wrfmi_state &s) {
   return n < 4 ? 4 : n;
 }
 static inline dim3 g_prep(const Dm &d) { return col_grid(d, d.ids - 1, d.jds - 1, kColBlk); }
-static inline dim3 g_col(const Dm &d) { return col_grid(d, d.ids, d.jds, kColBlk); }
+static inline dim3 g_col(const Dm &d) { return col_grid(d, col_ia(d), d.jds, kColBlk); }
 // (i, j) tiles of kColBlk over i = ids..ide, j = jds..jde, and nk levels in z
 static inline dim3 g_pt3(const Dm &d, int nk) {
   const dim3 g = col_grid(d, d.ids, d.jds, kColBlk);
