@@ -173,10 +173,18 @@ DEVI Tile3 tile3_of(unsigned b, int tnx, int tny, int tnz) {
   return tile_of_logical(xcd_logical(b, (unsigned)tnx * (unsigned)tny * (unsigned)tnz), tnx, tnz);
 }
 
-// First i of the one-column-per-lane kernels' tiles: ids rounded down to a 32-point
-// (128-byte) boundary of the row, so that a wave's 64 points of one level start on a cache
-// line (rows are padded to a multiple of 32 points); lanes below ids return at once.
-HDI int col_ia(const Dm &d) { return d.ims + ((d.ids - d.ims) & ~31); }
+// First i of the one-column-per-lane kernels' tiles (64 columns wide): ids rounded down to a
+// 32-point (128-byte) boundary of the row, so that a wave's 64 points of one level start on
+// a cache line (rows are padded to a multiple of 32 points) and lanes below ids return at
+// once.  A launch over one domain keeps ids instead when the rounding takes one more tile to
+// cover the mass columns ids..ide-1 (widths that are multiples of 64): a tile row holding a
+// few columns only, marching the whole column, was measured to double the launch time of
+// the w solve on small domains.  A batched launch over many domains always rounds (measured
+// 3 % faster on the 64-member ensemble, where such tiles overlap other members' work).
+HDI int col_ia(const Dm &d, bool batched) {
+  const int ia = d.ims + ((d.ids - d.ims) & ~31), n = d.ide - d.ids;
+  return batched || ((n + d.ids - ia + 63) >> 6) == ((n + 63) >> 6) ? ia : d.ids;
+}
 DEVI int X3i(const Dm &d, int i, int k, int j) { return (i - d.ims) + d.ni * (k - d.kms) + d.sj * (j - d.jms); }
 DEVI int X2i(const Dm &d, int i, int j) { return (i - d.ims) + d.ni * (j - d.jms); }
 DEVI Off X3(const Dm &d, int i, int k, int j) { return Off{4u * (unsigned)X3i(d, i, k, j)}; }

@@ -248,7 +248,7 @@ __global__ void __launch_bounds__(256) k_ww_scan(ARGS args) {
   const KArgs &a = kargs(args, T);
   const Dm &d = a.d;
   const DState &s = dstate(a);
-  const int i = col_ia(d) + (int)(T.bx * blockDim.x + threadIdx.x);
+  const int i = col_ia(d, !std::is_same<ARGS, KArgs>::value) + (int)(T.bx * blockDim.x + threadIdx.x);
   const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
   const int ide = d.ide, jde = d.jde, kde = d.kde;
   if (i < d.ids || i > ide || j > jde) return;

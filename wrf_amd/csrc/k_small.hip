@@ -126,7 +126,7 @@ __global__ void __launch_bounds__(256) k_coef_w(ARGS args) {
   const KArgs &a = kargs(args, T);
   const Dm &d = a.d;
   const DState &s = dstate(a);
-  const int i = col_ia(d) + (int)(T.bx * blockDim.x + threadIdx.x);
+  const int i = col_ia(d, !std::is_same<ARGS, KArgs>::value) + (int)(T.bx * blockDim.x + threadIdx.x);
   const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
   const int ide = d.ide, jde = d.jde, kde = d.kde;
   if (i < d.ids || i > ide || j > jde) return;
@@ -527,7 +527,7 @@ __global__ void __launch_bounds__(256) k_ac_mu_ww(ARGS args) {
   const KArgs &a = kargs(args, T);
   const Dm &d = a.d;
   const DState &s = dstate(a);
-  const int i = col_ia(d) + (int)(T.bx * blockDim.x + threadIdx.x);
+  const int i = col_ia(d, !std::is_same<ARGS, KArgs>::value) + (int)(T.bx * blockDim.x + threadIdx.x);
   const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
   if (i < d.ids || i > d.ide - 1 || j > d.jde - 1) return;
   const int k_end = d.kde - 1;
@@ -997,7 +997,7 @@ __global__ void __launch_bounds__(256) k_ac_wsolve(ARGS args) {
   const KArgs &a = kargs(args, T);
   const Dm &d = a.d;
   const DState &s = dstate(a);
-  const int i = col_ia(d) + (int)(T.bx * blockDim.x + threadIdx.x);
+  const int i = col_ia(d, !std::is_same<ARGS, KArgs>::value) + (int)(T.bx * blockDim.x + threadIdx.x);
   const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
   const int kde = d.kde, k_end = kde - 1;
   if (i < d.ids || i > d.ide - 1 || j > d.jde - 1) return;
