@@ -3,7 +3,7 @@
 #include "dev_common.h"
 
 namespace wrfmi {
-constexpr int kAdvUvK = 8;  // levels per thread in k_advance_uv
+constexpr int kAdvUvK = 4;  // levels per thread in k_advance_uv
 constexpr int kAdvK = 8;     // levels per block in k_adv_scalar
 constexpr int kTendK = 8;    // levels per thread in k_tend_u/v/w/t
 constexpr int kAdvTileI = 63;  // i points per wave in k_adv_scalar (lane 63 supplies a face)
