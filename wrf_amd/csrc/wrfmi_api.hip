@@ -384,7 +384,15 @@ static int num_sound_steps(const wrfmi_config &c, const wrfmi_state &s) {
   return n < 4 ? 4 : n;
 }
 static inline dim3 g_prep(const Dm &d) { return col_grid(d, d.ids - 1, d.jds - 1, kColBlk); }
-static inline dim3 g_col(const Launcher &L) { return col_grid(L.a.d, col_ia(L.a.d, L.rec != nullptr), L.a.d.jds, kColBlk); }
+// workgroup of the one-column-per-lane kernels: 64 columns by WRFMI_COL_ROWS rows.  One
+// row per workgroup spreads the few waves of a column launch (2800 at 400x400) evenly over
+// the CUs; at 400x400x80 the w solve takes 2.97 ms per step with 1 row, 3.12 with 2 and
+// 3.05 with 4.
+#ifndef WRFMI_COL_ROWS
+#define WRFMI_COL_ROWS 1
+#endif
+static const dim3 kColumnBlk(64, WRFMI_COL_ROWS, 1);
+static inline dim3 g_col(const Launcher &L) { return col_grid(L.a.d, col_ia(L.a.d, L.rec != nullptr), L.a.d.jds, kColumnBlk); }
 // (i, j) tiles of kColBlk over i = ids..ide, j = jds..jde, and nk levels in z
 static inline dim3 g_pt3(const Dm &d, int nk) {
   const dim3 g = col_grid(d, d.ids, d.jds, kColBlk);
@@ -413,8 +421,8 @@ static void ph_step_prep(Launcher &L) {
   } else {
     L.launchK(KID_STEP_PREP, KR(k_step_prep), dim3(gp.x, gp.y, a.d.kde - a.d.kds + 1), kColBlk, a);
   }
-  if (a.c.precision_mode == 1) L.launchK(KID_WW_SCAN, KRP(k_ww_scan, true), g_col(L), kColBlk, a);
-  else L.launchK(KID_WW_SCAN, KRP(k_ww_scan, false), g_col(L), kColBlk, a);
+  if (a.c.precision_mode == 1) L.launchK(KID_WW_SCAN, KRP(k_ww_scan, true), g_col(L), kColumnBlk, a);
+  else L.launchK(KID_WW_SCAN, KRP(k_ww_scan, false), g_col(L), kColumnBlk, a);
   L.halo_begin();
   L.halo_add(s.ru, 'u', 1); L.halo_add(s.rv, 'v', 1); L.halo_add(s.rw, 'w', 1); L.halo_add(s.ww, 'w', 1);
   L.halo_add(s.php, 'w', 1); L.halo_add(s.alt, 't', 1); L.halo_add(s.p, 'p', 1);
@@ -467,8 +475,8 @@ static void ph_small_prep(Launcher &L) {
   // the per-routine entry point stores calc_coef_w's a, alpha, gamma as the reference does;
   // the whole step leaves them to k_ac_wsolve, which forms them on its way up each column
   const bool p64 = a.c.precision_mode == 1;
-  if (L.whole_step) L.launchK(KID_COEF_W, p64 ? KRC(k_coef_w, double, false) : KRC(k_coef_w, float, false), g_col(L), kColBlk, a);
-  else L.launchK(KID_COEF_W, p64 ? KRC(k_coef_w, double, true) : KRC(k_coef_w, float, true), g_col(L), kColBlk, a);
+  if (L.whole_step) L.launchK(KID_COEF_W, p64 ? KRC(k_coef_w, double, false) : KRC(k_coef_w, float, false), g_col(L), kColumnBlk, a);
+  else L.launchK(KID_COEF_W, p64 ? KRC(k_coef_w, double, true) : KRC(k_coef_w, float, true), g_col(L), kColumnBlk, a);
   L.halo_begin();
   L.halo_add(s.ru_tend, 'u', 1); L.halo_add(s.rv_tend, 'v', 1); L.halo_add(s.ph_2, 'w', 1);
   L.halo_add(s.al, 'p', 1); L.halo_add(s.p, 'p', 1); L.halo_add(s.t_1, 'p', 1); L.halo_add(s.t_save, 't', 1);
@@ -490,12 +498,12 @@ static void ph_advance_mu_t_w(Launcher &L) {
   const int nk = a.d.kde - a.d.kds + 1;
   if (L.vec) L.launchK(KID_AC_DIV, KR(k_ac_div_x4), g_vec3(a, (nk + kDivK - 1) / kDivK), kColBlk, a);
   else L.launchK(KID_AC_DIV, KR(k_ac_div), g_pt3(a.d, nk), kColBlk, a);
-  if (p64) L.launchK(KID_AC_MU_WW, KRP(k_ac_mu_ww, true), g_col(L), kColBlk, a);
-  else L.launchK(KID_AC_MU_WW, KRP(k_ac_mu_ww, false), g_col(L), kColBlk, a);
+  if (p64) L.launchK(KID_AC_MU_WW, KRP(k_ac_mu_ww, true), g_col(L), kColumnBlk, a);
+  else L.launchK(KID_AC_MU_WW, KRP(k_ac_mu_ww, false), g_col(L), kColumnBlk, a);
   if (L.vec) L.launchK(KID_AC_THETA, KR(k_ac_theta_x4), g_vec3(a, nk), kColBlk, a);
   else L.launchK(KID_AC_THETA, KR(k_ac_theta), g_pt3(a.d, nk), kColBlk, a);
-  if (p64) L.launchK(KID_AC_WSOLVE, KRP(k_ac_wsolve, double), g_col(L), kColBlk, a);
-  else L.launchK(KID_AC_WSOLVE, KRP(k_ac_wsolve, float), g_col(L), kColBlk, a);
+  if (p64) L.launchK(KID_AC_WSOLVE, KRP(k_ac_wsolve, double), g_col(L), kColumnBlk, a);
+  else L.launchK(KID_AC_WSOLVE, KRP(k_ac_wsolve, float), g_col(L), kColumnBlk, a);
   L.halo_begin();
   L.halo_add(s.ph_2, 'w', 1); L.halo_add(s.al, 'p', 1); L.halo_add(s.p, 'p', 1);
   L.halo_add(s.muts, 't', 0); L.halo_add(s.mu_2, 't', 0); L.halo_add(s.mudf, 't', 0);
