@@ -258,6 +258,15 @@ static inline dim3 col_grid(const Dm &d, int i0, int j0, dim3 blk) {
   return dim3((ni + blk.x - 1) / blk.x, (nj + blk.y - 1) / blk.y, 1);
 }
 static const dim3 kColBlk(64, 4, 1);
+// workgroup of the four-points-per-lane kernels: one 256-point row per wave, WRFMI_VEC_ROWS
+// rows.  With one row, the tiles of neighbouring rows follow each other more closely in the
+// XCD-contiguous tile order, and a launch's waves spread evenly over the CUs: at 400x400x80
+// the step takes 17.45 ms with 1 row, 17.46 with 2 and 17.67 with 4 (8 and 16 were slower
+// still).
+#ifndef WRFMI_VEC_ROWS
+#define WRFMI_VEC_ROWS 1
+#endif
+static const dim3 kVecBlk(64, WRFMI_VEC_ROWS, 1);
 
 // ------------------------------------------------------------------ kernel timing (diagnostic)
 enum KernelId {
@@ -407,7 +416,7 @@ static inline dim3 g_vec(const KArgs &a, int ibase, int jlow, int rows, int nk) 
   return dim3((d.ide - ibase + w) / w, (d.jde - d.jds + 1 + jlow + rows - 1) / rows, nk);
 }
 static inline dim3 g_vec3(const KArgs &a, int nk) {
-  return g_vec(a, a.d.ims + 4 * ((a.d.ids - a.d.ims) >> 2), 0, 4, nk);
+  return g_vec(a, a.d.ims + 4 * ((a.d.ids - a.d.ims) >> 2), 0, WRFMI_VEC_ROWS, nk);
 }
 
 static void ph_step_prep(Launcher &L) {
@@ -417,7 +426,7 @@ static void ph_step_prep(Launcher &L) {
   if (L.vec) {
     // one extra group of four below ids (mut at ids-1), rows from jds-1
     const int ia0 = a.d.ims + 4 * ((a.d.ids - a.d.ims) >> 2) - 4;
-    L.launchK(KID_STEP_PREP, KR(k_step_prep_x4), g_vec(a, ia0, 1, 4, a.d.kde - a.d.kds + 1), kColBlk, a);
+    L.launchK(KID_STEP_PREP, KR(k_step_prep_x4), g_vec(a, ia0, 1, WRFMI_VEC_ROWS, a.d.kde - a.d.kds + 1), kVecBlk, a);
   } else {
     L.launchK(KID_STEP_PREP, KR(k_step_prep), dim3(gp.x, gp.y, a.d.kde - a.d.kds + 1), kColBlk, a);
   }
@@ -443,10 +452,10 @@ static void ph_tendency(Launcher &L) {
   if (L.vec && (per || L.zone_ok)) {
     // four-point kernels; on a domain with an open direction they leave the boundary zone
     // to the one-point kernels' zone form
-    L.launchK(KID_TEND_U, KR(k_tend_u_x4), g_vec3(a, nk), kColBlk, a);
-    L.launchK(KID_TEND_V, KR(k_tend_v_x4), g_vec3(a, nk), kColBlk, a);
-    L.launchK(KID_TEND_W, KR(k_tend_w_x4), g_vec3(a, nk), kColBlk, a);
-    L.launchK(KID_TEND_T, KR(k_tend_t_x4), g_vec3(a, nk), kColBlk, a);
+    L.launchK(KID_TEND_U, KR(k_tend_u_x4), g_vec3(a, nk), kVecBlk, a);
+    L.launchK(KID_TEND_V, KR(k_tend_v_x4), g_vec3(a, nk), kVecBlk, a);
+    L.launchK(KID_TEND_W, KR(k_tend_w_x4), g_vec3(a, nk), kVecBlk, a);
+    L.launchK(KID_TEND_T, KR(k_tend_t_x4), g_vec3(a, nk), kVecBlk, a);
     if (!per) {
       const dim3 gz = g_zone(a, nk);  // one level per thread
       L.launchK(KID_TEND_U, KRZ(k_tend_u), gz, kColBlk, a);
@@ -466,8 +475,8 @@ static void ph_small_prep(Launcher &L) {
   const wrfmi_state &s = a.s;
   const int nk = a.d.kde - a.d.kds + 1;
   if (L.vec) {
-    L.launchK(KID_SMALL_PREP, KR(k_small_prep_x4), g_vec3(a, nk), kColBlk, a);
-    L.launchK(KID_P_RHO0, KR(k_p_rho0_x4), g_vec3(a, nk - 1), kColBlk, a);
+    L.launchK(KID_SMALL_PREP, KR(k_small_prep_x4), g_vec3(a, nk), kVecBlk, a);
+    L.launchK(KID_P_RHO0, KR(k_p_rho0_x4), g_vec3(a, nk - 1), kVecBlk, a);
   } else {
     L.launchK(KID_SMALL_PREP, KR(k_small_prep), g_pt3(a.d, nk), kColBlk, a);
     L.launchK(KID_P_RHO0, KR(k_p_rho0), g_pt3(a.d, nk - 1), kColBlk, a);
@@ -487,7 +496,7 @@ static void ph_advance_uv(Launcher &L) {
   const bool per = L.a.c.periodic_x && L.a.c.periodic_y;
   const int nkb = (L.a.d.kde - L.a.d.kds + kAdvUvK - 1) / kAdvUvK;
   if (L.vec)
-    L.launchK(KID_ADVANCE_UV, per ? KRP(k_advance_uv_x4, true) : KRP(k_advance_uv_x4, false), g_vec3(L.a, nkb), kColBlk, L.a);
+    L.launchK(KID_ADVANCE_UV, per ? KRP(k_advance_uv_x4, true) : KRP(k_advance_uv_x4, false), g_vec3(L.a, nkb), kVecBlk, L.a);
   else
     L.launchK(KID_ADVANCE_UV, per ? KRP(k_advance_uv, true) : KRP(k_advance_uv, false), g_pt3(L.a.d, nkb), kColBlk, L.a);
 }
@@ -496,11 +505,11 @@ static void ph_advance_mu_t_w(Launcher &L) {
   const wrfmi_state &s = a.s;
   const bool p64 = a.c.precision_mode == 1;
   const int nk = a.d.kde - a.d.kds + 1;
-  if (L.vec) L.launchK(KID_AC_DIV, KR(k_ac_div_x4), g_vec3(a, (nk + kDivK - 1) / kDivK), kColBlk, a);
+  if (L.vec) L.launchK(KID_AC_DIV, KR(k_ac_div_x4), g_vec3(a, (nk + kDivK - 1) / kDivK), kVecBlk, a);
   else L.launchK(KID_AC_DIV, KR(k_ac_div), g_pt3(a.d, nk), kColBlk, a);
   if (p64) L.launchK(KID_AC_MU_WW, KRP(k_ac_mu_ww, true), g_col(L), kColumnBlk, a);
   else L.launchK(KID_AC_MU_WW, KRP(k_ac_mu_ww, false), g_col(L), kColumnBlk, a);
-  if (L.vec) L.launchK(KID_AC_THETA, KR(k_ac_theta_x4), g_vec3(a, nk), kColBlk, a);
+  if (L.vec) L.launchK(KID_AC_THETA, KR(k_ac_theta_x4), g_vec3(a, nk), kVecBlk, a);
   else L.launchK(KID_AC_THETA, KR(k_ac_theta), g_pt3(a.d, nk), kColBlk, a);
   if (p64) L.launchK(KID_AC_WSOLVE, KRP(k_ac_wsolve, double), g_col(L), kColumnBlk, a);
   else L.launchK(KID_AC_WSOLVE, KRP(k_ac_wsolve, float), g_col(L), kColumnBlk, a);
@@ -512,7 +521,7 @@ static void ph_advance_mu_t_w(Launcher &L) {
 static void ph_small_finish(Launcher &L) {
   const KArgs &a = L.a;
   const wrfmi_state &s = a.s;
-  if (L.vec) L.launchK(KID_SMALL_FINISH, KR(k_small_finish_x4), g_vec3(a, a.d.kde - a.d.kds + 1), kColBlk, a);
+  if (L.vec) L.launchK(KID_SMALL_FINISH, KR(k_small_finish_x4), g_vec3(a, a.d.kde - a.d.kds + 1), kVecBlk, a);
   else L.launchK(KID_SMALL_FINISH, KR(k_small_finish), g_pt3(a.d, a.d.kde - a.d.kds + 1), kColBlk, a);
   if (a.rk_step == 3) {
     L.halo_begin();
@@ -534,18 +543,18 @@ static void ph_scalar(Launcher &L) {
   } else {
     L.launchK(KID_ADV_SCALAR, per ? KRP(k_adv_scalar, true) : KRP(k_adv_scalar, false), ga, dim3(64, kAdvRows, 1), L.a);
   }
-  if (L.vec) L.launchK(KID_UPDATE_SCALAR, KR(k_update_scalar_x4), g_vec3(L.a, L.a.d.kde - L.a.d.kds), kColBlk, L.a);
+  if (L.vec) L.launchK(KID_UPDATE_SCALAR, KR(k_update_scalar_x4), g_vec3(L.a, L.a.d.kde - L.a.d.kds), kVecBlk, L.a);
   else L.launchK(KID_UPDATE_SCALAR, KR(k_update_scalar), g_pt3(L.a.d, L.a.d.kde - L.a.d.kds), kColBlk, L.a);
 }
 // every species' rk_scalar_tend and rk_update_scalar (species are independent: the order
 // over species does not change any value)
 static void ph_scalar_all(Launcher &L) {
   const Dm &d = L.a.d;
-  L.launchK(KID_ADV_SCALAR, KR(k_adv_scalar_all_x4), g_vec3(L.a, d.kde - d.kds + 1), kColBlk, L.a);
-  L.launchK(KID_UPDATE_SCALAR, KR(k_update_scalar_all_x4), g_vec3(L.a, d.kde - d.kds), kColBlk, L.a);
+  L.launchK(KID_ADV_SCALAR, KR(k_adv_scalar_all_x4), g_vec3(L.a, d.kde - d.kds + 1), kVecBlk, L.a);
+  L.launchK(KID_UPDATE_SCALAR, KR(k_update_scalar_all_x4), g_vec3(L.a, d.kde - d.kds), kVecBlk, L.a);
 }
 static void ph_p_rho_phi(Launcher &L) {
-  if (L.vec) L.launchK(KID_P_RHO_PHI, KR(k_p_rho_phi_x4), g_vec3(L.a, L.a.d.kde - L.a.d.kds), kColBlk, L.a);
+  if (L.vec) L.launchK(KID_P_RHO_PHI, KR(k_p_rho_phi_x4), g_vec3(L.a, L.a.d.kde - L.a.d.kds), kVecBlk, L.a);
   else L.launchK(KID_P_RHO_PHI, KR(k_p_rho_phi), g_pt3(L.a.d, L.a.d.kde - L.a.d.kds), kColBlk, L.a);
 }
 static void ph_bc_dry_2(Launcher &L, int kind) {
