@@ -247,7 +247,7 @@ template <typename T> DEVI T imax(T a, T b) { return a > b ? a : b; }
 
 // Vector kernels: a lane holds the four points i0..i0+3, where i0 - ims is a multiple of four
 // (so that the lane's memory is a 16-byte group); vec_ia0 is the first such i at or below ids.
-DEVI int vec_ia0(const Dm &d) { return d.ims + 4 * ((d.ids - d.ims) >> 2); }
+HDI int vec_ia0(const Dm &d) { return d.ims + 4 * ((d.ids - d.ims) >> 2); }
 // store of the lane's points that lie in [lo, hi]
 DEVI void st4m(const FPtr<float> &p, Off o, const F4 &v, int i0, int lo, int hi) {
   if (i0 >= lo && i0 + 3 <= hi) {

@@ -266,6 +266,15 @@ static const dim3 kColBlk(64, 4, 1);
 #ifndef WRFMI_VEC_ROWS
 #define WRFMI_VEC_ROWS 1
 #endif
+// k_adv_scalar_all_x4 and k_tend_t_x4 keep four rows: their 5th-order y stencils read the
+// j-3..j+3 rows of several fields, which the four waves of a workgroup share in L2 (with
+// one row: 1.41 and 1.11 ms per step at 400x400x80, with four: 1.33 and 1.08)
+#ifndef WRFMI_ADVALL_ROWS
+#define WRFMI_ADVALL_ROWS 4
+#endif
+#ifndef WRFMI_TENDT_ROWS
+#define WRFMI_TENDT_ROWS 4
+#endif
 static const dim3 kVecBlk(64, WRFMI_VEC_ROWS, 1);
 
 // ------------------------------------------------------------------ kernel timing (diagnostic)
@@ -455,7 +464,7 @@ static void ph_tendency(Launcher &L) {
     L.launchK(KID_TEND_U, KR(k_tend_u_x4), g_vec3(a, nk), kVecBlk, a);
     L.launchK(KID_TEND_V, KR(k_tend_v_x4), g_vec3(a, nk), kVecBlk, a);
     L.launchK(KID_TEND_W, KR(k_tend_w_x4), g_vec3(a, nk), kVecBlk, a);
-    L.launchK(KID_TEND_T, KR(k_tend_t_x4), g_vec3(a, nk), kVecBlk, a);
+    L.launchK(KID_TEND_T, KR(k_tend_t_x4), g_vec(a, vec_ia0(a.d), 0, WRFMI_TENDT_ROWS, nk), dim3(64, WRFMI_TENDT_ROWS, 1), a);
     if (!per) {
       const dim3 gz = g_zone(a, nk);  // one level per thread
       L.launchK(KID_TEND_U, KRZ(k_tend_u), gz, kColBlk, a);
@@ -550,7 +559,7 @@ static void ph_scalar(Launcher &L) {
 // over species does not change any value)
 static void ph_scalar_all(Launcher &L) {
   const Dm &d = L.a.d;
-  L.launchK(KID_ADV_SCALAR, KR(k_adv_scalar_all_x4), g_vec3(L.a, d.kde - d.kds + 1), kVecBlk, L.a);
+  L.launchK(KID_ADV_SCALAR, KR(k_adv_scalar_all_x4), g_vec(L.a, vec_ia0(d), 0, WRFMI_ADVALL_ROWS, d.kde - d.kds + 1), dim3(64, WRFMI_ADVALL_ROWS, 1), L.a);
   L.launchK(KID_UPDATE_SCALAR, KR(k_update_scalar_all_x4), g_vec3(L.a, d.kde - d.kds), kVecBlk, L.a);
 }
 static void ph_p_rho_phi(Launcher &L) {
