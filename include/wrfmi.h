@@ -134,9 +134,19 @@ int wrfmi_solve_em_step(wrfmi_state *st, const wrfmi_config *cfg,
                         const wrfmi_dims *d, void *stream, int sync);
 
 /* Same step, replayed from a hipGraph captured on the first call for this
- * (state, config, dims) triple.  Later calls must pass the same pointers. */
+ * (state, config, dims) triple, compared through the device pointers the state
+ * translates to (a registered array whose mirror was replaced gets a new graph). */
 int wrfmi_solve_em_step_graph(wrfmi_state *st, const wrfmi_config *cfg,
                               const wrfmi_dims *d, void *stream, int sync);
+
+/* Work arrays.  The library keeps a few grid-sized work arrays per domain (a domain
+ * is known by the device address of its u_2 array and its field size), allocated on
+ * the domain's first call: steps of different domains may run concurrently on
+ * different streams.  wrfmi_domain_release frees a domain's work arrays and cached
+ * graphs (waiting for the device first); a later call allocates them again.  No
+ * counterpart in the reference, whose work arrays are solve_em's stack and i1 locals
+ * (Registry/Registry.EM_COMMON:186-348). */
+int wrfmi_domain_release(const wrfmi_state *st);
 
 /* ---- batched ensembles ---------------------------------------------------
  * No counterpart in the reference, which advances one domain per solve_em call
@@ -231,7 +241,8 @@ int wrfmi_timing_collect(void);
 int wrfmi_timing_get(int id, long long *launches, double *total_ms);
 
 /* Non-finite scan of the prognostic state (u_2, v_2, w_2, t_2, ph_2, mu_2);
- * returns WRFMI_E_NONFINITE and the count in *n_bad when any is found. */
+ * returns WRFMI_E_NONFINITE and the count in *n_bad when any is found.  Waits for all
+ * work queued on the device before it scans. */
 int wrfmi_check_finite(const wrfmi_state *st, const wrfmi_dims *d, long long *n_bad);
 
 #ifdef __cplusplus

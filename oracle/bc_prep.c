@@ -200,21 +200,24 @@ void o_rk_phys_bc_dry_1(ostate *s, const cfg_t *c, const dims_t *d) {
 }
 
 /* module_big_step_utilities_em.F:26-180 */
+/* calc_mu_uv (module_big_step_utilities_em.F:26-180): as calc_mu_uv_1 with mub added, in
+ * the reference's four tile cases (they differ only at an edge tile's first or last point) */
 static void calc_mu_uv(const cfg_t *c, const wreal *mu, const wreal *mub, wreal *muu, wreal *muv,
                        const dims_t *d) {
   const int its = d->its, ite = d->ite, jts = d->jts, jte = d->jte;
   const int ids = d->ids, ide = d->ide, jds = d->jds, jde = d->jde;
   int itf = ite, jtf = IMIN(jte, jde - 1);
 #define MUUF(i, im, j) (R(0.5) * (((A2(mu, i, j) + A2(mu, im, j)) + A2(mub, i, j)) + A2(mub, im, j)))
-  /* single tile: its == ids and ite == ide */
-  if (its != ids || ite != ide) { /* general tile form not needed by the oracle */ }
+  const int ilo = its == ids ? its + 1 : its, ihi = ite == ide ? itf - 1 : itf;
   for (int j = jts; j <= jtf; j++)
-    for (int i = its + 1; i <= itf - 1; i++) A2(muu, i, j) = MUUF(i, i - 1, j);
-  {
+    for (int i = ilo; i <= ihi; i++) A2(muu, i, j) = MUUF(i, i - 1, j);
+  if (its == ids) {
     int i = its, im = its;
     if (c->periodic_x) im = its - 1;
     for (int j = jts; j <= jtf; j++) A2(muu, i, j) = MUUF(i, im, j);
-    i = ite; im = ite - 1;
+  }
+  if (ite == ide) {
+    int i = ite, im = ite - 1;
     if (c->periodic_x) im = ite;
     /* MUU(i,j) = 0.5*(MU(i-1,j)+MU(im,j)+MUB(i-1,j)+MUB(im,j)) */
     for (int j = jts; j <= jtf; j++)
@@ -222,15 +225,18 @@ static void calc_mu_uv(const cfg_t *c, const wreal *mu, const wreal *mub, wreal 
   }
 #undef MUUF
   itf = IMIN(ite, ide - 1); jtf = jte;
-  for (int j = jts + 1; j <= jtf - 1; j++)
+  const int jlo = jts == jds ? jts + 1 : jts, jhi = jte == jde ? jtf - 1 : jtf;
+  for (int j = jlo; j <= jhi; j++)
     for (int i = its; i <= itf; i++)
       A2(muv, i, j) = R(0.5) * (((A2(mu, i, j) + A2(mu, i, j - 1)) + A2(mub, i, j)) + A2(mub, i, j - 1));
-  {
+  if (jts == jds) {
     int j = jts, jm = jts;
     if (c->periodic_y) jm = jts - 1;
     for (int i = its; i <= itf; i++)
       A2(muv, i, j) = R(0.5) * (((A2(mu, i, j) + A2(mu, i, jm)) + A2(mub, i, j)) + A2(mub, i, jm));
-    j = jte; jm = jte - 1;
+  }
+  if (jte == jde) {
+    int j = jte, jm = jte - 1;
     if (c->periodic_y) jm = jte;
     for (int i = its; i <= itf; i++)
       A2(muv, i, j) = R(0.5) * (((A2(mu, i, j - 1) + A2(mu, i, jm)) + A2(mub, i, j - 1)) + A2(mub, i, jm));

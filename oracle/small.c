@@ -553,27 +553,36 @@ void o_sumflux(ostate *s, const dims_t *d, int iteration, int nsmall) {
   }
 }
 
-/* module_big_step_utilities_em.F:184-321 with mu = muts, muu = muus, muv = muvs (solve_em.F:1732) */
+/* module_big_step_utilities_em.F:184-321 with mu = muts, muu = muus, muv = muvs (solve_em.F:1732).
+ * The reference's four cases (tile at neither, the first, the last or both domain edges)
+ * differ only at i = its when its == ids and at i = ite when ite == ide (likewise in j);
+ * every other point takes the centred average. */
 void o_calc_mu_uv_1(ostate *s, const cfg_t *c, const dims_t *d) {
-  const int ide = d->ide, jde = d->jde;
+  const int ids = d->ids, ide = d->ide, jds = d->jds, jde = d->jde;
   const int its = d->its, ite = d->ite, jts = d->jts, jte = d->jte;
   wreal *mu = s->muts, *muu = s->muus, *muv = s->muvs;
   int itf = ite, jtf = IMIN(jte, jde - 1);
+  const int ilo = its == ids ? its + 1 : its, ihi = ite == ide ? itf - 1 : itf;
   for (int j = jts; j <= jtf; j++)
-    for (int i = its + 1; i <= itf - 1; i++) A2(muu, i, j) = R(0.5) * (A2(mu, i, j) + A2(mu, i - 1, j));
-  {
+    for (int i = ilo; i <= ihi; i++) A2(muu, i, j) = R(0.5) * (A2(mu, i, j) + A2(mu, i - 1, j));
+  if (its == ids) {
     int i = its, im = c->periodic_x ? its - 1 : its;
     for (int j = jts; j <= jtf; j++) A2(muu, i, j) = R(0.5) * (A2(mu, i, j) + A2(mu, im, j));
-    i = ite; im = c->periodic_x ? ite : ite - 1;
+  }
+  if (ite == ide) {
+    int i = ite, im = c->periodic_x ? ite : ite - 1;
     for (int j = jts; j <= jtf; j++) A2(muu, i, j) = R(0.5) * (A2(mu, i - 1, j) + A2(mu, im, j));
   }
   itf = IMIN(ite, ide - 1); jtf = jte;
-  for (int j = jts + 1; j <= jtf - 1; j++)
+  const int jlo = jts == jds ? jts + 1 : jts, jhi = jte == jde ? jtf - 1 : jtf;
+  for (int j = jlo; j <= jhi; j++)
     for (int i = its; i <= itf; i++) A2(muv, i, j) = R(0.5) * (A2(mu, i, j) + A2(mu, i, j - 1));
-  {
+  if (jts == jds) {
     int j = jts, jm = c->periodic_y ? jts - 1 : jts;
     for (int i = its; i <= itf; i++) A2(muv, i, j) = R(0.5) * (A2(mu, i, j) + A2(mu, i, jm));
-    j = jte; jm = c->periodic_y ? jte : jte - 1;
+  }
+  if (jte == jde) {
+    int j = jte, jm = c->periodic_y ? jte : jte - 1;
     for (int i = its; i <= itf; i++) A2(muv, i, j) = R(0.5) * (A2(mu, i, j - 1) + A2(mu, i, jm));
   }
 }

@@ -20,7 +20,53 @@
  */
 #include <stdlib.h>
 #include <math.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include "oracle.h"
+
+/* Tiling as in solve_em: every compute routine is called once per tile inside an OpenMP
+ * loop over the tiles (e.g. solve_em.F:876-911 for rk_tendency), with the tile's
+ * its..ite / jts..jte in the index sextets.  The tiles split the domain's j range into
+ * num_tiles rows of tiles (the 1-D "Y" strategy of frame/module_tiles.F:334-347, with
+ * num_tiles = the OpenMP thread count, module_tiles.F:271-278); the last tile ends at
+ * jde, the staggered end.  Results do not depend on the tiling (each tile writes only its
+ * own points from inputs of earlier phases).  The halo fills stay whole-domain calls. */
+static int g_num_tiles = 0;  /* 0: one tile per OpenMP thread */
+int oracle_set_num_tiles(int n) { g_num_tiles = n < 0 ? 0 : n; return 0; }
+int oracle_num_tiles(const dims_t *d) {
+  int n = g_num_tiles;
+#ifdef _OPENMP
+  if (n == 0) n = omp_get_max_threads();
+#else
+  if (n == 0) n = 1;
+#endif
+  const int rows = d->jde - d->jds + 1;
+  if (n > rows / 2) n = rows / 2 > 0 ? rows / 2 : 1;
+  return n;
+}
+static dims_t tile_dims(const dims_t *d, int t, int n) {
+  dims_t r = *d;
+  const int rows = d->jde - d->jds + 1, base = rows / n, extra = rows % n;
+  r.jts = d->jds + t * base + (t < extra ? t : extra);
+  r.jte = r.jts + base + (t < extra ? 1 : 0) - 1;
+  return r;
+}
+/* run CALL once per tile with `d` bound to the tile's dims; status codes are OR-ed */
+#define TILED(CALL)                                                     \
+  do {                                                                  \
+    const dims_t *dfull_ = d;                                           \
+    const int nt_ = oracle_num_tiles(dfull_);                           \
+    int rct_ = 0;                                                       \
+    _Pragma("omp parallel for schedule(static) reduction(|:rct_)")      \
+    for (int t_ = 0; t_ < nt_; t_++) {                                  \
+      const dims_t dt_ = tile_dims(dfull_, t_, nt_);                    \
+      const dims_t *d = &dt_;                                           \
+      rct_ |= (CALL);                                                   \
+    }                                                                   \
+    if (rct_) return rct_;                                              \
+  } while (0)
+#define TILEDV(CALL) TILED(((CALL), 0))
 
 int o_num_sound_steps(const cfg_t *c, const ostate *s) {
   if (c->time_step_sound != 0) return c->time_step_sound;
@@ -46,17 +92,17 @@ int o_solve_em_step(ostate *s, const cfg_t *c, const dims_t *d) {
     else if (rk_step == 2) { dt_rk = R(0.5) * s->dt; dts_rk = dts; nsmall = nsound / 2; }
     else { dt_rk = s->dt; dts_rk = dts; nsmall = nsound; }
 
-    o_rk_step_prep(s, c, d, rk_step);
+    TILEDV(o_rk_step_prep(s, c, d, rk_step));
     o_rk_phys_bc_dry_1(s, c, d);
     o_set_physical_bc3d(s->al, 'p', c, d);
     o_set_physical_bc3d(s->ph_2, 'w', c, d);
 
-    if ((rc = o_rk_tendency(s, c, d, rk_step))) return rc;
-    o_rk_addtend_dry(s, d, rk_step);
+    TILED(o_rk_tendency(s, c, d, rk_step));
+    TILEDV(o_rk_addtend_dry(s, d, rk_step));
 
-    o_small_step_prep(s, d, rk_step);
-    o_calc_p_rho(s, c, d, 0);
-    if (c->non_hydrostatic) o_calc_coef_w(s, c, d, dts_rk);
+    TILEDV(o_small_step_prep(s, d, rk_step));
+    TILEDV(o_calc_p_rho(s, c, d, 0));
+    if (c->non_hydrostatic) TILEDV(o_calc_coef_w(s, c, d, dts_rk));
 
     o_set_physical_bc3d(s->ru_tend, 'u', c, d);
     o_set_physical_bc3d(s->rv_tend, 'v', c, d);
@@ -70,11 +116,11 @@ int o_solve_em_step(ostate *s, const cfg_t *c, const dims_t *d) {
     o_set_physical_bc2d(s->mudf, 't', c, d);
 
     for (int it = 1; it <= nsmall; it++) {
-      o_advance_uv(s, c, d, dts_rk);
-      o_advance_mu_t(s, c, d, dts_rk);
-      if (c->non_hydrostatic) o_advance_w(s, c, d, dts_rk);
-      o_sumflux(s, d, it, nsmall);
-      o_calc_p_rho(s, c, d, it);
+      TILEDV(o_advance_uv(s, c, d, dts_rk));
+      TILEDV(o_advance_mu_t(s, c, d, dts_rk));
+      if (c->non_hydrostatic) TILEDV(o_advance_w(s, c, d, dts_rk));
+      TILEDV(o_sumflux(s, d, it, nsmall));
+      TILEDV(o_calc_p_rho(s, c, d, it));
       o_set_physical_bc3d(s->ph_2, 'w', c, d);
       o_set_physical_bc3d(s->al, 'p', c, d);
       o_set_physical_bc3d(s->p, 'p', c, d);
@@ -83,8 +129,8 @@ int o_solve_em_step(ostate *s, const cfg_t *c, const dims_t *d) {
       o_set_physical_bc2d(s->mudf, 't', c, d);
     }
 
-    o_calc_mu_uv_1(s, c, d);
-    o_small_step_finish(s, d, nsmall, dts_rk, rk_step, rk_order);
+    TILEDV(o_calc_mu_uv_1(s, c, d));
+    TILEDV(o_small_step_finish(s, d, nsmall, dts_rk, rk_step, rk_order));
     if (rk_step == rk_order) {
       o_set_physical_bc3d(s->ru_m, 'u', c, d);
       o_set_physical_bc3d(s->rv_m, 'v', c, d);
@@ -94,11 +140,11 @@ int o_solve_em_step(ostate *s, const cfg_t *c, const dims_t *d) {
     }
 
     for (int m = 0; m < s->n_species; m++) {
-      if ((rc = o_rk_scalar_tend(s, c, d, m))) return rc;
-      o_rk_update_scalar(s, d, m, rk_step, dt_rk);
+      TILED(o_rk_scalar_tend(s, c, d, m));
+      TILEDV(o_rk_update_scalar(s, d, m, rk_step, dt_rk));
     }
 
-    o_calc_p_rho_phi(s, c, d);
+    TILEDV(o_calc_p_rho_phi(s, c, d));
 
     if (rk_step < rk_order) {
       o_set_physical_bc3d(s->u_2, 'u', c, d);
@@ -111,7 +157,7 @@ int o_solve_em_step(ostate *s, const cfg_t *c, const dims_t *d) {
     }
   }
   /* solve_em.F:4180-4207 recomputes calc_p_rho_phi from unchanged inputs (same values) */
-  o_calc_p_rho_phi(s, c, d);
+  TILEDV(o_calc_p_rho_phi(s, c, d));
   o_set_physical_bc3d(s->u_1, 'u', c, d);
   o_set_physical_bc3d(s->u_2, 'u', c, d);
   o_set_physical_bc3d(s->v_1, 'v', c, d);
@@ -126,7 +172,7 @@ int o_solve_em_step(ostate *s, const cfg_t *c, const dims_t *d) {
   o_set_physical_bc2d(s->mu_2, 't', c, d);
   bc_species(s, c, d);
   /* solve_em.F:4734-4750: the surface w is reset from the final u, v (interior only) */
-  o_set_w_surface(s->w_2, s, c, d);
+  TILEDV(o_set_w_surface(s->w_2, s, c, d));
   return 0;
 }
 
@@ -144,22 +190,21 @@ int oracle_solve_em_steps(ostate *s, const cfg_t *c, const dims_t *d, int nsteps
 /* Per-routine groups, in the order and with the halo fills of o_solve_em_step, for the
  * phase-by-phase parity tests of the GPU entry points of the same names. */
 int oracle_rk_step_prep(ostate *s, const cfg_t *c, const dims_t *d, int rk_step) {
-  o_rk_step_prep(s, c, d, rk_step);
+  TILEDV(o_rk_step_prep(s, c, d, rk_step));
   o_rk_phys_bc_dry_1(s, c, d);
   o_set_physical_bc3d(s->al, 'p', c, d);
   o_set_physical_bc3d(s->ph_2, 'w', c, d);
   return 0;
 }
 int oracle_rk_tendency(ostate *s, const cfg_t *c, const dims_t *d, int rk_step) {
-  int rc = o_rk_tendency(s, c, d, rk_step);
-  if (rc) return rc;
-  o_rk_addtend_dry(s, d, rk_step);
+  TILED(o_rk_tendency(s, c, d, rk_step));
+  TILEDV(o_rk_addtend_dry(s, d, rk_step));
   return 0;
 }
 int oracle_small_step_prep(ostate *s, const cfg_t *c, const dims_t *d, int rk_step, double dts) {
-  o_small_step_prep(s, d, rk_step);
-  o_calc_p_rho(s, c, d, 0);
-  if (c->non_hydrostatic) o_calc_coef_w(s, c, d, (wreal)dts);
+  TILEDV(o_small_step_prep(s, d, rk_step));
+  TILEDV(o_calc_p_rho(s, c, d, 0));
+  if (c->non_hydrostatic) TILEDV(o_calc_coef_w(s, c, d, (wreal)dts));
   o_set_physical_bc3d(s->ru_tend, 'u', c, d);
   o_set_physical_bc3d(s->rv_tend, 'v', c, d);
   o_set_physical_bc3d(s->ph_2, 'w', c, d);
@@ -173,14 +218,14 @@ int oracle_small_step_prep(ostate *s, const cfg_t *c, const dims_t *d, int rk_st
   return 0;
 }
 int oracle_advance_uv(ostate *s, const cfg_t *c, const dims_t *d, double dts) {
-  o_advance_uv(s, c, d, (wreal)dts);
+  TILEDV(o_advance_uv(s, c, d, (wreal)dts));
   return 0;
 }
 int oracle_advance_mu_t_w(ostate *s, const cfg_t *c, const dims_t *d, double dts, int iteration, int n_small) {
-  o_advance_mu_t(s, c, d, (wreal)dts);
-  if (c->non_hydrostatic) o_advance_w(s, c, d, (wreal)dts);
-  o_sumflux(s, d, iteration, n_small);
-  o_calc_p_rho(s, c, d, iteration);
+  TILEDV(o_advance_mu_t(s, c, d, (wreal)dts));
+  if (c->non_hydrostatic) TILEDV(o_advance_w(s, c, d, (wreal)dts));
+  TILEDV(o_sumflux(s, d, iteration, n_small));
+  TILEDV(o_calc_p_rho(s, c, d, iteration));
   o_set_physical_bc3d(s->ph_2, 'w', c, d);
   o_set_physical_bc3d(s->al, 'p', c, d);
   o_set_physical_bc3d(s->p, 'p', c, d);
@@ -190,8 +235,8 @@ int oracle_advance_mu_t_w(ostate *s, const cfg_t *c, const dims_t *d, double dts
   return 0;
 }
 int oracle_small_step_finish(ostate *s, const cfg_t *c, const dims_t *d, int rk_step, int n_small, double dts) {
-  o_calc_mu_uv_1(s, c, d);
-  o_small_step_finish(s, d, n_small, (wreal)dts, rk_step, 3);
+  TILEDV(o_calc_mu_uv_1(s, c, d));
+  TILEDV(o_small_step_finish(s, d, n_small, (wreal)dts, rk_step, 3));
   if (rk_step == 3) {
     o_set_physical_bc3d(s->ru_m, 'u', c, d);
     o_set_physical_bc3d(s->rv_m, 'v', c, d);
@@ -202,17 +247,16 @@ int oracle_small_step_finish(ostate *s, const cfg_t *c, const dims_t *d, int rk_
   return 0;
 }
 int oracle_rk_scalar_step(ostate *s, const cfg_t *c, const dims_t *d, int ispec, int rk_step, double dt_rk) {
-  int rc = o_rk_scalar_tend(s, c, d, ispec);
-  if (rc) return rc;
-  o_rk_update_scalar(s, d, ispec, rk_step, (wreal)dt_rk);
+  TILED(o_rk_scalar_tend(s, c, d, ispec));
+  TILEDV(o_rk_update_scalar(s, d, ispec, rk_step, (wreal)dt_rk));
   return 0;
 }
 int oracle_calc_p_rho_phi(ostate *s, const cfg_t *c, const dims_t *d) {
-  o_calc_p_rho_phi(s, c, d);
+  TILEDV(o_calc_p_rho_phi(s, c, d));
   return 0;
 }
 int oracle_set_w_surface(ostate *s, const cfg_t *c, const dims_t *d) {
-  o_set_w_surface(s->w_2, s, c, d);
+  TILEDV(o_set_w_surface(s->w_2, s, c, d));
   return 0;
 }
 int oracle_phys_bc_dry_2(ostate *s, const cfg_t *c, const dims_t *d, int kind) {

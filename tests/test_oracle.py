@@ -377,3 +377,35 @@ def test_advance_w_surface_term_closed_form(oracle32):
     exp = U * (ht[j0:j0 + spec.ny][:, ii + 1] - ht[j0:j0 + spec.ny][:, ii - 1]) / (2.0 * spec.dx)
     assert np.max(np.abs(exp)) > 0.3
     assert np.max(np.abs(w1 - exp)) <= 1e-5 * np.max(np.abs(exp)), np.max(np.abs(w1 - exp))
+
+
+TILE_CASES = {
+    "periodic": dict(nx=24, ny=20, nz=12, bc="periodic", n_passive=2, n_bubbles=2, bubble_amp=2.0, seed=11),
+    "open": dict(nx=22, ny=19, nz=12, bc="open", n_bubbles=1, seed=12),
+    "terrain_yopen": dict(nx=20, ny=23, nz=12, dx=1000.0, dy=1000.0, dt=3.0, bc_x="periodic", bc_y="open",
+                          hybrid_opt=2, sounding="stable", n_ridges=2, ridge_a=(3000.0, 8000.0), n_passive=1,
+                          n_bubbles=0, noise_amp=0.0, seed=22),
+}
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("case", sorted(TILE_CASES))
+def test_oracle_tiling_invariance(prec, case):
+    """The oracle calls every compute routine once per j-tile inside an OpenMP loop, as
+    solve_em does; the reference's routines write only their tile's points from inputs of
+    earlier phases, so the result of two steps is bit for bit that of a single tile for
+    any tile count (3 and 7 tiles here, rows of unequal height)."""
+    from pyoracle import Oracle
+    o = Oracle(prec)
+    o.lib.oracle_set_num_tiles.argtypes = [ctypes.c_int]
+    hs = grid.build_initial_state(grid.DomainSpec(**TILE_CASES[case]))
+    try:
+        o.lib.oracle_set_num_tiles(1)
+        one = o.run(hs, 2)
+        for nt in (3, 7):
+            o.lib.oracle_set_num_tiles(nt)
+            many = o.run(hs, 2)
+            bad = [k for k in one if not np.array_equal(one[k], many[k])]
+            assert not bad, f"{nt} tiles: {bad}"
+    finally:
+        o.lib.oracle_set_num_tiles(0)

@@ -39,6 +39,9 @@ static int fail(int code, const char *fmt, ...) {
 
 // ------------------------------------------------------------------ registry of host arrays
 struct Mirror { void *dev; size_t bytes; };
+// Before a mirror is freed: drop the cached graphs and work arrays of domains that use it,
+// and mark the ensembles that use it stale (g_mu held, device idle).
+static void drop_device_range(uintptr_t lo, uintptr_t hi);
 static std::map<uintptr_t, Mirror> g_reg;  // host base address -> mirror
 
 static void *translate(void *p) {
@@ -62,6 +65,8 @@ extern "C" int wrfmi_register_field(void *host_ptr, size_t bytes) {
       HIPCHK(hipMemcpy(it->second.dev, host_ptr, bytes, hipMemcpyHostToDevice));
       return 0;
     }
+    HIPCHK(hipDeviceSynchronize());
+    drop_device_range((uintptr_t)it->second.dev, (uintptr_t)it->second.dev + it->second.bytes);
     HIPCHK(hipFree(it->second.dev));
     g_reg.erase(it);
   }
@@ -83,6 +88,8 @@ extern "C" int wrfmi_unregister_field(void *host_ptr) {
   std::lock_guard<std::mutex> lk(g_mu);
   auto it = g_reg.find((uintptr_t)host_ptr);
   if (it == g_reg.end()) return fail(WRFMI_E_POINTER, "unregister: pointer was not registered");
+  HIPCHK(hipDeviceSynchronize());
+  drop_device_range((uintptr_t)it->second.dev, (uintptr_t)it->second.dev + it->second.bytes);
   HIPCHK(hipFree(it->second.dev));
   g_reg.erase(it);
   return 0;
@@ -104,16 +111,20 @@ extern "C" int wrfmi_sync_to_host(void *host_ptr) {
 }
 
 // ------------------------------------------------------------------ work arrays
-// Library-owned 3-D work arrays (one grid field each), kept per field size until
-// wrfmi_finalize.  Allocated before any launch is enqueued or captured.
+// Library-owned 3-D work arrays (one grid field each) of one domain.  A domain is known by
+// the device address of its u_2 array and its field size, so two domains never share work
+// arrays, and steps of different domains may run concurrently on different streams.  They
+// are allocated on the domain's first call, before any launch is enqueued or captured, and
+// kept until wrfmi_domain_release or wrfmi_finalize.
 struct Scratch { float *dv, *rhs, *lo; };
-static std::map<size_t, Scratch> g_scratch;
+typedef std::pair<uintptr_t, size_t> DomainKey;  // (device address of u_2, elements per field)
+static std::map<DomainKey, Scratch> g_scratch;
 
-static int get_scratch(size_t nelem, Scratch *out) {
-  auto it = g_scratch.find(nelem);
+static int get_scratch(DomainKey key, Scratch *out) {
+  auto it = g_scratch.find(key);
   if (it == g_scratch.end()) {
     Scratch sc{nullptr, nullptr, nullptr};
-    const size_t bytes = nelem * sizeof(float);
+    const size_t bytes = key.second * sizeof(float);
     if (hipMalloc(&sc.dv, bytes) != hipSuccess || hipMalloc(&sc.rhs, bytes) != hipSuccess ||
         hipMalloc(&sc.lo, bytes) != hipSuccess) {
       (void)hipGetLastError();
@@ -121,39 +132,51 @@ static int get_scratch(size_t nelem, Scratch *out) {
       if (sc.rhs) (void)hipFree(sc.rhs);
       return fail(WRFMI_E_DEVICE, "cannot allocate %zu bytes of work arrays", 3 * bytes);
     }
-    it = g_scratch.emplace(nelem, sc).first;
+    it = g_scratch.emplace(key, sc).first;
   }
   *out = it->second;
   return 0;
 }
 // Advection tendencies of species 0 .. n-2 for the all-species scalar kernels: n-1 grid
-// fields per (field size, species count), allocated on first use.
-static std::map<std::pair<size_t, int>, float *> g_qt;
-static int get_qt(size_t nelem, int nspecies, float **out) {
-  const auto key = std::make_pair(nelem, nspecies);
-  auto it = g_qt.find(key);
+// fields per (domain, species count), allocated on first use.
+static std::map<std::pair<DomainKey, int>, float *> g_qt;
+static int get_qt(DomainKey key, int nspecies, float **out) {
+  const auto qk = std::make_pair(key, nspecies);
+  auto it = g_qt.find(qk);
   if (it == g_qt.end()) {
     float *p = nullptr;
-    const size_t bytes = nelem * sizeof(float) * (size_t)(nspecies - 1);
+    const size_t bytes = key.second * sizeof(float) * (size_t)(nspecies - 1);
     if (hipMalloc(&p, bytes) != hipSuccess) {
       (void)hipGetLastError();
       return fail(WRFMI_E_DEVICE, "cannot allocate %zu bytes of species work arrays", bytes);
     }
-    it = g_qt.emplace(key, p).first;
+    it = g_qt.emplace(qk, p).first;
   }
   *out = it->second;
   return 0;
 }
-static void free_scratch() {
-  for (auto &kv : g_scratch) {
-    (void)hipFree(kv.second.dv);
-    (void)hipFree(kv.second.rhs);
-    (void)hipFree(kv.second.lo);
+// free the work arrays of domains whose u_2 lies in [lo, hi)
+static void free_scratch_in(uintptr_t lo, uintptr_t hi) {
+  for (auto it = g_scratch.begin(); it != g_scratch.end();) {
+    if (it->first.first >= lo && it->first.first < hi) {
+      (void)hipFree(it->second.dv);
+      (void)hipFree(it->second.rhs);
+      (void)hipFree(it->second.lo);
+      it = g_scratch.erase(it);
+    } else {
+      ++it;
+    }
   }
-  g_scratch.clear();
-  for (auto &kv : g_qt) (void)hipFree(kv.second);
-  g_qt.clear();
+  for (auto it = g_qt.begin(); it != g_qt.end();) {
+    if (it->first.first.first >= lo && it->first.first.first < hi) {
+      (void)hipFree(it->second);
+      it = g_qt.erase(it);
+    } else {
+      ++it;
+    }
+  }
 }
+static void free_scratch() { free_scratch_in(0, ~(uintptr_t)0); }
 
 // ------------------------------------------------------------------ library
 static hipStream_t g_stream = nullptr;
@@ -216,7 +239,10 @@ static int make_dm(const wrfmi_dims *w, Dm *d) {
     return fail(WRFMI_E_DIMS, "the GPU path processes the whole patch: tile must equal the domain");
   if (w->ids - w->ims < 4 || w->jds - w->jms < 4 || w->ime - w->ide < 4 || w->jme - w->jde < 4)
     return fail(WRFMI_E_DIMS, "memory dims need a halo of at least 4 (bdyzone) around the domain");
-  if (w->kms != w->kds || w->kme < w->kde) return fail(WRFMI_E_DIMS, "memory k range must start at kds and reach kde");
+  // the kernels take level 1 as the surface (k_ac_wsolve's surface w from u, v at levels
+  // 1..3, calc_coef_w's bottom row), as the reference's vertical loops do
+  if (w->kds != 1 || w->kms != 1) return fail(WRFMI_E_DIMS, "kds and kms must be 1 (level 1 is the surface)");
+  if (w->kme < w->kde) return fail(WRFMI_E_DIMS, "memory k range must reach kde");
   if (w->kde - w->kds < 4 || w->ide - w->ids < 7 || w->jde - w->jds < 7) return fail(WRFMI_E_DIMS, "domain too small");
   d->ids = w->ids; d->ide = w->ide; d->jds = w->jds; d->jde = w->jde; d->kds = w->kds; d->kde = w->kde;
   d->ims = w->ims; d->ime = w->ime; d->jms = w->jms; d->jme = w->jme; d->kms = w->kms; d->kme = w->kme;
@@ -652,8 +678,12 @@ static bool vec_layout(const wrfmi_state &s, const Dm &d, const KArgs &a) {
   return ok;
 }
 
-// Validate and translate the arguments into L; the work arrays are the library's shared
-// ones for this field size unless own_scratch is given (ensemble members each have theirs).
+static inline size_t field_elems(const Dm &d) { return (size_t)d.sj * (size_t)(d.jme - d.jms + 1); }
+
+// Validate and translate the arguments into L (g_mu held).  The work arrays are this
+// domain's own (get_scratch) unless own_scratch is given: ensemble members each have theirs,
+// and the allocation-free entry point passes the static array.  With own_scratch and no
+// own_qt, the species are advected one by one (no species work arrays).
 static int prepare(const wrfmi_state *st_in, const wrfmi_config *cfg, const wrfmi_dims *dims, Launcher &L, wrfmi_state &s,
                    const Scratch *own_scratch = nullptr, float *own_qt = nullptr) {
   int rc;
@@ -665,9 +695,10 @@ static int prepare(const wrfmi_state *st_in, const wrfmi_config *cfg, const wrfm
   L.a.s = s;
   L.a.c = *cfg;
   L.a.d = d;
+  const DomainKey key((uintptr_t)s.u_2, field_elems(d));
   Scratch sc;
   if (own_scratch) sc = *own_scratch;
-  else if ((rc = get_scratch((size_t)d.sj * (size_t)(d.jme - d.jms + 1), &sc))) return rc;
+  else if ((rc = get_scratch(key, &sc))) return rc;
   L.a.scr_dv = FPtr<float>(sc.dv);
   L.a.scr_rhs = FPtr<float>(sc.rhs);
   L.a.scr_lo = FPtr<float>(sc.lo);
@@ -676,14 +707,13 @@ static int prepare(const wrfmi_state *st_in, const wrfmi_config *cfg, const wrfm
   L.h.open_xs = cfg->open_xs; L.h.open_xe = cfg->open_xe; L.h.open_ys = cfg->open_ys; L.h.open_ye = cfg->open_ye;
   L.vec = vec_layout(s, d, L.a);
   // the all-species scalar kernels (doubly periodic, aligned layout, more than one species)
-  L.scal_all = L.vec && cfg->periodic_x && cfg->periodic_y && s.n_species > 1;
+  L.scal_all = L.vec && cfg->periodic_x && cfg->periodic_y && s.n_species > 1 && (own_qt || !own_scratch);
   L.zone_ok = (cfg->periodic_x || d.ide - d.ids + 1 >= 2 * kZone + 2) && (cfg->periodic_y || d.jde - d.jds + 1 >= 2 * kZone + 2);
   if (L.scal_all) {
-    const size_t nelem = (size_t)d.sj * (size_t)(d.jme - d.jms + 1);
     float *qt = own_qt;
-    if (!qt && (rc = get_qt(nelem, s.n_species, &qt))) return rc;
+    if (!qt && (rc = get_qt(key, s.n_species, &qt))) return rc;
     L.a.scr_qt = qt;
-    L.a.qt_stride = nelem;
+    L.a.qt_stride = field_elems(d);
   }
   return 0;
 }
@@ -727,9 +757,14 @@ extern "C" int wrfmi_solve_em_step_graph(wrfmi_state *st, const wrfmi_config *cf
   hipStream_t hs = (hipStream_t)stream;
   GraphEntry *hit = nullptr;
   {
+    // graphs are cached under the translated (device) pointers: a registered host array
+    // whose mirror was replaced no longer matches the graph recorded on the old mirror
     std::lock_guard<std::mutex> lk(g_mu);
+    wrfmi_state tr;
+    int rc;
+    if ((rc = translate_state(st, &tr))) return rc;
     for (GraphEntry *e : g_graphs)
-      if (!memcmp(&e->s, st, sizeof(*st)) && !memcmp(&e->c, cfg, sizeof(*cfg)) && !memcmp(&e->d, dims, sizeof(*dims))) hit = e;
+      if (!memcmp(&e->s, &tr, sizeof(tr)) && !memcmp(&e->c, cfg, sizeof(*cfg)) && !memcmp(&e->d, dims, sizeof(*dims))) hit = e;
   }
   if (!hit) {
     Launcher L;
@@ -748,7 +783,7 @@ extern "C" int wrfmi_solve_em_step_graph(wrfmi_state *st, const wrfmi_config *cf
     HIPCHK(hipStreamEndCapture(cs, &g));
     hipGraphExec_t x;
     HIPCHK(hipGraphInstantiate(&x, g, nullptr, nullptr, 0));
-    hit = new GraphEntry{*st, *cfg, *dims, g, x};
+    hit = new GraphEntry{s, *cfg, *dims, g, x};
     std::lock_guard<std::mutex> lk(g_mu);
     g_graphs.push_back(hit);
   }
@@ -779,6 +814,8 @@ struct wrfmi_ensemble {
   void *table = nullptr;      // device: member arguments, offsets, tables
   hipGraph_t g = nullptr;
   hipGraphExec_t x = nullptr;
+  std::vector<wrfmi_state> members;  // translated (device) pointers of every member
+  bool stale = false;         // a registered array a member uses was re-registered or unregistered
 };
 static void ens_free(wrfmi_ensemble *e) {
   if (e->x) (void)hipGraphExecDestroy(e->x);
@@ -834,6 +871,7 @@ extern "C" int wrfmi_ensemble_create(int n_members, const wrfmi_state *states, c
       rc = fail(rc, "member %d: %s", m, msg.c_str());
       break;
     }
+    e->members.push_back(s);
   }
   // one kernel form for all members: the four-points-per-lane kernels only if every
   // member's layout allows them
@@ -943,6 +981,7 @@ extern "C" int wrfmi_ensemble_step(wrfmi_ensemble *e, void *stream, int sync) {
   {
     std::lock_guard<std::mutex> lk(g_mu);
     if (!e || !ens_known(e)) return fail(WRFMI_E_POINTER, "not an ensemble of this library");
+    if (e->stale) return fail(WRFMI_E_POINTER, "a registered array of this ensemble was replaced or unregistered");
   }
   ens_enqueue(e, (hipStream_t)stream);
   HIPCHK(hipGetLastError());
@@ -955,6 +994,7 @@ extern "C" int wrfmi_ensemble_step_graph(wrfmi_ensemble *e, void *stream, int sy
   {
     std::lock_guard<std::mutex> lk(g_mu);
     if (!e || !ens_known(e)) return fail(WRFMI_E_POINTER, "not an ensemble of this library");
+    if (e->stale) return fail(WRFMI_E_POINTER, "a registered array of this ensemble was replaced or unregistered");
     if (!e->x) {
       if (!g_stream) HIPCHK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
       HIPCHK(hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal));
@@ -978,6 +1018,53 @@ extern "C" int wrfmi_ensemble_destroy(wrfmi_ensemble *e) {
       return 0;
     }
   return fail(WRFMI_E_POINTER, "not an ensemble of this library");
+}
+
+static bool state_touches(const wrfmi_state &s, uintptr_t lo, uintptr_t hi) {
+  auto in = [&](const void *p) { return (uintptr_t)p >= lo && (uintptr_t)p < hi; };
+  bool t = in(s.a) || in(s.alpha) || in(s.gamma);
+#define TOUCH(n) t = t || in(s.n);
+  WRFMI_FIELDS_3D(TOUCH) WRFMI_FIELDS_2D(TOUCH) WRFMI_FIELDS_1D(TOUCH)
+#undef TOUCH
+  for (int m = 0; m < s.n_species && m < WRFMI_MAX_SPECIES; m++) t = t || in(s.q[m]) || in(s.q_old[m]) || in(s.q_tend[m]);
+  return t;
+}
+static void drop_device_range(uintptr_t lo, uintptr_t hi) {
+  for (size_t n = 0; n < g_graphs.size();) {
+    GraphEntry *e = g_graphs[n];
+    if (state_touches(e->s, lo, hi)) {
+      (void)hipGraphExecDestroy(e->x);
+      (void)hipGraphDestroy(e->g);
+      delete e;
+      g_graphs.erase(g_graphs.begin() + n);
+    } else {
+      n++;
+    }
+  }
+  free_scratch_in(lo, hi);
+  for (wrfmi_ensemble *e : g_ensembles)
+    for (const wrfmi_state &m : e->members) e->stale = e->stale || state_touches(m, lo, hi);
+}
+
+// Release what the library holds for one domain: its work arrays and its cached graphs.
+extern "C" int wrfmi_domain_release(const wrfmi_state *st) {
+  if (!st) return fail(WRFMI_E_POINTER, "null argument");
+  std::lock_guard<std::mutex> lk(g_mu);
+  const uintptr_t u2 = (uintptr_t)translate((void *)st->u_2);
+  HIPCHK(hipDeviceSynchronize());
+  for (size_t n = 0; n < g_graphs.size();) {
+    GraphEntry *e = g_graphs[n];
+    if ((uintptr_t)e->s.u_2 == u2) {
+      (void)hipGraphExecDestroy(e->x);
+      (void)hipGraphDestroy(e->g);
+      delete e;
+      g_graphs.erase(g_graphs.begin() + n);
+    } else {
+      n++;
+    }
+  }
+  free_scratch_in(u2, u2 + 1);
+  return 0;
 }
 
 // ------------------------------------------------------------------ per-routine entry points
@@ -1142,12 +1229,13 @@ extern "C" int wrfmi_check_finite(const wrfmi_state *st_in, const wrfmi_dims *di
   int rc;
   Dm d;
   wrfmi_state s;
-  {
-    std::lock_guard<std::mutex> lk(g_mu);
-    if ((rc = make_dm(dims, &d))) return rc;
-    if ((rc = translate_state(st_in, &s))) return rc;
-    if (!g_counter) HIPCHK(hipMalloc(&g_counter, sizeof(unsigned long long)));
-  }
+  // the scan waits for all work queued on the device (steps may sit on non-blocking
+  // streams), and holds the library lock so that concurrent scans do not share the counter
+  std::lock_guard<std::mutex> lk(g_mu);
+  if ((rc = make_dm(dims, &d))) return rc;
+  if ((rc = translate_state(st_in, &s))) return rc;
+  if (!g_counter) HIPCHK(hipMalloc(&g_counter, sizeof(unsigned long long)));
+  HIPCHK(hipDeviceSynchronize());
   unsigned long long zero = 0, cnt = 0;
   HIPCHK(hipMemcpy(g_counter, &zero, sizeof zero, hipMemcpyHostToDevice));
   dim3 blk(64, 1, 1), grd((d.ide - d.ids + 63) / 64, d.jde - d.jds, 1);

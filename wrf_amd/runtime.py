@@ -129,6 +129,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         fn.argtypes = [vp, vp, ctypes.c_int]
     lib.wrfmi_ensemble_destroy.argtypes = [vp]
     lib.wrfmi_check_finite.argtypes = [vp, vp, ctypes.POINTER(ctypes.c_longlong)]
+    lib.wrfmi_domain_release.argtypes = [vp]
     lib.wrfmi_kernel_count.restype = ctypes.c_int
     lib.wrfmi_kernel_name.restype = ctypes.c_char_p
     lib.wrfmi_kernel_name.argtypes = [ctypes.c_int]
@@ -240,6 +241,11 @@ class DeviceDomain:
                 raise ValueError(f"{k}: shape {tuple(src.shape)} does not match {tuple(t.shape)}")
             t.copy_(src.to(self.device))
         torch.cuda.synchronize(self.device)
+
+    def release(self):
+        """Free the library's work arrays and cached graphs of this domain (it may be
+        stepped again afterwards: they are allocated again on the next call)."""
+        check(self.lib.wrfmi_domain_release(ctypes.byref(self.state)))
 
     def check_finite(self) -> int:
         nbad = ctypes.c_longlong(0)
