@@ -13,26 +13,42 @@
  * Build: oracle/Makefile (gcc -O2 -ffp-contract=off -fno-fast-math).
  * Every function cites the reference file:line it follows.
  *
- * How each part is pinned (tests/test_oracle.py):
+ * How each part is pinned (tests/test_oracle.py, tests/oracle_pins.py; the mutation test
+ * tests/test_oracle_mutations.py builds this source with 17 plausible mistakes planted and
+ * shows that every one fails at least one pin):
  *   o_advect_scalar  bit for bit against the reference's own advect_scalar, compiled in
  *                    place (oracle/_ref) and through tests/golden/advect_scalar_ref.npz:
  *                    periodic and open, both signs of time_step, field_old = field and not.
- *   o_advect_u/_v    bit for bit equal to o_advect_scalar in uniform flow (same fluxes,
- *                    shifted stencil), plus symmetry/translation/rest-state checks.
- *   o_advect_w       horizontal part equal to o_advect_scalar to rounding in uniform
- *                    flow; vertical part bit for bit on levels 2..kde-3.
- *   whole step       resting atmosphere stays at rest (FP32 and FP64); uniform state stays
- *                    uniform; mirror and x/y-transpose symmetry of a centred bubble
- *                    (periodic and open); periodic translation invariance (bitwise);
+ *   o_advect_u/v/w   bit for bit equal to o_advect_scalar in uniform flow (same fluxes,
+ *                    shifted stencil); a constant field in a non-uniform mass flux on a
+ *                    stretched grid gives -C div(mass flux) on every level, advect_w's
+ *                    lowest level and lid half-cell included; a field linear in eta is
+ *                    carried exactly through advect_u/v's second-order boundary faces.
+ *   o_calc_coef_w    eigenvalues of the implicit operator against the continuum vertical
+ *                    acoustic modes of an isothermal column; row sums; self-adjointness.
+ *   o_advance_w      the off-centred trapezoidal scheme of the w-phi pair with
+ *                    calc_coef_w's operator; a hydrostatically balanced perturbation
+ *                    exerts no force.
+ *   o_calc_p_rho     the first-order change of the full equation of state.
+ *   o_advance_uv, horizontal_pressure_gradient  the four-term eta-coordinate pressure
+ *                    gradient in closed form (terrain-slope term included), plus the
+ *                    external-mode filter.
+ *   o_advance_mu_t   mass budget, omega'' from continuity, uniform theta preserved.
+ *   o_small_step_prep/finish  the coupling round trip returns time level n.
+ *   rk_update_scalar, sumflux  a uniform mixing ratio stays uniform through whole steps.
+ *   whole step       resting atmosphere stays at rest (flat and over terrain); uniform
+ *                    state stays uniform; mirror and x/y-transpose symmetry of a centred
+ *                    bubble (periodic and open); periodic translation invariance (bitwise);
  *                    dry-air and species mass conservation; the EOS reproduces the
  *                    independently built hydrostatic state; FP32 and FP64 builds agree.
  * Parity unpinned (no reference build, fixture or closed form covers them, and the
  * benchmark configurations do not exercise them): the w_damping = 1 branch of w_damp,
- * damp_opt = 3 in advance_w, top_lid = 1, phi_adv_z = 2, hypsometric_opt = 1,
- * use_theta_m = 0, Coriolis with f or e non-zero, map factors other than 1, and
- * advect_w's lowest and top three levels.  The terrain terms (the surface-w term of
- * advance_w and set_w_surface, with their cf1..cf3 extrapolations) are pinned by their
- * closed form in uniform flow, U * (ht(i+1) - ht(i-1)) / (2 dx) (tests/test_oracle.py).
+ * damp_opt = 3 in advance_w, top_lid = 1, phi_adv_z = 2, hypsometric_opt = 2's
+ * log-pressure form of calc_p_rho_phi beyond the initial state it reproduces,
+ * use_theta_m = 0, Coriolis with f or e non-zero, curvature with map factors other than
+ * 1.  The terrain terms (the surface-w term of advance_w and set_w_surface, with their
+ * cf1..cf3 extrapolations) are pinned by their closed form in uniform flow,
+ * U * (ht(i+1) - ht(i-1)) / (2 dx) (tests/test_oracle.py).
  */
 #ifndef WRF_ORACLE_H
 #define WRF_ORACLE_H

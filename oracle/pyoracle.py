@@ -26,10 +26,11 @@ def _load(name):
 
 
 class Oracle:
-    def __init__(self, precision: str = "f32"):
+    def __init__(self, precision: str = "f32", path: str | None = None):
+        """`path` loads another build of the same sources (the mutation tests' builds)."""
         self.precision = precision
         libs = {"f32": "liboracle_f32.so", "f32cr": "liboracle_f32cr.so", "f64": "liboracle_f64.so"}
-        self.lib = _load(libs[precision])
+        self.lib = ctypes.CDLL(path) if path else _load(libs[precision])
         single = precision != "f64"
         self.dtype = np.float32 if single else np.float64
         self.ctype = ctypes.c_float if single else ctypes.c_double
