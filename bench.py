@@ -13,8 +13,8 @@ the 3-D arrays it must read and write once per launch.
 ensemble = member-steps/s of the 64-member uneven ensemble, member by member with the plain
 kernels and batched (one launch per kernel over all members, hipGraph replay).
 terrain (with --terrain) = the 1200x1200x80 terrain case, FP32 and precision forms.
-cpu_baseline = the FP32 C restatement of the same step (the oracle: one core), timed on
-a 100x100x80 sub-domain of the same workload.
+cpu_baseline = the FP32 C restatement of the same step (the oracle), OpenMP over the host
+cores with solve_em's j-tiling, on the same workload: median of 3 single steps.
 """
 import argparse
 import json
@@ -61,27 +61,45 @@ def b_alg(n_sound, n_species):
     return 4 * (327 + 12 * n_species + 51 * n_ac)
 
 
-def cpu_baseline(seconds_target=15.0):
-    """Oracle (FP32 C restatement, one thread) on a 100x100x80 periodic sub-domain of the
-    benchmark workload; returns the cpu_baseline object."""
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(hs, runs=3):
+    """The oracle (FP32 C restatement of the step in the reference's operation order) on the
+    full workload, OpenMP over the host cores this process may use, with solve_em's j-tiling
+    (one tile per thread, frame/module_tiles.F:271-278): one untimed step, then the median
+    of `runs` timed single steps."""
+    import ctypes
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from pyoracle import Oracle
-    from wrf_amd import grid
-    spec = grid.large_c1(nx=100, ny=100, n_bubbles=2)
-    hs = grid.build_initial_state(spec)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
     o = Oracle("f32")
     sess = o.session(hs)
-    t0 = time.perf_counter()
+    o.lib.oracle_num_tiles.argtypes = [ctypes.c_void_p]
+    tiles = o.lib.oracle_num_tiles(ctypes.byref(hs.dims))
     sess.step(1)
-    t1 = time.perf_counter()
-    n = max(1, min(20, int(seconds_target / max(t1 - t0, 1e-3))))
-    t0 = time.perf_counter()
-    sess.step(n)
-    dt = time.perf_counter() - t0
+    times = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        sess.step(1)
+        times.append(time.perf_counter() - t0)
+    dt = sorted(times)[len(times) // 2]
+    spec = hs.spec
     pts = spec.nx * spec.ny * spec.nz
-    return {"value": pts * n / dt / 1e6, "unit": "Mpoint-RK3-steps/s", "cores": 1, "kind": "port",
-            "sample": f"FP32 C restatement of the step (oracle), 1 thread, 100x100x80 periodic sub-domain of the "
-                      f"workload (qv + 6 scalars), {n} RK3 steps after 1 untimed, {dt:.1f} s"}
+    return {"value": pts / dt / 1e6, "unit": "Mpoint-RK3-steps/s", "cores": threads, "kind": "port",
+            "omp_threads": threads, "tiles": tiles, "cpu_model": _cpu_model(),
+            "seconds_per_step": [round(t, 3) for t in times],
+            "sample": f"FP32 C restatement of the step (oracle), OpenMP {threads} threads, {tiles} j-tiles, the full "
+                      f"{spec.nx}x{spec.ny}x{spec.nz} workload (qv + {spec.n_passive} scalars), median of {runs} "
+                      f"single RK3 steps after 1 untimed"}
 
 
 def ensemble_leg(runtime, grid, torch, n_members, steps, log):
@@ -325,7 +343,7 @@ def main():
         out["terrain"] = terrain_leg(runtime, grid, torch, args.terrain_steps, log)
     if not args.no_cpu_baseline:
         log("timing the CPU baseline ...")
-        out["cpu_baseline"] = cpu_baseline()
+        out["cpu_baseline"] = cpu_baseline(hs)
     print(json.dumps(out), flush=True)
 
 

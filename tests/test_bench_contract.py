@@ -32,3 +32,13 @@ def test_more_than_one_gpu_is_refused():
                        timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stderr
+
+
+def test_cpu_baseline_reports_threads_tiles_and_median():
+    from wrf_amd import grid
+    hs = grid.build_initial_state(grid.large_c1(nx=24, ny=20, nz=10, n_bubbles=1))
+    cb = bench.cpu_baseline(hs, runs=3)
+    assert cb["kind"] == "port" and cb["unit"] == "Mpoint-RK3-steps/s"
+    assert cb["cores"] == cb["omp_threads"] >= 1 and cb["tiles"] >= 1
+    assert len(cb["seconds_per_step"]) == 3 and cb["value"] > 0
+    assert "24x20x10" in cb["sample"] and cb["cpu_model"]
