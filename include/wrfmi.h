@@ -139,6 +139,17 @@ int wrfmi_solve_em_step(wrfmi_state *st, const wrfmi_config *cfg,
 int wrfmi_solve_em_step_graph(wrfmi_state *st, const wrfmi_config *cfg,
                               const wrfmi_dims *d, void *stream, int sync);
 
+/* Same step (bit for bit), for callers that cannot allocate device memory (code under
+ * hipGraph capture, tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  Its work arrays live in a static device array of the
+ * library, 48 MiB on every GPU it runs on, held until the process ends; the call makes no
+ * hipMalloc, hipFree or other allocating call.  Limit: memory dims of at most 4,194,304
+ * (2^22) elements per field (WRFMI_E_DIMS beyond).  The species are advected one by one.
+ * Kernel timing mode is refused.  All calls on one GPU share the array: calls that
+ * overlap in time on one GPU (different streams or threads) race. */
+int wrfmi_solve_em_step_noalloc(wrfmi_state *st, const wrfmi_config *cfg,
+                                const wrfmi_dims *d, void *stream, int sync);
+
 /* Work arrays.  The library keeps a few grid-sized work arrays per domain (a domain
  * is known by the device address of its u_2 array and its field size), allocated on
  * the domain's first call: steps of different domains may run concurrently on

@@ -61,7 +61,7 @@ inline hipError_t hipStreamDestroy(hipStream_t) { return hipSuccess; }
 inline hipError_t hipPointerGetAttributes(hipPointerAttribute_t *a, const void *) { a->type = hipMemoryTypeDevice; return hipSuccess; }
 inline hipError_t hipGetDevice(int *d) { *d = 0; return hipSuccess; }
 // a static device array is ordinary host memory here: its address is the symbol's own
-#define __align__(n) alignas(n)
+#define __align__(n) __attribute__((aligned(n)))
 #define HIP_SYMBOL(x) (&(x))
 inline hipError_t hipGetSymbolAddress(void **p, const void *sym) { *p = const_cast<void *>(sym); return hipSuccess; }
 // events are not emulated (kernel timing mode is a GPU-only diagnostic)

@@ -123,7 +123,39 @@ def ensemble_check(order: int, nsteps: int):
     sys.exit(1 if fails else 0)
 
 
+def noalloc_check(order: int):
+    """The allocation-free whole step (work arrays in the library's static array) is bit for
+    bit the allocating whole step, on two domains in a row, the second smaller."""
+    import ctypes
+    cases = [dict(nx=22, ny=14, nz=10, bc="periodic", n_passive=2, n_bubbles=2, bubble_amp=2.0, seed=31),
+             dict(nx=13, ny=12, nz=8, bc="open", n_bubbles=1, seed=32)]
+    fails = []
+    for n, kw in enumerate(cases):
+        hs = grid.build_initial_state(grid.DomainSpec(**kw))
+        ref = EmuDomain(hs, order=order)
+        ref.step(2)
+        want = ref.download()
+        del ref
+        e = EmuDomain(hs, order=order)
+        fn = e.lib.wrfmi_solve_em_step_noalloc
+        fn.argtypes = [ctypes.c_void_p] * 4 + [ctypes.c_int]
+        for _ in range(2):
+            e._chk(fn(ctypes.byref(e.state), ctypes.byref(e.cfg), ctypes.byref(e.dims), None, 1))
+        got = e.download()
+        bad = [k for k in want if not np.array_equal(got[k], want[k])]
+        if bad:
+            fails.append(f"case {n}: the allocation-free step differs in {bad}")
+        del e
+    for f in fails:
+        print("MISMATCH", f)
+    print("OK noalloc" if not fails else "FAILED noalloc", "order", order, flush=True)
+    sys.exit(1 if fails else 0)
+
+
 def main():
+    if sys.argv[1] == "noalloc":
+        noalloc_check(int(sys.argv[2]))
+        return
     if sys.argv[1] == "registry":
         registry_check()
         return

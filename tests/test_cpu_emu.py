@@ -51,3 +51,11 @@ def test_emulated_ensemble_matches_single_domains(emu_env, order):
     assert r.returncode == 0, r.stdout[-4000:] + r.stderr[-4000:]
     assert "OK ensemble" in r.stdout
 
+
+
+@pytest.mark.parametrize("order", [0, 1])
+def test_emulated_noalloc_step_matches_allocating_step(emu_env, order):
+    r = subprocess.run([sys.executable, os.path.join(HERE, "emu_phase_check.py"), "noalloc", str(order)],
+                       env=emu_env, capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stdout[-4000:] + r.stderr[-4000:]
+    assert "OK noalloc" in r.stdout

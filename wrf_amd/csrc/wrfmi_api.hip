@@ -181,12 +181,14 @@ static void free_scratch() { free_scratch_in(0, ~(uintptr_t)0); }
 // ------------------------------------------------------------------ library
 static hipStream_t g_stream = nullptr;
 static unsigned long long *g_counter = nullptr;
+static int noalloc_scratch(float **out);
 
 extern "C" int wrfmi_init(void) {
   std::lock_guard<std::mutex> lk(g_mu);
   if (!g_stream) HIPCHK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
   if (!g_counter) HIPCHK(hipMalloc(&g_counter, sizeof(unsigned long long)));
-  return 0;
+  float *p = nullptr;
+  return noalloc_scratch(&p);  // the allocation-free step's work array on this device
 }
 struct GraphEntry;
 static std::vector<GraphEntry *> g_graphs;
@@ -726,6 +728,63 @@ extern "C" int wrfmi_solve_em_step(wrfmi_state *st, const wrfmi_config *cfg, con
   {
     std::lock_guard<std::mutex> lk(g_mu);
     if ((rc = prepare(st, cfg, dims, L, s))) return rc;
+  }
+  L.st = (hipStream_t)stream;
+  enqueue_step(L);
+  HIPCHK(hipGetLastError());
+  if (sync) HIPCHK(hipStreamSynchronize(L.st));
+  return 0;
+}
+
+// ------------------------------------------------------------------ allocation-free step
+// The whole step with its three work arrays (dvdxi, rhs, and the FP64 low part of the
+// precision form) in `scratch`, a static device array of three fields of at most
+// kNoAllocElems elements each (48 MiB on every GPU the library runs on, held until the
+// process ends), so that no call allocates.  In host code `scratch` names a host-side
+// stand-in: its device address comes from hipGetSymbolAddress, once per device (at
+// wrfmi_init, or at the entry point's first call on a device), and only that address is
+// passed to kernels.  Nothing in it needs to start at zero: every kernel writes its part of
+// a work array before it or a later kernel reads that part.  The species are advected one
+// by one, so no species work arrays are needed.  Calls on one GPU share the array, so calls
+// that overlap in time on one GPU (two streams, two threads) race.
+constexpr size_t kNoAllocElems = (size_t)1 << 22;
+static __device__ __align__(256) float scratch[3 * kNoAllocElems];
+constexpr int kMaxDevices = 64;
+static float *g_scratch_dev[kMaxDevices];  // device address of `scratch` per device (g_mu)
+
+// device address of `scratch` on the current device (g_mu held)
+static int noalloc_scratch(float **out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(WRFMI_E_DEVICE, "device %d beyond the %d the library supports", dev, kMaxDevices);
+  if (!g_scratch_dev[dev]) {
+    float *p = nullptr;
+    HIPCHK(hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(WRFMI_E_DEVICE, "no device address for the static work array");
+    g_scratch_dev[dev] = p;
+  }
+  *out = g_scratch_dev[dev];
+  return 0;
+}
+
+extern "C" int wrfmi_solve_em_step_noalloc(wrfmi_state *st, const wrfmi_config *cfg, const wrfmi_dims *dims, void *stream,
+                                           int sync) {
+  if (!st || !cfg || !dims) return fail(WRFMI_E_POINTER, "null argument");
+  Launcher L;
+  wrfmi_state s;
+  int rc;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (g_timing) return fail(WRFMI_E_BADOPT, "kernel timing mode creates events: not available without allocation");
+    Dm d;
+    if ((rc = make_dm(dims, &d))) return rc;
+    if (field_elems(d) > kNoAllocElems)
+      return fail(WRFMI_E_DIMS, "memory dims hold %zu elements per field; the allocation-free step takes at most %zu",
+                  field_elems(d), kNoAllocElems);
+    float *p = nullptr;
+    if ((rc = noalloc_scratch(&p))) return rc;
+    const Scratch sc{p, p + kNoAllocElems, p + 2 * kNoAllocElems};
+    if ((rc = prepare(st, cfg, dims, L, s, &sc, nullptr))) return rc;
   }
   L.st = (hipStream_t)stream;
   enqueue_step(L);
