@@ -199,9 +199,10 @@ int wrfmi_rk_tendency(wrfmi_state *st, const wrfmi_config *cfg, const wrfmi_dims
  * fills of solve_em.F:1181-1264. */
 int wrfmi_small_step_prep(wrfmi_state *st, const wrfmi_config *cfg, const wrfmi_dims *d,
                           int rk_step, float dts, void *stream);
-/* advance_uv (module_small_step_em.F:654-967), called at solve_em.F:1279. */
-int wrfmi_advance_uv(wrfmi_state *st, const wrfmi_config *cfg, const wrfmi_dims *d,
-                     float dts, void *stream);
+/* advance_uv (module_small_step_em.F:654-967), called at solve_em.F:1279; the entry point
+ * with advance_uv's own argument list is wrfmi_advance_uv (wrfmi_routines.h). */
+int wrfmi_advance_uv_state(wrfmi_state *st, const wrfmi_config *cfg, const wrfmi_dims *d,
+                           float dts, void *stream);
 /* One column pass: advance_mu_t (969-1175), advance_w (1178-1469), sumflux
  * (1473-1633), calc_p_rho (438-568) for acoustic substep `iteration` of
  * `n_small` (called at solve_em.F:1392, 1499, 1566, 1623), and the fills
@@ -218,6 +219,11 @@ int wrfmi_small_step_finish(wrfmi_state *st, const wrfmi_config *cfg, const wrfm
  * (called at solve_em.F:2215-2309 and 2781-2856). */
 int wrfmi_rk_scalar_step(wrfmi_state *st, const wrfmi_config *cfg, const wrfmi_dims *d,
                          int ispec, int rk_step, float dt_rk, void *stream);
+/* rk_scalar_tend alone (module_em.F:1096-1442: advect_tend = advect_scalar of species
+ * `ispec` with ru_m, rv_m, ww_m, from zero), called at solve_em.F:2215 and 2781; a host
+ * then calls rk_update_scalar with its own argument list (wrfmi_rk_update_scalar). */
+int wrfmi_rk_scalar_tend(wrfmi_state *st, const wrfmi_config *cfg, const wrfmi_dims *d,
+                         int ispec, int rk_step, void *stream);
 /* calc_p_rho_phi (module_big_step_utilities_em.F:953-1223), called at solve_em.F:2965. */
 int wrfmi_calc_p_rho_phi(wrfmi_state *st, const wrfmi_config *cfg, const wrfmi_dims *d,
                          void *stream);

@@ -90,7 +90,12 @@ class OracleSession:
             fn.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_int if k == "i" else ctypes.c_double for k in kinds]
             fn.restype = ctypes.c_int
 
+    # the library's state-struct form of advance_uv is wrfmi_advance_uv_state (its
+    # wrfmi_advance_uv keeps the reference's argument list)
+    _ALIAS = {"advance_uv_state": "advance_uv"}
+
     def phase(self, name: str, *args):
+        name = self._ALIAS.get(name, name)
         rc = getattr(self.o.lib, "oracle_" + name)(ctypes.byref(self.st), ctypes.byref(self.cfg),
                                                    ctypes.byref(self.dims), *args)
         if rc != 0:

@@ -8,7 +8,31 @@
 #include <ucontext.h>
 
 #include <cstdlib>
+#include <map>
+#include <mutex>
 #include <vector>
+
+// hipMalloc'd blocks (start -> size) for hipMemGetAddressRange
+static std::map<uintptr_t, size_t> g_allocs;
+static std::mutex g_alloc_mu;
+void emu_track(void *p, size_t n) {
+  std::lock_guard<std::mutex> lk(g_alloc_mu);
+  g_allocs[(uintptr_t)p] = n;
+}
+void emu_untrack(void *p) {
+  std::lock_guard<std::mutex> lk(g_alloc_mu);
+  g_allocs.erase((uintptr_t)p);
+}
+hipError_t hipMemGetAddressRange(hipDeviceptr_t *base, size_t *size, hipDeviceptr_t p) {
+  std::lock_guard<std::mutex> lk(g_alloc_mu);
+  auto it = g_allocs.upper_bound((uintptr_t)p);
+  if (it == g_allocs.begin()) return 1;
+  --it;
+  if ((uintptr_t)p >= it->first + it->second) return 1;
+  *base = (void *)it->first;
+  *size = it->second;
+  return hipSuccess;
+}
 
 thread_local emu_uint3 blockIdx, threadIdx;
 thread_local dim3 blockDim, gridDim;

@@ -50,9 +50,23 @@ struct hipPointerAttribute_t { hipMemoryType type; };
 
 inline const char *hipGetErrorString(hipError_t e) { return e ? "emulated HIP error" : "no error"; }
 inline hipError_t hipGetLastError() { return hipSuccess; }
-inline hipError_t hipMalloc(void **p, size_t n) { *p = std::malloc(n); return *p ? hipSuccess : 2; }
+// device allocations are malloc'd blocks, recorded so that hipMemGetAddressRange can find them
+void emu_track(void *p, size_t n);
+void emu_untrack(void *p);
+inline hipError_t hipMalloc(void **p, size_t n) {
+  *p = std::malloc(n);
+  if (!*p) return 2;
+  emu_track(*p, n);
+  return hipSuccess;
+}
 template <typename T> inline hipError_t hipMalloc(T **p, size_t n) { return hipMalloc((void **)p, n); }
-inline hipError_t hipFree(void *p) { std::free(p); return hipSuccess; }
+inline hipError_t hipFree(void *p) {
+  emu_untrack(p);
+  std::free(p);
+  return hipSuccess;
+}
+typedef void *hipDeviceptr_t;
+hipError_t hipMemGetAddressRange(hipDeviceptr_t *base, size_t *size, hipDeviceptr_t p);
 inline hipError_t hipMemcpy(void *d, const void *s, size_t n, hipMemcpyKind) { std::memcpy(d, s, n); return hipSuccess; }
 inline hipError_t hipDeviceSynchronize() { return hipSuccess; }
 inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }

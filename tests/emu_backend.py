@@ -66,6 +66,10 @@ class EmuDomain:
             self.arrays[k][...] = v
             self._chk(self.lib.wrfmi_sync_to_device(self.arrays[k].ctypes.data))
 
+    def addr(self, name: str) -> int:
+        """Address of the registered host array `name` (the library uses its mirror)."""
+        return self.arrays[name].ctypes.data
+
     def download(self) -> dict:
         for v in self.arrays.values():
             self._chk(self.lib.wrfmi_sync_to_host(v.ctypes.data))

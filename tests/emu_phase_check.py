@@ -152,7 +152,64 @@ def noalloc_check(order: int):
     sys.exit(1 if fails else 0)
 
 
+def routines_check(case: str, order: int):
+    """The entry points with the reference's argument lists against the oracle routines of
+    the same name, then one RK3 step of the Fortran solve_em-shaped host (which calls them)
+    against the oracle's whole step."""
+    import ctypes
+    from routine_parity import routine_parity
+    spec = grid.DomainSpec(**CASES[case])
+    hs = grid.build_initial_state(spec)
+    fails = []
+    for rk in (1, 2):
+        emu = EmuDomain(hs, order=order)
+        fails += [f"rk_step {rk}: {f}" for f in routine_parity(emu, Oracle("f32cr").session(hs), hs, rk_step=rk)]
+        del emu
+    emu = EmuDomain(hs, order=order)
+    flib = ctypes.CDLL(os.path.join(HERE, "cpu_emu", "libwrfmi_fortran_emu.so"))
+    flib.wrfmi_fortran_solve_em_step.argtypes = [ctypes.c_void_p] * 3
+    for _ in range(2):
+        rc = flib.wrfmi_fortran_solve_em_step(ctypes.byref(emu.state), ctypes.byref(emu.cfg), ctypes.byref(emu.dims))
+        if rc != 0:
+            fails.append(f"Fortran host step failed ({rc}): {emu.lib.wrfmi_last_error().decode()}")
+            break
+    if not fails:
+        got = emu.download()
+        exp = Oracle("f32cr").run(hs, 2)
+        bad = [k for k in exp if not np.array_equal(got[k], exp[k])]
+        if bad:
+            fails.append(f"Fortran host: two steps differ from the oracle in {bad}")
+    for f in fails:
+        print("MISMATCH", f)
+    print("OK routines" if not fails else "FAILED routines", case, "order", order, flush=True)
+    sys.exit(1 if fails else 0)
+
+
+def coef_width_check():
+    """A state whose calc_coef_w arrays hold 4-byte elements is refused in precision mode 1
+    (whose column solve writes 8-byte gamma) before any launch, and accepted in mode 0."""
+    import ctypes
+    hs = grid.build_initial_state(grid.DomainSpec(nx=12, ny=10, nz=8, bc="periodic", n_bubbles=1, seed=41))
+    emu = EmuDomain(hs, order=0)
+    emu.lib.wrfmi_solve_em_step.argtypes = [ctypes.c_void_p] * 4 + [ctypes.c_int]
+    emu.cfg.precision_mode = 1
+    rc = emu.lib.wrfmi_solve_em_step(ctypes.byref(emu.state), ctypes.byref(emu.cfg), ctypes.byref(emu.dims), None, 1)
+    msg = emu.lib.wrfmi_last_error().decode()
+    ok = rc == 4 and "8-byte" in msg
+    emu.cfg.precision_mode = 0
+    rc0 = emu.lib.wrfmi_solve_em_step(ctypes.byref(emu.state), ctypes.byref(emu.cfg), ctypes.byref(emu.dims), None, 1)
+    ok = ok and rc0 == 0
+    print("OK coefwidth" if ok else f"FAILED coefwidth rc={rc} msg={msg!r} rc0={rc0}", flush=True)
+    sys.exit(0 if ok else 1)
+
+
 def main():
+    if sys.argv[1] == "coefwidth":
+        coef_width_check()
+        return
+    if sys.argv[1] == "routines":
+        routines_check(sys.argv[2], int(sys.argv[3]))
+        return
     if sys.argv[1] == "noalloc":
         noalloc_check(int(sys.argv[2]))
         return

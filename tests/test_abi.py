@@ -13,15 +13,21 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def declared_entry_points():
-    text = open(os.path.join(ROOT, "include", "wrfmi.h")).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(wrfmi_\w+)\s*\(", text)))
+    names = set()
+    for h in ("wrfmi.h", "wrfmi_routines.h"):
+        text = open(os.path.join(ROOT, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names |= set(re.findall(r"\b(wrfmi_\w+)\s*\(", text))
+    return sorted(names)
 
 
 def test_header_declares_entry_points():
     names = declared_entry_points()
     assert "wrfmi_solve_em_step" in names and "wrfmi_advance_mu_t_w" in names
-    assert len(names) >= 25
+    for routine in ("advect_u", "advect_v", "advect_w", "advect_scalar", "rk_update_scalar", "advance_uv",
+                    "advance_mu_t", "advance_w", "sumflux", "calc_p_rho", "calc_coef_w"):
+        assert "wrfmi_" + routine in names
+    assert len(names) >= 40
 
 
 def test_library_exports_every_declared_symbol():
@@ -91,3 +97,19 @@ def test_step_phase_sequence():
     assert names.count("rk_scalar_step") == 3
     assert names[-3:] == ["calc_p_rho_phi", "phys_bc_dry_2", "set_w_surface"]
     assert runtime.num_sound_steps(type(hs_cfg)(), 6.0, 1000.0, 1000.0) == 4
+
+
+def test_reference_argument_entry_points_check_before_the_device():
+    """A null array argument is refused with WRFMI_E_POINTER, and an option outside the
+    implemented set with WRFMI_E_BADOPT, before any launch (no GPU needed)."""
+    spec = grid.DomainSpec(nx=10, ny=10, nz=6)
+    cfg, dims = grid.make_config(spec), grid.make_dims(spec)
+    lib = runtime.load_library()
+    d = [ctypes.c_int(getattr(dims, n)) for n, _ in dims._fields_]
+    dref = [ctypes.byref(x) for x in d]
+    one = ctypes.c_float(1.0)
+    ts = ctypes.c_int(6)
+    args = [None] * 9 + [ctypes.byref(ts), ctypes.byref(cfg)] + [None] * 8 + [ctypes.byref(one)] * 2 + [None]
+    assert lib.wrfmi_advect_u(*(args + dref)) == runtime.WRFMI_E_POINTER
+    cfg.h_mom_adv_order = 3
+    assert lib.wrfmi_advect_u(*(args + dref)) == runtime.WRFMI_E_BADOPT

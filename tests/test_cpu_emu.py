@@ -59,3 +59,22 @@ def test_emulated_noalloc_step_matches_allocating_step(emu_env, order):
                        env=emu_env, capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stdout[-4000:] + r.stderr[-4000:]
     assert "OK noalloc" in r.stdout
+
+
+@pytest.mark.parametrize("case,order", [("periodic", 0), ("open", 1), ("terrain_yopen", 0), ("periodic_unaligned", 1)])
+def test_emulated_reference_argument_entry_points(emu_env, case, order):
+    """The entry points with the reference's argument lists against the oracle routines of
+    the same name, and two steps of the Fortran solve_em host that calls them."""
+    r = subprocess.run([sys.executable, os.path.join(HERE, "emu_phase_check.py"), "routines", case, str(order)],
+                       env=emu_env, capture_output=True, text=True, timeout=900)
+    assert r.returncode == 0, r.stdout[-4000:] + r.stderr[-4000:]
+    assert "OK routines" in r.stdout
+
+
+def test_emulated_coefficient_width_checked(emu_env):
+    """Regression: precision mode 1 on a state with 4-byte calc_coef_w arrays is refused
+    (WRFMI_E_POINTER) before any kernel could write 8-byte gamma past the array end."""
+    r = subprocess.run([sys.executable, os.path.join(HERE, "emu_phase_check.py"), "coefwidth"],
+                       env=emu_env, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-4000:] + r.stderr[-4000:]
+    assert "OK coefwidth" in r.stdout

@@ -116,6 +116,7 @@ struct KArgs {
   float *scr_qt;        // advect_tend of species 0 .. n_species-2 (all-species scalar kernels),
   unsigned long long qt_stride;  // one grid field (elements) apart
   int tnx, tny, tnz;  // tiles of the current launch in i, j and k (see tile3)
+  int routine;        // Routine: R_FUSED in the whole step and the grouped entry points
 };
 
 // One field of a batched halo fill (set_physical_bc3d/2d on up to 16 fields per launch)

@@ -109,7 +109,9 @@ __global__ void __launch_bounds__(256) k_adv_scalar(ARGS args) {
       fy_up = buf[row + 1][lane];
     }
     if (writer) {
-      float t = 0.f;
+      // advect_scalar alone (a per-routine entry point) adds to the tendency array's values;
+      // rk_scalar_tend's call starts from zero
+      float t = a.routine == R_ADVECT ? tend[X3(d, i, k, j)] : 0.f;
       if (mass && k <= ktf) {
         if (ydiv) t = t - msftx * rdy * (fy_up - fy);
         if (xdiv) t = t - msftx * rdx * (fx_up - fx);

@@ -117,6 +117,39 @@ __global__ void __launch_bounds__(256) k_p_rho0(ARGS args) {
   s.pm1[o] = pp;
 }
 
+// calc_p_rho alone (module_small_step_em.F:438-568, non-hydrostatic), for the per-routine
+// entry point with the reference's argument list: mu and mut are the arrays bound to
+// mu_2 and muts, t_1 to t_save, step = the iteration field (0 at small_step_prep).  The
+// whole step computes the same values in k_p_rho0 (step 0) and k_ac_wsolve (steps >= 1).
+template <typename ARGS>
+__global__ void __launch_bounds__(256) k_calc_p_rho(ARGS args) {
+  Tile3 T;
+  const KArgs &a = kargs(args, T);
+  const Dm &d = a.d;
+  const DState &s = dstate(a);
+  const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
+  const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
+  const int k = d.kds + (int)T.bz;
+  if (i > d.ide - 1 || j > d.jde - 1 || k > d.kde - 1) return;
+  const Off o = X3(d, i, k, j), o2 = X2(d, i, j);
+  const float mu = s.mu_2[o2], mut = s.muts[o2];
+  const float c1 = s.c1h[X1(d, k)], c2 = s.c2h[X1(d, k)];
+  const float alt = s.alt[o];
+  const float al = -1.f / (c1 * mut + c2) * (alt * (c1 * mu) + s.rdnw[X1(d, k)] * (s.ph_2[o + d.ni] - s.ph_2[o]));
+  s.al[o] = al;
+  const float t1 = s.t_save[o];
+  float pp = s.c2a[o] * (alt * (s.t_2[o] - (c1 * mu) * t1) / ((c1 * mut + c2) * (K_T0 + t1)) - al);
+  if (a.iteration == 0) {
+    s.p[o] = pp;
+    s.pm1[o] = pp;
+  } else {
+    const float ptmp = pp;
+    pp = pp + a.c.smdiv * (pp - s.pm1[o]);
+    s.p[o] = pp;
+    s.pm1[o] = ptmp;
+  }
+}
+
 // small_step_prep's 2-D updates and, with COEF, calc_coef_w: one thread per (i, j) column.
 // The whole-step driver launches it without COEF: k_ac_wsolve forms the same coefficients
 // on its way up each column from values it loads anyway.
@@ -133,18 +166,21 @@ __global__ void __launch_bounds__(256) k_coef_w(ARGS args) {
   const bool mi = i <= ide - 1, mj = j <= jde - 1, mass = mi && mj;
   const CPtr<float> c1h = s.c1h, c2h = s.c2h, c1f = s.c1f, c2f = s.c2f;
   const Off o2 = X2(d, i, j);
-  const MuS m = small_mu(a, i, j);
-  if (mass) {
-    if (a.rk_step == 1) {
-      s.mu_1[o2] = s.mu_2[o2];
-      s.mudf[o2] = 0.f;
+  // calc_coef_w alone (a per-routine entry point): no small_step_prep updates
+  if (a.routine != R_COEF_W) {
+    const MuS m = small_mu(a, i, j);
+    if (mass) {
+      if (a.rk_step == 1) {
+        s.mu_1[o2] = s.mu_2[o2];
+        s.mudf[o2] = 0.f;
+      }
+      s.muts[o2] = m.muts;
+      s.mu_save[o2] = s.mu_2[o2];
+      s.mu_2[o2] = m.mu_new;
     }
-    s.muts[o2] = m.muts;
-    s.mu_save[o2] = s.mu_2[o2];
-    s.mu_2[o2] = m.mu_new;
+    if (mj) s.muus[o2] = m.muus;
+    if (mi) s.muvs[o2] = m.muvs;
   }
-  if (mj) s.muus[o2] = m.muus;
-  if (mi) s.muvs[o2] = m.muvs;
   if (!COEF || !mass || !a.c.non_hydrostatic) return;
   const float mut = s.mut[o2];
   const FPtr<CT> A((CT *)s.a), AL((CT *)s.alpha), GA((CT *)s.gamma);
@@ -507,7 +543,9 @@ __global__ void __launch_bounds__(256) k_ac_div(ARGS args) {
   const int k = d.kds + (int)T.bz;
   if (i > d.ide || j > d.jde || k > d.kde) return;
   const Off o2 = X2(d, i, j), o = X3(d, i, k, j);
-  sumflux_uv(a, s, i, j, k, o, o2);
+  // per-routine entry points: advance_mu_t without sumflux, or sumflux alone
+  if (a.routine != R_MU_T) sumflux_uv(a, s, i, j, k, o, o2);
+  if (a.routine == R_SUMFLUX) return;
   if (i > d.ide - 1 || j > d.jde - 1 || k > d.kde - 1) return;
   const FPtr<float> u = s.u_2, u_1 = s.u_save, v = s.v_2, v_1 = s.v_save;
   const float msftx = s.msftx[o2], msfty = s.msfty[o2], rdx = s.rdx, rdy = s.rdy;
@@ -567,7 +605,8 @@ __global__ void __launch_bounds__(256) k_ac_mu_ww(ARGS args) {
       });
 }
 
-// theta update of advance_mu_t (1125-1174), t_2ave and rhs of advance_w (1312-1371)
+// theta update of advance_mu_t (1125-1174), t_2ave of advance_w (1312-1320) and the ww_m
+// part of sumflux; in the per-routine forms (KArgs::routine) one of them
 template <typename ARGS>
 __global__ void __launch_bounds__(256) k_ac_theta(ARGS args) {
   Tile3 T;
@@ -582,16 +621,27 @@ __global__ void __launch_bounds__(256) k_ac_theta(ARGS args) {
   const wrfmi_config &c = a.c;
   const Off o2 = X2(d, i, j), o = X3(d, i, k, j);
   const float dts = a.dts, epssm = c.epssm;
-  const float msftx = s.msftx[o2], msfty = s.msfty[o2], rdx = s.rdx, rdy = s.rdy;
+  const float rdx = s.rdx, rdy = s.rdy;
   const FPtr<float> ww = s.ww, t_1 = s.t_save;
-  {  // sumflux of omega (module_small_step_em.F:1473-1633) at this mass point
+  const int routine = a.routine;
+  if (routine == R_FUSED || routine == R_SUMFLUX) {  // sumflux of omega (module_small_step_em.F:1473-1633) at this mass point
     const int it = a.iteration, ns = a.nsmall;
     float wwm = it == 1 ? 0.f : s.ww_m[o];
     wwm = wwm + ww[o];
     if (it == ns) wwm = wwm / (float)ns + s.ww1[o];
     s.ww_m[o] = wwm;
   }
-  if (k <= k_end) {
+  if (routine == R_W_TAVE && k <= k_end) {
+    // advance_w alone: t_2ave from the updated theta and advance_mu_t's t_ave (1312-1320),
+    // the expressions of the fused form below
+    const float t = s.t_2[o], tv = s.t_2save[o];
+    const float muave = s.muave[o2], muts = s.muts[o2];
+    float v1 = .5f * ((1.f + epssm) * t + (1.f - epssm) * tv);
+    v1 = (v1 + (s.c1h[X1(d, k)] * muave) * K_T0) / ((s.c1h[X1(d, k)] * muts + s.c2h[X1(d, k)]) * (K_T0 + t_1[o]));
+    s.t_2save[o] = v1;
+  }
+  if ((routine == R_FUSED || routine == R_MU_T) && k <= k_end) {
+    const float msftx = s.msftx[o2], msfty = s.msfty[o2];
     const FPtr<float> u = s.u_2, v = s.v_2;
     const float tv = s.t_2[o];
     float t = tv + msfty * dts * s.t_tend[o];
@@ -602,7 +652,7 @@ __global__ void __launch_bounds__(256) k_ac_theta(ARGS args) {
                           .5f * rdx * (u[o + 1] * (t_1[o + 1] + t_1[o]) - u[o] * (t_1[o] + t_1[o - 1]))) +
                  s.rdnw[X1(d, k)] * (wdtn_hi - wdtn_lo));
     s.t_2[o] = t;
-    if (c.non_hydrostatic) {
+    if (c.non_hydrostatic && routine == R_FUSED) {
       // t_2ave (advance_w 1312-1320): the old theta is t_2save's value from advance_mu_t
       const float muave = s.muave[o2], muts = s.muts[o2];
       float v1 = .5f * ((1.f + epssm) * t + (1.f - epssm) * tv);
@@ -991,7 +1041,7 @@ __global__ void __launch_bounds__(256) k_ac_theta_x4(ARGS args) {
 #ifndef WRFMI_WSOLVE_PF
 #define WRFMI_WSOLVE_PF 4
 #endif
-template <typename ARGS, typename CT>
+template <typename ARGS, typename CT, bool PRHO>
 __global__ void __launch_bounds__(256) k_ac_wsolve(ARGS args) {
   Tile3 T;
   const KArgs &a = kargs(args, T);
@@ -1159,7 +1209,9 @@ __global__ void __launch_bounds__(256) k_ac_wsolve(ARGS args) {
         }
       });
   // downward: back substitution, ph, calc_p_rho.  Fields: 0 = forward value of w (plus its
-  // low part in FP64), 1 = gamma, 2 = rhs, 3 = t_2, 4 = t_save, 5 = alt, 6 = c2a, 7 = pm1
+  // low part in FP64), 1 = gamma, 2 = rhs, 3 = t_2, 4 = t_save, 5 = alt, 6 = c2a, 7 = pm1.
+  // Without PRHO (advance_w alone, a per-routine entry point) calc_p_rho is left out and
+  // fields 3 to 7 are not loaded.
   const float muts = s.muts[o2], mu = s.mu_2[o2];
   const float smdiv = c.smdiv;
   CT wup = wfe;
@@ -1174,11 +1226,11 @@ __global__ void __launch_bounds__(256) k_ac_wsolve(ARGS args) {
             return (CT)w[o] + (CT)lo[o];
           case 1: return GA[o];
           case 2: return (CT)rhs[o];
-          case 3: return (CT)s.t_2[o];
-          case 4: return (CT)s.t_save[o];
-          case 5: return (CT)s.alt[o];
-          case 6: return (CT)s.c2a[o];
-          default: return (CT)s.pm1[o];
+          case 3: return PRHO ? (CT)s.t_2[o] : CT(0);
+          case 4: return PRHO ? (CT)s.t_save[o] : CT(0);
+          case 5: return PRHO ? (CT)s.alt[o] : CT(0);
+          case 6: return PRHO ? (CT)s.c2a[o] : CT(0);
+          default: return PRHO ? (CT)s.pm1[o] : CT(0);
         }
       },
       [&](int k, const CT *v) {
@@ -1203,7 +1255,7 @@ __global__ void __launch_bounds__(256) k_ac_wsolve(ARGS args) {
         } else {
           phk = ph[o];
         }
-        if (k <= k_end) {
+        if (PRHO && k <= k_end) {
           const float cc = c1h[X1(d, k)], c2 = c2h[X1(d, k)];
           const float alt = (float)v[5];
           const float al = -1.f / (cc * muts + c2) * (alt * (cc * mu) + rdnw[X1(d, k)] * (ph_up - phk));
@@ -1286,8 +1338,11 @@ WRFMI_INST_P2(k_coef_w, float, false)
 WRFMI_INST_P2(k_coef_w, double, false)
 WRFMI_INST_P(k_ac_mu_ww, false)
 WRFMI_INST_P(k_ac_mu_ww, true)
-WRFMI_INST_P(k_ac_wsolve, float)
-WRFMI_INST_P(k_ac_wsolve, double)
+WRFMI_INST_P2(k_ac_wsolve, float, true)
+WRFMI_INST_P2(k_ac_wsolve, double, true)
+WRFMI_INST_P2(k_ac_wsolve, float, false)
+WRFMI_INST_P2(k_ac_wsolve, double, false)
+WRFMI_INST(k_calc_p_rho)
 
 WRFMI_INST(k_small_prep)
 WRFMI_INST(k_p_rho0)

@@ -142,10 +142,12 @@ __global__ void __launch_bounds__(256) k_tend_u(ARGS args) {
   };
   // the vertical flux and dpn at the lower w level are carried up the column segment
   float vf_lo = (row_ok && k0 <= ktf && i >= iv0 && i <= iv1) ? vflux(k0) : 0.f;
-  float dpn_lo = (row_ok && k0 <= ktf && nh && i >= ih0 && i <= ih1) ? dpn(k0) : 0.f;
+  // advect_u alone (a per-routine entry point): added to the tendency array's values
+  const bool adv_only = a.routine == R_ADVECT;
+  float dpn_lo = (row_ok && k0 <= ktf && nh && !adv_only && i >= ih0 && i <= ih1) ? dpn(k0) : 0.f;
   for (int k = k0; k <= k1; k++) {
     const Off o = X3(d, i, k, j);
-    float t = 0.f;
+    float t = adv_only ? s.ru_tend[o] : 0.f;
     float vf_hi = 0.f, dpn_hi = 0.f;
     if (row_ok && k <= ktf) {
       // advect_u: y flux divergence
@@ -198,7 +200,7 @@ __global__ void __launch_bounds__(256) k_tend_u(ARGS args) {
         t = t - s.rdnw[X1(d, k)] * (vf_hi - vf_lo);
       }
       // horizontal_pressure_gradient (x)
-      if (i >= ih0 && i <= ih1) {
+      if (!adv_only && i >= ih0 && i <= ih1) {
         const float m = s.msfux[X2(d, i, j)] / s.msfuy[X2(d, i, j)];
         float dpx = m * .5f * rdx * (c1 * s.muu[X2(d, i, j)] + c2) *
                     ((ph[X3(d, i, k + 1, j)] - ph[X3(d, i - 1, k + 1, j)] + ph[o] - ph[X3(d, i - 1, k, j)]) +
@@ -214,21 +216,21 @@ __global__ void __launch_bounds__(256) k_tend_u(ARGS args) {
         }
       }
       // coriolis
-      if (i >= ic0 && i <= ic1) {
+      if (!adv_only && i >= ic0 && i <= ic1) {
         t = t + (s.msfux[X2(d, i, j)] / s.msfuy[X2(d, i, j)]) * 0.5f * (s.f[X2(d, i, j)] + s.f[X2(d, i - 1, j)]) * 0.25f *
                     (rv[X3(d, i - 1, k, j + 1)] + rv[X3(d, i, k, j + 1)] + rv[X3(d, i - 1, k, j)] + rv[X3(d, i, k, j)]) -
             0.5f * (s.e[X2(d, i, j)] + s.e[X2(d, i - 1, j)]) * 0.5f * (s.cosa[X2(d, i, j)] + s.cosa[X2(d, i - 1, j)]) * 0.25f *
                 (rw[X3(d, i - 1, k + 1, j)] + rw[X3(d, i - 1, k, j)] + rw[X3(d, i, k + 1, j)] + rw[X3(d, i, k, j)]);
       }
       // curvature
-      if (i >= ik0 && i <= ik1) {
+      if (!adv_only && i >= ik0 && i <= ik1) {
         t = t + 0.5f * (vx(i, k, j) + vx(i - 1, k, j)) * 0.25f *
                     (rv[X3(d, i - 1, k, j + 1)] + rv[X3(d, i, k, j + 1)] + rv[X3(d, i - 1, k, j)] + rv[X3(d, i, k, j)]) -
             u[o] * K_RERADIUS * 0.25f * (rw[X3(d, i - 1, k + 1, j)] + rw[X3(d, i - 1, k, j)] + rw[X3(d, i, k + 1, j)] + rw[X3(d, i, k, j)]);
       }
     }
     // rk_addtend_dry
-    if (row_ok && k <= kte - 1) {
+    if (!adv_only && row_ok && k <= kte - 1) {
       const float msf = s.msfuy[X2(d, i, j)];
       float tf = s.ru_tendf[o];
       if (a.rk_step == 1) {
@@ -547,10 +549,12 @@ __global__ void __launch_bounds__(256) k_tend_v(ARGS args) {
     return .5f * (s.fnm[X1(d, k)] * (p[X3(d, i, k, j - 1)] + p[X3(d, i, k, j)]) + s.fnp[X1(d, k)] * (p[X3(d, i, k - 1, j - 1)] + p[X3(d, i, k - 1, j)]));
   };
   float vf_lo = (col_ok && k0 <= ktf && j >= jv0 && j <= jv1) ? vflux(k0) : 0.f;
-  float dpn_lo = (col_ok && k0 <= ktf && nh && j >= jh0 && j <= jh1) ? dpn(k0) : 0.f;
+  // advect_v alone (a per-routine entry point): added to the tendency array's values
+  const bool adv_only = a.routine == R_ADVECT;
+  float dpn_lo = (col_ok && k0 <= ktf && nh && !adv_only && j >= jh0 && j <= jh1) ? dpn(k0) : 0.f;
   for (int k = k0; k <= k1; k++) {
     const Off o = X3(d, i, k, j);
-    float t = 0.f, vf_hi = 0.f, dpn_hi = 0.f;
+    float t = adv_only ? s.rv_tend[o] : 0.f, vf_hi = 0.f, dpn_hi = 0.f;
     if (col_ok && k <= ktf) {
       if (j >= jy0 && j <= jy1) {
         const float mrdy = s.msfvy[X2(d, i, j)] * rdy;
@@ -597,7 +601,7 @@ __global__ void __launch_bounds__(256) k_tend_v(ARGS args) {
         vf_hi = vflux(k + 1);
         t = t - (s.msfvy[X2(d, i, j)] / s.msfvx[X2(d, i, j)]) * s.rdnw[X1(d, k)] * (vf_hi - vf_lo);
       }
-      if (j >= jh0 && j <= jh1) {
+      if (!adv_only && j >= jh0 && j <= jh1) {
         const float m = s.msfvy[X2(d, i, j)] / s.msfvx[X2(d, i, j)];
         float dpy = m * .5f * rdy * (c1 * s.muv[X2(d, i, j)] + c2) *
                     ((ph[X3(d, i, k + 1, j)] - ph[X3(d, i, k + 1, j - 1)] + ph[o] - ph[X3(d, i, k, j - 1)]) +
@@ -612,7 +616,7 @@ __global__ void __launch_bounds__(256) k_tend_v(ARGS args) {
           t = t - dpy;
         }
       }
-      if (j >= jc0 && j <= jc1) {
+      if (!adv_only && j >= jc0 && j <= jc1) {
         const float m = s.msfvy[X2(d, i, j)] / s.msfvx[X2(d, i, j)];
         t = t - m * 0.5f * (s.f[X2(d, i, j)] + s.f[X2(d, i, j - 1)]) * 0.25f *
                     (ru[X3(d, i, k, j)] + ru[X3(d, i + 1, k, j)] + ru[X3(d, i, k, j - 1)] + ru[X3(d, i + 1, k, j - 1)]) +
@@ -623,7 +627,7 @@ __global__ void __launch_bounds__(256) k_tend_v(ARGS args) {
             m * v[o] * K_RERADIUS * 0.25f * (rw[X3(d, i, k + 1, j - 1)] + rw[X3(d, i, k, j - 1)] + rw[X3(d, i, k + 1, j)] + rw[X3(d, i, k, j)]);
       }
     }
-    if (col_ok && k <= kte - 1) {
+    if (!adv_only && col_ok && k <= kte - 1) {
       float tf = s.rv_tendf[o];
       if (a.rk_step == 1) {
         tf = tf + 0.f * s.msfvx[X2(d, i, j)];  // v_save was zeroed by rk_tendency
@@ -886,13 +890,15 @@ __global__ void __launch_bounds__(256) k_tend_w(ARGS args) {
     const float vel = 0.5f * (rom[X3(d, i, k, j)] + rom[X3(d, i, k - 1, j)]);
     return vel * flux3(w[X3(d, i, k - 2, j)], w[X3(d, i, k - 1, j)], w[X3(d, i, k, j)], w[X3(d, i, k + 1, j)], -vel, ts);
   };
+  // advect_w alone (a per-routine entry point): added to the tendency array's values
+  const bool adv_only = a.routine == R_ADVECT;
   const float msfty = mass ? s.msfty[X2(d, i, j)] : 1.f;
-  const float mu = mass ? s.mu_2[X2(d, i, j)] : 0.f, mub = mass ? s.mub[X2(d, i, j)] : 0.f;
+  const float mu = (mass && !adv_only) ? s.mu_2[X2(d, i, j)] : 0.f, mub = (mass && !adv_only) ? s.mub[X2(d, i, j)] : 0.f;
   const float mut = mass ? s.mut[X2(d, i, j)] : 0.f;
   float vf_c = (mass && imax(k0, kts + 1) <= kte) ? vflux(imax(k0, kts + 1)) : 0.f;  // vflux at level k
   for (int k = k0; k <= k1; k++) {
     const Off o = X3(d, i, k, j);
-    float t = 0.f;
+    float t = adv_only ? s.rw_tend[o] : 0.f;
     if (mass) {
       if (k >= kts + 1) {
         if (j >= jy0 && j <= jy1) {
@@ -963,7 +969,8 @@ __global__ void __launch_bounds__(256) k_tend_w(ARGS args) {
           t = t + 2.f * s.rdn[X1(d, k - 1)] * (vf_c);
         }
         // pg_buoy_w
-        if (k == kde) {
+        if (adv_only) {
+        } else if (k == kde) {
           // The top row reads the moisture coefficient cqw(kde-1) of calc_cq before the
           // thread of level kde-1 replaces it by 1/(1+cqw): recompute it as calc_cq does
           // (module_big_step_utilities_em.F:787-906) from the moist species.
@@ -989,14 +996,14 @@ __global__ void __launch_bounds__(256) k_tend_w(ARGS args) {
                        cq2 * (s.c1f[X1(d, k)] * mub + s.c2f[X1(d, k)]));
         }
         // w_damp
-        if (a.c.w_damping == 1 && k <= kde - 1) {
+        if (!adv_only && a.c.w_damping == 1 && k <= kde - 1) {
           const float w_damp_on = a.c.zadvect_implicit > 0 ? a.c.w_crit_cfl : K_W_BETA;
           const float cf = s.c1f[X1(d, k)] * mut + s.c2f[X1(d, k)];
           const float vert_cfl = fabsf(s.ww[o] / cf * s.rdnw[X1(d, k)] * a.dt);
           if (vert_cfl > w_damp_on) t = t - sgn1(w[o]) * K_W_ALPHA * (vert_cfl - a.c.w_crit_cfl) * cf;
         }
         // coriolis and curvature
-        if (k <= ktf) {
+        if (!adv_only && k <= ktf) {
           const float fm = fzm[X1(d, k)], fp = fzp[X1(d, k)];
           t = t + s.e[X2(d, i, j)] *
                       (s.cosa[X2(d, i, j)] * 0.5f * (fm * (ru[o] + ru[o + 1]) + fp * (ru[o - d.ni] + ru[o - d.ni + 1])) -
@@ -1009,12 +1016,14 @@ __global__ void __launch_bounds__(256) k_tend_w(ARGS args) {
         }
       }
       // rk_addtend_dry (rw and ph parts share this loop in the reference; ph is in k_tend_t)
-      float tf = s.rw_tendf[o];
-      if (a.rk_step == 1) {
-        tf = tf + 0.f * msfty;  // w_save was zeroed by rk_tendency
-        s.rw_tendf[o] = tf;
+      if (!adv_only) {
+        float tf = s.rw_tendf[o];
+        if (a.rk_step == 1) {
+          tf = tf + 0.f * msfty;  // w_save was zeroed by rk_tendency
+          s.rw_tendf[o] = tf;
+        }
+        t = t + tf / msfty;
       }
-      t = t + tf / msfty;
     }
     s.rw_tend[o] = t;
   }

@@ -13,6 +13,17 @@ constexpr int kVecTileI = 64 * kVecW;  // i points per wave in the vector kernel
 constexpr int kAdvVK = 4;      // levels per thread in k_adv_scalar_x4
 constexpr int kHaloK = 8;      // levels per thread in k_halo
 constexpr int kDivK = 4;       // levels per thread in k_ac_div_x4
+// The reference routine a launch computes on its own, for the per-routine entry points
+// with the reference's argument lists (KArgs::routine); R_FUSED is the whole step's form,
+// with each kernel's fused parts.
+enum Routine : int {
+  R_FUSED = 0,
+  R_ADVECT = 1,   // k_tend_u/v/w, k_adv_scalar: advect_u/v/w/scalar only, added to the tendency array's values
+  R_MU_T = 2,     // k_ac_div, k_ac_theta: advance_mu_t only (no sumflux; t_ave = theta before the update)
+  R_W_TAVE = 3,   // k_ac_theta: advance_w's time-averaged theta only (from t_2 and advance_mu_t's t_ave)
+  R_SUMFLUX = 4,  // k_ac_div, k_ac_theta: sumflux only
+  R_COEF_W = 5,   // k_coef_w: calc_coef_w only (no small_step_prep 2-D updates)
+};
 // Every state kernel is a template on its argument type: KArgs for one domain, BatchArgs
 // for an ensemble launch over the tiles of all members (k_halo: HaloArgs / BatchHalo).
 // ZONE = true is the one-point kernel's form over the boundary zone of a domain with an
@@ -40,7 +51,8 @@ template <typename A> __global__ void k_ac_div_x4(A);
 template <typename A> __global__ void k_small_prep_x4(A);
 template <typename A> __global__ void k_p_rho0_x4(A);
 template <typename A> __global__ void k_ac_theta_x4(A);
-template <typename A, typename CT> __global__ void k_ac_wsolve(A);
+template <typename A, typename CT, bool PRHO = true> __global__ void k_ac_wsolve(A);
+template <typename A> __global__ void k_calc_p_rho(A);
 template <typename A> __global__ void k_small_finish(A);
 template <typename A> __global__ void k_small_finish_x4(A);
 template <typename A, bool PER, bool ZONE = false> __global__ void k_adv_scalar(A);

@@ -13,7 +13,7 @@
 #include <mutex>
 #include <string>
 #include <vector>
-#include "kernels.h"
+#include "host_internal.h"
 
 
 using namespace wrfmi;
@@ -567,7 +567,8 @@ static void ph_small_finish(Launcher &L) {
     L.halo_flush();
   }
 }
-static void ph_scalar(Launcher &L) {
+// rk_scalar_tend's advect_scalar of species L.a.species into advect_tend
+static void ph_scalar_tend(Launcher &L) {
   const bool per = L.a.c.periodic_x && L.a.c.periodic_y;
   const Dm &d = L.a.d;
   const dim3 ga((d.ide - d.ids + 1 + kAdvTileI - 1) / kAdvTileI, (d.jde - d.jds + 1 + kAdvRows - 1) / kAdvRows,
@@ -580,6 +581,9 @@ static void ph_scalar(Launcher &L) {
   } else {
     L.launchK(KID_ADV_SCALAR, per ? KRP(k_adv_scalar, true) : KRP(k_adv_scalar, false), ga, dim3(64, kAdvRows, 1), L.a);
   }
+}
+static void ph_scalar(Launcher &L) {
+  ph_scalar_tend(L);
   if (L.vec) L.launchK(KID_UPDATE_SCALAR, KR(k_update_scalar_x4), g_vec3(L.a, L.a.d.kde - L.a.d.kds), kVecBlk, L.a);
   else L.launchK(KID_UPDATE_SCALAR, KR(k_update_scalar), g_pt3(L.a.d, L.a.d.kde - L.a.d.kds), kColBlk, L.a);
 }
@@ -682,6 +686,29 @@ static bool vec_layout(const wrfmi_state &s, const Dm &d, const KArgs &a) {
 
 static inline size_t field_elems(const Dm &d) { return (size_t)d.sj * (size_t)(d.jme - d.jms + 1); }
 
+// calc_coef_w's a, alpha and gamma hold 4-byte elements in precision mode 0 and 8-byte
+// elements in mode 1 (the column solve writes gamma at that width): each must lie in a device
+// allocation that reaches the end of the field at that width, or the kernels would write past
+// it.  hipMemGetAddressRange gives the allocation that holds the pointer.
+static int check_coef_arrays(const wrfmi_state &s, const wrfmi_config &c, const Dm &d) {
+  const size_t need = field_elems(d) * (c.precision_mode == 1 ? 8 : 4);
+  const void *arrs[3] = {s.a, s.alpha, s.gamma};
+  const char *names[3] = {"a", "alpha", "gamma"};
+  for (int n = 0; n < 3; n++) {
+    hipDeviceptr_t base = nullptr;
+    size_t size = 0;
+    if (hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)arrs[n]) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(WRFMI_E_POINTER, "%s is not in a device allocation", names[n]);
+    }
+    if ((uintptr_t)arrs[n] + need > (uintptr_t)base + size)
+      return fail(WRFMI_E_POINTER, "%s: its allocation ends %zu bytes after it; precision_mode %d needs %zu (%s elements)",
+                  names[n], (size_t)((uintptr_t)base + size - (uintptr_t)arrs[n]), c.precision_mode, need,
+                  c.precision_mode == 1 ? "8-byte" : "4-byte");
+  }
+  return 0;
+}
+
 // Validate and translate the arguments into L (g_mu held).  The work arrays are this
 // domain's own (get_scratch) unless own_scratch is given: ensemble members each have theirs,
 // and the allocation-free entry point passes the static array.  With own_scratch and no
@@ -697,6 +724,7 @@ static int prepare(const wrfmi_state *st_in, const wrfmi_config *cfg, const wrfm
   L.a.s = s;
   L.a.c = *cfg;
   L.a.d = d;
+  if ((rc = check_coef_arrays(s, *cfg, d))) return rc;
   const DomainKey key((uintptr_t)s.u_2, field_elems(d));
   Scratch sc;
   if (own_scratch) sc = *own_scratch;
@@ -1173,7 +1201,7 @@ extern "C" int wrfmi_small_step_prep(wrfmi_state *st, const wrfmi_config *cfg, c
     return 0;
   });
 }
-extern "C" int wrfmi_advance_uv(wrfmi_state *st, const wrfmi_config *cfg, const wrfmi_dims *d, float dts, void *stream) {
+extern "C" int wrfmi_advance_uv_state(wrfmi_state *st, const wrfmi_config *cfg, const wrfmi_dims *d, float dts, void *stream) {
   return run_phase(st, cfg, d, stream, [&](Launcher &L) {
     L.a.dts = dts;
     ph_advance_uv(L);
@@ -1211,6 +1239,17 @@ extern "C" int wrfmi_rk_scalar_step(wrfmi_state *st, const wrfmi_config *cfg, co
     L.a.species = ispec;
     L.a.dt_rk = dt_rk;
     ph_scalar(L);
+    return 0;
+  });
+}
+extern "C" int wrfmi_rk_scalar_tend(wrfmi_state *st, const wrfmi_config *cfg, const wrfmi_dims *d, int ispec, int rk_step,
+                                    void *stream) {
+  return run_phase(st, cfg, d, stream, [&](Launcher &L) {
+    if (int rc = check_rk(rk_step)) return rc;
+    if (ispec < 0 || ispec >= L.a.s.n_species) return fail(WRFMI_E_BADOPT, "species %d outside 0..%d", ispec, L.a.s.n_species - 1);
+    L.a.rk_step = rk_step;
+    L.a.species = ispec;
+    ph_scalar_tend(L);
     return 0;
   });
 }
@@ -1341,3 +1380,51 @@ extern "C" int wrfmi_timing_get(int id, long long *launches, double *total_ms) {
   *total_ms = g_t_ms[id];
   return 0;
 }
+
+// ------------------------------------------------------------------ services of host_internal.h
+// for the entry points with the reference's argument lists (wrfmi_routines.hip)
+static thread_local hipStream_t t_routine_stream = nullptr;
+extern "C" int wrfmi_set_stream(void *stream) {
+  t_routine_stream = (hipStream_t)stream;
+  return 0;
+}
+namespace wrfmi {
+namespace host {
+int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+std::mutex &lock() { return g_mu; }
+void *translate(void *p) { return ::translate(p); }
+int check_cfg(const wrfmi_config *c) { return ::check_cfg(c); }
+int make_dm(const wrfmi_dims *w, Dm *d) { return ::make_dm(w, d); }
+int domain_scratch(const void *key, const Dm &d, float **dv, float **rhs, float **lo) {
+  Scratch sc;
+  if (int rc = get_scratch(DomainKey((uintptr_t)key, field_elems(d)), &sc)) return rc;
+  *dv = sc.dv;
+  *rhs = sc.rhs;
+  *lo = sc.lo;
+  return 0;
+}
+int device_array(const void *p, const char *name) {
+  hipPointerAttribute_t at;
+  if (hipPointerGetAttributes(&at, p) != hipSuccess || at.type != hipMemoryTypeDevice) {
+    (void)hipGetLastError();
+    return wrfmi::host::fail(WRFMI_E_POINTER, "argument %s is neither a device array nor a registered host array", name);
+  }
+  return 0;
+}
+hipStream_t routine_stream() { return t_routine_stream; }
+void launch_tiles(void (*kern)(KArgs), dim3 g, dim3 b, KArgs a, hipStream_t st) {
+  a.tnx = (int)g.x;
+  a.tny = (int)g.y;
+  a.tnz = (int)g.z;
+  hipLaunchKernelGGL(kern, dim3(g.x * g.y * g.z, 1, 1), b, 0, st, a);
+}
+}  // namespace host
+}  // namespace wrfmi

@@ -40,10 +40,11 @@ PHASE_ARGS = {
     "rk_step_prep": "i",          # rk_step
     "rk_tendency": "i",           # rk_step
     "small_step_prep": "if",      # rk_step, dts
-    "advance_uv": "f",            # dts
+    "advance_uv_state": "f",      # dts
     "advance_mu_t_w": "fii",      # dts, iteration, n_small
     "small_step_finish": "iif",   # rk_step, n_small, dts
     "rk_scalar_step": "iif",      # ispec, rk_step, dt_rk
+    "rk_scalar_tend": "ii",       # ispec, rk_step
     "calc_p_rho_phi": "",
     "phys_bc_dry_2": "i",         # kind
     "set_w_surface": "",
@@ -77,7 +78,7 @@ def step_phases(cfg, scal: dict, n_species: int):
         out.append(("rk_tendency", (rk,)))
         out.append(("small_step_prep", (rk, float(dts))))
         for it in range(1, nsmall + 1):
-            out.append(("advance_uv", (float(dts),)))
+            out.append(("advance_uv_state", (float(dts),)))
             out.append(("advance_mu_t_w", (float(dts), it, nsmall)))
         out.append(("small_step_finish", (rk, nsmall, float(dts))))
         for m in range(n_species):
@@ -230,6 +231,10 @@ class DeviceDomain:
         fn = getattr(self.lib, "wrfmi_" + name)
         h = self.stream_handle(stream)
         check(fn(ctypes.byref(self.state), ctypes.byref(self.cfg), ctypes.byref(self.dims), *args, ctypes.c_void_p(h)))
+
+    def addr(self, name: str) -> int:
+        """Device address of the field `name`."""
+        return self.t[name].data_ptr()
 
     def upload(self, arrays: dict):
         """Overwrite device fields with host arrays of the same names and shapes."""
