@@ -262,6 +262,16 @@ int wrfmi_timing_get(int id, long long *launches, double *total_ms);
  * work queued on the device before it scans. */
 int wrfmi_check_finite(const wrfmi_state *st, const wrfmi_dims *d, long long *n_bad);
 
+/* Domain totals in FP64 (diagnostic; no counterpart routine in the reference):
+ * out[0] dry-air mass [kg], out[1] theta budget sum(dm * theta) [kg K], out[2] kinetic,
+ * out[3] internal, out[4] potential and out[5] total energy [J] of the mass columns
+ * i = ids..ide-1, j = jds..jde-1, with layer dry mass dm = (c1h*mut + c2h)*(-dnw)/g times the
+ * cell area dx*dy/(msftx*msfty).  FP64 accumulation throughout, except the theta budget's
+ * column sums, which are FP32 with compensated (two-sum) accumulation; reductions run in a
+ * fixed order (reproducible).  Enqueued on `stream`, then waits for it. */
+int wrfmi_domain_totals(const wrfmi_state *st, const wrfmi_config *cfg, const wrfmi_dims *d,
+                        void *stream, double *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -203,7 +203,45 @@ def coef_width_check():
     sys.exit(0 if ok else 1)
 
 
+def totals_check():
+    """wrfmi_domain_totals against the same sums in float64 numpy, before and after two
+    steps of a doubly periodic domain, over which dry mass and the theta budget are
+    conserved (no diabatic heating, flux-form transport)."""
+    import ctypes
+    from parity import domain_totals_numpy
+    hs = grid.build_initial_state(grid.DomainSpec(nx=18, ny=14, nz=10, bc="periodic", n_passive=1, n_bubbles=2,
+                                                  bubble_amp=2.0, seed=43))
+    emu = EmuDomain(hs, order=0)
+    emu.lib.wrfmi_domain_totals.argtypes = [ctypes.c_void_p] * 5
+    fails = []
+    res = []
+    for stage in range(2):
+        out = (ctypes.c_double * 6)()
+        emu._chk(emu.lib.wrfmi_domain_totals(ctypes.byref(emu.state), ctypes.byref(emu.cfg), ctypes.byref(emu.dims),
+                                             None, out))
+        got = np.array(out[:])
+        exp = domain_totals_numpy(emu.download(), hs)
+        tol = np.array([1e-12, 1e-6, 1e-12, 1e-12, 1e-12, 1e-12])
+        rel = np.abs(got - exp) / np.abs(exp)
+        if np.any(rel > tol):
+            fails.append(f"stage {stage}: relative differences {rel} exceed {tol}")
+        res.append(got)
+        emu.step(2)
+    drift_mass = abs(res[1][0] - res[0][0]) / res[0][0]
+    drift_theta = abs(res[1][1] - res[0][1]) / res[0][1]
+    if drift_mass > 1e-6 or drift_theta > 1e-5:
+        fails.append(f"periodic drifts: dry mass {drift_mass:.2e}, theta budget {drift_theta:.2e}")
+    for f in fails:
+        print("MISMATCH", f)
+    print("OK totals" if not fails else "FAILED totals", f"drift mass {drift_mass:.2e} theta {drift_theta:.2e}",
+          flush=True)
+    sys.exit(1 if fails else 0)
+
+
 def main():
+    if sys.argv[1] == "totals":
+        totals_check()
+        return
     if sys.argv[1] == "coefwidth":
         coef_width_check()
         return

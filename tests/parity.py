@@ -75,3 +75,35 @@ def phase_parity(backend, session, phases, nsteps: int = 1, stop_at_first: bool 
                 if stop_at_first:
                     return fails
     return fails
+
+
+def domain_totals_numpy(arr: dict, hs) -> np.ndarray:
+    """The domain totals of wrfmi_domain_totals (dry mass, theta budget, kinetic, internal,
+    potential and total energy) computed in float64 with numpy from the host arrays `arr`
+    (the state's fields by name) of a domain with dims and scalars of `hs`."""
+    d = hs.dims
+    g, rd = 9.81, 287.0
+    cp = 7.0 * rd / 2.0
+    cv = cp - rd
+    i0, j0 = d.ids - d.ims, d.jds - d.jms
+    nx, ny, nk = d.ide - d.ids, d.jde - d.jds, d.kde - 1
+    f = {k: np.asarray(v, np.float64) for k, v in arr.items()}
+    col = lambda a: a[j0:j0 + ny, i0:i0 + nx]  # noqa: E731
+    lev = lambda a, dk=0, di=0, dj=0: a[j0 + dj:j0 + dj + ny, dk:dk + nk, i0 + di:i0 + di + nx]  # noqa: E731
+    mut = col(f["mub"]) + col(f["mu_2"])
+    area = hs.scal["dx"] * hs.scal["dy"] / (col(f["msftx"]) * col(f["msfty"]))
+    c1h, c2h, dnw = f["c1h"][:nk], f["c2h"][:nk], f["dnw"][:nk]
+    dm = (c1h[None, :, None] * mut[:, None, :] + c2h[None, :, None]) * (-dnw[None, :, None]) / g
+    theta = lev(f["t_2"]) + 300.0
+    thd = theta / (1.0 + (461.6 / 287.0) * lev(f["q0"])) if (hs.cfg.use_theta_m and hs.n_moist >= 1) else theta
+    temp = thd * ((lev(f["p"]) + lev(f["pb"])) / 1.0e5) ** (rd / cp)
+    ub = 0.5 * (lev(f["u_2"]) + lev(f["u_2"], di=1))
+    vb = 0.5 * (lev(f["v_2"]) + lev(f["v_2"], dj=1))
+    wb = 0.5 * (lev(f["w_2"]) + lev(f["w_2"], dk=1))
+    phi = f["ph_2"] + f["phb"]
+    pe = dm * 0.5 * (lev(phi) + lev(phi, dk=1))
+    a3 = area[:, None, :]
+    out = np.array([np.sum(dm * a3), np.sum(dm * theta * a3), np.sum(dm * 0.5 * (ub ** 2 + vb ** 2 + wb ** 2) * a3),
+                    np.sum(dm * cv * temp * a3), np.sum(pe * a3), 0.0])
+    out[5] = out[2] + out[3] + out[4]
+    return out

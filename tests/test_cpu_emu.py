@@ -78,3 +78,12 @@ def test_emulated_coefficient_width_checked(emu_env):
                        env=emu_env, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout[-4000:] + r.stderr[-4000:]
     assert "OK coefwidth" in r.stdout
+
+
+def test_emulated_domain_totals(emu_env):
+    """wrfmi_domain_totals against float64 numpy sums, and conservation of dry mass and the
+    theta budget over two steps of a doubly periodic domain."""
+    r = subprocess.run([sys.executable, os.path.join(HERE, "emu_phase_check.py"), "totals"],
+                       env=emu_env, capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stdout[-4000:] + r.stderr[-4000:]
+    assert "OK totals" in r.stdout

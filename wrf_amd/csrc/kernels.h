@@ -66,6 +66,10 @@ template <typename A> __global__ void k_p_rho_phi_x4(A);
 template <typename A> __global__ void k_halo(A);
 template <typename A> __global__ void k_set_w_surface(A);
 __global__ void k_count_nonfinite(const float *a, Dm d, int kend, int is3d, unsigned long long *cnt);
+// domain totals (k_diag.hip): dry mass, theta budget, kinetic, internal, potential, total
+constexpr int kNTotals = 6;
+__global__ void k_domain_totals(KArgs a, double *part);
+__global__ void k_domain_totals_final(const double *part, int nblocks, double *out);
 }  // namespace wrfmi
 
 // explicit instantiations, in the source file that defines the kernel

@@ -181,6 +181,8 @@ static void free_scratch() { free_scratch_in(0, ~(uintptr_t)0); }
 // ------------------------------------------------------------------ library
 static hipStream_t g_stream = nullptr;
 static unsigned long long *g_counter = nullptr;
+static double *g_diag = nullptr;  // wrfmi_domain_totals' partial sums (g_mu)
+static size_t g_diag_bytes = 0;
 static int noalloc_scratch(float **out);
 
 extern "C" int wrfmi_init(void) {
@@ -202,6 +204,7 @@ extern "C" int wrfmi_finalize(void) {
   for (auto &kv : g_reg) (void)hipFree(kv.second.dev);
   g_reg.clear();
   if (g_counter) { (void)hipFree(g_counter); g_counter = nullptr; }
+  if (g_diag) { (void)hipFree(g_diag); g_diag = nullptr; g_diag_bytes = 0; }
   if (g_stream) { (void)hipStreamDestroy(g_stream); g_stream = nullptr; }
   return 0;
 }
@@ -1345,6 +1348,42 @@ extern "C" int wrfmi_check_finite(const wrfmi_state *st_in, const wrfmi_dims *di
   HIPCHK(hipMemcpy(&cnt, g_counter, sizeof cnt, hipMemcpyDeviceToHost));
   *n_bad = (long long)cnt;
   return cnt ? fail(WRFMI_E_NONFINITE, "%llu non-finite values in the prognostic state", cnt) : 0;
+}
+
+// ------------------------------------------------------------------ domain totals (FP64)
+
+extern "C" int wrfmi_domain_totals(const wrfmi_state *st_in, const wrfmi_config *cfg, const wrfmi_dims *dims,
+                                   void *stream, double *out) {
+  if (!st_in || !cfg || !dims || !out) return fail(WRFMI_E_POINTER, "null argument");
+  std::lock_guard<std::mutex> lk(g_mu);
+  int rc;
+  Dm d;
+  wrfmi_state s;
+  if ((rc = make_dm(dims, &d))) return rc;
+  if ((rc = translate_state(st_in, &s))) return rc;
+  const int ncol = (d.ide - d.ids) * (d.jde - d.jds);
+  const int nblocks = (ncol + 255) / 256;
+  const size_t need = (size_t)(nblocks + 1) * kNTotals * sizeof(double);
+  if (g_diag_bytes < need) {
+    if (g_diag) HIPCHK(hipFree(g_diag));
+    g_diag = nullptr;
+    g_diag_bytes = 0;
+    HIPCHK(hipMalloc(&g_diag, need));
+    g_diag_bytes = need;
+  }
+  KArgs a;
+  memset(&a, 0, sizeof a);
+  a.s = s;
+  a.c = *cfg;
+  a.d = d;
+  const hipStream_t hs = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_domain_totals, dim3(nblocks, 1, 1), dim3(256, 1, 1), 0, hs, a, g_diag + kNTotals);
+  hipLaunchKernelGGL(k_domain_totals_final, dim3(1, 1, 1), dim3(256, 1, 1), 0, hs, (const double *)(g_diag + kNTotals),
+                     nblocks, g_diag);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, g_diag, kNTotals * sizeof(double), hipMemcpyDeviceToHost, hs));
+  HIPCHK(hipStreamSynchronize(hs));
+  return 0;
 }
 
 // ------------------------------------------------------------------ kernel timing entry points

@@ -131,6 +131,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.wrfmi_ensemble_destroy.argtypes = [vp]
     lib.wrfmi_check_finite.argtypes = [vp, vp, ctypes.POINTER(ctypes.c_longlong)]
     lib.wrfmi_domain_release.argtypes = [vp]
+    lib.wrfmi_domain_totals.argtypes = [vp, vp, vp, vp, ctypes.POINTER(ctypes.c_double)]
     lib.wrfmi_kernel_count.restype = ctypes.c_int
     lib.wrfmi_kernel_name.restype = ctypes.c_char_p
     lib.wrfmi_kernel_name.argtypes = [ctypes.c_int]
@@ -251,6 +252,15 @@ class DeviceDomain:
         """Free the library's work arrays and cached graphs of this domain (it may be
         stepped again afterwards: they are allocated again on the next call)."""
         check(self.lib.wrfmi_domain_release(ctypes.byref(self.state)))
+
+    TOTALS = ("dry_mass_kg", "theta_budget_kg_K", "kinetic_J", "internal_J", "potential_J", "total_energy_J")
+
+    def domain_totals(self, stream=None) -> dict:
+        """FP64 domain totals of the current state (wrfmi_domain_totals)."""
+        out = (ctypes.c_double * 6)()
+        check(self.lib.wrfmi_domain_totals(ctypes.byref(self.state), ctypes.byref(self.cfg), ctypes.byref(self.dims),
+                                           ctypes.c_void_p(self.stream_handle(stream)), out))
+        return dict(zip(self.TOTALS, out[:]))
 
     def check_finite(self) -> int:
         nbad = ctypes.c_longlong(0)
