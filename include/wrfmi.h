@@ -243,6 +243,22 @@ int wrfmi_set_physical_bc3d(float *field, char var, const wrfmi_config *cfg,
 int wrfmi_set_physical_bc2d(float *field, char var, const wrfmi_config *cfg,
                             const wrfmi_dims *d, void *stream);
 
+/* ---- launch shapes (no counterpart in the reference, whose tile decomposition is
+ * frame/module_tiles.F:200-460) ----------------------------------------------------
+ * Rows per workgroup of the four-point and column kernels, levels per thread of the
+ * four-point advance_uv and column divergence (k_advance_uv, k_ac_div: 2, 4 or 8), and the
+ * column solve's prefetch depth (k_ac_wsolve: 2, 4 or 8), per kernel (the names of
+ * wrfmi_kernel_name) and domain shape in mass points (nx, ny, nz; 0 matches any).  Every
+ * shape gives the same results bit for bit; wrf_amd/autotune.py measures the candidates and
+ * writes the table.  wrfmi_init loads $WRFMI_TUNING_FILE, else wrfmi_tuning.txt beside the
+ * library if present (lines "kernel nx ny nz rows levels prefetch").  A later entry takes
+ * precedence; ensembles use the default shapes.  A step already captured in a graph keeps
+ * the shapes it was captured with. */
+int wrfmi_tuning_set(const char *kernel, int nx, int ny, int nz, int rows, int levels, int prefetch);
+int wrfmi_tuning_clear(void);
+int wrfmi_tuning_load(const char *path);
+int wrfmi_tuning_get(const char *kernel, int nx, int ny, int nz, int *rows, int *levels, int *prefetch);
+
 /* ---- kernel timing (diagnostic; no counterpart in the reference) ----------
  * While enabled, every launch of a selected kernel by the eager entry points
  * above is bracketed by a HIP event pair on its own stream; the graph entry

@@ -238,7 +238,57 @@ def totals_check():
     sys.exit(1 if fails else 0)
 
 
+def shapes_check(case: str):
+    """Tile-shape invariance: with every kernel's rows per workgroup at 2 and at 4, and the
+    levels per thread and prefetch depth at each tunable value, one step is bit for bit the
+    oracle's (the launch shapes the autotuner chooses among)."""
+    import ctypes
+    kernels = ["k_step_prep", "k_ww_scan", "k_tend_u", "k_tend_v", "k_tend_w", "k_tend_t", "k_small_prep", "k_p_rho0",
+               "k_coef_w", "k_advance_uv", "k_ac_div", "k_ac_mu_ww", "k_ac_theta", "k_ac_wsolve", "k_small_finish",
+               "k_adv_scalar", "k_update_scalar", "k_p_rho_phi"]
+    default = {k: (1, 1, 0) for k in kernels}
+    default.update({"k_tend_t": (4, 1, 0), "k_adv_scalar": (4, 1, 0), "k_advance_uv": (1, 4, 0), "k_ac_div": (1, 4, 0),
+                    "k_ac_wsolve": (1, 1, 4)})
+    sets = [("rows 2", {k: (2, v[1], v[2]) for k, v in default.items()}),
+            ("rows 4", {k: (4, v[1], v[2]) for k, v in default.items()})]
+    for lv in (2, 8):
+        sets.append((f"levels {lv}", {"k_advance_uv": (1, lv, 0), "k_ac_div": (2, lv, 0)}))
+    for pf in (2, 8):
+        sets.append((f"prefetch {pf}", {"k_ac_wsolve": (2, 1, pf)}))
+    hs = grid.build_initial_state(grid.DomainSpec(**CASES[case]))
+    exp = Oracle("f32cr").run(hs, 1)
+    fails = []
+    for label, table in sets:
+        emu = EmuDomain(hs, order=1)
+        lib = emu.lib
+        lib.wrfmi_tuning_set.argtypes = [ctypes.c_char_p] + [ctypes.c_int] * 6
+        lib.wrfmi_tuning_clear()
+        for k, (r, lv, pf) in table.items():
+            emu._chk(lib.wrfmi_tuning_set(k.encode(), 0, 0, 0, r, lv, pf))
+        emu.step(1)
+        got = emu.download()
+        lib.wrfmi_tuning_clear()
+        bad = [k for k in exp if k not in ("a", "alpha") and not np.array_equal(got[k], exp[k])]
+        if bad:
+            fails.append(f"{label}: differs from the oracle in {bad}")
+        del emu
+    # shapes outside the offered set are refused
+    import ctypes as C
+    lib = C.CDLL(os.path.join(HERE, "cpu_emu", "libwrfmi_emu.so"))
+    lib.wrfmi_tuning_set.argtypes = [C.c_char_p] + [C.c_int] * 6
+    if lib.wrfmi_tuning_set(b"k_ac_wsolve", 0, 0, 0, 3, 1, 4) != 1 or lib.wrfmi_tuning_set(b"k_tend_u", 0, 0, 0, 1, 2, 0) != 1 \
+            or lib.wrfmi_tuning_set(b"k_nonexistent", 0, 0, 0, 1, 1, 0) != 1:
+        fails.append("a shape outside the offered set was accepted")
+    for f in fails:
+        print("MISMATCH", f)
+    print("OK shapes" if not fails else "FAILED shapes", case, flush=True)
+    sys.exit(1 if fails else 0)
+
+
 def main():
+    if sys.argv[1] == "shapes":
+        shapes_check(sys.argv[2])
+        return
     if sys.argv[1] == "totals":
         totals_check()
         return

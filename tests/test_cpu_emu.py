@@ -87,3 +87,14 @@ def test_emulated_domain_totals(emu_env):
                        env=emu_env, capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stdout[-4000:] + r.stderr[-4000:]
     assert "OK totals" in r.stdout
+
+
+@pytest.mark.parametrize("case", ["periodic", "open"])
+def test_emulated_launch_shapes_give_identical_results(emu_env, case):
+    """Tile-shape invariance: every launch shape the autotuner chooses among (rows per
+    workgroup, levels per thread, prefetch depth) gives one step bit for bit equal to the
+    oracle's; shapes outside the offered set are refused."""
+    r = subprocess.run([sys.executable, os.path.join(HERE, "emu_phase_check.py"), "shapes", case],
+                       env=emu_env, capture_output=True, text=True, timeout=1500)
+    assert r.returncode == 0, r.stdout[-4000:] + r.stderr[-4000:]
+    assert "OK shapes" in r.stdout

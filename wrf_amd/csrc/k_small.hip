@@ -343,7 +343,7 @@ __global__ void __launch_bounds__(256) k_advance_uv(ARGS args) {
 // advance_uv with four i points per lane (16-byte loads): the same expressions as
 // k_advance_uv element by element.  Values at i-1 come from the group of four below (o-4),
 // values at j-1 from the row below; p and ph at level k+1 are carried to the next level.
-template <typename ARGS, bool PER>
+template <typename ARGS, bool PER, int KS>
 __global__ void __launch_bounds__(256) k_advance_uv_x4(ARGS args) {
   Tile3 T;
   const KArgs &a = kargs(args, T);
@@ -351,11 +351,11 @@ __global__ void __launch_bounds__(256) k_advance_uv_x4(ARGS args) {
   const DState &s = dstate(a);
   const int i0 = vec_ia0(d) + kVecW * (T.bx * 64 + (int)threadIdx.x);
   const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
-  const int k0 = d.kds + (int)T.bz * kAdvUvK;
+  const int k0 = d.kds + (int)T.bz * KS;
   const int ids = d.ids, ide = d.ide, jds = d.jds, jde = d.jde, kde = d.kde;
   const int its = ids, ite = ide, jts = jds, jte = jde;
   if (i0 > ide || j > jde || k0 > kde - 1) return;
-  const int k1 = imin(k0 + kAdvUvK - 1, kde - 1);
+  const int k1 = imin(k0 + KS - 1, kde - 1);
   const wrfmi_config &c = a.c;
   const Geo g = geo<PER>(c);
   const int i_start = its, i_end = imin(ite, ide - 1), j_start = jts, j_end = imin(jte, jde - 1), i_endu = ite, j_endv = jte;
@@ -899,7 +899,7 @@ __global__ void __launch_bounds__(256) k_p_rho0_x4(ARGS args) {
 // the same expressions in the same order as the one-point kernels above; i neighbours come
 // from the groups of four on either side.
 
-template <typename ARGS>
+template <typename ARGS, int KS>
 __global__ void __launch_bounds__(256) k_ac_div_x4(ARGS args) {
   Tile3 T;
   const KArgs &a = kargs(args, T);
@@ -907,17 +907,17 @@ __global__ void __launch_bounds__(256) k_ac_div_x4(ARGS args) {
   const DState &s = dstate(a);
   const int i0 = vec_ia0(d) + kVecW * (T.bx * 64 + (int)threadIdx.x);
   const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
-  const int k0 = d.kds + (int)T.bz * kDivK;
+  const int k0 = d.kds + (int)T.bz * KS;
   const int ids = d.ids, ide = d.ide, jde = d.jde, kde = d.kde;
   if (i0 > ide || j > jde || k0 > kde) return;
-  const int k1 = imin(k0 + kDivK - 1, kde);
+  const int k1 = imin(k0 + KS - 1, kde);
   const Off o2 = X2(d, i0, j), ob = X3(d, i0, 0, j);  // level k at ob + k*ni
   const int ni = d.ni;
   const bool mj = j <= jde - 1;
   const int it = a.iteration, ns = a.nsmall;
   const float nsf = (float)ns;
   const float rdx = s.rdx, rdy = s.rdy;
-  // the column's 2-D fields, loaded once for its kDivK levels
+  // the column's 2-D fields, loaded once for its KS levels
   const F4 muu = s.muu.ld<F4>(o2), msfuy = s.msfuy.ld<F4>(o2), muv = s.muv.ld<F4>(o2), mvi = s.msfvx_inv.ld<F4>(o2);
   const F4 muur = s.muu.ld<F4>(o2 + 4), msfuyr = s.msfuy.ld<F4>(o2 + 4);
   const F4 muvn = s.muv.ld<F4>(o2 + ni), mvin = s.msfvx_inv.ld<F4>(o2 + ni);
@@ -1038,10 +1038,7 @@ __global__ void __launch_bounds__(256) k_ac_theta_x4(ARGS args) {
 // (438-568).  CT = double keeps the forward value in FP64, its low-order part parked in scr_lo.
 // levels of loads in flight ahead of each march of k_ac_wsolve (2, 4, 6 and 8 time within
 // 2 % of each other at 400x400x80: the kernel is bound by the bytes it moves)
-#ifndef WRFMI_WSOLVE_PF
-#define WRFMI_WSOLVE_PF 4
-#endif
-template <typename ARGS, typename CT, bool PRHO>
+template <typename ARGS, typename CT, bool PRHO, int PF>
 __global__ void __launch_bounds__(256) k_ac_wsolve(ARGS args) {
   Tile3 T;
   const KArgs &a = kargs(args, T);
@@ -1126,7 +1123,7 @@ __global__ void __launch_bounds__(256) k_ac_wsolve(ARGS args) {
   auto rdnwC = [&](int k) -> CT { return (CT)rdnw[X1(d, k)]; };
   auto rdnC = [&](int k) -> CT { return (CT)rdn[X1(d, k)]; };
   CT gam = CT(0);
-  column_march_up<11, float, WRFMI_WSOLVE_PF>(
+  column_march_up<11, float, PF>(
       2, k_end + 1,
       [&](int f, int k) -> float {
         const Off o = ob + k * ni, op = ob + imin(k + 1, kde) * ni, opp = ob + imin(k + 2, kde) * ni;
@@ -1216,7 +1213,7 @@ __global__ void __launch_bounds__(256) k_ac_wsolve(ARGS args) {
   const float smdiv = c.smdiv;
   CT wup = wfe;
   float ph_up = 0.f;  // final ph one level above
-  column_march_down<8, CT, WRFMI_WSOLVE_PF>(
+  column_march_down<8, CT, PF>(
       1, k_end + 1,
       [&](int f, int k) -> CT {
         const Off o = ob + k * ni;
@@ -1330,25 +1327,42 @@ __global__ void __launch_bounds__(256) k_small_finish(ARGS args) {
 
 WRFMI_INST_P(k_advance_uv, false)
 WRFMI_INST_P(k_advance_uv, true)
-WRFMI_INST_P(k_advance_uv_x4, false)
-WRFMI_INST_P(k_advance_uv_x4, true)
+// four-point advance_uv and the column divergence: 2, 4 or 8 levels per thread (a launch
+// shape the autotuner chooses); the column solve: prefetch depth 2, 4 or 8
+#define WRFMI_INST_UV(P, KS) \
+  template __global__ void k_advance_uv_x4<KArgs, P, KS>(KArgs); \
+  template __global__ void k_advance_uv_x4<BatchArgs, P, KS>(BatchArgs);
+WRFMI_INST_UV(false, 2)
+WRFMI_INST_UV(false, 4)
+WRFMI_INST_UV(false, 8)
+WRFMI_INST_UV(true, 2)
+WRFMI_INST_UV(true, 4)
+WRFMI_INST_UV(true, 8)
 WRFMI_INST_P2(k_coef_w, float, true)
 WRFMI_INST_P2(k_coef_w, double, true)
 WRFMI_INST_P2(k_coef_w, float, false)
 WRFMI_INST_P2(k_coef_w, double, false)
 WRFMI_INST_P(k_ac_mu_ww, false)
 WRFMI_INST_P(k_ac_mu_ww, true)
-WRFMI_INST_P2(k_ac_wsolve, float, true)
-WRFMI_INST_P2(k_ac_wsolve, double, true)
-WRFMI_INST_P2(k_ac_wsolve, float, false)
-WRFMI_INST_P2(k_ac_wsolve, double, false)
+#define WRFMI_INST_WS(CT, PRHO, PF) \
+  template __global__ void k_ac_wsolve<KArgs, CT, PRHO, PF>(KArgs); \
+  template __global__ void k_ac_wsolve<BatchArgs, CT, PRHO, PF>(BatchArgs);
+WRFMI_INST_WS(float, true, 2)
+WRFMI_INST_WS(float, true, 4)
+WRFMI_INST_WS(float, true, 8)
+WRFMI_INST_WS(double, true, 2)
+WRFMI_INST_WS(double, true, 4)
+WRFMI_INST_WS(double, true, 8)
+WRFMI_INST_WS(float, false, 4)
 WRFMI_INST(k_calc_p_rho)
 
 WRFMI_INST(k_small_prep)
 WRFMI_INST(k_p_rho0)
 WRFMI_INST(k_ac_div)
 WRFMI_INST(k_ac_theta)
-WRFMI_INST(k_ac_div_x4)
+WRFMI_INST_P(k_ac_div_x4, 2)
+WRFMI_INST_P(k_ac_div_x4, 4)
+WRFMI_INST_P(k_ac_div_x4, 8)
 WRFMI_INST(k_small_prep_x4)
 WRFMI_INST(k_p_rho0_x4)
 WRFMI_INST(k_ac_theta_x4)

@@ -3,7 +3,7 @@
 #include "dev_common.h"
 
 namespace wrfmi {
-constexpr int kAdvUvK = 4;  // levels per thread in k_advance_uv
+constexpr int kAdvUvK = 4;  // levels per thread in k_advance_uv (the four-point form: a launch shape)
 constexpr int kAdvK = 8;     // levels per block in k_adv_scalar
 constexpr int kTendK = 8;    // levels per thread in k_tend_u/v/w/t
 constexpr int kAdvTileI = 63;  // i points per wave in k_adv_scalar (lane 63 supplies a face)
@@ -12,7 +12,7 @@ constexpr int kVecW = 4;       // i points per lane in the vector kernels (16-by
 constexpr int kVecTileI = 64 * kVecW;  // i points per wave in the vector kernels
 constexpr int kAdvVK = 4;      // levels per thread in k_adv_scalar_x4
 constexpr int kHaloK = 8;      // levels per thread in k_halo
-constexpr int kDivK = 4;       // levels per thread in k_ac_div_x4
+constexpr int kDivK = 4;       // default levels per thread in k_ac_div_x4 (a launch shape)
 // The reference routine a launch computes on its own, for the per-routine entry points
 // with the reference's argument lists (KArgs::routine); R_FUSED is the whole step's form,
 // with each kernel's fused parts.
@@ -43,15 +43,15 @@ template <typename A> __global__ void k_small_prep(A);
 template <typename A> __global__ void k_p_rho0(A);
 template <typename A, typename CT, bool COEF> __global__ void k_coef_w(A);
 template <typename A, bool PER> __global__ void k_advance_uv(A);
-template <typename A, bool PER> __global__ void k_advance_uv_x4(A);
+template <typename A, bool PER, int KS> __global__ void k_advance_uv_x4(A);
 template <typename A> __global__ void k_ac_div(A);
 template <typename A, bool COMP> __global__ void k_ac_mu_ww(A);
 template <typename A> __global__ void k_ac_theta(A);
-template <typename A> __global__ void k_ac_div_x4(A);
+template <typename A, int KS> __global__ void k_ac_div_x4(A);
 template <typename A> __global__ void k_small_prep_x4(A);
 template <typename A> __global__ void k_p_rho0_x4(A);
 template <typename A> __global__ void k_ac_theta_x4(A);
-template <typename A, typename CT, bool PRHO = true> __global__ void k_ac_wsolve(A);
+template <typename A, typename CT, bool PRHO = true, int PF = 4> __global__ void k_ac_wsolve(A);
 template <typename A> __global__ void k_calc_p_rho(A);
 template <typename A> __global__ void k_small_finish(A);
 template <typename A> __global__ void k_small_finish_x4(A);

@@ -4,7 +4,9 @@
 // captured hipGraph), with halo fills at the points where the reference calls
 // set_physical_bc3d/2d.
 #include <hip/hip_runtime.h>
+#include <dlfcn.h>
 #include <stdarg.h>
+#include <stdlib.h>
 #include <stdio.h>
 #include <string.h>
 #include <math.h>
@@ -185,8 +187,31 @@ static double *g_diag = nullptr;  // wrfmi_domain_totals' partial sums (g_mu)
 static size_t g_diag_bytes = 0;
 static int noalloc_scratch(float **out);
 
+static int tune_load(const char *path);
+static bool g_tune_loaded = false;
 extern "C" int wrfmi_init(void) {
   std::lock_guard<std::mutex> lk(g_mu);
+  if (!g_tune_loaded) {
+    // the autotuner's table: $WRFMI_TUNING_FILE, else wrfmi_tuning.txt beside this library
+    g_tune_loaded = true;
+    const char *env = getenv("WRFMI_TUNING_FILE");
+    std::string path;
+    if (env && *env) {
+      path = env;
+    } else {
+      Dl_info info;
+      if (dladdr((void *)&wrfmi_init, &info) && info.dli_fname) {
+        path = info.dli_fname;
+        const size_t slash = path.rfind('/');
+        path = (slash == std::string::npos ? std::string(".") : path.substr(0, slash)) + "/wrfmi_tuning.txt";
+        FILE *f = fopen(path.c_str(), "r");
+        if (f) fclose(f);
+        else path.clear();
+      }
+    }
+    if (!path.empty())
+      if (int rc = tune_load(path.c_str())) return rc;
+  }
   if (!g_stream) HIPCHK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
   if (!g_counter) HIPCHK(hipMalloc(&g_counter, sizeof(unsigned long long)));
   float *p = nullptr;
@@ -307,6 +332,10 @@ static const dim3 kColBlk(64, 4, 1);
 #define WRFMI_TENDT_ROWS 4
 #endif
 static const dim3 kVecBlk(64, WRFMI_VEC_ROWS, 1);
+// workgroup of the one-column-per-lane kernels: 64 columns by WRFMI_COL_ROWS rows (see g_col)
+#ifndef WRFMI_COL_ROWS
+#define WRFMI_COL_ROWS 1
+#endif
 
 // ------------------------------------------------------------------ kernel timing (diagnostic)
 enum KernelId {
@@ -345,6 +374,7 @@ struct KRef {
 #define KRP(K, P) KRef{K<KArgs, P>, K<BatchArgs, P>}
 #define KRZ(K) KRef{K<KArgs, false, true>, K<BatchArgs, false, true>}
 #define KRC(K, P, Q) KRef{K<KArgs, P, Q>, K<BatchArgs, P, Q>}
+#define KRWS(CT, PF) KRef{k_ac_wsolve<KArgs, CT, true, PF>, k_ac_wsolve<BatchArgs, CT, true, PF>}
 
 // One launch of an ensemble member, as recorded for a batched launch (wrfmi_ensemble_*).
 struct RecOp {
@@ -361,10 +391,21 @@ struct RecMember {
   std::vector<HaloArgs> h;
 };
 
+// Launch shapes: rows per workgroup of the four-point and column kernels, levels per thread
+// of the four-point advance_uv and column divergence, and the column solve's prefetch depth.
+// The autotuner (wrf_amd/autotune.py) picks them per kernel and domain shape; every shape
+// computes the same values in the same order (only the assignment of points to threads and
+// the distance loads run ahead change), so the results do not depend on them.
+struct Shape {
+  int rows, levels, prefetch;
+};
+static Shape default_shape(int kid);
+
 struct Launcher {
   hipStream_t st;
   KArgs a;
   HaloArgs h;
+  Shape sh[KID_COUNT];
   int err = 0;
   RecMember *rec = nullptr;  // set: record the launches instead of enqueueing them
   bool vec = false;          // the layout allows the four-points-per-lane kernels
@@ -422,6 +463,85 @@ struct Launcher {
   }
 };
 
+// ------------------------------------------------------------------ the launch-shape table
+static Shape default_shape(int kid) {
+  switch (kid) {
+    case KID_TEND_T: return Shape{WRFMI_TENDT_ROWS, 1, 0};
+    case KID_ADV_SCALAR: return Shape{WRFMI_ADVALL_ROWS, 1, 0};
+    case KID_ADVANCE_UV: return Shape{WRFMI_VEC_ROWS, kAdvUvK, 0};
+    case KID_AC_DIV: return Shape{WRFMI_VEC_ROWS, kDivK, 0};
+    case KID_AC_WSOLVE: return Shape{WRFMI_COL_ROWS, 1, 4};
+    case KID_WW_SCAN:
+    case KID_AC_MU_WW:
+    case KID_COEF_W: return Shape{WRFMI_COL_ROWS, 1, 0};
+    default: return Shape{WRFMI_VEC_ROWS, 1, 0};
+  }
+}
+// which of a kernel's shape parameters are tunable, and the values each may take
+static bool tunable_rows(int kid) { return kid != KID_HALO && kid != KID_W_SURFACE; }
+static bool tunable_levels(int kid) { return kid == KID_ADVANCE_UV || kid == KID_AC_DIV; }
+static bool tunable_prefetch(int kid) { return kid == KID_AC_WSOLVE; }
+struct TuneEntry {
+  int kid, nx, ny, nz;  // domain shape in mass points; 0 matches any
+  Shape s;
+};
+static std::vector<TuneEntry> g_tune;  // (g_mu) a later entry takes precedence
+static Shape shape_for(int kid, const Dm &d) {
+  Shape s = default_shape(kid);
+  const int nx = d.ide - d.ids, ny = d.jde - d.jds, nz = d.kde - d.kds;
+  for (const TuneEntry &e : g_tune)
+    if (e.kid == kid && (!e.nx || e.nx == nx) && (!e.ny || e.ny == ny) && (!e.nz || e.nz == nz)) s = e.s;
+  return s;
+}
+static int kernel_id(const char *name) {
+  for (int n = 0; n < KID_COUNT; n++)
+    if (name && !strcmp(name, kKernelNames[n])) return n;
+  return -1;
+}
+static int tune_add(const char *kernel, int nx, int ny, int nz, int rows, int levels, int prefetch) {
+  const int kid = kernel_id(kernel);
+  if (kid < 0) return fail(WRFMI_E_BADOPT, "no kernel named '%s'", kernel ? kernel : "(null)");
+  if (!tunable_rows(kid)) return fail(WRFMI_E_BADOPT, "%s has no tunable launch shape", kernel);
+  if (rows != 1 && rows != 2 && rows != 4) return fail(WRFMI_E_BADOPT, "%s: rows per workgroup %d (1, 2 or 4)", kernel, rows);
+  const Shape def = default_shape(kid);
+  if (tunable_levels(kid) ? (levels != 2 && levels != 4 && levels != 8) : levels != def.levels)
+    return fail(WRFMI_E_BADOPT, "%s: levels per thread %d not available", kernel, levels);
+  if (tunable_prefetch(kid) ? (prefetch != 2 && prefetch != 4 && prefetch != 8) : prefetch != def.prefetch)
+    return fail(WRFMI_E_BADOPT, "%s: prefetch depth %d not available", kernel, prefetch);
+  if (nx < 0 || ny < 0 || nz < 0) return fail(WRFMI_E_BADOPT, "negative domain shape");
+  g_tune.push_back(TuneEntry{kid, nx, ny, nz, Shape{rows, levels, prefetch}});
+  return 0;
+}
+// Lines "kernel nx ny nz rows levels prefetch" ('#' starts a comment).  The whole file is
+// checked before any entry is kept.
+static int tune_load(const char *path) {
+  FILE *f = fopen(path, "r");
+  if (!f) return fail(WRFMI_E_BADOPT, "cannot open launch-shape table %s", path);
+  std::vector<TuneEntry> saved = g_tune;
+  char line[256];
+  int lineno = 0, rc = 0;
+  while (!rc && fgets(line, sizeof line, f)) {
+    lineno++;
+    char *hash = strchr(line, '#');
+    if (hash) *hash = 0;
+    char name[64];
+    int nx, ny, nz, rows, levels, prefetch;
+    const int n = sscanf(line, "%63s %d %d %d %d %d %d", name, &nx, &ny, &nz, &rows, &levels, &prefetch);
+    if (n <= 0) continue;
+    if (n != 7) {
+      rc = fail(WRFMI_E_BADOPT, "%s:%d: expected 'kernel nx ny nz rows levels prefetch'", path, lineno);
+      break;
+    }
+    if ((rc = tune_add(name, nx, ny, nz, rows, levels, prefetch))) {
+      std::string msg = g_err;
+      rc = fail(rc, "%s:%d: %s", path, lineno, msg.c_str());
+    }
+  }
+  fclose(f);
+  if (rc) g_tune = saved;
+  return rc;
+}
+
 // ------------------------------------------------------------------ phases of one RK substep
 // Each enqueues the kernels of one group of solve_em calls, then the halo fills the
 // reference makes after that group.
@@ -437,11 +557,13 @@ static inline dim3 g_prep(const Dm &d) { return col_grid(d, d.ids - 1, d.jds - 1
 // row per workgroup spreads the few waves of a column launch (2800 at 400x400) evenly over
 // the CUs; at 400x400x80 the w solve takes 2.97 ms per step with 1 row, 3.12 with 2 and
 // 3.05 with 4.
-#ifndef WRFMI_COL_ROWS
-#define WRFMI_COL_ROWS 1
-#endif
 static const dim3 kColumnBlk(64, WRFMI_COL_ROWS, 1);
 static inline dim3 g_col(const Launcher &L) { return col_grid(L.a.d, col_ia(L.a.d, L.rec != nullptr), L.a.d.jds, kColumnBlk); }
+// the tuned forms: workgroup and grid of kernel `kid` with its launch shape
+static inline dim3 cblk(const Launcher &L, int kid) { return dim3(64, (unsigned)L.sh[kid].rows, 1); }
+static inline dim3 gcol(const Launcher &L, int kid) {
+  return col_grid(L.a.d, col_ia(L.a.d, L.rec != nullptr), L.a.d.jds, cblk(L, kid));
+}
 // (i, j) tiles of kColBlk over i = ids..ide, j = jds..jde, and nk levels in z
 static inline dim3 g_pt3(const Dm &d, int nk) {
   const dim3 g = col_grid(d, d.ids, d.jds, kColBlk);
@@ -458,6 +580,8 @@ static inline dim3 g_vec(const KArgs &a, int ibase, int jlow, int rows, int nk) 
 static inline dim3 g_vec3(const KArgs &a, int nk) {
   return g_vec(a, a.d.ims + 4 * ((a.d.ids - a.d.ims) >> 2), 0, WRFMI_VEC_ROWS, nk);
 }
+static inline dim3 vblk(const Launcher &L, int kid) { return dim3(64, (unsigned)L.sh[kid].rows, 1); }
+static inline dim3 gvec(const Launcher &L, int kid, int nk) { return g_vec(L.a, vec_ia0(L.a.d), 0, L.sh[kid].rows, nk); }
 
 static void ph_step_prep(Launcher &L) {
   const KArgs &a = L.a;
@@ -466,12 +590,13 @@ static void ph_step_prep(Launcher &L) {
   if (L.vec) {
     // one extra group of four below ids (mut at ids-1), rows from jds-1
     const int ia0 = a.d.ims + 4 * ((a.d.ids - a.d.ims) >> 2) - 4;
-    L.launchK(KID_STEP_PREP, KR(k_step_prep_x4), g_vec(a, ia0, 1, WRFMI_VEC_ROWS, a.d.kde - a.d.kds + 1), kVecBlk, a);
+    L.launchK(KID_STEP_PREP, KR(k_step_prep_x4), g_vec(a, ia0, 1, L.sh[KID_STEP_PREP].rows, a.d.kde - a.d.kds + 1),
+              vblk(L, KID_STEP_PREP), a);
   } else {
     L.launchK(KID_STEP_PREP, KR(k_step_prep), dim3(gp.x, gp.y, a.d.kde - a.d.kds + 1), kColBlk, a);
   }
-  if (a.c.precision_mode == 1) L.launchK(KID_WW_SCAN, KRP(k_ww_scan, true), g_col(L), kColumnBlk, a);
-  else L.launchK(KID_WW_SCAN, KRP(k_ww_scan, false), g_col(L), kColumnBlk, a);
+  if (a.c.precision_mode == 1) L.launchK(KID_WW_SCAN, KRP(k_ww_scan, true), gcol(L, KID_WW_SCAN), cblk(L, KID_WW_SCAN), a);
+  else L.launchK(KID_WW_SCAN, KRP(k_ww_scan, false), gcol(L, KID_WW_SCAN), cblk(L, KID_WW_SCAN), a);
   L.halo_begin();
   L.halo_add(s.ru, 'u', 1); L.halo_add(s.rv, 'v', 1); L.halo_add(s.rw, 'w', 1); L.halo_add(s.ww, 'w', 1);
   L.halo_add(s.php, 'w', 1); L.halo_add(s.alt, 't', 1); L.halo_add(s.p, 'p', 1);
@@ -492,10 +617,10 @@ static void ph_tendency(Launcher &L) {
   if (L.vec && (per || L.zone_ok)) {
     // four-point kernels; on a domain with an open direction they leave the boundary zone
     // to the one-point kernels' zone form
-    L.launchK(KID_TEND_U, KR(k_tend_u_x4), g_vec3(a, nk), kVecBlk, a);
-    L.launchK(KID_TEND_V, KR(k_tend_v_x4), g_vec3(a, nk), kVecBlk, a);
-    L.launchK(KID_TEND_W, KR(k_tend_w_x4), g_vec3(a, nk), kVecBlk, a);
-    L.launchK(KID_TEND_T, KR(k_tend_t_x4), g_vec(a, vec_ia0(a.d), 0, WRFMI_TENDT_ROWS, nk), dim3(64, WRFMI_TENDT_ROWS, 1), a);
+    L.launchK(KID_TEND_U, KR(k_tend_u_x4), gvec(L, KID_TEND_U, nk), vblk(L, KID_TEND_U), a);
+    L.launchK(KID_TEND_V, KR(k_tend_v_x4), gvec(L, KID_TEND_V, nk), vblk(L, KID_TEND_V), a);
+    L.launchK(KID_TEND_W, KR(k_tend_w_x4), gvec(L, KID_TEND_W, nk), vblk(L, KID_TEND_W), a);
+    L.launchK(KID_TEND_T, KR(k_tend_t_x4), gvec(L, KID_TEND_T, nk), vblk(L, KID_TEND_T), a);
     if (!per) {
       const dim3 gz = g_zone(a, nk);  // one level per thread
       L.launchK(KID_TEND_U, KRZ(k_tend_u), gz, kColBlk, a);
@@ -515,8 +640,8 @@ static void ph_small_prep(Launcher &L) {
   const wrfmi_state &s = a.s;
   const int nk = a.d.kde - a.d.kds + 1;
   if (L.vec) {
-    L.launchK(KID_SMALL_PREP, KR(k_small_prep_x4), g_vec3(a, nk), kVecBlk, a);
-    L.launchK(KID_P_RHO0, KR(k_p_rho0_x4), g_vec3(a, nk - 1), kVecBlk, a);
+    L.launchK(KID_SMALL_PREP, KR(k_small_prep_x4), gvec(L, KID_SMALL_PREP, nk), vblk(L, KID_SMALL_PREP), a);
+    L.launchK(KID_P_RHO0, KR(k_p_rho0_x4), gvec(L, KID_P_RHO0, nk - 1), vblk(L, KID_P_RHO0), a);
   } else {
     L.launchK(KID_SMALL_PREP, KR(k_small_prep), g_pt3(a.d, nk), kColBlk, a);
     L.launchK(KID_P_RHO0, KR(k_p_rho0), g_pt3(a.d, nk - 1), kColBlk, a);
@@ -524,8 +649,12 @@ static void ph_small_prep(Launcher &L) {
   // the per-routine entry point stores calc_coef_w's a, alpha, gamma as the reference does;
   // the whole step leaves them to k_ac_wsolve, which forms them on its way up each column
   const bool p64 = a.c.precision_mode == 1;
-  if (L.whole_step) L.launchK(KID_COEF_W, p64 ? KRC(k_coef_w, double, false) : KRC(k_coef_w, float, false), g_col(L), kColumnBlk, a);
-  else L.launchK(KID_COEF_W, p64 ? KRC(k_coef_w, double, true) : KRC(k_coef_w, float, true), g_col(L), kColumnBlk, a);
+  if (L.whole_step)
+    L.launchK(KID_COEF_W, p64 ? KRC(k_coef_w, double, false) : KRC(k_coef_w, float, false), gcol(L, KID_COEF_W),
+              cblk(L, KID_COEF_W), a);
+  else
+    L.launchK(KID_COEF_W, p64 ? KRC(k_coef_w, double, true) : KRC(k_coef_w, float, true), gcol(L, KID_COEF_W),
+              cblk(L, KID_COEF_W), a);
   L.halo_begin();
   L.halo_add(s.ru_tend, 'u', 1); L.halo_add(s.rv_tend, 'v', 1); L.halo_add(s.ph_2, 'w', 1);
   L.halo_add(s.al, 'p', 1); L.halo_add(s.p, 'p', 1); L.halo_add(s.t_1, 'p', 1); L.halo_add(s.t_save, 't', 1);
@@ -535,9 +664,14 @@ static void ph_small_prep(Launcher &L) {
 static void ph_advance_uv(Launcher &L) {
   const bool per = L.a.c.periodic_x && L.a.c.periodic_y;
   const int nkb = (L.a.d.kde - L.a.d.kds + kAdvUvK - 1) / kAdvUvK;
-  if (L.vec)
-    L.launchK(KID_ADVANCE_UV, per ? KRP(k_advance_uv_x4, true) : KRP(k_advance_uv_x4, false), g_vec3(L.a, nkb), kVecBlk, L.a);
-  else
+  if (L.vec) {
+    const int ks = L.sh[KID_ADVANCE_UV].levels;
+    const int nkv = (L.a.d.kde - L.a.d.kds + ks - 1) / ks;
+    const KRef k = ks == 2 ? (per ? KRC(k_advance_uv_x4, true, 2) : KRC(k_advance_uv_x4, false, 2))
+                   : ks == 8 ? (per ? KRC(k_advance_uv_x4, true, 8) : KRC(k_advance_uv_x4, false, 8))
+                             : (per ? KRC(k_advance_uv_x4, true, 4) : KRC(k_advance_uv_x4, false, 4));
+    L.launchK(KID_ADVANCE_UV, k, gvec(L, KID_ADVANCE_UV, nkv), vblk(L, KID_ADVANCE_UV), L.a);
+  } else
     L.launchK(KID_ADVANCE_UV, per ? KRP(k_advance_uv, true) : KRP(k_advance_uv, false), g_pt3(L.a.d, nkb), kColBlk, L.a);
 }
 static void ph_advance_mu_t_w(Launcher &L) {
@@ -545,14 +679,23 @@ static void ph_advance_mu_t_w(Launcher &L) {
   const wrfmi_state &s = a.s;
   const bool p64 = a.c.precision_mode == 1;
   const int nk = a.d.kde - a.d.kds + 1;
-  if (L.vec) L.launchK(KID_AC_DIV, KR(k_ac_div_x4), g_vec3(a, (nk + kDivK - 1) / kDivK), kVecBlk, a);
+  if (L.vec) {
+    const int ks = L.sh[KID_AC_DIV].levels;
+    const KRef k = ks == 2 ? KRP(k_ac_div_x4, 2) : ks == 8 ? KRP(k_ac_div_x4, 8) : KRP(k_ac_div_x4, 4);
+    L.launchK(KID_AC_DIV, k, gvec(L, KID_AC_DIV, (nk + ks - 1) / ks), vblk(L, KID_AC_DIV), a);
+  }
   else L.launchK(KID_AC_DIV, KR(k_ac_div), g_pt3(a.d, nk), kColBlk, a);
-  if (p64) L.launchK(KID_AC_MU_WW, KRP(k_ac_mu_ww, true), g_col(L), kColumnBlk, a);
-  else L.launchK(KID_AC_MU_WW, KRP(k_ac_mu_ww, false), g_col(L), kColumnBlk, a);
-  if (L.vec) L.launchK(KID_AC_THETA, KR(k_ac_theta_x4), g_vec3(a, nk), kVecBlk, a);
+  if (p64) L.launchK(KID_AC_MU_WW, KRP(k_ac_mu_ww, true), gcol(L, KID_AC_MU_WW), cblk(L, KID_AC_MU_WW), a);
+  else L.launchK(KID_AC_MU_WW, KRP(k_ac_mu_ww, false), gcol(L, KID_AC_MU_WW), cblk(L, KID_AC_MU_WW), a);
+  if (L.vec) L.launchK(KID_AC_THETA, KR(k_ac_theta_x4), gvec(L, KID_AC_THETA, nk), vblk(L, KID_AC_THETA), a);
   else L.launchK(KID_AC_THETA, KR(k_ac_theta), g_pt3(a.d, nk), kColBlk, a);
-  if (p64) L.launchK(KID_AC_WSOLVE, KRP(k_ac_wsolve, double), g_col(L), kColumnBlk, a);
-  else L.launchK(KID_AC_WSOLVE, KRP(k_ac_wsolve, float), g_col(L), kColumnBlk, a);
+  {
+    const int pf = L.sh[KID_AC_WSOLVE].prefetch;
+    KRef k;
+    if (p64) k = pf == 2 ? KRWS(double, 2) : pf == 8 ? KRWS(double, 8) : KRWS(double, 4);
+    else k = pf == 2 ? KRWS(float, 2) : pf == 8 ? KRWS(float, 8) : KRWS(float, 4);
+    L.launchK(KID_AC_WSOLVE, k, gcol(L, KID_AC_WSOLVE), cblk(L, KID_AC_WSOLVE), a);
+  }
   L.halo_begin();
   L.halo_add(s.ph_2, 'w', 1); L.halo_add(s.al, 'p', 1); L.halo_add(s.p, 'p', 1);
   L.halo_add(s.muts, 't', 0); L.halo_add(s.mu_2, 't', 0); L.halo_add(s.mudf, 't', 0);
@@ -561,7 +704,9 @@ static void ph_advance_mu_t_w(Launcher &L) {
 static void ph_small_finish(Launcher &L) {
   const KArgs &a = L.a;
   const wrfmi_state &s = a.s;
-  if (L.vec) L.launchK(KID_SMALL_FINISH, KR(k_small_finish_x4), g_vec3(a, a.d.kde - a.d.kds + 1), kVecBlk, a);
+  if (L.vec)
+    L.launchK(KID_SMALL_FINISH, KR(k_small_finish_x4), gvec(L, KID_SMALL_FINISH, a.d.kde - a.d.kds + 1),
+              vblk(L, KID_SMALL_FINISH), a);
   else L.launchK(KID_SMALL_FINISH, KR(k_small_finish), g_pt3(a.d, a.d.kde - a.d.kds + 1), kColBlk, a);
   if (a.rk_step == 3) {
     L.halo_begin();
@@ -578,8 +723,8 @@ static void ph_scalar_tend(Launcher &L) {
                 (d.kde - d.kds + 1 + kAdvK - 1) / kAdvK);
   if (L.vec && (per || L.zone_ok)) {
     const int ia0 = d.ims + 4 * ((d.ids - d.ims) >> 2);
-    const dim3 gv = g_vec(L.a, ia0, 0, kAdvRows, (d.kde - d.kds + kAdvVK) / kAdvVK);
-    L.launchK(KID_ADV_SCALAR, KR(k_adv_scalar_x4), gv, dim3(64, kAdvRows, 1), L.a);
+    const dim3 gv = g_vec(L.a, ia0, 0, L.sh[KID_ADV_SCALAR].rows, (d.kde - d.kds + kAdvVK) / kAdvVK);
+    L.launchK(KID_ADV_SCALAR, KR(k_adv_scalar_x4), gv, vblk(L, KID_ADV_SCALAR), L.a);
     if (!per) L.launchK(KID_ADV_SCALAR, KRZ(k_adv_scalar), g_zone(L.a, d.kde - d.kds + 1), dim3(64, kAdvRows, 1), L.a);
   } else {
     L.launchK(KID_ADV_SCALAR, per ? KRP(k_adv_scalar, true) : KRP(k_adv_scalar, false), ga, dim3(64, kAdvRows, 1), L.a);
@@ -587,18 +732,22 @@ static void ph_scalar_tend(Launcher &L) {
 }
 static void ph_scalar(Launcher &L) {
   ph_scalar_tend(L);
-  if (L.vec) L.launchK(KID_UPDATE_SCALAR, KR(k_update_scalar_x4), g_vec3(L.a, L.a.d.kde - L.a.d.kds), kVecBlk, L.a);
+  if (L.vec)
+    L.launchK(KID_UPDATE_SCALAR, KR(k_update_scalar_x4), gvec(L, KID_UPDATE_SCALAR, L.a.d.kde - L.a.d.kds),
+              vblk(L, KID_UPDATE_SCALAR), L.a);
   else L.launchK(KID_UPDATE_SCALAR, KR(k_update_scalar), g_pt3(L.a.d, L.a.d.kde - L.a.d.kds), kColBlk, L.a);
 }
 // every species' rk_scalar_tend and rk_update_scalar (species are independent: the order
 // over species does not change any value)
 static void ph_scalar_all(Launcher &L) {
   const Dm &d = L.a.d;
-  L.launchK(KID_ADV_SCALAR, KR(k_adv_scalar_all_x4), g_vec(L.a, vec_ia0(d), 0, WRFMI_ADVALL_ROWS, d.kde - d.kds + 1), dim3(64, WRFMI_ADVALL_ROWS, 1), L.a);
-  L.launchK(KID_UPDATE_SCALAR, KR(k_update_scalar_all_x4), g_vec3(L.a, d.kde - d.kds), kVecBlk, L.a);
+  L.launchK(KID_ADV_SCALAR, KR(k_adv_scalar_all_x4), gvec(L, KID_ADV_SCALAR, d.kde - d.kds + 1), vblk(L, KID_ADV_SCALAR), L.a);
+  L.launchK(KID_UPDATE_SCALAR, KR(k_update_scalar_all_x4), gvec(L, KID_UPDATE_SCALAR, d.kde - d.kds),
+            vblk(L, KID_UPDATE_SCALAR), L.a);
 }
 static void ph_p_rho_phi(Launcher &L) {
-  if (L.vec) L.launchK(KID_P_RHO_PHI, KR(k_p_rho_phi_x4), g_vec3(L.a, L.a.d.kde - L.a.d.kds), kVecBlk, L.a);
+  if (L.vec)
+    L.launchK(KID_P_RHO_PHI, KR(k_p_rho_phi_x4), gvec(L, KID_P_RHO_PHI, L.a.d.kde - L.a.d.kds), vblk(L, KID_P_RHO_PHI), L.a);
   else L.launchK(KID_P_RHO_PHI, KR(k_p_rho_phi), g_pt3(L.a.d, L.a.d.kde - L.a.d.kds), kColBlk, L.a);
 }
 static void ph_bc_dry_2(Launcher &L, int kind) {
@@ -717,7 +866,7 @@ static int check_coef_arrays(const wrfmi_state &s, const wrfmi_config &c, const 
 // and the allocation-free entry point passes the static array.  With own_scratch and no
 // own_qt, the species are advected one by one (no species work arrays).
 static int prepare(const wrfmi_state *st_in, const wrfmi_config *cfg, const wrfmi_dims *dims, Launcher &L, wrfmi_state &s,
-                   const Scratch *own_scratch = nullptr, float *own_qt = nullptr) {
+                   const Scratch *own_scratch = nullptr, float *own_qt = nullptr, bool default_shapes = false) {
   int rc;
   if ((rc = check_cfg(cfg))) return rc;
   Dm d;
@@ -738,6 +887,8 @@ static int prepare(const wrfmi_state *st_in, const wrfmi_config *cfg, const wrfm
   L.h.d = d;
   L.h.periodic_x = cfg->periodic_x; L.h.periodic_y = cfg->periodic_y;
   L.h.open_xs = cfg->open_xs; L.h.open_xe = cfg->open_xe; L.h.open_ys = cfg->open_ys; L.h.open_ye = cfg->open_ye;
+  // ensemble members are recorded with the default shapes (one launch sequence for all)
+  for (int k = 0; k < KID_COUNT; k++) L.sh[k] = default_shapes ? default_shape(k) : shape_for(k, d);
   L.vec = vec_layout(s, d, L.a);
   // the all-species scalar kernels (doubly periodic, aligned layout, more than one species)
   L.scal_all = L.vec && cfg->periodic_x && cfg->periodic_y && s.n_species > 1 && (own_qt || !own_scratch);
@@ -956,7 +1107,7 @@ extern "C" int wrfmi_ensemble_create(int n_members, const wrfmi_state *states, c
       e->work.push_back(qt);
     }
     wrfmi_state s;
-    if ((rc = prepare(&states[m], &cfgs[m], &dims[m], Ls[m], s, &sc, qt))) {
+    if ((rc = prepare(&states[m], &cfgs[m], &dims[m], Ls[m], s, &sc, qt, true))) {
       std::string msg = g_err;
       rc = fail(rc, "member %d: %s", m, msg.c_str());
       break;
@@ -1383,6 +1534,36 @@ extern "C" int wrfmi_domain_totals(const wrfmi_state *st_in, const wrfmi_config 
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(out, g_diag, kNTotals * sizeof(double), hipMemcpyDeviceToHost, hs));
   HIPCHK(hipStreamSynchronize(hs));
+  return 0;
+}
+
+// ------------------------------------------------------------------ launch-shape table entry points
+extern "C" int wrfmi_tuning_set(const char *kernel, int nx, int ny, int nz, int rows, int levels, int prefetch) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  return tune_add(kernel, nx, ny, nz, rows, levels, prefetch);
+}
+extern "C" int wrfmi_tuning_clear(void) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_tune.clear();
+  return 0;
+}
+extern "C" int wrfmi_tuning_load(const char *path) {
+  if (!path) return fail(WRFMI_E_POINTER, "null path");
+  std::lock_guard<std::mutex> lk(g_mu);
+  return tune_load(path);
+}
+extern "C" int wrfmi_tuning_get(const char *kernel, int nx, int ny, int nz, int *rows, int *levels, int *prefetch) {
+  if (!rows || !levels || !prefetch) return fail(WRFMI_E_POINTER, "null output");
+  std::lock_guard<std::mutex> lk(g_mu);
+  const int kid = kernel_id(kernel);
+  if (kid < 0) return fail(WRFMI_E_BADOPT, "no kernel named '%s'", kernel ? kernel : "(null)");
+  Dm d;
+  memset(&d, 0, sizeof d);
+  d.ids = 1; d.ide = nx + 1; d.jds = 1; d.jde = ny + 1; d.kds = 1; d.kde = nz + 1;
+  const Shape s = shape_for(kid, d);
+  *rows = s.rows;
+  *levels = s.levels;
+  *prefetch = s.prefetch;
   return 0;
 }
 

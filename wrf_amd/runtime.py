@@ -136,6 +136,9 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.wrfmi_kernel_name.restype = ctypes.c_char_p
     lib.wrfmi_kernel_name.argtypes = [ctypes.c_int]
     lib.wrfmi_timing_enable.argtypes = [ctypes.c_uint]
+    lib.wrfmi_tuning_set.argtypes = [ctypes.c_char_p] + [ctypes.c_int] * 6
+    lib.wrfmi_tuning_load.argtypes = [ctypes.c_char_p]
+    lib.wrfmi_tuning_get.argtypes = [ctypes.c_char_p] + [ctypes.c_int] * 3 + [ctypes.POINTER(ctypes.c_int)] * 3
     lib.wrfmi_timing_get.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_double)]
     if lib.wrfmi_state_sizeof() != ctypes.sizeof(State32):
         raise RuntimeError("wrfmi_state layout of the library does not match wrf_amd.fields")
