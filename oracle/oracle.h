@@ -19,6 +19,12 @@
  *   o_advect_scalar  bit for bit against the reference's own advect_scalar, compiled in
  *                    place (oracle/_ref) and through tests/golden/advect_scalar_ref.npz:
  *                    periodic and open, both signs of time_step, field_old = field and not.
+ *   o_advect_scalar_pd  bit for bit against the reference's own advect_scalar_pd the same way
+ *                    (tests/golden/advect_scalar_pd_ref.npz, tendency and its h/z split);
+ *                    unlimited it is the pinned o_advect_scalar; the limiter keeps the
+ *                    updated mass non-negative and the periodic total conserved.
+ *   o_rk_update_scalar_pd + the last-substep flow  a physics tendency is applied once over the
+ *                    step; an empty-celled tracer stays non-negative through whole steps.
  *   o_advect_u/v/w   bit for bit equal to o_advect_scalar in uniform flow (same fluxes,
  *                    shifted stencil); a constant field in a non-uniform mass flux on a
  *                    stretched grid gives -C div(mass flux) on every level, advect_w's
@@ -144,6 +150,17 @@ int o_advect_scalar(const wreal *field, const wreal *field_old, wreal *tend, con
                     const wreal *msfty, const wreal *fzm, const wreal *fzp, wreal rdx, wreal rdy,
                     const wreal *rdzw, const dims_t *d);
 
+/* positive-definite form (order 5 horizontal / 3 vertical); h_tend and z_tend are written
+ * when tenddec is non-zero */
+int o_advect_scalar_pd(const wreal *field, const wreal *field_old, wreal *tend, wreal *h_tend, wreal *z_tend,
+                       const wreal *ru, const wreal *rv, const wreal *rom, const wreal *c1, const wreal *c2,
+                       const wreal *mut, const wreal *mub, const wreal *mu_old, int time_step, const cfg_t *c,
+                       int tenddec, const wreal *msfux, const wreal *msfuy, const wreal *msfvx, const wreal *msfvy,
+                       const wreal *msftx, const wreal *msfty, const wreal *fzm, const wreal *fzp, wreal rdx, wreal rdy,
+                       const wreal *rdzw, wreal dt, const dims_t *d);
+void o_rk_update_scalar_pd(wreal *scalar, wreal *sc_tend, const wreal *c1, const wreal *c2, const wreal *mu_old,
+                           const wreal *mu_new, const wreal *mu_base, wreal dt, const dims_t *d);
+
 /* ---- rk_tendency and rk_addtend_dry ---- */
 int o_rk_tendency(ostate *s, const cfg_t *c, const dims_t *d, int rk_step);
 void o_rk_addtend_dry(ostate *s, const dims_t *d, int rk_step);
@@ -160,7 +177,7 @@ void o_calc_mu_uv_1(ostate *s, const cfg_t *c, const dims_t *d);
 void o_small_step_finish(ostate *s, const dims_t *d, int nsmall, wreal dts, int rk_step, int rk_order);
 
 /* ---- scalars and EOS ---- */
-int o_rk_scalar_tend(ostate *s, const cfg_t *c, const dims_t *d, int ispec);
+int o_rk_scalar_tend(ostate *s, const cfg_t *c, const dims_t *d, int ispec, int rk_step, wreal dt_rk);
 void o_rk_update_scalar(ostate *s, const dims_t *d, int ispec, int rk_step, wreal dt_rk);
 void o_calc_p_rho_phi(ostate *s, const cfg_t *c, const dims_t *d);
 

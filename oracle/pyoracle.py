@@ -3,7 +3,7 @@
 Loads oracle/liboracle_f32.so (reference-order FP32 restatement),
 liboracle_f32cr.so (the same with the EOS's LOG and ** rounded once from double),
 liboracle_f64.so (same source in FP64, the -r8 analogue), and the reference's own
-advect_scalar from oracle/_ref/libref_advect.so when that was built.
+advect_scalar and advect_scalar_pd from oracle/_ref/libref_advect.so when that was built.
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this.
 """
 from __future__ import annotations
@@ -79,7 +79,7 @@ class OracleSession:
 
     _ARGS = {"rk_step_prep": "i", "rk_tendency": "i", "small_step_prep": "id", "advance_uv": "d",
              "advance_mu_t_w": "dii", "small_step_finish": "iid", "rk_scalar_step": "iid",
-             "calc_p_rho_phi": "", "phys_bc_dry_2": "i", "set_w_surface": ""}
+             "rk_scalar_pd_prep": "d", "calc_p_rho_phi": "", "phys_bc_dry_2": "i", "set_w_surface": ""}
 
     def __init__(self, oracle: Oracle, hs):
         self.o = oracle
@@ -108,20 +108,36 @@ class OracleSession:
             raise RuntimeError(f"oracle solve_em step failed with status {rc}")
 
 
+def _flags(cfg):
+    return (ctypes.c_int * 15)(cfg.periodic_x, cfg.periodic_y, cfg.symmetric_xs, cfg.symmetric_xe, cfg.symmetric_ys,
+                               cfg.symmetric_ye, cfg.open_xs, cfg.open_xe, cfg.open_ys, cfg.open_ye, cfg.specified,
+                               cfg.nested, cfg.polar, cfg.h_sca_adv_order, cfg.v_sca_adv_order)
+
+
 class RefAdvect:
-    """The reference's own advect_scalar (compiled in place, ADVECT_KERNEL mode)."""
+    """The reference's own advect_scalar and advect_scalar_pd (compiled in place,
+    ADVECT_KERNEL mode)."""
 
     def __init__(self):
         self.lib = _load(os.path.join("_ref", "libref_advect.so"))
         self.fn = self.lib.ref_advect_scalar
         self.fn.restype = None
+        self.fn_pd = self.lib.ref_advect_scalar_pd
+        self.fn_pd.restype = None
+
+    def advect_scalar_pd(self, field, field_old, tend, h_tend, z_tend, ru, rv, rom, c1, c2, mut, mub, mu_old,
+                         time_step, cfg, tenddec, msf, fzm, fzp, rdx, rdy, rdzw, dt, dims):
+        d = (ctypes.c_int * 18)(*[getattr(dims, n) for n, _ in dims._fields_])
+        f = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+        self.fn_pd(f(field), f(field_old), f(tend), f(h_tend), f(z_tend), f(ru), f(rv), f(rom), f(c1), f(c2), f(mut),
+                   f(mub), f(mu_old), ctypes.c_int(time_step), _flags(cfg), ctypes.c_int(int(tenddec)),
+                   f(msf["msfux"]), f(msf["msfuy"]), f(msf["msfvx"]), f(msf["msfvy"]), f(msf["msftx"]),
+                   f(msf["msfty"]), f(fzm), f(fzp), ctypes.c_float(rdx), ctypes.c_float(rdy), f(rdzw),
+                   ctypes.c_float(dt), d)
 
     def advect_scalar(self, field, field_old, tend, ru, rv, rom, c1, c2, mut, time_step, cfg, msf, fzm, fzp,
                       rdx, rdy, rdzw, dims):
-        flags = (ctypes.c_int * 15)(cfg.periodic_x, cfg.periodic_y, cfg.symmetric_xs, cfg.symmetric_xe,
-                                    cfg.symmetric_ys, cfg.symmetric_ye, cfg.open_xs, cfg.open_xe, cfg.open_ys,
-                                    cfg.open_ye, cfg.specified, cfg.nested, cfg.polar, cfg.h_sca_adv_order,
-                                    cfg.v_sca_adv_order)
+        flags = _flags(cfg)
         d = (ctypes.c_int * 18)(*[getattr(dims, n) for n, _ in dims._fields_])
         f = lambda a: a.ctypes.data_as(ctypes.c_void_p)
         self.fn(f(field), f(field_old), f(tend), f(ru), f(rv), f(rom), f(c1), f(c2), f(mut),

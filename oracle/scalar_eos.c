@@ -11,12 +11,20 @@
 #include "oracle.h"
 
 /* advect_tend is zeroed over the tile (zero_tend, module_em.F:1224) and then filled */
-int o_rk_scalar_tend(ostate *s, const cfg_t *c, const dims_t *d, int m) {
+int o_rk_scalar_tend(ostate *s, const cfg_t *c, const dims_t *d, int m, int rk_step, wreal dt_rk) {
   int adv_opt = m < s->n_moist ? c->moist_adv_opt : c->scalar_adv_opt;
-  if (adv_opt != 0 || c->zadvect_implicit || c->diff_opt != 0 || c->diff_6th_opt != 0) return WRFMI_E_BADOPT;
+  if ((adv_opt != 0 && adv_opt != 1) || c->zadvect_implicit || c->diff_opt != 0 || c->diff_6th_opt != 0)
+    return WRFMI_E_BADOPT;
   for (int j = d->jts; j <= d->jte; j++)
     for (int k = d->kts; k <= d->kte; k++)
       for (int i = d->its; i <= d->ite; i++) A3(s->advect_tend, i, k, j) = R(0.);
+  /* module_em.F:1264-1280: the positive-definite form on the last RK substep, with the
+   * time-n values, mub and mu_1 of the limiter; dt_step is the substep's dt */
+  if (adv_opt == 1 && rk_step == c->rk_ord) {
+    return o_advect_scalar_pd(s->q[m], s->q_old[m], s->advect_tend, NULL, NULL, s->ru_m, s->rv_m, s->ww_m, s->c1h,
+                              s->c2h, s->muts, s->mub, s->mu_1, c->time_step, c, 0, s->msfux, s->msfuy, s->msfvx,
+                              s->msfvy, s->msftx, s->msfty, s->fnm, s->fnp, s->rdx, s->rdy, s->rdnw, dt_rk, d);
+  }
   return o_advect_scalar(s->q[m], s->q[m], s->advect_tend, s->ru_m, s->rv_m, s->ww_m, s->c1h, s->c2h, s->muts,
                          c->time_step, c, s->msfux, s->msfuy, s->msfvx, s->msfvy, s->msftx, s->msfty, s->fnm, s->fnp,
                          s->rdx, s->rdy, s->rdnw, d);

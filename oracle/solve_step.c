@@ -10,6 +10,7 @@
  *   halo fills                                    solve_em.F:1181-1264
  *   acoustic loop          solve_em.F:1266-1719
  *   calc_mu_uv_1, small_step_finish, rk3 fills    solve_em.F:1732-1800
+ *   positive-definite scalar pre-update + fills   solve_em.F:1850-1975
  *   scalar advection/update                       solve_em.F:2198-2364, 2777-2950
  *   calc_p_rho_phi                                solve_em.F:2959-2979
  *   rk_phys_bc_dry_2 + species fills (rk < 3)     solve_em.F:3240-3340
@@ -79,6 +80,24 @@ static void bc_species(ostate *s, const cfg_t *c, const dims_t *d) {
   for (int m = 0; m < s->n_species; m++) o_set_physical_bc3d(s->q[m], 'p', c, d);
 }
 
+static int pd_species(const ostate *s, const cfg_t *c, int m) {
+  return (m < s->n_moist ? c->moist_adv_opt : c->scalar_adv_opt) == 1;
+}
+
+/* solve_em.F:1850-1975: before the last RK substep, a species advected in positive-definite
+ * form takes its physics tendency into its time-n values (rk_update_scalar_pd with mu_1 as
+ * both masses, tendency zeroed), then those values get their halo */
+static void pd_prep(ostate *s, const cfg_t *c, const dims_t *d, wreal dt_rk) {
+  for (int m = 0; m < s->n_species; m++) {
+    if (!pd_species(s, c, m)) continue;
+    for (int t_ = 0, nt_ = oracle_num_tiles(d); t_ < nt_; t_++) {
+      const dims_t dt_ = tile_dims(d, t_, nt_);
+      o_rk_update_scalar_pd(s->q_old[m], s->q_tend[m], s->c1h, s->c2h, s->mu_1, s->mu_1, s->mub, dt_rk, &dt_);
+    }
+    o_set_physical_bc3d(s->q_old[m], 'p', c, d);
+  }
+}
+
 int o_solve_em_step(ostate *s, const cfg_t *c, const dims_t *d) {
   int rc;
   if (c->rk_ord != 3 || c->specified || c->nested || c->polar) return WRFMI_E_BADOPT;
@@ -139,8 +158,9 @@ int o_solve_em_step(ostate *s, const cfg_t *c, const dims_t *d) {
       o_set_physical_bc2d(s->muts, 't', c, d);
     }
 
+    if (rk_step == rk_order) pd_prep(s, c, d, dt_rk);
     for (int m = 0; m < s->n_species; m++) {
-      TILED(o_rk_scalar_tend(s, c, d, m));
+      TILED(o_rk_scalar_tend(s, c, d, m, rk_step, dt_rk));
       TILEDV(o_rk_update_scalar(s, d, m, rk_step, dt_rk));
     }
 
@@ -246,8 +266,12 @@ int oracle_small_step_finish(ostate *s, const cfg_t *c, const dims_t *d, int rk_
   }
   return 0;
 }
+int oracle_rk_scalar_pd_prep(ostate *s, const cfg_t *c, const dims_t *d, double dt_rk) {
+  pd_prep(s, c, d, (wreal)dt_rk);
+  return 0;
+}
 int oracle_rk_scalar_step(ostate *s, const cfg_t *c, const dims_t *d, int ispec, int rk_step, double dt_rk) {
-  TILED(o_rk_scalar_tend(s, c, d, ispec));
+  TILED(o_rk_scalar_tend(s, c, d, ispec, rk_step, (wreal)dt_rk));
   TILEDV(o_rk_update_scalar(s, d, ispec, rk_step, (wreal)dt_rk));
   return 0;
 }

@@ -45,6 +45,19 @@ pin_constant_mixing_ratio   the whole RK3 step with scalar transport (rk_update_
 pin_pressure_gradient       rk_tendency's horizontal pressure gradient
                             (module_big_step_utilities_em.F:2183-2415): the same closed form
                             as advance_uv's, for the large-step perturbation fields.
+pin_advect_scalar_pd        advect_scalar_pd (module_advect_em.F:6069-7885): with no cell
+                            limited (a positive smooth field) it is the pinned advect_scalar
+                            to rounding, periodic and open; on a tracer with many empty cells
+                            the limiter engages, the updated mass (c1 (mub + mu_old) + c2)
+                            q_old + dt msfty tend stays non-negative where the unlimited
+                            scheme's goes negative, and the periodic domain total is conserved.
+pin_pd_whole_step           the last-substep positive-definite transport of the whole step
+                            (solve_em.F:1850-1975, rk_update_scalar_pd module_em.F:1803-1916):
+                            a physics tendency on a resting atmosphere is applied once over the
+                            step, exactly as the unlimited path applies it; a tracer with empty
+                            cells stays non-negative through whole steps of a bubble flow where
+                            the unlimited path makes it negative; constant mixing ratios stay
+                            constant (pin_constant_mixing_ratio runs both forms).
 pin_resting_over_terrain    the whole step over terrain: an atmosphere at rest in
                             hydrostatic balance with flat isentropes stays (nearly) at rest
                             over a ridge on hybrid levels.
@@ -730,11 +743,13 @@ def pin_constant_mixing_ratio(o):
     """A uniform mixing ratio stays uniform through whole RK3 steps of a divergent flow
     (rising bubbles, sheared winds): the scalar fluxes are the time-averaged mass fluxes of
     the acoustic steps (sumflux), so the scalar tendency is C times the mass tendency, and
-    rk_update_scalar's (c1 mu + c2) coupling returns C.  Periodic and open boundaries."""
-    for bc in ("periodic", "open"):
+    rk_update_scalar's (c1 mu + c2) coupling returns C.  Periodic and open boundaries, with
+    the unlimited and the positive-definite transport."""
+    for bc, pd in (("periodic", 0), ("open", 0), ("periodic", 1), ("open", 1)):
         spec = grid.DomainSpec(nx=14, ny=12, nz=14, bc=bc, n_passive=2, n_bubbles=2, bubble_amp=3.0, seed=9,
                                hybrid_opt=2)
         hs = grid.build_initial_state(spec)
+        hs.cfg.moist_adv_opt = hs.cfg.scalar_adv_opt = pd
         for m in range(1, hs.n_species):
             hs.q[m][...] = 0.5 + 0.25 * m
             hs.q_old[m][...] = hs.q[m]
@@ -749,7 +764,126 @@ def pin_constant_mixing_ratio(o):
         for m in range(1, hs.n_species):
             q = np.asarray(a[f"q{m}"], np.float64)[sl]
             C = 0.5 + 0.25 * m
-            assert np.abs(q - C).max() <= 1e-11 * C, (bc, m, np.abs(q - C).max())
+            assert np.abs(q - C).max() <= 1e-11 * C, (bc, pd, m, np.abs(q - C).max())
+
+
+def _interior3(d):
+    return (slice(d.jds - d.jms, d.jde - d.jms), slice(0, d.kde - 1), slice(d.ids - d.ims, d.ide - d.ims))
+
+
+def _advect_pd_and_unlimited(o, d, c, A):
+    """advect_scalar_pd and advect_scalar (field_old = field for the latter) of the arrays A."""
+    vp, ci = ctypes.c_void_p, ctypes.c_int
+    P = lambda x: x.ctypes.data_as(vp)  # noqa: E731
+    L = o.lib
+    L.o_advect_scalar_pd.argtypes = [vp] * 13 + [ci, vp, ci] + [vp] * 8 + [o.ctype, o.ctype, vp, o.ctype, vp]
+    L.o_advect_scalar.argtypes = [vp] * 9 + [ci, vp] + [vp] * 8 + [o.ctype, o.ctype, vp, vp]
+    msf = [P(A[m]) for m in ("msfux", "msfuy", "msfvx", "msfvy", "msftx", "msfty")]
+    tp, th = np.zeros_like(A["field"]), np.zeros_like(A["field"])
+    assert L.o_advect_scalar_pd(P(A["field"]), P(A["field_old"]), P(tp), None, None, P(A["ru"]), P(A["rv"]), P(A["rom"]),
+                                P(A["c1"]), P(A["c2"]), P(A["mut"]), P(A["mub"]), P(A["mu_old"]), c.time_step,
+                                ctypes.byref(c), 0, *msf, P(A["fzm"]), P(A["fzp"]), float(A["rdx"]), float(A["rdy"]),
+                                P(A["rdzw"]), float(A["dt"]), ctypes.byref(d)) == 0
+    assert L.o_advect_scalar(P(A["field"]), P(A["field"]), P(th), P(A["ru"]), P(A["rv"]), P(A["rom"]), P(A["c1"]),
+                             P(A["c2"]), P(A["mut"]), c.time_step, ctypes.byref(c), *msf, P(A["fzm"]), P(A["fzp"]),
+                             float(A["rdx"]), float(A["rdy"]), P(A["rdzw"]), ctypes.byref(d)) == 0
+    return tp, th
+
+
+def pin_advect_scalar_pd(o):
+    """advect_scalar_pd: (1) where no cell is limited it is the high-order scheme of the pinned
+    advect_scalar (its fluxes are high - low plus low); (2) the limiter keeps the updated mass
+    of every cell non-negative, which the unlimited scheme does not on the same tracer; (3) the
+    limited fluxes stay single-valued per face, so the periodic domain total is conserved."""
+    from parity import pd_advect_inputs
+    for bc, ts in (("periodic", -6), ("open", 6)):
+        d, c, a = pd_advect_inputs(bc, 14, 12, 8, 43, same_old=True)
+        c.time_step = ts
+        A = {k: np.asarray(v, o.dtype) for k, v in a.items()}
+        jj, kk, ii = np.meshgrid(*[np.arange(n) for n in A["field"].shape], indexing="ij")
+        A["field"] = (1e-3 * (1 + 0.3 * np.sin(0.7 * ii + 0.3) * np.cos(0.5 * jj) + 0.1 * np.sin(kk))).astype(o.dtype)
+        grid._halo_fill_np(A["field"], d, bc == "periodic", False, False)
+        A["field_old"] = A["field"].copy()
+        tp, th = _advect_pd_and_unlimited(o, d, c, A)
+        S = _interior3(d)
+        assert np.count_nonzero(th[S]) == th[S].size
+        _close(tp[S], th[S], 1e-13, f"unlimited advect_scalar_pd vs advect_scalar ({bc})")
+    for seed in (41, 42):
+        d, c, a = pd_advect_inputs("periodic", 14, 12, 8, seed, same_old=True)
+        c.time_step = 6
+        A = {k: np.asarray(v, o.dtype) for k, v in a.items()}
+        tp, th = _advect_pd_and_unlimited(o, d, c, A)
+        S = _interior3(d)
+        S2 = (S[0], S[2])
+        nk = d.kde - 1
+        M = A["c1"][None, :nk, None] * (A["mub"][S2] + A["mu_old"][S2])[:, None, :] + A["c2"][None, :nk, None]
+        base = M * A["field_old"][S]
+        upd = lambda t: base + float(A["dt"]) * A["msfty"][S2][:, None, :] * t[S]  # noqa: E731
+        scale = base.max()
+        assert upd(tp).min() >= -1e-12 * scale, ("positive-definite update negative", upd(tp).min() / scale)
+        assert upd(th).min() < -1e-3 * scale, "the unlimited scheme stays positive: the case does not test the limiter"
+        differ = np.abs(tp[S] - th[S]) > 1e-9 * np.abs(th[S]).max()
+        assert 0.1 < differ.mean() < 0.95, differ.mean()
+        W = 1.0 / (A["msftx"][S2][:, None, :] * np.abs(A["rdzw"][None, :nk, None]))
+        tot = abs(float((W * tp[S]).sum())) / float((W * np.abs(tp[S])).sum())
+        assert tot < 1e-13, ("periodic total of the limited transport", tot)
+
+
+def _rest_state(bc, n_passive, seed=5):
+    spec = grid.DomainSpec(nx=12, ny=10, nz=12, bc=bc, n_passive=n_passive, n_bubbles=0, noise_amp=0.0,
+                           wind_scale=0.0, seed=seed, hybrid_opt=2)
+    return grid.build_initial_state(spec)
+
+
+def pin_pd_whole_step(o):
+    """The positive-definite path of the whole step.  (1) Resting atmosphere, uniform mixing
+    ratios, a physics tendency that is tau times the column mass: after one step q = C + dt tau
+    with either transport (the positive-definite path takes it into the time-n values once,
+    with the full dt, and zeroes it; the unlimited path adds it on every substep from time n).
+    (2) A tracer with many empty cells in a bubble flow stays non-negative through whole steps
+    with the positive-definite transport, while the unlimited transport makes it negative."""
+    for pd in (0, 1):
+        hs = _rest_state("periodic", 2)
+        hs.cfg.moist_adv_opt = hs.cfg.scalar_adv_opt = pd
+        d = hs.dims
+        nk = d.kde - 1
+        sess = o.session(hs)
+        _float64_levels(sess.arrays, hs)
+        a = sess.arrays
+        M = (np.asarray(a["c1h"])[None, :nk, None] * (np.asarray(a["mu_1"]) + np.asarray(a["mub"]))[:, None, :]
+             + np.asarray(a["c2h"])[None, :nk, None])
+        taus = {1: 2e-6, 2: -3e-6}
+        for m, tau in taus.items():
+            a[f"q{m}"][...] = 0.25 * m
+            a[f"q_old{m}"][...] = 0.25 * m
+            a[f"q_tend{m}"][...] = 0.0
+            a[f"q_tend{m}"][:, :nk, :] = tau * M
+        sess.step(1)
+        S = _interior3(d)
+        dt = float(hs.scal["dt"])
+        for m, tau in taus.items():
+            q = np.asarray(a[f"q{m}"], np.float64)[S]
+            exp = 0.25 * m + dt * tau
+            # the transport keeps C to 1e-11 (pin_constant_mixing_ratio); dt tau is 1e-5 of C
+            assert np.abs(q - exp).max() <= 1e-11 * 0.25 * m + 1e-6 * abs(dt * tau), (pd, m, np.abs(q - exp).max())
+    mins = {}
+    for pd in (0, 1):
+        spec = grid.DomainSpec(nx=16, ny=14, nz=12, bc="periodic", n_passive=1, n_bubbles=2, bubble_amp=4.0, seed=11,
+                               wind_scale=1.5, hybrid_opt=2)
+        hs = grid.build_initial_state(spec)
+        hs.cfg.moist_adv_opt = hs.cfg.scalar_adv_opt = pd
+        rng = np.random.default_rng(3)
+        q = hs.q[1]
+        q[...] = (1e-3 * np.maximum(0.0, rng.normal(0.0, 1.0, q.shape)) * (rng.uniform(0, 1, q.shape) < 0.5))
+        grid._halo_fill_np(q, hs.dims, True, False, False)
+        hs.q_old[1][...] = q
+        sess = o.session(hs)
+        _float64_levels(sess.arrays, hs)
+        sess.step(3)
+        S = _interior3(hs.dims)
+        mins[pd] = float(np.asarray(sess.arrays["q1"])[S].min())
+    assert mins[1] >= -1e-15, ("positive-definite transport went negative", mins[1])
+    assert mins[0] < -1e-6, ("the unlimited transport stays positive: the case does not test the limiter", mins[0])
 
 
 def pin_resting_over_terrain(o):
@@ -780,4 +914,5 @@ def pin_resting_over_terrain(o):
 PINS = [pin_advect_constant_field, pin_advect_linear_eta, pin_coef_w_spectrum, pin_advance_w_theta_scheme,
         pin_advance_w_hydrostatic, pin_calc_p_rho_linearization, pin_advance_uv_pgf, pin_pressure_gradient,
         pin_advance_mu_t_budget,
-        pin_small_step_round_trip, pin_constant_mixing_ratio, pin_resting_over_terrain]
+        pin_small_step_round_trip, pin_constant_mixing_ratio, pin_resting_over_terrain, pin_advect_scalar_pd,
+        pin_pd_whole_step]

@@ -107,3 +107,41 @@ def domain_totals_numpy(arr: dict, hs) -> np.ndarray:
                     np.sum(dm * cv * temp * a3), np.sum(pe * a3), 0.0])
     out[5] = out[2] + out[3] + out[4]
     return out
+
+
+MSF6 = ("msfux", "msfuy", "msfvx", "msfvy", "msftx", "msfty")
+
+
+def pd_advect_inputs(bc: str, nx: int, ny: int, nz: int, seed: int, same_old: bool = False):
+    """Seeded inputs of advect_scalar_pd on a small domain: a non-negative tracer in blobs
+    with many zero cells (so that the limiter engages at the blob edges), mass fluxes at
+    Courant numbers of about a tenth, masses of a real column.  All halos are filled as the
+    model fills them (periodic wrap or zero-gradient copy), so a periodic case conserves.
+    Returns (dims, config, arrays) with float32 arrays and the float32 scalars rdx, rdy, dt."""
+    from wrf_amd import grid
+    spec = grid.DomainSpec(nx=nx, ny=ny, nz=nz, bc=bc, pad_i=1, seed=seed)
+    d, c = grid.make_dims(spec), grid.make_config(spec)
+    per = bc == "periodic"
+    s3, s2, s1 = grid.shape3(d), grid.shape2(d), grid.shape1(d)
+    rng = np.random.default_rng(seed)
+    u = lambda sh, lo, hi: rng.uniform(lo, hi, sh).astype(np.float32)  # noqa: E731
+
+    def blobs():
+        q = np.maximum(0.0, rng.normal(0.0, 1.0, s3)) * (rng.uniform(0, 1, s3) < 0.45)
+        return (1e-3 * q).astype(np.float32)
+
+    # ru = mu u / msf with u up to 17 m/s; rom = mu deta/dt with w of a few m/s
+    a = dict(field=blobs(), field_old=blobs(), ru=u(s3, -1.5e6, 1.5e6), rv=u(s3, -1.5e6, 1.5e6), rom=u(s3, -40.0, 40.0),
+             mut=u(s2, 8.8e4, 9.2e4), mub=u(s2, 8.6e4, 9.0e4), mu_old=u(s2, 1.5e3, 2.5e3), c1=u(s1, 0.6, 1.0),
+             c2=u(s1, 0.0, 5e2), fzm=u(s1, 0.4, 0.6), rdzw=u(s1, -32.0, -24.0))
+    if same_old:
+        a["field_old"] = a["field"].copy()
+    a["fzp"] = (1 - a["fzm"]).astype(np.float32)
+    for m in MSF6:
+        a[m] = u(s2, 0.97, 1.03)
+    for k, (si, sj) in dict(field=(0, 0), field_old=(0, 0), ru=(1, 0), rv=(0, 1), rom=(0, 0), mut=(0, 0), mub=(0, 0),
+                            mu_old=(0, 0), msfux=(1, 0), msfuy=(1, 0), msfvx=(0, 1), msfvy=(0, 1), msftx=(0, 0),
+                            msfty=(0, 0)).items():
+        grid._halo_fill_np(a[k], d, per, bool(si), bool(sj))
+    a["rdx"], a["rdy"], a["dt"] = np.float32(1 / 1000.), np.float32(1 / 1250.), np.float32(6.0)
+    return d, c, a

@@ -1,8 +1,9 @@
 """Pins of the CPU oracle (oracle/*.c) -- the checks that make it a trustworthy checker.
 
-* advect_scalar: bit for bit against the reference's own advect_scalar, compiled in place
-  (oracle/_ref, when the reference tree is mounted) and through the committed fixture
-  tests/golden/advect_scalar_ref.npz that the same build wrote.
+* advect_scalar and advect_scalar_pd: bit for bit against the reference's own routines,
+  compiled in place (oracle/_ref, when the reference tree is mounted) and through the
+  committed fixtures tests/golden/advect_scalar_ref.npz and advect_scalar_pd_ref.npz that the
+  same build wrote.
 * The whole step and the routines the reference cannot be compiled for here (they need
   the Registry-generated module_configure): properties the equations fix --
   a resting hydrostatic atmosphere stays at rest; a horizontally uniform state stays
@@ -83,6 +84,62 @@ def test_advect_scalar_matches_reference_build(oracle32):
         t_or = np.zeros(s3, np.float32)
         _oracle_advect_scalar(oracle32, g, 0, t_or)
         assert np.array_equal(t_ref.view(np.int32), t_or.view(np.int32)), bc
+
+
+def _oracle_advect_scalar_pd(o, g, n):
+    """o_advect_scalar_pd on case n of a fixture dict `g`; returns (tendency, h, z)."""
+    d = grid.Dims(*[int(x) for x in g[f"c{n}_dims"]])
+    c = grid.make_config(grid.DomainSpec(bc="periodic" if int(g[f"c{n}_bc"]) == 0 else "open"))
+    ts = int(g[f"c{n}_time_step"])
+    c.time_step = ts
+    vp = ctypes.c_void_p
+    P = lambda a: a.ctypes.data_as(vp)
+    L = o.lib
+    L.o_advect_scalar_pd.argtypes = ([vp] * 13 + [ctypes.c_int, vp, ctypes.c_int] + [vp] * 8
+                                     + [ctypes.c_float, ctypes.c_float, vp, ctypes.c_float, vp])
+    a = {k: np.ascontiguousarray(g[f"c{n}_{k}"]) for k in
+         ("field", "field_old", "ru", "rv", "rom", "c1", "c2", "mut", "mub", "mu_old", "fzm", "fzp", "rdzw") + MSF}
+    out = [np.zeros_like(a["field"]) for _ in range(3)]
+    rc = L.o_advect_scalar_pd(P(a["field"]), P(a["field_old"]), *[P(x) for x in out], P(a["ru"]), P(a["rv"]),
+                              P(a["rom"]), P(a["c1"]), P(a["c2"]), P(a["mut"]), P(a["mub"]), P(a["mu_old"]), ts,
+                              ctypes.byref(c), 1, *[P(a[m]) for m in MSF], P(a["fzm"]), P(a["fzp"]),
+                              float(g[f"c{n}_rdx"]), float(g[f"c{n}_rdy"]), P(a["rdzw"]), float(g[f"c{n}_dt"]),
+                              ctypes.byref(d))
+    assert rc == 0
+    return out
+
+
+@pytest.mark.parametrize("n", range(4))
+def test_advect_scalar_pd_matches_reference_fixture(oracle32, n):
+    """The positive-definite form against the reference's own advect_scalar_pd (fixture
+    tests/golden/advect_scalar_pd_ref.npz): tendency and its horizontal/vertical split."""
+    g = np.load(os.path.join(GOLDEN, "advect_scalar_pd_ref.npz"))
+    for name, got in zip(("tendency", "h_tendency", "z_tendency"), _oracle_advect_scalar_pd(oracle32, g, n)):
+        assert np.array_equal(got.view(np.int32), g[f"c{n}_{name}"].view(np.int32)), name
+        assert np.count_nonzero(got) > 100, name
+
+
+def test_advect_scalar_pd_matches_reference_build(oracle32):
+    """Fresh inputs of other shapes through the reference build itself (needs oracle/_ref)."""
+    ref_lib = os.path.join(os.path.dirname(HERE), "oracle", "_ref", "libref_advect.so")
+    if not os.path.exists(ref_lib):
+        pytest.skip("reference advection kernel not built (no reference tree)")
+    from parity import pd_advect_inputs
+    from pyoracle import RefAdvect
+    ref = RefAdvect()
+    for bc, ts, shape, seed in (("periodic", 6, (15, 12, 9), 51), ("open", -12, (13, 14, 7), 52),
+                                ("open", 6, (16, 11, 10), 53)):
+        d, c, a = pd_advect_inputs(bc, *shape, seed)
+        c.time_step = ts
+        outs = [np.zeros_like(a["field"]) for _ in range(3)]
+        ref.advect_scalar_pd(a["field"], a["field_old"], *outs, a["ru"], a["rv"], a["rom"], a["c1"], a["c2"], a["mut"],
+                             a["mub"], a["mu_old"], ts, c, 1, {m: a[m] for m in MSF}, a["fzm"], a["fzp"],
+                             float(a["rdx"]), float(a["rdy"]), a["rdzw"], float(a["dt"]), d)
+        g = {f"c0_{k}": v for k, v in a.items()}
+        g.update(c0_dims=np.array([getattr(d, n) for n, _ in d._fields_], np.int32), c0_time_step=np.int32(ts),
+                 c0_bc=np.int32(0 if bc == "periodic" else 1))
+        for name, r, o in zip(("tendency", "h", "z"), outs, _oracle_advect_scalar_pd(oracle32, g, 0)):
+            assert np.array_equal(r.view(np.int32), o.view(np.int32)), (bc, name)
 
 
 def test_sounding_matches_reference_input_sounding():

@@ -18,7 +18,7 @@ import oracle_pins
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE = os.path.join(ROOT, "oracle")
-SRCS = ["bc_prep.c", "advect.c", "tend.c", "small.c", "scalar_eos.c", "solve_step.c"]
+SRCS = ["bc_prep.c", "advect.c", "advect_pd.c", "tend.c", "small.c", "scalar_eos.c", "solve_step.c"]
 CFLAGS = ["-O1", "-fPIC", "-std=c11", "-ffp-contract=off", "-fno-fast-math", "-w", "-DWREAL=double",
           "-DWREAL_IS_DOUBLE=1"]
 
@@ -65,6 +65,20 @@ MUTANTS = [
      "A3(tend, i, k, j) + A1(rdzu, k - 1) * (T2(vflux, i, k));", 1),
     ("rk_update_scalar_old_mu_for_new", "scalar_eos.c",
      "wreal munew = A2(s->mu_2, i, j) + A2(s->mub, i, j);", "wreal munew = A2(s->mu_1, i, j) + A2(s->mub, i, j);", 1),
+    ("pd_limiter_scales_inflow_face", "advect_pd.c",
+     "if (T3(fqx, i + 1, k, j) > R(0.)) T3(fqx, i + 1, k, j) = scale * T3(fqx, i + 1, k, j);",
+     "if (T3(fqx, i + 1, k, j) < R(0.)) T3(fqx, i + 1, k, j) = scale * T3(fqx, i + 1, k, j);", 1),
+    ("pd_low_order_base_mass_perturbation_dropped", "advect_pd.c",
+     "+ (A1(c1, k) * A2(mu_old, i, j))) * A3(field_old, i, k, j)", "+ R(0.)) * A3(field_old, i, k, j)", 1),
+    ("pd_upwind_flux_downwind", "advect_pd.c",
+     "* qm1 + R(0.5) * rmax(R(-1.0), (cr - (wreal)fabs((double)cr))) * q0;",
+     "* q0 + R(0.5) * rmax(R(-1.0), (cr - (wreal)fabs((double)cr))) * qm1;", 1),
+    ("pd_flux5_upwind_term_sign", "advect_pd.c",
+     "return pflux6(qm3, qm2, qm1, q0, qp1, qp2) -", "return pflux6(qm3, qm2, qm1, q0, qp1, qp2) +", 1),
+    ("pd_prep_tendency_not_zeroed", "advect_pd.c", "A3(sc_tend, i, k, j) = R(0.);", "", 1),
+    ("pd_prep_new_mass_for_old", "solve_step.c",
+     "s->c1h, s->c2h, s->mu_1, s->mu_1, s->mub, dt_rk, &dt_);", "s->c1h, s->c2h, s->mu_2, s->mu_1, s->mub, dt_rk, &dt_);",
+     1),
     ("pressure_gradient_half_dropped", "tend.c",
      "wreal dpx = (A2(s->msfux, i, j) / A2(s->msfuy, i, j)) * R(.5) * rdx *",
      "wreal dpx = (A2(s->msfux, i, j) / A2(s->msfuy, i, j)) * rdx *", 1),
