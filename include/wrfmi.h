@@ -121,7 +121,9 @@ int wrfmi_sync_to_host(void *host_ptr);
  * rhs_ph, horizontal_pressure_gradient, pg_buoy_w, w_damp, coriolis, curvature),
  * rk_addtend_dry, small_step_prep, calc_p_rho, calc_coef_w, the acoustic loop
  * (advance_uv, advance_mu_t, advance_w, sumflux, calc_p_rho), calc_mu_uv_1,
- * small_step_finish, rk_scalar_tend/rk_update_scalar for every species,
+ * small_step_finish, rk_scalar_tend/rk_update_scalar for every species (with
+ * rk_update_scalar_pd and advect_scalar_pd on the last substep for the species whose
+ * moist_adv_opt / scalar_adv_opt is 1),
  * calc_p_rho_phi, the set_physical_bc3d/2d halo fills and set_w_surface
  * (solve_em.F:4740).  The physics tendencies *_tendf, mu_tendf and h_diabatic are
  * read as given (the reference zeroes them at rk_step 1 when physics is off:
@@ -214,16 +216,26 @@ int wrfmi_advance_mu_t_w(wrfmi_state *st, const wrfmi_config *cfg, const wrfmi_d
  * mut, muts fills (called at solve_em.F:1732-1800). */
 int wrfmi_small_step_finish(wrfmi_state *st, const wrfmi_config *cfg, const wrfmi_dims *d,
                             int rk_step, int n_small, float dts, void *stream);
-/* rk_scalar_tend (module_em.F:1096-1442, advect_scalar with ru_m, rv_m, ww_m)
- * followed by rk_update_scalar (1587-1799) for species `ispec`
- * (called at solve_em.F:2215-2309 and 2781-2856). */
+/* rk_scalar_tend (module_em.F:1096-1442, advect_scalar with ru_m, rv_m, ww_m, or at
+ * rk_step 3 advect_scalar_pd as for wrfmi_rk_scalar_tend below) followed by
+ * rk_update_scalar (1587-1799) for species `ispec` (called at solve_em.F:2215-2309 and
+ * 2781-2856). */
 int wrfmi_rk_scalar_step(wrfmi_state *st, const wrfmi_config *cfg, const wrfmi_dims *d,
                          int ispec, int rk_step, float dt_rk, void *stream);
 /* rk_scalar_tend alone (module_em.F:1096-1442: advect_tend = advect_scalar of species
  * `ispec` with ru_m, rv_m, ww_m, from zero), called at solve_em.F:2215 and 2781; a host
- * then calls rk_update_scalar with its own argument list (wrfmi_rk_update_scalar). */
+ * then calls rk_update_scalar with its own argument list (wrfmi_rk_update_scalar).  At
+ * rk_step 3 a species whose option (moist_adv_opt for the n_moist moist species,
+ * scalar_adv_opt for the others) is 1 takes advect_scalar_pd (module_advect_em.F:
+ * 6069-7885) with muts, mub, mu_1 and dt_rk instead (module_em.F:1264-1280). */
 int wrfmi_rk_scalar_tend(wrfmi_state *st, const wrfmi_config *cfg, const wrfmi_dims *d,
-                         int ispec, int rk_step, void *stream);
+                         int ispec, int rk_step, float dt_rk, void *stream);
+/* Before the last substep's species transport (solve_em.F:1850-1975): for every species in
+ * positive-definite form, rk_update_scalar_pd (module_em.F:1803-1916) takes q_tend into
+ * q_old with mu_1 + mub as both masses and zeroes q_tend, then q_old gets its halo.
+ * dt_rk is the last substep's length (dt). */
+int wrfmi_rk_scalar_pd_prep(wrfmi_state *st, const wrfmi_config *cfg, const wrfmi_dims *d,
+                            float dt_rk, void *stream);
 /* calc_p_rho_phi (module_big_step_utilities_em.F:953-1223), called at solve_em.F:2965. */
 int wrfmi_calc_p_rho_phi(wrfmi_state *st, const wrfmi_config *cfg, const wrfmi_dims *d,
                          void *stream);

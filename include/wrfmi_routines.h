@@ -74,6 +74,25 @@ int wrfmi_rk_update_scalar(const int *scs, const int *sce, float *scalar_1, floa
                            const float *c2, const float *mu_old, const float *mu_new, const float *mu_base,
                            const int *rk_step, const float *dt, const int *spec_zone,
                            const wrfmi_config *config_flags, const int *tenddec, WRFMI_DIMS_REF);
+/* advect_scalar_pd, dyn_em/module_advect_em.F:6069-6083 (module_em.F:1266, rk_scalar_tend on
+ * the last RK substep for moist_adv_opt / scalar_adv_opt 1): tendency += the positive-definite
+ * flux divergence (5th-order horizontal, 3rd-order vertical, Smolarkiewicz upwind fluxes from
+ * field_old and the limiter with mub + mu_old).  tenddec's split diagnostics are refused.
+ * The limiter's scales use a work array of the domain known by `tendency`. */
+int wrfmi_advect_scalar_pd(const float *field, const float *field_old, float *tendency, float *h_tendency,
+                           float *z_tendency, const float *ru, const float *rv, const float *rom, const float *c1,
+                           const float *c2, const float *mut, const float *mub, const float *mu_old,
+                           const int *time_step, const wrfmi_config *config_flags, const int *tenddec,
+                           const float *msfux, const float *msfuy, const float *msfvx, const float *msfvy,
+                           const float *msftx, const float *msfty, const float *fzm, const float *fzp,
+                           const float *rdx, const float *rdy, const float *rdzw, const float *dt, WRFMI_DIMS_REF);
+/* rk_update_scalar_pd, dyn_em/module_em.F:1803-1811 (solve_em.F:1858, 1921 with mu_1 as both
+ * mu_old and mu_new): scalar = ((c1 (mu_old + mu_base) + c2) scalar + dt sc_tend) /
+ * (c1 (mu_new + mu_base) + c2) over the mass points, then sc_tend = 0; species scs..sce. */
+int wrfmi_rk_update_scalar_pd(const int *scs, const int *sce, float *scalar, float *sc_tend, const float *c1,
+                              const float *c2, const float *mu_old, const float *mu_new, const float *mu_base,
+                              const int *rk_step, const float *dt, const int *spec_zone,
+                              const wrfmi_config *config_flags, WRFMI_DIMS_REF);
 /* advance_uv, dyn_em/module_small_step_em.F:654-672 (solve_em.F:1279). */
 int wrfmi_advance_uv(float *u, const float *ru_tend, float *v, const float *rv_tend, const float *p, const float *pb,
                      const float *ph, const float *php, const float *alt, const float *al, const float *mu,

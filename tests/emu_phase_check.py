@@ -37,7 +37,39 @@ CASES = {
     "terrain_yopen": dict(nx=13, ny=14, nz=9, dx=2000.0, dy=2000.0, dt=6.0, bc_x="periodic", bc_y="open",
                           hybrid_opt=2, sounding="stable", n_ridges=2, ridge_a=(3000.0, 8000.0), n_passive=2,
                           n_bubbles=0, noise_amp=0.0, seed=20),
+    # positive-definite transport on the last substep (moist_adv_opt, scalar_adv_opt), passive
+    # scalars half empty and seeded physics tendencies (pd_parity.pd_state)
+    "pd_periodic": dict(nx=14, ny=11, nz=10, bc="periodic", n_passive=2, n_bubbles=2, bubble_amp=2.0, seed=26,
+                        pd=(1, 1)),
+    "pd_open": dict(nx=13, ny=12, nz=9, bc="open", n_passive=1, n_bubbles=1, seed=27, pd=(1, 1)),
+    "pd_mixed_unaligned": dict(nx=14, ny=11, nz=8, bc_x="open", bc_y="periodic", n_passive=1, n_bubbles=1, seed=28,
+                               pad_i=1, pd=(0, 1)),
 }
+
+
+def build_case(case: str):
+    """The initial state of CASES[case] (pd_parity.pd_state for the positive-definite cases)."""
+    kw = dict(CASES[case])
+    pd = kw.pop("pd", None)
+    if pd is None:
+        return grid.build_initial_state(grid.DomainSpec(**kw))
+    from pd_parity import pd_state
+    return pd_state(kw, *pd, seed=kw["seed"])
+
+
+def pd_routines_check(order: int):
+    """wrfmi_advect_scalar_pd against the reference's fixture and the oracle, and
+    wrfmi_rk_update_scalar_pd against the oracle."""
+    import ctypes
+    from emu_backend import EMU_LIB
+    from pd_parity import fixture_failures, update_pd_failures
+    lib = ctypes.CDLL(EMU_LIB)
+    lib.emu_set_launch_order(order)
+    fails = fixture_failures(lib, Oracle("f32")) + update_pd_failures(lib, Oracle("f32"))
+    for f in fails:
+        print("MISMATCH", f)
+    print("OK pd_routines" if not fails else "FAILED pd_routines", "order", order, flush=True)
+    sys.exit(1 if fails else 0)
 
 
 def registry_check():
@@ -158,8 +190,7 @@ def routines_check(case: str, order: int):
     against the oracle's whole step."""
     import ctypes
     from routine_parity import routine_parity
-    spec = grid.DomainSpec(**CASES[case])
-    hs = grid.build_initial_state(spec)
+    hs = build_case(case)
     fails = []
     for rk in (1, 2):
         emu = EmuDomain(hs, order=order)
@@ -307,11 +338,13 @@ def main():
     if sys.argv[1] == "ensemble":
         ensemble_check(int(sys.argv[2]), int(sys.argv[3]))
         return
+    if sys.argv[1] == "pd_routines":
+        pd_routines_check(int(sys.argv[2]))
+        return
     case, order, nsteps = sys.argv[1], int(sys.argv[2]), int(sys.argv[3])
-    spec = grid.DomainSpec(**CASES[case])
-    hs = grid.build_initial_state(spec)
+    hs = build_case(case)
     emu = EmuDomain(hs, order=order)
-    phases = runtime.step_phases(hs.cfg, hs.scal, hs.n_species)
+    phases = runtime.step_phases(hs.cfg, hs.scal, hs.n_species, hs.n_moist)
     # bitwise against the oracle whose EOS rounds LOG and ** once from double, as the
     # kernels do; then the whole step through wrfmi_solve_em_step
     sess = Oracle("f32cr").session(hs)

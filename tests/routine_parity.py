@@ -236,7 +236,7 @@ def routine_parity(backend, session, hs, rk_step=2):
             return fails
     session.phase("small_step_finish", rk_step, nsmall, dts)
     for m in range(hs.n_species):
-        O.state_routine("rk_scalar_tend", ctypes.c_int(m))  # advect_tend of species m
+        O.state_routine("rk_scalar_tend", ctypes.c_int(m), ctypes.c_int(rk_step), O.R(dt_rk))  # advect_tend of m
         check(f"rk_update_scalar species {m}", lambda: C.rk_update_scalar(m, rk_step, float(dt_rk)),
               lambda: O.rk_update_scalar(m, rk_step, float(dt_rk)))
     return fails

@@ -62,7 +62,8 @@ def test_ensemble_arguments_checked_before_the_device():
     ("specified", 1, "lateral"),
     ("diff_opt", 1, "diffusion"),
     ("damp_opt", 3, "damp_opt"),
-    ("moist_adv_opt", 1, "advection options"),
+    ("moist_adv_opt", 2, "advection options"),
+    ("scalar_adv_opt", 3, "advection options"),
 ])
 def test_unsupported_options_are_rejected(field, value, msg):
     lib = runtime.load_library()
@@ -97,6 +98,16 @@ def test_step_phase_sequence():
     assert names.count("rk_scalar_step") == 3
     assert names[-3:] == ["calc_p_rho_phi", "phys_bc_dry_2", "set_w_surface"]
     assert runtime.num_sound_steps(type(hs_cfg)(), 6.0, 1000.0, 1000.0) == 4
+    # positive-definite species: rk_update_scalar_pd's group once, before the last substep's species
+    hs_cfg.scalar_adv_opt = 1
+    ph = runtime.step_phases(hs_cfg, dict(dt=12.0, dx=2000.0, dy=2000.0), 3, n_moist=1)
+    names = [n for n, _ in ph]
+    assert names.count("rk_scalar_pd_prep") == 1
+    i = names.index("rk_scalar_pd_prep")
+    assert names[i - 1] == "small_step_finish" and names[i + 1:i + 4] == ["rk_scalar_step"] * 3
+    assert [a for n, a in ph if n == "rk_scalar_step"][-1][1] == 3
+    hs_cfg.scalar_adv_opt = 0
+    assert "rk_scalar_pd_prep" not in [n for n, _ in runtime.step_phases(hs_cfg, dict(dt=12.0, dx=2000.0, dy=2000.0), 3)]
 
 
 def test_reference_argument_entry_points_check_before_the_device():

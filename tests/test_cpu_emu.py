@@ -29,7 +29,8 @@ def emu_env():
                                                ("periodic_unaligned", 0, 1), ("open_unaligned", 1, 1),
                                                ("periodic_tight", 1, 1), ("open_tight", 0, 1),
                                                ("terrain", 0, 2), ("terrain_yopen", 1, 1),
-                                               ("periodic_wide", 1, 1), ("open_wide", 0, 1), ("open_narrow", 1, 2)])
+                                               ("periodic_wide", 1, 1), ("open_wide", 0, 1), ("open_narrow", 1, 2),
+                                               ("pd_periodic", 0, 2), ("pd_open", 1, 1), ("pd_mixed_unaligned", 0, 1)])
 def test_emulated_kernels_match_oracle(emu_env, case, order, nsteps):
     r = subprocess.run([sys.executable, os.path.join(HERE, "emu_phase_check.py"), case, str(order), str(nsteps)],
                        env=emu_env, capture_output=True, text=True, timeout=600)
@@ -61,7 +62,8 @@ def test_emulated_noalloc_step_matches_allocating_step(emu_env, order):
     assert "OK noalloc" in r.stdout
 
 
-@pytest.mark.parametrize("case,order", [("periodic", 0), ("open", 1), ("terrain_yopen", 0), ("periodic_unaligned", 1)])
+@pytest.mark.parametrize("case,order", [("periodic", 0), ("open", 1), ("terrain_yopen", 0), ("periodic_unaligned", 1),
+                                        ("pd_open", 1)])
 def test_emulated_reference_argument_entry_points(emu_env, case, order):
     """The entry points with the reference's argument lists against the oracle routines of
     the same name, and two steps of the Fortran solve_em host that calls them."""
@@ -98,3 +100,13 @@ def test_emulated_launch_shapes_give_identical_results(emu_env, case):
                        env=emu_env, capture_output=True, text=True, timeout=1500)
     assert r.returncode == 0, r.stdout[-4000:] + r.stderr[-4000:]
     assert "OK shapes" in r.stdout
+
+
+@pytest.mark.parametrize("order", [0, 1])
+def test_emulated_positive_definite_routines(emu_env, order):
+    """wrfmi_advect_scalar_pd against the reference's own advect_scalar_pd (fixture) and the
+    oracle, wrfmi_rk_update_scalar_pd against the oracle, bit for bit."""
+    r = subprocess.run([sys.executable, os.path.join(HERE, "emu_phase_check.py"), "pd_routines", str(order)],
+                       env=emu_env, capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stdout[-4000:] + r.stderr[-4000:]
+    assert "OK pd_routines" in r.stdout

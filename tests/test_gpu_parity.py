@@ -58,7 +58,7 @@ def test_per_routine_bitwise(gpu, case):
     print(f"per-routine parity, case {case}", flush=True)
     dev = runtime.DeviceDomain(hs)
     sess = Oracle("f32cr").session(hs)
-    phases = runtime.step_phases(hs.cfg, hs.scal, hs.n_species)
+    phases = runtime.step_phases(hs.cfg, hs.scal, hs.n_species, hs.n_moist)
     fails = phase_parity(dev, sess, phases, nsteps=2, eos_ulps=1)
     assert not fails, "\n".join(fails)
 
@@ -71,7 +71,7 @@ def test_per_routine_vs_libm_oracle(gpu, oracle32, case):
     print(f"per-routine parity (libm EOS), case {case}", flush=True)
     dev = runtime.DeviceDomain(hs)
     sess = oracle32.session(hs)
-    phases = runtime.step_phases(hs.cfg, hs.scal, hs.n_species)
+    phases = runtime.step_phases(hs.cfg, hs.scal, hs.n_species, hs.n_moist)
     fails = phase_parity(dev, sess, phases, nsteps=1)
     assert not fails, "\n".join(fails)
 

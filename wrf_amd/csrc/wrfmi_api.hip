@@ -254,8 +254,10 @@ static int check_cfg(const wrfmi_config *c) {
     return fail(WRFMI_E_BADOPT, "lateral boundaries: periodic and open are implemented (not specified/nested/polar/symmetric)");
   if (!c->periodic_x && !(c->open_xs && c->open_xe)) return fail(WRFMI_E_BADOPT, "x boundaries must be periodic or open on both sides");
   if (!c->periodic_y && !(c->open_ys && c->open_ye)) return fail(WRFMI_E_BADOPT, "y boundaries must be periodic or open on both sides");
-  if (c->moist_adv_opt != 0 || c->scalar_adv_opt != 0 || c->momentum_adv_opt != 1)
-    return fail(WRFMI_E_BADOPT, "advection options: only the original (non-PD, non-WENO) fluxes are implemented");
+  if (c->moist_adv_opt < 0 || c->moist_adv_opt > 1 || c->scalar_adv_opt < 0 || c->scalar_adv_opt > 1 ||
+      c->momentum_adv_opt != 1)
+    return fail(WRFMI_E_BADOPT, "advection options: the original fluxes and positive-definite scalar transport "
+                "(moist/scalar_adv_opt 0 or 1, momentum_adv_opt 1) are implemented, not monotonic or WENO");
   if (!c->non_hydrostatic) return fail(WRFMI_E_BADOPT, "hydrostatic mode is not implemented");
   if (c->diff_opt != 0 || c->diff_6th_opt != 0 || c->damp_opt == 3 || c->zadvect_implicit || c->pert_coriolis || c->map_proj == 6)
     return fail(WRFMI_E_BADOPT, "diffusion, damp_opt=3, IEVA, perturbation Coriolis and lat-lon grids are outside the implemented set");
@@ -341,12 +343,14 @@ static const dim3 kVecBlk(64, WRFMI_VEC_ROWS, 1);
 enum KernelId {
   KID_STEP_PREP, KID_TEND_U, KID_TEND_V, KID_TEND_W, KID_TEND_T, KID_SMALL_PREP, KID_ADVANCE_UV,
   KID_AC_DIV, KID_SMALL_FINISH, KID_ADV_SCALAR, KID_UPDATE_SCALAR, KID_P_RHO_PHI, KID_HALO, KID_P_RHO0,
-  KID_COEF_W, KID_WW_SCAN, KID_AC_MU_WW, KID_AC_THETA, KID_AC_WSOLVE, KID_W_SURFACE, KID_COUNT
+  KID_COEF_W, KID_WW_SCAN, KID_AC_MU_WW, KID_AC_THETA, KID_AC_WSOLVE, KID_W_SURFACE, KID_PD_SCALE, KID_PD_TEND,
+  KID_UPDATE_SCALAR_PD, KID_COUNT
 };
 static const char *kKernelNames[KID_COUNT] = {
   "k_step_prep", "k_tend_u", "k_tend_v", "k_tend_w", "k_tend_t", "k_small_prep", "k_advance_uv",
   "k_ac_div", "k_small_finish", "k_adv_scalar", "k_update_scalar", "k_p_rho_phi", "k_halo", "k_p_rho0",
-  "k_coef_w", "k_ww_scan", "k_ac_mu_ww", "k_ac_theta", "k_ac_wsolve", "k_set_w_surface"};
+  "k_coef_w", "k_ww_scan", "k_ac_mu_ww", "k_ac_theta", "k_ac_wsolve", "k_set_w_surface", "k_pd_scale", "k_pd_tend",
+  "k_update_scalar_pd"};
 struct EvPair { int id; hipEvent_t a, b; };
 static unsigned g_timing = 0;  // bit n set: time launches of kernel n
 static std::vector<EvPair> g_ev_pending;
@@ -478,7 +482,9 @@ static Shape default_shape(int kid) {
   }
 }
 // which of a kernel's shape parameters are tunable, and the values each may take
-static bool tunable_rows(int kid) { return kid != KID_HALO && kid != KID_W_SURFACE; }
+static bool tunable_rows(int kid) {
+  return kid != KID_HALO && kid != KID_W_SURFACE && kid != KID_PD_SCALE && kid != KID_PD_TEND && kid != KID_UPDATE_SCALAR_PD;
+}
 static bool tunable_levels(int kid) { return kid == KID_ADVANCE_UV || kid == KID_AC_DIV; }
 static bool tunable_prefetch(int kid) { return kid == KID_AC_WSOLVE; }
 struct TuneEntry {
@@ -715,8 +721,51 @@ static void ph_small_finish(Launcher &L) {
     L.halo_flush();
   }
 }
-// rk_scalar_tend's advect_scalar of species L.a.species into advect_tend
+// species m's transport option: moist species follow moist_adv_opt, the others
+// scalar_adv_opt; 1 is the positive-definite form on the last substep
+static bool pd_species(const KArgs &a, int m) {
+  return (m < a.s.n_moist ? a.c.moist_adv_opt : a.c.scalar_adv_opt) == 1;
+}
+static bool any_pd(const KArgs &a) {
+  for (int m = 0; m < a.s.n_species; m++)
+    if (pd_species(a, m)) return true;
+  return false;
+}
+// solve_em.F:1850-1975: before the last substep's transport, rk_update_scalar_pd takes each
+// positive-definite species' physics tendency into its time-n values (mu_1 as both masses,
+// dt_rk, tendency zeroed), then those values get their halo.  a.dt_rk is the substep's dt.
+static void ph_scalar_pd_prep(Launcher &L) {
+  const wrfmi_state &s = L.a.s;
+  const Dm &d = L.a.d;
+  KArgs a = L.a;
+  a.s.mu_2 = a.s.mu_1;  // solve_em passes mu_1 as both masses
+  for (int m = 0; m < s.n_species; m++) {
+    if (!pd_species(a, m)) continue;
+    a.species = m;
+    L.launchK(KID_UPDATE_SCALAR_PD, KR(k_update_scalar_pd), g_pt3(d, d.kde - d.kds), kColBlk, a);
+  }
+  L.halo_begin();
+  for (int m = 0; m < s.n_species; m++)
+    if (pd_species(L.a, m)) L.halo_add(s.q_old[m], 'p', 1);
+  L.halo_flush();
+}
+// advect_scalar_pd of species L.a.species into advect_tend (module_em.F:1264-1280): the
+// limiter's scales into the dvdxi work array, then the limited divergence
+static void ph_scalar_tend_pd(Launcher &L) {
+  const Dm &d = L.a.d;
+  const int nkz = (d.kde - d.kds + kPdK - 1) / kPdK;
+  const dim3 gs = col_grid(d, d.ids - 1, d.jds - 1, kColBlk);
+  L.launchK(KID_PD_SCALE, KR(k_pd_scale), dim3(gs.x, gs.y, nkz), kColBlk, L.a);
+  const int nkt = (d.kde - d.kds + 1 + kPdK - 1) / kPdK;
+  L.launchK(KID_PD_TEND, KR(k_pd_tend), g_pt3(d, nkt), kColBlk, L.a);
+}
+// rk_scalar_tend's advect_scalar of species L.a.species into advect_tend (the
+// positive-definite form on the last substep for a species that asks for it)
 static void ph_scalar_tend(Launcher &L) {
+  if (L.a.rk_step == 3 && pd_species(L.a, L.a.species)) {
+    ph_scalar_tend_pd(L);
+    return;
+  }
   const bool per = L.a.c.periodic_x && L.a.c.periodic_y;
   const Dm &d = L.a.d;
   const dim3 ga((d.ide - d.ids + 1 + kAdvTileI - 1) / kAdvTileI, (d.jde - d.jds + 1 + kAdvRows - 1) / kAdvRows,
@@ -795,7 +844,9 @@ static void enqueue_step(Launcher &L) {
       ph_advance_mu_t_w(L);
     }
     ph_small_finish(L);
-    if (L.scal_all) {
+    const bool pd = rk == 3 && any_pd(a);
+    if (pd) ph_scalar_pd_prep(L);
+    if (L.scal_all && !pd) {
       ph_scalar_all(L);
     } else {
       for (int m = 0; m < a.s.n_species; m++) {
@@ -1397,13 +1448,22 @@ extern "C" int wrfmi_rk_scalar_step(wrfmi_state *st, const wrfmi_config *cfg, co
   });
 }
 extern "C" int wrfmi_rk_scalar_tend(wrfmi_state *st, const wrfmi_config *cfg, const wrfmi_dims *d, int ispec, int rk_step,
-                                    void *stream) {
+                                    float dt_rk, void *stream) {
   return run_phase(st, cfg, d, stream, [&](Launcher &L) {
     if (int rc = check_rk(rk_step)) return rc;
     if (ispec < 0 || ispec >= L.a.s.n_species) return fail(WRFMI_E_BADOPT, "species %d outside 0..%d", ispec, L.a.s.n_species - 1);
     L.a.rk_step = rk_step;
     L.a.species = ispec;
+    L.a.dt_rk = dt_rk;
     ph_scalar_tend(L);
+    return 0;
+  });
+}
+extern "C" int wrfmi_rk_scalar_pd_prep(wrfmi_state *st, const wrfmi_config *cfg, const wrfmi_dims *d, float dt_rk,
+                                       void *stream) {
+  return run_phase(st, cfg, d, stream, [&](Launcher &L) {
+    L.a.dt_rk = dt_rk;
+    ph_scalar_pd_prep(L);
     return 0;
   });
 }

@@ -1,5 +1,6 @@
 // Per-routine entry points that keep the reference's argument lists (declared in
-// include/wrfmi.h): advect_u/v/w/scalar, rk_update_scalar, advance_uv, advance_mu_t,
+// include/wrfmi_routines.h): advect_u/v/w/scalar, advect_scalar_pd, rk_update_scalar,
+// rk_update_scalar_pd, advance_uv, advance_mu_t,
 // advance_w, sumflux, calc_p_rho and calc_coef_w, each with the arguments, in the order, of
 // the Fortran subroutine it replaces, all by reference (as a Fortran caller passes them).
 //
@@ -96,10 +97,11 @@ struct Call {
     a.routine = routine;
     return a;
   }
-  void scratch(KArgs &a) {
+  // the work arrays of the domain known by the device array `key` (by default its u_2 or w_2)
+  void scratch(KArgs &a, const void *key = nullptr) {
     if (rc) return;
     float *dv = nullptr, *rhs = nullptr, *lo = nullptr;
-    const void *key = s.u_2 ? (const void *)s.u_2 : (const void *)s.w_2;
+    if (!key) key = s.u_2 ? (const void *)s.u_2 : (const void *)s.w_2;
     if ((rc = domain_scratch(key, d, &dv, &rhs, &lo))) return;
     a.scr_dv = FPtr<float>(dv);
     a.scr_rhs = FPtr<float>(rhs);
@@ -272,6 +274,84 @@ extern "C" int wrfmi_rk_update_scalar(const int *scs, const int *sce, float *sca
     a.species = m;
     launch_tiles(k_update_scalar<KArgs>, pt3(C.d, C.d.kde - C.d.kds), kPtBlk, a, st);
   }
+  return finish(C);
+}
+
+extern "C" int wrfmi_rk_update_scalar_pd(const int *scs, const int *sce, float *scalar, float *sc_tend, const float *c1,
+                                         const float *c2, const float *mu_old, const float *mu_new, const float *mu_base,
+                                         const int *rk_step, const float *dt, const int *spec_zone,
+                                         const wrfmi_config *config_flags, WRFMI_DIMS_REF) {
+  const int *const dims[18] = {DIMS_PTRS};
+  Call C(config_flags, dims);
+  wrfmi_state &s = C.s;
+  const int s0 = C.iscalar(scs, "scs"), s1 = C.iscalar(sce, "sce");
+  C.iscalar(rk_step, "rk_step");      // not read by the routine
+  C.iscalar(spec_zone, "spec_zone");  // its loop bounds apply to specified boundaries only
+  if (!C.rc && (s1 < s0 || s1 - s0 >= WRFMI_MAX_SPECIES))
+    C.rc = fail(WRFMI_E_BADOPT, "scs..sce = %d..%d: between 1 and %d species", s0, s1, WRFMI_MAX_SPECIES);
+  C.bind(s.c1h, c1, "c1"); C.bind(s.c2h, c2, "c2");
+  C.bind(s.mu_1, mu_old, "mu_old"); C.bind(s.mu_2, mu_new, "mu_new"); C.bind(s.mub, mu_base, "mu_base");
+  float dt_rk = 0.f;
+  C.scalar(dt_rk, dt, "dt");
+  if (C.rc) return C.rc;
+  // scalar(ims:ime, kms:kme, jms:jme, scs:sce): species im at (im - scs) fields
+  const size_t field = (size_t)C.d.sj * (size_t)(C.d.jme - C.d.jms + 1);
+  const int n = s1 - s0 + 1;
+  for (int m = 0; m < n; m++) {
+    C.bind(s.q_old[m], scalar ? scalar + m * field : nullptr, "scalar");
+    C.bind(s.q_tend[m], sc_tend ? sc_tend + m * field : nullptr, "sc_tend");
+  }
+  s.n_species = n;
+  if (C.rc) return C.rc;
+  KArgs a = C.args(R_FUSED);
+  a.dt_rk = dt_rk;
+  const hipStream_t st = routine_stream();
+  for (int m = 0; m < n; m++) {
+    a.species = m;
+    launch_tiles(k_update_scalar_pd<KArgs>, pt3(C.d, C.d.kde - C.d.kds), kPtBlk, a, st);
+  }
+  return finish(C);
+}
+
+extern "C" int wrfmi_advect_scalar_pd(const float *field, const float *field_old, float *tendency, float *h_tendency,
+                                      float *z_tendency, const float *ru, const float *rv, const float *rom,
+                                      const float *c1, const float *c2, const float *mut, const float *mub,
+                                      const float *mu_old, const int *time_step, const wrfmi_config *config_flags,
+                                      const int *tenddec, const float *msfux, const float *msfuy, const float *msfvx,
+                                      const float *msfvy, const float *msftx, const float *msfty, const float *fzm,
+                                      const float *fzp, const float *rdx, const float *rdy, const float *rdzw,
+                                      const float *dt, WRFMI_DIMS_REF) {
+  const int *const dims[18] = {DIMS_PTRS};
+  Call C(config_flags, dims);
+  wrfmi_state &s = C.s;
+  (void)h_tendency; (void)z_tendency; (void)msfux; (void)msfuy; (void)msfvx; (void)msfvy;
+  if (!C.rc && C.iscalar(tenddec, "tenddec"))
+    C.rc = fail(WRFMI_E_BADOPT, "tenddec: the split horizontal/vertical tendency diagnostics are not implemented");
+  C.bind(s.q[0], field, "field"); C.bind(s.q_old[0], field_old, "field_old");
+  C.bind(s.advect_tend, tendency, "tendency");
+  C.bind(s.ru_m, ru, "ru"); C.bind(s.rv_m, rv, "rv"); C.bind(s.ww_m, rom, "rom");
+  C.bind(s.c1h, c1, "c1"); C.bind(s.c2h, c2, "c2");
+  C.bind(s.muts, mut, "mut"); C.bind(s.mub, mub, "mub"); C.bind(s.mu_1, mu_old, "mu_old");
+  C.bind(s.msftx, msftx, "msftx"); C.bind(s.msfty, msfty, "msfty");
+  C.bind(s.fnm, fzm, "fzm"); C.bind(s.fnp, fzp, "fzp"); C.bind(s.rdnw, rdzw, "rdzw");
+  C.scalar(s.rdx, rdx, "rdx");
+  C.scalar(s.rdy, rdy, "rdy");
+  float dt_step = 0.f;
+  C.scalar(dt_step, dt, "dt");
+  C.c.time_step = C.iscalar(time_step, "time_step");
+  s.n_species = 1;
+  if (C.rc) return C.rc;
+  KArgs a = C.args(R_ADVECT);
+  a.dt_rk = dt_step;
+  // the limiter's scales go to the dvdxi work array of the domain known by its tendency array
+  // (advect_tend at every call site)
+  C.scratch(a, s.advect_tend);
+  if (C.rc) return C.rc;
+  const Dm &d = C.d;
+  const hipStream_t st = routine_stream();
+  const dim3 gs = col_grid(d, d.ids - 1, d.jds - 1, kPtBlk);
+  launch_tiles(k_pd_scale<KArgs>, dim3(gs.x, gs.y, (d.kde - d.kds + kPdK - 1) / kPdK), kPtBlk, a, st);
+  launch_tiles(k_pd_tend<KArgs>, pt3(d, (d.kde - d.kds + 1 + kPdK - 1) / kPdK), kPtBlk, a, st);
   return finish(C);
 }
 

@@ -163,10 +163,26 @@ contains
 
       if (chk(wrfmi_small_step_finish(st, cfg, dm, rk_step, number_of_small_timesteps, dts_rk, stream))) exit rk_loop
 
-      ! moist and passive species: rk_scalar_tend, then rk_update_scalar for one species
+      ! positive-definite transport (moist_adv_opt / scalar_adv_opt 1): before the last
+      ! substep's transport, each such species takes its physics tendency into its time-n
+      ! values with mu_1 as both masses, and those get their halo (solve_em.F:1850-1975)
+      if (rk_step == 3) then
+        do im = 1, st%n_species
+          if (.not. pd_species(im)) cycle
+          call c_f_pointer(st%q_old(im), q_old, [n3])
+          call c_f_pointer(st%q_tend(im), q_tend, [n3])
+          if (chk(wrfmi_rk_update_scalar_pd(one, one, q_old, q_tend, c1h, c2h, mu_1, mu_1, mub, rk_step, dt_rk, &
+              spec_zone, cfg, ids, ide, jds, jde, kds, kde, ims, ime, jms, jme, kms, kme, its, ite, jts, jte, &
+              kts, kte))) exit rk_loop
+          if (chk(wrfmi_set_physical_bc3d(st%q_old(im), 'p', cfg, dm, stream))) exit rk_loop
+        end do
+      end if
+
+      ! moist and passive species: rk_scalar_tend (advect_scalar_pd on the last substep for a
+      ! positive-definite species), then rk_update_scalar for one species
       ! (solve_em.F:2215-2309, 2781-2856)
       do im = 1, st%n_species
-        if (chk(wrfmi_rk_scalar_tend(st, cfg, dm, im - 1, rk_step, stream))) exit rk_loop
+        if (chk(wrfmi_rk_scalar_tend(st, cfg, dm, im - 1, rk_step, dt_rk, stream))) exit rk_loop
         call c_f_pointer(st%q(im), q, [n3])
         call c_f_pointer(st%q_old(im), q_old, [n3])
         call c_f_pointer(st%q_tend(im), q_tend, [n3])
@@ -190,6 +206,16 @@ contains
     if (chk(wrfmi_set_w_surface(st, cfg, dm, stream))) return
 
   contains
+
+    ! species im (1-based; moist species first) is advected in positive-definite form
+    logical function pd_species(im)
+      integer(c_int), intent(in) :: im
+      if (im <= st%n_moist) then
+        pd_species = cfg%moist_adv_opt == 1
+      else
+        pd_species = cfg%scalar_adv_opt == 1
+      end if
+    end function pd_species
 
     ! true (and the status kept as the function result) when a call failed: the reference's
     ! host would call wrf_error_fatal with wrfmi_last_error() here

@@ -44,7 +44,8 @@ PHASE_ARGS = {
     "advance_mu_t_w": "fii",      # dts, iteration, n_small
     "small_step_finish": "iif",   # rk_step, n_small, dts
     "rk_scalar_step": "iif",      # ispec, rk_step, dt_rk
-    "rk_scalar_tend": "ii",       # ispec, rk_step
+    "rk_scalar_tend": "iif",      # ispec, rk_step, dt_rk
+    "rk_scalar_pd_prep": "f",     # dt_rk
     "calc_p_rho_phi": "",
     "phys_bc_dry_2": "i",         # kind
     "set_w_surface": "",
@@ -60,7 +61,13 @@ def num_sound_steps(cfg, dt: float, dx: float, dy: float) -> int:
     return max(n, 4)
 
 
-def step_phases(cfg, scal: dict, n_species: int):
+def pd_species(cfg, n_moist: int, m: int) -> bool:
+    """Species m takes the positive-definite transport on the last substep: moist species
+    (m < n_moist) follow moist_adv_opt, the others scalar_adv_opt (1 = positive definite)."""
+    return (cfg.moist_adv_opt if m < n_moist else cfg.scalar_adv_opt) == 1
+
+
+def step_phases(cfg, scal: dict, n_species: int, n_moist: int = 1):
     """The per-routine calls of one RK3 time step, in solve_em order, as (name, args) with
     FP32 substep lengths computed as the reference computes them (solve_em.F:601-615)."""
     f32 = np.float32
@@ -81,6 +88,8 @@ def step_phases(cfg, scal: dict, n_species: int):
             out.append(("advance_uv_state", (float(dts),)))
             out.append(("advance_mu_t_w", (float(dts), it, nsmall)))
         out.append(("small_step_finish", (rk, nsmall, float(dts))))
+        if rk == 3 and any(pd_species(cfg, n_moist, m) for m in range(n_species)):
+            out.append(("rk_scalar_pd_prep", (float(dt_rk),)))
         for m in range(n_species):
             out.append(("rk_scalar_step", (m, rk, float(dt_rk))))
         out.append(("calc_p_rho_phi", ()))
