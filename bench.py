@@ -12,6 +12,8 @@ events on its launch stream over the timed steps, against 8 TB/s; its algorithmi
 the 3-D arrays it must read and write once per launch.
 ensemble = member-steps/s of the 64-member uneven ensemble, member by member with the plain
 kernels and batched (one launch per kernel over all members, hipGraph replay).
+positive_definite = the same workload with positive-definite species transport on the last
+substep (moist_adv_opt = scalar_adv_opt = 1), and its three kernels against 8 TB/s.
 terrain (with --terrain) = the 1200x1200x80 terrain case, FP32 and precision forms.
 cpu_baseline = the FP32 C restatement of the same step (the oracle), OpenMP over the host
 cores with solve_em's j-tiling, on the same workload: median of 3 single steps.
@@ -156,6 +158,53 @@ def ensemble_leg(runtime, grid, torch, n_members, steps, log):
     return out
 
 
+# positive-definite transport (moist_adv_opt = scalar_adv_opt = 1), per species on the last
+# RK substep: 3-D FP32 arrays each kernel must read + write once per launch
+PD_FIELDS = {
+    "k_update_scalar_pd": 4,  # q_old q_tend -> q_old q_tend
+    "k_pd_scale": 6,          # q q_old ru_m rv_m ww_m -> scale
+    "k_pd_tend": 7,           # q q_old ru_m rv_m ww_m scale -> advect_tend
+}
+
+
+def pd_leg(runtime, dev, torch, stream, steps, pts, log):
+    """The same workload with every species in positive-definite form (advect_scalar_pd and
+    rk_update_scalar_pd on the last substep, moist_adv_opt = scalar_adv_opt = 1): ms per step
+    over `steps` steps after one untimed step, then the three kernels' average launch times
+    (HIP events on the launch stream, a separate run of `steps` steps) against 8 TB/s with
+    their algorithmic bytes."""
+    cfg = dev.cfg
+    saved = (cfg.moist_adv_opt, cfg.scalar_adv_opt)
+    cfg.moist_adv_opt = cfg.scalar_adv_opt = 1
+    try:
+        dev.step(1, stream=stream)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        dev.step(steps, stream=stream, sync=False)
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        with runtime.KernelTimer(list(PD_FIELDS)) as timer:
+            dev.step(steps, stream=stream)
+            kt = timer.collect()
+        nbad = dev.check_finite()
+        if nbad:
+            raise RuntimeError(f"positive definite: {nbad} non-finite values after the timed steps")
+    finally:
+        cfg.moist_adv_opt, cfg.scalar_adv_opt = saved
+    out = {"ms_per_step": round(1e3 * elapsed / steps, 4), "mpoint_steps_per_s": round(pts * steps / elapsed / 1e6, 1),
+           "kernels": {}}
+    for k, nf in PD_FIELDS.items():
+        n, ms = kt[k]
+        avg = ms / max(n, 1)
+        gbs = nf * 4 * pts / (avg * 1e-3) / 1e9
+        out["kernels"][k] = {"launches_per_step": n // steps, "avg_launch_ms": round(avg, 4),
+                             "alg_bytes_per_launch": nf * 4 * pts, "achieved_gbs": round(gbs, 1),
+                             "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    log(f"  positive definite: {out['ms_per_step']} ms/step; " + ", ".join(
+        f"{k} {v['avg_launch_ms']} ms ({v['achieved_gbs']} GB/s)" for k, v in out["kernels"].items()))
+    return out
+
+
 def terrain_leg(runtime, grid, torch, steps, log):
     """The 1200x1200x80 terrain case (hybrid levels, 12 Witch-of-Agnesi ridges, periodic y /
     open x, dx = 500 m, dt = 3 s, qv + 10 scalars): Mpoint-RK3-steps/s over `steps` steps
@@ -220,6 +269,7 @@ def main():
                     help="add the 1200x1200x80 terrain leg (about 50 GB of HBM, a minute of host set-up)")
     ap.add_argument("--terrain-steps", type=int, default=20)
     ap.add_argument("--terrain-only", action="store_true", help="run only the terrain leg (profiling)")
+    ap.add_argument("--no-pd", action="store_true", help="skip the positive-definite transport leg")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after the timed steps, copy a buffer of this many MiB once (calibrates HBM counters)")
     args = ap.parse_args()
@@ -332,6 +382,8 @@ def main():
         dev.step(args.steps, graph=True, stream=stream, sync=False)
         torch.cuda.synchronize()
         out["graph_ms_per_step"] = round(1e3 * (time.perf_counter() - t0) / args.steps, 4)
+    if not args.no_pd:
+        out["positive_definite"] = pd_leg(runtime, dev, torch, stream, args.steps, pts, log)
     if args.ensemble_members > 0:
         del dev
         torch.cuda.empty_cache()

@@ -168,8 +168,9 @@ int wrfmi_domain_release(const wrfmi_state *st);
  * wrfmi_solve_em_step (bit for bit), with one launch per kernel of the step
  * covering the tiles of every member.  states/cfgs/dims are arrays of
  * n_members structs, copied at creation: later steps use those pointers.
- * Members must share the boundary type, precision_mode, n_species and the
- * number of acoustic steps (WRFMI_E_BADOPT otherwise).  Each member gets its
+ * Members must share the boundary type, precision_mode, n_species and n_moist with
+ * their transport options (moist_adv_opt, scalar_adv_opt), and the number of acoustic
+ * steps (WRFMI_E_BADOPT otherwise).  Each member gets its
  * own work arrays.  _step enqueues the step eagerly; _step_graph replays a
  * hipGraph captured on its first call. */
 typedef struct wrfmi_ensemble wrfmi_ensemble;

@@ -14,7 +14,7 @@
  * Every function cites the reference file:line it follows.
  *
  * How each part is pinned (tests/test_oracle.py, tests/oracle_pins.py; the mutation test
- * tests/test_oracle_mutations.py builds this source with 17 plausible mistakes planted and
+ * tests/test_oracle_mutations.py builds this source with 23 plausible mistakes planted and
  * shows that every one fails at least one pin):
  *   o_advect_scalar  bit for bit against the reference's own advect_scalar, compiled in
  *                    place (oracle/_ref) and through tests/golden/advect_scalar_ref.npz:

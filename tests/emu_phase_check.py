@@ -42,6 +42,9 @@ CASES = {
     "pd_periodic": dict(nx=14, ny=11, nz=10, bc="periodic", n_passive=2, n_bubbles=2, bubble_amp=2.0, seed=26,
                         pd=(1, 1)),
     "pd_open": dict(nx=13, ny=12, nz=9, bc="open", n_passive=1, n_bubbles=1, seed=27, pd=(1, 1)),
+    # rows of 32 floats: the last four-point group holds ide-1 and three points past it
+    "pd_periodic_tight": dict(nx=21, ny=9, nz=9, bc="periodic", n_passive=1, n_bubbles=1, seed=29, pad_i=4,
+                              pd=(1, 1)),
     "pd_mixed_unaligned": dict(nx=14, ny=11, nz=8, bc_x="open", bc_y="periodic", n_passive=1, n_bubbles=1, seed=28,
                                pad_i=1, pd=(0, 1)),
 }
@@ -99,16 +102,23 @@ def registry_check():
     print("OK registry", flush=True)
 
 
-def ensemble_check(order: int, nsteps: int):
+def ensemble_check(order: int, nsteps: int, pd: bool = False):
     """Batched ensemble (wrfmi_ensemble_*): members of different sizes advanced together
-    are bit for bit the members advanced one by one; mismatched members are refused."""
+    are bit for bit the members advanced one by one; mismatched members are refused.  With
+    `pd`, every member's species take the positive-definite transport."""
     import ctypes
 
     from wrf_amd.fields import Config, Dims, State32
     specs = [grid.DomainSpec(nx=14, ny=11, nz=10, bc="open", n_bubbles=1, seed=21),
              grid.DomainSpec(nx=9, ny=13, nz=10, bc="open", n_bubbles=1, bubble_amp=1.5, seed=22),
              grid.DomainSpec(nx=12, ny=9, nz=12, bc="open", n_bubbles=1, bubble_amp=3.5, seed=23)]
-    hss = [grid.build_initial_state(s) for s in specs]
+    if pd:
+        from pd_parity import pd_state
+        for s in specs:
+            s.n_passive = 1
+        hss = [pd_state(s, 1, 1, seed=s.seed) for s in specs]
+    else:
+        hss = [grid.build_initial_state(s) for s in specs]
     singles = []
     for hs in hss:
         e = EmuDomain(hs, order=order)
@@ -149,6 +159,16 @@ def ensemble_check(order: int, nsteps: int):
     rc = lib.wrfmi_ensemble_create(2, ctypes.byref(states2), ctypes.byref(cfgs2), ctypes.byref(dims2), ctypes.byref(ens))
     if rc != 1:
         fails.append(f"mismatched members: expected WRFMI_E_BADOPT, got {rc}")
+    if pd:
+        # the same boundaries and species with the unlimited transport: other kernels, refused
+        cfgs3 = (Config * 2)(members[0].cfg, members[1].cfg)
+        cfgs3[1].moist_adv_opt = cfgs3[1].scalar_adv_opt = 0
+        states3 = (State32 * 2)(members[0].state, members[1].state)
+        dims3 = (Dims * 2)(members[0].dims, members[1].dims)
+        rc = lib.wrfmi_ensemble_create(2, ctypes.byref(states3), ctypes.byref(cfgs3), ctypes.byref(dims3),
+                                       ctypes.byref(ens))
+        if rc != 1 or "transport" not in lib.wrfmi_last_error().decode():
+            fails.append(f"members with different transport options: expected WRFMI_E_BADOPT, got {rc}")
     for f in fails:
         print("MISMATCH", f)
     print("OK ensemble" if not fails else "FAILED ensemble", "order", order, flush=True)
@@ -335,8 +355,8 @@ def main():
     if sys.argv[1] == "registry":
         registry_check()
         return
-    if sys.argv[1] == "ensemble":
-        ensemble_check(int(sys.argv[2]), int(sys.argv[3]))
+    if sys.argv[1] in ("ensemble", "ensemble_pd"):
+        ensemble_check(int(sys.argv[2]), int(sys.argv[3]), pd=sys.argv[1] == "ensemble_pd")
         return
     if sys.argv[1] == "pd_routines":
         pd_routines_check(int(sys.argv[2]))

@@ -30,7 +30,8 @@ def emu_env():
                                                ("periodic_tight", 1, 1), ("open_tight", 0, 1),
                                                ("terrain", 0, 2), ("terrain_yopen", 1, 1),
                                                ("periodic_wide", 1, 1), ("open_wide", 0, 1), ("open_narrow", 1, 2),
-                                               ("pd_periodic", 0, 2), ("pd_open", 1, 1), ("pd_mixed_unaligned", 0, 1)])
+                                               ("pd_periodic", 0, 2), ("pd_open", 1, 1), ("pd_mixed_unaligned", 0, 1),
+                                               ("pd_periodic_tight", 1, 2)])
 def test_emulated_kernels_match_oracle(emu_env, case, order, nsteps):
     r = subprocess.run([sys.executable, os.path.join(HERE, "emu_phase_check.py"), case, str(order), str(nsteps)],
                        env=emu_env, capture_output=True, text=True, timeout=600)
@@ -45,9 +46,9 @@ def test_emulated_host_array_registry(emu_env):
     assert "OK registry" in r.stdout
 
 
-@pytest.mark.parametrize("order", [0, 1])
-def test_emulated_ensemble_matches_single_domains(emu_env, order):
-    r = subprocess.run([sys.executable, os.path.join(HERE, "emu_phase_check.py"), "ensemble", str(order), "2"],
+@pytest.mark.parametrize("mode,order", [("ensemble", 0), ("ensemble", 1), ("ensemble_pd", 1)])
+def test_emulated_ensemble_matches_single_domains(emu_env, mode, order):
+    r = subprocess.run([sys.executable, os.path.join(HERE, "emu_phase_check.py"), mode, str(order), "2"],
                        env=emu_env, capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stdout[-4000:] + r.stderr[-4000:]
     assert "OK ensemble" in r.stdout

@@ -294,8 +294,296 @@ __global__ void __launch_bounds__(256) k_update_scalar_pd(ARGS args) {
   sc[o] = ((c1 * muold + c2) * sc[o] + a.dt_rk * tendency) / (c1 * munew + c2);
 }
 
+// ---------------------------------------------------------------------------------------
+// Four points per lane, doubly periodic domains of the whole step.  A lane holds the cells
+// i0..i0+3 (16-byte groups) of one row and marches kPdVK levels.  The five x faces of its
+// cells come from 12-point windows of q and 4-point groups of q_old, ru_m, mut and msfty;
+// the y faces from the rows j-3..j+3; the z face is carried up the column.  Every face is
+// the same expression, in the same order, as in PdFaces.  The limiter's scales are needed
+// on the interior cells only: the halo cells ids-1, ide (and rows jds-1, jde) of a periodic
+// domain hold exact copies of the interior inputs (the whole step fills every halo before
+// this transport), so the reference's scales there equal those of the cells they copy, and
+// k_pd_tend_x4 reads those.
+
+struct PdLane {
+  const DState &s;
+  const Dm &d;
+  FPtr<float> f, fo, ru, rv, rom, mut;
+  float ts, dt, rdx, rdy;
+  int i0, j;
+  Off o2;         // (i0, j) in a 2-D field
+  float mutw[6];  // mut at i0-1 .. i0+4 of row j
+  float msfw[6];  // msfty at i0-1 .. i0+4 of row j
+  F4 mut_s, mut_n, mxw_s, mxw_c, mxw_n;  // mut rows j-1, j+1; msftx rows j-1, j, j+1
+
+  DEVI void load2d() {
+    const F4 ml = mut.ld<F4>(o2 - 4), mc = mut.ld<F4>(o2), mr = mut.ld<F4>(o2 + 4);
+    const F4 sl = s.msfty.ld<F4>(o2 - 4), sc = s.msfty.ld<F4>(o2), sr = s.msfty.ld<F4>(o2 + 4);
+    mutw[0] = ml.v[3];
+    msfw[0] = sl.v[3];
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+      mutw[n + 1] = mc.v[n];
+      msfw[n + 1] = sc.v[n];
+    }
+    mutw[5] = mr.v[0];
+    msfw[5] = sr.v[0];
+    const int ni = d.ni;
+    mut_s = mut.ld<F4>(o2 - ni);
+    mut_n = mut.ld<F4>(o2 + ni);
+    mxw_s = s.msftx.ld<F4>(o2 - ni);
+    mxw_c = s.msftx.ld<F4>(o2);
+    mxw_n = s.msftx.ld<F4>(o2 + ni);
+  }
+  // x faces i0..i0+4 of level k (o = (i0, k, j)): fq = high - upwind, fl = upwind
+  DEVI void fx(Off o, float c1, float c2, float fq[5], float fl[5]) const {
+    const F4 wl = f.ld<F4>(o - 4), wc = f.ld<F4>(o), wr = f.ld<F4>(o + 4);
+    const F4 ol = fo.ld<F4>(o - 4), oc = fo.ld<F4>(o), orr = fo.ld<F4>(o + 4);
+    const F4 rc = ru.ld<F4>(o), rr = ru.ld<F4>(o + 4);
+    float W[12], FO[6];
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+      W[n] = wl.v[n];
+      W[n + 4] = wc.v[n];
+      W[n + 8] = wr.v[n];
+      FO[n + 1] = oc.v[n];
+    }
+    FO[0] = ol.v[3];
+    FO[5] = orr.v[0];
+#pragma unroll
+    for (int n = 0; n < 5; n++) {
+      const float dx = 2.f / (msfw[n + 1] + msfw[n]) / rdx;
+      const float mu = 0.5f * ((c1 * mutw[n + 1] + c2) + (c1 * mutw[n] + c2));
+      const float vel = n < 4 ? rc.v[n] : rr.v[0];
+      const float cr = vel * dt / dx / mu;
+      fl[n] = mu * (dx / dt) * pd_upwind(FO[n], FO[n + 1], cr);
+      fq[n] = vel * pd_flux5(W[n + 1], W[n + 2], W[n + 3], W[n + 4], W[n + 5], W[n + 6], vel, ts);
+      fq[n] = fq[n] - fl[n];
+    }
+  }
+  // y faces j (lo) and j+1 (hi) of level k for the lane's four cells
+  DEVI void fy(Off o, float c1, float c2, const F4 &q0, F4 &fq0, F4 &fl0, F4 &fq1, F4 &fl1) const {
+    const int sj = d.sj;
+    const F4 ym3 = f.ld<F4>(o - 3 * sj), ym2 = f.ld<F4>(o - 2 * sj), ym1 = f.ld<F4>(o - sj);
+    const F4 yp1 = f.ld<F4>(o + sj), yp2 = f.ld<F4>(o + 2 * sj), yp3 = f.ld<F4>(o + 3 * sj);
+    const F4 os = fo.ld<F4>(o - sj), oc = fo.ld<F4>(o), on = fo.ld<F4>(o + sj);
+    const F4 v0 = rv.ld<F4>(o), v1 = rv.ld<F4>(o + sj);
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+      const float mc = mutw[n + 1];
+      {
+        const float dy = 2.f / (mxw_c.v[n] + mxw_s.v[n]) / rdy;
+        const float mu = 0.5f * ((c1 * mc + c2) + (c1 * mut_s.v[n] + c2));
+        const float vel = v0.v[n];
+        const float cr = vel * dt / dy / mu;
+        fl0.v[n] = mu * (dy / dt) * pd_upwind(os.v[n], oc.v[n], cr);
+        fq0.v[n] = vel * pd_flux5(ym3.v[n], ym2.v[n], ym1.v[n], q0.v[n], yp1.v[n], yp2.v[n], vel, ts) - fl0.v[n];
+      }
+      {
+        const float dy = 2.f / (mxw_n.v[n] + mxw_c.v[n]) / rdy;
+        const float mu = 0.5f * ((c1 * mut_n.v[n] + c2) + (c1 * mc + c2));
+        const float vel = v1.v[n];
+        const float cr = vel * dt / dy / mu;
+        fl1.v[n] = mu * (dy / dt) * pd_upwind(oc.v[n], on.v[n], cr);
+        fq1.v[n] = vel * pd_flux5(ym2.v[n], ym1.v[n], q0.v[n], yp1.v[n], yp2.v[n], yp3.v[n], vel, ts) - fl1.v[n];
+      }
+    }
+  }
+  // z face at w level kk of the lane's cells, from q at kk-2..kk+1, q_old at kk-1, kk
+  DEVI void fz(int kk, const F4 &qm2, const F4 &qm1, const F4 &q0, const F4 &qp1, const F4 &om1, const F4 &o0,
+               const F4 &w, F4 &fq, F4 &fl) const {
+    const int kts = d.kds, ktf = d.kde - 1;
+    const bool zero = kk <= kts || kk >= d.kde, low = kk == kts + 1 || kk == ktf;
+    const int kc = imax(kts + 1, imin(kk, d.kde - 1));
+    const float c1 = s.c1h[X1(d, kc)], c2 = s.c2h[X1(d, kc)];
+    const float dz = 2.f / (s.rdnw[X1(d, kc)] + s.rdnw[X1(d, kc - 1)]);
+    const float fm = s.fnm[X1(d, kc)], fp = s.fnp[X1(d, kc)];
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+      const float m = mutw[n + 1];
+      const float mu = 0.5f * ((c1 * m + c2) + (c1 * m + c2));
+      const float vel = w.v[n];
+      const float cr = vel * dt / dz / mu;
+      const float l = mu * (dz / dt) * pd_upwind(om1.v[n], o0.v[n], cr);
+      const float h = low ? w.v[n] * (fm * q0.v[n] + fp * qm1.v[n])
+                          : vel * pd_flux3(qm2.v[n], qm1.v[n], q0.v[n], qp1.v[n], -vel, ts);
+      fl.v[n] = zero ? 0.f : l;
+      fq.v[n] = zero ? 0.f : h - l;
+    }
+  }
+};
+
+DEVI PdLane pd_lane(const KArgs &a, int i0, int j) {
+  const DState &s = dstate(a);
+  const int sp = a.species;
+  PdLane L{s, a.d, s.q[sp], s.q_old[sp], s.ru_m, s.rv_m, s.ww_m, s.muts, a.c.time_step >= 0 ? 1.f : -1.f,
+           a.dt_rk, s.rdx, s.rdy, i0, j, X2(a.d, i0, j)};
+  L.load2d();
+  return L;
+}
+
+// The interior cells' scales (ids..ide-1, jds..jde-1), kPdVK levels per thread.
+template <typename ARGS>
+__global__ void __launch_bounds__(256) k_pd_scale_x4(ARGS args) {
+  Tile3 T;
+  const KArgs &a = kargs(args, T);
+  const Dm &d = a.d;
+  const DState &s = dstate(a);
+  const int i0 = vec_ia0(d) + kVecW * (T.bx * 64 + (int)threadIdx.x);
+  const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
+  const int kts = d.kds, ktf = d.kde - 1;
+  const int k0 = kts + T.bz * kPdVK, k1 = imin(k0 + kPdVK - 1, ktf);
+  if (i0 > d.ide - 1 || j > d.jde - 1 || k0 > ktf) return;
+  const PdLane P = pd_lane(a, i0, j);
+  const FPtr<float> S = a.scr_dv;
+  const F4 mub = s.mub.ld<F4>(P.o2), mu_old = s.mu_1.ld<F4>(P.o2), msftx = P.mxw_c;
+  const float rdx = s.rdx, rdy = s.rdy, dt = P.dt, eps = 1.e-20f;
+  const int ni = d.ni;
+  const Off ob = X3(d, i0, 0, j);
+  auto lq = [&](const FPtr<float> &arr, int kk) { return arr.ld<F4>(ob + imax(kts, imin(kk, ktf)) * ni); };
+  auto lw = [&](int kk) { return P.rom.ld<F4>(ob + imax(kts, imin(kk, d.kde)) * ni); };
+  F4 qm2 = lq(P.f, k0 - 2), qm1 = lq(P.f, k0 - 1), q0 = lq(P.f, k0), qp1 = lq(P.f, k0 + 1);
+  F4 om1 = lq(P.fo, k0 - 1), o0 = lq(P.fo, k0);
+  F4 zq_lo, zl_lo;
+  P.fz(k0, qm2, qm1, q0, qp1, om1, o0, lw(k0), zq_lo, zl_lo);
+  for (int k = k0; k <= k1; k++) {
+    const Off o = ob + k * ni;
+    const float c1 = s.c1h[X1(d, k)], c2 = s.c2h[X1(d, k)], rdzw = s.rdnw[X1(d, k)];
+    float xq[5], xl[5];
+    P.fx(o, c1, c2, xq, xl);
+    F4 yq0, yl0, yq1, yl1;
+    P.fy(o, c1, c2, q0, yq0, yl0, yq1, yl1);
+    const F4 qp2 = lq(P.f, k + 2), op1 = lq(P.fo, k + 1);
+    F4 zq1, zl1;
+    P.fz(k + 1, qm1, q0, qp1, qp2, o0, op1, lw(k + 1), zq1, zl1);
+    F4 sc;
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+      const float msfty = P.msfw[n + 1];
+      const float ph_low = ((c1 * mub.v[n] + c2) + (c1 * mu_old.v[n])) * o0.v[n] -
+                           dt * (msftx.v[n] * msfty * (rdx * (xl[n + 1] - xl[n]) + rdy * (yl1.v[n] - yl0.v[n])) +
+                                 msfty * rdzw * (zl1.v[n] - zl_lo.v[n]));
+      const float flux_out = dt * ((msftx.v[n] * msfty) * (rdx * (fmax_r(0.f, xq[n + 1]) - fmin_r(0.f, xq[n])) +
+                                                           rdy * (fmax_r(0.f, yq1.v[n]) - fmin_r(0.f, yq0.v[n]))) +
+                                   msfty * rdzw * (fmin_r(0.f, zq1.v[n]) - fmax_r(0.f, zq_lo.v[n])));
+      sc.v[n] = flux_out > ph_low ? fmax_r(0.f, ph_low / (flux_out + eps)) : 1.f;
+    }
+    st4m(S, o, sc, i0, d.ids, d.ide - 1);
+    qm2 = qm1;
+    qm1 = q0;
+    q0 = qp1;
+    qp1 = qp2;
+    om1 = o0;
+    o0 = op1;
+    zq_lo = zq1;
+    zl_lo = zl1;
+  }
+}
+
+// advect_scalar_pd's tendency on a doubly periodic domain of the whole step (rk_scalar_tend's
+// form: from zero, zero at the points i = ide, j = jde, k = kde), kPdVK levels per thread.
+template <typename ARGS>
+__global__ void __launch_bounds__(256) k_pd_tend_x4(ARGS args) {
+  Tile3 T;
+  const KArgs &a = kargs(args, T);
+  const Dm &d = a.d;
+  const DState &s = dstate(a);
+  const int i0 = vec_ia0(d) + kVecW * (T.bx * 64 + (int)threadIdx.x);
+  const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
+  const int kts = d.kds, kte = d.kde, ktf = kte - 1;
+  const int k0 = kts + T.bz * kPdVK, k1 = imin(k0 + kPdVK - 1, kte);
+  if (i0 > d.ide || j > d.jde || k0 > kte) return;
+  const FPtr<float> S = a.scr_dv, tend = s.advect_tend;
+  const int ni = d.ni, nx = d.ide - d.ids, ny = d.jde - d.jds;
+  const Off ob = X3(d, i0, 0, j);
+  const F4 zero4{{0.f, 0.f, 0.f, 0.f}};
+  if (j == d.jde || k0 > ktf) {  // row jde, or the level kde alone: zeros only
+    for (int k = k0; k <= k1; k++) st4m(tend, ob + k * ni, zero4, i0, d.ids, d.ide);
+    return;
+  }
+  const PdLane P = pd_lane(a, i0, j);
+  const F4 msftx = P.mxw_c;
+  const float rdx = s.rdx, rdy = s.rdy;
+  // the scale rows j-1 and j+1, wrapped into the interior rows
+  const int js = j - 1 < d.jds ? j - 1 + ny : j - 1, jn = j + 1 > d.jde - 1 ? j + 1 - ny : j + 1;
+  const Off obs = X3(d, i0, 0, js), obn = X3(d, i0, 0, jn);
+  // the scales of the cells i0-1 .. i0+4, each wrapped into the interior columns (a lane
+  // whose cells all lie inside reads its own group and the two neighbours as they are)
+  const bool inner = i0 - 1 >= d.ids && i0 + 4 <= d.ide - 1;
+  int ix[6];
+#pragma unroll
+  for (int m = 0; m < 6; m++) {
+    const int i = i0 - 1 + m;
+    ix[m] = i < d.ids ? i + nx : i > d.ide - 1 ? i - nx : i;
+  }
+  // cells i0..i0+3 past ide-1 store a zero; their own scales are not used
+  auto lq = [&](const FPtr<float> &arr, int kk) { return arr.ld<F4>(ob + imax(kts, imin(kk, ktf)) * ni); };
+  auto lw = [&](int kk) { return P.rom.ld<F4>(ob + imax(kts, imin(kk, kte)) * ni); };
+  auto lim = [](float fq, float s_up, float s_dn) { return fq > 0.f ? s_up * fq : fq < 0.f ? s_dn * fq : fq; };
+  const int kz = imin(k0, ktf);
+  F4 qm2 = lq(P.f, kz - 2), qm1 = lq(P.f, kz - 1), q0 = lq(P.f, kz), qp1 = lq(P.f, kz + 1);
+  F4 om1 = lq(P.fo, kz - 1), o0 = lq(P.fo, kz);
+  F4 s0 = S.ld<F4>(ob + kz * ni);
+  F4 zq_lo, zl_lo;
+  P.fz(kz, qm2, qm1, q0, qp1, om1, o0, lw(kz), zq_lo, zl_lo);
+  if (kz > kts) {
+    const F4 sm = S.ld<F4>(ob + (kz - 1) * ni);
+#pragma unroll
+    for (int n = 0; n < 4; n++) zq_lo.v[n] = lim(zq_lo.v[n], s0.v[n], sm.v[n]);
+  }
+  for (int k = kz; k <= imin(k1, ktf); k++) {
+    const Off o = ob + k * ni;
+    const float c1 = s.c1h[X1(d, k)], c2 = s.c2h[X1(d, k)], rdzw = s.rdnw[X1(d, k)];
+    const F4 qp2 = lq(P.f, k + 2), op1 = lq(P.fo, k + 1);
+    F4 zq1, zl1;
+    P.fz(k + 1, qm1, q0, qp1, qp2, o0, op1, lw(k + 1), zq1, zl1);
+    const F4 sp = S.ld<F4>(ob + imin(k + 1, ktf) * ni);
+    float xq[5], xl[5];
+    P.fx(o, c1, c2, xq, xl);
+    F4 yq0, yl0, yq1, yl1;
+    P.fy(o, c1, c2, q0, yq0, yl0, yq1, yl1);
+    const F4 ss = S.ld<F4>(obs + k * ni), sn = S.ld<F4>(obn + k * ni);
+    float SX[6];
+    if (inner) {
+      SX[0] = S[o - 1];
+      SX[5] = S[o + 4];
+#pragma unroll
+      for (int n = 0; n < 4; n++) SX[n + 1] = s0.v[n];
+    } else {
+#pragma unroll
+      for (int m = 0; m < 6; m++) SX[m] = S[X3(d, ix[m], k, j)];
+    }
+    F4 t;
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+      float tt = 0.f;
+      const float zhi = k + 1 <= ktf ? lim(zq1.v[n], sp.v[n], s0.v[n]) : zq1.v[n];
+      tt = tt - rdzw * (zhi - zq_lo.v[n] + zl1.v[n] - zl_lo.v[n]);
+      zq1.v[n] = zhi;
+      const float x0 = lim(xq[n], SX[n], SX[n + 1]), x1 = lim(xq[n + 1], SX[n + 1], SX[n + 2]);
+      tt = tt - msftx.v[n] * (rdx * (x1 - x0 + xl[n + 1] - xl[n]));
+      const float y0 = lim(yq0.v[n], ss.v[n], s0.v[n]), y1 = lim(yq1.v[n], s0.v[n], sn.v[n]);
+      tt = tt - msftx.v[n] * (rdy * (y1 - y0 + yl1.v[n] - yl0.v[n]));
+      t.v[n] = i0 + n <= d.ide - 1 ? tt : 0.f;
+    }
+    st4m(tend, o, t, i0, d.ids, d.ide);
+    qm2 = qm1;
+    qm1 = q0;
+    q0 = qp1;
+    qp1 = qp2;
+    om1 = o0;
+    o0 = op1;
+    zq_lo = zq1;
+    zl_lo = zl1;
+    s0 = sp;
+  }
+  if (k1 == kte) st4m(tend, ob + kte * ni, zero4, i0, d.ids, d.ide);
+}
+
 WRFMI_INST(k_pd_scale)
 WRFMI_INST(k_pd_tend)
+WRFMI_INST(k_pd_scale_x4)
+WRFMI_INST(k_pd_tend_x4)
 WRFMI_INST(k_update_scalar_pd)
 
 }  // namespace wrfmi

@@ -14,6 +14,7 @@ constexpr int kAdvVK = 4;      // levels per thread in k_adv_scalar_x4
 constexpr int kHaloK = 8;      // levels per thread in k_halo
 constexpr int kDivK = 4;       // default levels per thread in k_ac_div_x4 (a launch shape)
 constexpr int kPdK = 8;        // levels per thread in k_pd_scale and k_pd_tend
+constexpr int kPdVK = 4;       // levels per thread in k_pd_scale_x4 and k_pd_tend_x4
 // The reference routine a launch computes on its own, for the per-routine entry points
 // with the reference's argument lists (KArgs::routine); R_FUSED is the whole step's form,
 // with each kernel's fused parts.
@@ -64,6 +65,8 @@ template <typename A> __global__ void k_adv_scalar_all_x4(A);
 template <typename A> __global__ void k_update_scalar_all_x4(A);
 template <typename A> __global__ void k_pd_scale(A);
 template <typename A> __global__ void k_pd_tend(A);
+template <typename A> __global__ void k_pd_scale_x4(A);
+template <typename A> __global__ void k_pd_tend_x4(A);
 template <typename A> __global__ void k_update_scalar_pd(A);
 template <typename A> __global__ void k_p_rho_phi(A);
 template <typename A> __global__ void k_p_rho_phi_x4(A);

@@ -750,9 +750,17 @@ static void ph_scalar_pd_prep(Launcher &L) {
   L.halo_flush();
 }
 // advect_scalar_pd of species L.a.species into advect_tend (module_em.F:1264-1280): the
-// limiter's scales into the dvdxi work array, then the limited divergence
+// limiter's scales into the dvdxi work array, then the limited divergence.  A doubly
+// periodic aligned domain takes the four-point forms, which read the halo cells' scales
+// from the interior cells they copy (the phases before this one have filled every halo).
 static void ph_scalar_tend_pd(Launcher &L) {
   const Dm &d = L.a.d;
+  if (L.vec && L.a.c.periodic_x && L.a.c.periodic_y) {
+    const int nks = (d.kde - d.kds + kPdVK - 1) / kPdVK, nkt = (d.kde - d.kds + 1 + kPdVK - 1) / kPdVK;
+    L.launchK(KID_PD_SCALE, KR(k_pd_scale_x4), g_vec(L.a, vec_ia0(d), -1, WRFMI_VEC_ROWS, nks), kVecBlk, L.a);
+    L.launchK(KID_PD_TEND, KR(k_pd_tend_x4), g_vec(L.a, vec_ia0(d), 0, WRFMI_VEC_ROWS, nkt), kVecBlk, L.a);
+    return;
+  }
   const int nkz = (d.kde - d.kds + kPdK - 1) / kPdK;
   const dim3 gs = col_grid(d, d.ids - 1, d.jds - 1, kColBlk);
   L.launchK(KID_PD_SCALE, KR(k_pd_scale), dim3(gs.x, gs.y, nkz), kColBlk, L.a);
@@ -1179,7 +1187,8 @@ extern "C" int wrfmi_ensemble_create(int n_members, const wrfmi_state *states, c
     enqueue_step(Ls[m]);
   }
   // every member must have produced the same launch sequence (same kernels and blocks):
-  // members share the boundary type, precision mode, species count and acoustic steps
+  // members share the boundary type, precision mode, species count, species transport
+  // options and acoustic steps
   for (int m = 1; m < n_members && !rc; m++) {
     bool same = rec[m].ops.size() == rec[0].ops.size();
     for (size_t o = 0; same && o < rec[0].ops.size(); o++)
@@ -1187,8 +1196,8 @@ extern "C" int wrfmi_ensemble_create(int n_members, const wrfmi_state *states, c
              rec[m].ops[o].blk.y == rec[0].ops[o].blk.y && rec[m].ops[o].halo == rec[0].ops[o].halo &&
              (!rec[m].ops[o].halo || rec[m].h[rec[m].ops[o].arg].n == rec[0].h[rec[0].ops[o].arg].n);
     if (!same)
-      rc = fail(WRFMI_E_BADOPT, "member %d: members must share the boundary type, precision mode, number of species "
-                "and number of acoustic steps", m);
+      rc = fail(WRFMI_E_BADOPT, "member %d: members must share the boundary type, precision mode, number of species, "
+                "species transport options and number of acoustic steps", m);
   }
   if (rc) {
     ens_free(e);
