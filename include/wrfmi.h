@@ -258,7 +258,8 @@ int wrfmi_set_physical_bc2d(float *field, char var, const wrfmi_config *cfg,
 
 /* ---- launch shapes (no counterpart in the reference, whose tile decomposition is
  * frame/module_tiles.F:200-460) ----------------------------------------------------
- * Rows per workgroup of the four-point and column kernels, levels per thread of the
+ * Rows per workgroup of the four-point and column kernels (1, 2 or 4; 8 too for the 5th-order
+ * stencil kernels k_tend_u/v/w/t and k_adv_scalar), levels per thread of the
  * four-point advance_uv and column divergence (k_advance_uv, k_ac_div: 2, 4 or 8), and the
  * column solve's prefetch depth (k_ac_wsolve: 2, 4 or 8), per kernel (the names of
  * wrfmi_kernel_name) and domain shape in mass points (nx, ny, nz; 0 matches any).  Every

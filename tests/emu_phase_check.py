@@ -290,9 +290,10 @@ def totals_check():
 
 
 def shapes_check(case: str):
-    """Tile-shape invariance: with every kernel's rows per workgroup at 2 and at 4, and the
-    levels per thread and prefetch depth at each tunable value, one step is bit for bit the
-    oracle's (the launch shapes the autotuner chooses among)."""
+    """Tile-shape invariance: with every kernel's rows per workgroup at 2 and at 4 (and the
+    stencil kernels' at 8), and the levels per thread and prefetch depth at each
+    tunable value, one step is bit for bit the oracle's (the launch shapes the autotuner
+    chooses among)."""
     import ctypes
     kernels = ["k_step_prep", "k_ww_scan", "k_tend_u", "k_tend_v", "k_tend_w", "k_tend_t", "k_small_prep", "k_p_rho0",
                "k_coef_w", "k_advance_uv", "k_ac_div", "k_ac_mu_ww", "k_ac_theta", "k_ac_wsolve", "k_small_finish",
@@ -302,6 +303,8 @@ def shapes_check(case: str):
                     "k_ac_wsolve": (1, 1, 4)})
     sets = [("rows 2", {k: (2, v[1], v[2]) for k, v in default.items()}),
             ("rows 4", {k: (4, v[1], v[2]) for k, v in default.items()})]
+    tall = ("k_tend_u", "k_tend_v", "k_tend_w", "k_tend_t", "k_adv_scalar")
+    sets.append(("rows 8", {k: (8, 1, 0) for k in tall}))
     for lv in (2, 8):
         sets.append((f"levels {lv}", {"k_advance_uv": (1, lv, 0), "k_ac_div": (2, lv, 0)}))
     for pf in (2, 8):
@@ -328,7 +331,8 @@ def shapes_check(case: str):
     lib = C.CDLL(os.path.join(HERE, "cpu_emu", "libwrfmi_emu.so"))
     lib.wrfmi_tuning_set.argtypes = [C.c_char_p] + [C.c_int] * 6
     if lib.wrfmi_tuning_set(b"k_ac_wsolve", 0, 0, 0, 3, 1, 4) != 1 or lib.wrfmi_tuning_set(b"k_tend_u", 0, 0, 0, 1, 2, 0) != 1 \
-            or lib.wrfmi_tuning_set(b"k_nonexistent", 0, 0, 0, 1, 1, 0) != 1:
+            or lib.wrfmi_tuning_set(b"k_nonexistent", 0, 0, 0, 1, 1, 0) != 1 \
+            or lib.wrfmi_tuning_set(b"k_ac_theta", 0, 0, 0, 8, 1, 0) != 1 or lib.wrfmi_tuning_set(b"k_tend_v", 0, 0, 0, 16, 1, 0) != 1:
         fails.append("a shape outside the offered set was accepted")
     for f in fails:
         print("MISMATCH", f)

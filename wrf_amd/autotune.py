@@ -1,7 +1,8 @@
 """Launch-shape autotuner of the MI355X dynamics step.
 
 For one domain shape it measures, kernel by kernel, the candidate launch shapes the library
-offers (rows per workgroup of the four-point and column kernels, levels per thread of the
+offers (rows per workgroup of the four-point and column kernels -- up to 8 for the 5th-order
+stencil kernels, 4 for the others --, levels per thread of the
 four-point advance_uv and column divergence, prefetch depth of the column solve; see
 wrfmi_tuning_set in include/wrfmi.h) with the library's own per-kernel HIP-event timing, keeps
 the fastest for each kernel in turn (coordinate descent, starting from the current table), and
@@ -28,6 +29,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_TABLE = os.path.join(HERE, "wrfmi_tuning.txt")
 
 ROWS = (1, 2, 4)
+TALL_ROWS = (1, 2, 4, 8)  # the four-point stencil kernels take workgroups of up to 8 rows
+TALL = {"k_tend_u", "k_tend_v", "k_tend_w", "k_tend_t", "k_adv_scalar"}
 LEVELS = (2, 4, 8)
 PREFETCH = (2, 4, 8)
 # the tunable kernels: which parameter besides rows each has
@@ -72,7 +75,7 @@ def tune(spec, steps=3, log=print):
     best = {}
     for kernel, extra in KERNELS.items():
         rows0, lev0, pf0 = current_shape(lib, kernel, nx, ny, nz)
-        cands = [(r, lev0, pf0) for r in ROWS]
+        cands = [(r, lev0, pf0) for r in (TALL_ROWS if kernel in TALL else ROWS)]
         if extra == "levels":
             cands += [(rows0, lv, pf0) for lv in LEVELS]
         if extra == "prefetch":

@@ -152,7 +152,7 @@ __global__ void __launch_bounds__(256) k_adv_scalar(ARGS args) {
 // offset is a multiple of four floats; the host checks the alignment and the padding the
 // windows need.
 template <typename ARGS>
-__global__ void __launch_bounds__(256) k_adv_scalar_x4(ARGS args) {
+__global__ void __launch_bounds__(kTallBlk) k_adv_scalar_x4(ARGS args) {
   Tile3 T;
   const KArgs &a = kargs(args, T);
   const Dm &d = a.d;
@@ -279,7 +279,7 @@ __global__ void __launch_bounds__(256) k_update_scalar(ARGS args) {
 // tendency goes to the work array m (the last species to advect_tend, as after the
 // reference's species loop); element by element the fluxes of k_adv_scalar_x4.
 template <typename ARGS>
-__global__ void __launch_bounds__(256) k_adv_scalar_all_x4(ARGS args) {
+__global__ void __launch_bounds__(kTallBlk) k_adv_scalar_all_x4(ARGS args) {
   Tile3 T;
   const KArgs &a = kargs(args, T);
   const Dm &d = a.d;

@@ -250,7 +250,7 @@ __global__ void __launch_bounds__(256) k_tend_u(ARGS args) {
 // PER = true (5th-order fluxes on every face, no lateral-boundary terms); the vertical
 // flux and dpn at both w levels of the level are formed directly.
 template <typename ARGS>
-__global__ void __launch_bounds__(256) k_tend_u_x4(ARGS args) {
+__global__ void __launch_bounds__(kTallBlk) k_tend_u_x4(ARGS args) {
   Tile3 T;
   const KArgs &a = kargs(args, T);
   const Dm &d = a.d;
@@ -644,7 +644,7 @@ __global__ void __launch_bounds__(256) k_tend_v(ARGS args) {
 // k_tend_v for the doubly periodic case with four i points per lane, one level per thread;
 // element by element the terms of k_tend_v with PER = true, in its order.
 template <typename ARGS>
-__global__ void __launch_bounds__(256) k_tend_v_x4(ARGS args) {
+__global__ void __launch_bounds__(kTallBlk) k_tend_v_x4(ARGS args) {
   Tile3 T;
   const KArgs &a = kargs(args, T);
   const Dm &d = a.d;
@@ -1032,7 +1032,7 @@ __global__ void __launch_bounds__(256) k_tend_w(ARGS args) {
 // k_tend_w for the doubly periodic case with four i points per lane, one level per thread;
 // element by element the terms of k_tend_w with PER = true, in its order.
 template <typename ARGS>
-__global__ void __launch_bounds__(256) k_tend_w_x4(ARGS args) {
+__global__ void __launch_bounds__(kTallBlk) k_tend_w_x4(ARGS args) {
   Tile3 T;
   const KArgs &a = kargs(args, T);
   const Dm &d = a.d;
@@ -1463,7 +1463,7 @@ __global__ void __launch_bounds__(256) k_tend_t(ARGS args) {
 // (order-6 horizontal form) + rk_addtend_dry into ph_tend, and mu_tend/mu_save at level
 // kts.  Element by element the terms of k_tend_t with PER = true, in its order.
 template <typename ARGS>
-__global__ void __launch_bounds__(256) k_tend_t_x4(ARGS args) {
+__global__ void __launch_bounds__(kTallBlk) k_tend_t_x4(ARGS args) {
   Tile3 T;
   const KArgs &a = kargs(args, T);
   const Dm &d = a.d;

@@ -15,6 +15,11 @@ constexpr int kHaloK = 8;      // levels per thread in k_halo
 constexpr int kDivK = 4;       // default levels per thread in k_ac_div_x4 (a launch shape)
 constexpr int kPdK = 8;        // levels per thread in k_pd_scale and k_pd_tend
 constexpr int kPdVK = 4;       // levels per thread in k_pd_scale_x4 and k_pd_tend_x4
+// The four-point 5th-order stencil kernels (k_tend_u/v/w/t_x4, k_adv_scalar_x4,
+// k_adv_scalar_all_x4) take workgroups of up to 8 rows of 256 points (a launch shape): a
+// taller tile re-reads fewer of the j+-3 halo rows its stencils share with the tiles above
+// and below.  (At 1024 threads the compiler would cap them at 128 VGPRs and spill.)
+constexpr int kTallBlk = 512;
 // The reference routine a launch computes on its own, for the per-routine entry points
 // with the reference's argument lists (KArgs::routine); R_FUSED is the whole step's form,
 // with each kernel's fused parts.

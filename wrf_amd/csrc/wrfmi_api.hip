@@ -486,6 +486,10 @@ static bool tunable_rows(int kid) {
   return kid != KID_HALO && kid != KID_W_SURFACE && kid != KID_PD_SCALE && kid != KID_PD_TEND && kid != KID_UPDATE_SCALAR_PD;
 }
 static bool tunable_levels(int kid) { return kid == KID_ADVANCE_UV || kid == KID_AC_DIV; }
+// the four-point stencil kernels built for workgroups of up to kTallBlk threads
+static bool tall_rows(int kid) {
+  return kid == KID_TEND_U || kid == KID_TEND_V || kid == KID_TEND_W || kid == KID_TEND_T || kid == KID_ADV_SCALAR;
+}
 static bool tunable_prefetch(int kid) { return kid == KID_AC_WSOLVE; }
 struct TuneEntry {
   int kid, nx, ny, nz;  // domain shape in mass points; 0 matches any
@@ -508,7 +512,8 @@ static int tune_add(const char *kernel, int nx, int ny, int nz, int rows, int le
   const int kid = kernel_id(kernel);
   if (kid < 0) return fail(WRFMI_E_BADOPT, "no kernel named '%s'", kernel ? kernel : "(null)");
   if (!tunable_rows(kid)) return fail(WRFMI_E_BADOPT, "%s has no tunable launch shape", kernel);
-  if (rows != 1 && rows != 2 && rows != 4) return fail(WRFMI_E_BADOPT, "%s: rows per workgroup %d (1, 2 or 4)", kernel, rows);
+  if (tall_rows(kid) ? (rows != 1 && rows != 2 && rows != 4 && rows != 8) : (rows != 1 && rows != 2 && rows != 4))
+    return fail(WRFMI_E_BADOPT, "%s: rows per workgroup %d (1, 2, 4%s)", kernel, rows, tall_rows(kid) ? " or 8" : "");
   const Shape def = default_shape(kid);
   if (tunable_levels(kid) ? (levels != 2 && levels != 4 && levels != 8) : levels != def.levels)
     return fail(WRFMI_E_BADOPT, "%s: levels per thread %d not available", kernel, levels);
