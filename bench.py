@@ -12,8 +12,10 @@ events on its launch stream over the timed steps, against 8 TB/s; its algorithmi
 the 3-D arrays it must read and write once per launch.
 ensemble = member-steps/s of the 64-member uneven ensemble, member by member with the plain
 kernels and batched (one launch per kernel over all members, hipGraph replay).
-positive_definite = the same workload with positive-definite species transport on the last
-substep (moist_adv_opt = scalar_adv_opt = 1), and its three kernels against 8 TB/s.
+positive_definite, mixing, w_damping = the same workload with positive-definite species
+transport on the last substep (moist_adv_opt = scalar_adv_opt = 1), with sixth-order
+diffusion and Rayleigh damping on the first (diff_6th_opt 2, damp_opt 2), and with advance_w's
+damping layer (damp_opt 3): ms per step and the kernels the options add, against 8 TB/s.
 terrain (with --terrain) = the 1200x1200x80 terrain case, FP32 and precision forms.
 cpu_baseline = the FP32 C restatement of the same step (the oracle), OpenMP over the host
 cores with solve_em's j-tiling, on the same workload: median of 3 single steps.
@@ -158,24 +160,40 @@ def ensemble_leg(runtime, grid, torch, n_members, steps, log):
     return out
 
 
-# positive-definite transport (moist_adv_opt = scalar_adv_opt = 1), per species on the last
-# RK substep: 3-D FP32 arrays each kernel must read + write once per launch
-PD_FIELDS = {
-    "k_update_scalar_pd": 4,  # q_old q_tend -> q_old q_tend
-    "k_pd_scale": 6,          # q q_old ru_m rv_m ww_m -> scale
-    "k_pd_tend": 7,           # q q_old ru_m rv_m ww_m scale -> advect_tend
+# legs that time the same workload with other options of the step: (config options, the
+# kernels the options add or change, with the 3-D FP32 arrays each must read + write once
+# per launch -- None where the work depends on the options' depth, then only the launch
+# time is reported)
+OPTION_LEGS = {
+    # positive-definite transport, per species on the last RK substep
+    "positive_definite": (dict(moist_adv_opt=1, scalar_adv_opt=1), {
+        "k_update_scalar_pd": 4,  # q_old q_tend -> q_old q_tend
+        "k_pd_scale": 6,          # q q_old ru_m rv_m ww_m -> scale
+        "k_pd_tend": 7,           # q q_old ru_m rv_m ww_m scale -> advect_tend
+    }),
+    # sixth-order monotone diffusion of u v w theta and the species, Rayleigh damping of the
+    # top 5 km toward the base profiles, both on the first RK substep
+    "mixing": (dict(diff_6th_opt=2, damp_opt=2, zdamp=5000.0, dampcoef=0.2), {
+        "k_diff6": 3,             # field tend -> tend
+        "k_rayleigh": None,       # the levels of the damping layer only
+    }),
+    # advance_w's implicit Rayleigh damping of w in the top 5 km (damp_opt 3)
+    "w_damping": (dict(damp_opt=3, zdamp=5000.0, dampcoef=0.2), {
+        "k_ac_wsolve": None,
+    }),
 }
 
 
-def pd_leg(runtime, dev, torch, stream, steps, pts, log):
-    """The same workload with every species in positive-definite form (advect_scalar_pd and
-    rk_update_scalar_pd on the last substep, moist_adv_opt = scalar_adv_opt = 1): ms per step
-    over `steps` steps after one untimed step, then the three kernels' average launch times
-    (HIP events on the launch stream, a separate run of `steps` steps) against 8 TB/s with
-    their algorithmic bytes."""
+def option_leg(runtime, dev, torch, stream, steps, pts, log, name):
+    """The same workload with OPTION_LEGS[name]'s options: ms per step over `steps` steps
+    after one untimed step, then the kernels' average launch times (HIP events on the launch
+    stream, a separate run of `steps` steps) against 8 TB/s with their algorithmic bytes.
+    The domain's options are restored afterwards (the state has then advanced)."""
+    opts, fields = OPTION_LEGS[name]
     cfg = dev.cfg
-    saved = (cfg.moist_adv_opt, cfg.scalar_adv_opt)
-    cfg.moist_adv_opt = cfg.scalar_adv_opt = 1
+    saved = {k: getattr(cfg, k) for k in opts}
+    for k, v in opts.items():
+        setattr(cfg, k, v)
     try:
         dev.step(1, stream=stream)
         torch.cuda.synchronize()
@@ -183,25 +201,29 @@ def pd_leg(runtime, dev, torch, stream, steps, pts, log):
         dev.step(steps, stream=stream, sync=False)
         torch.cuda.synchronize()
         elapsed = time.perf_counter() - t0
-        with runtime.KernelTimer(list(PD_FIELDS)) as timer:
+        with runtime.KernelTimer(list(fields)) as timer:
             dev.step(steps, stream=stream)
             kt = timer.collect()
         nbad = dev.check_finite()
         if nbad:
-            raise RuntimeError(f"positive definite: {nbad} non-finite values after the timed steps")
+            raise RuntimeError(f"{name}: {nbad} non-finite values after the timed steps")
     finally:
-        cfg.moist_adv_opt, cfg.scalar_adv_opt = saved
-    out = {"ms_per_step": round(1e3 * elapsed / steps, 4), "mpoint_steps_per_s": round(pts * steps / elapsed / 1e6, 1),
-           "kernels": {}}
-    for k, nf in PD_FIELDS.items():
+        for k, v in saved.items():
+            setattr(cfg, k, v)
+    out = {"options": opts, "ms_per_step": round(1e3 * elapsed / steps, 4),
+           "mpoint_steps_per_s": round(pts * steps / elapsed / 1e6, 1), "kernels": {}}
+    for k, nf in fields.items():
         n, ms = kt[k]
         avg = ms / max(n, 1)
-        gbs = nf * 4 * pts / (avg * 1e-3) / 1e9
-        out["kernels"][k] = {"launches_per_step": n // steps, "avg_launch_ms": round(avg, 4),
-                             "alg_bytes_per_launch": nf * 4 * pts, "achieved_gbs": round(gbs, 1),
-                             "frac": round(gbs / HBM_PEAK_GBS, 4)}
-    log(f"  positive definite: {out['ms_per_step']} ms/step; " + ", ".join(
-        f"{k} {v['avg_launch_ms']} ms ({v['achieved_gbs']} GB/s)" for k, v in out["kernels"].items()))
+        rec = {"launches_per_step": n // steps, "avg_launch_ms": round(avg, 4)}
+        if nf is not None:
+            gbs = nf * 4 * pts / (avg * 1e-3) / 1e9
+            rec.update({"alg_bytes_per_launch": nf * 4 * pts, "achieved_gbs": round(gbs, 1),
+                        "frac": round(gbs / HBM_PEAK_GBS, 4)})
+        out["kernels"][k] = rec
+    log(f"  {name}: {out['ms_per_step']} ms/step; " + ", ".join(
+        f"{k} {v['avg_launch_ms']} ms" + (f" ({v['achieved_gbs']} GB/s)" if "achieved_gbs" in v else "")
+        for k, v in out["kernels"].items()))
     return out
 
 
@@ -269,7 +291,8 @@ def main():
                     help="add the 1200x1200x80 terrain leg (about 50 GB of HBM, a minute of host set-up)")
     ap.add_argument("--terrain-steps", type=int, default=20)
     ap.add_argument("--terrain-only", action="store_true", help="run only the terrain leg (profiling)")
-    ap.add_argument("--no-pd", action="store_true", help="skip the positive-definite transport leg")
+    ap.add_argument("--no-options", action="store_true",
+                    help="skip the option legs (positive-definite transport, mixing, w damping)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after the timed steps, copy a buffer of this many MiB once (calibrates HBM counters)")
     args = ap.parse_args()
@@ -382,8 +405,9 @@ def main():
         dev.step(args.steps, graph=True, stream=stream, sync=False)
         torch.cuda.synchronize()
         out["graph_ms_per_step"] = round(1e3 * (time.perf_counter() - t0) / args.steps, 4)
-    if not args.no_pd:
-        out["positive_definite"] = pd_leg(runtime, dev, torch, stream, args.steps, pts, log)
+    if not args.no_options:
+        for name in OPTION_LEGS:
+            out[name] = option_leg(runtime, dev, torch, stream, args.steps, pts, log, name)
     if args.ensemble_members > 0:
         del dev
         torch.cuda.empty_cache()

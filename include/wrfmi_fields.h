@@ -34,10 +34,12 @@
   X(msfux) X(msfuy) X(msfvx) X(msfvx_inv) X(msfvy) X(msftx) X(msfty)            \
   X(f) X(e) X(sina) X(cosa) X(xlat)
 
-/* 1-D vertical coefficient arrays */
+/* 1-D vertical arrays: level coefficients, and the base-state profiles of the upper-level
+ * Rayleigh damper (u, v, theta perturbation and height of a reference column) */
 #define WRFMI_FIELDS_1D(X)                                                      \
   X(znu) X(znw) X(dnw) X(rdnw) X(dn) X(rdn) X(fnm) X(fnp)                       \
-  X(c1h) X(c2h) X(c1f) X(c2f) X(c3h) X(c4h) X(c3f) X(c4f)
+  X(c1h) X(c2h) X(c1f) X(c2f) X(c3h) X(c4h) X(c3f) X(c4f)                       \
+  X(u_base) X(v_base) X(t_base) X(z_base)
 
 /* REAL scalars of the domain (grid%<name>) */
 #define WRFMI_SCALARS_R(X)                                                      \

@@ -161,6 +161,15 @@ int o_advect_scalar_pd(const wreal *field, const wreal *field_old, wreal *tend, 
 void o_rk_update_scalar_pd(wreal *scalar, wreal *sc_tend, const wreal *c1, const wreal *c2, const wreal *mu_old,
                            const wreal *mu_new, const wreal *mu_base, wreal dt, const dims_t *d);
 
+/* ---- forward-step mixing and damping of the first RK substep (mixing.c) ---- */
+#define WRFMI_ORACLE_MAXK 1024
+void o_sixth_order_diffusion(char name, const wreal *field, wreal *tend, const wreal *mut, wreal dt, const cfg_t *c,
+                             const wreal *c1, const wreal *c2, int diff_6th_opt, wreal diff_6th_factor,
+                             const wreal *phb, const wreal *ph, wreal rdx, wreal rdy, const wreal *msftx,
+                             const wreal *msfty, const wreal *msfux, const wreal *msfuy, const wreal *msfvx,
+                             const wreal *msfvy, const dims_t *d);
+void o_rk_rayleigh_damp(ostate *s, const cfg_t *c, const dims_t *d);
+
 /* ---- rk_tendency and rk_addtend_dry ---- */
 int o_rk_tendency(ostate *s, const cfg_t *c, const dims_t *d, int rk_step);
 void o_rk_addtend_dry(ostate *s, const dims_t *d, int rk_step);

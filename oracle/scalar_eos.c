@@ -13,8 +13,15 @@
 /* advect_tend is zeroed over the tile (zero_tend, module_em.F:1224) and then filled */
 int o_rk_scalar_tend(ostate *s, const cfg_t *c, const dims_t *d, int m, int rk_step, wreal dt_rk) {
   int adv_opt = m < s->n_moist ? c->moist_adv_opt : c->scalar_adv_opt;
-  if ((adv_opt != 0 && adv_opt != 1) || c->zadvect_implicit || c->diff_opt != 0 || c->diff_6th_opt != 0)
+  if ((adv_opt != 0 && adv_opt != 1) || c->zadvect_implicit || c->diff_opt != 0 || c->diff_6th_opt < 0 ||
+      c->diff_6th_opt > 2)
     return WRFMI_E_BADOPT;
+  /* module_em.F:1421-1436: at rk_step 1, 6th-order diffusion of the species into its
+   * tendency (the physics tendency array), with rk_scalar_tend's mut = muts and dt_step */
+  if (rk_step == 1 && c->diff_6th_opt != 0)
+    o_sixth_order_diffusion('m', s->q[m], s->q_tend[m], s->muts, dt_rk, c, s->c1h, s->c2h, c->diff_6th_opt,
+                            (wreal)c->diff_6th_factor, s->phb, s->ph_2, s->rdx, s->rdy, s->msftx, s->msfty, s->msfux,
+                            s->msfuy, s->msfvx, s->msfvy, d);
   for (int j = d->jts; j <= d->jte; j++)
     for (int k = d->kts; k <= d->kte; k++)
       for (int i = d->its; i <= d->ite; i++) A3(s->advect_tend, i, k, j) = R(0.);

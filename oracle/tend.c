@@ -424,10 +424,12 @@ static int curvature(ostate *s, const cfg_t *c, const dims_t *d) {
   return 0;
 }
 
-/* dyn_em/module_em.F:190-955, physics off (diff_opt = 0, damp_opt /= 2, rad_nudge = 0, IEVA off) */
+/* dyn_em/module_em.F:190-955, physics off (diff_opt = 0, rad_nudge = 0, IEVA off); at
+ * rk_step 1 the forward-step terms of 800-950: sixth_order_diffusion (diff_6th_opt 1, 2) and
+ * rk_rayleigh_damp (damp_opt 2) into the *_tendf arrays */
 int o_rk_tendency(ostate *s, const cfg_t *c, const dims_t *d, int rk_step) {
-  (void)rk_step;
-  if (c->zadvect_implicit || c->momentum_adv_opt != 1 || c->diff_opt != 0 || c->diff_6th_opt != 0 || c->pert_coriolis)
+  if (c->zadvect_implicit || c->momentum_adv_opt != 1 || c->diff_opt != 0 || c->diff_6th_opt < 0 ||
+      c->diff_6th_opt > 2 || c->damp_opt == 1 || c->pert_coriolis || d->kde + 1 >= WRFMI_ORACLE_MAXK)
     return WRFMI_E_BADOPT;
   zero3(s->ru_tend, d); zero3(s->rv_tend, d); zero3(s->rw_tend, d); zero3(s->t_tend, d); zero3(s->ph_tend, d);
   zero3(s->u_save, d); zero3(s->v_save, d); zero3(s->w_save, d); zero3(s->ph_save, d); zero3(s->t_save, d);
@@ -453,7 +455,21 @@ int o_rk_tendency(ostate *s, const cfg_t *c, const dims_t *d, int rk_step) {
   if (c->non_hydrostatic) pg_buoy_w(s, d);
   w_damp(s, c, d);
   coriolis(s, c, d);
-  return curvature(s, c, d);
+  rc = curvature(s, c, d);
+  if (rc || rk_step != 1) return rc;
+  if (c->diff_6th_opt != 0) {
+    const wreal f6 = (wreal)c->diff_6th_factor;
+#define SIX(nm, fld, tnd, c1, c2) \
+    o_sixth_order_diffusion(nm, s->fld, s->tnd, s->mut, s->dt, c, s->c1, s->c2, c->diff_6th_opt, f6, s->phb, s->ph_2, \
+                            s->rdx, s->rdy, s->msftx, s->msfty, s->msfux, s->msfuy, s->msfvx, s->msfvy, d)
+    SIX('u', u_2, ru_tendf, c1h, c2h);
+    SIX('v', v_2, rv_tendf, c1h, c2h);
+    if (c->non_hydrostatic) SIX('w', w_2, rw_tendf, c1f, c2f);
+    SIX('m', t_2, t_tendf, c1h, c2h);
+#undef SIX
+  }
+  if (c->damp_opt == 2) o_rk_rayleigh_damp(s, c, d);
+  return 0;
 }
 
 /* dyn_em/module_em.F:959-1092 (c1 = c1h, c2 = c2h) */

@@ -45,6 +45,17 @@ CASES = {
     # rows of 32 floats: the last four-point group holds ide-1 and three points past it
     "pd_periodic_tight": dict(nx=21, ny=9, nz=9, bc="periodic", n_passive=1, n_bubbles=1, seed=29, pad_i=4,
                               pd=(1, 1)),
+    # forward-step mixing and damping of the first substep: 6th-order diffusion (monotone form,
+    # slope tapering), the Rayleigh damper toward the base profiles, advance_w's damp_opt 3
+    "mix_periodic": dict(nx=14, ny=11, nz=12, bc="periodic", n_passive=1, n_bubbles=2, bubble_amp=2.0, seed=35,
+                         opts=dict(diff_6th_opt=2, diff_6th_slopeopt=1, damp_opt=2, zdamp=6000.0)),
+    "mix_open": dict(nx=15, ny=13, nz=10, bc="open", n_passive=1, n_bubbles=1, seed=36,
+                     opts=dict(diff_6th_opt=1, damp_opt=3, zdamp=5000.0, dampcoef=0.3)),
+    "mix_terrain": dict(nx=16, ny=12, nz=10, dx=2000.0, dy=2000.0, dt=6.0, bc_x="open", bc_y="periodic",
+                        hybrid_opt=2, sounding="stable", n_ridges=3, ridge_a=(3000.0, 8000.0), n_passive=1,
+                        n_bubbles=0, noise_amp=0.0, seed=37,
+                        opts=dict(diff_6th_opt=2, diff_6th_slopeopt=1, diff_6th_thresh=0.02, damp_opt=2,
+                                  zdamp=6000.0)),
     "pd_mixed_unaligned": dict(nx=14, ny=11, nz=8, bc_x="open", bc_y="periodic", n_passive=1, n_bubbles=1, seed=28,
                                pad_i=1, pd=(0, 1)),
 }
@@ -54,10 +65,15 @@ def build_case(case: str):
     """The initial state of CASES[case] (pd_parity.pd_state for the positive-definite cases)."""
     kw = dict(CASES[case])
     pd = kw.pop("pd", None)
+    opts = kw.pop("opts", {})
     if pd is None:
-        return grid.build_initial_state(grid.DomainSpec(**kw))
-    from pd_parity import pd_state
-    return pd_state(kw, *pd, seed=kw["seed"])
+        hs = grid.build_initial_state(grid.DomainSpec(**kw))
+    else:
+        from pd_parity import pd_state
+        hs = pd_state(kw, *pd, seed=kw["seed"])
+    for k, v in opts.items():
+        setattr(hs.cfg, k, v)
+    return hs
 
 
 def pd_routines_check(order: int):

@@ -911,8 +911,171 @@ def pin_resting_over_terrain(o):
     assert wmax < 0.04, wmax
 
 
+def _sixth(D, name, field, mut, dt=6.0, opt=1, factor=0.12, slope=0, thresh=0.1):
+    """sixth_order_diffusion of `field` (D's precision) from a zero tendency."""
+    o = D.o
+    vp = ctypes.c_void_p
+    fn = o.lib.o_sixth_order_diffusion
+    fn.argtypes = ([ctypes.c_char] + [vp] * 3 + [o.ctype] + [vp] * 3 + [ctypes.c_int, o.ctype, vp, vp, o.ctype, o.ctype]
+                   + [vp] * 7)
+    fn.restype = None
+    D.cfg.diff_6th_slopeopt, D.cfg.diff_6th_thresh = slope, thresh
+    P = lambda a: a.ctypes.data_as(vp)  # noqa: E731
+    f = np.ascontiguousarray(field, dtype=o.dtype)
+    m = np.ascontiguousarray(mut, dtype=o.dtype)
+    tend = np.zeros_like(f)
+    c1, c2 = ("c1f", "c2f") if name == "w" else ("c1h", "c2h")
+    fn(name.encode(), P(f), P(tend), P(m), dt, ctypes.byref(D.cfg), P(D.a[c1]), P(D.a[c2]), opt, factor,
+       P(D.a["phb"]), P(D.a["ph_2"]), D.hs.scal["rdx"], D.hs.scal["rdy"],
+       *[P(D.a[n]) for n in ("msftx", "msfty", "msfux", "msfuy", "msfvx", "msfvy")], ctypes.byref(D.d))
+    return tend
+
+
+def pin_sixth_order_diffusion(o):
+    """sixth_order_diffusion (module_big_step_utilities_em.F:6220-6636):
+    (1) a 2dx checkerboard A(-1)^(i+j) on uniform mu is damped at the rate the coefficient
+        states: tendency = -factor mu A(-1)^(i+j) / dt (1/(64 2 dt) per direction times the
+        operator's -64), for u, v, w and mass points;
+    (2) with non-uniform mu the tendency is a flux difference: its periodic domain sum
+        vanishes (the face masses of neighbouring points agree);
+    (3) diff_6th_opt = 2 on a unit spike keeps only the two down-gradient faces next to it:
+        -40 coef mu at the spike, +10 coef mu at its four neighbours, zero elsewhere;
+    (4) diff_6th_slopeopt = 1 with a uniform x-slope of phb scales the x part by
+        1 - slope / (9.81 thresh) and leaves y alone."""
+    D = Dom(o, nx=12, ny=10, nz=6)
+    D.set_levels(stretched_dnw(6, seed=3))
+    d = D.d
+    sh = D.a["u_2"].shape
+    J, K, I = np.meshgrid(np.arange(sh[0]) + d.jms, np.arange(sh[1]) + d.kms, np.arange(sh[2]) + d.ims, indexing="ij")
+    chk = np.where((I + J) % 2 == 0, 1.0, -1.0)
+    M, A, B, dt, factor = 9.0e4, 2.5, 7.0, 6.0, 0.12
+    mut = np.full(D.a["mut"].shape, M)
+    ny, nx, nk = d.jde - d.jds, d.ide - d.ids, d.kde - 1
+    J0, I0 = d.jds - d.jms, d.ids - d.ims
+    for name in ("u", "v", "w", "m"):
+        t = _sixth(D, name, B + A * chk, mut, dt=dt, factor=factor)
+        k0 = 1 if name == "w" else 0
+        sl = (slice(J0, J0 + ny), slice(k0, nk), slice(I0, I0 + nx))
+        _close(t[sl], -factor * M * A * chk[sl] / dt, 1e-12, f"checkerboard damping, {name}")
+        assert np.all(t[:, :k0, :] == 0.0)
+    rng = np.random.default_rng(4)
+    f = rng.uniform(-3.0, 3.0, sh)
+    grid._halo_fill_np(f, d, True, False, False)
+    mut_r = rng.uniform(8.5e4, 9.5e4, D.a["mut"].shape)
+    grid._halo_fill_np(mut_r, d, True, False, False)
+    for name in ("u", "v", "w", "m"):
+        t = _sixth(D, name, f, mut_r, dt=dt, factor=factor)
+        k0 = 1 if name == "w" else 0
+        sl = (slice(J0, J0 + ny), slice(k0, nk), slice(I0, I0 + nx))
+        tot = abs(float(t[sl].sum())) / float(np.abs(t[sl]).sum())
+        assert tot < 1e-12, (name, tot)
+    spike = np.zeros(sh)
+    js, is_ = J0 + 4, I0 + 5
+    spike[js, :, is_] = 1.0
+    t = _sixth(D, "m", spike, mut, dt=dt, opt=2, factor=factor)
+    coef = factor * 0.015625 / (2.0 * dt)
+    exp = np.zeros(sh)
+    exp[js, :nk, is_] = -40.0 * coef * M
+    for dj, di in ((0, -1), (0, 1), (-1, 0), (1, 0)):
+        exp[js + dj, :nk, is_ + di] = 10.0 * coef * M
+    _close(t, exp, 1e-12, "diff_6th_opt 2 on a spike")
+    thresh, slope_frac = float(np.float32(0.1)), 0.5  # the config holds thresh as a REAL
+    phb = np.asarray(D.a["phb"])
+    saved = phb.copy()
+    phb[...] = (slope_frac * 9.81 * thresh) * (I - d.ids) * D.dx
+    t = _sixth(D, "m", B + A * chk, mut, dt=dt, factor=factor, slope=1, thresh=thresh)
+    phb[...] = saved
+    sl = (slice(J0, J0 + ny), slice(0, nk), slice(I0, I0 + nx))
+    _close(t[sl], -factor * M * A * chk[sl] / (2.0 * dt) * ((1.0 - slope_frac) + 1.0), 1e-12, "slope tapering")
+
+
+def _damp_levels(D, ztop=20000.0):
+    """Flat levels equally spaced in height (phb = g z, ph = 0): w-level and mass heights."""
+    kde = D.kde
+    zw = np.linspace(0.0, ztop, kde)
+    D.a["phb"][...] = 0.0
+    D.a["phb"][:, :kde, :] = (G * zw)[None, :, None]
+    D.a["ph_2"][...] = 0.0
+    return zw, 0.5 * (zw[1:] + zw[:-1])
+
+
+def pin_rayleigh_damp(o):
+    """rk_rayleigh_damp (module_big_step_utilities_em.F:5836-6110) on flat levels with linear
+    base profiles (the reference value interpolated from them is exact): at the mass levels
+    within zdamp of the top, u, v and theta are relaxed at the rate
+    (c1 mu + c2) dampcoef sin^2(pi/2 (1 - (ztop - z)/zdamp)) toward a + b z, w at its own
+    levels toward zero, and nothing changes below the layer."""
+    D = Dom(o, nx=6, ny=5, nz=24)
+    D.set_levels(stretched_dnw(24, seed=5))
+    kde, nk = D.kde, D.kde - 1
+    zw, zm = _damp_levels(D)
+    ztop, zdamp, dampcoef = zw[-1], 5000.0, 0.2
+    D.cfg.zdamp, D.cfg.dampcoef = zdamp, dampcoef
+    dampcoef = float(D.cfg.dampcoef)  # the config holds it as a REAL
+    base = {"u_base": (3.0, 1.5e-3), "v_base": (-2.0, 4.0e-4), "t_base": (1.0, 2.0e-4)}
+    for n, (a, b) in base.items():
+        D.put1(**{n: a + b * zm})
+    D.put1(z_base=zm)
+    rng = np.random.default_rng(6)
+    for n in ("u_2", "v_2", "w_2", "t_2"):
+        D.a[n][...] = rng.uniform(-5.0, 5.0, D.a[n].shape)
+    mu = {"muu": 8.8e4, "muv": 9.1e4, "mut": 9.0e4}
+    for n, v in mu.items():
+        D.a[n][...] = v
+    c1h, c2h = np.asarray(D.a["c1h"])[:nk], np.asarray(D.a["c2h"])[:nk]
+    c1f, c2f = np.asarray(D.a["c1f"])[:kde], np.asarray(D.a["c2f"])[:kde]
+    for n in ("ru_tendf", "rv_tendf", "rw_tendf", "t_tendf"):
+        D.a[n][...] = 0.0
+    D.call("o_rk_rayleigh_damp")
+    rate = lambda z: dampcoef * np.sin(0.5 * np.pi * (1.0 - np.minimum(1.0, (ztop - z) / zdamp))) ** 2  # noqa: E731
+    layer = zm >= ztop - zdamp
+    assert 2 <= layer.sum() < nk
+    for fld, tnd, bn, mun in (("u_2", "ru_tendf", "u_base", "muu"), ("v_2", "rv_tendf", "v_base", "muv"),
+                              ("t_2", "t_tendf", "t_base", "mut")):
+        a, b = base[bn]
+        col = np.asarray(D.a[fld], np.float64)[D.cj + 1, :nk, D.ci + 1]
+        got = np.asarray(D.a[tnd], np.float64)[D.cj + 1, :nk, D.ci + 1]
+        exp = np.where(layer, -(c1h * mu[mun] + c2h) * rate(zm) * (col - (a + b * zm)), 0.0)
+        _close(got, exp, 1e-10, f"Rayleigh damping of {fld}")
+    wl = zw >= ztop - zdamp
+    col = np.asarray(D.a["w_2"], np.float64)[D.cj + 1, :kde, D.ci + 1]
+    got = np.asarray(D.a["rw_tendf"], np.float64)[D.cj + 1, :kde, D.ci + 1]
+    _close(got, np.where(wl, -(c1f * mu["mut"] + c2f) * rate(zw) * col, 0.0), 1e-10, "Rayleigh damping of w")
+
+
+def pin_advance_w_damp_opt3(o):
+    """advance_w's damp_opt = 3 branch (module_small_step_em.F:1445-1458) with w_save = 0: the
+    final w at each level within zdamp of the top is the undamped one over
+    1 + dts dampcoef sin^2(pi/2 (z - (ztop - zdamp))/zdamp), below it unchanged, and phi
+    follows w through its own update."""
+    res = {}
+    for damp in (0, 3):
+        D, q = _isothermal_column(o)
+        kde, mu, dts, eps = D.kde, q["mu"], q["dts"], q["epssm"]
+        zz = np.linspace(0.0, 1.0, kde)
+        D.a["w_2"][...] = (mu * 0.2 * np.sin(np.pi * zz))[None, :, None]
+        for n in ("t_2", "t_2save", "ww", "ph_tend", "rw_tend", "u_2", "v_2", "ph_save", "muave", "ht", "ph_2",
+                  "w_save"):
+            D.a[n][...] = 0.0
+        D.a["t_save"][...] = 5.0
+        zw = np.linspace(0.0, q["depth"], kde)
+        D.a["phb"][...] = 0.0
+        D.a["phb"][:, :kde, :] = (G * zw)[None, :, None]
+        D.cfg.damp_opt, D.cfg.dampcoef, D.cfg.zdamp = damp, 0.2, 0.3 * q["depth"]
+        D.call("o_advance_w", dts, kinds="r")
+        res[damp] = (np.asarray(D.a["w_2"], np.float64)[D.cj, :kde, D.ci].copy(),
+                     np.asarray(D.a["ph_2"], np.float64)[D.cj, :kde, D.ci].copy(), zw, dts, eps, mu, D.cfg.zdamp)
+    (w0, p0, zw, dts, eps, mu, zdamp), (w3, p3, *_rest) = res[0], res[3]
+    hbot = zw[-1] - zdamp
+    wt = np.where(zw >= hbot, dts * float(np.float32(0.2)) * np.sin(0.5 * np.pi * (zw - hbot) / zdamp) ** 2, 0.0)
+    k = slice(1, kde)
+    assert np.count_nonzero(wt[k]) >= 3 and np.count_nonzero(wt[k] == 0) >= 3
+    _close(w3[k], w0[k] / (1.0 + wt[k]), 1e-10, "damped w")
+    _close(p3[k] - p0[k], 0.5 * dts * G * (1.0 + eps) * (w3[k] - w0[k]) / mu, 1e-8, "phi of the damped w")
+
+
 PINS = [pin_advect_constant_field, pin_advect_linear_eta, pin_coef_w_spectrum, pin_advance_w_theta_scheme,
         pin_advance_w_hydrostatic, pin_calc_p_rho_linearization, pin_advance_uv_pgf, pin_pressure_gradient,
         pin_advance_mu_t_budget,
         pin_small_step_round_trip, pin_constant_mixing_ratio, pin_resting_over_terrain, pin_advect_scalar_pd,
-        pin_pd_whole_step]
+        pin_pd_whole_step, pin_sixth_order_diffusion, pin_rayleigh_damp, pin_advance_w_damp_opt3]

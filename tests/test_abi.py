@@ -61,7 +61,7 @@ def test_ensemble_arguments_checked_before_the_device():
     ("h_mom_adv_order", 6, "orders"),
     ("specified", 1, "lateral"),
     ("diff_opt", 1, "diffusion"),
-    ("damp_opt", 3, "damp_opt"),
+    ("damp_opt", 1, "damp_opt"),
     ("moist_adv_opt", 2, "advection options"),
     ("scalar_adv_opt", 3, "advection options"),
 ])

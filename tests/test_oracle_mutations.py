@@ -18,7 +18,7 @@ import oracle_pins
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE = os.path.join(ROOT, "oracle")
-SRCS = ["bc_prep.c", "advect.c", "advect_pd.c", "tend.c", "small.c", "scalar_eos.c", "solve_step.c"]
+SRCS = ["bc_prep.c", "advect.c", "advect_pd.c", "mixing.c", "tend.c", "small.c", "scalar_eos.c", "solve_step.c"]
 CFLAGS = ["-O1", "-fPIC", "-std=c11", "-ffp-contract=off", "-fno-fast-math", "-w", "-DWREAL=double",
           "-DWREAL_IS_DOUBLE=1"]
 
@@ -79,6 +79,19 @@ MUTANTS = [
     ("pd_prep_new_mass_for_old", "solve_step.c",
      "s->c1h, s->c2h, s->mu_1, s->mu_1, s->mub, dt_rk, &dt_);", "s->c1h, s->c2h, s->mu_2, s->mu_1, s->mub, dt_rk, &dt_);",
      1),
+    ("sixth_order_coefficient_half_dropped", "mixing.c",
+     "const wreal coef = diff_6th_factor * R(0.015625) / (R(2.0) * dt);",
+     "const wreal coef = diff_6th_factor * R(0.015625) / dt;", 1),
+    ("sixth_order_u_face_mass_shifted", "mixing.c", "m0 = MU(i - 1, j);", "m0 = MU(i, j);", 1),
+    ("sixth_order_limiter_sign", "mixing.c",
+     "if (fx0 * (F(i, k, j) - F(i - 1, k, j)) <= R(0.0)) fx0 = R(0.0);",
+     "if (fx0 * (F(i, k, j) - F(i - 1, k, j)) >= R(0.0)) fx0 = R(0.0);", 1),
+    ("rayleigh_interpolation_offset", "mixing.c",
+     "(z00[k] - A1(z_base, k2)) / (A1(z_base, k2 + 1) - A1(z_base, k2));",
+     "(z00[k] - A1(z_base, k2 + 1)) / (A1(z_base, k2 + 1) - A1(z_base, k2));", 1),
+    ("rayleigh_full_layer", "mixing.c",
+     "wreal dcoef = R(1.0) - rminw(R(1.0), (ztop - z00[k]) / zdamp);", "wreal dcoef = R(1.0);", 1),
+    ("damp_opt3_half_depth", "small.c", "wreal hbot = htop - hdepth;", "wreal hbot = htop - R(0.5) * hdepth;", 1),
     ("pressure_gradient_half_dropped", "tend.c",
      "wreal dpx = (A2(s->msfux, i, j) / A2(s->msfuy, i, j)) * R(.5) * rdx *",
      "wreal dpx = (A2(s->msfux, i, j) / A2(s->msfuy, i, j)) * rdx *", 1),

@@ -1211,6 +1211,11 @@ __global__ void __launch_bounds__(256) k_ac_wsolve(ARGS args) {
   // fields 3 to 7 are not loaded.
   const float muts = s.muts[o2], mu = s.mu_2[o2];
   const float smdiv = c.smdiv;
+  // damp_opt = 3 (1445-1458): the back-substituted w relaxed toward w_save in the layer
+  // within zdamp of the model top (heights from ph_save + phb), after the whole sweep
+  const bool damp3 = c.damp_opt == 3;
+  const float dampmag = dts * c.dampcoef, hdepth = c.zdamp;
+  const float htop = damp3 ? (ph1f[ob + kde * ni] + phbf[ob + kde * ni]) / K_G : 0.f;
   CT wup = wfe;
   float ph_up = 0.f;  // final ph one level above
   column_march_down<8, CT, PF>(
@@ -1245,6 +1250,17 @@ __global__ void __launch_bounds__(256) k_ac_wsolve(ARGS args) {
             const CT wn = v[0] - v[1] * wup;
             wup = wn;
             wk = (float)wn;
+            w[o] = wk;
+          }
+          if (damp3) {
+            const float hk = (ph1f[o] + phbf[o]) / K_G;
+            const float hbot = htop - hdepth;
+            float dampwt = 0.f;
+            if (hk >= hbot) {
+              const float sn = (float)sin((double)(.5f * (4.f * 0.785398185f) * (hk - hbot) / hdepth));
+              dampwt = dampmag * sn * sn;
+            }
+            wk = (wk - dampwt * (c1f[X1(d, k)] * mut + c2f[X1(d, k)]) * s.w_save[o]) / (1.f + dampwt);
             w[o] = wk;
           }
           phk = (float)v[2] + msfty * .5f * dts * K_G * (1.f + epssm) * wk / (c1f[X1(d, k)] * muts + c2f[X1(d, k)]);

@@ -453,8 +453,9 @@ extern "C" int wrfmi_advance_w(float *w, const float *rw_tend, const float *ww, 
   const int *const dims[18] = {DIMS_PTRS};
   Call C(config_flags, dims);
   wrfmi_state &s = C.s;
-  (void)w_save; (void)c3h; (void)c4h; (void)c3f; (void)c4f; (void)alb; (void)dnw; (void)a; (void)alpha;
+  (void)c3h; (void)c4h; (void)c3f; (void)c4f; (void)alb; (void)dnw; (void)a; (void)alpha;
   C.bind(s.w_2, w, "w"); C.bind(s.rw_tend, rw_tend, "rw_tend"); C.bind(s.ww, ww, "ww");
+  C.bind(s.w_save, w_save, "w_save");  // read by the damp_opt = 3 branch
   C.bind(s.u_2, u, "u"); C.bind(s.v_2, v, "v"); C.bind(s.mu_2, mu1, "mu1"); C.bind(s.mut, mut, "mut");
   C.bind(s.muave, muave, "muave"); C.bind(s.muts, muts, "muts");
   C.bind(s.c1h, c1h, "c1h"); C.bind(s.c2h, c2h, "c2h"); C.bind(s.c1f, c1f, "c1f"); C.bind(s.c2f, c2f, "c2f");

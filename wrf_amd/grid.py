@@ -142,6 +142,7 @@ def make_config(spec: DomainSpec) -> Config:
     c.time_step = int(round(spec.dt))
     c.epssm, c.smdiv, c.emdiv = 0.1, 0.1, 0.01
     c.dampcoef, c.zdamp, c.w_crit_cfl = 0.2, 5000.0, 1.2
+    c.diff_6th_factor, c.diff_6th_thresh, c.diff_6th_slopeopt = 0.12, 0.10, 0
     c.precision_mode = spec.precision_mode
     return c
 
@@ -435,6 +436,22 @@ def _vertical_coords(hs: HostState, spec: DomainSpec, p_top: float):
                 c1h=c1h, c2h=c2h, c3h=c3h, c4h=c4h, c1f=c1f, c2f=c2f, c3f=c3f, c4f=c4f)
 
 
+def _base_profiles(hs: HostState):
+    """The base-state profiles of the upper-level Rayleigh damper from the column (ids, jds)
+    of the initial state, in REAL arithmetic (module_initialize_ideal.F:1579-1585): u, v and
+    the theta perturbation of that column, and the height of its mass levels."""
+    d = hs.dims
+    i, j = d.ids - d.ims, d.jds - d.jms
+    nz = d.kde - d.kds
+    f32, g = np.float32, np.float32(9.81)
+    A1, A3 = hs.a1, hs.a3
+    for n, src in (("t_base", "t_1"), ("u_base", "u_1"), ("v_base", "v_1")):
+        A1[n][:nz] = A3[src][j, :nz, i]
+    phb, ph1 = A3["phb"][j, :, i], A3["ph_1"][j, :, i]
+    for k in range(nz):
+        A1["z_base"][k] = f32(0.5) * (((phb[k] + phb[k + 1]) + ph1[k]) + ph1[k + 1]) / g
+
+
 def build_initial_state(spec: DomainSpec) -> HostState:
     """Analytic supercell-type initial state (see the module docstring); a spec with the
     stable sounding goes to build_terrain_state."""
@@ -599,6 +616,7 @@ def build_initial_state(spec: DomainSpec) -> HostState:
         hs.q_old[m][...] = hs.q[m]
     for n in ("mub", "mu_1", "mu_2"):
         _halo_fill_np(A2[n], d, per, False, False)
+    _base_profiles(hs)
     return hs
 
 
@@ -743,6 +761,7 @@ def build_terrain_state(spec: DomainSpec) -> HostState:
     for w, u, v in (("w_1", "u_1", "v_1"), ("w_2", "u_2", "v_2")):
         set_w_surface_np(hs, A3[w], A3[u], A3[v], fill=True)
         _halo_fill_np(A3[w], d, per, False, False)
+    _base_profiles(hs)
     return hs
 
 
