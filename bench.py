@@ -12,9 +12,10 @@ events on its launch stream over the timed steps, against 8 TB/s; its algorithmi
 the 3-D arrays it must read and write once per launch.
 ensemble = member-steps/s of the 64-member uneven ensemble, member by member with the plain
 kernels and batched (one launch per kernel over all members, hipGraph replay).
-positive_definite, mixing, w_damping = the same workload with positive-definite species
-transport on the last substep (moist_adv_opt = scalar_adv_opt = 1), with sixth-order
-diffusion and Rayleigh damping on the first (diff_6th_opt 2, damp_opt 2), and with advance_w's
+positive_definite, mixing, diffusion, w_damping = the same workload with positive-definite
+species transport on the last substep (moist_adv_opt = scalar_adv_opt = 1), with sixth-order
+diffusion and Rayleigh damping on the first (diff_6th_opt 2, damp_opt 2), with constant-
+coefficient second-order diffusion on the first (diff_opt 1, km_opt 1), and with advance_w's
 damping layer (damp_opt 3): ms per step and the kernels the options add, against 8 TB/s.
 terrain (with --terrain) = the 1200x1200x80 terrain case, FP32 and precision forms.
 cpu_baseline = the FP32 C restatement of the same step (the oracle), OpenMP over the host
@@ -177,6 +178,11 @@ OPTION_LEGS = {
         "k_diff6": 3,             # field tend -> tend
         "k_rayleigh": None,       # the levels of the damping layer only
     }),
+    # second-order diffusion with constant eddy coefficients (diff_opt 1, km_opt 1) of u v w
+    # theta and the species on the first RK substep
+    "diffusion": (dict(diff_opt=1, km_opt=1, khdif=300.0, kvdif=50.0), {
+        "k_diff2": 4,             # field alt tend -> tend
+    }),
     # advance_w's implicit Rayleigh damping of w in the top 5 km (damp_opt 3)
     "w_damping": (dict(damp_opt=3, zdamp=5000.0, dampcoef=0.2), {
         "k_ac_wsolve": None,
@@ -292,7 +298,7 @@ def main():
     ap.add_argument("--terrain-steps", type=int, default=20)
     ap.add_argument("--terrain-only", action="store_true", help="run only the terrain leg (profiling)")
     ap.add_argument("--no-options", action="store_true",
-                    help="skip the option legs (positive-definite transport, mixing, w damping)")
+                    help="skip the option legs (positive-definite transport, mixing, diffusion, w damping)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after the timed steps, copy a buffer of this many MiB once (calibrates HBM counters)")
     args = ap.parse_args()

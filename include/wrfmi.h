@@ -65,8 +65,11 @@ typedef struct wrfmi_config {
   int time_step_sound;   /* 0 = derive from dt and dx as solve_em.F:446-463 */
   int time_step;         /* namelist integer time_step (its sign enters the upwind fluxes) */
   int diff_6th_slopeopt; /* 6th-order diffusion tapered on steep terrain (1) or not (0) */
+  int km_opt;            /* eddy coefficients of diff_opt 1: 1 = constant khdif, kvdif */
+  int bl_pbl_physics;    /* 0: diff_opt 1 also diffuses vertically (no boundary-layer scheme) */
   float epssm, smdiv, emdiv, dampcoef, zdamp, w_crit_cfl;
   float diff_6th_factor, diff_6th_thresh;
+  float khdif, kvdif;    /* horizontal and vertical eddy viscosity of km_opt 1 (m2/s) */
   /* 0: FP32 everywhere, reference operation order.
    * 1: FP32 bulk, FP64 calc_coef_w + advance_w tridiagonal solve,
    *    compensated (two-sum) column mass-divergence integrals. */

@@ -25,7 +25,8 @@
   X(ru_tend) X(rv_tend) X(rw_tend) X(ph_tend) X(t_tend)                         \
   X(ru_tendf) X(rv_tendf) X(rw_tendf) X(ph_tendf) X(t_tendf)                    \
   X(h_diabatic)                                                                 \
-  X(cqu) X(cqv) X(cqw) X(c2a) X(pm1) X(ww1) X(t_2save) X(advect_tend)
+  X(cqu) X(cqv) X(cqw) X(c2a) X(pm1) X(ww1) X(t_2save) X(advect_tend)            \
+  X(t_init)
 
 /* 2-D fields */
 #define WRFMI_FIELDS_2D(X)                                                      \
@@ -34,12 +35,14 @@
   X(msfux) X(msfuy) X(msfvx) X(msfvx_inv) X(msfvy) X(msftx) X(msfty)            \
   X(f) X(e) X(sina) X(cosa) X(xlat)
 
-/* 1-D vertical arrays: level coefficients, and the base-state profiles of the upper-level
- * Rayleigh damper (u, v, theta perturbation and height of a reference column) */
+/* 1-D vertical arrays: level coefficients, the base-state profiles of the upper-level
+ * Rayleigh damper (u, v, theta perturbation and height of a reference column; u_base and
+ * v_base are also the reference of diff_opt 1's vertical diffusion) and the QV profile
+ * that diff_opt 1 diffuses QV relative to */
 #define WRFMI_FIELDS_1D(X)                                                      \
   X(znu) X(znw) X(dnw) X(rdnw) X(dn) X(rdn) X(fnm) X(fnp)                       \
   X(c1h) X(c2h) X(c1f) X(c2f) X(c3h) X(c4h) X(c3f) X(c4f)                       \
-  X(u_base) X(v_base) X(t_base) X(z_base)
+  X(u_base) X(v_base) X(t_base) X(z_base) X(qv_base)
 
 /* REAL scalars of the domain (grid%<name>) */
 #define WRFMI_SCALARS_R(X)                                                      \

@@ -14,7 +14,7 @@
  * Every function cites the reference file:line it follows.
  *
  * How each part is pinned (tests/test_oracle.py, tests/oracle_pins.py; the mutation test
- * tests/test_oracle_mutations.py builds this source with 23 plausible mistakes planted and
+ * tests/test_oracle_mutations.py builds this source with 35 plausible mistakes planted and
  * shows that every one fails at least one pin):
  *   o_advect_scalar  bit for bit against the reference's own advect_scalar, compiled in
  *                    place (oracle/_ref) and through tests/golden/advect_scalar_ref.npz:
@@ -42,6 +42,15 @@
  *   o_advance_mu_t   mass budget, omega'' from continuity, uniform theta preserved.
  *   o_small_step_prep/finish  the coupling round trip returns time level n.
  *   rk_update_scalar, sumflux  a uniform mixing ratio stays uniform through whole steps.
+ *   o_sixth_order_diffusion  the checkerboard damping rate, flux form (periodic sum zero),
+ *                    the monotone limiter on a spike, the slope tapering factor.
+ *   o_rk_rayleigh_damp  the relaxation rate sin^2 profile toward linear base profiles.
+ *   advance_w damp_opt 3  the damped w over 1 + dts dampcoef sin^2(...) in the layer.
+ *   o_horizontal_diffusion(_3dmp)  the discrete Laplacian's eigenvalue on cosines (v's
+ *                    y part without the column mass, as the reference has it), flux form,
+ *                    the base subtracted.
+ *   o_vertical_diffusion*  2 a mu K for a z^2 profile on interior levels, zero gradient
+ *                    below the lowest level, no flux through the top, bases subtracted.
  *   whole step       resting atmosphere stays at rest (flat and over terrain); uniform
  *                    state stays uniform; mirror and x/y-transpose symmetry of a centred
  *                    bubble (periodic and open); periodic translation invariance (bitwise);
@@ -49,7 +58,8 @@
  *                    independently built hydrostatic state; FP32 and FP64 builds agree.
  * Parity unpinned (no reference build, fixture or closed form covers them, and the
  * benchmark configurations do not exercise them): the w_damping = 1 branch of w_damp,
- * damp_opt = 3 in advance_w, top_lid = 1, phi_adv_z = 2, hypsometric_opt = 2's
+ * the open- and specified-boundary loop bounds of the diffusion routines (their closed
+ * forms run on periodic domains), top_lid = 1, phi_adv_z = 2, hypsometric_opt = 2's
  * log-pressure form of calc_p_rho_phi beyond the initial state it reproduces,
  * use_theta_m = 0, Coriolis with f or e non-zero, curvature with map factors other than
  * 1.  The terrain terms (the surface-w term of advance_w and set_w_surface, with their
@@ -169,6 +179,31 @@ void o_sixth_order_diffusion(char name, const wreal *field, wreal *tend, const w
                              const wreal *msfty, const wreal *msfux, const wreal *msfuy, const wreal *msfvx,
                              const wreal *msfvy, const dims_t *d);
 void o_rk_rayleigh_damp(ostate *s, const cfg_t *c, const dims_t *d);
+void o_horizontal_diffusion(char name, const wreal *field, wreal *tend, const wreal *mut, const wreal *c1,
+                            const wreal *c2, const cfg_t *c, const wreal *msfux, const wreal *msfuy,
+                            const wreal *msfvx, const wreal *msfvx_inv, const wreal *msfvy, const wreal *msftx,
+                            const wreal *msfty, wreal khdif, const wreal *xkmhd, wreal rdx, wreal rdy,
+                            const dims_t *d);
+void o_horizontal_diffusion_3dmp(char name, const wreal *field, wreal *tend, const wreal *mut, const wreal *c1,
+                                 const wreal *c2, const cfg_t *c, const wreal *base_3d, const wreal *msfux,
+                                 const wreal *msfuy, const wreal *msfvx, const wreal *msfvx_inv, const wreal *msfvy,
+                                 const wreal *msftx, const wreal *msfty, wreal khdif, const wreal *xkmhd, wreal rdx,
+                                 wreal rdy, const dims_t *d);
+void o_vertical_diffusion(char name, const wreal *field, wreal *tend, const cfg_t *c, const wreal *c1,
+                          const wreal *c2, const wreal *alt, const wreal *mut, const wreal *rdn, const wreal *rdnw,
+                          wreal kvdif, const dims_t *d);
+void o_vertical_diffusion_mp(const wreal *field, wreal *tend, const cfg_t *c, const wreal *base, const wreal *c1,
+                             const wreal *c2, const wreal *alt, const wreal *mut, const wreal *rdn, const wreal *rdnw,
+                             wreal kvdif, const dims_t *d);
+void o_vertical_diffusion_3dmp(const wreal *field, wreal *tend, const cfg_t *c, const wreal *base_3d,
+                               const wreal *c1, const wreal *c2, const wreal *alt, const wreal *mut, const wreal *rdn,
+                               const wreal *rdnw, wreal kvdif, const dims_t *d);
+void o_vertical_diffusion_u(const wreal *field, wreal *tend, const cfg_t *c, const wreal *u_base, const wreal *c1h,
+                            const wreal *c2h, const wreal *alt, const wreal *muu, const wreal *rdn, const wreal *rdnw,
+                            wreal kvdif, const dims_t *d);
+void o_vertical_diffusion_v(const wreal *field, wreal *tend, const cfg_t *c, const wreal *v_base, const wreal *c1h,
+                            const wreal *c2h, const wreal *alt, const wreal *muv, const wreal *rdn, const wreal *rdnw,
+                            wreal kvdif, const dims_t *d);
 
 /* ---- rk_tendency and rk_addtend_dry ---- */
 int o_rk_tendency(ostate *s, const cfg_t *c, const dims_t *d, int rk_step);

@@ -1,6 +1,7 @@
 /*
  * Oracle: scalar transport and the equation of state.  TEST INFRASTRUCTURE.
- *   rk_scalar_tend    dyn_em/module_em.F:1096-1442 (advect_scalar branch, no diffusion)
+ *   rk_scalar_tend    dyn_em/module_em.F:1096-1442 (advect_scalar / advect_scalar_pd, diff_opt 1
+ *                     with km_opt 1, sixth-order diffusion)
  *   rk_update_scalar  dyn_em/module_em.F:1587-1799
  *   calc_p_rho_phi    dyn_em/module_big_step_utilities_em.F:953-1223 (non-hydrostatic)
  * Species s->q[m] (m < n_moist are moist species, q[0] = QV); at the solve_em.F
@@ -13,9 +14,29 @@
 /* advect_tend is zeroed over the tile (zero_tend, module_em.F:1224) and then filled */
 int o_rk_scalar_tend(ostate *s, const cfg_t *c, const dims_t *d, int m, int rk_step, wreal dt_rk) {
   int adv_opt = m < s->n_moist ? c->moist_adv_opt : c->scalar_adv_opt;
-  if ((adv_opt != 0 && adv_opt != 1) || c->zadvect_implicit || c->diff_opt != 0 || c->diff_6th_opt < 0 ||
-      c->diff_6th_opt > 2)
+  if ((adv_opt != 0 && adv_opt != 1) || c->zadvect_implicit || c->diff_opt < 0 || c->diff_opt > 1 ||
+      (c->diff_opt == 1 && c->km_opt != 1) || c->diff_6th_opt < 0 || c->diff_6th_opt > 2)
     return WRFMI_E_BADOPT;
+  /* module_em.F:1380-1415: at rk_step 1 with diff_opt 1, horizontal diffusion with xkhh
+   * (khdif/prandtl under km_opt 1) and, with no boundary-layer scheme, vertical diffusion
+   * with kvdif/prandtl: of qv - qv_base for QV, of the species itself otherwise */
+  if (rk_step == 1 && c->diff_opt == 1) {
+    const wreal prandtl = R(1.) / R(3.0);
+    const wreal khdq = (wreal)c->khdif / prandtl, kvdq = (wreal)c->kvdif / prandtl;
+    const size_t n = (size_t)NI_ * NK_ * (size_t)(d->jme - d->jms + 1);
+    wreal *xkhh = (wreal *)malloc(n * sizeof(wreal));
+    for (size_t q = 0; q < n; q++) xkhh[q] = khdq;
+    o_horizontal_diffusion('m', s->q[m], s->q_tend[m], s->muts, s->c1h, s->c2h, c, s->msfux, s->msfuy, s->msfvx,
+                           s->msfvx_inv, s->msfvy, s->msftx, s->msfty, khdq, xkhh, s->rdx, s->rdy, d);
+    free(xkhh);
+    if (c->bl_pbl_physics == 0) {
+      if (m == 0 && s->n_moist > 0)
+        o_vertical_diffusion_mp(s->q[m], s->q_tend[m], c, s->qv_base, s->c1h, s->c2h, s->alt, s->muts, s->rdn,
+                                s->rdnw, kvdq, d);
+      else
+        o_vertical_diffusion('m', s->q[m], s->q_tend[m], c, s->c1h, s->c2h, s->alt, s->muts, s->rdn, s->rdnw, kvdq, d);
+    }
+  }
   /* module_em.F:1421-1436: at rk_step 1, 6th-order diffusion of the species into its
    * tendency (the physics tendency array), with rk_scalar_tend's mut = muts and dt_step */
   if (rk_step == 1 && c->diff_6th_opt != 0)

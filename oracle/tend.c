@@ -424,12 +424,53 @@ static int curvature(ostate *s, const cfg_t *c, const dims_t *d) {
   return 0;
 }
 
-/* dyn_em/module_em.F:190-955, physics off (diff_opt = 0, rad_nudge = 0, IEVA off); at
- * rk_step 1 the forward-step terms of 800-950: sixth_order_diffusion (diff_6th_opt 1, 2) and
- * rk_rayleigh_damp (damp_opt 2) into the *_tendf arrays */
+/* isotropic_km (dyn_em/module_diffusion_em.F:1718-1772, km_opt 1) with phy_bc's fill of the
+ * halo: xkmh = khdif and xkhh = khdif/prandtl everywhere (prandtl = 1./3.0,
+ * share/module_model_constants.F:86) */
+static wreal *const_field(wreal v, const dims_t *d) {
+  const size_t n = (size_t)NI_ * NK_ * (size_t)(d->jme - d->jms + 1);
+  wreal *a = (wreal *)malloc(n * sizeof(wreal));
+  for (size_t q = 0; q < n; q++) a[q] = v;
+  return a;
+}
+
+/* dyn_em/module_em.F:799-877: diff_opt 1, the horizontal diffusion of u, v, w and theta
+ * (theta relative to t_init) and, with no boundary-layer scheme, their vertical diffusion */
+static void diff_opt1(ostate *s, const cfg_t *c, const dims_t *d) {
+  const wreal khdif = (wreal)c->khdif, kvdif = (wreal)c->kvdif;
+  wreal *xkmh = const_field(khdif, d), *xkhh = const_field(khdif / (R(1.) / R(3.0)), d);
+#define HD(nm, fld, tnd, c1, c2) \
+  o_horizontal_diffusion(nm, s->fld, s->tnd, s->mut, s->c1, s->c2, c, s->msfux, s->msfuy, s->msfvx, s->msfvx_inv, \
+                         s->msfvy, s->msftx, s->msfty, khdif, xkmh, s->rdx, s->rdy, d)
+  HD('u', u_2, ru_tendf, c1h, c2h);
+  HD('v', v_2, rv_tendf, c1h, c2h);
+  HD('w', w_2, rw_tendf, c1f, c2f);
+#undef HD
+  const wreal khdq = R(3.) * khdif;
+  o_horizontal_diffusion_3dmp('m', s->t_2, s->t_tendf, s->mut, s->c1h, s->c2h, c, s->t_init, s->msfux, s->msfuy,
+                              s->msfvx, s->msfvx_inv, s->msfvy, s->msftx, s->msfty, khdq, xkhh, s->rdx, s->rdy, d);
+  if (c->bl_pbl_physics == 0) {
+    o_vertical_diffusion_u(s->u_2, s->ru_tendf, c, s->u_base, s->c1h, s->c2h, s->alt, s->muu, s->rdn, s->rdnw, kvdif,
+                           d);
+    o_vertical_diffusion_v(s->v_2, s->rv_tendf, c, s->v_base, s->c1h, s->c2h, s->alt, s->muv, s->rdn, s->rdnw, kvdif,
+                           d);
+    if (c->non_hydrostatic)
+      o_vertical_diffusion('w', s->w_2, s->rw_tendf, c, s->c1f, s->c2f, s->alt, s->mut, s->rdn, s->rdnw, kvdif, d);
+    const wreal kvdq = R(3.) * kvdif;
+    o_vertical_diffusion_3dmp(s->t_2, s->t_tendf, c, s->t_init, s->c1h, s->c2h, s->alt, s->mut, s->rdn, s->rdnw, kvdq,
+                              d);
+  }
+  free(xkmh);
+  free(xkhh);
+}
+
+/* dyn_em/module_em.F:190-955, physics off (rad_nudge = 0, IEVA off); at rk_step 1 the
+ * forward-step terms of 797-950: diff_opt 1 with km_opt 1, sixth_order_diffusion
+ * (diff_6th_opt 1, 2) and rk_rayleigh_damp (damp_opt 2) into the *_tendf arrays */
 int o_rk_tendency(ostate *s, const cfg_t *c, const dims_t *d, int rk_step) {
-  if (c->zadvect_implicit || c->momentum_adv_opt != 1 || c->diff_opt != 0 || c->diff_6th_opt < 0 ||
-      c->diff_6th_opt > 2 || c->damp_opt == 1 || c->pert_coriolis || d->kde + 1 >= WRFMI_ORACLE_MAXK)
+  if (c->zadvect_implicit || c->momentum_adv_opt != 1 || c->diff_opt < 0 || c->diff_opt > 1 ||
+      (c->diff_opt == 1 && c->km_opt != 1) || c->diff_6th_opt < 0 || c->diff_6th_opt > 2 || c->damp_opt == 1 ||
+      c->pert_coriolis || d->kde + 1 >= WRFMI_ORACLE_MAXK)
     return WRFMI_E_BADOPT;
   zero3(s->ru_tend, d); zero3(s->rv_tend, d); zero3(s->rw_tend, d); zero3(s->t_tend, d); zero3(s->ph_tend, d);
   zero3(s->u_save, d); zero3(s->v_save, d); zero3(s->w_save, d); zero3(s->ph_save, d); zero3(s->t_save, d);
@@ -457,6 +498,7 @@ int o_rk_tendency(ostate *s, const cfg_t *c, const dims_t *d, int rk_step) {
   coriolis(s, c, d);
   rc = curvature(s, c, d);
   if (rc || rk_step != 1) return rc;
+  if (c->diff_opt == 1) diff_opt1(s, c, d);
   if (c->diff_6th_opt != 0) {
     const wreal f6 = (wreal)c->diff_6th_factor;
 #define SIX(nm, fld, tnd, c1, c2) \

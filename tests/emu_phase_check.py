@@ -56,6 +56,18 @@ CASES = {
                         n_bubbles=0, noise_amp=0.0, seed=37,
                         opts=dict(diff_6th_opt=2, diff_6th_slopeopt=1, diff_6th_thresh=0.02, damp_opt=2,
                                   zdamp=6000.0)),
+    # diff_opt 1 with constant eddy coefficients (km_opt 1): second-order horizontal and
+    # vertical diffusion of u, v, w, theta (relative to t_init) and the species (QV relative
+    # to qv_base) on the first substep, with 6th-order diffusion and damping on top
+    "diff_periodic": dict(nx=14, ny=11, nz=12, bc="periodic", n_passive=1, n_bubbles=2, bubble_amp=2.0, seed=38,
+                          opts=dict(diff_opt=1, km_opt=1, khdif=300.0, kvdif=50.0)),
+    "diff_open": dict(nx=15, ny=13, nz=10, bc="open", n_passive=1, n_bubbles=1, seed=39,
+                      opts=dict(diff_opt=1, km_opt=1, khdif=200.0, kvdif=80.0, diff_6th_opt=2, damp_opt=3,
+                                zdamp=5000.0)),
+    "diff_terrain": dict(nx=16, ny=12, nz=10, dx=2000.0, dy=2000.0, dt=6.0, bc_x="open", bc_y="periodic",
+                         hybrid_opt=2, sounding="stable", n_ridges=3, ridge_a=(3000.0, 8000.0), n_passive=1,
+                         n_bubbles=0, noise_amp=0.0, seed=40,
+                         opts=dict(diff_opt=1, km_opt=1, khdif=500.0, kvdif=30.0, damp_opt=2, zdamp=6000.0)),
     "pd_mixed_unaligned": dict(nx=14, ny=11, nz=8, bc_x="open", bc_y="periodic", n_passive=1, n_bubbles=1, seed=28,
                                pad_i=1, pd=(0, 1)),
 }

@@ -989,6 +989,171 @@ def pin_sixth_order_diffusion(o):
     _close(t[sl], -factor * M * A * chk[sl] / (2.0 * dt) * ((1.0 - slope_frac) + 1.0), 1e-12, "slope tapering")
 
 
+# ----------------------------------------------------------------------------- diff_opt 1
+def _hdiff(D, name, field, mut, xk, base=None):
+    """horizontal_diffusion (or _3dmp with `base`) of `field` with the coefficient array `xk`
+    from a zero tendency."""
+    o = D.o
+    vp = ctypes.c_void_p
+    P = lambda a: a.ctypes.data_as(vp)  # noqa: E731
+    c = lambda x: np.ascontiguousarray(x, dtype=o.dtype)  # noqa: E731
+    f, m, k = c(field), c(mut), c(xk)
+    tend = np.zeros_like(f)
+    c1, c2 = ("c1f", "c2f") if name == "w" else ("c1h", "c2h")
+    msf = [P(D.a[n]) for n in ("msfux", "msfuy", "msfvx", "msfvx_inv", "msfvy", "msftx", "msfty")]
+    if base is None:
+        fn = o.lib.o_horizontal_diffusion
+        fn.argtypes = [ctypes.c_char] + [vp] * 6 + [vp] * 7 + [o.ctype, vp, o.ctype, o.ctype, vp]
+        fn(name.encode(), P(f), P(tend), P(m), P(D.a[c1]), P(D.a[c2]), ctypes.byref(D.cfg), *msf, 0.0, P(k),
+           1.0 / D.dx, 1.0 / D.dy, ctypes.byref(D.d))
+    else:
+        b = c(base)
+        fn = o.lib.o_horizontal_diffusion_3dmp
+        fn.argtypes = [ctypes.c_char] + [vp] * 7 + [vp] * 7 + [o.ctype, vp, o.ctype, o.ctype, vp]
+        fn(name.encode(), P(f), P(tend), P(m), P(D.a[c1]), P(D.a[c2]), ctypes.byref(D.cfg), P(b), *msf, 0.0, P(k),
+           1.0 / D.dx, 1.0 / D.dy, ctypes.byref(D.d))
+    fn.restype = None
+    return tend
+
+
+def pin_horizontal_diffusion(o):
+    """horizontal_diffusion and horizontal_diffusion_3dmp (module_big_step_utilities_em.F:
+    2715-3060), diff_opt 1:
+    (1) f = A cos(2 pi i/nx) + B cos(2 pi j/ny) with uniform mu and eddy viscosity K is an
+        eigenfunction of the second difference: tendency = mu K (-(4/dx^2) sin^2(pi/nx) A
+        cos(...) - (4/dy^2) sin^2(pi/ny) B cos(...)), for u, v, w and mass points -- except
+        that the reference's v component takes its y part without the column mass (its
+        mkrdym/mkrdyp, module_big_step_utilities_em.F:2822-2823, have no c1 mu + c2 factor);
+    (2) with non-uniform mu and K the tendency is a flux difference: its periodic domain sum
+        vanishes (both points beside a face weight it with the same mass and K);
+    (3) the 3dmp form diffuses field - base: base + f gives (1)'s tendency for any base."""
+    D = Dom(o, nx=10, ny=8, nz=5)
+    D.set_levels(stretched_dnw(5, seed=7))
+    d = D.d
+    sh = D.a["u_2"].shape
+    J, _, I = np.meshgrid(np.arange(sh[0]) + d.jms, np.arange(sh[1]) + d.kms, np.arange(sh[2]) + d.ims, indexing="ij")
+    nx, ny, nk = d.ide - d.ids, d.jde - d.jds, d.kde - 1
+    A, B, M, K = 2.0, -1.5, 9.0e4, 75.0
+    cx, cy = np.cos(2 * np.pi * (I - d.ids) / nx), np.cos(2 * np.pi * (J - d.jds) / ny)
+    f = A * cx + B * cy
+    lam_x = -K * (4.0 / D.dx ** 2) * np.sin(np.pi / nx) ** 2 * A * cx
+    lam_y = -K * (4.0 / D.dy ** 2) * np.sin(np.pi / ny) ** 2 * B * cy
+    lam = M * (lam_x + lam_y)
+    mut = np.full(D.a["mut"].shape, M)
+    xk = np.full(sh, K)
+    J0, I0 = d.jds - d.jms, d.ids - d.ims
+    for name in ("u", "v", "w", "m"):
+        t = _hdiff(D, name, f, mut, xk)
+        k0 = 1 if name == "w" else 0
+        sl = (slice(J0, J0 + ny), slice(k0, nk), slice(I0, I0 + nx))
+        exp = M * lam_x + lam_y if name == "v" else lam
+        _close(t[sl], exp[sl], 1e-10, f"second-difference eigenvalue, {name}")
+        assert np.all(t[:, :k0, :] == 0.0) and np.all(t[:, nk:, :] == 0.0)
+    rng = np.random.default_rng(8)
+    g = rng.uniform(-3.0, 3.0, sh)
+    mut_r = rng.uniform(8.5e4, 9.5e4, D.a["mut"].shape)
+    xk_r = rng.uniform(20.0, 200.0, sh)
+    for a in (g, xk_r):
+        grid._halo_fill_np(a, d, True, False, False)
+    grid._halo_fill_np(mut_r, d, True, False, False)
+    for name in ("u", "v", "w", "m"):
+        t = _hdiff(D, name, g, mut_r, xk_r)
+        k0 = 1 if name == "w" else 0
+        sl = (slice(J0, J0 + ny), slice(k0, nk), slice(I0, I0 + nx))
+        tot = abs(float(t[sl].sum())) / float(np.abs(t[sl]).sum())
+        assert tot < 1e-12, (name, tot)
+    base = rng.uniform(280.0, 320.0, sh)
+    grid._halo_fill_np(base, d, True, False, False)
+    t = _hdiff(D, "m", base + f, mut, xk, base=base)
+    sl = (slice(J0, J0 + ny), slice(0, nk), slice(I0, I0 + nx))
+    _close(t[sl], lam[sl], 1e-9, "3dmp: field - base diffused")
+
+
+def _vdiff(D, kind, field, mu, kv, base=None):
+    """One of the vertical diffusion routines from a zero tendency: kind 'w' or 'm'
+    (vertical_diffusion), 'mp' (base a profile), '3dmp' (base a 3-D array), 'u', 'v'."""
+    o = D.o
+    vp = ctypes.c_void_p
+    P = lambda a: a.ctypes.data_as(vp)  # noqa: E731
+    c = lambda x: np.ascontiguousarray(x, dtype=o.dtype)  # noqa: E731
+    f, m = c(field), c(mu)
+    tend = np.zeros_like(f)
+    c1, c2 = ("c1f", "c2f") if kind == "w" else ("c1h", "c2h")
+    tail = [P(D.a["alt"]), P(m), P(D.a["rdn"]), P(D.a["rdnw"]), kv, ctypes.byref(D.d)]
+    tys = [vp] * 4 + [o.ctype, vp]
+    if kind in ("w", "m"):
+        fn = o.lib.o_vertical_diffusion
+        fn.argtypes = [ctypes.c_char] + [vp] * 5 + tys
+        fn(kind.encode(), P(f), P(tend), ctypes.byref(D.cfg), P(D.a[c1]), P(D.a[c2]), *tail)
+    else:
+        fn = {"mp": o.lib.o_vertical_diffusion_mp, "3dmp": o.lib.o_vertical_diffusion_3dmp,
+              "u": o.lib.o_vertical_diffusion_u, "v": o.lib.o_vertical_diffusion_v}[kind]
+        fn.argtypes = [vp] * 6 + tys
+        fn(P(f), P(tend), ctypes.byref(D.cfg), P(c(base)), P(D.a[c1]), P(D.a[c2]), *tail)
+    fn.restype = None
+    return tend
+
+
+def pin_vertical_diffusion(o):
+    """vertical_diffusion ('w', 'm'), _mp, _3dmp, _u and _v (module_big_step_utilities_em.F:
+    3064-3581), diff_opt 1, on equally spaced eta levels with an inverse density alpha that
+    varies in x and y: with dz = mu alpha |deta|/g the layer depth in each column (alpha
+    averaged to the u and v columns), a profile a z^2 (plus the routine's base profile or
+    3-D base, random) has the continuum tendency mu K f'' = 2 a mu K on every interior level;
+    the lowest mass level gets none (zero gradient below it), the flux through the model top
+    is zero (the top mass level of 'm', mp and 3dmp and the top w level below the lid lose
+    mu K f'(z) at the face under them over their depth), and u and v leave their top level
+    alone."""
+    nz = 8
+    D = Dom(o, nx=6, ny=5, nz=nz)
+    D.set_levels(np.full(nz, -1.0 / nz))
+    d = D.d
+    sh = D.a["u_2"].shape
+    nk, G_ = d.kde - 1, G
+    M, K, a = 9.0e4, 30.0, 2.0e-6
+    rng = np.random.default_rng(9)
+    I = np.arange(sh[2]) + d.ims
+    J = np.arange(sh[0]) + d.jms
+    alpha = 0.8 + 0.05 * np.sin(0.7 * I)[None, :] + 0.04 * np.cos(0.9 * J)[:, None]  # (nj, ni)
+    D.a["alt"][...] = alpha[:, None, :]
+    mu = np.full(D.a["mut"].shape, M)
+    deta = 1.0 / nz
+    ci, cj = D.ci, D.cj
+    sl_i, sl_j = slice(ci, ci + d.ide - d.ids), slice(cj, cj + d.jde - d.jds)
+    cases = {"m": alpha, "mp": alpha, "3dmp": alpha, "w": alpha,
+             "u": 0.5 * (alpha + np.roll(alpha, 1, axis=1)), "v": 0.5 * (alpha + np.roll(alpha, 1, axis=0))}
+    for kind, al_col in cases.items():
+        dz = M * al_col * deta / G_  # (nj, ni)
+        if kind == "w":
+            z = np.arange(sh[1])[None, :, None] * dz[:, None, :]
+        else:
+            z = (np.arange(sh[1]) + 0.5)[None, :, None] * dz[:, None, :]
+        f = a * z ** 2
+        base = None
+        if kind in ("mp", "u", "v"):
+            base = np.zeros(sh[1])
+            base[:nk] = rng.uniform(-5.0, 5.0, nk)
+            f = f + base[None, :, None]
+        elif kind == "3dmp":
+            base = rng.uniform(280.0, 320.0, sh)
+            f = f + base
+        t = _vdiff(D, kind, f, mu, K, base)
+        inner = t[sl_j, :, sl_i]
+        dzi = dz[sl_j, sl_i]
+        exp = np.zeros_like(inner)
+        if kind == "w":
+            exp[:, 1:nk - 1, :] = 2.0 * a * M * K
+            zt = (nk - 1.5) * dzi  # the mass level between the top two w levels below the lid
+            exp[:, nk - 1, :] = -M * K * 2.0 * a * zt / dzi
+        elif kind in ("u", "v"):
+            exp[:, 1:nk - 1, :] = 2.0 * a * M * K
+        else:
+            exp[:, 1:nk - 1, :] = 2.0 * a * M * K
+            zt = (nk - 1) * dzi  # the face under the top mass level
+            exp[:, nk - 1, :] = -M * K * 2.0 * a * zt / dzi
+        _close(inner, exp, 1e-9, f"vertical diffusion, {kind}")
+
+
 def _damp_levels(D, ztop=20000.0):
     """Flat levels equally spaced in height (phb = g z, ph = 0): w-level and mass heights."""
     kde = D.kde
@@ -1078,4 +1243,5 @@ PINS = [pin_advect_constant_field, pin_advect_linear_eta, pin_coef_w_spectrum, p
         pin_advance_w_hydrostatic, pin_calc_p_rho_linearization, pin_advance_uv_pgf, pin_pressure_gradient,
         pin_advance_mu_t_budget,
         pin_small_step_round_trip, pin_constant_mixing_ratio, pin_resting_over_terrain, pin_advect_scalar_pd,
-        pin_pd_whole_step, pin_sixth_order_diffusion, pin_rayleigh_damp, pin_advance_w_damp_opt3]
+        pin_pd_whole_step, pin_sixth_order_diffusion, pin_rayleigh_damp, pin_advance_w_damp_opt3,
+        pin_horizontal_diffusion, pin_vertical_diffusion]

@@ -56,11 +56,24 @@ def test_ensemble_arguments_checked_before_the_device():
     assert lib.wrfmi_ensemble_step(ctypes.c_void_p(1234), None, 0) == runtime.WRFMI_E_POINTER
 
 
+def test_constant_coefficient_diffusion_only():
+    """diff_opt 1 is implemented with km_opt 1 (constant khdif, kvdif); other eddy-coefficient
+    schemes are refused before any launch."""
+    lib = runtime.load_library()
+    spec = grid.DomainSpec(nx=10, ny=10, nz=6)
+    cfg, dims = grid.make_config(spec), grid.make_dims(spec)
+    cfg.diff_opt, cfg.km_opt = 1, 4
+    st = fields.State32()
+    rc = lib.wrfmi_solve_em_step(ctypes.byref(st), ctypes.byref(cfg), ctypes.byref(dims), None, 1)
+    assert rc == runtime.WRFMI_E_BADOPT
+    assert b"km_opt" in lib.wrfmi_last_error()
+
+
 @pytest.mark.parametrize("field,value,msg", [
     ("rk_ord", 2, "rk_ord"),
     ("h_mom_adv_order", 6, "orders"),
     ("specified", 1, "lateral"),
-    ("diff_opt", 1, "diffusion"),
+    ("diff_opt", 2, "TKE"),
     ("damp_opt", 1, "damp_opt"),
     ("moist_adv_opt", 2, "advection options"),
     ("scalar_adv_opt", 3, "advection options"),

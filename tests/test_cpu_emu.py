@@ -32,7 +32,8 @@ def emu_env():
                                                ("periodic_wide", 1, 1), ("open_wide", 0, 1), ("open_narrow", 1, 2),
                                                ("pd_periodic", 0, 2), ("pd_open", 1, 1), ("pd_mixed_unaligned", 0, 1),
                                                ("pd_periodic_tight", 1, 2), ("mix_periodic", 0, 2),
-                                               ("mix_open", 1, 2), ("mix_terrain", 0, 1)])
+                                               ("mix_open", 1, 2), ("mix_terrain", 0, 1), ("diff_periodic", 0, 2),
+                                               ("diff_open", 1, 2), ("diff_terrain", 0, 1)])
 def test_emulated_kernels_match_oracle(emu_env, case, order, nsteps):
     r = subprocess.run([sys.executable, os.path.join(HERE, "emu_phase_check.py"), case, str(order), str(nsteps)],
                        env=emu_env, capture_output=True, text=True, timeout=600)
@@ -65,7 +66,7 @@ def test_emulated_noalloc_step_matches_allocating_step(emu_env, order):
 
 
 @pytest.mark.parametrize("case,order", [("periodic", 0), ("open", 1), ("terrain_yopen", 0), ("periodic_unaligned", 1),
-                                        ("pd_open", 1), ("mix_open", 1)])
+                                        ("pd_open", 1), ("mix_open", 1), ("diff_open", 0)])
 def test_emulated_reference_argument_entry_points(emu_env, case, order):
     """The entry points with the reference's argument lists against the oracle routines of
     the same name, and two steps of the Fortran solve_em host that calls them."""

@@ -1,8 +1,9 @@
 """Mixing and damping of the first RK substep on the gfx950 path (needs an MI355X):
-sixth-order diffusion (diff_6th_opt 1 and 2, with and without slope tapering), the Rayleigh
-damper toward the base profiles (damp_opt 2) and advance_w's implicit upper damping layer
-(damp_opt 3), against the FP32 oracle bit for bit: per routine, over three steps, under a
-hipGraph replay and at 400x400x80.
+second-order diffusion with constant eddy coefficients (diff_opt 1, km_opt 1), sixth-order
+diffusion (diff_6th_opt 1 and 2, with and without slope tapering), the Rayleigh damper toward
+the base profiles (damp_opt 2) and advance_w's implicit upper damping layer (damp_opt 3),
+against the FP32 oracle bit for bit: per routine, over three steps, under a hipGraph replay
+and at 400x400x80.
 """
 import numpy as np
 import pytest
@@ -23,9 +24,17 @@ CASES = {
                          hybrid_opt=2, sounding="stable", n_ridges=3, ridge_a=(3000.0, 8000.0), n_passive=1,
                          n_bubbles=0, noise_amp=0.0, seed=43),
                     dict(diff_6th_opt=2, diff_6th_slopeopt=1, diff_6th_thresh=0.02, damp_opt=2, zdamp=6000.0), None),
+    "diff_periodic": (dict(nx=24, ny=20, nz=16, bc="periodic", n_passive=1, n_bubbles=2, bubble_amp=2.0, seed=45),
+                      dict(diff_opt=1, km_opt=1, khdif=300.0, kvdif=50.0), None),
+    "diff_terrain": (dict(nx=24, ny=16, nz=14, dx=2000.0, dy=2000.0, dt=6.0, bc_x="open", bc_y="periodic",
+                          hybrid_opt=2, sounding="stable", n_ridges=3, ridge_a=(3000.0, 8000.0), n_passive=1,
+                          n_bubbles=0, noise_amp=0.0, seed=46),
+                     dict(diff_opt=1, km_opt=1, khdif=500.0, kvdif=30.0, diff_6th_opt=1, damp_opt=2, zdamp=6000.0),
+                     None),
     # mixing together with positive-definite transport of every species
     "mix_pd": (dict(nx=24, ny=20, nz=16, bc="periodic", n_passive=2, n_bubbles=2, bubble_amp=2.0, seed=44),
-               dict(diff_6th_opt=2, damp_opt=3, zdamp=6000.0, dampcoef=0.2), (1, 1)),
+               dict(diff_opt=1, km_opt=1, khdif=300.0, kvdif=50.0, diff_6th_opt=2, damp_opt=3, zdamp=6000.0,
+                    dampcoef=0.2), (1, 1)),
 }
 
 
@@ -74,15 +83,16 @@ def test_mixing_steps_bitwise(gpu, case):
     assert not bad, f"arrays differ from the oracle after three steps: {bad}"
 
 
-def test_mixing_changes_the_step(gpu):
+@pytest.mark.parametrize("case", ["mix_open", "diff_periodic"])
+def test_mixing_changes_the_step(gpu, case):
     """The options act: the same state stepped with mixing and damping off differs from the
     one stepped with them on (so that the bitwise tests above compare the mixing itself)."""
-    kw, opts, _ = CASES["mix_open"]
+    kw, opts, _ = CASES[case]
     on, off = runtime.DeviceDomain(_state(kw, opts, None)), runtime.DeviceDomain(_state(kw, {}, None))
     on.step(1)
     off.step(1)
     a, b = on.download(), off.download()
-    for k in ("u_2", "v_2", "w_2", "t_2"):
+    for k in ("u_2", "v_2", "w_2", "t_2", "q0"):
         assert not np.array_equal(a[k], b[k]), k
 
 
@@ -102,7 +112,8 @@ def test_mixing_graph_replay_matches_eager(gpu):
 
 
 @pytest.mark.parametrize("opts", [dict(diff_6th_opt=2, diff_6th_slopeopt=1, damp_opt=2, zdamp=5000.0),
-                                  dict(diff_6th_opt=1, damp_opt=3, zdamp=5000.0, dampcoef=0.2)])
+                                  dict(diff_opt=1, km_opt=1, khdif=300.0, kvdif=50.0, diff_6th_opt=1, damp_opt=3,
+                                       zdamp=5000.0, dampcoef=0.2)])
 def test_mixing_full_size_bitwise(gpu, opts):
     """400x400x80 with the mixing and damping options on: one step bitwise equal to the
     oracle's."""

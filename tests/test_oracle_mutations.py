@@ -91,6 +91,21 @@ MUTANTS = [
      "(z00[k] - A1(z_base, k2 + 1)) / (A1(z_base, k2 + 1) - A1(z_base, k2));", 1),
     ("rayleigh_full_layer", "mixing.c",
      "wreal dcoef = R(1.0) - rminw(R(1.0), (ztop - z00[k]) / zdamp);", "wreal dcoef = R(1.0);", 1),
+    ("hdiff_u_mass_at_own_point", "mixing.c",
+     "mkrdxm = (A2(msftx, i - 1, j) / A2(msfty, i - 1, j)) * CM(i - 1, j) * XK(i - 1, k, j) * rdx;",
+     "mkrdxm = (A2(msftx, i - 1, j) / A2(msfty, i - 1, j)) * CM(i, j) * XK(i - 1, k, j) * rdx;", 1),
+    ("hdiff_mass_k_average_half_dropped", "mixing.c",
+     "mkrdxm = (A2(msfux, i, j) / A2(msfuy, i, j)) * R(0.5) * (XK(i, k, j) + XK(i - 1, k, j)) * R(0.5) *\n"
+     "                   (CM(i, j) + CM(i - 1, j)) * rdx;",
+     "mkrdxm = (A2(msfux, i, j) / A2(msfuy, i, j)) * (XK(i, k, j) + XK(i - 1, k, j)) * R(0.5) *\n"
+     "                   (CM(i, j) + CM(i - 1, j)) * rdx;", 1),
+    ("vdiff_flux_below_lowest_level_zeroed", "mixing.c", "vflux[kts - 1] = vflux[kts];\n      vflux[ktf] = R(0.);\n      for (int k = kts; k <= ktf; k++)",
+     "vflux[kts - 1] = R(0.);\n      vflux[ktf] = R(0.);\n      for (int k = kts; k <= ktf; k++)", 1),
+    ("vdiff_uv_base_sign", "mixing.c", "- A1(base, k + 1) + A1(base, k));", "+ A1(base, k + 1) - A1(base, k));", 1),
+    ("vdiff_uv_level_density_unaveraged", "mixing.c", "(R(0.5) * (A3(alt, im, k, jm) + A3(alt, i, k, j))) *",
+     "(R(0.5) * (A3(alt, i, k, j) + A3(alt, i, k, j))) *", 1),
+    ("vdiff_w_top_level_skipped", "mixing.c", "for (int k = kts + 1; k <= ktf; k++)\n        A3(tend, i, k, j) = A3(tend, i, k, j) + A1(rdn, k)",
+     "for (int k = kts + 1; k <= ktf - 1; k++)\n        A3(tend, i, k, j) = A3(tend, i, k, j) + A1(rdn, k)", 1),
     ("damp_opt3_half_depth", "small.c", "wreal hbot = htop - hdepth;", "wreal hbot = htop - R(0.5) * hdepth;", 1),
     ("pressure_gradient_half_dropped", "tend.c",
      "wreal dpx = (A2(s->msfux, i, j) / A2(s->msfuy, i, j)) * R(.5) * rdx *",

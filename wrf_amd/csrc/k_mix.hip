@@ -1,8 +1,11 @@
 // Forward-step mixing and damping of the first RK substep, added into the *_tendf arrays
-// before rk_addtend_dry reads them (dyn_em/module_em.F:800-950, 1421-1436):
+// before rk_addtend_dry reads them (dyn_em/module_em.F:797-950, 1380-1436):
 //   k_diff6   : sixth_order_diffusion (module_big_step_utilities_em.F:6220-6636), one point
 //               per thread, for u, v, w, theta (rk_tendency's call with the full dt) and a
 //               species (rk_scalar_tend's call into its tendency, with muts and dt_rk)
+//   k_diff2   : diff_opt 1 with km_opt 1, horizontal_diffusion and the vertical_diffusion
+//               routines (module_big_step_utilities_em.F:2715-3581), one point per thread,
+//               for u, v, w, theta and a species
 //   k_rayleigh: rk_rayleigh_damp (module_big_step_utilities_em.F:5836-6110), one column per
 //               thread: u, v, theta relaxed toward the base profiles interpolated to the
 //               heights of the levels in the damping layer, w toward zero
@@ -150,6 +153,180 @@ __global__ void __launch_bounds__(256) k_diff6(ARGS args) {
   tend[o] = tend[o] + tx + ty;
 }
 
+// K: 0 = u, 1 = v, 2 = w, 3 = theta (rk_tendency, module_em.F:799-877), 4 = species
+// a.species (rk_scalar_tend, module_em.F:1380-1415).  diff_opt 1 with km_opt 1: the
+// horizontal diffusion (horizontal_diffusion, and horizontal_diffusion_3dmp of theta -
+// t_init) with the constant eddy viscosity of isotropic_km, then, with no boundary-layer
+// scheme, the vertical diffusion (vertical_diffusion_u/_v, vertical_diffusion 'w' and 'm',
+// _3dmp of theta - t_init, _mp of QV - qv_base), both added to the point's tendency in the
+// reference's order.  The coefficient averages are formed from the constant as the
+// reference forms them from its (constant) coefficient arrays.
+template <typename ARGS, int K>
+__global__ void __launch_bounds__(256) k_diff2(ARGS args) {
+  Tile3 T;
+  const KArgs &a = kargs(args, T);
+  const Dm &d = a.d;
+  const DState &s = dstate(a);
+  const int i = d.ids + (int)(T.bx * blockDim.x + threadIdx.x);
+  const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
+  const int k = d.kds + (int)T.bz;
+  const wrfmi_config &c = a.c;
+  const int ids = d.ids, ide = d.ide, jds = d.jds, jde = d.jde, kts = d.kds, ktf = d.kde - 1;
+  const bool spec = c.specified || c.nested;
+  // horizontal_diffusion's bounds (module_big_step_utilities_em.F:2772-2914)
+  int hi0 = ids, hi1 = ide - 1, hj0 = jds, hj1 = jde - 1, hk0 = kts;
+  if (K == 0) {
+    hi1 = ide;
+    if (c.open_xs || spec) hi0 = ids + 1;
+    if (c.open_xe || spec) hi1 = ide - 1;
+    if (c.open_ys || spec) hj0 = jds + 1;
+    if (c.open_ye || spec) hj1 = jde - 2;
+    if (c.periodic_x) { hi0 = ids; hi1 = ide; }
+  } else if (K == 1) {
+    hj1 = jde;
+    if (c.open_xs || spec) hi0 = ids + 1;
+    if (c.open_xe || spec) hi1 = ide - 2;
+    if (c.open_ys || spec) hj0 = jds + 1;
+    if (c.open_ye || spec) hj1 = jde - 1;
+    if (c.periodic_x) { hi0 = ids; hi1 = ide - 1; }
+    if (c.polar) { hj0 = jds + 1; hj1 = jde - 1; }
+  } else {
+    if (c.open_xs || spec) hi0 = ids + 1;
+    if (c.open_xe || spec) hi1 = ide - 2;
+    if (c.open_ys || spec) hj0 = jds + 1;
+    if (c.open_ye || spec) hj1 = jde - 2;
+    if (c.periodic_x) { hi0 = ids; hi1 = ide - 1; }
+    if (K == 2) hk0 = kts + 1;
+  }
+  // the vertical routines' bounds (3064-3581)
+  int vi0 = ids, vi1 = ide - 1, vj0 = jds, vj1 = jde - 1, vk0 = kts, vk1 = ktf;
+  if (K == 0) {
+    vi1 = ide;
+    if (c.open_xs || spec) vi0 = ids + 1;
+    if (c.open_xe || spec) vi1 = ide - 1;
+    if (c.periodic_x) { vi0 = ids; vi1 = ide; }
+    vk1 = ktf - 1;
+  } else if (K == 1) {
+    if (c.open_ys || spec) vj0 = jds + 1;
+    if (c.open_ye || spec) vj1 = jde - 1;
+    vk1 = ktf - 1;
+  } else if (K == 2) {
+    vk0 = kts + 1;
+  }
+  const bool hor = i >= hi0 && i <= hi1 && j >= hj0 && j <= hj1 && k >= hk0 && k <= ktf;
+  const bool ver = c.bl_pbl_physics == 0 && (K != 2 || c.non_hydrostatic) && i >= vi0 && i <= vi1 && j >= vj0 &&
+                   j <= vj1 && k >= vk0 && k <= vk1;
+  if (!hor && !ver) return;
+  const FPtr<float> f = K == 0 ? s.u_2 : K == 1 ? s.v_2 : K == 2 ? s.w_2 : K == 3 ? s.t_2 : s.q[a.species];
+  FPtr<float> tend = K == 0 ? s.ru_tendf : K == 1 ? s.rv_tendf : K == 2 ? s.rw_tendf : K == 3 ? s.t_tendf
+                                                                                            : s.q_tend[a.species];
+  const FPtr<float> mut = K == 4 ? s.muts : s.mut;
+  const float c1 = K == 2 ? s.c1f[X1(d, k)] : s.c1h[X1(d, k)], c2 = K == 2 ? s.c2f[X1(d, k)] : s.c2h[X1(d, k)];
+  // prandtl = 1./3.0 (share/module_model_constants.F:86); xkmh = khdif for u, v, w and
+  // xkhh = khdif/prandtl for theta and the species (isotropic_km)
+  const float prandtl = 1.0f / 3.0f;
+  const float xk = K <= 2 ? c.khdif : c.khdif / prandtl;
+  const float kv = K <= 2 ? c.kvdif : K == 3 ? 3.0f * c.kvdif : c.kvdif / prandtl;
+  const Off o = X3(d, i, k, j);
+  const int sj = d.sj, ni = d.ni;
+  const bool qv = K == 4 && a.species == 0 && s.n_moist > 0;
+  float t = tend[o];
+  if (hor) {
+    auto F = [&](int di, int dj) { return f[o + di + dj * sj]; };
+    auto CM = [&](int di, int dj) { return c1 * mut[X2(d, i + di, j + dj)] + c2; };
+    auto M2 = [&](const FPtr<float> &m, int di, int dj) { return m[X2(d, i + di, j + dj)]; };
+    const float rdx = s.rdx, rdy = s.rdy;
+    float mkrdxm, mkrdxp, mrdx, mkrdym, mkrdyp, mrdy;
+    if (K == 0) {
+      mkrdxm = (M2(s.msftx, -1, 0) / M2(s.msfty, -1, 0)) * CM(-1, 0) * xk * rdx;
+      mkrdxp = (M2(s.msftx, 0, 0) / M2(s.msfty, 0, 0)) * CM(0, 0) * xk * rdx;
+      mrdx = M2(s.msfux, 0, 0) * M2(s.msfuy, 0, 0) * rdx;
+      mkrdym = ((M2(s.msfuy, 0, 0) + M2(s.msfuy, 0, -1)) / (M2(s.msfux, 0, 0) + M2(s.msfux, 0, -1))) * 0.25f *
+               (CM(0, 0) + CM(0, -1) + CM(-1, -1) + CM(-1, 0)) * 0.25f * (xk + xk + xk + xk) * rdy;
+      mkrdyp = ((M2(s.msfuy, 0, 0) + M2(s.msfuy, 0, 1)) / (M2(s.msfux, 0, 0) + M2(s.msfux, 0, 1))) * 0.25f *
+               (CM(0, 0) + CM(0, 1) + CM(-1, 1) + CM(-1, 0)) * 0.25f * (xk + xk + xk + xk) * rdy;
+      mrdy = M2(s.msfux, 0, 0) * M2(s.msfuy, 0, 0) * rdy;
+    } else if (K == 1) {
+      mkrdxm = ((M2(s.msfvx, 0, 0) + M2(s.msfvx, -1, 0)) / (M2(s.msfvy, 0, 0) + M2(s.msfvy, -1, 0))) * 0.25f *
+               (CM(0, 0) + CM(0, -1) + CM(-1, -1) + CM(-1, 0)) * 0.25f * (xk + xk + xk + xk) * rdx;
+      mkrdxp = ((M2(s.msfvx, 0, 0) + M2(s.msfvx, 1, 0)) / (M2(s.msfvy, 0, 0) + M2(s.msfvy, 1, 0))) * 0.25f *
+               (CM(0, 0) + CM(0, -1) + CM(1, -1) + CM(1, 0)) * 0.25f * (xk + xk + xk + xk) * rdx;
+      mrdx = M2(s.msfvx, 0, 0) * M2(s.msfvy, 0, 0) * rdx;
+      // the reference's y part of v carries no column mass (2822-2823)
+      mkrdym = (M2(s.msfty, 0, -1) / M2(s.msftx, 0, -1)) * xk * rdy;
+      mkrdyp = (M2(s.msfty, 0, 0) / M2(s.msftx, 0, 0)) * xk * rdy;
+      mrdy = M2(s.msfvx, 0, 0) * M2(s.msfvy, 0, 0) * rdy;
+    } else if (K == 2) {
+      mkrdxm = (M2(s.msfux, 0, 0) / M2(s.msfuy, 0, 0)) * 0.25f * (CM(0, 0) + CM(-1, 0) + CM(0, 0) + CM(-1, 0)) *
+               0.25f * (xk + xk + xk + xk) * rdx;
+      mkrdxp = (M2(s.msfux, 1, 0) / M2(s.msfuy, 1, 0)) * 0.25f * (CM(1, 0) + CM(0, 0) + CM(1, 0) + CM(0, 0)) * 0.25f *
+               (xk + xk + xk + xk) * rdx;
+      mrdx = M2(s.msftx, 0, 0) * M2(s.msfty, 0, 0) * rdx;
+      mkrdym = (M2(s.msfvy, 0, 0) * M2(s.msfvx_inv, 0, 0)) * 0.25f * (CM(0, 0) + CM(0, -1) + CM(0, 0) + CM(0, -1)) *
+               0.25f * (xk + xk + xk + xk) * rdy;
+      mkrdyp = (M2(s.msfvy, 0, 1) * M2(s.msfvx_inv, 0, 1)) * 0.25f * (CM(0, 1) + CM(0, 0) + CM(0, 1) + CM(0, 0)) *
+               0.25f * (xk + xk + xk + xk) * rdy;
+      mrdy = M2(s.msftx, 0, 0) * M2(s.msfty, 0, 0) * rdy;
+    } else {
+      mkrdxm = (M2(s.msfux, 0, 0) / M2(s.msfuy, 0, 0)) * 0.5f * (xk + xk) * 0.5f * (CM(0, 0) + CM(-1, 0)) * rdx;
+      mkrdxp = (M2(s.msfux, 1, 0) / M2(s.msfuy, 1, 0)) * 0.5f * (xk + xk) * 0.5f * (CM(1, 0) + CM(0, 0)) * rdx;
+      mrdx = M2(s.msftx, 0, 0) * M2(s.msfty, 0, 0) * rdx;
+      mkrdym = (M2(s.msfvy, 0, 0) * M2(s.msfvx_inv, 0, 0)) * 0.5f * (xk + xk) * 0.5f * (CM(0, 0) + CM(0, -1)) * rdy;
+      mkrdyp = (M2(s.msfvy, 0, 1) * M2(s.msfvx_inv, 0, 1)) * 0.5f * (xk + xk) * 0.5f * (CM(0, 1) + CM(0, 0)) * rdy;
+      mrdy = M2(s.msftx, 0, 0) * M2(s.msfty, 0, 0) * rdy;
+    }
+    if (K == 3) {
+      // field - t_init (horizontal_diffusion_3dmp)
+      auto B = [&](int di, int dj) { return s.t_init[o + di + dj * sj]; };
+      t = t + (mrdx * (mkrdxp * (F(1, 0) - F(0, 0) - B(1, 0) + B(0, 0)) - mkrdxm * (F(0, 0) - F(-1, 0) - B(0, 0) + B(-1, 0))) +
+               mrdy * (mkrdyp * (F(0, 1) - F(0, 0) - B(0, 1) + B(0, 0)) - mkrdym * (F(0, 0) - F(0, -1) - B(0, 0) + B(0, -1))));
+    } else {
+      t = t + (mrdx * (mkrdxp * (F(1, 0) - F(0, 0)) - mkrdxm * (F(0, 0) - F(-1, 0))) +
+               mrdy * (mkrdyp * (F(0, 1) - F(0, 0)) - mkrdym * (F(0, 0) - F(0, -1))));
+    }
+  }
+  if (ver) {
+    const float g = K_G;
+    auto AL = [&](int dk, int di, int dj) { return s.alt[o + dk * ni + di + dj * sj]; };
+    if (K == 0 || K == 1) {
+      const int di = K == 0 ? -1 : 0, dj = K == 1 ? -1 : 0;
+      const CPtr<float> &base = K == 0 ? s.u_base : s.v_base;
+      // vflux(kk) between levels kk and kk+1 (kk = kts-1 takes vflux(kts))
+      auto vflux = [&](int kk) {
+        if (kk < kts) kk = kts;
+        const int dk = kk - k;
+        return kv * s.rdn[X1(d, kk + 1)] / (0.25f * (AL(dk, 0, 0) + AL(dk, di, dj) + AL(dk + 1, 0, 0) + AL(dk + 1, di, dj))) *
+               (f[o + (dk + 1) * ni] - f[o + dk * ni] - base[X1(d, kk + 1)] + base[X1(d, kk)]);
+      };
+      const float mu = K == 0 ? s.muu[X2(d, i, j)] : s.muv[X2(d, i, j)];
+      t = t + g * g * s.rdnw[X1(d, k)] / (s.c1h[X1(d, k)] * mu + s.c2h[X1(d, k)]) / (0.5f * (AL(0, di, dj) + AL(0, 0, 0))) *
+                  (vflux(k) - vflux(k - 1));
+    } else if (K == 2) {
+      auto vflux = [&](int kk) {
+        if (kk >= ktf) return 0.0f;
+        const int dk = kk - k;
+        return (kv / AL(dk, 0, 0)) * s.rdnw[X1(d, kk)] * (f[o + (dk + 1) * ni] - f[o + dk * ni]);
+      };
+      t = t + s.rdn[X1(d, k)] * g * g / (c1 * mut[X2(d, i, j)] + c2) / (0.5f * (AL(0, 0, 0) + AL(-1, 0, 0))) *
+                  (vflux(k) - vflux(k - 1));
+    } else {
+      // 'm', _mp (QV - qv_base) and _3dmp (theta - t_init); zero gradient below the lowest
+      // level, no flux through the top
+      auto vflux = [&](int kk) {
+        if (kk < kts) kk = kts;
+        if (kk >= ktf) return 0.0f;
+        const int dk = kk - k;
+        float df = f[o + (dk + 1) * ni] - f[o + dk * ni];
+        if (qv) df = df - s.qv_base[X1(d, kk + 1)] + s.qv_base[X1(d, kk)];
+        if (K == 3) df = df - s.t_init[o + (dk + 1) * ni] + s.t_init[o + dk * ni];
+        return kv * s.rdn[X1(d, kk + 1)] / (0.5f * (AL(dk, 0, 0) + AL(dk + 1, 0, 0))) * df;
+      };
+      t = t + g * g / (c1 * mut[X2(d, i, j)] + c2) / AL(0, 0, 0) * s.rdnw[X1(d, k)] * (vflux(k) - vflux(k - 1));
+    }
+  }
+  tend[o] = t;
+}
+
 // the damping of one field in the layer of one column: levels ktf downward while the
 // level's height z00(k) is within zdamp of the top (the contiguous layer the reference's
 // DO WHILE finds), the reference value interpolated from the base profile `base`
@@ -238,5 +415,10 @@ WRFMI_INST_P(k_diff6, 2)
 WRFMI_INST_P(k_diff6, 3)
 WRFMI_INST_P(k_diff6, 4)
 WRFMI_INST(k_rayleigh)
+WRFMI_INST_P(k_diff2, 0)
+WRFMI_INST_P(k_diff2, 1)
+WRFMI_INST_P(k_diff2, 2)
+WRFMI_INST_P(k_diff2, 3)
+WRFMI_INST_P(k_diff2, 4)
 
 }  // namespace wrfmi

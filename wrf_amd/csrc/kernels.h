@@ -75,6 +75,7 @@ template <typename A> __global__ void k_pd_tend_x4(A);
 template <typename A> __global__ void k_update_scalar_pd(A);
 template <typename A, int K> __global__ void k_diff6(A);
 template <typename A> __global__ void k_rayleigh(A);
+template <typename A, int K> __global__ void k_diff2(A);
 template <typename A> __global__ void k_p_rho_phi(A);
 template <typename A> __global__ void k_p_rho_phi_x4(A);
 template <typename A> __global__ void k_halo(A);

@@ -259,10 +259,12 @@ static int check_cfg(const wrfmi_config *c) {
     return fail(WRFMI_E_BADOPT, "advection options: the original fluxes and positive-definite scalar transport "
                 "(moist/scalar_adv_opt 0 or 1, momentum_adv_opt 1) are implemented, not monotonic or WENO");
   if (!c->non_hydrostatic) return fail(WRFMI_E_BADOPT, "hydrostatic mode is not implemented");
-  if (c->diff_opt != 0 || c->diff_6th_opt < 0 || c->diff_6th_opt > 2 || c->damp_opt == 1 || c->damp_opt < 0 ||
-      c->damp_opt > 3 || c->zadvect_implicit || c->pert_coriolis || c->map_proj == 6)
-    return fail(WRFMI_E_BADOPT, "diff_opt 1/2 (2nd-order and TKE diffusion), damp_opt 1, IEVA, perturbation Coriolis and "
-                "lat-lon grids are outside the implemented set (diff_6th_opt 0-2 and damp_opt 0, 2, 3 are implemented)");
+  if (c->diff_opt < 0 || c->diff_opt > 1 || (c->diff_opt == 1 && c->km_opt != 1) || c->diff_6th_opt < 0 ||
+      c->diff_6th_opt > 2 || c->damp_opt == 1 || c->damp_opt < 0 || c->damp_opt > 3 || c->zadvect_implicit ||
+      c->pert_coriolis || c->map_proj == 6)
+    return fail(WRFMI_E_BADOPT, "diff_opt 2 (TKE diffusion), km_opt other than 1, damp_opt 1, IEVA, perturbation "
+                "Coriolis and lat-lon grids are outside the implemented set (diff_opt 0 and 1 with km_opt 1, "
+                "diff_6th_opt 0-2 and damp_opt 0, 2, 3 are implemented)");
   if (c->hypsometric_opt != 1 && c->hypsometric_opt != 2) return fail(WRFMI_E_BADOPT, "hypsometric_opt must be 1 or 2");
   if (c->precision_mode != 0 && c->precision_mode != 1) return fail(WRFMI_E_BADOPT, "precision_mode must be 0 or 1");
   return 0;
@@ -346,13 +348,13 @@ enum KernelId {
   KID_STEP_PREP, KID_TEND_U, KID_TEND_V, KID_TEND_W, KID_TEND_T, KID_SMALL_PREP, KID_ADVANCE_UV,
   KID_AC_DIV, KID_SMALL_FINISH, KID_ADV_SCALAR, KID_UPDATE_SCALAR, KID_P_RHO_PHI, KID_HALO, KID_P_RHO0,
   KID_COEF_W, KID_WW_SCAN, KID_AC_MU_WW, KID_AC_THETA, KID_AC_WSOLVE, KID_W_SURFACE, KID_PD_SCALE, KID_PD_TEND,
-  KID_UPDATE_SCALAR_PD, KID_DIFF6, KID_RAYLEIGH, KID_COUNT
+  KID_UPDATE_SCALAR_PD, KID_DIFF6, KID_RAYLEIGH, KID_DIFF2, KID_COUNT
 };
 static const char *kKernelNames[KID_COUNT] = {
   "k_step_prep", "k_tend_u", "k_tend_v", "k_tend_w", "k_tend_t", "k_small_prep", "k_advance_uv",
   "k_ac_div", "k_small_finish", "k_adv_scalar", "k_update_scalar", "k_p_rho_phi", "k_halo", "k_p_rho0",
   "k_coef_w", "k_ww_scan", "k_ac_mu_ww", "k_ac_theta", "k_ac_wsolve", "k_set_w_surface", "k_pd_scale", "k_pd_tend",
-  "k_update_scalar_pd", "k_diff6", "k_rayleigh"};
+  "k_update_scalar_pd", "k_diff6", "k_rayleigh", "k_diff2"};
 struct EvPair { int id; hipEvent_t a, b; };
 static unsigned g_timing = 0;  // bit n set: time launches of kernel n
 static std::vector<EvPair> g_ev_pending;
@@ -486,7 +488,7 @@ static Shape default_shape(int kid) {
 // which of a kernel's shape parameters are tunable, and the values each may take
 static bool tunable_rows(int kid) {
   return kid != KID_HALO && kid != KID_W_SURFACE && kid != KID_PD_SCALE && kid != KID_PD_TEND && kid != KID_UPDATE_SCALAR_PD &&
-         kid != KID_DIFF6 && kid != KID_RAYLEIGH;
+         kid != KID_DIFF6 && kid != KID_RAYLEIGH && kid != KID_DIFF2;
 }
 static bool tunable_levels(int kid) { return kid == KID_ADVANCE_UV || kid == KID_AC_DIV; }
 // the four-point stencil kernels built for workgroups of up to kTallBlk threads
@@ -623,12 +625,19 @@ static inline dim3 g_zone(const KArgs &a, int nkz) {
   const unsigned n = zone_count(a.d, a.c.periodic_x, a.c.periodic_y);
   return dim3((n + 255) / 256, 1, nkz);
 }
-// rk_tendency's forward-step terms at rk_step 1 (module_em.F:800-950), into the *_tendf
-// arrays before rk_addtend_dry (fused into k_tend_*) reads them: 6th-order diffusion of u,
-// v, w, theta, then the Rayleigh damper
+// rk_tendency's forward-step terms at rk_step 1 (module_em.F:797-950), into the *_tendf
+// arrays before rk_addtend_dry (fused into k_tend_*) reads them: diff_opt 1's second-order
+// diffusion of u, v, w, theta, then 6th-order diffusion, then the Rayleigh damper
 static void ph_mixing(Launcher &L) {
   const KArgs &a = L.a;
   const Dm &d = a.d;
+  if (a.c.diff_opt == 1) {
+    const dim3 g = g_pt3(d, d.kde - d.kds);
+    L.launchK(KID_DIFF2, KRP(k_diff2, 0), g, kColBlk, a);
+    L.launchK(KID_DIFF2, KRP(k_diff2, 1), g, kColBlk, a);
+    L.launchK(KID_DIFF2, KRP(k_diff2, 2), g, kColBlk, a);
+    L.launchK(KID_DIFF2, KRP(k_diff2, 3), g, kColBlk, a);
+  }
   if (a.c.diff_6th_opt != 0) {
     const dim3 g = g_pt3(d, d.kde - d.kds);
     L.launchK(KID_DIFF6, KRP(k_diff6, 0), g, kColBlk, a);
@@ -638,9 +647,11 @@ static void ph_mixing(Launcher &L) {
   }
   if (a.c.damp_opt == 2) L.launchK(KID_RAYLEIGH, KR(k_rayleigh), g_pt3(d, 1), kColBlk, a);
 }
-// rk_scalar_tend's 6th-order diffusion of species a.species into its tendency at rk_step 1
-// (module_em.F:1421-1436)
+// rk_scalar_tend's diff_opt 1 and 6th-order diffusion of species a.species into its
+// tendency at rk_step 1 (module_em.F:1380-1436)
 static void ph_scalar_mixing(Launcher &L) {
+  if (L.a.rk_step == 1 && L.a.c.diff_opt == 1)
+    L.launchK(KID_DIFF2, KRP(k_diff2, 4), g_pt3(L.a.d, L.a.d.kde - L.a.d.kds), kColBlk, L.a);
   if (L.a.rk_step == 1 && L.a.c.diff_6th_opt != 0)
     L.launchK(KID_DIFF6, KRP(k_diff6, 4), g_pt3(L.a.d, L.a.d.kde - L.a.d.kds), kColBlk, L.a);
 }

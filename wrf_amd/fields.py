@@ -52,8 +52,9 @@ _CFG_INT = ("periodic_x", "periodic_y", "symmetric_xs", "symmetric_xe", "symmetr
             "moist_adv_opt", "scalar_adv_opt", "momentum_adv_opt",
             "non_hydrostatic", "top_lid", "phi_adv_z", "hypsometric_opt", "use_theta_m",
             "damp_opt", "w_damping", "diff_opt", "diff_6th_opt", "zadvect_implicit", "pert_coriolis",
-            "map_proj", "time_step_sound", "time_step", "diff_6th_slopeopt")
-_CFG_REAL = ("epssm", "smdiv", "emdiv", "dampcoef", "zdamp", "w_crit_cfl", "diff_6th_factor", "diff_6th_thresh")
+            "map_proj", "time_step_sound", "time_step", "diff_6th_slopeopt", "km_opt", "bl_pbl_physics")
+_CFG_REAL = ("epssm", "smdiv", "emdiv", "dampcoef", "zdamp", "w_crit_cfl", "diff_6th_factor", "diff_6th_thresh",
+             "khdif", "kvdif")
 
 
 class Config(ctypes.Structure):

@@ -143,6 +143,7 @@ def make_config(spec: DomainSpec) -> Config:
     c.epssm, c.smdiv, c.emdiv = 0.1, 0.1, 0.01
     c.dampcoef, c.zdamp, c.w_crit_cfl = 0.2, 5000.0, 1.2
     c.diff_6th_factor, c.diff_6th_thresh, c.diff_6th_slopeopt = 0.12, 0.10, 0
+    c.km_opt, c.bl_pbl_physics, c.khdif, c.kvdif = 1, 0, 0.0, 0.0
     c.precision_mode = spec.precision_mode
     return c
 
@@ -438,8 +439,8 @@ def _vertical_coords(hs: HostState, spec: DomainSpec, p_top: float):
 
 def _base_profiles(hs: HostState):
     """The base-state profiles of the upper-level Rayleigh damper from the column (ids, jds)
-    of the initial state, in REAL arithmetic (module_initialize_ideal.F:1579-1585): u, v and
-    the theta perturbation of that column, and the height of its mass levels."""
+    of the initial state, in REAL arithmetic (module_initialize_ideal.F:1579-1585): u, v,
+    qv and the theta perturbation of that column, and the height of its mass levels."""
     d = hs.dims
     i, j = d.ids - d.ims, d.jds - d.jms
     nz = d.kde - d.kds
@@ -447,6 +448,7 @@ def _base_profiles(hs: HostState):
     A1, A3 = hs.a1, hs.a3
     for n, src in (("t_base", "t_1"), ("u_base", "u_1"), ("v_base", "v_1")):
         A1[n][:nz] = A3[src][j, :nz, i]
+    A1["qv_base"][:nz] = hs.q[0][j, :nz, i] if hs.n_species > 0 else 0.0
     phb, ph1 = A3["phb"][j, :, i], A3["ph_1"][j, :, i]
     for k in range(nz):
         A1["z_base"][k] = f32(0.5) * (((phb[k] + phb[k + 1]) + ph1[k]) + ph1[k + 1]) / g
@@ -589,6 +591,7 @@ def build_initial_state(spec: DomainSpec) -> HostState:
     A3["phb"][:, : nz + 1, :] = phb.astype(np.float32)[None, :, None]
     A3["pb"][:, :nz, :] = pb.astype(np.float32)[None, :, None]
     A3["alb"][:, :nz, :] = alb.astype(np.float32)[None, :, None]
+    A3["t_init"][:, :nz, :] = t_init.astype(np.float32)[None, :, None]
     A3["u_2"][:, :nz, :] = uz.astype(np.float32)[None, :, None]
     A3["v_2"][:, :nz, :] = vz.astype(np.float32)[None, :, None]
     A3["u_1"][...] = A3["u_2"]
@@ -661,7 +664,8 @@ def build_terrain_state(spec: DomainSpec) -> HostState:
         p_surf = np.interp(ht, zs, pdry)
         mub = p_surf - p_top
         pb = [c3h[k] * mub + c4h[k] + p_top for k in range(nz)]
-        alb = [(R_D / P1000) * _interp_p(thdry, pdry, pb[k]) * (pb[k] / P1000) ** CVPM for k in range(nz)]
+        t_init = [_interp_p(thdry, pdry, pb[k]) - T0 for k in range(nz)]
+        alb = [(R_D / P1000) * (t_init[k] + T0) * (pb[k] / P1000) ** CVPM for k in range(nz)]
         phb = [G * ht]
         for k in range(nz):
             phb.append(phb[k] - dnw[k] * (c1h[k] * mub + c2h[k]) * alb[k])
@@ -707,6 +711,7 @@ def build_terrain_state(spec: DomainSpec) -> HostState:
             A3["t_2"][js, k, is_] = ((T[k] + T0) * (1.0 + RVOVRD * QV[k]) - T0).astype(f32)
             A3["pb"][js, k, is_] = pb[k].astype(f32)
             A3["alb"][js, k, is_] = alb[k].astype(f32)
+            A3["t_init"][js, k, is_] = t_init[k].astype(f32)
             A3["phb"][js, k, is_] = phb[k].astype(f32)
             hs.q[0][js, k, is_] = QV[k].astype(f32)
             ph_lo = ph_hi
@@ -750,7 +755,7 @@ def build_terrain_state(spec: DomainSpec) -> HostState:
     workers = max(1, min(16, os.cpu_count() or 1))
     with concurrent.futures.ThreadPoolExecutor(workers) as pool:
         list(pool.map(build_rows, range(len(blocks))))
-        names = ["u_1", "u_2", "v_1", "v_2", "t_1", "t_2", "ph_1", "ph_2", "phb", "pb", "alb", "p", "al", "alt",
+        names = ["u_1", "u_2", "v_1", "v_2", "t_1", "t_2", "ph_1", "ph_2", "phb", "pb", "alb", "t_init", "p", "al", "alt",
                  "mub", "mu_1", "mu_2", "ht", "msfux", "msfuy", "msfvx", "msfvx_inv", "msfvy", "msftx", "msfty",
                  "cosa"] + [f"q{m}" for m in range(hs.n_species)]
         # the time-level copies read halo-free sources: run them before their sources' fills
