@@ -35,7 +35,8 @@ enum {
   WRFMI_E_DIMS = 2,      /* index sextets inconsistent with each other or the halo */
   WRFMI_E_DEVICE = 3,    /* HIP runtime error */
   WRFMI_E_POINTER = 4,   /* null, or neither device nor registered host pointer */
-  WRFMI_E_NONFINITE = 5  /* non-finite value found by wrfmi_check_finite */
+  WRFMI_E_NONFINITE = 5, /* non-finite value found by wrfmi_check_finite */
+  WRFMI_E_IO = 6         /* a history file could not be created or written */
 };
 
 /* Index sextets of the reference (dyn_em/solve_em.F:101-114):

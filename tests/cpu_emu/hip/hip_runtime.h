@@ -65,6 +65,22 @@ inline hipError_t hipFree(void *p) {
   std::free(p);
   return hipSuccess;
 }
+// pinned host memory is ordinary host memory here
+enum { hipHostMallocDefault = 0 };
+inline hipError_t hipHostMalloc(void **p, size_t n, unsigned) {
+  *p = std::malloc(n);
+  return *p ? hipSuccess : 2;
+}
+template <typename T> inline hipError_t hipHostMalloc(T **p, size_t n, unsigned f) { return hipHostMalloc((void **)p, n, f); }
+inline hipError_t hipHostFree(void *p) {
+  std::free(p);
+  return hipSuccess;
+}
+inline unsigned __float_as_uint(float x) {
+  unsigned u;
+  std::memcpy(&u, &x, 4);
+  return u;
+}
 typedef void *hipDeviceptr_t;
 hipError_t hipMemGetAddressRange(hipDeviceptr_t *base, size_t *size, hipDeviceptr_t p);
 inline hipError_t hipMemcpy(void *d, const void *s, size_t n, hipMemcpyKind) { std::memcpy(d, s, n); return hipSuccess; }
