@@ -317,6 +317,33 @@ def totals_check():
     sys.exit(1 if fails else 0)
 
 
+def history_check_emu(case: str, order: int):
+    """wrfmi_history_* on the emulated kernels: two records (the initial state and the state
+    after one step) read back with scipy's netCDF reader and compared with the arrays."""
+    import tempfile
+    from history_check import history_failures
+    from wrf_amd.runtime import History
+    hs = build_case(case)
+    emu = EmuDomain(hs, order=order)
+    start = "2026-06-01_12:00:00"
+    recs = []
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "wrfout_d01")
+        with History(emu.lib, emu.state, emu.cfg, emu.dims, path, start) as h:
+            for r in range(2):
+                date = f"2026-06-01_12:00:{6 * r:02d}"
+                h.write(date, 0.1 * r)
+                recs.append((date, 0.1 * r, emu.download()))
+                if r == 0:
+                    emu.step(1)
+            assert h.records == 2
+        fails = history_failures(path, recs, hs.dims, hs.cfg, hs.scal, hs.n_moist, hs.n_species, start)
+    for f in fails:
+        print("MISMATCH", f)
+    print("OK history" if not fails else "FAILED history", case, flush=True)
+    sys.exit(1 if fails else 0)
+
+
 def shapes_check(case: str):
     """Tile-shape invariance: with every kernel's rows per workgroup at 2 and at 4 (and the
     stencil kernels' at 8), and the levels per thread and prefetch depth at each
@@ -372,6 +399,8 @@ def main():
     if sys.argv[1] == "shapes":
         shapes_check(sys.argv[2])
         return
+    if sys.argv[1] == "history":
+        history_check_emu(sys.argv[2], int(sys.argv[3]))
     if sys.argv[1] == "totals":
         totals_check()
         return

@@ -14,7 +14,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def declared_entry_points():
     names = set()
-    for h in ("wrfmi.h", "wrfmi_routines.h"):
+    for h in ("wrfmi.h", "wrfmi_routines.h", "wrfmi_history.h"):
         text = open(os.path.join(ROOT, "include", h)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
         names |= set(re.findall(r"\b(wrfmi_\w+)\s*\(", text))

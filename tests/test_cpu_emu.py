@@ -113,3 +113,13 @@ def test_emulated_positive_definite_routines(emu_env, order):
                        env=emu_env, capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stdout[-4000:] + r.stderr[-4000:]
     assert "OK pd_routines" in r.stdout
+
+
+@pytest.mark.parametrize("case,order", [("periodic", 0), ("terrain_yopen", 1)])
+def test_emulated_history_file(emu_env, case, order):
+    """wrfmi_history_*: two records of the reference's wrfout layout, read back with scipy's
+    netCDF reader, equal to the state bit for bit."""
+    r = subprocess.run([sys.executable, os.path.join(HERE, "emu_phase_check.py"), "history", case, str(order)],
+                       env=emu_env, capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stdout[-4000:] + r.stderr[-4000:]
+    assert "OK history" in r.stdout

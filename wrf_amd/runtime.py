@@ -27,6 +27,7 @@ WRFMI_E_DIMS = 2
 WRFMI_E_DEVICE = 3
 WRFMI_E_POINTER = 4
 WRFMI_E_NONFINITE = 5
+WRFMI_E_IO = 6
 
 
 class WrfmiError(RuntimeError):
@@ -155,9 +156,13 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     return lib
 
 
-def check(rc: int):
+def check(rc: int, lib=None):
+    """Raises WrfmiError with the library's message unless rc is WRFMI_OK (lib: the library
+    that returned rc, the GPU library by default)."""
     if rc != WRFMI_OK:
-        raise WrfmiError(rc, load_library().wrfmi_last_error().decode())
+        lib = lib or load_library()
+        lib.wrfmi_last_error.restype = ctypes.c_char_p
+        raise WrfmiError(rc, lib.wrfmi_last_error().decode())
 
 
 class KernelTimer:
@@ -292,6 +297,46 @@ class DeviceDomain:
     def hbm_bytes(self) -> int:
         return sum(v.numel() * v.element_size() for v in self.t.values())
 
+
+
+class History:
+    """A history file in the reference's wrfout layout (wrfmi_history_open/_write/_close,
+    include/wrfmi_history.h): `lib`, `state`, `cfg`, `dims` are a domain's library handle
+    and ctypes structs (DeviceDomain's, or a registered-host-array domain's)."""
+
+    def __init__(self, lib, state, cfg, dims, path, start_date="2000-01-01_00:00:00", species_names=None):
+        vp = ctypes.c_void_p
+        lib.wrfmi_history_open.argtypes = [ctypes.c_char_p, vp, vp, vp, ctypes.c_char_p, vp, vp]
+        lib.wrfmi_history_write.argtypes = [vp, vp, ctypes.c_char_p, ctypes.c_float, vp]
+        lib.wrfmi_history_close.argtypes = [vp]
+        lib.wrfmi_history_records.argtypes = [vp]
+        self.lib, self.state = lib, state
+        names = None
+        if species_names is not None:
+            self._names = [n.encode() if n else None for n in species_names]
+            names = (ctypes.c_char_p * len(self._names))(*self._names)
+        self.h = ctypes.c_void_p()
+        check(lib.wrfmi_history_open(str(path).encode(), ctypes.byref(state), ctypes.byref(cfg), ctypes.byref(dims),
+                                     start_date.encode(), names, ctypes.byref(self.h)), lib)
+
+    def write(self, date, xtime, stream=None):
+        check(self.lib.wrfmi_history_write(self.h, ctypes.byref(self.state), date.encode(), float(xtime),
+                                           ctypes.c_void_p(stream or 0)), self.lib)
+
+    @property
+    def records(self):
+        return self.lib.wrfmi_history_records(self.h)
+
+    def close(self):
+        if self.h:
+            check(self.lib.wrfmi_history_close(self.h), self.lib)
+            self.h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 class DeviceEnsemble:
