@@ -71,11 +71,36 @@ typedef struct wrfmi_config {
   float epssm, smdiv, emdiv, dampcoef, zdamp, w_crit_cfl;
   float diff_6th_factor, diff_6th_thresh;
   float khdif, kvdif;    /* horizontal and vertical eddy viscosity of km_opt 1 (m2/s) */
+  /* specified / nested lateral boundaries: the boundary zone's width and its split into
+   * the specified rows and the relaxation rows (namelist spec_bdy_width, spec_zone,
+   * relax_zone); have_bcs_moist / have_bcs_scalar: the moist species other than QV / the
+   * passive scalars take boundary arrays too (otherwise flow_dep_bdy) */
+  int spec_bdy_width, spec_zone, relax_zone, have_bcs_moist, have_bcs_scalar;
   /* 0: FP32 everywhere, reference operation order.
    * 1: FP32 bulk, FP64 calc_coef_w + advance_w tridiagonal solve,
    *    compensated (two-sum) column mass-divergence integrals. */
   int precision_mode;
 } wrfmi_config;
+
+/* Lateral boundary arrays of a domain with specified (or nested) boundaries: the
+ * Registry's <field>_b[xs|xe|ys|ye] values and <field>_bt[..] tendencies
+ * (Registry/Registry.EM_COMMON), side index 0..3 = xs, xe, ys, ye.  x sides are Fortran
+ * arrays (jms:jme, kds:kde, spec_bdy_width), y sides (ims:ime, kds:kde, spec_bdy_width); mu
+ * has one level.  The values are the coupled ones solve_em relaxes toward (u, v: the
+ * mu-coupled ru, rv; ph, t: times c1 mu + c2; mu itself).  dtbc is grid%dtbc, the time since
+ * the boundary values' valid time: the whole step adds dt to it first, as solve_em.F:377
+ * does; fcx, gcx are lbc_fcx_gcx's relaxation weights (module_bc_em.F:1297-1341). */
+#define WRFMI_MAX_BDY_WIDTH 16
+#define WRFMI_LBC_FIELDS(X) X(u) X(v) X(ph) X(t) X(w) X(mu)
+typedef struct wrfmi_lbc {
+  float dtbc;
+  float fcx[WRFMI_MAX_BDY_WIDTH], gcx[WRFMI_MAX_BDY_WIDTH];
+#define WRFMI_LBC_DECL(n) float *n##_b[4]; float *n##_bt[4];
+  WRFMI_LBC_FIELDS(WRFMI_LBC_DECL)
+#undef WRFMI_LBC_DECL
+  float *q_b[WRFMI_MAX_SPECIES][4];
+  float *q_bt[WRFMI_MAX_SPECIES][4];
+} wrfmi_lbc;
 
 /* One domain's arrays.  Members are named after the reference's grid fields. */
 typedef struct wrfmi_state {
@@ -96,6 +121,9 @@ typedef struct wrfmi_state {
   float *q[WRFMI_MAX_SPECIES];
   float *q_old[WRFMI_MAX_SPECIES];
   float *q_tend[WRFMI_MAX_SPECIES];
+  /* lateral boundary arrays (host memory struct; its arrays device or registered host
+   * memory), read when cfg->specified or cfg->nested; NULL otherwise */
+  wrfmi_lbc *lbc;
 } wrfmi_state;
 
 /* ---- library ----------------------------------------------------------- */

@@ -82,6 +82,17 @@ typedef WREAL wreal;
 typedef wrfmi_dims dims_t;
 typedef wrfmi_config cfg_t;
 
+/* wrfmi_lbc with WREAL arrays and weights */
+typedef struct olbc {
+  wreal dtbc;
+  wreal fcx[WRFMI_MAX_BDY_WIDTH], gcx[WRFMI_MAX_BDY_WIDTH];
+#define O_LBC(n) wreal *n##_b[4]; wreal *n##_bt[4];
+  WRFMI_LBC_FIELDS(O_LBC)
+#undef O_LBC
+  wreal *q_b[WRFMI_MAX_SPECIES][4];
+  wreal *q_bt[WRFMI_MAX_SPECIES][4];
+} olbc;
+
 /* state with WREAL arrays; member order identical to wrfmi_state */
 typedef struct ostate {
 #define O_PTR(n) wreal *n;
@@ -97,6 +108,7 @@ typedef struct ostate {
   wreal *q[WRFMI_MAX_SPECIES];
   wreal *q_old[WRFMI_MAX_SPECIES];
   wreal *q_tend[WRFMI_MAX_SPECIES];
+  olbc *lbc;
 } ostate;
 
 #define NI_ (d->ime - d->ims + 1)
@@ -204,6 +216,22 @@ void o_vertical_diffusion_u(const wreal *field, wreal *tend, const cfg_t *c, con
 void o_vertical_diffusion_v(const wreal *field, wreal *tend, const cfg_t *c, const wreal *v_base, const wreal *c1h,
                             const wreal *c2h, const wreal *alt, const wreal *muv, const wreal *rdn, const wreal *rdnw,
                             wreal kvdif, const dims_t *d);
+
+/* ---- specified / nested lateral boundaries (lbc.c) ---- */
+void o_relax_bdytend(const wreal *field, wreal *tend, wreal *const *b, wreal *const *bt, char var, const cfg_t *c,
+                     int spec_zone, int relax_zone, wreal dtbc, const wreal *fcx, const wreal *gcx, const dims_t *d);
+void o_spec_bdytend(wreal *tend, wreal *const *bt, char var, const cfg_t *c, int spec_zone, const dims_t *d);
+void o_spec_bdyupdate(wreal *field, const wreal *tend, wreal dt, char var, const cfg_t *c, int spec_zone,
+                      const dims_t *d);
+void o_spec_bdyupdate_ph(const wreal *ph_save, wreal *field, const wreal *tend, const wreal *mu_tend,
+                         const wreal *muts, const wreal *c1, const wreal *c2, wreal dt, const cfg_t *c, int spec_zone,
+                         const dims_t *d);
+void o_zero_grad_bdy(wreal *field, char var, const cfg_t *c, int spec_zone, const dims_t *d);
+void o_flow_dep_bdy(wreal *field, const wreal *u, const wreal *v, const cfg_t *c, int spec_zone, const dims_t *d);
+void o_relax_bdy_dry(ostate *s, const cfg_t *c, const dims_t *d);
+void o_spec_bdy_dry(ostate *s, const cfg_t *c, const dims_t *d);
+void o_relax_spec_bdy_scalar(ostate *s, const cfg_t *c, const dims_t *d, int m);
+int o_species_has_bcs(const ostate *s, const cfg_t *c, int m);
 
 /* ---- rk_tendency and rk_addtend_dry ---- */
 int o_rk_tendency(ostate *s, const cfg_t *c, const dims_t *d, int rk_step);

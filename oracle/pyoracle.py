@@ -13,7 +13,9 @@ import os
 
 import numpy as np
 
-from wrf_amd.fields import (FIELDS_1D, FIELDS_2D, FIELDS_3D, MAX_SPECIES, SCALARS_R, State32, State64)
+from wrf_amd.fields import (FIELDS_1D, FIELDS_2D, FIELDS_3D, MAX_SPECIES, SCALARS_R, State32, State64,
+                             make_lbc_struct)
+from wrf_amd.grid import lbc_struct
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 
@@ -35,6 +37,7 @@ class Oracle:
         self.dtype = np.float32 if single else np.float64
         self.ctype = ctypes.c_float if single else ctypes.c_double
         self.State = State32 if single else State64
+        self.Lbc = make_lbc_struct(self.ctype, "OLbc32" if single else "OLbc64")
         self.lib.oracle_state_sizeof.restype = ctypes.c_size_t
         if self.lib.oracle_state_sizeof() != ctypes.sizeof(self.State):
             raise RuntimeError("oracle state layout does not match wrf_amd.fields")
@@ -58,6 +61,10 @@ class Oracle:
             st.q[m] = arrs[f"q{m}"].ctypes.data_as(P)
             st.q_old[m] = arrs[f"q_old{m}"].ctypes.data_as(P)
             st.q_tend[m] = arrs[f"q_tend{m}"].ctypes.data_as(P)
+        if hs.lbc is not None:
+            # the olbc struct lives as long as the state struct (kept on it)
+            st._lbc = lbc_struct(hs.lbc, self.Lbc, lambda n: arrs[n].ctypes.data, self.ctype)
+            st.lbc = ctypes.addressof(st._lbc)
         return arrs, st
 
     def session(self, hs):

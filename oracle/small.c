@@ -260,9 +260,17 @@ void o_advance_uv(ostate *s, const cfg_t *c, const dims_t *d, wreal dts) {
   const int ids = d->ids, ide = d->ide, jds = d->jds, jde = d->jde, kde = d->kde;
   const int its = d->its, ite = d->ite, jts = d->jts, jte = d->jte, kts = d->kts, kte = d->kte;
   int i_start, i_end, j_start, j_end, k_start, k_end, i_endu, j_endv;
-  if (c->nested || c->specified) return; /* specified/nested bounds are out of scope */
-  i_start = its; i_end = IMIN(ite, ide - 1); j_start = jts; j_end = IMIN(jte, jde - 1);
-  k_start = kts; k_end = kte - 1; i_endu = ite; j_endv = jte;
+  if (c->nested || c->specified) {
+    /* module_small_step_em.F:734-747: the spec zone is left to spec_bdyupdate */
+    const int sz = c->spec_zone;
+    i_start = IMAX(its, ids + sz); i_end = IMIN(ite, ide - sz - 1); j_start = IMAX(jts, jds + sz);
+    j_end = IMIN(jte, jde - sz - 1); k_start = kts; k_end = IMIN(kte, kde - 1); i_endu = IMIN(ite, ide - sz);
+    j_endv = IMIN(jte, jde - sz);
+    if (c->periodic_x) { i_start = its; i_end = IMIN(ite, ide - 1); i_endu = ite; }
+  } else {
+    i_start = its; i_end = IMIN(ite, ide - 1); j_start = jts; j_end = IMIN(jte, jde - 1);
+    k_start = kts; k_end = kte - 1; i_endu = ite; j_endv = jte;
+  }
   int i_start_up = i_start, i_end_up = i_endu, j_start_vp = j_start, j_end_vp = j_endv;
   if ((c->open_xs || c->symmetric_xs) && its == ids) i_start_up = i_start_up + 1;
   if ((c->open_xe || c->symmetric_xe) && ite == ide) i_end_up = i_end_up - 1;

@@ -98,9 +98,30 @@ static void pd_prep(ostate *s, const cfg_t *c, const dims_t *d, wreal dt_rk) {
   }
 }
 
+/* the spec-zone updates of the acoustic substep (solve_em.F:1343-1365, 1459-1490,
+ * 1583-1620) */
+static void spec_uv(ostate *s, const cfg_t *c, const dims_t *d, wreal dts_rk) {
+  o_spec_bdyupdate(s->u_2, s->ru_tend, dts_rk, 'u', c, c->spec_zone, d);
+  o_spec_bdyupdate(s->v_2, s->rv_tend, dts_rk, 'v', c, c->spec_zone, d);
+}
+static void spec_mu_t(ostate *s, const cfg_t *c, const dims_t *d, wreal dts_rk) {
+  o_spec_bdyupdate(s->t_2, s->t_tend, dts_rk, 't', c, c->spec_zone, d);
+  o_spec_bdyupdate(s->mu_2, s->mu_tend, dts_rk, 'm', c, c->spec_zone, d);
+  o_spec_bdyupdate(s->muts, s->mu_tend, dts_rk, 'm', c, c->spec_zone, d);
+}
+static void spec_w_ph(ostate *s, const cfg_t *c, const dims_t *d, wreal dts_rk) {
+  if (!c->non_hydrostatic) return;
+  o_spec_bdyupdate_ph(s->ph_save, s->ph_2, s->ph_tend, s->mu_tend, s->muts, s->c1f, s->c2f, dts_rk, c, c->spec_zone, d);
+  if (c->specified) o_zero_grad_bdy(s->w_2, 'w', c, c->spec_zone, d);
+  else o_spec_bdyupdate(s->w_2, s->rw_tend, dts_rk, 'h', c, c->spec_zone, d);
+}
+
 int o_solve_em_step(ostate *s, const cfg_t *c, const dims_t *d) {
   int rc;
-  if (c->rk_ord != 3 || c->specified || c->nested || c->polar) return WRFMI_E_BADOPT;
+  const int lbc = c->specified || c->nested;
+  if (c->rk_ord != 3 || c->polar || (lbc && !s->lbc)) return WRFMI_E_BADOPT;
+  /* solve_em.F:377 */
+  if (lbc) s->lbc->dtbc = s->lbc->dtbc + s->dt;
   const int rk_order = 3;
   int nsound = o_num_sound_steps(c, s);
   wreal dts = s->dt / (wreal)nsound;
@@ -117,7 +138,11 @@ int o_solve_em_step(ostate *s, const cfg_t *c, const dims_t *d) {
     o_set_physical_bc3d(s->ph_2, 'w', c, d);
 
     TILED(o_rk_tendency(s, c, d, rk_step));
+    /* solve_em.F:937-1008: relaxation at rk_step 1 into the *_save arrays and mu_tend,
+     * then the spec zone's tendencies */
+    if (lbc && rk_step == 1) o_relax_bdy_dry(s, c, d);
     TILEDV(o_rk_addtend_dry(s, d, rk_step));
+    if (lbc) o_spec_bdy_dry(s, c, d);
 
     TILEDV(o_small_step_prep(s, d, rk_step));
     TILEDV(o_calc_p_rho(s, c, d, 0));
@@ -136,9 +161,12 @@ int o_solve_em_step(ostate *s, const cfg_t *c, const dims_t *d) {
 
     for (int it = 1; it <= nsmall; it++) {
       TILEDV(o_advance_uv(s, c, d, dts_rk));
+      if (lbc) spec_uv(s, c, d, dts_rk);
       TILEDV(o_advance_mu_t(s, c, d, dts_rk));
+      if (lbc) spec_mu_t(s, c, d, dts_rk);
       if (c->non_hydrostatic) TILEDV(o_advance_w(s, c, d, dts_rk));
       TILEDV(o_sumflux(s, d, it, nsmall));
+      if (lbc) spec_w_ph(s, c, d, dts_rk);
       TILEDV(o_calc_p_rho(s, c, d, it));
       o_set_physical_bc3d(s->ph_2, 'w', c, d);
       o_set_physical_bc3d(s->al, 'p', c, d);
@@ -161,7 +189,11 @@ int o_solve_em_step(ostate *s, const cfg_t *c, const dims_t *d) {
     if (rk_step == rk_order) pd_prep(s, c, d, dt_rk);
     for (int m = 0; m < s->n_species; m++) {
       TILED(o_rk_scalar_tend(s, c, d, m, rk_step, dt_rk));
+      /* solve_em.F:2261-2296, 2808-2845 */
+      if (lbc && rk_step == 1 && o_species_has_bcs(s, c, m)) o_relax_spec_bdy_scalar(s, c, d, m);
       TILEDV(o_rk_update_scalar(s, d, m, rk_step, dt_rk));
+      /* solve_em.F:2342-2355, 2886-2930 */
+      if (c->specified && !o_species_has_bcs(s, c, m)) o_flow_dep_bdy(s->q[m], s->ru_m, s->rv_m, c, c->spec_zone, d);
     }
 
     TILEDV(o_calc_p_rho_phi(s, c, d));

@@ -1239,9 +1239,251 @@ def pin_advance_w_damp_opt3(o):
     _close(p3[k] - p0[k], 0.5 * dts * G * (1.0 + eps) * (w3[k] - w0[k]) / mu, 1e-8, "phi of the damped w")
 
 
+
+
+
+# ----------------------------------------------------------------------------- lateral boundaries
+def _lbc_sides(D, nk, width, fn):
+    """Four boundary arrays (x sides (width, nk, nj), y sides (width, nk, ni)) in oracle
+    precision, entry [s][b, k, ij] = fn(side, b + 1, k, ij)."""
+    d = D.d
+    nj, ni = d.jme - d.jms + 1, d.ime - d.ims + 1
+    out = []
+    for s_ in range(4):
+        n = nj if s_ < 2 else ni
+        a = np.zeros((width, nk, n))
+        for b in range(width):
+            for k in range(nk):
+                a[b, k, :] = fn(s_, b + 1, k, np.arange(n) + (d.jms if s_ < 2 else d.ims))
+        out.append(np.ascontiguousarray(a, dtype=D.o.dtype))
+    return out
+
+
+def _ptrs4(arrs):
+    P = ctypes.c_void_p
+    return (P * 4)(*[a.ctypes.data for a in arrs])
+
+
+def pin_lateral_relaxation(o):
+    """relax_bdytend (share/module_bc.F:1221-1427) with boundary arrays that equal the field's
+    own rows (value + dtbc * tendency = field): no tendency anywhere; a deviation delta of
+    the field at one point of a relaxation row relaxes it at fcx + 4 gcx and adds gcx delta
+    to each of its four neighbours inside the relaxation rows (the 5-point smoother), and
+    nothing else changes -- on every side, for mass and u-staggered fields."""
+    D = Dom(o, nx=14, ny=12, nz=4, bc="specified")
+    d, cfg = D.d, D.cfg
+    width, sz, rz = 5, 1, 4
+    cfg.spec_bdy_width, cfg.spec_zone, cfg.relax_zone = width, sz, rz
+    rng = np.random.default_rng(21)
+    fcx = np.zeros(16)
+    gcx = np.zeros(16)
+    fcx[1:5] = rng.uniform(0.01, 0.02, 4)
+    gcx[1:5] = rng.uniform(0.001, 0.002, 4)
+    dtbc = 7.0
+    nk = d.kde - d.kds + 1
+    fn = o.lib.o_relax_bdytend
+    vp = ctypes.c_void_p
+    fn.argtypes = [vp, vp, vp, vp, ctypes.c_char, vp, ctypes.c_int, ctypes.c_int, o.ctype, vp, vp, vp]
+    fn.restype = None
+    for var in ("t", "u"):
+        f = rng.uniform(-2.0, 2.0, D.a["u_2"].shape)
+        ibe = d.ide if var == "u" else d.ide - 1
+        jbe = d.jde - 1
+
+        def row(s_, b, k, ij):
+            # the field's own values on row/column b of side s_
+            if s_ == 0:
+                return f[ij - d.jms, k, d.ids + b - 1 - d.ims]
+            if s_ == 1:
+                return f[ij - d.jms, k, ibe - (b - 1) - d.ims]
+            if s_ == 2:
+                return f[d.jds + b - 1 - d.jms, k, ij - d.ims]
+            return f[jbe - (b - 1) - d.jms, k, ij - d.ims]
+        bt = _lbc_sides(D, nk, width, lambda s_, b, k, ij: rng.uniform(-1, 1, ij.size))
+        bv = _lbc_sides(D, nk, width, lambda s_, b, k, ij: row(s_, b, k, ij))
+        for s_ in range(4):
+            bv[s_] = np.ascontiguousarray(bv[s_] - dtbc * bt[s_], dtype=o.dtype)
+
+        def relax(field):
+            fld = np.ascontiguousarray(field, dtype=o.dtype)
+            tend = np.zeros_like(fld)
+            fn(fld.ctypes.data, tend.ctypes.data, _ptrs4(bv), _ptrs4(bt), var.encode(), ctypes.byref(cfg), sz, rz,
+               dtbc, np.ascontiguousarray(fcx, dtype=o.dtype).ctypes.data,
+               np.ascontiguousarray(gcx, dtype=o.dtype).ctypes.data, ctypes.byref(d))
+            return tend
+        t0 = relax(f)
+        assert np.abs(t0).max() <= 1e-12 * np.abs(f).max(), (var, np.abs(t0).max())
+        # one deviation per side, on the second relaxation row, away from the corners
+        mid_i, mid_j = (d.ids + ibe) // 2, (d.jds + jbe) // 2
+        for (i, j, bd) in ((d.ids + 2, mid_j, 2), (ibe - 2, mid_j, 2), (mid_i, d.jds + 2, 2), (mid_i, jbe - 2, 2)):
+            g = f.copy()
+            delta = 0.5
+            k = 1
+            g[j - d.jms, k, i - d.ims] += delta
+            t = relax(g)
+            exp = np.zeros_like(t)
+            exp[j - d.jms, k, i - d.ims] = -(fcx[bd] + 4 * gcx[bd]) * delta
+            for di, dj in ((1, 0), (-1, 0), (0, 1), (0, -1)):
+                ii, jj = i + di, j + dj
+                bnd = min(ii - d.ids, ibe - ii, jj - d.jds, jbe - jj)
+                if sz <= bnd <= rz - 1:
+                    exp[jj - d.jms, k, ii - d.ims] = gcx[bnd] * delta
+            _close(t, exp, 1e-9, f"relaxation of a point deviation, {var}, ({i}, {j})")
+
+
+def _zone_side(i, j, ibs, ibe, jbs, jbe):
+    """The side whose spec rows hold (i, j) in the reference's frame: the y sides own their
+    full rows between the x sides' diagonals, the x sides the rest."""
+    dy = min(j - jbs, jbe - j)
+    dx = min(i - ibs, ibe - i)
+    if dy <= dx:
+        return (2 if j - jbs <= jbe - j else 3), dy
+    return (0 if i - ibs <= ibe - i else 1), dx
+
+
+def pin_spec_zone(o):
+    """spec_bdytend + spec_bdyupdate (share/module_bc.F:1430-2064), zero_grad_bdy and
+    flow_dep_bdy (2219-2456) on spec_zone 2: exactly the frame of width spec_zone changes;
+    each frame point takes the tendency of its side's array at its distance (the y sides
+    own the corners), advances by dt times it, takes the value of the nearest point of the
+    first inner row (zero gradient), and under flow_dep_bdy keeps that value where the
+    flow through its side leaves the domain and is zeroed where it enters."""
+    D = Dom(o, nx=12, ny=10, nz=3, bc="specified")
+    d, cfg = D.d, D.cfg
+    width, sz = 5, 2
+    cfg.spec_bdy_width, cfg.spec_zone = width, sz
+    rng = np.random.default_rng(22)
+    nk = d.kde - d.kds + 1
+    vp = ctypes.c_void_p
+    L = o.lib
+    L.o_spec_bdytend.argtypes = [vp, vp, ctypes.c_char, vp, ctypes.c_int, vp]
+    L.o_spec_bdyupdate.argtypes = [vp, vp, o.ctype, ctypes.c_char, vp, ctypes.c_int, vp]
+    L.o_zero_grad_bdy.argtypes = [vp, ctypes.c_char, vp, ctypes.c_int, vp]
+    L.o_flow_dep_bdy.argtypes = [vp, vp, vp, vp, ctypes.c_int, vp]
+    bt = _lbc_sides(D, nk, width, lambda s_, b, k, ij: 100.0 * s_ + 10.0 * b + 0.01 * ij + 0.001 * k)
+    sh = D.a["u_2"].shape
+    base = np.ascontiguousarray(rng.uniform(-1, 1, sh), dtype=o.dtype)
+    ibs, ibe, jbs, jbe = d.ids, d.ide - 1, d.jds, d.jde - 1
+    tend = base.copy()
+    L.o_spec_bdytend(tend.ctypes.data, _ptrs4(bt), b"t", ctypes.byref(cfg), sz, ctypes.byref(d))
+    fld = base.copy()
+    dt = 3.0
+    L.o_spec_bdyupdate(fld.ctypes.data, tend.ctypes.data, dt, b"t", ctypes.byref(cfg), sz, ctypes.byref(d))
+    zg = base.copy()
+    L.o_zero_grad_bdy(zg.ctypes.data, b"t", ctypes.byref(cfg), sz, ctypes.byref(d))
+    u = np.ascontiguousarray(rng.uniform(-1, 1, sh), dtype=o.dtype)
+    v = np.ascontiguousarray(rng.uniform(-1, 1, sh), dtype=o.dtype)
+    fd = base.copy()
+    L.o_flow_dep_bdy(fd.ctypes.data, u.ctypes.data, v.ctypes.data, ctypes.byref(cfg), sz, ctypes.byref(d))
+    for j in range(d.jms, d.jme + 1):
+        for i in range(d.ims, d.ime + 1):
+            J, I = j - d.jms, i - d.ims
+            inside = ibs <= i <= ibe and jbs <= j <= jbe
+            side, dist = _zone_side(i, j, ibs, ibe, jbs, jbe) if inside else (None, None)
+            in_frame = inside and dist < sz
+            for k in range(0, nk - 1):
+                if not in_frame:
+                    assert tend[J, k, I] == base[J, k, I] and fld[J, k, I] == base[J, k, I]
+                    assert zg[J, k, I] == base[J, k, I] and fd[J, k, I] == base[J, k, I]
+                    continue
+                ij = j if side < 2 else i
+                arr_ij = ij - (d.jms if side < 2 else d.ims)
+                exp_t = bt[side][dist, k, arr_ij]
+                assert tend[J, k, I] == exp_t, (i, j, k)
+                assert abs(fld[J, k, I] - (base[J, k, I] + dt * exp_t)) <= 1e-12 * abs(exp_t) * dt + 1e-12
+                ii = min(max(i, ibs + sz), ibe - sz)
+                jj = min(max(j, jbs + sz), jbe - sz)
+                src = {2: (ii, jbs + sz), 3: (ii, jbe - sz), 0: (ibs + sz, jj), 1: (ibe - sz, jj)}[side]
+                inner = base[src[1] - d.jms, k, src[0] - d.ims]
+                assert zg[J, k, I] == inner, ("zero gradient", i, j, k)
+                flow = {2: -v[J, k, I], 3: v[J + 1, k, I], 0: -u[J, k, I], 1: u[J, k, I + 1]}[side]
+                assert fd[J, k, I] == (inner if flow > 0 else 0.0), ("flow dependent", i, j, k)
+
+
+def pin_spec_bdyupdate_ph(o):
+    """spec_bdyupdate_ph (dyn_em/module_bc_em.F:17-157): with no mass change the spec zone's
+    geopotential advances by dt tend/(c1 mu + c2) (the coupled tendency decoupled); with no
+    tendency and no ph_save, a mass change keeps the coupled value (c1 mu + c2) ph."""
+    D = Dom(o, nx=10, ny=8, nz=5, bc="specified")
+    D.set_levels(stretched_dnw(5, seed=4))
+    d, cfg = D.d, D.cfg
+    cfg.spec_zone = 1
+    rng = np.random.default_rng(23)
+    sh = D.a["u_2"].shape
+    c1, c2 = np.asarray(D.a["c1f"], np.float64), np.asarray(D.a["c2f"], np.float64)
+    c1[:] = rng.uniform(0.5, 1.0, c1.shape)
+    c2[:] = rng.uniform(0.0, 5e3, c2.shape)
+    D.a["c1f"][...] = c1
+    D.a["c2f"][...] = c2
+    muts = np.ascontiguousarray(rng.uniform(8e4, 9e4, D.a["mut"].shape), dtype=o.dtype)
+    vp = ctypes.c_void_p
+    fn = o.lib.o_spec_bdyupdate_ph
+    fn.argtypes = [vp] * 7 + [o.ctype, vp, ctypes.c_int, vp]
+    c = lambda x: np.ascontiguousarray(x, dtype=o.dtype)  # noqa: E731
+    ph0 = c(rng.uniform(-50, 50, sh))
+    tend = c(rng.uniform(-1e5, 1e5, sh))
+    zero3, zero2 = c(np.zeros(sh)), c(np.zeros(D.a["mut"].shape))
+    dt = 2.0
+    frame = np.zeros(sh[::2], bool)
+    frame[d.jds - d.jms, d.ids - d.ims:d.ide - d.ims] = True
+    frame[d.jde - 1 - d.jms, d.ids - d.ims:d.ide - d.ims] = True
+    frame[d.jds - d.jms:d.jde - d.jms, d.ids - d.ims] = True
+    frame[d.jds - d.jms:d.jde - d.jms, d.ide - 1 - d.ims] = True
+    kk = slice(0, d.kde)
+    M = (c1[None, kk, None] * muts[:, None, :] + c2[None, kk, None])
+    ph = ph0.copy()
+    fn(zero3.ctypes.data, ph.ctypes.data, tend.ctypes.data, zero2.ctypes.data, muts.ctypes.data,
+       D.a["c1f"].ctypes.data, D.a["c2f"].ctypes.data, dt, ctypes.byref(cfg), 1, ctypes.byref(d))
+    exp = ph0.copy()
+    exp[:, kk, :] = np.where(frame[:, None, :], ph0[:, kk, :] + dt * tend[:, kk, :] / M, ph0[:, kk, :])
+    _close(ph, exp, 1e-12, "spec zone ph with constant mass")
+    mu_tend = c(rng.uniform(-20, 20, D.a["mut"].shape))
+    ph = ph0.copy()
+    fn(zero3.ctypes.data, ph.ctypes.data, zero3.ctypes.data, mu_tend.ctypes.data, muts.ctypes.data,
+       D.a["c1f"].ctypes.data, D.a["c2f"].ctypes.data, dt, ctypes.byref(cfg), 1, ctypes.byref(d))
+    M_old = (c1[None, kk, None] * (muts - dt * mu_tend)[:, None, :] + c2[None, kk, None])
+    exp = ph0.copy()
+    exp[:, kk, :] = np.where(frame[:, None, :], ph0[:, kk, :] * M_old / M, ph0[:, kk, :])
+    _close(ph, exp, 1e-12, "spec zone ph keeps its coupled value")
+
+
+def pin_specified_frame_held(o):
+    """Whole steps with specified boundaries and zero boundary tendencies: the spec zone
+    (the outermost row of every side) keeps its values of time n through both steps -- u,
+    v, theta, mu and phi to the rounding of their coupling and decoupling (spec_bdy_dry
+    gives them the boundary tendency 0 and the acoustic steps advance them by dts times
+    it) -- while the interior evolves (w's zero-gradient copy is pinned with
+    zero_grad_bdy itself)."""
+    hs = grid.build_initial_state(grid.DomainSpec(nx=16, ny=14, nz=10, bc="specified", n_bubbles=1, bubble_amp=3.0,
+                                                  n_passive=1, seed=24))
+    grid.build_lbc(hs, seed=25, tend_scale=0.0, dtype=o.dtype)
+    out = o.run(hs, 2)
+    d = hs.dims
+    J0, I0 = d.jds - d.jms, d.ids - d.ims
+    nx, ny, nz = d.ide - d.ids, d.jde - d.jds, d.kde - d.kds
+    frames = {  # (array, i extent, j extent, levels)
+        "u_2": (nx + 1, ny, nz), "v_2": (nx, ny + 1, nz), "t_2": (nx, ny, nz), "ph_2": (nx, ny, nz + 1)}
+    moved = 0.0
+    for k, (ex, ey, ez) in frames.items():
+        a0, a1 = hs.a3[k].astype(np.float64), out[k]
+        for sl in ((slice(J0, J0 + 1), slice(0, ez), slice(I0, I0 + ex)),
+                   (slice(J0 + ey - 1, J0 + ey), slice(0, ez), slice(I0, I0 + ex)),
+                   (slice(J0, J0 + ey), slice(0, ez), slice(I0, I0 + 1)),
+                   (slice(J0, J0 + ey), slice(0, ez), slice(I0 + ex - 1, I0 + ex))):
+            assert np.abs(a1[sl] - a0[sl]).max() <= 1e-12 * (np.abs(a0).max() + 1.0), f"{k}: the spec zone changed"
+        inner = (slice(J0 + 3, J0 + ey - 3), slice(0, ez), slice(I0 + 3, I0 + ex - 3))
+        moved = max(moved, float(np.abs(a1[inner] - a0[inner]).max()))
+    assert moved > 1e-3, "the interior did not evolve"
+    m0, m1 = hs.a2["mu_2"].astype(np.float64), out["mu_2"]
+    for sl in ((J0, slice(I0, I0 + nx)), (J0 + ny - 1, slice(I0, I0 + nx)), (slice(J0, J0 + ny), I0),
+               (slice(J0, J0 + ny), I0 + nx - 1)):
+        assert np.abs(m1[sl] - m0[sl]).max() <= 1e-12 * np.abs(m0).max(), "mu: the spec zone changed"
+
+
 PINS = [pin_advect_constant_field, pin_advect_linear_eta, pin_coef_w_spectrum, pin_advance_w_theta_scheme,
         pin_advance_w_hydrostatic, pin_calc_p_rho_linearization, pin_advance_uv_pgf, pin_pressure_gradient,
         pin_advance_mu_t_budget,
         pin_small_step_round_trip, pin_constant_mixing_ratio, pin_resting_over_terrain, pin_advect_scalar_pd,
         pin_pd_whole_step, pin_sixth_order_diffusion, pin_rayleigh_damp, pin_advance_w_damp_opt3,
-        pin_horizontal_diffusion, pin_vertical_diffusion]
+        pin_horizontal_diffusion, pin_vertical_diffusion, pin_lateral_relaxation, pin_spec_zone,
+        pin_spec_bdyupdate_ph, pin_specified_frame_held]

@@ -18,7 +18,7 @@ import oracle_pins
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE = os.path.join(ROOT, "oracle")
-SRCS = ["bc_prep.c", "advect.c", "advect_pd.c", "mixing.c", "tend.c", "small.c", "scalar_eos.c", "solve_step.c"]
+SRCS = ["bc_prep.c", "advect.c", "advect_pd.c", "mixing.c", "lbc.c", "tend.c", "small.c", "scalar_eos.c", "solve_step.c"]
 CFLAGS = ["-O1", "-fPIC", "-std=c11", "-ffp-contract=off", "-fno-fast-math", "-w", "-DWREAL=double",
           "-DWREAL_IS_DOUBLE=1"]
 
@@ -106,6 +106,15 @@ MUTANTS = [
      "(R(0.5) * (A3(alt, i, k, j) + A3(alt, i, k, j))) *", 1),
     ("vdiff_w_top_level_skipped", "mixing.c", "for (int k = kts + 1; k <= ktf; k++)\n        A3(tend, i, k, j) = A3(tend, i, k, j) + A1(rdn, k)",
      "for (int k = kts + 1; k <= ktf - 1; k++)\n        A3(tend, i, k, j) = A3(tend, i, k, j) + A1(rdn, k)", 1),
+    ("relax_row_below_taken_from_row_above", "lbc.c", "f3 = FLS(2, i, b_dist, i, k, j - 1),",
+     "f3 = FLS(2, i, b_dist + 2, i, k, j - 1),", 1),
+    ("relax_smoother_sign", "lbc.c", "+ fcx[(bd)] * (f0) - gcx[(bd)] *", "+ fcx[(bd)] * (f0) + gcx[(bd)] *", 1),
+    ("spec_zone_y_sides_take_the_corners", "lbc.c", "b_limit = px_ ? 0 : b_dist, side = 2;", "b_limit = 0, side = 2;", 1),
+    ("zero_grad_copies_the_frame_itself", "lbc.c",
+     "A3(field, i, k, j) = A3(field, ii, k, side == 2 ? z.jbs + spec_zone : z.jbe - spec_zone);",
+     "A3(field, i, k, j) = A3(field, ii, k, side == 2 ? z.jbs + spec_zone - 1 : z.jbe - spec_zone);", 1),
+    ("spec_ph_old_mass_sign", "lbc.c", "A2(muts, i, j) - dt * A2(mu_tend, i, j);", "A2(muts, i, j) + dt * A2(mu_tend, i, j);", 1),
+    ("flow_dep_inflow_kept", "lbc.c", "side == 2 ? A3(v, i, k, j) < R(0.)", "side == 2 ? A3(v, i, k, j) > R(0.)", 1),
     ("damp_opt3_half_depth", "small.c", "wreal hbot = htop - hdepth;", "wreal hbot = htop - R(0.5) * hdepth;", 1),
     ("pressure_gradient_half_dropped", "tend.c",
      "wreal dpx = (A2(s->msfux, i, j) / A2(s->msfuy, i, j)) * R(.5) * rdx *",

@@ -95,6 +95,7 @@ struct DState {
   FPtr<float> q[WRFMI_MAX_SPECIES];
   FPtr<float> q_old[WRFMI_MAX_SPECIES];
   FPtr<float> q_tend[WRFMI_MAX_SPECIES];
+  const void *lbc;  // host memory: the boundary kernels take their arrays by argument
 };
 static_assert(sizeof(DState) == sizeof(wrfmi_state), "DState must mirror wrfmi_state");
 

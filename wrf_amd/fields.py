@@ -53,6 +53,7 @@ _CFG_INT = ("periodic_x", "periodic_y", "symmetric_xs", "symmetric_xe", "symmetr
             "non_hydrostatic", "top_lid", "phi_adv_z", "hypsometric_opt", "use_theta_m",
             "damp_opt", "w_damping", "diff_opt", "diff_6th_opt", "zadvect_implicit", "pert_coriolis",
             "map_proj", "time_step_sound", "time_step", "diff_6th_slopeopt", "km_opt", "bl_pbl_physics")
+_CFG_INT2 = ("spec_bdy_width", "spec_zone", "relax_zone", "have_bcs_moist", "have_bcs_scalar")
 _CFG_REAL = ("epssm", "smdiv", "emdiv", "dampcoef", "zdamp", "w_crit_cfl", "diff_6th_factor", "diff_6th_thresh",
              "khdif", "kvdif")
 
@@ -61,7 +62,7 @@ class Config(ctypes.Structure):
     """wrfmi_config: the config_flags fields the in-scope routines read."""
 
     _fields_ = ([(n, ctypes.c_int) for n in _CFG_INT] + [(n, ctypes.c_float) for n in _CFG_REAL]
-                + [("precision_mode", ctypes.c_int)])
+                + [(n, ctypes.c_int) for n in _CFG_INT2] + [("precision_mode", ctypes.c_int)])
 
 
 def make_state_struct(real_ctype, name: str):
@@ -73,6 +74,21 @@ def make_state_struct(real_ctype, name: str):
     fields += [("a", ctypes.c_void_p), ("alpha", ctypes.c_void_p), ("gamma", ctypes.c_void_p)]
     fields += [("n_moist", ctypes.c_int), ("n_species", ctypes.c_int)]
     fields += [(n, P * MAX_SPECIES) for n in SPECIES_ARRAYS]
+    fields += [("lbc", ctypes.c_void_p)]
+    return type(name, (ctypes.Structure,), {"_fields_": fields})
+
+
+MAX_BDY_WIDTH = 16
+LBC_FIELDS = ("u", "v", "ph", "t", "w", "mu")
+
+
+def make_lbc_struct(real_ctype, name: str):
+    """The ctypes mirror of ``wrfmi_lbc`` (real_ctype=c_float) or of the oracle's olbc."""
+    P = ctypes.POINTER(real_ctype)
+    fields = [("dtbc", real_ctype), ("fcx", real_ctype * MAX_BDY_WIDTH), ("gcx", real_ctype * MAX_BDY_WIDTH)]
+    for n in LBC_FIELDS:
+        fields += [(n + "_b", P * 4), (n + "_bt", P * 4)]
+    fields += [("q_b", (P * 4) * MAX_SPECIES), ("q_bt", (P * 4) * MAX_SPECIES)]
     return type(name, (ctypes.Structure,), {"_fields_": fields})
 
 

@@ -12,6 +12,7 @@ input_sounding file, computed in float64 and rounded once to float32.
 """
 from __future__ import annotations
 
+import ctypes
 import dataclasses
 import math
 
@@ -43,7 +44,8 @@ class DomainSpec:
     dt: float = 12.0
     ztop: float = 20000.0
     time_step_sound: int = 6
-    bc: str = "periodic"      # "periodic" | "open" (both directions unless bc_x/bc_y say otherwise)
+    bc: str = "periodic"      # "periodic" | "open" | "specified" (both directions unless bc_x/bc_y
+                              # say otherwise; "specified" in y with x "specified" or "periodic")
     bc_x: str = ""            # per-direction override of bc
     bc_y: str = ""
     hybrid_opt: int = 0
@@ -115,8 +117,14 @@ def make_config(spec: DomainSpec) -> Config:
     diffusion and damping off, as the benchmark configurations state."""
     c = Config()
     for b in (spec.bcx, spec.bcy):
-        if b not in ("periodic", "open"):
-            raise ValueError(f"lateral boundary {b!r}: periodic or open")
+        if b not in ("periodic", "open", "specified"):
+            raise ValueError(f"lateral boundary {b!r}: periodic, open or specified")
+    if "specified" in (spec.bcx, spec.bcy) and not (spec.bcy == "specified" and spec.bcx in ("specified", "periodic")):
+        raise ValueError("specified boundaries: on all four sides, or in y with a periodic x")
+    c.specified = int(spec.bcy == "specified")
+    # namelist defaults (Registry/Registry.EM_COMMON: spec_bdy_width 5, spec_zone 1, relax_zone 4)
+    c.spec_bdy_width, c.spec_zone, c.relax_zone = 5, 1, 4
+    c.have_bcs_moist = c.have_bcs_scalar = 0
     c.periodic_x = int(spec.bcx == "periodic")
     c.periodic_y = int(spec.bcy == "periodic")
     c.open_xs = c.open_xe = int(spec.bcx == "open")
@@ -299,6 +307,7 @@ class HostState:
         self.q_old = [np.zeros(shape3(d), np.float32) for _ in range(self.n_species)]
         self.q_tend = [np.zeros(shape3(d), np.float32) for _ in range(self.n_species)]
         self.coef_dtype = np.float64 if spec.precision_mode == 1 else np.float32
+        self.lbc = None  # lateral boundary arrays of a specified domain (build_lbc)
         self.a = np.zeros(shape3(d), self.coef_dtype)
         self.alpha = np.zeros(shape3(d), self.coef_dtype)
         self.gamma = np.zeros(shape3(d), self.coef_dtype)
@@ -314,6 +323,8 @@ class HostState:
             out[f"q_old{m}"] = self.q_old[m]
             out[f"q_tend{m}"] = self.q_tend[m]
         out["a"], out["alpha"], out["gamma"] = self.a, self.alpha, self.gamma
+        if self.lbc is not None:
+            out.update(lbc_arrays(self.lbc))
         return out
 
     def copy(self):
@@ -329,6 +340,11 @@ class HostState:
         other.q_tend = [x.copy() for x in self.q_tend]
         other.coef_dtype = self.coef_dtype
         other.a, other.alpha, other.gamma = self.a.copy(), self.alpha.copy(), self.gamma.copy()
+        other.lbc = None
+        if self.lbc is not None:
+            other.lbc = {k: ([[a.copy() for a in sides] for sides in v] if k in ("q_b", "q_bt")
+                             else [a.copy() for a in v] if isinstance(v, list) else np.array(v, copy=True))
+                         for k, v in self.lbc.items()}
         return other
 
 
@@ -849,3 +865,128 @@ def terrain_c4(**kw) -> DomainSpec:
                 n_bubbles=0, noise_amp=0.0, seed=1005)
     base.update(kw)
     return DomainSpec(**base)
+
+
+# ----------------------------------------------------------------------------- lateral boundaries
+LBC_FIELDS = ("u", "v", "ph", "t", "w", "mu")
+_SIDES = ("xs", "xe", "ys", "ye")
+
+
+def lbc_fcx_gcx(spec_bdy_width, spec_zone, relax_zone, dt, spec_exp=0.0):
+    """The relaxation weights of lbc_fcx_gcx (dyn_em/module_bc_em.F:1297-1341, specified),
+    in REAL arithmetic; entries 0..spec_zone-1 (the spec zone) stay zero."""
+    f32 = np.float32
+    fcx = np.zeros(16, f32)
+    gcx = np.zeros(16, f32)
+    for loop in range(spec_zone + 1, spec_zone + relax_zone + 1):
+        n, m = f32(spec_zone + relax_zone - loop), f32(relax_zone - 1)
+        w = f32(np.exp(f32(-(loop - (spec_zone + 1))) * f32(spec_exp)))
+        fcx[loop - 1] = (f32(0.1) / f32(dt) * n / m) * w
+        gcx[loop - 1] = (f32(1.0) / f32(dt) / f32(50.) * n / m) * w
+    return fcx, gcx
+
+
+def build_lbc(hs, seed: int = 0, value_noise: float = 1e-3, tend_scale: float = 2e-4, dtype=np.float32):
+    """Synthetic lateral boundary arrays for a specified domain: per field the coupled
+    values of the initial state's boundary rows (u, v times c1 mu + c2, ph, t and w times
+    c1 mu + c2, mu) perturbed by a relative `value_noise`, and tendencies of a relative
+    `tend_scale` per second (seeded).  The species with boundary arrays (QV, and the others
+    under have_bcs_moist / have_bcs_scalar) take theirs the same way.  The products are
+    formed in `dtype` (float64 for an FP64 oracle, where they then equal its own mass
+    weighting exactly).  Sets hs.lbc."""
+    d, cfg = hs.dims, hs.cfg
+    w = int(cfg.spec_bdy_width)
+    rng = np.random.default_rng(seed)
+    f32 = np.float32
+    A3 = {k: v.astype(dtype) for k, v in hs.a3.items() if k in ("u_2", "v_2", "ph_2", "w_2", "t_2")}
+    A2 = {k: hs.a2[k].astype(dtype) for k in ("mub", "mu_2")}
+    A1 = {k: hs.a1[k].astype(dtype) for k in ("c1h", "c2h", "c1f", "c2f")}
+    nkb = d.kde - d.kds + 1
+    ci, cj = d.ids - d.ims, d.jds - d.jms
+    mut = A2["mub"] + A2["mu_2"]
+    c1h, c2h, c1f, c2f = (A1[n] for n in ("c1h", "c2h", "c1f", "c2f"))
+
+    def coupled(name):
+        if name == "u":
+            return A3["u_2"] * (c1h[None, :, None] * mut[:, None, :] + c2h[None, :, None])
+        if name == "v":
+            return A3["v_2"] * (c1h[None, :, None] * mut[:, None, :] + c2h[None, :, None])
+        if name in ("ph", "w"):
+            return A3[name + "_2"] * (c1f[None, :, None] * mut[:, None, :] + c2f[None, :, None])
+        if name == "t":
+            return A3["t_2"] * (c1h[None, :, None] * mut[:, None, :] + c2h[None, :, None])
+        if name == "mu":
+            return A2["mu_2"][:, None, :]
+        m = int(name[1:])
+        return hs.q[m].astype(dtype) * (c1h[None, :, None] * mut[:, None, :] + c2h[None, :, None])
+
+    def sides(name):
+        full = coupled(name)  # (nj, nk, ni)
+        nk = 1 if name == "mu" else nkb
+        ibe = d.ide - d.ims if name == "u" else d.ide - 1 - d.ims
+        jbe = d.jde - d.jms if name == "v" else d.jde - 1 - d.jms
+        vals, tends = [], []
+        for s_ in range(4):
+            if s_ < 2:
+                cols = [ci + b if s_ == 0 else ibe - b for b in range(w)]
+                a = np.stack([full[:, :nk, c] for c in cols])  # (w, nj, nk)
+                a = a.transpose(0, 2, 1)                       # (w, nk, nj)
+            else:
+                rows = [cj + b if s_ == 2 else jbe - b for b in range(w)]
+                a = np.stack([full[r, :nk, :] for r in rows])  # (w, nk, ni)
+            scale = np.abs(a).max() or 1.0
+            if value_noise:
+                a = a * (1.0 + value_noise * rng.standard_normal(a.shape))
+            vals.append(np.ascontiguousarray(a.astype(dtype)))
+            tends.append(np.ascontiguousarray((tend_scale * scale * rng.standard_normal(a.shape)).astype(dtype)))
+        return vals, tends
+
+    lbc = {"dtbc": np.float32(0.0)}
+    lbc["fcx"], lbc["gcx"] = lbc_fcx_gcx(w, int(cfg.spec_zone), int(cfg.relax_zone), hs.scal["dt"])
+    for n in LBC_FIELDS:
+        lbc[n + "_b"], lbc[n + "_bt"] = sides(n)
+    lbc["q_b"] = [None] * hs.n_species
+    lbc["q_bt"] = [None] * hs.n_species
+    for m in range(hs.n_species):
+        moist = m < hs.n_moist
+        if cfg.nested or (m == 0 and moist) or (cfg.have_bcs_moist if moist else cfg.have_bcs_scalar):
+            lbc["q_b"][m], lbc["q_bt"][m] = sides(f"q{m}")
+    hs.lbc = lbc
+    return lbc
+
+
+def lbc_arrays(lbc) -> dict:
+    """The boundary arrays of `lbc` by flat name (lbc_<field>_b<side>, lbc_q<m>_b<side>)."""
+    out = {}
+    for n in LBC_FIELDS:
+        for s_ in range(4):
+            out[f"lbc_{n}_b{s_}"] = lbc[n + "_b"][s_]
+            out[f"lbc_{n}_bt{s_}"] = lbc[n + "_bt"][s_]
+    for m, sides in enumerate(lbc["q_b"]):
+        if sides is not None:
+            for s_ in range(4):
+                out[f"lbc_q{m}_b{s_}"] = sides[s_]
+                out[f"lbc_q{m}_bt{s_}"] = lbc["q_bt"][m][s_]
+    return out
+
+
+def lbc_struct(lbc, struct_type, ptr, ctype=ctypes.c_float):
+    """A ctypes wrfmi_lbc (or oracle olbc) for `lbc` whose arrays are ptr(flat name)."""
+    import ctypes as ct
+    L = struct_type()
+    L.dtbc = float(lbc["dtbc"])
+    for k in range(16):
+        L.fcx[k] = float(lbc["fcx"][k])
+        L.gcx[k] = float(lbc["gcx"][k])
+    P = ct.POINTER(ctype)
+    for n in LBC_FIELDS:
+        for s_ in range(4):
+            getattr(L, n + "_b")[s_] = ct.cast(ptr(f"lbc_{n}_b{s_}"), P)
+            getattr(L, n + "_bt")[s_] = ct.cast(ptr(f"lbc_{n}_bt{s_}"), P)
+    for m, sides in enumerate(lbc["q_b"]):
+        if sides is not None:
+            for s_ in range(4):
+                L.q_b[m][s_] = ct.cast(ptr(f"lbc_q{m}_b{s_}"), P)
+                L.q_bt[m][s_] = ct.cast(ptr(f"lbc_q{m}_bt{s_}"), P)
+    return L
+
