@@ -17,6 +17,7 @@ species transport on the last substep (moist_adv_opt = scalar_adv_opt = 1), with
 diffusion and Rayleigh damping on the first (diff_6th_opt 2, damp_opt 2), with constant-
 coefficient second-order diffusion on the first (diff_opt 1, km_opt 1), and with advance_w's
 damping layer (damp_opt 3): ms per step and the kernels the options add, against 8 TB/s.
+specified = the same workload with specified lateral boundaries (synthetic boundary arrays).
 terrain (with --terrain) = the 1200x1200x80 terrain case, FP32 and precision forms.
 cpu_baseline = the FP32 C restatement of the same step (the oracle), OpenMP over the host
 cores with solve_em's j-tiling, on the same workload: median of 3 single steps.
@@ -233,6 +234,40 @@ def option_leg(runtime, dev, torch, stream, steps, pts, log, name):
     return out
 
 
+def specified_leg(runtime, grid, torch, steps, log, nx, ny, nz):
+    """The same workload with specified lateral boundaries on all four sides (synthetic
+    boundary arrays of the initial state's boundary rows): ms per step over `steps` steps
+    after one untimed step, and the boundary kernel's launches (HIP events, a separate run).
+    The specified path runs the one-point kernel forms, which carry the spec-zone bounds."""
+    spec = grid.large_c1(nx=nx, ny=ny, nz=nz, bc="specified")
+    hs = grid.build_initial_state(spec)
+    grid.build_lbc(hs, seed=1003)
+    pts = spec.nx * spec.ny * spec.nz
+    dev = runtime.DeviceDomain(hs)
+    stream = torch.cuda.current_stream()
+    dev.step(1, stream=stream)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    dev.step(steps, stream=stream, sync=False)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    with runtime.KernelTimer(["k_lbc"]) as timer:
+        dev.step(steps, stream=stream)
+        kt = timer.collect()
+    nbad = dev.check_finite()
+    if nbad:
+        raise RuntimeError(f"specified: {nbad} non-finite values after the timed steps")
+    n, ms = kt["k_lbc"]
+    out = {"boundaries": "specified on all four sides, spec_bdy_width 5, spec_zone 1, relax_zone 4",
+           "ms_per_step": round(1e3 * elapsed / steps, 4), "mpoint_steps_per_s": round(pts * steps / elapsed / 1e6, 1),
+           "k_lbc": {"launches_per_step": n // steps, "ms_per_step": round(ms / steps, 4)}}
+    log(f"  specified: {out['ms_per_step']} ms/step; k_lbc {out['k_lbc']['launches_per_step']} launches, "
+        f"{out['k_lbc']['ms_per_step']} ms per step")
+    del dev
+    torch.cuda.empty_cache()
+    return out
+
+
 def terrain_leg(runtime, grid, torch, steps, log):
     """The 1200x1200x80 terrain case (hybrid levels, 12 Witch-of-Agnesi ridges, periodic y /
     open x, dx = 500 m, dt = 3 s, qv + 10 scalars): Mpoint-RK3-steps/s over `steps` steps
@@ -298,7 +333,8 @@ def main():
     ap.add_argument("--terrain-steps", type=int, default=20)
     ap.add_argument("--terrain-only", action="store_true", help="run only the terrain leg (profiling)")
     ap.add_argument("--no-options", action="store_true",
-                    help="skip the option legs (positive-definite transport, mixing, diffusion, w damping)")
+                    help="skip the option legs (positive-definite transport, mixing, diffusion, w damping, "
+                         "specified boundaries)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after the timed steps, copy a buffer of this many MiB once (calibrates HBM counters)")
     args = ap.parse_args()
@@ -414,6 +450,7 @@ def main():
     if not args.no_options:
         for name in OPTION_LEGS:
             out[name] = option_leg(runtime, dev, torch, stream, args.steps, pts, log, name)
+        out["specified"] = specified_leg(runtime, grid, torch, args.steps, log, spec.nx, spec.ny, spec.nz)
     if args.ensemble_members > 0:
         del dev
         torch.cuda.empty_cache()

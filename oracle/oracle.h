@@ -14,7 +14,7 @@
  * Every function cites the reference file:line it follows.
  *
  * How each part is pinned (tests/test_oracle.py, tests/oracle_pins.py; the mutation test
- * tests/test_oracle_mutations.py builds this source with 35 plausible mistakes planted and
+ * tests/test_oracle_mutations.py builds this source with 41 plausible mistakes planted and
  * shows that every one fails at least one pin):
  *   o_advect_scalar  bit for bit against the reference's own advect_scalar, compiled in
  *                    place (oracle/_ref) and through tests/golden/advect_scalar_ref.npz:
@@ -51,6 +51,16 @@
  *                    the base subtracted.
  *   o_vertical_diffusion*  2 a mu K for a z^2 profile on interior levels, zero gradient
  *                    below the lowest level, no flux through the top, bases subtracted.
+ *   o_relax_bdytend  boundary arrays equal to the field give no tendency; a deviation at
+ *                    one relaxation point relaxes at fcx + 4 gcx and feeds gcx to its four
+ *                    neighbours (the 5-point smoother), on every side.
+ *   o_spec_bdytend/_bdyupdate, o_zero_grad_bdy, o_flow_dep_bdy  exactly the frame of width
+ *                    spec_zone changes, by dt times its side's tendency (corners split along
+ *                    the diagonal); zero gradient from the first inner row; outflow kept, inflow
+ *                    zeroed.
+ *   o_spec_bdyupdate_ph  the coupled tendency decoupled by (c1 mu + c2); the coupled value
+ *                    kept under a mass change.
+ *   specified whole step  zero boundary tendencies hold the spec zone at time level n.
  *   whole step       resting atmosphere stays at rest (flat and over terrain); uniform
  *                    state stays uniform; mirror and x/y-transpose symmetry of a centred
  *                    bubble (periodic and open); periodic translation invariance (bitwise);
@@ -59,7 +69,8 @@
  * Parity unpinned (no reference build, fixture or closed form covers them, and the
  * benchmark configurations do not exercise them): the w_damping = 1 branch of w_damp,
  * the open- and specified-boundary loop bounds of the diffusion routines (their closed
- * forms run on periodic domains), top_lid = 1, phi_adv_z = 2, hypsometric_opt = 2's
+ * forms run on periodic domains), the relaxation and spec zones of nested boundaries
+ * (refused by the GPU library), top_lid = 1, phi_adv_z = 2, hypsometric_opt = 2's
  * log-pressure form of calc_p_rho_phi beyond the initial state it reproduces,
  * use_theta_m = 0, Coriolis with f or e non-zero, curvature with map factors other than
  * 1.  The terrain terms (the surface-w term of advance_w and set_w_surface, with their

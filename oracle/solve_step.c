@@ -249,8 +249,12 @@ int oracle_rk_step_prep(ostate *s, const cfg_t *c, const dims_t *d, int rk_step)
   return 0;
 }
 int oracle_rk_tendency(ostate *s, const cfg_t *c, const dims_t *d, int rk_step) {
+  const int lbc = c->specified || c->nested;
+  if (lbc && !s->lbc) return WRFMI_E_BADOPT;
   TILED(o_rk_tendency(s, c, d, rk_step));
+  if (lbc && rk_step == 1) o_relax_bdy_dry(s, c, d);
   TILEDV(o_rk_addtend_dry(s, d, rk_step));
+  if (lbc) o_spec_bdy_dry(s, c, d);
   return 0;
 }
 int oracle_small_step_prep(ostate *s, const cfg_t *c, const dims_t *d, int rk_step, double dts) {
@@ -271,12 +275,16 @@ int oracle_small_step_prep(ostate *s, const cfg_t *c, const dims_t *d, int rk_st
 }
 int oracle_advance_uv(ostate *s, const cfg_t *c, const dims_t *d, double dts) {
   TILEDV(o_advance_uv(s, c, d, (wreal)dts));
+  if (c->specified || c->nested) spec_uv(s, c, d, (wreal)dts);
   return 0;
 }
 int oracle_advance_mu_t_w(ostate *s, const cfg_t *c, const dims_t *d, double dts, int iteration, int n_small) {
+  const int lbc = c->specified || c->nested;
   TILEDV(o_advance_mu_t(s, c, d, (wreal)dts));
+  if (lbc) spec_mu_t(s, c, d, (wreal)dts);
   if (c->non_hydrostatic) TILEDV(o_advance_w(s, c, d, (wreal)dts));
   TILEDV(o_sumflux(s, d, iteration, n_small));
+  if (lbc) spec_w_ph(s, c, d, (wreal)dts);
   TILEDV(o_calc_p_rho(s, c, d, iteration));
   o_set_physical_bc3d(s->ph_2, 'w', c, d);
   o_set_physical_bc3d(s->al, 'p', c, d);
@@ -303,8 +311,12 @@ int oracle_rk_scalar_pd_prep(ostate *s, const cfg_t *c, const dims_t *d, double 
   return 0;
 }
 int oracle_rk_scalar_step(ostate *s, const cfg_t *c, const dims_t *d, int ispec, int rk_step, double dt_rk) {
+  const int lbc = c->specified || c->nested;
   TILED(o_rk_scalar_tend(s, c, d, ispec, rk_step, (wreal)dt_rk));
+  if (lbc && rk_step == 1 && o_species_has_bcs(s, c, ispec)) o_relax_spec_bdy_scalar(s, c, d, ispec);
   TILEDV(o_rk_update_scalar(s, d, ispec, rk_step, (wreal)dt_rk));
+  if (c->specified && !o_species_has_bcs(s, c, ispec))
+    o_flow_dep_bdy(s->q[ispec], s->ru_m, s->rv_m, c, c->spec_zone, d);
   return 0;
 }
 int oracle_calc_p_rho_phi(ostate *s, const cfg_t *c, const dims_t *d) {

@@ -8,7 +8,7 @@ import os
 
 import numpy as np
 
-from wrf_amd.fields import FIELDS_1D, FIELDS_2D, FIELDS_3D, SCALARS_R, State32
+from wrf_amd.fields import FIELDS_1D, FIELDS_2D, FIELDS_3D, SCALARS_R, Lbc32, State32
 from wrf_amd.runtime import PHASE_ARGS
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -49,6 +49,10 @@ class EmuDomain:
             st.q[m] = self.arrays[f"q{m}"].ctypes.data_as(P)
             st.q_old[m] = self.arrays[f"q_old{m}"].ctypes.data_as(P)
             st.q_tend[m] = self.arrays[f"q_tend{m}"].ctypes.data_as(P)
+        if hs.lbc is not None:
+            from wrf_amd.grid import lbc_struct
+            self.lbc = lbc_struct(hs.lbc, Lbc32, lambda n: self.arrays[n].ctypes.data)
+            st.lbc = ctypes.addressof(self.lbc)
         self.state = st
 
     def __del__(self):

@@ -68,6 +68,12 @@ CASES = {
                          hybrid_opt=2, sounding="stable", n_ridges=3, ridge_a=(3000.0, 8000.0), n_passive=1,
                          n_bubbles=0, noise_amp=0.0, seed=40,
                          opts=dict(diff_opt=1, km_opt=1, khdif=500.0, kvdif=30.0, damp_opt=2, zdamp=6000.0)),
+    # specified lateral boundaries (relaxation and spec zones, flow-dependent species)
+    "spec_all": dict(nx=16, ny=14, nz=10, bc="specified", n_passive=1, n_bubbles=2, bubble_amp=2.0, seed=41, lbc=51),
+    "spec_channel": dict(nx=15, ny=13, nz=9, bc_x="periodic", bc_y="specified", n_passive=1, n_bubbles=1, seed=42,
+                         lbc=52),
+    "spec_mixed": dict(nx=17, ny=12, nz=10, bc="specified", n_passive=2, n_bubbles=1, seed=43, lbc=53, pd=(1, 1),
+                       opts=dict(have_bcs_scalar=1, diff_6th_opt=2, damp_opt=3, zdamp=5000.0)),
     "pd_mixed_unaligned": dict(nx=14, ny=11, nz=8, bc_x="open", bc_y="periodic", n_passive=1, n_bubbles=1, seed=28,
                                pad_i=1, pd=(0, 1)),
 }
@@ -78,6 +84,7 @@ def build_case(case: str):
     kw = dict(CASES[case])
     pd = kw.pop("pd", None)
     opts = kw.pop("opts", {})
+    lbc = kw.pop("lbc", None)
     if pd is None:
         hs = grid.build_initial_state(grid.DomainSpec(**kw))
     else:
@@ -85,6 +92,8 @@ def build_case(case: str):
         hs = pd_state(kw, *pd, seed=kw["seed"])
     for k, v in opts.items():
         setattr(hs.cfg, k, v)
+    if lbc is not None:
+        grid.build_lbc(hs, seed=lbc)
     return hs
 
 

@@ -1344,10 +1344,11 @@ def _zone_side(i, j, ibs, ibe, jbs, jbe):
 def pin_spec_zone(o):
     """spec_bdytend + spec_bdyupdate (share/module_bc.F:1430-2064), zero_grad_bdy and
     flow_dep_bdy (2219-2456) on spec_zone 2: exactly the frame of width spec_zone changes;
-    each frame point takes the tendency of its side's array at its distance (the y sides
-    own the corners), advances by dt times it, takes the value of the nearest point of the
-    first inner row (zero gradient), and under flow_dep_bdy keeps that value where the
-    flow through its side leaves the domain and is zeroed where it enters."""
+    each frame point takes the tendency of its side's array at its distance (corners split
+    along the diagonal, the y sides taking it), advances by dt times it, takes the value
+    of the nearest point of the first inner row (zero gradient), and under flow_dep_bdy
+    keeps that value where the flow through its side leaves the domain and is zeroed
+    where it enters."""
     D = Dom(o, nx=12, ny=10, nz=3, bc="specified")
     d, cfg = D.d, D.cfg
     width, sz = 5, 2

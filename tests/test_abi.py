@@ -69,10 +69,34 @@ def test_constant_coefficient_diffusion_only():
     assert b"km_opt" in lib.wrfmi_last_error()
 
 
+def _specified(lib):
+    spec = grid.DomainSpec(nx=12, ny=10, nz=6, bc="specified")
+    cfg, dims = grid.make_config(spec), grid.make_dims(spec)
+    return cfg, dims
+
+
+def test_specified_options_checked():
+    """Specified boundaries: spec_zone 1 and relax_zone >= 2 only; the boundary arrays are
+    required; graph replay and ensembles refuse them (dtbc advances every step)."""
+    lib = runtime.load_library()
+    cfg, dims = _specified(lib)
+    st = fields.State32()
+    cfg.spec_zone = 2
+    rc = lib.wrfmi_solve_em_step(ctypes.byref(st), ctypes.byref(cfg), ctypes.byref(dims), None, 1)
+    assert rc == runtime.WRFMI_E_BADOPT and b"spec_zone" in lib.wrfmi_last_error()
+    cfg.spec_zone = 1
+    rc = lib.wrfmi_solve_em_step_graph(ctypes.byref(st), ctypes.byref(cfg), ctypes.byref(dims), None, 1)
+    assert rc == runtime.WRFMI_E_BADOPT and b"graph" in lib.wrfmi_last_error()
+    h = ctypes.c_void_p()
+    rc = lib.wrfmi_ensemble_create(1, ctypes.byref(st), ctypes.byref(cfg), ctypes.byref(dims), ctypes.byref(h))
+    assert rc == runtime.WRFMI_E_BADOPT and b"specified" in lib.wrfmi_last_error()
+
+
 @pytest.mark.parametrize("field,value,msg", [
     ("rk_ord", 2, "rk_ord"),
     ("h_mom_adv_order", 6, "orders"),
-    ("specified", 1, "lateral"),
+    ("specified", 1, "specified"),
+    ("nested", 1, "lateral"),
     ("diff_opt", 2, "TKE"),
     ("damp_opt", 1, "damp_opt"),
     ("moist_adv_opt", 2, "advection options"),

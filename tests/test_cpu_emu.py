@@ -33,7 +33,8 @@ def emu_env():
                                                ("pd_periodic", 0, 2), ("pd_open", 1, 1), ("pd_mixed_unaligned", 0, 1),
                                                ("pd_periodic_tight", 1, 2), ("mix_periodic", 0, 2),
                                                ("mix_open", 1, 2), ("mix_terrain", 0, 1), ("diff_periodic", 0, 2),
-                                               ("diff_open", 1, 2), ("diff_terrain", 0, 1)])
+                                               ("diff_open", 1, 2), ("diff_terrain", 0, 1), ("spec_all", 0, 2),
+                                               ("spec_all", 1, 1), ("spec_channel", 1, 2), ("spec_mixed", 0, 2)])
 def test_emulated_kernels_match_oracle(emu_env, case, order, nsteps):
     r = subprocess.run([sys.executable, os.path.join(HERE, "emu_phase_check.py"), case, str(order), str(nsteps)],
                        env=emu_env, capture_output=True, text=True, timeout=600)

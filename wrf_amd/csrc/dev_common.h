@@ -224,9 +224,9 @@ DEVI const KArgs &kargs(const BatchArgs &b, Tile3 &T) {
 }
 
 // Lateral-boundary flags of the whole-patch tile (its = ids, ite = ide, ...).  The host
-// accepts periodic or open boundaries only, so symmetric, specified, nested and polar are
-// compile-time false here; PER = true is the doubly periodic specialisation, in which every
-// open-boundary and degraded-order branch is compiled out.
+// accepts periodic, open and specified boundaries, so symmetric and polar are compile-time
+// false here; PER = true is the doubly periodic specialisation, in which every
+// open/specified-boundary and degraded-order branch is compiled out.
 struct Geo {
   bool per_x, per_y, sym_xs, sym_xe, sym_ys, sym_ye, open_xs, open_xe, open_ys, open_ye, spec, nested, polar;
 };
@@ -240,7 +240,9 @@ DEVI Geo geo(const wrfmi_config &c) {
   g.open_xe = PER ? false : (bool)c.open_xe;
   g.open_ys = PER ? false : (bool)c.open_ys;
   g.open_ye = PER ? false : (bool)c.open_ye;
-  g.nested = g.spec = g.polar = false;
+  g.spec = PER ? false : (bool)(c.specified || c.nested);
+  g.nested = PER ? false : (bool)c.nested;
+  g.polar = false;
   return g;
 }
 

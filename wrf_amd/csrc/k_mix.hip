@@ -36,24 +36,27 @@ __global__ void __launch_bounds__(256) k_diff6(ARGS args) {
   const int ids = d.ids, ide = d.ide, jds = d.jds, jde = d.jde, ktf = d.kde - 1;
   // loop bounds of the reference for the whole-patch tile (periodic or open boundaries)
   int i0 = ids, i1 = ide - 1, j0 = jds, j1 = jde - 1, k0 = d.kds;
+  const bool spec = c.specified || c.nested;
   if (K == 0) {
     i1 = ide;
     if (c.open_xs) i0 = ids + 3;
     if (c.open_ys) j0 = jds + 3;
     if (c.open_xe) i1 = ide - 3;
     if (c.open_ye) j1 = jde - 4;
+    if (spec) { i0 = ids + 3; i1 = ide - 3; j0 = jds + 3; j1 = jde - 4; }
   } else if (K == 1) {
     j1 = jde;
     if (c.open_xs) i0 = ids + 3;
     if (c.open_ys) j0 = jds + 3;
     if (c.open_xe) i1 = ide - 4;
     if (c.open_ye) j1 = jde - 3;
-    if (c.open_xs) { i0 = ids + 3; i1 = ide - 4; j0 = jds + 3; j1 = jde - 3; }  // 6380-6385
+    if (c.open_xs || spec) { i0 = ids + 3; i1 = ide - 4; j0 = jds + 3; j1 = jde - 3; }  // 6380-6385
   } else {
     if (c.open_xs) i0 = ids + 3;
     if (c.open_ys) j0 = jds + 3;
     if (c.open_xe) i1 = ide - 4;
     if (c.open_ye) j1 = jde - 4;
+    if (spec) { i0 = ids + 3; i1 = ide - 4; j0 = jds + 3; j1 = jde - 4; }
     if (K == 2) k0 = d.kds + 1;
   }
   if (i < i0 || i > i1 || j < j0 || j > j1 || k < k0 || k > ktf) return;

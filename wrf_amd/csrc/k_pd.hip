@@ -222,21 +222,22 @@ __global__ void __launch_bounds__(256) k_pd_tend(ARGS args) {
     const Off o = X3(d, i, k, j);
     float t = add ? tend[o] : 0.f;
     if (k <= ktf) {
-      // radiation terms of open boundaries (7255-7326), before the flux divergence
-      if (!per_x && i == d.ids) {
+      // radiation terms of open boundaries (7255-7326; not of specified ones), before the
+      // flux divergence
+      if (a.c.open_xs && i == d.ids) {
         const float ub = fmin_r(0.5f * (ru[o] + ru[o + 1]), 0.f);
         t = t - rdx * (ub * (fo[o + 1] - fo[o]) + f[o] * (ru[o + 1] - ru[o]));
       }
-      if (!per_x && i == d.ide - 1) {
+      if (a.c.open_xe && i == d.ide - 1) {
         const Off oe = X3(d, d.ide, k, j);
         const float ub = fmax_r(0.5f * (ru[oe - 1] + ru[oe]), 0.f);
         t = t - rdx * (ub * (fo[o] - fo[o - 1]) + f[o] * (ru[oe] - ru[oe - 1]));
       }
-      if (!per_y && j == d.jds) {
+      if (a.c.open_ys && j == d.jds) {
         const float vb = fmin_r(0.5f * (rv[o] + rv[o + d.sj]), 0.f);
         t = t - rdy * (vb * (fo[o + d.sj] - fo[o]) + f[o] * (rv[o + d.sj] - rv[o]));
       }
-      if (!per_y && j == d.jde - 1) {
+      if (a.c.open_ye && j == d.jde - 1) {
         const Off on = X3(d, i, k, d.jde);
         const float vb = fmax_r(0.5f * (rv[on - d.sj] + rv[on]), 0.f);
         t = t - rdy * (vb * (fo[o] - fo[o - d.sj]) + f[o] * (rv[on] - rv[on - d.sj]));

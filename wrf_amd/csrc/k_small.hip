@@ -241,7 +241,21 @@ __global__ void __launch_bounds__(256) k_advance_uv(ARGS args) {
   const int k1 = imin(k0 + kAdvUvK - 1, kde - 1);
   const wrfmi_config &c = a.c;
   const Geo g = geo<PER>(c);
-  const int i_start = its, i_end = imin(ite, ide - 1), j_start = jts, j_end = imin(jte, jde - 1), i_endu = ite, j_endv = jte;
+  int i_start = its, i_end = imin(ite, ide - 1), j_start = jts, j_end = imin(jte, jde - 1), i_endu = ite, j_endv = jte;
+  if (c.specified || c.nested) {  // module_small_step_em.F:734-747: the spec zone is spec_bdyupdate's
+    const int sz = c.spec_zone;
+    i_start = imax(its, ids + sz);
+    i_end = imin(ite, ide - sz - 1);
+    j_start = imax(jts, jds + sz);
+    j_end = imin(jte, jde - sz - 1);
+    i_endu = imin(ite, ide - sz);
+    j_endv = imin(jte, jde - sz);
+    if (c.periodic_x) {
+      i_start = its;
+      i_end = imin(ite, ide - 1);
+      i_endu = ite;
+    }
+  }
   int i_start_up = i_start, i_end_up = i_endu, j_start_vp = j_start, j_end_vp = j_endv;
   if ((g.open_xs || g.sym_xs) && its == ids) i_start_up++;
   if ((g.open_xe || g.sym_xe) && ite == ide) i_end_up--;
@@ -261,7 +275,7 @@ __global__ void __launch_bounds__(256) k_advance_uv(ARGS args) {
   // stay inside the memory patch, and only the stores are masked: loads under per-lane
   // conditions would make the compiler wait for all loads in flight at each join.
   const bool u_on = j >= j_start && j <= j_end;
-  const bool v_on = i <= i_end && j >= j_start_v_tend && j <= j_end_v_tend;
+  const bool v_on = i >= i_start && i <= i_end && j >= j_start_v_tend && j <= j_end_v_tend;
   const bool utend_on = i >= i_start_u_tend && i <= i_end_u_tend, upg_on = i >= i_start_up && i <= i_end_up;
   const bool vpg_on = j >= j_start_vp && j <= j_end_vp;
   // column constants (map-factor ratios, column masses, the mudf term)
@@ -568,6 +582,7 @@ __global__ void __launch_bounds__(256) k_ac_mu_ww(ARGS args) {
   const int i = col_ia(d, !std::is_same<ARGS, KArgs>::value) + (int)(T.bx * blockDim.x + threadIdx.x);
   const int j = d.jds + (int)(T.by * blockDim.y + threadIdx.y);
   if (i < d.ids || i > d.ide - 1 || j > d.jde - 1) return;
+  if (spec_frame(a.c, d, i, j)) return;
   const int k_end = d.kde - 1;
   const Off o2 = X2(d, i, j);
   const Off ob = X3(d, i, 0, j);  // level k of the column is at ob + k*ni
@@ -631,7 +646,7 @@ __global__ void __launch_bounds__(256) k_ac_theta(ARGS args) {
     if (it == ns) wwm = wwm / (float)ns + s.ww1[o];
     s.ww_m[o] = wwm;
   }
-  if (routine == R_W_TAVE && k <= k_end) {
+  if (routine == R_W_TAVE && k <= k_end && !spec_frame(c, d, i, j)) {
     // advance_w alone: t_2ave from the updated theta and advance_mu_t's t_ave (1312-1320),
     // the expressions of the fused form below
     const float t = s.t_2[o], tv = s.t_2save[o];
@@ -640,7 +655,7 @@ __global__ void __launch_bounds__(256) k_ac_theta(ARGS args) {
     v1 = (v1 + (s.c1h[X1(d, k)] * muave) * K_T0) / ((s.c1h[X1(d, k)] * muts + s.c2h[X1(d, k)]) * (K_T0 + t_1[o]));
     s.t_2save[o] = v1;
   }
-  if ((routine == R_FUSED || routine == R_MU_T) && k <= k_end) {
+  if ((routine == R_FUSED || routine == R_MU_T) && k <= k_end && !spec_frame(c, d, i, j)) {
     const float msftx = s.msftx[o2], msfty = s.msfty[o2];
     const FPtr<float> u = s.u_2, v = s.v_2;
     const float tv = s.t_2[o];
@@ -1050,6 +1065,8 @@ __global__ void __launch_bounds__(256) k_ac_wsolve(ARGS args) {
   if (i < d.ids || i > d.ide - 1 || j > d.jde - 1) return;
   const wrfmi_config &c = a.c;
   if (!c.non_hydrostatic) return;
+  // a specified domain's outermost columns: spec_bdyupdate_ph and calc_p_rho in k_lbc
+  if (spec_frame(c, d, i, j)) return;
   FPtr<float> w = s.w_2, ph = s.ph_2;
   FPtr<float> lo = a.scr_lo;
   const FPtr<float> rhs = a.scr_rhs;

@@ -234,7 +234,8 @@ __global__ void __launch_bounds__(256) k_tend_u(ARGS args) {
       const float msf = s.msfuy[X2(d, i, j)];
       float tf = s.ru_tendf[o];
       if (a.rk_step == 1) {
-        tf = tf + 0.f * msf;  // u_save was zeroed by rk_tendency (module_em.F:395)
+        // u_save: zeroed by rk_tendency (module_em.F:395), or relax_bdy_dry's relaxation
+        tf = tf + ((a.c.specified || a.c.nested) ? s.u_save[o] : 0.f) * msf;
         s.ru_tendf[o] = tf;
       }
       t = t + tf / msf;
@@ -630,7 +631,7 @@ __global__ void __launch_bounds__(256) k_tend_v(ARGS args) {
     if (!adv_only && col_ok && k <= kte - 1) {
       float tf = s.rv_tendf[o];
       if (a.rk_step == 1) {
-        tf = tf + 0.f * s.msfvx[X2(d, i, j)];  // v_save was zeroed by rk_tendency
+        tf = tf + ((a.c.specified || a.c.nested) ? s.v_save[o] : 0.f) * s.msfvx[X2(d, i, j)];  // v_save: zero or relaxation
         s.rv_tendf[o] = tf;
       }
       t = t + tf * s.msfvx_inv[X2(d, i, j)];
@@ -1019,7 +1020,7 @@ __global__ void __launch_bounds__(256) k_tend_w(ARGS args) {
       if (!adv_only) {
         float tf = s.rw_tendf[o];
         if (a.rk_step == 1) {
-          tf = tf + 0.f * msfty;  // w_save was zeroed by rk_tendency
+          tf = tf + ((a.c.specified || a.c.nested) ? s.w_save[o] : 0.f) * msfty;  // w_save: zero or relaxation
           s.rw_tendf[o] = tf;
         }
         t = t + tf / msfty;
@@ -1316,12 +1317,19 @@ __global__ void __launch_bounds__(256) k_tend_t(ARGS args) {
   float wd_c = (mass && k0 >= 2 && k0 <= kte - 1) ? wdwn(k0) : 0.f;     // rhs_ph's wdwn at level k
   for (int k = k0; k <= k1; k++) {
     const Off o = X3(d, i, k, j);
-    // zero_tend of the *_save arrays over the whole tile
-    s.u_save[o] = 0.f;
-    s.v_save[o] = 0.f;
-    s.w_save[o] = 0.f;
-    s.ph_save[o] = 0.f;
-    s.t_save[o] = 0.f;
+    // relax_bdy_dry's relaxation of theta and phi (specified boundaries, rk_step 1), which
+    // rk_addtend_dry adds to the forward tendencies; zero otherwise
+    const bool relax = (a.c.specified || a.c.nested) && a.rk_step == 1;
+    const float t_sv = relax ? s.t_save[o] : 0.f, ph_sv = relax ? s.ph_save[o] : 0.f;
+    // zero_tend of the *_save arrays over the whole tile (with the relaxation they hold its
+    // values, zero outside the relaxation rows, as they are after rk_addtend_dry)
+    if (!relax) {
+      s.u_save[o] = 0.f;
+      s.v_save[o] = 0.f;
+      s.w_save[o] = 0.f;
+      s.ph_save[o] = 0.f;
+      s.t_save[o] = 0.f;
+    }
     float t = 0.f, pt = 0.f, vf_hi = 0.f;
     if (mass) {
       if (k <= ktf) {
@@ -1360,7 +1368,7 @@ __global__ void __launch_bounds__(256) k_tend_t(ARGS args) {
         // rk_addtend_dry (theta)
         float tf = s.t_tendf[o];
         if (a.rk_step == 1) {
-          tf = tf + 0.f;  // t_save was zeroed by rk_tendency
+          tf = tf + t_sv;  // t_save: zeroed by rk_tendency, or the relaxation
           s.t_tendf[o] = tf;
         }
         t = t + tf / msfty + (s.c1h[X1(d, k)] * mut + s.c2h[X1(d, k)]) * s.h_diabatic[o] / msfty;
@@ -1443,7 +1451,7 @@ __global__ void __launch_bounds__(256) k_tend_t(ARGS args) {
       }
       float ptf = s.ph_tendf[o];
       if (a.rk_step == 1) {
-        ptf = ptf + 0.f;  // ph_save was zeroed by rk_tendency
+        ptf = ptf + ph_sv;  // ph_save: zeroed by rk_tendency, or the relaxation
         s.ph_tendf[o] = ptf;
       }
       pt = pt + ptf / msfty;
@@ -1452,9 +1460,12 @@ __global__ void __launch_bounds__(256) k_tend_t(ARGS args) {
     s.ph_tend[o] = pt;
   }
   if (k0 == kts) {
+    // mu_tend: zeroed by rk_tendency, plus relax_bdy_dry's relaxation of mu at rk_step 1
+    // (k_lbc leaves it in mu_save), then rk_addtend_dry's mu_tendf
+    const float mu_rx = ((a.c.specified || a.c.nested) && a.rk_step == 1) ? s.mu_save[X2(d, i, j)] : 0.f;
     s.mu_tend[X2(d, i, j)] = 0.f;
     s.mu_save[X2(d, i, j)] = 0.f;
-    if (mass) s.mu_tend[X2(d, i, j)] = 0.f + s.mu_tendf[X2(d, i, j)];
+    if (mass) s.mu_tend[X2(d, i, j)] = mu_rx + s.mu_tendf[X2(d, i, j)];
   }
 }
 

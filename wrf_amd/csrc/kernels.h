@@ -75,6 +75,30 @@ template <typename A> __global__ void k_pd_tend_x4(A);
 template <typename A> __global__ void k_update_scalar_pd(A);
 template <typename A, int K> __global__ void k_diff6(A);
 template <typename A> __global__ void k_rayleigh(A);
+
+// the outermost mass row of a specified (or nested) domain, which advance_mu_t and
+// advance_w leave to the boundary updates (module_small_step_em.F:1054-1063, 1273-1282)
+DEVI bool spec_frame(const wrfmi_config &c, const Dm &d, int i, int j) {
+  if (!(c.specified || c.nested)) return false;
+  return (!c.periodic_x && (i == d.ids || i == d.ide - 1)) || j == d.jds || j == d.jde - 1;
+}
+
+// specified lateral boundaries (k_lbc.hip)
+enum LbcMode : int { LBC_RELAX, LBC_SPEC_TEND, LBC_UPDATE, LBC_PH_PRHO, LBC_ZERO_GRAD, LBC_FLOW_DEP };
+struct LbcArgs {
+  KArgs a;
+  const float *b[4], *bt[4];  // the field's boundary values and tendencies (xs, xe, ys, ye)
+  float fcx[WRFMI_MAX_BDY_WIDTH], gcx[WRFMI_MAX_BDY_WIDTH];
+  float dtbc, dt;
+  int var;            // staggering: 'u', 'v', 'h' (w levels), 'w', 'm' (2-D), other mass levels
+  int mode;           // LbcMode
+  int massw;          // LBC_RELAX: weight the field by c1 mut + c2 (c1f, c2f for 'h')
+  int whole;          // LBC_RELAX: write the whole target (zero outside the relaxation rows)
+  const float *src;   // LBC_RELAX: the relaxed field
+  const float *tend;  // LBC_UPDATE: the tendency
+  float *dst;         // the array written (LBC_ZERO_GRAD, LBC_FLOW_DEP: also read)
+};
+__global__ void k_lbc(LbcArgs);
 template <typename A, int K> __global__ void k_diff2(A);
 template <typename A> __global__ void k_p_rho_phi(A);
 template <typename A> __global__ void k_p_rho_phi_x4(A);

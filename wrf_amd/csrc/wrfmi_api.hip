@@ -250,10 +250,19 @@ static int check_cfg(const wrfmi_config *c) {
   if (c->h_mom_adv_order != 5 || c->v_mom_adv_order != 3 || c->h_sca_adv_order != 5 || c->v_sca_adv_order != 3)
     return fail(WRFMI_E_BADOPT, "advection orders h/v = %d/%d (momentum), %d/%d (scalar): only 5/3 is implemented",
                 c->h_mom_adv_order, c->v_mom_adv_order, c->h_sca_adv_order, c->v_sca_adv_order);
-  if (c->specified || c->nested || c->polar || c->symmetric_xs || c->symmetric_xe || c->symmetric_ys || c->symmetric_ye)
-    return fail(WRFMI_E_BADOPT, "lateral boundaries: periodic and open are implemented (not specified/nested/polar/symmetric)");
-  if (!c->periodic_x && !(c->open_xs && c->open_xe)) return fail(WRFMI_E_BADOPT, "x boundaries must be periodic or open on both sides");
-  if (!c->periodic_y && !(c->open_ys && c->open_ye)) return fail(WRFMI_E_BADOPT, "y boundaries must be periodic or open on both sides");
+  if (c->nested || c->polar || c->symmetric_xs || c->symmetric_xe || c->symmetric_ys || c->symmetric_ye)
+    return fail(WRFMI_E_BADOPT, "lateral boundaries: periodic, open and specified are implemented (not nested/polar/symmetric)");
+  if (c->specified) {
+    if (c->periodic_y || c->open_xs || c->open_xe || c->open_ys || c->open_ye)
+      return fail(WRFMI_E_BADOPT, "specified boundaries hold all four sides (or the y sides with a periodic x)");
+    if (c->spec_zone != 1 || c->relax_zone < 2 || c->spec_bdy_width < c->spec_zone + c->relax_zone ||
+        c->spec_bdy_width > WRFMI_MAX_BDY_WIDTH)
+      return fail(WRFMI_E_BADOPT, "specified boundaries: spec_zone 1, relax_zone >= 2 and spec_bdy_width >= spec_zone + "
+                  "relax_zone (at most %d) are implemented", WRFMI_MAX_BDY_WIDTH);
+  } else {
+    if (!c->periodic_x && !(c->open_xs && c->open_xe)) return fail(WRFMI_E_BADOPT, "x boundaries must be periodic or open on both sides");
+    if (!c->periodic_y && !(c->open_ys && c->open_ye)) return fail(WRFMI_E_BADOPT, "y boundaries must be periodic or open on both sides");
+  }
   if (c->moist_adv_opt < 0 || c->moist_adv_opt > 1 || c->scalar_adv_opt < 0 || c->scalar_adv_opt > 1 ||
       c->momentum_adv_opt != 1)
     return fail(WRFMI_E_BADOPT, "advection options: the original fluxes and positive-definite scalar transport "
@@ -348,13 +357,13 @@ enum KernelId {
   KID_STEP_PREP, KID_TEND_U, KID_TEND_V, KID_TEND_W, KID_TEND_T, KID_SMALL_PREP, KID_ADVANCE_UV,
   KID_AC_DIV, KID_SMALL_FINISH, KID_ADV_SCALAR, KID_UPDATE_SCALAR, KID_P_RHO_PHI, KID_HALO, KID_P_RHO0,
   KID_COEF_W, KID_WW_SCAN, KID_AC_MU_WW, KID_AC_THETA, KID_AC_WSOLVE, KID_W_SURFACE, KID_PD_SCALE, KID_PD_TEND,
-  KID_UPDATE_SCALAR_PD, KID_DIFF6, KID_RAYLEIGH, KID_DIFF2, KID_COUNT
+  KID_UPDATE_SCALAR_PD, KID_DIFF6, KID_RAYLEIGH, KID_DIFF2, KID_LBC, KID_COUNT
 };
 static const char *kKernelNames[KID_COUNT] = {
   "k_step_prep", "k_tend_u", "k_tend_v", "k_tend_w", "k_tend_t", "k_small_prep", "k_advance_uv",
   "k_ac_div", "k_small_finish", "k_adv_scalar", "k_update_scalar", "k_p_rho_phi", "k_halo", "k_p_rho0",
   "k_coef_w", "k_ww_scan", "k_ac_mu_ww", "k_ac_theta", "k_ac_wsolve", "k_set_w_surface", "k_pd_scale", "k_pd_tend",
-  "k_update_scalar_pd", "k_diff6", "k_rayleigh", "k_diff2"};
+  "k_update_scalar_pd", "k_diff6", "k_rayleigh", "k_diff2", "k_lbc"};
 struct EvPair { int id; hipEvent_t a, b; };
 static unsigned g_timing = 0;  // bit n set: time launches of kernel n
 static std::vector<EvPair> g_ev_pending;
@@ -420,6 +429,8 @@ struct Launcher {
   bool scal_all = false;     // species advected and updated by the all-species kernels
   bool zone_ok = false;      // each open direction is wide enough for a boundary zone on each side
   bool whole_step = false;   // enqueued by the whole-step driver (work arrays stay internal)
+  bool lbc_on = false;       // specified boundaries: lbc holds the boundary arrays' device addresses
+  wrfmi_lbc lbc;
   // launch with an optional event pair around it (kernel timing mode)
   template <typename K, typename... A>
   void launch(int id, K kern, dim3 g, dim3 b, A... args) {
@@ -488,7 +499,7 @@ static Shape default_shape(int kid) {
 // which of a kernel's shape parameters are tunable, and the values each may take
 static bool tunable_rows(int kid) {
   return kid != KID_HALO && kid != KID_W_SURFACE && kid != KID_PD_SCALE && kid != KID_PD_TEND && kid != KID_UPDATE_SCALAR_PD &&
-         kid != KID_DIFF6 && kid != KID_RAYLEIGH && kid != KID_DIFF2;
+         kid != KID_DIFF6 && kid != KID_RAYLEIGH && kid != KID_DIFF2 && kid != KID_LBC;
 }
 static bool tunable_levels(int kid) { return kid == KID_ADVANCE_UV || kid == KID_AC_DIV; }
 // the four-point stencil kernels built for workgroups of up to kTallBlk threads
@@ -599,6 +610,12 @@ static inline dim3 g_vec3(const KArgs &a, int nk) {
 static inline dim3 vblk(const Launcher &L, int kid) { return dim3(64, (unsigned)L.sh[kid].rows, 1); }
 static inline dim3 gvec(const Launcher &L, int kid, int nk) { return g_vec(L.a, vec_ia0(L.a.d), 0, L.sh[kid].rows, nk); }
 
+// whether species m takes boundary arrays (solve_em.F:2261-2262, 2808-2810)
+static bool species_bcs(const wrfmi_config &c, const wrfmi_state &s, int m) {
+  if (c.nested) return true;
+  if (m == 0 && s.n_moist > 0) return true;
+  return m < s.n_moist ? c.have_bcs_moist : c.have_bcs_scalar;
+}
 static void ph_step_prep(Launcher &L) {
   const KArgs &a = L.a;
   const wrfmi_state &s = a.s;
@@ -655,8 +672,58 @@ static void ph_scalar_mixing(Launcher &L) {
   if (L.a.rk_step == 1 && L.a.c.diff_6th_opt != 0)
     L.launchK(KID_DIFF6, KRP(k_diff6, 4), g_pt3(L.a.d, L.a.d.kde - L.a.d.kds), kColBlk, L.a);
 }
+// k_lbc over the columns of the domain (see k_lbc.hip for the modes)
+static void lbc_launch(Launcher &L, int mode, int var, float *dst, const float *src, const float *tend, float dt,
+                       float *const *b, float *const *bt, int massw, int whole) {
+  LbcArgs A;
+  memset(&A, 0, sizeof A);
+  A.a = L.a;
+  for (int n = 0; n < 4; n++) {
+    A.b[n] = b ? b[n] : nullptr;
+    A.bt[n] = bt ? bt[n] : nullptr;
+  }
+  memcpy(A.fcx, L.lbc.fcx, sizeof A.fcx);
+  memcpy(A.gcx, L.lbc.gcx, sizeof A.gcx);
+  A.dtbc = L.lbc.dtbc;
+  A.dt = dt;
+  A.var = var;
+  A.mode = mode;
+  A.massw = massw;
+  A.whole = whole;
+  A.src = src;
+  A.tend = tend;
+  A.dst = dst;
+  const Dm &d = L.a.d;
+  L.launch(KID_LBC, k_lbc, dim3((d.ide - d.ids + 1 + 63) / 64, (d.jde - d.jds + 1 + 3) / 4, 1), dim3(64, 4, 1), A);
+}
+// relax_bdy_dry at rk_step 1 (solve_em.F:937-962), into the *_save arrays that k_tend_*
+// add to the forward tendencies (rk_addtend_dry) and, for mu, into mu_save (k_tend_t adds it
+// to mu_tend); every save array is written whole, as rk_tendency zeroes them
+static void lbc_relax_dry(Launcher &L) {
+  const wrfmi_state &s = L.a.s;
+  wrfmi_lbc &B = L.lbc;
+  const bool nest = L.a.c.nested;
+  lbc_launch(L, LBC_RELAX, 'u', s.u_save, s.ru, nullptr, 0.f, B.u_b, B.u_bt, 0, 1);
+  lbc_launch(L, LBC_RELAX, 'v', s.v_save, s.rv, nullptr, 0.f, B.v_b, B.v_bt, 0, 1);
+  lbc_launch(L, LBC_RELAX, 'h', s.ph_save, s.ph_2, nullptr, 0.f, B.ph_b, B.ph_bt, 1, 1);
+  lbc_launch(L, LBC_RELAX, 't', s.t_save, s.t_2, nullptr, 0.f, B.t_b, B.t_bt, 1, 1);
+  lbc_launch(L, LBC_RELAX, 'm', s.mu_save, s.mu_2, nullptr, 0.f, B.mu_b, B.mu_bt, 0, 1);
+  lbc_launch(L, LBC_RELAX, 'h', s.w_save, s.w_2, nullptr, 0.f, nest ? B.w_b : nullptr, nest ? B.w_bt : nullptr, 1, 1);
+}
+// spec_bdy_dry after rk_addtend_dry (solve_em.F:982-1008)
+static void lbc_spec_dry(Launcher &L) {
+  const wrfmi_state &s = L.a.s;
+  wrfmi_lbc &B = L.lbc;
+  lbc_launch(L, LBC_SPEC_TEND, 'u', s.ru_tend, nullptr, nullptr, 0.f, nullptr, B.u_bt, 0, 0);
+  lbc_launch(L, LBC_SPEC_TEND, 'v', s.rv_tend, nullptr, nullptr, 0.f, nullptr, B.v_bt, 0, 0);
+  lbc_launch(L, LBC_SPEC_TEND, 'h', s.ph_tend, nullptr, nullptr, 0.f, nullptr, B.ph_bt, 0, 0);
+  lbc_launch(L, LBC_SPEC_TEND, 't', s.t_tend, nullptr, nullptr, 0.f, nullptr, B.t_bt, 0, 0);
+  lbc_launch(L, LBC_SPEC_TEND, 'm', s.mu_tend, nullptr, nullptr, 0.f, nullptr, B.mu_bt, 0, 0);
+  if (L.a.c.nested) lbc_launch(L, LBC_SPEC_TEND, 'h', s.rw_tend, nullptr, nullptr, 0.f, nullptr, B.w_bt, 0, 0);
+}
 static void ph_tendency(Launcher &L) {
   if (L.a.rk_step == 1) ph_mixing(L);
+  if (L.lbc_on && L.a.rk_step == 1) lbc_relax_dry(L);
   const KArgs &a = L.a;
   const int nk = a.d.kde - a.d.kds + 1, nkt = (nk + kTendK - 1) / kTendK;
   const bool per = a.c.periodic_x && a.c.periodic_y;
@@ -681,6 +748,7 @@ static void ph_tendency(Launcher &L) {
   L.launchK(KID_TEND_V, per ? KRP(k_tend_v, true) : KRP(k_tend_v, false), gt, kColBlk, a);
   L.launchK(KID_TEND_W, per ? KRP(k_tend_w, true) : KRP(k_tend_w, false), gt, kColBlk, a);
   L.launchK(KID_TEND_T, per ? KRP(k_tend_t, true) : KRP(k_tend_t, false), gt, kColBlk, a);
+  if (L.lbc_on) lbc_spec_dry(L);
 }
 static void ph_small_prep(Launcher &L) {
   const KArgs &a = L.a;
@@ -720,6 +788,11 @@ static void ph_advance_uv(Launcher &L) {
     L.launchK(KID_ADVANCE_UV, k, gvec(L, KID_ADVANCE_UV, nkv), vblk(L, KID_ADVANCE_UV), L.a);
   } else
     L.launchK(KID_ADVANCE_UV, per ? KRP(k_advance_uv, true) : KRP(k_advance_uv, false), g_pt3(L.a.d, nkb), kColBlk, L.a);
+  if (L.lbc_on) {  // spec_bdyupdate of u and v (solve_em.F:1343-1365)
+    const wrfmi_state &s = L.a.s;
+    lbc_launch(L, LBC_UPDATE, 'u', s.u_2, nullptr, s.ru_tend, L.a.dts, nullptr, nullptr, 0, 0);
+    lbc_launch(L, LBC_UPDATE, 'v', s.v_2, nullptr, s.rv_tend, L.a.dts, nullptr, nullptr, 0, 0);
+  }
 }
 static void ph_advance_mu_t_w(Launcher &L) {
   const KArgs &a = L.a;
@@ -736,12 +809,24 @@ static void ph_advance_mu_t_w(Launcher &L) {
   else L.launchK(KID_AC_MU_WW, KRP(k_ac_mu_ww, false), gcol(L, KID_AC_MU_WW), cblk(L, KID_AC_MU_WW), a);
   if (L.vec) L.launchK(KID_AC_THETA, KR(k_ac_theta_x4), gvec(L, KID_AC_THETA, nk), vblk(L, KID_AC_THETA), a);
   else L.launchK(KID_AC_THETA, KR(k_ac_theta), g_pt3(a.d, nk), kColBlk, a);
+  if (L.lbc_on) {  // spec_bdyupdate of theta, mu and muts (solve_em.F:1459-1490)
+    lbc_launch(L, LBC_UPDATE, 't', s.t_2, nullptr, s.t_tend, a.dts, nullptr, nullptr, 0, 0);
+    lbc_launch(L, LBC_UPDATE, 'm', s.mu_2, nullptr, s.mu_tend, a.dts, nullptr, nullptr, 0, 0);
+    lbc_launch(L, LBC_UPDATE, 'm', s.muts, nullptr, s.mu_tend, a.dts, nullptr, nullptr, 0, 0);
+  }
   {
     const int pf = L.sh[KID_AC_WSOLVE].prefetch;
     KRef k;
     if (p64) k = pf == 2 ? KRWS(double, 2) : pf == 8 ? KRWS(double, 8) : KRWS(double, 4);
     else k = pf == 2 ? KRWS(float, 2) : pf == 8 ? KRWS(float, 8) : KRWS(float, 4);
     L.launchK(KID_AC_WSOLVE, k, gcol(L, KID_AC_WSOLVE), cblk(L, KID_AC_WSOLVE), a);
+  }
+  if (L.lbc_on && a.c.non_hydrostatic) {
+    // spec_bdyupdate_ph, zero_grad_bdy of w (solve_em.F:1583-1620), and calc_p_rho on the
+    // spec-zone columns k_ac_wsolve left out
+    lbc_launch(L, LBC_PH_PRHO, 't', s.ph_2, nullptr, nullptr, a.dts, nullptr, nullptr, 0, 0);
+    if (a.c.specified) lbc_launch(L, LBC_ZERO_GRAD, 'w', s.w_2, nullptr, nullptr, 0.f, nullptr, nullptr, 0, 0);
+    else lbc_launch(L, LBC_UPDATE, 'h', s.w_2, nullptr, s.rw_tend, a.dts, nullptr, nullptr, 0, 0);
   }
   L.halo_begin();
   L.halo_add(s.ph_2, 'w', 1); L.halo_add(s.al, 'p', 1); L.halo_add(s.p, 'p', 1);
@@ -831,10 +916,19 @@ static void ph_scalar_tend(Launcher &L) {
 }
 static void ph_scalar(Launcher &L) {
   ph_scalar_tend(L);
+  const int m = L.a.species;
+  const wrfmi_state &s = L.a.s;
+  const bool bcs = L.lbc_on && species_bcs(L.a.c, s, m);
+  if (bcs && L.a.rk_step == 1) {  // relax_bdy_scalar, spec_bdy_scalar (solve_em.F:2261-2296)
+    lbc_launch(L, LBC_RELAX, 'q', s.q_tend[m], s.q[m], nullptr, 0.f, L.lbc.q_b[m], L.lbc.q_bt[m], 1, 0);
+    lbc_launch(L, LBC_SPEC_TEND, 'q', s.q_tend[m], nullptr, nullptr, 0.f, nullptr, L.lbc.q_bt[m], 0, 0);
+  }
   if (L.vec)
     L.launchK(KID_UPDATE_SCALAR, KR(k_update_scalar_x4), gvec(L, KID_UPDATE_SCALAR, L.a.d.kde - L.a.d.kds),
               vblk(L, KID_UPDATE_SCALAR), L.a);
   else L.launchK(KID_UPDATE_SCALAR, KR(k_update_scalar), g_pt3(L.a.d, L.a.d.kde - L.a.d.kds), kColBlk, L.a);
+  // flow_dep_bdy of a species without boundary arrays (solve_em.F:2342-2355, 2886-2930)
+  if (L.a.c.specified && !bcs) lbc_launch(L, LBC_FLOW_DEP, 'q', s.q[m], nullptr, nullptr, 0.f, nullptr, nullptr, 0, 0);
 }
 // every species' rk_scalar_tend and rk_update_scalar (species are independent: the order
 // over species does not change any value)
@@ -970,6 +1064,28 @@ static int check_coef_arrays(const wrfmi_state &s, const wrfmi_config &c, const 
 // domain's own (get_scratch) unless own_scratch is given: ensemble members each have theirs,
 // and the allocation-free entry point passes the static array.  With own_scratch and no
 // own_qt, the species are advected one by one (no species work arrays).
+// the state's boundary arrays at their device addresses
+static int translate_lbc(const wrfmi_state *st, const wrfmi_config &c, wrfmi_lbc *out) {
+  if (!st->lbc) return fail(WRFMI_E_POINTER, "specified boundaries: the state's lbc (boundary arrays) is null");
+  *out = *st->lbc;
+  auto tr = [&](float *&p, const char *name, int side) -> int {
+    p = p ? (float *)translate((void *)p) : nullptr;
+    if (!p) return fail(WRFMI_E_POINTER, "specified boundaries: %s[%d] is null or not device memory", name, side);
+    return 0;
+  };
+  int rc;
+  for (int sd = 0; sd < 4; sd++) {
+#define TRL(n) if ((rc = tr(out->n##_b[sd], #n "_b", sd)) || (rc = tr(out->n##_bt[sd], #n "_bt", sd))) return rc;
+    TRL(u) TRL(v) TRL(ph) TRL(t) TRL(mu)
+    if (c.nested) { TRL(w) }
+#undef TRL
+    for (int m = 0; m < st->n_species; m++)
+      if (species_bcs(c, *st, m))
+        if ((rc = tr(out->q_b[m][sd], "q_b", sd)) || (rc = tr(out->q_bt[m][sd], "q_bt", sd))) return rc;
+  }
+  return 0;
+}
+
 static int prepare(const wrfmi_state *st_in, const wrfmi_config *cfg, const wrfmi_dims *dims, Launcher &L, wrfmi_state &s,
                    const Scratch *own_scratch = nullptr, float *own_qt = nullptr, bool default_shapes = false) {
   int rc;
@@ -992,13 +1108,19 @@ static int prepare(const wrfmi_state *st_in, const wrfmi_config *cfg, const wrfm
   L.a.scr_lo = FPtr<float>(sc.lo);
   L.h.d = d;
   L.h.periodic_x = cfg->periodic_x; L.h.periodic_y = cfg->periodic_y;
-  L.h.open_xs = cfg->open_xs; L.h.open_xe = cfg->open_xe; L.h.open_ys = cfg->open_ys; L.h.open_ye = cfg->open_ye;
+  // set_physical_bc3d/2d fill a specified side as an open one (share/module_bc.F:217-242)
+  const int spx = cfg->specified && !cfg->periodic_x, spy = cfg->specified;
+  L.h.open_xs = cfg->open_xs || spx; L.h.open_xe = cfg->open_xe || spx;
+  L.h.open_ys = cfg->open_ys || spy; L.h.open_ye = cfg->open_ye || spy;
+  L.lbc_on = cfg->specified || cfg->nested;
+  if (L.lbc_on && (rc = translate_lbc(st_in, *cfg, &L.lbc))) return rc;
   // ensemble members are recorded with the default shapes (one launch sequence for all)
   for (int k = 0; k < KID_COUNT; k++) L.sh[k] = default_shapes ? default_shape(k) : shape_for(k, d);
   L.vec = vec_layout(s, d, L.a);
   // the all-species scalar kernels (doubly periodic, aligned layout, more than one species)
   L.scal_all = L.vec && cfg->periodic_x && cfg->periodic_y && s.n_species > 1 && (own_qt || !own_scratch);
   L.zone_ok = (cfg->periodic_x || d.ide - d.ids + 1 >= 2 * kZone + 2) && (cfg->periodic_y || d.jde - d.jds + 1 >= 2 * kZone + 2);
+  if (L.lbc_on) L.vec = L.scal_all = L.zone_ok = false;  // the one-point kernels carry the spec-zone bounds
   if (L.scal_all) {
     float *qt = own_qt;
     if (!qt && (rc = get_qt(key, s.n_species, &qt))) return rc;
@@ -1006,6 +1128,13 @@ static int prepare(const wrfmi_state *st_in, const wrfmi_config *cfg, const wrfm
     L.a.qt_stride = field_elems(d);
   }
   return 0;
+}
+
+// solve_em.F:377: the step starts by advancing grid%dtbc by dt (the host's boundary struct)
+static void lbc_begin_step(Launcher &L, wrfmi_state *st) {
+  if (!L.lbc_on) return;
+  st->lbc->dtbc = st->lbc->dtbc + st->dt;
+  L.lbc.dtbc = st->lbc->dtbc;
 }
 
 extern "C" int wrfmi_solve_em_step(wrfmi_state *st, const wrfmi_config *cfg, const wrfmi_dims *dims, void *stream, int sync) {
@@ -1016,6 +1145,7 @@ extern "C" int wrfmi_solve_em_step(wrfmi_state *st, const wrfmi_config *cfg, con
   {
     std::lock_guard<std::mutex> lk(g_mu);
     if ((rc = prepare(st, cfg, dims, L, s))) return rc;
+    lbc_begin_step(L, st);
   }
   L.st = (hipStream_t)stream;
   enqueue_step(L);
@@ -1073,6 +1203,7 @@ extern "C" int wrfmi_solve_em_step_noalloc(wrfmi_state *st, const wrfmi_config *
     if ((rc = noalloc_scratch(&p))) return rc;
     const Scratch sc{p, p + kNoAllocElems, p + 2 * kNoAllocElems};
     if ((rc = prepare(st, cfg, dims, L, s, &sc, nullptr))) return rc;
+    lbc_begin_step(L, st);
   }
   L.st = (hipStream_t)stream;
   enqueue_step(L);
@@ -1100,6 +1231,9 @@ static void free_graphs() {
 
 extern "C" int wrfmi_solve_em_step_graph(wrfmi_state *st, const wrfmi_config *cfg, const wrfmi_dims *dims, void *stream, int sync) {
   if (!st || !cfg || !dims) return fail(WRFMI_E_POINTER, "null argument");
+  if (cfg->specified || cfg->nested)
+    return fail(WRFMI_E_BADOPT, "specified boundaries: dtbc changes every step, so the step is not replayed as a graph "
+                "(use wrfmi_solve_em_step)");
   if (g_timing) return fail(WRFMI_E_BADOPT, "kernel timing mode records events on eager launches only");
   hipStream_t hs = (hipStream_t)stream;
   GraphEntry *hit = nullptr;
@@ -1181,6 +1315,9 @@ extern "C" int wrfmi_ensemble_create(int n_members, const wrfmi_state *states, c
                                      const wrfmi_dims *dims, wrfmi_ensemble **out) {
   if (!states || !cfgs || !dims || !out) return fail(WRFMI_E_POINTER, "null argument");
   if (n_members < 1) return fail(WRFMI_E_BADOPT, "an ensemble needs at least one member");
+  for (int m = 0; m < n_members; m++)
+    if (cfgs[m].specified || cfgs[m].nested)
+      return fail(WRFMI_E_BADOPT, "member %d: specified boundaries are stepped one domain at a time (wrfmi_solve_em_step)", m);
   *out = nullptr;
   std::lock_guard<std::mutex> lk(g_mu);
   wrfmi_ensemble *e = new wrfmi_ensemble;

@@ -54,6 +54,13 @@ struct Call {
     int *p = &w.ids;
     for (int n = 0; n < 18; n++) p[n] = *dims[n];
     if ((rc = check_cfg(&c))) return;
+    // the boundary-zone updates between these routines (spec_bdyupdate, relax_bdy_dry, ...)
+    // run inside the state-struct entry points
+    if (c.specified || c.nested) {
+      rc = fail(WRFMI_E_BADOPT, "specified boundaries: use the state-struct entry points (wrfmi_solve_em_step, "
+                "wrfmi_advance_uv_state, ...), which carry the boundary arrays");
+      return;
+    }
     rc = make_dm(&w, &d);
   }
   // bind array argument `name` to a state slot (device address after registry translation)

@@ -98,3 +98,7 @@ State64 = make_state_struct(ctypes.c_double, "State64")
 
 def state_field_count() -> int:
     return len(FIELDS_3D) + len(FIELDS_2D) + len(FIELDS_1D)
+
+
+Lbc32 = make_lbc_struct(ctypes.c_float, "Lbc32")
+

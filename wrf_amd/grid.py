@@ -342,7 +342,8 @@ class HostState:
         other.a, other.alpha, other.gamma = self.a.copy(), self.alpha.copy(), self.gamma.copy()
         other.lbc = None
         if self.lbc is not None:
-            other.lbc = {k: ([[a.copy() for a in sides] for sides in v] if k in ("q_b", "q_bt")
+            other.lbc = {k: ([None if sides is None else [a.copy() for a in sides] for sides in v]
+                             if k in ("q_b", "q_bt")
                              else [a.copy() for a in v] if isinstance(v, list) else np.array(v, copy=True))
                          for k, v in self.lbc.items()}
         return other

@@ -16,7 +16,7 @@ import os
 
 import numpy as np
 
-from .fields import FIELDS_1D, FIELDS_2D, FIELDS_3D, SCALARS_R, Config, Dims, State32
+from .fields import FIELDS_1D, FIELDS_2D, FIELDS_3D, SCALARS_R, Config, Dims, Lbc32, State32
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libwrfmi.so")
@@ -228,6 +228,11 @@ class DeviceDomain:
             st.q[m] = ctypes.cast(self.t[f"q{m}"].data_ptr(), P)
             st.q_old[m] = ctypes.cast(self.t[f"q_old{m}"].data_ptr(), P)
             st.q_tend[m] = ctypes.cast(self.t[f"q_tend{m}"].data_ptr(), P)
+        if hs.lbc is not None:
+            # the lateral boundary arrays of a specified domain (the struct lives in host memory)
+            from .grid import lbc_struct
+            self.lbc = lbc_struct(hs.lbc, Lbc32, lambda n: self.t[n].data_ptr())
+            st.lbc = ctypes.addressof(self.lbc)
         self.state = st
         self.scal = dict(hs.scal)
 
