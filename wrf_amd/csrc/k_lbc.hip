@@ -1,6 +1,8 @@
 // Specified lateral boundaries (the boundary-zone routines solve_em calls when
-// config_flags%specified): one thread per (i, j) column of a field's staggering, marching
-// its levels.  Modes:
+// config_flags%specified).  k_lbc: one thread per point (i, j, k) of the frame of rows
+// along the edges that the mode touches (of the whole extent for the relaxation into the
+// save arrays), up to six fields per launch (blockIdx.z); k_lbc_ph: one workgroup per
+// spec-zone column.  Modes:
 //   LBC_RELAX      relax_bdytend (share/module_bc.F:1221-1427) of ru, rv, ph or t (mass
 //                  weighted as relax_bdy_dry's mass_weight does, dyn_em/module_bc_em.F:
 //                  1716-1745), mu or a species into the target: the *_save arrays (written
@@ -11,6 +13,7 @@
 //   LBC_UPDATE     spec_bdyupdate (1955-2064): field += dt * tendency in the spec zone
 //   LBC_PH_PRHO    spec_bdyupdate_ph (dyn_em/module_bc_em.F:17-157) on the spec-zone
 //                  columns, then calc_p_rho there (k_ac_wsolve leaves these columns alone)
+//                  (k_lbc_ph, which also takes zero_grad_bdy of w)
 //   LBC_ZERO_GRAD  zero_grad_bdy of w (share/module_bc.F:2219-2332)
 //   LBC_FLOW_DEP   flow_dep_bdy of a species (2335-2456)
 // Zone membership follows the reference's loops with the tile equal to the domain: side
@@ -67,11 +70,135 @@ DEVI float lbc_at(const Dm &d, const float *arr, int side, int ij, int k, int b,
   return arr[((long)(b - 1) * nkb + kk) * n + o];
 }
 
+__global__ void __launch_bounds__(256) k_lbc(LbcArgs L) {
+  const KArgs &a = L.a;
+  const Dm &d = a.d;
+  const DState &s = dstate(a);
+  const LbcJob &J = L.job[blockIdx.z];
+  const LbcGeo z = lbc_geo(d, J.var);
+  const bool px = a.c.periodic_x, twod = J.var == 'm';
+  const int k = d.kds + (int)blockIdx.y;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  int i, j;
+  if (L.whole) {
+    // a whole-domain target is written over zero_tend's extent (ids..ide, jds..jde, every
+    // level); everything else stays inside the variable's staggering
+    const int ni = d.ide - d.ids + 1;
+    if (t >= (long)ni * (d.jde - d.jds + 1)) return;
+    i = d.ids + (int)(t % ni);
+    j = d.jds + (int)(t / ni);
+  } else {
+    const LbcRing r = lbc_ring(z.ibs, z.ibe, z.jbs, z.jbe, px, L.width);
+    if (t >= r.count) return;
+    const long ny = (long)(r.rb + r.rt) * r.ni;
+    if (t < ny) {
+      const int row = (int)(t / r.ni);
+      i = z.ibs + (int)(t % r.ni);
+      j = row < r.rb ? z.jbs + row : z.jbe - (row - r.rb);
+    } else {
+      const int nc = r.cl + r.cr, row = (int)((t - ny) / nc), c = (int)((t - ny) % nc);
+      j = z.jbs + r.rb + row;
+      i = c < r.cl ? z.ibs + c : z.ibe - (c - r.cl);
+    }
+  }
+  const bool inside = i <= z.ibe && j <= z.jbe;
+  const int sz = a.c.spec_zone, rz = a.c.relax_zone;
+  const Off o2 = X2(d, i, j);
+  // element offsets are byte offsets on a base pointer (FPtr)
+  const FPtr<float> SRC((float *)J.src), TEND((float *)J.tend);
+  FPtr<float> DST(J.dst);
+  auto at = [&](const FPtr<float> &p, int ii, int kk, int jj) -> float {
+    return twod ? p[X2(d, ii, jj)] : p[X3(d, ii, kk, jj)];
+  };
+  if (L.mode == LBC_RELAX) {
+    if (k > (L.whole && !twod ? d.kde : z.ktf)) return;
+    // the relaxed field at a point: ru, rv, mu, or the mass-weighted ph, t, species
+    auto fld = [&](int ii, int kk, int jj) -> float {
+      const float f = at(SRC, ii, kk, jj);
+      if (!J.massw) return f;
+      const float c1 = J.var == 'h' ? s.c1f[X1(d, kk)] : s.c1h[X1(d, kk)];
+      const float c2 = J.var == 'h' ? s.c2f[X1(d, kk)] : s.c2h[X1(d, kk)];
+      return f * (c1 * s.mut[X2(d, ii, jj)] + c2);
+    };
+    float tv = L.whole ? 0.f : at(DST, i, k, j);
+    bool any = false;
+    int b, side = -1;
+    while (inside && k <= z.ktf && J.b[0] && (side = lbc_side(z, px, i, j, sz, rz - 1, side, &b)) >= 0) {
+      auto fls = [&](int ij, int bb, int fi, int fj) -> float {
+        return lbc_at(d, J.b[side], side, ij, k, bb, twod) + L.dtbc * lbc_at(d, J.bt[side], side, ij, k, bb, twod) -
+               fld(fi, k, fj);
+      };
+      float f0, f1, f2, f3, f4;
+      if (side >= 2) {
+        const int im1 = imax(i - 1, z.ibs), ip1 = imin(i + 1, z.ibe), dj = side == 2 ? 1 : -1;
+        f0 = fls(i, b + 1, i, j);
+        f1 = fls(im1, b + 1, im1, j);
+        f2 = fls(ip1, b + 1, ip1, j);
+        f3 = fls(i, b, i, j - dj);
+        f4 = fls(i, b + 2, i, j + dj);
+      } else {
+        const int di = side == 0 ? 1 : -1;
+        f0 = fls(j, b + 1, i, j);
+        f1 = fls(j - 1, b + 1, i, j - 1);
+        f2 = fls(j + 1, b + 1, i, j + 1);
+        f3 = fls(j, b, i - di, j);
+        f4 = fls(j, b + 2, i + di, j);
+      }
+      tv = tv + L.fcx[b] * f0 - L.gcx[b] * (f1 + f2 + f3 + f4 - 4.f * f0);
+      any = true;
+    }
+    if (any || L.whole) {
+      if (twod) DST[o2] = tv;
+      else DST[X3(d, i, k, j)] = tv;
+    }
+    return;
+  }
+  if (!inside || k > z.ktf) return;
+  if (L.mode == LBC_FLOW_DEP || L.mode == LBC_ZERO_GRAD) {
+    int b, side = -1, last = -1;
+    while ((side = lbc_side(z, px, i, j, 0, sz - 1, side, &b)) >= 0) last = side;
+    if (last < 0) return;
+    int si, sj;
+    if (last >= 2) {
+      si = imin(imax(i, z.ibs + sz), z.ibe - sz);
+      if (px) si = i;
+      sj = last == 2 ? z.jbs + sz : z.jbe - sz;
+    } else {
+      sj = imin(imax(j, z.jbs + sz), z.jbe - sz);
+      si = last == 0 ? z.ibs + sz : z.ibe - sz;
+    }
+    const float inner = DST[X3(d, si, k, sj)];
+    float v = inner;
+    if (L.mode == LBC_FLOW_DEP) {
+      const FPtr<float> u = s.ru_m, vv = s.rv_m;
+      const bool out = last == 2 ? vv[X3(d, i, k, j)] < 0.f : last == 3 ? vv[X3(d, i, k, j + 1)] > 0.f
+                     : last == 0 ? u[X3(d, i, k, j)] < 0.f : u[X3(d, i + 1, k, j)] > 0.f;
+      v = out ? inner : 0.f;
+    }
+    DST[X3(d, i, k, j)] = v;
+    return;
+  }
+  // LBC_SPEC_TEND, LBC_UPDATE: the spec zone (later sides overwrite earlier)
+  int b = 0, side = -1, last = -1, lb = 0;
+  while ((side = lbc_side(z, px, i, j, 0, sz - 1, side, &b)) >= 0) {
+    last = side;
+    lb = b;
+    if (L.mode == LBC_UPDATE) {  // each side's loop adds dt * tendency once
+      if (twod) DST[o2] = DST[o2] + L.dt * TEND[o2];
+      else DST[X3(d, i, k, j)] = DST[X3(d, i, k, j)] + L.dt * TEND[X3(d, i, k, j)];
+    }
+  }
+  if (last < 0 || L.mode == LBC_UPDATE) return;
+  const float v = lbc_at(d, J.bt[last], last, last < 2 ? j : i, k, lb + 1, twod);
+  if (twod) DST[o2] = v;
+  else DST[X3(d, i, k, j)] = v;
+}
+
 // calc_coef_w's a, alpha and gamma (module_small_step_em.F:570-652) of column (i, j) at the
-// width CT, in k_coef_w's expressions: the whole step forms them inside k_ac_wsolve, which
-// leaves the spec-zone columns to this kernel
+// width CT, in k_coef_w's expressions, over the workgroup's lanes: every level's
+// coefficients at once, the forward recurrence on lane 0
 template <typename CT>
-DEVI void coef_w_column(const KArgs &a, const DState &s, int i, int j) {
+DEVI void coef_w_column(const KArgs &a, const DState &s, int i, int j, CT *sa, CT *sb, CT *sc) {
   const Dm &d = a.d;
   const int kde = d.kde;
   const CPtr<float> c1h = s.c1h, c2h = s.c2h, c1f = s.c1f, c2f = s.c2f;
@@ -87,164 +214,92 @@ DEVI void coef_w_column(const KArgs &a, const DState &s, int i, int j) {
   auto rdn = [&](int k) -> CT { return (CT)s.rdn[X1(d, k)]; };
   auto C2A = [&](int k) -> CT { return (CT)s.c2a[X3(d, i, k, j)]; };
   auto CQW = [&](int k) -> CT { return (CT)s.cqw[X3(d, i, k, j)]; };
-  {
-    const int k = kde - 1;
-    A[X3(d, i, 2, j)] = CT(0);
-    A[X3(d, i, kde, j)] = -CT(2) * cof * (rdnw(kde - 1) * rdnw(kde - 1)) * C2A(kde - 1) * lid / (cH(k) * cF(k));
-    GA[X3(d, i, 1, j)] = CT(0);
+  for (int k = 2 + (int)threadIdx.x; k <= kde; k += blockDim.x) {
+    if (k == kde) {
+      const int km = kde - 1;
+      const CT top = -CT(2) * cof * (rdnw(kde - 1) * rdnw(kde - 1)) * C2A(kde - 1) * lid / (cH(km) * cF(km));
+      A[X3(d, i, kde, j)] = top;
+      sa[k] = top;
+      sb[k] = CT(1) + CT(2) * cof * (rdnw(kde - 1) * rdnw(kde - 1)) * C2A(kde - 1) / (cH(k - 1) * cF(k));
+    } else {
+      const CT ak = k >= 3 ? -CQW(k) * cof * rdn(k) * rdnw(k - 1) * C2A(k - 1) / (cH(k - 1) * cF(k - 1)) : CT(0);
+      A[X3(d, i, k, j)] = ak;  // A(2) = 0
+      sa[k] = ak;
+      sb[k] = CT(1) + CQW(k) * cof * rdn(k) *
+                          (rdnw(k) * C2A(k) / (cH(k) * cF(k)) + rdnw(k - 1) * C2A(k - 1) / (cH(k - 1) * cF(k)));
+      sc[k] = -CQW(k) * cof * rdn(k) * rdnw(k) * C2A(k) / (cH(k) * cF(k + 1));
+    }
   }
-  CT gam = CT(0);
-  for (int k = 2; k <= kde - 1; k++) {
-    const CT ak = k >= 3 ? -CQW(k) * cof * rdn(k) * rdnw(k - 1) * C2A(k - 1) / (cH(k - 1) * cF(k - 1)) : CT(0);
-    if (k >= 3) A[X3(d, i, k, j)] = ak;
-    const CT b = CT(1) + CQW(k) * cof * rdn(k) *
-                             (rdnw(k) * C2A(k) / (cH(k) * cF(k)) + rdnw(k - 1) * C2A(k - 1) / (cH(k - 1) * cF(k)));
-    const CT c = -CQW(k) * cof * rdn(k) * rdnw(k) * C2A(k) / (cH(k) * cF(k + 1));
-    const CT alpha = CT(1) / (b - ak * gam);
-    AL[X3(d, i, k, j)] = alpha;
-    gam = c * alpha;
-    GA[X3(d, i, k, j)] = gam;
+  if (threadIdx.x == 0) GA[X3(d, i, 1, j)] = CT(0);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    CT gam = CT(0);
+    for (int k = 2; k <= kde - 1; k++) {
+      const CT alpha = CT(1) / (sb[k] - sa[k] * gam);
+      sb[k] = alpha;
+      gam = sc[k] * alpha;
+      sc[k] = gam;
+    }
+    const CT alpha = CT(1) / (sb[kde] - sa[kde] * gam);
+    sb[kde] = alpha;
+    sc[kde] = CT(0) * alpha;
   }
-  {
-    const int k = kde;
-    const CT b = CT(1) + CT(2) * cof * (rdnw(kde - 1) * rdnw(kde - 1)) * C2A(kde - 1) / (cH(k - 1) * cF(k));
-    const CT alpha = CT(1) / (b - A[X3(d, i, kde, j)] * gam);
-    AL[X3(d, i, kde, j)] = alpha;
-    GA[X3(d, i, kde, j)] = CT(0) * alpha;
+  __syncthreads();
+  for (int k = 2 + (int)threadIdx.x; k <= kde; k += blockDim.x) {
+    AL[X3(d, i, k, j)] = sb[k];
+    GA[X3(d, i, k, j)] = sc[k];
   }
 }
 
-__global__ void __launch_bounds__(256) k_lbc(LbcArgs L) {
+// LBC_PH_PRHO (mass staggering, spec_zone 1): the spec-zone columns advance_w leaves out --
+// calc_coef_w, spec_bdyupdate_ph, calc_p_rho and (zero_grad_w) zero_grad_bdy of w
+__global__ void __launch_bounds__(128) k_lbc_ph(LbcArgs L) {
   const KArgs &a = L.a;
   const Dm &d = a.d;
   const DState &s = dstate(a);
-  const int i = d.ids + (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  const int j = d.jds + (int)(blockIdx.y * blockDim.y + threadIdx.y);
-  const LbcGeo z = lbc_geo(d, L.var);
-  if (i > d.ide || j > d.jde) return;
-  // a whole-domain target is written over zero_tend's extent (its..ite, jts..jte, every
-  // level); everything else stays inside the variable's staggering
-  const bool inside = i <= z.ibe && j <= z.jbe;
-  if (!inside && !(L.mode == LBC_RELAX && L.whole)) return;
-  const bool px = a.c.periodic_x, twod = L.var == 'm';
-  const int sz = a.c.spec_zone, rz = a.c.relax_zone;
+  const LbcGeo z = lbc_geo(d, L.job[0].var);
+  const bool px = a.c.periodic_x;
+  const int sz = a.c.spec_zone;
+  const LbcRing r = lbc_ring(z.ibs, z.ibe, z.jbs, z.jbe, px, L.width);
+  const long t = blockIdx.x;
+  if (t >= r.count) return;
+  int i, j;
+  const long ny = (long)(r.rb + r.rt) * r.ni;
+  if (t < ny) {
+    const int row = (int)(t / r.ni);
+    i = z.ibs + (int)(t % r.ni);
+    j = row < r.rb ? z.jbs + row : z.jbe - (row - r.rb);
+  } else {
+    const int nc = r.cl + r.cr, row = (int)((t - ny) / nc), c = (int)((t - ny) % nc);
+    j = z.jbs + r.rb + row;
+    i = c < r.cl ? z.ibs + c : z.ibe - (c - r.cl);
+  }
+  int b, side = -1, last = -1;
+  while ((side = lbc_side(z, px, i, j, 0, sz - 1, side, &b)) >= 0) last = side;
+  if (last < 0) return;  // uniform over the workgroup
+  __shared__ double sa[kLbcMaxK + 1], sb[kLbcMaxK + 1], sc[kLbcMaxK + 1];
+  __shared__ float sph[kLbcMaxK + 1];
+  if (a.c.precision_mode == 1) coef_w_column<double>(a, s, i, j, sa, sb, sc);
+  else coef_w_column<float>(a, s, i, j, (float *)sa, (float *)sb, (float *)sc);
   const Off o2 = X2(d, i, j);
-  // element offsets are byte offsets on a base pointer (FPtr)
-  const FPtr<float> SRC((float *)L.src), TEND((float *)L.tend);
-  FPtr<float> DST(L.dst);
-  auto at = [&](const FPtr<float> &p, int ii, int k, int jj) -> float {
-    return twod ? p[X2(d, ii, jj)] : p[X3(d, ii, k, jj)];
-  };
-  if (L.mode == LBC_RELAX) {
-    // the relaxed field at a point: ru, rv, mu, or the mass-weighted ph, t, species
-    auto fld = [&](int ii, int k, int jj) -> float {
-      const float f = at(SRC, ii, k, jj);
-      if (!L.massw) return f;
-      const float c1 = L.var == 'h' ? s.c1f[X1(d, k)] : s.c1h[X1(d, k)];
-      const float c2 = L.var == 'h' ? s.c2f[X1(d, k)] : s.c2h[X1(d, k)];
-      return f * (c1 * s.mut[X2(d, ii, jj)] + c2);
-    };
-    const int kmax = L.whole && !twod ? d.kde : z.ktf;
-    for (int k = d.kds; k <= kmax; k++) {
-      float t = L.whole ? 0.f : at(DST, i, k, j);
-      bool any = false;
-      int b, side = -1;
-      while (inside && k <= z.ktf && L.b[0] && (side = lbc_side(z, px, i, j, sz, rz - 1, side, &b)) >= 0) {
-        auto fls = [&](int ij, int bb, int fi, int fj) -> float {
-          return lbc_at(d, L.b[side], side, ij, k, bb, twod) + L.dtbc * lbc_at(d, L.bt[side], side, ij, k, bb, twod) -
-                 fld(fi, k, fj);
-        };
-        float f0, f1, f2, f3, f4;
-        if (side >= 2) {
-          const int im1 = imax(i - 1, z.ibs), ip1 = imin(i + 1, z.ibe), dj = side == 2 ? 1 : -1;
-          f0 = fls(i, b + 1, i, j);
-          f1 = fls(im1, b + 1, im1, j);
-          f2 = fls(ip1, b + 1, ip1, j);
-          f3 = fls(i, b, i, j - dj);
-          f4 = fls(i, b + 2, i, j + dj);
-        } else {
-          const int di = side == 0 ? 1 : -1;
-          f0 = fls(j, b + 1, i, j);
-          f1 = fls(j - 1, b + 1, i, j - 1);
-          f2 = fls(j + 1, b + 1, i, j + 1);
-          f3 = fls(j, b, i - di, j);
-          f4 = fls(j, b + 2, i + di, j);
-        }
-        t = t + L.fcx[b] * f0 - L.gcx[b] * (f1 + f2 + f3 + f4 - 4.f * f0);
-        any = true;
-      }
-      if (any || L.whole) {
-        if (twod) DST[o2] = t;
-        else DST[X3(d, i, k, j)] = t;
-      }
-    }
-    return;
-  }
-  if (L.mode == LBC_FLOW_DEP || L.mode == LBC_ZERO_GRAD) {
-    int b, side = -1, last = -1, lb = 0;
-    while ((side = lbc_side(z, px, i, j, 0, sz - 1, side, &b)) >= 0) { last = side; lb = b; }
-    if (last < 0) return;
-    (void)lb;
-    for (int k = d.kds; k <= z.ktf; k++) {
-      int si, sj;
-      if (last >= 2) {
-        si = imin(imax(i, z.ibs + sz), z.ibe - sz);
-        if (px) si = i;
-        sj = last == 2 ? z.jbs + sz : z.jbe - sz;
-      } else {
-        sj = imin(imax(j, z.jbs + sz), z.jbe - sz);
-        si = last == 0 ? z.ibs + sz : z.ibe - sz;
-      }
-      const float inner = DST[X3(d, si, k, sj)];
-      float v = inner;
-      if (L.mode == LBC_FLOW_DEP) {
-        const FPtr<float> u = s.ru_m, vv = s.rv_m;
-        const bool out = last == 2 ? vv[X3(d, i, k, j)] < 0.f : last == 3 ? vv[X3(d, i, k, j + 1)] > 0.f
-                       : last == 0 ? u[X3(d, i, k, j)] < 0.f : u[X3(d, i + 1, k, j)] > 0.f;
-        v = out ? inner : 0.f;
-      }
-      DST[X3(d, i, k, j)] = v;
-    }
-    return;
-  }
-  // LBC_SPEC_TEND, LBC_UPDATE, LBC_PH_PRHO: the spec zone (later sides overwrite earlier)
-  int b = 0, side = -1, last = -1, lb = 0;
-  while ((side = lbc_side(z, px, i, j, 0, sz - 1, side, &b)) >= 0) {
-    last = side;
-    lb = b;
-    if (L.mode == LBC_UPDATE)  // each side's loop adds dt * tendency once
-      for (int k = d.kds; k <= z.ktf; k++) {
-        if (twod) DST[o2] = DST[o2] + L.dt * TEND[o2];
-        else DST[X3(d, i, k, j)] = DST[X3(d, i, k, j)] + L.dt * TEND[X3(d, i, k, j)];
-      }
-  }
-  if (last < 0 || L.mode == LBC_UPDATE) return;
-  if (L.mode == LBC_SPEC_TEND) {
-    for (int k = d.kds; k <= z.ktf; k++) {
-      const float v = lbc_at(d, L.bt[last], last, last < 2 ? j : i, k, lb + 1, twod);
-      if (twod) DST[o2] = v;
-      else DST[X3(d, i, k, j)] = v;
-    }
-    return;
-  }
-  // LBC_PH_PRHO (mass staggering, spec_zone 1: the columns advance_w leaves out)
-  if (a.c.precision_mode == 1) coef_w_column<double>(a, s, i, j);
-  else coef_w_column<float>(a, s, i, j);
   const float dt = L.dt, muts = s.muts[o2], mu_tend = s.mu_tend[o2];
-  for (int k = d.kds; k <= d.kde; k++) {
+  for (int k = d.kds + (int)threadIdx.x; k <= d.kde; k += blockDim.x) {
     const Off o = X3(d, i, k, j);
     const float c1 = s.c1f[X1(d, k)], c2 = s.c2f[X1(d, k)];
     const float mu_old = muts - dt * mu_tend;
-    s.ph_2[o] = s.ph_2[o] * (c1 * mu_old + c2) / (c1 * muts + c2) + dt * s.ph_tend[o] / (c1 * muts + c2) +
-                s.ph_save[o] * ((c1 * mu_old + c2) / (c1 * muts + c2) - 1.f);
+    const float ph = s.ph_2[o] * (c1 * mu_old + c2) / (c1 * muts + c2) + dt * s.ph_tend[o] / (c1 * muts + c2) +
+                     s.ph_save[o] * ((c1 * mu_old + c2) / (c1 * muts + c2) - 1.f);
+    s.ph_2[o] = ph;
+    sph[k - d.kds] = ph;
   }
+  __syncthreads();
   // calc_p_rho (module_small_step_em.F:438-568) for the column, as k_calc_p_rho forms it
   const float mu = s.mu_2[o2];
-  for (int k = d.kds; k <= d.kde - 1; k++) {
+  for (int k = d.kds + (int)threadIdx.x; k <= d.kde - 1; k += blockDim.x) {
     const Off o = X3(d, i, k, j);
     const float c1 = s.c1h[X1(d, k)], c2 = s.c2h[X1(d, k)];
     const float alt = s.alt[o];
-    const float al = -1.f / (c1 * muts + c2) * (alt * (c1 * mu) + s.rdnw[X1(d, k)] * (s.ph_2[o + d.ni] - s.ph_2[o]));
+    const float al = -1.f / (c1 * muts + c2) * (alt * (c1 * mu) + s.rdnw[X1(d, k)] * (sph[k + 1 - d.kds] - sph[k - d.kds]));
     s.al[o] = al;
     const float t1 = s.t_save[o];
     float pp = s.c2a[o] * (alt * (s.t_2[o] - (c1 * mu) * t1) / ((c1 * muts + c2) * (K_T0 + t1)) - al);
@@ -253,6 +308,18 @@ __global__ void __launch_bounds__(256) k_lbc(LbcArgs L) {
     s.p[o] = pp;
     s.pm1[o] = ptmp;
   }
+  if (!L.zero_grad_w) return;
+  // zero_grad_bdy of w (share/module_bc.F:2219-2332): the frame takes the first inner row
+  int si, sj;
+  if (last >= 2) {
+    si = px ? i : imin(imax(i, z.ibs + sz), z.ibe - sz);
+    sj = last == 2 ? z.jbs + sz : z.jbe - sz;
+  } else {
+    sj = imin(imax(j, z.jbs + sz), z.jbe - sz);
+    si = last == 0 ? z.ibs + sz : z.ibe - sz;
+  }
+  FPtr<float> W(s.w_2);
+  for (int k = d.kds + (int)threadIdx.x; k <= d.kde; k += blockDim.x) W[X3(d, i, k, j)] = W[X3(d, si, k, sj)];
 }
 
 }  // namespace wrfmi

@@ -85,20 +85,49 @@ DEVI bool spec_frame(const wrfmi_config &c, const Dm &d, int i, int j) {
 
 // specified lateral boundaries (k_lbc.hip)
 enum LbcMode : int { LBC_RELAX, LBC_SPEC_TEND, LBC_UPDATE, LBC_PH_PRHO, LBC_ZERO_GRAD, LBC_FLOW_DEP };
-struct LbcArgs {
-  KArgs a;
+// one field of a boundary launch
+struct LbcJob {
   const float *b[4], *bt[4];  // the field's boundary values and tendencies (xs, xe, ys, ye)
-  float fcx[WRFMI_MAX_BDY_WIDTH], gcx[WRFMI_MAX_BDY_WIDTH];
-  float dtbc, dt;
-  int var;            // staggering: 'u', 'v', 'h' (w levels), 'w', 'm' (2-D), other mass levels
-  int mode;           // LbcMode
-  int massw;          // LBC_RELAX: weight the field by c1 mut + c2 (c1f, c2f for 'h')
-  int whole;          // LBC_RELAX: write the whole target (zero outside the relaxation rows)
   const float *src;   // LBC_RELAX: the relaxed field
   const float *tend;  // LBC_UPDATE: the tendency
   float *dst;         // the array written (LBC_ZERO_GRAD, LBC_FLOW_DEP: also read)
+  int var;            // staggering: 'u', 'v', 'h' (w levels), 'w', 'm' (2-D), other mass levels
+  int massw;          // LBC_RELAX: weight the field by c1 mut + c2 (c1f, c2f for 'h')
 };
+constexpr int kLbcJobs = 6;     // fields per launch
+constexpr int kLbcMaxK = 256;   // levels of the k_lbc_ph column (LDS)
+struct LbcArgs {
+  KArgs a;
+  LbcJob job[kLbcJobs];
+  float fcx[WRFMI_MAX_BDY_WIDTH], gcx[WRFMI_MAX_BDY_WIDTH];
+  float dtbc, dt;
+  int mode;           // LbcMode
+  int njobs;
+  int whole;          // LBC_RELAX: write the whole targets (zero outside the relaxation rows)
+  int width;          // rows from the edges the launch covers (spec_zone, or relax_zone)
+  int zero_grad_w;    // k_lbc_ph: also zero_grad_bdy of w
+};
+static_assert(sizeof(LbcArgs) <= 4096, "kernel arguments");
+// the frame of `width` rows along the edges of the extent [ibs, ibe] x [jbs, jbe] (no
+// x columns when periodic in x), enumerated y rows first: rb rows at jbs, rt at jbe, then
+// the nmid rows between with cl columns at ibs and cr at ibe
+struct LbcRing {
+  int ni, rb, rt, cl, cr;
+  long count;
+};
+HDI LbcRing lbc_ring(int ibs, int ibe, int jbs, int jbe, bool px, int width) {
+  LbcRing r;
+  const int nj = jbe - jbs + 1;
+  r.ni = ibe - ibs + 1;
+  r.rb = width < nj ? width : nj;
+  r.rt = width < nj - r.rb ? width : nj - r.rb;
+  r.cl = px ? 0 : (width < r.ni ? width : r.ni);
+  r.cr = px ? 0 : (width < r.ni - r.cl ? width : r.ni - r.cl);
+  r.count = (long)(r.rb + r.rt) * r.ni + (long)(nj - r.rb - r.rt) * (r.cl + r.cr);
+  return r;
+}
 __global__ void k_lbc(LbcArgs);
+__global__ void k_lbc_ph(LbcArgs);
 template <typename A, int K> __global__ void k_diff2(A);
 template <typename A> __global__ void k_p_rho_phi(A);
 template <typename A> __global__ void k_p_rho_phi_x4(A);

@@ -10,6 +10,7 @@
 #include <stdio.h>
 #include <string.h>
 #include <math.h>
+#include <initializer_list>
 #include <iterator>
 #include <map>
 #include <mutex>
@@ -673,28 +674,46 @@ static void ph_scalar_mixing(Launcher &L) {
     L.launchK(KID_DIFF6, KRP(k_diff6, 4), g_pt3(L.a.d, L.a.d.kde - L.a.d.kds), kColBlk, L.a);
 }
 // k_lbc over the columns of the domain (see k_lbc.hip for the modes)
-static void lbc_launch(Launcher &L, int mode, int var, float *dst, const float *src, const float *tend, float dt,
-                       float *const *b, float *const *bt, int massw, int whole) {
+static LbcJob lbc_job(int var, float *dst, const float *src = nullptr, const float *tend = nullptr,
+                      float *const *b = nullptr, float *const *bt = nullptr, int massw = 0) {
+  LbcJob J;
+  memset(&J, 0, sizeof J);
+  for (int n = 0; n < 4; n++) {
+    J.b[n] = b ? b[n] : nullptr;
+    J.bt[n] = bt ? bt[n] : nullptr;
+  }
+  J.src = src;
+  J.tend = tend;
+  J.dst = dst;
+  J.var = var;
+  J.massw = massw;
+  return J;
+}
+// one boundary launch over up to kLbcJobs fields: the whole extent (whole), else the frame
+// of `width` rows; LBC_PH_PRHO is k_lbc_ph, one workgroup per spec-zone column
+static void lbc_launch(Launcher &L, int mode, float dt, int whole, int width, std::initializer_list<LbcJob> jobs,
+                       int zero_grad_w = 0) {
   LbcArgs A;
   memset(&A, 0, sizeof A);
   A.a = L.a;
-  for (int n = 0; n < 4; n++) {
-    A.b[n] = b ? b[n] : nullptr;
-    A.bt[n] = bt ? bt[n] : nullptr;
-  }
+  for (const LbcJob &J : jobs) A.job[A.njobs++] = J;
   memcpy(A.fcx, L.lbc.fcx, sizeof A.fcx);
   memcpy(A.gcx, L.lbc.gcx, sizeof A.gcx);
   A.dtbc = L.lbc.dtbc;
   A.dt = dt;
-  A.var = var;
   A.mode = mode;
-  A.massw = massw;
   A.whole = whole;
-  A.src = src;
-  A.tend = tend;
-  A.dst = dst;
+  A.width = width;
+  A.zero_grad_w = zero_grad_w;
   const Dm &d = L.a.d;
-  L.launch(KID_LBC, k_lbc, dim3((d.ide - d.ids + 1 + 63) / 64, (d.jde - d.jds + 1 + 3) / 4, 1), dim3(64, 4, 1), A);
+  // the frame of the widest staggering bounds every field's frame
+  const LbcRing r = lbc_ring(d.ids, d.ide, d.jds, d.jde, L.a.c.periodic_x, width);
+  if (mode == LBC_PH_PRHO) {
+    L.launch(KID_LBC, k_lbc_ph, dim3((unsigned)r.count, 1, 1), dim3(128, 1, 1), A);
+    return;
+  }
+  const long n = whole ? (long)(d.ide - d.ids + 1) * (d.jde - d.jds + 1) : r.count;
+  L.launch(KID_LBC, k_lbc, dim3((unsigned)((n + 255) / 256), d.kde - d.kds + 1, A.njobs), dim3(256, 1, 1), A);
 }
 // relax_bdy_dry at rk_step 1 (solve_em.F:937-962), into the *_save arrays that k_tend_*
 // add to the forward tendencies (rk_addtend_dry) and, for mu, into mu_save (k_tend_t adds it
@@ -703,23 +722,22 @@ static void lbc_relax_dry(Launcher &L) {
   const wrfmi_state &s = L.a.s;
   wrfmi_lbc &B = L.lbc;
   const bool nest = L.a.c.nested;
-  lbc_launch(L, LBC_RELAX, 'u', s.u_save, s.ru, nullptr, 0.f, B.u_b, B.u_bt, 0, 1);
-  lbc_launch(L, LBC_RELAX, 'v', s.v_save, s.rv, nullptr, 0.f, B.v_b, B.v_bt, 0, 1);
-  lbc_launch(L, LBC_RELAX, 'h', s.ph_save, s.ph_2, nullptr, 0.f, B.ph_b, B.ph_bt, 1, 1);
-  lbc_launch(L, LBC_RELAX, 't', s.t_save, s.t_2, nullptr, 0.f, B.t_b, B.t_bt, 1, 1);
-  lbc_launch(L, LBC_RELAX, 'm', s.mu_save, s.mu_2, nullptr, 0.f, B.mu_b, B.mu_bt, 0, 1);
-  lbc_launch(L, LBC_RELAX, 'h', s.w_save, s.w_2, nullptr, 0.f, nest ? B.w_b : nullptr, nest ? B.w_bt : nullptr, 1, 1);
+  lbc_launch(L, LBC_RELAX, 0.f, 1, 0,
+             {lbc_job('u', s.u_save, s.ru, nullptr, B.u_b, B.u_bt), lbc_job('v', s.v_save, s.rv, nullptr, B.v_b, B.v_bt),
+              lbc_job('h', s.ph_save, s.ph_2, nullptr, B.ph_b, B.ph_bt, 1),
+              lbc_job('t', s.t_save, s.t_2, nullptr, B.t_b, B.t_bt, 1), lbc_job('m', s.mu_save, s.mu_2, nullptr, B.mu_b, B.mu_bt),
+              lbc_job('h', s.w_save, s.w_2, nullptr, nest ? B.w_b : nullptr, nest ? B.w_bt : nullptr, 1)});
 }
 // spec_bdy_dry after rk_addtend_dry (solve_em.F:982-1008)
 static void lbc_spec_dry(Launcher &L) {
   const wrfmi_state &s = L.a.s;
   wrfmi_lbc &B = L.lbc;
-  lbc_launch(L, LBC_SPEC_TEND, 'u', s.ru_tend, nullptr, nullptr, 0.f, nullptr, B.u_bt, 0, 0);
-  lbc_launch(L, LBC_SPEC_TEND, 'v', s.rv_tend, nullptr, nullptr, 0.f, nullptr, B.v_bt, 0, 0);
-  lbc_launch(L, LBC_SPEC_TEND, 'h', s.ph_tend, nullptr, nullptr, 0.f, nullptr, B.ph_bt, 0, 0);
-  lbc_launch(L, LBC_SPEC_TEND, 't', s.t_tend, nullptr, nullptr, 0.f, nullptr, B.t_bt, 0, 0);
-  lbc_launch(L, LBC_SPEC_TEND, 'm', s.mu_tend, nullptr, nullptr, 0.f, nullptr, B.mu_bt, 0, 0);
-  if (L.a.c.nested) lbc_launch(L, LBC_SPEC_TEND, 'h', s.rw_tend, nullptr, nullptr, 0.f, nullptr, B.w_bt, 0, 0);
+  const int sz = L.a.c.spec_zone;
+  lbc_launch(L, LBC_SPEC_TEND, 0.f, 0, sz,
+             {lbc_job('u', s.ru_tend, nullptr, nullptr, nullptr, B.u_bt), lbc_job('v', s.rv_tend, nullptr, nullptr, nullptr, B.v_bt),
+              lbc_job('h', s.ph_tend, nullptr, nullptr, nullptr, B.ph_bt), lbc_job('t', s.t_tend, nullptr, nullptr, nullptr, B.t_bt),
+              lbc_job('m', s.mu_tend, nullptr, nullptr, nullptr, B.mu_bt)});
+  if (L.a.c.nested) lbc_launch(L, LBC_SPEC_TEND, 0.f, 0, sz, {lbc_job('h', s.rw_tend, nullptr, nullptr, nullptr, B.w_bt)});
 }
 static void ph_tendency(Launcher &L) {
   if (L.a.rk_step == 1) ph_mixing(L);
@@ -790,8 +808,8 @@ static void ph_advance_uv(Launcher &L) {
     L.launchK(KID_ADVANCE_UV, per ? KRP(k_advance_uv, true) : KRP(k_advance_uv, false), g_pt3(L.a.d, nkb), kColBlk, L.a);
   if (L.lbc_on) {  // spec_bdyupdate of u and v (solve_em.F:1343-1365)
     const wrfmi_state &s = L.a.s;
-    lbc_launch(L, LBC_UPDATE, 'u', s.u_2, nullptr, s.ru_tend, L.a.dts, nullptr, nullptr, 0, 0);
-    lbc_launch(L, LBC_UPDATE, 'v', s.v_2, nullptr, s.rv_tend, L.a.dts, nullptr, nullptr, 0, 0);
+    lbc_launch(L, LBC_UPDATE, L.a.dts, 0, L.a.c.spec_zone,
+               {lbc_job('u', s.u_2, nullptr, s.ru_tend), lbc_job('v', s.v_2, nullptr, s.rv_tend)});
   }
 }
 static void ph_advance_mu_t_w(Launcher &L) {
@@ -810,9 +828,9 @@ static void ph_advance_mu_t_w(Launcher &L) {
   if (L.vec) L.launchK(KID_AC_THETA, KR(k_ac_theta_x4), gvec(L, KID_AC_THETA, nk), vblk(L, KID_AC_THETA), a);
   else L.launchK(KID_AC_THETA, KR(k_ac_theta), g_pt3(a.d, nk), kColBlk, a);
   if (L.lbc_on) {  // spec_bdyupdate of theta, mu and muts (solve_em.F:1459-1490)
-    lbc_launch(L, LBC_UPDATE, 't', s.t_2, nullptr, s.t_tend, a.dts, nullptr, nullptr, 0, 0);
-    lbc_launch(L, LBC_UPDATE, 'm', s.mu_2, nullptr, s.mu_tend, a.dts, nullptr, nullptr, 0, 0);
-    lbc_launch(L, LBC_UPDATE, 'm', s.muts, nullptr, s.mu_tend, a.dts, nullptr, nullptr, 0, 0);
+    lbc_launch(L, LBC_UPDATE, a.dts, 0, a.c.spec_zone,
+               {lbc_job('t', s.t_2, nullptr, s.t_tend), lbc_job('m', s.mu_2, nullptr, s.mu_tend),
+                lbc_job('m', s.muts, nullptr, s.mu_tend)});
   }
   {
     const int pf = L.sh[KID_AC_WSOLVE].prefetch;
@@ -824,9 +842,8 @@ static void ph_advance_mu_t_w(Launcher &L) {
   if (L.lbc_on && a.c.non_hydrostatic) {
     // spec_bdyupdate_ph, zero_grad_bdy of w (solve_em.F:1583-1620), and calc_p_rho on the
     // spec-zone columns k_ac_wsolve left out
-    lbc_launch(L, LBC_PH_PRHO, 't', s.ph_2, nullptr, nullptr, a.dts, nullptr, nullptr, 0, 0);
-    if (a.c.specified) lbc_launch(L, LBC_ZERO_GRAD, 'w', s.w_2, nullptr, nullptr, 0.f, nullptr, nullptr, 0, 0);
-    else lbc_launch(L, LBC_UPDATE, 'h', s.w_2, nullptr, s.rw_tend, a.dts, nullptr, nullptr, 0, 0);
+    lbc_launch(L, LBC_PH_PRHO, a.dts, 0, a.c.spec_zone, {lbc_job('t', s.ph_2)}, a.c.specified);
+    if (!a.c.specified) lbc_launch(L, LBC_UPDATE, a.dts, 0, a.c.spec_zone, {lbc_job('h', s.w_2, nullptr, s.rw_tend)});
   }
   L.halo_begin();
   L.halo_add(s.ph_2, 'w', 1); L.halo_add(s.al, 'p', 1); L.halo_add(s.p, 'p', 1);
@@ -920,15 +937,15 @@ static void ph_scalar(Launcher &L) {
   const wrfmi_state &s = L.a.s;
   const bool bcs = L.lbc_on && species_bcs(L.a.c, s, m);
   if (bcs && L.a.rk_step == 1) {  // relax_bdy_scalar, spec_bdy_scalar (solve_em.F:2261-2296)
-    lbc_launch(L, LBC_RELAX, 'q', s.q_tend[m], s.q[m], nullptr, 0.f, L.lbc.q_b[m], L.lbc.q_bt[m], 1, 0);
-    lbc_launch(L, LBC_SPEC_TEND, 'q', s.q_tend[m], nullptr, nullptr, 0.f, nullptr, L.lbc.q_bt[m], 0, 0);
+    lbc_launch(L, LBC_RELAX, 0.f, 0, L.a.c.relax_zone, {lbc_job('q', s.q_tend[m], s.q[m], nullptr, L.lbc.q_b[m], L.lbc.q_bt[m], 1)});
+    lbc_launch(L, LBC_SPEC_TEND, 0.f, 0, L.a.c.spec_zone, {lbc_job('q', s.q_tend[m], nullptr, nullptr, nullptr, L.lbc.q_bt[m])});
   }
   if (L.vec)
     L.launchK(KID_UPDATE_SCALAR, KR(k_update_scalar_x4), gvec(L, KID_UPDATE_SCALAR, L.a.d.kde - L.a.d.kds),
               vblk(L, KID_UPDATE_SCALAR), L.a);
   else L.launchK(KID_UPDATE_SCALAR, KR(k_update_scalar), g_pt3(L.a.d, L.a.d.kde - L.a.d.kds), kColBlk, L.a);
   // flow_dep_bdy of a species without boundary arrays (solve_em.F:2342-2355, 2886-2930)
-  if (L.a.c.specified && !bcs) lbc_launch(L, LBC_FLOW_DEP, 'q', s.q[m], nullptr, nullptr, 0.f, nullptr, nullptr, 0, 0);
+  if (L.a.c.specified && !bcs) lbc_launch(L, LBC_FLOW_DEP, 0.f, 0, L.a.c.spec_zone, {lbc_job('q', s.q[m])});
 }
 // every species' rk_scalar_tend and rk_update_scalar (species are independent: the order
 // over species does not change any value)
@@ -1113,6 +1130,8 @@ static int prepare(const wrfmi_state *st_in, const wrfmi_config *cfg, const wrfm
   L.h.open_xs = cfg->open_xs || spx; L.h.open_xe = cfg->open_xe || spx;
   L.h.open_ys = cfg->open_ys || spy; L.h.open_ye = cfg->open_ye || spy;
   L.lbc_on = cfg->specified || cfg->nested;
+  if (L.lbc_on && d.kde > kLbcMaxK)
+    return fail(WRFMI_E_DIMS, "specified boundaries: at most %d levels (kde %d)", kLbcMaxK, d.kde);
   if (L.lbc_on && (rc = translate_lbc(st_in, *cfg, &L.lbc))) return rc;
   // ensemble members are recorded with the default shapes (one launch sequence for all)
   for (int k = 0; k < KID_COUNT; k++) L.sh[k] = default_shapes ? default_shape(k) : shape_for(k, d);
