@@ -372,7 +372,18 @@ __global__ void __launch_bounds__(256) k_advance_uv_x4(ARGS args) {
   const int k1 = imin(k0 + KS - 1, kde - 1);
   const wrfmi_config &c = a.c;
   const Geo g = geo<PER>(c);
-  const int i_start = its, i_end = imin(ite, ide - 1), j_start = jts, j_end = imin(jte, jde - 1), i_endu = ite, j_endv = jte;
+  int i_start = its, i_end = imin(ite, ide - 1), j_start = jts, j_end = imin(jte, jde - 1), i_endu = ite, j_endv = jte;
+  if (c.specified || c.nested) {  // module_small_step_em.F:734-747: the spec zone is spec_bdyupdate's
+    const int sz = c.spec_zone;
+    j_start = imax(jts, jds + sz);
+    j_end = imin(jte, jde - sz - 1);
+    j_endv = imin(jte, jde - sz);
+    if (!c.periodic_x) {
+      i_start = imax(its, ids + sz);
+      i_end = imin(ite, ide - sz - 1);
+      i_endu = imin(ite, ide - sz);
+    }
+  }
   int i_start_up = i_start, i_end_up = i_endu, j_start_vp = j_start, j_end_vp = j_endv;
   if ((g.open_xs || g.sym_xs) && its == ids) i_start_up++;
   if ((g.open_xe || g.sym_xe) && ite == ide) i_end_up--;
@@ -504,7 +515,7 @@ __global__ void __launch_bounds__(256) k_advance_uv_x4(ARGS args) {
         const float vpg = v - dts * cqv.v[n] * dpxy + (c1 * mudf_yv[n]);
         vn.v[n] = vpg_on ? vpg : v;
       }
-      if (v_on) st4m(s.v_2, o, vn, i0, ids, i_end);
+      if (v_on) st4m(s.v_2, o, vn, i0, i_start, i_end);
     }
     pc = pcp; hc = hcp; pl = plp; hl = hlp; ps = psp; hs = hsp;
   }
@@ -1041,8 +1052,12 @@ __global__ void __launch_bounds__(256) k_ac_theta_x4(ARGS args) {
       v1 = (v1 + (c1 * muave.v[n]) * K_T0) / ((c1 * muts.v[n] + c2) * (K_T0 + t1.v[n]));
       t2s.v[n] = c.non_hydrostatic ? v1 : tv.v[n];
     }
-    st4m(s.t_2, o, tn, i0, ids, ide - 1);
-    st4m(s.t_2save, o, t2s, i0, ids, ide - 1);
+    // a specified domain's outermost rows and columns are spec_bdyupdate's
+    const bool spec = c.specified || c.nested, sx = spec && !c.periodic_x;
+    if (!(spec && (j == d.jds || j == jde - 1))) {
+      st4m(s.t_2, o, tn, i0, sx ? ids + 1 : ids, sx ? ide - 2 : ide - 1);
+      st4m(s.t_2save, o, t2s, i0, sx ? ids + 1 : ids, sx ? ide - 2 : ide - 1);
+    }
   }
 }
 

@@ -760,6 +760,7 @@ static void ph_tendency(Launcher &L) {
       L.launchK(KID_TEND_W, KRZ(k_tend_w), gz, kColBlk, a);
       L.launchK(KID_TEND_T, KRZ(k_tend_t), gz, kColBlk, a);
     }
+    if (L.lbc_on) lbc_spec_dry(L);
     return;
   }
   L.launchK(KID_TEND_U, per ? KRP(k_tend_u, true) : KRP(k_tend_u, false), gt, kColBlk, a);
@@ -1139,7 +1140,15 @@ static int prepare(const wrfmi_state *st_in, const wrfmi_config *cfg, const wrfm
   // the all-species scalar kernels (doubly periodic, aligned layout, more than one species)
   L.scal_all = L.vec && cfg->periodic_x && cfg->periodic_y && s.n_species > 1 && (own_qt || !own_scratch);
   L.zone_ok = (cfg->periodic_x || d.ide - d.ids + 1 >= 2 * kZone + 2) && (cfg->periodic_y || d.jde - d.jds + 1 >= 2 * kZone + 2);
-  if (L.lbc_on) L.vec = L.scal_all = L.zone_ok = false;  // the one-point kernels carry the spec-zone bounds
+  // specified boundaries: the four-point kernels carry the spec-zone bounds; the boundary
+  // zone's one-point kernels take the relaxation rows (added to the tendencies), so those
+  // must lie inside it.  With a periodic x, rhs_ph still narrows its x range near the x
+  // edges (module_big_step_utilities_em.F:1937-1938, 2023-2048 test `specified` alone), which only the
+  // one-point k_tend_t has: the tendencies then run one-point over the whole domain.
+  if (L.lbc_on) {
+    L.scal_all = false;
+    if (cfg->relax_zone > kZone || cfg->periodic_x) L.zone_ok = false;
+  }
   if (L.scal_all) {
     float *qt = own_qt;
     if (!qt && (rc = get_qt(key, s.n_species, &qt))) return rc;
