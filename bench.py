@@ -332,6 +332,7 @@ def main():
                     help="add the 1200x1200x80 terrain leg (about 50 GB of HBM, a minute of host set-up)")
     ap.add_argument("--terrain-steps", type=int, default=20)
     ap.add_argument("--terrain-only", action="store_true", help="run only the terrain leg (profiling)")
+    ap.add_argument("--specified-only", action="store_true", help="run only the specified-boundary leg (profiling)")
     ap.add_argument("--no-options", action="store_true",
                     help="skip the option legs (positive-definite transport, mixing, diffusion, w damping, "
                          "specified boundaries)")
@@ -354,6 +355,9 @@ def main():
         return
     if args.terrain_only:
         print(json.dumps({"terrain": terrain_leg(runtime, grid, torch, args.terrain_steps, log)}))
+        return
+    if args.specified_only:
+        print(json.dumps({"specified": specified_leg(runtime, grid, torch, args.steps, log, args.nx, args.ny, args.nz)}))
         return
     spec = grid.large_c1(nx=args.nx, ny=args.ny, nz=args.nz)
     log(f"building the {spec.nx}x{spec.ny}x{spec.nz} initial state ...")

@@ -74,29 +74,32 @@ __global__ void __launch_bounds__(256) k_lbc(LbcArgs L) {
   const KArgs &a = L.a;
   const Dm &d = a.d;
   const DState &s = dstate(a);
-  const LbcJob &J = L.job[blockIdx.z];
+  // whole: blockIdx.z = job * nk + level, (64, 4) threads over (i, j); frame: blockIdx.y =
+  // level, blockIdx.z = job, threads over the frame's points
+  const int nk = d.kde - d.kds + 1;
+  const int jb = L.whole ? (int)blockIdx.z / nk : (int)blockIdx.z;
+  const LbcJob &J = L.job[jb];
   const LbcGeo z = lbc_geo(d, J.var);
   const bool px = a.c.periodic_x, twod = J.var == 'm';
-  const int k = d.kds + (int)blockIdx.y;
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int kb = d.kds + (L.whole ? (int)blockIdx.z - jb * nk : (int)blockIdx.y);
   int i, j;
   if (L.whole) {
     // a whole-domain target is written over zero_tend's extent (ids..ide, jds..jde, every
     // level); everything else stays inside the variable's staggering
-    const int ni = d.ide - d.ids + 1;
-    if (t >= (long)ni * (d.jde - d.jds + 1)) return;
-    i = d.ids + (int)(t % ni);
-    j = d.jds + (int)(t / ni);
+    i = d.ids + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    j = d.jds + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+    if (i > d.ide || j > d.jde) return;
   } else {
     const LbcRing r = lbc_ring(z.ibs, z.ibe, z.jbs, z.jbe, px, L.width);
-    if (t >= r.count) return;
-    const long ny = (long)(r.rb + r.rt) * r.ni;
+    const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (t >= (int)r.count) return;
+    const int ny = (r.rb + r.rt) * r.ni;
     if (t < ny) {
-      const int row = (int)(t / r.ni);
-      i = z.ibs + (int)(t % r.ni);
+      const int row = t / r.ni;
+      i = z.ibs + (t - row * r.ni);
       j = row < r.rb ? z.jbs + row : z.jbe - (row - r.rb);
     } else {
-      const int nc = r.cl + r.cr, row = (int)((t - ny) / nc), c = (int)((t - ny) % nc);
+      const int nc = r.cl + r.cr, row = (t - ny) / nc, c = (t - ny) - row * nc;
       j = z.jbs + r.rb + row;
       i = c < r.cl ? z.ibs + c : z.ibe - (c - r.cl);
     }
@@ -111,48 +114,53 @@ __global__ void __launch_bounds__(256) k_lbc(LbcArgs L) {
     return twod ? p[X2(d, ii, jj)] : p[X3(d, ii, kk, jj)];
   };
   if (L.mode == LBC_RELAX) {
-    if (k > (L.whole && !twod ? d.kde : z.ktf)) return;
-    // the relaxed field at a point: ru, rv, mu, or the mass-weighted ph, t, species
-    auto fld = [&](int ii, int kk, int jj) -> float {
-      const float f = at(SRC, ii, kk, jj);
-      if (!J.massw) return f;
-      const float c1 = J.var == 'h' ? s.c1f[X1(d, kk)] : s.c1h[X1(d, kk)];
-      const float c2 = J.var == 'h' ? s.c2f[X1(d, kk)] : s.c2h[X1(d, kk)];
-      return f * (c1 * s.mut[X2(d, ii, jj)] + c2);
-    };
-    float tv = L.whole ? 0.f : at(DST, i, k, j);
-    bool any = false;
-    int b, side = -1;
-    while (inside && k <= z.ktf && J.b[0] && (side = lbc_side(z, px, i, j, sz, rz - 1, side, &b)) >= 0) {
-      auto fls = [&](int ij, int bb, int fi, int fj) -> float {
-        return lbc_at(d, J.b[side], side, ij, k, bb, twod) + L.dtbc * lbc_at(d, J.bt[side], side, ij, k, bb, twod) -
-               fld(fi, k, fj);
+    const int kmax = L.whole && !twod ? d.kde : z.ktf;
+    for (int k = kb; k <= imin(kb, kmax); k++) {  // this thread's level
+      // the relaxed field at a point: ru, rv, mu, or the mass-weighted ph, t, species
+      auto fld = [&](int ii, int kk, int jj) -> float {
+        const float f = at(SRC, ii, kk, jj);
+        if (!J.massw) return f;
+        const float c1 = J.var == 'h' ? s.c1f[X1(d, kk)] : s.c1h[X1(d, kk)];
+        const float c2 = J.var == 'h' ? s.c2f[X1(d, kk)] : s.c2h[X1(d, kk)];
+        return f * (c1 * s.mut[X2(d, ii, jj)] + c2);
       };
-      float f0, f1, f2, f3, f4;
-      if (side >= 2) {
-        const int im1 = imax(i - 1, z.ibs), ip1 = imin(i + 1, z.ibe), dj = side == 2 ? 1 : -1;
-        f0 = fls(i, b + 1, i, j);
-        f1 = fls(im1, b + 1, im1, j);
-        f2 = fls(ip1, b + 1, ip1, j);
-        f3 = fls(i, b, i, j - dj);
-        f4 = fls(i, b + 2, i, j + dj);
-      } else {
-        const int di = side == 0 ? 1 : -1;
-        f0 = fls(j, b + 1, i, j);
-        f1 = fls(j - 1, b + 1, i, j - 1);
-        f2 = fls(j + 1, b + 1, i, j + 1);
-        f3 = fls(j, b, i - di, j);
-        f4 = fls(j, b + 2, i + di, j);
+      float tv = L.whole ? 0.f : at(DST, i, k, j);
+      bool any = false;
+      int b, side = -1;
+      // only points within relax_zone of an edge can lie in a relaxation row
+      const bool near = inside && (j - z.jbs < rz || z.jbe - j < rz || (!px && (i - z.ibs < rz || z.ibe - i < rz)));
+      while (near && k <= z.ktf && J.b[0] && (side = lbc_side(z, px, i, j, sz, rz - 1, side, &b)) >= 0) {
+        auto fls = [&](int ij, int bb, int fi, int fj) -> float {
+          return lbc_at(d, J.b[side], side, ij, k, bb, twod) + L.dtbc * lbc_at(d, J.bt[side], side, ij, k, bb, twod) -
+                 fld(fi, k, fj);
+        };
+        float f0, f1, f2, f3, f4;
+        if (side >= 2) {
+          const int im1 = imax(i - 1, z.ibs), ip1 = imin(i + 1, z.ibe), dj = side == 2 ? 1 : -1;
+          f0 = fls(i, b + 1, i, j);
+          f1 = fls(im1, b + 1, im1, j);
+          f2 = fls(ip1, b + 1, ip1, j);
+          f3 = fls(i, b, i, j - dj);
+          f4 = fls(i, b + 2, i, j + dj);
+        } else {
+          const int di = side == 0 ? 1 : -1;
+          f0 = fls(j, b + 1, i, j);
+          f1 = fls(j - 1, b + 1, i, j - 1);
+          f2 = fls(j + 1, b + 1, i, j + 1);
+          f3 = fls(j, b, i - di, j);
+          f4 = fls(j, b + 2, i + di, j);
+        }
+        tv = tv + L.fcx[b] * f0 - L.gcx[b] * (f1 + f2 + f3 + f4 - 4.f * f0);
+        any = true;
       }
-      tv = tv + L.fcx[b] * f0 - L.gcx[b] * (f1 + f2 + f3 + f4 - 4.f * f0);
-      any = true;
-    }
-    if (any || L.whole) {
-      if (twod) DST[o2] = tv;
-      else DST[X3(d, i, k, j)] = tv;
+      if (any || L.whole) {
+        if (twod) DST[o2] = tv;
+        else DST[X3(d, i, k, j)] = tv;
+      }
     }
     return;
   }
+  const int k = kb;
   if (!inside || k > z.ktf) return;
   if (L.mode == LBC_FLOW_DEP || L.mode == LBC_ZERO_GRAD) {
     int b, side = -1, last = -1;

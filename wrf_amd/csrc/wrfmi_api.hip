@@ -712,8 +712,11 @@ static void lbc_launch(Launcher &L, int mode, float dt, int whole, int width, st
     L.launch(KID_LBC, k_lbc_ph, dim3((unsigned)r.count, 1, 1), dim3(128, 1, 1), A);
     return;
   }
-  const long n = whole ? (long)(d.ide - d.ids + 1) * (d.jde - d.jds + 1) : r.count;
-  L.launch(KID_LBC, k_lbc, dim3((unsigned)((n + 255) / 256), d.kde - d.kds + 1, A.njobs), dim3(256, 1, 1), A);
+  const int nk = d.kde - d.kds + 1;
+  if (whole)
+    L.launch(KID_LBC, k_lbc, dim3((d.ide - d.ids + 64) / 64, (d.jde - d.jds + 4) / 4, nk * A.njobs), dim3(64, 4, 1), A);
+  else
+    L.launch(KID_LBC, k_lbc, dim3((unsigned)((r.count + 255) / 256), nk, A.njobs), dim3(256, 1, 1), A);
 }
 // relax_bdy_dry at rk_step 1 (solve_em.F:937-962), into the *_save arrays that k_tend_*
 // add to the forward tendencies (rk_addtend_dry) and, for mu, into mu_save (k_tend_t adds it
