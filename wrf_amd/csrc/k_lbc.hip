@@ -5,10 +5,10 @@
 // spec-zone column.  Modes:
 //   LBC_RELAX      relax_bdytend (share/module_bc.F:1221-1427) of ru, rv, ph or t (mass
 //                  weighted as relax_bdy_dry's mass_weight does, dyn_em/module_bc_em.F:
-//                  1716-1745), mu or a species into the target: the *_save arrays (written
-//                  over the whole domain, zero outside the relaxation rows, as rk_tendency
-//                  left them), mu_save for mu (which k_tend_t adds to mu_tend), q_tend;
-//                  without boundary arrays (w of a specified domain) only the zeros
+//                  1716-1745), mu or a species added to the target: the *_save arrays
+//                  (after LBC_ZERO), mu_save for mu (which k_tend_t adds to mu_tend), q_tend;
+//                  with `whole`, written over the whole extent, zero outside the rows
+//   LBC_ZERO       zero_tend of the save arrays over the whole extent, every level
 //   LBC_SPEC_TEND  spec_bdytend (1430-1549): the spec zone's tendency is the boundary one
 //   LBC_UPDATE     spec_bdyupdate (1955-2064): field += dt * tendency in the spec zone
 //   LBC_PH_PRHO    spec_bdyupdate_ph (dyn_em/module_bc_em.F:17-157) on the spec-zone
@@ -77,11 +77,11 @@ __global__ void __launch_bounds__(256) k_lbc(LbcArgs L) {
   // whole: blockIdx.z = job * nk + level, (64, 4) threads over (i, j); frame: blockIdx.y =
   // level, blockIdx.z = job, threads over the frame's points
   const int nk = d.kde - d.kds + 1;
-  const int jb = L.whole ? (int)blockIdx.z / nk : (int)blockIdx.z;
+  const int jb = L.whole && L.mode != LBC_ZERO ? (int)blockIdx.z / nk : (int)blockIdx.z;
   const LbcJob &J = L.job[jb];
   const LbcGeo z = lbc_geo(d, J.var);
   const bool px = a.c.periodic_x, twod = J.var == 'm';
-  const int kb = d.kds + (L.whole ? (int)blockIdx.z - jb * nk : (int)blockIdx.y);
+  const int kb = d.kds + (L.mode == LBC_ZERO ? 0 : L.whole ? (int)blockIdx.z - jb * nk : (int)blockIdx.y);
   int i, j;
   if (L.whole) {
     // a whole-domain target is written over zero_tend's extent (ids..ide, jds..jde, every
@@ -103,6 +103,13 @@ __global__ void __launch_bounds__(256) k_lbc(LbcArgs L) {
       j = z.jbs + r.rb + row;
       i = c < r.cl ? z.ibs + c : z.ibe - (c - r.cl);
     }
+  }
+  if (L.mode == LBC_ZERO) {  // zero_tend of a save array over the whole extent, a column per thread
+    FPtr<float> D(J.dst);
+    if (twod) D[X2(d, i, j)] = 0.f;
+    else
+      for (int k = d.kds; k <= d.kde; k++) D[X3(d, i, k, j)] = 0.f;
+    return;
   }
   const bool inside = i <= z.ibe && j <= z.jbe;
   const int sz = a.c.spec_zone, rz = a.c.relax_zone;

@@ -84,7 +84,7 @@ DEVI bool spec_frame(const wrfmi_config &c, const Dm &d, int i, int j) {
 }
 
 // specified lateral boundaries (k_lbc.hip)
-enum LbcMode : int { LBC_RELAX, LBC_SPEC_TEND, LBC_UPDATE, LBC_PH_PRHO, LBC_ZERO_GRAD, LBC_FLOW_DEP };
+enum LbcMode : int { LBC_RELAX, LBC_SPEC_TEND, LBC_UPDATE, LBC_PH_PRHO, LBC_ZERO_GRAD, LBC_FLOW_DEP, LBC_ZERO };
 // one field of a boundary launch
 struct LbcJob {
   const float *b[4], *bt[4];  // the field's boundary values and tendencies (xs, xe, ys, ye)
@@ -103,7 +103,7 @@ struct LbcArgs {
   float dtbc, dt;
   int mode;           // LbcMode
   int njobs;
-  int whole;          // LBC_RELAX: write the whole targets (zero outside the relaxation rows)
+  int whole;          // LBC_ZERO: the whole extent (ids..ide, jds..jde, every level)
   int width;          // rows from the edges the launch covers (spec_zone, or relax_zone)
   int zero_grad_w;    // k_lbc_ph: also zero_grad_bdy of w
 };

@@ -714,22 +714,27 @@ static void lbc_launch(Launcher &L, int mode, float dt, int whole, int width, st
   }
   const int nk = d.kde - d.kds + 1;
   if (whole)
-    L.launch(KID_LBC, k_lbc, dim3((d.ide - d.ids + 64) / 64, (d.jde - d.jds + 4) / 4, nk * A.njobs), dim3(64, 4, 1), A);
+    L.launch(KID_LBC, k_lbc, dim3((d.ide - d.ids + 64) / 64, (d.jde - d.jds + 4) / 4, (mode == LBC_ZERO ? 1 : nk) * A.njobs),
+             dim3(64, 4, 1), A);
   else
     L.launch(KID_LBC, k_lbc, dim3((unsigned)((r.count + 255) / 256), nk, A.njobs), dim3(256, 1, 1), A);
 }
 // relax_bdy_dry at rk_step 1 (solve_em.F:937-962), into the *_save arrays that k_tend_*
 // add to the forward tendencies (rk_addtend_dry) and, for mu, into mu_save (k_tend_t adds it
-// to mu_tend); every save array is written whole, as rk_tendency zeroes them
+// to mu_tend): the save arrays are zeroed over the whole extent, as rk_tendency's zero_tend
+// leaves them, then the relaxation rows are written
 static void lbc_relax_dry(Launcher &L) {
   const wrfmi_state &s = L.a.s;
   wrfmi_lbc &B = L.lbc;
   const bool nest = L.a.c.nested;
-  lbc_launch(L, LBC_RELAX, 0.f, 1, 0,
+  lbc_launch(L, LBC_ZERO, 0.f, 1, 0,
+             {lbc_job('u', s.u_save), lbc_job('v', s.v_save), lbc_job('h', s.ph_save), lbc_job('t', s.t_save),
+              lbc_job('m', s.mu_save), lbc_job('h', s.w_save)});
+  lbc_launch(L, LBC_RELAX, 0.f, 0, L.a.c.relax_zone,
              {lbc_job('u', s.u_save, s.ru, nullptr, B.u_b, B.u_bt), lbc_job('v', s.v_save, s.rv, nullptr, B.v_b, B.v_bt),
               lbc_job('h', s.ph_save, s.ph_2, nullptr, B.ph_b, B.ph_bt, 1),
-              lbc_job('t', s.t_save, s.t_2, nullptr, B.t_b, B.t_bt, 1), lbc_job('m', s.mu_save, s.mu_2, nullptr, B.mu_b, B.mu_bt),
-              lbc_job('h', s.w_save, s.w_2, nullptr, nest ? B.w_b : nullptr, nest ? B.w_bt : nullptr, 1)});
+              lbc_job('t', s.t_save, s.t_2, nullptr, B.t_b, B.t_bt, 1), lbc_job('m', s.mu_save, s.mu_2, nullptr, B.mu_b, B.mu_bt)});
+  if (nest) lbc_launch(L, LBC_RELAX, 0.f, 0, L.a.c.relax_zone, {lbc_job('h', s.w_save, s.w_2, nullptr, B.w_b, B.w_bt, 1)});
 }
 // spec_bdy_dry after rk_addtend_dry (solve_em.F:982-1008)
 static void lbc_spec_dry(Launcher &L) {
