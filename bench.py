@@ -17,7 +17,8 @@ species transport on the last substep (moist_adv_opt = scalar_adv_opt = 1), with
 diffusion and Rayleigh damping on the first (diff_6th_opt 2, damp_opt 2), with constant-
 coefficient second-order diffusion on the first (diff_opt 1, km_opt 1), and with advance_w's
 damping layer (damp_opt 3): ms per step and the kernels the options add, against 8 TB/s.
-specified = the same workload with specified lateral boundaries (synthetic boundary arrays).
+specified = the same workload with specified lateral boundaries (synthetic boundary arrays),
+at most 20 timed steps.
 terrain (with --terrain) = the 1200x1200x80 terrain case, FP32 and precision forms.
 cpu_baseline = the FP32 C restatement of the same step (the oracle), OpenMP over the host
 cores with solve_em's j-tiling, on the same workload: median of 3 single steps.
@@ -234,6 +235,13 @@ def option_leg(runtime, dev, torch, stream, steps, pts, log, name):
     return out
 
 
+# the specified-boundary leg's timed steps: its synthetic boundary arrays (the initial
+# state's boundary rows with noise, and random tendencies) do not balance the interior, and
+# the step itself (the oracle as well as the GPU) reaches |w| > 300 m/s after 100 steps of
+# 6 s on this setup; 2 x 20 + 1 steps stay well before that
+SPECIFIED_STEPS = 20
+
+
 def specified_leg(runtime, grid, torch, steps, log, nx, ny, nz):
     """The same workload with specified lateral boundaries on all four sides (synthetic
     boundary arrays of the initial state's boundary rows): ms per step over `steps` steps
@@ -357,7 +365,8 @@ def main():
         print(json.dumps({"terrain": terrain_leg(runtime, grid, torch, args.terrain_steps, log)}))
         return
     if args.specified_only:
-        print(json.dumps({"specified": specified_leg(runtime, grid, torch, args.steps, log, args.nx, args.ny, args.nz)}))
+        print(json.dumps({"specified": specified_leg(runtime, grid, torch, min(args.steps, SPECIFIED_STEPS), log,
+                                                     args.nx, args.ny, args.nz)}))
         return
     spec = grid.large_c1(nx=args.nx, ny=args.ny, nz=args.nz)
     log(f"building the {spec.nx}x{spec.ny}x{spec.nz} initial state ...")
@@ -454,7 +463,11 @@ def main():
     if not args.no_options:
         for name in OPTION_LEGS:
             out[name] = option_leg(runtime, dev, torch, stream, args.steps, pts, log, name)
-        out["specified"] = specified_leg(runtime, grid, torch, args.steps, log, spec.nx, spec.ny, spec.nz)
+        try:
+            out["specified"] = specified_leg(runtime, grid, torch, min(args.steps, SPECIFIED_STEPS), log,
+                                             spec.nx, spec.ny, spec.nz)
+        except RuntimeError as e:
+            out["specified"] = {"error": str(e)}
     if args.ensemble_members > 0:
         del dev
         torch.cuda.empty_cache()
