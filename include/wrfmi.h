@@ -89,7 +89,10 @@ typedef struct wrfmi_config {
  * has one level.  The values are the coupled ones solve_em relaxes toward (u, v: the
  * mu-coupled ru, rv; ph, t: times c1 mu + c2; mu itself).  dtbc is grid%dtbc, the time since
  * the boundary values' valid time: the whole step adds dt to it first, as solve_em.F:377
- * does; fcx, gcx are lbc_fcx_gcx's relaxation weights (module_bc_em.F:1297-1341). */
+ * does; fcx, gcx are lbc_fcx_gcx's relaxation weights (module_bc_em.F:1297-1341).
+ * Implemented: specified boundaries with spec_zone 1 and at most 256 levels; nested
+ * boundaries, and specified ones under graph capture or in ensembles, return
+ * WRFMI_E_BADOPT. */
 #define WRFMI_MAX_BDY_WIDTH 16
 #define WRFMI_LBC_FIELDS(X) X(u) X(v) X(ph) X(t) X(w) X(mu)
 typedef struct wrfmi_lbc {
